@@ -1,0 +1,837 @@
+/* CPU oracle for the FIREBALL 2c/3c assembly path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Plain C, binary64, compiled with -ffp-contract=off.  Each routine restates
+ * the reference routine named in its comment (paths relative to the
+ * reference's src/ tree), keeping the reference's evaluation order so that the
+ * results agree with the reference's Intel/flang FP64 build to rounding.
+ * Only tests/, the smoke check and the CPU-baseline leg of bench.py load this.
+ */
+#include "fb_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define P_EQ2 14.39975            /* include/constants.h:4 */
+#define P_OVERLAP 1               /* include/interactions_2c.h:5-14 */
+#define P_KINETIC 2
+#define P_VNA_ONTOPL 3
+#define P_VNA_ONTOPR 4
+#define P_VNA_ATOM 5
+#define P_DIPOLE_Z 10
+#define P_BCNA 3                  /* include/interactions_3c.h:8 */
+#define MAXTYPE2 26
+#define MAXSUB2 10
+#define MAXTYPE3 5
+#define MAXSUB3 7
+#define P_MBOX 4                  /* c.SYSTEM/M_configuraciones.f90:359 */
+
+typedef struct {
+  int nME, nx;
+  double xmax, dx;
+  int *mu, *nu, *mv;
+  double *data; /* [nx][nME] */
+} cell2c;
+
+typedef struct {
+  int nME, nx, ny;
+  double xmax, ymax;
+  int *mu, *nu, *mv;
+  double *data; /* [ny][nx][nME] */
+} cell3c;
+
+struct fbo_tables {
+  int ns;
+  int nssh[16];
+  int lssh[16][FBO_MAXSH];
+  double rcA[16][FBO_MAXSH];
+  int norb[16];
+  int have_d;
+  cell2c **c2; /* [ns][ns][MAXTYPE2][MAXSUB2] */
+  cell3c **c3; /* [ns][ns][ns][MAXTYPE3][MAXSUB3][FBO_MAXTHETA+1] */
+};
+
+struct fbo_system {
+  const fbo_tables *t;
+  int natoms, icluster, mbeta_max;
+  double *ratom; /* [N][3] */
+  int *imass;    /* 1-based species */
+  double lattice[9];
+  double *dQ;    /* [N][FBO_MAXSH] */
+  double *xl;    /* [mbeta_max+1][3] */
+  /* neighbour lists (CSR over iatom) */
+  int *neighn, *noff, *neigh_j, *neigh_b, *neigh_back, *neigh_self;
+  int64_t npairs;
+  /* common-neighbour lists (CSR over ialpha) */
+  int *ncommon, *coff, *ci_j, *ci_b, *cj_j, *cj_b, *cm;
+  int64_t ntrip;
+  /* blocks */
+  int64_t *boff; /* per pair */
+  int64_t blen;
+  double *mat[5];
+  int done[5];
+  int nfallback;
+};
+
+static inline cell2c *C2(const fbo_tables *t, int is, int js, int it, int isub) {
+  if (is < 1 || js < 1 || is > t->ns || js > t->ns || it < 0 || it >= MAXTYPE2 || isub < 0 ||
+      isub >= MAXSUB2)
+    return NULL;
+  return t->c2[(((size_t)(is - 1) * t->ns + (js - 1)) * MAXTYPE2 + it) * MAXSUB2 + isub];
+}
+
+static inline cell3c *C3(const fbo_tables *t, int is, int js, int ks, int it, int isub, int ith) {
+  if (it < 0 || it >= MAXTYPE3 || isub < 0 || isub >= MAXSUB3 || ith < 1 || ith > FBO_MAXTHETA)
+    return NULL;
+  size_t k = ((((size_t)(is - 1) * t->ns + (js - 1)) * t->ns + (ks - 1)) * MAXTYPE3 + it);
+  return t->c3[(k * MAXSUB3 + isub) * (FBO_MAXTHETA + 1) + ith];
+}
+
+fbo_tables *fbo_tables_new(int nspecies, const int *nssh, const int *lssh, const double *rcutoffA) {
+  if (nspecies < 1 || nspecies > 16) return NULL;
+  fbo_tables *t = calloc(1, sizeof(*t));
+  t->ns = nspecies;
+  for (int i = 0; i < nspecies; i++) {
+    t->nssh[i] = nssh[i];
+    t->norb[i] = 0;
+    for (int k = 0; k < nssh[i]; k++) {
+      t->lssh[i][k] = lssh[i * FBO_MAXSH + k];
+      t->rcA[i][k] = rcutoffA[i * FBO_MAXSH + k];
+      t->norb[i] += 2 * t->lssh[i][k] + 1;
+      if (t->lssh[i][k] == 2) t->have_d = 1; /* c.SYSTEM/M_species.f90:1212-1217 */
+    }
+  }
+  size_t n2 = (size_t)nspecies * nspecies * MAXTYPE2 * MAXSUB2;
+  size_t n3 = (size_t)nspecies * nspecies * nspecies * MAXTYPE3 * MAXSUB3 * (FBO_MAXTHETA + 1);
+  t->c2 = calloc(n2, sizeof(cell2c *));
+  t->c3 = calloc(n3, sizeof(cell3c *));
+  return t;
+}
+
+void fbo_tables_free(fbo_tables *t) {
+  if (!t) return;
+  size_t n2 = (size_t)t->ns * t->ns * MAXTYPE2 * MAXSUB2;
+  size_t n3 = (size_t)t->ns * t->ns * t->ns * MAXTYPE3 * MAXSUB3 * (FBO_MAXTHETA + 1);
+  for (size_t i = 0; i < n2; i++)
+    if (t->c2[i]) { free(t->c2[i]->mu); free(t->c2[i]->nu); free(t->c2[i]->mv); free(t->c2[i]->data); free(t->c2[i]); }
+  for (size_t i = 0; i < n3; i++)
+    if (t->c3[i]) { free(t->c3[i]->mu); free(t->c3[i]->nu); free(t->c3[i]->mv); free(t->c3[i]->data); free(t->c3[i]); }
+  free(t->c2); free(t->c3); free(t);
+}
+
+static int *dupi(const int *a, int n) { int *b = malloc(sizeof(int) * (n ? n : 1)); memcpy(b, a, sizeof(int) * n); return b; }
+
+int fbo_add_cell2c(fbo_tables *t, int is, int js, int itype, int isub, int nME, int nx, double xmax,
+                   const int *mu, const int *nu, const int *mvalue, const double *data) {
+  if (!t || is < 1 || js < 1 || is > t->ns || js > t->ns || itype < 0 || itype >= MAXTYPE2 ||
+      isub < 0 || isub >= MAXSUB2 || nx < 7 || nME < 1)
+    return FBO_E_ARG;
+  cell2c *c = calloc(1, sizeof(*c));
+  c->nME = nME; c->nx = nx; c->xmax = xmax;
+  c->dx = xmax / (double)(nx - 1); /* e.FDATA/M_Fdata_2c.f90:222 */
+  c->mu = dupi(mu, nME); c->nu = dupi(nu, nME); c->mv = dupi(mvalue, nME);
+  c->data = malloc(sizeof(double) * nx * nME);
+  memcpy(c->data, data, sizeof(double) * nx * nME);
+  t->c2[(((size_t)(is - 1) * t->ns + (js - 1)) * MAXTYPE2 + itype) * MAXSUB2 + isub] = c;
+  return FBO_OK;
+}
+
+int fbo_add_cell3c(fbo_tables *t, int is, int js, int ks, int itype, int isub, int itheta, int nME,
+                   int nx, int ny, double xmax, double ymax, const int *mu, const int *nu,
+                   const int *mvalue, const double *data) {
+  if (!t || is < 1 || js < 1 || ks < 1 || is > t->ns || js > t->ns || ks > t->ns || itype < 0 ||
+      itype >= MAXTYPE3 || isub < 0 || isub >= MAXSUB3 || itheta < 1 || itheta > FBO_MAXTHETA ||
+      nx < 4 || ny < 4)
+    return FBO_E_ARG;
+  cell3c *c = calloc(1, sizeof(*c));
+  c->nME = nME; c->nx = nx; c->ny = ny; c->xmax = xmax; c->ymax = ymax;
+  c->mu = dupi(mu, nME); c->nu = dupi(nu, nME); c->mv = dupi(mvalue, nME);
+  c->data = malloc(sizeof(double) * nx * ny * nME);
+  memcpy(c->data, data, sizeof(double) * nx * ny * nME);
+  size_t k = ((((size_t)(is - 1) * t->ns + (js - 1)) * t->ns + (ks - 1)) * MAXTYPE3 + itype);
+  t->c3[(k * MAXSUB3 + isub) * (FBO_MAXTHETA + 1) + itheta] = c;
+  return FBO_OK;
+}
+
+/* ---------------------------------------------------------------- helpers */
+
+/* b.FUNCTIONS/distance.f90:72-73 */
+double fbo_distance(const double *a, const double *b) {
+  double d1 = b[0] - a[0], d2 = b[1] - a[1], d3 = b[2] - a[2];
+  return sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+}
+
+/* b.FUNCTIONS/magnitude.f90: sqrt(dot_product(a, a)) */
+static double magnitude(const double *a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
+/* b.FUNCTIONS/a_cross_b.f90 */
+static void cross(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* b.FUNCTIONS/smoother.f90:124-136 */
+double fbo_smoother(double r, double rend, double x) {
+  double rbegin = x * rend;
+  if (r > rend) return 0.0;
+  if (r < rbegin) return 1.0;
+  double ratio = (r - rbegin) / (rend - rbegin);
+  double sm = 1.0 - ratio * ratio;
+  return sm * sm;
+}
+
+/* b.FUNCTIONS/epsilon.f90:73-177.  spe is column-major 3x3: spe[3*j + i] = spe(i+1, j+1). */
+static int epsilon_impl(const double *r1, const double *r2, double *spe) {
+  double r1_mag = magnitude(r1), r2_mag = magnitude(r2);
+  double zp[3], yp[3] = {0, 0, 0}, xp[3], r1h[3];
+  int fallback = 0;
+  if (r2_mag < 1.0e-4) { /* :123-133 */
+    for (int i = 0; i < 9; i++) spe[i] = 0.0;
+    spe[0] = spe[4] = spe[8] = 1.0;
+    return 1;
+  }
+  for (int i = 0; i < 3; i++) zp[i] = r2[i] / r2_mag;
+  double yp_mag = 0.0;
+  if (r1_mag > 1.0e-4) {
+    for (int i = 0; i < 3; i++) r1h[i] = r1[i] / r1_mag;
+    cross(zp, r1h, yp);
+    yp_mag = magnitude(yp);
+  }
+  if (yp_mag < 1.0e-4) { /* :148-168 */
+    fallback = 1;
+    if (fabs(zp[0]) > 1.0e-4) {
+      yp[0] = -(zp[1] + zp[2]) / zp[0]; yp[1] = 1.0; yp[2] = 1.0;
+    } else if (fabs(zp[1]) > 1.0e-4) {
+      yp[0] = 1.0; yp[1] = -(zp[0] + zp[2]) / zp[1]; yp[2] = 1.0;
+    } else {
+      yp[0] = 1.0; yp[1] = 1.0; yp[2] = -(zp[0] + zp[1]) / zp[2];
+    }
+    yp_mag = magnitude(yp);
+  }
+  for (int i = 0; i < 3; i++) yp[i] = yp[i] / yp_mag;
+  cross(yp, zp, xp);
+  for (int i = 0; i < 3; i++) { spe[i] = xp[i]; spe[3 + i] = yp[i]; spe[6 + i] = zp[i]; }
+  return fallback;
+}
+
+void fbo_epsilon(const double *r1, const double *r2, double *spe) { (void)epsilon_impl(r1, r2, spe); }
+
+/* The d-orbital rotation tensor amat(3,3,5) of include/constants.h:26-35 written
+ * out by its non-zero entries: amat(ix,jx,imu) with 1-based ix, jx. */
+static double amat(int ix, int jx, int imu) {
+  const double h = 0.5, c = 1.0 / (2.0 * sqrt(3.0)), c2 = 2.0 / (2.0 * sqrt(3.0));
+  switch (imu) {
+    case 1: return ((ix == 1 && jx == 2) || (ix == 2 && jx == 1)) ? h : 0.0;
+    case 2: return ((ix == 2 && jx == 3) || (ix == 3 && jx == 2)) ? h : 0.0;
+    case 3: return (ix == jx) ? (ix == 3 ? c2 : -c) : 0.0;
+    case 4: return ((ix == 1 && jx == 3) || (ix == 3 && jx == 1)) ? h : 0.0;
+    case 5: return (ix == jx && ix < 3) ? (ix == 1 ? h : -h) : 0.0;
+  }
+  return 0.0;
+}
+
+/* c.SYSTEM/M_rotations.f90:221-313.  eps, pmat column-major 3x3; dmat column-major 5x5. */
+void fbo_twister(const double *eps, int have_d, double *dmat, double *pmat) {
+#define E(i, j) eps[3 * ((j) - 1) + (i) - 1]
+#define PM(i, j) pmat[3 * ((j) - 1) + (i) - 1]
+#define DM(i, j) dmat[5 * ((j) - 1) + (i) - 1]
+  PM(1, 1) = E(2, 2); PM(1, 2) = E(2, 3); PM(1, 3) = E(2, 1);
+  PM(2, 1) = E(3, 2); PM(2, 2) = E(3, 3); PM(2, 3) = E(3, 1);
+  PM(3, 1) = E(1, 2); PM(3, 2) = E(1, 3); PM(3, 3) = E(1, 1);
+  if (!have_d) return;
+  for (int imu = 1; imu <= 5; imu++) {
+    double l11 = 0, l12 = 0, l13 = 0, l32 = 0, l33 = 0;
+    for (int jx = 1; jx <= 3; jx++)
+      for (int ix = 1; ix <= 3; ix++) {
+        double a = amat(ix, jx, imu);
+        if (a != 0.0) {
+          l11 = l11 + a * E(jx, 1) * E(ix, 1);
+          l12 = l12 + a * E(jx, 1) * E(ix, 2);
+          l13 = l13 + a * E(jx, 1) * E(ix, 3);
+          l32 = l32 + a * E(jx, 3) * E(ix, 2);
+          l33 = l33 + a * E(jx, 3) * E(ix, 3);
+        }
+      }
+    DM(imu, 1) = 2.0 * l12;
+    DM(imu, 2) = 2.0 * l32;
+    DM(imu, 3) = sqrt(3.0) * l33;
+    DM(imu, 4) = 2.0 * l13;
+    DM(imu, 5) = 2.0 * l11 + l33;
+  }
+#undef E
+#undef PM
+#undef DM
+}
+
+/* c.SYSTEM/M_rotations.f90:341-382: 5x5 column-major choice by l */
+static void chooser(int l, const double *dmat, const double *pmat, double *r) {
+  for (int i = 0; i < 25; i++) r[i] = 0.0;
+  if (l == 0) r[0] = 1.0;
+  else if (l == 1) { for (int j = 0; j < 3; j++) for (int i = 0; i < 3; i++) r[5 * j + i] = pmat[3 * j + i]; }
+  else if (l == 2) { for (int i = 0; i < 25; i++) r[i] = dmat[i]; }
+}
+
+/* c.SYSTEM/M_rotations.f90:98-177: X = L M R^T per shell pair, loop order m2, m1, k2, k1. */
+void fbo_rotate(const fbo_tables *t, int in1, int in3, const double *eps, int norb_mu, int norb_nu,
+                const double *M, double *X) {
+  double dmat[25] = {0}, pmat[9], left[25], right[25];
+  fbo_twister(eps, t->have_d, dmat, pmat);
+  for (int i = 0; i < norb_mu * norb_nu; i++) X[i] = 0.0;
+  int n1 = 0;
+  for (int issh = 0; issh < t->nssh[in1 - 1]; issh++) {
+    int l1 = t->lssh[in1 - 1][issh];
+    chooser(l1, dmat, pmat, left);
+    int n2 = 0;
+    for (int jssh = 0; jssh < t->nssh[in3 - 1]; jssh++) {
+      int l2 = t->lssh[in3 - 1][jssh];
+      chooser(l2, dmat, pmat, right);
+      for (int m2 = 1; m2 <= 2 * l2 + 1; m2++)
+        for (int m1 = 1; m1 <= 2 * l1 + 1; m1++)
+          for (int k2 = 1; k2 <= 2 * l2 + 1; k2++)
+            for (int k1 = 1; k1 <= 2 * l1 + 1; k1++) {
+              double *x = &X[(n2 + k2 - 1) * norb_mu + (n1 + k1 - 1)];
+              *x = *x + left[5 * (m1 - 1) + (k1 - 1)] * M[(n2 + m2 - 1) * norb_mu + (n1 + m1 - 1)] *
+                            right[5 * (m2 - 1) + (k2 - 1)];
+            }
+      n2 += 2 * l2 + 1;
+    }
+    n1 += 2 * l1 + 1;
+  }
+}
+
+/* e.FDATA/M_Fdata_2c.f90:302-415: 7-point Lagrange interpolation + recovery. */
+int fbo_getMEs_2c(const fbo_tables *t, int is, int js, int iint, int isub, double x, int norb_mu,
+                  int norb_nu, double *hmbox) {
+  const double P_tolerance = 1.0e-5;
+  const cell2c *c = C2(t, is, js, iint, isub);
+  if (!c) return FBO_E_NO_TABLE;
+  if (x < 0.0) return FBO_E_X_NEGATIVE;
+  if (x > c->xmax * (1.0 + P_tolerance)) return FBO_E_X_OUT_OF_RANGE;
+  /* the x < P_tolerance fill (:354-362) is overwritten below, as in the reference */
+  int imid = (int)(x / c->dx) + 1;
+  int ileft = imid - 3, iright = imid + 3;
+  if (ileft < 1) { ileft = 1; iright = 7; }
+  else if (iright > c->nx) { ileft = c->nx - 6; iright = c->nx; }
+  double xx[7], F[64];
+  for (int ip = ileft; ip <= iright; ip++) xx[ip - ileft] = (double)(ip - 1) * c->dx;
+  for (int k = 0; k < c->nME; k++) F[k] = 0.0;
+  for (int isum = ileft; isum <= iright; isum++) {
+    double prod = 1.0;
+    for (int iprod = ileft; iprod <= iright; iprod++) {
+      if (iprod != isum) {
+        double pden = 1.0 / (xx[isum - ileft] - xx[iprod - ileft]);
+        prod = prod * (x - xx[iprod - ileft]) * pden;
+      }
+    }
+    const double *row = c->data + (size_t)(isum - 1) * c->nME;
+    for (int k = 0; k < c->nME; k++) F[k] = F[k] + row[k] * prod;
+  }
+  for (int i = 0; i < norb_mu * norb_nu; i++) hmbox[i] = 0.0;
+  for (int k = 0; k < c->nME; k++) hmbox[(c->nu[k] - 1) * norb_mu + (c->mu[k] - 1)] = F[k];
+  return FBO_OK;
+}
+
+/* e.FDATA/M_Fdata_3c.f90:293-517: adaptive bivariate interpolation of the five
+ * Legendre theta cells, Legendre sum, sin(theta) on mvalue=1 MEs. */
+int fbo_getMEs_3c(const fbo_tables *t, int is, int js, int ks, int iint, int isub, double x,
+                  double y, int norb_mu, int norb_nu, double cost, double *hmbox) {
+  const double P_tiny = 1.0e-6;
+  const double P_small = 1.0 - 4.0; /* "1.0d0-4" in the reference: -3.0, so method 2 never runs */
+  double p[5];
+  double cost2 = cost * cost;
+  double sint = sqrt(fmax(1.0e-5, 1.0 - cost2));
+  p[0] = 1.0;
+  p[1] = cost;
+  p[2] = 0.5 * (3.0 * cost2 - 1.0);
+  p[3] = 0.5 * cost * (5.0 * cost2 - 3.0);
+  p[4] = 0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2));
+  const cell3c *c1 = C3(t, is, js, ks, iint, isub, 1);
+  if (!c1) return FBO_E_NO_TABLE;
+  int nME = c1->nME, nx = c1->nx, ny = c1->ny;
+  double F[64];
+  for (int k = 0; k < nME; k++) F[k] = 0.0;
+  double dx = c1->xmax / (nx - 1.0), dy = c1->ymax / (ny - 1.0);
+  int ix = (int)(x / dx) + 1, iy = (int)(y / dy) + 1;
+  if (ix < 2) ix = 2;
+  if (iy < 2) iy = 2;
+  if (ix > nx - 2) ix = nx - 2;
+  if (iy > ny - 2) iy = ny - 2;
+  double px = x / dx - ix + 1.0, py = y / dy - iy + 1.0;
+#define T3(c, i, j, m) (c)->data[((size_t)((j) - 1) * nx + ((i) - 1)) * nME + (m)]
+  double gx = (T3(c1, ix + 1, iy, 0) - T3(c1, ix, iy, 0)) / dx;
+  double gy = (T3(c1, ix, iy + 1, 0) - T3(c1, ix, iy, 0)) / dy;
+  double gg = gx * gx + gy * gy;
+  const cell3c *c = c1;
+  for (int ith = 1; ith <= FBO_MAXTHETA; ith++) {
+    c = C3(t, is, js, ks, iint, isub, ith);
+    if (!c) return FBO_E_NO_TABLE;
+    for (int m = 0; m < nME; m++) {
+      double interim;
+      if (gg < P_tiny) { /* method 1 */
+        interim = ((1.0 - px - py) * T3(c, ix, iy, m) + px * T3(c, ix + 1, iy, m)) +
+                  py * T3(c, ix, iy + 1, m);
+      } else if (gg <= P_small) { /* method 2: unreachable, kept for fidelity */
+        interim = (py * (py - 1) * 0.5 * T3(c, ix, iy - 1, m) + px * (px - 1) * 0.5 * T3(c, ix - 1, iy, m) +
+                   (1.0 + px * py - px * px - py * py) * T3(c, ix, iy, m) +
+                   px * (px - 2.0 * py + 1.0) * 0.5 * T3(c, ix + 1, iy, m) +
+                   py * (py - 2.0 * px + 1.0) * 0.5 * T3(c, ix, iy + 1, m) + px * py * T3(c, ix + 1, iy + 1, m));
+      } else { /* method 3: 4-point in y at four x rows, then 4-point in x */
+        double g[4];
+        for (int k = -1; k <= 2; k++) {
+          double f1m1 = T3(c, ix + k, iy - 1, m);
+          double f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+          double f0p3 = 3.0 * T3(c, ix + k, iy, m), f0p6 = 2.0 * f0p3;
+          double f1p3 = 3.0 * T3(c, ix + k, iy + 1, m), f1p6 = 2.0 * f1p3;
+          double f2p1 = T3(c, ix + k, iy + 2, m);
+          double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+          double bb2 = f1m3 - f0p6 + f1p3;
+          double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+          double bb0 = f0p6;
+          g[k + 1] = ((bb3 * py + bb2) * py + bb1) * py + bb0;
+        }
+        double f1m1 = g[0], f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+        double f0p3 = 3.0 * g[1], f0p6 = 2.0 * f0p3;
+        double f1p3 = 3.0 * g[2], f1p6 = 2.0 * f1p3;
+        double f2p1 = g[3];
+        double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+        double bb2 = f1m3 - f0p6 + f1p3;
+        double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+        double bb0 = f0p6;
+        interim = (((bb3 * px + bb2) * px + bb1) * px + bb0) / 36.0;
+      }
+      F[m] = F[m] + p[ith - 1] * interim;
+    }
+  }
+#undef T3
+  for (int i = 0; i < norb_mu * norb_nu; i++) hmbox[i] = 0.0;
+  for (int k = 0; k < c->nME; k++) {
+    double v = (c->mv[k] != 1) ? F[k] : F[k] * sint;
+    hmbox[(c->nu[k] - 1) * norb_mu + (c->mu[k] - 1)] = v;
+  }
+  return FBO_OK;
+}
+
+/* c.SYSTEM/make_cells.f90:115-144.  xl_out holds (mbeta_max+1)*3 values. */
+int fbo_make_cells(const double *lat, int icluster, double *xl) {
+  xl[0] = xl[1] = xl[2] = 0.0;
+  if (icluster == 1) return 0;
+  int mbeta = 1;
+  for (int iz = -P_MBOX; iz <= P_MBOX; iz++)
+    for (int iy = -P_MBOX; iy <= P_MBOX; iy++)
+      for (int ix = -P_MBOX; ix <= P_MBOX; ix++) {
+        if (ix == 0 && iy == 0 && iz == 0) continue;
+        for (int k = 0; k < 3; k++)
+          xl[3 * mbeta + k] = (double)ix * lat[0 + k] + (double)iy * lat[3 + k] + (double)iz * lat[6 + k];
+        mbeta++;
+      }
+  return mbeta - 1;
+}
+
+/* ------------------------------------------------------------- structure */
+
+fbo_system *fbo_system_new(const fbo_tables *t, int natoms, const double *ratom, const int *imass,
+                           const double *lattice, int icluster, const double *dQ) {
+  if (!t || natoms < 1) return NULL;
+  for (int i = 0; i < natoms; i++)
+    if (imass[i] < 1 || imass[i] > t->ns) return NULL;
+  fbo_system *s = calloc(1, sizeof(*s));
+  s->t = t;
+  s->natoms = natoms;
+  s->icluster = icluster;
+  s->ratom = malloc(sizeof(double) * 3 * natoms);
+  memcpy(s->ratom, ratom, sizeof(double) * 3 * natoms);
+  s->imass = dupi(imass, natoms);
+  memcpy(s->lattice, lattice, sizeof(double) * 9);
+  s->dQ = calloc((size_t)natoms * FBO_MAXSH, sizeof(double));
+  if (dQ) memcpy(s->dQ, dQ, sizeof(double) * natoms * FBO_MAXSH);
+  int nimg = (icluster == 1) ? 1 : (2 * P_MBOX + 1) * (2 * P_MBOX + 1) * (2 * P_MBOX + 1);
+  s->xl = calloc((size_t)nimg * 3, sizeof(double));
+  s->mbeta_max = fbo_make_cells(lattice, icluster, s->xl);
+  return s;
+}
+
+void fbo_system_free(fbo_system *s) {
+  if (!s) return;
+  free(s->ratom); free(s->imass); free(s->dQ); free(s->xl);
+  free(s->neighn); free(s->noff); free(s->neigh_j); free(s->neigh_b); free(s->neigh_back); free(s->neigh_self);
+  free(s->ncommon); free(s->coff); free(s->ci_j); free(s->ci_b); free(s->cj_j); free(s->cj_b); free(s->cm);
+  free(s->boff);
+  for (int i = 0; i < 5; i++) free(s->mat[i]);
+  free(s);
+}
+
+static double rcut_max(const fbo_tables *t, int in) {
+  double r = -99.0;
+  for (int k = 0; k < t->nssh[in - 1]; k++) r = fmax(r, t->rcA[in - 1][k]);
+  return r;
+}
+
+static double rcut_min(const fbo_tables *t, int in) {
+  double r = 99.0;
+  for (int k = 0; k < t->nssh[in - 1]; k++) r = fmin(r, t->rcA[in - 1][k]);
+  return r;
+}
+
+#define RAT(s, i) (&(s)->ratom[3 * ((i) - 1)])
+#define XL(s, b) (&(s)->xl[3 * (b)])
+
+/* c.SYSTEM/M_neighbors.f90:246-519 (find_neigh_max + find_neighbors) and
+ * :548-812 (find_common_max + find_common_neighbors). */
+int fbo_build_lists(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  int N = s->natoms;
+  s->neighn = calloc(N, sizeof(int));
+  s->noff = calloc(N + 1, sizeof(int));
+  /* pass 1: count (find_neigh_max) */
+  for (int ia = 1; ia <= N; ia++) {
+    int in1 = s->imass[ia - 1];
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    double rci = rcut_max(t, in1);
+    int num = 0;
+    for (int mb = 0; mb <= s->mbeta_max; mb++)
+      for (int ja = 1; ja <= N; ja++) {
+        int in2 = s->imass[ja - 1];
+        double r2[3];
+        for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+        double rcj = rcut_max(t, in2);
+        double z = fbo_distance(r1, r2);
+        double range_max = rci + rcj - 0.01;
+        if (ia != ja && z < 0.5) return FBO_E_ATOMS_TOO_CLOSE;
+        if (z < range_max) num++;
+      }
+    s->neighn[ia - 1] = num;
+    s->noff[ia] = s->noff[ia - 1] + num;
+  }
+  s->npairs = s->noff[N];
+  s->neigh_j = malloc(sizeof(int) * (s->npairs + 1));
+  s->neigh_b = malloc(sizeof(int) * (s->npairs + 1));
+  s->neigh_back = malloc(sizeof(int) * (s->npairs + 1));
+  s->neigh_self = malloc(sizeof(int) * N);
+  /* pass 2: fill */
+  for (int ia = 1; ia <= N; ia++) {
+    int in1 = s->imass[ia - 1];
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    double rci = rcut_max(t, in1);
+    int num = 0, off = s->noff[ia - 1];
+    for (int mb = 0; mb <= s->mbeta_max; mb++)
+      for (int ja = 1; ja <= N; ja++) {
+        int in2 = s->imass[ja - 1];
+        double r2[3];
+        for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+        double rcj = rcut_max(t, in2);
+        double z = fbo_distance(r1, r2);
+        double range_max = rci + rcj - 0.01;
+        if (z < range_max) {
+          s->neigh_j[off + num] = ja;
+          s->neigh_b[off + num] = mb;
+          num++;
+        }
+      }
+  }
+  /* neigh_self and neigh_back (:457-515) */
+  for (int ia = 1; ia <= N; ia++) s->neigh_self[ia - 1] = -999;
+  for (int ia = 1; ia <= N; ia++) {
+    int off = s->noff[ia - 1];
+    for (int in = 1; in <= s->neighn[ia - 1]; in++) {
+      int mb = s->neigh_b[off + in - 1], ja = s->neigh_j[off + in - 1];
+      s->neigh_back[off + in - 1] = -999;
+      if (ia == ja && mb == 0) {
+        if (s->neigh_self[ia - 1] != -999) return FBO_E_NEIGH_SELF;
+        s->neigh_self[ia - 1] = in;
+      }
+      int joff = s->noff[ja - 1];
+      for (int jn = 1; jn <= s->neighn[ja - 1]; jn++) {
+        int jb = s->neigh_b[joff + jn - 1], ka = s->neigh_j[joff + jn - 1];
+        if (ia != ka || fabs(XL(s, mb)[0] + XL(s, jb)[0]) > 1.0e-3 ||
+            fabs(XL(s, mb)[1] + XL(s, jb)[1]) > 1.0e-3 || fabs(XL(s, mb)[2] + XL(s, jb)[2]) > 1.0e-3) {
+        } else {
+          if (s->neigh_back[off + in - 1] != -999) return FBO_E_NEIGH_BACK;
+          s->neigh_back[off + in - 1] = jn;
+        }
+      }
+    }
+    for (int in = 1; in <= s->neighn[ia - 1]; in++)
+      if (s->neigh_back[off + in - 1] == -999) return FBO_E_NEIGH_BACK;
+  }
+  for (int ia = 1; ia <= N; ia++)
+    if (s->neigh_self[ia - 1] == -999) return FBO_E_NEIGH_SELF;
+
+  /* common neighbours: one pass storing into growable arrays */
+  s->ncommon = calloc(N, sizeof(int));
+  s->coff = calloc(N + 1, sizeof(int));
+  int64_t cap = 1024, n = 0;
+  int *a_ij = malloc(sizeof(int) * cap), *a_ib = malloc(sizeof(int) * cap), *a_jj = malloc(sizeof(int) * cap),
+      *a_jb = malloc(sizeof(int) * cap), *a_m = malloc(sizeof(int) * cap);
+  for (int ial = 1; ial <= N; ial++) {
+    int off = s->noff[ial - 1], nn = s->neighn[ial - 1], num = 0;
+    for (int in = 1; in <= nn; in++) {
+      int ia = s->neigh_j[off + in - 1], ib = s->neigh_b[off + in - 1];
+      if (ia == ial && ib == 0) continue;
+      int in1 = s->imass[ia - 1];
+      double r1[3];
+      for (int k = 0; k < 3; k++) r1[k] = RAT(s, ia)[k] + XL(s, ib)[k];
+      double rci = rcut_max(t, in1);
+      for (int jn = 1; jn <= nn; jn++) {
+        int ja = s->neigh_j[off + jn - 1], jb = s->neigh_b[off + jn - 1];
+        if ((ja == ial && jb == 0) || in == jn) continue;
+        int in2 = s->imass[ja - 1];
+        double r2[3];
+        for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, jb)[k];
+        double rcj = rcut_max(t, in2);
+        double z = fbo_distance(r1, r2);
+        double range_max = rci + rcj - 0.01;
+        if (z < range_max) {
+          int m = -9999;
+          int ioff = s->noff[ia - 1];
+          for (int kn = 1; kn <= s->neighn[ia - 1]; kn++) {
+            int ka = s->neigh_j[ioff + kn - 1], kb = s->neigh_b[ioff + kn - 1];
+            double r3[3];
+            for (int k = 0; k < 3; k++) r3[k] = XL(s, kb)[k] + RAT(s, ka)[k] - RAT(s, ia)[k];
+            if (fabs(r3[0] - r2[0] + r1[0]) < 1.0e-4 && fabs(r3[1] - r2[1] + r1[1]) < 1.0e-4 &&
+                fabs(r3[2] - r2[2] + r1[2]) < 1.0e-4) {
+              m = kn;
+              break;
+            }
+          }
+          if (m == -9999) continue; /* dropped (:791-792) */
+          if (n == cap) {
+            cap *= 2;
+            a_ij = realloc(a_ij, sizeof(int) * cap); a_ib = realloc(a_ib, sizeof(int) * cap);
+            a_jj = realloc(a_jj, sizeof(int) * cap); a_jb = realloc(a_jb, sizeof(int) * cap);
+            a_m = realloc(a_m, sizeof(int) * cap);
+          }
+          a_ij[n] = ia; a_ib[n] = ib; a_jj[n] = ja; a_jb[n] = jb; a_m[n] = m;
+          n++; num++;
+        }
+      }
+    }
+    s->ncommon[ial - 1] = num;
+    s->coff[ial] = s->coff[ial - 1] + num;
+  }
+  s->ntrip = n;
+  s->ci_j = a_ij; s->ci_b = a_ib; s->cj_j = a_jj; s->cj_b = a_jb; s->cm = a_m;
+
+  /* block offsets */
+  s->boff = malloc(sizeof(int64_t) * (s->npairs + 1));
+  int64_t o = 0;
+  for (int ia = 1; ia <= N; ia++) {
+    int nmu = t->norb[s->imass[ia - 1] - 1];
+    for (int in = 0; in < s->neighn[ia - 1]; in++) {
+      int p = s->noff[ia - 1] + in;
+      s->boff[p] = o;
+      o += (int64_t)nmu * t->norb[s->imass[s->neigh_j[p] - 1] - 1];
+    }
+  }
+  s->boff[s->npairs] = o;
+  s->blen = o;
+  for (int i = 0; i < 5; i++) { free(s->mat[i]); s->mat[i] = calloc(o ? o : 1, sizeof(double)); s->done[i] = 0; }
+  return FBO_OK;
+}
+
+int64_t fbo_npairs(const fbo_system *s) { return s->npairs; }
+int64_t fbo_ntriplets(const fbo_system *s) { return s->ntrip; }
+int64_t fbo_block_len(const fbo_system *s) { return s->blen; }
+int fbo_frame_fallbacks(const fbo_system *s) { return s->nfallback; }
+
+int fbo_get_lists(const fbo_system *s, int *neighn, int *neigh_j, int *neigh_b, int *neigh_back,
+                  int *neigh_self, int *ncommon, int *iatom_j, int *iatom_b, int *jatom_j,
+                  int *jatom_b, int *mneigh) {
+  if (!s->neighn) return FBO_E_ORDER;
+  int N = s->natoms;
+  if (neighn) memcpy(neighn, s->neighn, sizeof(int) * N);
+  if (neigh_j) memcpy(neigh_j, s->neigh_j, sizeof(int) * s->npairs);
+  if (neigh_b) memcpy(neigh_b, s->neigh_b, sizeof(int) * s->npairs);
+  if (neigh_back) memcpy(neigh_back, s->neigh_back, sizeof(int) * s->npairs);
+  if (neigh_self) memcpy(neigh_self, s->neigh_self, sizeof(int) * N);
+  if (ncommon) memcpy(ncommon, s->ncommon, sizeof(int) * N);
+  if (iatom_j) memcpy(iatom_j, s->ci_j, sizeof(int) * s->ntrip);
+  if (iatom_b) memcpy(iatom_b, s->ci_b, sizeof(int) * s->ntrip);
+  if (jatom_j) memcpy(jatom_j, s->cj_j, sizeof(int) * s->ntrip);
+  if (jatom_b) memcpy(jatom_b, s->cj_b, sizeof(int) * s->ntrip);
+  if (mneigh) memcpy(mneigh, s->cm, sizeof(int) * s->ntrip);
+  return FBO_OK;
+}
+
+int fbo_get_blocks(const fbo_system *s, int matrix, double *out) {
+  if (matrix < 0 || matrix > 4 || !s->mat[matrix]) return FBO_E_ARG;
+  memcpy(out, s->mat[matrix], sizeof(double) * s->blen);
+  return FBO_OK;
+}
+
+/* ------------------------------------------------------------ assemblers */
+
+/* frame of a pair: sighat along r2 - r1 (default z when z < 1e-5), epsilon(r2, sighat) */
+static double pair_frame(fbo_system *s, const double *r1, const double *r2, double *eps) {
+  double z = fbo_distance(r1, r2), sighat[3];
+  if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+  else for (int k = 0; k < 3; k++) sighat[k] = (r2[k] - r1[k]) / z;
+  s->nfallback += epsilon_impl(r2, sighat, eps);
+  return z;
+}
+
+/* j.ASSEMBLERS/M_assemble_2c_DOGS.f90:87-209 (S), :232-353 (T), :376-499 (dipole_z) */
+static int assemble_2c_simple(fbo_system *s, int interaction, int matrix) {
+  const fbo_tables *t = s->t;
+  if (!s->neighn) return FBO_E_ORDER;
+  double sm[81], eps[9];
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1];
+    for (int in = 0; in < s->neighn[ia - 1]; in++) {
+      int p = s->noff[ia - 1] + in;
+      int mb = s->neigh_b[p], ja = s->neigh_j[p];
+      double r2[3];
+      for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+      int in2 = s->imass[ja - 1], nnu = t->norb[in2 - 1];
+      double z = pair_frame(s, r1, r2, eps);
+      int rc = fbo_getMEs_2c(t, in1, in2, interaction, 0, z, nmu, nnu, sm);
+      if (rc) return rc;
+      fbo_rotate(t, in1, in2, eps, nmu, nnu, sm, s->mat[matrix] + s->boff[p]);
+    }
+  }
+  s->done[matrix] = 1;
+  return FBO_OK;
+}
+
+int fbo_assemble_S(fbo_system *s) { return assemble_2c_simple(s, P_OVERLAP, FBO_MAT_S); }
+int fbo_assemble_T(fbo_system *s) { return assemble_2c_simple(s, P_KINETIC, FBO_MAT_T); }
+int fbo_assemble_dipole_z(fbo_system *s) { return assemble_2c_simple(s, P_DIPOLE_Z, FBO_MAT_DIP); }
+
+/* j.ASSEMBLERS/M_assemble_2c_DOGS.f90:523-829 */
+int fbo_assemble_vna_2c(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  if (!s->neighn || !s->done[FBO_MAT_S]) return FBO_E_ORDER;
+  double *vb = s->mat[FBO_MAT_VNA], *vo = s->mat[FBO_MAT_VNAO];
+  memset(vb, 0, sizeof(double) * s->blen);
+  memset(vo, 0, sizeof(double) * s->blen);
+  double bm[81], bx[81], eps[9], emnpl[81];
+  /* ontop terms into blocko (:592-702) */
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1];
+    for (int in = 0; in < s->neighn[ia - 1]; in++) {
+      int p = s->noff[ia - 1] + in;
+      int mb = s->neigh_b[p], ja = s->neigh_j[p];
+      double r2[3];
+      for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+      int in2 = s->imass[ja - 1], nnu = t->norb[in2 - 1], nb = nmu * nnu;
+      double z = pair_frame(s, r1, r2, eps);
+      if (ia == ja && mb == 0) continue;
+      double *blk = vo + s->boff[p];
+      int rc = fbo_getMEs_2c(t, in1, in2, P_VNA_ONTOPL, 0, z, nmu, nnu, bm);
+      if (rc) return rc;
+      fbo_rotate(t, in1, in2, eps, nmu, nnu, bm, bx);
+      for (int i = 0; i < nb; i++) blk[i] = blk[i] + bx[i] * P_EQ2;
+      for (int isorp = 1; isorp <= t->nssh[in1 - 1]; isorp++) {
+        double dQ = s->dQ[(ia - 1) * FBO_MAXSH + isorp - 1];
+        rc = fbo_getMEs_2c(t, in1, in2, P_VNA_ONTOPL, isorp, z, nmu, nnu, bm);
+        if (rc) return rc;
+        fbo_rotate(t, in1, in2, eps, nmu, nnu, bm, bx);
+        for (int i = 0; i < nb; i++) blk[i] = blk[i] + dQ * bx[i] * P_EQ2;
+      }
+      rc = fbo_getMEs_2c(t, in1, in2, P_VNA_ONTOPR, 0, z, nmu, nnu, bm);
+      if (rc) return rc;
+      fbo_rotate(t, in1, in2, eps, nmu, nnu, bm, bx);
+      for (int i = 0; i < nb; i++) blk[i] = blk[i] + bx[i] * P_EQ2;
+      for (int isorp = 1; isorp <= t->nssh[in2 - 1]; isorp++) {
+        double dQ = s->dQ[(ja - 1) * FBO_MAXSH + isorp - 1];
+        rc = fbo_getMEs_2c(t, in1, in2, P_VNA_ONTOPR, isorp, z, nmu, nnu, bm);
+        if (rc) return rc;
+        fbo_rotate(t, in1, in2, eps, nmu, nnu, bm, bx);
+        for (int i = 0; i < nb; i++) blk[i] = blk[i] + dQ * bx[i] * P_EQ2;
+      }
+    }
+  }
+  /* atom terms into block(neigh_self) (:710-816) */
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1], nb = nmu * nmu;
+    int pself = s->noff[ia - 1] + s->neigh_self[ia - 1] - 1;
+    double *blk = vb + s->boff[pself];
+    const double *sself = s->mat[FBO_MAT_S] + s->boff[pself];
+    for (int in = 0; in < s->neighn[ia - 1]; in++) {
+      int p = s->noff[ia - 1] + in;
+      int mb = s->neigh_b[p], ja = s->neigh_j[p];
+      double r2[3];
+      for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+      int in2 = s->imass[ja - 1];
+      double z = pair_frame(s, r1, r2, eps);
+      double rend = rcut_min(t, in1) + rcut_min(t, in2);
+      double smooth = fbo_smoother(z, rend, 0.8);
+      for (int i = 0; i < nb; i++) emnpl[i] = 0.0;
+      if (z > 1.0e-4)
+        for (int i = 0; i < nb; i++) emnpl[i] = sself[i] / z;
+      int rc = fbo_getMEs_2c(t, in1, in2, P_VNA_ATOM, 0, z, nmu, nmu, bm);
+      if (rc) return rc;
+      fbo_rotate(t, in1, in1, eps, nmu, nmu, bm, bx);
+      for (int i = 0; i < nb; i++) blk[i] = blk[i] + bx[i] * P_EQ2;
+      for (int isorp = 1; isorp <= t->nssh[in2 - 1]; isorp++) {
+        double dQ = s->dQ[(ja - 1) * FBO_MAXSH + isorp - 1];
+        rc = fbo_getMEs_2c(t, in1, in2, P_VNA_ATOM, isorp, z, nmu, nmu, bm);
+        if (rc) return rc;
+        fbo_rotate(t, in1, in1, eps, nmu, nmu, bm, bx);
+        for (int i = 0; i < nb; i++)
+          blk[i] = blk[i] + P_EQ2 * dQ * ((1.0 - smooth) * emnpl[i] + smooth * bx[i]);
+      }
+    }
+  }
+  s->done[FBO_MAT_VNA] = s->done[FBO_MAT_VNAO] = 1;
+  return FBO_OK;
+}
+
+/* j.ASSEMBLERS/M_assemble_3c_DOGS.f90:86-340 (serial order: ialpha, then its common pairs) */
+int fbo_assemble_vna_3c(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  if (!s->neighn || !s->done[FBO_MAT_VNA] || !s->done[FBO_MAT_S] || !s->done[FBO_MAT_DIP]) return FBO_E_ORDER;
+  double bm[81], bx[81], eps[9], emnpl[81];
+  for (int ial = 1; ial <= s->natoms; ial++) {
+    int indna = s->imass[ial - 1];
+    const double *rna = RAT(s, ial);
+    for (int in = 0; in < s->ncommon[ial - 1]; in++) {
+      int c = s->coff[ial - 1] + in;
+      int mneigh = s->cm[c];
+      if (mneigh == 0) continue;
+      int ia = s->ci_j[c], ib = s->ci_b[c], ja = s->cj_j[c], jb = s->cj_b[c];
+      double r1[3], r2[3], r21[3], sighat[3], rnabc[3], rhat[3];
+      for (int k = 0; k < 3; k++) { r1[k] = RAT(s, ia)[k] + XL(s, ib)[k]; r2[k] = RAT(s, ja)[k] + XL(s, jb)[k]; }
+      int in1 = s->imass[ia - 1], in2 = s->imass[ja - 1];
+      int nmu = t->norb[in1 - 1], nnu = t->norb[in2 - 1], nb = nmu * nnu;
+      int p = s->noff[ia - 1] + mneigh - 1;
+      double *blk = s->mat[FBO_MAT_VNA] + s->boff[p];
+      const double *sb = s->mat[FBO_MAT_S] + s->boff[p];
+      const double *db = s->mat[FBO_MAT_DIP] + s->boff[p];
+      for (int k = 0; k < 3; k++) r21[k] = r2[k] - r1[k];
+      double z = fbo_distance(r1, r2);
+      if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+      else for (int k = 0; k < 3; k++) sighat[k] = r21[k] / z;
+      for (int k = 0; k < 3; k++) rnabc[k] = rna[k] - (r1[k] + r21[k] / 2.0);
+      double x = sqrt(rnabc[0] * rnabc[0] + rnabc[1] * rnabc[1] + rnabc[2] * rnabc[2]);
+      double d13 = fbo_distance(rna, r1), d23 = fbo_distance(rna, r2);
+      if (x < 1.0e-5) { rhat[0] = 0.0; rhat[1] = 0.0; rhat[2] = 0.0; }
+      else for (int k = 0; k < 3; k++) rhat[k] = rnabc[k] / x;
+      double cost = sighat[0] * rhat[0] + sighat[1] * rhat[1] + sighat[2] * rhat[2];
+      s->nfallback += epsilon_impl(rhat, sighat, eps);
+      double rc1 = rcut_min(t, in1), rc2 = rcut_min(t, in2), rc3 = rcut_min(t, indna);
+      double stinky = fbo_smoother(d13, rc1 + rc3, 0.8);
+      stinky = stinky * fbo_smoother(d23, rc2 + rc3, 0.8);
+      for (int i = 0; i < nb; i++) {
+        double sterm = sb[i] / 2.0, dterm = db[i] / z;
+        emnpl[i] = (sterm - dterm) / d13 + (sterm + dterm) / d23;
+      }
+      int rc = fbo_getMEs_3c(t, in1, in2, indna, P_BCNA, 0, x, z, nmu, nnu, cost, bm);
+      if (rc) return rc;
+      fbo_rotate(t, in1, in2, eps, nmu, nnu, bm, bx);
+      for (int i = 0; i < nb; i++) blk[i] = blk[i] + bx[i] * P_EQ2;
+      for (int isorp = 1; isorp <= t->nssh[indna - 1]; isorp++) {
+        rc = fbo_getMEs_3c(t, in1, in2, indna, P_BCNA, isorp, x, z, nmu, nnu, cost, bm);
+        if (rc) return rc;
+        fbo_rotate(t, in1, in2, eps, nmu, nnu, bm, bx);
+        double dQ = s->dQ[(ial - 1) * FBO_MAXSH + isorp - 1];
+        for (int i = 0; i < nb; i++)
+          blk[i] = blk[i] + dQ * (stinky * bx[i] + (1.0 - stinky) * emnpl[i]) * P_EQ2;
+      }
+    }
+  }
+  return FBO_OK;
+}

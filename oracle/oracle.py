@@ -1,0 +1,220 @@
+"""ctypes binding of the C oracle (oracle/libfboracle.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, the smoke check and the CPU-baseline leg of bench.py import this
+module; it is the checker, never the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfboracle.so")
+MAXSH = 4
+MATS = {"S": 0, "T": 1, "DIP": 2, "VNA": 3, "VNAO": 4}
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp, i64 = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int64
+        L.fbo_tables_new.restype = vp
+        L.fbo_tables_new.argtypes = [C.c_int, ip, ip, dp]
+        L.fbo_tables_free.argtypes = [vp]
+        L.fbo_add_cell2c.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, ip, ip, ip, dp]
+        L.fbo_add_cell3c.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, C.c_double, C.c_double, ip, ip, ip, dp]
+        L.fbo_distance.restype = C.c_double
+        L.fbo_distance.argtypes = [dp, dp]
+        L.fbo_smoother.restype = C.c_double
+        L.fbo_smoother.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.fbo_epsilon.argtypes = [dp, dp, dp]
+        L.fbo_twister.argtypes = [dp, C.c_int, dp, dp]
+        L.fbo_rotate.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp]
+        L.fbo_getMEs_2c.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, dp]
+        L.fbo_getMEs_3c.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                    C.c_int, C.c_int, C.c_double, dp]
+        L.fbo_make_cells.argtypes = [dp, C.c_int, dp]
+        L.fbo_system_new.restype = vp
+        L.fbo_system_new.argtypes = [vp, C.c_int, dp, ip, dp, C.c_int, dp]
+        L.fbo_system_free.argtypes = [vp]
+        L.fbo_build_lists.argtypes = [vp]
+        for f in ("fbo_npairs", "fbo_ntriplets", "fbo_block_len"):
+            getattr(L, f).restype = i64
+            getattr(L, f).argtypes = [vp]
+        L.fbo_get_lists.argtypes = [vp] + [ip] * 11
+        for f in ("fbo_assemble_S", "fbo_assemble_T", "fbo_assemble_dipole_z", "fbo_assemble_vna_2c",
+                  "fbo_assemble_vna_3c", "fbo_frame_fallbacks"):
+            getattr(L, f).argtypes = [vp]
+        L.fbo_get_blocks.argtypes = [vp, C.c_int, dp]
+        _lib = L
+    return _lib
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise OracleError(f"{what} failed with status {rc}")
+
+
+class Tables:
+    def __init__(self, ts):
+        L = lib()
+        self.ts = ts
+        ns = len(ts.species)
+        nssh = np.array([sp.nssh for sp in ts.species], np.int32)
+        lssh = np.zeros((ns, MAXSH), np.int32)
+        rc = np.zeros((ns, MAXSH), np.float64)
+        for i, sp in enumerate(ts.species):
+            lssh[i, :sp.nssh] = sp.lssh
+            rc[i, :sp.nssh] = sp.rcutoffA
+        self.h = L.fbo_tables_new(ns, _ip(nssh), _ip(lssh), _dp(rc))
+        keep = []
+        for (a, b, it, isub), c in ts.cells2c.items():
+            arrs = [np.ascontiguousarray(x, np.int32) for x in (c.mu, c.nu, c.mvalue)]
+            d = np.ascontiguousarray(c.data, np.float64)
+            keep.append((arrs, d))
+            _check(L.fbo_add_cell2c(self.h, a, b, it, isub, c.nME, c.nx, c.xmax, *(_ip(x) for x in arrs), _dp(d)),
+                   "add_cell2c")
+        for (a, b, cc, it, isub, ith), c in ts.cells3c.items():
+            arrs = [np.ascontiguousarray(x, np.int32) for x in (c.mu, c.nu, c.mvalue)]
+            d = np.ascontiguousarray(c.data, np.float64)
+            _check(L.fbo_add_cell3c(self.h, a, b, cc, it, isub, ith, c.nME, c.nx, c.ny, c.xmax, c.ymax,
+                                    *(_ip(x) for x in arrs), _dp(d)), "add_cell3c")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().fbo_tables_free(self.h)
+            self.h = None
+
+    def getMEs_2c(self, is_, js, iint, isub, x, nmu, nnu):
+        out = np.zeros(nmu * nnu)
+        _check(lib().fbo_getMEs_2c(self.h, is_, js, iint, isub, x, nmu, nnu, _dp(out)), "getMEs_2c")
+        return out.reshape(nnu, nmu).T
+
+    def getMEs_3c(self, is_, js, ks, iint, isub, x, y, nmu, nnu, cost):
+        out = np.zeros(nmu * nnu)
+        _check(lib().fbo_getMEs_3c(self.h, is_, js, ks, iint, isub, x, y, nmu, nnu, cost, _dp(out)), "getMEs_3c")
+        return out.reshape(nnu, nmu).T
+
+    def rotate(self, in1, in3, eps, M):
+        nmu, nnu = M.shape
+        Mf = np.asfortranarray(M, np.float64)
+        epsf = np.asfortranarray(eps, np.float64)
+        out = np.zeros(nmu * nnu)
+        lib().fbo_rotate(self.h, in1, in3, _dp(epsf.ravel(order="F")), nmu, nnu, _dp(Mf.ravel(order="F")), _dp(out))
+        return out.reshape(nnu, nmu).T
+
+
+def epsilon(r1, r2):
+    spe = np.zeros(9)
+    lib().fbo_epsilon(_dp(np.ascontiguousarray(r1, np.float64)), _dp(np.ascontiguousarray(r2, np.float64)), _dp(spe))
+    return spe.reshape(3, 3).T
+
+
+def twister(eps, have_d=True):
+    dmat = np.zeros(25)
+    pmat = np.zeros(9)
+    e = np.asfortranarray(eps, np.float64).ravel(order="F")
+    lib().fbo_twister(_dp(e), int(have_d), _dp(dmat), _dp(pmat))
+    return dmat.reshape(5, 5).T, pmat.reshape(3, 3).T
+
+
+def smoother(r, rend, x):
+    return lib().fbo_smoother(r, rend, x)
+
+
+def make_cells(lattice, icluster=0):
+    xl = np.zeros((729, 3))
+    lat = np.ascontiguousarray(lattice, np.float64)
+    n = lib().fbo_make_cells(_dp(lat), icluster, _dp(xl))
+    return xl[: n + 1].copy()
+
+
+class System:
+    """Oracle run of the full path on one structure."""
+
+    def __init__(self, tables: Tables, st):
+        L = lib()
+        self.tables = tables
+        self.st = st
+        ratom = np.ascontiguousarray(st.ratom, np.float64)
+        imass = np.ascontiguousarray(st.imass, np.int32)
+        lat = np.ascontiguousarray(st.lattice, np.float64)
+        dq = None if st.dQ is None else np.ascontiguousarray(st.dQ[:, :MAXSH], np.float64)
+        self.h = L.fbo_system_new(tables.h, st.natoms, _dp(ratom), _ip(imass), _dp(lat), st.icluster,
+                                  _dp(dq) if dq is not None else None)
+        if not self.h:
+            raise OracleError("fbo_system_new failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().fbo_system_free(self.h)
+            self.h = None
+
+    def build_lists(self):
+        _check(lib().fbo_build_lists(self.h), "build_lists")
+        L = lib()
+        n = self.st.natoms
+        npairs, ntrip = L.fbo_npairs(self.h), L.fbo_ntriplets(self.h)
+        out = {k: np.zeros(n, np.int32) for k in ("neighn", "neigh_self", "ncommon")}
+        for k in ("neigh_j", "neigh_b", "neigh_back"):
+            out[k] = np.zeros(max(npairs, 1), np.int32)
+        for k in ("com_iatom_j", "com_iatom_b", "com_jatom_j", "com_jatom_b", "com_mneigh"):
+            out[k] = np.zeros(max(ntrip, 1), np.int32)
+        _check(L.fbo_get_lists(self.h, _ip(out["neighn"]), _ip(out["neigh_j"]), _ip(out["neigh_b"]),
+                               _ip(out["neigh_back"]), _ip(out["neigh_self"]), _ip(out["ncommon"]),
+                               _ip(out["com_iatom_j"]), _ip(out["com_iatom_b"]), _ip(out["com_jatom_j"]),
+                               _ip(out["com_jatom_b"]), _ip(out["com_mneigh"])), "get_lists")
+        for k in ("neigh_j", "neigh_b", "neigh_back"):
+            out[k] = out[k][:npairs]
+        for k in ("com_iatom_j", "com_iatom_b", "com_jatom_j", "com_jatom_b", "com_mneigh"):
+            out[k] = out[k][:ntrip]
+        out["npairs"], out["ntriplets"] = int(npairs), int(ntrip)
+        self.lists = out
+        return out
+
+    def assemble(self, which=("S", "T", "DIP", "VNA2C", "VNA3C")):
+        L = lib()
+        fn = {"S": L.fbo_assemble_S, "T": L.fbo_assemble_T, "DIP": L.fbo_assemble_dipole_z,
+              "VNA2C": L.fbo_assemble_vna_2c, "VNA3C": L.fbo_assemble_vna_3c}
+        for w in which:
+            _check(fn[w](self.h), "assemble_" + w)
+
+    def blocks(self, mat):
+        n = lib().fbo_block_len(self.h)
+        out = np.zeros(max(n, 1))
+        _check(lib().fbo_get_blocks(self.h, MATS[mat], _dp(out)), "get_blocks")
+        return out[:n]
+
+    def run_all(self):
+        lists = self.build_lists()
+        self.assemble()
+        res = dict(lists)
+        for m in MATS:
+            res[m] = self.blocks(m)
+        res["frame_fallbacks"] = lib().fbo_frame_fallbacks(self.h)
+        return res

@@ -1,0 +1,105 @@
+"""Run the reference harness (oracle/_ref/fb_harness_ref) and read its output.
+
+Test infrastructure only: used by tests/, tests/golden/make_golden.py and the
+cpu_baseline leg of bench.py.  The harness binary is compiled from the
+reference's Fortran sources by ``make -C oracle ref`` (see oracle/Makefile).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(REPO, "thunder-master_amd"))
+
+from fireball_hip import fdata_io  # noqa: E402
+
+HARNESS_REF = os.path.join(HERE, "_ref", "fb_harness_ref")
+
+
+def available(binary=HARNESS_REF):
+    return os.path.isfile(binary) and os.access(binary, os.X_OK)
+
+
+class _Reader:
+    def __init__(self, buf):
+        self.buf = buf
+        self.pos = 0
+
+    def i32(self, n):
+        a = np.frombuffer(self.buf, np.int32, n, self.pos)
+        self.pos += 4 * n
+        return a.copy()
+
+    def f64(self, n):
+        a = np.frombuffer(self.buf, np.float64, n, self.pos)
+        self.pos += 8 * n
+        return a.copy()
+
+
+def parse_output(path, species, imass):
+    """Return a dict with flat CSR lists (0-based atom indices kept 1-based as in
+    the reference) and per-matrix flat block arrays in pair order."""
+    with open(path, "rb") as f:
+        r = _Reader(f.read())
+    natoms, mbeta_max, npairs, ncommon_tot, nrep = r.i32(5)
+    times = r.f64(6)
+    neighn = r.i32(natoms)
+    neigh_j, neigh_b, neigh_back = [], [], []
+    for ia in range(natoms):
+        n = int(neighn[ia])
+        neigh_j.append(r.i32(n)); neigh_b.append(r.i32(n)); neigh_back.append(r.i32(n))
+    neigh_self = r.i32(natoms)
+    ncommon = r.i32(natoms)
+    com = {k: [] for k in ("iatom_j", "iatom_b", "jatom_j", "jatom_b", "mneigh")}
+    for ia in range(natoms):
+        n = int(ncommon[ia])
+        for k in ("iatom_j", "iatom_b", "jatom_j", "jatom_b", "mneigh"):
+            com[k].append(r.i32(n))
+    cat = lambda xs: np.concatenate(xs) if xs else np.zeros(0, np.int32)  # noqa: E731
+    neigh_j, neigh_b, neigh_back = cat(neigh_j), cat(neigh_b), cat(neigh_back)
+    norb = np.array([sp.norb for sp in species], np.int64)
+    mats = {k: [] for k in ("S", "T", "DIP", "VNA", "VNAO")}
+    p = 0
+    for ia in range(natoms):
+        nmu = norb[imass[ia] - 1]
+        for _ in range(int(neighn[ia])):
+            nnu = norb[imass[neigh_j[p] - 1] - 1]
+            for k in ("S", "T", "DIP", "VNA", "VNAO"):
+                mats[k].append(r.f64(int(nmu * nnu)))
+            p += 1
+    assert r.pos == len(r.buf), "trailing bytes in harness output"
+    out = dict(natoms=int(natoms), mbeta_max=int(mbeta_max), npairs=int(npairs),
+               ntriplets=int(ncommon_tot), nrep=int(nrep),
+               times=dict(zip(("lists", "S", "T", "DIP", "VNA2C", "VNA3C"), times)),
+               neighn=neighn, neigh_j=neigh_j, neigh_b=neigh_b, neigh_back=neigh_back,
+               neigh_self=neigh_self, ncommon=ncommon,
+               **{"com_" + k: cat(v) for k, v in com.items()})
+    for k, v in mats.items():
+        out[k] = np.concatenate(v) if v else np.zeros(0)
+    return out
+
+
+def run(ts, structure, binary=HARNESS_REF, nrep=1, workdir=None, timeout=3600):
+    """Write tables + structure, run the harness, return the parsed output."""
+    own = workdir is None
+    tmp = tempfile.mkdtemp(prefix="fbref_") if own else workdir
+    fdir = os.path.join(tmp, "Fdata")
+    if not os.path.isdir(fdir):
+        fdata_io.write_fdata(ts, fdir)
+    inp = os.path.join(tmp, "harness.in")
+    fdata_io.write_harness_input(inp, fdir, ts, structure)
+    outp = os.path.join(tmp, "harness.out")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run([binary, inp, outp, str(nrep)], cwd=tmp, check=True, env=env,
+                   stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=timeout)
+    res = parse_output(outp, ts.species, structure.imass)
+    if own:
+        import shutil
+        shutil.rmtree(tmp, ignore_errors=True)
+    return res

@@ -1,0 +1,108 @@
+"""Write synthetic tables and structures in the reference's on-disk formats.
+
+Fdata (read by e.FDATA/M_Fdata_2c.f90:149-272 and e.FDATA/M_Fdata_3c.f90:143-264):
+
+* ``2c.Z1.Z2.dat``   number of 2c cells of the ordered species pair
+* ``2c.Z1.Z2.dir``   one record per cell: ``iindex itype isubtype fname nME nx xmax``
+* ``<fname>``        nx lines of nME values (grid point major)
+* ``TT_munu_2c.Z1.Z2.dat``  mu / nu / mvalue lines, one triple per cell of
+  interaction TT in directory order (the reader keeps the unit connected across
+  cells of one interaction, so the triples are consumed in sequence)
+* ``clPP.ZZ.dat``    Kleinman-Bylander coefficients (empty: no PP shells)
+* ``3c.Z1.Z2.Z3.dat`` / ``.dir`` with ``iindex itype isubtype itheta fname nME nx xmax ny ymax``;
+  data files hold ny*nx lines (iy outer, ix inner) of nME values.
+
+Values are written with 17 significant digits so that they read back to the
+same binary64 numbers.
+
+The structure file read by the oracle's Fortran harness is this project's own
+format (``write_harness_input``).
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from .synth import TableSet, Structure
+
+
+def _fmt(v):
+    return repr(float(v))
+
+
+def _write_rows(path, rows):
+    with open(path, "w") as f:
+        for row in rows:
+            f.write(" ".join(_fmt(v) for v in row))
+            f.write("\n")
+
+
+def write_fdata(ts: TableSet, outdir: str):
+    os.makedirs(outdir, exist_ok=True)
+    sp = ts.species
+    ns = len(sp)
+    for a in range(1, ns + 1):
+        za = sp[a - 1].nZ
+        open(os.path.join(outdir, f"clPP.{za:02d}.dat"), "w").close()
+        for b in range(1, ns + 1):
+            zb = sp[b - 1].nZ
+            keys = sorted(k for k in ts.cells2c if k[0] == a and k[1] == b)
+            keys.sort(key=lambda k: (k[2], k[3]))
+            with open(os.path.join(outdir, f"2c.{za:02d}.{zb:02d}.dat"), "w") as f:
+                f.write(f"{len(keys)}\n")
+            munu = {}
+            with open(os.path.join(outdir, f"2c.{za:02d}.{zb:02d}.dir"), "w") as fd:
+                for idx, k in enumerate(keys, start=1):
+                    c = ts.cells2c[k]
+                    fname = f"t2_{k[2]:02d}_{k[3]}_{za:02d}_{zb:02d}.dat"
+                    fd.write(f"{idx} {k[2]} {k[3]} {fname} {c.nME} {c.nx} {_fmt(c.xmax)}\n")
+                    _write_rows(os.path.join(outdir, fname), c.data)
+                    munu.setdefault(k[2], []).append(c)
+            for itype, cells in munu.items():
+                with open(os.path.join(outdir, f"{itype:02d}_munu_2c.{za:02d}.{zb:02d}.dat"), "w") as f:
+                    for c in cells:
+                        for arr in (c.mu, c.nu, c.mvalue):
+                            f.write(" ".join(str(int(v)) for v in arr) + "\n")
+            for cc in range(1, ns + 1):
+                zc = sp[cc - 1].nZ
+                keys3 = sorted(k for k in ts.cells3c if k[:3] == (a, b, cc))
+                keys3.sort(key=lambda k: (k[3], k[4], k[5]))
+                if not keys3:
+                    continue
+                with open(os.path.join(outdir, f"3c.{za:02d}.{zb:02d}.{zc:02d}.dat"), "w") as f:
+                    f.write(f"{len(keys3)}\n")
+                munu3 = {}
+                with open(os.path.join(outdir, f"3c.{za:02d}.{zb:02d}.{zc:02d}.dir"), "w") as fd:
+                    for idx, k in enumerate(keys3, start=1):
+                        c = ts.cells3c[k]
+                        fname = f"t3_{k[3]:02d}_{k[4]}_{k[5]}_{za:02d}_{zb:02d}_{zc:02d}.dat"
+                        fd.write(f"{idx} {k[3]} {k[4]} {k[5]} {fname} {c.nME} {c.nx} "
+                                 f"{_fmt(c.xmax)} {c.ny} {_fmt(c.ymax)}\n")
+                        _write_rows(os.path.join(outdir, fname), c.data.reshape(c.ny * c.nx, c.nME))
+                        munu3.setdefault(k[3], []).append(c)
+                for itype, cells in munu3.items():
+                    with open(os.path.join(outdir, f"{itype:02d}_munu_3c.{za:02d}.{zb:02d}.{zc:02d}.dat"), "w") as f:
+                        for c in cells:
+                            for arr in (c.mu, c.nu, c.mvalue):
+                                f.write(" ".join(str(int(v)) for v in arr) + "\n")
+
+
+def write_harness_input(path: str, fdata_dir: str, ts: TableSet, s: Structure):
+    """Input for the oracle's Fortran harness: species, lattice, atoms, shell charges."""
+    with open(path, "w") as f:
+        f.write(f"'{fdata_dir}'\n")
+        f.write(f"{len(ts.species)}\n")
+        for sp in ts.species:
+            f.write(f"{sp.nZ} {sp.nssh}\n")
+            f.write(" ".join(str(l) for l in sp.lssh) + "\n")
+            f.write(" ".join(_fmt(r) for r in sp.rcutoff) + "\n")
+        f.write(f"{s.natoms} {s.icluster}\n")
+        for i in range(3):
+            f.write(" ".join(_fmt(v) for v in s.lattice[i]) + "\n")
+        for ia in range(s.natoms):
+            f.write(f"{int(s.imass[ia])} " + " ".join(_fmt(v) for v in s.ratom[ia]) + "\n")
+        for ia in range(s.natoms):
+            nssh = ts.species[s.imass[ia] - 1].nssh
+            dq = np.zeros(nssh) if s.dQ is None else s.dQ[ia, :nssh]
+            f.write(" ".join(_fmt(v) for v in dq) + "\n")
