@@ -1,0 +1,125 @@
+"""Generate the golden fixtures from the reference's own Fortran (run in the
+build container, where the reference sources are present).
+
+    make -C oracle ref            # compile the reference modules + harness
+    python tests/golden/make_golden.py
+
+Each fixture is an .npz holding the inputs that are not re-derivable from a
+seed (explicit positions for the hand-made cases), the reference's neighbour
+and common-neighbour lists, its S/T/dipole_z/Vna blocks, and a digest of the
+synthetic tables so that a change of the table generator is detected.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+sys.path.insert(0, os.path.join(REPO, "thunder-master_amd"))
+
+import refrun  # noqa: E402
+from fireball_hip import synth  # noqa: E402
+
+LIST_KEYS = ("neighn", "neigh_j", "neigh_b", "neigh_back", "neigh_self", "ncommon", "com_iatom_j",
+             "com_iatom_b", "com_jatom_j", "com_jatom_b", "com_mneigh")
+MATS = ("S", "T", "DIP", "VNA", "VNAO")
+
+
+def tables_digest(ts):
+    h = hashlib.sha256()
+    for k in sorted(ts.cells2c):
+        c = ts.cells2c[k]
+        h.update(np.array(k, np.int64).tobytes())
+        h.update(np.ascontiguousarray(c.data, np.float64).tobytes())
+        h.update(np.ascontiguousarray(c.mu, np.int32).tobytes())
+        h.update(np.ascontiguousarray(c.nu, np.int32).tobytes())
+    for k in sorted(ts.cells3c):
+        c = ts.cells3c[k]
+        h.update(np.array(k, np.int64).tobytes())
+        h.update(np.ascontiguousarray(c.data, np.float64).tobytes())
+        h.update(np.ascontiguousarray(c.mvalue, np.int32).tobytes())
+    return h.hexdigest()
+
+
+def chain_cluster():
+    """Cluster with atoms on coordinate axes through the origin and a third atom
+    at an exact bond centre: exercises every epsilon_function fallback branch
+    (b.FUNCTIONS/epsilon.f90:123-168) and the x < 1e-5 branch of the 3c frame
+    (j.ASSEMBLERS/M_assemble_3c_DOGS.f90:222-228)."""
+    pos = [
+        [0.0, 0.0, 1.0], [0.0, 0.0, 3.3], [0.0, 0.0, 5.6],      # z chain, 3.3 is the bond centre of 1.0/5.6
+        [1.2, 0.0, 0.0], [3.4, 0.0, 0.0],                       # x axis
+        [0.0, -1.5, 0.0], [0.0, -3.7, 0.0],                     # y axis
+        [1.7, 1.9, 2.3], [-1.1, 2.6, 3.9],                      # generic
+    ]
+    imass = [1, 2, 1, 2, 1, 3, 2, 1, 3]
+    ratom = np.array(pos, np.float64)
+    return synth.Structure(ratom, np.array(imass, np.int32), np.diag([30.0, 30.0, 30.0]), icluster=1,
+                           name="chain")
+
+
+def cases():
+    si = [synth.SI]
+    ts_si = synth.make_tables(si, seed=2024)
+    ts_si.seed = 2024
+    out = []
+    s0 = synth.config(0)[1][0]
+    out.append(("si64", ts_si, s0, 8))                      # config 0, blocks of the first 8 atoms kept
+    s0q = synth.config(0)[1][0]
+    synth.random_dQ([s0q], si, seed=7)
+    out.append(("si64_dq", ts_si, s0q, 8))
+    s1 = synth.diamond_si((1, 1, 1), sigma=0.05, seed=5)    # L = 5.43 A < range: images beyond +-1
+    out.append(("si8_cell", ts_si, s1, None))
+    sp3 = [synth.C_SP, synth.H_SP, synth.O_SP]
+    ts3 = synth.make_tables(sp3, seed=99)
+    ts3.seed = 99
+    box = synth.random_box({6: 12, 1: 20, 8: 8}, 1.0, 0.9, 11, sp3)
+    synth.random_dQ([box], sp3, seed=12)
+    out.append(("cho40_dq", ts3, box, None))
+    ch = chain_cluster()
+    out.append(("chain_cluster", ts3, ch, None))
+    return out
+
+
+def main():
+    if not refrun.available():
+        sys.exit("reference harness not built: run `make -C oracle ref` first")
+    for name, ts, st, keep_atoms in cases():
+        r = refrun.run(ts, st)
+        d = {k: np.asarray(r[k], np.int32) for k in LIST_KEYS}
+        d["natoms"] = np.int32(st.natoms)
+        d["mbeta_max"] = np.int32(r["mbeta_max"])
+        d["ratom"] = st.ratom
+        d["imass"] = st.imass
+        d["lattice"] = st.lattice
+        d["icluster"] = np.int32(st.icluster)
+        d["dQ"] = st.dQ if st.dQ is not None else np.zeros((st.natoms, 4))
+        d["tables_digest"] = np.array(tables_digest(ts))
+        d["table_seed"] = np.int64(ts.seed)
+        d["species_z"] = np.array([sp.nZ for sp in ts.species], np.int32)
+        norb = np.array([sp.norb for sp in ts.species])
+        sizes = np.array([norb[st.imass[ia] - 1] * norb[st.imass[j - 1] - 1]
+                          for ia in range(st.natoms) for j in r["neigh_j"][sum(r["neighn"][:ia]):sum(r["neighn"][:ia + 1])]])
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        if keep_atoms is None:
+            npair_keep = len(sizes)
+        else:
+            npair_keep = int(np.sum(r["neighn"][:keep_atoms]))
+        d["npair_blocks"] = np.int32(npair_keep)
+        for m in MATS:
+            d[m] = r[m][: offs[npair_keep]]
+            d[m + "_sum"] = np.float64(np.sum(r[m]))
+            d[m + "_sumsq"] = np.float64(np.sum(r[m] ** 2))
+            d[m + "_sumabs"] = np.float64(np.sum(np.abs(r[m])))
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **d)
+        print(f"{name}: natoms={st.natoms} pairs={r['npairs']} triplets={r['ntriplets']} "
+              f"blocks kept for {npair_keep} pairs")
+
+
+if __name__ == "__main__":
+    main()
