@@ -1,0 +1,142 @@
+/* fireball_hip.h -- C ABI of the MI355X (gfx950) two- and three-centre
+ * Hamiltonian / overlap assembler.
+ *
+ * The entry points replace, one for one, the reference's Fortran routines
+ * listed next to each declaration (paths relative to the reference's src/
+ * tree).  Arguments are plain pointers and sizes; arrays that mirror a
+ * reference array use the reference's conventions:
+ *   - atom, species, neighbour and matrix-element indices are 1-based;
+ *   - image indices (neigh_b, *_common_b) run 0..mbeta_max in make_cells order;
+ *   - a block is column-major norb_mu x norb_nu, blocks are packed in pair order
+ *     (iatom major, ineigh minor), exactly T_assemble_block%block per
+ *     (iatom, ineigh) of a.GLOBAL/M_assemble_blocks.f90:44-64;
+ *   - table data arrive in Fortran order, Fdata_2c(ix, iME) and
+ *     Fdata_3c(ix, iY, iME) (e.FDATA/M_Fdata_2c.f90:92, M_Fdata_3c.f90:89).
+ * Every function returns FB_OK (0) or a negative FB_E_* status instead of the
+ * reference's `stop`; fb_last_error() gives the message.  A context is
+ * single-threaded; work is ordered on the context's HIP stream.
+ */
+#ifndef FIREBALL_HIP_H
+#define FIREBALL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FB_MAXSH 4          /* shells per species */
+#define FB_MAXSPECIES 8
+
+enum fb_status {
+  FB_OK = 0,
+  FB_E_ARG = -1,
+  FB_E_X_NEGATIVE = -2,       /* e.FDATA/M_Fdata_2c.f90:347-348 */
+  FB_E_X_OUT_OF_RANGE = -3,   /* e.FDATA/M_Fdata_2c.f90:349-353 */
+  FB_E_ATOMS_TOO_CLOSE = -4,  /* c.SYSTEM/M_neighbors.f90:421-430 */
+  FB_E_NEIGH_SELF = -5,       /* c.SYSTEM/M_neighbors.f90:462-465, 509-515 */
+  FB_E_NEIGH_BACK = -6,       /* c.SYSTEM/M_neighbors.f90:478-506 */
+  FB_E_NO_TABLE = -7,         /* interaction cell absent from Fdata */
+  FB_E_ORDER = -8,            /* assemble_* called before its inputs exist */
+  FB_E_HIP = -9,              /* HIP runtime error */
+  FB_E_CAPACITY = -10,        /* a list exceeds a kernel limit */
+  FB_E_UNSUPPORTED = -11      /* e.g. a shell with l > 2 */
+};
+
+enum fb_matrix {            /* which block array */
+  FB_MAT_S = 0,             /* s%overlap(:)%neighbors(:)%block */
+  FB_MAT_T = 1,             /* s%kinetic(:)%neighbors(:)%block */
+  FB_MAT_DIP = 2,           /* s%dipole_z(:)%neighbors(:)%block */
+  FB_MAT_VNA = 3,           /* s%vna(:)%neighbors(:)%block  (atom term on neigh_self, 3c elsewhere) */
+  FB_MAT_VNAO = 4           /* s%vna(:)%neighbors(:)%blocko (ontop terms) */
+};
+
+enum fb_precision {
+  FB_PREC_FP64 = 0,         /* plain binary64 kernels (parity anchor) */
+  FB_PREC_MIXED = 1         /* FP32 interpolation/rotation, FP64 geometry + accumulation + blocks */
+};
+
+enum fb_method3c {
+  FB_3C_TABLE = 0,          /* bivariate table interpolation (reference method) */
+  FB_3C_MLP = 1             /* learned surrogate with table fallback */
+};
+
+typedef struct fb_ctx fb_ctx;
+
+/* --- context ------------------------------------------------------------ */
+int fb_create(fb_ctx **ctx, int device);
+int fb_destroy(fb_ctx *ctx);
+const char *fb_last_error(const fb_ctx *ctx);
+int fb_set_stream(fb_ctx *ctx, void *hip_stream);      /* NULL: the context's own stream */
+int fb_synchronize(fb_ctx *ctx);
+int fb_set_precision(fb_ctx *ctx, int mode);          /* enum fb_precision */
+int fb_set_3c_method(fb_ctx *ctx, int method);        /* enum fb_method3c */
+
+/* --- species: read_info, c.SYSTEM/M_species.f90:1057-1251 ----------------
+ * lssh and rcutoffA are [nspecies][FB_MAXSH]; rcutoffA in Angstrom (the
+ * reference's shell(issh)%rcutoffA). */
+int fb_set_species(fb_ctx *ctx, int nspecies, const int *nZ, const int *nssh, const int *lssh,
+                   const double *rcutoffA);
+
+/* --- tables --------------------------------------------------------------
+ * fb_set_table_2c: one T_Fdata_cell_2c of Fdata_bundle_2c(is, js), as filled by
+ *   read_Fdata_2c, e.FDATA/M_Fdata_2c.f90:149-272 (index_2c(itype, isub)).
+ * fb_set_table_3c: one T_Fdata_cell_3c of Fdata_bundle_3c(is, js, ks), as filled
+ *   by read_Fdata_3c, e.FDATA/M_Fdata_3c.f90:143-264 (index_3c(itype, isub, itheta)).
+ * fb_commit_tables packs all cells into device slabs; call it once after the
+ * last fb_set_table_* and before any assemble. */
+int fb_set_table_2c(fb_ctx *ctx, int is, int js, int itype, int isub, int nME, int nx, double xmax,
+                    const int *mu, const int *nu, const int *mvalue, const double *fdata);
+int fb_set_table_3c(fb_ctx *ctx, int is, int js, int ks, int itype, int isub, int itheta, int nME,
+                    int nx, double xmax, int ny, double ymax, const int *mu, const int *nu,
+                    const int *mvalue, const double *fdata);
+int fb_commit_tables(fb_ctx *ctx);
+
+/* --- structures -----------------------------------------------------------
+ * One call per structure (T_structure after read_positions / make_cells /
+ * read_charges, c.SYSTEM/M_configuraciones.f90:604-922, make_cells.f90:51-157).
+ * ratom [natoms][3] in Angstrom, already origin-shifted as read_positions
+ * leaves them; imass 1-based species; lattice rows a1, a2, a3; dQ
+ * [natoms][FB_MAXSH] shell charges (NULL: neutral).  Several structures form
+ * one batch: lists and assembly then cover all of them in single launches. */
+int fb_clear_structures(fb_ctx *ctx);
+int fb_add_structure(fb_ctx *ctx, int natoms, const double *ratom, const int *imass,
+                     const double *lattice, int icluster, const double *dQ, int *istruct);
+int fb_set_positions(fb_ctx *ctx, int istruct, const double *ratom);   /* new geometry, same atoms */
+int fb_set_charges(fb_ctx *ctx, int istruct, const double *dQ);
+
+/* --- lists: driver_neighbors, c.SYSTEM/M_neighbors.f90:93-141 ------------
+ * (find_neigh_max/find_neighbors :246-519, find_common_max/
+ * find_common_neighbors :548-812), for every structure of the batch. */
+int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntriplets_total);
+int fb_get_list_sizes(fb_ctx *ctx, int istruct, int64_t *npairs, int64_t *ntriplets, int *mbeta_max);
+int fb_get_neighbors(fb_ctx *ctx, int istruct, int *neighn, int *neigh_j, int *neigh_b,
+                     int *neigh_back, int *neigh_self);
+int fb_get_common(fb_ctx *ctx, int istruct, int *ncommon, int *iatom_common_j, int *iatom_common_b,
+                  int *jatom_common_j, int *jatom_common_b, int *neigh_common);
+
+/* --- assembly ---------------------------------------------------------- */
+int fb_assemble_S(fb_ctx *ctx);          /* j.ASSEMBLERS/M_assemble_2c_DOGS.f90:87-209   */
+int fb_assemble_T(fb_ctx *ctx);          /* j.ASSEMBLERS/M_assemble_2c_DOGS.f90:232-353  */
+int fb_assemble_dipole_z(fb_ctx *ctx);   /* j.ASSEMBLERS/M_assemble_2c_DOGS.f90:376-499  */
+int fb_assemble_vna_2c(fb_ctx *ctx);     /* j.ASSEMBLERS/M_assemble_2c_DOGS.f90:523-829  */
+int fb_assemble_vna_3c(fb_ctx *ctx);     /* j.ASSEMBLERS/M_assemble_3c_DOGS.f90:86-340   */
+/* S, T, dipole_z and vna_2c in one 2c launch, then vna_3c: the H+S build. */
+int fb_assemble_all(fb_ctx *ctx);
+
+/* --- results ------------------------------------------------------------ */
+int fb_get_block_len(fb_ctx *ctx, int istruct, int64_t *len);
+int fb_get_blocks(fb_ctx *ctx, int istruct, int matrix, double *out);
+int fb_get_device_blocks(fb_ctx *ctx, int matrix, void **dptr, int64_t *len);
+
+/* --- measurement -------------------------------------------------------
+ * With timing on, every kernel launch of the library is bracketed by HIP
+ * events on the launch stream; fb_get_timings returns the last call's
+ * kernels (name, ms), at most max entries; names is max*32 chars. */
+int fb_enable_timing(fb_ctx *ctx, int on);
+int fb_get_timings(fb_ctx *ctx, int max, char *names, float *ms, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,138 @@
+"""GPU parity: the HIP assembler through its C ABI against the golden vectors
+of the reference Fortran and against the oracle.
+
+Lists must be bit-identical; binary64 blocks agree to 1e-12 eV (the values
+are O(10) eV, so this is ~1e-13 relative: rounding only)."""
+import numpy as np
+import pytest
+
+import golden_cases
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+LIST_KEYS = ("neighn", "neigh_j", "neigh_b", "neigh_back", "neigh_self", "ncommon", "com_iatom_j",
+             "com_iatom_b", "com_jatom_j", "com_jatom_b", "com_mneigh")
+MATS = ("S", "T", "DIP", "VNA", "VNAO")
+TOL64 = 1e-12
+
+
+def _gpu(ts, structs, precision="fp64"):
+    from fireball_hip.lib import Assembler
+    asm = Assembler(ts, precision=precision)
+    for st in structs:
+        asm.add_structure(st)
+    asm.build_lists()
+    asm.assemble_all()
+    return asm
+
+
+@pytest.mark.parametrize("name", golden_cases.NAMES)
+def test_golden_fp64(golden, name):
+    print(f"case {name}", flush=True)
+    g = golden(name)
+    ts, st = golden_cases.inputs(g)
+    asm = _gpu(ts, [st])
+    r = asm.run_all(0)
+    for k in LIST_KEYS:
+        assert np.array_equal(r[k], g[k]), k
+    n = len(g["S"])
+    o = oracle.System(oracle.Tables(ts), st).run_all()
+    for m in MATS:
+        np.testing.assert_allclose(r[m][:n], g[m], rtol=0, atol=TOL64, err_msg=m)
+        np.testing.assert_allclose(r[m], o[m], rtol=0, atol=TOL64, err_msg=m)
+
+
+def test_separate_entry_points_match_fused(golden):
+    g = golden("cho40_dq")
+    ts, st = golden_cases.inputs(g)
+    from fireball_hip.lib import Assembler
+    asm = Assembler(ts)
+    asm.add_structure(st)
+    asm.build_lists()
+    asm.assemble_S()
+    asm.assemble_T()
+    asm.assemble_dipole_z()
+    asm.assemble_vna_2c()
+    asm.assemble_vna_3c()
+    sep = {m: asm.blocks(m) for m in MATS}
+    asm.assemble_all()
+    for m in MATS:
+        np.testing.assert_allclose(asm.blocks(m), sep[m], rtol=0, atol=TOL64, err_msg=m)
+
+
+def test_batch_of_structures_matches_one_by_one():
+    from fireball_hip import synth
+    sp = [synth.SI]
+    ts = synth.make_tables(sp)
+    structs = [synth.diamond_si(c, sigma=0.07, seed=k) for k, c in enumerate([(1, 1, 1), (2, 1, 1), (2, 2, 1), (1, 1, 3)])]
+    structs[1].icluster = 1
+    asm = _gpu(ts, structs)
+    tab = oracle.Tables(ts)
+    for k, st in enumerate(structs):
+        print(f"batch member {k}", flush=True)
+        r = asm.run_all(k)
+        o = oracle.System(tab, st).run_all()
+        for key in LIST_KEYS:
+            assert np.array_equal(r[key], o[key]), (k, key)
+        for m in MATS:
+            np.testing.assert_allclose(r[m], o[m], rtol=0, atol=TOL64, err_msg=f"{k} {m}")
+
+
+def test_si216_against_oracle():
+    from fireball_hip import synth
+    sp = [synth.SI]
+    ts = synth.make_tables(sp)
+    st = synth.diamond_si((3, 3, 3), sigma=0.08, seed=4)
+    synth.random_dQ([st], sp, seed=9)
+    asm = _gpu(ts, [st])
+    r = asm.run_all(0)
+    o = oracle.System(oracle.Tables(ts), st).run_all()
+    for key in LIST_KEYS:
+        assert np.array_equal(r[key], o[key]), key
+    for m in MATS:
+        np.testing.assert_allclose(r[m], o[m], rtol=0, atol=TOL64, err_msg=m)
+
+
+def test_error_atoms_too_close():
+    from fireball_hip import synth
+    from fireball_hip.lib import Assembler, FireballError
+    ts = synth.make_tables([synth.SI], with_3c=False)
+    pos = np.array([[1.0, 1.0, 1.0], [1.3, 1.0, 1.0], [3.0, 3.0, 3.0]])
+    st = synth.Structure(pos, np.ones(3, np.int32), np.eye(3) * 20, 1)
+    asm = Assembler(ts)
+    asm.add_structure(st)
+    with pytest.raises(FireballError) as e:
+        asm.build_lists()
+    assert e.value.code == -4
+
+
+def test_error_x_out_of_range():
+    from fireball_hip import synth
+    from fireball_hip.lib import Assembler, FireballError
+    ts = synth.make_tables([synth.SI], with_3c=False)
+    for c in ts.cells2c.values():
+        c.xmax = 2.0                       # shorter than the neighbour range
+    pos = np.array([[1.0, 1.0, 1.0], [3.5, 1.0, 1.0]])
+    st = synth.Structure(pos, np.ones(2, np.int32), np.eye(3) * 20, 1)
+    asm = Assembler(ts)
+    asm.add_structure(st)
+    asm.build_lists()
+    with pytest.raises(FireballError) as e:
+        asm.assemble_S()
+    assert e.value.code == -3
+
+
+def test_order_errors():
+    from fireball_hip import synth
+    from fireball_hip.lib import Assembler, FireballError
+    ts = synth.make_tables([synth.SI])
+    asm = Assembler(ts)
+    asm.add_structure(synth.diamond_si((1, 1, 1), seed=1))
+    with pytest.raises(FireballError) as e:
+        asm.assemble_S()
+    assert e.value.code == -8
+    asm.build_lists()
+    with pytest.raises(FireballError) as e:
+        asm.assemble_vna_3c()
+    assert e.value.code == -8

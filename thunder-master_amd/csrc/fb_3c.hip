@@ -1,0 +1,225 @@
+// Three-centre neutral-atom (bcna) assembly, j.ASSEMBLERS/M_assemble_3c_DOGS.f90:86-340.
+//
+// Triplets arrive sorted by their target block (iatom, mneigh), stably, so a
+// segment of consecutive triplets holds every contribution to one block in the
+// reference's accumulation order (ialpha ascending, then the common-pair
+// order).  A workgroup owns the segments that start inside its tile of
+// consecutive triplets: it computes the contributions of their triplets in
+// chunks, and one thread per segment adds them in order and stores the block
+// once.  No atomics; results are deterministic.
+//
+// k_3c_fp64 is the plain binary64 kernel: one thread per triplet evaluates the
+// reference's interpolation (e.FDATA/M_Fdata_3c.f90:293-517) literally.
+#include "fb_device.h"
+
+#define T3 128
+#define NB (FB_MO * FB_MO)
+#define NME3 10   // max non-zero 3c matrix elements for s/p species
+
+__device__ __forceinline__ int group3c_id(const DView &v, int in1, int in2, int in3, int isub) {
+  return v.idx3c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * v.sp.ns + (in3 - 1)) * FB_MAXSUB3 + isub];
+}
+
+// Legendre P0..P4 and sin(theta) (M_Fdata_3c.f90:347-354)
+__device__ __forceinline__ void d_legendre(double cost, double *p, double *sint) {
+  double cost2 = cost * cost;
+  *sint = sqrt(fmax(1.0e-5, 1.0 - cost2));
+  p[0] = 1.0;
+  p[1] = cost;
+  p[2] = 0.5 * (3.0 * cost2 - 1.0);
+  p[3] = 0.5 * cost * (5.0 * cost2 - 3.0);
+  p[4] = 0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2));
+}
+
+// getMEs_Fdata_3c in binary64 with the reference's method selection and
+// evaluation order; writes the recovered molecular-frame block hm (nmu x nnu).
+__device__ inline int d_getMEs_3c(const DView &v, int gid, double x, double y, double cost, int nmu, int nnu,
+                                  double *hm) {
+  if (gid < 0) return ST_NO_TABLE;
+  const DGroup3c g = v.grp3c[gid];
+  const int nME = g.nME, nx = g.nx, ny = g.ny;
+  double p[5], sint;
+  d_legendre(cost, p, &sint);
+  const double dx = g.xmax / (nx - 1.0), dy = g.ymax / (ny - 1.0);
+  int ix = (int)(x / dx) + 1, iy = (int)(y / dy) + 1;
+  if (ix < 2) ix = 2;
+  if (iy < 2) iy = 2;
+  if (ix > nx - 2) ix = nx - 2;
+  if (iy > ny - 2) iy = ny - 2;
+  const double px = x / dx - ix + 1.0, py = y / dy - iy + 1.0;
+  const double *tab = v.tab3c + g.off;
+  const int64_t tstride = (int64_t)ny * nx * nME;
+#define T3D(th, i, j, m) tab[(th) * tstride + ((int64_t)((j) - 1) * nx + ((i) - 1)) * nME + (m)]
+  const double gx = (T3D(0, ix + 1, iy, 0) - T3D(0, ix, iy, 0)) / dx;
+  const double gy = (T3D(0, ix, iy + 1, 0) - T3D(0, ix, iy, 0)) / dy;
+  const double gg = gx * gx + gy * gy;
+  double F[NME3];
+  for (int m = 0; m < nME; m++) F[m] = 0.0;
+  for (int th = 0; th < FB_NTHETA; th++) {
+    for (int m = 0; m < nME; m++) {
+      double interim;
+      if (gg < 1.0e-6) {
+        interim = ((1.0 - px - py) * T3D(th, ix, iy, m) + px * T3D(th, ix + 1, iy, m)) + py * T3D(th, ix, iy + 1, m);
+      } else {
+        double gk[4];
+        for (int k = -1; k <= 2; k++) {
+          double f1m1 = T3D(th, ix + k, iy - 1, m);
+          double f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+          double f0p3 = 3.0 * T3D(th, ix + k, iy, m), f0p6 = 2.0 * f0p3;
+          double f1p3 = 3.0 * T3D(th, ix + k, iy + 1, m), f1p6 = 2.0 * f1p3;
+          double f2p1 = T3D(th, ix + k, iy + 2, m);
+          double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+          double bb2 = f1m3 - f0p6 + f1p3;
+          double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+          double bb0 = f0p6;
+          gk[k + 1] = ((bb3 * py + bb2) * py + bb1) * py + bb0;
+        }
+        double f1m1 = gk[0], f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+        double f0p3 = 3.0 * gk[1], f0p6 = 2.0 * f0p3;
+        double f1p3 = 3.0 * gk[2], f1p6 = 2.0 * f1p3;
+        double f2p1 = gk[3];
+        double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+        double bb2 = f1m3 - f0p6 + f1p3;
+        double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+        double bb0 = f0p6;
+        interim = (((bb3 * px + bb2) * px + bb1) * px + bb0) / 36.0;
+      }
+      F[m] = F[m] + p[th] * interim;
+    }
+  }
+#undef T3D
+  for (int i = 0; i < nmu * nnu; i++) hm[i] = 0.0;
+  for (int k = 0; k < nME; k++) {
+    double val = (v.mv[g.munu + k] != 1) ? F[k] : F[k] * sint;
+    hm[(v.nu[g.munu + k] - 1) * nmu + (v.mu[g.munu + k] - 1)] = val;
+  }
+  return 0;
+}
+
+// triplet geometry, M_assemble_3c_DOGS.f90:175-271
+struct Geo3 {
+  double x, z, cost, d13, d23, stinky;
+  double eps[9];
+  int in1, in2, indna, nmu, nnu;
+};
+
+__device__ inline void d_geo3(const DView &v, int4 tr, Geo3 &G) {
+  const int ial = tr.x, gia = tr.y, gja = tr.z;
+  const int ib = tr.w & 0xffff, jb = (tr.w >> 16) & 0xffff;
+  const int s = v.atom_struct[ial];
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  double r1[3], r2[3], rna[3], r21[3], sighat[3], rnabc[3], rhat[3];
+  for (int k = 0; k < 3; k++) {
+    r1[k] = v.ratom[3 * gia + k] + xls[3 * ib + k];
+    r2[k] = v.ratom[3 * gja + k] + xls[3 * jb + k];
+    rna[k] = v.ratom[3 * ial + k];
+    r21[k] = r2[k] - r1[k];
+  }
+  G.in1 = v.imass[gia];
+  G.in2 = v.imass[gja];
+  G.indna = v.imass[ial];
+  G.nmu = v.sp.norb[G.in1 - 1];
+  G.nnu = v.sp.norb[G.in2 - 1];
+  G.z = d_distance(r1, r2);
+  if (G.z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+  else for (int k = 0; k < 3; k++) sighat[k] = r21[k] / G.z;
+  for (int k = 0; k < 3; k++) rnabc[k] = rna[k] - (r1[k] + r21[k] / 2.0);
+  G.x = sqrt(rnabc[0] * rnabc[0] + rnabc[1] * rnabc[1] + rnabc[2] * rnabc[2]);
+  G.d13 = d_distance(rna, r1);
+  G.d23 = d_distance(rna, r2);
+  if (G.x < 1.0e-5) { rhat[0] = 0.0; rhat[1] = 0.0; rhat[2] = 0.0; }
+  else for (int k = 0; k < 3; k++) rhat[k] = rnabc[k] / G.x;
+  G.cost = sighat[0] * rhat[0] + sighat[1] * rhat[1] + sighat[2] * rhat[2];
+  d_epsilon(rhat, sighat, G.eps);
+  double rc1 = v.sp.rcmin[G.in1 - 1], rc2 = v.sp.rcmin[G.in2 - 1], rc3 = v.sp.rcmin[G.indna - 1];
+  G.stinky = d_smoother(G.d13, rc1 + rc3, 0.8);
+  G.stinky = G.stinky * d_smoother(G.d23, rc2 + rc3, 0.8);
+}
+
+// segment range owned by the workgroup of tile [ts, te)
+__device__ __forceinline__ bool tile_segments(const DView &v, int64_t ts, int64_t te, int64_t &first, int64_t &last,
+                                              int64_t &r0, int64_t &r1) {
+  int64_t sgs = v.trip_seg[ts];
+  first = (v.seg_off[sgs] == ts) ? sgs : sgs + 1;
+  last = v.trip_seg[te - 1];
+  if (first > last) return false;
+  r0 = v.seg_off[first];
+  r1 = v.seg_off[last + 1];
+  return true;
+}
+
+__global__ __launch_bounds__(T3) void k_3c_fp64(DView v) {
+  __shared__ double s_c[T3][NB + 1];
+  const int64_t ts = (int64_t)blockIdx.x * T3;
+  const int64_t te = min(ts + T3, v.ntrip);
+  int64_t first, last, r0, r1;
+  if (!tile_segments(v, ts, te, first, last, r0, r1)) return;
+  const int64_t my = first + threadIdx.x;
+  const bool own = my <= last;
+  int64_t a = 0, b = 0;
+  if (own) { a = v.seg_off[my]; b = v.seg_off[my + 1]; }
+  double acc[NB];
+  for (int i = 0; i < NB; i++) acc[i] = 0.0;
+  int status = 0;
+  for (int64_t c0 = r0; c0 < r1; c0 += T3) {
+    const int64_t t = c0 + threadIdx.x;
+    double c[NB];
+    for (int i = 0; i < NB; i++) c[i] = 0.0;
+    if (t < r1) {
+      const int4 tr = v.trip[t];
+      Geo3 G;
+      d_geo3(v, tr, G);
+      double pm[9], hm[NB], bx[NB];
+      d_pmat<double>(G.eps, pm);
+      const int nb = G.nmu * G.nnu;
+      int gid = group3c_id(v, G.in1, G.in2, G.indna, 0);
+      status |= d_getMEs_3c(v, gid, G.x, G.z, G.cost, G.nmu, G.nnu, hm);
+      d_rotate<double>(v.sp, G.in1, G.in2, pm, hm, G.nmu, bx);
+      for (int i = 0; i < nb; i++) c[i] = bx[i] * FB_P_EQ2;
+      bool charged = false;
+      for (int isorp = 1; isorp <= v.sp.nssh[G.indna - 1]; isorp++)
+        charged |= (v.dQ[tr.x * FB_MAXSH + isorp - 1] != 0.0);
+      if (charged) {
+        const int p = v.trip_pair[t];
+        const double *sb = v.mat[FB_MAT_S] + v.boff[p];
+        const double *db = v.mat[FB_MAT_DIP] + v.boff[p];
+        for (int isorp = 1; isorp <= v.sp.nssh[G.indna - 1]; isorp++) {
+          double dQ = v.dQ[tr.x * FB_MAXSH + isorp - 1];
+          if (dQ == 0.0) continue;
+          gid = group3c_id(v, G.in1, G.in2, G.indna, isorp);
+          status |= d_getMEs_3c(v, gid, G.x, G.z, G.cost, G.nmu, G.nnu, hm);
+          d_rotate<double>(v.sp, G.in1, G.in2, pm, hm, G.nmu, bx);
+          for (int i = 0; i < nb; i++) {
+            double sterm = sb[i] / 2.0, dterm = db[i] / G.z;
+            double emnpl = (sterm - dterm) / G.d13 + (sterm + dterm) / G.d23;
+            c[i] = c[i] + dQ * (G.stinky * bx[i] + (1.0 - G.stinky) * emnpl) * FB_P_EQ2;
+          }
+        }
+      }
+    }
+    for (int i = 0; i < NB; i++) s_c[threadIdx.x][i] = c[i];
+    __syncthreads();
+    if (own) {
+      int64_t k0 = max(a, c0), k1 = min(b, c0 + T3);
+      for (int64_t k = k0; k < k1; k++)
+        for (int i = 0; i < NB; i++) acc[i] = acc[i] + s_c[k - c0][i];
+    }
+    __syncthreads();
+  }
+  if (own) {
+    const int p = v.seg_pair[my];
+    const int ia = v.pair_atom[p];
+    const int s = v.atom_struct[ia];
+    const int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
+    const int nb = v.sp.norb[v.imass[ia] - 1] * v.sp.norb[v.imass[ja] - 1];
+    double *dst = v.mat[FB_MAT_VNA] + v.boff[p];
+    for (int i = 0; i < nb; i++) dst[i] = acc[i];
+  }
+  if (status) atomicOr(v.status, status);
+}
+
+extern "C" int fbk_3c_fp64(const DView &v, hipStream_t st) {
+  if (v.ntrip == 0) return 0;
+  hipLaunchKernelGGL(k_3c_fp64, dim3((unsigned)((v.ntrip + T3 - 1) / T3)), dim3(T3), 0, st, v);
+  return (int)hipGetLastError();
+}

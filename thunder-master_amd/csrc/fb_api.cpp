@@ -1,0 +1,987 @@
+// Host side of the C ABI declared in include/fireball_hip.h.
+//
+// Owns the device copies of species, tables, structures, lists and blocks,
+// orders every launch on one HIP stream and turns kernel status bits into the
+// FB_E_* codes that replace the reference's `stop` statements.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "fb_internal.h"
+
+// kernel launchers (fb_lists.hip, fb_2c.hip, fb_3c.hip, fb_3c_mixed.hip)
+extern "C" {
+int fbk_neighbors_count(const DView &v, int *neighn, int *status, hipStream_t st);
+int fbk_neighbors_fill(const DView &v, const int64_t *noff, int *neigh_j, int *neigh_b, int *neigh_self,
+                       int *pair_atom, int *status, hipStream_t st);
+int fbk_neigh_back(const DView &v, int *neigh_back, int *status, hipStream_t st);
+int fbk_common_count(const DView &v, int *ncommon, int *status, hipStream_t st);
+int fbk_common_fill(const DView &v, const int64_t *toff, int4 *trip, int *trip_pair, int *status, hipStream_t st);
+int fbk_block_sizes(const DView &v, int64_t *sz, hipStream_t st);
+int fbk_scan_i32(const int *in, int64_t *out, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t st);
+int fbk_scan_i64(const int64_t *in, int64_t *out, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t st);
+int fbk_sort_pairs(const int *keys_in, int *keys_out, const int *vals_in, int *vals_out, int64_t n, int end_bit,
+                   void *tmp, size_t *tmp_bytes, hipStream_t st);
+int fbk_scan_incl_i32(const int *in, int *out, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t st);
+int fbk_iota(int *a, int64_t n, hipStream_t st);
+int fbk_seg_mark(int64_t ntrip, const int *sorted_pair, int *flag, hipStream_t st);
+int fbk_seg_fill(int64_t ntrip, const int *sorted_pair, const int *segid_incl, int64_t *seg_off, int *seg_pair,
+                 int *trip_seg, hipStream_t st);
+int fbk_gather_trip(int64_t ntrip, const int *perm, const int4 *in, int4 *out, hipStream_t st);
+int fbk_fill_i32(int *a, int64_t n, int val, hipStream_t st);
+int fbk_2c(const DView &v, int mask, hipStream_t st);
+int fbk_3c_fp64(const DView &v, hipStream_t st);
+}
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      cap = 0;
+    }
+    size_t b = bytes ? bytes : 16;
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) cap = b;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <typename T> T *as() const { return (T *)p; }
+};
+
+struct HCell2c {
+  int is, js, itype, isub, nME, nx;
+  double xmax;
+  std::vector<int> mu, nu, mv;
+  std::vector<double> data;  // [nx][nME]
+};
+
+struct HCell3c {
+  int is, js, ks, itype, isub, itheta, nME, nx, ny;
+  double xmax, ymax;
+  std::vector<int> mu, nu, mv;
+  std::vector<double> data;  // [ny][nx][nME]
+};
+
+struct HStruct {
+  int natoms, icluster;
+  std::vector<double> ratom;
+  std::vector<int> imass;
+  double lattice[9];
+  std::vector<double> dQ;  // [natoms][FB_MAXSH]
+};
+
+int slot_of(int itype) {
+  switch (itype) {
+    case 1: return SLOT_S;
+    case 2: return SLOT_T;
+    case 3: return SLOT_ONL;
+    case 4: return SLOT_ONR;
+    case 5: return SLOT_ATOM;
+    case 10: return SLOT_DIP;
+  }
+  return -1;
+}
+
+}  // namespace
+
+struct fb_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  std::string err;
+  int precision = FB_PREC_FP64;
+  int method3c = FB_3C_TABLE;
+
+  DSpecies sp{};
+  int nZ[FB_MAXSPECIES] = {0};
+  bool species_set = false;
+  std::vector<HCell2c> c2;
+  std::vector<HCell3c> c3;
+  bool tables_ok = false;
+  DevBuf d_cell2c, d_idx2c, d_grp3c, d_idx3c, d_tab2c, d_tab2c32, d_tab3c, d_tab3c32, d_mu, d_nu, d_mv;
+
+  std::vector<HStruct> structs;
+  bool structs_dirty = true;
+  std::vector<DStruct> h_st;
+  std::vector<int> h_imass;
+  int natoms = 0;
+  DevBuf d_st, d_xl, d_ratom, d_imass, d_atom_struct, d_dQ;
+
+  DevBuf d_neighn, d_noff, d_neigh_j, d_neigh_b, d_neigh_back, d_neigh_self, d_pair_atom, d_bsz, d_boff;
+  DevBuf d_ncommon, d_toff, d_trip_orig, d_tpair_orig, d_trip, d_tpair, d_perm_in, d_perm, d_flag, d_segincl,
+      d_seg_off, d_seg_pair, d_trip_seg;
+  DevBuf d_tmp, d_status;
+  DevBuf d_mat[5];
+  int64_t npairs = 0, ntrip = 0, nseg = 0, blen = 0;
+  std::vector<int64_t> h_noff, h_toff, h_boff_struct;
+  bool lists_ok = false;
+  int done[5] = {0, 0, 0, 0, 0};
+
+  bool timing = false;
+  struct Ev {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Ev> events;
+  std::vector<hipEvent_t> pool;
+
+  int fail(int code, const char *what, hipError_t e = hipSuccess) {
+    err = what;
+    if (e != hipSuccess) {
+      err += ": ";
+      err += hipGetErrorString(e);
+    }
+    return code;
+  }
+
+  DView view() const {
+    DView v{};
+    v.sp = sp;
+    v.cell2c = d_cell2c.as<DCell2c>();
+    v.idx2c = d_idx2c.as<int>();
+    v.grp3c = d_grp3c.as<DGroup3c>();
+    v.idx3c = d_idx3c.as<int>();
+    v.tab2c = d_tab2c.as<double>();
+    v.tab2c32 = d_tab2c32.as<float>();
+    v.tab3c = d_tab3c.as<double>();
+    v.tab3c32 = d_tab3c32.as<float>();
+    v.mu = d_mu.as<int>();
+    v.nu = d_nu.as<int>();
+    v.mv = d_mv.as<int>();
+    v.nstruct = (int)structs.size();
+    v.natoms = natoms;
+    v.st = d_st.as<DStruct>();
+    v.xl = d_xl.as<double>();
+    v.ratom = d_ratom.as<double>();
+    v.imass = d_imass.as<int>();
+    v.atom_struct = d_atom_struct.as<int>();
+    v.dQ = d_dQ.as<double>();
+    v.npairs = npairs;
+    v.neighn = d_neighn.as<int>();
+    v.noff = d_noff.as<int64_t>();
+    v.neigh_j = d_neigh_j.as<int>();
+    v.neigh_b = d_neigh_b.as<int>();
+    v.neigh_back = d_neigh_back.as<int>();
+    v.neigh_self = d_neigh_self.as<int>();
+    v.pair_atom = d_pair_atom.as<int>();
+    v.boff = d_boff.as<int64_t>();
+    v.ntrip = ntrip;
+    v.nseg = nseg;
+    v.trip = d_trip.as<int4>();
+    v.trip_pair = d_tpair.as<int>();
+    v.seg_off = d_seg_off.as<int64_t>();
+    v.seg_pair = d_seg_pair.as<int>();
+    v.trip_seg = d_trip_seg.as<int>();
+    for (int i = 0; i < 5; i++) v.mat[i] = d_mat[i].as<double>();
+    v.status = d_status.as<int>();
+    return v;
+  }
+
+  hipEvent_t ev() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+
+  // run a launcher, bracketed by events when timing is on
+  template <typename F> int launch(const char *name, F f) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (timing) {
+      a = ev();
+      b = ev();
+      (void)hipEventRecord(a, stream);
+    }
+    int rc = f();
+    if (timing) {
+      (void)hipEventRecord(b, stream);
+      events.push_back({name, a, b});
+    }
+    if (rc != 0) return fail(FB_E_HIP, name, (hipError_t)rc);
+    return FB_OK;
+  }
+
+  void reset_events() {
+    for (auto &e : events) {
+      pool.push_back(e.a);
+      pool.push_back(e.b);
+    }
+    events.clear();
+  }
+
+  int check_status(const char *where) {
+    int st = 0;
+    hipError_t e = hipMemcpyAsync(&st, d_status.p, sizeof(int), hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess) return fail(FB_E_HIP, where, e);
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return fail(FB_E_HIP, where, e);
+    if (st == 0) return FB_OK;
+    std::string w(where);
+    if (st & ST_TOO_CLOSE) return fail(FB_E_ATOMS_TOO_CLOSE, (w + ": atoms dangerously close (< 0.5 A)").c_str());
+    if (st & ST_BACK) return fail(FB_E_NEIGH_BACK, (w + ": neigh_back missing or not unique").c_str());
+    if (st & ST_SELF) return fail(FB_E_NEIGH_SELF, (w + ": neigh_self missing").c_str());
+    if (st & ST_X_NEG) return fail(FB_E_X_NEGATIVE, (w + ": negative x in 2c interpolation").c_str());
+    if (st & ST_X_RANGE) return fail(FB_E_X_OUT_OF_RANGE, (w + ": x beyond the table range").c_str());
+    if (st & ST_NO_TABLE) return fail(FB_E_NO_TABLE, (w + ": interaction table missing").c_str());
+    if (st & ST_CAPACITY) return fail(FB_E_CAPACITY, (w + ": list exceeds kernel capacity").c_str());
+    return fail(FB_E_HIP, where);
+  }
+
+  int clear_status() {
+    hipError_t e = hipMemsetAsync(d_status.p, 0, sizeof(int), stream);
+    if (e != hipSuccess) return fail(FB_E_HIP, "status clear", e);
+    return FB_OK;
+  }
+
+  ~fb_ctx() {
+    DevBuf *bufs[] = {&d_cell2c, &d_idx2c, &d_grp3c, &d_idx3c, &d_tab2c, &d_tab2c32, &d_tab3c, &d_tab3c32,
+                      &d_mu, &d_nu, &d_mv, &d_st, &d_xl, &d_ratom, &d_imass, &d_atom_struct, &d_dQ,
+                      &d_neighn, &d_noff, &d_neigh_j, &d_neigh_b, &d_neigh_back, &d_neigh_self, &d_pair_atom,
+                      &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
+                      &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_tmp,
+                      &d_status};
+    for (DevBuf *b : bufs) b->release();
+    for (auto &m : d_mat) m.release();
+    reset_events();
+    for (auto e : pool) (void)hipEventDestroy(e);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+  }
+};
+
+// ------------------------------------------------------------------ helpers
+static int upload(fb_ctx *ctx, DevBuf &b, const void *src, size_t bytes, const char *what) {
+  hipError_t e = b.ensure(bytes);
+  if (e != hipSuccess) return ctx->fail(FB_E_HIP, what, e);
+  if (bytes) {
+    e = hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return ctx->fail(FB_E_HIP, what, e);
+  }
+  return FB_OK;
+}
+
+#define RC(x)                 \
+  do {                        \
+    int rc_ = (x);            \
+    if (rc_ != FB_OK) return rc_; \
+  } while (0)
+
+// make_cells, c.SYSTEM/make_cells.f90:115-144 (same expression order)
+static int make_cells(const double *lat, int icluster, double *xl) {
+  xl[0] = xl[1] = xl[2] = 0.0;
+  if (icluster == 1) return 0;
+  int mbeta = 1;
+  for (int iz = -4; iz <= 4; iz++)
+    for (int iy = -4; iy <= 4; iy++)
+      for (int ix = -4; ix <= 4; ix++) {
+        if (ix == 0 && iy == 0 && iz == 0) continue;
+        for (int k = 0; k < 3; k++)
+          xl[3 * mbeta + k] = (double)ix * lat[k] + (double)iy * lat[3 + k] + (double)iz * lat[6 + k];
+        mbeta++;
+      }
+  return mbeta - 1;
+}
+
+// ------------------------------------------------------------------- C ABI
+extern "C" {
+
+int fb_create(fb_ctx **out, int device) {
+  if (!out) return FB_E_ARG;
+  *out = nullptr;
+  fb_ctx *ctx = new fb_ctx();
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete ctx;
+    return FB_E_HIP;
+  }
+  e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return FB_E_HIP;
+  }
+  ctx->stream = ctx->own_stream;
+  e = ctx->d_status.ensure(sizeof(int) * 4);
+  if (e != hipSuccess) {
+    delete ctx;
+    return FB_E_HIP;
+  }
+  *out = ctx;
+  return FB_OK;
+}
+
+int fb_destroy(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return FB_OK;
+}
+
+const char *fb_last_error(const fb_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int fb_set_stream(fb_ctx *ctx, void *s) {
+  if (!ctx) return FB_E_ARG;
+  ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+  return FB_OK;
+}
+
+int fb_synchronize(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  return FB_OK;
+}
+
+int fb_set_precision(fb_ctx *ctx, int mode) {
+  if (!ctx || (mode != FB_PREC_FP64 && mode != FB_PREC_MIXED)) return FB_E_ARG;
+  ctx->precision = mode;
+  return FB_OK;
+}
+
+int fb_set_3c_method(fb_ctx *ctx, int method) {
+  if (!ctx || (method != FB_3C_TABLE && method != FB_3C_MLP)) return FB_E_ARG;
+  ctx->method3c = method;
+  return FB_OK;
+}
+
+int fb_set_species(fb_ctx *ctx, int ns, const int *nZ, const int *nssh, const int *lssh, const double *rcA) {
+  if (!ctx || ns < 1 || ns > FB_MAXSPECIES || !nssh || !lssh || !rcA) return ctx ? ctx->fail(FB_E_ARG, "species") : FB_E_ARG;
+  DSpecies sp{};
+  sp.ns = ns;
+  for (int i = 0; i < ns; i++) {
+    if (nssh[i] < 1 || nssh[i] > FB_MAXSH) return ctx->fail(FB_E_ARG, "species: nssh out of range");
+    sp.nssh[i] = nssh[i];
+    sp.norb[i] = 0;
+    sp.rcmax[i] = -99.0;
+    sp.rcmin[i] = 99.0;
+    for (int k = 0; k < nssh[i]; k++) {
+      int l = lssh[i * FB_MAXSH + k];
+      if (l < 0 || l > 2) return ctx->fail(FB_E_UNSUPPORTED, "species: shell with l > 2");
+      if (l == 2) sp.have_d = 1;
+      sp.lssh[i][k] = l;
+      sp.norb[i] += 2 * l + 1;
+      // c.SYSTEM/M_neighbors.f90:299-304 (max) and M_assemble_2c_DOGS.f90:764-771 (min)
+      sp.rcmax[i] = std::max(sp.rcmax[i], rcA[i * FB_MAXSH + k]);
+      sp.rcmin[i] = std::min(sp.rcmin[i], rcA[i * FB_MAXSH + k]);
+    }
+    if (sp.norb[i] > 4) return ctx->fail(FB_E_UNSUPPORTED, "species: more than 4 orbitals (s and p shells only)");
+    ctx->nZ[i] = nZ ? nZ[i] : 0;
+  }
+  ctx->sp = sp;
+  ctx->species_set = true;
+  ctx->tables_ok = false;
+  ctx->lists_ok = false;
+  return FB_OK;
+}
+
+int fb_set_table_2c(fb_ctx *ctx, int is, int js, int itype, int isub, int nME, int nx, double xmax, const int *mu,
+                    const int *nu, const int *mvalue, const double *fdata) {
+  if (!ctx || !ctx->species_set) return ctx ? ctx->fail(FB_E_ORDER, "set_table_2c before set_species") : FB_E_ARG;
+  if (is < 1 || js < 1 || is > ctx->sp.ns || js > ctx->sp.ns || isub < 0 || isub >= FB_MAXSUB2 || nx < 7 ||
+      nME < 1 || nME > 16 || !mu || !nu || !mvalue || !fdata)
+    return ctx->fail(FB_E_ARG, "set_table_2c: bad arguments");
+  HCell2c c;
+  c.is = is; c.js = js; c.itype = itype; c.isub = isub; c.nME = nME; c.nx = nx; c.xmax = xmax;
+  c.mu.assign(mu, mu + nME);
+  c.nu.assign(nu, nu + nME);
+  c.mv.assign(mvalue, mvalue + nME);
+  c.data.resize((size_t)nx * nME);
+  for (int m = 0; m < nME; m++)
+    for (int ix = 0; ix < nx; ix++) c.data[(size_t)ix * nME + m] = fdata[(size_t)m * nx + ix];
+  for (auto &o : ctx->c2)
+    if (o.is == is && o.js == js && o.itype == itype && o.isub == isub) { o = std::move(c); ctx->tables_ok = false; return FB_OK; }
+  ctx->c2.push_back(std::move(c));
+  ctx->tables_ok = false;
+  return FB_OK;
+}
+
+int fb_set_table_3c(fb_ctx *ctx, int is, int js, int ks, int itype, int isub, int itheta, int nME, int nx,
+                    double xmax, int ny, double ymax, const int *mu, const int *nu, const int *mvalue,
+                    const double *fdata) {
+  if (!ctx || !ctx->species_set) return ctx ? ctx->fail(FB_E_ORDER, "set_table_3c before set_species") : FB_E_ARG;
+  if (is < 1 || js < 1 || ks < 1 || is > ctx->sp.ns || js > ctx->sp.ns || ks > ctx->sp.ns || isub < 0 ||
+      isub >= FB_MAXSUB3 || itheta < 1 || itheta > FB_NTHETA || nx < 4 || ny < 4 || nME < 1 || nME > 10 || !mu ||
+      !nu || !mvalue || !fdata)
+    return ctx->fail(FB_E_ARG, "set_table_3c: bad arguments");
+  HCell3c c;
+  c.is = is; c.js = js; c.ks = ks; c.itype = itype; c.isub = isub; c.itheta = itheta;
+  c.nME = nME; c.nx = nx; c.ny = ny; c.xmax = xmax; c.ymax = ymax;
+  c.mu.assign(mu, mu + nME);
+  c.nu.assign(nu, nu + nME);
+  c.mv.assign(mvalue, mvalue + nME);
+  c.data.resize((size_t)nx * ny * nME);
+  for (int m = 0; m < nME; m++)
+    for (int iy = 0; iy < ny; iy++)
+      for (int ix = 0; ix < nx; ix++)
+        c.data[((size_t)iy * nx + ix) * nME + m] = fdata[((size_t)m * ny + iy) * nx + ix];
+  for (auto &o : ctx->c3)
+    if (o.is == is && o.js == js && o.ks == ks && o.itype == itype && o.isub == isub && o.itheta == itheta) {
+      o = std::move(c);
+      ctx->tables_ok = false;
+      return FB_OK;
+    }
+  ctx->c3.push_back(std::move(c));
+  ctx->tables_ok = false;
+  return FB_OK;
+}
+
+int fb_commit_tables(fb_ctx *ctx) {
+  if (!ctx || !ctx->species_set) return ctx ? ctx->fail(FB_E_ORDER, "commit_tables before set_species") : FB_E_ARG;
+  FB_HIP(hipSetDevice(ctx->device));
+  const int ns = ctx->sp.ns;
+  std::vector<int> idx2((size_t)ns * ns * FB_NSLOT2 * FB_MAXSUB2, -1);
+  std::vector<DCell2c> cells;
+  std::vector<double> tab2;
+  std::vector<int> mu, nu, mv;
+  for (auto &c : ctx->c2) {
+    int slot = slot_of(c.itype);
+    if (slot < 0) continue;  // interactions outside the assembled path
+    {
+      // the block of the atom term lives on the first atom's orbitals (make_munu_atom)
+      int rows = ctx->sp.norb[c.is - 1], cols = (slot == SLOT_ATOM) ? rows : ctx->sp.norb[c.js - 1];
+      for (int k = 0; k < c.nME; k++)
+        if (c.mu[k] < 1 || c.mu[k] > rows || c.nu[k] < 1 || c.nu[k] > cols)
+          return ctx->fail(FB_E_ARG, "commit_tables: 2c mu/nu outside the orbital block");
+    }
+    DCell2c d{};
+    d.nME = c.nME;
+    d.nx = c.nx;
+    d.xmax = c.xmax;
+    d.dx = c.xmax / (double)(c.nx - 1);  // e.FDATA/M_Fdata_2c.f90:222
+    d.off = (int64_t)tab2.size();
+    d.off32 = d.off;
+    d.munu = (int)mu.size();
+    tab2.insert(tab2.end(), c.data.begin(), c.data.end());
+    mu.insert(mu.end(), c.mu.begin(), c.mu.end());
+    nu.insert(nu.end(), c.nu.begin(), c.nu.end());
+    mv.insert(mv.end(), c.mv.begin(), c.mv.end());
+    idx2[(((size_t)(c.is - 1) * ns + (c.js - 1)) * FB_NSLOT2 + slot) * FB_MAXSUB2 + c.isub] = (int)cells.size();
+    cells.push_back(d);
+  }
+  // 3c: group the five theta cells of each (is, js, ks, isub) of bcna (itype 3)
+  std::map<std::tuple<int, int, int, int>, std::vector<const HCell3c *>> groups;
+  for (auto &c : ctx->c3)
+    if (c.itype == 3) groups[std::make_tuple(c.is, c.js, c.ks, c.isub)].push_back(&c);
+  std::vector<int> idx3((size_t)ns * ns * ns * FB_MAXSUB3, -1);
+  std::vector<DGroup3c> grps;
+  std::vector<double> tab3;
+  std::vector<float> tab3f;
+  for (auto &kv : groups) {
+    const HCell3c *th[FB_NTHETA] = {nullptr};
+    for (auto *c : kv.second) th[c->itheta - 1] = c;
+    for (int t = 0; t < FB_NTHETA; t++)
+      if (!th[t]) return ctx->fail(FB_E_NO_TABLE, "commit_tables: a 3c group lacks a theta cell");
+    const HCell3c *c1 = th[0];
+    for (int t = 0; t < FB_NTHETA; t++)
+      for (int k = 0; k < th[t]->nME; k++)
+        if (th[t]->mu[k] < 1 || th[t]->mu[k] > ctx->sp.norb[th[t]->is - 1] || th[t]->nu[k] < 1 ||
+            th[t]->nu[k] > ctx->sp.norb[th[t]->js - 1])
+          return ctx->fail(FB_E_ARG, "commit_tables: 3c mu/nu outside the orbital block");
+    for (int t = 1; t < FB_NTHETA; t++)
+      if (th[t]->nME != c1->nME || th[t]->nx != c1->nx || th[t]->ny != c1->ny || th[t]->xmax != c1->xmax ||
+          th[t]->ymax != c1->ymax)
+        return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: theta cells of a 3c group differ in grid");
+    DGroup3c g{};
+    g.nME = c1->nME;
+    g.nx = c1->nx;
+    g.ny = c1->ny;
+    g.xmax = c1->xmax;
+    g.ymax = c1->ymax;
+    g.dx = c1->xmax / (c1->nx - 1.0);  // e.FDATA/M_Fdata_3c.f90:367-368
+    g.dy = c1->ymax / (c1->ny - 1.0);
+    g.off = (int64_t)tab3.size();
+    g.off32 = (int64_t)tab3f.size();
+    // the reference recovers the block with the mu/nu/mvalue of the last theta cell
+    const HCell3c *cl = th[FB_NTHETA - 1];
+    g.munu = (int)mu.size();
+    mu.insert(mu.end(), cl->mu.begin(), cl->mu.end());
+    nu.insert(nu.end(), cl->nu.begin(), cl->nu.end());
+    mv.insert(mv.end(), cl->mv.begin(), cl->mv.end());
+    for (int t = 0; t < FB_NTHETA; t++) tab3.insert(tab3.end(), th[t]->data.begin(), th[t]->data.end());
+    size_t plane = (size_t)g.nx * g.ny;
+    tab3f.resize(tab3f.size() + plane * FB_NTHETA * g.nME);
+    float *dst = tab3f.data() + g.off32;
+    for (size_t q = 0; q < plane; q++)
+      for (int t = 0; t < FB_NTHETA; t++)
+        for (int m = 0; m < g.nME; m++) dst[(q * FB_NTHETA + t) * g.nME + m] = (float)th[t]->data[q * g.nME + m];
+    int is, js, ks, isub;
+    std::tie(is, js, ks, isub) = kv.first;
+    idx3[(((size_t)(is - 1) * ns + (js - 1)) * ns + (ks - 1)) * FB_MAXSUB3 + isub] = (int)grps.size();
+    grps.push_back(g);
+  }
+  std::vector<float> tab2f(tab2.begin(), tab2.end());
+  RC(upload(ctx, ctx->d_cell2c, cells.data(), cells.size() * sizeof(DCell2c), "upload cells2c"));
+  RC(upload(ctx, ctx->d_idx2c, idx2.data(), idx2.size() * sizeof(int), "upload idx2c"));
+  RC(upload(ctx, ctx->d_grp3c, grps.data(), grps.size() * sizeof(DGroup3c), "upload groups3c"));
+  RC(upload(ctx, ctx->d_idx3c, idx3.data(), idx3.size() * sizeof(int), "upload idx3c"));
+  RC(upload(ctx, ctx->d_tab2c, tab2.data(), tab2.size() * sizeof(double), "upload tab2c"));
+  RC(upload(ctx, ctx->d_tab2c32, tab2f.data(), tab2f.size() * sizeof(float), "upload tab2c32"));
+  RC(upload(ctx, ctx->d_tab3c, tab3.data(), tab3.size() * sizeof(double), "upload tab3c"));
+  RC(upload(ctx, ctx->d_tab3c32, tab3f.data(), tab3f.size() * sizeof(float), "upload tab3c32"));
+  RC(upload(ctx, ctx->d_mu, mu.data(), mu.size() * sizeof(int), "upload mu"));
+  RC(upload(ctx, ctx->d_nu, nu.data(), nu.size() * sizeof(int), "upload nu"));
+  RC(upload(ctx, ctx->d_mv, mv.data(), mv.size() * sizeof(int), "upload mvalue"));
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->tables_ok = true;
+  return FB_OK;
+}
+
+int fb_clear_structures(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  ctx->structs.clear();
+  ctx->structs_dirty = true;
+  ctx->lists_ok = false;
+  return FB_OK;
+}
+
+int fb_add_structure(fb_ctx *ctx, int natoms, const double *ratom, const int *imass, const double *lattice,
+                     int icluster, const double *dQ, int *istruct) {
+  if (!ctx || !ctx->species_set) return ctx ? ctx->fail(FB_E_ORDER, "add_structure before set_species") : FB_E_ARG;
+  if (natoms < 1 || !ratom || !imass || !lattice) return ctx->fail(FB_E_ARG, "add_structure: bad arguments");
+  HStruct s;
+  s.natoms = natoms;
+  s.icluster = icluster;
+  s.ratom.assign(ratom, ratom + 3 * (size_t)natoms);
+  s.imass.assign(imass, imass + natoms);
+  for (int i = 0; i < natoms; i++)
+    if (imass[i] < 1 || imass[i] > ctx->sp.ns) return ctx->fail(FB_E_ARG, "add_structure: species out of range");
+  std::memcpy(s.lattice, lattice, sizeof(s.lattice));
+  s.dQ.assign((size_t)natoms * FB_MAXSH, 0.0);
+  if (dQ) std::memcpy(s.dQ.data(), dQ, sizeof(double) * natoms * FB_MAXSH);
+  ctx->structs.push_back(std::move(s));
+  if (istruct) *istruct = (int)ctx->structs.size() - 1;
+  ctx->structs_dirty = true;
+  ctx->lists_ok = false;
+  return FB_OK;
+}
+
+int fb_set_positions(fb_ctx *ctx, int is, const double *ratom) {
+  if (!ctx || is < 0 || is >= (int)ctx->structs.size() || !ratom) return FB_E_ARG;
+  auto &s = ctx->structs[is];
+  std::memcpy(s.ratom.data(), ratom, sizeof(double) * 3 * s.natoms);
+  ctx->structs_dirty = true;
+  ctx->lists_ok = false;
+  return FB_OK;
+}
+
+int fb_set_charges(fb_ctx *ctx, int is, const double *dQ) {
+  if (!ctx || is < 0 || is >= (int)ctx->structs.size()) return FB_E_ARG;
+  auto &s = ctx->structs[is];
+  std::fill(s.dQ.begin(), s.dQ.end(), 0.0);
+  if (dQ) std::memcpy(s.dQ.data(), dQ, sizeof(double) * s.natoms * FB_MAXSH);
+  ctx->structs_dirty = true;  // charges change no list, but must be re-uploaded
+  for (int i = 0; i < 5; i++) ctx->done[i] = 0;
+  return FB_OK;
+}
+
+static int upload_structures(fb_ctx *ctx) {
+  const int ns = (int)ctx->structs.size();
+  int natoms = 0;
+  for (auto &s : ctx->structs) natoms += s.natoms;
+  ctx->natoms = natoms;
+  std::vector<double> ratom((size_t)natoms * 3), dq((size_t)natoms * FB_MAXSH), xl((size_t)ns * FB_NIMG * 3, 0.0);
+  std::vector<int> imass(natoms), astr(natoms);
+  ctx->h_st.assign(ns, DStruct{});
+  int off = 0;
+  for (int k = 0; k < ns; k++) {
+    const HStruct &s = ctx->structs[k];
+    DStruct &d = ctx->h_st[k];
+    d.atom_off = off;
+    d.natoms = s.natoms;
+    d.icluster = s.icluster;
+    d.mbeta_max = make_cells(s.lattice, s.icluster, &xl[(size_t)k * FB_NIMG * 3]);
+    d.rcmax_all = -99.0;
+    for (int q = 0; q < 3; q++) { d.bmin[q] = 1e300; d.bmax[q] = -1e300; }
+    for (int i = 0; i < s.natoms; i++) {
+      for (int q = 0; q < 3; q++) {
+        double x = s.ratom[3 * i + q];
+        ratom[3 * (size_t)(off + i) + q] = x;
+        d.bmin[q] = std::min(d.bmin[q], x);
+        d.bmax[q] = std::max(d.bmax[q], x);
+      }
+      imass[off + i] = s.imass[i];
+      astr[off + i] = k;
+      d.rcmax_all = std::max(d.rcmax_all, ctx->sp.rcmax[s.imass[i] - 1]);
+      for (int q = 0; q < FB_MAXSH; q++) dq[(size_t)(off + i) * FB_MAXSH + q] = s.dQ[(size_t)i * FB_MAXSH + q];
+    }
+    off += s.natoms;
+  }
+  ctx->h_imass = imass;
+  RC(upload(ctx, ctx->d_ratom, ratom.data(), ratom.size() * sizeof(double), "upload ratom"));
+  RC(upload(ctx, ctx->d_imass, imass.data(), imass.size() * sizeof(int), "upload imass"));
+  RC(upload(ctx, ctx->d_atom_struct, astr.data(), astr.size() * sizeof(int), "upload atom_struct"));
+  RC(upload(ctx, ctx->d_dQ, dq.data(), dq.size() * sizeof(double), "upload dQ"));
+  RC(upload(ctx, ctx->d_xl, xl.data(), xl.size() * sizeof(double), "upload xl"));
+  RC(upload(ctx, ctx->d_st, ctx->h_st.data(), ctx->h_st.size() * sizeof(DStruct), "upload structs"));
+  ctx->structs_dirty = false;
+  return FB_OK;
+}
+
+static int ensure_tmp(fb_ctx *ctx, size_t bytes) {
+  hipError_t e = ctx->d_tmp.ensure(bytes);
+  if (e != hipSuccess) return ctx->fail(FB_E_HIP, "temporary storage", e);
+  return FB_OK;
+}
+
+static int scan_i32(fb_ctx *ctx, const int *in, int64_t *out, int64_t n) {
+  size_t bytes = 0;
+  int rc = fbk_scan_i32(in, out, n, nullptr, &bytes, ctx->stream);
+  if (rc) return ctx->fail(FB_E_HIP, "scan sizing", (hipError_t)rc);
+  RC(ensure_tmp(ctx, bytes));
+  bytes = ctx->d_tmp.cap;
+  return ctx->launch("scan", [&] { return fbk_scan_i32(in, out, n, ctx->d_tmp.p, &bytes, ctx->stream); });
+}
+
+static int scan_i64(fb_ctx *ctx, const int64_t *in, int64_t *out, int64_t n) {
+  size_t bytes = 0;
+  int rc = fbk_scan_i64(in, out, n, nullptr, &bytes, ctx->stream);
+  if (rc) return ctx->fail(FB_E_HIP, "scan sizing", (hipError_t)rc);
+  RC(ensure_tmp(ctx, bytes));
+  bytes = ctx->d_tmp.cap;
+  return ctx->launch("scan", [&] { return fbk_scan_i64(in, out, n, ctx->d_tmp.p, &bytes, ctx->stream); });
+}
+
+int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
+  if (!ctx) return FB_E_ARG;
+  if (!ctx->tables_ok) return ctx->fail(FB_E_ORDER, "build_lists before commit_tables");
+  if (ctx->structs.empty()) return ctx->fail(FB_E_ORDER, "build_lists without structures");
+  FB_HIP(hipSetDevice(ctx->device));
+  ctx->reset_events();
+  RC(upload_structures(ctx));
+  const int N = ctx->natoms;
+  RC(ctx->clear_status());
+  hipStream_t st = ctx->stream;
+  FB_HIP(ctx->d_neighn.ensure(sizeof(int) * N));
+  FB_HIP(ctx->d_noff.ensure(sizeof(int64_t) * (N + 1)));
+  FB_HIP(ctx->d_neigh_self.ensure(sizeof(int) * N));
+  DView v = ctx->view();
+  RC(ctx->launch("neighbors_count", [&] { return fbk_neighbors_count(v, ctx->d_neighn.as<int>(), v.status, st); }));
+  RC(scan_i32(ctx, ctx->d_neighn.as<int>(), ctx->d_noff.as<int64_t>(), N));
+  ctx->h_noff.resize(N + 1);
+  FB_HIP(hipMemcpyAsync(ctx->h_noff.data(), ctx->d_noff.p, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost, st));
+  RC(ctx->check_status("find_neighbors"));
+  ctx->npairs = ctx->h_noff[N];
+  const int64_t P = ctx->npairs;
+  FB_HIP(ctx->d_neigh_j.ensure(sizeof(int) * P));
+  FB_HIP(ctx->d_neigh_b.ensure(sizeof(int) * P));
+  FB_HIP(ctx->d_neigh_back.ensure(sizeof(int) * P));
+  FB_HIP(ctx->d_pair_atom.ensure(sizeof(int) * P));
+  FB_HIP(ctx->d_bsz.ensure(sizeof(int64_t) * P));
+  FB_HIP(ctx->d_boff.ensure(sizeof(int64_t) * (P + 1)));
+  v = ctx->view();
+  RC(ctx->launch("fill_self", [&] { return fbk_fill_i32(ctx->d_neigh_self.as<int>(), N, -999, st); }));
+  RC(ctx->launch("neighbors_fill", [&] {
+    return fbk_neighbors_fill(v, ctx->d_noff.as<int64_t>(), ctx->d_neigh_j.as<int>(), ctx->d_neigh_b.as<int>(),
+                              ctx->d_neigh_self.as<int>(), ctx->d_pair_atom.as<int>(), v.status, st);
+  }));
+  RC(ctx->launch("neigh_back", [&] { return fbk_neigh_back(v, ctx->d_neigh_back.as<int>(), v.status, st); }));
+  {
+    std::vector<int> self(N);
+    FB_HIP(hipMemcpyAsync(self.data(), ctx->d_neigh_self.p, sizeof(int) * N, hipMemcpyDeviceToHost, st));
+    RC(ctx->check_status("find_neighbors"));
+    for (int i = 0; i < N; i++)
+      if (self[i] == -999) return ctx->fail(FB_E_NEIGH_SELF, "find_neighbors: neigh_self missing");
+  }
+  // common neighbours
+  FB_HIP(ctx->d_ncommon.ensure(sizeof(int) * N));
+  FB_HIP(ctx->d_toff.ensure(sizeof(int64_t) * (N + 1)));
+  RC(ctx->launch("common_count", [&] { return fbk_common_count(v, ctx->d_ncommon.as<int>(), v.status, st); }));
+  RC(scan_i32(ctx, ctx->d_ncommon.as<int>(), ctx->d_toff.as<int64_t>(), N));
+  ctx->h_toff.resize(N + 1);
+  FB_HIP(hipMemcpyAsync(ctx->h_toff.data(), ctx->d_toff.p, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost, st));
+  RC(ctx->check_status("find_common_neighbors"));
+  ctx->ntrip = ctx->h_toff[N];
+  const int64_t T = ctx->ntrip;
+  if (P >= (1ll << 31) || T >= (1ll << 31)) return ctx->fail(FB_E_CAPACITY, "more than 2^31 pairs or triplets");
+  FB_HIP(ctx->d_trip_orig.ensure(sizeof(int4) * T));
+  FB_HIP(ctx->d_tpair_orig.ensure(sizeof(int) * T));
+  FB_HIP(ctx->d_trip.ensure(sizeof(int4) * T));
+  FB_HIP(ctx->d_tpair.ensure(sizeof(int) * T));
+  FB_HIP(ctx->d_perm_in.ensure(sizeof(int) * T));
+  FB_HIP(ctx->d_perm.ensure(sizeof(int) * T));
+  FB_HIP(ctx->d_flag.ensure(sizeof(int) * T));
+  FB_HIP(ctx->d_segincl.ensure(sizeof(int) * T));
+  FB_HIP(ctx->d_seg_off.ensure(sizeof(int64_t) * (T + 1)));
+  FB_HIP(ctx->d_seg_pair.ensure(sizeof(int) * T));
+  FB_HIP(ctx->d_trip_seg.ensure(sizeof(int) * T));
+  RC(ctx->launch("common_fill", [&] {
+    return fbk_common_fill(v, ctx->d_toff.as<int64_t>(), ctx->d_trip_orig.as<int4>(), ctx->d_tpair_orig.as<int>(),
+                           v.status, st);
+  }));
+  ctx->nseg = 0;
+  if (T > 0) {
+    // stable radix sort of triplet ids by target pair
+    int end_bit = 1;
+    while ((1ll << end_bit) <= P) end_bit++;
+    RC(ctx->launch("iota", [&] { return fbk_iota(ctx->d_perm_in.as<int>(), T, st); }));
+    size_t bytes = 0;
+    int rc = fbk_sort_pairs(ctx->d_tpair_orig.as<int>(), ctx->d_tpair.as<int>(), ctx->d_perm_in.as<int>(),
+                            ctx->d_perm.as<int>(), T, end_bit, nullptr, &bytes, st);
+    if (rc) return ctx->fail(FB_E_HIP, "sort sizing", (hipError_t)rc);
+    size_t b2 = 0;
+    rc = fbk_scan_incl_i32(ctx->d_flag.as<int>(), ctx->d_segincl.as<int>(), T, nullptr, &b2, st);
+    if (rc) return ctx->fail(FB_E_HIP, "scan sizing", (hipError_t)rc);
+    RC(ensure_tmp(ctx, std::max(bytes, b2)));
+    bytes = ctx->d_tmp.cap;
+    RC(ctx->launch("sort_triplets", [&] {
+      return fbk_sort_pairs(ctx->d_tpair_orig.as<int>(), ctx->d_tpair.as<int>(), ctx->d_perm_in.as<int>(),
+                            ctx->d_perm.as<int>(), T, end_bit, ctx->d_tmp.p, &bytes, st);
+    }));
+    RC(ctx->launch("seg_mark", [&] { return fbk_seg_mark(T, ctx->d_tpair.as<int>(), ctx->d_flag.as<int>(), st); }));
+    b2 = ctx->d_tmp.cap;
+    RC(ctx->launch("seg_scan", [&] {
+      return fbk_scan_incl_i32(ctx->d_flag.as<int>(), ctx->d_segincl.as<int>(), T, ctx->d_tmp.p, &b2, st);
+    }));
+    int nseg = 0;
+    FB_HIP(hipMemcpyAsync(&nseg, ctx->d_segincl.as<int>() + (T - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+    RC(ctx->launch("seg_fill", [&] {
+      return fbk_seg_fill(T, ctx->d_tpair.as<int>(), ctx->d_segincl.as<int>(), ctx->d_seg_off.as<int64_t>(),
+                          ctx->d_seg_pair.as<int>(), ctx->d_trip_seg.as<int>(), st);
+    }));
+    RC(ctx->launch("gather_triplets", [&] {
+      return fbk_gather_trip(T, ctx->d_perm.as<int>(), ctx->d_trip_orig.as<int4>(), ctx->d_trip.as<int4>(), st);
+    }));
+    FB_HIP(hipStreamSynchronize(st));
+    ctx->nseg = nseg;
+  }
+  // block offsets
+  v = ctx->view();
+  RC(ctx->launch("block_sizes", [&] { return fbk_block_sizes(v, ctx->d_bsz.as<int64_t>(), st); }));
+  RC(scan_i64(ctx, ctx->d_bsz.as<int64_t>(), ctx->d_boff.as<int64_t>(), P));
+  FB_HIP(hipMemcpyAsync(&ctx->blen, ctx->d_boff.as<int64_t>() + P, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  RC(ctx->check_status("find_common_neighbors"));
+  ctx->h_boff_struct.resize(ctx->structs.size() + 1);
+  for (size_t k = 0; k < ctx->structs.size(); k++) {
+    int64_t p0 = ctx->h_noff[ctx->h_st[k].atom_off];
+    FB_HIP(hipMemcpy(&ctx->h_boff_struct[k], ctx->d_boff.as<int64_t>() + p0, sizeof(int64_t), hipMemcpyDeviceToHost));
+  }
+  ctx->h_boff_struct[ctx->structs.size()] = ctx->blen;
+  for (int i = 0; i < 5; i++) {
+    FB_HIP(ctx->d_mat[i].ensure(sizeof(double) * std::max<int64_t>(ctx->blen, 1)));
+    ctx->done[i] = 0;
+  }
+  ctx->lists_ok = true;
+  if (npairs_total) *npairs_total = P;
+  if (ntrip_total) *ntrip_total = T;
+  return FB_OK;
+}
+
+int fb_get_list_sizes(fb_ctx *ctx, int is, int64_t *npairs, int64_t *ntrip, int *mbeta_max) {
+  if (!ctx || !ctx->lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "lists not built") : FB_E_ARG;
+  if (is < 0 || is >= (int)ctx->structs.size()) return ctx->fail(FB_E_ARG, "structure index");
+  const DStruct &d = ctx->h_st[is];
+  if (npairs) *npairs = ctx->h_noff[d.atom_off + d.natoms] - ctx->h_noff[d.atom_off];
+  if (ntrip) *ntrip = ctx->h_toff[d.atom_off + d.natoms] - ctx->h_toff[d.atom_off];
+  if (mbeta_max) *mbeta_max = d.mbeta_max;
+  return FB_OK;
+}
+
+static int d2h(fb_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!dst || bytes == 0) return FB_OK;
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return ctx->fail(FB_E_HIP, "device to host copy", e);
+  return FB_OK;
+}
+
+int fb_get_neighbors(fb_ctx *ctx, int is, int *neighn, int *neigh_j, int *neigh_b, int *neigh_back, int *neigh_self) {
+  if (!ctx || !ctx->lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "lists not built") : FB_E_ARG;
+  if (is < 0 || is >= (int)ctx->structs.size()) return ctx->fail(FB_E_ARG, "structure index");
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  const DStruct &d = ctx->h_st[is];
+  int64_t p0 = ctx->h_noff[d.atom_off], p1 = ctx->h_noff[d.atom_off + d.natoms];
+  RC(d2h(ctx, neighn, ctx->d_neighn.as<int>() + d.atom_off, sizeof(int) * d.natoms));
+  RC(d2h(ctx, neigh_j, ctx->d_neigh_j.as<int>() + p0, sizeof(int) * (p1 - p0)));
+  RC(d2h(ctx, neigh_b, ctx->d_neigh_b.as<int>() + p0, sizeof(int) * (p1 - p0)));
+  RC(d2h(ctx, neigh_back, ctx->d_neigh_back.as<int>() + p0, sizeof(int) * (p1 - p0)));
+  RC(d2h(ctx, neigh_self, ctx->d_neigh_self.as<int>() + d.atom_off, sizeof(int) * d.natoms));
+  return FB_OK;
+}
+
+int fb_get_common(fb_ctx *ctx, int is, int *ncommon, int *iatom_j, int *iatom_b, int *jatom_j, int *jatom_b,
+                  int *neigh_common) {
+  if (!ctx || !ctx->lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "lists not built") : FB_E_ARG;
+  if (is < 0 || is >= (int)ctx->structs.size()) return ctx->fail(FB_E_ARG, "structure index");
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  const DStruct &d = ctx->h_st[is];
+  int64_t t0 = ctx->h_toff[d.atom_off], t1 = ctx->h_toff[d.atom_off + d.natoms];
+  int64_t n = t1 - t0;
+  RC(d2h(ctx, ncommon, ctx->d_ncommon.as<int>() + d.atom_off, sizeof(int) * d.natoms));
+  std::vector<int4> tr(n);
+  std::vector<int> tp(n);
+  RC(d2h(ctx, tr.data(), ctx->d_trip_orig.as<int4>() + t0, sizeof(int4) * n));
+  RC(d2h(ctx, tp.data(), ctx->d_tpair_orig.as<int>() + t0, sizeof(int) * n));
+  for (int64_t t = 0; t < n; t++) {
+    if (iatom_j) iatom_j[t] = tr[t].y - d.atom_off + 1;
+    if (jatom_j) jatom_j[t] = tr[t].z - d.atom_off + 1;
+    if (iatom_b) iatom_b[t] = tr[t].w & 0xffff;
+    if (jatom_b) jatom_b[t] = (tr[t].w >> 16) & 0xffff;
+    if (neigh_common) neigh_common[t] = (int)(tp[t] - ctx->h_noff[tr[t].y] + 1);
+  }
+  return FB_OK;
+}
+
+// ---------------------------------------------------------------- assembly
+static int pre_assemble(fb_ctx *ctx, const char *what) {
+  if (!ctx) return FB_E_ARG;
+  if (!ctx->lists_ok) return ctx->fail(FB_E_ORDER, (std::string(what) + " before build_lists").c_str());
+  FB_HIP(hipSetDevice(ctx->device));
+  if (ctx->structs_dirty) RC(upload_structures(ctx));
+  RC(ctx->clear_status());
+  return FB_OK;
+}
+
+static int run_2c(fb_ctx *ctx, int mask, const char *name) {
+  DView v = ctx->view();
+  return ctx->launch(name, [&] { return fbk_2c(v, mask, ctx->stream); });
+}
+
+static int any_charge(fb_ctx *ctx) {
+  for (auto &s : ctx->structs)
+    for (double q : s.dQ)
+      if (q != 0.0) return 1;
+  return 0;
+}
+
+static int run_3c(fb_ctx *ctx) {
+  DView v = ctx->view();
+  return ctx->launch("assemble_3c_fp64", [&] { return fbk_3c_fp64(v, ctx->stream); });
+}
+
+int fb_assemble_S(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  ctx->reset_events();
+  RC(pre_assemble(ctx, "assemble_S"));
+  RC(run_2c(ctx, 1, "assemble_2c_S"));
+  RC(ctx->check_status("assemble_S"));
+  ctx->done[FB_MAT_S] = 1;
+  return FB_OK;
+}
+
+int fb_assemble_T(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  ctx->reset_events();
+  RC(pre_assemble(ctx, "assemble_T"));
+  RC(run_2c(ctx, 2, "assemble_2c_T"));
+  RC(ctx->check_status("assemble_T"));
+  ctx->done[FB_MAT_T] = 1;
+  return FB_OK;
+}
+
+int fb_assemble_dipole_z(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  ctx->reset_events();
+  RC(pre_assemble(ctx, "assemble_dipole_z"));
+  RC(run_2c(ctx, 4, "assemble_2c_dipole_z"));
+  RC(ctx->check_status("assemble_dipole_z"));
+  ctx->done[FB_MAT_DIP] = 1;
+  return FB_OK;
+}
+
+int fb_assemble_vna_2c(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  ctx->reset_events();
+  RC(pre_assemble(ctx, "assemble_vna_2c"));
+  if (!ctx->done[FB_MAT_S]) return ctx->fail(FB_E_ORDER, "assemble_vna_2c needs assemble_S first");
+  RC(run_2c(ctx, 8, "assemble_2c_vna"));
+  RC(ctx->check_status("assemble_vna_2c"));
+  ctx->done[FB_MAT_VNA] = ctx->done[FB_MAT_VNAO] = 1;
+  return FB_OK;
+}
+
+int fb_assemble_vna_3c(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  ctx->reset_events();
+  RC(pre_assemble(ctx, "assemble_vna_3c"));
+  if (!ctx->done[FB_MAT_VNA] || !ctx->done[FB_MAT_S] || !ctx->done[FB_MAT_DIP])
+    return ctx->fail(FB_E_ORDER, "assemble_vna_3c needs assemble_S, assemble_dipole_z and assemble_vna_2c first");
+  RC(run_3c(ctx));
+  RC(ctx->check_status("assemble_vna_3c"));
+  return FB_OK;
+}
+
+int fb_assemble_all(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  ctx->reset_events();
+  RC(pre_assemble(ctx, "assemble_all"));
+  if (any_charge(ctx)) {
+    RC(run_2c(ctx, 7, "assemble_2c_STD"));
+    RC(run_2c(ctx, 8, "assemble_2c_vna"));
+  } else {
+    RC(run_2c(ctx, 15, "assemble_2c"));
+  }
+  RC(run_3c(ctx));
+  RC(ctx->check_status("assemble_all"));
+  for (int i = 0; i < 5; i++) ctx->done[i] = 1;
+  return FB_OK;
+}
+
+int fb_get_block_len(fb_ctx *ctx, int is, int64_t *len) {
+  if (!ctx || !ctx->lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "lists not built") : FB_E_ARG;
+  if (is < 0 || is >= (int)ctx->structs.size() || !len) return ctx->fail(FB_E_ARG, "structure index");
+  *len = ctx->h_boff_struct[is + 1] - ctx->h_boff_struct[is];
+  return FB_OK;
+}
+
+int fb_get_blocks(fb_ctx *ctx, int is, int matrix, double *out) {
+  if (!ctx || !ctx->lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "lists not built") : FB_E_ARG;
+  if (is < 0 || is >= (int)ctx->structs.size() || matrix < 0 || matrix > 4 || !out)
+    return ctx->fail(FB_E_ARG, "get_blocks arguments");
+  if (!ctx->done[matrix]) return ctx->fail(FB_E_ORDER, "get_blocks: matrix not assembled");
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  int64_t b0 = ctx->h_boff_struct[is], b1 = ctx->h_boff_struct[is + 1];
+  return d2h(ctx, out, ctx->d_mat[matrix].as<double>() + b0, sizeof(double) * (b1 - b0));
+}
+
+int fb_get_device_blocks(fb_ctx *ctx, int matrix, void **dptr, int64_t *len) {
+  if (!ctx || !ctx->lists_ok || matrix < 0 || matrix > 4) return FB_E_ARG;
+  if (dptr) *dptr = ctx->d_mat[matrix].p;
+  if (len) *len = ctx->blen;
+  return FB_OK;
+}
+
+int fb_enable_timing(fb_ctx *ctx, int on) {
+  if (!ctx) return FB_E_ARG;
+  ctx->timing = on != 0;
+  return FB_OK;
+}
+
+int fb_get_timings(fb_ctx *ctx, int max, char *names, float *ms, int *count) {
+  if (!ctx) return FB_E_ARG;
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  int n = 0;
+  for (auto &e : ctx->events) {
+    if (n >= max) break;
+    float t = 0.f;
+    FB_HIP(hipEventElapsedTime(&t, e.a, e.b));
+    if (ms) ms[n] = t;
+    if (names) {
+      std::strncpy(names + 32 * n, e.name.c_str(), 31);
+      names[32 * n + 31] = 0;
+    }
+    n++;
+  }
+  if (count) *count = n;
+  return FB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// Internal data layout shared by the host library and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fireball_hip.h"
+
+#define FB_NIMG 729           // (2*4+1)^3 images, c.SYSTEM/make_cells.f90:126
+#define FB_NTHETA 5           // include/interactions_3c.h:4
+#define FB_MAXSUB2 10         // include/interactions_2c.h:3
+#define FB_MAXSUB3 7          // include/interactions_3c.h:3
+#define FB_NSLOT2 6           // 2c interactions used on the path
+#define FB_P_EQ2 14.39975     // include/constants.h:4
+
+// 2c interaction slots (device-side dense index of the interaction ids of
+// include/interactions_2c.h: overlap 1, kinetic 2, vna_ontopL 3, vna_ontopR 4,
+// vna_atom 5, dipole_z 10)
+enum { SLOT_S = 0, SLOT_T = 1, SLOT_ONL = 2, SLOT_ONR = 3, SLOT_ATOM = 4, SLOT_DIP = 5 };
+
+// status bits raised by kernels (one word per context)
+enum {
+  ST_X_RANGE = 1, ST_X_NEG = 2, ST_TOO_CLOSE = 4, ST_SELF = 8, ST_BACK = 16, ST_NO_TABLE = 32,
+  ST_CAPACITY = 64
+};
+
+struct DSpecies {
+  int ns;
+  int have_d;
+  int nssh[FB_MAXSPECIES];
+  int lssh[FB_MAXSPECIES][FB_MAXSH];
+  int norb[FB_MAXSPECIES];
+  double rcmax[FB_MAXSPECIES];   // max shell rcutoffA (neighbour range)
+  double rcmin[FB_MAXSPECIES];   // min shell rcutoffA (smoothing)
+};
+
+struct DCell2c {
+  int nME, nx;
+  double xmax, dx;
+  int64_t off;       // first value in the FP64 slab, layout [nx][nME]
+  int64_t off32;     // same cell in the FP32 slab
+  int munu;          // first entry in mu/nu/mvalue
+  int pad;
+};
+
+// one (is, js, ks, isub) group of the five Legendre theta cells
+struct DGroup3c {
+  int nME, nx, ny, munu;
+  double xmax, ymax, dx, dy;
+  int64_t off;       // FP64 slab, layout [theta][ny][nx][nME]
+  int64_t off32;     // FP32 slab, layout [ny][nx][theta][nME]
+};
+
+struct DStruct {
+  int atom_off, natoms, mbeta_max, icluster;
+  int64_t pair_off, trip_off;
+  double bmin[3], bmax[3];
+  double rcmax_all;
+};
+
+// all device pointers of a context, passed by value to kernels
+struct DView {
+  DSpecies sp;
+  // tables
+  const DCell2c *cell2c;        // cells
+  const int *idx2c;             // [ns][ns][FB_NSLOT2][FB_MAXSUB2] -> cell or -1
+  const DGroup3c *grp3c;
+  const int *idx3c;             // [ns][ns][ns][FB_MAXSUB3] -> group or -1 (itype = bcna)
+  const double *tab2c;
+  const float *tab2c32;
+  const double *tab3c;
+  const float *tab3c32;
+  const int *mu, *nu, *mv;
+  // structures
+  int nstruct, natoms;
+  const DStruct *st;
+  const double *xl;             // [nstruct][FB_NIMG][3]
+  const double *ratom;          // [natoms][3]
+  const int *imass;             // [natoms]
+  const int *atom_struct;       // [natoms]
+  const double *dQ;             // [natoms][FB_MAXSH]
+  // neighbour lists (CSR over global atoms, per-structure 1-based atom ids)
+  int64_t npairs;
+  const int *neighn;            // [natoms]
+  const int64_t *noff;          // [natoms+1] global pair offset
+  const int *neigh_j, *neigh_b, *neigh_back, *neigh_self;
+  const int *pair_atom;         // [npairs] global iatom (0-based)
+  const int64_t *boff;          // [npairs+1] block offsets (doubles)
+  // triplets, sorted by target pair (segment order = reference accumulation order)
+  int64_t ntrip, nseg;
+  const int4 *trip;             // [ntrip] (ialpha_g, iatom_g, jatom_g, packed beta: ibeta | jbeta<<16)
+  const int *trip_pair;         // [ntrip] target pair (global)
+  const int64_t *seg_off;       // [nseg+1]
+  const int *seg_pair;          // [nseg] target pair of each segment
+  const int *trip_seg;          // [ntrip] segment of each sorted triplet
+  // outputs
+  double *mat[5];
+  int *status;
+};
+
+#define FB_HIP(call)                                                      \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) return ctx->fail(FB_E_HIP, #call, e_);          \
+  } while (0)

@@ -1,0 +1,354 @@
+// Neighbour and common-neighbour (triplet) lists on the GPU, bit-identical to
+// c.SYSTEM/M_neighbors.f90 (find_neighbors :356-519, find_common_neighbors
+// :681-812).
+//
+// One workgroup per central atom walks the candidates in the reference's order
+// (image mbeta outer, atom jatom inner; or ineigh outer, jneigh inner for the
+// triplets) in chunks of the workgroup size; an ordered compaction (wave
+// ballot + cross-wave prefix in LDS) appends the accepted candidates, so no
+// sort is needed and the list order is the reference's.  Distances use the
+// reference's expression with contraction off.  Images whose bounding box lies
+// farther than the cutoff (plus a 1e-6 A margin) from the central atom are
+// skipped: they cannot contain a neighbour.
+#include "fb_device.h"
+
+#define LWG 256
+
+// ordered block-wide exclusive scan of a predicate; returns this thread's
+// position, writes the block total to *total (all threads)
+__device__ __forceinline__ int block_rank(bool pred, int *s_wave, int *total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long m = __ballot(pred);
+  int before = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) s_wave[wave] = __popcll(m);
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < LWG / 64; w++) {
+    int c = s_wave[w];
+    if (w < wave) off += c;
+    tot += c;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + before;
+}
+
+__device__ __forceinline__ bool image_may_hit(const DStruct &S, const double *xlb, const double *r1,
+                                              double reach) {
+  double d2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double lo = S.bmin[k] + xlb[k], hi = S.bmax[k] + xlb[k];
+    double d = fmax(0.0, fmax(lo - r1[k], r1[k] - hi));
+    d2 += d * d;
+  }
+  return d2 <= reach * reach;
+}
+
+// FILL = false: count neighbours (find_neigh_max); FILL = true: write them.
+template <bool FILL>
+__global__ __launch_bounds__(LWG) void k_neighbors(DView v, int *neighn_out, const int64_t *noff,
+                                                   int *neigh_j, int *neigh_b, int *neigh_self,
+                                                   int *pair_atom, int *status) {
+  __shared__ int s_wave[LWG / 64];
+  const int ia = blockIdx.x;  // global atom
+  const int s = v.atom_struct[ia];
+  const DStruct S = v.st[s];
+  const int li = ia - S.atom_off;  // local 0-based
+  const int in1 = v.imass[ia];
+  double r1[3] = {v.ratom[3 * ia], v.ratom[3 * ia + 1], v.ratom[3 * ia + 2]};
+  const double rci = v.sp.rcmax[in1 - 1];
+  const double reach = rci + S.rcmax_all + 1.0e-6;
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  int count = 0;
+  int64_t base = FILL ? noff[ia] : 0;
+  for (int mb = 0; mb <= S.mbeta_max; mb++) {
+    const double *xlb = xls + 3 * mb;
+    if (!image_may_hit(S, xlb, r1, reach)) continue;
+    for (int j0 = 0; j0 < S.natoms; j0 += LWG) {
+      int lj = j0 + threadIdx.x;
+      bool pass = false;
+      if (lj < S.natoms) {
+        int ja = S.atom_off + lj;
+        double r2[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) r2[k] = v.ratom[3 * ja + k] + xlb[k];
+        double z = d_distance(r1, r2);
+        double range_max = rci + v.sp.rcmax[v.imass[ja] - 1] - 0.01;
+        if (lj != li && z < 0.5) atomicOr(status, ST_TOO_CLOSE);
+        pass = z < range_max;
+      }
+      int tot;
+      int pos = block_rank(pass, s_wave, &tot);
+      if (FILL && pass) {
+        int64_t p = base + count + pos;
+        neigh_j[p] = lj + 1;
+        neigh_b[p] = mb;
+        pair_atom[p] = ia;
+        if (lj == li && mb == 0) neigh_self[ia] = count + pos + 1;
+      }
+      count += tot;
+    }
+  }
+  if (!FILL && threadIdx.x == 0) neighn_out[ia] = count;
+}
+
+// neigh_back, c.SYSTEM/M_neighbors.f90:469-486: the jneigh of jatom that
+// points back at (iatom, -xl(mbeta)) within 1e-3 A per component.
+__global__ void k_neigh_back(DView v, int *neigh_back, int *status) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= v.npairs) return;
+  int ia = v.pair_atom[p];
+  int s = v.atom_struct[ia];
+  const DStruct S = v.st[s];
+  int li1 = ia - S.atom_off + 1;
+  int ja = S.atom_off + v.neigh_j[p] - 1;
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  const double *xm = xls + 3 * v.neigh_b[p];
+  int found = -999;
+  int64_t jo = v.noff[ja];
+  int nj = v.neighn[ja];
+  for (int jn = 0; jn < nj; jn++) {
+    int ka = v.neigh_j[jo + jn];
+    const double *xj = xls + 3 * v.neigh_b[jo + jn];
+    if (li1 != ka || fabs(xm[0] + xj[0]) > 1.0e-3 || fabs(xm[1] + xj[1]) > 1.0e-3 ||
+        fabs(xm[2] + xj[2]) > 1.0e-3) {
+    } else {
+      if (found != -999) atomicOr(status, ST_BACK);
+      found = jn + 1;
+    }
+  }
+  if (found == -999) atomicOr(status, ST_BACK);
+  neigh_back[p] = found;
+}
+
+// common neighbours of ialpha.  FILL = false counts (find_common_max plus the
+// drop rule of find_common_neighbors, so the counts are exact); FILL = true
+// writes the triplet and its target pair.
+template <bool FILL>
+__global__ __launch_bounds__(LWG) void k_common(DView v, int *ncommon_out, const int64_t *toff,
+                                                int4 *trip, int *trip_pair, int *status) {
+  __shared__ int s_wave[LWG / 64];
+  const int ial = blockIdx.x;
+  const int s = v.atom_struct[ial];
+  const DStruct S = v.st[s];
+  const int lal = ial - S.atom_off + 1;  // 1-based local
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  const int64_t o = v.noff[ial];
+  const int n = v.neighn[ial];
+  const int64_t ncand = (int64_t)n * n;
+  int count = 0;
+  int64_t base = FILL ? toff[ial] : 0;
+  for (int64_t c0 = 0; c0 < ncand; c0 += LWG) {
+    int64_t c = c0 + threadIdx.x;
+    bool pass = false;
+    int ia = 0, ib = 0, ja = 0, jb = 0, m = -9999;
+    if (c < ncand) {
+      int in = (int)(c / n), jn = (int)(c % n);
+      ia = v.neigh_j[o + in];
+      ib = v.neigh_b[o + in];
+      ja = v.neigh_j[o + jn];
+      jb = v.neigh_b[o + jn];
+      if (!(ia == lal && ib == 0) && !(ja == lal && jb == 0) && in != jn) {
+        int gia = S.atom_off + ia - 1, gja = S.atom_off + ja - 1;
+        double r1[3], r2[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          r1[k] = v.ratom[3 * gia + k] + xls[3 * ib + k];
+          r2[k] = v.ratom[3 * gja + k] + xls[3 * jb + k];
+        }
+        double z = d_distance(r1, r2);
+        double range_max = v.sp.rcmax[v.imass[gia] - 1] + v.sp.rcmax[v.imass[gja] - 1] - 0.01;
+        if (z < range_max) {
+          int64_t io = v.noff[gia];
+          int ni = v.neighn[gia];
+          for (int kn = 0; kn < ni; kn++) {
+            int ka = v.neigh_j[io + kn];
+            int kb = v.neigh_b[io + kn];
+            int gka = S.atom_off + ka - 1;
+            double r3[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) r3[k] = xls[3 * kb + k] + v.ratom[3 * gka + k] - v.ratom[3 * gia + k];
+            if (fabs(r3[0] - r2[0] + r1[0]) < 1.0e-4 && fabs(r3[1] - r2[1] + r1[1]) < 1.0e-4 &&
+                fabs(r3[2] - r2[2] + r1[2]) < 1.0e-4) {
+              m = kn + 1;
+              break;
+            }
+          }
+          pass = (m != -9999);
+        }
+      }
+    }
+    int tot;
+    int pos = block_rank(pass, s_wave, &tot);
+    if (FILL && pass) {
+      int64_t t = base + count + pos;
+      int gia = S.atom_off + ia - 1, gja = S.atom_off + ja - 1;
+      trip[t] = make_int4(ial, gia, gja, (ib & 0xffff) | (jb << 16));
+      trip_pair[t] = (int)(v.noff[gia] + m - 1);
+      // mneigh is recovered from the target pair: m = trip_pair - noff[iatom] + 1
+    }
+    count += tot;
+  }
+  if (!FILL && threadIdx.x == 0) ncommon_out[ial] = count;
+}
+
+// segment bookkeeping after the stable sort of triplets by target pair
+__global__ void k_seg_mark(int64_t ntrip, const int *sorted_pair, int *flag) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntrip) return;
+  flag[t] = (t == 0 || sorted_pair[t] != sorted_pair[t - 1]) ? 1 : 0;
+}
+
+__global__ void k_seg_fill(int64_t ntrip, const int *sorted_pair, const int *segid_incl, int64_t *seg_off,
+                           int *seg_pair, int *trip_seg) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntrip) return;
+  int sid = segid_incl[t] - 1;
+  trip_seg[t] = sid;
+  if (t == 0 || sorted_pair[t] != sorted_pair[t - 1]) {
+    seg_off[sid] = t;
+    seg_pair[sid] = sorted_pair[t];
+  }
+  if (t == ntrip - 1) seg_off[sid + 1] = ntrip;
+}
+
+__global__ void k_gather_trip(int64_t ntrip, const int *perm, const int4 *trip_in, int4 *trip_out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntrip) return;
+  trip_out[t] = trip_in[perm[t]];
+}
+
+__global__ void k_iota(int64_t n, int *a) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = (int)t;
+}
+
+// block offsets: boff[p] = sum of norb_i * norb_j over earlier pairs
+__global__ void k_block_sizes(DView v, int64_t *sz) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= v.npairs) return;
+  int ia = v.pair_atom[p];
+  int s = v.atom_struct[ia];
+  int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
+  sz[p] = (int64_t)v.sp.norb[v.imass[ia] - 1] * v.sp.norb[v.imass[ja] - 1];
+}
+
+// ----------------------------------------------------------------- launchers
+#include <hipcub/hipcub.hpp>
+
+extern "C" {
+
+int fbk_neighbors_count(const DView &v, int *neighn, int *status, hipStream_t st) {
+  hipLaunchKernelGGL(k_neighbors<false>, dim3(v.natoms), dim3(LWG), 0, st, v, neighn, (const int64_t *)nullptr,
+                     (int *)nullptr, (int *)nullptr, (int *)nullptr, (int *)nullptr, status);
+  return (int)hipGetLastError();
+}
+
+int fbk_neighbors_fill(const DView &v, const int64_t *noff, int *neigh_j, int *neigh_b, int *neigh_self,
+                       int *pair_atom, int *status, hipStream_t st) {
+  hipLaunchKernelGGL(k_neighbors<true>, dim3(v.natoms), dim3(LWG), 0, st, v, (int *)nullptr, noff, neigh_j,
+                     neigh_b, neigh_self, pair_atom, status);
+  return (int)hipGetLastError();
+}
+
+int fbk_neigh_back(const DView &v, int *neigh_back, int *status, hipStream_t st) {
+  if (v.npairs == 0) return 0;
+  hipLaunchKernelGGL(k_neigh_back, dim3((unsigned)((v.npairs + 255) / 256)), dim3(256), 0, st, v, neigh_back,
+                     status);
+  return (int)hipGetLastError();
+}
+
+int fbk_common_count(const DView &v, int *ncommon, int *status, hipStream_t st) {
+  hipLaunchKernelGGL(k_common<false>, dim3(v.natoms), dim3(LWG), 0, st, v, ncommon, (const int64_t *)nullptr,
+                     (int4 *)nullptr, (int *)nullptr, status);
+  return (int)hipGetLastError();
+}
+
+int fbk_common_fill(const DView &v, const int64_t *toff, int4 *trip, int *trip_pair, int *status,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_common<true>, dim3(v.natoms), dim3(LWG), 0, st, v, (int *)nullptr, toff, trip, trip_pair,
+                     status);
+  return (int)hipGetLastError();
+}
+
+int fbk_block_sizes(const DView &v, int64_t *sz, hipStream_t st) {
+  if (v.npairs == 0) return 0;
+  hipLaunchKernelGGL(k_block_sizes, dim3((unsigned)((v.npairs + 255) / 256)), dim3(256), 0, st, v, sz);
+  return (int)hipGetLastError();
+}
+
+// exclusive prefix sums (int -> int64 and int64 -> int64); out has n+1 entries
+int fbk_scan_i32(const int *in, int64_t *out, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t st) {
+  // out[0] = 0, out[i+1] = sum(in[0..i])
+  if (tmp == nullptr) {
+    return (int)hipcub::DeviceScan::InclusiveSum(nullptr, *tmp_bytes, in, out + 1, (int)n, st);
+  }
+  hipError_t e = hipMemsetAsync(out, 0, sizeof(int64_t), st);
+  if (e != hipSuccess) return (int)e;
+  if (n == 0) return 0;
+  return (int)hipcub::DeviceScan::InclusiveSum(tmp, *tmp_bytes, in, out + 1, (int)n, st);
+}
+
+int fbk_scan_i64(const int64_t *in, int64_t *out, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t st) {
+  if (tmp == nullptr) {
+    return (int)hipcub::DeviceScan::InclusiveSum(nullptr, *tmp_bytes, in, out + 1, (int)n, st);
+  }
+  hipError_t e = hipMemsetAsync(out, 0, sizeof(int64_t), st);
+  if (e != hipSuccess) return (int)e;
+  if (n == 0) return 0;
+  return (int)hipcub::DeviceScan::InclusiveSum(tmp, *tmp_bytes, in, out + 1, (int)n, st);
+}
+
+// stable sort of triplet ids by target pair; then segments
+int fbk_sort_pairs(const int *keys_in, int *keys_out, const int *vals_in, int *vals_out, int64_t n, int end_bit,
+                   void *tmp, size_t *tmp_bytes, hipStream_t st) {
+  return (int)hipcub::DeviceRadixSort::SortPairs(tmp, *tmp_bytes, keys_in, keys_out, vals_in, vals_out, (int)n,
+                                                 0, end_bit, st);
+}
+
+int fbk_scan_incl_i32(const int *in, int *out, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t st) {
+  return (int)hipcub::DeviceScan::InclusiveSum(tmp, *tmp_bytes, in, out, (int)n, st);
+}
+
+int fbk_iota(int *a, int64_t n, hipStream_t st) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a);
+  return (int)hipGetLastError();
+}
+
+int fbk_seg_mark(int64_t ntrip, const int *sorted_pair, int *flag, hipStream_t st) {
+  if (ntrip == 0) return 0;
+  hipLaunchKernelGGL(k_seg_mark, dim3((unsigned)((ntrip + 255) / 256)), dim3(256), 0, st, ntrip, sorted_pair,
+                     flag);
+  return (int)hipGetLastError();
+}
+
+int fbk_seg_fill(int64_t ntrip, const int *sorted_pair, const int *segid_incl, int64_t *seg_off, int *seg_pair,
+                 int *trip_seg, hipStream_t st) {
+  if (ntrip == 0) return 0;
+  hipLaunchKernelGGL(k_seg_fill, dim3((unsigned)((ntrip + 255) / 256)), dim3(256), 0, st, ntrip, sorted_pair,
+                     segid_incl, seg_off, seg_pair, trip_seg);
+  return (int)hipGetLastError();
+}
+
+int fbk_gather_trip(int64_t ntrip, const int *perm, const int4 *in, int4 *out, hipStream_t st) {
+  if (ntrip == 0) return 0;
+  hipLaunchKernelGGL(k_gather_trip, dim3((unsigned)((ntrip + 255) / 256)), dim3(256), 0, st, ntrip, perm, in,
+                     out);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"
+
+__global__ void k_fill_i32(int64_t n, int *a, int val) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = val;
+}
+
+extern "C" int fbk_fill_i32(int *a, int64_t n, int val, hipStream_t st) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, val);
+  return (int)hipGetLastError();
+}
