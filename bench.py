@@ -1,0 +1,181 @@
+"""Benchmark of the FIREBALL 2c/3c H+S assembly on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--precision mixed|fp64]
+
+A step is one H+S build of the workload -- S, T, dipole_z, Vna-2c (ontop +
+atom) and Vna-3c blocks for every pair of the structure -- with positions,
+tables and lists already resident in HBM.  The default workload is the
+4096-atom periodic Si supercell (8x8x8 diamond cells, displacements
+N(0, 0.08 A), sp3 basis, rc = 5.4 bohr, synthetic smooth 2c/3c Fdata).
+``value`` is 3c triplets per second of whole H+S builds; ms_per_step is the H+S
+build time.  The plain binary64 kernels run beside the selected precision and
+the per-matrix max-abs difference is reported.
+
+Prints one JSON line.  Only one GPU is supported: --gpus N > 1 exits.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "thunder-master_amd"))
+
+from fireball_hip import synth  # noqa: E402
+from fireball_hip.lib import Assembler  # noqa: E402
+
+METRIC = "3c triplets/s + H+S build ms at <=1e-6 eV err; FP32/MLP vs plain FP64, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    0: "Si64 periodic diamond 2x2x2 (a=5.43 A, sigma 0.05 A, seed 0), sp3, rc 5.4 bohr",
+    1: "Si4096 periodic diamond 8x8x8 (a=5.43 A, sigma 0.08 A, seed 0), sp3, rc 5.4 bohr",
+    2: "CHNO3000 random box 1.0 g/cm3 (C:H:N:O 30:50:10:10, seed 1), rc 4.5/5.0 bohr",
+    3: "batch of 512 structures (16..1024 atoms, half Si, half CHNO, seed 2)",
+    4: "aC8192 random box 2.0 g/cm3 (seed 3), sp3, rc 5.0 bohr",
+}
+MATS = ("S", "T", "DIP", "VNA", "VNAO")
+
+
+def algorithmic_bytes_3c(asm, nstruct):
+    """Compulsory HBM bytes of one 3c launch: the sorted triplet records
+    (int4 + target pair + segment id = 24 B) and one store per targeted block."""
+    ntrip = 0
+    tgt_bytes = 0
+    for k in range(nstruct):
+        L = asm.lists(k)
+        ntrip += L["ntriplets"]
+        if L["ntriplets"] == 0:
+            continue
+        st = asm.structs[k]
+        norb = np.array([sp.norb for sp in asm.tables.species])
+        off = np.concatenate([[0], np.cumsum(L["neighn"])])
+        atom_of_pair = np.repeat(np.arange(st.natoms), L["neighn"])
+        tgt = np.unique(off[L["com_iatom_j"] - 1] + L["com_mneigh"] - 1)
+        nmu = norb[st.imass[atom_of_pair[tgt]] - 1]
+        nnu = norb[st.imass[L["neigh_j"][tgt] - 1] - 1]
+        tgt_bytes += int(np.sum(nmu * nnu * 8))
+    return 24 * ntrip + tgt_bytes, ntrip
+
+
+def time_builds(asm, steps, warmup):
+    asm.enable_timing(True)
+    for _ in range(warmup):
+        asm.assemble_all()
+    asm.synchronize()
+    kern = {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        asm.assemble_all()
+        for name, ms in asm.timings():
+            kern.setdefault(name, []).append(ms)
+    asm.synchronize()
+    t1 = time.perf_counter()
+    asm.enable_timing(False)
+    return (t1 - t0) * 1e3 / steps, {k: float(np.mean(v)) for k, v in kern.items()}
+
+
+def cpu_baseline():
+    """The reference's own Fortran (serial flang build, oracle/_ref) on a
+    bounded sample: a 512-atom Si cell (4x4x4 diamond, sigma 0.08 A)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import refrun
+    st = synth.diamond_si((4, 4, 4), sigma=0.08, seed=0)
+    ts = synth.make_tables([synth.SI])
+    if refrun.available():
+        nrep = 3
+        r = refrun.run(ts, st, nrep=nrep)
+        t = sum(r["times"][k] for k in ("S", "T", "DIP", "VNA2C", "VNA3C")) / nrep
+        return {"value": r["ntriplets"] / t, "unit": "triplets/s", "cores": 1, "kind": "reference",
+                "sample": f"Si512 4x4x4 diamond (sigma 0.08 A): {r['ntriplets']} triplets, H+S build "
+                          f"{t * 1e3:.1f} ms (assemble_S/T/dipole_z/vna_2c/vna_3c, serial flang -O2, "
+                          f"mean of {nrep}); neighbour lists {r['times']['lists']:.2f} s not included",
+                "h_s_build_ms": t * 1e3}
+    import oracle
+    o = oracle.System(oracle.Tables(ts), st)
+    o.build_lists()
+    t0 = time.perf_counter()
+    o.assemble()
+    t = time.perf_counter() - t0
+    return {"value": o.lists["ntriplets"] / t, "unit": "triplets/s", "cores": 1, "kind": "port",
+            "sample": f"Si512 4x4x4 diamond: C oracle H+S build {t * 1e3:.1f} ms", "h_s_build_ms": t * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--precision", default="mixed", choices=("mixed", "fp64"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-compare", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    species, structs = synth.config(args.config)
+    ts = synth.make_tables(species)
+    asm = Assembler(ts, precision=args.precision)
+    for st in structs:
+        asm.add_structure(st)
+    t0 = time.perf_counter()
+    npairs, ntrip = asm.build_lists()
+    lists_ms = (time.perf_counter() - t0) * 1e3
+    print(f"# lists: {npairs} pairs, {ntrip} triplets in {lists_ms:.1f} ms", file=sys.stderr, flush=True)
+
+    ms, kern = time_builds(asm, args.steps, args.warmup)
+    print(f"# {args.precision}: {ms:.3f} ms per H+S build; kernels {kern}", file=sys.stderr, flush=True)
+    res_main = {m: np.concatenate([asm.blocks(m, k) for k in range(len(structs))]) for m in MATS}
+
+    extra = {}
+    if not args.no_fp64_compare and args.precision != "fp64":
+        asm.set_precision("fp64")
+        ms64, kern64 = time_builds(asm, args.steps, args.warmup)
+        res64 = {m: np.concatenate([asm.blocks(m, k) for k in range(len(structs))]) for m in MATS}
+        err = {m: float(np.max(np.abs(res_main[m] - res64[m]))) for m in MATS}
+        extra.update(fp64_ms_per_step=ms64, speedup_vs_fp64=ms64 / ms, max_abs_err_vs_fp64_eV=err,
+                     fp64_kernels_ms=kern64)
+        asm.set_precision(args.precision)
+        print(f"# fp64: {ms64:.3f} ms; err {err}", file=sys.stderr, flush=True)
+
+    # dominant kernel: the 3c launch
+    k3 = [k for k in kern if k.startswith("assemble_3c")]
+    dom = max(kern, key=kern.get) if not k3 else k3[0]
+    bytes3c, _ = algorithmic_bytes_3c(asm, len(structs))
+    achieved = bytes3c / (kern[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+            "kernel_ms": kern[dom], "algorithmic_bytes": bytes3c}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline()
+        except Exception as e:  # the baseline is reported, never required
+            cpu = {"error": str(e)}
+
+    out = {
+        "metric": METRIC, "value": ntrip / (ms * 1e-3), "unit": "triplets/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS[args.config], "natoms": int(sum(s.natoms for s in structs)),
+                   "nstructures": len(structs), "npairs": npairs, "ntriplets": ntrip,
+                   "precision": args.precision, "h_s_build_ms": ms, "lists_ms": lists_ms,
+                   "kernels_ms": kern, **extra},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
