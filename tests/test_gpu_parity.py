@@ -136,3 +136,64 @@ def test_order_errors():
     with pytest.raises(FireballError) as e:
         asm.assemble_vna_3c()
     assert e.value.code == -8
+
+
+# ----------------------------------------------------------- mixed precision
+TOL_MIXED = 1.0e-6      # eV, max-abs per block element (the bar of the FP32 path)
+
+
+@pytest.mark.parametrize("name", golden_cases.NAMES)
+def test_golden_mixed(golden, name):
+    print(f"mixed case {name}", flush=True)
+    g = golden(name)
+    ts, st = golden_cases.inputs(g)
+    asm = _gpu(ts, [st], precision="mixed")
+    r = asm.run_all(0)
+    for k in LIST_KEYS:
+        assert np.array_equal(r[k], g[k]), k
+    o = oracle.System(oracle.Tables(ts), st).run_all()
+    for m in MATS:
+        err = float(np.max(np.abs(r[m] - o[m])))
+        assert err <= TOL_MIXED, (m, err)
+    # the 2c blocks stay binary64 in the mixed path
+    for m in ("S", "T", "DIP", "VNAO"):
+        np.testing.assert_allclose(r[m], o[m], rtol=0, atol=TOL64, err_msg=m)
+
+
+def test_si4096_mixed_vs_fp64():
+    """Config-1 scale: the mixed path against the binary64 GPU path (itself
+    bit-identical to the oracle at the smaller sizes above)."""
+    from fireball_hip import synth
+    species, structs = synth.config(1)
+    ts = synth.make_tables(species)
+    asm = _gpu(ts, structs, precision="mixed")
+    mixed = asm.blocks("VNA")
+    asm.set_precision("fp64")
+    asm.assemble_all()
+    ref = asm.blocks("VNA")
+    err = float(np.max(np.abs(mixed - ref)))
+    print(f"Si4096 mixed vs fp64 VNA max-abs {err:.3e} eV", flush=True)
+    assert err <= TOL_MIXED
+    # size-independent invariants of the lists at full size
+    L = asm.lists(0)
+    off = np.concatenate([[0], np.cumsum(L["neighn"])])
+    back = off[L["neigh_j"] - 1] + L["neigh_back"] - 1
+    atom_of_pair = np.repeat(np.arange(1, structs[0].natoms + 1), L["neighn"])
+    assert np.array_equal(L["neigh_j"][back], atom_of_pair)
+    assert np.array_equal(back[back], np.arange(len(back)))
+
+
+def test_generic_shells_against_oracle():
+    """Species outside the s / sp fast path (two s shells) run the generic kernels."""
+    from fireball_hip import synth
+    sp = [synth.Species(3, [0, 0], [5.0, 4.6]), synth.C_SP]
+    ts = synth.make_tables(sp, seed=17)
+    st = synth.random_box({3: 6, 6: 10}, 0.8, 1.0, 21, sp)
+    synth.random_dQ([st], sp, seed=3)
+    asm = _gpu(ts, [st])
+    r = asm.run_all(0)
+    o = oracle.System(oracle.Tables(ts), st).run_all()
+    for key in LIST_KEYS:
+        assert np.array_equal(r[key], o[key]), key
+    for m in MATS:
+        np.testing.assert_allclose(r[m], o[m], rtol=0, atol=TOL64, err_msg=m)

@@ -223,3 +223,167 @@ extern "C" int fbk_3c_fp64(const DView &v, hipStream_t st) {
   hipLaunchKernelGGL(k_3c_fp64, dim3((unsigned)((v.ntrip + T3 - 1) / T3)), dim3(T3), 0, st, v);
   return (int)hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// s/p species with canonical tables: the same algorithm, with compile-time
+// matrix-element counts and the register-resident rotation of fb_sp.h (same
+// operations and order as the generic path, without private-memory arrays).
+#include "fb_sp.h"
+
+template <int NME>
+__device__ __forceinline__ int getMEs_3c_sp(const DView &v, int gid, double x, double y, double cost, double *F) {
+  if (gid < 0) return ST_NO_TABLE;
+  const DGroup3c g = v.grp3c[gid];
+  const int nx = g.nx, ny = g.ny;
+  double p[5], sint;
+  d_legendre(cost, p, &sint);
+  const double dx = g.xmax / (nx - 1.0), dy = g.ymax / (ny - 1.0);
+  int ix = (int)(x / dx) + 1, iy = (int)(y / dy) + 1;
+  if (ix < 2) ix = 2;
+  if (iy < 2) iy = 2;
+  if (ix > nx - 2) ix = nx - 2;
+  if (iy > ny - 2) iy = ny - 2;
+  const double px = x / dx - ix + 1.0, py = y / dy - iy + 1.0;
+  const double *tab = v.tab3c + g.off;
+  const int64_t tstride = (int64_t)ny * nx * NME;
+#define T3S(th, i, j, m) tab[(th) * tstride + ((int64_t)((j) - 1) * nx + ((i) - 1)) * NME + (m)]
+  const double gx = (T3S(0, ix + 1, iy, 0) - T3S(0, ix, iy, 0)) / dx;
+  const double gy = (T3S(0, ix, iy + 1, 0) - T3S(0, ix, iy, 0)) / dy;
+  const double gg = gx * gx + gy * gy;
+#pragma unroll
+  for (int m = 0; m < NME; m++) F[m] = 0.0;
+  for (int th = 0; th < FB_NTHETA; th++) {
+#pragma unroll
+    for (int m = 0; m < NME; m++) {
+      double interim;
+      if (gg < 1.0e-6) {
+        interim = ((1.0 - px - py) * T3S(th, ix, iy, m) + px * T3S(th, ix + 1, iy, m)) + py * T3S(th, ix, iy + 1, m);
+      } else {
+        double gk[4];
+#pragma unroll
+        for (int k = -1; k <= 2; k++) {
+          double f1m1 = T3S(th, ix + k, iy - 1, m);
+          double f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+          double f0p3 = 3.0 * T3S(th, ix + k, iy, m), f0p6 = 2.0 * f0p3;
+          double f1p3 = 3.0 * T3S(th, ix + k, iy + 1, m), f1p6 = 2.0 * f1p3;
+          double f2p1 = T3S(th, ix + k, iy + 2, m);
+          double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+          double bb2 = f1m3 - f0p6 + f1p3;
+          double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+          double bb0 = f0p6;
+          gk[k + 1] = ((bb3 * py + bb2) * py + bb1) * py + bb0;
+        }
+        double f1m1 = gk[0], f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+        double f0p3 = 3.0 * gk[1], f0p6 = 2.0 * f0p3;
+        double f1p3 = 3.0 * gk[2], f1p6 = 2.0 * f1p3;
+        double f2p1 = gk[3];
+        double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+        double bb2 = f1m3 - f0p6 + f1p3;
+        double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+        double bb0 = f0p6;
+        interim = (((bb3 * px + bb2) * px + bb1) * px + bb0) / 36.0;
+      }
+      F[m] = F[m] + p[th] * interim;
+    }
+  }
+#undef T3S
+  // sin(theta) on the mvalue = 1 entries (the last nME - nME(delta m = 0) of the canonical order)
+  constexpr int NM0 = (NME == 10) ? 6 : (NME == 3 ? 2 : 1);
+#pragma unroll
+  for (int m = NM0; m < NME; m++) F[m] = F[m] * sint;
+  return 0;
+}
+
+template <int CLS>
+__device__ __forceinline__ int triplet_fp64_sp(const DView &v, int64_t t, const int4 tr, const Geo3 &G, double *c) {
+  constexpr int NME = nme3c_of(CLS);
+  constexpr int NBL = ((CLS & 2) ? 4 : 1) * ((CLS & 1) ? 4 : 1);
+  double pm[9], F[NME], bx[16];
+  d_pmat<double>(G.eps, pm);
+  int status = getMEs_3c_sp<NME>(v, group3c_id(v, G.in1, G.in2, G.indna, 0), G.x, G.z, G.cost, F);
+  rot3c_sp<double>(CLS, pm, F, bx);
+#pragma unroll
+  for (int i = 0; i < NBL; i++) c[i] = bx[i] * FB_P_EQ2;
+  bool charged = false;
+  for (int isorp = 1; isorp <= v.sp.nssh[G.indna - 1]; isorp++) charged |= (v.dQ[tr.x * FB_MAXSH + isorp - 1] != 0.0);
+  if (charged) {
+    const int p = v.trip_pair[t];
+    const double *sb = v.mat[FB_MAT_S] + v.boff[p];
+    const double *db = v.mat[FB_MAT_DIP] + v.boff[p];
+    for (int isorp = 1; isorp <= v.sp.nssh[G.indna - 1]; isorp++) {
+      double dQ = v.dQ[tr.x * FB_MAXSH + isorp - 1];
+      if (dQ == 0.0) continue;
+      status |= getMEs_3c_sp<NME>(v, group3c_id(v, G.in1, G.in2, G.indna, isorp), G.x, G.z, G.cost, F);
+      rot3c_sp<double>(CLS, pm, F, bx);
+#pragma unroll
+      for (int i = 0; i < NBL; i++) {
+        double sterm = sb[i] / 2.0, dterm = db[i] / G.z;
+        double emnpl = (sterm - dterm) / G.d13 + (sterm + dterm) / G.d23;
+        c[i] = c[i] + dQ * (G.stinky * bx[i] + (1.0 - G.stinky) * emnpl) * FB_P_EQ2;
+      }
+    }
+  }
+  return status;
+}
+
+__global__ __launch_bounds__(T3) void k_3c_fp64_sp(DView v) {
+  __shared__ double s_c[T3][NB + 1];
+  const int64_t ts = (int64_t)blockIdx.x * T3;
+  const int64_t te = min(ts + T3, v.ntrip);
+  int64_t first, last, r0, r1;
+  if (!tile_segments(v, ts, te, first, last, r0, r1)) return;
+  const int64_t my = first + threadIdx.x;
+  const bool own = my <= last;
+  int64_t a = 0, b = 0;
+  if (own) { a = v.seg_off[my]; b = v.seg_off[my + 1]; }
+  double acc[NB];
+#pragma unroll
+  for (int i = 0; i < NB; i++) acc[i] = 0.0;
+  int status = 0;
+  for (int64_t c0 = r0; c0 < r1; c0 += T3) {
+    const int64_t t = c0 + threadIdx.x;
+    double c[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) c[i] = 0.0;
+    if (t < r1) {
+      const int4 tr = v.trip[t];
+      Geo3 G;
+      d_geo3(v, tr, G);
+      const int cls = (G.nmu == 4 ? 2 : 0) + (G.nnu == 4 ? 1 : 0);
+      switch (cls) {
+        case 3: status |= triplet_fp64_sp<3>(v, t, tr, G, c); break;
+        case 2: status |= triplet_fp64_sp<2>(v, t, tr, G, c); break;
+        case 1: status |= triplet_fp64_sp<1>(v, t, tr, G, c); break;
+        default: status |= triplet_fp64_sp<0>(v, t, tr, G, c); break;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; i++) s_c[threadIdx.x][i] = c[i];
+    __syncthreads();
+    if (own) {
+      int64_t k0 = max(a, c0), k1 = min(b, c0 + T3);
+      for (int64_t k = k0; k < k1; k++)
+#pragma unroll
+        for (int i = 0; i < NB; i++) acc[i] = acc[i] + s_c[k - c0][i];
+    }
+    __syncthreads();
+  }
+  if (own) {
+    const int p = v.seg_pair[my];
+    const int ia = v.pair_atom[p];
+    const int s = v.atom_struct[ia];
+    const int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
+    const int nb = v.sp.norb[v.imass[ia] - 1] * v.sp.norb[v.imass[ja] - 1];
+    double *dst = v.mat[FB_MAT_VNA] + v.boff[p];
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+      if (i < nb) dst[i] = acc[i];
+  }
+  if (status) atomicOr(v.status, status);
+}
+
+extern "C" int fbk_3c_fp64_sp(const DView &v, hipStream_t st) {
+  if (v.ntrip == 0) return 0;
+  hipLaunchKernelGGL(k_3c_fp64_sp, dim3((unsigned)((v.ntrip + T3 - 1) / T3)), dim3(T3), 0, st, v);
+  return (int)hipGetLastError();
+}

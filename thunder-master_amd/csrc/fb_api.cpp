@@ -38,6 +38,9 @@ int fbk_gather_trip(int64_t ntrip, const int *perm, const int4 *in, int4 *out, h
 int fbk_fill_i32(int *a, int64_t n, int val, hipStream_t st);
 int fbk_2c(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64(const DView &v, hipStream_t st);
+int fbk_3c_mixed(const DView &v, int chunk, hipStream_t st);
+int fbk_2c_sp(const DView &v, int mask, hipStream_t st);
+int fbk_3c_fp64_sp(const DView &v, hipStream_t st);
 }
 
 namespace {
@@ -100,6 +103,56 @@ int slot_of(int itype) {
   return -1;
 }
 
+// canonical (mu, nu, mvalue) lists of make_munu (c.SYSTEM/M_integrals_2c.f90:297-335)
+// and make_munu_3c (c.SYSTEM/M_integrals_3c.f90:299-477), s and p shells
+struct MeList {
+  std::vector<int> mu, nu, mv;
+};
+
+MeList munu2c(const std::vector<int> &li, const std::vector<int> &lj) {
+  MeList r;
+  int n1 = 0;
+  for (int l1 : li) {
+    n1 += l1 + 1;
+    int n2 = 0;
+    for (int l2 : lj) {
+      n2 += l2 + 1;
+      for (int m = -std::min(l1, l2); m <= std::min(l1, l2); m++) {
+        r.mu.push_back(n1 + m);
+        r.nu.push_back(n2 + m);
+        r.mv.push_back(0);
+      }
+      n2 += l2;
+    }
+    n1 += l1;
+  }
+  return r;
+}
+
+MeList munu3c_sp(const std::vector<int> &li, const std::vector<int> &lj) {
+  MeList r = munu2c(li, lj);
+  int n1 = 0;
+  for (int l1 : li) {
+    n1 += l1 + 1;
+    int n2 = 0;
+    for (int l2 : lj) {
+      n2 += l2 + 1;
+      if (l1 == 0 && l2 != 0) { r.mu.push_back(n1); r.nu.push_back(n2 + 1); r.mv.push_back(1); }
+      if (l1 == 1) {
+        r.mu.push_back(n1 + 1); r.nu.push_back(n2); r.mv.push_back(1);
+        if (l2 != 0) { r.mu.push_back(n1); r.nu.push_back(n2 + 1); r.mv.push_back(1); }
+      }
+      n2 += l2;
+    }
+    n1 += l1;
+  }
+  return r;
+}
+
+bool same_list(const MeList &a, const std::vector<int> &mu, const std::vector<int> &nu, const std::vector<int> &mv) {
+  return a.mu == mu && a.nu == nu && a.mv == mv;
+}
+
 }  // namespace
 
 struct fb_ctx {
@@ -108,6 +161,7 @@ struct fb_ctx {
   std::string err;
   int precision = FB_PREC_FP64;
   int method3c = FB_3C_TABLE;
+  int chunk3c = 128;      // triplets per workgroup chunk of the mixed 3c kernel
 
   DSpecies sp{};
   int nZ[FB_MAXSPECIES] = {0};
@@ -116,6 +170,9 @@ struct fb_ctx {
   std::vector<HCell3c> c3;
   bool tables_ok = false;
   DevBuf d_cell2c, d_idx2c, d_grp3c, d_idx3c, d_tab2c, d_tab2c32, d_tab3c, d_tab3c32, d_mu, d_nu, d_mv;
+  DevBuf d_grid2c, d_pden, d_wg2c;
+  int canon_sp = 0;
+  int nwg2c = 0;
 
   std::vector<HStruct> structs;
   bool structs_dirty = true;
@@ -165,6 +222,11 @@ struct fb_ctx {
     v.mu = d_mu.as<int>();
     v.nu = d_nu.as<int>();
     v.mv = d_mv.as<int>();
+    v.grid2c = d_grid2c.as<DGrid2c>();
+    v.pden = d_pden.as<double>();
+    v.canon_sp = canon_sp;
+    v.wg2c = d_wg2c.as<int>();
+    v.nwg2c = nwg2c;
     v.nstruct = (int)structs.size();
     v.natoms = natoms;
     v.st = d_st.as<DStruct>();
@@ -256,7 +318,7 @@ struct fb_ctx {
 
   ~fb_ctx() {
     DevBuf *bufs[] = {&d_cell2c, &d_idx2c, &d_grp3c, &d_idx3c, &d_tab2c, &d_tab2c32, &d_tab3c, &d_tab3c32,
-                      &d_mu, &d_nu, &d_mv, &d_st, &d_xl, &d_ratom, &d_imass, &d_atom_struct, &d_dQ,
+                      &d_mu, &d_nu, &d_mv, &d_grid2c, &d_pden, &d_wg2c, &d_st, &d_xl, &d_ratom, &d_imass, &d_atom_struct, &d_dQ,
                       &d_neighn, &d_noff, &d_neigh_j, &d_neigh_b, &d_neigh_back, &d_neigh_self, &d_pair_atom,
                       &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
                       &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_tmp,
@@ -526,11 +588,64 @@ int fb_commit_tables(fb_ctx *ctx) {
         for (int m = 0; m < g.nME; m++) dst[(q * FB_NTHETA + t) * g.nME + m] = (float)th[t]->data[q * g.nME + m];
     int is, js, ks, isub;
     std::tie(is, js, ks, isub) = kv.first;
+    // the mixed kernel addresses every charge subtype with the neutral group's stencil
+    if (isub > 0) {
+      auto k0 = groups.find(std::make_tuple(is, js, ks, 0));
+      if (k0 == groups.end() || k0->second[0]->nx != g.nx || k0->second[0]->ny != g.ny ||
+          k0->second[0]->nME != g.nME || k0->second[0]->xmax != g.xmax || k0->second[0]->ymax != g.ymax)
+        return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: 3c charge subtypes must share the neutral grid");
+    }
     idx3[(((size_t)(is - 1) * ns + (js - 1)) * ns + (ks - 1)) * FB_MAXSUB3 + isub] = (int)grps.size();
     grps.push_back(g);
   }
+  // distinct 2c grids and their Lagrange denominators (host binary64, same
+  // expression as e.FDATA/M_Fdata_2c.f90:379-385)
+  std::vector<DGrid2c> grids;
+  std::vector<double> pden;
+  for (auto &d : cells) {
+    int gidx = -1;
+    for (size_t k = 0; k < grids.size(); k++)
+      if (grids[k].nx == d.nx && grids[k].dx == d.dx) gidx = (int)k;
+    if (gidx < 0) {
+      DGrid2c gr{};
+      gr.nx = d.nx;
+      gr.dx = d.dx;
+      gr.pden_off = (int64_t)pden.size();
+      for (int ileft = 1; ileft <= d.nx - 6; ileft++) {
+        double xx[7];
+        for (int k = 0; k < 7; k++) xx[k] = (double)(ileft + k - 1) * d.dx;
+        for (int is = 0; is < 7; is++)
+          for (int ip = 0; ip < 7; ip++) pden.push_back(ip == is ? 0.0 : 1.0 / (xx[is] - xx[ip]));
+      }
+      gidx = (int)grids.size();
+      grids.push_back(gr);
+    }
+    d.grid = gidx;
+  }
+  // s / p species with canonical matrix-element order -> register-resident kernels
+  int canon = 1;
+  std::vector<std::vector<int>> shells(ns);
+  for (int i = 0; i < ns; i++) {
+    for (int k = 0; k < ctx->sp.nssh[i]; k++) shells[i].push_back(ctx->sp.lssh[i][k]);
+    bool ok = (shells[i] == std::vector<int>{0}) || (shells[i] == std::vector<int>{0, 1});
+    if (!ok) canon = 0;
+  }
+  if (canon) {
+    for (auto &c : ctx->c2) {
+      int slot = slot_of(c.itype);
+      if (slot < 0) continue;
+      MeList e = (slot == SLOT_ATOM) ? munu2c(shells[c.is - 1], shells[c.is - 1])
+                                     : munu2c(shells[c.is - 1], shells[c.js - 1]);
+      if (!same_list(e, c.mu, c.nu, c.mv)) canon = 0;
+    }
+    for (auto &c : ctx->c3) {
+      if (c.itype != 3) continue;
+      MeList e = munu3c_sp(shells[c.is - 1], shells[c.js - 1]);
+      if (!same_list(e, c.mu, c.nu, c.mv)) canon = 0;
+    }
+  }
+  ctx->canon_sp = canon;
   std::vector<float> tab2f(tab2.begin(), tab2.end());
-  RC(upload(ctx, ctx->d_cell2c, cells.data(), cells.size() * sizeof(DCell2c), "upload cells2c"));
   RC(upload(ctx, ctx->d_idx2c, idx2.data(), idx2.size() * sizeof(int), "upload idx2c"));
   RC(upload(ctx, ctx->d_grp3c, grps.data(), grps.size() * sizeof(DGroup3c), "upload groups3c"));
   RC(upload(ctx, ctx->d_idx3c, idx3.data(), idx3.size() * sizeof(int), "upload idx3c"));
@@ -538,6 +653,9 @@ int fb_commit_tables(fb_ctx *ctx) {
   RC(upload(ctx, ctx->d_tab2c32, tab2f.data(), tab2f.size() * sizeof(float), "upload tab2c32"));
   RC(upload(ctx, ctx->d_tab3c, tab3.data(), tab3.size() * sizeof(double), "upload tab3c"));
   RC(upload(ctx, ctx->d_tab3c32, tab3f.data(), tab3f.size() * sizeof(float), "upload tab3c32"));
+  RC(upload(ctx, ctx->d_cell2c, cells.data(), cells.size() * sizeof(DCell2c), "upload cells2c"));
+  RC(upload(ctx, ctx->d_grid2c, grids.data(), grids.size() * sizeof(DGrid2c), "upload grids2c"));
+  RC(upload(ctx, ctx->d_pden, pden.data(), pden.size() * sizeof(double), "upload pden"));
   RC(upload(ctx, ctx->d_mu, mu.data(), mu.size() * sizeof(int), "upload mu"));
   RC(upload(ctx, ctx->d_nu, nu.data(), nu.size() * sizeof(int), "upload nu"));
   RC(upload(ctx, ctx->d_mv, mv.data(), mv.size() * sizeof(int), "upload mvalue"));
@@ -780,6 +898,28 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
     FB_HIP(ctx->d_mat[i].ensure(sizeof(double) * std::max<int64_t>(ctx->blen, 1)));
     ctx->done[i] = 0;
   }
+  // 2c workgroups: whole atoms, at most 16 atoms and 256 pairs each (an atom
+  // with more than 256 neighbours gets a workgroup of its own)
+  {
+    std::vector<int> wg;
+    wg.push_back(0);
+    int na = 0;
+    int64_t np = 0;
+    for (int a = 0; a < N; a++) {
+      int64_t n = ctx->h_noff[a + 1] - ctx->h_noff[a];
+      if (na > 0 && (na == 16 || np + n > 256)) {
+        wg.push_back(a);
+        na = 0;
+        np = 0;
+      }
+      na++;
+      np += n;
+    }
+    wg.push_back(N);
+    ctx->nwg2c = (int)wg.size() - 1;
+    RC(upload(ctx, ctx->d_wg2c, wg.data(), wg.size() * sizeof(int), "upload wg2c"));
+    FB_HIP(hipStreamSynchronize(st));
+  }
   ctx->lists_ok = true;
   if (npairs_total) *npairs_total = P;
   if (ntrip_total) *ntrip_total = T;
@@ -852,6 +992,7 @@ static int pre_assemble(fb_ctx *ctx, const char *what) {
 
 static int run_2c(fb_ctx *ctx, int mask, const char *name) {
   DView v = ctx->view();
+  if (ctx->canon_sp) return ctx->launch(name, [&] { return fbk_2c_sp(v, mask, ctx->stream); });
   return ctx->launch(name, [&] { return fbk_2c(v, mask, ctx->stream); });
 }
 
@@ -864,7 +1005,13 @@ static int any_charge(fb_ctx *ctx) {
 
 static int run_3c(fb_ctx *ctx) {
   DView v = ctx->view();
-  return ctx->launch("assemble_3c_fp64", [&] { return fbk_3c_fp64(v, ctx->stream); });
+  if (ctx->canon_sp) {
+    if (ctx->precision == FB_PREC_MIXED)
+      return ctx->launch("assemble_3c_mixed", [&] { return fbk_3c_mixed(v, ctx->chunk3c, ctx->stream); });
+    return ctx->launch("assemble_3c_fp64", [&] { return fbk_3c_fp64_sp(v, ctx->stream); });
+  }
+  // generic shells: binary64 path only
+  return ctx->launch("assemble_3c_fp64_generic", [&] { return fbk_3c_fp64(v, ctx->stream); });
 }
 
 int fb_assemble_S(fb_ctx *ctx) {

@@ -40,7 +40,16 @@ struct DCell2c {
   int64_t off;       // first value in the FP64 slab, layout [nx][nME]
   int64_t off32;     // same cell in the FP32 slab
   int munu;          // first entry in mu/nu/mvalue
-  int pad;
+  int grid;          // distinct (nx, dx) grid, see DGrid2c
+};
+
+// Lagrange denominators 1/(xx_is - xx_ip) of every 7-point window of a grid,
+// evaluated on the host with the reference's expression
+// (e.FDATA/M_Fdata_2c.f90:379-389); layout [ileft-1][is][ip]
+struct DGrid2c {
+  int nx, pad;
+  double dx;
+  int64_t pden_off;
 };
 
 // one (is, js, ks, isub) group of the five Legendre theta cells
@@ -71,6 +80,9 @@ struct DView {
   const double *tab3c;
   const float *tab3c32;
   const int *mu, *nu, *mv;
+  const DGrid2c *grid2c;
+  const double *pden;
+  int canon_sp;                 // every species s or sp, every table in canonical ME order
   // structures
   int nstruct, natoms;
   const DStruct *st;
@@ -86,6 +98,8 @@ struct DView {
   const int *neigh_j, *neigh_b, *neigh_back, *neigh_self;
   const int *pair_atom;         // [npairs] global iatom (0-based)
   const int64_t *boff;          // [npairs+1] block offsets (doubles)
+  const int *wg2c;              // [nwg2c+1] first atom of each 2c workgroup
+  int nwg2c;
   // triplets, sorted by target pair (segment order = reference accumulation order)
   int64_t ntrip, nseg;
   const int4 *trip;             // [ntrip] (ialpha_g, iatom_g, jatom_g, packed beta: ibeta | jbeta<<16)

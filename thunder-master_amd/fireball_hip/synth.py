@@ -36,7 +36,7 @@ P_MAXTHETA = 5              # include/interactions_3c.h:4
 P_OVERLAP, P_KINETIC, P_VNA_ONTOPL, P_VNA_ONTOPR, P_VNA_ATOM, P_DIPOLE_Z = 1, 2, 3, 4, 5, 10
 P_BCNA = 3
 
-MASS = {1: 1.008, 6: 12.011, 7: 14.007, 8: 15.999, 14: 28.0855}
+MASS = {1: 1.008, 3: 6.94, 6: 12.011, 7: 14.007, 8: 15.999, 14: 28.0855}
 SYMBOL = {1: "H", 6: "C", 7: "N", 8: "O", 14: "Si"}
 AMU_G = 1.66053906660e-24
 
