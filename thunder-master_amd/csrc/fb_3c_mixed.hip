@@ -30,7 +30,9 @@ struct TPrm {
   int base, ncol, rowstride;
   int nme, cls, m1, gsub;     // ME count, pair class, method-1 flag, charged (1) or not (0)
   int p;                      // target pair
-  float wx[4], wy[4];         // Lagrange weights; method 1 uses wx = (1-px-py, px, py)
+  float wx[4], wy[4];         // cubic Lagrange weights (method 3)
+  float w1[3];                // (1-px-py, px, py) of the 3-point rule (method 1)
+  int m1k[FB_MAXSH];          // method-1 flag of each charged group (its own gradient test)
   float P[5], sint;
   float pm[9];
   float dq[FB_MAXSH];         // shell charges of the third atom
@@ -54,11 +56,11 @@ __device__ __forceinline__ void cubic_w(double p, float *w) {
 
 // one (triplet, ME) stencil over the five theta cells, Legendre-summed
 __device__ __forceinline__ float interp_me(const float *__restrict__ tab, int base, int rowstride, int ncol, int nme,
-                                           int m, const TPrm &q) {
+                                           int m, const TPrm &q, int m1) {
   float acc = 0.f;
-  if (q.m1) {
+  if (m1) {
     const float *c00 = tab + base + rowstride + ncol + m;   // (ix, iy)
-    const float w0 = q.wx[0], w1 = q.wx[1], w2 = q.wx[2];
+    const float w0 = q.w1[0], w1 = q.w1[1], w2 = q.w1[2];
 #pragma unroll
     for (int th = 0; th < FB_NTHETA; th++) {
       const float *c = c00 + th * nme;
@@ -184,27 +186,22 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       const double gg = gx * gx + gy * gy;
       q.m1 = (gg < 1.0e-6) ? 1 : 0;
       float w[4];
-      if (q.m1) {
-        q.wx[0] = (float)(1.0 - px - py);
-        q.wx[1] = (float)px;
-        q.wx[2] = (float)py;
-        q.wx[3] = 0.f;
-        q.wy[0] = q.wy[1] = q.wy[2] = q.wy[3] = 0.f;
-      } else {
-        cubic_w(px, w);
+      q.w1[0] = (float)(1.0 - px - py);
+      q.w1[1] = (float)px;
+      q.w1[2] = (float)py;
+      cubic_w(px, w);
 #pragma unroll
-        for (int k = 0; k < 4; k++) q.wx[k] = w[k];
-        cubic_w(py, w);
+      for (int k = 0; k < 4; k++) q.wx[k] = w[k];
+      cubic_w(py, w);
 #pragma unroll
-        for (int k = 0; k < 4; k++) q.wy[k] = w[k];
-      }
+      for (int k = 0; k < 4; k++) q.wy[k] = w[k];
       q.ncol = FB_NTHETA * g.nME;
       q.rowstride = g.nx * q.ncol;
       q.base = (int)(g.off32 + ((int64_t)(iy - 2) * g.nx + (ix - 2)) * q.ncol);
       // charged terms (M_assemble_3c_DOGS.f90:250-318)
       q.gsub = 0;
 #pragma unroll
-      for (int k = 0; k < FB_MAXSH; k++) { q.dq[k] = 0.f; q.gk[k] = 0; }
+      for (int k = 0; k < FB_MAXSH; k++) { q.dq[k] = 0.f; q.gk[k] = 0; q.m1k[k] = 0; }
       for (int isorp = 1; isorp <= v.sp.nssh[indna - 1]; isorp++) {
         double dq = v.dQ[ial * FB_MAXSH + isorp - 1];
         if (dq == 0.0) continue;
@@ -214,7 +211,14 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
           continue;
         }
         q.dq[isorp - 1] = (float)dq;
-        q.gk[isorp - 1] = (int)(v.grp3c[gk].off32 - g.off32);
+        const DGroup3c gq = v.grp3c[gk];
+        q.gk[isorp - 1] = (int)(gq.off32 - g.off32);
+        // getMEs_Fdata_3c chooses the method from each table's own gradient
+        const double *tq = v.tab3c + gq.off;
+        const double h00 = tq[((int64_t)(iy - 1) * g.nx + (ix - 1)) * g.nME];
+        const double hx = (tq[((int64_t)(iy - 1) * g.nx + ix) * g.nME] - h00) / dx;
+        const double hy = (tq[((int64_t)iy * g.nx + (ix - 1)) * g.nME] - h00) / dy;
+        q.m1k[isorp - 1] = (hx * hx + hy * hy < 1.0e-6) ? 1 : 0;
         q.gsub = 1;
       }
       if (q.gsub) {
@@ -232,12 +236,13 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       const int t = it / NME3, m = it - t * NME3;
       const TPrm &q = sq[t];
       if (m >= q.nme) continue;
-      float F = interp_me(v.tab3c32, q.base, q.rowstride, q.ncol, q.nme, m, q);
+      float F = interp_me(v.tab3c32, q.base, q.rowstride, q.ncol, q.nme, m, q, q.m1);
       if (q.gsub) {
         for (int k = 0; k < FB_MAXSH; k++) {
           const float dq = q.dq[k];
           if (dq == 0.f) continue;
-          F = __builtin_fmaf(dq * q.stinky, interp_me(v.tab3c32, q.base + q.gk[k], q.rowstride, q.ncol, q.nme, m, q), F);
+          F = __builtin_fmaf(dq * q.stinky,
+                             interp_me(v.tab3c32, q.base + q.gk[k], q.rowstride, q.ncol, q.nme, m, q, q.m1k[k]), F);
         }
       }
       sF[t][m] = F;
