@@ -65,20 +65,36 @@ def algorithmic_bytes_3c(asm, nstruct):
 
 
 def time_builds(asm, steps, warmup):
+    """K back-to-back H+S builds (enqueued asynchronously) with every kernel
+    bracketed by HIP events on the launch stream; returns wall ms per build and
+    the mean duration of each kernel."""
     asm.enable_timing(True)
     for _ in range(warmup):
-        asm.assemble_all()
+        asm.assemble_all(sync=False)
     asm.synchronize()
+    asm.timings(4096)                     # drop the warm-up events
     kern = {}
     t0 = time.perf_counter()
     for _ in range(steps):
-        asm.assemble_all()
-        for name, ms in asm.timings():
-            kern.setdefault(name, []).append(ms)
+        asm.assemble_all(sync=False)
     asm.synchronize()
     t1 = time.perf_counter()
+    for name, ms in asm.timings(4096):
+        kern.setdefault(name, []).append(ms)
     asm.enable_timing(False)
-    return (t1 - t0) * 1e3 / steps, {k: float(np.mean(v)) for k, v in kern.items()}
+    return (t1 - t0) * 1e3 / steps, {k: float(np.mean(v)) for k, v in kern.items() if k != "clear_status"}
+
+
+def time_graph(asm, steps, warmup):
+    """K H+S builds replayed from the captured HIP graph, no per-kernel events."""
+    for _ in range(warmup):
+        asm.assemble_all(sync=False)
+    asm.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        asm.assemble_all(sync=False)
+    asm.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
 
 
 def cpu_baseline():
@@ -131,8 +147,15 @@ def main():
     lists_ms = (time.perf_counter() - t0) * 1e3
     print(f"# lists: {npairs} pairs, {ntrip} triplets in {lists_ms:.1f} ms", file=sys.stderr, flush=True)
 
+    chunk = None
+    if args.precision == "mixed":
+        asm.assemble_all()
+        chunk = asm.autotune(5)
+        print(f"# autotuned 3c chunk: {chunk} triplets per workgroup", file=sys.stderr, flush=True)
     ms, kern = time_builds(asm, args.steps, args.warmup)
-    print(f"# {args.precision}: {ms:.3f} ms per H+S build; kernels {kern}", file=sys.stderr, flush=True)
+    ms_graph = time_graph(asm, args.steps, args.warmup)
+    print(f"# {args.precision}: {ms:.3f} ms per H+S build (graph replay {ms_graph:.3f} ms); kernels {kern}",
+          file=sys.stderr, flush=True)
     res_main = {m: np.concatenate([asm.blocks(m, k) for k in range(len(structs))]) for m in MATS}
 
     extra = {}
@@ -170,6 +193,7 @@ def main():
         "config": {"workload": WORKLOADS[args.config], "natoms": int(sum(s.natoms for s in structs)),
                    "nstructures": len(structs), "npairs": npairs, "ntriplets": ntrip,
                    "precision": args.precision, "h_s_build_ms": ms, "lists_ms": lists_ms,
+                   "chunk3c": chunk, "ms_per_step_graph_replay": ms_graph,
                    "kernels_ms": kern, **extra},
         "roofline": roof,
         "cpu_baseline": cpu,

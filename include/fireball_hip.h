@@ -121,7 +121,10 @@ int fb_assemble_T(fb_ctx *ctx);          /* j.ASSEMBLERS/M_assemble_2c_DOGS.f90:
 int fb_assemble_dipole_z(fb_ctx *ctx);   /* j.ASSEMBLERS/M_assemble_2c_DOGS.f90:376-499  */
 int fb_assemble_vna_2c(fb_ctx *ctx);     /* j.ASSEMBLERS/M_assemble_2c_DOGS.f90:523-829  */
 int fb_assemble_vna_3c(fb_ctx *ctx);     /* j.ASSEMBLERS/M_assemble_3c_DOGS.f90:86-340   */
-/* S, T, dipole_z and vna_2c in one 2c launch, then vna_3c: the H+S build. */
+/* S, T, dipole_z and vna_2c in one 2c launch, then vna_3c: the H+S build.
+ * Asynchronous: it returns once the build is enqueued (replayed from a
+ * captured HIP graph unless timing is on); a kernel-detected error is returned
+ * by the next fb_synchronize, fb_get_blocks or assemble call. */
 int fb_assemble_all(fb_ctx *ctx);
 
 /* --- results ------------------------------------------------------------ */
@@ -129,10 +132,22 @@ int fb_get_block_len(fb_ctx *ctx, int istruct, int64_t *len);
 int fb_get_blocks(fb_ctx *ctx, int istruct, int matrix, double *out);
 int fb_get_device_blocks(fb_ctx *ctx, int matrix, void **dptr, int64_t *len);
 
+/* --- tuning ------------------------------------------------------------
+ * No counterpart in the reference (its assembler has no launch parameters).
+ * FB_TUNE_CHUNK3C: triplets per workgroup chunk of the mixed 3c kernel (64,
+ * 128 or 256), which sets its LDS footprint.  fb_autotune times every
+ * candidate on the current lists (nrep launches each, HIP events) and keeps
+ * the fastest; it returns the chosen value in *chosen. */
+enum fb_tune { FB_TUNE_CHUNK3C = 0 };
+int fb_set_tuning(fb_ctx *ctx, int param, int value);
+int fb_get_tuning(fb_ctx *ctx, int param, int *value);
+int fb_autotune(fb_ctx *ctx, int nrep, int *chosen);
+
 /* --- measurement -------------------------------------------------------
  * With timing on, every kernel launch of the library is bracketed by HIP
- * events on the launch stream; fb_get_timings returns the last call's
- * kernels (name, ms), at most max entries; names is max*32 chars. */
+ * events on the launch stream; fb_get_timings synchronizes and returns the
+ * kernels launched since the previous fb_get_timings (name, ms), at most max
+ * entries; names is max*32 chars. */
 int fb_enable_timing(fb_ctx *ctx, int on);
 int fb_get_timings(fb_ctx *ctx, int max, char *names, float *ms, int *count);
 

@@ -197,3 +197,46 @@ def test_generic_shells_against_oracle():
         assert np.array_equal(r[key], o[key]), key
     for m in MATS:
         np.testing.assert_allclose(r[m], o[m], rtol=0, atol=TOL64, err_msg=m)
+
+
+def test_graph_replays_match_eager():
+    """assemble_all replays a captured HIP graph; every replay, with eager
+    builds and host synchronisations in between, must equal the eager result."""
+    from fireball_hip import synth
+    from fireball_hip.lib import Assembler
+    sp = [synth.SI]
+    ts = synth.make_tables(sp)
+    st = synth.diamond_si((2, 2, 2), sigma=0.06, seed=11)
+    for precision in ("fp64", "mixed"):
+        asm = Assembler(ts, precision=precision)
+        asm.add_structure(st)
+        asm.build_lists()
+        asm.enable_timing(True)            # timing on: eager launches
+        asm.assemble_all()
+        asm.timings(64)
+        eager = {m: asm.blocks(m) for m in MATS}
+        asm.enable_timing(False)           # timing off: graph replays
+        for rep in range(4):
+            asm.assemble_all(sync=(rep % 2 == 0))
+            asm.synchronize()
+            for m in MATS:
+                assert np.array_equal(asm.blocks(m), eager[m]), (precision, rep, m)
+            asm.assemble_S()               # eager work between replays
+            asm.synchronize()
+
+
+def test_async_error_surfaces_at_synchronize():
+    from fireball_hip import synth
+    from fireball_hip.lib import Assembler, FireballError
+    ts = synth.make_tables([synth.SI])
+    for c in ts.cells2c.values():
+        c.xmax = 2.0
+    pos = np.array([[1.0, 1.0, 1.0], [3.5, 1.0, 1.0], [1.0, 3.4, 1.0]])
+    st = synth.Structure(pos, np.ones(3, np.int32), np.eye(3) * 20, 1)
+    asm = Assembler(ts)
+    asm.add_structure(st)
+    asm.build_lists()
+    asm.assemble_all(sync=False)
+    with pytest.raises(FireballError) as e:
+        asm.synchronize()
+    assert e.value.code == -3

@@ -11,11 +11,11 @@
 //      (gradient of ME 1 of theta cell 1 against 1e-6, from the binary64
 //      table), so every discrete decision equals the reference's.  Stencil
 //      weights, Legendre P0..P4 and the p rotation go to LDS in FP32.
-//   B  one lane per (triplet, matrix element): 4x4 (or 3-point) stencil of
-//      the FP32 slab [iy][ix][theta][ME] for the five theta cells and the
-//      Legendre sum; sin(theta) on mvalue = 1 elements.
-//   C  one lane per triplet: Slater-Koster rotation (fb_sp.h) in FP32, times
-//      P_eq2, plus the charged terms.
+//   B  one lane per (triplet, theta, matrix element): 4x4 (or 3-point)
+//      stencil of the FP32 slab [iy][ix][theta][ME]; the lanes of one triplet
+//      read the 5*nME contiguous values of each stencil point together.
+//   C  one lane per triplet: Legendre sum, sin(theta) on mvalue = 1 elements,
+//      Slater-Koster rotation (fb_sp.h) in FP32, times P_eq2, charged terms.
 //   D  one lane per owned segment: binary64 accumulation of the FP32
 //      contributions in segment order; one store of the final block.
 // Binary64 adds run at the FP32 rate on gfx950 without packed math, so the
@@ -25,6 +25,7 @@
 
 #define NB (FB_MO * FB_MO)
 #define NME3 10
+#define NS3 (FB_NTHETA * NME3)
 
 struct TPrm {
   int base, ncol, rowstride;
@@ -54,38 +55,27 @@ __device__ __forceinline__ void cubic_w(double p, float *w) {
   w[3] = (float)(pp1 * p * pm1 / 6.0);
 }
 
-// one (triplet, ME) stencil over the five theta cells, Legendre-summed
-__device__ __forceinline__ float interp_me(const float *__restrict__ tab, int base, int rowstride, int ncol, int nme,
-                                           int m, const TPrm &q, int m1) {
-  float acc = 0.f;
+// one (triplet, theta, ME) slot s = theta*nME + ME of the stencil: the lanes of
+// a triplet read the 5*nME contiguous values of every stencil point together
+__device__ __forceinline__ float interp_slot(const float *__restrict__ tab, int base, int rowstride, int ncol, int s,
+                                             const TPrm &q, int m1) {
   if (m1) {
-    const float *c00 = tab + base + rowstride + ncol + m;   // (ix, iy)
-    const float w0 = q.w1[0], w1 = q.w1[1], w2 = q.w1[2];
-#pragma unroll
-    for (int th = 0; th < FB_NTHETA; th++) {
-      const float *c = c00 + th * nme;
-      float s = w0 * c[0];
-      s = __builtin_fmaf(w1, c[ncol], s);
-      s = __builtin_fmaf(w2, c[rowstride], s);
-      acc = __builtin_fmaf(q.P[th], s, acc);
-    }
-    return acc;
+    const float *c00 = tab + base + rowstride + ncol + s;   // (ix, iy)
+    float v = q.w1[0] * c00[0];
+    v = __builtin_fmaf(q.w1[1], c00[ncol], v);
+    return __builtin_fmaf(q.w1[2], c00[rowstride], v);
   }
+  const float *c = tab + base + s;
   const float wy0 = q.wy[0], wy1 = q.wy[1], wy2 = q.wy[2], wy3 = q.wy[3];
+  float acc = 0.f;
 #pragma unroll
-  for (int th = 0; th < FB_NTHETA; th++) {
-    const float *c = tab + base + th * nme + m;
-    float s = 0.f;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const float *col = c + i * ncol;
-      float g = wy0 * col[0];
-      g = __builtin_fmaf(wy1, col[rowstride], g);
-      g = __builtin_fmaf(wy2, col[2 * rowstride], g);
-      g = __builtin_fmaf(wy3, col[3 * rowstride], g);
-      s = __builtin_fmaf(q.wx[i], g, s);
-    }
-    acc = __builtin_fmaf(q.P[th], s, acc);
+  for (int i = 0; i < 4; i++) {
+    const float *col = c + i * ncol;
+    float g = wy0 * col[0];
+    g = __builtin_fmaf(wy1, col[rowstride], g);
+    g = __builtin_fmaf(wy2, col[2 * rowstride], g);
+    g = __builtin_fmaf(wy3, col[3 * rowstride], g);
+    acc = __builtin_fmaf(q.wx[i], g, acc);
   }
   return acc;
 }
@@ -93,7 +83,7 @@ __device__ __forceinline__ float interp_me(const float *__restrict__ tab, int ba
 template <int CH>
 __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   __shared__ TPrm sq[CH];
-  __shared__ float sF[CH][NME3 + 1];
+  __shared__ float sT[CH][NS3 + 1];
   __shared__ float sC[CH][NB + 1];
   const int tid = threadIdx.x;
   const int64_t ts = (int64_t)blockIdx.x * CH;
@@ -231,21 +221,20 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       }
     }
     __syncthreads();
-    // ---------------- phase B: interpolation, one lane per (triplet, ME)
-    for (int it = tid; it < nt * NME3; it += CH) {
-      const int t = it / NME3, m = it - t * NME3;
+    // ---------------- phase B: interpolation, one lane per (triplet, theta, ME)
+    for (int it = tid; it < nt * NS3; it += CH) {
+      const int t = it / NS3, sl = it - t * NS3;
       const TPrm &q = sq[t];
-      if (m >= q.nme) continue;
-      float F = interp_me(v.tab3c32, q.base, q.rowstride, q.ncol, q.nme, m, q, q.m1);
+      if (sl >= q.ncol) continue;
+      float F = interp_slot(v.tab3c32, q.base, q.rowstride, q.ncol, sl, q, q.m1);
       if (q.gsub) {
         for (int k = 0; k < FB_MAXSH; k++) {
           const float dq = q.dq[k];
           if (dq == 0.f) continue;
-          F = __builtin_fmaf(dq * q.stinky,
-                             interp_me(v.tab3c32, q.base + q.gk[k], q.rowstride, q.ncol, q.nme, m, q, q.m1k[k]), F);
+          F = __builtin_fmaf(dq * q.stinky, interp_slot(v.tab3c32, q.base + q.gk[k], q.rowstride, q.ncol, sl, q, q.m1k[k]), F);
         }
       }
-      sF[t][m] = F;
+      sT[t][sl] = F;
     }
     __syncthreads();
     // ---------------- phase C: rotation, one lane per triplet
@@ -254,7 +243,15 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       float F[NME3], pm[9], X[16];
       const int nm0 = (q.cls == 3) ? 6 : (q.cls == 0 ? 1 : 2);
 #pragma unroll
-      for (int m = 0; m < NME3; m++) F[m] = (m < q.nme) ? sF[tid][m] * (m >= nm0 ? q.sint : 1.f) : 0.f;
+      for (int m = 0; m < NME3; m++) {
+        float f = 0.f;
+        if (m < q.nme) {
+#pragma unroll
+          for (int th = 0; th < FB_NTHETA; th++) f = __builtin_fmaf(q.P[th], sT[tid][th * q.nme + m], f);
+          if (m >= nm0) f *= q.sint;
+        }
+        F[m] = f;
+      }
 #pragma unroll
       for (int k = 0; k < 9; k++) pm[k] = q.pm[k];
 #pragma unroll

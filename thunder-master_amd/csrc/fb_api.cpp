@@ -191,6 +191,16 @@ struct fb_ctx {
   bool lists_ok = false;
   int done[5] = {0, 0, 0, 0, 0};
 
+  // deferred status of asynchronous H+S builds
+  int *h_status = nullptr;      // pinned
+  bool pending = false;
+  // captured H+S build
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long graph_key = -1;
+  long lists_gen = 0;
+  bool use_graph = true;
+
   bool timing = false;
   struct Ev {
     std::string name;
@@ -280,6 +290,12 @@ struct fb_ctx {
       (void)hipEventRecord(b, stream);
       events.push_back({name, a, b});
     }
+    if (events.size() > 8192) {  // timing left on without readers: keep the newest
+      hipStreamSynchronize(stream);
+      pool.push_back(events.front().a);
+      pool.push_back(events.front().b);
+      events.erase(events.begin());
+    }
     if (rc != 0) return fail(FB_E_HIP, name, (hipError_t)rc);
     return FB_OK;
   }
@@ -298,6 +314,19 @@ struct fb_ctx {
     if (e != hipSuccess) return fail(FB_E_HIP, where, e);
     e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return fail(FB_E_HIP, where, e);
+    return decode_status(st, where);
+  }
+
+  // result of the last asynchronous build, if one is outstanding
+  int settle(const char *where) {
+    if (!pending) return FB_OK;
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return fail(FB_E_HIP, where, e);
+    pending = false;
+    return decode_status(*h_status, where);
+  }
+
+  int decode_status(int st, const char *where) {
     if (st == 0) return FB_OK;
     std::string w(where);
     if (st & ST_TOO_CLOSE) return fail(FB_E_ATOMS_TOO_CLOSE, (w + ": atoms dangerously close (< 0.5 A)").c_str());
@@ -327,6 +356,9 @@ struct fb_ctx {
     for (auto &m : d_mat) m.release();
     reset_events();
     for (auto e : pool) (void)hipEventDestroy(e);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (h_status) (void)hipHostFree(h_status);
     if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 };
@@ -384,10 +416,12 @@ int fb_create(fb_ctx **out, int device) {
   }
   ctx->stream = ctx->own_stream;
   e = ctx->d_status.ensure(sizeof(int) * 4);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_status, sizeof(int) * 4);
   if (e != hipSuccess) {
     delete ctx;
     return FB_E_HIP;
   }
+  ctx->h_status[0] = 0;
   *out = ctx;
   return FB_OK;
 }
@@ -405,13 +439,14 @@ const char *fb_last_error(const fb_ctx *ctx) { return ctx ? ctx->err.c_str() : "
 int fb_set_stream(fb_ctx *ctx, void *s) {
   if (!ctx) return FB_E_ARG;
   ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+  ctx->graph_key = -1;  // a captured graph is tied to its stream's device
   return FB_OK;
 }
 
 int fb_synchronize(fb_ctx *ctx) {
   if (!ctx) return FB_E_ARG;
   FB_HIP(hipStreamSynchronize(ctx->stream));
-  return FB_OK;
+  return ctx->settle("assemble_all");
 }
 
 int fb_set_precision(fb_ctx *ctx, int mode) {
@@ -784,7 +819,6 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
   if (!ctx->tables_ok) return ctx->fail(FB_E_ORDER, "build_lists before commit_tables");
   if (ctx->structs.empty()) return ctx->fail(FB_E_ORDER, "build_lists without structures");
   FB_HIP(hipSetDevice(ctx->device));
-  ctx->reset_events();
   RC(upload_structures(ctx));
   const int N = ctx->natoms;
   RC(ctx->clear_status());
@@ -921,6 +955,7 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
     FB_HIP(hipStreamSynchronize(st));
   }
   ctx->lists_ok = true;
+  ctx->lists_gen++;
   if (npairs_total) *npairs_total = P;
   if (ntrip_total) *ntrip_total = T;
   return FB_OK;
@@ -985,6 +1020,7 @@ static int pre_assemble(fb_ctx *ctx, const char *what) {
   if (!ctx) return FB_E_ARG;
   if (!ctx->lists_ok) return ctx->fail(FB_E_ORDER, (std::string(what) + " before build_lists").c_str());
   FB_HIP(hipSetDevice(ctx->device));
+  RC(ctx->settle("previous assemble_all"));
   if (ctx->structs_dirty) RC(upload_structures(ctx));
   RC(ctx->clear_status());
   return FB_OK;
@@ -1016,7 +1052,6 @@ static int run_3c(fb_ctx *ctx) {
 
 int fb_assemble_S(fb_ctx *ctx) {
   if (!ctx) return FB_E_ARG;
-  ctx->reset_events();
   RC(pre_assemble(ctx, "assemble_S"));
   RC(run_2c(ctx, 1, "assemble_2c_S"));
   RC(ctx->check_status("assemble_S"));
@@ -1026,7 +1061,6 @@ int fb_assemble_S(fb_ctx *ctx) {
 
 int fb_assemble_T(fb_ctx *ctx) {
   if (!ctx) return FB_E_ARG;
-  ctx->reset_events();
   RC(pre_assemble(ctx, "assemble_T"));
   RC(run_2c(ctx, 2, "assemble_2c_T"));
   RC(ctx->check_status("assemble_T"));
@@ -1036,7 +1070,6 @@ int fb_assemble_T(fb_ctx *ctx) {
 
 int fb_assemble_dipole_z(fb_ctx *ctx) {
   if (!ctx) return FB_E_ARG;
-  ctx->reset_events();
   RC(pre_assemble(ctx, "assemble_dipole_z"));
   RC(run_2c(ctx, 4, "assemble_2c_dipole_z"));
   RC(ctx->check_status("assemble_dipole_z"));
@@ -1046,7 +1079,6 @@ int fb_assemble_dipole_z(fb_ctx *ctx) {
 
 int fb_assemble_vna_2c(fb_ctx *ctx) {
   if (!ctx) return FB_E_ARG;
-  ctx->reset_events();
   RC(pre_assemble(ctx, "assemble_vna_2c"));
   if (!ctx->done[FB_MAT_S]) return ctx->fail(FB_E_ORDER, "assemble_vna_2c needs assemble_S first");
   RC(run_2c(ctx, 8, "assemble_2c_vna"));
@@ -1057,7 +1089,6 @@ int fb_assemble_vna_2c(fb_ctx *ctx) {
 
 int fb_assemble_vna_3c(fb_ctx *ctx) {
   if (!ctx) return FB_E_ARG;
-  ctx->reset_events();
   RC(pre_assemble(ctx, "assemble_vna_3c"));
   if (!ctx->done[FB_MAT_VNA] || !ctx->done[FB_MAT_S] || !ctx->done[FB_MAT_DIP])
     return ctx->fail(FB_E_ORDER, "assemble_vna_3c needs assemble_S, assemble_dipole_z and assemble_vna_2c first");
@@ -1066,18 +1097,54 @@ int fb_assemble_vna_3c(fb_ctx *ctx) {
   return FB_OK;
 }
 
-int fb_assemble_all(fb_ctx *ctx) {
-  if (!ctx) return FB_E_ARG;
-  ctx->reset_events();
-  RC(pre_assemble(ctx, "assemble_all"));
-  if (any_charge(ctx)) {
+// the launches of one H+S build (status clear, 2c, 3c), enqueued on the stream
+static int enqueue_build(fb_ctx *ctx, int charged) {
+  RC(ctx->launch("clear_status", [&] { return fbk_fill_i32(ctx->d_status.as<int>(), 1, 0, ctx->stream); }));
+  if (charged) {
     RC(run_2c(ctx, 7, "assemble_2c_STD"));
     RC(run_2c(ctx, 8, "assemble_2c_vna"));
   } else {
     RC(run_2c(ctx, 15, "assemble_2c"));
   }
   RC(run_3c(ctx));
-  RC(ctx->check_status("assemble_all"));
+  FB_HIP(hipMemcpyAsync(ctx->h_status, ctx->d_status.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  return FB_OK;
+}
+
+// Asynchronous: returns once the build is enqueued.  A status raised by the
+// kernels (x beyond a table, missing table) is returned by the next
+// fb_synchronize, fb_get_blocks or assemble call.  Without per-kernel timing
+// the build is replayed from a captured HIP graph.
+int fb_assemble_all(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  if (!ctx->lists_ok) return ctx->fail(FB_E_ORDER, "assemble_all before build_lists");
+  FB_HIP(hipSetDevice(ctx->device));
+  RC(ctx->settle("previous assemble_all"));
+  if (ctx->structs_dirty) RC(upload_structures(ctx));
+  const int charged = any_charge(ctx);
+  if (ctx->timing || !ctx->use_graph || ctx->stream == nullptr) {
+    RC(enqueue_build(ctx, charged));
+  } else {
+    const long key = ((ctx->lists_gen * 4 + ctx->precision) * 2 + charged) * 512 + ctx->chunk3c;
+    if (ctx->graph_key != key || !ctx->gexec) {
+      if (ctx->gexec) { (void)hipGraphExecDestroy(ctx->gexec); ctx->gexec = nullptr; }
+      if (ctx->graph) { (void)hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
+      FB_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_build(ctx, charged);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+      if (rc != FB_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return ctx->fail(FB_E_HIP, "graph capture", e);
+      ctx->graph = g;
+      FB_HIP(hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
+      ctx->graph_key = key;
+    }
+    FB_HIP(hipGraphLaunch(ctx->gexec, ctx->stream));
+  }
+  ctx->pending = true;
   for (int i = 0; i < 5; i++) ctx->done[i] = 1;
   return FB_OK;
 }
@@ -1095,6 +1162,7 @@ int fb_get_blocks(fb_ctx *ctx, int is, int matrix, double *out) {
     return ctx->fail(FB_E_ARG, "get_blocks arguments");
   if (!ctx->done[matrix]) return ctx->fail(FB_E_ORDER, "get_blocks: matrix not assembled");
   FB_HIP(hipStreamSynchronize(ctx->stream));
+  RC(ctx->settle("assemble_all"));
   int64_t b0 = ctx->h_boff_struct[is], b1 = ctx->h_boff_struct[is + 1];
   return d2h(ctx, out, ctx->d_mat[matrix].as<double>() + b0, sizeof(double) * (b1 - b0));
 }
@@ -1103,6 +1171,64 @@ int fb_get_device_blocks(fb_ctx *ctx, int matrix, void **dptr, int64_t *len) {
   if (!ctx || !ctx->lists_ok || matrix < 0 || matrix > 4) return FB_E_ARG;
   if (dptr) *dptr = ctx->d_mat[matrix].p;
   if (len) *len = ctx->blen;
+  return FB_OK;
+}
+
+int fb_set_tuning(fb_ctx *ctx, int param, int value) {
+  if (!ctx) return FB_E_ARG;
+  if (param == FB_TUNE_CHUNK3C) {
+    if (value != 64 && value != 128 && value != 256) return ctx->fail(FB_E_ARG, "chunk must be 64, 128 or 256");
+    ctx->chunk3c = value;
+    return FB_OK;
+  }
+  return ctx->fail(FB_E_ARG, "unknown tuning parameter");
+}
+
+int fb_get_tuning(fb_ctx *ctx, int param, int *value) {
+  if (!ctx || !value) return FB_E_ARG;
+  if (param == FB_TUNE_CHUNK3C) {
+    *value = ctx->chunk3c;
+    return FB_OK;
+  }
+  return ctx->fail(FB_E_ARG, "unknown tuning parameter");
+}
+
+int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
+  if (!ctx) return FB_E_ARG;
+  if (!ctx->lists_ok || !ctx->done[FB_MAT_S] || !ctx->done[FB_MAT_DIP] || !ctx->done[FB_MAT_VNA])
+    return ctx->fail(FB_E_ORDER, "autotune needs lists and the 2c blocks (run an assemble first)");
+  if (nrep < 1) nrep = 5;
+  FB_HIP(hipSetDevice(ctx->device));
+  RC(ctx->settle("assemble_all"));
+  const int cand[3] = {64, 128, 256};
+  float best = 1e30f;
+  int pick = ctx->chunk3c;
+  hipEvent_t a, b;
+  FB_HIP(hipEventCreate(&a));
+  FB_HIP(hipEventCreate(&b));
+  DView v = ctx->view();
+  for (int c : cand) {
+    int rc = fbk_3c_mixed(v, c, ctx->stream);  // warm-up
+    if (rc) return ctx->fail(FB_E_HIP, "autotune", (hipError_t)rc);
+    FB_HIP(hipEventRecord(a, ctx->stream));
+    for (int r = 0; r < nrep; r++) {
+      rc = fbk_3c_mixed(v, c, ctx->stream);
+      if (rc) return ctx->fail(FB_E_HIP, "autotune", (hipError_t)rc);
+    }
+    FB_HIP(hipEventRecord(b, ctx->stream));
+    FB_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    FB_HIP(hipEventElapsedTime(&ms, a, b));
+    if (ms < best) {
+      best = ms;
+      pick = c;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  ctx->chunk3c = pick;
+  if (chosen) *chosen = pick;
+  RC(ctx->check_status("autotune"));
   return FB_OK;
 }
 
@@ -1115,6 +1241,7 @@ int fb_enable_timing(fb_ctx *ctx, int on) {
 int fb_get_timings(fb_ctx *ctx, int max, char *names, float *ms, int *count) {
   if (!ctx) return FB_E_ARG;
   FB_HIP(hipStreamSynchronize(ctx->stream));
+  RC(ctx->settle("assemble_all"));
   int n = 0;
   for (auto &e : ctx->events) {
     if (n >= max) break;
@@ -1128,6 +1255,7 @@ int fb_get_timings(fb_ctx *ctx, int max, char *names, float *ms, int *count) {
     n++;
   }
   if (count) *count = n;
+  ctx->reset_events();
   return FB_OK;
 }
 

@@ -66,6 +66,8 @@ def lib():
             "fb_get_block_len": [vp, C.c_int, i64p], "fb_get_blocks": [vp, C.c_int, C.c_int, dp],
             "fb_get_device_blocks": [vp, C.c_int, C.POINTER(vp), i64p],
             "fb_enable_timing": [vp, C.c_int],
+            "fb_set_tuning": [vp, C.c_int, C.c_int], "fb_get_tuning": [vp, C.c_int, ip],
+            "fb_autotune": [vp, C.c_int, ip],
             "fb_get_timings": [vp, C.c_int, C.c_char_p, C.POINTER(C.c_float), ip],
         }
         for name, args in sig.items():
@@ -218,8 +220,12 @@ class Assembler:
     def assemble_vna_3c(self):
         self._check(self.L.fb_assemble_vna_3c(self.h))
 
-    def assemble_all(self):
+    def assemble_all(self, sync=True):
+        """One H+S build; with sync=False it only enqueues (errors surface at
+        the next synchronize / blocks / assemble call)."""
         self._check(self.L.fb_assemble_all(self.h))
+        if sync:
+            self.synchronize()
 
     def synchronize(self):
         self._check(self.L.fb_synchronize(self.h))
@@ -230,6 +236,21 @@ class Assembler:
         out = np.zeros(max(n.value, 1))
         self._check(self.L.fb_get_blocks(self.h, istruct, FB_MAT[mat], _dp(out)))
         return out[:n.value]
+
+    def set_chunk3c(self, chunk):
+        self._check(self.L.fb_set_tuning(self.h, 0, int(chunk)))
+
+    def chunk3c(self):
+        v = C.c_int()
+        self._check(self.L.fb_get_tuning(self.h, 0, C.byref(v)))
+        return v.value
+
+    def autotune(self, nrep=5):
+        """Time every workgroup chunk of the mixed 3c kernel on the current
+        lists and keep the fastest; returns the chosen chunk."""
+        v = C.c_int()
+        self._check(self.L.fb_autotune(self.h, int(nrep), C.byref(v)))
+        return v.value
 
     def enable_timing(self, on=True):
         self._check(self.L.fb_enable_timing(self.h, int(on)))
