@@ -5,17 +5,19 @@
 // blocks whose triplet segment starts in its tile and sums each segment in the
 // reference's order), but the work inside a chunk of triplets is spread over
 // all lanes in four phases:
-//   A  one lane per triplet: binary64 geometry -- bond and third-atom vectors,
-//      x = |r_na - bond centre|, y = bond length, the frame epsilon(rhat,
-//      sighat) -- and the reference's grid cell (ix, iy) and method choice
-//      (gradient of ME 1 of theta cell 1 against 1e-6, from the binary64
-//      table), so every discrete decision equals the reference's.  Stencil
-//      weights, Legendre P0..P4 and the p rotation go to LDS in FP32.
-//   B  one lane per (triplet, theta, matrix element): 4x4 (or 3-point)
-//      stencil of the FP32 slab [iy][ix][theta][ME]; the lanes of one triplet
-//      read the 5*nME contiguous values of each stencil point together.
-//   C  one lane per triplet: Legendre sum, sin(theta) on mvalue = 1 elements,
-//      Slater-Koster rotation (fb_sp.h) in FP32, times P_eq2, charged terms.
+//   A  one lane per triplet: geometry.  Positions, bond and third-atom vectors,
+//      x = |r_na - bond centre| and y = bond length are binary64 with the
+//      reference's expressions, and so are the decisions taken on them: the
+//      grid cell (ix, iy) = int(x/dx)+1, int(y/dy)+1 and the method choice
+//      (gradient of ME 1 of theta cell 1 of each table against 1e-6, from the
+//      binary64 table).  Unit vectors and the frame epsilon(rhat, sighat) are
+//      binary64 with reciprocal multiplies; weights, Legendre P0..P4 and the
+//      p rotation go to LDS in FP32.
+//   B  one lane per (triplet, matrix element): the 16 (or 3) stencil points of
+//      the FP32 slab [iy][ix][ME][theta], one float4 + one float per point,
+//      Legendre-folded at each point, then the x/y Lagrange weights.
+//   C  one lane per triplet: sin(theta) on mvalue = 1 elements, Slater-Koster
+//      rotation (fb_sp.h) in FP32, times P_eq2, charged terms.
 //   D  one lane per owned segment: binary64 accumulation of the FP32
 //      contributions in segment order; one store of the final block.
 // Binary64 adds run at the FP32 rate on gfx950 without packed math, so the
@@ -25,20 +27,23 @@
 
 #define NB (FB_MO * FB_MO)
 #define NME3 10
-#define NS3 (FB_NTHETA * NME3)
+#define TH8 8     // theta values per (point, ME) in the FP32 slab: 5 used, 3 padding
 
 struct TPrm {
-  int base, ncol, rowstride;
-  int nme, cls, m1, gsub;     // ME count, pair class, method-1 flag, charged (1) or not (0)
-  int p;                      // target pair
+  unsigned base;              // FP32 slab index (floats) of stencil corner (ix-1, iy-1), neutral group
+  unsigned rowstride;         // floats per iy row of the slab
+  int cls, nme, m1, p;        // pair class, ME count, method-1 flag, target pair
   float wx[4], wy[4];         // cubic Lagrange weights (method 3)
   float w1[3];                // (1-px-py, px, py) of the 3-point rule (method 1)
-  int m1k[FB_MAXSH];          // method-1 flag of each charged group (its own gradient test)
   float P[5], sint;
   float pm[9];
-  float dq[FB_MAXSH];         // shell charges of the third atom
-  float stinky, d13, d23, z;
-  int gk[FB_MAXSH];           // FP32 slab offset difference of each charged group
+};
+
+struct TChg {                 // charged third atom (DOGS terms)
+  float dq[FB_MAXSH];
+  int gk[FB_MAXSH];           // slab offset of each charged group relative to the neutral one
+  int m1k[FB_MAXSH];
+  float stinky, d13, d23, z, dqs;
 };
 
 __device__ __forceinline__ int grp_id(const DView &v, int in1, int in2, int in3, int isub) {
@@ -48,43 +53,63 @@ __device__ __forceinline__ int grp_id(const DView &v, int in1, int in2, int in3,
 // 4-point Lagrange weights on nodes -1, 0, 1, 2 (the Horner form of
 // M_Fdata_3c.f90:439-477 is this polynomial, divided by 6 per direction)
 __device__ __forceinline__ void cubic_w(double p, float *w) {
-  double pm1 = p - 1.0, pm2 = p - 2.0, pp1 = p + 1.0;
-  w[0] = (float)(-p * pm1 * pm2 / 6.0);
-  w[1] = (float)(pp1 * pm1 * pm2 / 2.0);
-  w[2] = (float)(-pp1 * p * pm2 / 2.0);
-  w[3] = (float)(pp1 * p * pm1 / 6.0);
+  const double pm1 = p - 1.0, pm2 = p - 2.0, pp1 = p + 1.0, s6 = 1.0 / 6.0;
+  w[0] = (float)(-p * pm1 * pm2 * s6);
+  w[1] = (float)(pp1 * pm1 * pm2 * 0.5);
+  w[2] = (float)(-pp1 * p * pm2 * 0.5);
+  w[3] = (float)(pp1 * p * pm1 * s6);
 }
 
-// one (triplet, theta, ME) slot s = theta*nME + ME of the stencil: the lanes of
-// a triplet read the 5*nME contiguous values of every stencil point together
-__device__ __forceinline__ float interp_slot(const float *__restrict__ tab, int base, int rowstride, int ncol, int s,
-                                             const TPrm &q, int m1) {
+// Buffer resource over the FP32 slab: loads take one 32-bit byte offset in a
+// VGPR plus an immediate, and addresses outside the slab read as zero.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slab_rsrc(const float *tab, int64_t nfloats) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)tab, (short)0, (int)(nfloats * 4), 0x00020000);
+}
+
+// Legendre fold of one (point, ME): sum_theta P_theta T_theta; off in bytes
+__device__ __forceinline__ float fold(__amdgpu_buffer_rsrc_t r, unsigned off, const float *P) {
+  const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+  const float b = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(off + 16u), 0, 0));
+  float v = P[0] * a.x;
+  v = __builtin_fmaf(P[1], a.y, v);
+  v = __builtin_fmaf(P[2], a.z, v);
+  v = __builtin_fmaf(P[3], a.w, v);
+  return __builtin_fmaf(P[4], b, v);
+}
+
+// one (triplet, ME): 16-point tensor stencil (method 3) or 3-point rule (method 1);
+// base and rowstride in floats
+template <int NME>
+__device__ __forceinline__ float stencil(__amdgpu_buffer_rsrc_t r, unsigned base, unsigned rowstride, int m,
+                                         const float *wx, const float *wy, const float *w1, const float *P, int m1) {
+  constexpr unsigned PT = NME * TH8 * 4;   // bytes per ix step
+  const unsigned o = (base + (unsigned)m * TH8) * 4u, rb = rowstride * 4u;
   if (m1) {
-    const float *c00 = tab + base + rowstride + ncol + s;   // (ix, iy)
-    float v = q.w1[0] * c00[0];
-    v = __builtin_fmaf(q.w1[1], c00[ncol], v);
-    return __builtin_fmaf(q.w1[2], c00[rowstride], v);
+    const unsigned c = o + rb + PT;   // (ix, iy)
+    float v = w1[0] * fold(r, c, P);
+    v = __builtin_fmaf(w1[1], fold(r, c + PT, P), v);
+    return __builtin_fmaf(w1[2], fold(r, c + rb, P), v);
   }
-  const float *c = tab + base + s;
-  const float wy0 = q.wy[0], wy1 = q.wy[1], wy2 = q.wy[2], wy3 = q.wy[3];
   float acc = 0.f;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const float *col = c + i * ncol;
-    float g = wy0 * col[0];
-    g = __builtin_fmaf(wy1, col[rowstride], g);
-    g = __builtin_fmaf(wy2, col[2 * rowstride], g);
-    g = __builtin_fmaf(wy3, col[3 * rowstride], g);
-    acc = __builtin_fmaf(q.wx[i], g, acc);
+  for (int j = 0; j < 4; j++) {
+    const unsigned row = o + (unsigned)j * rb;
+    float g = wx[0] * fold(r, row, P);
+    g = __builtin_fmaf(wx[1], fold(r, row + PT, P), g);
+    g = __builtin_fmaf(wx[2], fold(r, row + 2 * PT, P), g);
+    g = __builtin_fmaf(wx[3], fold(r, row + 3 * PT, P), g);
+    acc = __builtin_fmaf(wy[j], g, acc);
   }
   return acc;
 }
 
-template <int CH>
+template <int CH, bool CHARGED>
 __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   __shared__ TPrm sq[CH];
-  __shared__ float sT[CH][NS3 + 1];
-  __shared__ float sC[CH][NB + 1];
+  __shared__ float sF[CH][NME3 + 1];
+  __shared__ TChg sx[CHARGED ? CH : 1];
+  float(*sC)[NB] = reinterpret_cast<float(*)[NB]>(&sq[0]);   // contributions reuse the parameter storage
+  static_assert(sizeof(TPrm) >= NB * sizeof(float), "contribution row must fit in a parameter row");
   const int tid = threadIdx.x;
   const int64_t ts = (int64_t)blockIdx.x * CH;
   const int64_t te = min(ts + CH, v.ntrip);
@@ -105,6 +130,7 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
 #pragma unroll
   for (int i = 0; i < NB; i++) acc[i] = 0.0;
   int status = 0;
+  const __amdgpu_buffer_rsrc_t tab = slab_rsrc(v.tab3c32, v.ntab3c32);
 
   for (int64_t c0 = r0; c0 < r1; c0 += CH) {
     const int nt = (int)min((int64_t)CH, r1 - c0);
@@ -117,7 +143,7 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       const int ib = tr.w & 0xffff, jb = (tr.w >> 16) & 0xffff;
       const int s = v.atom_struct[ial];
       const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
-      double r1v[3], r2v[3], rna[3], r21[3], sighat[3], rnabc[3], rhat[3], eps[9];
+      double r1v[3], r2v[3], rna[3], r21[3], sighat[3], rnabc[3], rhat[3];
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         r1v[k] = v.ratom[3 * gia + k] + xls[3 * ib + k];
@@ -131,23 +157,52 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       const double z = d_distance(r1v, r2v);
       if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
       else {
+        const double iz = 1.0 / z;
 #pragma unroll
-        for (int k = 0; k < 3; k++) sighat[k] = r21[k] / z;
+        for (int k = 0; k < 3; k++) sighat[k] = r21[k] * iz;
       }
 #pragma unroll
-      for (int k = 0; k < 3; k++) rnabc[k] = rna[k] - (r1v[k] + r21[k] / 2.0);
+      for (int k = 0; k < 3; k++) rnabc[k] = rna[k] - (r1v[k] + r21[k] * 0.5);
       const double x = sqrt(rnabc[0] * rnabc[0] + rnabc[1] * rnabc[1] + rnabc[2] * rnabc[2]);
       if (x < 1.0e-5) { rhat[0] = 0.0; rhat[1] = 0.0; rhat[2] = 0.0; }
       else {
+        const double ix_ = 1.0 / x;
 #pragma unroll
-        for (int k = 0; k < 3; k++) rhat[k] = rnabc[k] / x;
+        for (int k = 0; k < 3; k++) rhat[k] = rnabc[k] * ix_;
       }
       const double cost = sighat[0] * rhat[0] + sighat[1] * rhat[1] + sighat[2] * rhat[2];
-      d_epsilon(rhat, sighat, eps);
-      float pmf[9];
-      d_pmat<float>(eps, pmf);
+      {
+        // epsilon(rhat, sighat), b.FUNCTIONS/epsilon.f90:120-177, with the
+        // reference's thresholds (sighat is a unit vector, so the r2 = 0 branch
+        // cannot occur)
+        double zp[3], yp[3] = {0.0, 0.0, 0.0}, xp[3];
+        const double izm = 1.0 / d_mag(sighat);
 #pragma unroll
-      for (int k = 0; k < 9; k++) q.pm[k] = pmf[k];
+        for (int k = 0; k < 3; k++) zp[k] = sighat[k] * izm;
+        double ypm = 0.0;
+        const double r1m = d_mag(rhat);
+        if (r1m > 1.0e-4) {
+          const double ir = 1.0 / r1m;
+          const double rh[3] = {rhat[0] * ir, rhat[1] * ir, rhat[2] * ir};
+          d_cross(zp, rh, yp);
+          ypm = d_mag(yp);
+        }
+        if (ypm < 1.0e-4) {
+          if (fabs(zp[0]) > 1.0e-4) { yp[0] = -(zp[1] + zp[2]) / zp[0]; yp[1] = 1.0; yp[2] = 1.0; }
+          else if (fabs(zp[1]) > 1.0e-4) { yp[0] = 1.0; yp[1] = -(zp[0] + zp[2]) / zp[1]; yp[2] = 1.0; }
+          else { yp[0] = 1.0; yp[1] = 1.0; yp[2] = -(zp[0] + zp[1]) / zp[2]; }
+          ypm = d_mag(yp);
+        }
+        const double iy_ = 1.0 / ypm;
+#pragma unroll
+        for (int k = 0; k < 3; k++) yp[k] *= iy_;
+        d_cross(yp, zp, xp);
+        const double eps[9] = {xp[0], xp[1], xp[2], yp[0], yp[1], yp[2], zp[0], zp[1], zp[2]};
+        float pmf[9];
+        d_pmat<float>(eps, pmf);
+#pragma unroll
+        for (int k = 0; k < 9; k++) q.pm[k] = pmf[k];
+      }
       const double cost2 = cost * cost;
       q.sint = (float)sqrt(fmax(1.0e-5, 1.0 - cost2));
       q.P[0] = 1.f;
@@ -162,122 +217,138 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       }
       const DGroup3c g = v.grp3c[gid];
       q.nme = g.nME;
+      // grid cell and method, binary64 as M_Fdata_3c.f90:367-385
       const double dx = g.xmax / (g.nx - 1.0), dy = g.ymax / (g.ny - 1.0);
-      int ix = (int)(x / dx) + 1, iy = (int)(z / dy) + 1;
+      const double qx = x / dx, qy = z / dy;
+      int ix = (int)qx + 1, iy = (int)qy + 1;
       if (ix < 2) ix = 2;
       if (iy < 2) iy = 2;
       if (ix > g.nx - 2) ix = g.nx - 2;
       if (iy > g.ny - 2) iy = g.ny - 2;
-      const double px = x / dx - ix + 1.0, py = z / dy - iy + 1.0;
+      const double px = qx - ix + 1.0, py = qy - iy + 1.0;
       const double *t64 = v.tab3c + g.off;
-      const double f00 = t64[((int64_t)(iy - 1) * g.nx + (ix - 1)) * g.nME];
-      const double gx = (t64[((int64_t)(iy - 1) * g.nx + ix) * g.nME] - f00) / dx;
-      const double gy = (t64[((int64_t)iy * g.nx + (ix - 1)) * g.nME] - f00) / dy;
-      const double gg = gx * gx + gy * gy;
-      q.m1 = (gg < 1.0e-6) ? 1 : 0;
-      float w[4];
+      const int64_t i00 = ((int64_t)(iy - 1) * g.nx + (ix - 1)) * g.nME;
+      const double f00 = t64[i00];
+      const double gx = (t64[i00 + g.nME] - f00) / dx;
+      const double gy = (t64[i00 + (int64_t)g.nx * g.nME] - f00) / dy;
+      q.m1 = (gx * gx + gy * gy < 1.0e-6) ? 1 : 0;
       q.w1[0] = (float)(1.0 - px - py);
       q.w1[1] = (float)px;
       q.w1[2] = (float)py;
-      cubic_w(px, w);
+      cubic_w(px, q.wx);
+      cubic_w(py, q.wy);
+      q.rowstride = (unsigned)(g.nx * g.nME * TH8);
+      q.base = (unsigned)(g.off32 + ((int64_t)(iy - 2) * g.nx + (ix - 2)) * g.nME * TH8);
+      if (CHARGED) {
+        TChg &c = sx[tid];
+        c.dqs = 0.f;
 #pragma unroll
-      for (int k = 0; k < 4; k++) q.wx[k] = w[k];
-      cubic_w(py, w);
-#pragma unroll
-      for (int k = 0; k < 4; k++) q.wy[k] = w[k];
-      q.ncol = FB_NTHETA * g.nME;
-      q.rowstride = g.nx * q.ncol;
-      q.base = (int)(g.off32 + ((int64_t)(iy - 2) * g.nx + (ix - 2)) * q.ncol);
-      // charged terms (M_assemble_3c_DOGS.f90:250-318)
-      q.gsub = 0;
-#pragma unroll
-      for (int k = 0; k < FB_MAXSH; k++) { q.dq[k] = 0.f; q.gk[k] = 0; q.m1k[k] = 0; }
-      for (int isorp = 1; isorp <= v.sp.nssh[indna - 1]; isorp++) {
-        double dq = v.dQ[ial * FB_MAXSH + isorp - 1];
-        if (dq == 0.0) continue;
-        int gk = grp_id(v, in1, in2, indna, isorp);
-        if (gk < 0) {
-          status |= ST_NO_TABLE;
-          continue;
+        for (int k = 0; k < FB_MAXSH; k++) { c.dq[k] = 0.f; c.gk[k] = 0; c.m1k[k] = 0; }
+        for (int isorp = 1; isorp <= v.sp.nssh[indna - 1]; isorp++) {
+          double dq = v.dQ[ial * FB_MAXSH + isorp - 1];
+          if (dq == 0.0) continue;
+          int gk = grp_id(v, in1, in2, indna, isorp);
+          if (gk < 0) {
+            status |= ST_NO_TABLE;
+            continue;
+          }
+          const DGroup3c gq = v.grp3c[gk];
+          c.dq[isorp - 1] = (float)dq;
+          c.dqs += (float)dq;
+          c.gk[isorp - 1] = (int)(gq.off32 - g.off32);
+          // getMEs_Fdata_3c chooses the method from each table's own gradient
+          const double *tq = v.tab3c + gq.off;
+          const double h00 = tq[i00];
+          const double hx = (tq[i00 + g.nME] - h00) / dx;
+          const double hy = (tq[i00 + (int64_t)g.nx * g.nME] - h00) / dy;
+          c.m1k[isorp - 1] = (hx * hx + hy * hy < 1.0e-6) ? 1 : 0;
         }
-        q.dq[isorp - 1] = (float)dq;
-        const DGroup3c gq = v.grp3c[gk];
-        q.gk[isorp - 1] = (int)(gq.off32 - g.off32);
-        // getMEs_Fdata_3c chooses the method from each table's own gradient
-        const double *tq = v.tab3c + gq.off;
-        const double h00 = tq[((int64_t)(iy - 1) * g.nx + (ix - 1)) * g.nME];
-        const double hx = (tq[((int64_t)(iy - 1) * g.nx + ix) * g.nME] - h00) / dx;
-        const double hy = (tq[((int64_t)iy * g.nx + (ix - 1)) * g.nME] - h00) / dy;
-        q.m1k[isorp - 1] = (hx * hx + hy * hy < 1.0e-6) ? 1 : 0;
-        q.gsub = 1;
-      }
-      if (q.gsub) {
         const double d13 = d_distance(rna, r1v), d23 = d_distance(rna, r2v);
         double rc1 = v.sp.rcmin[in1 - 1], rc2 = v.sp.rcmin[in2 - 1], rc3 = v.sp.rcmin[indna - 1];
-        q.stinky = (float)(d_smoother(d13, rc1 + rc3, 0.8) * d_smoother(d23, rc2 + rc3, 0.8));
-        q.d13 = (float)d13;
-        q.d23 = (float)d23;
-        q.z = (float)z;
+        c.stinky = (float)(d_smoother(d13, rc1 + rc3, 0.8) * d_smoother(d23, rc2 + rc3, 0.8));
+        c.d13 = (float)d13;
+        c.d23 = (float)d23;
+        c.z = (float)z;
       }
     }
     __syncthreads();
-    // ---------------- phase B: interpolation, one lane per (triplet, theta, ME)
-    for (int it = tid; it < nt * NS3; it += CH) {
-      const int t = it / NS3, sl = it - t * NS3;
+    // ---------------- phase B: interpolation, one lane per (triplet, ME)
+    for (int it = tid; it < nt * NME3; it += CH) {
+      const int t = it / NME3, m = it - t * NME3;
       const TPrm &q = sq[t];
-      if (sl >= q.ncol) continue;
-      float F = interp_slot(v.tab3c32, q.base, q.rowstride, q.ncol, sl, q, q.m1);
-      if (q.gsub) {
+      if (m >= q.nme) continue;
+      float wx[4], wy[4], w1[3], P[5];
+#pragma unroll
+      for (int k = 0; k < 4; k++) { wx[k] = q.wx[k]; wy[k] = q.wy[k]; }
+#pragma unroll
+      for (int k = 0; k < 3; k++) w1[k] = q.w1[k];
+#pragma unroll
+      for (int k = 0; k < 5; k++) P[k] = q.P[k];
+      const unsigned base = q.base, rs = q.rowstride;
+      float F;
+      switch (q.nme) {
+        case 10: F = stencil<10>(tab, base, rs, m, wx, wy, w1, P, q.m1); break;
+        case 3: F = stencil<3>(tab, base, rs, m, wx, wy, w1, P, q.m1); break;
+        default: F = stencil<1>(tab, base, rs, m, wx, wy, w1, P, q.m1); break;
+      }
+      if (CHARGED) {
+        const TChg &c = sx[t];
         for (int k = 0; k < FB_MAXSH; k++) {
-          const float dq = q.dq[k];
+          const float dq = c.dq[k];
           if (dq == 0.f) continue;
-          F = __builtin_fmaf(dq * q.stinky, interp_slot(v.tab3c32, q.base + q.gk[k], q.rowstride, q.ncol, sl, q, q.m1k[k]), F);
+          const unsigned bk = base + (unsigned)c.gk[k];
+          float Fk;
+          switch (q.nme) {
+            case 10: Fk = stencil<10>(tab, bk, rs, m, wx, wy, w1, P, c.m1k[k]); break;
+            case 3: Fk = stencil<3>(tab, bk, rs, m, wx, wy, w1, P, c.m1k[k]); break;
+            default: Fk = stencil<1>(tab, bk, rs, m, wx, wy, w1, P, c.m1k[k]); break;
+          }
+          F = __builtin_fmaf(dq * c.stinky, Fk, F);
         }
       }
-      sT[t][sl] = F;
+      sF[t][m] = F;
     }
     __syncthreads();
     // ---------------- phase C: rotation, one lane per triplet
+    float X[16];
+    int cls = 0, p = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) X[i] = 0.f;
     if (tid < nt) {
       const TPrm &q = sq[tid];
-      float F[NME3], pm[9], X[16];
-      const int nm0 = (q.cls == 3) ? 6 : (q.cls == 0 ? 1 : 2);
+      float F[NME3], pm[9];
+      cls = q.cls;
+      p = q.p;
+      const int nm0 = (cls == 3) ? 6 : (cls == 0 ? 1 : 2);
+      const float sint = q.sint;
 #pragma unroll
-      for (int m = 0; m < NME3; m++) {
-        float f = 0.f;
-        if (m < q.nme) {
-#pragma unroll
-          for (int th = 0; th < FB_NTHETA; th++) f = __builtin_fmaf(q.P[th], sT[tid][th * q.nme + m], f);
-          if (m >= nm0) f *= q.sint;
-        }
-        F[m] = f;
-      }
+      for (int m = 0; m < NME3; m++) F[m] = (m < q.nme) ? sF[tid][m] * (m >= nm0 ? sint : 1.f) : 0.f;
 #pragma unroll
       for (int k = 0; k < 9; k++) pm[k] = q.pm[k];
-#pragma unroll
-      for (int i = 0; i < 16; i++) X[i] = 0.f;
-      switch (q.cls) {
+      switch (cls) {
         case 3: rot3c_sp<float>(3, pm, F, X); break;
         case 2: rot3c_sp<float>(2, pm, F, X); break;
         case 1: rot3c_sp<float>(1, pm, F, X); break;
         default: rot3c_sp<float>(0, pm, F, X); break;
       }
-      const int nb = (q.cls & 2 ? 4 : 1) * (q.cls & 1 ? 4 : 1);
-      if (q.gsub) {
-        float dqs = 0.f;
-#pragma unroll
-        for (int k = 0; k < FB_MAXSH; k++) dqs += q.dq[k];
-        const double *sbp = v.mat[FB_MAT_S] + v.boff[q.p];
-        const double *dbp = v.mat[FB_MAT_DIP] + v.boff[q.p];
+    }
+    __syncthreads();   // every lane has read its parameters before sC overwrites them
+    if (tid < nt) {
+      const int nb = (cls & 2 ? 4 : 1) * (cls & 1 ? 4 : 1);
+      if (CHARGED) {
+        const TChg &c = sx[tid];
+        const double *sbp = v.mat[FB_MAT_S] + v.boff[p];
+        const double *dbp = v.mat[FB_MAT_DIP] + v.boff[p];
 #pragma unroll
         for (int e = 0; e < 16; e++) {
-          if (e >= nb) break;
-          double sterm = sbp[e] / 2.0, dterm = dbp[e] / (double)q.z;
-          double emnpl = (sterm - dterm) / (double)q.d13 + (sterm + dterm) / (double)q.d23;
-          X[e] = X[e] * (float)FB_P_EQ2 + (float)(dqs * (1.0 - q.stinky) * emnpl * FB_P_EQ2);
+          float val = 0.f;
+          if (e < nb) {
+            double sterm = sbp[e] * 0.5, dterm = dbp[e] / (double)c.z;
+            double emnpl = (sterm - dterm) / (double)c.d13 + (sterm + dterm) / (double)c.d23;
+            val = X[e] * (float)FB_P_EQ2 + (float)(c.dqs * (1.0 - c.stinky) * emnpl * FB_P_EQ2);
+          }
+          sC[tid][e] = val;
         }
-#pragma unroll
-        for (int e = 0; e < 16; e++) sC[tid][e] = (e < nb) ? X[e] : 0.f;
       } else {
 #pragma unroll
         for (int e = 0; e < 16; e++) sC[tid][e] = X[e] * (float)FB_P_EQ2;
@@ -307,14 +378,19 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   if (status) atomicOr(v.status, status);
 }
 
-extern "C" int fbk_3c_mixed(const DView &v, int chunk, hipStream_t st) {
-  if (v.ntrip == 0) return 0;
+template <bool CHARGED>
+static int launch_mixed(const DView &v, int chunk, hipStream_t st) {
   if (chunk == 256) {
-    hipLaunchKernelGGL(k_3c_mixed<256>, dim3((unsigned)((v.ntrip + 255) / 256)), dim3(256), 0, st, v);
+    hipLaunchKernelGGL((k_3c_mixed<256, CHARGED>), dim3((unsigned)((v.ntrip + 255) / 256)), dim3(256), 0, st, v);
   } else if (chunk == 64) {
-    hipLaunchKernelGGL(k_3c_mixed<64>, dim3((unsigned)((v.ntrip + 63) / 64)), dim3(64), 0, st, v);
+    hipLaunchKernelGGL((k_3c_mixed<64, CHARGED>), dim3((unsigned)((v.ntrip + 63) / 64)), dim3(64), 0, st, v);
   } else {
-    hipLaunchKernelGGL(k_3c_mixed<128>, dim3((unsigned)((v.ntrip + 127) / 128)), dim3(128), 0, st, v);
+    hipLaunchKernelGGL((k_3c_mixed<128, CHARGED>), dim3((unsigned)((v.ntrip + 127) / 128)), dim3(128), 0, st, v);
   }
   return (int)hipGetLastError();
+}
+
+extern "C" int fbk_3c_mixed(const DView &v, int chunk, int charged, hipStream_t st) {
+  if (v.ntrip == 0) return 0;
+  return charged ? launch_mixed<true>(v, chunk, st) : launch_mixed<false>(v, chunk, st);
 }

@@ -38,7 +38,7 @@ int fbk_gather_trip(int64_t ntrip, const int *perm, const int4 *in, int4 *out, h
 int fbk_fill_i32(int *a, int64_t n, int val, hipStream_t st);
 int fbk_2c(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64(const DView &v, hipStream_t st);
-int fbk_3c_mixed(const DView &v, int chunk, hipStream_t st);
+int fbk_3c_mixed(const DView &v, int chunk, int charged, hipStream_t st);
 int fbk_2c_sp(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64_sp(const DView &v, hipStream_t st);
 }
@@ -172,6 +172,7 @@ struct fb_ctx {
   DevBuf d_cell2c, d_idx2c, d_grp3c, d_idx3c, d_tab2c, d_tab2c32, d_tab3c, d_tab3c32, d_mu, d_nu, d_mv;
   DevBuf d_grid2c, d_pden, d_wg2c;
   int canon_sp = 0;
+  int64_t ntab3c32 = 0;
   int nwg2c = 0;
 
   std::vector<HStruct> structs;
@@ -229,6 +230,7 @@ struct fb_ctx {
     v.tab2c32 = d_tab2c32.as<float>();
     v.tab3c = d_tab3c.as<double>();
     v.tab3c32 = d_tab3c32.as<float>();
+    v.ntab3c32 = ntab3c32;
     v.mu = d_mu.as<int>();
     v.nu = d_nu.as<int>();
     v.mv = d_mv.as<int>();
@@ -615,12 +617,14 @@ int fb_commit_tables(fb_ctx *ctx) {
     nu.insert(nu.end(), cl->nu.begin(), cl->nu.end());
     mv.insert(mv.end(), cl->mv.begin(), cl->mv.end());
     for (int t = 0; t < FB_NTHETA; t++) tab3.insert(tab3.end(), th[t]->data.begin(), th[t]->data.end());
+    // FP32 slab [iy][ix][ME][theta], theta padded to 8 so that the five
+    // values of one (point, ME) load as one float4 + one float
     size_t plane = (size_t)g.nx * g.ny;
-    tab3f.resize(tab3f.size() + plane * FB_NTHETA * g.nME);
+    tab3f.resize(tab3f.size() + plane * g.nME * 8, 0.0f);
     float *dst = tab3f.data() + g.off32;
     for (size_t q = 0; q < plane; q++)
-      for (int t = 0; t < FB_NTHETA; t++)
-        for (int m = 0; m < g.nME; m++) dst[(q * FB_NTHETA + t) * g.nME + m] = (float)th[t]->data[q * g.nME + m];
+      for (int m = 0; m < g.nME; m++)
+        for (int t = 0; t < FB_NTHETA; t++) dst[(q * g.nME + m) * 8 + t] = (float)th[t]->data[q * g.nME + m];
     int is, js, ks, isub;
     std::tie(is, js, ks, isub) = kv.first;
     // the mixed kernel addresses every charge subtype with the neutral group's stencil
@@ -687,7 +691,10 @@ int fb_commit_tables(fb_ctx *ctx) {
   RC(upload(ctx, ctx->d_tab2c, tab2.data(), tab2.size() * sizeof(double), "upload tab2c"));
   RC(upload(ctx, ctx->d_tab2c32, tab2f.data(), tab2f.size() * sizeof(float), "upload tab2c32"));
   RC(upload(ctx, ctx->d_tab3c, tab3.data(), tab3.size() * sizeof(double), "upload tab3c"));
+  if (tab3f.size() * sizeof(float) >= (size_t)INT32_MAX)
+    return ctx->fail(FB_E_CAPACITY, "commit_tables: FP32 3c slab above 2 GiB");
   RC(upload(ctx, ctx->d_tab3c32, tab3f.data(), tab3f.size() * sizeof(float), "upload tab3c32"));
+  ctx->ntab3c32 = (int64_t)tab3f.size();
   RC(upload(ctx, ctx->d_cell2c, cells.data(), cells.size() * sizeof(DCell2c), "upload cells2c"));
   RC(upload(ctx, ctx->d_grid2c, grids.data(), grids.size() * sizeof(DGrid2c), "upload grids2c"));
   RC(upload(ctx, ctx->d_pden, pden.data(), pden.size() * sizeof(double), "upload pden"));
@@ -1043,7 +1050,7 @@ static int run_3c(fb_ctx *ctx) {
   DView v = ctx->view();
   if (ctx->canon_sp) {
     if (ctx->precision == FB_PREC_MIXED)
-      return ctx->launch("assemble_3c_mixed", [&] { return fbk_3c_mixed(v, ctx->chunk3c, ctx->stream); });
+      return ctx->launch("assemble_3c_mixed", [&] { return fbk_3c_mixed(v, ctx->chunk3c, any_charge(ctx), ctx->stream); });
     return ctx->launch("assemble_3c_fp64", [&] { return fbk_3c_fp64_sp(v, ctx->stream); });
   }
   // generic shells: binary64 path only
@@ -1207,12 +1214,13 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   FB_HIP(hipEventCreate(&a));
   FB_HIP(hipEventCreate(&b));
   DView v = ctx->view();
+  const int charged = any_charge(ctx);
   for (int c : cand) {
-    int rc = fbk_3c_mixed(v, c, ctx->stream);  // warm-up
+    int rc = fbk_3c_mixed(v, c, charged, ctx->stream);  // warm-up
     if (rc) return ctx->fail(FB_E_HIP, "autotune", (hipError_t)rc);
     FB_HIP(hipEventRecord(a, ctx->stream));
     for (int r = 0; r < nrep; r++) {
-      rc = fbk_3c_mixed(v, c, ctx->stream);
+      rc = fbk_3c_mixed(v, c, charged, ctx->stream);
       if (rc) return ctx->fail(FB_E_HIP, "autotune", (hipError_t)rc);
     }
     FB_HIP(hipEventRecord(b, ctx->stream));

@@ -79,6 +79,7 @@ struct DView {
   const float *tab2c32;
   const double *tab3c;
   const float *tab3c32;
+  int64_t ntab3c32;             // floats in the FP32 3c slab
   const int *mu, *nu, *mv;
   const DGrid2c *grid2c;
   const double *pden;
