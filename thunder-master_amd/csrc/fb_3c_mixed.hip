@@ -18,8 +18,8 @@
 //      Legendre-folded at each point, then the x/y Lagrange weights.
 //   C  one lane per triplet: sin(theta) on mvalue = 1 elements, Slater-Koster
 //      rotation (fb_sp.h) in FP32, times P_eq2, charged terms.
-//   D  one lane per owned segment: binary64 accumulation of the FP32
-//      contributions in segment order; one store of the final block.
+//   D  one lane per (owned segment, block element): binary64 accumulation of
+//      the FP32 contributions in segment order in LDS; one store per element.
 // Binary64 adds run at the FP32 rate on gfx950 without packed math, so the
 // accumulation is plain binary64 rather than a compensated FP32 sum.
 #include "fb_device.h"
@@ -108,6 +108,7 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   __shared__ TPrm sq[CH];
   __shared__ float sF[CH][NME3 + 1];
   __shared__ TChg sx[CHARGED ? CH : 1];
+  __shared__ int64_t sSeg[CH + 1];
   float(*sC)[NB] = reinterpret_cast<float(*)[NB]>(&sq[0]);   // contributions reuse the parameter storage
   static_assert(sizeof(TPrm) >= NB * sizeof(float), "contribution row must fit in a parameter row");
   const int tid = threadIdx.x;
@@ -122,13 +123,14 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
     r0 = v.seg_off[first];
     r1 = v.seg_off[last + 1];
   }
-  const int64_t my = first + tid;
-  const bool own = my <= last;
-  int64_t sa = 0, sb = 0;
-  if (own) { sa = v.seg_off[my]; sb = v.seg_off[my + 1]; }
+  // owned segments [first, last] (at most CH: each starts inside the tile).
+  // Accumulator slot sl = (segment, block element); lane tid owns the slots
+  // tid + j*CH, j < NB, in registers.
+  const int nown = (int)(last - first + 1);
+  for (int k = tid; k <= nown; k += CH) sSeg[k] = v.seg_off[first + k];
   double acc[NB];
 #pragma unroll
-  for (int i = 0; i < NB; i++) acc[i] = 0.0;
+  for (int j = 0; j < NB; j++) acc[j] = 0.0;
   int status = 0;
   const __amdgpu_buffer_rsrc_t tab = slab_rsrc(v.tab3c32, v.ntab3c32);
 
@@ -175,16 +177,12 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
         // epsilon(rhat, sighat), b.FUNCTIONS/epsilon.f90:120-177, with the
         // reference's thresholds (sighat is a unit vector, so the r2 = 0 branch
         // cannot occur)
-        double zp[3], yp[3] = {0.0, 0.0, 0.0}, xp[3];
-        const double izm = 1.0 / d_mag(sighat);
-#pragma unroll
-        for (int k = 0; k < 3; k++) zp[k] = sighat[k] * izm;
+        // sighat and rhat are unit vectors already (rhat = 0 when x < 1e-5),
+        // so the normalisations of epsilon.f90:136-145 change them by rounding only
+        double zp[3] = {sighat[0], sighat[1], sighat[2]}, yp[3] = {0.0, 0.0, 0.0}, xp[3];
         double ypm = 0.0;
-        const double r1m = d_mag(rhat);
-        if (r1m > 1.0e-4) {
-          const double ir = 1.0 / r1m;
-          const double rh[3] = {rhat[0] * ir, rhat[1] * ir, rhat[2] * ir};
-          d_cross(zp, rh, yp);
+        if (x >= 1.0e-5) {
+          d_cross(zp, rhat, yp);
           ypm = d_mag(yp);
         }
         if (ypm < 1.0e-4) {
@@ -355,25 +353,33 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       }
     }
     __syncthreads();
-    // ---------------- phase D: ordered binary64 accumulation per owned segment
-    if (own) {
-      int64_t k0 = max(sa, c0), k1 = min(sb, c0 + nt);
-      for (int64_t k = k0; k < k1; k++)
+    // ---------------- phase D: ordered binary64 accumulation, one lane per
+    // (owned segment, block element), segment order preserved
 #pragma unroll
-        for (int i = 0; i < NB; i++) acc[i] += (double)sC[k - c0][i];
+    for (int j = 0; j < NB; j++) {
+      const int sl = tid + j * CH;
+      if (sl < nown * NB) {
+        const int sg = sl / NB, e = sl - sg * NB;
+        const int64_t k0 = max(sSeg[sg], c0), k1 = min(sSeg[sg + 1], c0 + nt);
+        double a = acc[j];
+        for (int64_t k = k0; k < k1; k++) a += (double)sC[k - c0][e];
+        acc[j] = a;
+      }
     }
     __syncthreads();
   }
-  if (own) {
-    const int p = v.seg_pair[my];
-    const int ia = v.pair_atom[p];
-    const int s = v.atom_struct[ia];
-    const int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
-    const int nb = v.sp.norb[v.imass[ia] - 1] * v.sp.norb[v.imass[ja] - 1];
-    double *dst = v.mat[FB_MAT_VNA] + v.boff[p];
 #pragma unroll
-    for (int i = 0; i < NB; i++)
-      if (i < nb) dst[i] = acc[i];
+  for (int j = 0; j < NB; j++) {
+    const int sl = tid + j * CH;
+    if (sl < nown * NB) {
+      const int sg = sl / NB, e = sl - sg * NB;
+      const int p = v.seg_pair[first + sg];
+      const int ia = v.pair_atom[p];
+      const int s = v.atom_struct[ia];
+      const int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
+      const int nb = v.sp.norb[v.imass[ia] - 1] * v.sp.norb[v.imass[ja] - 1];
+      if (e < nb) v.mat[FB_MAT_VNA][v.boff[p] + e] = acc[j];
+    }
   }
   if (status) atomicOr(v.status, status);
 }

@@ -40,6 +40,7 @@ int fbk_2c(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64(const DView &v, hipStream_t st);
 int fbk_3c_mixed(const DView &v, int chunk, int charged, hipStream_t st);
 int fbk_2c_sp(const DView &v, int mask, hipStream_t st);
+int fbk_2c_job(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64_sp(const DView &v, hipStream_t st);
 }
 
@@ -939,8 +940,8 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
     FB_HIP(ctx->d_mat[i].ensure(sizeof(double) * std::max<int64_t>(ctx->blen, 1)));
     ctx->done[i] = 0;
   }
-  // 2c workgroups: whole atoms, at most 16 atoms and 256 pairs each (an atom
-  // with more than 256 neighbours gets a workgroup of its own)
+  // 2c workgroups: whole atoms, at most 16 atoms and FB_2C_PCAP pairs each (an
+  // atom with more neighbours gets a workgroup of its own, run in passes)
   {
     std::vector<int> wg;
     wg.push_back(0);
@@ -948,7 +949,7 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
     int64_t np = 0;
     for (int a = 0; a < N; a++) {
       int64_t n = ctx->h_noff[a + 1] - ctx->h_noff[a];
-      if (na > 0 && (na == 16 || np + n > 256)) {
+      if (na > 0 && (na == 16 || np + n > FB_2C_PCAP)) {
         wg.push_back(a);
         na = 0;
         np = 0;
