@@ -383,187 +383,3 @@ extern "C" int fbk_2c_sp(const DView &v, int mask, hipStream_t st) {
   hipLaunchKernelGGL(k_2c_sp, dim3(v.nwg2c), dim3(WG2), 0, st, v, mask);
   return (int)hipGetLastError();
 }
-
-// ---------------------------------------------------------------------------
-// Job-parallel variant: the pairs of a workgroup (whole atoms, at most
-// FB_2C_PCAP pairs and 16 atoms) get their frame from one lane each, then one
-// lane per (pair, job) -- S, T, dipole_z, Vna ontop, Vna atom -- interpolates,
-// rotates and stores; the atom terms are summed per atom in neighbour order.
-// Five times the lanes of the pair-per-lane kernel for the same work, which
-// is what this latency-bound, store-heavy kernel needs.
-struct PairPrm {
-  double pm[9];
-  double z;
-  int64_t bo;
-  int ia, ja, in1, in2, cls, self;
-};
-
-template <int CLS>
-__device__ __forceinline__ int job2c(const DView &v, int job, const PairPrm &q, double *atom_c) {
-  constexpr int NMU = (CLS & 2) ? 4 : 1, NNU = (CLS & 1) ? 4 : 1, NBL = NMU * NNU;
-  constexpr int CLSA = (CLS & 2) ? CLS_SPSP : CLS_SS;
-  const int ia = q.ia, ja = q.ja, in1 = q.in1, in2 = q.in2;
-  const double z = q.z;
-  const int64_t bo = q.bo;
-  double pm[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) pm[k] = q.pm[k];
-  int status = 0, grid_cached = -1, ileft = 1;
-  double prod[7], X[16];
-  if (job <= 2) {
-    const int slot = job == 0 ? SLOT_S : (job == 1 ? SLOT_T : SLOT_DIP);
-    const int mat = job == 0 ? FB_MAT_S : (job == 1 ? FB_MAT_T : FB_MAT_DIP);
-    status |= block2c<CLS>(v, cell2c_id(v, in1, in2, slot, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-    for (int i = 0; i < NBL; i++) v.mat[mat][bo + i] = X[i];
-  } else if (job == 3) {
-    double acc[NBL];
-#pragma unroll
-    for (int i = 0; i < NBL; i++) acc[i] = 0.0;
-    if (!q.self) {
-      status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_ONL, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-      for (int i = 0; i < NBL; i++) acc[i] = acc[i] + X[i] * FB_P_EQ2;
-      for (int isorp = 1; isorp <= v.sp.nssh[in1 - 1]; isorp++) {
-        double dQ = v.dQ[ia * FB_MAXSH + isorp - 1];
-        if (dQ == 0.0) continue;
-        status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_ONL, isorp), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-        for (int i = 0; i < NBL; i++) acc[i] = acc[i] + dQ * X[i] * FB_P_EQ2;
-      }
-      status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_ONR, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-      for (int i = 0; i < NBL; i++) acc[i] = acc[i] + X[i] * FB_P_EQ2;
-      for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
-        double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
-        if (dQ == 0.0) continue;
-        status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_ONR, isorp), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-        for (int i = 0; i < NBL; i++) acc[i] = acc[i] + dQ * X[i] * FB_P_EQ2;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < NBL; i++) v.mat[FB_MAT_VNAO][bo + i] = acc[i];
-    if (!q.self) {
-#pragma unroll
-      for (int i = 0; i < NBL; i++) v.mat[FB_MAT_VNA][bo + i] = 0.0;
-    }
-  } else {
-    constexpr int NA = NMU * NMU;
-    status |= block2c<CLSA>(v, cell2c_id(v, in1, in2, SLOT_ATOM, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-    for (int i = 0; i < NA; i++) atom_c[i] = X[i] * FB_P_EQ2;
-    bool charged = false;
-    for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) charged |= (v.dQ[ja * FB_MAXSH + isorp - 1] != 0.0);
-    if (charged) {
-      const int64_t pself = v.noff[ia] + v.neigh_self[ia] - 1;
-      const double *sself = v.mat[FB_MAT_S] + v.boff[pself];
-      const double smooth = d_smoother(z, v.sp.rcmin[in1 - 1] + v.sp.rcmin[in2 - 1], 0.8);
-      for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
-        double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
-        status |= block2c<CLSA>(v, cell2c_id(v, in1, in2, SLOT_ATOM, isorp), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-        for (int i = 0; i < NA; i++) {
-          double emnpl = (z > 1.0e-4) ? sself[i] / z : 0.0;
-          atom_c[i] = atom_c[i] + FB_P_EQ2 * dQ * ((1.0 - smooth) * emnpl + smooth * X[i]);
-        }
-      }
-    }
-  }
-  return status;
-}
-
-#define WGJ 256
-#define NJOB 5
-
-__global__ __launch_bounds__(WGJ) void k_2c_job(DView v, int mask) {
-  __shared__ PairPrm sp_[FB_2C_PCAP];
-  __shared__ double s_at[FB_2C_PCAP][17];
-  const int tid = threadIdx.x;
-  const int a0 = v.wg2c[blockIdx.x], a1 = v.wg2c[blockIdx.x + 1];
-  const int64_t pb = v.noff[a0], pe = v.noff[a1];
-  const int ra = a0 + (tid >> 4), re = tid & 15;
-  const bool rown = ra < a1;
-  int64_t rp0 = 0, rp1 = 0;
-  int rna = 0;
-  if (rown) {
-    rp0 = v.noff[ra];
-    rp1 = v.noff[ra + 1];
-    rna = v.sp.norb[v.imass[ra] - 1];
-  }
-  double racc = 0.0;
-  int status = 0;
-  for (int64_t c0 = pb; c0 < pe; c0 += FB_2C_PCAP) {
-    const int np = (int)min((int64_t)FB_2C_PCAP, pe - c0);
-    if (tid < np) {
-      PairPrm &q = sp_[tid];
-      const int64_t p = c0 + tid;
-      const int ia = v.pair_atom[p];
-      const int s = v.atom_struct[ia];
-      const int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
-      const int mb = v.neigh_b[p];
-      const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
-      double r1[3], r2[3], sighat[3], eps[9], pm[9];
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        r1[k] = v.ratom[3 * ia + k];
-        r2[k] = v.ratom[3 * ja + k] + xls[3 * mb + k];
-      }
-      const double z = d_distance(r1, r2);
-      if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
-      else {
-#pragma unroll
-        for (int k = 0; k < 3; k++) sighat[k] = (r2[k] - r1[k]) / z;
-      }
-      d_epsilon(r2, sighat, eps);
-      d_pmat<double>(eps, pm);
-#pragma unroll
-      for (int k = 0; k < 9; k++) q.pm[k] = pm[k];
-      q.z = z;
-      q.bo = v.boff[p];
-      q.ia = ia;
-      q.ja = ja;
-      q.in1 = v.imass[ia];
-      q.in2 = v.imass[ja];
-      q.cls = (v.sp.norb[q.in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[q.in2 - 1] == 4 ? 1 : 0);
-      q.self = (ja == ia && mb == 0) ? 1 : 0;
-    }
-    __syncthreads();
-    for (int it = tid; it < np * NJOB; it += WGJ) {
-      const int k = it / NJOB, job = it - k * NJOB;
-      const bool on = (job <= 2) ? ((mask >> job) & 1) : ((mask >> 3) & 1);
-      if (!on) continue;
-      const PairPrm &q = sp_[k];
-      double atom_c[16];
-#pragma unroll
-      for (int i = 0; i < 16; i++) atom_c[i] = 0.0;
-      switch (q.cls) {
-        case 3: status |= job2c<3>(v, job, q, atom_c); break;
-        case 2: status |= job2c<2>(v, job, q, atom_c); break;
-        case 1: status |= job2c<1>(v, job, q, atom_c); break;
-        default: status |= job2c<0>(v, job, q, atom_c); break;
-      }
-      if (job == 4) {
-#pragma unroll
-        for (int i = 0; i < 16; i++) s_at[k][i] = atom_c[i];
-      }
-    }
-    __syncthreads();
-    if ((mask & 8) && rown && re < rna * rna) {
-      const int64_t k0 = max(rp0, c0), k1 = min(rp1, c0 + np);
-      for (int64_t k = k0; k < k1; k++) racc = racc + s_at[k - c0][re];
-    }
-    __syncthreads();
-  }
-  if ((mask & 8) && rown && re < rna * rna) {
-    const int64_t pself = v.noff[ra] + v.neigh_self[ra] - 1;
-    v.mat[FB_MAT_VNA][v.boff[pself] + re] = racc;
-  }
-  if (status) atomicOr(v.status, status);
-}
-
-extern "C" int fbk_2c_job(const DView &v, int mask, hipStream_t st) {
-  if (v.nwg2c == 0) return 0;
-  hipLaunchKernelGGL(k_2c_job, dim3(v.nwg2c), dim3(WGJ), 0, st, v, mask);
-  return (int)hipGetLastError();
-}

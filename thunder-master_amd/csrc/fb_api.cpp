@@ -40,7 +40,6 @@ int fbk_2c(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64(const DView &v, hipStream_t st);
 int fbk_3c_mixed(const DView &v, int chunk, int charged, hipStream_t st);
 int fbk_2c_sp(const DView &v, int mask, hipStream_t st);
-int fbk_2c_job(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64_sp(const DView &v, hipStream_t st);
 }
 

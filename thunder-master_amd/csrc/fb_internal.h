@@ -12,7 +12,7 @@
 #define FB_MAXSUB3 7          // include/interactions_3c.h:3
 #define FB_NSLOT2 6           // 2c interactions used on the path
 #define FB_P_EQ2 14.39975     // include/constants.h:4
-#define FB_2C_PCAP 48         // pairs per 2c workgroup pass (job-parallel kernel)
+#define FB_2C_PCAP 256        // pairs per 2c workgroup pass
 
 // 2c interaction slots (device-side dense index of the interaction ids of
 // include/interactions_2c.h: overlap 1, kinetic 2, vna_ontopL 3, vna_ontopR 4,
