@@ -66,7 +66,7 @@ typedef struct fb_ctx fb_ctx;
 /* --- context ------------------------------------------------------------ */
 int fb_create(fb_ctx **ctx, int device);
 int fb_destroy(fb_ctx *ctx);
-const char *fb_last_error(const fb_ctx *ctx);
+const char *fb_last_error(const fb_ctx *ctx);          /* NULL: why this thread's last fb_create failed */
 int fb_set_stream(fb_ctx *ctx, void *hip_stream);      /* NULL: the context's own stream */
 int fb_synchronize(fb_ctx *ctx);
 int fb_set_precision(fb_ctx *ctx, int mode);          /* enum fb_precision */
@@ -103,7 +103,7 @@ int fb_clear_structures(fb_ctx *ctx);
 int fb_add_structure(fb_ctx *ctx, int natoms, const double *ratom, const int *imass,
                      const double *lattice, int icluster, const double *dQ, int *istruct);
 int fb_set_positions(fb_ctx *ctx, int istruct, const double *ratom);   /* new geometry, same atoms */
-int fb_set_charges(fb_ctx *ctx, int istruct, const double *dQ);
+int fb_set_charges(fb_ctx *ctx, int istruct, const double *dQ);      /* invalidates Vna only */
 
 /* --- lists: driver_neighbors, c.SYSTEM/M_neighbors.f90:93-141 ------------
  * (find_neigh_max/find_neighbors :246-519, find_common_max/

@@ -1,0 +1,107 @@
+"""The Fortran drop-in (thunder-master_amd/fortran: modules M_assemble_2c and
+M_assemble_3c over the C ABI) linked into the same harness program as the
+reference's own modules.
+
+The harness (oracle/ref/fb_harness.f90) runs the reference's read_Fdata_2c /
+read_Fdata_3c, make_cells and driver_neighbors, then calls assemble_S, T,
+dipole_z, vna_2c and vna_3c through whichever M_assemble_2c / M_assemble_3c it
+was linked with, and dumps lists and blocks.  oracle/_ref/fb_harness_ref uses
+the reference's modules, oracle/_ref/fb_harness_hip the drop-in ones.  Both
+are built by `make -C oracle ref hip`.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import golden_cases
+import refrun
+
+HIP_HARNESS = os.path.join(os.path.dirname(refrun.HARNESS_REF), "fb_harness_hip")
+LIST_KEYS = ("neighn", "neigh_j", "neigh_b", "neigh_back", "neigh_self", "ncommon", "com_iatom_j",
+             "com_iatom_b", "com_jatom_j", "com_jatom_b", "com_mneigh")
+MATS = ("S", "T", "DIP", "VNA", "VNAO")
+TOL64 = 1e-12
+TOL_MIXED = 1e-6
+
+need_hip = pytest.mark.skipif(not refrun.available(HIP_HARNESS), reason="make -C oracle hip not built")
+
+
+def _run_hip(ts, st, nrep=1, env=None):
+    print(f"hip harness natoms={st.natoms} nrep={nrep} env={env}", flush=True)
+    full = dict(os.environ, **(env or {}))
+    with tempfile.TemporaryDirectory(prefix="fbhip_") as d:
+        fdir = os.path.join(d, "Fdata")
+        from fireball_hip import fdata_io
+        fdata_io.write_fdata(ts, fdir)
+        inp = os.path.join(d, "harness.in")
+        fdata_io.write_harness_input(inp, fdir, ts, st)
+        outp = os.path.join(d, "harness.out")
+        r = subprocess.run([HIP_HARNESS, inp, outp, str(nrep)], cwd=d, env=full, capture_output=True,
+                           text=True, timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError(f"hip harness exit {r.returncode}: {r.stdout[-2000:]} {r.stderr[-2000:]}")
+        return refrun.parse_output(outp, ts.species, st.imass)
+
+
+@need_hip
+def test_hip_harness_fails_loudly_without_gpu():
+    """With no GPU visible the drop-in stops with the library's message and a
+    nonzero status; it never falls back to CPU assembly."""
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is visible here")
+    from fireball_hip import synth
+    ts = synth.make_tables([synth.SI])
+    st = synth.diamond_si((1, 1, 1), sigma=0.05, seed=0)
+    with pytest.raises(RuntimeError, match="fb_create"):
+        _run_hip(ts, st)
+
+
+@need_hip
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", golden_cases.NAMES)
+def test_dropin_matches_reference_golden(golden, name):
+    print(f"case {name}", flush=True)
+    g = golden(name)
+    ts, st = golden_cases.inputs(g)
+    r = _run_hip(ts, st)
+    for k in LIST_KEYS:
+        assert np.array_equal(r[k], g[k]), k
+    n = len(g["S"])
+    for m in MATS:
+        np.testing.assert_allclose(r[m][:n], g[m], rtol=0, atol=TOL64, err_msg=m)
+        # whole-matrix checksums of the golden vectors cover the blocks beyond n
+        assert abs(float(np.sum(r[m])) - float(g[m + "_sum"])) <= 1e-9 * max(1.0, float(g[m + "_sumabs"])), m
+
+
+@need_hip
+@pytest.mark.gpu
+def test_dropin_matches_reference_live_repeated():
+    """Si216 with shell charges, assembled twice in one process (the second
+    pass exercises destroy_assemble_2c / destroy_assemble_vna and the cached
+    lists), against the reference modules run on the same input."""
+    if not refrun.available():
+        pytest.skip("reference harness not built")
+    from fireball_hip import synth
+    ts = synth.make_tables([synth.SI])
+    st = synth.diamond_si((3, 3, 3), sigma=0.08, seed=5)
+    synth.random_dQ([st], [synth.SI], seed=5)
+    ref = refrun.run(ts, st)
+    hip = _run_hip(ts, st, nrep=2)
+    for k in LIST_KEYS:
+        assert np.array_equal(hip[k], ref[k]), k
+    for m in MATS:
+        np.testing.assert_allclose(hip[m], ref[m], rtol=0, atol=TOL64, err_msg=m)
+
+
+@need_hip
+@pytest.mark.gpu
+def test_dropin_mixed_precision_within_tolerance(golden):
+    g = golden("cho40_dq")
+    ts, st = golden_cases.inputs(g)
+    r = _run_hip(ts, st, env={"FIREBALL_HIP_PRECISION": "mixed"})
+    n = len(g["S"])
+    for m in MATS:
+        np.testing.assert_allclose(r[m][:n], g[m], rtol=0, atol=TOL_MIXED, err_msg=m)

@@ -401,28 +401,28 @@ static int make_cells(const double *lat, int icluster, double *xl) {
 // ------------------------------------------------------------------- C ABI
 extern "C" {
 
+// why the last fb_create of this thread failed (fb_last_error(NULL) returns it)
+static thread_local std::string g_create_err = "null context";
+
+static int create_failed(fb_ctx *ctx, const char *what, hipError_t e) {
+  delete ctx;
+  g_create_err = std::string("fb_create: ") + what + ": " + hipGetErrorString(e);
+  return FB_E_HIP;
+}
+
 int fb_create(fb_ctx **out, int device) {
   if (!out) return FB_E_ARG;
   *out = nullptr;
   fb_ctx *ctx = new fb_ctx();
   ctx->device = device;
   hipError_t e = hipSetDevice(device);
-  if (e != hipSuccess) {
-    delete ctx;
-    return FB_E_HIP;
-  }
+  if (e != hipSuccess) return create_failed(ctx, "hipSetDevice", e);
   e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
-  if (e != hipSuccess) {
-    delete ctx;
-    return FB_E_HIP;
-  }
+  if (e != hipSuccess) return create_failed(ctx, "stream", e);
   ctx->stream = ctx->own_stream;
   e = ctx->d_status.ensure(sizeof(int) * 4);
   if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_status, sizeof(int) * 4);
-  if (e != hipSuccess) {
-    delete ctx;
-    return FB_E_HIP;
-  }
+  if (e != hipSuccess) return create_failed(ctx, "status buffers", e);
   ctx->h_status[0] = 0;
   *out = ctx;
   return FB_OK;
@@ -436,7 +436,7 @@ int fb_destroy(fb_ctx *ctx) {
   return FB_OK;
 }
 
-const char *fb_last_error(const fb_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *fb_last_error(const fb_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 
 int fb_set_stream(fb_ctx *ctx, void *s) {
   if (!ctx) return FB_E_ARG;
@@ -750,7 +750,9 @@ int fb_set_charges(fb_ctx *ctx, int is, const double *dQ) {
   std::fill(s.dQ.begin(), s.dQ.end(), 0.0);
   if (dQ) std::memcpy(s.dQ.data(), dQ, sizeof(double) * s.natoms * FB_MAXSH);
   ctx->structs_dirty = true;  // charges change no list, but must be re-uploaded
-  for (int i = 0; i < 5; i++) ctx->done[i] = 0;
+  // only the Vna blocks depend on the shell charges: S, T and dipole_z stay valid,
+  // as in a self-consistent loop that re-assembles Vna per charge update
+  ctx->done[FB_MAT_VNA] = ctx->done[FB_MAT_VNAO] = 0;
   return FB_OK;
 }
 

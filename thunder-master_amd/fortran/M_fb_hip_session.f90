@@ -1,0 +1,283 @@
+! GPU session shared by the drop-in M_assemble_2c / M_assemble_3c modules.
+!
+! One library context per process.  On first use it takes the species from
+! M_species (species(:)%shell(:)%lssh, %rcutoffA) and every Fdata cell of the
+! assembled interactions from Fdata_bundle_2c / Fdata_bundle_3c
+! (e.FDATA/M_Fdata_2c.f90:95-110, e.FDATA/M_Fdata_3c.f90:88-105), so the
+! caller's read_Fdata_2c / read_Fdata_3c remain the only table reader.
+!
+! Per call it mirrors the caller's T_structure into the library: a change of
+! natoms, species, lattice or icluster re-adds the structure, a change of
+! positions only replaces them, and either rebuilds the lists on the GPU.  The
+! rebuilt lists must equal the caller's s%neighbors (driver_neighbors,
+! c.SYSTEM/M_neighbors.f90:93-141), pair for pair and triplet for triplet:
+! blocks are returned in that pair order, so a mismatch stops the run.
+!
+! Environment: FIREBALL_HIP_DEVICE (default 0), FIREBALL_HIP_PRECISION
+! ("fp64", default, or "mixed").
+        module M_fb_hip_session
+        use iso_c_binding
+        use fb_hip_c
+        use M_assemble_blocks
+        use M_species
+        use M_configuraciones
+        use M_Fdata_2c
+        use M_Fdata_3c
+        implicit none
+
+        type(c_ptr), save :: fbh = c_null_ptr
+        integer, save :: c_natoms = -1, c_icluster = -1
+        integer(c_int), allocatable, save :: c_imass(:)
+        real(c_double), allocatable, save :: c_ratom(:, :)
+        real(c_double), save :: c_lattice(3, 3)
+        logical, save :: have_lists = .false.
+        integer(c_int64_t), save :: npairs_all = 0, ntrip_all = 0
+        real(c_double), allocatable, save :: fb_buf(:)
+
+        ! 2c interactions of the assembled path (include/interactions_2c.h:5-14)
+        integer, parameter :: n_itype2c = 6
+        integer, parameter :: itype2c(n_itype2c) = [1, 2, 3, 4, 5, 10]
+        ! 3c: P_bcna (include/interactions_3c.h:8) over five Legendre angles
+        integer, parameter :: itype3c_bcna = 3, ntheta3c = 5
+
+        contains
+
+        subroutine fb_session_open ()
+        integer(c_int) :: dev, prec
+        integer ispecies, issh, nsp, is, js, ks, k, itype, isub, itheta, idx, st
+        integer(c_int), allocatable :: nZ(:), nssh(:), lssh(:, :)
+        real(c_double), allocatable :: rcA(:, :)
+        character(len=32) :: val
+        type(T_Fdata_cell_2c), pointer :: c2
+        type(T_Fdata_cell_3c), pointer :: c3
+
+        if (c_associated(fbh)) return
+        dev = 0
+        call get_environment_variable ('FIREBALL_HIP_DEVICE', val, status = st)
+        if (st .eq. 0) read (val, *) dev
+        call fb_check (fb_create(fbh, dev), fbh, 'fb_create')
+        prec = FB_PREC_FP64
+        call get_environment_variable ('FIREBALL_HIP_PRECISION', val, status = st)
+        if (st .eq. 0 .and. trim(val) .eq. 'mixed') prec = FB_PREC_MIXED
+        call fb_check (fb_set_precision(fbh, prec), fbh, 'fb_set_precision')
+
+        nsp = size(species)
+        allocate (nZ(nsp), nssh(nsp), lssh(FB_MAXSH, nsp), rcA(FB_MAXSH, nsp))
+        lssh = 0
+        rcA = 0.0d0
+        do ispecies = 1, nsp
+          nZ(ispecies) = species(ispecies)%nZ
+          nssh(ispecies) = species(ispecies)%nssh
+          if (nssh(ispecies) .gt. FB_MAXSH) error stop ' fireball_hip: more than 4 shells '
+          do issh = 1, species(ispecies)%nssh
+            lssh(issh, ispecies) = species(ispecies)%shell(issh)%lssh
+            rcA(issh, ispecies) = species(ispecies)%shell(issh)%rcutoffA
+          end do
+        end do
+        call fb_check (fb_set_species(fbh, nsp, nZ, nssh, lssh, rcA), fbh, 'fb_set_species')
+
+        do is = 1, nsp
+          do js = 1, nsp
+            do k = 1, n_itype2c
+              itype = itype2c(k)
+              do isub = 0, ubound(Fdata_bundle_2c(is, js)%index_2c, 2)
+                idx = Fdata_bundle_2c(is, js)%index_2c(itype, isub)
+                if (idx .le. 0) cycle
+                c2 => Fdata_bundle_2c(is, js)%Fdata_cell_2c(idx)
+                call fb_check (fb_set_table_2c(fbh, is, js, itype, isub, c2%nME, c2%nx,    &
+     &            c2%xmax, c2%mu_2c, c2%nu_2c, c2%mvalue_2c, c2%Fdata_2c), fbh,            &
+     &            'fb_set_table_2c')
+              end do
+            end do
+            do ks = 1, nsp
+              do isub = 0, species(ks)%nssh
+                do itheta = 1, ntheta3c
+                  idx = Fdata_bundle_3c(is, js, ks)%index_3c(itype3c_bcna, isub, itheta)
+                  if (idx .le. 0) cycle
+                  c3 => Fdata_bundle_3c(is, js, ks)%Fdata_cell_3c(idx)
+                  call fb_check (fb_set_table_3c(fbh, is, js, ks, itype3c_bcna, isub,      &
+     &              itheta, c3%nME, c3%nx, c3%xmax, c3%ny, c3%ymax, c3%mu_3c, c3%nu_3c,    &
+     &              c3%mvalue_3c, c3%Fdata_3c), fbh, 'fb_set_table_3c')
+                end do
+              end do
+            end do
+          end do
+        end do
+        call fb_check (fb_commit_tables(fbh), fbh, 'fb_commit_tables')
+        deallocate (nZ, nssh, lssh, rcA)
+        end subroutine fb_session_open
+
+! Mirror the caller's structure; rebuild and check the lists when it changed.
+        subroutine fb_session_sync (s)
+        type(T_structure), target :: s
+        integer iatom, i
+        logical same_atoms, same_pos
+        integer(c_int) :: idx
+        real(c_double), allocatable :: r(:, :), dq(:, :)
+        integer(c_int), allocatable :: im(:)
+
+        call fb_session_open ()
+        allocate (r(3, s%natoms), im(s%natoms))
+        do iatom = 1, s%natoms
+          r(:, iatom) = s%atom(iatom)%ratom
+          im(iatom) = s%atom(iatom)%imass
+        end do
+        same_atoms = s%natoms .eq. c_natoms .and. s%icluster .eq. c_icluster
+        if (same_atoms) then
+          same_atoms = all(im .eq. c_imass)
+          do i = 1, 3
+            same_atoms = same_atoms .and. all(s%lattice(i)%a .eq. c_lattice(:, i))
+          end do
+        end if
+        same_pos = .false.
+        if (same_atoms) same_pos = all(r .eq. c_ratom)
+        if (same_atoms .and. same_pos .and. have_lists) then
+          deallocate (r, im)
+          return
+        end if
+
+        if (.not. same_atoms) then
+          if (allocated(c_imass)) deallocate (c_imass, c_ratom)
+          allocate (c_imass(s%natoms), c_ratom(3, s%natoms))
+          c_natoms = s%natoms
+          c_icluster = s%icluster
+          c_imass = im
+          do i = 1, 3
+            c_lattice(:, i) = s%lattice(i)%a
+          end do
+          allocate (dq(FB_MAXSH, s%natoms))
+          call shell_charges (s, dq)
+          call fb_check (fb_clear_structures(fbh), fbh, 'fb_clear_structures')
+          call fb_check (fb_add_structure(fbh, c_natoms, r, c_imass, c_lattice,          &
+     &      c_icluster, dq, idx), fbh, 'fb_add_structure')
+          deallocate (dq)
+        else
+          call fb_check (fb_set_positions(fbh, 0, r), fbh, 'fb_set_positions')
+        end if
+        c_ratom = r
+        call fb_check (fb_build_lists(fbh, npairs_all, ntrip_all), fbh, 'fb_build_lists')
+        have_lists = .true.
+        call check_lists (s)
+        deallocate (r, im)
+        end subroutine fb_session_sync
+
+! Hand the current shell charges to the library (Vna only depends on them).
+        subroutine fb_session_charges (s)
+        type(T_structure), target :: s
+        real(c_double), allocatable :: dq(:, :)
+        allocate (dq(FB_MAXSH, s%natoms))
+        call shell_charges (s, dq)
+        call fb_check (fb_set_charges(fbh, 0, dq), fbh, 'fb_set_charges')
+        deallocate (dq)
+        end subroutine fb_session_charges
+
+        subroutine shell_charges (s, dq)
+        type(T_structure), target :: s
+        real(c_double), intent(out) :: dq(FB_MAXSH, s%natoms)
+        integer iatom, issh
+        dq = 0.0d0
+        do iatom = 1, s%natoms
+          if (.not. associated(s%atom(iatom)%shell)) cycle
+          do issh = 1, min(size(s%atom(iatom)%shell), FB_MAXSH)
+            dq(issh, iatom) = s%atom(iatom)%shell(issh)%dQ
+          end do
+        end do
+        end subroutine shell_charges
+
+! The library's lists against the caller's: neighbors (neighn, neigh_j,
+! neigh_b) and common neighbors (ncommon and the five per-triplet arrays).
+        subroutine check_lists (s)
+        type(T_structure), target :: s
+        integer(c_int), allocatable :: nn(:), nj(:), nb(:), nback(:), nself(:)
+        integer(c_int), allocatable :: nc(:), cij(:), cib(:), cjj(:), cjb(:), cm(:)
+        integer iatom, p, n
+        logical ok
+
+        allocate (nn(s%natoms), nself(s%natoms), nc(s%natoms))
+        allocate (nj(max(npairs_all, 1_8)), nb(max(npairs_all, 1_8)), nback(max(npairs_all, 1_8)))
+        allocate (cij(max(ntrip_all, 1_8)), cib(max(ntrip_all, 1_8)), cjj(max(ntrip_all, 1_8)))
+        allocate (cjb(max(ntrip_all, 1_8)), cm(max(ntrip_all, 1_8)))
+        call fb_check (fb_get_neighbors(fbh, 0, nn, nj, nb, nback, nself), fbh, 'fb_get_neighbors')
+        call fb_check (fb_get_common(fbh, 0, nc, cij, cib, cjj, cjb, cm), fbh, 'fb_get_common')
+        ok = .true.
+        p = 0
+        do iatom = 1, s%natoms
+          n = s%neighbors(iatom)%neighn
+          if (n .ne. nn(iatom)) then
+            ok = .false.
+            exit
+          end if
+          if (any(s%neighbors(iatom)%neigh_j(1:n) .ne. nj(p+1:p+n))) ok = .false.
+          if (any(s%neighbors(iatom)%neigh_b(1:n) .ne. nb(p+1:p+n))) ok = .false.
+          p = p + n
+        end do
+        if (ok) then
+          p = 0
+          do iatom = 1, s%natoms
+            n = s%neighbors(iatom)%ncommon
+            if (n .ne. nc(iatom)) then
+              ok = .false.
+              exit
+            end if
+            if (n .eq. 0) cycle
+            if (any(s%neighbors(iatom)%iatom_common_j(1:n) .ne. cij(p+1:p+n))) ok = .false.
+            if (any(s%neighbors(iatom)%iatom_common_b(1:n) .ne. cib(p+1:p+n))) ok = .false.
+            if (any(s%neighbors(iatom)%jatom_common_j(1:n) .ne. cjj(p+1:p+n))) ok = .false.
+            if (any(s%neighbors(iatom)%jatom_common_b(1:n) .ne. cjb(p+1:p+n))) ok = .false.
+            if (any(s%neighbors(iatom)%neigh_common(1:n) .ne. cm(p+1:p+n))) ok = .false.
+            p = p + n
+          end do
+        end if
+        deallocate (nn, nj, nb, nback, nself, nc, cij, cib, cjj, cjb, cm)
+        if (.not. ok) then
+          write (*,*) ' fireball_hip: the GPU neighbor lists differ from s%neighbors'
+          error stop ' fireball_hip failure '
+        end if
+        end subroutine check_lists
+
+! Fetch one assembled matrix (pair order of s%neighbors, each block column-major
+! norb_mu x norb_nu) into the session buffer.
+        subroutine fb_session_fetch (imat)
+        integer(c_int), intent(in) :: imat
+        integer(c_int64_t) :: blen
+        call fb_check (fb_get_block_len(fbh, 0, blen), fbh, 'fb_get_block_len')
+        if (allocated(fb_buf)) then
+          if (size(fb_buf, kind=8) .lt. blen) deallocate (fb_buf)
+        end if
+        if (.not. allocated(fb_buf)) allocate (fb_buf(max(blen, 1_8)))
+        call fb_check (fb_get_blocks(fbh, 0, imat, fb_buf), fbh, 'fb_get_blocks')
+        end subroutine fb_session_fetch
+
+! Allocate (when asked) and fill the blocks of one T_assemble_neighbors array
+! from the session buffer; which = 0 -> %block, 1 -> %blocko.
+        subroutine fb_session_store (s, m, which, alloc)
+        type(T_structure), target :: s
+        type(T_assemble_neighbors), pointer :: m(:)
+        integer, intent(in) :: which
+        logical, intent(in) :: alloc
+        integer iatom, ineigh, jatom, nmu, nnu, nb
+        integer(c_int64_t) :: off
+        type(T_assemble_block), pointer :: pb
+
+        off = 0
+        do iatom = 1, s%natoms
+          nmu = species(s%atom(iatom)%imass)%norb_max
+          if (alloc .and. which .eq. 0) allocate (m(iatom)%neighbors(s%neighbors(iatom)%neighn))
+          do ineigh = 1, s%neighbors(iatom)%neighn
+            jatom = s%neighbors(iatom)%neigh_j(ineigh)
+            nnu = species(s%atom(jatom)%imass)%norb_max
+            nb = nmu*nnu
+            pb => m(iatom)%neighbors(ineigh)
+            if (which .eq. 0) then
+              if (alloc) allocate (pb%block(nmu, nnu))
+              pb%block = reshape(fb_buf(off+1:off+nb), [nmu, nnu])
+            else
+              if (alloc) allocate (pb%blocko(nmu, nnu))
+              pb%blocko = reshape(fb_buf(off+1:off+nb), [nmu, nnu])
+            end if
+            off = off + nb
+          end do
+        end do
+        end subroutine fb_session_store
+
+        end module M_fb_hip_session
