@@ -48,8 +48,10 @@ MATS = ("S", "T", "DIP", "VNA", "VNAO")
 
 
 def algorithmic_bytes_3c(asm, nstruct):
-    """Compulsory HBM bytes of one 3c launch: the sorted triplet records
-    (int4 + target pair + segment id = 24 B) and one store per targeted block."""
+    """Algorithmic HBM bytes of one 3c launch: 24 B per triplet (six int32:
+    ialpha, iatom, jatom, the two cells and the target pair) plus one binary64
+    read-modify-write (16 B) per element of every targeted Vna block, the
+    accumulate-into-existing-blocks semantics of assemble_vna_3c."""
     ntrip = 0
     tgt_bytes = 0
     for k in range(nstruct):
@@ -64,7 +66,7 @@ def algorithmic_bytes_3c(asm, nstruct):
         tgt = np.unique(off[L["com_iatom_j"] - 1] + L["com_mneigh"] - 1)
         nmu = norb[st.imass[atom_of_pair[tgt]] - 1]
         nnu = norb[st.imass[L["neigh_j"][tgt] - 1] - 1]
-        tgt_bytes += int(np.sum(nmu * nnu * 8))
+        tgt_bytes += int(np.sum(nmu * nnu * 16))
     return 24 * ntrip + tgt_bytes, ntrip
 
 
@@ -99,6 +101,21 @@ def time_graph(asm, steps, warmup):
         asm.assemble_all(sync=False)
     asm.synchronize()
     return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def gamma_eigen_shift(asm, precision_a, precision_b):
+    """Eigenvalue shift of the Gamma-point folded H = T + Vna (block + blocko)
+    of structure 0 between two precisions, from the library's own fold and
+    rocSOLVER dsyevd on the upper triangle; returns max |lambda_a - lambda_b|
+    in eV and the orbital count.  S is binary64 in both paths."""
+    lam = {}
+    for prec in (precision_a, precision_b):
+        asm.set_precision(prec)
+        asm.assemble_all()
+        lam[prec] = asm.gamma_eigenvalues("H", 0)
+    asm.set_precision(precision_a)
+    return {"H": float(np.max(np.abs(lam[precision_a] - lam[precision_b]))), "norb": int(lam[precision_a].size),
+            "lambda_range_eV": [float(lam[precision_b][0]), float(lam[precision_b][-1])]}
 
 
 def measured_traffic(kernel, config):
@@ -165,6 +182,7 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=("mixed", "fp64"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--no-eig", action="store_true", help="skip the Gamma-point eigenvalue shift")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one elementwise copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -201,6 +219,12 @@ def main():
         ms64, kern64 = time_builds(asm, args.steps, args.warmup)
         res64 = {m: np.concatenate([asm.blocks(m, k) for k in range(len(structs))]) for m in MATS}
         err = {m: float(np.max(np.abs(res_main[m] - res64[m]))) for m in MATS}
+        frob = {m: float(np.linalg.norm(res_main[m] - res64[m]) / max(np.linalg.norm(res64[m]), 1e-300))
+                for m in MATS}
+        t0 = time.perf_counter()
+        eig = gamma_eigen_shift(asm, args.precision, "fp64") if not args.no_eig else None
+        print(f"# eigenvalue shift {eig} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+        extra.update(rel_frobenius_vs_fp64=frob, gamma_eigen_shift_eV=eig)
         extra.update(fp64_ms_per_step=ms64, speedup_vs_fp64=ms64 / ms, max_abs_err_vs_fp64_eV=err,
                      fp64_kernels_ms=kern64)
         asm.set_precision(args.precision)

@@ -132,6 +132,18 @@ int fb_get_block_len(fb_ctx *ctx, int istruct, int64_t *len);
 int fb_get_blocks(fb_ctx *ctx, int istruct, int matrix, double *out);
 int fb_get_device_blocks(fb_ctx *ctx, int matrix, void **dptr, int64_t *len);
 
+/* --- consumer: Gamma-point eigenvalues ------------------------------------
+ * Folds the blocks of structure istruct at k = 0 into a dense binary64 matrix
+ * (build_Hmatrix, h.SOLVESH/M_kspace_Lowdin.f90:180-271, 630-724) -- which = 0:
+ * S, which = 1: H = (T + Vna block) + Vna blocko, the in-scope terms -- and
+ * returns its eigenvalues in ascending order from rocSOLVER dsyevd on the
+ * upper triangle, the triangle dsyev('V','U') reads in the reference
+ * (M_diagonalization_Lowdin_gamma.f90:173-178).  *n_out receives the orbital
+ * count; w must hold that many values.  rocSOLVER is loaded on first use
+ * (FB_E_UNSUPPORTED when absent).  The n x n matrix is allocated for the call
+ * and freed before it returns. */
+int fb_gamma_eigenvalues(fb_ctx *ctx, int istruct, int which, double *w, int64_t nw, int64_t *n_out);
+
 /* --- tuning ------------------------------------------------------------
  * No counterpart in the reference (its assembler has no launch parameters).
  * FB_TUNE_CHUNK3C: triplets per workgroup chunk of the mixed 3c kernel (64,

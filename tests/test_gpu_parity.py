@@ -240,3 +240,40 @@ def test_async_error_surfaces_at_synchronize():
     with pytest.raises(FireballError) as e:
         asm.synchronize()
     assert e.value.code == -3
+
+
+def _fold_upper(blocks, neighn, neigh_j, imass, norb_sp):
+    """Gamma fold of pair blocks into the upper triangle, symmetric completion."""
+    no = norb_sp[np.asarray(imass) - 1]
+    off = np.concatenate([[0], np.cumsum(no)])
+    n = int(off[-1])
+    A = np.zeros((n, n))
+    ia = np.repeat(np.arange(len(neighn)), neighn)
+    b = 0
+    for p in range(len(ia)):
+        i, j = ia[p], neigh_j[p] - 1
+        nm, nn = no[i], no[j]
+        A[off[i]:off[i] + nm, off[j]:off[j] + nn] += blocks[b:b + nm * nn].reshape(nn, nm).T
+        b += nm * nn
+    U = np.triu(A)
+    return U + np.triu(A, 1).T
+
+
+@pytest.mark.parametrize("name", ["si64", "cho40_dq"])
+def test_gamma_eigenvalues_match_numpy(golden, name):
+    """Gamma fold + rocSOLVER dsyevd (upper triangle) against numpy eigvalsh of
+    the same fold of the oracle's blocks."""
+    print(f"case {name}", flush=True)
+    g = golden(name)
+    ts, st = golden_cases.inputs(g)
+    asm = _gpu(ts, [st])
+    o = oracle.System(oracle.Tables(ts), st).run_all()
+    norb_sp = np.array([sp.norb for sp in ts.species])
+    for which, mats in (("S", ("S",)), ("H", ("T", "VNA", "VNAO"))):
+        ref_blocks = o[mats[0]].copy()
+        for m in mats[1:]:
+            ref_blocks = ref_blocks + o[m]
+        ref = np.linalg.eigvalsh(_fold_upper(ref_blocks, o["neighn"], o["neigh_j"], st.imass, norb_sp))
+        w = asm.gamma_eigenvalues(which)
+        assert w.shape == ref.shape
+        np.testing.assert_allclose(w, ref, rtol=0, atol=1e-9 * max(1.0, float(np.max(np.abs(ref)))), err_msg=which)

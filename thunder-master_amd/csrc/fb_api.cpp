@@ -3,7 +3,9 @@
 // Owns the device copies of species, tables, structures, lists and blocks,
 // orders every launch on one HIP stream and turns kernel status bits into the
 // FB_E_* codes that replace the reference's `stop` statements.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -41,6 +43,8 @@ int fbk_3c_fp64(const DView &v, hipStream_t st);
 int fbk_3c_mixed(const DView &v, int chunk, int charged, hipStream_t st);
 int fbk_2c_sp(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64_sp(const DView &v, hipStream_t st);
+int fbk_gamma_fold(const DView &v, int a0, int natoms, int which, const int *orb_off, int n, double *A,
+                   hipStream_t st);
 }
 
 namespace {
@@ -1155,6 +1159,94 @@ int fb_assemble_all(fb_ctx *ctx) {
   }
   ctx->pending = true;
   for (int i = 0; i < 5; i++) ctx->done[i] = 1;
+  return FB_OK;
+}
+
+// rocSOLVER is loaded on first use, so the library itself does not depend on it
+namespace {
+struct Solver {
+  bool tried = false, ok = false;
+  decltype(&rocblas_create_handle) create = nullptr;
+  decltype(&rocblas_destroy_handle) destroy = nullptr;
+  decltype(&rocblas_set_stream) set_stream = nullptr;
+  decltype(&rocsolver_dsyevd) dsyevd = nullptr;
+  std::string why;
+};
+Solver g_solver;
+
+bool load_solver() {
+  Solver &S = g_solver;
+  if (S.tried) return S.ok;
+  S.tried = true;
+  void *hb = dlopen("librocblas.so.5", RTLD_NOW | RTLD_LOCAL);
+  if (!hb) hb = dlopen("/opt/rocm/lib/librocblas.so.5", RTLD_NOW | RTLD_LOCAL);
+  void *hs = dlopen("librocsolver.so.0", RTLD_NOW | RTLD_LOCAL);
+  if (!hs) hs = dlopen("/opt/rocm/lib/librocsolver.so.0", RTLD_NOW | RTLD_LOCAL);
+  if (!hb || !hs) {
+    S.why = "rocBLAS/rocSOLVER not found";
+    return false;
+  }
+  S.create = (decltype(S.create))dlsym(hb, "rocblas_create_handle");
+  S.destroy = (decltype(S.destroy))dlsym(hb, "rocblas_destroy_handle");
+  S.set_stream = (decltype(S.set_stream))dlsym(hb, "rocblas_set_stream");
+  S.dsyevd = (decltype(S.dsyevd))dlsym(hs, "rocsolver_dsyevd");
+  S.ok = S.create && S.destroy && S.set_stream && S.dsyevd;
+  if (!S.ok) S.why = "rocBLAS/rocSOLVER symbols missing";
+  return S.ok;
+}
+}  // namespace
+
+int fb_gamma_eigenvalues(fb_ctx *ctx, int is, int which, double *w, int64_t nw, int64_t *n_out) {
+  if (!ctx || !ctx->lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "lists not built") : FB_E_ARG;
+  if (is < 0 || is >= (int)ctx->structs.size() || (which != 0 && which != 1))
+    return ctx->fail(FB_E_ARG, "gamma_eigenvalues arguments");
+  if (which == 0 && !ctx->done[FB_MAT_S]) return ctx->fail(FB_E_ORDER, "gamma_eigenvalues: S not assembled");
+  if (which == 1 && !(ctx->done[FB_MAT_T] && ctx->done[FB_MAT_VNA]))
+    return ctx->fail(FB_E_ORDER, "gamma_eigenvalues: T and Vna not assembled");
+  FB_HIP(hipSetDevice(ctx->device));
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  RC(ctx->settle("assemble_all"));
+  const DStruct &st = ctx->h_st[is];
+  std::vector<int> orb(st.natoms + 1, 0);
+  for (int i = 0; i < st.natoms; i++) orb[i + 1] = orb[i] + ctx->sp.norb[ctx->h_imass[st.atom_off + i] - 1];
+  const int n = orb[st.natoms];
+  if (n_out) *n_out = n;
+  if (!w || nw < n) return ctx->fail(FB_E_ARG, "gamma_eigenvalues: output shorter than the orbital count");
+  if (!load_solver()) return ctx->fail(FB_E_UNSUPPORTED, g_solver.why.c_str());
+  FB_HIP(hipStreamSynchronize(ctx->stream));  // buffers freed below must not be in use by queued work
+  DevBuf dA, dorb, dD, dE, dinfo;
+  struct Release {  // the dense matrix is transient: free every buffer on return
+    DevBuf *b[5];
+    ~Release() {
+      for (DevBuf *x : b) x->release();
+    }
+  } release_all{{&dA, &dorb, &dD, &dE, &dinfo}};
+  hipError_t e = dA.ensure(sizeof(double) * (size_t)n * n);
+  if (e == hipSuccess) e = dorb.ensure(sizeof(int) * orb.size());
+  if (e == hipSuccess) e = dD.ensure(sizeof(double) * n);
+  if (e == hipSuccess) e = dE.ensure(sizeof(double) * n);
+  if (e == hipSuccess) e = dinfo.ensure(sizeof(int));
+  if (e != hipSuccess) return ctx->fail(FB_E_HIP, "gamma_eigenvalues buffers", e);
+  FB_HIP(hipMemcpyAsync(dorb.p, orb.data(), sizeof(int) * orb.size(), hipMemcpyHostToDevice, ctx->stream));
+  DView v = ctx->view();
+  int rc = fbk_gamma_fold(v, st.atom_off, st.natoms, which, dorb.as<int>(), n, dA.as<double>(), ctx->stream);
+  if (rc) return ctx->fail(FB_E_HIP, "gamma fold launch", (hipError_t)rc);
+  rocblas_handle h = nullptr;
+  if (g_solver.create(&h) != rocblas_status_success) return ctx->fail(FB_E_HIP, "rocblas_create_handle");
+  rocblas_status rs = g_solver.set_stream(h, ctx->stream);
+  if (rs == rocblas_status_success)
+    rs = g_solver.dsyevd(h, rocblas_evect_none, rocblas_fill_upper, n, dA.as<double>(), n, dD.as<double>(),
+                         dE.as<double>(), dinfo.as<int>());
+  int info = 0;
+  if (rs == rocblas_status_success) {
+    e = hipMemcpyAsync(w, dD.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  }
+  g_solver.destroy(h);
+  if (rs != rocblas_status_success) return ctx->fail(FB_E_HIP, "rocsolver_dsyevd");
+  if (e != hipSuccess) return ctx->fail(FB_E_HIP, "gamma_eigenvalues copy", e);
+  if (info != 0) return ctx->fail(FB_E_HIP, "rocsolver_dsyevd did not converge");
   return FB_OK;
 }
 

@@ -69,6 +69,7 @@ def lib():
             "fb_set_tuning": [vp, C.c_int, C.c_int], "fb_get_tuning": [vp, C.c_int, ip],
             "fb_autotune": [vp, C.c_int, ip],
             "fb_get_timings": [vp, C.c_int, C.c_char_p, C.POINTER(C.c_float), ip],
+            "fb_gamma_eigenvalues": [vp, C.c_int, C.c_int, dp, C.c_int64, i64p],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -236,6 +237,17 @@ class Assembler:
         out = np.zeros(max(n.value, 1))
         self._check(self.L.fb_get_blocks(self.h, istruct, FB_MAT[mat], _dp(out)))
         return out[:n.value]
+
+    def gamma_eigenvalues(self, which="H", istruct=0):
+        """Eigenvalues (ascending) of the Gamma-point folded S or
+        H = T + Vna block + Vna blocko of one structure, on the GPU."""
+        st = self.structs[istruct]
+        norb = np.array([sp.norb for sp in self.tables.species])
+        n = C.c_int64(int(norb[np.asarray(st.imass) - 1].sum()))
+        w = np.zeros(max(n.value, 1))
+        self._check(self.L.fb_gamma_eigenvalues(self.h, istruct, {"S": 0, "H": 1}[which], _dp(w), n.value,
+                                                C.byref(n)))
+        return w[:n.value]
 
     def set_chunk3c(self, chunk):
         self._check(self.L.fb_set_tuning(self.h, 0, int(chunk)))
