@@ -144,6 +144,44 @@ int fb_get_device_blocks(fb_ctx *ctx, int matrix, void **dptr, int64_t *len);
  * and freed before it returns. */
 int fb_gamma_eigenvalues(fb_ctx *ctx, int istruct, int which, double *w, int64_t nw, int64_t *n_out);
 
+/* --- learned 3c surrogate ------------------------------------------------
+ * No counterpart in the reference.  One network per neutral 3c group (species
+ * triple): inputs (x, y, cos theta), two hidden layers of 64 SiLU units,
+ * outputs the five Legendre theta cells of every matrix element that
+ * getMEs_Fdata_3c interpolates (e.FDATA/M_Fdata_3c.f90:391-489).
+ * fb_mlp_train samples nsamples points of every neutral group from the
+ * binary64 table path, trains in FP32 on the GPU (Adam, batch, nsteps, cosine
+ * learning-rate decay from lr), stores BF16 weights, and calibrates the
+ * largest |network - table| of every table cell from fresh samples.  With
+ * fb_set_3c_method(FB_3C_MLP) and mixed precision, the 3c launch evaluates
+ * the network on MFMA for each triplet whose bound
+ *   2 * P_eq2 * sqrt(nME) * sum_theta |P_theta(cos)| * cell error
+ * is at most the tolerance (fb_mlp_set_tolerance, eV, default 1e-4) and the
+ * table stencil for every other triplet.  Charged builds use the tables.
+ * fb_mlp_info: 12 values per group: is, js, ks, isub, trained, nsamples,
+ *   nsteps, loss before, loss after (normalised MSE), max cell error (eV),
+ *   training ms, calibration samples.
+ * fb_mlp_stats: triplets that took the network and triplets seen by MLP 3c
+ *   launches since the previous call.
+ * fb_mlp_groups: number of 3c groups (network slots).
+ * fb_mlp_samples: n training samples of one group as fb_mlp_train draws them
+ *   (X [n][3] normalised inputs, T [n][64] theta cells per output slot).
+ * fb_mlp_gradients / fb_mlp_forward run the training kernels (FP32 forward and
+ *   backward of one batch, mean squared error over the valid output slots) and
+ *   the BF16 MFMA inference on caller-given parameters (layout: W1[64][3],
+ *   b1[64], W2[64][64], b2[64], W3[64][64], b3[64]; X [n][3], T and Y [n][64]). */
+int fb_mlp_train(fb_ctx *ctx, int64_t nsamples, int nsteps, int batch, double lr, uint64_t seed);
+int fb_mlp_groups(fb_ctx *ctx, int *ngroups);
+int fb_mlp_samples(fb_ctx *ctx, int group, int64_t n, uint64_t seed, float *X, float *T);
+int fb_mlp_set_tolerance(fb_ctx *ctx, double tau_ev);
+int fb_mlp_info(fb_ctx *ctx, int group, double *out, int nout);
+int fb_mlp_get_params(fb_ctx *ctx, int group, float *params, float *oscale);
+int fb_mlp_stats(fb_ctx *ctx, int64_t *n_network, int64_t *n_seen);
+int fb_mlp_gradients(fb_ctx *ctx, int nme, const float *params, int n, const float *X, const float *T,
+                     float *grads, double *loss);
+int fb_mlp_forward(fb_ctx *ctx, int nme, const float *params, const float *oscale, int n, const float *X,
+                   float *Y);
+
 /* --- tuning ------------------------------------------------------------
  * No counterpart in the reference (its assembler has no launch parameters).
  * FB_TUNE_CHUNK3C: triplets per workgroup chunk of the mixed 3c kernel (64,

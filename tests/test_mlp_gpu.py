@@ -1,0 +1,130 @@
+"""The learned 3c surrogate on the GPU, checked against the CPU restatement
+(oracle/mlp_oracle.py) and the table oracle (oracle/fb_oracle.c):
+
+* training samples equal the table interpolation they are drawn from;
+* the FP32 forward/backward kernels give the oracle's loss and gradient;
+* the BF16 MFMA inference equals the oracle's emulation of its arithmetic
+  (this also pins the MFMA operand layout);
+* after training, MLP assembly stays within 1e-4 eV of the binary64 table
+  path, serves triplets the untrained network cannot, and with tolerance 0
+  reproduces the table kernel exactly."""
+import numpy as np
+import pytest
+
+import golden_cases
+import mlp_oracle as mo
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _asm(ts, precision="mixed", method="table"):
+    from fireball_hip.lib import Assembler
+    return Assembler(ts, precision=precision, method3c=method)
+
+
+def _si_tables():
+    from fireball_hip import synth
+    return synth.make_tables([synth.SI])
+
+
+def test_samples_match_table_oracle():
+    ts = _si_tables()
+    asm = _asm(ts)
+    X, T = asm.mlp_samples(0, 256, seed=11)
+    o = oracle.Tables(ts)
+    c = ts.cells3c[(1, 1, 1, 3, 0, 1)]
+    nme = c.nME
+    worst = 0.0
+    for i in range(64):
+        x = (float(X[i, 0]) + 1.0) * c.xmax / 2.0
+        y = (float(X[i, 1]) + 1.0) * c.ymax / 2.0
+        cost = float(X[i, 2])
+        hm = o.getMEs_3c(1, 1, 1, 3, 0, x, y, 4, 4, cost)
+        P = [1.0, cost, 0.5 * (3 * cost ** 2 - 1), 0.5 * cost * (5 * cost ** 2 - 3),
+             0.125 * (3 - cost ** 2 * (30 - 35 * cost ** 2))]
+        sint = np.sqrt(max(1e-5, 1 - cost ** 2))
+        for m in range(nme):
+            folded = 0.0
+            for o_ in range(64):
+                ok, th, mm = mo.slot(o_, nme)
+                if ok and mm == m:
+                    folded += P[th] * float(T[i, o_])
+            ref = hm[c.mu[m] - 1, c.nu[m] - 1] / (sint if c.mvalue[m] == 1 else 1.0)
+            worst = max(worst, abs(folded - ref))
+    # inputs are stored in FP32, so (x, y) are recovered to ~1e-7 relative
+    assert worst < 2e-5, worst
+
+
+def test_training_kernels_match_oracle_gradient():
+    rng = np.random.default_rng(5)
+    p = rng.normal(0, 0.25, mo.NP).astype(np.float32)
+    X = rng.uniform(-1, 1, (300, 3)).astype(np.float32)
+    T = rng.normal(0, 1, (300, 64)).astype(np.float32)
+    asm = _asm(_si_tables())
+    for nme in (10, 3):
+        loss, g = asm.mlp_gradients(nme, p, X, T)
+        lref, gref = mo.loss_and_grads(p.astype(np.float64), X, T, nme)
+        assert abs(loss - lref) <= 1e-5 * lref
+        for s, e in ((mo.P_W1, mo.P_B1), (mo.P_B1, mo.P_W2), (mo.P_W2, mo.P_B2), (mo.P_B2, mo.P_W3),
+                     (mo.P_W3, mo.P_B3), (mo.P_B3, mo.NP)):
+            scale = max(np.max(np.abs(gref[s:e])), 1e-12)
+            assert np.max(np.abs(g[s:e] - gref[s:e])) <= 2e-5 * scale, (nme, s)
+
+
+def test_mfma_inference_matches_bf16_emulation():
+    rng = np.random.default_rng(9)
+    p = rng.normal(0, 0.3, mo.NP).astype(np.float32)
+    X = rng.uniform(-1, 1, (200, 3)).astype(np.float32)
+    osc = rng.uniform(0.5, 2.0, 64).astype(np.float32)
+    asm = _asm(_si_tables())
+    Y = asm.mlp_forward(10, p, X, osc)
+    ref = mo.infer_bf16(p, X, osc)
+    d = np.abs(Y - ref)
+    # identical BF16 quantisation; the residue is FP32 summation order and the
+    # rare activation that rounds to the neighbouring BF16 value
+    assert np.median(d) < 1e-5 * np.max(np.abs(ref)), np.median(d)
+    assert np.max(d) < 2e-2 * np.max(np.abs(ref)), np.max(d)
+
+
+def test_trained_surrogate_within_tolerance_and_gains_over_untrained(golden):
+    g = golden("si64")
+    ts, st = golden_cases.inputs(g)
+
+    def run(nsteps, tau):
+        asm = _asm(ts, precision="mixed", method="mlp")
+        asm.add_structure(st)
+        asm.build_lists()
+        asm.train_mlp(nsamples=1 << 20, nsteps=nsteps, batch=8192, lr=3e-3, seed=3)
+        asm.set_mlp_tolerance(tau)
+        asm.assemble_all()
+        nn, seen = asm.mlp_stats()
+        return asm, asm.blocks("VNA"), nn / max(seen, 1)
+
+    ref = _asm(ts, precision="fp64")
+    ref.add_structure(st)
+    ref.build_lists()
+    ref.assemble_all()
+    vna64 = ref.blocks("VNA")
+    tab = _asm(ts, precision="mixed")
+    tab.add_structure(st)
+    tab.build_lists()
+    tab.assemble_all()
+    vna_tab = tab.blocks("VNA")
+
+    asm, vna, frac = run(4000, 1e-4)
+    info = asm.mlp_info(0)
+    err = float(np.max(np.abs(vna - vna64)))
+    print(f"trained: loss {info['loss_before']:.3e} -> {info['loss_after']:.3e}, "
+          f"max cell err {info['max_cell_err_eV']:.3e} eV, network fraction {frac:.3f}, "
+          f"max |VNA - fp64| {err:.3e} eV", flush=True)
+    _, vna_u, frac_u = run(1, 1e-4)
+    err_u = float(np.max(np.abs(vna_u - vna64)))
+    print(f"untrained: network fraction {frac_u:.3f}, max |VNA - fp64| {err_u:.3e} eV", flush=True)
+    assert info["trained"] == 1 and info["loss_after"] < 0.1 * info["loss_before"]
+    assert err <= 1e-4 and err_u <= 1e-4
+    assert frac > frac_u
+    # tolerance 0: every triplet falls back, identical to the table kernel
+    _, vna0, frac0 = run(1, 0.0)
+    assert frac0 == 0.0
+    assert np.array_equal(vna0, vna_tab)

@@ -23,6 +23,7 @@
 // Binary64 adds run at the FP32 rate on gfx950 without packed math, so the
 // accumulation is plain binary64 rather than a compensated FP32 sum.
 #include "fb_device.h"
+#include "fb_mlp.h"
 #include "fb_sp.h"
 
 #define NB (FB_MO * FB_MO)
@@ -103,12 +104,17 @@ __device__ __forceinline__ float stencil(__amdgpu_buffer_rsrc_t r, unsigned base
   return acc;
 }
 
-template <int CH, bool CHARGED>
+// MLP: the learned surrogate replaces the stencil of phase B for every triplet
+// whose calibrated bound passes (neutral builds only); the others keep it.
+template <int CH, bool CHARGED, bool MLP = false>
 __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   __shared__ TPrm sq[CH];
   __shared__ float sF[CH][NME3 + 1];
   __shared__ TChg sx[CHARGED ? CH : 1];
   __shared__ int64_t sSeg[CH + 1];
+  __shared__ float sU[MLP ? CH : 1][4];
+  __shared__ int sG[MLP ? CH : 1];
+  __shared__ unsigned sCnt[2];
   float(*sC)[NB] = reinterpret_cast<float(*)[NB]>(&sq[0]);   // contributions reuse the parameter storage
   static_assert(sizeof(TPrm) >= NB * sizeof(float), "contribution row must fit in a parameter row");
   const int tid = threadIdx.x;
@@ -133,6 +139,7 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   for (int j = 0; j < NB; j++) acc[j] = 0.0;
   int status = 0;
   const __amdgpu_buffer_rsrc_t tab = slab_rsrc(v.tab3c32, v.ntab3c32);
+  if (MLP && tid < 2) sCnt[tid] = 0u;
 
   for (int64_t c0 = r0; c0 < r1; c0 += CH) {
     const int nt = (int)min((int64_t)CH, r1 - c0);
@@ -237,6 +244,24 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       cubic_w(py, q.wy);
       q.rowstride = (unsigned)(g.nx * g.nME * TH8);
       q.base = (unsigned)(g.off32 + ((int64_t)(iy - 2) * g.nx + (ix - 2)) * g.nME * TH8);
+      if (MLP) {
+        // per-triplet bound: P_eq2 * sqrt(nME) (rotation, Frobenius) * sum |P_theta|
+        // * calibrated cell error, with a factor 2 of margin over the sampled maximum
+        const DMlp *w = v.mlp + gid;
+        bool ok = w->ready && x <= g.xmax && z <= g.ymax;
+        if (ok) {
+          const float ce = v.mlp_err[w->err_off + (int64_t)(iy - 1) * g.nx + (ix - 1)];
+          const float sp = fabsf(q.P[0]) + fabsf(q.P[1]) + fabsf(q.P[2]) + fabsf(q.P[3]) + fabsf(q.P[4]);
+          ok = 2.0f * (float)FB_P_EQ2 * sqrtf((float)g.nME) * sp * ce <= v.mlp_tau;
+        }
+        sU[tid][0] = (float)(2.0 * x / g.xmax - 1.0);
+        sU[tid][1] = (float)(2.0 * z / g.ymax - 1.0);
+        sU[tid][2] = (float)cost;
+        sU[tid][3] = ok ? mlp_envelope(w, (float)x, (float)z) : 0.f;
+        sG[tid] = ok ? gid : -1;
+        atomicAdd(&sCnt[0], ok ? 1u : 0u);
+        atomicAdd(&sCnt[1], 1u);
+      }
       if (CHARGED) {
         TChg &c = sx[tid];
         c.dqs = 0.f;
@@ -275,6 +300,7 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       const int t = it / NME3, m = it - t * NME3;
       const TPrm &q = sq[t];
       if (m >= q.nme) continue;
+      if (MLP && sG[t] >= 0) continue;
       float wx[4], wy[4], w1[3], P[5];
 #pragma unroll
       for (int k = 0; k < 4; k++) { wx[k] = q.wx[k]; wy[k] = q.wy[k]; }
@@ -305,6 +331,42 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
         }
       }
       sF[t][m] = F;
+    }
+    if (MLP) {
+      // network evaluation, 16 triplets per wavefront pass; lane group l/16
+      // folds the matrix elements m = l/16 + 4k of its triplet
+      const int wv = tid >> 6, l = tid & 63, qg = l >> 4;
+      for (int blk = wv; blk * 16 < nt; blk += CH / 64) {
+        const int t = blk * 16 + (l & 15);
+        const int gsel = (t < nt) ? sG[t] : -1;
+        unsigned long long pend = __ballot(gsel >= 0);
+        while (pend) {
+          const int lead = __ffsll((long long)pend) - 1;
+          const int gcur = __shfl(gsel, lead);
+          const bool mine = (gsel == gcur);
+          const DMlp *w = v.mlp + gcur;
+          float out[16];
+          mlp16(w, mine ? sU[t][0] : 0.f, mine ? sU[t][1] : 0.f, mine ? sU[t][2] : 0.f, out);
+          if (mine) {
+            const TPrm &q = sq[t];
+            const float env = sU[t][3];
+#pragma unroll
+            for (int ml = 0; ml < 3; ml++) {
+              const int m = qg + 4 * ml;
+              if (m < q.nme) {
+                float F = 0.f;
+#pragma unroll
+                for (int th = 0; th < 5; th++) {
+                  const int r = ml * 5 + th, o = 16 * (r >> 2) + 4 * qg + (r & 3);
+                  F = __builtin_fmaf(q.P[th], out[r] * w->oscale[o] * env, F);
+                }
+                sF[t][m] = F;
+              }
+            }
+          }
+          pend &= ~__ballot(mine);
+        }
+      }
     }
     __syncthreads();
     // ---------------- phase C: rotation, one lane per triplet
@@ -381,6 +443,13 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       if (e < nb) v.mat[FB_MAT_VNA][v.boff[p] + e] = acc[j];
     }
   }
+  if (MLP) {
+    __syncthreads();
+    if (tid == 0) {
+      atomicAdd(&v.mlp_count[0], (unsigned long long)sCnt[0]);
+      atomicAdd(&v.mlp_count[1], (unsigned long long)sCnt[1]);
+    }
+  }
   if (status) atomicOr(v.status, status);
 }
 
@@ -392,6 +461,18 @@ static int launch_mixed(const DView &v, int chunk, hipStream_t st) {
     hipLaunchKernelGGL((k_3c_mixed<64, CHARGED>), dim3((unsigned)((v.ntrip + 63) / 64)), dim3(64), 0, st, v);
   } else {
     hipLaunchKernelGGL((k_3c_mixed<128, CHARGED>), dim3((unsigned)((v.ntrip + 127) / 128)), dim3(128), 0, st, v);
+  }
+  return (int)hipGetLastError();
+}
+
+extern "C" int fbk_3c_mixed_mlp(const DView &v, int chunk, hipStream_t st) {
+  if (v.ntrip == 0) return 0;
+  if (chunk == 256) {
+    hipLaunchKernelGGL((k_3c_mixed<256, false, true>), dim3((unsigned)((v.ntrip + 255) / 256)), dim3(256), 0, st, v);
+  } else if (chunk == 64) {
+    hipLaunchKernelGGL((k_3c_mixed<64, false, true>), dim3((unsigned)((v.ntrip + 63) / 64)), dim3(64), 0, st, v);
+  } else {
+    hipLaunchKernelGGL((k_3c_mixed<128, false, true>), dim3((unsigned)((v.ntrip + 127) / 128)), dim3(128), 0, st, v);
   }
   return (int)hipGetLastError();
 }

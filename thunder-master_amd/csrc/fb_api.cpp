@@ -8,6 +8,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -17,6 +18,9 @@
 #include <vector>
 
 #include "fb_internal.h"
+#include "fb_mlp.h"
+#include <array>
+#include <random>
 
 // kernel launchers (fb_lists.hip, fb_2c.hip, fb_3c.hip, fb_3c_mixed.hip)
 extern "C" {
@@ -45,6 +49,24 @@ int fbk_2c_sp(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64_sp(const DView &v, hipStream_t st);
 int fbk_gamma_fold(const DView &v, int a0, int natoms, int which, const int *orb_off, int n, double *A,
                    hipStream_t st);
+int fbk_3c_mixed_mlp(const DView &v, int chunk, hipStream_t st);
+int fbk_mlp_samples(const DView &v, int gid, int64_t n, unsigned long long seed, float *X, float *T, hipStream_t st);
+int fbk_mlp_colmaxabs(int64_t n, int ncol, const float *A, float *out, hipStream_t st);
+int fbk_mlp_rowmax(int64_t n, const float *T, float *M, hipStream_t st);
+int fbk_mlp_unenvelope(int64_t n, const float *X, float *T, float c, float kx, float ky, float xh, float yh,
+                       hipStream_t st);
+int fbk_mlp_scale_cols(int64_t n, int ncol, float *A, const float *inv, hipStream_t st);
+int fbk_mlp_gemm(int M, int N, int K, const float *A, int64_t sai, int64_t sak, const float *B, int64_t sbk,
+                 int64_t sbj, float *C, int64_t sci, int64_t scj, int nz, float *partial, hipStream_t st);
+int fbk_mlp_bias_silu(int64_t n, float *Z, float *A, const float *b, hipStream_t st);
+int fbk_mlp_out(int64_t n, int nme, float *Y, const float *b3, const float *T, float *dY, float *rowloss,
+                float inv_count, hipStream_t st);
+int fbk_mlp_silu_back(int64_t n, const float *dA, const float *Z, float *dZ, hipStream_t st);
+int fbk_mlp_adam(int np, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float c1,
+                 float c2, float eps, hipStream_t st);
+int fbk_mlp_infer(const DMlp *w, int64_t n, const float *X, float *Y, hipStream_t st);
+int fbk_mlp_calib(const DView &v, int gid, const DMlp *w, int64_t n, unsigned long long seed, unsigned *err,
+                  hipStream_t st);
 }
 
 namespace {
@@ -206,6 +228,17 @@ struct fb_ctx {
   long lists_gen = 0;
   bool use_graph = true;
 
+  // learned 3c surrogate
+  std::vector<DGroup3c> h_grp3c;
+  std::vector<std::array<int, 4>> h_grpkey;   // (is, js, ks, isub) of each group
+  DevBuf d_mlp, d_mlp_err, d_mlp_count;
+  std::vector<DMlp> h_mlp;
+  std::vector<std::vector<float>> h_mlp_params, h_mlp_err;
+  std::vector<std::array<double, 8>> mlp_stat;
+  bool mlp_ready = false;
+  double mlp_tau = 1.0e-4;
+  long mlp_gen = 0;
+
   bool timing = false;
   struct Ev {
     std::string name;
@@ -268,6 +301,10 @@ struct fb_ctx {
     v.seg_pair = d_seg_pair.as<int>();
     v.trip_seg = d_trip_seg.as<int>();
     for (int i = 0; i < 5; i++) v.mat[i] = d_mat[i].as<double>();
+    v.mlp = d_mlp.as<DMlp>();
+    v.mlp_err = d_mlp_err.as<float>();
+    v.mlp_tau = (float)mlp_tau;
+    v.mlp_count = d_mlp_count.as<unsigned long long>();
     v.status = d_status.as<int>();
     return v;
   }
@@ -357,6 +394,7 @@ struct fb_ctx {
                       &d_neighn, &d_noff, &d_neigh_j, &d_neigh_b, &d_neigh_back, &d_neigh_self, &d_pair_atom,
                       &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
                       &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_tmp,
+                      &d_mlp, &d_mlp_err, &d_mlp_count,
                       &d_status};
     for (DevBuf *b : bufs) b->release();
     for (auto &m : d_mat) m.release();
@@ -587,6 +625,7 @@ int fb_commit_tables(fb_ctx *ctx) {
     if (c.itype == 3) groups[std::make_tuple(c.is, c.js, c.ks, c.isub)].push_back(&c);
   std::vector<int> idx3((size_t)ns * ns * ns * FB_MAXSUB3, -1);
   std::vector<DGroup3c> grps;
+  std::vector<std::array<int, 4>> grpkey;
   std::vector<double> tab3;
   std::vector<float> tab3f;
   for (auto &kv : groups) {
@@ -640,6 +679,7 @@ int fb_commit_tables(fb_ctx *ctx) {
     }
     idx3[(((size_t)(is - 1) * ns + (js - 1)) * ns + (ks - 1)) * FB_MAXSUB3 + isub] = (int)grps.size();
     grps.push_back(g);
+    grpkey.push_back({is, js, ks, isub});
   }
   // distinct 2c grids and their Lagrange denominators (host binary64, same
   // expression as e.FDATA/M_Fdata_2c.f90:379-385)
@@ -706,6 +746,10 @@ int fb_commit_tables(fb_ctx *ctx) {
   RC(upload(ctx, ctx->d_nu, nu.data(), nu.size() * sizeof(int), "upload nu"));
   RC(upload(ctx, ctx->d_mv, mv.data(), mv.size() * sizeof(int), "upload mvalue"));
   FB_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->h_grp3c = grps;
+  ctx->h_grpkey = grpkey;
+  ctx->mlp_ready = false;   // networks belong to the previous tables
+  ctx->mlp_gen++;
   ctx->tables_ok = true;
   return FB_OK;
 }
@@ -1055,6 +1099,10 @@ static int any_charge(fb_ctx *ctx) {
 static int run_3c(fb_ctx *ctx) {
   DView v = ctx->view();
   if (ctx->canon_sp) {
+    // the learned surrogate serves the neutral bcna term of the mixed path;
+    // charged builds keep the table stencil for every subtype
+    if (ctx->precision == FB_PREC_MIXED && ctx->method3c == FB_3C_MLP && ctx->mlp_ready && !any_charge(ctx))
+      return ctx->launch("assemble_3c_mlp", [&] { return fbk_3c_mixed_mlp(v, ctx->chunk3c, ctx->stream); });
     if (ctx->precision == FB_PREC_MIXED)
       return ctx->launch("assemble_3c_mixed", [&] { return fbk_3c_mixed(v, ctx->chunk3c, any_charge(ctx), ctx->stream); });
     return ctx->launch("assemble_3c_fp64", [&] { return fbk_3c_fp64_sp(v, ctx->stream); });
@@ -1138,7 +1186,10 @@ int fb_assemble_all(fb_ctx *ctx) {
   if (ctx->timing || !ctx->use_graph || ctx->stream == nullptr) {
     RC(enqueue_build(ctx, charged));
   } else {
-    const long key = ((ctx->lists_gen * 4 + ctx->precision) * 2 + charged) * 512 + ctx->chunk3c;
+    // the graph holds kernel arguments by value: any change of lists, precision,
+    // charge state, chunk, 3c method or networks needs a new capture
+    const long key = (((((ctx->lists_gen * 4 + ctx->precision) * 2 + charged) * 512 + ctx->chunk3c) * 2 +
+                       ctx->method3c) * 4096) + (ctx->mlp_gen & 4095);
     if (ctx->graph_key != key || !ctx->gexec) {
       if (ctx->gexec) { (void)hipGraphExecDestroy(ctx->gexec); ctx->gexec = nullptr; }
       if (ctx->graph) { (void)hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
@@ -1247,6 +1298,402 @@ int fb_gamma_eigenvalues(fb_ctx *ctx, int is, int which, double *w, int64_t nw, 
   if (rs != rocblas_status_success) return ctx->fail(FB_E_HIP, "rocsolver_dsyevd");
   if (e != hipSuccess) return ctx->fail(FB_E_HIP, "gamma_eigenvalues copy", e);
   if (info != 0) return ctx->fail(FB_E_HIP, "rocsolver_dsyevd did not converge");
+  return FB_OK;
+}
+
+// ------------------------------------------------------- learned 3c surrogate
+namespace {
+
+struct MlpBufs {
+  DevBuf X, T, osc, inv, Z1, A1, Z2, A2, Y, dY, dA, dZ2, dZ1, rowloss, P, G, M, V, part, ones;
+  ~MlpBufs() {
+    DevBuf *b[] = {&X, &T, &osc, &inv, &Z1, &A1, &Z2, &A2, &Y, &dY, &dA, &dZ2, &dZ1, &rowloss, &P, &G, &M, &V, &part,
+                   &ones};
+    for (DevBuf *x : b) x->release();
+  }
+};
+
+#define MLP_RC(call)                                                      \
+  do {                                                                    \
+    int rc_ = (call);                                                     \
+    if (rc_) return ctx->fail(FB_E_HIP, #call, (hipError_t)rc_);          \
+  } while (0)
+
+int mlp_alloc_batch(fb_ctx *ctx, MlpBufs &b, int64_t B) {
+  const size_t h = sizeof(float) * (size_t)B * MLP_H, o = sizeof(float) * (size_t)B * MLP_O;
+  hipError_t e = hipSuccess;
+  DevBuf *hb[] = {&b.Z1, &b.A1, &b.Z2, &b.A2, &b.dA, &b.dZ2, &b.dZ1};
+  for (DevBuf *x : hb)
+    if (e == hipSuccess) e = x->ensure(h);
+  if (e == hipSuccess) e = b.Y.ensure(o);
+  if (e == hipSuccess) e = b.dY.ensure(o);
+  if (e == hipSuccess) e = b.rowloss.ensure(sizeof(float) * (size_t)B);
+  if (e == hipSuccess) e = b.P.ensure(sizeof(float) * MLP_NP);
+  if (e == hipSuccess) e = b.G.ensure(sizeof(float) * MLP_NP);
+  if (e == hipSuccess) e = b.part.ensure(sizeof(float) * 64 * MLP_H * MLP_O);
+  if (e == hipSuccess) e = b.ones.ensure(sizeof(float));
+  if (e != hipSuccess) return ctx->fail(FB_E_HIP, "network buffers", e);
+  const float one = 1.0f;
+  FB_HIP(hipMemcpyAsync(b.ones.p, &one, sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  return FB_OK;
+}
+
+// forward + backward of one batch (X [B][3], T [B][64]); gradients into b.G
+int mlp_fwd_bwd(fb_ctx *ctx, MlpBufs &b, int nme, int64_t B, const float *X, const float *T) {
+  hipStream_t st = ctx->stream;
+  float *P = b.P.as<float>(), *G = b.G.as<float>();
+  float *Z1 = b.Z1.as<float>(), *A1 = b.A1.as<float>(), *Z2 = b.Z2.as<float>(), *A2 = b.A2.as<float>();
+  float *Y = b.Y.as<float>(), *dY = b.dY.as<float>(), *dA = b.dA.as<float>();
+  float *dZ2 = b.dZ2.as<float>(), *dZ1 = b.dZ1.as<float>(), *part = b.part.as<float>();
+  const float *ones = b.ones.as<float>();
+  const int Bi = (int)B;
+  const int nz = (int)std::min<int64_t>(64, std::max<int64_t>(1, B / 256));
+  int nvalid = 0;
+  for (int o = 0; o < MLP_O; o++) {
+    int th, m;
+    nvalid += mlp_slot(o, nme, &th, &m) ? 1 : 0;
+  }
+  // forward
+  MLP_RC(fbk_mlp_gemm(Bi, MLP_H, MLP_IN, X, MLP_IN, 1, P + MLP_P_W1, 1, MLP_IN, Z1, MLP_H, 1, 1, nullptr, st));
+  MLP_RC(fbk_mlp_bias_silu(B, Z1, A1, P + MLP_P_B1, st));
+  MLP_RC(fbk_mlp_gemm(Bi, MLP_H, MLP_H, A1, MLP_H, 1, P + MLP_P_W2, 1, MLP_H, Z2, MLP_H, 1, 1, nullptr, st));
+  MLP_RC(fbk_mlp_bias_silu(B, Z2, A2, P + MLP_P_B2, st));
+  MLP_RC(fbk_mlp_gemm(Bi, MLP_O, MLP_H, A2, MLP_H, 1, P + MLP_P_W3, 1, MLP_H, Y, MLP_O, 1, 1, nullptr, st));
+  MLP_RC(fbk_mlp_out(B, nme, Y, P + MLP_P_B3, T, dY, b.rowloss.as<float>(), 1.0f / ((float)B * (float)nvalid), st));
+  // backward
+  MLP_RC(fbk_mlp_gemm(MLP_O, MLP_H, Bi, dY, 1, MLP_O, A2, MLP_H, 1, G + MLP_P_W3, MLP_H, 1, nz, part, st));
+  MLP_RC(fbk_mlp_gemm(1, MLP_O, Bi, ones, 0, 0, dY, MLP_O, 1, G + MLP_P_B3, MLP_O, 1, nz, part, st));
+  MLP_RC(fbk_mlp_gemm(Bi, MLP_H, MLP_O, dY, MLP_O, 1, P + MLP_P_W3, MLP_H, 1, dA, MLP_H, 1, 1, nullptr, st));
+  MLP_RC(fbk_mlp_silu_back(B, dA, Z2, dZ2, st));
+  MLP_RC(fbk_mlp_gemm(MLP_H, MLP_H, Bi, dZ2, 1, MLP_H, A1, MLP_H, 1, G + MLP_P_W2, MLP_H, 1, nz, part, st));
+  MLP_RC(fbk_mlp_gemm(1, MLP_H, Bi, ones, 0, 0, dZ2, MLP_H, 1, G + MLP_P_B2, MLP_H, 1, nz, part, st));
+  MLP_RC(fbk_mlp_gemm(Bi, MLP_H, MLP_H, dZ2, MLP_H, 1, P + MLP_P_W2, MLP_H, 1, dA, MLP_H, 1, 1, nullptr, st));
+  MLP_RC(fbk_mlp_silu_back(B, dA, Z1, dZ1, st));
+  MLP_RC(fbk_mlp_gemm(MLP_H, MLP_IN, Bi, dZ1, 1, MLP_H, X, MLP_IN, 1, G + MLP_P_W1, MLP_IN, 1, nz, part, st));
+  MLP_RC(fbk_mlp_gemm(1, MLP_H, Bi, ones, 0, 0, dZ1, MLP_H, 1, G + MLP_P_B1, MLP_H, 1, nz, part, st));
+  return FB_OK;
+}
+
+double mlp_batch_loss(fb_ctx *ctx, MlpBufs &b, int nme, int64_t B) {
+  std::vector<float> rl((size_t)B);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1.0;
+  if (hipMemcpy(rl.data(), b.rowloss.p, sizeof(float) * B, hipMemcpyDeviceToHost) != hipSuccess) return -1.0;
+  int nvalid = 0;
+  for (int o = 0; o < MLP_O; o++) {
+    int th, m;
+    nvalid += mlp_slot(o, nme, &th, &m) ? 1 : 0;
+  }
+  double s = 0.0;
+  for (float x : rl) s += x;
+  return s / ((double)B * nvalid);
+}
+
+void mlp_init_params(std::vector<float> &p, unsigned long long seed) {
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> u(-1.0, 1.0);
+  p.assign(MLP_NP, 0.0f);
+  const double l1 = std::sqrt(6.0 / (MLP_IN + MLP_H)), l2 = std::sqrt(6.0 / (MLP_H + MLP_H)),
+               l3 = std::sqrt(6.0 / (MLP_H + MLP_O));
+  for (int i = 0; i < MLP_H * MLP_IN; i++) p[MLP_P_W1 + i] = (float)(l1 * u(rng));
+  for (int i = 0; i < MLP_H * MLP_H; i++) p[MLP_P_W2 + i] = (float)(l2 * u(rng));
+  for (int i = 0; i < MLP_O * MLP_H; i++) p[MLP_P_W3 + i] = (float)(l3 * u(rng));
+}
+
+// device image of a network: FP32 first layer and biases, BF16 operand tiles
+void mlp_image(const std::vector<float> &p, const float *oscale, const DGroup3c &g, DMlp &w) {
+  std::memset(&w, 0, sizeof(w));
+  std::memcpy(w.w1, &p[MLP_P_W1], sizeof(w.w1));
+  std::memcpy(w.b1, &p[MLP_P_B1], sizeof(w.b1));
+  std::memcpy(w.b2, &p[MLP_P_B2], sizeof(w.b2));
+  std::memcpy(w.b3, &p[MLP_P_B3], sizeof(w.b3));
+  for (int o = 0; o < MLP_O; o++) w.oscale[o] = oscale ? oscale[o] : 1.0f;
+  w.xmax = (float)g.xmax;
+  w.ymax = (float)g.ymax;
+  w.xs = (float)(2.0 / g.xmax);
+  w.ys = (float)(2.0 / g.ymax);
+  w.nme = g.nME;
+  w.nx = g.nx;
+  w.ny = g.ny;
+  w.ready = 1;
+  for (int mt = 0; mt < 4; mt++)
+    for (int kb = 0; kb < 4; kb++)
+      for (int l = 0; l < 64; l++)
+        for (int j = 0; j < 4; j++) {
+          const int row = mt * 16 + (l & 15), col = kb * 16 + 4 * (l >> 4) + j;
+          w.w2op[mt][kb][l][j] = mlp_bf16(p[MLP_P_W2 + row * MLP_H + col]);
+          w.w3op[mt][kb][l][j] = mlp_bf16(p[MLP_P_W3 + row * MLP_H + col]);
+        }
+}
+
+}  // namespace
+
+int fb_mlp_train(fb_ctx *ctx, int64_t nsamples, int nsteps, int batch, double lr, uint64_t seed) {
+  if (!ctx || !ctx->tables_ok) return ctx ? ctx->fail(FB_E_ORDER, "mlp_train before commit_tables") : FB_E_ARG;
+  if (nsamples < batch || batch < 256 || nsteps < 1 || !(lr > 0.0))
+    return ctx->fail(FB_E_ARG, "mlp_train: need nsamples >= batch >= 256, nsteps >= 1, lr > 0");
+  if (!ctx->canon_sp) return ctx->fail(FB_E_UNSUPPORTED, "mlp_train: s/p species with canonical tables only");
+  FB_HIP(hipSetDevice(ctx->device));
+  RC(ctx->settle("assemble_all"));
+  const int ng = (int)ctx->h_grp3c.size();
+  ctx->h_mlp.assign(ng, DMlp{});
+  ctx->h_mlp_params.assign(ng, {});
+  ctx->h_mlp_err.assign(ng, {});
+  ctx->mlp_stat.assign(ng, std::array<double, 8>{});
+  DView v = ctx->view();
+  for (int gid = 0; gid < ng; gid++) {
+    if (ctx->h_grpkey[gid][3] != 0) continue;   // the neutral term only
+    const DGroup3c &g = ctx->h_grp3c[gid];
+    auto t0 = std::chrono::steady_clock::now();
+    MlpBufs b;
+    hipError_t e = b.X.ensure(sizeof(float) * (size_t)nsamples * MLP_IN);
+    if (e == hipSuccess) e = b.T.ensure(sizeof(float) * (size_t)nsamples * MLP_O);
+    if (e == hipSuccess) e = b.osc.ensure(sizeof(float) * MLP_O);
+    if (e == hipSuccess) e = b.inv.ensure(sizeof(float) * MLP_O);
+    if (e == hipSuccess) e = b.M.ensure(sizeof(float) * MLP_NP);
+    if (e == hipSuccess) e = b.V.ensure(sizeof(float) * MLP_NP);
+    if (e != hipSuccess) return ctx->fail(FB_E_HIP, "network sample buffers", e);
+    RC(mlp_alloc_batch(ctx, b, batch));
+    MLP_RC(fbk_mlp_samples(v, gid, nsamples, seed + 0x1000ull * gid, b.X.as<float>(), b.T.as<float>(), ctx->stream));
+    // envelope exp(c - kx x - ky y): least squares on log max_o |t_o| over a
+    // subset of the samples (decay rates clamped at zero)
+    double env[3] = {0.0, 0.0, 0.0};
+    {
+      const int64_t nf = std::min<int64_t>(nsamples, 1 << 18);
+      DevBuf dm;
+      FB_HIP(dm.ensure(sizeof(float) * (size_t)nf));
+      std::vector<float> hx((size_t)nf * MLP_IN), hm((size_t)nf);
+      int rc = fbk_mlp_rowmax(nf, b.T.as<float>(), dm.as<float>(), ctx->stream);
+      if (rc == 0) rc = (int)hipStreamSynchronize(ctx->stream);
+      if (rc == 0) rc = (int)hipMemcpy(hm.data(), dm.p, sizeof(float) * nf, hipMemcpyDeviceToHost);
+      if (rc == 0) rc = (int)hipMemcpy(hx.data(), b.X.p, sizeof(float) * nf * MLP_IN, hipMemcpyDeviceToHost);
+      dm.release();
+      if (rc) return ctx->fail(FB_E_HIP, "envelope fit data", (hipError_t)rc);
+      double A[3][3] = {{0}}, r[3] = {0};
+      for (int64_t i = 0; i < nf; i++) {
+        if (!(hm[i] > 1e-30f)) continue;
+        const double f[3] = {1.0, -(hx[i * 3] + 1.0) * 0.5 * g.xmax, -(hx[i * 3 + 1] + 1.0) * 0.5 * g.ymax};
+        const double lv = std::log((double)hm[i]);
+        for (int a = 0; a < 3; a++) {
+          r[a] += f[a] * lv;
+          for (int c = 0; c < 3; c++) A[a][c] += f[a] * f[c];
+        }
+      }
+      // 3x3 solve (Cramer)
+      auto det3 = [](double M[3][3]) {
+        return M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+               M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+      };
+      const double D = det3(A);
+      if (std::fabs(D) > 1e-300) {
+        for (int k = 0; k < 3; k++) {
+          double Mk[3][3];
+          for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++) Mk[a][c] = (c == k) ? r[a] : A[a][c];
+          env[k] = det3(Mk) / D;
+        }
+      }
+      env[1] = std::max(env[1], 0.0);
+      env[2] = std::max(env[2], 0.0);
+    }
+    MLP_RC(fbk_mlp_unenvelope(nsamples, b.X.as<float>(), b.T.as<float>(), (float)env[0], (float)env[1],
+                              (float)env[2], (float)(0.5 * g.xmax), (float)(0.5 * g.ymax), ctx->stream));
+    MLP_RC(fbk_mlp_colmaxabs(nsamples, MLP_O, b.T.as<float>(), b.osc.as<float>(), ctx->stream));
+    std::vector<float> osc(MLP_O), inv(MLP_O);
+    FB_HIP(hipStreamSynchronize(ctx->stream));
+    FB_HIP(hipMemcpy(osc.data(), b.osc.p, sizeof(float) * MLP_O, hipMemcpyDeviceToHost));
+    for (int o = 0; o < MLP_O; o++) {
+      if (!(osc[o] > 1e-30f)) osc[o] = 1.0f;
+      inv[o] = 1.0f / osc[o];
+    }
+    FB_HIP(hipMemcpyAsync(b.inv.p, inv.data(), sizeof(float) * MLP_O, hipMemcpyHostToDevice, ctx->stream));
+    MLP_RC(fbk_mlp_scale_cols(nsamples, MLP_O, b.T.as<float>(), b.inv.as<float>(), ctx->stream));
+    std::vector<float> p;
+    mlp_init_params(p, seed * 31ull + (unsigned long long)gid);
+    FB_HIP(hipMemcpyAsync(b.P.p, p.data(), sizeof(float) * MLP_NP, hipMemcpyHostToDevice, ctx->stream));
+    FB_HIP(hipMemsetAsync(b.M.p, 0, sizeof(float) * MLP_NP, ctx->stream));
+    FB_HIP(hipMemsetAsync(b.V.p, 0, sizeof(float) * MLP_NP, ctx->stream));
+    const int64_t nb = nsamples / batch;
+    const double b1 = 0.9, b2 = 0.999;
+    double loss0 = -1.0, loss = -1.0;
+    for (int step = 0; step < nsteps; step++) {
+      const int64_t off = (step % nb) * batch;
+      RC(mlp_fwd_bwd(ctx, b, g.nME, batch, b.X.as<float>() + off * MLP_IN, b.T.as<float>() + off * MLP_O));
+      if (step == 0) loss0 = mlp_batch_loss(ctx, b, g.nME, batch);
+      const double lr_t = lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(M_PI * step / nsteps)));
+      const double c1 = 1.0 / (1.0 - std::pow(b1, step + 1)), c2 = 1.0 / (1.0 - std::pow(b2, step + 1));
+      MLP_RC(fbk_mlp_adam(MLP_NP, b.P.as<float>(), b.G.as<float>(), b.M.as<float>(), b.V.as<float>(), (float)lr_t,
+                          (float)b1, (float)b2, (float)c1, (float)c2, 1e-8f, ctx->stream));
+    }
+    // loss of the trained network on the first batch
+    RC(mlp_fwd_bwd(ctx, b, g.nME, batch, b.X.as<float>(), b.T.as<float>()));
+    loss = mlp_batch_loss(ctx, b, g.nME, batch);
+    FB_HIP(hipMemcpy(p.data(), b.P.p, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost));
+    ctx->h_mlp_params[gid] = p;
+    mlp_image(p, osc.data(), g, ctx->h_mlp[gid]);
+    ctx->h_mlp[gid].ec = (float)env[0];
+    ctx->h_mlp[gid].ekx = (float)env[1];
+    ctx->h_mlp[gid].eky = (float)env[2];
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctx->mlp_stat[gid] = {1.0, (double)nsamples, (double)nsteps, loss0, loss, 0.0, ms, 0.0};
+  }
+  // calibration: per grid cell, the largest |network - table| of any theta cell
+  FB_HIP(hipSetDevice(ctx->device));
+  RC(upload(ctx, ctx->d_mlp, ctx->h_mlp.data(), sizeof(DMlp) * std::max(ng, 1), "upload networks"));
+  std::vector<float> errs;
+  for (int gid = 0; gid < ng; gid++) {
+    if (!ctx->h_mlp[gid].ready) continue;
+    const DGroup3c &g = ctx->h_grp3c[gid];
+    const size_t ncell = (size_t)g.nx * g.ny;
+    DevBuf de;
+    FB_HIP(de.ensure(sizeof(unsigned) * ncell));
+    FB_HIP(hipMemsetAsync(de.p, 0, sizeof(unsigned) * ncell, ctx->stream));
+    const int64_t ncal = std::max<int64_t>(1 << 20, (int64_t)ncell * 1024);
+    int rc = fbk_mlp_calib(v, gid, ctx->d_mlp.as<DMlp>() + gid, ncal, seed ^ 0xca1b7a7eull, de.as<unsigned>(),
+                           ctx->stream);
+    std::vector<float> ce(ncell);
+    if (rc == 0) rc = (int)hipStreamSynchronize(ctx->stream);
+    if (rc == 0) rc = (int)hipMemcpy(ce.data(), de.p, sizeof(float) * ncell, hipMemcpyDeviceToHost);
+    de.release();
+    if (rc) return ctx->fail(FB_E_HIP, "network calibration", (hipError_t)rc);
+    // cells no calibration sample reached (the clamped border rows) never
+    // take the network
+    for (float &x : ce)
+      if (!(x > 0.f)) x = 3.0e38f;
+    float emax = 0.f;
+    for (int iy = 2; iy <= g.ny - 2; iy++)
+      for (int ix = 2; ix <= g.nx - 2; ix++) emax = std::max(emax, ce[(size_t)(iy - 1) * g.nx + (ix - 1)]);
+    ctx->mlp_stat[gid][5] = emax;
+    ctx->mlp_stat[gid][7] = (double)ncal;
+    ctx->h_mlp[gid].err_off = (int64_t)errs.size();
+    ctx->h_mlp_err[gid] = ce;
+    errs.insert(errs.end(), ce.begin(), ce.end());
+  }
+  if (errs.empty()) errs.push_back(0.f);
+  RC(upload(ctx, ctx->d_mlp, ctx->h_mlp.data(), sizeof(DMlp) * std::max(ng, 1), "upload networks"));
+  RC(upload(ctx, ctx->d_mlp_err, errs.data(), sizeof(float) * errs.size(), "upload calibration"));
+  FB_HIP(ctx->d_mlp_count.ensure(2 * sizeof(unsigned long long)));
+  FB_HIP(hipMemsetAsync(ctx->d_mlp_count.p, 0, 2 * sizeof(unsigned long long), ctx->stream));
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->mlp_ready = true;
+  ctx->mlp_gen++;
+  return FB_OK;
+}
+
+int fb_mlp_groups(fb_ctx *ctx, int *ngroups) {
+  if (!ctx || !ngroups) return FB_E_ARG;
+  *ngroups = (int)ctx->h_grp3c.size();
+  return FB_OK;
+}
+
+int fb_mlp_samples(fb_ctx *ctx, int gid, int64_t n, uint64_t seed, float *X, float *T) {
+  if (!ctx || !ctx->tables_ok) return ctx ? ctx->fail(FB_E_ORDER, "mlp_samples before commit_tables") : FB_E_ARG;
+  if (gid < 0 || gid >= (int)ctx->h_grp3c.size() || n < 1 || !X || !T)
+    return ctx->fail(FB_E_ARG, "mlp_samples arguments");
+  FB_HIP(hipSetDevice(ctx->device));
+  RC(ctx->settle("assemble_all"));
+  DevBuf dx, dt;
+  struct Release {
+    DevBuf *b[2];
+    ~Release() {
+      for (DevBuf *x : b) x->release();
+    }
+  } release_all{{&dx, &dt}};
+  FB_HIP(dx.ensure(sizeof(float) * (size_t)n * MLP_IN));
+  FB_HIP(dt.ensure(sizeof(float) * (size_t)n * MLP_O));
+  int rc = fbk_mlp_samples(ctx->view(), gid, n, seed, dx.as<float>(), dt.as<float>(), ctx->stream);
+  if (rc) return ctx->fail(FB_E_HIP, "mlp_samples launch", (hipError_t)rc);
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  FB_HIP(hipMemcpy(X, dx.p, sizeof(float) * (size_t)n * MLP_IN, hipMemcpyDeviceToHost));
+  FB_HIP(hipMemcpy(T, dt.p, sizeof(float) * (size_t)n * MLP_O, hipMemcpyDeviceToHost));
+  return FB_OK;
+}
+
+int fb_mlp_set_tolerance(fb_ctx *ctx, double tau) {
+  if (!ctx || !(tau >= 0.0)) return FB_E_ARG;
+  ctx->mlp_tau = tau;
+  ctx->mlp_gen++;
+  return FB_OK;
+}
+
+int fb_mlp_info(fb_ctx *ctx, int gid, double *out, int nout) {
+  if (!ctx || !out || nout < 1) return FB_E_ARG;
+  if (gid < 0 || gid >= (int)ctx->h_grp3c.size()) return ctx->fail(FB_E_ARG, "mlp_info: group index");
+  double row[12] = {0};
+  for (int k = 0; k < 4; k++) row[k] = ctx->h_grpkey[gid][k];
+  if (gid < (int)ctx->mlp_stat.size())
+    for (int k = 0; k < 8; k++) row[4 + k] = ctx->mlp_stat[gid][k];
+  for (int k = 0; k < nout && k < 12; k++) out[k] = row[k];
+  return FB_OK;
+}
+
+int fb_mlp_get_params(fb_ctx *ctx, int gid, float *params, float *oscale) {
+  if (!ctx || gid < 0 || gid >= (int)ctx->h_mlp_params.size() || ctx->h_mlp_params[gid].empty())
+    return ctx ? ctx->fail(FB_E_ARG, "mlp_get_params: no network for this group") : FB_E_ARG;
+  if (params) std::memcpy(params, ctx->h_mlp_params[gid].data(), sizeof(float) * MLP_NP);
+  if (oscale) std::memcpy(oscale, ctx->h_mlp[gid].oscale, sizeof(float) * MLP_O);
+  return FB_OK;
+}
+
+int fb_mlp_stats(fb_ctx *ctx, int64_t *n_network, int64_t *n_seen) {
+  if (!ctx) return FB_E_ARG;
+  unsigned long long c[2] = {0, 0};
+  if (ctx->d_mlp_count.p) {
+    FB_HIP(hipStreamSynchronize(ctx->stream));
+    FB_HIP(hipMemcpy(c, ctx->d_mlp_count.p, sizeof(c), hipMemcpyDeviceToHost));
+    FB_HIP(hipMemset(ctx->d_mlp_count.p, 0, sizeof(c)));
+  }
+  if (n_network) *n_network = (int64_t)c[0];
+  if (n_seen) *n_seen = (int64_t)c[1];
+  return FB_OK;
+}
+
+int fb_mlp_gradients(fb_ctx *ctx, int nme, const float *params, int n, const float *X, const float *T,
+                     float *grads, double *loss) {
+  if (!ctx || !params || !X || !T || !grads || n < 1 || nme < 1 || nme > 12)
+    return ctx ? ctx->fail(FB_E_ARG, "mlp_gradients arguments") : FB_E_ARG;
+  FB_HIP(hipSetDevice(ctx->device));
+  RC(ctx->settle("assemble_all"));
+  MlpBufs b;
+  RC(mlp_alloc_batch(ctx, b, n));
+  FB_HIP(b.X.ensure(sizeof(float) * (size_t)n * MLP_IN));
+  FB_HIP(b.T.ensure(sizeof(float) * (size_t)n * MLP_O));
+  FB_HIP(hipMemcpyAsync(b.P.p, params, sizeof(float) * MLP_NP, hipMemcpyHostToDevice, ctx->stream));
+  FB_HIP(hipMemcpyAsync(b.X.p, X, sizeof(float) * (size_t)n * MLP_IN, hipMemcpyHostToDevice, ctx->stream));
+  FB_HIP(hipMemcpyAsync(b.T.p, T, sizeof(float) * (size_t)n * MLP_O, hipMemcpyHostToDevice, ctx->stream));
+  RC(mlp_fwd_bwd(ctx, b, nme, n, b.X.as<float>(), b.T.as<float>()));
+  const double l = mlp_batch_loss(ctx, b, nme, n);
+  if (loss) *loss = l;
+  FB_HIP(hipMemcpy(grads, b.G.p, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost));
+  return FB_OK;
+}
+
+int fb_mlp_forward(fb_ctx *ctx, int nme, const float *params, const float *oscale, int n, const float *X, float *Y) {
+  if (!ctx || !params || !X || !Y || n < 1 || nme < 1 || nme > 12)
+    return ctx ? ctx->fail(FB_E_ARG, "mlp_forward arguments") : FB_E_ARG;
+  FB_HIP(hipSetDevice(ctx->device));
+  RC(ctx->settle("assemble_all"));
+  std::vector<float> p(params, params + MLP_NP);
+  DGroup3c g{};
+  g.nME = nme;
+  g.xmax = g.ymax = 1.0;
+  DMlp w;
+  mlp_image(p, oscale, g, w);
+  DevBuf dw, dx, dy;
+  struct Release {
+    DevBuf *b[3];
+    ~Release() {
+      for (DevBuf *x : b) x->release();
+    }
+  } release_all{{&dw, &dx, &dy}};
+  FB_HIP(dw.ensure(sizeof(DMlp)));
+  FB_HIP(dx.ensure(sizeof(float) * (size_t)n * MLP_IN));
+  FB_HIP(dy.ensure(sizeof(float) * (size_t)n * MLP_O));
+  FB_HIP(hipMemcpyAsync(dw.p, &w, sizeof(DMlp), hipMemcpyHostToDevice, ctx->stream));
+  FB_HIP(hipMemcpyAsync(dx.p, X, sizeof(float) * (size_t)n * MLP_IN, hipMemcpyHostToDevice, ctx->stream));
+  int rc = fbk_mlp_infer(dw.as<DMlp>(), n, dx.as<float>(), dy.as<float>(), ctx->stream);
+  if (rc) return ctx->fail(FB_E_HIP, "mlp_forward launch", (hipError_t)rc);
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  FB_HIP(hipMemcpy(Y, dy.p, sizeof(float) * (size_t)n * MLP_O, hipMemcpyDeviceToHost));
   return FB_OK;
 }
 

@@ -109,6 +109,11 @@ struct DView {
   const int64_t *seg_off;       // [nseg+1]
   const int *seg_pair;          // [nseg] target pair of each segment
   const int *trip_seg;          // [ntrip] segment of each sorted triplet
+  // learned 3c surrogate (one DMlp per 3c group; ready = 0 where untrained)
+  const struct DMlp *mlp;
+  const float *mlp_err;         // calibrated per-cell error of each network
+  float mlp_tau;                // per-triplet bound above which the table path is used
+  unsigned long long *mlp_count;  // [2]: triplets on the network, triplets seen
   // outputs
   double *mat[5];
   int *status;

@@ -47,6 +47,7 @@ def lib():
             raise FireballError(-9, f"{LIB_PATH} is missing: build it with thunder-master_amd/build.py")
         L = C.CDLL(LIB_PATH)
         vp, ip, dp, i64p = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int64)
+        fp = C.POINTER(C.c_float)
         sig = {
             "fb_create": [C.POINTER(vp), C.c_int], "fb_destroy": [vp], "fb_set_stream": [vp, vp],
             "fb_synchronize": [vp], "fb_set_precision": [vp, C.c_int], "fb_set_3c_method": [vp, C.c_int],
@@ -70,6 +71,15 @@ def lib():
             "fb_autotune": [vp, C.c_int, ip],
             "fb_get_timings": [vp, C.c_int, C.c_char_p, C.POINTER(C.c_float), ip],
             "fb_gamma_eigenvalues": [vp, C.c_int, C.c_int, dp, C.c_int64, i64p],
+            "fb_mlp_train": [vp, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_uint64],
+            "fb_mlp_groups": [vp, ip],
+            "fb_mlp_samples": [vp, C.c_int, C.c_int64, C.c_uint64, fp, fp],
+            "fb_mlp_set_tolerance": [vp, C.c_double],
+            "fb_mlp_info": [vp, C.c_int, dp, C.c_int],
+            "fb_mlp_get_params": [vp, C.c_int, fp, fp],
+            "fb_mlp_stats": [vp, i64p, i64p],
+            "fb_mlp_gradients": [vp, C.c_int, fp, C.c_int, fp, fp, fp, dp],
+            "fb_mlp_forward": [vp, C.c_int, fp, fp, C.c_int, fp, fp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -88,6 +98,10 @@ def lib():
 
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def _dp(a):
@@ -248,6 +262,65 @@ class Assembler:
         self._check(self.L.fb_gamma_eigenvalues(self.h, istruct, {"S": 0, "H": 1}[which], _dp(w), n.value,
                                                 C.byref(n)))
         return w[:n.value]
+
+    # ---------------------------------------------------- learned 3c surrogate
+    MLP_NP = 8576
+    MLP_INFO = ("is", "js", "ks", "isub", "trained", "nsamples", "nsteps", "loss_before", "loss_after",
+                "max_cell_err_eV", "train_ms", "calib_samples")
+
+    def train_mlp(self, nsamples=1 << 21, nsteps=4000, batch=8192, lr=3e-3, seed=7):
+        """Train one network per neutral 3c group on samples of the table path
+        (GPU forward/backward/Adam kernels), then calibrate its error."""
+        self._check(self.L.fb_mlp_train(self.h, int(nsamples), int(nsteps), int(batch), float(lr), int(seed)))
+
+    def mlp_groups(self):
+        n = C.c_int()
+        self._check(self.L.fb_mlp_groups(self.h, C.byref(n)))
+        return n.value
+
+    def mlp_info(self, group):
+        out = np.zeros(12)
+        self._check(self.L.fb_mlp_info(self.h, int(group), _dp(out), 12))
+        return dict(zip(self.MLP_INFO, out.tolist()))
+
+    def mlp_params(self, group):
+        p = np.zeros(self.MLP_NP, np.float32)
+        osc = np.zeros(64, np.float32)
+        self._check(self.L.fb_mlp_get_params(self.h, int(group), _fp(p), _fp(osc)))
+        return p, osc
+
+    def mlp_samples(self, group, n, seed=1):
+        X = np.zeros((n, 3), np.float32)
+        T = np.zeros((n, 64), np.float32)
+        self._check(self.L.fb_mlp_samples(self.h, int(group), int(n), int(seed), _fp(X), _fp(T)))
+        return X, T
+
+    def set_mlp_tolerance(self, tau_ev):
+        self._check(self.L.fb_mlp_set_tolerance(self.h, float(tau_ev)))
+
+    def mlp_stats(self):
+        a, b = C.c_int64(), C.c_int64()
+        self._check(self.L.fb_mlp_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def mlp_gradients(self, nme, params, X, T):
+        p = np.ascontiguousarray(params, np.float32)
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(self.MLP_NP, np.float32)
+        loss = C.c_double()
+        self._check(self.L.fb_mlp_gradients(self.h, int(nme), _fp(p), X.shape[0], _fp(X), _fp(T), _fp(g),
+                                            C.byref(loss)))
+        return loss.value, g
+
+    def mlp_forward(self, nme, params, X, oscale=None):
+        p = np.ascontiguousarray(params, np.float32)
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], 64), np.float32)
+        osc = None if oscale is None else np.ascontiguousarray(oscale, np.float32)
+        self._check(self.L.fb_mlp_forward(self.h, int(nme), _fp(p), _fp(osc) if osc is not None else None,
+                                          X.shape[0], _fp(X), _fp(Y)))
+        return Y
 
     def set_chunk3c(self, chunk):
         self._check(self.L.fb_set_tuning(self.h, 0, int(chunk)))
