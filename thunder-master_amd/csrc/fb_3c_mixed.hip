@@ -38,7 +38,9 @@ struct TPrm {
   float w1[3];                // (1-px-py, px, py) of the 3-point rule (method 1)
   float P[5], sint;
   float pm[9];
+  int pad_;                   // odd row stride (33 words): lane-per-triplet accesses hit distinct banks
 };
+static_assert(sizeof(TPrm) == 33 * 4, "TPrm row stride must stay odd in words");
 
 struct TChg {                 // charged third atom (DOGS terms)
   float dq[FB_MAXSH];
@@ -115,8 +117,9 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   __shared__ float sU[MLP ? CH : 1][4];
   __shared__ int sG[MLP ? CH : 1];
   __shared__ unsigned sCnt[2];
-  float(*sC)[NB] = reinterpret_cast<float(*)[NB]>(&sq[0]);   // contributions reuse the parameter storage
-  static_assert(sizeof(TPrm) >= NB * sizeof(float), "contribution row must fit in a parameter row");
+  // contributions reuse the parameter storage, rows padded to 17 words
+  float(*sC)[NB + 1] = reinterpret_cast<float(*)[NB + 1]>(&sq[0]);
+  static_assert(sizeof(TPrm) >= (NB + 1) * sizeof(float), "contribution row must fit in a parameter row");
   const int tid = threadIdx.x;
   const int64_t ts = (int64_t)blockIdx.x * CH;
   const int64_t te = min(ts + CH, v.ntrip);
@@ -231,12 +234,10 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       if (ix > g.nx - 2) ix = g.nx - 2;
       if (iy > g.ny - 2) iy = g.ny - 2;
       const double px = qx - ix + 1.0, py = qy - iy + 1.0;
-      const double *t64 = v.tab3c + g.off;
-      const int64_t i00 = ((int64_t)(iy - 1) * g.nx + (ix - 1)) * g.nME;
-      const double f00 = t64[i00];
-      const double gx = (t64[i00 + g.nME] - f00) / dx;
-      const double gy = (t64[i00 + (int64_t)g.nx * g.nME] - f00) / dy;
-      q.m1 = (gx * gx + gy * gy < 1.0e-6) ? 1 : 0;
+      // method of the cell, precomputed from the binary64 table with the
+      // reference's gradient test (M_Fdata_3c.f90:383-385)
+      const int64_t icell = (int64_t)(iy - 1) * g.nx + (ix - 1);
+      q.m1 = v.meth3c[g.moff + icell];
       q.w1[0] = (float)(1.0 - px - py);
       q.w1[1] = (float)px;
       q.w1[2] = (float)py;
@@ -245,14 +246,14 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       q.rowstride = (unsigned)(g.nx * g.nME * TH8);
       q.base = (unsigned)(g.off32 + ((int64_t)(iy - 2) * g.nx + (ix - 2)) * g.nME * TH8);
       if (MLP) {
-        // per-triplet bound: P_eq2 * sqrt(nME) (rotation, Frobenius) * sum |P_theta|
-        // * calibrated cell error, with a factor 2 of margin over the sampled maximum
+        // per-triplet bound: P_eq2 x the calibrated Frobenius error of the
+        // cell's molecular-frame block (bounds every rotated element), with a
+        // factor 2 of margin over the sampled maximum
         const DMlp *w = v.mlp + gid;
         bool ok = w->ready && x <= g.xmax && z <= g.ymax;
         if (ok) {
           const float ce = v.mlp_err[w->err_off + (int64_t)(iy - 1) * g.nx + (ix - 1)];
-          const float sp = fabsf(q.P[0]) + fabsf(q.P[1]) + fabsf(q.P[2]) + fabsf(q.P[3]) + fabsf(q.P[4]);
-          ok = 2.0f * (float)FB_P_EQ2 * sqrtf((float)g.nME) * sp * ce <= v.mlp_tau;
+          ok = 2.0f * (float)FB_P_EQ2 * ce <= v.mlp_tau;
         }
         sU[tid][0] = (float)(2.0 * x / g.xmax - 1.0);
         sU[tid][1] = (float)(2.0 * z / g.ymax - 1.0);
@@ -280,11 +281,7 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
           c.dqs += (float)dq;
           c.gk[isorp - 1] = (int)(gq.off32 - g.off32);
           // getMEs_Fdata_3c chooses the method from each table's own gradient
-          const double *tq = v.tab3c + gq.off;
-          const double h00 = tq[i00];
-          const double hx = (tq[i00 + g.nME] - h00) / dx;
-          const double hy = (tq[i00 + (int64_t)g.nx * g.nME] - h00) / dy;
-          c.m1k[isorp - 1] = (hx * hx + hy * hy < 1.0e-6) ? 1 : 0;
+          c.m1k[isorp - 1] = v.meth3c[gq.moff + icell];
         }
         const double d13 = d_distance(rna, r1v), d23 = d_distance(rna, r2v);
         double rc1 = v.sp.rcmin[in1 - 1], rc2 = v.sp.rcmin[in2 - 1], rc3 = v.sp.rcmin[indna - 1];

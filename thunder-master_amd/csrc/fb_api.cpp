@@ -195,7 +195,7 @@ struct fb_ctx {
   std::vector<HCell2c> c2;
   std::vector<HCell3c> c3;
   bool tables_ok = false;
-  DevBuf d_cell2c, d_idx2c, d_grp3c, d_idx3c, d_tab2c, d_tab2c32, d_tab3c, d_tab3c32, d_mu, d_nu, d_mv;
+  DevBuf d_cell2c, d_idx2c, d_grp3c, d_idx3c, d_tab2c, d_tab2c32, d_tab3c, d_tab3c32, d_mu, d_nu, d_mv, d_meth3c;
   DevBuf d_grid2c, d_pden, d_wg2c;
   int canon_sp = 0;
   int64_t ntab3c32 = 0;
@@ -236,7 +236,7 @@ struct fb_ctx {
   std::vector<std::vector<float>> h_mlp_params, h_mlp_err;
   std::vector<std::array<double, 8>> mlp_stat;
   bool mlp_ready = false;
-  double mlp_tau = 1.0e-4;
+  double mlp_tau = 5.0e-5;
   long mlp_gen = 0;
 
   bool timing = false;
@@ -267,6 +267,7 @@ struct fb_ctx {
     v.tab2c32 = d_tab2c32.as<float>();
     v.tab3c = d_tab3c.as<double>();
     v.tab3c32 = d_tab3c32.as<float>();
+    v.meth3c = d_meth3c.as<unsigned char>();
     v.ntab3c32 = ntab3c32;
     v.mu = d_mu.as<int>();
     v.nu = d_nu.as<int>();
@@ -394,7 +395,7 @@ struct fb_ctx {
                       &d_neighn, &d_noff, &d_neigh_j, &d_neigh_b, &d_neigh_back, &d_neigh_self, &d_pair_atom,
                       &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
                       &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_tmp,
-                      &d_mlp, &d_mlp_err, &d_mlp_count,
+                      &d_mlp, &d_mlp_err, &d_mlp_count, &d_meth3c,
                       &d_status};
     for (DevBuf *b : bufs) b->release();
     for (auto &m : d_mat) m.release();
@@ -626,6 +627,7 @@ int fb_commit_tables(fb_ctx *ctx) {
   std::vector<int> idx3((size_t)ns * ns * ns * FB_MAXSUB3, -1);
   std::vector<DGroup3c> grps;
   std::vector<std::array<int, 4>> grpkey;
+  std::vector<unsigned char> meth;
   std::vector<double> tab3;
   std::vector<float> tab3f;
   for (auto &kv : groups) {
@@ -660,6 +662,20 @@ int fb_commit_tables(fb_ctx *ctx) {
     nu.insert(nu.end(), cl->nu.begin(), cl->nu.end());
     mv.insert(mv.end(), cl->mv.begin(), cl->mv.end());
     for (int t = 0; t < FB_NTHETA; t++) tab3.insert(tab3.end(), th[t]->data.begin(), th[t]->data.end());
+    // method of every interior cell: gradient of ME 1 of theta cell 1 against
+    // 1e-6 (e.FDATA/M_Fdata_3c.f90:383-385), the same binary64 expression the
+    // kernels would evaluate per triplet
+    g.moff = (int64_t)meth.size();
+    meth.resize(meth.size() + (size_t)g.nx * g.ny, 0);
+    for (int iy = 2; iy <= g.ny - 2; iy++)
+      for (int ix = 2; ix <= g.nx - 2; ix++) {
+        const double *t0 = th[0]->data.data();
+        const size_t i00 = ((size_t)(iy - 1) * g.nx + (ix - 1)) * g.nME;
+        const double f00 = t0[i00];
+        const double gx = (t0[i00 + g.nME] - f00) / g.dx;
+        const double gy = (t0[i00 + (size_t)g.nx * g.nME] - f00) / g.dy;
+        meth[g.moff + (size_t)(iy - 1) * g.nx + (ix - 1)] = (gx * gx + gy * gy < 1.0e-6) ? 1 : 0;
+      }
     // FP32 slab [iy][ix][ME][theta], theta padded to 8 so that the five
     // values of one (point, ME) load as one float4 + one float
     size_t plane = (size_t)g.nx * g.ny;
@@ -735,6 +751,7 @@ int fb_commit_tables(fb_ctx *ctx) {
   RC(upload(ctx, ctx->d_tab2c, tab2.data(), tab2.size() * sizeof(double), "upload tab2c"));
   RC(upload(ctx, ctx->d_tab2c32, tab2f.data(), tab2f.size() * sizeof(float), "upload tab2c32"));
   RC(upload(ctx, ctx->d_tab3c, tab3.data(), tab3.size() * sizeof(double), "upload tab3c"));
+  RC(upload(ctx, ctx->d_meth3c, meth.data(), meth.size(), "upload 3c methods"));
   if (tab3f.size() * sizeof(float) >= (size_t)INT32_MAX)
     return ctx->fail(FB_E_CAPACITY, "commit_tables: FP32 3c slab above 2 GiB");
   RC(upload(ctx, ctx->d_tab3c32, tab3f.data(), tab3f.size() * sizeof(float), "upload tab3c32"));

@@ -58,7 +58,8 @@ struct DGroup3c {
   int nME, nx, ny, munu;
   double xmax, ymax, dx, dy;
   int64_t off;       // FP64 slab, layout [theta][ny][nx][nME]
-  int64_t off32;     // FP32 slab, layout [ny][nx][theta][nME]
+  int64_t off32;     // FP32 slab, layout [ny][nx][ME][theta8]
+  int64_t moff;      // method table [ny][nx] (uint8, 1 = 3-point rule), e.FDATA/M_Fdata_3c.f90:383-385
 };
 
 struct DStruct {
@@ -80,6 +81,7 @@ struct DView {
   const float *tab2c32;
   const double *tab3c;
   const float *tab3c32;
+  const unsigned char *meth3c;  // per-cell method choice of every 3c group
   int64_t ntab3c32;             // floats in the FP32 3c slab
   const int *mu, *nu, *mv;
   const DGrid2c *grid2c;
