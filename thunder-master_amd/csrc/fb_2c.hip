@@ -113,7 +113,7 @@ __global__ __launch_bounds__(W2C) void k_2c(DView v, int mask) {
           }
         }
         for (int i = 0; i < nb; i++) v.mat[FB_MAT_VNAO][bo + i] = acc[i];
-        if (p != pself)
+        if (p != pself && !((mask & FB_2C_SKIP_TGT) && v.pair_tgt[p]))
           for (int i = 0; i < nb; i++) v.mat[FB_MAT_VNA][bo + i] = 0.0;
         // atom term <i|V_j|i> (M_assemble_2c_DOGS.f90:763-812)
         double rend = rc1min + v.sp.rcmin[in2 - 1];
@@ -229,7 +229,7 @@ __device__ __forceinline__ int block2c(const DView &v, int cid, double x, const 
 
 template <int CLS>
 __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, int in1, int in2, bool self,
-                                      double z, const double *pm, int64_t bo, double *atom_c) {
+                                      bool keep_vna, double z, const double *pm, int64_t bo, double *atom_c) {
   constexpr int NMU = (CLS & 2) ? 4 : 1, NNU = (CLS & 1) ? 4 : 1, NBL = NMU * NNU;
   constexpr int CLSA = (CLS & 2) ? CLS_SPSP : CLS_SS;   // atom term lives on iatom's orbitals
   int status = 0, grid_cached = -1, ileft = 1;
@@ -277,7 +277,7 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
     }
 #pragma unroll
     for (int i = 0; i < NBL; i++) v.mat[FB_MAT_VNAO][bo + i] = acc[i];
-    if (!self) {
+    if (!self && !keep_vna) {
 #pragma unroll
       for (int i = 0; i < NBL; i++) v.mat[FB_MAT_VNA][bo + i] = 0.0;
     }
@@ -306,7 +306,7 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
   return status;
 }
 
-__global__ __launch_bounds__(WG2) void k_2c_sp(DView v, int mask) {
+__global__ __launch_bounds__(WG2, 3) void k_2c_sp(DView v, int mask) {
   __shared__ double s_at[WG2][17];
   const int tid = threadIdx.x;
   const int a0 = v.wg2c[blockIdx.x], a1 = v.wg2c[blockIdx.x + 1];
@@ -353,11 +353,12 @@ __global__ __launch_bounds__(WG2) void k_2c_sp(DView v, int mask) {
       const bool self = (ja == ia && mb == 0);
       const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
       const int64_t bo = v.boff[p];
+      const bool keep = (mask & FB_2C_SKIP_TGT) && v.pair_tgt[p];
       switch (cls) {
-        case 3: status |= pair2c<3>(v, mask, ia, ja, in1, in2, self, z, pm, bo, atom_c); break;
-        case 2: status |= pair2c<2>(v, mask, ia, ja, in1, in2, self, z, pm, bo, atom_c); break;
-        case 1: status |= pair2c<1>(v, mask, ia, ja, in1, in2, self, z, pm, bo, atom_c); break;
-        default: status |= pair2c<0>(v, mask, ia, ja, in1, in2, self, z, pm, bo, atom_c); break;
+        case 3: status |= pair2c<3>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c); break;
+        case 2: status |= pair2c<2>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c); break;
+        case 1: status |= pair2c<1>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c); break;
+        default: status |= pair2c<0>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c); break;
       }
     }
     if (mask & 8) {
@@ -381,5 +382,315 @@ __global__ __launch_bounds__(WG2) void k_2c_sp(DView v, int mask) {
 extern "C" int fbk_2c_sp(const DView &v, int mask, hipStream_t st) {
   if (v.nwg2c == 0) return 0;
   hipLaunchKernelGGL(k_2c_sp, dim3(v.nwg2c), dim3(WG2), 0, st, v, mask);
+  return (int)hipGetLastError();
+}
+
+
+// bond geometry of pair p (M_assemble_2c_DOGS.f90:165-177)
+struct Pair2c {
+  int ia, ja, in1, in2, cls;
+  bool self;
+  double z;
+  double pm[9];
+  int64_t bo;
+};
+
+__device__ __forceinline__ void pair_geometry(const DView &v, int64_t p, Pair2c &g) {
+  const int ia = v.pair_atom[p];
+  const int s = v.atom_struct[ia];
+  const int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
+  const int mb = v.neigh_b[p];
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  double r1[3], r2[3], sighat[3], eps[9];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    r1[q] = v.ratom[3 * ia + q];
+    r2[q] = v.ratom[3 * ja + q] + xls[3 * mb + q];
+  }
+  g.z = d_distance(r1, r2);
+  if (g.z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+  else {
+#pragma unroll
+    for (int q = 0; q < 3; q++) sighat[q] = (r2[q] - r1[q]) / g.z;
+  }
+  d_epsilon(r2, sighat, eps);
+  d_pmat<double>(eps, g.pm);
+  g.ia = ia;
+  g.ja = ja;
+  g.in1 = v.imass[ia];
+  g.in2 = v.imass[ja];
+  g.self = (ja == ia && mb == 0);
+  g.cls = (v.sp.norb[g.in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[g.in2 - 1] == 4 ? 1 : 0);
+  g.bo = v.boff[p];
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged variant for s/p species (the default): the workgroup walks the
+// interactions in the reference's order (S, T, dipole_z, ontop L with its
+// charge subtypes, ontop R with its subtypes, atom with its subtypes).  For
+// each it stages every distinct table cell its lanes need in LDS with
+// contiguous loads and interpolates from there, so the per-lane gathers hit
+// LDS instead of the vector memory pipeline.  Lagrange reciprocals are formed
+// in-kernel with the reference's division.  Each lane writes its block into
+// an LDS row and the workgroup stores the contiguous block range of its pairs
+// with consecutive lanes on consecutive addresses.  Arithmetic and its order
+// per pair are those of k_2c_sp, so the blocks are the same bits.
+#define CELLCAP 4096   // doubles of one staged cell (nx * nME)
+#define ROWP 17        // doubles per LDS block row (odd: conflict-free lane rows)
+
+// window weights of M_Fdata_2c.f90:377-390: xx_k = (ileft+k-1)*dx,
+// pden = 1/(xx_is - xx_ip), prod = prod*(x - xx_ip)*pden in ip order
+__device__ __forceinline__ void lagrange7_div(double x, double dx, int ileft, double *prod) {
+  double xx[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) xx[k] = (double)(ileft + k - 1) * dx;
+#pragma unroll
+  for (int is = 0; is < 7; is++) {
+    double p = 1.0;
+#pragma unroll
+    for (int ip = 0; ip < 7; ip++)
+      if (ip != is) p = p * (x - xx[ip]) * (1.0 / (xx[is] - xx[ip]));
+    prod[is] = p;
+  }
+}
+
+template <int CLS>
+__device__ __forceinline__ void interp_rot(const double *base, const double *prod, const double *pm, double *X) {
+  constexpr int NME = nme2c_of(CLS);
+  double F[NME];
+#pragma unroll
+  for (int m = 0; m < NME; m++) F[m] = 0.0;
+#pragma unroll
+  for (int is = 0; is < 7; is++)
+#pragma unroll
+    for (int m = 0; m < NME; m++) F[m] = F[m] + base[is * NME + m] * prod[is];
+  rot2c_sp<double>(CLS, pm, F, X);
+}
+
+// One interaction slot: every lane of the workgroup calls it; a lane with
+// cell my >= 0 gets its rotated block in X (zeros otherwise).  The loop stages
+// one distinct cell per pass; the cell id is uniform, so its header loads are
+// scalar.
+__device__ inline int slot2c(const DView &v, int my, int cls, const Pair2c &g, double *s_cell, int *s_sel,
+                             int &grid_c, int &ileft, double *prod, double *X) {
+  int status = 0;
+  bool todo = my >= 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++) X[i] = 0.0;
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) *s_sel = -1;
+    __syncthreads();
+    if (todo) *s_sel = my;   // any one of the competing writes wins
+    __syncthreads();
+    const int c = *s_sel;
+    if (c < 0) break;
+    const DCell2c cc = v.cell2c[c];
+    const int nv = cc.nx * cc.nME;
+    const bool lds = nv <= CELLCAP;
+    if (lds)
+      for (int i = threadIdx.x; i < nv; i += WG2) s_cell[i] = v.tab2c[cc.off + i];
+    __syncthreads();
+    if (todo && my == c) {
+      todo = false;
+      const double x = g.z;
+      if (x < 0.0) {
+        status |= ST_X_NEG;
+      } else if (x > cc.xmax * (1.0 + 1.0e-5)) {
+        status |= ST_X_RANGE;
+      } else {
+        if (cc.grid != grid_c) {
+          const DGrid2c gr = v.grid2c[cc.grid];
+          ileft = window7(x, gr.dx, gr.nx);
+          lagrange7_div(x, gr.dx, ileft, prod);
+          grid_c = cc.grid;
+        }
+        // separate branches keep the LDS pointer in its address space (ds_read
+        // rather than flat loads)
+        if (lds) {
+          const double *base = s_cell + (ileft - 1) * cc.nME;
+          switch (cls) {
+            case 3: interp_rot<3>(base, prod, g.pm, X); break;
+            case 2: interp_rot<2>(base, prod, g.pm, X); break;
+            case 1: interp_rot<1>(base, prod, g.pm, X); break;
+            default: interp_rot<0>(base, prod, g.pm, X); break;
+          }
+        } else {
+          const double *base = v.tab2c + cc.off + (int64_t)(ileft - 1) * cc.nME;
+          switch (cls) {
+            case 3: interp_rot<3>(base, prod, g.pm, X); break;
+            case 2: interp_rot<2>(base, prod, g.pm, X); break;
+            case 1: interp_rot<1>(base, prod, g.pm, X); break;
+            default: interp_rot<0>(base, prod, g.pm, X); break;
+          }
+        }
+      }
+    }
+  }
+  return status;
+}
+
+// contiguous store of the chunk's block range [rb0, rb0 + len) from the LDS rows
+__device__ __forceinline__ void store_rows(double *dst, int len, const double *s_row, const unsigned short *s_map) {
+  for (int j = threadIdx.x; j < len; j += WG2) dst[j] = s_row[s_map[j]];
+}
+
+__global__ __launch_bounds__(WG2) void k_2c_lds(DView v, int mask) {
+  __shared__ double s_cell[CELLCAP];
+  __shared__ double s_row[WG2 * ROWP];          // lane blocks, then atom terms for the reduction
+  __shared__ unsigned short s_map[WG2 * 16];    // block range element -> LDS row element
+  __shared__ int s_sel;
+  const int tid = threadIdx.x;
+  const int a0 = v.wg2c[blockIdx.x], a1 = v.wg2c[blockIdx.x + 1];
+  const int64_t pb = v.noff[a0], pe = v.noff[a1];
+  // reduction slot of this thread: (atom a0 + tid/16, element tid%16)
+  const int ra = a0 + (tid >> 4), re = tid & 15;
+  const bool rown = ra < a1;
+  int64_t rp0 = 0, rp1 = 0;
+  int rna = 0;
+  if (rown) {
+    rp0 = v.noff[ra];
+    rp1 = v.noff[ra + 1];
+    rna = v.sp.norb[v.imass[ra] - 1];
+  }
+  int maxsh = 0;
+  for (int s = 0; s < v.sp.ns; s++) maxsh = max(maxsh, v.sp.nssh[s]);
+  double racc = 0.0;
+  int status = 0;
+  for (int64_t c0 = pb; c0 < pe; c0 += WG2) {
+    const int nt = (int)min((int64_t)WG2, pe - c0);
+    const bool act = tid < nt;
+    Pair2c g{};
+    int nb = 0, ro = 0;
+    const int64_t rb0 = v.boff[c0];
+    const int len = (int)(v.boff[c0 + nt] - rb0);
+    if (act) {
+      pair_geometry(v, c0 + tid, g);
+      nb = ((g.cls & 2) ? 4 : 1) * ((g.cls & 1) ? 4 : 1);
+      ro = (int)(g.bo - rb0);
+      for (int i = 0; i < nb; i++) s_map[ro + i] = (unsigned short)(tid * ROWP + i);
+    }
+    int grid_c = -1, ileft = 1;
+    double prod[7], X[16], acc[16];
+    bool charged = false;
+    double smooth = 0.0;
+    const double *sself = nullptr;
+    if (act && (mask & 8)) {
+      for (int isorp = 1; isorp <= v.sp.nssh[g.in2 - 1]; isorp++)
+        charged |= (v.dQ[g.ja * FB_MAXSH + isorp - 1] != 0.0);
+      if (charged) {
+        const int64_t pself = v.noff[g.ia] + v.neigh_self[g.ia] - 1;
+        sself = v.mat[FB_MAT_S] + v.boff[pself];
+        smooth = d_smoother(g.z, v.sp.rcmin[g.in1 - 1] + v.sp.rcmin[g.in2 - 1], 0.8);
+      }
+    }
+    // the slot sequence, uniform over the workgroup: S, T, dipole_z, then
+    // ontop L and R with their charge subtypes, then the atom term with its
+    // subtypes (per pair, the reference's order of accumulation)
+    const int nsl = (maxsh + 1);
+    const int nslots = 3 + ((mask & 8) ? 3 * nsl : 0);
+    for (int si = 0; si < nslots; si++) {
+      const int kind = si < 3 ? si : 3 + (si - 3) / nsl;       // 0 S 1 T 2 DIP 3 ONL 4 ONR 5 ATOM
+      const int isorp = si < 3 ? 0 : (si - 3) % nsl;
+      if (kind < 3 && !(mask & (1 << kind))) continue;
+      int my = -1, cls = g.cls;
+      double dQ = 1.0;
+      if (act) {
+        if (kind < 3) {
+          my = cell2c_id(v, g.in1, g.in2, kind == 0 ? SLOT_S : (kind == 1 ? SLOT_T : SLOT_DIP), 0);
+          if (my < 0) status |= ST_NO_TABLE;
+        } else if (kind < 5) {
+          if (!g.self) {
+            const int inq = kind == 3 ? g.in1 : g.in2, aq = kind == 3 ? g.ia : g.ja;
+            bool need = isorp == 0;
+            if (isorp > 0 && isorp <= v.sp.nssh[inq - 1]) {
+              dQ = v.dQ[aq * FB_MAXSH + isorp - 1];
+              need = dQ != 0.0;
+            }
+            if (need) {
+              my = cell2c_id(v, g.in1, g.in2, kind == 3 ? SLOT_ONL : SLOT_ONR, isorp);
+              if (my < 0) status |= ST_NO_TABLE;
+            }
+          }
+        } else if (isorp == 0 || (charged && isorp <= v.sp.nssh[g.in2 - 1])) {
+          my = cell2c_id(v, g.in1, g.in2, SLOT_ATOM, isorp);
+          if (my < 0) status |= ST_NO_TABLE;
+          if (isorp > 0) dQ = v.dQ[g.ja * FB_MAXSH + isorp - 1];
+          cls = (g.cls & 2) ? CLS_SPSP : CLS_SS;   // the atom term lives on iatom's orbitals
+        }
+      }
+      if (si == 3) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) acc[i] = 0.0;
+      }
+      status |= slot2c(v, my, cls, g, s_cell, &s_sel, grid_c, ileft, prod, X);
+      if (kind < 3) {
+        if (act)
+          for (int i = 0; i < nb; i++) s_row[tid * ROWP + i] = X[i];
+        __syncthreads();
+        store_rows(v.mat[kind == 0 ? FB_MAT_S : (kind == 1 ? FB_MAT_T : FB_MAT_DIP)] + rb0, len, s_row, s_map);
+      } else if (kind < 5) {
+        if (my >= 0) {
+          if (isorp == 0) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc[i] = acc[i] + X[i] * FB_P_EQ2;
+          } else {
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc[i] = acc[i] + dQ * X[i] * FB_P_EQ2;
+          }
+        }
+        if (si == 3 + 2 * nsl - 1) {
+          // blocko complete (M_assemble_2c_DOGS.f90:640-702); the Vna block is
+          // zero here, its self blocks receive the atom sums below
+          __syncthreads();
+          if (act)
+            for (int i = 0; i < nb; i++) s_row[tid * ROWP + i] = acc[i];
+          __syncthreads();
+          store_rows(v.mat[FB_MAT_VNAO] + rb0, len, s_row, s_map);
+          if (mask & FB_2C_SKIP_TGT) {
+            if (act && !g.self && !v.pair_tgt[c0 + tid])
+              for (int i = 0; i < nb; i++) v.mat[FB_MAT_VNA][g.bo + i] = 0.0;
+          } else {
+            for (int j = tid; j < len; j += WG2) v.mat[FB_MAT_VNA][rb0 + j] = 0.0;
+          }
+#pragma unroll
+          for (int i = 0; i < 16; i++) acc[i] = 0.0;   // reused for the atom term
+        }
+      } else if (my >= 0) {
+        // atom term <i|V_j|i> (M_assemble_2c_DOGS.f90:710-816)
+        const int na = (g.cls & 2) ? 16 : 1;
+        if (isorp == 0) {
+          for (int i = 0; i < na; i++) acc[i] = X[i] * FB_P_EQ2;
+        } else {
+          for (int i = 0; i < na; i++) {
+            double emnpl = (g.z > 1.0e-4) ? sself[i] / g.z : 0.0;
+            acc[i] = acc[i] + FB_P_EQ2 * dQ * ((1.0 - smooth) * emnpl + smooth * X[i]);
+          }
+        }
+      }
+    }
+    if (mask & 8) {
+      // ordered sum over this chunk's neighbours into the self block
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 16; i++) s_row[tid * ROWP + i] = act ? acc[i] : 0.0;
+      __syncthreads();
+      if (rown && re < rna * rna) {
+        const int64_t k0 = max(rp0, c0), k1 = min(rp1, c0 + (int64_t)nt);
+        for (int64_t k = k0; k < k1; k++) racc = racc + s_row[(int)(k - c0) * ROWP + re];
+      }
+    }
+    __syncthreads();
+  }
+  if ((mask & 8) && rown && re < rna * rna) {
+    const int64_t pself = v.noff[ra] + v.neigh_self[ra] - 1;
+    v.mat[FB_MAT_VNA][v.boff[pself] + re] = racc;
+  }
+  if (status) atomicOr(v.status, status);
+}
+
+extern "C" int fbk_2c_lds(const DView &v, int mask, hipStream_t st) {
+  if (v.nwg2c == 0) return 0;
+  hipLaunchKernelGGL(k_2c_lds, dim3(v.nwg2c), dim3(WG2), 0, st, v, mask);
   return (int)hipGetLastError();
 }

@@ -39,13 +39,14 @@ int fbk_scan_incl_i32(const int *in, int *out, int64_t n, void *tmp, size_t *tmp
 int fbk_iota(int *a, int64_t n, hipStream_t st);
 int fbk_seg_mark(int64_t ntrip, const int *sorted_pair, int *flag, hipStream_t st);
 int fbk_seg_fill(int64_t ntrip, const int *sorted_pair, const int *segid_incl, int64_t *seg_off, int *seg_pair,
-                 int *trip_seg, hipStream_t st);
+                 int *trip_seg, int *pair_tgt, hipStream_t st);
 int fbk_gather_trip(int64_t ntrip, const int *perm, const int4 *in, int4 *out, hipStream_t st);
 int fbk_fill_i32(int *a, int64_t n, int val, hipStream_t st);
 int fbk_2c(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64(const DView &v, hipStream_t st);
 int fbk_3c_mixed(const DView &v, int chunk, int charged, hipStream_t st);
 int fbk_2c_sp(const DView &v, int mask, hipStream_t st);
+int fbk_2c_lds(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64_sp(const DView &v, hipStream_t st);
 int fbk_gamma_fold(const DView &v, int a0, int natoms, int which, const int *orb_off, int n, double *A,
                    hipStream_t st);
@@ -184,10 +185,13 @@ bool same_list(const MeList &a, const std::vector<int> &mu, const std::vector<in
 struct fb_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr, stream = nullptr;
+  hipStream_t side = nullptr;          // second stream: 2c beside 3c in a build
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
   int precision = FB_PREC_FP64;
   int method3c = FB_3C_TABLE;
   int chunk3c = 128;      // triplets per workgroup chunk of the mixed 3c kernel
+  int k2c = 1;            // 2c kernel for s/p species: 0 LDS-staged cells, 1 register-resident
 
   DSpecies sp{};
   int nZ[FB_MAXSPECIES] = {0};
@@ -210,7 +214,7 @@ struct fb_ctx {
 
   DevBuf d_neighn, d_noff, d_neigh_j, d_neigh_b, d_neigh_back, d_neigh_self, d_pair_atom, d_bsz, d_boff;
   DevBuf d_ncommon, d_toff, d_trip_orig, d_tpair_orig, d_trip, d_tpair, d_perm_in, d_perm, d_flag, d_segincl,
-      d_seg_off, d_seg_pair, d_trip_seg;
+      d_seg_off, d_seg_pair, d_trip_seg, d_pair_tgt;
   DevBuf d_tmp, d_status;
   DevBuf d_mat[5];
   int64_t npairs = 0, ntrip = 0, nseg = 0, blen = 0;
@@ -300,6 +304,7 @@ struct fb_ctx {
     v.trip_pair = d_tpair.as<int>();
     v.seg_off = d_seg_off.as<int64_t>();
     v.seg_pair = d_seg_pair.as<int>();
+    v.pair_tgt = d_pair_tgt.as<int>();
     v.trip_seg = d_trip_seg.as<int>();
     for (int i = 0; i < 5; i++) v.mat[i] = d_mat[i].as<double>();
     v.mlp = d_mlp.as<DMlp>();
@@ -394,7 +399,7 @@ struct fb_ctx {
                       &d_mu, &d_nu, &d_mv, &d_grid2c, &d_pden, &d_wg2c, &d_st, &d_xl, &d_ratom, &d_imass, &d_atom_struct, &d_dQ,
                       &d_neighn, &d_noff, &d_neigh_j, &d_neigh_b, &d_neigh_back, &d_neigh_self, &d_pair_atom,
                       &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
-                      &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_tmp,
+                      &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_pair_tgt, &d_tmp,
                       &d_mlp, &d_mlp_err, &d_mlp_count, &d_meth3c,
                       &d_status};
     for (DevBuf *b : bufs) b->release();
@@ -405,6 +410,9 @@ struct fb_ctx {
     if (graph) (void)hipGraphDestroy(graph);
     if (h_status) (void)hipHostFree(h_status);
     if (own_stream) (void)hipStreamDestroy(own_stream);
+    if (side) (void)hipStreamDestroy(side);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
   }
 };
 
@@ -461,7 +469,10 @@ int fb_create(fb_ctx **out, int device) {
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return create_failed(ctx, "hipSetDevice", e);
   e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
-  if (e != hipSuccess) return create_failed(ctx, "stream", e);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
+  if (e != hipSuccess) return create_failed(ctx, "streams", e);
   ctx->stream = ctx->own_stream;
   e = ctx->d_status.ensure(sizeof(int) * 4);
   if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_status, sizeof(int) * 4);
@@ -950,6 +961,8 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
   FB_HIP(ctx->d_seg_off.ensure(sizeof(int64_t) * (T + 1)));
   FB_HIP(ctx->d_seg_pair.ensure(sizeof(int) * T));
   FB_HIP(ctx->d_trip_seg.ensure(sizeof(int) * T));
+  FB_HIP(ctx->d_pair_tgt.ensure(sizeof(int) * std::max<int64_t>(P, 1)));
+  RC(ctx->launch("clear_targets", [&] { return fbk_fill_i32(ctx->d_pair_tgt.as<int>(), P, 0, st); }));
   RC(ctx->launch("common_fill", [&] {
     return fbk_common_fill(v, ctx->d_toff.as<int64_t>(), ctx->d_trip_orig.as<int4>(), ctx->d_tpair_orig.as<int>(),
                            v.status, st);
@@ -982,7 +995,7 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
     FB_HIP(hipMemcpyAsync(&nseg, ctx->d_segincl.as<int>() + (T - 1), sizeof(int), hipMemcpyDeviceToHost, st));
     RC(ctx->launch("seg_fill", [&] {
       return fbk_seg_fill(T, ctx->d_tpair.as<int>(), ctx->d_segincl.as<int>(), ctx->d_seg_off.as<int64_t>(),
-                          ctx->d_seg_pair.as<int>(), ctx->d_trip_seg.as<int>(), st);
+                          ctx->d_seg_pair.as<int>(), ctx->d_trip_seg.as<int>(), ctx->d_pair_tgt.as<int>(), st);
     }));
     RC(ctx->launch("gather_triplets", [&] {
       return fbk_gather_trip(T, ctx->d_perm.as<int>(), ctx->d_trip_orig.as<int4>(), ctx->d_trip.as<int4>(), st);
@@ -1100,10 +1113,15 @@ static int pre_assemble(fb_ctx *ctx, const char *what) {
   return FB_OK;
 }
 
-static int run_2c(fb_ctx *ctx, int mask, const char *name) {
+static int launch_2c(fb_ctx *ctx, const DView &v, int mask, int variant, hipStream_t st) {
+  return variant == 1 ? fbk_2c_sp(v, mask, st) : fbk_2c_lds(v, mask, st);
+}
+
+static int run_2c(fb_ctx *ctx, int mask, const char *name, hipStream_t st = nullptr) {
+  if (!st) st = ctx->stream;
   DView v = ctx->view();
-  if (ctx->canon_sp) return ctx->launch(name, [&] { return fbk_2c_sp(v, mask, ctx->stream); });
-  return ctx->launch(name, [&] { return fbk_2c(v, mask, ctx->stream); });
+  if (ctx->canon_sp) return ctx->launch(name, [&] { return launch_2c(ctx, v, mask, ctx->k2c, st); });
+  return ctx->launch(name, [&] { return fbk_2c(v, mask, st); });
 }
 
 static int any_charge(fb_ctx *ctx) {
@@ -1175,16 +1193,26 @@ int fb_assemble_vna_3c(fb_ctx *ctx) {
   return FB_OK;
 }
 
-// the launches of one H+S build (status clear, 2c, 3c), enqueued on the stream
+// The launches of one H+S build (status clear, 2c, 3c), enqueued on the
+// stream.  Without per-kernel timing the 2c launch that the 3c launch does not
+// read (all of 2c when neutral; the Vna part when charged, whose terms read
+// S and dipole_z) runs on a second stream beside the 3c launch; it then leaves
+// the Vna blocks of 3c targets to the 3c launch, which overwrites them.
 static int enqueue_build(fb_ctx *ctx, int charged) {
   RC(ctx->launch("clear_status", [&] { return fbk_fill_i32(ctx->d_status.as<int>(), 1, 0, ctx->stream); }));
-  if (charged) {
-    RC(run_2c(ctx, 7, "assemble_2c_STD"));
-    RC(run_2c(ctx, 8, "assemble_2c_vna"));
+  const bool fork = !ctx->timing && ctx->side != nullptr;
+  if (charged) RC(run_2c(ctx, 7, "assemble_2c_STD"));
+  if (fork) {
+    FB_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    FB_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+    RC(run_2c(ctx, (charged ? 8 : 15) | FB_2C_SKIP_TGT, charged ? "assemble_2c_vna" : "assemble_2c", ctx->side));
+    RC(run_3c(ctx));
+    FB_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+    FB_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   } else {
-    RC(run_2c(ctx, 15, "assemble_2c"));
+    RC(run_2c(ctx, charged ? 8 : 15, charged ? "assemble_2c_vna" : "assemble_2c"));
+    RC(run_3c(ctx));
   }
-  RC(run_3c(ctx));
   FB_HIP(hipMemcpyAsync(ctx->h_status, ctx->d_status.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   return FB_OK;
 }
@@ -1746,6 +1774,12 @@ int fb_set_tuning(fb_ctx *ctx, int param, int value) {
     ctx->chunk3c = value;
     return FB_OK;
   }
+  if (param == FB_TUNE_2C_KERNEL) {
+    if (value != 0 && value != 1) return ctx->fail(FB_E_ARG, "2c kernel must be 0 (LDS-staged) or 1 (register)");
+    ctx->k2c = value;
+    ctx->mlp_gen++;   // captured graphs hold the old launch
+    return FB_OK;
+  }
   return ctx->fail(FB_E_ARG, "unknown tuning parameter");
 }
 
@@ -1753,6 +1787,10 @@ int fb_get_tuning(fb_ctx *ctx, int param, int *value) {
   if (!ctx || !value) return FB_E_ARG;
   if (param == FB_TUNE_CHUNK3C) {
     *value = ctx->chunk3c;
+    return FB_OK;
+  }
+  if (param == FB_TUNE_2C_KERNEL) {
+    *value = ctx->k2c;
     return FB_OK;
   }
   return ctx->fail(FB_E_ARG, "unknown tuning parameter");
@@ -1773,26 +1811,51 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   FB_HIP(hipEventCreate(&b));
   DView v = ctx->view();
   const int charged = any_charge(ctx);
+  const bool mlp = ctx->precision == FB_PREC_MIXED && ctx->method3c == FB_3C_MLP && ctx->mlp_ready && !charged;
+  auto run3 = [&](int c) { return mlp ? fbk_3c_mixed_mlp(v, c, ctx->stream) : fbk_3c_mixed(v, c, charged, ctx->stream); };
+  auto run2 = [&](int var) {
+    if (!charged) return launch_2c(ctx, v, 15, var, ctx->stream);
+    int rc = launch_2c(ctx, v, 7, var, ctx->stream);
+    return rc ? rc : launch_2c(ctx, v, 8, var, ctx->stream);
+  };
+  auto timed = [&](auto &&fn, float *ms) -> int {
+    int rc = fn();  // warm-up
+    if (rc) return rc;
+    hipError_t e = hipEventRecord(a, ctx->stream);
+    for (int r = 0; r < nrep && e == hipSuccess && rc == 0; r++) rc = fn();
+    if (rc) return rc;
+    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, a, b);
+    return (int)e;
+  };
   for (int c : cand) {
-    int rc = fbk_3c_mixed(v, c, charged, ctx->stream);  // warm-up
-    if (rc) return ctx->fail(FB_E_HIP, "autotune", (hipError_t)rc);
-    FB_HIP(hipEventRecord(a, ctx->stream));
-    for (int r = 0; r < nrep; r++) {
-      rc = fbk_3c_mixed(v, c, charged, ctx->stream);
-      if (rc) return ctx->fail(FB_E_HIP, "autotune", (hipError_t)rc);
-    }
-    FB_HIP(hipEventRecord(b, ctx->stream));
-    FB_HIP(hipEventSynchronize(b));
     float ms = 0.f;
-    FB_HIP(hipEventElapsedTime(&ms, a, b));
+    int rc = timed([&] { return run3(c); }, &ms);
+    if (rc) return ctx->fail(FB_E_HIP, "autotune 3c", (hipError_t)rc);
     if (ms < best) {
       best = ms;
       pick = c;
     }
   }
+  if (ctx->canon_sp) {
+    float best2 = 1e30f;
+    int pick2 = ctx->k2c;
+    for (int var = 0; var < 2; var++) {
+      float ms = 0.f;
+      int rc = timed([&] { return run2(var); }, &ms);
+      if (rc) return ctx->fail(FB_E_HIP, "autotune 2c", (hipError_t)rc);
+      if (ms < best2) {
+        best2 = ms;
+        pick2 = var;
+      }
+    }
+    ctx->k2c = pick2;
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   ctx->chunk3c = pick;
+  ctx->mlp_gen++;   // captured graphs hold the previous launch shapes
   if (chosen) *chosen = pick;
   RC(ctx->check_status("autotune"));
   return FB_OK;

@@ -13,6 +13,7 @@
 #define FB_NSLOT2 6           // 2c interactions used on the path
 #define FB_P_EQ2 14.39975     // include/constants.h:4
 #define FB_2C_PCAP 256        // pairs per 2c workgroup pass
+#define FB_2C_SKIP_TGT 16     // 2c mask bit: leave the Vna blocks of 3c targets to the 3c launch
 
 // 2c interaction slots (device-side dense index of the interaction ids of
 // include/interactions_2c.h: overlap 1, kinetic 2, vna_ontopL 3, vna_ontopR 4,
@@ -110,6 +111,7 @@ struct DView {
   const int *trip_pair;         // [ntrip] target pair (global)
   const int64_t *seg_off;       // [nseg+1]
   const int *seg_pair;          // [nseg] target pair of each segment
+  const int *pair_tgt;          // [npairs] 1 where some triplet accumulates into the pair's Vna block
   const int *trip_seg;          // [ntrip] segment of each sorted triplet
   // learned 3c surrogate (one DMlp per 3c group; ready = 0 where untrained)
   const struct DMlp *mlp;

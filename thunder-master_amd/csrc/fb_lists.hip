@@ -202,7 +202,7 @@ __global__ void k_seg_mark(int64_t ntrip, const int *sorted_pair, int *flag) {
 }
 
 __global__ void k_seg_fill(int64_t ntrip, const int *sorted_pair, const int *segid_incl, int64_t *seg_off,
-                           int *seg_pair, int *trip_seg) {
+                           int *seg_pair, int *trip_seg, int *pair_tgt) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntrip) return;
   int sid = segid_incl[t] - 1;
@@ -210,6 +210,7 @@ __global__ void k_seg_fill(int64_t ntrip, const int *sorted_pair, const int *seg
   if (t == 0 || sorted_pair[t] != sorted_pair[t - 1]) {
     seg_off[sid] = t;
     seg_pair[sid] = sorted_pair[t];
+    pair_tgt[sorted_pair[t]] = 1;
   }
   if (t == ntrip - 1) seg_off[sid + 1] = ntrip;
 }
@@ -326,10 +327,10 @@ int fbk_seg_mark(int64_t ntrip, const int *sorted_pair, int *flag, hipStream_t s
 }
 
 int fbk_seg_fill(int64_t ntrip, const int *sorted_pair, const int *segid_incl, int64_t *seg_off, int *seg_pair,
-                 int *trip_seg, hipStream_t st) {
+                 int *trip_seg, int *pair_tgt, hipStream_t st) {
   if (ntrip == 0) return 0;
   hipLaunchKernelGGL(k_seg_fill, dim3((unsigned)((ntrip + 255) / 256)), dim3(256), 0, st, ntrip, sorted_pair,
-                     segid_incl, seg_off, seg_pair, trip_seg);
+                     segid_incl, seg_off, seg_pair, trip_seg, pair_tgt);
   return (int)hipGetLastError();
 }
 

@@ -322,6 +322,14 @@ class Assembler:
                                           X.shape[0], _fp(X), _fp(Y)))
         return Y
 
+    def tuning(self, param):
+        v = C.c_int()
+        self._check(self.L.fb_get_tuning(self.h, int(param), C.byref(v)))
+        return v.value
+
+    def set_tuning(self, param, value):
+        self._check(self.L.fb_set_tuning(self.h, int(param), int(value)))
+
     def set_chunk3c(self, chunk):
         self._check(self.L.fb_set_tuning(self.h, 0, int(chunk)))
 
