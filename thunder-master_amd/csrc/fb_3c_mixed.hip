@@ -33,14 +33,14 @@
 struct TPrm {
   unsigned base;              // FP32 slab index (floats) of stencil corner (ix-1, iy-1), neutral group
   unsigned rowstride;         // floats per iy row of the slab
-  int cls, nme, m1, p;        // pair class, ME count, method-1 flag, target pair
-  float wx[4], wy[4];         // cubic Lagrange weights (method 3)
-  float w1[3];                // (1-px-py, px, py) of the 3-point rule (method 1)
+  int p;                      // target pair
+  unsigned char cls, nme, m1, pad_;   // pair class, ME count, method-1 flag
+  float px, py;               // position in the cell; phase B forms the weights
   float P[5], sint;
   float pm[9];
-  int pad_;                   // odd row stride (33 words): lane-per-triplet accesses hit distinct banks
 };
-static_assert(sizeof(TPrm) == 33 * 4, "TPrm row stride must stay odd in words");
+// 21 words: an odd row stride, so lane-per-triplet accesses hit distinct banks
+static_assert(sizeof(TPrm) == 21 * 4, "TPrm row stride must stay odd in words");
 
 struct TChg {                 // charged third atom (DOGS terms)
   float dq[FB_MAXSH];
@@ -55,12 +55,12 @@ __device__ __forceinline__ int grp_id(const DView &v, int in1, int in2, int in3,
 
 // 4-point Lagrange weights on nodes -1, 0, 1, 2 (the Horner form of
 // M_Fdata_3c.f90:439-477 is this polynomial, divided by 6 per direction)
-__device__ __forceinline__ void cubic_w(double p, float *w) {
-  const double pm1 = p - 1.0, pm2 = p - 2.0, pp1 = p + 1.0, s6 = 1.0 / 6.0;
-  w[0] = (float)(-p * pm1 * pm2 * s6);
-  w[1] = (float)(pp1 * pm1 * pm2 * 0.5);
-  w[2] = (float)(-pp1 * p * pm2 * 0.5);
-  w[3] = (float)(pp1 * p * pm1 * s6);
+__device__ __forceinline__ void cubic_w(float p, float *w) {
+  const float pm1 = p - 1.f, pm2 = p - 2.f, pp1 = p + 1.f, s6 = 1.f / 6.f;
+  w[0] = -p * pm1 * pm2 * s6;
+  w[1] = pp1 * pm1 * pm2 * 0.5f;
+  w[2] = -pp1 * p * pm2 * 0.5f;
+  w[3] = pp1 * p * pm1 * s6;
 }
 
 // Buffer resource over the FP32 slab: loads take one 32-bit byte offset in a
@@ -121,17 +121,10 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   float(*sC)[NB + 1] = reinterpret_cast<float(*)[NB + 1]>(&sq[0]);
   static_assert(sizeof(TPrm) >= (NB + 1) * sizeof(float), "contribution row must fit in a parameter row");
   const int tid = threadIdx.x;
-  const int64_t ts = (int64_t)blockIdx.x * CH;
-  const int64_t te = min(ts + CH, v.ntrip);
-  int64_t first, last, r0, r1;
-  {
-    int64_t sgs = v.trip_seg[ts];
-    first = (v.seg_off[sgs] == ts) ? sgs : sgs + 1;
-    last = v.trip_seg[te - 1];
-    if (first > last) return;
-    r0 = v.seg_off[first];
-    r1 = v.seg_off[last + 1];
-  }
+  // segment ownership of this tile, from the list build
+  const int4 td = v.tiles[CH == 64 ? 0 : (CH == 128 ? 1 : 2)][blockIdx.x];
+  const int64_t first = td.x, last = td.y, r0 = td.z, r1 = td.w;
+  if (first > last) return;
   // owned segments [first, last] (at most CH: each starts inside the tile).
   // Accumulator slot sl = (segment, block element); lane tid owns the slots
   // tid + j*CH, j < NB, in registers.
@@ -151,9 +144,10 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       TPrm &q = sq[tid];
       const int64_t t = c0 + tid;
       const int4 tr = v.trip[t];
+      const int aux = v.trip_aux[t];
       const int ial = tr.x, gia = tr.y, gja = tr.z;
       const int ib = tr.w & 0xffff, jb = (tr.w >> 16) & 0xffff;
-      const int s = v.atom_struct[ial];
+      const int s = aux >> 16;
       const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
       double r1v[3], r2v[3], rna[3], r21[3], sighat[3], rnabc[3], rhat[3];
 #pragma unroll
@@ -218,7 +212,8 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       q.P[2] = (float)(0.5 * (3.0 * cost2 - 1.0));
       q.P[3] = (float)(0.5 * cost * (5.0 * cost2 - 3.0));
       q.P[4] = (float)(0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2)));
-      int gid = grp_id(v, in1, in2, indna, 0);
+      int gid = aux & 0xffff;
+      if (gid == 0xffff) gid = -1;
       if (gid < 0) {
         status |= ST_NO_TABLE;
         gid = 0;
@@ -238,11 +233,8 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       // reference's gradient test (M_Fdata_3c.f90:383-385)
       const int64_t icell = (int64_t)(iy - 1) * g.nx + (ix - 1);
       q.m1 = v.meth3c[g.moff + icell];
-      q.w1[0] = (float)(1.0 - px - py);
-      q.w1[1] = (float)px;
-      q.w1[2] = (float)py;
-      cubic_w(px, q.wx);
-      cubic_w(py, q.wy);
+      q.px = (float)px;
+      q.py = (float)py;
       q.rowstride = (unsigned)(g.nx * g.nME * TH8);
       q.base = (unsigned)(g.off32 + ((int64_t)(iy - 2) * g.nx + (ix - 2)) * g.nME * TH8);
       if (MLP) {
@@ -299,10 +291,12 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       if (m >= q.nme) continue;
       if (MLP && sG[t] >= 0) continue;
       float wx[4], wy[4], w1[3], P[5];
-#pragma unroll
-      for (int k = 0; k < 4; k++) { wx[k] = q.wx[k]; wy[k] = q.wy[k]; }
-#pragma unroll
-      for (int k = 0; k < 3; k++) w1[k] = q.w1[k];
+      const float px = q.px, py = q.py;
+      cubic_w(px, wx);
+      cubic_w(py, wy);
+      w1[0] = (1.f - px) - py;
+      w1[1] = px;
+      w1[2] = py;
 #pragma unroll
       for (int k = 0; k < 5; k++) P[k] = q.P[k];
       const unsigned base = q.base, rs = q.rowstride;

@@ -41,6 +41,8 @@ int fbk_seg_mark(int64_t ntrip, const int *sorted_pair, int *flag, hipStream_t s
 int fbk_seg_fill(int64_t ntrip, const int *sorted_pair, const int *segid_incl, int64_t *seg_off, int *seg_pair,
                  int *trip_seg, int *pair_tgt, hipStream_t st);
 int fbk_gather_trip(int64_t ntrip, const int *perm, const int4 *in, int4 *out, hipStream_t st);
+int fbk_trip_aux(const DView &v, int *aux, hipStream_t st);
+int fbk_tiles(const DView &v, int ch, int4 *tiles, hipStream_t st);
 int fbk_fill_i32(int *a, int64_t n, int val, hipStream_t st);
 int fbk_2c(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64(const DView &v, hipStream_t st);
@@ -214,7 +216,7 @@ struct fb_ctx {
 
   DevBuf d_neighn, d_noff, d_neigh_j, d_neigh_b, d_neigh_back, d_neigh_self, d_pair_atom, d_bsz, d_boff;
   DevBuf d_ncommon, d_toff, d_trip_orig, d_tpair_orig, d_trip, d_tpair, d_perm_in, d_perm, d_flag, d_segincl,
-      d_seg_off, d_seg_pair, d_trip_seg, d_pair_tgt;
+      d_seg_off, d_seg_pair, d_trip_seg, d_pair_tgt, d_trip_aux, d_tiles[3];
   DevBuf d_tmp, d_status;
   DevBuf d_mat[5];
   int64_t npairs = 0, ntrip = 0, nseg = 0, blen = 0;
@@ -305,6 +307,8 @@ struct fb_ctx {
     v.seg_off = d_seg_off.as<int64_t>();
     v.seg_pair = d_seg_pair.as<int>();
     v.pair_tgt = d_pair_tgt.as<int>();
+    v.trip_aux = d_trip_aux.as<int>();
+    for (int k = 0; k < 3; k++) v.tiles[k] = d_tiles[k].as<int4>();
     v.trip_seg = d_trip_seg.as<int>();
     for (int i = 0; i < 5; i++) v.mat[i] = d_mat[i].as<double>();
     v.mlp = d_mlp.as<DMlp>();
@@ -399,7 +403,7 @@ struct fb_ctx {
                       &d_mu, &d_nu, &d_mv, &d_grid2c, &d_pden, &d_wg2c, &d_st, &d_xl, &d_ratom, &d_imass, &d_atom_struct, &d_dQ,
                       &d_neighn, &d_noff, &d_neigh_j, &d_neigh_b, &d_neigh_back, &d_neigh_self, &d_pair_atom,
                       &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
-                      &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_pair_tgt, &d_tmp,
+                      &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_pair_tgt, &d_trip_aux, &d_tiles[0], &d_tiles[1], &d_tiles[2], &d_tmp,
                       &d_mlp, &d_mlp_err, &d_mlp_count, &d_meth3c,
                       &d_status};
     for (DevBuf *b : bufs) b->release();
@@ -1002,6 +1006,19 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
     }));
     FB_HIP(hipStreamSynchronize(st));
     ctx->nseg = nseg;
+    // per-triplet group and structure, per-tile segment ownership (list
+    // properties: species and triplet order only)
+    FB_HIP(ctx->d_trip_aux.ensure(sizeof(int) * T));
+    v = ctx->view();
+    v.ntrip = T;
+    v.nseg = nseg;
+    RC(ctx->launch("trip_aux", [&] { return fbk_trip_aux(v, ctx->d_trip_aux.as<int>(), st); }));
+    const int chs[3] = {64, 128, 256};
+    for (int k = 0; k < 3; k++) {
+      FB_HIP(ctx->d_tiles[k].ensure(sizeof(int4) * ((T + chs[k] - 1) / chs[k])));
+      RC(ctx->launch("tiles", [&] { return fbk_tiles(v, chs[k], ctx->d_tiles[k].as<int4>(), st); }));
+    }
+    FB_HIP(hipStreamSynchronize(st));
   }
   // block offsets
   v = ctx->view();

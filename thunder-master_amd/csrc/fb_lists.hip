@@ -353,3 +353,44 @@ extern "C" int fbk_fill_i32(int *a, int64_t n, int val, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, val);
   return (int)hipGetLastError();
 }
+
+// species group (neutral bcna) and structure of every sorted triplet
+__global__ void k_trip_aux(DView v, int *aux) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= v.ntrip) return;
+  const int4 tr = v.trip[t];
+  const int in1 = v.imass[tr.y], in2 = v.imass[tr.z], in3 = v.imass[tr.x];
+  int gid = v.idx3c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * v.sp.ns + (in3 - 1)) * FB_MAXSUB3];
+  aux[t] = (gid < 0 ? 0xffff : gid) | (v.atom_struct[tr.x] << 16);
+}
+
+// segment ownership of every tile of ch consecutive sorted triplets: the
+// segments that start inside the tile, and the triplets they span
+__global__ void k_tiles(DView v, int ch, int4 *tiles) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ntile = (v.ntrip + ch - 1) / ch;
+  if (b >= ntile) return;
+  const int64_t ts = b * ch, te = min(ts + (int64_t)ch, v.ntrip);
+  const int64_t sgs = v.trip_seg[ts];
+  const int64_t first = (v.seg_off[sgs] == ts) ? sgs : sgs + 1;
+  const int64_t last = v.trip_seg[te - 1];
+  int4 d = make_int4((int)first, (int)last, 0, 0);
+  if (first <= last) {
+    d.z = (int)v.seg_off[first];
+    d.w = (int)v.seg_off[last + 1];
+  }
+  tiles[b] = d;
+}
+
+extern "C" int fbk_trip_aux(const DView &v, int *aux, hipStream_t st) {
+  if (v.ntrip == 0) return 0;
+  hipLaunchKernelGGL(k_trip_aux, dim3((unsigned)((v.ntrip + 255) / 256)), dim3(256), 0, st, v, aux);
+  return (int)hipGetLastError();
+}
+
+extern "C" int fbk_tiles(const DView &v, int ch, int4 *tiles, hipStream_t st) {
+  const int64_t ntile = (v.ntrip + ch - 1) / ch;
+  if (ntile == 0) return 0;
+  hipLaunchKernelGGL(k_tiles, dim3((unsigned)((ntile + 255) / 256)), dim3(256), 0, st, v, ch, tiles);
+  return (int)hipGetLastError();
+}
