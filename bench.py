@@ -106,11 +106,16 @@ def time_graph(asm, steps, warmup):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def gamma_eigen_shift(asm, precision_a, precision_b):
+def gamma_eigen_shift(asm, precision_a, precision_b, max_norb=20000):
     """Eigenvalue shift of the Gamma-point folded H = T + Vna (block + blocko)
     of structure 0 between two precisions, from the library's own fold and
     rocSOLVER dsyevd on the upper triangle; returns max |lambda_a - lambda_b|
-    in eV and the orbital count.  S is binary64 in both paths."""
+    in eV and the orbital count (None above max_norb orbitals, where the dense
+    solve alone takes tens of seconds).  S is binary64 in both paths."""
+    st = asm.structs[0]
+    norb = np.array([sp.norb for sp in asm.tables.species])
+    if int(norb[np.asarray(st.imass) - 1].sum()) > max_norb:
+        return None
     lam = {}
     for prec in (precision_a, precision_b):
         asm.set_precision(prec)
@@ -119,6 +124,44 @@ def gamma_eigen_shift(asm, precision_a, precision_b):
     asm.set_precision(precision_a)
     return {"H": float(np.max(np.abs(lam[precision_a] - lam[precision_b]))), "norb": int(lam[precision_a].size),
             "lambda_range_eV": [float(lam[precision_b][0]), float(lam[precision_b][-1])]}
+
+
+MLP_FLOPS_PER_EVAL = 2 * (64 * 64 + 64 * 64)   # the two 64-wide MFMA layers of one network evaluation
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def mlp_leg(asm, steps, warmup, vna64, table_kernel_ms, table_build_ms):
+    """Train the learned 3c surrogate on the workload's tables, then time the
+    same H+S build with it (table stencil wherever the calibrated bound exceeds
+    the tolerance) and report its error against binary64 and its MFMA use."""
+    t0 = time.perf_counter()
+    asm.set_3c_method("mlp")
+    asm.train_mlp(nsamples=1 << 21, nsteps=4000, batch=8192, lr=3e-3, seed=7)
+    train_s = time.perf_counter() - t0
+    asm.assemble_all()
+    asm.mlp_stats()
+    ms_events, kern = time_builds(asm, steps, warmup)
+    n_net, n_seen = asm.mlp_stats()
+    ms = time_graph(asm, steps, warmup)
+    asm.mlp_stats()
+    asm.assemble_all()
+    err = float(np.max(np.abs(np.concatenate([asm.blocks("VNA", k) for k in range(len(asm.structs))]) - vna64)))
+    kms = kern.get("assemble_3c_mlp", float("nan"))
+    per_launch = n_net / max(steps + warmup, 1)
+    groups = [asm.mlp_info(gid) for gid in range(asm.mlp_groups())]
+    trained = [g for g in groups if g["trained"] == 1]
+    asm.set_3c_method("table")
+    achieved = per_launch * MLP_FLOPS_PER_EVAL / (kms * 1e-3) / 1e12
+    return {"train_s": train_s, "networks": len(trained),
+            "loss_after_max": max((g["loss_after"] for g in trained), default=None),
+            "max_cell_err_eV": max((g["max_cell_err_eV"] for g in trained), default=None),
+            "tolerance_eV": 5e-5, "fraction_on_network": n_net / max(n_seen, 1),
+            "ms_per_step": ms, "kernel_ms": kms, "table_kernel_ms": table_kernel_ms,
+            "speedup_vs_table_kernel": table_kernel_ms / kms, "speedup_vs_table_build": table_build_ms / ms,
+            "max_abs_err_vs_fp64_eV": err,
+            "mfma": {"achieved_tflops": achieved, "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+                     "utilisation": achieved / MFMA_BF16_PEAK_TFLOPS,
+                     "flops_per_network_eval": MLP_FLOPS_PER_EVAL}}
 
 
 def measured_traffic(kernel, config):
@@ -186,6 +229,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-eig", action="store_true", help="skip the Gamma-point eigenvalue shift")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the learned 3c surrogate leg")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one elementwise copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -238,6 +282,17 @@ def main():
         asm.set_precision(args.precision)
         print(f"# fp64: {ms64:.3f} ms; err {err}", file=sys.stderr, flush=True)
 
+    mlp = None
+    if not args.no_mlp and args.precision == "mixed" and not args.no_fp64_compare:
+        try:
+            mlp = mlp_leg(asm, args.steps, args.warmup, res64["VNA"], kern.get("assemble_3c_mixed", float("nan")), ms)
+            print(f"# mlp: {mlp['ms_per_step']:.3f} ms per build, 3c kernel {mlp['kernel_ms']:.3f} ms "
+                  f"(table {mlp['table_kernel_ms']:.3f}), fraction {mlp['fraction_on_network']:.3f}, "
+                  f"err {mlp['max_abs_err_vs_fp64_eV']:.2e} eV, MFMA {mlp['mfma']['utilisation']:.2e}",
+                  file=sys.stderr, flush=True)
+        except Exception as e:  # reported, never required
+            mlp = {"error": str(e)}
+
     # dominant kernel: the 3c launch
     k3 = [k for k in kern if k.startswith("assemble_3c")]
     dom = max(kern, key=kern.get) if not k3 else k3[0]
@@ -263,7 +318,8 @@ def main():
                    "nstructures": len(structs), "npairs": npairs, "ntriplets": ntrip,
                    "precision": args.precision, "h_s_build_ms": ms, "lists_ms": lists_ms,
                    "chunk3c": chunk, "ms_per_step_with_kernel_events": ms_events,
-                   "kernels_ms": kern, **extra},
+                   "kernels_ms": kern, **extra, "mlp_3c": mlp,
+                   **({"structures_per_s": len(structs) / (ms * 1e-3)} if len(structs) > 1 else {})},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

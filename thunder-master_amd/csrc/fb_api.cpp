@@ -204,6 +204,7 @@ struct fb_ctx {
   DevBuf d_cell2c, d_idx2c, d_grp3c, d_idx3c, d_tab2c, d_tab2c32, d_tab3c, d_tab3c32, d_mu, d_nu, d_mv, d_meth3c;
   DevBuf d_grid2c, d_pden, d_wg2c;
   int canon_sp = 0;
+  int ncell2c = 0, ngrid2c = 0;
   int64_t ntab3c32 = 0;
   int nwg2c = 0;
 
@@ -266,6 +267,8 @@ struct fb_ctx {
     DView v{};
     v.sp = sp;
     v.cell2c = d_cell2c.as<DCell2c>();
+    v.ncell2c = ncell2c;
+    v.ngrid2c = ngrid2c;
     v.idx2c = d_idx2c.as<int>();
     v.grp3c = d_grp3c.as<DGroup3c>();
     v.idx3c = d_idx3c.as<int>();
@@ -625,6 +628,7 @@ int fb_commit_tables(fb_ctx *ctx) {
     d.nx = c.nx;
     d.xmax = c.xmax;
     d.dx = c.xmax / (double)(c.nx - 1);  // e.FDATA/M_Fdata_2c.f90:222
+    if (tab2.size() & 1) tab2.push_back(0.0);   // cells start on 16-byte boundaries (paired loads)
     d.off = (int64_t)tab2.size();
     d.off32 = d.off;
     d.munu = (int)mu.size();
@@ -772,6 +776,8 @@ int fb_commit_tables(fb_ctx *ctx) {
   RC(upload(ctx, ctx->d_tab3c32, tab3f.data(), tab3f.size() * sizeof(float), "upload tab3c32"));
   ctx->ntab3c32 = (int64_t)tab3f.size();
   RC(upload(ctx, ctx->d_cell2c, cells.data(), cells.size() * sizeof(DCell2c), "upload cells2c"));
+  ctx->ncell2c = (int)cells.size();
+  ctx->ngrid2c = (int)grids.size();
   RC(upload(ctx, ctx->d_grid2c, grids.data(), grids.size() * sizeof(DGrid2c), "upload grids2c"));
   RC(upload(ctx, ctx->d_pden, pden.data(), pden.size() * sizeof(double), "upload pden"));
   RC(upload(ctx, ctx->d_mu, mu.data(), mu.size() * sizeof(int), "upload mu"));
@@ -1791,6 +1797,11 @@ int fb_set_tuning(fb_ctx *ctx, int param, int value) {
     ctx->chunk3c = value;
     return FB_OK;
   }
+  if (param == FB_TUNE_GRAPH) {
+    if (value != 0 && value != 1) return ctx->fail(FB_E_ARG, "graph replay must be 0 or 1");
+    ctx->use_graph = value != 0;
+    return FB_OK;
+  }
   if (param == FB_TUNE_2C_KERNEL) {
     if (value != 0 && value != 1) return ctx->fail(FB_E_ARG, "2c kernel must be 0 (LDS-staged) or 1 (register)");
     ctx->k2c = value;
@@ -1808,6 +1819,10 @@ int fb_get_tuning(fb_ctx *ctx, int param, int *value) {
   }
   if (param == FB_TUNE_2C_KERNEL) {
     *value = ctx->k2c;
+    return FB_OK;
+  }
+  if (param == FB_TUNE_GRAPH) {
+    *value = ctx->use_graph ? 1 : 0;
     return FB_OK;
   }
   return ctx->fail(FB_E_ARG, "unknown tuning parameter");

@@ -75,6 +75,7 @@ struct DView {
   DSpecies sp;
   // tables
   const DCell2c *cell2c;        // cells
+  int ncell2c, ngrid2c;         // cell and grid counts (small: the 2c kernel caches them in LDS)
   const int *idx2c;             // [ns][ns][FB_NSLOT2][FB_MAXSUB2] -> cell or -1
   const DGroup3c *grp3c;
   const int *idx3c;             // [ns][ns][ns][FB_MAXSUB3] -> group or -1 (itype = bcna)

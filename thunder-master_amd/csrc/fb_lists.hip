@@ -394,3 +394,4 @@ extern "C" int fbk_tiles(const DView &v, int ch, int4 *tiles, hipStream_t st) {
   hipLaunchKernelGGL(k_tiles, dim3((unsigned)((ntile + 255) / 256)), dim3(256), 0, st, v, ch, tiles);
   return (int)hipGetLastError();
 }
+
