@@ -38,7 +38,7 @@ METRIC = "3c triplets/s + H+S build ms at <=1e-6 eV err; FP32/MLP vs plain FP64,
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and
 # scaled by the same counters over a copy of known size; one line per kernel:
-#   traffic <kernel name> <workload config> <bytes per launch>
+#   traffic <launch name>:<3c chunk> <workload config> <bytes per launch>
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "hbm_traffic.txt")
 WORKLOADS = {
     0: "Si64 periodic diamond 2x2x2 (a=5.43 A, sigma 0.05 A, seed 0), sp3, rc 5.4 bohr",
@@ -164,14 +164,16 @@ def mlp_leg(asm, steps, warmup, vna64, table_kernel_ms, table_build_ms):
                      "flops_per_network_eval": MLP_FLOPS_PER_EVAL}}
 
 
-def measured_traffic(kernel, config):
-    """HBM bytes per launch of `kernel` on workload `config` from the committed
-    counter summary, or None when it holds no such line."""
+def measured_traffic(kernel, config, chunk):
+    """HBM bytes per launch of `kernel` (at 3c chunk `chunk`) on workload
+    `config` from the committed counter summary, or None when it holds no
+    such line."""
+    key = f"{kernel}:{chunk}" if chunk else kernel
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 w = line.split()
-                if len(w) == 4 and w[0] == "traffic" and w[1] == kernel and w[2] == str(config):
+                if len(w) == 4 and w[0] == "traffic" and w[1] == key and w[2] == str(config):
                     return float(w[3])
     except OSError:
         pass
@@ -299,7 +301,7 @@ def main():
     bytes3c, _ = algorithmic_bytes_3c(asm, len(structs))
     achieved = bytes3c / (kern[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, args.config), "kernel": dom,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, args.config, chunk), "kernel": dom,
             "kernel_ms": kern[dom], "algorithmic_bytes": bytes3c}
 
     cpu = None

@@ -91,6 +91,13 @@ int fb_set_table_3c(fb_ctx *ctx, int is, int js, int ks, int itype, int isub, in
                     int nx, double xmax, int ny, double ymax, const int *mu, const int *nu,
                     const int *mvalue, const double *fdata);
 int fb_commit_tables(fb_ctx *ctx);
+/* fb_read_fdata: read_Fdata_2c + read_Fdata_3c (e.FDATA/M_Fdata_2c.f90:149-272,
+ * M_Fdata_3c.f90:143-264) from a directory in the reference's text layout
+ * (2c.Z1.Z2.dat/.dir, 3c.Z1.Z2.Z3.dat/.dir, data files, TT_munu_*.dat) for
+ * the species of fb_set_species (their nZ), loading the interactions of this
+ * path through fb_set_table_2c / fb_set_table_3c.  Call fb_commit_tables
+ * afterwards.  A missing or malformed file fails with FB_E_NO_TABLE. */
+int fb_read_fdata(fb_ctx *ctx, const char *fdata_dir);
 
 /* --- structures -----------------------------------------------------------
  * One call per structure (T_structure after read_positions / make_cells /

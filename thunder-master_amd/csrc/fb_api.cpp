@@ -19,6 +19,7 @@
 
 #include "fb_internal.h"
 #include "fb_mlp.h"
+#include "fb_fdata.h"
 #include <array>
 #include <random>
 
@@ -603,6 +604,37 @@ int fb_set_table_3c(fb_ctx *ctx, int is, int js, int ks, int itype, int isub, in
   ctx->c3.push_back(std::move(c));
   ctx->tables_ok = false;
   return FB_OK;
+}
+
+namespace {
+bool fdata_type2c(int itype) { return slot_of(itype) >= 0; }
+bool fdata_type3c(int itype) { return itype == 3; }   // bcna, include/interactions_3c.h:8
+
+struct CtxSink : FdataSink {
+  fb_ctx *ctx;
+  explicit CtxSink(fb_ctx *c) : ctx(c) {}
+  int cell2c(int is, int js, int itype, int isub, int nME, int nx, double xmax, const int *mu, const int *nu,
+             const int *mv, const double *fdata) override {
+    return fb_set_table_2c(ctx, is, js, itype, isub, nME, nx, xmax, mu, nu, mv, fdata);
+  }
+  int cell3c(int is, int js, int ks, int itype, int isub, int itheta, int nME, int nx, double xmax, int ny,
+             double ymax, const int *mu, const int *nu, const int *mv, const double *fdata) override {
+    return fb_set_table_3c(ctx, is, js, ks, itype, isub, itheta, nME, nx, xmax, ny, ymax, mu, nu, mv, fdata);
+  }
+};
+}  // namespace
+
+int fb_read_fdata(fb_ctx *ctx, const char *fdata_dir) {
+  if (!ctx || !fdata_dir) return FB_E_ARG;
+  if (!ctx->species_set) return ctx->fail(FB_E_ORDER, "read_fdata before set_species");
+  for (int i = 0; i < ctx->sp.ns; i++)
+    if (ctx->nZ[i] < 1) return ctx->fail(FB_E_ARG, "read_fdata: species need their atomic numbers");
+  CtxSink sink(ctx);
+  std::string err;
+  FdataWanted want{fdata_type2c, fdata_type3c};
+  int rc = fb_fdata_read(fdata_dir, ctx->sp.ns, ctx->nZ, want, sink, err);
+  if (rc == -1) return ctx->fail(FB_E_NO_TABLE, ("read_fdata: " + err).c_str());
+  return rc;
 }
 
 int fb_commit_tables(fb_ctx *ctx) {

@@ -24,7 +24,7 @@ import os
 
 import numpy as np
 
-from .synth import TableSet, Structure
+from .synth import Cell2c, Cell3c, Structure, TableSet
 
 
 def _fmt(v):
@@ -106,3 +106,111 @@ def write_harness_input(path: str, fdata_dir: str, ts: TableSet, s: Structure):
             nssh = ts.species[s.imass[ia] - 1].nssh
             dq = np.zeros(nssh) if s.dQ is None else s.dQ[ia, :nssh]
             f.write(" ".join(_fmt(v) for v in dq) + "\n")
+
+
+# ---------------------------------------------------------------- reader
+class _ListReader:
+    """Fortran list-directed input over one text file: every read starts at a
+    new record and takes values across records; blanks and commas separate,
+    d/D exponents and r*c repeats are accepted, '/' ends a read."""
+
+    def __init__(self, path):
+        self.path = path
+        with open(path) as f:
+            self.lines = f.read().splitlines()
+        self.pos = 0
+
+    def values(self, n):
+        out = []
+        while len(out) < n:
+            if self.pos >= len(self.lines):
+                raise ValueError(f"short record in {self.path}")
+            line = self.lines[self.pos]
+            self.pos += 1
+            stop = False
+            for tok in line.replace(",", " ").split():
+                if tok.startswith("/"):
+                    stop = True
+                    break
+                if "*" in tok and tok.split("*", 1)[0].isdigit():
+                    r, v = tok.split("*", 1)
+                    out.extend([v] * int(r))
+                else:
+                    out.append(tok.strip("'\""))
+                if len(out) >= n:
+                    break
+            if stop:
+                break
+        if len(out) < n:
+            raise ValueError(f"short record in {self.path}")
+        return out[:n]
+
+
+def _f(tok):
+    return float(tok.replace("d", "e").replace("D", "e"))
+
+
+def read_fdata(fdir, species, types2c=(1, 2, 3, 4, 5, 10), types3c=(3,)):
+    """TableSet of the given species (their nZ) from an Fdata directory, with
+    the reference reader's semantics (e.FDATA/M_Fdata_2c.f90:149-272,
+    M_Fdata_3c.f90:143-264): (itype, isubtype[, itheta]) names the cell at
+    directory position iindex, and the mu/nu/mvalue unit stays open across
+    cells, restarting a file whenever the interaction changes (create.x
+    directories keep each interaction's cells together)."""
+    ts = TableSet(species=list(species))
+    z = [sp.nZ for sp in species]
+    ns = len(species)
+    for a in range(1, ns + 1):
+        for b in range(1, ns + 1):
+            za, zb = z[a - 1], z[b - 1]
+            n = int(_ListReader(os.path.join(fdir, f"2c.{za:02d}.{zb:02d}.dat")).values(1)[0])
+            d = _ListReader(os.path.join(fdir, f"2c.{za:02d}.{zb:02d}.dir"))
+            unit = None
+            cells, index = [], {}
+            for _ in range(n):
+                t = d.values(7)
+                iindex, itype, isub, fname = int(t[0]), int(t[1]), int(t[2]), t[3]
+                nme, nx, xmax = int(t[4]), int(t[5]), _f(t[6])
+                data = None
+                if itype in types2c:
+                    r = _ListReader(os.path.join(fdir, fname))
+                    data = np.array([[_f(v) for v in r.values(nme)] for _ in range(nx)])
+                mpath = os.path.join(fdir, f"{itype:02d}_munu_2c.{za:02d}.{zb:02d}.dat")
+                if unit is None or unit.path != mpath:
+                    unit = _ListReader(mpath)
+                mu, nu, mv = (np.array([int(v) for v in unit.values(nme)], np.int32) for _ in range(3))
+                cells.append((itype, Cell2c(nme, nx, xmax, mu, nu, mv, data)))
+                index[(itype, isub)] = iindex
+            for (itype, isub), iindex in index.items():
+                if itype in types2c:
+                    ctype, cell = cells[iindex - 1]
+                    if ctype not in types2c:
+                        raise ValueError("2c index points at a cell of another interaction")
+                    ts.cells2c[(a, b, itype, isub)] = cell
+            for k in range(1, ns + 1):
+                zk = z[k - 1]
+                n3 = int(_ListReader(os.path.join(fdir, f"3c.{za:02d}.{zb:02d}.{zk:02d}.dat")).values(1)[0])
+                d3 = _ListReader(os.path.join(fdir, f"3c.{za:02d}.{zb:02d}.{zk:02d}.dir"))
+                unit3 = None
+                cells3, index3 = [], {}
+                for _ in range(n3):
+                    t = d3.values(10)
+                    iindex, itype, isub, ith, fname = int(t[0]), int(t[1]), int(t[2]), int(t[3]), t[4]
+                    nme, nx, xmax, ny, ymax = int(t[5]), int(t[6]), _f(t[7]), int(t[8]), _f(t[9])
+                    data = None
+                    if itype in types3c:
+                        r = _ListReader(os.path.join(fdir, fname))
+                        data = np.array([[_f(v) for v in r.values(nme)] for _ in range(nx * ny)]).reshape(ny, nx, nme)
+                    mpath = os.path.join(fdir, f"{itype:02d}_munu_3c.{za:02d}.{zb:02d}.{zk:02d}.dat")
+                    if unit3 is None or unit3.path != mpath:
+                        unit3 = _ListReader(mpath)
+                    mu, nu, mv = (np.array([int(v) for v in unit3.values(nme)], np.int32) for _ in range(3))
+                    cells3.append((itype, Cell3c(nme, nx, ny, xmax, ymax, mu, nu, mv, data)))
+                    index3[(itype, isub, ith)] = iindex
+                for (itype, isub, ith), iindex in index3.items():
+                    if itype in types3c:
+                        ctype, cell = cells3[iindex - 1]
+                        if ctype not in types3c:
+                            raise ValueError("3c index points at a cell of another interaction")
+                        ts.cells3c[(a, b, k, itype, isub, ith)] = cell
+    return ts

@@ -71,6 +71,7 @@ def lib():
             "fb_autotune": [vp, C.c_int, ip],
             "fb_get_timings": [vp, C.c_int, C.c_char_p, C.POINTER(C.c_float), ip],
             "fb_gamma_eigenvalues": [vp, C.c_int, C.c_int, dp, C.c_int64, i64p],
+            "fb_read_fdata": [vp, C.c_char_p],
             "fb_mlp_train": [vp, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_uint64],
             "fb_mlp_groups": [vp, ip],
             "fb_mlp_samples": [vp, C.c_int, C.c_int64, C.c_uint64, fp, fp],
@@ -111,7 +112,10 @@ def _dp(a):
 class Assembler:
     """One context of the GPU assembler: species + tables + a batch of structures."""
 
-    def __init__(self, tables, device=0, precision="fp64", method3c="table"):
+    def __init__(self, tables, device=0, precision="fp64", method3c="table", fdata_dir=None):
+        """tables: a TableSet whose species and cells are uploaded; with
+        fdata_dir, only its species are used and the cells are read from that
+        directory in the reference's Fdata layout (fb_read_fdata)."""
         self.L = lib()
         self.h = C.c_void_p()
         rc = self.L.fb_create(C.byref(self.h), device)
@@ -120,7 +124,11 @@ class Assembler:
         self.tables = tables
         self.structs = []
         self._set_species(tables)
-        self._set_tables(tables)
+        if fdata_dir is None:
+            self._set_tables(tables)
+        else:
+            self._check(self.L.fb_read_fdata(self.h, os.fsencode(fdata_dir)))
+            self._check(self.L.fb_commit_tables(self.h))
         self.set_precision(precision)
         self.set_3c_method(method3c)
 
