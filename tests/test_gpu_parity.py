@@ -277,3 +277,45 @@ def test_gamma_eigenvalues_match_numpy(golden, name):
         w = asm.gamma_eigenvalues(which)
         assert w.shape == ref.shape
         np.testing.assert_allclose(w, ref, rtol=0, atol=1e-9 * max(1.0, float(np.max(np.abs(ref)))), err_msg=which)
+
+
+def _list_invariants(asm, istruct, st):
+    """Size-independent properties of the lists of one structure: neigh_back
+    is an involution onto the mirrored pair, and every triplet's mneigh names
+    iatom's neighbour (jatom, jbeta - ibeta)."""
+    from fireball_hip import synth  # noqa: F401
+    import oracle as orc
+    L = asm.lists(istruct)
+    off = np.concatenate([[0], np.cumsum(L["neighn"])])
+    back = off[L["neigh_j"] - 1] + L["neigh_back"] - 1
+    atom_of_pair = np.repeat(np.arange(1, st.natoms + 1), L["neighn"])
+    assert np.array_equal(L["neigh_j"][back], atom_of_pair)
+    assert np.array_equal(back[back], np.arange(len(back)))
+    xl = orc.make_cells(np.asarray(st.lattice), st.icluster)
+    assert np.allclose(xl[L["neigh_b"][back]], -xl[L["neigh_b"]], atol=1e-9)
+    tgt = off[L["com_iatom_j"] - 1] + L["com_mneigh"] - 1
+    assert np.array_equal(L["neigh_j"][tgt], L["com_jatom_j"])
+    d = xl[L["neigh_b"][tgt]] - (xl[L["com_jatom_b"]] - xl[L["com_iatom_b"]])
+    assert np.max(np.abs(d)) < 1e-4
+
+
+@pytest.mark.parametrize("cfg", [2, 4, 3])
+def test_full_size_workloads_mixed_vs_fp64(cfg):
+    """CHNO3000, aC8192 and the 512-structure batch at their real sizes: the
+    mixed path within 1e-6 eV of binary64 on every matrix, and the list
+    invariants on (a sample of) the structures."""
+    from fireball_hip import synth
+    print(f"config {cfg}", flush=True)
+    species, structs = synth.config(cfg)
+    ts = synth.make_tables(species)
+    asm = _gpu(ts, structs, precision="mixed")
+    ks = range(len(structs)) if len(structs) < 8 else [0, 1, 2, len(structs) // 2, len(structs) - 1]
+    mixed = {m: [asm.blocks(m, k) for k in ks] for m in MATS}
+    asm.set_precision("fp64")
+    asm.assemble_all()
+    for m in MATS:
+        err = max(float(np.max(np.abs(a - asm.blocks(m, k)))) for a, k in zip(mixed[m], ks))
+        print(f"config {cfg} {m} mixed vs fp64 max-abs {err:.3e} eV", flush=True)
+        assert err <= TOL_MIXED, m
+    for k in ks:
+        _list_invariants(asm, k, structs[k])
