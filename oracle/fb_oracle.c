@@ -71,6 +71,14 @@ struct fbo_system {
   double *mat[5];
   int done[5];
   int nfallback;
+  /* forces: density matrix input and derivative outputs */
+  double *rho;      /* [blen] per-pair density blocks, block layout */
+  double *dmat[3];  /* [3*blen] Dblock(3, norb_mu, norb_nu) of S, T, dipole_z */
+  double *fatom;    /* [npairs][3] forces%vna_atom(:, ineigh) */
+  double *fontop;   /* [npairs][3] forces%vna_ontop(:, ineigh) */
+  double *f3[3];    /* [natoms][3] forces%f3naa, f3nab, f3nac */
+  int ddone[5];
+  int ndwarn;       /* Depsilon_2c/3c colinear guards taken (WARNINGS file lines in the reference) */
 };
 
 static inline cell2c *C2(const fbo_tables *t, int is, int js, int it, int isub) {
@@ -458,6 +466,8 @@ void fbo_system_free(fbo_system *s) {
   free(s->ncommon); free(s->coff); free(s->ci_j); free(s->ci_b); free(s->cj_j); free(s->cj_b); free(s->cm);
   free(s->boff);
   for (int i = 0; i < 5; i++) free(s->mat[i]);
+  free(s->rho); free(s->fatom); free(s->fontop);
+  for (int i = 0; i < 3; i++) { free(s->dmat[i]); free(s->f3[i]); }
   free(s);
 }
 
@@ -626,6 +636,15 @@ int fbo_build_lists(fbo_system *s) {
   s->boff[s->npairs] = o;
   s->blen = o;
   for (int i = 0; i < 5; i++) { free(s->mat[i]); s->mat[i] = calloc(o ? o : 1, sizeof(double)); s->done[i] = 0; }
+  free(s->rho); s->rho = NULL;
+  free(s->fatom); free(s->fontop);
+  s->fatom = calloc((size_t)(3 * s->npairs + 1), sizeof(double));
+  s->fontop = calloc((size_t)(3 * s->npairs + 1), sizeof(double));
+  for (int i = 0; i < 3; i++) {
+    free(s->dmat[i]); s->dmat[i] = calloc((size_t)(3 * o + 1), sizeof(double));
+    free(s->f3[i]); s->f3[i] = calloc((size_t)(3 * N), sizeof(double));
+  }
+  for (int i = 0; i < 5; i++) s->ddone[i] = 0;
   return FBO_OK;
 }
 
@@ -835,3 +854,700 @@ int fbo_assemble_vna_3c(fbo_system *s) {
   }
   return FBO_OK;
 }
+
+/* ================================================================ forces
+ * Derivative chain of k.DASSEMBLERS/M_Dassemble_2c_DOGS.f90 and
+ * M_Dassemble_3c_DOGS.f90.  Layout conventions (Fortran, column-major):
+ *   deps(ix, imu, k)         deps[(ix-1) + 3*(imu-1) + 9*(k-1)]
+ *   Dpmat(kx, i, j)          Dpmat[(kx-1) + 3*(i-1) + 9*(j-1)]
+ *   Ddmat(kx, i, j)          Ddmat[(kx-1) + 3*(i-1) + 15*(j-1)]
+ *   Dblock(ix, imu, inu)     D[(ix-1) + 3*((imu-1) + norb_mu*(inu-1))]
+ */
+#define EP(e, i, j) (e)[3 * ((j) - 1) + (i) - 1]
+#define DE(d, ix, imu, k) (d)[((ix) - 1) + 3 * ((imu) - 1) + 9 * ((k) - 1)]
+
+/* include/constants.h: xlevi (Levi-Civita) and delk (Kronecker) */
+static double levi(int i, int j, int k) {
+  if (i == j || j == k || i == k) return 0.0;
+  return ((i == 1 && j == 2) || (i == 2 && j == 3) || (i == 3 && j == 1)) ? 1.0 : -1.0;
+}
+static double delk(int i, int j) { return i == j ? 1.0 : 0.0; }
+
+/* b.FUNCTIONS/Dsmoother.f90 */
+double fbo_Dsmoother(double r, double rend, double x) {
+  double rbegin = x * rend;
+  if (r > rend) return 0.0;
+  if (r < rbegin) return 0.0;
+  double ratio = (r - rbegin) / (rend - rbegin);
+  return -(2.0 * (1.0 - ratio * ratio) * (2.0 * ratio) / (rend - rbegin));
+}
+
+/* b.FUNCTIONS/Depsilon_2c.f90: deps/dr1 of the 2c frame epsilon(r2, sighat),
+ * r1 and r2 absolute positions.  Returns 1 when the colinear guard zeroed it. */
+int fbo_Depsilon_2c(const double *r1, const double *r2, double z, const double *eps, double *deps) {
+  for (int i = 0; i < 27; i++) deps[i] = 0.0;
+  if (z < 1.0e-5) return 0;
+  double ca[3];
+  ca[0] = r2[1] * r1[2] - r2[2] * r1[1];
+  ca[1] = r2[2] * r1[0] - r2[0] * r1[2];
+  ca[2] = r2[0] * r1[1] - r2[1] * r1[0];
+  double crossmag = sqrt(ca[0] * ca[0] + ca[1] * ca[1] + ca[2] * ca[2]);
+  if (fabs(crossmag) < 1.0e-3) return 1;
+  double r2mag2 = r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2];
+  double r1dotr2 = r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2];
+  double iz = 1.0 / z, ic = 1.0 / crossmag;
+  for (int ix = 1; ix <= 3; ix++)
+    for (int imu = 1; imu <= 3; imu++) {
+      double sum = levi(ix, imu, 1) * r2[0] + levi(ix, imu, 2) * r2[1] + levi(ix, imu, 3) * r2[2];
+      DE(deps, ix, imu, 1) = (EP(eps, imu, 1) * EP(eps, ix, 3) * iz) -
+                             (EP(eps, imu, 1) * (ic * ic)) * (r2mag2 * r1[ix - 1] - r1dotr2 * r2[ix - 1]) +
+                             iz * ic *
+                                 (delk(imu, ix) * (r2mag2 - r1dotr2) - r1[imu - 1] * r2[ix - 1] -
+                                  r2[imu - 1] * r2[ix - 1] + 2.0 * r1[ix - 1] * r2[imu - 1]);
+      DE(deps, ix, imu, 2) = ic * (sum + (EP(eps, imu, 2) * ic) * (r1dotr2 * r2[ix - 1] - r2mag2 * r1[ix - 1]));
+      DE(deps, ix, imu, 3) = -((delk(imu, ix) - EP(eps, imu, 3) * EP(eps, ix, 3)) * iz);
+    }
+  return 0;
+}
+
+/* b.FUNCTIONS/Depsilon_3c.f90: dera3 = deps/dratm, der13 = deps/dr1 of the 3c
+ * frame epsilon(rhat, sighat).  Returns 1 when the colinear guard zeroed them. */
+int fbo_Depsilon_3c(const double *r1, const double *r2, const double *r21, double z, const double *ratm,
+                    const double *rnabc, const double *eps3, double *dera3, double *der13) {
+  for (int i = 0; i < 27; i++) { dera3[i] = 0.0; der13[i] = 0.0; }
+  double ca[3];
+  ca[0] = r21[1] * rnabc[2] - r21[2] * rnabc[1];
+  ca[1] = r21[2] * rnabc[0] - r21[0] * rnabc[2];
+  ca[2] = r21[0] * rnabc[1] - r21[1] * rnabc[0];
+  double crossmag = sqrt(ca[0] * ca[0] + ca[1] * ca[1] + ca[2] * ca[2]);
+  if (fabs(crossmag) < 1.0e-3) return 1;
+  double r2mag2 = r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2];
+  double r1mag2 = r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2];
+  double ratmmag2 = ratm[0] * ratm[0] + ratm[1] * ratm[1] + ratm[2] * ratm[2];
+  double r1dotr2 = r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2];
+  double r2dotratm = r2[0] * ratm[0] + r2[1] * ratm[1] + r2[2] * ratm[2];
+  double r1dotratm = r1[0] * ratm[0] + r1[1] * ratm[1] + r1[2] * ratm[2];
+  for (int ix = 1; ix <= 3; ix++)
+    for (int imu = 1; imu <= 3; imu++) {
+      DE(dera3, ix, imu, 3) = 0.0;
+      double sum = levi(ix, imu, 1) * (r2[0] - r1[0]) + levi(ix, imu, 2) * (r2[1] - r1[1]) +
+                   levi(ix, imu, 3) * (r2[2] - r1[2]);
+      DE(dera3, ix, imu, 2) =
+          (1.0 / crossmag) *
+          (sum - (EP(eps3, imu, 2) / crossmag) *
+                     ((r1mag2 + r2mag2 - 2.0 * r1dotr2) * ratm[ix - 1] +
+                      (r1dotr2 - r1mag2 - r2dotratm + r1dotratm) * r2[ix - 1] +
+                      (r1dotr2 - r2mag2 + r2dotratm - r1dotratm) * r1[ix - 1]));
+    }
+  for (int ix = 1; ix <= 3; ix++) {
+    DE(dera3, ix, 1, 1) = EP(eps3, 3, 3) * DE(dera3, ix, 2, 2) - EP(eps3, 2, 3) * DE(dera3, ix, 3, 2);
+    DE(dera3, ix, 2, 1) = EP(eps3, 1, 3) * DE(dera3, ix, 3, 2) - EP(eps3, 3, 3) * DE(dera3, ix, 1, 2);
+    DE(dera3, ix, 3, 1) = EP(eps3, 2, 3) * DE(dera3, ix, 1, 2) - EP(eps3, 1, 3) * DE(dera3, ix, 2, 2);
+  }
+  for (int ix = 1; ix <= 3; ix++)
+    for (int imu = 1; imu <= 3; imu++) {
+      DE(der13, ix, imu, 3) = (1.0 / z) * (EP(eps3, imu, 3) * EP(eps3, ix, 3) - delk(imu, ix));
+      double sum = levi(ix, imu, 1) * (ratm[0] - r2[0]) + levi(ix, imu, 2) * (ratm[1] - r2[1]) +
+                   levi(ix, imu, 3) * (ratm[2] - r2[2]);
+      DE(der13, ix, imu, 2) =
+          (1.0 / crossmag) *
+          (sum - (EP(eps3, imu, 2) / crossmag) *
+                     ((r2dotratm - r1dotratm + r1dotr2 - r2mag2) * ratm[ix - 1] +
+                      (r2dotratm + r1dotratm - r1dotr2 - ratmmag2) * r2[ix - 1] +
+                      (ratmmag2 - 2.0 * r2dotratm + r2mag2) * r1[ix - 1]));
+    }
+  for (int ix = 1; ix <= 3; ix++) {
+    DE(der13, ix, 1, 1) = EP(eps3, 3, 3) * DE(der13, ix, 2, 2) - EP(eps3, 3, 2) * DE(der13, ix, 2, 3) -
+                          EP(eps3, 2, 3) * DE(der13, ix, 3, 2) + EP(eps3, 2, 2) * DE(der13, ix, 3, 3);
+    DE(der13, ix, 2, 1) = EP(eps3, 1, 3) * DE(der13, ix, 3, 2) - EP(eps3, 1, 2) * DE(der13, ix, 3, 3) -
+                          EP(eps3, 3, 3) * DE(der13, ix, 1, 2) + EP(eps3, 3, 2) * DE(der13, ix, 1, 3);
+    DE(der13, ix, 3, 1) = EP(eps3, 2, 3) * DE(der13, ix, 1, 2) - EP(eps3, 2, 2) * DE(der13, ix, 1, 3) -
+                          EP(eps3, 1, 3) * DE(der13, ix, 2, 2) + EP(eps3, 1, 2) * DE(der13, ix, 2, 3);
+  }
+  return 0;
+}
+
+/* c.SYSTEM/M_Drotations.f90:203-298 (Dtwister): derivatives of the p and d
+ * rotation matrices. */
+void fbo_Dtwister(const double *eps, const double *deps, double *Ddmat, double *Dpmat) {
+#define DPM(kx, i, j) Dpmat[((kx) - 1) + 3 * ((i) - 1) + 9 * ((j) - 1)]
+#define DDM(kx, i, j) Ddmat[((kx) - 1) + 3 * ((i) - 1) + 15 * ((j) - 1)]
+  for (int kx = 1; kx <= 3; kx++) {
+    DPM(kx, 1, 1) = DE(deps, kx, 2, 2); DPM(kx, 1, 2) = DE(deps, kx, 2, 3); DPM(kx, 1, 3) = DE(deps, kx, 2, 1);
+    DPM(kx, 2, 1) = DE(deps, kx, 3, 2); DPM(kx, 2, 2) = DE(deps, kx, 3, 3); DPM(kx, 2, 3) = DE(deps, kx, 3, 1);
+    DPM(kx, 3, 1) = DE(deps, kx, 1, 2); DPM(kx, 3, 2) = DE(deps, kx, 1, 3); DPM(kx, 3, 3) = DE(deps, kx, 1, 1);
+  }
+  for (int imu = 1; imu <= 5; imu++)
+    for (int kx = 1; kx <= 3; kx++) {
+      double a12 = 0.0, a32 = 0.0, a33 = 0.0, a13 = 0.0, a11 = 0.0;
+      for (int ix = 1; ix <= 3; ix++)
+        for (int jx = 1; jx <= 3; jx++) {
+          double a = amat(jx, ix, imu);
+          if (a != 0.0) {
+            a12 = a12 + a * (DE(deps, kx, ix, 1) * EP(eps, jx, 2) + EP(eps, ix, 1) * DE(deps, kx, jx, 2));
+            a32 = a32 + a * (DE(deps, kx, ix, 3) * EP(eps, jx, 2) + EP(eps, ix, 3) * DE(deps, kx, jx, 2));
+            a33 = a33 + a * (DE(deps, kx, ix, 3) * EP(eps, jx, 3) + EP(eps, ix, 3) * DE(deps, kx, jx, 3));
+            a13 = a13 + a * (DE(deps, kx, ix, 1) * EP(eps, jx, 3) + EP(eps, ix, 1) * DE(deps, kx, jx, 3));
+            a11 = a11 + a * (DE(deps, kx, ix, 1) * EP(eps, jx, 1) + EP(eps, ix, 1) * DE(deps, kx, jx, 1));
+          }
+        }
+      DDM(kx, imu, 1) = 2.0 * a12;
+      DDM(kx, imu, 2) = 2.0 * a32;
+      DDM(kx, imu, 3) = sqrt(3.0) * a33;
+      DDM(kx, imu, 4) = 2.0 * a13;
+      DDM(kx, imu, 5) = 2.0 * a11 + a33;
+    }
+#undef DPM
+#undef DDM
+}
+
+/* c.SYSTEM/M_Drotations.f90:322-365 (Dchooser): 3x5x5 derivative rotation by l */
+static void Dchooser(int l, const double *Ddmat, const double *Dpmat, double *r) {
+  for (int i = 0; i < 75; i++) r[i] = 0.0;
+  if (l == 1) {
+    for (int j = 0; j < 3; j++)
+      for (int i = 0; i < 3; i++)
+        for (int kx = 0; kx < 3; kx++) r[kx + 3 * i + 15 * j] = Dpmat[kx + 3 * i + 9 * j];
+  } else if (l == 2) {
+    for (int i = 0; i < 75; i++) r[i] = Ddmat[i];
+  }
+}
+
+/* c.SYSTEM/M_Drotations.f90:102-153 (Drotate) and :386-478 (Dmatrix):
+ * Dmatx(ix) = dL/dr M R^T + L M dR^T/dr + L dM/dr R^T shell pair by shell pair,
+ * loop order m2, k2, m1, k1, ix. */
+void fbo_Drotate(const fbo_tables *t, int in1, int in2, const double *eps, const double *deps, int norb_mu,
+                 int norb_nu, const double *matm, const double *Dmatm, double *Dmatx) {
+  double dmat[25] = {0}, pmat[9], Ddmat[75], Dpmat[27];
+  double left[25], right[25], Dleft[75], Dright[75];
+  fbo_twister(eps, t->have_d, dmat, pmat);
+  fbo_Dtwister(eps, deps, Ddmat, Dpmat);
+  int n1 = 0;
+  for (int issh = 0; issh < t->nssh[in1 - 1]; issh++) {
+    int l1 = t->lssh[in1 - 1][issh];
+    chooser(l1, dmat, pmat, left);
+    Dchooser(l1, Ddmat, Dpmat, Dleft);
+    int n2 = 0;
+    for (int jssh = 0; jssh < t->nssh[in2 - 1]; jssh++) {
+      int l2 = t->lssh[in2 - 1][jssh];
+      chooser(l2, dmat, pmat, right);
+      Dchooser(l2, Ddmat, Dpmat, Dright);
+      for (int k2 = 1; k2 <= 2 * l2 + 1; k2++)
+        for (int k1 = 1; k1 <= 2 * l1 + 1; k1++)
+          for (int ix = 0; ix < 3; ix++) Dmatx[ix + 3 * ((n1 + k1 - 1) + norb_mu * (n2 + k2 - 1))] = 0.0;
+      for (int m2 = 1; m2 <= 2 * l2 + 1; m2++)
+        for (int k2 = 1; k2 <= 2 * l2 + 1; k2++)
+          for (int m1 = 1; m1 <= 2 * l1 + 1; m1++)
+            for (int k1 = 1; k1 <= 2 * l1 + 1; k1++)
+              for (int ix = 0; ix < 3; ix++) {
+                double *term = &Dmatx[ix + 3 * ((n1 + k1 - 1) + norb_mu * (n2 + k2 - 1))];
+                double mm = matm[(n1 + m1 - 1) + norb_mu * (n2 + m2 - 1)];
+                double dm = Dmatm[ix + 3 * ((n1 + m1 - 1) + norb_mu * (n2 + m2 - 1))];
+                double lf = left[(k1 - 1) + 5 * (m1 - 1)], rt = right[(k2 - 1) + 5 * (m2 - 1)];
+                double dl = Dleft[ix + 3 * (k1 - 1) + 15 * (m1 - 1)], dr = Dright[ix + 3 * (k2 - 1) + 15 * (m2 - 1)];
+                *term = *term + dl * rt * mm + lf * dr * mm + lf * rt * dm;
+              }
+      n2 += 2 * l2 + 1;
+    }
+    n1 += 2 * l1 + 1;
+  }
+}
+
+/* e.FDATA/M_Fdata_2c.f90:446-600: 7-point Lagrange value and x-derivative.
+ * Unlike the value routine, x < 1e-5 returns point 1 with a zero derivative. */
+int fbo_getDMEs_2c(const fbo_tables *t, int is, int js, int iint, int isub, double x, int norb_mu,
+                   int norb_nu, double *hmbox, double *Dhmbox) {
+  const double P_tolerance = 1.0e-5;
+  const cell2c *c = C2(t, is, js, iint, isub);
+  if (!c) return FBO_E_NO_TABLE;
+  if (x < 0.0) return FBO_E_X_NEGATIVE;
+  if (x > c->xmax * (1.0 + P_tolerance)) return FBO_E_X_OUT_OF_RANGE;
+  for (int i = 0; i < norb_mu * norb_nu; i++) { hmbox[i] = 0.0; Dhmbox[i] = 0.0; }
+  if (x < P_tolerance) {
+    for (int k = 0; k < c->nME; k++) hmbox[(c->nu[k] - 1) * norb_mu + (c->mu[k] - 1)] = c->data[k];
+    return FBO_OK;
+  }
+  int imid = (int)(x / c->dx) + 1;
+  int ileft = imid - 3, iright = imid + 3;
+  if (ileft < 1) { ileft = 1; iright = 7; }
+  else if (iright > c->nx) { ileft = c->nx - 6; iright = c->nx; }
+  double xx[7], pdenom[7], F[64], dF[64];
+  for (int ip = ileft; ip <= iright; ip++) xx[ip - ileft] = (double)(ip - 1) * c->dx;
+  for (int k = 0; k < c->nME; k++) { F[k] = 0.0; dF[k] = 0.0; }
+  for (int isum = ileft; isum <= iright; isum++) {
+    double prod = 1.0;
+    pdenom[isum - ileft] = 1.0;
+    for (int iprod = ileft; iprod <= iright; iprod++) {
+      if (iprod != isum) {
+        double pden = 1.0 / (xx[isum - ileft] - xx[iprod - ileft]);
+        pdenom[isum - ileft] = pdenom[isum - ileft] * pden;
+        prod = prod * (x - xx[iprod - ileft]) * pden;
+      }
+    }
+    const double *row = c->data + (size_t)(isum - 1) * c->nME;
+    for (int k = 0; k < c->nME; k++) F[k] = F[k] + row[k] * prod;
+  }
+  for (int k = 0; k < c->nME; k++) hmbox[(c->nu[k] - 1) * norb_mu + (c->mu[k] - 1)] = F[k];
+  for (int ipoint = ileft; ipoint <= iright; ipoint++) {
+    double xsum = 0.0;
+    for (int jpoint = ileft; jpoint <= iright; jpoint++) {
+      if (ipoint == jpoint) continue;
+      double xprod = 1.0;
+      for (int kpoint = ileft; kpoint <= iright; kpoint++) {
+        if (kpoint == jpoint || kpoint == ipoint) continue;
+        xprod = xprod * (x - xx[kpoint - ileft]);
+      }
+      xsum = xsum + xprod;
+    }
+    const double *row = c->data + (size_t)(ipoint - 1) * c->nME;
+    for (int k = 0; k < c->nME; k++) dF[k] = dF[k] + row[k] * pdenom[ipoint - ileft] * xsum;
+  }
+  for (int k = 0; k < c->nME; k++) Dhmbox[(c->nu[k] - 1) * norb_mu + (c->mu[k] - 1)] = dF[k];
+  return FBO_OK;
+}
+
+/* e.FDATA/M_Fdata_3c.f90:549-846: value and derivatives (cos theta, x, y) of
+ * the 3c interpolation.  The 3-point branch takes the x/y slopes of ME 1 of
+ * theta cell 1 for every matrix element (:641-642); the 6-point branch cannot
+ * be taken (P_small = 1.0d0-4 = -3 while gg >= 0). */
+int fbo_getDMEs_3c(const fbo_tables *t, int is, int js, int ks, int iint, int isub, double x, double y,
+                   int norb_mu, int norb_nu, double cost, double *hmbox, double *Dphmbox, double *Dxhmbox,
+                   double *Dyhmbox) {
+  const double P_tiny = 1.0e-6;
+  double p[5], Dp[5];
+  double cost2 = cost * cost;
+  double sint = sqrt(fmax(1.0e-5, 1.0 - cost2));
+  p[0] = 1.0;
+  p[1] = cost;
+  p[2] = 0.5 * (3.0 * cost2 - 1.0);
+  p[3] = 0.5 * cost * (5.0 * cost2 - 3.0);
+  p[4] = 0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2));
+  Dp[0] = 0.0;
+  Dp[1] = 1.0;
+  Dp[2] = 3.0 * cost;
+  Dp[3] = (15.0 * cost2 - 3.0) / 2.0;
+  Dp[4] = 0.5 * cost * (35.0 * cost2 - 15.0);
+  const cell3c *c1 = C3(t, is, js, ks, iint, isub, 1);
+  if (!c1) return FBO_E_NO_TABLE;
+  int nME = c1->nME, nx = c1->nx, ny = c1->ny;
+  double F[64], DpF[64], DxF[64], DyF[64];
+  for (int k = 0; k < nME; k++) { F[k] = 0.0; DpF[k] = 0.0; DxF[k] = 0.0; DyF[k] = 0.0; }
+  double dx = c1->xmax / (nx - 1.0), dy = c1->ymax / (ny - 1.0);
+  int ix = (int)(x / dx) + 1, iy = (int)(y / dy) + 1;
+  if (ix < 2) ix = 2;
+  if (iy < 2) iy = 2;
+  if (ix > nx - 2) ix = nx - 2;
+  if (iy > ny - 2) iy = ny - 2;
+  double px = x / dx - ix + 1.0, py = y / dy - iy + 1.0;
+#define T3(c, i, j, m) (c)->data[((size_t)((j) - 1) * nx + ((i) - 1)) * nME + (m)]
+  double gx = (T3(c1, ix + 1, iy, 0) - T3(c1, ix, iy, 0)) / dx;
+  double gy = (T3(c1, ix, iy + 1, 0) - T3(c1, ix, iy, 0)) / dy;
+  double gg = gx * gx + gy * gy;
+  const cell3c *c = c1;
+  for (int ith = 1; ith <= FBO_MAXTHETA; ith++) {
+    c = C3(t, is, js, ks, iint, isub, ith);
+    if (!c) return FBO_E_NO_TABLE;
+    double pt = p[ith - 1], dpt = Dp[ith - 1];
+    for (int m = 0; m < nME; m++) {
+      if (gg < P_tiny) {
+        double interim = ((1.0 - px - py) * T3(c, ix, iy, m) + px * T3(c, ix + 1, iy, m)) + py * T3(c, ix, iy + 1, m);
+        F[m] = F[m] + pt * interim;
+        DpF[m] = DpF[m] + dpt * interim;
+        DxF[m] = DxF[m] + pt * gx;
+        DyF[m] = DyF[m] + pt * gy;
+      } else {
+        double g[4], gp[4];
+        for (int k = -1; k <= 2; k++) {
+          double f1m1 = T3(c, ix + k, iy - 1, m);
+          double f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+          double f0p3 = 3.0 * T3(c, ix + k, iy, m), f0p6 = 2.0 * f0p3;
+          double f1p3 = 3.0 * T3(c, ix + k, iy + 1, m), f1p6 = 2.0 * f1p3;
+          double f2p1 = T3(c, ix + k, iy + 2, m);
+          double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+          double bb2 = f1m3 - f0p6 + f1p3;
+          double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+          double bb0 = f0p6;
+          g[k + 1] = ((bb3 * py + bb2) * py + bb1) * py + bb0;
+          gp[k + 1] = ((3.0 * bb3 * py + 2.0 * bb2) * py + bb1);
+        }
+        double f1m1 = g[0], f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+        double f0p3 = 3.0 * g[1], f0p6 = 2.0 * f0p3;
+        double f1p3 = 3.0 * g[2], f1p6 = 2.0 * f1p3;
+        double f2p1 = g[3];
+        double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+        double bb2 = f1m3 - f0p6 + f1p3;
+        double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+        double bb0 = f0p6;
+        double interim = (((bb3 * px + bb2) * px + bb1) * px + bb0) / 36.0;
+        F[m] = F[m] + pt * interim;
+        DpF[m] = DpF[m] + dpt * interim;
+        DxF[m] = DxF[m] + pt * ((3.0 * bb3 * px + 2.0 * bb2) * px + bb1) / (36.0 * dx);
+        f1m1 = gp[0]; f1m2 = 2.0 * f1m1; f1m3 = 3.0 * f1m1;
+        f0p3 = 3.0 * gp[1]; f0p6 = 2.0 * f0p3;
+        f1p3 = 3.0 * gp[2]; f1p6 = 2.0 * f1p3;
+        f2p1 = gp[3];
+        bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+        bb2 = f1m3 - f0p6 + f1p3;
+        bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+        bb0 = f0p6;
+        DyF[m] = DyF[m] + pt * (((bb3 * px + bb2) * px + bb1) * px + bb0) / (36.0 * dy);
+      }
+    }
+  }
+#undef T3
+  for (int i = 0; i < norb_mu * norb_nu; i++) { hmbox[i] = 0.0; Dphmbox[i] = 0.0; Dxhmbox[i] = 0.0; Dyhmbox[i] = 0.0; }
+  for (int k = 0; k < c->nME; k++) {
+    int o = (c->nu[k] - 1) * norb_mu + (c->mu[k] - 1);
+    if (c->mv[k] != 1) {
+      hmbox[o] = F[k]; Dphmbox[o] = DpF[k]; Dxhmbox[o] = DxF[k]; Dyhmbox[o] = DyF[k];
+    } else {
+      hmbox[o] = F[k] * sint;
+      Dphmbox[o] = DpF[k] * sint - cost * F[k] / sint;
+      Dxhmbox[o] = DxF[k] * sint;
+      Dyhmbox[o] = DyF[k] * sint;
+    }
+  }
+  return FBO_OK;
+}
+
+int fbo_set_denmat(fbo_system *s, const double *rho) {
+  if (!s->neighn) return FBO_E_ORDER;
+  free(s->rho);
+  s->rho = malloc(sizeof(double) * (size_t)(s->blen ? s->blen : 1));
+  memcpy(s->rho, rho, sizeof(double) * (size_t)s->blen);
+  return FBO_OK;
+}
+
+/* frame and its r1-derivative of a pair (sighat along r2 - r1, epsilon(r2, sighat)) */
+static double pair_dframe(fbo_system *s, const double *r1, const double *r2, double *eps, double *deps,
+                          double *eta) {
+  double z = fbo_distance(r1, r2), sighat[3];
+  if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+  else for (int k = 0; k < 3; k++) sighat[k] = (r2[k] - r1[k]) / z;
+  (void)epsilon_impl(r2, sighat, eps);
+  s->ndwarn += fbo_Depsilon_2c(r1, r2, z, eps, deps);
+  for (int k = 0; k < 3; k++) eta[k] = eps[6 + k];
+  return z;
+}
+
+/* k.DASSEMBLERS/M_Dassemble_2c_DOGS.f90:95-266 (S), :292-463 (T), :487-655 (dipole_z) */
+static int dassemble_2c_simple(fbo_system *s, int interaction, int d) {
+  const fbo_tables *t = s->t;
+  if (!s->neighn) return FBO_E_ORDER;
+  double sm[81], dsm[81], vdsm[243], eps[9], deps[27], eta[3];
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1];
+    for (int in = 0; in < s->neighn[ia - 1]; in++) {
+      int p = s->noff[ia - 1] + in;
+      int mb = s->neigh_b[p], ja = s->neigh_j[p];
+      double r2[3];
+      for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+      int in2 = s->imass[ja - 1], nnu = t->norb[in2 - 1], nb = nmu * nnu;
+      double z = pair_dframe(s, r1, r2, eps, deps, eta);
+      int rc = fbo_getDMEs_2c(t, in1, in2, interaction, 0, z, nmu, nnu, sm, dsm);
+      if (rc) return rc;
+      for (int i = 0; i < 3 * nb; i++) vdsm[i] = 0.0;
+      for (int i = 0; i < nb; i++)
+        if (z > 1.0e-3)
+          for (int k = 0; k < 3; k++) vdsm[k + 3 * i] = -eta[k] * dsm[i];
+      fbo_Drotate(t, in1, in2, eps, deps, nmu, nnu, sm, vdsm, s->dmat[d] + 3 * s->boff[p]);
+    }
+  }
+  s->ddone[d] = 1;
+  return FBO_OK;
+}
+
+int fbo_Dassemble_S(fbo_system *s) { return dassemble_2c_simple(s, P_OVERLAP, 0); }
+int fbo_Dassemble_T(fbo_system *s) { return dassemble_2c_simple(s, P_KINETIC, 1); }
+int fbo_Dassemble_dipole_z(fbo_system *s) { return dassemble_2c_simple(s, P_DIPOLE_Z, 2); }
+
+/* one (interaction, isorp) term of the ontop forces: f -= rho (dQ) vdx P_eq2 */
+static int ontop_term(fbo_system *s, int in1, int in2, int iint, int isorp, double z, const double *eps,
+                      const double *deps, const double *eta, const double *rho, double dQ, int use_dq, double *f) {
+  const fbo_tables *t = s->t;
+  int nmu = t->norb[in1 - 1], nnu = t->norb[in2 - 1], nb = nmu * nnu;
+  double bm[81], dbm[81], vdbm[243], vdx[243];
+  int rc = fbo_getDMEs_2c(t, in1, in2, iint, isorp, z, nmu, nnu, bm, dbm);
+  if (rc) return rc;
+  for (int i = 0; i < 3 * nb; i++) vdbm[i] = 0.0;
+  for (int i = 0; i < nb; i++)
+    if (z > 1.0e-3)
+      for (int k = 0; k < 3; k++) vdbm[k + 3 * i] = -eta[k] * dbm[i];
+  fbo_Drotate(t, in1, in2, eps, deps, nmu, nnu, bm, vdbm, vdx);
+  for (int i = 0; i < nb; i++)
+    for (int k = 0; k < 3; k++) {
+      if (use_dq) f[k] = f[k] - rho[i] * dQ * vdx[k + 3 * i] * P_EQ2;
+      else f[k] = f[k] - rho[i] * vdx[k + 3 * i] * P_EQ2;
+    }
+  return FBO_OK;
+}
+
+/* k.DASSEMBLERS/M_Dassemble_2c_DOGS.f90:679-1199: forces%vna_ontop (non-self
+ * pairs, ontop L and R with their charged shells, contracted with the pair's
+ * density block) and forces%vna_atom (every pair, contracted with the density
+ * block of neigh_self, smoothed monopole for the charged shells). */
+int fbo_Dassemble_vna_2c(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  if (!s->neighn || !s->rho || !s->done[FBO_MAT_S]) return FBO_E_ORDER;
+  double eps[9], deps[27], eta[3];
+  memset(s->fatom, 0, sizeof(double) * 3 * s->npairs);
+  memset(s->fontop, 0, sizeof(double) * 3 * s->npairs);
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    int in1 = s->imass[ia - 1];
+    for (int in = 0; in < s->neighn[ia - 1]; in++) {
+      int p = s->noff[ia - 1] + in;
+      int mb = s->neigh_b[p], ja = s->neigh_j[p];
+      double r2[3];
+      for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+      int in2 = s->imass[ja - 1];
+      double z = pair_dframe(s, r1, r2, eps, deps, eta);
+      if (ia == ja && mb == 0) continue;
+      const double *rho = s->rho + s->boff[p];
+      double *f = s->fontop + 3 * p;
+      int rc = ontop_term(s, in1, in2, P_VNA_ONTOPL, 0, z, eps, deps, eta, rho, 0.0, 0, f);
+      if (rc) return rc;
+      for (int isorp = 1; isorp <= t->nssh[in1 - 1]; isorp++) {
+        rc = ontop_term(s, in1, in2, P_VNA_ONTOPL, isorp, z, eps, deps, eta, rho,
+                        s->dQ[(ia - 1) * FBO_MAXSH + isorp - 1], 1, f);
+        if (rc) return rc;
+      }
+      rc = ontop_term(s, in1, in2, P_VNA_ONTOPR, 0, z, eps, deps, eta, rho, 0.0, 0, f);
+      if (rc) return rc;
+      for (int isorp = 1; isorp <= t->nssh[in2 - 1]; isorp++) {
+        rc = ontop_term(s, in1, in2, P_VNA_ONTOPR, isorp, z, eps, deps, eta, rho,
+                        s->dQ[(ja - 1) * FBO_MAXSH + isorp - 1], 1, f);
+        if (rc) return rc;
+      }
+    }
+  }
+  double bm[81], dbm[81], bx[81], vdbm[243], vdx[243], emnpl[81], vdemnpl[243];
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1], nb = nmu * nmu;
+    int pself = s->noff[ia - 1] + s->neigh_self[ia - 1] - 1;
+    const double *rho = s->rho + s->boff[pself];
+    const double *sself = s->mat[FBO_MAT_S] + s->boff[pself];
+    for (int in = 0; in < s->neighn[ia - 1]; in++) {
+      int p = s->noff[ia - 1] + in;
+      int mb = s->neigh_b[p], ja = s->neigh_j[p];
+      double r2[3];
+      for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+      int in2 = s->imass[ja - 1];
+      double z = pair_dframe(s, r1, r2, eps, deps, eta);
+      double rend = rcut_min(t, in1) + rcut_min(t, in2);
+      double smooth = fbo_smoother(z, rend, 0.8);
+      double Dsmooth = fbo_Dsmoother(z, rend, 0.8);
+      double *f = s->fatom + 3 * p;
+      int rc = fbo_getDMEs_2c(t, in1, in2, P_VNA_ATOM, 0, z, nmu, nmu, bm, dbm);
+      if (rc) return rc;
+      for (int i = 0; i < 3 * nb; i++) vdbm[i] = 0.0;
+      for (int i = 0; i < nb; i++)
+        if (z > 1.0e-3)
+          for (int k = 0; k < 3; k++) vdbm[k + 3 * i] = -eta[k] * dbm[i];
+      fbo_Drotate(t, in1, in1, eps, deps, nmu, nmu, bm, vdbm, vdx);
+      for (int i = 0; i < nb; i++)
+        for (int k = 0; k < 3; k++) f[k] = f[k] - rho[i] * vdx[k + 3 * i] * P_EQ2;
+      for (int isorp = 1; isorp <= t->nssh[in2 - 1]; isorp++) {
+        double dQ = s->dQ[(ja - 1) * FBO_MAXSH + isorp - 1];
+        rc = fbo_getDMEs_2c(t, in1, in2, P_VNA_ATOM, isorp, z, nmu, nmu, bm, dbm);
+        if (rc) return rc;
+        for (int i = 0; i < nb; i++) emnpl[i] = 0.0;
+        for (int i = 0; i < 3 * nb; i++) { vdbm[i] = 0.0; vdemnpl[i] = 0.0; }
+        for (int i = 0; i < nb; i++)
+          if (z > 1.0e-4) {
+            for (int k = 0; k < 3; k++) vdbm[k + 3 * i] = -eta[k] * dbm[i];
+            emnpl[i] = sself[i] / z;
+            for (int k = 0; k < 3; k++) vdemnpl[k + 3 * i] = eta[k] * sself[i] / (z * z);
+          }
+        fbo_rotate(t, in1, in1, eps, nmu, nmu, bm, bx);
+        fbo_Drotate(t, in1, in1, eps, deps, nmu, nmu, bm, vdbm, vdx);
+        for (int i = 0; i < nb; i++)
+          for (int k = 0; k < 3; k++)
+            f[k] = f[k] - rho[i] * P_EQ2 * dQ *
+                              (smooth * vdx[k + 3 * i] - eta[k] * Dsmooth * bx[i] + (1.0 - smooth) * vdemnpl[k + 3 * i] +
+                               eta[k] * Dsmooth * emnpl[i]);
+      }
+    }
+  }
+  s->ddone[3] = 1;
+  return FBO_OK;
+}
+
+/* triplet geometry of M_Dassemble_3c_DOGS.f90:305-362 (neutral loop) and
+ * :495-562 (charged loop: rhat from rna - (r1 + r2)/2 instead of rnabc) */
+typedef struct {
+  double r1[3], r2[3], r21[3], rna[3], rnabc[3], sighat[3], rhat[3], amt[3], bmt[3];
+  double z, x, cost, d13, d23;
+  double eps[9], depsA[27], depsB[27];
+} dgeo3;
+
+static void dgeo3_build(fbo_system *s, int ial, int ia, int ib, int ja, int jb, int charged, dgeo3 *G) {
+  for (int k = 0; k < 3; k++) {
+    G->r1[k] = RAT(s, ia)[k] + XL(s, ib)[k];
+    G->r2[k] = RAT(s, ja)[k] + XL(s, jb)[k];
+    G->rna[k] = RAT(s, ial)[k];
+    G->r21[k] = G->r2[k] - G->r1[k];
+  }
+  G->z = fbo_distance(G->r1, G->r2);
+  if (G->z < 1.0e-5) { G->sighat[0] = 0.0; G->sighat[1] = 0.0; G->sighat[2] = 1.0; }
+  else for (int k = 0; k < 3; k++) G->sighat[k] = G->r21[k] / G->z;
+  for (int k = 0; k < 3; k++) G->rnabc[k] = G->rna[k] - (G->r1[k] + G->r21[k] / 2.0);
+  G->x = sqrt(G->rnabc[0] * G->rnabc[0] + G->rnabc[1] * G->rnabc[1] + G->rnabc[2] * G->rnabc[2]);
+  G->d13 = fbo_distance(G->rna, G->r1);
+  G->d23 = fbo_distance(G->rna, G->r2);
+  if (G->x < 1.0e-5) { G->rhat[0] = 0.0; G->rhat[1] = 0.0; G->rhat[2] = 0.0; }
+  else if (charged) for (int k = 0; k < 3; k++) G->rhat[k] = (G->rna[k] - 0.5 * (G->r1[k] + G->r2[k])) / G->x;
+  else for (int k = 0; k < 3; k++) G->rhat[k] = G->rnabc[k] / G->x;
+  G->cost = G->sighat[0] * G->rhat[0] + G->sighat[1] * G->rhat[1] + G->sighat[2] * G->rhat[2];
+  (void)epsilon_impl(G->rhat, G->sighat, G->eps);
+  s->ndwarn += fbo_Depsilon_3c(G->r1, G->r2, G->r21, G->z, G->rna, G->rnabc, G->eps, G->depsA, G->depsB);
+  for (int k = 0; k < 3; k++) {
+    G->amt[k] = (G->x > 1.0e-5) ? (G->sighat[k] - G->cost * G->rhat[k]) / G->x : 0.0;
+    G->bmt[k] = (G->cost * G->sighat[k] - G->rhat[k]) / G->z;
+  }
+}
+
+/* f3naMa / f3naMb of the matrix elements of the cell (M_Dassemble_3c_DOGS.f90:388-399),
+ * rotated with depsA / depsB; f3naXc by Newton's third law */
+static int dforce3_terms(fbo_system *s, int in1, int in2, int indna, int isorp, const dgeo3 *G, double *bm,
+                         double *bx, double *Xa, double *Xb, double *Xc) {
+  const fbo_tables *t = s->t;
+  int nmu = t->norb[in1 - 1], nnu = t->norb[in2 - 1], nb = nmu * nnu;
+  double dp[81], dxm[81], dym[81], Ma[243], Mb[243];
+  int rc = fbo_getDMEs_3c(t, in1, in2, indna, P_BCNA, isorp, G->x, G->z, nmu, nnu, G->cost, bm, dp, dxm, dym);
+  if (rc) return rc;
+  if (bx) fbo_rotate(t, in1, in2, G->eps, nmu, nnu, bm, bx);
+  for (int i = 0; i < 3 * nb; i++) { Ma[i] = 0.0; Mb[i] = 0.0; }
+  const cell3c *c = C3(t, in1, in2, indna, P_BCNA, isorp, 1);
+  for (int k = 0; k < c->nME; k++) {
+    int o = (c->nu[k] - 1) * nmu + (c->mu[k] - 1);
+    for (int q = 0; q < 3; q++) Ma[q + 3 * o] = G->rhat[q] * dxm[o] + G->amt[q] * dp[o];
+    for (int q = 0; q < 3; q++) Mb[q + 3 * o] = -G->sighat[q] * dym[o] + G->bmt[q] * dp[o] - Ma[q + 3 * o] / 2.0;
+  }
+  fbo_Drotate(t, in1, in2, G->eps, G->depsA, nmu, nnu, bm, Ma, Xa);
+  fbo_Drotate(t, in1, in2, G->eps, G->depsB, nmu, nnu, bm, Mb, Xb);
+  for (int i = 0; i < 3 * nb; i++) Xc[i] = -Xa[i] - Xb[i];
+  return FBO_OK;
+}
+
+/* k.DASSEMBLERS/M_Dassemble_3c_DOGS.f90:136-793: f3naa (on ialpha), f3nab (on
+ * iatom) and f3nac (on jatom), neutral loop over all triplets first, then the
+ * charged loop with the smoothed monopole terms. */
+int fbo_Dassemble_vna_3c(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  if (!s->neighn || !s->rho || !s->done[FBO_MAT_S] || !s->done[FBO_MAT_DIP] || !s->ddone[0] || !s->ddone[2])
+    return FBO_E_ORDER;
+  double bm[81], bx[81], Xa[243], Xb[243], Xc[243];
+  for (int i = 0; i < 3; i++) memset(s->f3[i], 0, sizeof(double) * 3 * s->natoms);
+  for (int ial = 1; ial <= s->natoms; ial++) {
+    int indna = s->imass[ial - 1];
+    for (int in = 0; in < s->ncommon[ial - 1]; in++) {
+      int c = s->coff[ial - 1] + in;
+      int mneigh = s->cm[c];
+      if (mneigh == 0) continue;
+      int ia = s->ci_j[c], ja = s->cj_j[c];
+      int in1 = s->imass[ia - 1], in2 = s->imass[ja - 1];
+      int nmu = t->norb[in1 - 1], nnu = t->norb[in2 - 1];
+      const double *rho = s->rho + s->boff[s->noff[ia - 1] + mneigh - 1];
+      dgeo3 G;
+      dgeo3_build(s, ial, ia, s->ci_b[c], ja, s->cj_b[c], 0, &G);
+      int rc = dforce3_terms(s, in1, in2, indna, 0, &G, bm, NULL, Xa, Xb, Xc);
+      if (rc) return rc;
+      double *fa = s->f3[0] + 3 * (ial - 1), *fb = s->f3[1] + 3 * (ia - 1), *fc = s->f3[2] + 3 * (ja - 1);
+      for (int inu = 0; inu < nnu; inu++)
+        for (int imu = 0; imu < nmu; imu++) {
+          int o = imu + nmu * inu;
+          for (int k = 0; k < 3; k++) fa[k] = fa[k] - rho[o] * Xa[k + 3 * o] * P_EQ2;
+          for (int k = 0; k < 3; k++) fb[k] = fb[k] - rho[o] * Xb[k + 3 * o] * P_EQ2;
+          for (int k = 0; k < 3; k++) fc[k] = fc[k] - rho[o] * Xc[k + 3 * o] * P_EQ2;
+        }
+    }
+  }
+  double emnpl[81], dA[243], dB[243], dC[243];
+  for (int ial = 1; ial <= s->natoms; ial++) {
+    int indna = s->imass[ial - 1];
+    for (int in = 0; in < s->ncommon[ial - 1]; in++) {
+      int c = s->coff[ial - 1] + in;
+      int mneigh = s->cm[c];
+      if (mneigh == 0) continue;
+      int ia = s->ci_j[c], ja = s->cj_j[c];
+      int in1 = s->imass[ia - 1], in2 = s->imass[ja - 1];
+      int nmu = t->norb[in1 - 1], nnu = t->norb[in2 - 1], nb = nmu * nnu;
+      int64_t bo = s->boff[s->noff[ia - 1] + mneigh - 1];
+      const double *rho = s->rho + bo;
+      const double *sb = s->mat[FBO_MAT_S] + bo, *db = s->mat[FBO_MAT_DIP] + bo;
+      const double *sD = s->dmat[0] + 3 * bo, *dD = s->dmat[2] + 3 * bo;
+      dgeo3 G;
+      dgeo3_build(s, ial, ia, s->ci_b[c], ja, s->cj_b[c], 1, &G);
+      double rhatA1[3] = {0, 0, 0}, rhatA2[3] = {0, 0, 0};
+      if (G.d13 > 1.0e-5) for (int k = 0; k < 3; k++) rhatA1[k] = (G.rna[k] - G.r1[k]) / G.d13;
+      if (G.d23 > 1.0e-5) for (int k = 0; k < 3; k++) rhatA2[k] = (G.rna[k] - G.r2[k]) / G.d23;
+      double rend1 = rcut_min(t, in1) + rcut_min(t, indna), rend2 = rcut_min(t, in2) + rcut_min(t, indna);
+      double st13 = fbo_smoother(G.d13, rend1, 0.8), dst13 = fbo_Dsmoother(G.d13, rend1, 0.8);
+      double st23 = fbo_smoother(G.d23, rend2, 0.8), dst23 = fbo_Dsmoother(G.d23, rend2, 0.8);
+      double stinky = st13 * st23;
+      double dstnA[3], dstnB[3], dstnC[3];
+      for (int k = 0; k < 3; k++) {
+        dstnB[k] = -dst13 * st23 * rhatA1[k];
+        dstnC[k] = -st13 * dst23 * rhatA2[k];
+        dstnA[k] = -dstnB[k] - dstnC[k];
+      }
+      double d13sq = G.d13 * G.d13, d23sq = G.d23 * G.d23;
+      for (int i = 0; i < nb; i++) {
+        double sterm = sb[i] / 2.0, dterm = db[i] / G.z;
+        emnpl[i] = (sterm - dterm) / G.d13 + (sterm + dterm) / G.d23;
+        for (int k = 0; k < 3; k++) {
+          double spterm = sD[k + 3 * i] / 2.0;
+          double dpterm = dD[k + 3 * i] / G.z;
+          dpterm = dpterm + dterm * G.sighat[k] / G.z;
+          dA[k + 3 * i] = -((sterm - dterm) * rhatA1[k] / d13sq) - (sterm + dterm) * rhatA2[k] / d23sq;
+          dB[k + 3 * i] = (sterm - dterm) * rhatA1[k] / d13sq + (spterm - dpterm) / G.d13 + (spterm + dpterm) / G.d23;
+          dC[k + 3 * i] = -dA[k + 3 * i] - dB[k + 3 * i];
+        }
+      }
+      double *fa = s->f3[0] + 3 * (ial - 1), *fb = s->f3[1] + 3 * (ia - 1), *fc = s->f3[2] + 3 * (ja - 1);
+      for (int isorp = 1; isorp <= t->nssh[indna - 1]; isorp++) {
+        int rc = dforce3_terms(s, in1, in2, indna, isorp, &G, bm, bx, Xa, Xb, Xc);
+        if (rc) return rc;
+        double dQ = s->dQ[(ial - 1) * FBO_MAXSH + isorp - 1];
+        for (int inu = 0; inu < nnu; inu++)
+          for (int imu = 0; imu < nmu; imu++) {
+            int o = imu + nmu * inu;
+            double w = P_EQ2 * dQ * rho[o];
+            for (int k = 0; k < 3; k++)
+              fa[k] = fa[k] - w * (stinky * Xa[k + 3 * o] + dstnA[k] * bx[o] - dstnA[k] * emnpl[o] +
+                                   (1.0 - stinky) * dA[k + 3 * o]);
+            for (int k = 0; k < 3; k++)
+              fb[k] = fb[k] - w * (stinky * Xb[k + 3 * o] + dstnB[k] * bx[o] - dstnB[k] * emnpl[o] +
+                                   (1.0 - stinky) * dB[k + 3 * o]);
+            for (int k = 0; k < 3; k++)
+              fc[k] = fc[k] - w * (stinky * Xc[k + 3 * o] + dstnC[k] * bx[o] - dstnC[k] * emnpl[o] +
+                                   (1.0 - stinky) * dC[k + 3 * o]);
+          }
+      }
+    }
+  }
+  s->ddone[4] = 1;
+  return FBO_OK;
+}
+
+int fbo_get_dblocks(const fbo_system *s, int which, double *out) {
+  if (which < 0 || which > 2 || !s->dmat[which]) return FBO_E_ARG;
+  memcpy(out, s->dmat[which], sizeof(double) * 3 * (size_t)s->blen);
+  return FBO_OK;
+}
+
+int fbo_get_forces(const fbo_system *s, double *vna_atom, double *vna_ontop, double *f3naa, double *f3nab,
+                   double *f3nac) {
+  if (!s->neighn) return FBO_E_ORDER;
+  if (vna_atom) memcpy(vna_atom, s->fatom, sizeof(double) * 3 * (size_t)s->npairs);
+  if (vna_ontop) memcpy(vna_ontop, s->fontop, sizeof(double) * 3 * (size_t)s->npairs);
+  if (f3naa) memcpy(f3naa, s->f3[0], sizeof(double) * 3 * (size_t)s->natoms);
+  if (f3nab) memcpy(f3nab, s->f3[1], sizeof(double) * 3 * (size_t)s->natoms);
+  if (f3nac) memcpy(f3nac, s->f3[2], sizeof(double) * 3 * (size_t)s->natoms);
+  return FBO_OK;
+}
+
+int fbo_deps_warnings(const fbo_system *s) { return s->ndwarn; }

@@ -75,4 +75,30 @@ int fbo_assemble_vna_3c(fbo_system *s);
 int fbo_get_blocks(const fbo_system *s, int matrix, double *out);
 int fbo_frame_fallbacks(const fbo_system *s);
 
+/* forces (k.DASSEMBLERS): derivative primitives.  deps layouts are Fortran
+ * column-major deps(ix, imu, k); Dblocks are Dblock(ix, imu, inu) per pair. */
+double fbo_Dsmoother(double r, double rend, double x);
+int fbo_Depsilon_2c(const double *r1, const double *r2, double z, const double *eps, double *deps);
+int fbo_Depsilon_3c(const double *r1, const double *r2, const double *r21, double z, const double *ratm,
+                    const double *rnabc, const double *eps3, double *dera3, double *der13);
+void fbo_Dtwister(const double *eps, const double *deps, double *Ddmat, double *Dpmat);
+void fbo_Drotate(const fbo_tables *t, int in1, int in2, const double *eps, const double *deps, int norb_mu,
+                 int norb_nu, const double *matm, const double *Dmatm, double *Dmatx);
+int fbo_getDMEs_2c(const fbo_tables *t, int is, int js, int iint, int isub, double x, int norb_mu,
+                   int norb_nu, double *hmbox, double *Dhmbox);
+int fbo_getDMEs_3c(const fbo_tables *t, int is, int js, int ks, int iint, int isub, double x, double y,
+                   int norb_mu, int norb_nu, double cost, double *hmbox, double *Dphmbox, double *Dxhmbox,
+                   double *Dyhmbox);
+/* forces: density matrix in, Dblocks and force pieces out */
+int fbo_set_denmat(fbo_system *s, const double *rho);
+int fbo_Dassemble_S(fbo_system *s);
+int fbo_Dassemble_T(fbo_system *s);
+int fbo_Dassemble_dipole_z(fbo_system *s);
+int fbo_Dassemble_vna_2c(fbo_system *s);
+int fbo_Dassemble_vna_3c(fbo_system *s);
+int fbo_get_dblocks(const fbo_system *s, int which, double *out);
+int fbo_get_forces(const fbo_system *s, double *vna_atom, double *vna_ontop, double *f3naa, double *f3nab,
+                   double *f3nac);
+int fbo_deps_warnings(const fbo_system *s);
+
 #endif

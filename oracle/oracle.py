@@ -59,6 +59,21 @@ def lib():
                   "fbo_assemble_vna_3c", "fbo_frame_fallbacks"):
             getattr(L, f).argtypes = [vp]
         L.fbo_get_blocks.argtypes = [vp, C.c_int, dp]
+        L.fbo_Dsmoother.restype = C.c_double
+        L.fbo_Dsmoother.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.fbo_Depsilon_2c.argtypes = [dp, dp, C.c_double, dp, dp]
+        L.fbo_Depsilon_3c.argtypes = [dp, dp, dp, C.c_double, dp, dp, dp, dp, dp]
+        L.fbo_Dtwister.argtypes = [dp, dp, dp, dp]
+        L.fbo_Drotate.argtypes = [vp, C.c_int, C.c_int, dp, dp, C.c_int, C.c_int, dp, dp, dp]
+        L.fbo_getDMEs_2c.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, dp, dp]
+        L.fbo_getDMEs_3c.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                     C.c_int, C.c_int, C.c_double, dp, dp, dp, dp]
+        L.fbo_set_denmat.argtypes = [vp, dp]
+        for f in ("fbo_Dassemble_S", "fbo_Dassemble_T", "fbo_Dassemble_dipole_z", "fbo_Dassemble_vna_2c",
+                  "fbo_Dassemble_vna_3c", "fbo_deps_warnings"):
+            getattr(L, f).argtypes = [vp]
+        L.fbo_get_dblocks.argtypes = [vp, C.c_int, dp]
+        L.fbo_get_forces.argtypes = [vp, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -120,6 +135,31 @@ class Tables:
         _check(lib().fbo_getMEs_3c(self.h, is_, js, ks, iint, isub, x, y, nmu, nnu, cost, _dp(out)), "getMEs_3c")
         return out.reshape(nnu, nmu).T
 
+    def getDMEs_2c(self, is_, js, iint, isub, x, nmu, nnu):
+        """(value, d/dx) blocks of getDMEs_Fdata_2c."""
+        h, d = np.zeros(nmu * nnu), np.zeros(nmu * nnu)
+        _check(lib().fbo_getDMEs_2c(self.h, is_, js, iint, isub, x, nmu, nnu, _dp(h), _dp(d)), "getDMEs_2c")
+        return h.reshape(nnu, nmu).T, d.reshape(nnu, nmu).T
+
+    def getDMEs_3c(self, is_, js, ks, iint, isub, x, y, nmu, nnu, cost):
+        """(value, d/dcos, d/dx, d/dy) blocks of getDMEs_Fdata_3c."""
+        out = [np.zeros(nmu * nnu) for _ in range(4)]
+        _check(lib().fbo_getDMEs_3c(self.h, is_, js, ks, iint, isub, x, y, nmu, nnu, cost, *(_dp(o) for o in out)),
+               "getDMEs_3c")
+        return tuple(o.reshape(nnu, nmu).T for o in out)
+
+    def Drotate(self, in1, in2, eps, deps, M, DM):
+        """Drotate: M (nmu, nnu), DM and deps as (3, ...) arrays indexed [ix, ...];
+        returns Dmatx[ix, imu, inu]."""
+        nmu, nnu = M.shape
+        out = np.zeros(3 * nmu * nnu)
+        e = np.asfortranarray(eps, np.float64).ravel(order="F")
+        de = np.asfortranarray(deps, np.float64).ravel(order="F")
+        Mf = np.asfortranarray(M, np.float64).ravel(order="F")
+        DMf = np.asfortranarray(DM, np.float64).ravel(order="F")
+        lib().fbo_Drotate(self.h, in1, in2, _dp(e), _dp(de), nmu, nnu, _dp(Mf), _dp(DMf), _dp(out))
+        return out.reshape(nnu, nmu, 3).transpose(2, 1, 0)
+
     def rotate(self, in1, in3, eps, M):
         nmu, nnu = M.shape
         Mf = np.asfortranarray(M, np.float64)
@@ -145,6 +185,28 @@ def twister(eps, have_d=True):
 
 def smoother(r, rend, x):
     return lib().fbo_smoother(r, rend, x)
+
+
+def Dsmoother(r, rend, x):
+    return lib().fbo_Dsmoother(r, rend, x)
+
+
+def Depsilon_2c(r1, r2, z, eps):
+    """deps[ix, imu, k] = d eps(imu, k) / d r1(ix); second value: colinear guard taken."""
+    d = np.zeros(27)
+    e = np.asfortranarray(eps, np.float64).ravel(order="F")
+    w = lib().fbo_Depsilon_2c(_dp(np.ascontiguousarray(r1, np.float64)), _dp(np.ascontiguousarray(r2, np.float64)),
+                              z, _dp(e), _dp(d))
+    return d.reshape(3, 3, 3).transpose(2, 1, 0), bool(w)
+
+
+def Depsilon_3c(r1, r2, r21, z, ratm, rnabc, eps):
+    """(d eps/d ratm, d eps/d r1) as [ix, imu, k] arrays, and the colinear flag."""
+    a, b = np.zeros(27), np.zeros(27)
+    c = lambda v: _dp(np.ascontiguousarray(v, np.float64))  # noqa: E731
+    e = np.asfortranarray(eps, np.float64).ravel(order="F")
+    w = lib().fbo_Depsilon_3c(c(r1), c(r2), c(r21), z, c(ratm), c(rnabc), _dp(e), _dp(a), _dp(b))
+    return a.reshape(3, 3, 3).transpose(2, 1, 0), b.reshape(3, 3, 3).transpose(2, 1, 0), bool(w)
 
 
 def make_cells(lattice, icluster=0):
@@ -209,6 +271,42 @@ class System:
         out = np.zeros(max(n, 1))
         _check(lib().fbo_get_blocks(self.h, MATS[mat], _dp(out)), "get_blocks")
         return out[:n]
+
+    def set_denmat(self, rho):
+        rho = np.ascontiguousarray(rho, np.float64)
+        assert rho.size == lib().fbo_block_len(self.h)
+        self._rho = rho
+        _check(lib().fbo_set_denmat(self.h, _dp(rho)), "set_denmat")
+
+    def dassemble(self, which=("S", "T", "DIP", "VNA2C", "VNA3C")):
+        L = lib()
+        fn = {"S": L.fbo_Dassemble_S, "T": L.fbo_Dassemble_T, "DIP": L.fbo_Dassemble_dipole_z,
+              "VNA2C": L.fbo_Dassemble_vna_2c, "VNA3C": L.fbo_Dassemble_vna_3c}
+        for w in which:
+            _check(fn[w](self.h), "Dassemble_" + w)
+
+    def dblocks(self, mat):
+        n = 3 * lib().fbo_block_len(self.h)
+        out = np.zeros(max(n, 1))
+        _check(lib().fbo_get_dblocks(self.h, {"S": 0, "T": 1, "DIP": 2}[mat], _dp(out)), "get_dblocks")
+        return out[:n]
+
+    def forces(self):
+        L = lib()
+        npairs, n = L.fbo_npairs(self.h), self.st.natoms
+        fa, fo = np.zeros((max(npairs, 1), 3)), np.zeros((max(npairs, 1), 3))
+        f3 = [np.zeros((n, 3)) for _ in range(3)]
+        _check(L.fbo_get_forces(self.h, _dp(fa), _dp(fo), *(_dp(x) for x in f3)), "get_forces")
+        return {"vna_atom": fa[:npairs], "vna_ontop": fo[:npairs], "f3naa": f3[0], "f3nab": f3[1], "f3nac": f3[2]}
+
+    def run_forces(self, rho):
+        """Dassemble_S/T/dipole_z/vna_2c/vna_3c after run_all(); Dblocks and forces."""
+        self.set_denmat(rho)
+        self.dassemble()
+        res = {"D" + m: self.dblocks(m) for m in ("S", "T", "DIP")}
+        res.update(self.forces())
+        res["deps_warnings"] = lib().fbo_deps_warnings(self.h)
+        return res
 
     def run_all(self):
         lists = self.build_lists()

@@ -103,3 +103,63 @@ def run(ts, structure, binary=HARNESS_REF, nrep=1, workdir=None, timeout=3600):
         import shutil
         shutil.rmtree(tmp, ignore_errors=True)
     return res
+
+
+HARNESS_DREF = os.path.join(HERE, "_ref", "fb_dharness_ref")
+
+
+def parse_forces(path, species, imass, neighn, neigh_j):
+    """Dblocks (S, T, DIP) in pair order as flat [3*block] arrays, vna_atom and
+    vna_ontop as [npairs, 3], f3naa/f3nab/f3nac as [natoms, 3]."""
+    with open(path, "rb") as f:
+        r = _Reader(f.read())
+    natoms, npairs, ntrip, nrep = r.i32(4)
+    times = r.f64(5)
+    norb = np.array([sp.norb for sp in species], np.int64)
+    D = {k: [] for k in ("DS", "DT", "DDIP")}
+    p = 0
+    for ia in range(natoms):
+        nmu = norb[imass[ia] - 1]
+        for _ in range(int(neighn[ia])):
+            nnu = norb[imass[neigh_j[p] - 1] - 1]
+            for k in ("DS", "DT", "DDIP"):
+                D[k].append(r.f64(int(3 * nmu * nnu)))
+            p += 1
+    fa, fo = [], []
+    for ia in range(natoms):
+        n = int(neighn[ia])
+        fa.append(r.f64(3 * n).reshape(n, 3))
+        fo.append(r.f64(3 * n).reshape(n, 3))
+    f3 = r.f64(9 * natoms).reshape(natoms, 3, 3)
+    assert r.pos == len(r.buf), "trailing bytes in force harness output"
+    out = {k: (np.concatenate(v) if v else np.zeros(0)) for k, v in D.items()}
+    out["vna_atom"] = np.concatenate(fa) if fa else np.zeros((0, 3))
+    out["vna_ontop"] = np.concatenate(fo) if fo else np.zeros((0, 3))
+    out["f3naa"], out["f3nab"], out["f3nac"] = f3[:, 0].copy(), f3[:, 1].copy(), f3[:, 2].copy()
+    out["times"] = dict(zip(("DS", "DT", "DDIP", "DVNA2C", "DVNA3C"), times))
+    out["nrep"] = int(nrep)
+    out["ntriplets"] = int(ntrip)
+    return out
+
+
+def run_forces(ts, structure, rho, lists, binary=HARNESS_DREF, nrep=1, workdir=None, timeout=3600):
+    """Run the force harness on (tables, structure) with density blocks rho
+    (pair order of `lists`, the reference's neighbour lists)."""
+    own = workdir is None
+    tmp = tempfile.mkdtemp(prefix="fbdref_") if own else workdir
+    fdir = os.path.join(tmp, "Fdata")
+    if not os.path.isdir(fdir):
+        fdata_io.write_fdata(ts, fdir)
+    inp = os.path.join(tmp, "harness.in")
+    fdata_io.write_harness_input(inp, fdir, ts, structure)
+    rhof = os.path.join(tmp, "denmat.bin")
+    np.ascontiguousarray(rho, np.float64).tofile(rhof)
+    outp = os.path.join(tmp, "dharness.out")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run([binary, inp, rhof, outp, str(nrep)], cwd=tmp, check=True, env=env,
+                   stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=timeout)
+    res = parse_forces(outp, ts.species, structure.imass, lists["neighn"], lists["neigh_j"])
+    if own:
+        import shutil
+        shutil.rmtree(tmp, ignore_errors=True)
+    return res

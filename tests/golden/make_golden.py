@@ -1,8 +1,9 @@
 """Generate the golden fixtures from the reference's own Fortran (run in the
 build container, where the reference sources are present).
 
-    make -C oracle ref            # compile the reference modules + harness
-    python tests/golden/make_golden.py
+    make -C oracle ref dref       # compile the reference modules + harnesses
+    python tests/golden/make_golden.py            # H/S fixtures and forces
+    python tests/golden/make_golden.py --forces   # force fixtures only
 
 Each fixture is an .npz holding the inputs that are not re-derivable from a
 seed (explicit positions for the hand-made cases), the reference's neighbour
@@ -86,9 +87,45 @@ def cases():
     return out
 
 
+DENMAT_SEED = 11
+DKEYS = ("DS", "DT", "DDIP")
+FKEYS = ("vna_atom", "vna_ontop", "f3naa", "f3nab", "f3nac")
+
+
+def forces_fixture(name, ts, st, keep_atoms):
+    """<name>_forces.npz: the reference's Dassemble_* outputs for the synthetic
+    density matrix synth.denmat(seed DENMAT_SEED) on the reference's lists;
+    Dblocks of the pairs of the first keep_atoms atoms, checksums of all."""
+    r = refrun.run(ts, st)
+    rho = synth.denmat(ts.species, st.imass, r["neighn"], r["neigh_j"], r["neigh_back"], seed=DENMAT_SEED)
+    f = refrun.run_forces(ts, st, rho, r)
+    norb = np.array([sp.norb for sp in ts.species])
+    pair_atom = np.repeat(np.arange(st.natoms), r["neighn"])
+    sizes = 3 * norb[st.imass[pair_atom] - 1] * norb[st.imass[r["neigh_j"] - 1] - 1]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    npair_keep = len(sizes) if keep_atoms is None else int(np.sum(r["neighn"][:keep_atoms]))
+    d = {"denmat_seed": np.int64(DENMAT_SEED), "npair_dblocks": np.int32(npair_keep)}
+    for k in DKEYS:
+        d[k] = f[k][: offs[npair_keep]]
+        d[k + "_sum"] = np.float64(np.sum(f[k]))
+        d[k + "_sumsq"] = np.float64(np.sum(f[k] ** 2))
+        d[k + "_sumabs"] = np.float64(np.sum(np.abs(f[k])))
+    for k in FKEYS:
+        d[k] = f[k]
+    np.savez_compressed(os.path.join(HERE, f"{name}_forces.npz"), **d)
+    print(f"{name}_forces: Dblocks kept for {npair_keep} pairs")
+
+
 def main():
     if not refrun.available():
         sys.exit("reference harness not built: run `make -C oracle ref` first")
+    want_forces = os.path.isfile(refrun.HARNESS_DREF)
+    if "--forces" in sys.argv:
+        if not want_forces:
+            sys.exit("force harness not built: run `make -C oracle dref` first")
+        for name, ts, st, keep_atoms in cases():
+            forces_fixture(name, ts, st, keep_atoms if keep_atoms is not None else (10 if st.natoms > 32 else None))
+        return
     for name, ts, st, keep_atoms in cases():
         r = refrun.run(ts, st)
         d = {k: np.asarray(r[k], np.int32) for k in LIST_KEYS}
@@ -119,6 +156,8 @@ def main():
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **d)
         print(f"{name}: natoms={st.natoms} pairs={r['npairs']} triplets={r['ntriplets']} "
               f"blocks kept for {npair_keep} pairs")
+        if want_forces:
+            forces_fixture(name, ts, st, keep_atoms if keep_atoms is not None else (10 if st.natoms > 32 else None))
 
 
 if __name__ == "__main__":

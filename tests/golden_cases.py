@@ -15,3 +15,4 @@ def inputs(g):
     st = synth.Structure(np.array(g["ratom"]), np.array(g["imass"], np.int32), np.array(g["lattice"]),
                          int(g["icluster"]), dQ=dq)
     return ts, st
+

@@ -447,3 +447,36 @@ def random_dQ(structs, species, seed=7, scale=0.05):
             dq[ia, :sp_.nssh] = rng.uniform(-scale, scale, size=sp_.nssh)
         s.dQ = dq
     return structs
+
+
+def denmat(species, imass, neighn, neigh_j, neigh_back, seed=11, scale=0.5):
+    """Synthetic density matrix for the force assemblers: one block per pair in
+    the pair order of the neighbour lists (iatom major), column-major
+    norb_mu x norb_nu, symmetric under the pair reversal of neigh_back
+    (rho(j, i) = rho(i, j)^T), entries uniform in [-scale, scale] scaled by
+    exp(-|pair index distance| / 8) so that the blocks are not all alike."""
+    norb = np.array([sp.norb for sp in species], np.int64)
+    imass = np.asarray(imass)
+    neighn = np.asarray(neighn, np.int64)
+    neigh_j = np.asarray(neigh_j, np.int64)
+    neigh_back = np.asarray(neigh_back, np.int64)
+    noff = np.concatenate([[0], np.cumsum(neighn)])
+    pair_atom = np.repeat(np.arange(len(neighn)), neighn)
+    nmu = norb[imass[pair_atom] - 1]
+    nnu = norb[imass[neigh_j - 1] - 1]
+    boff = np.concatenate([[0], np.cumsum(nmu * nnu)])
+    back = noff[neigh_j - 1] + neigh_back - 1
+    rng = np.random.default_rng(seed)
+    rho = np.zeros(int(boff[-1]))
+    for p in range(len(neigh_j)):
+        q = int(back[p])
+        if q < p:
+            continue
+        a, b = int(nmu[p]), int(nnu[p])
+        blk = rng.uniform(-scale, scale, size=(a, b)) * np.exp(-abs(int(neigh_j[p]) - 1 - int(pair_atom[p])) / 8.0)
+        if q == p:
+            blk = 0.5 * (blk + blk.T)
+        rho[boff[p]:boff[p + 1]] = blk.ravel(order="F")
+        if q != p:
+            rho[boff[q]:boff[q + 1]] = blk.T.ravel(order="F")
+    return rho
