@@ -139,6 +139,29 @@ int fb_get_block_len(fb_ctx *ctx, int istruct, int64_t *len);
 int fb_get_blocks(fb_ctx *ctx, int istruct, int matrix, double *out);
 int fb_get_device_blocks(fb_ctx *ctx, int matrix, void **dptr, int64_t *len);
 
+/* --- forces: the k.DASSEMBLERS force assemblers --------------------------
+ * Call order as in the reference's MD driver (n.NAMD/M_nonadiabatic_mdet.f90:
+ * 2065-2101): the H+S assembly above, fb_set_denmat for every structure, then
+ * fb_dassemble_S, _T, _dipole_z, _vna_2c, _vna_3c (or fb_dassemble_all).
+ * The density matrix (s%denmat) comes in the block layout of fb_get_blocks;
+ * a new list build discards it.  Outputs:
+ *   Dblocks      Dblock(3, norb_mu, norb_nu) per pair, pair order, 3 x the
+ *                block length (fb_get_dblocks; matrix FB_MAT_S/T/DIP)
+ *   vna_atom     forces%vna_atom(3, ineigh) per pair, [npairs][3]
+ *   vna_ontop    forces%vna_ontop(3, ineigh) per pair, [npairs][3]
+ *   f3naa/b/c    forces%f3naa / f3nab / f3nac per atom, [natoms][3]
+ * The force kernels compute in binary64 whatever fb_set_precision says. */
+int fb_set_denmat(fb_ctx *ctx, int istruct, const double *rho);  /* s%denmat(i)%neighbors(k)%block */
+int fb_dassemble_S(fb_ctx *ctx);          /* k.DASSEMBLERS/M_Dassemble_2c_DOGS.f90:95-266   */
+int fb_dassemble_T(fb_ctx *ctx);          /* k.DASSEMBLERS/M_Dassemble_2c_DOGS.f90:292-463  */
+int fb_dassemble_dipole_z(fb_ctx *ctx);   /* k.DASSEMBLERS/M_Dassemble_2c_DOGS.f90:487-655  */
+int fb_dassemble_vna_2c(fb_ctx *ctx);     /* k.DASSEMBLERS/M_Dassemble_2c_DOGS.f90:679-1199 */
+int fb_dassemble_vna_3c(fb_ctx *ctx);     /* k.DASSEMBLERS/M_Dassemble_3c_DOGS.f90:136-793  */
+int fb_dassemble_all(fb_ctx *ctx);        /* all five: one 2c launch, the 3c launch, per-atom sums */
+int fb_get_dblocks(fb_ctx *ctx, int istruct, int matrix, double *out);
+int fb_get_forces(fb_ctx *ctx, int istruct, double *vna_atom, double *vna_ontop, double *f3naa, double *f3nab,
+                  double *f3nac);
+
 /* --- consumer: Gamma-point eigenvalues ------------------------------------
  * Folds the blocks of structure istruct at k = 0 into a dense binary64 matrix
  * (build_Hmatrix, h.SOLVESH/M_kspace_Lowdin.f90:180-271, 630-724) -- which = 0:

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "fb_internal.h"
+#include "fb_forces.h"
 #include "fb_mlp.h"
 #include "fb_fdata.h"
 #include <array>
@@ -226,6 +227,12 @@ struct fb_ctx {
   bool lists_ok = false;
   int done[5] = {0, 0, 0, 0, 0};
 
+  // forces (k.DASSEMBLERS): density matrix in, Dblocks and force pieces out
+  DevBuf d_rho, d_dmat[3], d_fatom, d_fontop, d_f3t, d_f3, d_jkey, d_jkey_s, d_iota_t, d_perm_j, d_ioff, d_joff;
+  long forces_gen = -1;          // lists_gen the force buffers and orderings belong to
+  std::vector<char> rho_ok;      // per structure: density blocks uploaded
+  int ddone[5] = {0, 0, 0, 0, 0};  // Dblock S, T, dipole_z; vna_2c forces; vna_3c forces
+
   // deferred status of asynchronous H+S builds
   int *h_status = nullptr;      // pinned
   bool pending = false;
@@ -262,6 +269,24 @@ struct fb_ctx {
       err += hipGetErrorString(e);
     }
     return code;
+  }
+
+  DForce fview() const {
+    DForce f{};
+    f.rho = d_rho.as<double>();
+    for (int i = 0; i < 3; i++) f.dmat[i] = d_dmat[i].as<double>();
+    f.fatom = d_fatom.as<double>();
+    f.fontop = d_fontop.as<double>();
+    f.f3t = d_f3t.as<double>();
+    f.f3 = d_f3.as<double>();
+    f.trip = d_trip_orig.as<int4>();
+    f.tpair = d_tpair_orig.as<int>();
+    f.toff = d_toff.as<int64_t>();
+    f.perm = d_perm.as<int>();
+    f.ioff = d_ioff.as<int64_t>();
+    f.perm_j = d_perm_j.as<int>();
+    f.joff = d_joff.as<int64_t>();
+    return f;
   }
 
   DView view() const {
@@ -412,6 +437,9 @@ struct fb_ctx {
                       &d_status};
     for (DevBuf *b : bufs) b->release();
     for (auto &m : d_mat) m.release();
+    DevBuf *fbufs[] = {&d_rho, &d_dmat[0], &d_dmat[1], &d_dmat[2], &d_fatom, &d_fontop, &d_f3t, &d_f3, &d_jkey,
+                       &d_jkey_s, &d_iota_t, &d_perm_j, &d_ioff, &d_joff};
+    for (DevBuf *b : fbufs) b->release();
     reset_events();
     for (auto e : pool) (void)hipEventDestroy(e);
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -871,6 +899,7 @@ int fb_set_charges(fb_ctx *ctx, int is, const double *dQ) {
   // only the Vna blocks depend on the shell charges: S, T and dipole_z stay valid,
   // as in a self-consistent loop that re-assembles Vna per charge update
   ctx->done[FB_MAT_VNA] = ctx->done[FB_MAT_VNAO] = 0;
+  ctx->ddone[3] = ctx->ddone[4] = 0;   // the Vna force pieces too
   return FB_OK;
 }
 
@@ -1819,6 +1848,182 @@ int fb_get_device_blocks(fb_ctx *ctx, int matrix, void **dptr, int64_t *len) {
   if (!ctx || !ctx->lists_ok || matrix < 0 || matrix > 4) return FB_E_ARG;
   if (dptr) *dptr = ctx->d_mat[matrix].p;
   if (len) *len = ctx->blen;
+  return FB_OK;
+}
+
+// ------------------------------------------------------------------ forces
+// Buffers of the force assemblers and the two triplet orderings of the
+// per-atom sums (iatom: the target-pair order of the lists; jatom: a stable
+// radix sort of the list order by jatom), built once per list generation.
+static int prep_forces(fb_ctx *ctx) {
+  if (ctx->forces_gen == ctx->lists_gen) return FB_OK;
+  hipStream_t st = ctx->stream;
+  const int64_t P = ctx->npairs, T = ctx->ntrip, B = ctx->blen;
+  const int N = ctx->natoms;
+  auto one = [](int64_t n) { return (size_t)std::max<int64_t>(n, 1); };
+  FB_HIP(ctx->d_rho.ensure(sizeof(double) * one(B)));
+  for (int i = 0; i < 3; i++) FB_HIP(ctx->d_dmat[i].ensure(sizeof(double) * one(3 * B)));
+  FB_HIP(ctx->d_fatom.ensure(sizeof(double) * one(3 * P)));
+  FB_HIP(ctx->d_fontop.ensure(sizeof(double) * one(3 * P)));
+  FB_HIP(ctx->d_f3t.ensure(sizeof(double) * one(9 * T)));
+  FB_HIP(ctx->d_f3.ensure(sizeof(double) * one(9 * (int64_t)N)));
+  FB_HIP(ctx->d_ioff.ensure(sizeof(int64_t) * (N + 1)));
+  FB_HIP(ctx->d_joff.ensure(sizeof(int64_t) * (N + 1)));
+  FB_HIP(ctx->d_jkey.ensure(sizeof(int) * one(T)));
+  FB_HIP(ctx->d_jkey_s.ensure(sizeof(int) * one(T)));
+  FB_HIP(ctx->d_iota_t.ensure(sizeof(int) * one(T)));
+  FB_HIP(ctx->d_perm_j.ensure(sizeof(int) * one(T)));
+  if (T > 0) {
+    RC(ctx->launch("trip_jkey", [&] { return fbk_trip_jkey(T, ctx->d_trip_orig.as<int4>(), ctx->d_jkey.as<int>(), st); }));
+    RC(ctx->launch("iota", [&] { return fbk_iota(ctx->d_iota_t.as<int>(), T, st); }));
+    int end_bit = 1;
+    while ((1ll << end_bit) <= N) end_bit++;
+    size_t bytes = 0;
+    int rc = fbk_sort_pairs(ctx->d_jkey.as<int>(), ctx->d_jkey_s.as<int>(), ctx->d_iota_t.as<int>(),
+                            ctx->d_perm_j.as<int>(), T, end_bit, nullptr, &bytes, st);
+    if (rc) return ctx->fail(FB_E_HIP, "jatom sort sizing", (hipError_t)rc);
+    RC(ensure_tmp(ctx, bytes));
+    bytes = ctx->d_tmp.cap;
+    RC(ctx->launch("sort_jatom", [&] {
+      return fbk_sort_pairs(ctx->d_jkey.as<int>(), ctx->d_jkey_s.as<int>(), ctx->d_iota_t.as<int>(),
+                            ctx->d_perm_j.as<int>(), T, end_bit, ctx->d_tmp.p, &bytes, st);
+    }));
+  }
+  RC(ctx->launch("f3_offsets", [&] {
+    return fbk_f3_offsets(N, T, ctx->d_noff.as<int64_t>(), ctx->d_tpair.as<int>(), ctx->d_jkey_s.as<int>(),
+                          ctx->d_ioff.as<int64_t>(), ctx->d_joff.as<int64_t>(), st);
+  }));
+  FB_HIP(hipStreamSynchronize(st));
+  ctx->rho_ok.assign(ctx->structs.size(), 0);
+  for (int i = 0; i < 5; i++) ctx->ddone[i] = 0;
+  ctx->forces_gen = ctx->lists_gen;
+  return FB_OK;
+}
+
+int fb_set_denmat(fb_ctx *ctx, int is, const double *rho) {
+  if (!ctx) return FB_E_ARG;
+  if (!ctx->lists_ok) return ctx->fail(FB_E_ORDER, "set_denmat before build_lists");
+  if (is < 0 || is >= (int)ctx->structs.size() || !rho) return ctx->fail(FB_E_ARG, "set_denmat arguments");
+  FB_HIP(hipSetDevice(ctx->device));
+  RC(ctx->settle("previous assemble_all"));
+  RC(prep_forces(ctx));
+  const int64_t b0 = ctx->h_boff_struct[is], b1 = ctx->h_boff_struct[is + 1];
+  FB_HIP(hipMemcpyAsync(ctx->d_rho.as<double>() + b0, rho, sizeof(double) * (b1 - b0), hipMemcpyHostToDevice,
+                        ctx->stream));
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->rho_ok[is] = 1;
+  ctx->ddone[3] = ctx->ddone[4] = 0;
+  return FB_OK;
+}
+
+static int pre_dassemble(fb_ctx *ctx, const char *what) {
+  RC(pre_assemble(ctx, what));
+  RC(prep_forces(ctx));
+  return FB_OK;
+}
+
+static bool all_rho(const fb_ctx *ctx) {
+  for (char c : ctx->rho_ok)
+    if (!c) return false;
+  return !ctx->rho_ok.empty();
+}
+
+static int dassemble_2c(fb_ctx *ctx, int bit, const char *what) {
+  if (!ctx) return FB_E_ARG;
+  RC(pre_dassemble(ctx, what));
+  DView v = ctx->view();
+  DForce f = ctx->fview();
+  RC(ctx->launch(what, [&] { return fbk_d2c(v, f, 1 << bit, ctx->stream); }));
+  RC(ctx->check_status(what));
+  ctx->ddone[bit] = 1;
+  return FB_OK;
+}
+
+int fb_dassemble_S(fb_ctx *ctx) { return dassemble_2c(ctx, 0, "Dassemble_S"); }
+int fb_dassemble_T(fb_ctx *ctx) { return dassemble_2c(ctx, 1, "Dassemble_T"); }
+int fb_dassemble_dipole_z(fb_ctx *ctx) { return dassemble_2c(ctx, 2, "Dassemble_dipole_z"); }
+
+int fb_dassemble_vna_2c(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  RC(pre_dassemble(ctx, "Dassemble_vna_2c"));
+  if (!all_rho(ctx)) return ctx->fail(FB_E_ORDER, "Dassemble_vna_2c needs the density matrix (fb_set_denmat)");
+  if (!ctx->done[FB_MAT_S]) return ctx->fail(FB_E_ORDER, "Dassemble_vna_2c needs assemble_S first");
+  DView v = ctx->view();
+  DForce f = ctx->fview();
+  RC(ctx->launch("Dassemble_vna_2c", [&] { return fbk_d2c(v, f, 8, ctx->stream); }));
+  RC(ctx->check_status("Dassemble_vna_2c"));
+  ctx->ddone[3] = 1;
+  return FB_OK;
+}
+
+static int run_d3c(fb_ctx *ctx) {
+  DView v = ctx->view();
+  DForce f = ctx->fview();
+  RC(ctx->launch("Dassemble_vna_3c", [&] { return fbk_d3c(v, f, ctx->stream); }));
+  RC(ctx->launch("forces_3c_sum", [&] { return fbk_f3_reduce(v, f, ctx->stream); }));
+  return FB_OK;
+}
+
+int fb_dassemble_vna_3c(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  RC(pre_dassemble(ctx, "Dassemble_vna_3c"));
+  if (!all_rho(ctx)) return ctx->fail(FB_E_ORDER, "Dassemble_vna_3c needs the density matrix (fb_set_denmat)");
+  if (!ctx->done[FB_MAT_S] || !ctx->done[FB_MAT_DIP] || !ctx->ddone[0] || !ctx->ddone[2])
+    return ctx->fail(FB_E_ORDER,
+                     "Dassemble_vna_3c needs assemble_S, assemble_dipole_z, Dassemble_S and Dassemble_dipole_z first");
+  RC(run_d3c(ctx));
+  RC(ctx->check_status("Dassemble_vna_3c"));
+  ctx->ddone[4] = 1;
+  return FB_OK;
+}
+
+int fb_dassemble_all(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  RC(pre_dassemble(ctx, "Dassemble_all"));
+  if (!all_rho(ctx)) return ctx->fail(FB_E_ORDER, "Dassemble_all needs the density matrix (fb_set_denmat)");
+  if (!ctx->done[FB_MAT_S] || !ctx->done[FB_MAT_DIP])
+    return ctx->fail(FB_E_ORDER, "Dassemble_all needs assemble_S and assemble_dipole_z first");
+  DView v = ctx->view();
+  DForce f = ctx->fview();
+  RC(ctx->launch("Dassemble_2c", [&] { return fbk_d2c(v, f, 15, ctx->stream); }));
+  RC(run_d3c(ctx));
+  RC(ctx->check_status("Dassemble_all"));
+  for (int i = 0; i < 5; i++) ctx->ddone[i] = 1;
+  return FB_OK;
+}
+
+int fb_get_dblocks(fb_ctx *ctx, int is, int matrix, double *out) {
+  if (!ctx || !ctx->lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "lists not built") : FB_E_ARG;
+  if (is < 0 || is >= (int)ctx->structs.size() || matrix < 0 || matrix > 2 || !out)
+    return ctx->fail(FB_E_ARG, "get_dblocks arguments");
+  if (ctx->forces_gen != ctx->lists_gen || !ctx->ddone[matrix])
+    return ctx->fail(FB_E_ORDER, "get_dblocks: Dblocks not assembled");
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  const int64_t b0 = ctx->h_boff_struct[is], b1 = ctx->h_boff_struct[is + 1];
+  return d2h(ctx, out, ctx->d_dmat[matrix].as<double>() + 3 * b0, sizeof(double) * 3 * (b1 - b0));
+}
+
+int fb_get_forces(fb_ctx *ctx, int is, double *vna_atom, double *vna_ontop, double *f3naa, double *f3nab,
+                  double *f3nac) {
+  if (!ctx || !ctx->lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "lists not built") : FB_E_ARG;
+  if (is < 0 || is >= (int)ctx->structs.size()) return ctx->fail(FB_E_ARG, "get_forces arguments");
+  if (ctx->forces_gen != ctx->lists_gen) return ctx->fail(FB_E_ORDER, "get_forces: forces not assembled");
+  if ((vna_atom || vna_ontop) && !ctx->ddone[3]) return ctx->fail(FB_E_ORDER, "get_forces: Dassemble_vna_2c not run");
+  if ((f3naa || f3nab || f3nac) && !ctx->ddone[4]) return ctx->fail(FB_E_ORDER, "get_forces: Dassemble_vna_3c not run");
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  const DStruct &d = ctx->h_st[is];
+  const int64_t p0 = ctx->h_noff[d.atom_off], p1 = ctx->h_noff[d.atom_off + d.natoms];
+  RC(d2h(ctx, vna_atom, ctx->d_fatom.as<double>() + 3 * p0, sizeof(double) * 3 * (p1 - p0)));
+  RC(d2h(ctx, vna_ontop, ctx->d_fontop.as<double>() + 3 * p0, sizeof(double) * 3 * (p1 - p0)));
+  if (f3naa || f3nab || f3nac) {
+    std::vector<double> h((size_t)9 * d.natoms);
+    RC(d2h(ctx, h.data(), ctx->d_f3.as<double>() + (size_t)9 * d.atom_off, sizeof(double) * h.size()));
+    double *dst[3] = {f3naa, f3nab, f3nac};
+    for (int r = 0; r < 3; r++)
+      if (dst[r])
+        for (int a = 0; a < d.natoms; a++)
+          for (int k = 0; k < 3; k++) dst[r][3 * a + k] = h[(size_t)9 * a + 3 * r + k];
+  }
   return FB_OK;
 }
 

@@ -1,0 +1,718 @@
+// Force assemblers of k.DASSEMBLERS in binary64 on gfx950:
+//   Dassemble_S / _T / _dipole_z   Dblock(3, norb_mu, norb_nu) per pair
+//                                  (M_Dassemble_2c_DOGS.f90:95-655)
+//   Dassemble_vna_2c               forces%vna_ontop / vna_atom per pair
+//                                  (M_Dassemble_2c_DOGS.f90:679-1199)
+//   Dassemble_vna_3c               forces%f3naa / f3nab / f3nac per atom
+//                                  (M_Dassemble_3c_DOGS.f90:136-793)
+// with the derivative helpers getDMEs_Fdata_2c (e.FDATA/M_Fdata_2c.f90:446-600),
+// getDMEs_Fdata_3c (e.FDATA/M_Fdata_3c.f90:549-846), Depsilon_2c / Depsilon_3c
+// (b.FUNCTIONS), Dtwister / Drotate (c.SYSTEM/M_Drotations.f90:102-298) and
+// Dsmoother (b.FUNCTIONS/Dsmoother.f90).  Expressions keep the reference's
+// evaluation order (the library is compiled with -ffp-contract=off).
+//
+// Layout: 2c work is one wavefront per central atom with one lane per pair
+// (the atom's pairs are contiguous), so the per-pair outputs (Dblocks, the two
+// force vectors) are stored without any reduction.  3c work is one thread per
+// triplet in the list (reference) order; each thread contracts its rotated
+// derivative blocks with the density block of its target pair at once, so a
+// triplet leaves nine numbers (the forces on ialpha, iatom and jatom), and a
+// second kernel sums them per atom in three fixed orders (list order for
+// ialpha, target-pair order for iatom, a stable jatom order for jatom) with a
+// fixed wavefront tree: deterministic, no atomics.
+#include "fb_device.h"
+#include "fb_forces.h"
+
+#define WD2 64
+#define NBK (FB_MO * FB_MO)
+#define NME3F 10
+
+// Levi-Civita and Kronecker tensors of include/constants.h (xlevi, delk), 1-based
+__device__ __forceinline__ double d_levi(int i, int j, int k) {
+  if (i == j || j == k || i == k) return 0.0;
+  return ((i == 1 && j == 2) || (i == 2 && j == 3) || (i == 3 && j == 1)) ? 1.0 : -1.0;
+}
+__device__ __forceinline__ double d_delk(int i, int j) { return i == j ? 1.0 : 0.0; }
+
+#define EPS_(e, i, j) (e)[3 * ((j) - 1) + (i) - 1]
+#define DEP_(d, ix, imu, k) (d)[((ix) - 1) + 3 * ((imu) - 1) + 9 * ((k) - 1)]
+
+__device__ __forceinline__ double d_Dsmoother(double r, double rend, double x) {
+  double rbegin = x * rend;
+  if (r > rend) return 0.0;
+  if (r < rbegin) return 0.0;
+  double ratio = (r - rbegin) / (rend - rbegin);
+  return -(2.0 * (1.0 - ratio * ratio) * (2.0 * ratio) / (rend - rbegin));
+}
+
+// deps(ix, imu, k) = d eps(imu, k) / d r1(ix) of the 2c frame epsilon(r2, sighat)
+__device__ inline void d_Depsilon_2c(const double *r1, const double *r2, double z, const double *eps, double *deps) {
+#pragma unroll
+  for (int i = 0; i < 27; i++) deps[i] = 0.0;
+  if (z < 1.0e-5) return;
+  double ca0 = r2[1] * r1[2] - r2[2] * r1[1];
+  double ca1 = r2[2] * r1[0] - r2[0] * r1[2];
+  double ca2 = r2[0] * r1[1] - r2[1] * r1[0];
+  double crossmag = sqrt(ca0 * ca0 + ca1 * ca1 + ca2 * ca2);
+  if (fabs(crossmag) < 1.0e-3) return;
+  double r2mag2 = r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2];
+  double r1dotr2 = r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2];
+  double iz = 1.0 / z, ic = 1.0 / crossmag;
+#pragma unroll
+  for (int ix = 1; ix <= 3; ix++)
+#pragma unroll
+    for (int imu = 1; imu <= 3; imu++) {
+      double sum = d_levi(ix, imu, 1) * r2[0] + d_levi(ix, imu, 2) * r2[1] + d_levi(ix, imu, 3) * r2[2];
+      DEP_(deps, ix, imu, 1) = (EPS_(eps, imu, 1) * EPS_(eps, ix, 3) * iz) -
+                               (EPS_(eps, imu, 1) * (ic * ic)) * (r2mag2 * r1[ix - 1] - r1dotr2 * r2[ix - 1]) +
+                               iz * ic *
+                                   (d_delk(imu, ix) * (r2mag2 - r1dotr2) - r1[imu - 1] * r2[ix - 1] -
+                                    r2[imu - 1] * r2[ix - 1] + 2.0 * r1[ix - 1] * r2[imu - 1]);
+      DEP_(deps, ix, imu, 2) = ic * (sum + (EPS_(eps, imu, 2) * ic) * (r1dotr2 * r2[ix - 1] - r2mag2 * r1[ix - 1]));
+      DEP_(deps, ix, imu, 3) = -((d_delk(imu, ix) - EPS_(eps, imu, 3) * EPS_(eps, ix, 3)) * iz);
+    }
+}
+
+// dera3 = d eps / d ratm and der13 = d eps / d r1 of the 3c frame epsilon(rhat, sighat)
+__device__ inline void d_Depsilon_3c(const double *r1, const double *r2, const double *r21, double z,
+                                     const double *ratm, const double *rnabc, const double *eps3, double *dera3,
+                                     double *der13) {
+#pragma unroll
+  for (int i = 0; i < 27; i++) { dera3[i] = 0.0; der13[i] = 0.0; }
+  double ca0 = r21[1] * rnabc[2] - r21[2] * rnabc[1];
+  double ca1 = r21[2] * rnabc[0] - r21[0] * rnabc[2];
+  double ca2 = r21[0] * rnabc[1] - r21[1] * rnabc[0];
+  double crossmag = sqrt(ca0 * ca0 + ca1 * ca1 + ca2 * ca2);
+  if (fabs(crossmag) < 1.0e-3) return;
+  double r2mag2 = r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2];
+  double r1mag2 = r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2];
+  double ratmmag2 = ratm[0] * ratm[0] + ratm[1] * ratm[1] + ratm[2] * ratm[2];
+  double r1dotr2 = r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2];
+  double r2dotratm = r2[0] * ratm[0] + r2[1] * ratm[1] + r2[2] * ratm[2];
+  double r1dotratm = r1[0] * ratm[0] + r1[1] * ratm[1] + r1[2] * ratm[2];
+#pragma unroll
+  for (int ix = 1; ix <= 3; ix++)
+#pragma unroll
+    for (int imu = 1; imu <= 3; imu++) {
+      double sum = d_levi(ix, imu, 1) * (r2[0] - r1[0]) + d_levi(ix, imu, 2) * (r2[1] - r1[1]) +
+                   d_levi(ix, imu, 3) * (r2[2] - r1[2]);
+      DEP_(dera3, ix, imu, 2) =
+          (1.0 / crossmag) * (sum - (EPS_(eps3, imu, 2) / crossmag) *
+                                        ((r1mag2 + r2mag2 - 2.0 * r1dotr2) * ratm[ix - 1] +
+                                         (r1dotr2 - r1mag2 - r2dotratm + r1dotratm) * r2[ix - 1] +
+                                         (r1dotr2 - r2mag2 + r2dotratm - r1dotratm) * r1[ix - 1]));
+    }
+#pragma unroll
+  for (int ix = 1; ix <= 3; ix++) {
+    DEP_(dera3, ix, 1, 1) = EPS_(eps3, 3, 3) * DEP_(dera3, ix, 2, 2) - EPS_(eps3, 2, 3) * DEP_(dera3, ix, 3, 2);
+    DEP_(dera3, ix, 2, 1) = EPS_(eps3, 1, 3) * DEP_(dera3, ix, 3, 2) - EPS_(eps3, 3, 3) * DEP_(dera3, ix, 1, 2);
+    DEP_(dera3, ix, 3, 1) = EPS_(eps3, 2, 3) * DEP_(dera3, ix, 1, 2) - EPS_(eps3, 1, 3) * DEP_(dera3, ix, 2, 2);
+  }
+#pragma unroll
+  for (int ix = 1; ix <= 3; ix++)
+#pragma unroll
+    for (int imu = 1; imu <= 3; imu++) {
+      DEP_(der13, ix, imu, 3) = (1.0 / z) * (EPS_(eps3, imu, 3) * EPS_(eps3, ix, 3) - d_delk(imu, ix));
+      double sum = d_levi(ix, imu, 1) * (ratm[0] - r2[0]) + d_levi(ix, imu, 2) * (ratm[1] - r2[1]) +
+                   d_levi(ix, imu, 3) * (ratm[2] - r2[2]);
+      DEP_(der13, ix, imu, 2) =
+          (1.0 / crossmag) * (sum - (EPS_(eps3, imu, 2) / crossmag) *
+                                        ((r2dotratm - r1dotratm + r1dotr2 - r2mag2) * ratm[ix - 1] +
+                                         (r2dotratm + r1dotratm - r1dotr2 - ratmmag2) * r2[ix - 1] +
+                                         (ratmmag2 - 2.0 * r2dotratm + r2mag2) * r1[ix - 1]));
+    }
+#pragma unroll
+  for (int ix = 1; ix <= 3; ix++) {
+    DEP_(der13, ix, 1, 1) = EPS_(eps3, 3, 3) * DEP_(der13, ix, 2, 2) - EPS_(eps3, 3, 2) * DEP_(der13, ix, 2, 3) -
+                            EPS_(eps3, 2, 3) * DEP_(der13, ix, 3, 2) + EPS_(eps3, 2, 2) * DEP_(der13, ix, 3, 3);
+    DEP_(der13, ix, 2, 1) = EPS_(eps3, 1, 3) * DEP_(der13, ix, 3, 2) - EPS_(eps3, 1, 2) * DEP_(der13, ix, 3, 3) -
+                            EPS_(eps3, 3, 3) * DEP_(der13, ix, 1, 2) + EPS_(eps3, 3, 2) * DEP_(der13, ix, 1, 3);
+    DEP_(der13, ix, 3, 1) = EPS_(eps3, 2, 3) * DEP_(der13, ix, 1, 2) - EPS_(eps3, 2, 2) * DEP_(der13, ix, 1, 3) -
+                            EPS_(eps3, 1, 3) * DEP_(der13, ix, 2, 2) + EPS_(eps3, 1, 2) * DEP_(der13, ix, 2, 3);
+  }
+}
+
+// derivative (component ix, 1-based) of the p rotation: Dpmat(ix, :, :) of
+// Dtwister, the (y, z, x) permutation of deps(ix, :, :)
+__device__ __forceinline__ void d_dpmat(const double *deps, int ix, double *dpm) {
+  dpm[0] = DEP_(deps, ix, 2, 2); dpm[3] = DEP_(deps, ix, 2, 3); dpm[6] = DEP_(deps, ix, 2, 1);
+  dpm[1] = DEP_(deps, ix, 3, 2); dpm[4] = DEP_(deps, ix, 3, 3); dpm[7] = DEP_(deps, ix, 3, 1);
+  dpm[2] = DEP_(deps, ix, 1, 2); dpm[5] = DEP_(deps, ix, 1, 3); dpm[8] = DEP_(deps, ix, 1, 1);
+}
+
+// Drotate, one derivative component: X = dL M R^T + L M dR^T + L dM R^T shell
+// pair by shell pair with the reference's loop order (m2, k2, m1, k1) and the
+// s-shell rotation 1, derivative 0 of Dchooser.  M, DM, X column-major, ld nmu.
+__device__ inline void d_Drot(const DSpecies &sp, int in1, int in2, const double *pm, const double *dpm,
+                              const double *M, const double *DM, int nmu, double *X) {
+  const int nnu = sp.norb[in2 - 1];
+  for (int i = 0; i < nmu * nnu; i++) X[i] = 0.0;
+  int n1 = 0;
+  for (int issh = 0; issh < sp.nssh[in1 - 1]; issh++) {
+    const int l1 = sp.lssh[in1 - 1][issh];
+    int n2 = 0;
+    for (int jssh = 0; jssh < sp.nssh[in2 - 1]; jssh++) {
+      const int l2 = sp.lssh[in2 - 1][jssh];
+      for (int m2 = 0; m2 < 2 * l2 + 1; m2++)
+        for (int k2 = 0; k2 < 2 * l2 + 1; k2++)
+          for (int m1 = 0; m1 < 2 * l1 + 1; m1++)
+            for (int k1 = 0; k1 < 2 * l1 + 1; k1++) {
+              const double lf = (l1 == 0) ? 1.0 : pm[3 * m1 + k1];
+              const double dl = (l1 == 0) ? 0.0 : dpm[3 * m1 + k1];
+              const double rt = (l2 == 0) ? 1.0 : pm[3 * m2 + k2];
+              const double dr = (l2 == 0) ? 0.0 : dpm[3 * m2 + k2];
+              const int o = (n2 + m2) * nmu + (n1 + m1);
+              double *x = &X[(n2 + k2) * nmu + (n1 + k1)];
+              *x = *x + dl * rt * M[o] + lf * dr * M[o] + lf * rt * DM[o];
+            }
+      n2 += 2 * l2 + 1;
+    }
+    n1 += 2 * l1 + 1;
+  }
+}
+
+// getDMEs_Fdata_2c on one cell: values and x-derivatives of the nME elements
+__device__ inline int d_getD2c(const DCell2c &c, const double *__restrict__ tab, double x, double *F, double *dF) {
+  if (x < 0.0) return ST_X_NEG;
+  if (x > c.xmax * (1.0 + 1.0e-5)) return ST_X_RANGE;
+  const double *base = tab + c.off;
+  if (x < 1.0e-5) {
+    for (int m = 0; m < c.nME; m++) { F[m] = base[m]; dF[m] = 0.0; }
+    return 0;
+  }
+  int imid = (int)(x / c.dx) + 1;
+  int ileft = imid - 3, iright = imid + 3;
+  if (ileft < 1) { ileft = 1; iright = 7; }
+  else if (iright > c.nx) { ileft = c.nx - 6; iright = c.nx; }
+  double xx[7], pdenom[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) xx[k] = (double)(ileft + k - 1) * c.dx;
+  for (int m = 0; m < c.nME; m++) { F[m] = 0.0; dF[m] = 0.0; }
+#pragma unroll
+  for (int is = 0; is < 7; is++) {
+    double prod = 1.0;
+    pdenom[is] = 1.0;
+#pragma unroll
+    for (int ip = 0; ip < 7; ip++) {
+      if (ip != is) {
+        double pden = 1.0 / (xx[is] - xx[ip]);
+        pdenom[is] = pdenom[is] * pden;
+        prod = prod * (x - xx[ip]) * pden;
+      }
+    }
+    const double *row = base + (int64_t)(ileft - 1 + is) * c.nME;
+    for (int m = 0; m < c.nME; m++) F[m] = F[m] + row[m] * prod;
+  }
+#pragma unroll
+  for (int ipt = 0; ipt < 7; ipt++) {
+    double xsum = 0.0;
+#pragma unroll
+    for (int jpt = 0; jpt < 7; jpt++) {
+      if (ipt == jpt) continue;
+      double xprod = 1.0;
+#pragma unroll
+      for (int kpt = 0; kpt < 7; kpt++) {
+        if (kpt == jpt || kpt == ipt) continue;
+        xprod = xprod * (x - xx[kpt]);
+      }
+      xsum = xsum + xprod;
+    }
+    const double *row = base + (int64_t)(ileft - 1 + ipt) * c.nME;
+    for (int m = 0; m < c.nME; m++) dF[m] = dF[m] + row[m] * pdenom[ipt] * xsum;
+  }
+  return 0;
+}
+
+__device__ __forceinline__ int cell2c_of(const DView &v, int in1, int in2, int slot, int isub) {
+  return v.idx2c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * FB_NSLOT2 + slot) * FB_MAXSUB2 + isub];
+}
+
+// value and r1-derivative blocks (molecular frame) of one 2c interaction:
+// hm = getDMEs value, vd(ix) = -eta(ix) dF/dx where z > zcut (else 0)
+__device__ inline int d_dblock2c(const DView &v, int cid, int nmu, int nnu, double z, double *hm, double *dhm) {
+  if (cid < 0) return ST_NO_TABLE;
+  const DCell2c c = v.cell2c[cid];
+  double F[16], dF[16];
+  int st = d_getD2c(c, v.tab2c, z, F, dF);
+  if (st) return st;
+  for (int i = 0; i < nmu * nnu; i++) { hm[i] = 0.0; dhm[i] = 0.0; }
+  for (int k = 0; k < c.nME; k++) {
+    const int o = (v.nu[c.munu + k] - 1) * nmu + (v.mu[c.munu + k] - 1);
+    hm[o] = F[k];
+    dhm[o] = dF[k];
+  }
+  return 0;
+}
+
+// f(ix) -= rho(o) [dq] X_ix(o) P_eq2 over o, with X_ix the Drotate of (hm, -eta dhm)
+__device__ inline void d_contract_2c(const DView &v, int in1, int in2, int nmu, double z, double zcut,
+                                     const double *pm, const double *deps, const double *eta, const double *hm,
+                                     const double *dhm, const double *rho, double dq, bool use_dq, double *f) {
+  const int nb = nmu * v.sp.norb[in2 - 1];
+  double DM[NBK], X[NBK], dpm[9];
+#pragma unroll
+  for (int ix = 0; ix < 3; ix++) {
+    for (int i = 0; i < nb; i++) DM[i] = (z > zcut) ? -eta[ix] * dhm[i] : 0.0;
+    d_dpmat(deps, ix + 1, dpm);
+    d_Drot(v.sp, in1, in2, pm, dpm, hm, DM, nmu, X);
+    for (int i = 0; i < nb; i++) {
+      if (use_dq) f[ix] = f[ix] - rho[i] * dq * X[i] * FB_P_EQ2;
+      else f[ix] = f[ix] - rho[i] * X[i] * FB_P_EQ2;
+    }
+  }
+}
+
+// mask bits: 1 Dblock S, 2 Dblock T, 4 Dblock dipole_z, 8 vna_ontop / vna_atom
+__global__ __launch_bounds__(WD2) void k_d2c(DView v, DForce F, int mask) {
+  const int ia = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int s = v.atom_struct[ia];
+  const DStruct St = v.st[s];
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  const int in1 = v.imass[ia];
+  const int nmu = v.sp.norb[in1 - 1];
+  const int n = v.neighn[ia];
+  const int64_t p0 = v.noff[ia];
+  const double r1[3] = {v.ratom[3 * ia], v.ratom[3 * ia + 1], v.ratom[3 * ia + 2]};
+  const int64_t pself = p0 + v.neigh_self[ia] - 1;
+  int status = 0;
+  for (int k = lane; k < n; k += WD2) {
+    const int64_t p = p0 + k;
+    const int ja = St.atom_off + v.neigh_j[p] - 1;
+    const int mb = v.neigh_b[p];
+    const int in2 = v.imass[ja];
+    const int nnu = v.sp.norb[in2 - 1];
+    const int nb = nmu * nnu;
+    double r2[3];
+    for (int q = 0; q < 3; q++) r2[q] = v.ratom[3 * ja + q] + xls[3 * mb + q];
+    double z = d_distance(r1, r2), sighat[3], eps[9], pm[9], deps[27], eta[3];
+    if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+    else for (int q = 0; q < 3; q++) sighat[q] = (r2[q] - r1[q]) / z;
+    d_epsilon(r2, sighat, eps);
+    d_Depsilon_2c(r1, r2, z, eps, deps);
+    d_pmat<double>(eps, pm);
+    for (int q = 0; q < 3; q++) eta[q] = eps[6 + q];
+    const int64_t bo = v.boff[p];
+    double hm[NBK], dhm[NBK], DM[NBK], X[NBK], dpm[9];
+    const int slots[3] = {SLOT_S, SLOT_T, SLOT_DIP};
+    for (int d = 0; d < 3; d++) {
+      if (!(mask & (1 << d))) continue;
+      int st = d_dblock2c(v, cell2c_of(v, in1, in2, slots[d], 0), nmu, nnu, z, hm, dhm);
+      status |= st;
+      if (st) continue;
+      double *D = F.dmat[d] + 3 * bo;
+      for (int ix = 0; ix < 3; ix++) {
+        for (int i = 0; i < nb; i++) DM[i] = (z > 1.0e-3) ? -eta[ix] * dhm[i] : 0.0;
+        d_dpmat(deps, ix + 1, dpm);
+        d_Drot(v.sp, in1, in2, pm, dpm, hm, DM, nmu, X);
+        for (int i = 0; i < nb; i++) D[ix + 3 * i] = X[i];
+      }
+    }
+    if (mask & 8) {
+      // ontop L and R, contracted with the pair's density block (:760-999)
+      double fo[3] = {0.0, 0.0, 0.0};
+      const bool self = (v.neigh_j[p] == ia - St.atom_off + 1 && mb == 0);
+      if (!self) {
+        const double *rho = F.rho + bo;
+        for (int side = 0; side < 2; side++) {
+          const int slot = side ? SLOT_ONR : SLOT_ONL;
+          const int qa = side ? ja : ia;
+          const int inq = side ? in2 : in1;
+          int st = d_dblock2c(v, cell2c_of(v, in1, in2, slot, 0), nmu, nnu, z, hm, dhm);
+          status |= st;
+          if (!st) d_contract_2c(v, in1, in2, nmu, z, 1.0e-3, pm, deps, eta, hm, dhm, rho, 0.0, false, fo);
+          for (int isorp = 1; isorp <= v.sp.nssh[inq - 1]; isorp++) {
+            const double dQ = v.dQ[qa * FB_MAXSH + isorp - 1];
+            if (dQ == 0.0) continue;
+            st = d_dblock2c(v, cell2c_of(v, in1, in2, slot, isorp), nmu, nnu, z, hm, dhm);
+            status |= st;
+            if (!st) d_contract_2c(v, in1, in2, nmu, z, 1.0e-3, pm, deps, eta, hm, dhm, rho, dQ, true, fo);
+          }
+        }
+      }
+      // atom term <i|V_j|i>, contracted with the density block of neigh_self (:1005-1190)
+      double fa[3] = {0.0, 0.0, 0.0};
+      const double *rho_s = F.rho + v.boff[pself];
+      int st = d_dblock2c(v, cell2c_of(v, in1, in2, SLOT_ATOM, 0), nmu, nmu, z, hm, dhm);
+      status |= st;
+      if (!st) d_contract_2c(v, in1, in1, nmu, z, 1.0e-3, pm, deps, eta, hm, dhm, rho_s, 0.0, false, fa);
+      bool charged = false;
+      for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) charged |= (v.dQ[ja * FB_MAXSH + isorp - 1] != 0.0);
+      if (charged) {
+        const double rend = v.sp.rcmin[in1 - 1] + v.sp.rcmin[in2 - 1];
+        const double smooth = d_smoother(z, rend, 0.8), Dsmooth = d_Dsmoother(z, rend, 0.8);
+        const double *sself = v.mat[FB_MAT_S] + v.boff[pself];
+        const int na = nmu * nmu;
+        for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
+          const double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
+          if (dQ == 0.0) continue;
+          st = d_dblock2c(v, cell2c_of(v, in1, in2, SLOT_ATOM, isorp), nmu, nmu, z, hm, dhm);
+          status |= st;
+          if (st) continue;
+          double bx[NBK];
+          d_rotate<double>(v.sp, in1, in1, pm, hm, nmu, bx);
+          for (int ix = 0; ix < 3; ix++) {
+            for (int i = 0; i < na; i++) DM[i] = (z > 1.0e-4) ? -eta[ix] * dhm[i] : 0.0;
+            d_dpmat(deps, ix + 1, dpm);
+            d_Drot(v.sp, in1, in1, pm, dpm, hm, DM, nmu, X);
+            for (int i = 0; i < na; i++) {
+              const double emnpl = (z > 1.0e-4) ? sself[i] / z : 0.0;
+              const double vdem = (z > 1.0e-4) ? eta[ix] * sself[i] / (z * z) : 0.0;
+              fa[ix] = fa[ix] - rho_s[i] * FB_P_EQ2 * dQ *
+                                    (smooth * X[i] - eta[ix] * Dsmooth * bx[i] + (1.0 - smooth) * vdem +
+                                     eta[ix] * Dsmooth * emnpl);
+            }
+          }
+        }
+      }
+      for (int q = 0; q < 3; q++) {
+        F.fontop[3 * p + q] = fo[q];
+        F.fatom[3 * p + q] = fa[q];
+      }
+    }
+  }
+  if (status) atomicOr(v.status, status);
+}
+
+// ---------------------------------------------------------------- 3c
+
+__device__ __forceinline__ int group3c_of(const DView &v, int in1, int in2, int in3, int isub) {
+  return v.idx3c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * v.sp.ns + (in3 - 1)) * FB_MAXSUB3 + isub];
+}
+
+// getDMEs_Fdata_3c: value, d/dcos, d/dx, d/dy blocks (molecular frame).  The
+// 3-point rule takes the slopes of ME 1 of theta cell 1 for every element.
+__device__ inline int d_getD3c(const DView &v, int gid, double x, double y, double cost, int nmu, int nnu,
+                               double *hm, double *dpm_, double *dxm, double *dym) {
+  if (gid < 0) return ST_NO_TABLE;
+  const DGroup3c g = v.grp3c[gid];
+  const int nME = g.nME, nx = g.nx, ny = g.ny;
+  double p[5], Dp[5];
+  const double cost2 = cost * cost;
+  const double sint = sqrt(fmax(1.0e-5, 1.0 - cost2));
+  p[0] = 1.0;
+  p[1] = cost;
+  p[2] = 0.5 * (3.0 * cost2 - 1.0);
+  p[3] = 0.5 * cost * (5.0 * cost2 - 3.0);
+  p[4] = 0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2));
+  Dp[0] = 0.0;
+  Dp[1] = 1.0;
+  Dp[2] = 3.0 * cost;
+  Dp[3] = (15.0 * cost2 - 3.0) / 2.0;
+  Dp[4] = 0.5 * cost * (35.0 * cost2 - 15.0);
+  const double dx = g.xmax / (nx - 1.0), dy = g.ymax / (ny - 1.0);
+  int ix = (int)(x / dx) + 1, iy = (int)(y / dy) + 1;
+  if (ix < 2) ix = 2;
+  if (iy < 2) iy = 2;
+  if (ix > nx - 2) ix = nx - 2;
+  if (iy > ny - 2) iy = ny - 2;
+  const double px = x / dx - ix + 1.0, py = y / dy - iy + 1.0;
+  const double *tab = v.tab3c + g.off;
+  const int64_t tstride = (int64_t)ny * nx * nME;
+#define T3F(th, i, j, m) tab[(th) * tstride + ((int64_t)((j) - 1) * nx + ((i) - 1)) * nME + (m)]
+  const double gx = (T3F(0, ix + 1, iy, 0) - T3F(0, ix, iy, 0)) / dx;
+  const double gy = (T3F(0, ix, iy + 1, 0) - T3F(0, ix, iy, 0)) / dy;
+  const double gg = gx * gx + gy * gy;
+  double Fv[NME3F], DpF[NME3F], DxF[NME3F], DyF[NME3F];
+  for (int m = 0; m < nME; m++) { Fv[m] = 0.0; DpF[m] = 0.0; DxF[m] = 0.0; DyF[m] = 0.0; }
+  for (int th = 0; th < FB_NTHETA; th++) {
+    const double pt = p[th], dpt = Dp[th];
+    for (int m = 0; m < nME; m++) {
+      if (gg < 1.0e-6) {
+        const double interim =
+            ((1.0 - px - py) * T3F(th, ix, iy, m) + px * T3F(th, ix + 1, iy, m)) + py * T3F(th, ix, iy + 1, m);
+        Fv[m] = Fv[m] + pt * interim;
+        DpF[m] = DpF[m] + dpt * interim;
+        DxF[m] = DxF[m] + pt * gx;
+        DyF[m] = DyF[m] + pt * gy;
+      } else {
+        double gk[4], gp[4];
+        for (int k = -1; k <= 2; k++) {
+          double f1m1 = T3F(th, ix + k, iy - 1, m);
+          double f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+          double f0p3 = 3.0 * T3F(th, ix + k, iy, m), f0p6 = 2.0 * f0p3;
+          double f1p3 = 3.0 * T3F(th, ix + k, iy + 1, m), f1p6 = 2.0 * f1p3;
+          double f2p1 = T3F(th, ix + k, iy + 2, m);
+          double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+          double bb2 = f1m3 - f0p6 + f1p3;
+          double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+          double bb0 = f0p6;
+          gk[k + 1] = ((bb3 * py + bb2) * py + bb1) * py + bb0;
+          gp[k + 1] = ((3.0 * bb3 * py + 2.0 * bb2) * py + bb1);
+        }
+        double f1m1 = gk[0], f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+        double f0p3 = 3.0 * gk[1], f0p6 = 2.0 * f0p3;
+        double f1p3 = 3.0 * gk[2], f1p6 = 2.0 * f1p3;
+        double f2p1 = gk[3];
+        double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+        double bb2 = f1m3 - f0p6 + f1p3;
+        double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+        double bb0 = f0p6;
+        const double interim = (((bb3 * px + bb2) * px + bb1) * px + bb0) / 36.0;
+        Fv[m] = Fv[m] + pt * interim;
+        DpF[m] = DpF[m] + dpt * interim;
+        DxF[m] = DxF[m] + pt * ((3.0 * bb3 * px + 2.0 * bb2) * px + bb1) / (36.0 * dx);
+        f1m1 = gp[0]; f1m2 = 2.0 * f1m1; f1m3 = 3.0 * f1m1;
+        f0p3 = 3.0 * gp[1]; f0p6 = 2.0 * f0p3;
+        f1p3 = 3.0 * gp[2]; f1p6 = 2.0 * f1p3;
+        f2p1 = gp[3];
+        bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+        bb2 = f1m3 - f0p6 + f1p3;
+        bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+        bb0 = f0p6;
+        DyF[m] = DyF[m] + pt * (((bb3 * px + bb2) * px + bb1) * px + bb0) / (36.0 * dy);
+      }
+    }
+  }
+#undef T3F
+  for (int i = 0; i < nmu * nnu; i++) { hm[i] = 0.0; dpm_[i] = 0.0; dxm[i] = 0.0; dym[i] = 0.0; }
+  for (int k = 0; k < nME; k++) {
+    const int o = (v.nu[g.munu + k] - 1) * nmu + (v.mu[g.munu + k] - 1);
+    if (v.mv[g.munu + k] != 1) {
+      hm[o] = Fv[k]; dpm_[o] = DpF[k]; dxm[o] = DxF[k]; dym[o] = DyF[k];
+    } else {
+      hm[o] = Fv[k] * sint;
+      dpm_[o] = DpF[k] * sint - cost * Fv[k] / sint;
+      dxm[o] = DxF[k] * sint;
+      dym[o] = DyF[k] * sint;
+    }
+  }
+  return 0;
+}
+
+struct DGeo3 {
+  double r1[3], r2[3], rna[3], sighat[3], rhat[3], amt[3], bmt[3];
+  double z, x, cost, d13, d23;
+  double eps[9], depsA[27], depsB[27];
+};
+
+// triplet frame and its derivatives; charged = the rhat expression of the
+// charged loop (M_Dassemble_3c_DOGS.f90:526), rnabc / x otherwise (:336)
+__device__ inline void d_dgeo3(const DView &v, int4 tr, bool charged, DGeo3 &G) {
+  const int ial = tr.x, gia = tr.y, gja = tr.z;
+  const int ib = tr.w & 0xffff, jb = (tr.w >> 16) & 0xffff;
+  const int s = v.atom_struct[ial];
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  double r21[3], rnabc[3];
+  for (int k = 0; k < 3; k++) {
+    G.r1[k] = v.ratom[3 * gia + k] + xls[3 * ib + k];
+    G.r2[k] = v.ratom[3 * gja + k] + xls[3 * jb + k];
+    G.rna[k] = v.ratom[3 * ial + k];
+    r21[k] = G.r2[k] - G.r1[k];
+  }
+  G.z = d_distance(G.r1, G.r2);
+  if (G.z < 1.0e-5) { G.sighat[0] = 0.0; G.sighat[1] = 0.0; G.sighat[2] = 1.0; }
+  else for (int k = 0; k < 3; k++) G.sighat[k] = r21[k] / G.z;
+  for (int k = 0; k < 3; k++) rnabc[k] = G.rna[k] - (G.r1[k] + r21[k] / 2.0);
+  G.x = sqrt(rnabc[0] * rnabc[0] + rnabc[1] * rnabc[1] + rnabc[2] * rnabc[2]);
+  G.d13 = d_distance(G.rna, G.r1);
+  G.d23 = d_distance(G.rna, G.r2);
+  if (G.x < 1.0e-5) { G.rhat[0] = 0.0; G.rhat[1] = 0.0; G.rhat[2] = 0.0; }
+  else if (charged) for (int k = 0; k < 3; k++) G.rhat[k] = (G.rna[k] - 0.5 * (G.r1[k] + G.r2[k])) / G.x;
+  else for (int k = 0; k < 3; k++) G.rhat[k] = rnabc[k] / G.x;
+  G.cost = G.sighat[0] * G.rhat[0] + G.sighat[1] * G.rhat[1] + G.sighat[2] * G.rhat[2];
+  d_epsilon(G.rhat, G.sighat, G.eps);
+  d_Depsilon_3c(G.r1, G.r2, r21, G.z, G.rna, rnabc, G.eps, G.depsA, G.depsB);
+  for (int k = 0; k < 3; k++) {
+    G.amt[k] = (G.x > 1.0e-5) ? (G.sighat[k] - G.cost * G.rhat[k]) / G.x : 0.0;
+    G.bmt[k] = (G.cost * G.sighat[k] - G.rhat[k]) / G.z;
+  }
+}
+
+// f3naMa / f3naMb of component ix (M_Dassemble_3c_DOGS.f90:388-399) on the
+// elements of the group's cell, rotated into Xa / Xb
+__device__ inline void d_f3_rot(const DView &v, const DGroup3c &g, int in1, int in2, int nmu, const DGeo3 &G,
+                                const double *pm, int ix, const double *hm, const double *dp, const double *dxm,
+                                const double *dym, double *Xa, double *Xb) {
+  const int nb = nmu * v.sp.norb[in2 - 1];
+  double Ma[NBK], Mb[NBK], dpm[9];
+  for (int i = 0; i < nb; i++) { Ma[i] = 0.0; Mb[i] = 0.0; }
+  for (int k = 0; k < g.nME; k++) {
+    const int o = (v.nu[g.munu + k] - 1) * nmu + (v.mu[g.munu + k] - 1);
+    Ma[o] = G.rhat[ix] * dxm[o] + G.amt[ix] * dp[o];
+    Mb[o] = -G.sighat[ix] * dym[o] + G.bmt[ix] * dp[o] - Ma[o] / 2.0;
+  }
+  d_dpmat(G.depsA, ix + 1, dpm);
+  d_Drot(v.sp, in1, in2, pm, dpm, hm, Ma, nmu, Xa);
+  d_dpmat(G.depsB, ix + 1, dpm);
+  d_Drot(v.sp, in1, in2, pm, dpm, hm, Mb, nmu, Xb);
+}
+
+#define T3D 64
+
+__global__ __launch_bounds__(T3D) void k_d3c(DView v, DForce F) {
+  const int64_t t = (int64_t)blockIdx.x * T3D + threadIdx.x;
+  if (t >= v.ntrip) return;
+  const int4 tr = F.trip[t];
+  const int tp = F.tpair[t];
+  const int in1 = v.imass[tr.y], in2 = v.imass[tr.z], indna = v.imass[tr.x];
+  const int nmu = v.sp.norb[in1 - 1], nnu = v.sp.norb[in2 - 1], nb = nmu * nnu;
+  const int64_t bo = v.boff[tp];
+  const double *rho = F.rho + bo;
+  int status = 0;
+  double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0}, fc[3] = {0.0, 0.0, 0.0};
+  double hm[NBK], dp[NBK], dxm[NBK], dym[NBK], Xa[NBK], Xb[NBK], pm[9];
+  {
+    DGeo3 G;
+    d_dgeo3(v, tr, false, G);
+    d_pmat<double>(G.eps, pm);
+    const int gid = group3c_of(v, in1, in2, indna, 0);
+    int st = d_getD3c(v, gid, G.x, G.z, G.cost, nmu, nnu, hm, dp, dxm, dym);
+    status |= st;
+    if (!st) {
+      const DGroup3c g = v.grp3c[gid];
+      for (int ix = 0; ix < 3; ix++) {
+        d_f3_rot(v, g, in1, in2, nmu, G, pm, ix, hm, dp, dxm, dym, Xa, Xb);
+        for (int i = 0; i < nb; i++) {
+          const double xc = -Xa[i] - Xb[i];
+          fa[ix] = fa[ix] - rho[i] * Xa[i] * FB_P_EQ2;
+          fb[ix] = fb[ix] - rho[i] * Xb[i] * FB_P_EQ2;
+          fc[ix] = fc[ix] - rho[i] * xc * FB_P_EQ2;
+        }
+      }
+    }
+  }
+  bool charged = false;
+  for (int isorp = 1; isorp <= v.sp.nssh[indna - 1]; isorp++) charged |= (v.dQ[tr.x * FB_MAXSH + isorp - 1] != 0.0);
+  if (charged) {
+    // smoothed monopole terms (M_Dassemble_3c_DOGS.f90:528-776)
+    DGeo3 G;
+    d_dgeo3(v, tr, true, G);
+    d_pmat<double>(G.eps, pm);
+    double rA1[3] = {0.0, 0.0, 0.0}, rA2[3] = {0.0, 0.0, 0.0};
+    if (G.d13 > 1.0e-5) for (int k = 0; k < 3; k++) rA1[k] = (G.rna[k] - G.r1[k]) / G.d13;
+    if (G.d23 > 1.0e-5) for (int k = 0; k < 3; k++) rA2[k] = (G.rna[k] - G.r2[k]) / G.d23;
+    const double rc3 = v.sp.rcmin[indna - 1];
+    const double rend1 = v.sp.rcmin[in1 - 1] + rc3, rend2 = v.sp.rcmin[in2 - 1] + rc3;
+    const double st13 = d_smoother(G.d13, rend1, 0.8), dst13 = d_Dsmoother(G.d13, rend1, 0.8);
+    const double st23 = d_smoother(G.d23, rend2, 0.8), dst23 = d_Dsmoother(G.d23, rend2, 0.8);
+    const double stinky = st13 * st23;
+    double dstnA[3], dstnB[3], dstnC[3];
+    for (int k = 0; k < 3; k++) {
+      dstnB[k] = -dst13 * st23 * rA1[k];
+      dstnC[k] = -st13 * dst23 * rA2[k];
+      dstnA[k] = -dstnB[k] - dstnC[k];
+    }
+    const double *sb = v.mat[FB_MAT_S] + bo, *db = v.mat[FB_MAT_DIP] + bo;
+    const double *sD = F.dmat[0] + 3 * bo, *dD = F.dmat[2] + 3 * bo;
+    const double d13sq = G.d13 * G.d13, d23sq = G.d23 * G.d23;
+    for (int isorp = 1; isorp <= v.sp.nssh[indna - 1]; isorp++) {
+      const double dQ = v.dQ[tr.x * FB_MAXSH + isorp - 1];
+      if (dQ == 0.0) continue;
+      const int gid = group3c_of(v, in1, in2, indna, isorp);
+      int st = d_getD3c(v, gid, G.x, G.z, G.cost, nmu, nnu, hm, dp, dxm, dym);
+      status |= st;
+      if (st) continue;
+      const DGroup3c g = v.grp3c[gid];
+      double bx[NBK];
+      d_rotate<double>(v.sp, in1, in2, pm, hm, nmu, bx);
+      for (int ix = 0; ix < 3; ix++) {
+        d_f3_rot(v, g, in1, in2, nmu, G, pm, ix, hm, dp, dxm, dym, Xa, Xb);
+        for (int i = 0; i < nb; i++) {
+          const double sterm = sb[i] / 2.0, dterm = db[i] / G.z;
+          const double emnpl = (sterm - dterm) / G.d13 + (sterm + dterm) / G.d23;
+          const double spterm = sD[ix + 3 * i] / 2.0;
+          double dpterm = dD[ix + 3 * i] / G.z;
+          dpterm = dpterm + dterm * G.sighat[ix] / G.z;
+          const double dA = -((sterm - dterm) * rA1[ix] / d13sq) - (sterm + dterm) * rA2[ix] / d23sq;
+          const double dB = (sterm - dterm) * rA1[ix] / d13sq + (spterm - dpterm) / G.d13 + (spterm + dpterm) / G.d23;
+          const double dC = -dA - dB;
+          const double xc = -Xa[i] - Xb[i];
+          const double w = FB_P_EQ2 * dQ * rho[i];
+          fa[ix] = fa[ix] - w * (stinky * Xa[i] + dstnA[ix] * bx[i] - dstnA[ix] * emnpl + (1.0 - stinky) * dA);
+          fb[ix] = fb[ix] - w * (stinky * Xb[i] + dstnB[ix] * bx[i] - dstnB[ix] * emnpl + (1.0 - stinky) * dB);
+          fc[ix] = fc[ix] - w * (stinky * xc + dstnC[ix] * bx[i] - dstnC[ix] * emnpl + (1.0 - stinky) * dC);
+        }
+      }
+    }
+  }
+  double *o = F.f3t + 9 * t;
+  for (int k = 0; k < 3; k++) { o[k] = fa[k]; o[3 + k] = fb[k]; o[6 + k] = fc[k]; }
+  if (status) atomicOr(v.status, status);
+}
+
+// per-atom sums of the triplet forces: one wavefront per atom, lane-strided
+// partial sums in a fixed order and a fixed butterfly, so results repeat bit
+// for bit from run to run
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x = x + __shfl_xor(x, off, 64);
+  return x;
+}
+
+__global__ __launch_bounds__(64) void k_f3_reduce(DView v, DForce F) {
+  const int a = blockIdx.x;
+  const int lane = threadIdx.x;
+  double s[9];
+  for (int k = 0; k < 9; k++) s[k] = 0.0;
+  for (int64_t t = F.toff[a] + lane; t < F.toff[a + 1]; t += 64)
+    for (int k = 0; k < 3; k++) s[k] = s[k] + F.f3t[9 * t + k];
+  for (int64_t q = F.ioff[a] + lane; q < F.ioff[a + 1]; q += 64) {
+    const int64_t t = F.perm[q];
+    for (int k = 0; k < 3; k++) s[3 + k] = s[3 + k] + F.f3t[9 * t + 3 + k];
+  }
+  for (int64_t q = F.joff[a] + lane; q < F.joff[a + 1]; q += 64) {
+    const int64_t t = F.perm_j[q];
+    for (int k = 0; k < 3; k++) s[6 + k] = s[6 + k] + F.f3t[9 * t + 6 + k];
+  }
+  for (int k = 0; k < 9; k++) {
+    const double r = wave_sum(s[k]);
+    if (lane == 0) F.f3[9 * a + k] = r;
+  }
+}
+
+// triplet ranges per atom in the target-pair order (iatom) and in the jatom
+// order: lower bounds of the atom's first pair / id in the sorted keys
+__device__ __forceinline__ int64_t lower_bound_i32(const int *a, int64_t n, int64_t key) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_f3_offsets(int natoms, int64_t ntrip, const int64_t *noff, const int *tpair_sorted,
+                             const int *jkey_sorted, int64_t *ioff, int64_t *joff) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a > natoms) return;
+  ioff[a] = lower_bound_i32(tpair_sorted, ntrip, noff[a]);
+  joff[a] = lower_bound_i32(jkey_sorted, ntrip, a);
+}
+
+__global__ void k_trip_jkey(int64_t ntrip, const int4 *trip, int *key) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntrip) key[t] = trip[t].z;
+}
+
+extern "C" int fbk_d2c(const DView &v, const DForce &f, int mask, hipStream_t st) {
+  if (v.natoms == 0) return 0;
+  hipLaunchKernelGGL(k_d2c, dim3(v.natoms), dim3(WD2), 0, st, v, f, mask);
+  return (int)hipGetLastError();
+}
+
+extern "C" int fbk_d3c(const DView &v, const DForce &f, hipStream_t st) {
+  if (v.ntrip == 0) return 0;
+  hipLaunchKernelGGL(k_d3c, dim3((unsigned)((v.ntrip + T3D - 1) / T3D)), dim3(T3D), 0, st, v, f);
+  return (int)hipGetLastError();
+}
+
+extern "C" int fbk_f3_reduce(const DView &v, const DForce &f, hipStream_t st) {
+  if (v.natoms == 0) return 0;
+  hipLaunchKernelGGL(k_f3_reduce, dim3(v.natoms), dim3(64), 0, st, v, f);
+  return (int)hipGetLastError();
+}
+
+extern "C" int fbk_f3_offsets(int natoms, int64_t ntrip, const int64_t *noff, const int *tpair_sorted,
+                              const int *jkey_sorted, int64_t *ioff, int64_t *joff, hipStream_t st) {
+  hipLaunchKernelGGL(k_f3_offsets, dim3((unsigned)((natoms + 1 + 255) / 256)), dim3(256), 0, st, natoms, ntrip,
+                     noff, tpair_sorted, jkey_sorted, ioff, joff);
+  return (int)hipGetLastError();
+}
+
+extern "C" int fbk_trip_jkey(int64_t ntrip, const int4 *trip, int *key, hipStream_t st) {
+  if (ntrip == 0) return 0;
+  hipLaunchKernelGGL(k_trip_jkey, dim3((unsigned)((ntrip + 255) / 256)), dim3(256), 0, st, ntrip, trip, key);
+  return (int)hipGetLastError();
+}

@@ -81,6 +81,11 @@ def lib():
             "fb_mlp_stats": [vp, i64p, i64p],
             "fb_mlp_gradients": [vp, C.c_int, fp, C.c_int, fp, fp, fp, dp],
             "fb_mlp_forward": [vp, C.c_int, fp, fp, C.c_int, fp, fp],
+            "fb_set_denmat": [vp, C.c_int, dp],
+            "fb_dassemble_S": [vp], "fb_dassemble_T": [vp], "fb_dassemble_dipole_z": [vp],
+            "fb_dassemble_vna_2c": [vp], "fb_dassemble_vna_3c": [vp], "fb_dassemble_all": [vp],
+            "fb_get_dblocks": [vp, C.c_int, C.c_int, dp],
+            "fb_get_forces": [vp, C.c_int, dp, dp, dp, dp, dp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -259,6 +264,40 @@ class Assembler:
         out = np.zeros(max(n.value, 1))
         self._check(self.L.fb_get_blocks(self.h, istruct, FB_MAT[mat], _dp(out)))
         return out[:n.value]
+
+    # ---------------------------------------------------------------- forces
+    def set_denmat(self, rho, istruct=0):
+        """Density blocks of one structure (s%denmat), block layout of blocks()."""
+        n = C.c_int64()
+        self._check(self.L.fb_get_block_len(self.h, istruct, C.byref(n)))
+        rho = np.ascontiguousarray(rho, np.float64)
+        if rho.size != n.value:
+            raise FireballError(-1, f"density matrix has {rho.size} values, the blocks {n.value}")
+        self._check(self.L.fb_set_denmat(self.h, istruct, _dp(rho)))
+
+    def dassemble(self, which=("S", "T", "DIP", "VNA2C", "VNA3C")):
+        fn = {"S": self.L.fb_dassemble_S, "T": self.L.fb_dassemble_T, "DIP": self.L.fb_dassemble_dipole_z,
+              "VNA2C": self.L.fb_dassemble_vna_2c, "VNA3C": self.L.fb_dassemble_vna_3c}
+        for w in which:
+            self._check(fn[w](self.h))
+
+    def dassemble_all(self):
+        self._check(self.L.fb_dassemble_all(self.h))
+
+    def dblocks(self, mat, istruct=0):
+        n = C.c_int64()
+        self._check(self.L.fb_get_block_len(self.h, istruct, C.byref(n)))
+        out = np.zeros(max(3 * n.value, 1))
+        self._check(self.L.fb_get_dblocks(self.h, istruct, FB_MAT[mat], _dp(out)))
+        return out[:3 * n.value]
+
+    def forces(self, istruct=0):
+        npairs = self.list_sizes(istruct)[0]
+        n = self.structs[istruct].natoms
+        fa, fo = np.zeros((max(npairs, 1), 3)), np.zeros((max(npairs, 1), 3))
+        f3 = [np.zeros((n, 3)) for _ in range(3)]
+        self._check(self.L.fb_get_forces(self.h, istruct, _dp(fa), _dp(fo), *(_dp(x) for x in f3)))
+        return {"vna_atom": fa[:npairs], "vna_ontop": fo[:npairs], "f3naa": f3[0], "f3nab": f3[1], "f3nac": f3[2]}
 
     def gamma_eigenvalues(self, which="H", istruct=0):
         """Eigenvalues (ascending) of the Gamma-point folded S or
