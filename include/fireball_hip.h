@@ -220,14 +220,16 @@ int fb_mlp_forward(fb_ctx *ctx, int nme, const float *params, const float *oscal
  * No counterpart in the reference (its assembler has no launch parameters).
  * FB_TUNE_CHUNK3C: triplets per workgroup chunk of the mixed 3c kernel (64,
  * 128 or 256), which sets its LDS footprint.  FB_TUNE_2C_KERNEL: the 2c
- * kernel for s/p species, 0 = table cells staged in LDS (default), 1 =
+ * kernel for s/p species, 0 = table cells staged in LDS, 1 (default) =
  * register-resident with per-lane table gathers; both give the same bits.
  * FB_TUNE_GRAPH: 1 (default) replays fb_assemble_all from a captured HIP
- * graph, 0 enqueues its launches directly.
+ * graph, 0 enqueues its launches directly.  FB_TUNE_FORCE_KERNEL: the force
+ * kernels for s/p species, 1 (default) = compile-time block shapes, 0 = the
+ * generic shell loops; both give the same values.
  * fb_autotune times every candidate of both on the current lists (nrep
  * launches each, HIP events) and keeps the fastest; it returns the chosen
  * chunk in *chosen. */
-enum fb_tune { FB_TUNE_CHUNK3C = 0, FB_TUNE_2C_KERNEL = 1, FB_TUNE_GRAPH = 2 };
+enum fb_tune { FB_TUNE_CHUNK3C = 0, FB_TUNE_2C_KERNEL = 1, FB_TUNE_GRAPH = 2, FB_TUNE_FORCE_KERNEL = 3 };
 int fb_set_tuning(fb_ctx *ctx, int param, int value);
 int fb_get_tuning(fb_ctx *ctx, int param, int *value);
 int fb_autotune(fb_ctx *ctx, int nrep, int *chosen);

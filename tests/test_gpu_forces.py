@@ -160,3 +160,28 @@ def test_full_size_force_invariants(cfg):
     tot = f1["f3naa"] + f1["f3nab"] + f1["f3nac"]
     scale = np.abs(f1["f3naa"]).sum() + np.abs(f1["f3nab"]).sum() + np.abs(f1["f3nac"]).sum()
     assert np.max(np.abs(tot.sum(axis=0))) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("name", ["cho40_dq", "si64_dq", "chain_cluster"])
+def test_sp_kernels_equal_generic_kernels(golden, name):
+    """The s/p force kernels (compile-time block shapes) against the generic
+    shell-loop kernels: the same operations in the same order."""
+    g = golden(name)
+    ts, st = golden_cases.inputs(g)
+    rho = _rho(ts, st, {k: g[k] for k in ("neighn", "neigh_j", "neigh_back")})
+    a = _gpu_forces(ts, [st], [rho])
+    from fireball_hip.lib import Assembler
+    b = Assembler(ts, precision="fp64")
+    b.add_structure(st)
+    b.build_lists()
+    b.assemble_all()
+    b.set_tuning(3, 0)
+    b.set_denmat(rho)
+    b.dassemble_all()
+    for m in DKEYS.values():
+        np.testing.assert_array_equal(a.dblocks(m), b.dblocks(m))
+    fa, fb = a.forces(), b.forces()
+    for k in ("vna_atom", "vna_ontop"):
+        np.testing.assert_array_equal(fa[k], fb[k])
+    for k in ("f3naa", "f3nab", "f3nac"):
+        np.testing.assert_allclose(fa[k], fb[k], rtol=0, atol=1e-13 * max(1.0, np.max(np.abs(fb[k]))))

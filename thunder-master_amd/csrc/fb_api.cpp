@@ -232,6 +232,7 @@ struct fb_ctx {
   long forces_gen = -1;          // lists_gen the force buffers and orderings belong to
   std::vector<char> rho_ok;      // per structure: density blocks uploaded
   int ddone[5] = {0, 0, 0, 0, 0};  // Dblock S, T, dipole_z; vna_2c forces; vna_3c forces
+  int force_kernel = 1;            // s/p species: 1 register-resident kernels, 0 the generic ones
 
   // deferred status of asynchronous H+S builds
   int *h_status = nullptr;      // pinned
@@ -1928,12 +1929,19 @@ static bool all_rho(const fb_ctx *ctx) {
   return !ctx->rho_ok.empty();
 }
 
+// s/p species with canonical tables take the register-resident kernels
+static int launch_d2c(fb_ctx *ctx, int mask, const char *what) {
+  DView v = ctx->view();
+  DForce f = ctx->fview();
+  if (ctx->canon_sp && ctx->force_kernel == 1)
+    return ctx->launch(what, [&] { return fbk_d2c_sp(v, f, mask, ctx->stream); });
+  return ctx->launch(what, [&] { return fbk_d2c(v, f, mask, ctx->stream); });
+}
+
 static int dassemble_2c(fb_ctx *ctx, int bit, const char *what) {
   if (!ctx) return FB_E_ARG;
   RC(pre_dassemble(ctx, what));
-  DView v = ctx->view();
-  DForce f = ctx->fview();
-  RC(ctx->launch(what, [&] { return fbk_d2c(v, f, 1 << bit, ctx->stream); }));
+  RC(launch_d2c(ctx, 1 << bit, what));
   RC(ctx->check_status(what));
   ctx->ddone[bit] = 1;
   return FB_OK;
@@ -1948,9 +1956,7 @@ int fb_dassemble_vna_2c(fb_ctx *ctx) {
   RC(pre_dassemble(ctx, "Dassemble_vna_2c"));
   if (!all_rho(ctx)) return ctx->fail(FB_E_ORDER, "Dassemble_vna_2c needs the density matrix (fb_set_denmat)");
   if (!ctx->done[FB_MAT_S]) return ctx->fail(FB_E_ORDER, "Dassemble_vna_2c needs assemble_S first");
-  DView v = ctx->view();
-  DForce f = ctx->fview();
-  RC(ctx->launch("Dassemble_vna_2c", [&] { return fbk_d2c(v, f, 8, ctx->stream); }));
+  RC(launch_d2c(ctx, 8, "Dassemble_vna_2c"));
   RC(ctx->check_status("Dassemble_vna_2c"));
   ctx->ddone[3] = 1;
   return FB_OK;
@@ -1959,7 +1965,10 @@ int fb_dassemble_vna_2c(fb_ctx *ctx) {
 static int run_d3c(fb_ctx *ctx) {
   DView v = ctx->view();
   DForce f = ctx->fview();
-  RC(ctx->launch("Dassemble_vna_3c", [&] { return fbk_d3c(v, f, ctx->stream); }));
+  if (ctx->canon_sp && ctx->force_kernel == 1)
+    RC(ctx->launch("Dassemble_vna_3c", [&] { return fbk_d3c_sp(v, f, any_charge(ctx), ctx->stream); }));
+  else
+    RC(ctx->launch("Dassemble_vna_3c", [&] { return fbk_d3c(v, f, ctx->stream); }));
   RC(ctx->launch("forces_3c_sum", [&] { return fbk_f3_reduce(v, f, ctx->stream); }));
   return FB_OK;
 }
@@ -1983,9 +1992,7 @@ int fb_dassemble_all(fb_ctx *ctx) {
   if (!all_rho(ctx)) return ctx->fail(FB_E_ORDER, "Dassemble_all needs the density matrix (fb_set_denmat)");
   if (!ctx->done[FB_MAT_S] || !ctx->done[FB_MAT_DIP])
     return ctx->fail(FB_E_ORDER, "Dassemble_all needs assemble_S and assemble_dipole_z first");
-  DView v = ctx->view();
-  DForce f = ctx->fview();
-  RC(ctx->launch("Dassemble_2c", [&] { return fbk_d2c(v, f, 15, ctx->stream); }));
+  RC(launch_d2c(ctx, 15, "Dassemble_2c"));
   RC(run_d3c(ctx));
   RC(ctx->check_status("Dassemble_all"));
   for (int i = 0; i < 5; i++) ctx->ddone[i] = 1;
@@ -2045,6 +2052,11 @@ int fb_set_tuning(fb_ctx *ctx, int param, int value) {
     ctx->mlp_gen++;   // captured graphs hold the old launch
     return FB_OK;
   }
+  if (param == FB_TUNE_FORCE_KERNEL) {
+    if (value != 0 && value != 1) return ctx->fail(FB_E_ARG, "force kernel must be 0 (generic) or 1 (s/p)");
+    ctx->force_kernel = value;
+    return FB_OK;
+  }
   return ctx->fail(FB_E_ARG, "unknown tuning parameter");
 }
 
@@ -2060,6 +2072,10 @@ int fb_get_tuning(fb_ctx *ctx, int param, int *value) {
   }
   if (param == FB_TUNE_GRAPH) {
     *value = ctx->use_graph ? 1 : 0;
+    return FB_OK;
+  }
+  if (param == FB_TUNE_FORCE_KERNEL) {
+    *value = ctx->force_kernel;
     return FB_OK;
   }
   return ctx->fail(FB_E_ARG, "unknown tuning parameter");

@@ -716,3 +716,873 @@ extern "C" int fbk_trip_jkey(int64_t ntrip, const int4 *trip, int *key, hipStrea
   hipLaunchKernelGGL(k_trip_jkey, dim3((unsigned)((ntrip + 255) / 256)), dim3(256), 0, st, ntrip, trip, key);
   return (int)hipGetLastError();
 }
+
+// ===========================================================================
+// s/p species with canonical tables (the usual case): compile-time block
+// shapes, closed-form Drotate, the Lagrange denominators of the host table,
+// and the frame derivatives of one Cartesian component at a time, so that
+// everything stays in registers.  The operations and their order are those of
+// the generic kernels above (terms that the generic loops add as exact zeros
+// are skipped), so the two paths agree to the last bit apart from signs of
+// zero.
+#include "fb_sp.h"
+
+#define PP_(k, m) pm[3 * (m) + (k)]
+#define DD_(k, m) dpm[3 * (m) + (k)]
+
+// one nonzero molecular element (m1, m2) of the p-p block: the three product-rule terms
+__device__ __forceinline__ double drot_pp_term(double x, const double *pm, const double *dpm, int k, int l, int m1,
+                                               int m2, double Mv, double DMv) {
+  x = x + (DD_(k, m1) * PP_(l, m2)) * Mv;
+  x = x + (PP_(k, m1) * DD_(l, m2)) * Mv;
+  x = x + (PP_(k, m1) * PP_(l, m2)) * DMv;
+  return x;
+}
+
+// Drotate of a 2c block, canonical ME order {ss, s-pz, pz-s, py-py, pz-pz, px-px}
+template <int CLS>
+__device__ __forceinline__ void drot2c_sp(const double *pm, const double *dpm, const double *F, const double *DM,
+                                          double *X) {
+  X[0] = DM[0];
+  if (CLS == CLS_SPSP) {
+#pragma unroll
+    for (int l = 0; l < 3; l++) X[(1 + l) * 4] = DD_(l, 1) * F[1] + PP_(l, 1) * DM[1];
+#pragma unroll
+    for (int k = 0; k < 3; k++) X[1 + k] = DD_(k, 1) * F[2] + PP_(k, 1) * DM[2];
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        double x = 0.0;
+        x = drot_pp_term(x, pm, dpm, k, l, 0, 0, F[3], DM[3]);
+        x = drot_pp_term(x, pm, dpm, k, l, 1, 1, F[4], DM[4]);
+        x = drot_pp_term(x, pm, dpm, k, l, 2, 2, F[5], DM[5]);
+        X[(1 + l) * 4 + 1 + k] = x;
+      }
+  } else if (CLS == CLS_SSP) {
+#pragma unroll
+    for (int l = 0; l < 3; l++) X[1 + l] = DD_(l, 1) * F[1] + PP_(l, 1) * DM[1];
+  } else if (CLS == CLS_SPS) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) X[1 + k] = DD_(k, 1) * F[1] + PP_(k, 1) * DM[1];
+  }
+}
+
+// Drotate of a 3c block, canonical ME order
+// {ss, s-pz, pz-s, py-py, pz-pz, px-px, s-px, px-s, px-pz, pz-px} (s-sp: {ss, s-pz, s-px}; sp-s: {ss, pz-s, px-s})
+template <int CLS>
+__device__ __forceinline__ void drot3c_sp(const double *pm, const double *dpm, const double *F, const double *DM,
+                                          double *X) {
+  X[0] = DM[0];
+  if (CLS == CLS_SPSP) {
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      double x = DD_(l, 1) * F[1];
+      x = x + PP_(l, 1) * DM[1];
+      x = x + DD_(l, 2) * F[6];
+      x = x + PP_(l, 2) * DM[6];
+      X[(1 + l) * 4] = x;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double x = DD_(k, 1) * F[2];
+      x = x + PP_(k, 1) * DM[2];
+      x = x + DD_(k, 2) * F[7];
+      x = x + PP_(k, 2) * DM[7];
+      X[1 + k] = x;
+    }
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        double x = 0.0;
+        x = drot_pp_term(x, pm, dpm, k, l, 0, 0, F[3], DM[3]);
+        x = drot_pp_term(x, pm, dpm, k, l, 1, 1, F[4], DM[4]);
+        x = drot_pp_term(x, pm, dpm, k, l, 2, 1, F[8], DM[8]);
+        x = drot_pp_term(x, pm, dpm, k, l, 1, 2, F[9], DM[9]);
+        x = drot_pp_term(x, pm, dpm, k, l, 2, 2, F[5], DM[5]);
+        X[(1 + l) * 4 + 1 + k] = x;
+      }
+  } else if (CLS == CLS_SSP) {
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      double x = DD_(l, 1) * F[1];
+      x = x + PP_(l, 1) * DM[1];
+      x = x + DD_(l, 2) * F[2];
+      x = x + PP_(l, 2) * DM[2];
+      X[1 + l] = x;
+    }
+  } else if (CLS == CLS_SPS) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double x = DD_(k, 1) * F[1];
+      x = x + PP_(k, 1) * DM[1];
+      x = x + DD_(k, 2) * F[2];
+      x = x + PP_(k, 2) * DM[2];
+      X[1 + k] = x;
+    }
+  }
+}
+#undef PP_
+#undef DD_
+
+// 7-point window, products and derivative weights of one grid (M_Fdata_2c.f90:470-540)
+struct DLag7 {
+  int grid, ileft;
+  double prod[7], pdenom[7], xsum[7];
+};
+
+__device__ __forceinline__ void dlag7_build(const DView &v, int grid, double x, DLag7 &L) {
+  const DGrid2c gr = v.grid2c[grid];
+  int imid = (int)(x / gr.dx) + 1;
+  int ileft = imid - 3;
+  if (ileft < 1) ileft = 1;
+  else if (imid + 3 > gr.nx) ileft = gr.nx - 6;
+  const double *pd = v.pden + gr.pden_off + (int64_t)(ileft - 1) * 49;
+  double xx[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) xx[k] = (double)(ileft + k - 1) * gr.dx;
+#pragma unroll
+  for (int is = 0; is < 7; is++) {
+    double p = 1.0, q = 1.0;
+#pragma unroll
+    for (int ip = 0; ip < 7; ip++)
+      if (ip != is) {
+        q = q * pd[is * 7 + ip];
+        p = p * (x - xx[ip]) * pd[is * 7 + ip];
+      }
+    L.prod[is] = p;
+    L.pdenom[is] = q;
+  }
+#pragma unroll
+  for (int ipt = 0; ipt < 7; ipt++) {
+    double xs = 0.0;
+#pragma unroll
+    for (int jpt = 0; jpt < 7; jpt++) {
+      if (ipt == jpt) continue;
+      double xp = 1.0;
+#pragma unroll
+      for (int kpt = 0; kpt < 7; kpt++)
+        if (kpt != jpt && kpt != ipt) xp = xp * (x - xx[kpt]);
+      xs = xs + xp;
+    }
+    L.xsum[ipt] = xs;
+  }
+  L.grid = grid;
+  L.ileft = ileft;
+}
+
+// values and x-derivatives of one canonical 2c cell (getDMEs_Fdata_2c)
+template <int NME>
+__device__ __forceinline__ int dinterp2c_sp(const DView &v, int cid, double x, DLag7 &L, double *F, double *dF) {
+  if (cid < 0) return ST_NO_TABLE;
+  const DCell2c c = v.cell2c[cid];
+  if (x < 0.0) return ST_X_NEG;
+  if (x > c.xmax * (1.0 + 1.0e-5)) return ST_X_RANGE;
+  const double *tab = v.tab2c + c.off;
+  if (x < 1.0e-5) {
+#pragma unroll
+    for (int m = 0; m < NME; m++) { F[m] = tab[m]; dF[m] = 0.0; }
+    return 0;
+  }
+  if (c.grid != L.grid) dlag7_build(v, c.grid, x, L);
+  const double *base = tab + (int64_t)(L.ileft - 1) * NME;
+#pragma unroll
+  for (int m = 0; m < NME; m++) { F[m] = 0.0; dF[m] = 0.0; }
+#pragma unroll
+  for (int is = 0; is < 7; is++)
+#pragma unroll
+    for (int m = 0; m < NME; m++) F[m] = F[m] + base[is * NME + m] * L.prod[is];
+#pragma unroll
+  for (int is = 0; is < 7; is++)
+#pragma unroll
+    for (int m = 0; m < NME; m++) dF[m] = dF[m] + base[is * NME + m] * L.pdenom[is] * L.xsum[is];
+  return 0;
+}
+
+// d/dr1 of one rotated 2c interaction, component ix, contracted: f -= rho [dq] X P_eq2
+template <int CLS>
+__device__ __forceinline__ void dcontract2c_sp(const double *pm, const double *deps, const double *eta, double z,
+                                               double zcut, const double *F, const double *dF, const double *rho,
+                                               double dq, bool use_dq, double *f) {
+  constexpr int NBL = ((CLS & 2) ? 4 : 1) * ((CLS & 1) ? 4 : 1);
+  constexpr int NME = nme2c_of(CLS);
+  double DM[NME], X[16], dpm[9];
+#pragma unroll
+  for (int ix = 0; ix < 3; ix++) {
+#pragma unroll
+    for (int m = 0; m < NME; m++) DM[m] = (z > zcut) ? -eta[ix] * dF[m] : 0.0;
+    d_dpmat(deps, ix + 1, dpm);
+    drot2c_sp<CLS>(pm, dpm, F, DM, X);
+#pragma unroll
+    for (int i = 0; i < NBL; i++) {
+      if (use_dq) f[ix] = f[ix] - rho[i] * dq * X[i] * FB_P_EQ2;
+      else f[ix] = f[ix] - rho[i] * X[i] * FB_P_EQ2;
+    }
+  }
+}
+
+template <int CLS>
+__device__ __forceinline__ int pair_d2c_sp(const DView &v, const DForce &Fo, int mask, int64_t p, int ia, int ja,
+                                           int in1, int in2, bool self, double z, const double *pm,
+                                           const double *deps, const double *eta) {
+  constexpr int NMU = (CLS & 2) ? 4 : 1, NNU = (CLS & 1) ? 4 : 1, NBL = NMU * NNU;
+  constexpr int NME = nme2c_of(CLS);
+  constexpr int CLSA = (CLS & 2) ? CLS_SPSP : CLS_SS;   // atom term on iatom's orbitals
+  constexpr int NMEA = nme2c_of(CLSA), NA = NMU * NMU;
+  int status = 0;
+  DLag7 L;
+  L.grid = -1;
+  L.ileft = 1;
+  double F[6], dF[6], X[16], DM[6], dpm[9];
+  const int64_t bo = v.boff[p];
+  const int slots[3] = {SLOT_S, SLOT_T, SLOT_DIP};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (!(mask & (1 << d))) continue;
+    int st = dinterp2c_sp<NME>(v, cell2c_of(v, in1, in2, slots[d], 0), z, L, F, dF);
+    status |= st;
+    if (st) continue;
+    double *D = Fo.dmat[d] + 3 * bo;
+#pragma unroll
+    for (int ix = 0; ix < 3; ix++) {
+#pragma unroll
+      for (int m = 0; m < NME; m++) DM[m] = (z > 1.0e-3) ? -eta[ix] * dF[m] : 0.0;
+      d_dpmat(deps, ix + 1, dpm);
+      drot2c_sp<CLS>(pm, dpm, F, DM, X);
+#pragma unroll
+      for (int i = 0; i < NBL; i++) D[ix + 3 * i] = X[i];
+    }
+  }
+  if (mask & 8) {
+    double fo[3] = {0.0, 0.0, 0.0}, fa[3] = {0.0, 0.0, 0.0};
+    if (!self) {
+      double rho[NBL];
+#pragma unroll
+      for (int i = 0; i < NBL; i++) rho[i] = Fo.rho[bo + i];
+#pragma unroll
+      for (int side = 0; side < 2; side++) {
+        const int slot = side ? SLOT_ONR : SLOT_ONL;
+        const int qa = side ? ja : ia;
+        const int inq = side ? in2 : in1;
+        int st = dinterp2c_sp<NME>(v, cell2c_of(v, in1, in2, slot, 0), z, L, F, dF);
+        status |= st;
+        if (!st) dcontract2c_sp<CLS>(pm, deps, eta, z, 1.0e-3, F, dF, rho, 0.0, false, fo);
+        for (int isorp = 1; isorp <= v.sp.nssh[inq - 1]; isorp++) {
+          const double dQ = v.dQ[qa * FB_MAXSH + isorp - 1];
+          if (dQ == 0.0) continue;
+          st = dinterp2c_sp<NME>(v, cell2c_of(v, in1, in2, slot, isorp), z, L, F, dF);
+          status |= st;
+          if (!st) dcontract2c_sp<CLS>(pm, deps, eta, z, 1.0e-3, F, dF, rho, dQ, true, fo);
+        }
+      }
+    }
+    const int64_t pself = v.noff[ia] + v.neigh_self[ia] - 1;
+    double rs[NA];
+#pragma unroll
+    for (int i = 0; i < NA; i++) rs[i] = Fo.rho[v.boff[pself] + i];
+    int st = dinterp2c_sp<NMEA>(v, cell2c_of(v, in1, in2, SLOT_ATOM, 0), z, L, F, dF);
+    status |= st;
+    if (!st) dcontract2c_sp<CLSA>(pm, deps, eta, z, 1.0e-3, F, dF, rs, 0.0, false, fa);
+    bool charged = false;
+    for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) charged |= (v.dQ[ja * FB_MAXSH + isorp - 1] != 0.0);
+    if (charged) {
+      const double rend = v.sp.rcmin[in1 - 1] + v.sp.rcmin[in2 - 1];
+      const double smooth = d_smoother(z, rend, 0.8), Dsmooth = d_Dsmoother(z, rend, 0.8);
+      const double *sself = v.mat[FB_MAT_S] + v.boff[pself];
+      for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
+        const double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
+        if (dQ == 0.0) continue;
+        st = dinterp2c_sp<NMEA>(v, cell2c_of(v, in1, in2, SLOT_ATOM, isorp), z, L, F, dF);
+        status |= st;
+        if (st) continue;
+        double bx[16];
+        rot2c_sp<double>(CLSA, pm, F, bx);
+#pragma unroll
+        for (int ix = 0; ix < 3; ix++) {
+#pragma unroll
+          for (int m = 0; m < NMEA; m++) DM[m] = (z > 1.0e-4) ? -eta[ix] * dF[m] : 0.0;
+          d_dpmat(deps, ix + 1, dpm);
+          drot2c_sp<CLSA>(pm, dpm, F, DM, X);
+#pragma unroll
+          for (int i = 0; i < NA; i++) {
+            const double emnpl = (z > 1.0e-4) ? sself[i] / z : 0.0;
+            const double vdem = (z > 1.0e-4) ? eta[ix] * sself[i] / (z * z) : 0.0;
+            fa[ix] = fa[ix] - rs[i] * FB_P_EQ2 * dQ *
+                                  (smooth * X[i] - eta[ix] * Dsmooth * bx[i] + (1.0 - smooth) * vdem +
+                                   eta[ix] * Dsmooth * emnpl);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      Fo.fontop[3 * p + q] = fo[q];
+      Fo.fatom[3 * p + q] = fa[q];
+    }
+  }
+  return status;
+}
+
+#define WD2S 128
+
+__global__ __launch_bounds__(WD2S) void k_d2c_sp(DView v, DForce Fo, int mask) {
+  const int64_t p = (int64_t)blockIdx.x * WD2S + threadIdx.x;
+  if (p >= v.npairs) return;
+  const int ia = v.pair_atom[p];
+  const int s = v.atom_struct[ia];
+  const int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
+  const int mb = v.neigh_b[p];
+  const int in1 = v.imass[ia], in2 = v.imass[ja];
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  double r1[3], r2[3], sighat[3], eps[9], pm[9], deps[27], eta[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    r1[q] = v.ratom[3 * ia + q];
+    r2[q] = v.ratom[3 * ja + q] + xls[3 * mb + q];
+  }
+  const double z = d_distance(r1, r2);
+  if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+  else {
+#pragma unroll
+    for (int q = 0; q < 3; q++) sighat[q] = (r2[q] - r1[q]) / z;
+  }
+  d_epsilon(r2, sighat, eps);
+  d_Depsilon_2c(r1, r2, z, eps, deps);
+  d_pmat<double>(eps, pm);
+#pragma unroll
+  for (int q = 0; q < 3; q++) eta[q] = eps[6 + q];
+  const bool self = (ja == ia && mb == 0);
+  const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
+  int status;
+  switch (cls) {
+    case 3: status = pair_d2c_sp<3>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta); break;
+    case 2: status = pair_d2c_sp<2>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta); break;
+    case 1: status = pair_d2c_sp<1>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta); break;
+    default: status = pair_d2c_sp<0>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta); break;
+  }
+  if (status) atomicOr(v.status, status);
+}
+
+// ---------------------------------------------------------------- 3c, s/p
+
+// Depsilon_3c restricted to one Cartesian component ix (1-based): the
+// (ix, :, :) slices of dera3 and der13 depend on that component only
+struct D3Scal {
+  double crossmag, r2mag2, r1mag2, ratmmag2, r1dotr2, r2dotratm, r1dotratm;
+  bool zero;
+};
+
+__device__ __forceinline__ void d3_scal(const double *r1, const double *r2, const double *r21, const double *ratm,
+                                        const double *rnabc, D3Scal &S) {
+  double ca0 = r21[1] * rnabc[2] - r21[2] * rnabc[1];
+  double ca1 = r21[2] * rnabc[0] - r21[0] * rnabc[2];
+  double ca2 = r21[0] * rnabc[1] - r21[1] * rnabc[0];
+  S.crossmag = sqrt(ca0 * ca0 + ca1 * ca1 + ca2 * ca2);
+  S.zero = fabs(S.crossmag) < 1.0e-3;
+  S.r2mag2 = r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2];
+  S.r1mag2 = r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2];
+  S.ratmmag2 = ratm[0] * ratm[0] + ratm[1] * ratm[1] + ratm[2] * ratm[2];
+  S.r1dotr2 = r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2];
+  S.r2dotratm = r2[0] * ratm[0] + r2[1] * ratm[1] + r2[2] * ratm[2];
+  S.r1dotratm = r1[0] * ratm[0] + r1[1] * ratm[1] + r1[2] * ratm[2];
+}
+
+// dA[imu + 3*(k-1)] = dera3(ix, imu, k), dB = der13(ix, imu, k)
+__device__ __forceinline__ void d3_slice(const double *r1, const double *r2, double z, const double *ratm,
+                                         const double *eps3, const D3Scal &S, int ix, double *dA, double *dB) {
+#define E3(i, j) eps3[3 * ((j) - 1) + (i) - 1]
+#define SA(imu, k) dA[((imu) - 1) + 3 * ((k) - 1)]
+#define SB(imu, k) dB[((imu) - 1) + 3 * ((k) - 1)]
+#pragma unroll
+  for (int i = 0; i < 9; i++) { dA[i] = 0.0; dB[i] = 0.0; }
+  if (S.zero) return;
+  const double cm = S.crossmag;
+#pragma unroll
+  for (int imu = 1; imu <= 3; imu++) {
+    double sum = d_levi(ix, imu, 1) * (r2[0] - r1[0]) + d_levi(ix, imu, 2) * (r2[1] - r1[1]) +
+                 d_levi(ix, imu, 3) * (r2[2] - r1[2]);
+    SA(imu, 2) = (1.0 / cm) * (sum - (E3(imu, 2) / cm) *
+                                         ((S.r1mag2 + S.r2mag2 - 2.0 * S.r1dotr2) * ratm[ix - 1] +
+                                          (S.r1dotr2 - S.r1mag2 - S.r2dotratm + S.r1dotratm) * r2[ix - 1] +
+                                          (S.r1dotr2 - S.r2mag2 + S.r2dotratm - S.r1dotratm) * r1[ix - 1]));
+  }
+  SA(1, 1) = E3(3, 3) * SA(2, 2) - E3(2, 3) * SA(3, 2);
+  SA(2, 1) = E3(1, 3) * SA(3, 2) - E3(3, 3) * SA(1, 2);
+  SA(3, 1) = E3(2, 3) * SA(1, 2) - E3(1, 3) * SA(2, 2);
+#pragma unroll
+  for (int imu = 1; imu <= 3; imu++) {
+    SB(imu, 3) = (1.0 / z) * (E3(imu, 3) * E3(ix, 3) - d_delk(imu, ix));
+    double sum = d_levi(ix, imu, 1) * (ratm[0] - r2[0]) + d_levi(ix, imu, 2) * (ratm[1] - r2[1]) +
+                 d_levi(ix, imu, 3) * (ratm[2] - r2[2]);
+    SB(imu, 2) = (1.0 / cm) * (sum - (E3(imu, 2) / cm) *
+                                         ((S.r2dotratm - S.r1dotratm + S.r1dotr2 - S.r2mag2) * ratm[ix - 1] +
+                                          (S.r2dotratm + S.r1dotratm - S.r1dotr2 - S.ratmmag2) * r2[ix - 1] +
+                                          (S.ratmmag2 - 2.0 * S.r2dotratm + S.r2mag2) * r1[ix - 1]));
+  }
+  SB(1, 1) = E3(3, 3) * SB(2, 2) - E3(3, 2) * SB(2, 3) - E3(2, 3) * SB(3, 2) + E3(2, 2) * SB(3, 3);
+  SB(2, 1) = E3(1, 3) * SB(3, 2) - E3(1, 2) * SB(3, 3) - E3(3, 3) * SB(1, 2) + E3(3, 2) * SB(1, 3);
+  SB(3, 1) = E3(2, 3) * SB(1, 2) - E3(2, 2) * SB(1, 3) - E3(1, 3) * SB(2, 2) + E3(1, 2) * SB(2, 3);
+#undef E3
+#undef SA
+#undef SB
+}
+
+// Dtwister p part from a slice s[imu + 3*(k-1)] = deps(ix, imu, k)
+__device__ __forceinline__ void dpmat_slice(const double *s, double *dpm) {
+#define SL(imu, k) s[((imu) - 1) + 3 * ((k) - 1)]
+  dpm[0] = SL(2, 2); dpm[3] = SL(2, 3); dpm[6] = SL(2, 1);
+  dpm[1] = SL(3, 2); dpm[4] = SL(3, 3); dpm[7] = SL(3, 1);
+  dpm[2] = SL(1, 2); dpm[5] = SL(1, 3); dpm[8] = SL(1, 1);
+#undef SL
+}
+
+// getDMEs_Fdata_3c of one canonical group: values and d/dcos, d/dx, d/dy
+template <int NME>
+__device__ __forceinline__ int dgetD3c_sp(const DView &v, int gid, double x, double y, double cost, double *F,
+                                          double *DpF, double *DxF, double *DyF) {
+  if (gid < 0) return ST_NO_TABLE;
+  const DGroup3c g = v.grp3c[gid];
+  const int nx = g.nx, ny = g.ny;
+  double p[5], Dp[5];
+  const double cost2 = cost * cost;
+  const double sint = sqrt(fmax(1.0e-5, 1.0 - cost2));
+  p[0] = 1.0;
+  p[1] = cost;
+  p[2] = 0.5 * (3.0 * cost2 - 1.0);
+  p[3] = 0.5 * cost * (5.0 * cost2 - 3.0);
+  p[4] = 0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2));
+  Dp[0] = 0.0;
+  Dp[1] = 1.0;
+  Dp[2] = 3.0 * cost;
+  Dp[3] = (15.0 * cost2 - 3.0) / 2.0;
+  Dp[4] = 0.5 * cost * (35.0 * cost2 - 15.0);
+  const double dx = g.xmax / (nx - 1.0), dy = g.ymax / (ny - 1.0);
+  int ix = (int)(x / dx) + 1, iy = (int)(y / dy) + 1;
+  if (ix < 2) ix = 2;
+  if (iy < 2) iy = 2;
+  if (ix > nx - 2) ix = nx - 2;
+  if (iy > ny - 2) iy = ny - 2;
+  const double px = x / dx - ix + 1.0, py = y / dy - iy + 1.0;
+  const double *tab = v.tab3c + g.off;
+  const int64_t tstride = (int64_t)ny * nx * NME;
+#define T3S(th, i, j, m) tab[(th) * tstride + ((int64_t)((j) - 1) * nx + ((i) - 1)) * NME + (m)]
+  const double gx = (T3S(0, ix + 1, iy, 0) - T3S(0, ix, iy, 0)) / dx;
+  const double gy = (T3S(0, ix, iy + 1, 0) - T3S(0, ix, iy, 0)) / dy;
+  const bool three = (gx * gx + gy * gy) < 1.0e-6;
+#pragma unroll
+  for (int m = 0; m < NME; m++) { F[m] = 0.0; DpF[m] = 0.0; DxF[m] = 0.0; DyF[m] = 0.0; }
+  for (int th = 0; th < FB_NTHETA; th++) {
+    const double pt = p[th], dpt = Dp[th];
+#pragma unroll
+    for (int m = 0; m < NME; m++) {
+      if (three) {
+        const double interim =
+            ((1.0 - px - py) * T3S(th, ix, iy, m) + px * T3S(th, ix + 1, iy, m)) + py * T3S(th, ix, iy + 1, m);
+        F[m] = F[m] + pt * interim;
+        DpF[m] = DpF[m] + dpt * interim;
+        DxF[m] = DxF[m] + pt * gx;
+        DyF[m] = DyF[m] + pt * gy;
+      } else {
+        double gk[4], gp[4];
+#pragma unroll
+        for (int k = -1; k <= 2; k++) {
+          double f1m1 = T3S(th, ix + k, iy - 1, m);
+          double f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+          double f0p3 = 3.0 * T3S(th, ix + k, iy, m), f0p6 = 2.0 * f0p3;
+          double f1p3 = 3.0 * T3S(th, ix + k, iy + 1, m), f1p6 = 2.0 * f1p3;
+          double f2p1 = T3S(th, ix + k, iy + 2, m);
+          double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+          double bb2 = f1m3 - f0p6 + f1p3;
+          double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+          double bb0 = f0p6;
+          gk[k + 1] = ((bb3 * py + bb2) * py + bb1) * py + bb0;
+          gp[k + 1] = ((3.0 * bb3 * py + 2.0 * bb2) * py + bb1);
+        }
+        double f1m1 = gk[0], f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+        double f0p3 = 3.0 * gk[1], f0p6 = 2.0 * f0p3;
+        double f1p3 = 3.0 * gk[2], f1p6 = 2.0 * f1p3;
+        double f2p1 = gk[3];
+        double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+        double bb2 = f1m3 - f0p6 + f1p3;
+        double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+        double bb0 = f0p6;
+        const double interim = (((bb3 * px + bb2) * px + bb1) * px + bb0) / 36.0;
+        F[m] = F[m] + pt * interim;
+        DpF[m] = DpF[m] + dpt * interim;
+        DxF[m] = DxF[m] + pt * ((3.0 * bb3 * px + 2.0 * bb2) * px + bb1) / (36.0 * dx);
+        f1m1 = gp[0]; f1m2 = 2.0 * f1m1; f1m3 = 3.0 * f1m1;
+        f0p3 = 3.0 * gp[1]; f0p6 = 2.0 * f0p3;
+        f1p3 = 3.0 * gp[2]; f1p6 = 2.0 * f1p3;
+        f2p1 = gp[3];
+        bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+        bb2 = f1m3 - f0p6 + f1p3;
+        bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+        bb0 = f0p6;
+        DyF[m] = DyF[m] + pt * (((bb3 * px + bb2) * px + bb1) * px + bb0) / (36.0 * dy);
+      }
+    }
+  }
+#undef T3S
+  constexpr int NM0 = (NME == 10) ? 6 : (NME == 3 ? 2 : 1);
+#pragma unroll
+  for (int m = NM0; m < NME; m++) {
+    DpF[m] = DpF[m] * sint - cost * F[m] / sint;
+    F[m] = F[m] * sint;
+    DxF[m] = DxF[m] * sint;
+    DyF[m] = DyF[m] * sint;
+  }
+  return 0;
+}
+
+struct DGeo3s {
+  double r1[3], r2[3], rna[3], r21[3], rnabc[3], sighat[3], rhat[3], amt[3], bmt[3];
+  double z, x, cost, d13, d23;
+  double eps[9];
+};
+
+__device__ __forceinline__ void d_dgeo3s(const DView &v, int4 tr, bool charged, DGeo3s &G) {
+  const int ial = tr.x, gia = tr.y, gja = tr.z;
+  const int ib = tr.w & 0xffff, jb = (tr.w >> 16) & 0xffff;
+  const int s = v.atom_struct[ial];
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    G.r1[k] = v.ratom[3 * gia + k] + xls[3 * ib + k];
+    G.r2[k] = v.ratom[3 * gja + k] + xls[3 * jb + k];
+    G.rna[k] = v.ratom[3 * ial + k];
+    G.r21[k] = G.r2[k] - G.r1[k];
+  }
+  G.z = d_distance(G.r1, G.r2);
+  if (G.z < 1.0e-5) { G.sighat[0] = 0.0; G.sighat[1] = 0.0; G.sighat[2] = 1.0; }
+  else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) G.sighat[k] = G.r21[k] / G.z;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) G.rnabc[k] = G.rna[k] - (G.r1[k] + G.r21[k] / 2.0);
+  G.x = sqrt(G.rnabc[0] * G.rnabc[0] + G.rnabc[1] * G.rnabc[1] + G.rnabc[2] * G.rnabc[2]);
+  G.d13 = d_distance(G.rna, G.r1);
+  G.d23 = d_distance(G.rna, G.r2);
+  if (G.x < 1.0e-5) { G.rhat[0] = 0.0; G.rhat[1] = 0.0; G.rhat[2] = 0.0; }
+  else if (charged) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) G.rhat[k] = (G.rna[k] - 0.5 * (G.r1[k] + G.r2[k])) / G.x;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) G.rhat[k] = G.rnabc[k] / G.x;
+  }
+  G.cost = G.sighat[0] * G.rhat[0] + G.sighat[1] * G.rhat[1] + G.sighat[2] * G.rhat[2];
+  d_epsilon(G.rhat, G.sighat, G.eps);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    G.amt[k] = (G.x > 1.0e-5) ? (G.sighat[k] - G.cost * G.rhat[k]) / G.x : 0.0;
+    G.bmt[k] = (G.cost * G.sighat[k] - G.rhat[k]) / G.z;
+  }
+}
+
+// element (row, col) of the rotated derivative of a 3c block: drot3c_sp one
+// element at a time (same terms, same order), so that a triplet never holds a
+// whole rotated block
+template <int CLS>
+__device__ __forceinline__ double drot3c_el(int row, int col, const double *pm, const double *dpm, const double *F,
+                                            const double *DM) {
+#define PP_(k, m) pm[3 * (m) + (k)]
+#define DD_(k, m) dpm[3 * (m) + (k)]
+  if (row == 0 && col == 0) return DM[0];
+  if (CLS == CLS_SPSP) {
+    if (row == 0) {
+      const int l = col - 1;
+      double x = DD_(l, 1) * F[1];
+      x = x + PP_(l, 1) * DM[1];
+      x = x + DD_(l, 2) * F[6];
+      return x + PP_(l, 2) * DM[6];
+    }
+    if (col == 0) {
+      const int k = row - 1;
+      double x = DD_(k, 1) * F[2];
+      x = x + PP_(k, 1) * DM[2];
+      x = x + DD_(k, 2) * F[7];
+      return x + PP_(k, 2) * DM[7];
+    }
+    const int k = row - 1, l = col - 1;
+    double x = 0.0;
+    x = drot_pp_term(x, pm, dpm, k, l, 0, 0, F[3], DM[3]);
+    x = drot_pp_term(x, pm, dpm, k, l, 1, 1, F[4], DM[4]);
+    x = drot_pp_term(x, pm, dpm, k, l, 2, 1, F[8], DM[8]);
+    x = drot_pp_term(x, pm, dpm, k, l, 1, 2, F[9], DM[9]);
+    return drot_pp_term(x, pm, dpm, k, l, 2, 2, F[5], DM[5]);
+  }
+  if (CLS == CLS_SSP) {
+    const int l = col - 1;
+    double x = DD_(l, 1) * F[1];
+    x = x + PP_(l, 1) * DM[1];
+    x = x + DD_(l, 2) * F[2];
+    return x + PP_(l, 2) * DM[2];
+  }
+  if (CLS == CLS_SPS) {
+    const int k = row - 1;
+    double x = DD_(k, 1) * F[1];
+    x = x + PP_(k, 1) * DM[1];
+    x = x + DD_(k, 2) * F[2];
+    return x + PP_(k, 2) * DM[2];
+  }
+  return 0.0;
+}
+
+// element (row, col) of the rotated 3c value block (rot3c_sp)
+template <int CLS>
+__device__ __forceinline__ double rot3c_el(int row, int col, const double *pm, const double *F) {
+  if (row == 0 && col == 0) return F[0];
+  if (CLS == CLS_SPSP) {
+    if (row == 0) return F[1] * PP_(col - 1, 1) + F[6] * PP_(col - 1, 2);
+    if (col == 0) return PP_(row - 1, 1) * F[2] + PP_(row - 1, 2) * F[7];
+    const int k = row - 1, l = col - 1;
+    double x = (PP_(k, 0) * F[3]) * PP_(l, 0);
+    x = x + (PP_(k, 1) * F[4]) * PP_(l, 1);
+    x = x + (PP_(k, 2) * F[8]) * PP_(l, 1);
+    x = x + (PP_(k, 1) * F[9]) * PP_(l, 2);
+    return x + (PP_(k, 2) * F[5]) * PP_(l, 2);
+  }
+  if (CLS == CLS_SSP) return F[1] * PP_(col - 1, 1) + F[2] * PP_(col - 1, 2);
+  if (CLS == CLS_SPS) return PP_(row - 1, 1) * F[1] + PP_(row - 1, 2) * F[2];
+  return 0.0;
+#undef PP_
+#undef DD_
+}
+
+#define T3N 64
+#define NI3 (4 * NME3F)   // staged interpolation values per triplet: F, dF/dcos, dF/dx, dF/dy
+
+// Interpolation of one group into the thread's LDS column: sI[q * NME + m][lane]
+template <int NME>
+__device__ __forceinline__ int stage_d3c(const DView &v, int gid, double x, double y, double cost,
+                                         double (*sI)[T3N], int lane) {
+  double F[NME], DpF[NME], DxF[NME], DyF[NME];
+  const int st = dgetD3c_sp<NME>(v, gid, x, y, cost, F, DpF, DxF, DyF);
+#pragma unroll
+  for (int m = 0; m < NME; m++) {
+    sI[m][lane] = F[m];
+    sI[NME + m][lane] = DpF[m];
+    sI[2 * NME + m][lane] = DxF[m];
+    sI[3 * NME + m][lane] = DyF[m];
+  }
+  return st;
+}
+
+// (x, z, cos theta) of a triplet: the interpolation arguments only
+__device__ __forceinline__ void xyc3(const DView &v, int4 tr, bool charged, double &x, double &z, double &cost) {
+  DGeo3s G;
+  d_dgeo3s(v, tr, charged, G);
+  x = G.x;
+  z = G.z;
+  cost = G.cost;
+}
+
+// neutral bcna forces of one triplet from its staged interpolation
+template <int CLS>
+__device__ __forceinline__ void force3c_neutral(const DView &v, const DForce &Fo, int4 tr, int64_t bo,
+                                                double (*sI)[T3N], int lane, double *fa, double *fb, double *fc) {
+  constexpr int NME = nme3c_of(CLS);
+  constexpr int NMU = (CLS & 2) ? 4 : 1, NNU = (CLS & 1) ? 4 : 1;
+  DGeo3s G;
+  d_dgeo3s(v, tr, false, G);
+  D3Scal S;
+  d3_scal(G.r1, G.r2, G.r21, G.rna, G.rnabc, S);
+  double pm[9];
+  d_pmat<double>(G.eps, pm);
+#pragma unroll 1
+  for (int ix = 0; ix < 3; ix++) {
+    double sA[9], sB[9], dpA[9], dpB[9], F[NME], Ma[NME], Mb[NME];
+    d3_slice(G.r1, G.r2, G.z, G.rna, G.eps, S, ix + 1, sA, sB);
+    dpmat_slice(sA, dpA);
+    dpmat_slice(sB, dpB);
+#pragma unroll
+    for (int m = 0; m < NME; m++) {
+      F[m] = sI[m][lane];
+      const double dp = sI[NME + m][lane];
+      Ma[m] = G.rhat[ix] * sI[2 * NME + m][lane] + G.amt[ix] * dp;
+      Mb[m] = -G.sighat[ix] * sI[3 * NME + m][lane] + G.bmt[ix] * dp - Ma[m] / 2.0;
+    }
+    double a = fa[ix], b = fb[ix], c = fc[ix];
+#pragma unroll
+    for (int col = 0; col < NNU; col++)
+#pragma unroll
+      for (int row = 0; row < NMU; row++) {
+        const double xa = drot3c_el<CLS>(row, col, pm, dpA, F, Ma);
+        const double xb = drot3c_el<CLS>(row, col, pm, dpB, F, Mb);
+        const double r = Fo.rho[bo + row + NMU * col];
+        const double xc = -xa - xb;
+        a = a - r * xa * FB_P_EQ2;
+        b = b - r * xb * FB_P_EQ2;
+        c = c - r * xc * FB_P_EQ2;
+      }
+    fa[ix] = a;
+    fb[ix] = b;
+    fc[ix] = c;
+  }
+}
+
+__global__ __launch_bounds__(T3N) void k_d3c_sp(DView v, DForce Fo) {
+  __shared__ double sI[NI3][T3N];
+  const int lane = threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * T3N + lane;
+  if (t >= v.ntrip) return;
+  const int4 tr = Fo.trip[t];
+  const int64_t bo = v.boff[Fo.tpair[t]];
+  const int in1 = v.imass[tr.y], in2 = v.imass[tr.z], indna = v.imass[tr.x];
+  const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
+  const int gid = group3c_of(v, in1, in2, indna, 0);
+  double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0}, fc[3] = {0.0, 0.0, 0.0};
+  double x, z, cost;
+  xyc3(v, tr, false, x, z, cost);
+  int status;
+  switch (cls) {
+    case 3: status = stage_d3c<10>(v, gid, x, z, cost, sI, lane); break;
+    case 2:
+    case 1: status = stage_d3c<3>(v, gid, x, z, cost, sI, lane); break;
+    default: status = stage_d3c<1>(v, gid, x, z, cost, sI, lane); break;
+  }
+  if (!status) {
+    switch (cls) {
+      case 3: force3c_neutral<3>(v, Fo, tr, bo, sI, lane, fa, fb, fc); break;
+      case 2: force3c_neutral<2>(v, Fo, tr, bo, sI, lane, fa, fb, fc); break;
+      case 1: force3c_neutral<1>(v, Fo, tr, bo, sI, lane, fa, fb, fc); break;
+      default: force3c_neutral<0>(v, Fo, tr, bo, sI, lane, fa, fb, fc); break;
+    }
+  }
+  double *o = Fo.f3t + 9 * t;
+#pragma unroll
+  for (int k = 0; k < 3; k++) { o[k] = fa[k]; o[3 + k] = fb[k]; o[6 + k] = fc[k]; }
+  if (status) atomicOr(v.status, status);
+}
+
+// charged shells of a triplet (M_Dassemble_3c_DOGS.f90:460-776): continues the
+// triplet's neutral sums in f3t with every isorp whose dQ is not zero
+template <int CLS>
+__device__ __forceinline__ void force3c_charged(const DView &v, const DForce &Fo, int4 tr, int64_t bo, int in1,
+                                                int in2, int indna, double dQ, double (*sI)[T3N], int lane,
+                                                double *fa, double *fb, double *fc) {
+  constexpr int NME = nme3c_of(CLS);
+  constexpr int NMU = (CLS & 2) ? 4 : 1, NNU = (CLS & 1) ? 4 : 1;
+  DGeo3s G;
+  d_dgeo3s(v, tr, true, G);
+  D3Scal S;
+  d3_scal(G.r1, G.r2, G.r21, G.rna, G.rnabc, S);
+  double pm[9];
+  d_pmat<double>(G.eps, pm);
+  double rA1[3] = {0.0, 0.0, 0.0}, rA2[3] = {0.0, 0.0, 0.0};
+  if (G.d13 > 1.0e-5) for (int k = 0; k < 3; k++) rA1[k] = (G.rna[k] - G.r1[k]) / G.d13;
+  if (G.d23 > 1.0e-5) for (int k = 0; k < 3; k++) rA2[k] = (G.rna[k] - G.r2[k]) / G.d23;
+  const double rc3 = v.sp.rcmin[indna - 1];
+  const double rend1 = v.sp.rcmin[in1 - 1] + rc3, rend2 = v.sp.rcmin[in2 - 1] + rc3;
+  const double st13 = d_smoother(G.d13, rend1, 0.8), dst13 = d_Dsmoother(G.d13, rend1, 0.8);
+  const double st23 = d_smoother(G.d23, rend2, 0.8), dst23 = d_Dsmoother(G.d23, rend2, 0.8);
+  const double stinky = st13 * st23;
+  const double *sb = v.mat[FB_MAT_S] + bo, *db = v.mat[FB_MAT_DIP] + bo;
+  const double *sD = Fo.dmat[0] + 3 * bo, *dD = Fo.dmat[2] + 3 * bo;
+  const double d13sq = G.d13 * G.d13, d23sq = G.d23 * G.d23;
+#pragma unroll 1
+  for (int ix = 0; ix < 3; ix++) {
+    double sA[9], sB[9], dpA[9], dpB[9], F[NME], Ma[NME], Mb[NME];
+    d3_slice(G.r1, G.r2, G.z, G.rna, G.eps, S, ix + 1, sA, sB);
+    dpmat_slice(sA, dpA);
+    dpmat_slice(sB, dpB);
+#pragma unroll
+    for (int m = 0; m < NME; m++) {
+      F[m] = sI[m][lane];
+      const double dp = sI[NME + m][lane];
+      Ma[m] = G.rhat[ix] * sI[2 * NME + m][lane] + G.amt[ix] * dp;
+      Mb[m] = -G.sighat[ix] * sI[3 * NME + m][lane] + G.bmt[ix] * dp - Ma[m] / 2.0;
+    }
+    const double dstnB = -dst13 * st23 * rA1[ix];
+    const double dstnC = -st13 * dst23 * rA2[ix];
+    const double dstnA = -dstnB - dstnC;
+    double a = fa[ix], b = fb[ix], c = fc[ix];
+#pragma unroll
+    for (int col = 0; col < NNU; col++)
+#pragma unroll
+      for (int row = 0; row < NMU; row++) {
+        const int i = row + NMU * col;
+        const double xa = drot3c_el<CLS>(row, col, pm, dpA, F, Ma);
+        const double xb = drot3c_el<CLS>(row, col, pm, dpB, F, Mb);
+        const double bx = rot3c_el<CLS>(row, col, pm, F);
+        const double sterm = sb[i] / 2.0, dterm = db[i] / G.z;
+        const double emnpl = (sterm - dterm) / G.d13 + (sterm + dterm) / G.d23;
+        const double spterm = sD[ix + 3 * i] / 2.0;
+        double dpterm = dD[ix + 3 * i] / G.z;
+        dpterm = dpterm + dterm * G.sighat[ix] / G.z;
+        const double dA = -((sterm - dterm) * rA1[ix] / d13sq) - (sterm + dterm) * rA2[ix] / d23sq;
+        const double dB = (sterm - dterm) * rA1[ix] / d13sq + (spterm - dpterm) / G.d13 + (spterm + dpterm) / G.d23;
+        const double dC = -dA - dB;
+        const double xc = -xa - xb;
+        const double w = FB_P_EQ2 * dQ * Fo.rho[bo + i];
+        a = a - w * (stinky * xa + dstnA * bx - dstnA * emnpl + (1.0 - stinky) * dA);
+        b = b - w * (stinky * xb + dstnB * bx - dstnB * emnpl + (1.0 - stinky) * dB);
+        c = c - w * (stinky * xc + dstnC * bx - dstnC * emnpl + (1.0 - stinky) * dC);
+      }
+    fa[ix] = a;
+    fb[ix] = b;
+    fc[ix] = c;
+  }
+}
+
+__global__ __launch_bounds__(T3N) void k_d3c_sp_q(DView v, DForce Fo) {
+  __shared__ double sI[NI3][T3N];
+  const int lane = threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * T3N + lane;
+  if (t >= v.ntrip) return;
+  const int4 tr = Fo.trip[t];
+  const int indna = v.imass[tr.x];
+  bool charged = false;
+  for (int isorp = 1; isorp <= v.sp.nssh[indna - 1]; isorp++) charged |= (v.dQ[tr.x * FB_MAXSH + isorp - 1] != 0.0);
+  if (!charged) return;
+  const int64_t bo = v.boff[Fo.tpair[t]];
+  const int in1 = v.imass[tr.y], in2 = v.imass[tr.z];
+  const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
+  double *o = Fo.f3t + 9 * t;
+  double fa[3], fb[3], fc[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) { fa[k] = o[k]; fb[k] = o[3 + k]; fc[k] = o[6 + k]; }
+  double x, z, cost;
+  xyc3(v, tr, true, x, z, cost);
+  int status = 0;
+  for (int isorp = 1; isorp <= v.sp.nssh[indna - 1]; isorp++) {
+    const double dQ = v.dQ[tr.x * FB_MAXSH + isorp - 1];
+    if (dQ == 0.0) continue;
+    const int gid = group3c_of(v, in1, in2, indna, isorp);
+    int st;
+    switch (cls) {
+      case 3: st = stage_d3c<10>(v, gid, x, z, cost, sI, lane); break;
+      case 2:
+      case 1: st = stage_d3c<3>(v, gid, x, z, cost, sI, lane); break;
+      default: st = stage_d3c<1>(v, gid, x, z, cost, sI, lane); break;
+    }
+    status |= st;
+    if (st) continue;
+    switch (cls) {
+      case 3: force3c_charged<3>(v, Fo, tr, bo, in1, in2, indna, dQ, sI, lane, fa, fb, fc); break;
+      case 2: force3c_charged<2>(v, Fo, tr, bo, in1, in2, indna, dQ, sI, lane, fa, fb, fc); break;
+      case 1: force3c_charged<1>(v, Fo, tr, bo, in1, in2, indna, dQ, sI, lane, fa, fb, fc); break;
+      default: force3c_charged<0>(v, Fo, tr, bo, in1, in2, indna, dQ, sI, lane, fa, fb, fc); break;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) { o[k] = fa[k]; o[3 + k] = fb[k]; o[6 + k] = fc[k]; }
+  if (status) atomicOr(v.status, status);
+}
+
+extern "C" int fbk_d2c_sp(const DView &v, const DForce &f, int mask, hipStream_t st) {
+  if (v.npairs == 0) return 0;
+  hipLaunchKernelGGL(k_d2c_sp, dim3((unsigned)((v.npairs + WD2S - 1) / WD2S)), dim3(WD2S), 0, st, v, f, mask);
+  return (int)hipGetLastError();
+}
+
+// charged = 0: the neutral pass only (every shell charge of the batch is zero)
+extern "C" int fbk_d3c_sp(const DView &v, const DForce &f, int charged, hipStream_t st) {
+  if (v.ntrip == 0) return 0;
+  const dim3 grid((unsigned)((v.ntrip + T3N - 1) / T3N));
+  hipLaunchKernelGGL(k_d3c_sp, grid, dim3(T3N), 0, st, v, f);
+  if (charged) hipLaunchKernelGGL(k_d3c_sp_q, grid, dim3(T3N), 0, st, v, f);
+  return (int)hipGetLastError();
+}
