@@ -164,6 +164,83 @@ def mlp_leg(asm, steps, warmup, vna64, table_kernel_ms, table_build_ms):
                      "flops_per_network_eval": MLP_FLOPS_PER_EVAL}}
 
 
+FP64_PEAK_TFLOPS = 78.6
+# FP64 operations per triplet of the neutral 3c force kernel (FMA counted as 2),
+# from rocprofv3 --pmc SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 over one launch on
+# the 4096-atom Si workload; committed with the counter summary
+FORCE3C_FLOPS_FILE = os.path.join(REPO, "profiles", "forces_fp64_ops.txt")
+
+
+def force3c_flops_per_triplet():
+    try:
+        with open(FORCE3C_FLOPS_FILE) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 2 and w[0] == "fp64_ops_per_triplet":
+                    return float(w[1])
+    except OSError:
+        pass
+    return None
+
+
+def forces_leg(asm, structs, steps, warmup):
+    """One force build = fb_dassemble_all (Dblocks of S, T and dipole_z, the
+    vna_atom / vna_ontop pair forces and the per-atom f3naa/f3nab/f3nac of
+    Dassemble_vna_3c, binary64) on the current H+S blocks, with a synthetic
+    symmetric density matrix; wall time over K builds and per-kernel times from
+    an event-bracketed pass."""
+    for k, st in enumerate(structs):
+        L = asm.lists(k)
+        asm.set_denmat(synth.denmat(asm.tables.species, st.imass, L["neighn"], L["neigh_j"], L["neigh_back"]), k)
+    for _ in range(warmup):
+        asm.dassemble_all()
+    asm.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        asm.dassemble_all()
+    asm.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    asm.enable_timing(True)
+    asm.timings(4096)
+    for _ in range(max(1, min(steps, 5))):
+        asm.dassemble_all()
+    asm.synchronize()
+    kern = {}
+    for name, t in asm.timings(4096):
+        kern.setdefault(name, []).append(t)
+    asm.enable_timing(False)
+    kern = {k: float(np.mean(v)) for k, v in kern.items() if k != "clear_status"}
+    ntrip = sum(asm.list_sizes(k)[1] for k in range(len(structs)))
+    out = {"ms_per_step": ms, "triplets_per_s": ntrip / (ms * 1e-3), "kernels_ms": kern, "dtype": "f64"}
+    fl = force3c_flops_per_triplet()
+    k3 = kern.get("Dassemble_vna_3c")
+    if fl and k3:
+        tf = fl * ntrip / (k3 * 1e-3) / 1e12
+        out["roofline_3c"] = {"bound": "fp64", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": tf / FP64_PEAK_TFLOPS, "ops_per_triplet": fl}
+    return out
+
+
+def force_cpu_baseline():
+    """The reference's force assemblers (serial flang build) on the same
+    bounded sample as the H+S baseline: Dassemble_S/T/dipole_z/vna_2c/vna_3c."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import refrun
+    if not refrun.available(refrun.HARNESS_DREF):
+        return None
+    st = synth.diamond_si((4, 4, 4), sigma=0.08, seed=0)
+    ts = synth.make_tables([synth.SI])
+    lists = refrun.run(ts, st)
+    rho = synth.denmat(ts.species, st.imass, lists["neighn"], lists["neigh_j"], lists["neigh_back"])
+    nrep = 3
+    r = refrun.run_forces(ts, st, rho, lists, nrep=nrep)
+    t = sum(r["times"].values()) / nrep
+    return {"value": r["ntriplets"] / t, "unit": "triplets/s", "cores": 1, "kind": "reference",
+            "sample": f"Si512 4x4x4 diamond (sigma 0.08 A): {r['ntriplets']} triplets, force build "
+                      f"{t * 1e3:.1f} ms (Dassemble_S/T/dipole_z/vna_2c/vna_3c, serial flang -O2, mean of {nrep})",
+            "force_build_ms": t * 1e3}
+
+
 def measured_traffic(kernel, config, chunk):
     """HBM bytes per launch of `kernel` (at 3c chunk `chunk`) on workload
     `config` from the committed counter summary, or None when it holds no
@@ -232,6 +309,7 @@ def main():
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-eig", action="store_true", help="skip the Gamma-point eigenvalue shift")
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned 3c surrogate leg")
+    ap.add_argument("--no-forces", action="store_true", help="skip the force-assembler leg")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one elementwise copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -295,6 +373,17 @@ def main():
         except Exception as e:  # reported, never required
             mlp = {"error": str(e)}
 
+    forces = None
+    if not args.no_forces:
+        try:
+            forces = forces_leg(asm, structs, args.steps, args.warmup)
+            if not args.no_cpu_baseline:
+                forces["cpu_baseline"] = force_cpu_baseline()
+            print(f"# forces: {forces['ms_per_step']:.3f} ms per force build; kernels {forces['kernels_ms']}",
+                  file=sys.stderr, flush=True)
+        except Exception as e:  # reported, never required
+            forces = {"error": str(e)}
+
     # dominant kernel: the 3c launch
     k3 = [k for k in kern if k.startswith("assemble_3c")]
     dom = max(kern, key=kern.get) if not k3 else k3[0]
@@ -320,7 +409,7 @@ def main():
                    "nstructures": len(structs), "npairs": npairs, "ntriplets": ntrip,
                    "precision": args.precision, "h_s_build_ms": ms, "lists_ms": lists_ms,
                    "chunk3c": chunk, "ms_per_step_with_kernel_events": ms_events,
-                   "kernels_ms": kern, **extra, "mlp_3c": mlp,
+                   "kernels_ms": kern, **extra, "mlp_3c": mlp, "forces": forces,
                    **({"structures_per_s": len(structs) / (ms * 1e-3)} if len(structs) > 1 else {})},
         "roofline": roof,
         "cpu_baseline": cpu,

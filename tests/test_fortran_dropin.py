@@ -105,3 +105,61 @@ def test_dropin_mixed_precision_within_tolerance(golden):
     n = len(g["S"])
     for m in MATS:
         np.testing.assert_allclose(r[m][:n], g[m], rtol=0, atol=TOL_MIXED, err_msg=m)
+
+
+# ---------------------------------------------------------------- forces
+DHIP_HARNESS = os.path.join(os.path.dirname(refrun.HARNESS_REF), "fb_dharness_hip")
+need_dhip = pytest.mark.skipif(not refrun.available(DHIP_HARNESS), reason="make -C oracle hip not built")
+DKEYS = ("DS", "DT", "DDIP")
+F2KEYS = ("vna_atom", "vna_ontop")
+F3KEYS = ("f3naa", "f3nab", "f3nac")
+
+
+def _scale(a):
+    return max(1.0, float(np.max(np.abs(a)))) if np.size(a) else 1.0
+
+
+@need_dhip
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", golden_cases.NAMES)
+def test_force_dropin_matches_reference_golden(golden, name):
+    """The force harness (oracle/ref/fb_dharness.f90) linked with the drop-in
+    M_Dassemble_2c / M_Dassemble_3c: Dblocks and force pieces against the
+    reference's own Fortran (golden fixtures)."""
+    from fireball_hip import synth
+    print(f"force case {name}", flush=True)
+    g, gf = golden(name), golden(name + "_forces")
+    ts, st = golden_cases.inputs(g)
+    lists = {k: g[k] for k in ("neighn", "neigh_j", "neigh_back")}
+    rho = synth.denmat(ts.species, st.imass, lists["neighn"], lists["neigh_j"], lists["neigh_back"],
+                       seed=int(gf["denmat_seed"]))
+    r = refrun.run_forces(ts, st, rho, lists, binary=DHIP_HARNESS)
+    for k in DKEYS:
+        n = len(gf[k])
+        np.testing.assert_allclose(r[k][:n], gf[k], rtol=0, atol=1e-12 * _scale(gf[k]), err_msg=k)
+    for k in F2KEYS:
+        np.testing.assert_allclose(r[k], gf[k], rtol=0, atol=1e-12 * _scale(gf[k]), err_msg=k)
+    for k in F3KEYS:
+        np.testing.assert_allclose(r[k], gf[k], rtol=0, atol=1e-11 * _scale(gf[k]), err_msg=k)
+
+
+@need_dhip
+@pytest.mark.gpu
+def test_force_dropin_matches_reference_live_repeated():
+    """Si216 with shell charges, force assemblers run twice in one process (the
+    second pass exercises destroy_Dassemble_2c and the re-uploaded density
+    matrix), against the reference's force modules on the same input."""
+    if not refrun.available(refrun.HARNESS_DREF):
+        pytest.skip("reference force harness not built")
+    from fireball_hip import synth
+    ts = synth.make_tables([synth.SI])
+    st = synth.diamond_si((3, 3, 3), sigma=0.08, seed=5)
+    synth.random_dQ([st], [synth.SI], seed=5)
+    lists = refrun.run(ts, st)
+    rho = synth.denmat(ts.species, st.imass, lists["neighn"], lists["neigh_j"], lists["neigh_back"], seed=3)
+    ref = refrun.run_forces(ts, st, rho, lists)
+    hip = refrun.run_forces(ts, st, rho, lists, binary=DHIP_HARNESS, nrep=2)
+    for k in DKEYS + F2KEYS:
+        np.testing.assert_allclose(hip[k], ref[k], rtol=0, atol=1e-12 * _scale(ref[k]), err_msg=k)
+    for k in F3KEYS:
+        np.testing.assert_allclose(hip[k], ref[k], rtol=0, atol=1e-11 * _scale(ref[k]), err_msg=k)

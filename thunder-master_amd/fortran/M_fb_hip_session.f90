@@ -280,4 +280,55 @@
         end do
         end subroutine fb_session_store
 
+! Hand the caller's density matrix (s%denmat, the blocks the force assemblers
+! contract with) to the library, in the pair order of s%neighbors.
+        subroutine fb_session_denmat (s)
+        type(T_structure), target :: s
+        integer iatom, ineigh, nb
+        integer(c_int64_t) :: blen, off
+        real(c_double), allocatable :: rho(:)
+        call fb_check (fb_get_block_len(fbh, 0, blen), fbh, 'fb_get_block_len')
+        allocate (rho(max(blen, 1_8)))
+        off = 0
+        do iatom = 1, s%natoms
+          do ineigh = 1, s%neighbors(iatom)%neighn
+            nb = size(s%denmat(iatom)%neighbors(ineigh)%block)
+            rho(off+1:off+nb) = reshape(s%denmat(iatom)%neighbors(ineigh)%block, [nb])
+            off = off + nb
+          end do
+        end do
+        if (off .ne. blen) error stop ' fireball_hip: s%denmat does not match the block layout '
+        call fb_check (fb_set_denmat(fbh, 0, rho), fbh, 'fb_set_denmat')
+        deallocate (rho)
+        end subroutine fb_session_denmat
+
+! Allocate and fill Dblock(3, norb_mu, norb_nu) of one T_assemble_neighbors
+! array from the library's Dblocks of matrix imat.
+        subroutine fb_session_dstore (s, m, imat)
+        type(T_structure), target :: s
+        type(T_assemble_neighbors), pointer :: m(:)
+        integer(c_int), intent(in) :: imat
+        integer iatom, ineigh, jatom, nmu, nnu, nb
+        integer(c_int64_t) :: blen, off
+        real(c_double), allocatable :: d(:)
+        type(T_assemble_block), pointer :: pb
+        call fb_check (fb_get_block_len(fbh, 0, blen), fbh, 'fb_get_block_len')
+        allocate (d(max(3*blen, 1_8)))
+        call fb_check (fb_get_dblocks(fbh, 0, imat, d), fbh, 'fb_get_dblocks')
+        off = 0
+        do iatom = 1, s%natoms
+          nmu = species(s%atom(iatom)%imass)%norb_max
+          do ineigh = 1, s%neighbors(iatom)%neighn
+            jatom = s%neighbors(iatom)%neigh_j(ineigh)
+            nnu = species(s%atom(jatom)%imass)%norb_max
+            nb = 3*nmu*nnu
+            pb => m(iatom)%neighbors(ineigh)
+            allocate (pb%Dblock(3, nmu, nnu))
+            pb%Dblock = reshape(d(off+1:off+nb), [3, nmu, nnu])
+            off = off + nb
+          end do
+        end do
+        deallocate (d)
+        end subroutine fb_session_dstore
+
         end module M_fb_hip_session

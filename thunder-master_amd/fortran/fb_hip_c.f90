@@ -148,6 +148,55 @@
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx
           end function
+
+          integer(c_int) function fb_get_list_sizes (ctx, is, npairs, ntrip, mbeta_max)           &
+     &        bind(c, name='fb_get_list_sizes')
+            import :: c_ptr, c_int, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: is
+            integer(c_int64_t) :: npairs, ntrip
+            integer(c_int) :: mbeta_max
+          end function
+
+          integer(c_int) function fb_set_denmat (ctx, is, rho) bind(c, name='fb_set_denmat')
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: is
+            real(c_double) :: rho(*)
+          end function
+          integer(c_int) function fb_dassemble_S (ctx) bind(c, name='fb_dassemble_S')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+          end function
+          integer(c_int) function fb_dassemble_T (ctx) bind(c, name='fb_dassemble_T')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+          end function
+          integer(c_int) function fb_dassemble_dipole_z (ctx) bind(c, name='fb_dassemble_dipole_z')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+          end function
+          integer(c_int) function fb_dassemble_vna_2c (ctx) bind(c, name='fb_dassemble_vna_2c')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+          end function
+          integer(c_int) function fb_dassemble_vna_3c (ctx) bind(c, name='fb_dassemble_vna_3c')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+          end function
+          integer(c_int) function fb_get_dblocks (ctx, is, matrix, out) bind(c, name='fb_get_dblocks')
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: is, matrix
+            real(c_double) :: out(*)
+          end function
+          integer(c_int) function fb_get_forces (ctx, is, vna_atom, vna_ontop, f3naa, f3nab, f3nac) &
+     &        bind(c, name='fb_get_forces')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: is
+            type(c_ptr), value :: vna_atom, vna_ontop, f3naa, f3nab, f3nac
+          end function
         end interface
 
         contains
