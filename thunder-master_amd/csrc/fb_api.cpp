@@ -228,7 +228,8 @@ struct fb_ctx {
   int done[5] = {0, 0, 0, 0, 0};
 
   // forces (k.DASSEMBLERS): density matrix in, Dblocks and force pieces out
-  DevBuf d_rho, d_dmat[3], d_fatom, d_fontop, d_f3t, d_f3, d_jkey, d_jkey_s, d_iota_t, d_perm_j, d_ioff, d_joff;
+  DevBuf d_rho, d_dmat[3], d_fatom, d_fontop, d_f3a, d_f3b, d_f3c, d_f3, d_jkey, d_jkey_s, d_iota_t, d_perm_j,
+      d_inv_perm, d_inv_perm_j, d_ioff, d_joff;
   long forces_gen = -1;          // lists_gen the force buffers and orderings belong to
   std::vector<char> rho_ok;      // per structure: density blocks uploaded
   int ddone[5] = {0, 0, 0, 0, 0};  // Dblock S, T, dipole_z; vna_2c forces; vna_3c forces
@@ -278,14 +279,16 @@ struct fb_ctx {
     for (int i = 0; i < 3; i++) f.dmat[i] = d_dmat[i].as<double>();
     f.fatom = d_fatom.as<double>();
     f.fontop = d_fontop.as<double>();
-    f.f3t = d_f3t.as<double>();
+    f.f3a = d_f3a.as<double>();
+    f.f3b = d_f3b.as<double>();
+    f.f3c = d_f3c.as<double>();
     f.f3 = d_f3.as<double>();
     f.trip = d_trip_orig.as<int4>();
     f.tpair = d_tpair_orig.as<int>();
     f.toff = d_toff.as<int64_t>();
-    f.perm = d_perm.as<int>();
+    f.inv_perm = d_inv_perm.as<int>();
     f.ioff = d_ioff.as<int64_t>();
-    f.perm_j = d_perm_j.as<int>();
+    f.inv_perm_j = d_inv_perm_j.as<int>();
     f.joff = d_joff.as<int64_t>();
     return f;
   }
@@ -438,8 +441,8 @@ struct fb_ctx {
                       &d_status};
     for (DevBuf *b : bufs) b->release();
     for (auto &m : d_mat) m.release();
-    DevBuf *fbufs[] = {&d_rho, &d_dmat[0], &d_dmat[1], &d_dmat[2], &d_fatom, &d_fontop, &d_f3t, &d_f3, &d_jkey,
-                       &d_jkey_s, &d_iota_t, &d_perm_j, &d_ioff, &d_joff};
+    DevBuf *fbufs[] = {&d_rho, &d_dmat[0], &d_dmat[1], &d_dmat[2], &d_fatom, &d_fontop, &d_f3a, &d_f3b, &d_f3c,
+                       &d_f3, &d_jkey, &d_jkey_s, &d_iota_t, &d_perm_j, &d_inv_perm, &d_inv_perm_j, &d_ioff, &d_joff};
     for (DevBuf *b : fbufs) b->release();
     reset_events();
     for (auto e : pool) (void)hipEventDestroy(e);
@@ -1866,7 +1869,11 @@ static int prep_forces(fb_ctx *ctx) {
   for (int i = 0; i < 3; i++) FB_HIP(ctx->d_dmat[i].ensure(sizeof(double) * one(3 * B)));
   FB_HIP(ctx->d_fatom.ensure(sizeof(double) * one(3 * P)));
   FB_HIP(ctx->d_fontop.ensure(sizeof(double) * one(3 * P)));
-  FB_HIP(ctx->d_f3t.ensure(sizeof(double) * one(9 * T)));
+  FB_HIP(ctx->d_f3a.ensure(sizeof(double) * one(3 * T)));
+  FB_HIP(ctx->d_f3b.ensure(sizeof(double) * one(3 * T)));
+  FB_HIP(ctx->d_f3c.ensure(sizeof(double) * one(3 * T)));
+  FB_HIP(ctx->d_inv_perm.ensure(sizeof(int) * one(T)));
+  FB_HIP(ctx->d_inv_perm_j.ensure(sizeof(int) * one(T)));
   FB_HIP(ctx->d_f3.ensure(sizeof(double) * one(9 * (int64_t)N)));
   FB_HIP(ctx->d_ioff.ensure(sizeof(int64_t) * (N + 1)));
   FB_HIP(ctx->d_joff.ensure(sizeof(int64_t) * (N + 1)));
@@ -1888,6 +1895,11 @@ static int prep_forces(fb_ctx *ctx) {
     RC(ctx->launch("sort_jatom", [&] {
       return fbk_sort_pairs(ctx->d_jkey.as<int>(), ctx->d_jkey_s.as<int>(), ctx->d_iota_t.as<int>(),
                             ctx->d_perm_j.as<int>(), T, end_bit, ctx->d_tmp.p, &bytes, st);
+    }));
+    // the inverse orders: where each triplet's iatom / jatom record goes
+    RC(ctx->launch("invert_perm", [&] { return fbk_invert_perm(T, ctx->d_perm.as<int>(), ctx->d_inv_perm.as<int>(), st); }));
+    RC(ctx->launch("invert_perm", [&] {
+      return fbk_invert_perm(T, ctx->d_perm_j.as<int>(), ctx->d_inv_perm_j.as<int>(), st);
     }));
   }
   RC(ctx->launch("f3_offsets", [&] {

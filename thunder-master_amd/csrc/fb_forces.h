@@ -8,14 +8,16 @@ struct DForce {
   const double *rho;        // [blen] density blocks, block layout of the H/S blocks
   double *dmat[3];          // [3*blen] Dblock(3, norb_mu, norb_nu) of S, T, dipole_z
   double *fatom, *fontop;   // [npairs][3] forces%vna_atom / vna_ontop
-  double *f3t;              // [ntrip][9] per-triplet forces on ialpha, iatom, jatom (list order)
+  double *f3a;              // [ntrip][3] per-triplet force on ialpha, list order
+  double *f3b;              // [ntrip][3] per-triplet force on iatom, target-pair order
+  double *f3c;              // [ntrip][3] per-triplet force on jatom, jatom order
   double *f3;               // [natoms][9] f3naa, f3nab, f3nac
   const int4 *trip;         // triplets in list order
   const int *tpair;         // target pair of each triplet, list order
   const int64_t *toff;      // [natoms+1] list-order range of each ialpha
-  const int *perm;          // target-pair order -> list order
+  const int *inv_perm;      // list order -> target-pair order
   const int64_t *ioff;      // [natoms+1] range of each iatom in target-pair order
-  const int *perm_j;        // jatom order -> list order
+  const int *inv_perm_j;    // list order -> jatom order
   const int64_t *joff;      // [natoms+1] range of each jatom in jatom order
 };
 
@@ -28,4 +30,5 @@ int fbk_f3_reduce(const DView &v, const DForce &f, hipStream_t st);
 int fbk_f3_offsets(int natoms, int64_t ntrip, const int64_t *noff, const int *tpair_sorted, const int *jkey_sorted,
                    int64_t *ioff, int64_t *joff, hipStream_t st);
 int fbk_trip_jkey(int64_t ntrip, const int4 *trip, int *key, hipStream_t st);
+int fbk_invert_perm(int64_t n, const int *perm, int *inv, hipStream_t st);
 }

@@ -34,6 +34,19 @@ __device__ __forceinline__ double d_levi(int i, int j, int k) {
 }
 __device__ __forceinline__ double d_delk(int i, int j) { return i == j ? 1.0 : 0.0; }
 
+// v[k] for a k that is not a compile-time constant, without addressing a
+// private array (which would put it in scratch memory): a blend with exact
+// 0/1 weights, equal to v[k] (finite values)
+__device__ __forceinline__ double sel3(const double *v, int k) {
+  const double w0 = (k == 0) ? 1.0 : 0.0, w1 = (k == 1) ? 1.0 : 0.0, w2 = (k == 2) ? 1.0 : 0.0;
+  return (v[0] * w0 + v[1] * w1) + v[2] * w2;
+}
+__device__ __forceinline__ void put3(double *v, int k, double x) {
+  v[0] = (k == 0) ? x : v[0];
+  v[1] = (k == 1) ? x : v[1];
+  v[2] = (k == 2) ? x : v[2];
+}
+
 #define EPS_(e, i, j) (e)[3 * ((j) - 1) + (i) - 1]
 #define DEP_(d, ix, imu, k) (d)[((ix) - 1) + 3 * ((imu) - 1) + 9 * ((k) - 1)]
 
@@ -626,8 +639,12 @@ __global__ __launch_bounds__(T3D) void k_d3c(DView v, DForce F) {
       }
     }
   }
-  double *o = F.f3t + 9 * t;
-  for (int k = 0; k < 3; k++) { o[k] = fa[k]; o[3 + k] = fb[k]; o[6 + k] = fc[k]; }
+  const int64_t ob = 3 * (int64_t)F.inv_perm[t], oc = 3 * (int64_t)F.inv_perm_j[t];
+  for (int k = 0; k < 3; k++) {
+    F.f3a[3 * t + k] = fa[k];
+    F.f3b[ob + k] = fb[k];
+    F.f3c[oc + k] = fc[k];
+  }
   if (status) atomicOr(v.status, status);
 }
 
@@ -644,17 +661,18 @@ __global__ __launch_bounds__(64) void k_f3_reduce(DView v, DForce F) {
   const int a = blockIdx.x;
   const int lane = threadIdx.x;
   double s[9];
+#pragma unroll
   for (int k = 0; k < 9; k++) s[k] = 0.0;
+  // each role's records of this atom are contiguous in its own array
   for (int64_t t = F.toff[a] + lane; t < F.toff[a + 1]; t += 64)
-    for (int k = 0; k < 3; k++) s[k] = s[k] + F.f3t[9 * t + k];
-  for (int64_t q = F.ioff[a] + lane; q < F.ioff[a + 1]; q += 64) {
-    const int64_t t = F.perm[q];
-    for (int k = 0; k < 3; k++) s[3 + k] = s[3 + k] + F.f3t[9 * t + 3 + k];
-  }
-  for (int64_t q = F.joff[a] + lane; q < F.joff[a + 1]; q += 64) {
-    const int64_t t = F.perm_j[q];
-    for (int k = 0; k < 3; k++) s[6 + k] = s[6 + k] + F.f3t[9 * t + 6 + k];
-  }
+#pragma unroll
+    for (int k = 0; k < 3; k++) s[k] = s[k] + F.f3a[3 * t + k];
+  for (int64_t q = F.ioff[a] + lane; q < F.ioff[a + 1]; q += 64)
+#pragma unroll
+    for (int k = 0; k < 3; k++) s[3 + k] = s[3 + k] + F.f3b[3 * q + k];
+  for (int64_t q = F.joff[a] + lane; q < F.joff[a + 1]; q += 64)
+#pragma unroll
+    for (int k = 0; k < 3; k++) s[6 + k] = s[6 + k] + F.f3c[3 * q + k];
   for (int k = 0; k < 9; k++) {
     const double r = wave_sum(s[k]);
     if (lane == 0) F.f3[9 * a + k] = r;
@@ -679,6 +697,11 @@ __global__ void k_f3_offsets(int natoms, int64_t ntrip, const int64_t *noff, con
   if (a > natoms) return;
   ioff[a] = lower_bound_i32(tpair_sorted, ntrip, noff[a]);
   joff[a] = lower_bound_i32(jkey_sorted, ntrip, a);
+}
+
+__global__ void k_invert_perm(int64_t n, const int *perm, int *inv) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) inv[perm[s]] = (int)s;
 }
 
 __global__ void k_trip_jkey(int64_t ntrip, const int4 *trip, int *key) {
@@ -708,6 +731,12 @@ extern "C" int fbk_f3_offsets(int natoms, int64_t ntrip, const int64_t *noff, co
                               const int *jkey_sorted, int64_t *ioff, int64_t *joff, hipStream_t st) {
   hipLaunchKernelGGL(k_f3_offsets, dim3((unsigned)((natoms + 1 + 255) / 256)), dim3(256), 0, st, natoms, ntrip,
                      noff, tpair_sorted, jkey_sorted, ioff, joff);
+  return (int)hipGetLastError();
+}
+
+extern "C" int fbk_invert_perm(int64_t n, const int *perm, int *inv, hipStream_t st) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_invert_perm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, perm, inv);
   return (int)hipGetLastError();
 }
 
@@ -1098,27 +1127,29 @@ __device__ __forceinline__ void d3_slice(const double *r1, const double *r2, dou
   for (int i = 0; i < 9; i++) { dA[i] = 0.0; dB[i] = 0.0; }
   if (S.zero) return;
   const double cm = S.crossmag;
+  const double ra = sel3(ratm, ix - 1), r1x = sel3(r1, ix - 1), r2x = sel3(r2, ix - 1);
+  const double e3x = ix == 1 ? E3(1, 3) : (ix == 2 ? E3(2, 3) : E3(3, 3));
 #pragma unroll
   for (int imu = 1; imu <= 3; imu++) {
     double sum = d_levi(ix, imu, 1) * (r2[0] - r1[0]) + d_levi(ix, imu, 2) * (r2[1] - r1[1]) +
                  d_levi(ix, imu, 3) * (r2[2] - r1[2]);
     SA(imu, 2) = (1.0 / cm) * (sum - (E3(imu, 2) / cm) *
-                                         ((S.r1mag2 + S.r2mag2 - 2.0 * S.r1dotr2) * ratm[ix - 1] +
-                                          (S.r1dotr2 - S.r1mag2 - S.r2dotratm + S.r1dotratm) * r2[ix - 1] +
-                                          (S.r1dotr2 - S.r2mag2 + S.r2dotratm - S.r1dotratm) * r1[ix - 1]));
+                                         ((S.r1mag2 + S.r2mag2 - 2.0 * S.r1dotr2) * ra +
+                                          (S.r1dotr2 - S.r1mag2 - S.r2dotratm + S.r1dotratm) * r2x +
+                                          (S.r1dotr2 - S.r2mag2 + S.r2dotratm - S.r1dotratm) * r1x));
   }
   SA(1, 1) = E3(3, 3) * SA(2, 2) - E3(2, 3) * SA(3, 2);
   SA(2, 1) = E3(1, 3) * SA(3, 2) - E3(3, 3) * SA(1, 2);
   SA(3, 1) = E3(2, 3) * SA(1, 2) - E3(1, 3) * SA(2, 2);
 #pragma unroll
   for (int imu = 1; imu <= 3; imu++) {
-    SB(imu, 3) = (1.0 / z) * (E3(imu, 3) * E3(ix, 3) - d_delk(imu, ix));
+    SB(imu, 3) = (1.0 / z) * (E3(imu, 3) * e3x - d_delk(imu, ix));
     double sum = d_levi(ix, imu, 1) * (ratm[0] - r2[0]) + d_levi(ix, imu, 2) * (ratm[1] - r2[1]) +
                  d_levi(ix, imu, 3) * (ratm[2] - r2[2]);
     SB(imu, 2) = (1.0 / cm) * (sum - (E3(imu, 2) / cm) *
-                                         ((S.r2dotratm - S.r1dotratm + S.r1dotr2 - S.r2mag2) * ratm[ix - 1] +
-                                          (S.r2dotratm + S.r1dotratm - S.r1dotr2 - S.ratmmag2) * r2[ix - 1] +
-                                          (S.ratmmag2 - 2.0 * S.r2dotratm + S.r2mag2) * r1[ix - 1]));
+                                         ((S.r2dotratm - S.r1dotratm + S.r1dotr2 - S.r2mag2) * ra +
+                                          (S.r2dotratm + S.r1dotratm - S.r1dotr2 - S.ratmmag2) * r2x +
+                                          (S.ratmmag2 - 2.0 * S.r2dotratm + S.r2mag2) * r1x));
   }
   SB(1, 1) = E3(3, 3) * SB(2, 2) - E3(3, 2) * SB(2, 3) - E3(2, 3) * SB(3, 2) + E3(2, 2) * SB(3, 3);
   SB(2, 1) = E3(1, 3) * SB(3, 2) - E3(1, 2) * SB(3, 3) - E3(3, 3) * SB(1, 2) + E3(3, 2) * SB(1, 3);
@@ -1144,19 +1175,15 @@ __device__ __forceinline__ int dgetD3c_sp(const DView &v, int gid, double x, dou
   if (gid < 0) return ST_NO_TABLE;
   const DGroup3c g = v.grp3c[gid];
   const int nx = g.nx, ny = g.ny;
-  double p[5], Dp[5];
+  // Legendre P0..P4 and derivatives, selected per theta below (no private array)
   const double cost2 = cost * cost;
   const double sint = sqrt(fmax(1.0e-5, 1.0 - cost2));
-  p[0] = 1.0;
-  p[1] = cost;
-  p[2] = 0.5 * (3.0 * cost2 - 1.0);
-  p[3] = 0.5 * cost * (5.0 * cost2 - 3.0);
-  p[4] = 0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2));
-  Dp[0] = 0.0;
-  Dp[1] = 1.0;
-  Dp[2] = 3.0 * cost;
-  Dp[3] = (15.0 * cost2 - 3.0) / 2.0;
-  Dp[4] = 0.5 * cost * (35.0 * cost2 - 15.0);
+  const double p2 = 0.5 * (3.0 * cost2 - 1.0);
+  const double p3 = 0.5 * cost * (5.0 * cost2 - 3.0);
+  const double p4 = 0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2));
+  const double dp2 = 3.0 * cost;
+  const double dp3 = (15.0 * cost2 - 3.0) / 2.0;
+  const double dp4 = 0.5 * cost * (35.0 * cost2 - 15.0);
   const double dx = g.xmax / (nx - 1.0), dy = g.ymax / (ny - 1.0);
   int ix = (int)(x / dx) + 1, iy = (int)(y / dy) + 1;
   if (ix < 2) ix = 2;
@@ -1173,7 +1200,8 @@ __device__ __forceinline__ int dgetD3c_sp(const DView &v, int gid, double x, dou
 #pragma unroll
   for (int m = 0; m < NME; m++) { F[m] = 0.0; DpF[m] = 0.0; DxF[m] = 0.0; DyF[m] = 0.0; }
   for (int th = 0; th < FB_NTHETA; th++) {
-    const double pt = p[th], dpt = Dp[th];
+    const double pt = th == 0 ? 1.0 : (th == 1 ? cost : (th == 2 ? p2 : (th == 3 ? p3 : p4)));
+    const double dpt = th == 0 ? 0.0 : (th == 1 ? 1.0 : (th == 2 ? dp2 : (th == 3 ? dp3 : dp4)));
 #pragma unroll
     for (int m = 0; m < NME; m++) {
       if (three) {
@@ -1351,225 +1379,345 @@ __device__ __forceinline__ double rot3c_el(int row, int col, const double *pm, c
 #undef DD_
 }
 
-#define T3N 64
-#define NI3 (4 * NME3F)   // staged interpolation values per triplet: F, dF/dcos, dF/dx, dF/dy
+// ---------------------------------------------------------------------------
+// Four lanes per triplet.  Phase A: lane q interpolates matrix elements q, q+4,
+// q+8 of the group (value and its cos, x and y derivatives) into the
+// triplet's LDS row.  Phase B: lane q < 3 takes Cartesian component q: the
+// frame-derivative slices of that component, f3naMa / f3naMb, the two rotated
+// derivative blocks element by element, and the contraction with the density
+// block, so that no lane loops over components and every private value has a
+// compile-time index.  Lane 3 idles in phase B.
+#define T3Q 64                 // lanes per workgroup: 16 triplets
+#define SIROW (4 * NME3F + 1)  // LDS row per triplet (odd stride: no bank sharing)
 
-// Interpolation of one group into the thread's LDS column: sI[q * NME + m][lane]
-template <int NME>
-__device__ __forceinline__ int stage_d3c(const DView &v, int gid, double x, double y, double cost,
-                                         double (*sI)[T3N], int lane) {
-  double F[NME], DpF[NME], DxF[NME], DyF[NME];
-  const int st = dgetD3c_sp<NME>(v, gid, x, y, cost, F, DpF, DxF, DyF);
+// getDMEs_Fdata_3c of elements q, q+4, q+8 of one canonical group into row
+__device__ inline int stage_q(const DView &v, int gid, int nme, double x, double y, double cost, double *row, int q) {
+  if (gid < 0) return ST_NO_TABLE;
+  const DGroup3c g = v.grp3c[gid];
+  const int nx = g.nx, ny = g.ny;
+  const double cost2 = cost * cost;
+  const double sint = sqrt(fmax(1.0e-5, 1.0 - cost2));
+  const double p2 = 0.5 * (3.0 * cost2 - 1.0);
+  const double p3 = 0.5 * cost * (5.0 * cost2 - 3.0);
+  const double p4 = 0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2));
+  const double dp2 = 3.0 * cost;
+  const double dp3 = (15.0 * cost2 - 3.0) / 2.0;
+  const double dp4 = 0.5 * cost * (35.0 * cost2 - 15.0);
+  const double dx = g.xmax / (nx - 1.0), dy = g.ymax / (ny - 1.0);
+  int ix = (int)(x / dx) + 1, iy = (int)(y / dy) + 1;
+  if (ix < 2) ix = 2;
+  if (iy < 2) iy = 2;
+  if (ix > nx - 2) ix = nx - 2;
+  if (iy > ny - 2) iy = ny - 2;
+  const double px = x / dx - ix + 1.0, py = y / dy - iy + 1.0;
+  const double *tab = v.tab3c + g.off;
+  const int64_t tstride = (int64_t)ny * nx * nme;
+#define T3Q_(th, i, j, m) tab[(th) * tstride + ((int64_t)((j) - 1) * nx + ((i) - 1)) * nme + (m)]
+  const double gx = (T3Q_(0, ix + 1, iy, 0) - T3Q_(0, ix, iy, 0)) / dx;
+  const double gy = (T3Q_(0, ix, iy + 1, 0) - T3Q_(0, ix, iy, 0)) / dy;
+  const bool three = (gx * gx + gy * gy) < 1.0e-6;
+  const int nm0 = (nme == 10) ? 6 : (nme == 3 ? 2 : 1);   // canonical order: mvalue = 1 elements last
+  for (int m = q; m < nme; m += 4) {
+    double F = 0.0, DpF = 0.0, DxF = 0.0, DyF = 0.0;
+    for (int th = 0; th < FB_NTHETA; th++) {
+      const double pt = th == 0 ? 1.0 : (th == 1 ? cost : (th == 2 ? p2 : (th == 3 ? p3 : p4)));
+      const double dpt = th == 0 ? 0.0 : (th == 1 ? 1.0 : (th == 2 ? dp2 : (th == 3 ? dp3 : dp4)));
+      if (three) {
+        const double interim =
+            ((1.0 - px - py) * T3Q_(th, ix, iy, m) + px * T3Q_(th, ix + 1, iy, m)) + py * T3Q_(th, ix, iy + 1, m);
+        F = F + pt * interim;
+        DpF = DpF + dpt * interim;
+        DxF = DxF + pt * gx;
+        DyF = DyF + pt * gy;
+      } else {
+        double gk[4], gp[4];
+#pragma unroll
+        for (int k = -1; k <= 2; k++) {
+          double f1m1 = T3Q_(th, ix + k, iy - 1, m);
+          double f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+          double f0p3 = 3.0 * T3Q_(th, ix + k, iy, m), f0p6 = 2.0 * f0p3;
+          double f1p3 = 3.0 * T3Q_(th, ix + k, iy + 1, m), f1p6 = 2.0 * f1p3;
+          double f2p1 = T3Q_(th, ix + k, iy + 2, m);
+          double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+          double bb2 = f1m3 - f0p6 + f1p3;
+          double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+          double bb0 = f0p6;
+          gk[k + 1] = ((bb3 * py + bb2) * py + bb1) * py + bb0;
+          gp[k + 1] = ((3.0 * bb3 * py + 2.0 * bb2) * py + bb1);
+        }
+        double f1m1 = gk[0], f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+        double f0p3 = 3.0 * gk[1], f0p6 = 2.0 * f0p3;
+        double f1p3 = 3.0 * gk[2], f1p6 = 2.0 * f1p3;
+        double f2p1 = gk[3];
+        double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+        double bb2 = f1m3 - f0p6 + f1p3;
+        double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+        double bb0 = f0p6;
+        const double interim = (((bb3 * px + bb2) * px + bb1) * px + bb0) / 36.0;
+        F = F + pt * interim;
+        DpF = DpF + dpt * interim;
+        DxF = DxF + pt * ((3.0 * bb3 * px + 2.0 * bb2) * px + bb1) / (36.0 * dx);
+        f1m1 = gp[0]; f1m2 = 2.0 * f1m1; f1m3 = 3.0 * f1m1;
+        f0p3 = 3.0 * gp[1]; f0p6 = 2.0 * f0p3;
+        f1p3 = 3.0 * gp[2]; f1p6 = 2.0 * f1p3;
+        f2p1 = gp[3];
+        bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+        bb2 = f1m3 - f0p6 + f1p3;
+        bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+        bb0 = f0p6;
+        DyF = DyF + pt * (((bb3 * px + bb2) * px + bb1) * px + bb0) / (36.0 * dy);
+      }
+    }
+#undef T3Q_
+    if (m >= nm0) {
+      DpF = DpF * sint - cost * F / sint;
+      F = F * sint;
+      DxF = DxF * sint;
+      DyF = DyF * sint;
+    }
+    row[m] = F;
+    row[nme + m] = DpF;
+    row[2 * nme + m] = DxF;
+    row[3 * nme + m] = DyF;
+  }
+  return 0;
+}
+
+// the scalars of one Cartesian component q (0..2) of a triplet's geometry
+struct DGeoQ {
+  double rh, am, sg, bm;              // rhat, amt, sighat, bmt components
+  double r1q, r2q, raq, e3q;          // r1, r2, ratm components and eps(q+1, 3)
+  double rA1, rA2;                    // (rna - r1)/d13, (rna - r2)/d23 components (charged terms)
+};
+
+__device__ __forceinline__ double pick3(double a, double b, double c, int q) { return q == 0 ? a : (q == 1 ? b : c); }
+
+__device__ __forceinline__ void geo_q(const DGeo3s &G, int q, DGeoQ &Q) {
+  Q.rh = pick3(G.rhat[0], G.rhat[1], G.rhat[2], q);
+  Q.am = pick3(G.amt[0], G.amt[1], G.amt[2], q);
+  Q.sg = pick3(G.sighat[0], G.sighat[1], G.sighat[2], q);
+  Q.bm = pick3(G.bmt[0], G.bmt[1], G.bmt[2], q);
+  Q.r1q = pick3(G.r1[0], G.r1[1], G.r1[2], q);
+  Q.r2q = pick3(G.r2[0], G.r2[1], G.r2[2], q);
+  Q.raq = pick3(G.rna[0], G.rna[1], G.rna[2], q);
+  Q.e3q = pick3(G.eps[6], G.eps[7], G.eps[8], q);
+  const double a1x = G.d13 > 1.0e-5 ? (G.rna[0] - G.r1[0]) / G.d13 : 0.0;
+  const double a1y = G.d13 > 1.0e-5 ? (G.rna[1] - G.r1[1]) / G.d13 : 0.0;
+  const double a1z = G.d13 > 1.0e-5 ? (G.rna[2] - G.r1[2]) / G.d13 : 0.0;
+  const double a2x = G.d23 > 1.0e-5 ? (G.rna[0] - G.r2[0]) / G.d23 : 0.0;
+  const double a2y = G.d23 > 1.0e-5 ? (G.rna[1] - G.r2[1]) / G.d23 : 0.0;
+  const double a2z = G.d23 > 1.0e-5 ? (G.rna[2] - G.r2[2]) / G.d23 : 0.0;
+  Q.rA1 = pick3(a1x, a1y, a1z, q);
+  Q.rA2 = pick3(a2x, a2y, a2z, q);
+}
+
+// Depsilon_3c slices of component ix = q + 1 from component scalars
+__device__ __forceinline__ void d3_slice_q(const DGeo3s &G, const D3Scal &S, const DGeoQ &Q, int ix, double *dA,
+                                           double *dB) {
+#define E3(i, j) G.eps[3 * ((j) - 1) + (i) - 1]
+#define SA(imu, k) dA[((imu) - 1) + 3 * ((k) - 1)]
+#define SB(imu, k) dB[((imu) - 1) + 3 * ((k) - 1)]
+#pragma unroll
+  for (int i = 0; i < 9; i++) { dA[i] = 0.0; dB[i] = 0.0; }
+  if (S.zero) return;
+  const double cm = S.crossmag;
+#pragma unroll
+  for (int imu = 1; imu <= 3; imu++) {
+    double sum = d_levi(ix, imu, 1) * (G.r2[0] - G.r1[0]) + d_levi(ix, imu, 2) * (G.r2[1] - G.r1[1]) +
+                 d_levi(ix, imu, 3) * (G.r2[2] - G.r1[2]);
+    SA(imu, 2) = (1.0 / cm) * (sum - (E3(imu, 2) / cm) *
+                                         ((S.r1mag2 + S.r2mag2 - 2.0 * S.r1dotr2) * Q.raq +
+                                          (S.r1dotr2 - S.r1mag2 - S.r2dotratm + S.r1dotratm) * Q.r2q +
+                                          (S.r1dotr2 - S.r2mag2 + S.r2dotratm - S.r1dotratm) * Q.r1q));
+  }
+  SA(1, 1) = E3(3, 3) * SA(2, 2) - E3(2, 3) * SA(3, 2);
+  SA(2, 1) = E3(1, 3) * SA(3, 2) - E3(3, 3) * SA(1, 2);
+  SA(3, 1) = E3(2, 3) * SA(1, 2) - E3(1, 3) * SA(2, 2);
+#pragma unroll
+  for (int imu = 1; imu <= 3; imu++) {
+    SB(imu, 3) = (1.0 / G.z) * (E3(imu, 3) * Q.e3q - d_delk(imu, ix));
+    double sum = d_levi(ix, imu, 1) * (G.rna[0] - G.r2[0]) + d_levi(ix, imu, 2) * (G.rna[1] - G.r2[1]) +
+                 d_levi(ix, imu, 3) * (G.rna[2] - G.r2[2]);
+    SB(imu, 2) = (1.0 / cm) * (sum - (E3(imu, 2) / cm) *
+                                         ((S.r2dotratm - S.r1dotratm + S.r1dotr2 - S.r2mag2) * Q.raq +
+                                          (S.r2dotratm + S.r1dotratm - S.r1dotr2 - S.ratmmag2) * Q.r2q +
+                                          (S.ratmmag2 - 2.0 * S.r2dotratm + S.r2mag2) * Q.r1q));
+  }
+  SB(1, 1) = E3(3, 3) * SB(2, 2) - E3(3, 2) * SB(2, 3) - E3(2, 3) * SB(3, 2) + E3(2, 2) * SB(3, 3);
+  SB(2, 1) = E3(1, 3) * SB(3, 2) - E3(1, 2) * SB(3, 3) - E3(3, 3) * SB(1, 2) + E3(3, 2) * SB(1, 3);
+  SB(3, 1) = E3(2, 3) * SB(1, 2) - E3(2, 2) * SB(1, 3) - E3(1, 3) * SB(2, 2) + E3(1, 2) * SB(2, 3);
+#undef E3
+#undef SA
+#undef SB
+}
+
+// component q of the neutral (charged = false) or one charged shell's forces of
+// a triplet, continuing (a, b, c): f3naa, f3nab, f3nac sums
+template <int CLS, bool CHG>
+__device__ __forceinline__ void force3c_q(const DView &v, const DForce &Fo, const DGeo3s &G, const D3Scal &S,
+                                          const DGeoQ &Q, int q, int64_t bo, const double *row, double dQ,
+                                          double stinky, double dstA, double dstB, double dstC, double &a, double &b,
+                                          double &c) {
+  constexpr int NME = nme3c_of(CLS);
+  constexpr int NMU = (CLS & 2) ? 4 : 1, NNU = (CLS & 1) ? 4 : 1;
+  double pm[9], sA[9], sB[9], dpA[9], dpB[9], F[NME], Ma[NME], Mb[NME];
+  d_pmat<double>(G.eps, pm);
+  d3_slice_q(G, S, Q, q + 1, sA, sB);
+  dpmat_slice(sA, dpA);
+  dpmat_slice(sB, dpB);
 #pragma unroll
   for (int m = 0; m < NME; m++) {
-    sI[m][lane] = F[m];
-    sI[NME + m][lane] = DpF[m];
-    sI[2 * NME + m][lane] = DxF[m];
-    sI[3 * NME + m][lane] = DyF[m];
+    F[m] = row[m];
+    const double dp = row[NME + m];
+    Ma[m] = Q.rh * row[2 * NME + m] + Q.am * dp;
+    Mb[m] = -Q.sg * row[3 * NME + m] + Q.bm * dp - Ma[m] / 2.0;
   }
-  return st;
-}
-
-// (x, z, cos theta) of a triplet: the interpolation arguments only
-__device__ __forceinline__ void xyc3(const DView &v, int4 tr, bool charged, double &x, double &z, double &cost) {
-  DGeo3s G;
-  d_dgeo3s(v, tr, charged, G);
-  x = G.x;
-  z = G.z;
-  cost = G.cost;
-}
-
-// neutral bcna forces of one triplet from its staged interpolation
-template <int CLS>
-__device__ __forceinline__ void force3c_neutral(const DView &v, const DForce &Fo, int4 tr, int64_t bo,
-                                                double (*sI)[T3N], int lane, double *fa, double *fb, double *fc) {
-  constexpr int NME = nme3c_of(CLS);
-  constexpr int NMU = (CLS & 2) ? 4 : 1, NNU = (CLS & 1) ? 4 : 1;
-  DGeo3s G;
-  d_dgeo3s(v, tr, false, G);
-  D3Scal S;
-  d3_scal(G.r1, G.r2, G.r21, G.rna, G.rnabc, S);
-  double pm[9];
-  d_pmat<double>(G.eps, pm);
-#pragma unroll 1
-  for (int ix = 0; ix < 3; ix++) {
-    double sA[9], sB[9], dpA[9], dpB[9], F[NME], Ma[NME], Mb[NME];
-    d3_slice(G.r1, G.r2, G.z, G.rna, G.eps, S, ix + 1, sA, sB);
-    dpmat_slice(sA, dpA);
-    dpmat_slice(sB, dpB);
-#pragma unroll
-    for (int m = 0; m < NME; m++) {
-      F[m] = sI[m][lane];
-      const double dp = sI[NME + m][lane];
-      Ma[m] = G.rhat[ix] * sI[2 * NME + m][lane] + G.amt[ix] * dp;
-      Mb[m] = -G.sighat[ix] * sI[3 * NME + m][lane] + G.bmt[ix] * dp - Ma[m] / 2.0;
-    }
-    double a = fa[ix], b = fb[ix], c = fc[ix];
-#pragma unroll
-    for (int col = 0; col < NNU; col++)
-#pragma unroll
-      for (int row = 0; row < NMU; row++) {
-        const double xa = drot3c_el<CLS>(row, col, pm, dpA, F, Ma);
-        const double xb = drot3c_el<CLS>(row, col, pm, dpB, F, Mb);
-        const double r = Fo.rho[bo + row + NMU * col];
-        const double xc = -xa - xb;
-        a = a - r * xa * FB_P_EQ2;
-        b = b - r * xb * FB_P_EQ2;
-        c = c - r * xc * FB_P_EQ2;
-      }
-    fa[ix] = a;
-    fb[ix] = b;
-    fc[ix] = c;
-  }
-}
-
-__global__ __launch_bounds__(T3N) void k_d3c_sp(DView v, DForce Fo) {
-  __shared__ double sI[NI3][T3N];
-  const int lane = threadIdx.x;
-  const int64_t t = (int64_t)blockIdx.x * T3N + lane;
-  if (t >= v.ntrip) return;
-  const int4 tr = Fo.trip[t];
-  const int64_t bo = v.boff[Fo.tpair[t]];
-  const int in1 = v.imass[tr.y], in2 = v.imass[tr.z], indna = v.imass[tr.x];
-  const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
-  const int gid = group3c_of(v, in1, in2, indna, 0);
-  double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0}, fc[3] = {0.0, 0.0, 0.0};
-  double x, z, cost;
-  xyc3(v, tr, false, x, z, cost);
-  int status;
-  switch (cls) {
-    case 3: status = stage_d3c<10>(v, gid, x, z, cost, sI, lane); break;
-    case 2:
-    case 1: status = stage_d3c<3>(v, gid, x, z, cost, sI, lane); break;
-    default: status = stage_d3c<1>(v, gid, x, z, cost, sI, lane); break;
-  }
-  if (!status) {
-    switch (cls) {
-      case 3: force3c_neutral<3>(v, Fo, tr, bo, sI, lane, fa, fb, fc); break;
-      case 2: force3c_neutral<2>(v, Fo, tr, bo, sI, lane, fa, fb, fc); break;
-      case 1: force3c_neutral<1>(v, Fo, tr, bo, sI, lane, fa, fb, fc); break;
-      default: force3c_neutral<0>(v, Fo, tr, bo, sI, lane, fa, fb, fc); break;
-    }
-  }
-  double *o = Fo.f3t + 9 * t;
-#pragma unroll
-  for (int k = 0; k < 3; k++) { o[k] = fa[k]; o[3 + k] = fb[k]; o[6 + k] = fc[k]; }
-  if (status) atomicOr(v.status, status);
-}
-
-// charged shells of a triplet (M_Dassemble_3c_DOGS.f90:460-776): continues the
-// triplet's neutral sums in f3t with every isorp whose dQ is not zero
-template <int CLS>
-__device__ __forceinline__ void force3c_charged(const DView &v, const DForce &Fo, int4 tr, int64_t bo, int in1,
-                                                int in2, int indna, double dQ, double (*sI)[T3N], int lane,
-                                                double *fa, double *fb, double *fc) {
-  constexpr int NME = nme3c_of(CLS);
-  constexpr int NMU = (CLS & 2) ? 4 : 1, NNU = (CLS & 1) ? 4 : 1;
-  DGeo3s G;
-  d_dgeo3s(v, tr, true, G);
-  D3Scal S;
-  d3_scal(G.r1, G.r2, G.r21, G.rna, G.rnabc, S);
-  double pm[9];
-  d_pmat<double>(G.eps, pm);
-  double rA1[3] = {0.0, 0.0, 0.0}, rA2[3] = {0.0, 0.0, 0.0};
-  if (G.d13 > 1.0e-5) for (int k = 0; k < 3; k++) rA1[k] = (G.rna[k] - G.r1[k]) / G.d13;
-  if (G.d23 > 1.0e-5) for (int k = 0; k < 3; k++) rA2[k] = (G.rna[k] - G.r2[k]) / G.d23;
-  const double rc3 = v.sp.rcmin[indna - 1];
-  const double rend1 = v.sp.rcmin[in1 - 1] + rc3, rend2 = v.sp.rcmin[in2 - 1] + rc3;
-  const double st13 = d_smoother(G.d13, rend1, 0.8), dst13 = d_Dsmoother(G.d13, rend1, 0.8);
-  const double st23 = d_smoother(G.d23, rend2, 0.8), dst23 = d_Dsmoother(G.d23, rend2, 0.8);
-  const double stinky = st13 * st23;
   const double *sb = v.mat[FB_MAT_S] + bo, *db = v.mat[FB_MAT_DIP] + bo;
   const double *sD = Fo.dmat[0] + 3 * bo, *dD = Fo.dmat[2] + 3 * bo;
   const double d13sq = G.d13 * G.d13, d23sq = G.d23 * G.d23;
-#pragma unroll 1
-  for (int ix = 0; ix < 3; ix++) {
-    double sA[9], sB[9], dpA[9], dpB[9], F[NME], Ma[NME], Mb[NME];
-    d3_slice(G.r1, G.r2, G.z, G.rna, G.eps, S, ix + 1, sA, sB);
-    dpmat_slice(sA, dpA);
-    dpmat_slice(sB, dpB);
 #pragma unroll
-    for (int m = 0; m < NME; m++) {
-      F[m] = sI[m][lane];
-      const double dp = sI[NME + m][lane];
-      Ma[m] = G.rhat[ix] * sI[2 * NME + m][lane] + G.amt[ix] * dp;
-      Mb[m] = -G.sighat[ix] * sI[3 * NME + m][lane] + G.bmt[ix] * dp - Ma[m] / 2.0;
-    }
-    const double dstnB = -dst13 * st23 * rA1[ix];
-    const double dstnC = -st13 * dst23 * rA2[ix];
-    const double dstnA = -dstnB - dstnC;
-    double a = fa[ix], b = fb[ix], c = fc[ix];
+  for (int col = 0; col < NNU; col++)
 #pragma unroll
-    for (int col = 0; col < NNU; col++)
-#pragma unroll
-      for (int row = 0; row < NMU; row++) {
-        const int i = row + NMU * col;
-        const double xa = drot3c_el<CLS>(row, col, pm, dpA, F, Ma);
-        const double xb = drot3c_el<CLS>(row, col, pm, dpB, F, Mb);
-        const double bx = rot3c_el<CLS>(row, col, pm, F);
+    for (int row_ = 0; row_ < NMU; row_++) {
+      const int i = row_ + NMU * col;
+      const double xa = drot3c_el<CLS>(row_, col, pm, dpA, F, Ma);
+      const double xb = drot3c_el<CLS>(row_, col, pm, dpB, F, Mb);
+      const double xc = -xa - xb;
+      if (!CHG) {
+        const double r = Fo.rho[bo + i];
+        a = a - r * xa * FB_P_EQ2;
+        b = b - r * xb * FB_P_EQ2;
+        c = c - r * xc * FB_P_EQ2;
+      } else {
+        const double bx = rot3c_el<CLS>(row_, col, pm, F);
         const double sterm = sb[i] / 2.0, dterm = db[i] / G.z;
         const double emnpl = (sterm - dterm) / G.d13 + (sterm + dterm) / G.d23;
-        const double spterm = sD[ix + 3 * i] / 2.0;
-        double dpterm = dD[ix + 3 * i] / G.z;
-        dpterm = dpterm + dterm * G.sighat[ix] / G.z;
-        const double dA = -((sterm - dterm) * rA1[ix] / d13sq) - (sterm + dterm) * rA2[ix] / d23sq;
-        const double dB = (sterm - dterm) * rA1[ix] / d13sq + (spterm - dpterm) / G.d13 + (spterm + dpterm) / G.d23;
+        const double spterm = sD[q + 3 * i] / 2.0;
+        double dpterm = dD[q + 3 * i] / G.z;
+        dpterm = dpterm + dterm * Q.sg / G.z;
+        const double dA = -((sterm - dterm) * Q.rA1 / d13sq) - (sterm + dterm) * Q.rA2 / d23sq;
+        const double dB = (sterm - dterm) * Q.rA1 / d13sq + (spterm - dpterm) / G.d13 + (spterm + dpterm) / G.d23;
         const double dC = -dA - dB;
-        const double xc = -xa - xb;
         const double w = FB_P_EQ2 * dQ * Fo.rho[bo + i];
-        a = a - w * (stinky * xa + dstnA * bx - dstnA * emnpl + (1.0 - stinky) * dA);
-        b = b - w * (stinky * xb + dstnB * bx - dstnB * emnpl + (1.0 - stinky) * dB);
-        c = c - w * (stinky * xc + dstnC * bx - dstnC * emnpl + (1.0 - stinky) * dC);
+        a = a - w * (stinky * xa + dstA * bx - dstA * emnpl + (1.0 - stinky) * dA);
+        b = b - w * (stinky * xb + dstB * bx - dstB * emnpl + (1.0 - stinky) * dB);
+        c = c - w * (stinky * xc + dstC * bx - dstC * emnpl + (1.0 - stinky) * dC);
       }
-    fa[ix] = a;
-    fb[ix] = b;
-    fc[ix] = c;
+    }
+}
+
+template <bool CHG>
+__device__ __forceinline__ void force3c_dispatch(int cls, const DView &v, const DForce &Fo, const DGeo3s &G,
+                                                 const D3Scal &S, const DGeoQ &Q, int q, int64_t bo, const double *row,
+                                                 double dQ, double st, double dA, double dB, double dC, double &a,
+                                                 double &b, double &c) {
+  switch (cls) {
+    case 3: force3c_q<3, CHG>(v, Fo, G, S, Q, q, bo, row, dQ, st, dA, dB, dC, a, b, c); break;
+    case 2: force3c_q<2, CHG>(v, Fo, G, S, Q, q, bo, row, dQ, st, dA, dB, dC, a, b, c); break;
+    case 1: force3c_q<1, CHG>(v, Fo, G, S, Q, q, bo, row, dQ, st, dA, dB, dC, a, b, c); break;
+    default: force3c_q<0, CHG>(v, Fo, G, S, Q, q, bo, row, dQ, st, dA, dB, dC, a, b, c); break;
   }
 }
 
-__global__ __launch_bounds__(T3N) void k_d3c_sp_q(DView v, DForce Fo) {
-  __shared__ double sI[NI3][T3N];
-  const int lane = threadIdx.x;
-  const int64_t t = (int64_t)blockIdx.x * T3N + lane;
+// The per-triplet forces go to three arrays in the order in which they are
+// summed per atom: f3a (on ialpha) in list order, f3b (on iatom) in target-pair
+// order, f3c (on jatom) in jatom order; component q of each.
+__global__ __launch_bounds__(T3Q) void k_d3c_sp(DView v, DForce Fo) {
+  __shared__ double sI[T3Q / 4][SIROW];
+  const int slot = threadIdx.x >> 2, q = threadIdx.x & 3;
+  const int64_t t = (int64_t)blockIdx.x * (T3Q / 4) + slot;
+  if (t >= v.ntrip) return;
+  const int4 tr = Fo.trip[t];
+  const int in1 = v.imass[tr.y], in2 = v.imass[tr.z], indna = v.imass[tr.x];
+  const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
+  const int nme = cls == 3 ? 10 : (cls == 0 ? 1 : 3);
+  DGeo3s G;
+  d_dgeo3s(v, tr, false, G);
+  double *row = sI[slot];
+  int status = stage_q(v, group3c_of(v, in1, in2, indna, 0), nme, G.x, G.z, G.cost, row, q);
+  // the four lanes of a triplet sit in one wavefront: a wavefront barrier
+  // orders phase A's LDS writes before phase B's reads
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  status |= __shfl_xor(status, 1, 4) | __shfl_xor(status, 2, 4);
+  if (q < 3) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    if (!status) {
+      D3Scal S;
+      d3_scal(G.r1, G.r2, G.r21, G.rna, G.rnabc, S);
+      DGeoQ Q;
+      geo_q(G, q, Q);
+      force3c_dispatch<false>(cls, v, Fo, G, S, Q, q, v.boff[Fo.tpair[t]], row, 0.0, 0.0, 0.0, 0.0, 0.0, a, b, c);
+    }
+    Fo.f3a[3 * t + q] = a;
+    Fo.f3b[3 * (int64_t)Fo.inv_perm[t] + q] = b;
+    Fo.f3c[3 * (int64_t)Fo.inv_perm_j[t] + q] = c;
+  }
+  if (status && q == 0) atomicOr(v.status, status);
+}
+
+// charged shells (M_Dassemble_3c_DOGS.f90:460-776): continues each triplet's
+// neutral sums with every isorp whose dQ is not zero
+__global__ __launch_bounds__(T3Q) void k_d3c_sp_q(DView v, DForce Fo) {
+  __shared__ double sI[T3Q / 4][SIROW];
+  const int slot = threadIdx.x >> 2, q = threadIdx.x & 3;
+  const int64_t t = (int64_t)blockIdx.x * (T3Q / 4) + slot;
   if (t >= v.ntrip) return;
   const int4 tr = Fo.trip[t];
   const int indna = v.imass[tr.x];
   bool charged = false;
   for (int isorp = 1; isorp <= v.sp.nssh[indna - 1]; isorp++) charged |= (v.dQ[tr.x * FB_MAXSH + isorp - 1] != 0.0);
   if (!charged) return;
-  const int64_t bo = v.boff[Fo.tpair[t]];
   const int in1 = v.imass[tr.y], in2 = v.imass[tr.z];
   const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
-  double *o = Fo.f3t + 9 * t;
-  double fa[3], fb[3], fc[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) { fa[k] = o[k]; fb[k] = o[3 + k]; fc[k] = o[6 + k]; }
-  double x, z, cost;
-  xyc3(v, tr, true, x, z, cost);
+  const int nme = cls == 3 ? 10 : (cls == 0 ? 1 : 3);
+  const int64_t bo = v.boff[Fo.tpair[t]];
+  DGeo3s G;
+  d_dgeo3s(v, tr, true, G);
+  D3Scal S;
+  d3_scal(G.r1, G.r2, G.r21, G.rna, G.rnabc, S);
+  DGeoQ Q;
+  geo_q(G, q < 3 ? q : 0, Q);
+  const double rc3 = v.sp.rcmin[indna - 1];
+  const double rend1 = v.sp.rcmin[in1 - 1] + rc3, rend2 = v.sp.rcmin[in2 - 1] + rc3;
+  const double st13 = d_smoother(G.d13, rend1, 0.8), dst13 = d_Dsmoother(G.d13, rend1, 0.8);
+  const double st23 = d_smoother(G.d23, rend2, 0.8), dst23 = d_Dsmoother(G.d23, rend2, 0.8);
+  const double stinky = st13 * st23;
+  const double dstB = -dst13 * st23 * Q.rA1;
+  const double dstC = -st13 * dst23 * Q.rA2;
+  const double dstA = -dstB - dstC;
+  double *row = sI[slot];
+  double a = 0.0, b = 0.0, c = 0.0;
+  const int64_t ob = 3 * (int64_t)Fo.inv_perm[t] + q, oc = 3 * (int64_t)Fo.inv_perm_j[t] + q;
+  if (q < 3) {
+    a = Fo.f3a[3 * t + q];
+    b = Fo.f3b[ob];
+    c = Fo.f3c[oc];
+  }
   int status = 0;
   for (int isorp = 1; isorp <= v.sp.nssh[indna - 1]; isorp++) {
     const double dQ = v.dQ[tr.x * FB_MAXSH + isorp - 1];
     if (dQ == 0.0) continue;
-    const int gid = group3c_of(v, in1, in2, indna, isorp);
-    int st;
-    switch (cls) {
-      case 3: st = stage_d3c<10>(v, gid, x, z, cost, sI, lane); break;
-      case 2:
-      case 1: st = stage_d3c<3>(v, gid, x, z, cost, sI, lane); break;
-      default: st = stage_d3c<1>(v, gid, x, z, cost, sI, lane); break;
-    }
+    int st = stage_q(v, group3c_of(v, in1, in2, indna, isorp), nme, G.x, G.z, G.cost, row, q);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    st |= __shfl_xor(st, 1, 4) | __shfl_xor(st, 2, 4);
     status |= st;
-    if (st) continue;
-    switch (cls) {
-      case 3: force3c_charged<3>(v, Fo, tr, bo, in1, in2, indna, dQ, sI, lane, fa, fb, fc); break;
-      case 2: force3c_charged<2>(v, Fo, tr, bo, in1, in2, indna, dQ, sI, lane, fa, fb, fc); break;
-      case 1: force3c_charged<1>(v, Fo, tr, bo, in1, in2, indna, dQ, sI, lane, fa, fb, fc); break;
-      default: force3c_charged<0>(v, Fo, tr, bo, in1, in2, indna, dQ, sI, lane, fa, fb, fc); break;
-    }
+    if (!st && q < 3)
+      force3c_dispatch<true>(cls, v, Fo, G, S, Q, q, bo, row, dQ, stinky, dstA, dstB, dstC, a, b, c);
+    // the row is rewritten by the next shell: reads of this one complete first
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-#pragma unroll
-  for (int k = 0; k < 3; k++) { o[k] = fa[k]; o[3 + k] = fb[k]; o[6 + k] = fc[k]; }
-  if (status) atomicOr(v.status, status);
+  if (q < 3) {
+    Fo.f3a[3 * t + q] = a;
+    Fo.f3b[ob] = b;
+    Fo.f3c[oc] = c;
+  }
+  if (status && q == 0) atomicOr(v.status, status);
 }
 
 extern "C" int fbk_d2c_sp(const DView &v, const DForce &f, int mask, hipStream_t st) {
@@ -1581,8 +1729,8 @@ extern "C" int fbk_d2c_sp(const DView &v, const DForce &f, int mask, hipStream_t
 // charged = 0: the neutral pass only (every shell charge of the batch is zero)
 extern "C" int fbk_d3c_sp(const DView &v, const DForce &f, int charged, hipStream_t st) {
   if (v.ntrip == 0) return 0;
-  const dim3 grid((unsigned)((v.ntrip + T3N - 1) / T3N));
-  hipLaunchKernelGGL(k_d3c_sp, grid, dim3(T3N), 0, st, v, f);
-  if (charged) hipLaunchKernelGGL(k_d3c_sp_q, grid, dim3(T3N), 0, st, v, f);
+  const dim3 grid((unsigned)((v.ntrip + T3Q / 4 - 1) / (T3Q / 4)));
+  hipLaunchKernelGGL(k_d3c_sp, grid, dim3(T3Q), 0, st, v, f);
+  if (charged) hipLaunchKernelGGL(k_d3c_sp_q, grid, dim3(T3Q), 0, st, v, f);
   return (int)hipGetLastError();
 }
