@@ -1483,6 +1483,41 @@ __device__ inline int stage_q(const DView &v, int gid, int nme, double x, double
   return 0;
 }
 
+// (x, z, cos theta) of a triplet with the expressions of d_dgeo3s, without the
+// frame: what the interpolation needs
+__device__ __forceinline__ void xzc3(const DView &v, int4 tr, bool charged, double &x, double &z, double &cost) {
+  const int ial = tr.x, gia = tr.y, gja = tr.z;
+  const int ib = tr.w & 0xffff, jb = (tr.w >> 16) & 0xffff;
+  const int s = v.atom_struct[ial];
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  double r1[3], r2[3], rna[3], r21[3], sighat[3], rnabc[3], rhat[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    r1[k] = v.ratom[3 * gia + k] + xls[3 * ib + k];
+    r2[k] = v.ratom[3 * gja + k] + xls[3 * jb + k];
+    rna[k] = v.ratom[3 * ial + k];
+    r21[k] = r2[k] - r1[k];
+  }
+  z = d_distance(r1, r2);
+  if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+  else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) sighat[k] = r21[k] / z;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) rnabc[k] = rna[k] - (r1[k] + r21[k] / 2.0);
+  x = sqrt(rnabc[0] * rnabc[0] + rnabc[1] * rnabc[1] + rnabc[2] * rnabc[2]);
+  if (x < 1.0e-5) { rhat[0] = 0.0; rhat[1] = 0.0; rhat[2] = 0.0; }
+  else if (charged) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) rhat[k] = (rna[k] - 0.5 * (r1[k] + r2[k])) / x;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) rhat[k] = rnabc[k] / x;
+  }
+  cost = sighat[0] * rhat[0] + sighat[1] * rhat[1] + sighat[2] * rhat[2];
+}
+
 // the scalars of one Cartesian component q (0..2) of a triplet's geometry
 struct DGeoQ {
   double rh, am, sg, bm;              // rhat, amt, sighat, bmt components
@@ -1631,10 +1666,10 @@ __global__ __launch_bounds__(T3Q) void k_d3c_sp(DView v, DForce Fo) {
   const int in1 = v.imass[tr.y], in2 = v.imass[tr.z], indna = v.imass[tr.x];
   const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
   const int nme = cls == 3 ? 10 : (cls == 0 ? 1 : 3);
-  DGeo3s G;
-  d_dgeo3s(v, tr, false, G);
+  double x, z, cost;
+  xzc3(v, tr, false, x, z, cost);
   double *row = sI[slot];
-  int status = stage_q(v, group3c_of(v, in1, in2, indna, 0), nme, G.x, G.z, G.cost, row, q);
+  int status = stage_q(v, group3c_of(v, in1, in2, indna, 0), nme, x, z, cost, row, q);
   // the four lanes of a triplet sit in one wavefront: a wavefront barrier
   // orders phase A's LDS writes before phase B's reads
   __builtin_amdgcn_wave_barrier();
@@ -1644,6 +1679,8 @@ __global__ __launch_bounds__(T3Q) void k_d3c_sp(DView v, DForce Fo) {
   if (q < 3) {
     double a = 0.0, b = 0.0, c = 0.0;
     if (!status) {
+      DGeo3s G;
+      d_dgeo3s(v, tr, false, G);
       D3Scal S;
       d3_scal(G.r1, G.r2, G.r21, G.rna, G.rnabc, S);
       DGeoQ Q;
