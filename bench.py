@@ -183,6 +183,19 @@ def force3c_flops_per_triplet():
     return None
 
 
+def _force_kernel_ms(asm, n):
+    asm.enable_timing(True)
+    asm.timings(4096)
+    for _ in range(n):
+        asm.dassemble_all()
+    asm.synchronize()
+    kern = {}
+    for name, t in asm.timings(4096):
+        kern.setdefault(name, []).append(t)
+    asm.enable_timing(False)
+    return {k: float(np.mean(v)) for k, v in kern.items() if k != "clear_status"}
+
+
 def forces_leg(asm, structs, steps, warmup):
     """One force build = fb_dassemble_all (Dblocks of S, T and dipole_z, the
     vna_atom / vna_ontop pair forces and the per-atom f3naa/f3nab/f3nac of
@@ -200,24 +213,38 @@ def forces_leg(asm, structs, steps, warmup):
         asm.dassemble_all()
     asm.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
-    asm.enable_timing(True)
-    asm.timings(4096)
-    for _ in range(max(1, min(steps, 5))):
-        asm.dassemble_all()
-    asm.synchronize()
-    kern = {}
-    for name, t in asm.timings(4096):
-        kern.setdefault(name, []).append(t)
-    asm.enable_timing(False)
-    kern = {k: float(np.mean(v)) for k, v in kern.items() if k != "clear_status"}
+    kern = _force_kernel_ms(asm, max(1, min(steps, 5)))
+    kern64 = kern
     ntrip = sum(asm.list_sizes(k)[1] for k in range(len(structs)))
-    out = {"ms_per_step": ms, "triplets_per_s": ntrip / (ms * 1e-3), "kernels_ms": kern, "dtype": "f64"}
+    prec = asm.precision
+    out = {"ms_per_step": ms, "triplets_per_s": ntrip / (ms * 1e-3), "kernels_ms": kern,
+           "dtype": "f64" if prec == "fp64" else "f32 interpolation of the 3c forces, f64 otherwise"}
+    if prec != "fp64":
+        # the binary64 force build beside it, and the error of the mixed one
+        got = [asm.forces(k) for k in range(len(structs))]
+        asm.set_precision("fp64")
+        for _ in range(warmup):
+            asm.dassemble_all()
+        asm.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            asm.dassemble_all()
+        asm.synchronize()
+        ms64 = (time.perf_counter() - t0) * 1e3 / steps
+        kern64 = _force_kernel_ms(asm, max(1, min(steps, 5)))
+        ref = [asm.forces(k) for k in range(len(structs))]
+        asm.set_precision(prec)
+        err = {key: float(max(np.max(np.abs(g[key] - r[key])) for g, r in zip(got, ref)))
+               for key in ("vna_atom", "vna_ontop", "f3naa", "f3nab", "f3nac")}
+        out.update(fp64_ms_per_step=ms64, fp64_kernels_ms=kern64, speedup_vs_fp64=ms64 / ms,
+                   max_abs_err_vs_fp64_eV_per_A=err)
     fl = force3c_flops_per_triplet()
-    k3 = kern.get("Dassemble_vna_3c")
+    k3 = kern64.get("Dassemble_vna_3c")
     if fl and k3:
+        # the binary64 kernel, whose FP64 operation count the counter pass measured
         tf = fl * ntrip / (k3 * 1e-3) / 1e12
-        out["roofline_3c"] = {"bound": "fp64", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                              "frac": tf / FP64_PEAK_TFLOPS, "ops_per_triplet": fl}
+        out["roofline_3c_fp64"] = {"bound": "fp64", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                   "frac": tf / FP64_PEAK_TFLOPS, "ops_per_triplet": fl, "kernel_ms": k3}
     return out
 
 

@@ -185,3 +185,40 @@ def test_sp_kernels_equal_generic_kernels(golden, name):
         np.testing.assert_array_equal(fa[k], fb[k])
     for k in ("f3naa", "f3nab", "f3nac"):
         np.testing.assert_allclose(fa[k], fb[k], rtol=0, atol=1e-13 * max(1.0, np.max(np.abs(fb[k]))))
+
+
+TOL_FORCE_MIXED = 1e-6   # eV/A: mixed-precision 3c forces against binary64, per component
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2])
+def test_mixed_forces_within_tolerance(cfg):
+    """Mixed precision (FP32 interpolation of the 3c force kernel, binary64 frames,
+    frame derivatives, rotation and sums): Dblocks and pair forces are binary64 in
+    both modes and equal; the per-atom 3c forces are within TOL_FORCE_MIXED of
+    binary64."""
+    from fireball_hip.lib import Assembler
+    species, structs = synth.config(cfg)
+    ts = synth.make_tables(species)
+    st = structs[0]
+    print(f"config {cfg}", flush=True)
+    asm = Assembler(ts, precision="fp64")
+    asm.add_structure(st)
+    asm.build_lists()
+    asm.assemble_all()
+    L = asm.lists()
+    asm.set_denmat(_rho(ts, st, L))
+    asm.dassemble_all()
+    ref = asm.forces()
+    dref = {m: asm.dblocks(m) for m in DKEYS.values()}
+    asm.set_precision("mixed")
+    asm.assemble_all()
+    asm.dassemble_all()
+    got = asm.forces()
+    for m in DKEYS.values():
+        np.testing.assert_array_equal(asm.dblocks(m), dref[m])
+    for k in ("vna_atom", "vna_ontop"):
+        np.testing.assert_array_equal(got[k], ref[k])
+    for k in ("f3naa", "f3nab", "f3nac"):
+        err = float(np.max(np.abs(got[k] - ref[k])))
+        print(f"config {cfg} {k} mixed vs fp64 max-abs {err:.3e} (max |F| {np.max(np.abs(ref[k])):.3e})", flush=True)
+        assert err <= TOL_FORCE_MIXED, k

@@ -1978,7 +1978,9 @@ static int run_d3c(fb_ctx *ctx) {
   DView v = ctx->view();
   DForce f = ctx->fview();
   if (ctx->canon_sp && ctx->force_kernel == 1)
-    RC(ctx->launch("Dassemble_vna_3c", [&] { return fbk_d3c_sp(v, f, any_charge(ctx), ctx->stream); }));
+    RC(ctx->launch("Dassemble_vna_3c", [&] {
+      return fbk_d3c_sp(v, f, any_charge(ctx), ctx->precision == FB_PREC_MIXED, ctx->stream);
+    }));
   else
     RC(ctx->launch("Dassemble_vna_3c", [&] { return fbk_d3c(v, f, ctx->stream); }));
   RC(ctx->launch("forces_3c_sum", [&] { return fbk_f3_reduce(v, f, ctx->stream); }));

@@ -25,7 +25,7 @@ extern "C" {
 int fbk_d2c(const DView &v, const DForce &f, int mask, hipStream_t st);
 int fbk_d3c(const DView &v, const DForce &f, hipStream_t st);
 int fbk_d2c_sp(const DView &v, const DForce &f, int mask, hipStream_t st);
-int fbk_d3c_sp(const DView &v, const DForce &f, int charged, hipStream_t st);
+int fbk_d3c_sp(const DView &v, const DForce &f, int charged, int mixed, hipStream_t st);
 int fbk_f3_reduce(const DView &v, const DForce &f, hipStream_t st);
 int fbk_f3_offsets(int natoms, int64_t ntrip, const int64_t *noff, const int *tpair_sorted, const int *jkey_sorted,
                    int64_t *ioff, int64_t *joff, hipStream_t st);

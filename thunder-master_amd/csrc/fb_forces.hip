@@ -1518,6 +1518,149 @@ __device__ __forceinline__ void xzc3(const DView &v, int4 tr, bool charged, doub
   cost = sighat[0] * rhat[0] + sighat[1] * rhat[1] + sighat[2] * rhat[2];
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision (fb_set_precision FB_PREC_MIXED): the interpolation of phase
+// A in FP32 over the FP32 slab [iy][ix][ME][theta8] -- cubic Lagrange weights in
+// x and y and their derivatives (the Horner form of M_Fdata_3c.f90:636-713 is
+// this polynomial times 36), the Legendre fold of the value and of its cos theta
+// derivative at every stencil point.  The cell, the method, the interpolation
+// arguments and everything of phase B stay binary64.
+#define TH8F 8
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slab32(const float *tab, int64_t nfloats) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)tab, (short)0, (int)(nfloats * 4), 0x00020000);
+}
+
+// sum_theta P_theta T_theta and sum_theta dP_theta T_theta of one (point, ME); off in bytes
+__device__ __forceinline__ void fold2(__amdgpu_buffer_rsrc_t r, unsigned off, const float *P, const float *dP,
+                                      float &tv, float &tp) {
+  const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+  const float b = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(off + 16u), 0, 0));
+  float v = P[0] * a.x;
+  v = __builtin_fmaf(P[1], a.y, v);
+  v = __builtin_fmaf(P[2], a.z, v);
+  v = __builtin_fmaf(P[3], a.w, v);
+  tv = __builtin_fmaf(P[4], b, v);
+  float u = dP[1] * a.y;
+  u = __builtin_fmaf(dP[2], a.z, u);
+  u = __builtin_fmaf(dP[3], a.w, u);
+  tp = __builtin_fmaf(dP[4], b, u);
+}
+
+// weights of the 4-point cubic on nodes -1, 0, 1, 2 and their p-derivatives
+__device__ __forceinline__ void cubic_wd(float p, float *w, float *d) {
+  const float pm1 = p - 1.f, pm2 = p - 2.f, pp1 = p + 1.f, s6 = 1.f / 6.f, p2 = p * p;
+  w[0] = -p * pm1 * pm2 * s6;
+  w[1] = pp1 * pm1 * pm2 * 0.5f;
+  w[2] = -pp1 * p * pm2 * 0.5f;
+  w[3] = pp1 * p * pm1 * s6;
+  d[0] = -(3.f * p2 - 6.f * p + 2.f) * s6;
+  d[1] = (3.f * p2 - 4.f * p - 1.f) * 0.5f;
+  d[2] = -(3.f * p2 - 2.f * p - 2.f) * 0.5f;
+  d[3] = (3.f * p2 - 1.f) * s6;
+}
+
+// elements q, q+4, q+8 of one canonical group into row (binary64 values)
+__device__ inline int stage_q_mx(const DView &v, int gid, int nme, double x, double y, double cost, double *row,
+                                 int q) {
+  if (gid < 0) return ST_NO_TABLE;
+  const DGroup3c g = v.grp3c[gid];
+  const int nx = g.nx, ny = g.ny;
+  const double dx = g.xmax / (nx - 1.0), dy = g.ymax / (ny - 1.0);
+  int ix = (int)(x / dx) + 1, iy = (int)(y / dy) + 1;
+  if (ix < 2) ix = 2;
+  if (iy < 2) iy = 2;
+  if (ix > nx - 2) ix = nx - 2;
+  if (iy > ny - 2) iy = ny - 2;
+  const double px = x / dx - ix + 1.0, py = y / dy - iy + 1.0;
+  const int64_t icell = (int64_t)(iy - 1) * nx + (ix - 1);
+  const bool three = v.meth3c[g.moff + icell] != 0;
+  const double cost2 = cost * cost;
+  const float sint = (float)sqrt(fmax(1.0e-5, 1.0 - cost2)), cf = (float)cost;
+  float P[5], dP[5];
+  P[0] = 1.f;
+  P[1] = cf;
+  P[2] = (float)(0.5 * (3.0 * cost2 - 1.0));
+  P[3] = (float)(0.5 * cost * (5.0 * cost2 - 3.0));
+  P[4] = (float)(0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2)));
+  dP[0] = 0.f;
+  dP[1] = 1.f;
+  dP[2] = (float)(3.0 * cost);
+  dP[3] = (float)((15.0 * cost2 - 3.0) / 2.0);
+  dP[4] = (float)(0.5 * cost * (35.0 * cost2 - 15.0));
+  const __amdgpu_buffer_rsrc_t r = slab32(v.tab3c32, v.ntab3c32);
+  const unsigned PT = (unsigned)(nme * TH8F * 4);            // bytes per ix step
+  const unsigned RB = (unsigned)(nx * nme * TH8F * 4);       // bytes per iy step
+  const unsigned base = (unsigned)(g.off32 + ((int64_t)(iy - 2) * nx + (ix - 2)) * nme * TH8F) * 4u;
+  const int nm0 = (nme == 10) ? 6 : (nme == 3 ? 2 : 1);
+  float wx[4], wy[4], dwx[4], dwy[4];
+  cubic_wd((float)px, wx, dwx);
+  cubic_wd((float)py, wy, dwy);
+  float gxs = 0.f, gys = 0.f;
+  if (three) {
+    // the 3-point rule's slopes come from ME 1 of theta cell 1 (binary64 table)
+    const double *tab = v.tab3c + g.off;
+#define T3X(i, j) tab[((int64_t)((j) - 1) * nx + ((i) - 1)) * nme]
+    const double gx = (T3X(ix + 1, iy) - T3X(ix, iy)) / dx;
+    const double gy = (T3X(ix, iy + 1) - T3X(ix, iy)) / dy;
+#undef T3X
+    const float ps = (((P[0] + P[1]) + P[2]) + P[3]) + P[4];
+    gxs = (float)gx * ps;
+    gys = (float)gy * ps;
+  }
+  const float idx = (float)(1.0 / dx), idy = (float)(1.0 / dy);
+  for (int m = q; m < nme; m += 4) {
+    const unsigned o = base + (unsigned)(m * TH8F * 4);
+    float fv, fp, fx, fy;
+    if (three) {
+      const unsigned c = o + RB + PT;   // (ix, iy)
+      float t0, p0, t1, p1, t2, p2;
+      fold2(r, c, P, dP, t0, p0);
+      fold2(r, c + PT, P, dP, t1, p1);
+      fold2(r, c + RB, P, dP, t2, p2);
+      const float w0 = (1.f - (float)px) - (float)py;
+      fv = __builtin_fmaf((float)py, t2, __builtin_fmaf((float)px, t1, w0 * t0));
+      fp = __builtin_fmaf((float)py, p2, __builtin_fmaf((float)px, p1, w0 * p0));
+      fx = gxs;
+      fy = gys;
+    } else {
+      fv = 0.f;
+      fp = 0.f;
+      fx = 0.f;
+      fy = 0.f;
+#pragma unroll
+      for (int l = 0; l < 4; l++) {
+        float gv = 0.f, gp = 0.f, gd = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          float tv, tp;
+          fold2(r, o + (unsigned)l * RB + (unsigned)k * PT, P, dP, tv, tp);
+          gv = __builtin_fmaf(wx[k], tv, gv);
+          gp = __builtin_fmaf(wx[k], tp, gp);
+          gd = __builtin_fmaf(dwx[k], tv, gd);
+        }
+        fv = __builtin_fmaf(wy[l], gv, fv);
+        fp = __builtin_fmaf(wy[l], gp, fp);
+        fx = __builtin_fmaf(wy[l], gd, fx);
+        fy = __builtin_fmaf(dwy[l], gv, fy);
+      }
+      fx = fx * idx;
+      fy = fy * idy;
+    }
+    if (m >= nm0) {
+      fp = fp * sint - cf * fv / sint;
+      fv = fv * sint;
+      fx = fx * sint;
+      fy = fy * sint;
+    }
+    row[m] = fv;
+    row[nme + m] = fp;
+    row[2 * nme + m] = fx;
+    row[3 * nme + m] = fy;
+  }
+  return 0;
+}
+
 // the scalars of one Cartesian component q (0..2) of a triplet's geometry
 struct DGeoQ {
   double rh, am, sg, bm;              // rhat, amt, sighat, bmt components
@@ -1657,6 +1800,7 @@ __device__ __forceinline__ void force3c_dispatch(int cls, const DView &v, const 
 // The per-triplet forces go to three arrays in the order in which they are
 // summed per atom: f3a (on ialpha) in list order, f3b (on iatom) in target-pair
 // order, f3c (on jatom) in jatom order; component q of each.
+template <bool MX>
 __global__ __launch_bounds__(T3Q) void k_d3c_sp(DView v, DForce Fo) {
   __shared__ double sI[T3Q / 4][SIROW];
   const int slot = threadIdx.x >> 2, q = threadIdx.x & 3;
@@ -1669,7 +1813,8 @@ __global__ __launch_bounds__(T3Q) void k_d3c_sp(DView v, DForce Fo) {
   double x, z, cost;
   xzc3(v, tr, false, x, z, cost);
   double *row = sI[slot];
-  int status = stage_q(v, group3c_of(v, in1, in2, indna, 0), nme, x, z, cost, row, q);
+  const int gid0 = group3c_of(v, in1, in2, indna, 0);
+  int status = MX ? stage_q_mx(v, gid0, nme, x, z, cost, row, q) : stage_q(v, gid0, nme, x, z, cost, row, q);
   // the four lanes of a triplet sit in one wavefront: a wavefront barrier
   // orders phase A's LDS writes before phase B's reads
   __builtin_amdgcn_wave_barrier();
@@ -1763,11 +1908,13 @@ extern "C" int fbk_d2c_sp(const DView &v, const DForce &f, int mask, hipStream_t
   return (int)hipGetLastError();
 }
 
-// charged = 0: the neutral pass only (every shell charge of the batch is zero)
-extern "C" int fbk_d3c_sp(const DView &v, const DForce &f, int charged, hipStream_t st) {
+// charged = 0: the neutral pass only (every shell charge of the batch is zero);
+// mixed = 1: the FP32 interpolation of the neutral pass
+extern "C" int fbk_d3c_sp(const DView &v, const DForce &f, int charged, int mixed, hipStream_t st) {
   if (v.ntrip == 0) return 0;
   const dim3 grid((unsigned)((v.ntrip + T3Q / 4 - 1) / (T3Q / 4)));
-  hipLaunchKernelGGL(k_d3c_sp, grid, dim3(T3Q), 0, st, v, f);
+  if (mixed) hipLaunchKernelGGL(k_d3c_sp<true>, grid, dim3(T3Q), 0, st, v, f);
+  else hipLaunchKernelGGL(k_d3c_sp<false>, grid, dim3(T3Q), 0, st, v, f);
   if (charged) hipLaunchKernelGGL(k_d3c_sp_q, grid, dim3(T3Q), 0, st, v, f);
   return (int)hipGetLastError();
 }

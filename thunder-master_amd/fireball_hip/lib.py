@@ -179,6 +179,7 @@ class Assembler:
 
     def set_precision(self, precision):
         self._check(self.L.fb_set_precision(self.h, PREC[precision]))
+        self.precision = precision
 
     def set_3c_method(self, method):
         self._check(self.L.fb_set_3c_method(self.h, METHOD3C[method]))
