@@ -79,6 +79,13 @@ struct fbo_system {
   double *f3[3];    /* [natoms][3] forces%f3naa, f3nab, f3nac */
   int ddone[5];
   int ndwarn;       /* Depsilon_2c/3c colinear guards taken (WARNINGS file lines in the reference) */
+  /* McWEDA densities: input shell charges and the four density block arrays */
+  double *qin;      /* [N][FBO_MAXSH] s%atom(:)%shell(:)%Qin */
+  double *rho2[2];  /* [blen] rho_in, rho_bond: orbital blocks, block layout */
+  int64_t *wboff;   /* per pair: offset of the nssh_mu x nssh_nu shell block */
+  int64_t wlen;
+  double *wrho[2];  /* [wlen] rho_in_weighted, rho_bond_weighted */
+  int rdone[4];     /* rho_2c, rho_3c, rho_weighted_2c, rho_weighted_3c */
 };
 
 static inline cell2c *C2(const fbo_tables *t, int is, int js, int it, int isub) {
@@ -468,6 +475,8 @@ void fbo_system_free(fbo_system *s) {
   for (int i = 0; i < 5; i++) free(s->mat[i]);
   free(s->rho); free(s->fatom); free(s->fontop);
   for (int i = 0; i < 3; i++) { free(s->dmat[i]); free(s->f3[i]); }
+  free(s->qin); free(s->wboff);
+  for (int i = 0; i < 2; i++) { free(s->rho2[i]); free(s->wrho[i]); }
   free(s);
 }
 
@@ -645,6 +654,26 @@ int fbo_build_lists(fbo_system *s) {
     free(s->f3[i]); s->f3[i] = calloc((size_t)(3 * N), sizeof(double));
   }
   for (int i = 0; i < 5; i++) s->ddone[i] = 0;
+  /* McWEDA density blocks: orbital blocks in the block layout, shell blocks
+   * nssh_mu x nssh_nu per pair */
+  free(s->wboff);
+  s->wboff = malloc(sizeof(int64_t) * (s->npairs + 1));
+  int64_t w = 0;
+  for (int ia = 1; ia <= N; ia++) {
+    int nsi = t->nssh[s->imass[ia - 1] - 1];
+    for (int in = 0; in < s->neighn[ia - 1]; in++) {
+      int p = s->noff[ia - 1] + in;
+      s->wboff[p] = w;
+      w += (int64_t)nsi * t->nssh[s->imass[s->neigh_j[p] - 1] - 1];
+    }
+  }
+  s->wboff[s->npairs] = w;
+  s->wlen = w;
+  for (int i = 0; i < 2; i++) {
+    free(s->rho2[i]); s->rho2[i] = calloc(o ? o : 1, sizeof(double));
+    free(s->wrho[i]); s->wrho[i] = calloc(w ? w : 1, sizeof(double));
+  }
+  for (int i = 0; i < 4; i++) s->rdone[i] = 0;
   return FBO_OK;
 }
 
@@ -1551,3 +1580,248 @@ int fbo_get_forces(const fbo_system *s, double *vna_atom, double *vna_ontop, dou
 }
 
 int fbo_deps_warnings(const fbo_system *s) { return s->ndwarn; }
+
+/* ======================================================= McWEDA densities
+ * j.ASSEMBLERS/M_assemble_rho_McWEDA_DOGS.f90: the input density rho_in (and
+ * rho_bond) in orbital blocks, and its shell-averaged form rho_in_weighted (and
+ * rho_bond_weighted) in shell blocks, each a sum over the shells isorp of the
+ * contributing atom of Qin(isorp) times an interpolated (and, for orbital
+ * blocks, rotated) table.  Serial order: atoms, then neighbours or common
+ * pairs, then isorp; every term is added to the block as it is formed. */
+#define P_RHO_ONTOPL 15
+#define P_RHO_ONTOPR 16
+#define P_RHO_ATOM 17
+#define P_RHOS_ONTOPL 19
+#define P_RHOS_ONTOPR 20
+#define P_RHOS_ATOM 21
+#define P_RHO_3C 1
+#define P_RHOS_3C 2
+
+int fbo_set_qin(fbo_system *s, const double *qin) {
+  if (!s || !qin) return FBO_E_ARG;
+  if (!s->qin) s->qin = malloc(sizeof(double) * (size_t)s->natoms * FBO_MAXSH);
+  memcpy(s->qin, qin, sizeof(double) * (size_t)s->natoms * FBO_MAXSH);
+  for (int i = 0; i < 4; i++) s->rdone[i] = 0;
+  return FBO_OK;
+}
+
+#define QIN(s, ia, isorp) ((s)->qin[((ia) - 1) * FBO_MAXSH + (isorp) - 1])
+
+/* assemble_rho_2c, M_assemble_rho_McWEDA_DOGS.f90:90-341 */
+int fbo_assemble_rho_2c(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  if (!s->neighn || !s->qin || !s->wboff) return FBO_E_ORDER;
+  double *ri = s->rho2[0], *rb = s->rho2[1];
+  memset(ri, 0, sizeof(double) * (s->blen ? s->blen : 1));
+  memset(rb, 0, sizeof(double) * (s->blen ? s->blen : 1));
+  double bm[81], bx[81], eps[9];
+  /* ontop terms on every non-self pair (:166-247) */
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1];
+    for (int in = 0; in < s->neighn[ia - 1]; in++) {
+      int p = s->noff[ia - 1] + in;
+      int mb = s->neigh_b[p], ja = s->neigh_j[p];
+      double r2[3];
+      for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+      int in2 = s->imass[ja - 1], nnu = t->norb[in2 - 1], nb = nmu * nnu;
+      double z = pair_frame(s, r1, r2, eps);
+      if (ia == ja && mb == 0) continue;
+      double *bi = ri + s->boff[p], *bb = rb + s->boff[p];
+      for (int isorp = 1; isorp <= t->nssh[in1 - 1]; isorp++) {
+        double q = QIN(s, ia, isorp);
+        int rc = fbo_getMEs_2c(t, in1, in2, P_RHO_ONTOPL, isorp, z, nmu, nnu, bm);
+        if (rc) return rc;
+        fbo_rotate(t, in1, in2, eps, nmu, nnu, bm, bx);
+        for (int i = 0; i < nb; i++) { bi[i] = bi[i] + bx[i] * q; bb[i] = bb[i] + bx[i] * q; }
+      }
+      for (int isorp = 1; isorp <= t->nssh[in2 - 1]; isorp++) {
+        double q = QIN(s, ja, isorp);
+        int rc = fbo_getMEs_2c(t, in1, in2, P_RHO_ONTOPR, isorp, z, nmu, nnu, bm);
+        if (rc) return rc;
+        fbo_rotate(t, in1, in2, eps, nmu, nnu, bm, bx);
+        for (int i = 0; i < nb; i++) { bi[i] = bi[i] + bx[i] * q; bb[i] = bb[i] + bx[i] * q; }
+      }
+    }
+  }
+  /* atom terms on the self block: every neighbour into rho_in, the atom itself
+   * also into rho_bond (:250-335) */
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1], nb = nmu * nmu;
+    int pself = s->noff[ia - 1] + s->neigh_self[ia - 1] - 1;
+    double *bi = ri + s->boff[pself], *bb = rb + s->boff[pself];
+    for (int in = 0; in < s->neighn[ia - 1]; in++) {
+      int p = s->noff[ia - 1] + in;
+      int mb = s->neigh_b[p], ja = s->neigh_j[p];
+      double r2[3];
+      for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+      int in2 = s->imass[ja - 1];
+      double z = pair_frame(s, r1, r2, eps);
+      int self = (ia == ja && mb == 0);
+      for (int isorp = 1; isorp <= t->nssh[in2 - 1]; isorp++) {
+        double q = QIN(s, ja, isorp);
+        int rc = fbo_getMEs_2c(t, in1, in2, P_RHO_ATOM, isorp, z, nmu, nmu, bm);
+        if (rc) return rc;
+        fbo_rotate(t, in1, in1, eps, nmu, nmu, bm, bx);
+        for (int i = 0; i < nb; i++) bi[i] = bi[i] + bx[i] * q;
+        if (self)
+          for (int i = 0; i < nb; i++) bb[i] = bb[i] + bx[i] * q;
+      }
+    }
+  }
+  s->rdone[0] = 1;
+  s->rdone[1] = 0;
+  return FBO_OK;
+}
+
+/* triplet geometry of the density assemblers (M_assemble_rho_McWEDA_DOGS.f90:451-468) */
+static void rho3_geometry(fbo_system *s, int ial, int c, double *x, double *z, double *cost, double *eps) {
+  const double *rna = RAT(s, ial);
+  int ia = s->ci_j[c], ib = s->ci_b[c], ja = s->cj_j[c], jb = s->cj_b[c];
+  double r1[3], r2[3], r21[3], sighat[3], rnabc[3], rhat[3];
+  for (int k = 0; k < 3; k++) { r1[k] = RAT(s, ia)[k] + XL(s, ib)[k]; r2[k] = RAT(s, ja)[k] + XL(s, jb)[k]; }
+  for (int k = 0; k < 3; k++) r21[k] = r2[k] - r1[k];
+  *z = fbo_distance(r1, r2);
+  if (*z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+  else for (int k = 0; k < 3; k++) sighat[k] = r21[k] / *z;
+  for (int k = 0; k < 3; k++) rnabc[k] = rna[k] - (r1[k] + r21[k] / 2.0);
+  *x = sqrt(rnabc[0] * rnabc[0] + rnabc[1] * rnabc[1] + rnabc[2] * rnabc[2]);
+  if (*x < 1.0e-5) { rhat[0] = 0.0; rhat[1] = 0.0; rhat[2] = 0.0; }
+  else for (int k = 0; k < 3; k++) rhat[k] = rnabc[k] / *x;
+  *cost = sighat[0] * rhat[0] + sighat[1] * rhat[1] + sighat[2] * rhat[2];
+  s->nfallback += epsilon_impl(rhat, sighat, eps);
+}
+
+/* assemble_rho_3c, M_assemble_rho_McWEDA_DOGS.f90:362-511 */
+int fbo_assemble_rho_3c(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  if (!s->neighn || !s->rdone[0]) return FBO_E_ORDER;
+  double bm[81], bx[81], eps[9];
+  for (int ial = 1; ial <= s->natoms; ial++) {
+    int indna = s->imass[ial - 1];
+    for (int in = 0; in < s->ncommon[ial - 1]; in++) {
+      int c = s->coff[ial - 1] + in;
+      int mneigh = s->cm[c];
+      if (mneigh == 0) continue;
+      int ia = s->ci_j[c], ja = s->cj_j[c];
+      int in1 = s->imass[ia - 1], in2 = s->imass[ja - 1];
+      int nmu = t->norb[in1 - 1], nnu = t->norb[in2 - 1], nb = nmu * nnu;
+      double x, z, cost;
+      rho3_geometry(s, ial, c, &x, &z, &cost, eps);
+      double *blk = s->rho2[0] + s->boff[s->noff[ia - 1] + mneigh - 1];
+      for (int isorp = 1; isorp <= t->nssh[indna - 1]; isorp++) {
+        double q = QIN(s, ial, isorp);
+        int rc = fbo_getMEs_3c(t, in1, in2, indna, P_RHO_3C, isorp, x, z, nmu, nnu, cost, bm);
+        if (rc) return rc;
+        fbo_rotate(t, in1, in2, eps, nmu, nnu, bm, bx);
+        for (int i = 0; i < nb; i++) blk[i] = blk[i] + bx[i] * q;
+      }
+    }
+  }
+  s->rdone[1] = 1;
+  return FBO_OK;
+}
+
+/* assemble_rho_weighted_2c, M_assemble_rho_McWEDA_DOGS.f90:534-762 (no rotation:
+ * shell blocks are scalars per shell pair) */
+int fbo_assemble_rho_weighted_2c(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  if (!s->neighn || !s->qin || !s->wboff) return FBO_E_ORDER;
+  double *wi = s->wrho[0], *wb = s->wrho[1];
+  memset(wi, 0, sizeof(double) * (s->wlen ? s->wlen : 1));
+  memset(wb, 0, sizeof(double) * (s->wlen ? s->wlen : 1));
+  double bm[16];
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    int in1 = s->imass[ia - 1], nsi = t->nssh[in1 - 1];
+    for (int in = 0; in < s->neighn[ia - 1]; in++) {
+      int p = s->noff[ia - 1] + in;
+      int mb = s->neigh_b[p], ja = s->neigh_j[p];
+      double r2[3];
+      for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+      int in2 = s->imass[ja - 1], nsj = t->nssh[in2 - 1], nb = nsi * nsj;
+      double z = fbo_distance(r1, r2);
+      if (ia == ja && mb == 0) continue;
+      double *bi = wi + s->wboff[p], *bb = wb + s->wboff[p];
+      for (int isorp = 1; isorp <= nsi; isorp++) {
+        double q = QIN(s, ia, isorp);
+        int rc = fbo_getMEs_2c(t, in1, in2, P_RHOS_ONTOPL, isorp, z, nsi, nsj, bm);
+        if (rc) return rc;
+        for (int i = 0; i < nb; i++) { bi[i] = bi[i] + bm[i] * q; bb[i] = bb[i] + bm[i] * q; }
+      }
+      for (int isorp = 1; isorp <= nsj; isorp++) {
+        double q = QIN(s, ja, isorp);
+        int rc = fbo_getMEs_2c(t, in1, in2, P_RHOS_ONTOPR, isorp, z, nsi, nsj, bm);
+        if (rc) return rc;
+        for (int i = 0; i < nb; i++) { bi[i] = bi[i] + bm[i] * q; bb[i] = bb[i] + bm[i] * q; }
+      }
+    }
+  }
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    int in1 = s->imass[ia - 1], nsi = t->nssh[in1 - 1], nb = nsi * nsi;
+    int pself = s->noff[ia - 1] + s->neigh_self[ia - 1] - 1;
+    double *bi = wi + s->wboff[pself], *bb = wb + s->wboff[pself];
+    for (int in = 0; in < s->neighn[ia - 1]; in++) {
+      int p = s->noff[ia - 1] + in;
+      int mb = s->neigh_b[p], ja = s->neigh_j[p];
+      double r2[3];
+      for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+      int in2 = s->imass[ja - 1];
+      double z = fbo_distance(r1, r2);
+      int self = (ia == ja && mb == 0);
+      for (int isorp = 1; isorp <= t->nssh[in2 - 1]; isorp++) {
+        double q = QIN(s, ja, isorp);
+        int rc = fbo_getMEs_2c(t, in1, in2, P_RHOS_ATOM, isorp, z, nsi, nsi, bm);
+        if (rc) return rc;
+        for (int i = 0; i < nb; i++) bi[i] = bi[i] + bm[i] * q;
+        if (self)
+          for (int i = 0; i < nb; i++) bb[i] = bb[i] + bm[i] * q;
+      }
+    }
+  }
+  s->rdone[2] = 1;
+  s->rdone[3] = 0;
+  return FBO_OK;
+}
+
+/* assemble_rho_weighted_3c, M_assemble_rho_McWEDA_DOGS.f90:783-924 */
+int fbo_assemble_rho_weighted_3c(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  if (!s->neighn || !s->rdone[2]) return FBO_E_ORDER;
+  double bm[16], eps[9];
+  for (int ial = 1; ial <= s->natoms; ial++) {
+    int indna = s->imass[ial - 1];
+    for (int in = 0; in < s->ncommon[ial - 1]; in++) {
+      int c = s->coff[ial - 1] + in;
+      int mneigh = s->cm[c];
+      if (mneigh == 0) continue;
+      int ia = s->ci_j[c], ja = s->cj_j[c];
+      int in1 = s->imass[ia - 1], in2 = s->imass[ja - 1];
+      int nsi = t->nssh[in1 - 1], nsj = t->nssh[in2 - 1], nb = nsi * nsj;
+      double x, z, cost;
+      rho3_geometry(s, ial, c, &x, &z, &cost, eps);
+      double *blk = s->wrho[0] + s->wboff[s->noff[ia - 1] + mneigh - 1];
+      for (int isorp = 1; isorp <= t->nssh[indna - 1]; isorp++) {
+        double q = QIN(s, ial, isorp);
+        int rc = fbo_getMEs_3c(t, in1, in2, indna, P_RHOS_3C, isorp, x, z, nsi, nsj, cost, bm);
+        if (rc) return rc;
+        for (int i = 0; i < nb; i++) blk[i] = blk[i] + bm[i] * q;
+      }
+    }
+  }
+  s->rdone[3] = 1;
+  return FBO_OK;
+}
+
+int64_t fbo_shell_block_len(const fbo_system *s) { return s->wlen; }
+
+/* which: 0 rho_in, 1 rho_bond (orbital blocks, fbo_block_len doubles),
+ *        2 rho_in_weighted, 3 rho_bond_weighted (shell blocks, fbo_shell_block_len) */
+int fbo_get_rho(const fbo_system *s, int which, double *out) {
+  if (which < 0 || which > 3) return FBO_E_ARG;
+  if (which < 2) memcpy(out, s->rho2[which], sizeof(double) * s->blen);
+  else memcpy(out, s->wrho[which - 2], sizeof(double) * s->wlen);
+  return FBO_OK;
+}

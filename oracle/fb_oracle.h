@@ -101,4 +101,16 @@ int fbo_get_forces(const fbo_system *s, double *vna_atom, double *vna_ontop, dou
                    double *f3nac);
 int fbo_deps_warnings(const fbo_system *s);
 
+/* McWEDA densities (j.ASSEMBLERS/M_assemble_rho_McWEDA_DOGS.f90).  qin is
+ * [natoms][FBO_MAXSH] (s%atom(:)%shell(:)%Qin).  Orbital blocks (rho_in,
+ * rho_bond) use the block layout; shell blocks (rho_in_weighted,
+ * rho_bond_weighted) are column-major nssh_mu x nssh_nu per pair, pair order. */
+int fbo_set_qin(fbo_system *s, const double *qin);
+int fbo_assemble_rho_2c(fbo_system *s);
+int fbo_assemble_rho_3c(fbo_system *s);
+int fbo_assemble_rho_weighted_2c(fbo_system *s);
+int fbo_assemble_rho_weighted_3c(fbo_system *s);
+int64_t fbo_shell_block_len(const fbo_system *s);
+int fbo_get_rho(const fbo_system *s, int which, double *out);
+
 #endif

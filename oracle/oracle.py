@@ -74,6 +74,13 @@ def lib():
             getattr(L, f).argtypes = [vp]
         L.fbo_get_dblocks.argtypes = [vp, C.c_int, dp]
         L.fbo_get_forces.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.fbo_set_qin.argtypes = [vp, dp]
+        for f in ("fbo_assemble_rho_2c", "fbo_assemble_rho_3c", "fbo_assemble_rho_weighted_2c",
+                  "fbo_assemble_rho_weighted_3c"):
+            getattr(L, f).argtypes = [vp]
+        L.fbo_shell_block_len.restype = i64
+        L.fbo_shell_block_len.argtypes = [vp]
+        L.fbo_get_rho.argtypes = [vp, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -307,6 +314,37 @@ class System:
         res.update(self.forces())
         res["deps_warnings"] = lib().fbo_deps_warnings(self.h)
         return res
+
+    RHO = {"RHO_IN": 0, "RHO_BOND": 1, "RHOW_IN": 2, "RHOW_BOND": 3}
+
+    def set_qin(self, qin):
+        qin = np.ascontiguousarray(qin[:, :MAXSH], np.float64)
+        assert qin.shape == (self.st.natoms, MAXSH)
+        self._qin = qin
+        _check(lib().fbo_set_qin(self.h, _dp(qin)), "set_qin")
+
+    def assemble_rho(self, which=("RHO2C", "RHO3C", "RHOW2C", "RHOW3C")):
+        """assemble_rho_2c, _3c, assemble_rho_weighted_2c, _3c, in the order of
+        j.ASSEMBLERS/M_assemble_vxc_McWEDA_DOGS.f90:148-154."""
+        L = lib()
+        fn = {"RHO2C": L.fbo_assemble_rho_2c, "RHO3C": L.fbo_assemble_rho_3c,
+              "RHOW2C": L.fbo_assemble_rho_weighted_2c, "RHOW3C": L.fbo_assemble_rho_weighted_3c}
+        for w in which:
+            _check(fn[w](self.h), "assemble_" + w)
+
+    def rho(self, which):
+        L = lib()
+        k = self.RHO[which]
+        n = L.fbo_block_len(self.h) if k < 2 else L.fbo_shell_block_len(self.h)
+        out = np.zeros(max(n, 1))
+        _check(L.fbo_get_rho(self.h, k, _dp(out)), "get_rho")
+        return out[:n]
+
+    def run_rho(self, qin):
+        """The four McWEDA density assemblers after build_lists(); all four arrays."""
+        self.set_qin(qin)
+        self.assemble_rho()
+        return {k: self.rho(k) for k in self.RHO}
 
     def run_all(self):
         lists = self.build_lists()

@@ -163,3 +163,62 @@ def run_forces(ts, structure, rho, lists, binary=HARNESS_DREF, nrep=1, workdir=N
         import shutil
         shutil.rmtree(tmp, ignore_errors=True)
     return res
+
+
+HARNESS_RREF = os.path.join(HERE, "_ref", "fb_rharness_ref")
+
+
+def parse_rho(path, species, imass, neighn, neigh_j):
+    """rho_in, rho_bond (orbital blocks) and rho_in_weighted, rho_bond_weighted
+    (shell blocks nssh_mu x nssh_nu), each flat in pair order."""
+    with open(path, "rb") as f:
+        r = _Reader(f.read())
+    natoms, npairs, ntrip, nrep = r.i32(4)
+    times = r.f64(4)
+    norb = np.array([sp.norb for sp in species], np.int64)
+    nssh = np.array([sp.nssh for sp in species], np.int64)
+    out = {k: [] for k in ("RHO_IN", "RHO_BOND", "RHOW_IN", "RHOW_BOND")}
+    for size, keys in ((norb, ("RHO_IN", "RHO_BOND")), (nssh, ("RHOW_IN", "RHOW_BOND"))):
+        p = 0
+        for ia in range(natoms):
+            a = size[imass[ia] - 1]
+            for _ in range(int(neighn[ia])):
+                b = size[imass[neigh_j[p] - 1] - 1]
+                for k in keys:
+                    out[k].append(r.f64(int(a * b)))
+                p += 1
+    assert r.pos == len(r.buf), "trailing bytes in density harness output"
+    res = {k: (np.concatenate(v) if v else np.zeros(0)) for k, v in out.items()}
+    res["times"] = dict(zip(("RHO2C", "RHO3C", "RHOW2C", "RHOW3C"), times))
+    res["nrep"] = int(nrep)
+    res["ntriplets"] = int(ntrip)
+    return res
+
+
+def write_qin(path, qin, species, imass):
+    """Stream of binary64 shell charges, nssh values per atom in atom order."""
+    vals = [np.asarray(qin[ia, :species[imass[ia] - 1].nssh], np.float64) for ia in range(len(imass))]
+    np.concatenate(vals).tofile(path)
+
+
+def run_rho(ts, structure, qin, lists, binary=HARNESS_RREF, nrep=1, workdir=None, timeout=3600):
+    """Run the McWEDA density harness on (tables, structure) with shell charges
+    qin [natoms][4]; `lists` are the reference's neighbour lists (pair order)."""
+    own = workdir is None
+    tmp = tempfile.mkdtemp(prefix="fbrref_") if own else workdir
+    fdir = os.path.join(tmp, "Fdata")
+    if not os.path.isdir(fdir):
+        fdata_io.write_fdata(ts, fdir)
+    inp = os.path.join(tmp, "harness.in")
+    fdata_io.write_harness_input(inp, fdir, ts, structure)
+    qf = os.path.join(tmp, "qin.bin")
+    write_qin(qf, qin, ts.species, structure.imass)
+    outp = os.path.join(tmp, "rharness.out")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run([binary, inp, qf, outp, str(nrep)], cwd=tmp, check=True, env=env,
+                   stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=timeout)
+    res = parse_rho(outp, ts.species, structure.imass, lists["neighn"], lists["neigh_j"])
+    if own:
+        import shutil
+        shutil.rmtree(tmp, ignore_errors=True)
+    return res

@@ -4,6 +4,8 @@ build container, where the reference sources are present).
     make -C oracle ref dref       # compile the reference modules + harnesses
     python tests/golden/make_golden.py            # H/S fixtures and forces
     python tests/golden/make_golden.py --forces   # force fixtures only
+    make -C oracle rref                            # McWEDA density harness
+    python tests/golden/make_golden.py --rho      # density fixtures only
 
 Each fixture is an .npz holding the inputs that are not re-derivable from a
 seed (explicit positions for the hand-made cases), the reference's neighbour
@@ -116,7 +118,48 @@ def forces_fixture(name, ts, st, keep_atoms):
     print(f"{name}_forces: Dblocks kept for {npair_keep} pairs")
 
 
+RHO_TABLE_SEED = 4048
+QIN_SEED = 5
+RKEYS = ("RHO_IN", "RHO_BOND", "RHOW_IN", "RHOW_BOND")
+
+
+def rho_fixture(name, ts, st, keep_atoms):
+    """<name>_rho.npz: the reference's McWEDA density assemblers
+    (assemble_rho_2c, _3c, assemble_rho_weighted_2c, _3c) with the density
+    tables of synth.add_rho_tables(seed RHO_TABLE_SEED) and the shell charges
+    synth.shell_qin(seed QIN_SEED); orbital blocks of the pairs of the first
+    keep_atoms atoms, shell blocks of all pairs, checksums of all."""
+    import copy
+    tsr = copy.copy(ts)
+    tsr.cells2c = dict(ts.cells2c)
+    tsr.cells3c = dict(ts.cells3c)
+    synth.add_rho_tables(tsr, seed=RHO_TABLE_SEED)
+    r = refrun.run(ts, st)
+    qin = synth.shell_qin(st, ts.species, seed=QIN_SEED)
+    f = refrun.run_rho(tsr, st, qin, r)
+    norb = np.array([sp.norb for sp in ts.species])
+    pair_atom = np.repeat(np.arange(st.natoms), r["neighn"])
+    sizes = norb[st.imass[pair_atom] - 1] * norb[st.imass[r["neigh_j"] - 1] - 1]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    npair_keep = len(sizes) if keep_atoms is None else int(np.sum(r["neighn"][:keep_atoms]))
+    d = {"rho_table_seed": np.int64(RHO_TABLE_SEED), "qin": qin, "npair_blocks": np.int32(npair_keep),
+         "rho_tables_digest": np.array(tables_digest(tsr))}
+    for k in RKEYS:
+        d[k] = f[k][: offs[npair_keep]] if k in ("RHO_IN", "RHO_BOND") else f[k]
+        d[k + "_sum"] = np.float64(np.sum(f[k]))
+        d[k + "_sumsq"] = np.float64(np.sum(f[k] ** 2))
+        d[k + "_sumabs"] = np.float64(np.sum(np.abs(f[k])))
+    np.savez_compressed(os.path.join(HERE, f"{name}_rho.npz"), **d)
+    print(f"{name}_rho: orbital blocks kept for {npair_keep} pairs")
+
+
 def main():
+    if "--rho" in sys.argv:
+        if not os.path.isfile(refrun.HARNESS_RREF):
+            sys.exit("density harness not built: run `make -C oracle rref` first")
+        for name, ts, st, keep_atoms in cases():
+            rho_fixture(name, ts, st, keep_atoms if keep_atoms is not None else (10 if st.natoms > 32 else None))
+        return
     if not refrun.available():
         sys.exit("reference harness not built: run `make -C oracle ref` first")
     want_forces = os.path.isfile(refrun.HARNESS_DREF)

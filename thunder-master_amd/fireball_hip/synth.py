@@ -32,9 +32,11 @@ P_ABOHR = 0.529177          # include/constants.h:2
 P_EQ2 = 14.39975            # include/constants.h:4
 P_MAXTHETA = 5              # include/interactions_3c.h:4
 
-# interaction ids (include/interactions_2c.h:5-14, interactions_3c.h:8)
+# interaction ids (include/interactions_2c.h:5-25, interactions_3c.h:6-8)
 P_OVERLAP, P_KINETIC, P_VNA_ONTOPL, P_VNA_ONTOPR, P_VNA_ATOM, P_DIPOLE_Z = 1, 2, 3, 4, 5, 10
-P_BCNA = 3
+P_RHO_ONTOPL, P_RHO_ONTOPR, P_RHO_ATOM = 15, 16, 17
+P_RHOS_ONTOPL, P_RHOS_ONTOPR, P_RHOS_ATOM = 19, 20, 21
+P_RHO_3C, P_RHOS_3C, P_BCNA = 1, 2, 3
 
 MASS = {1: 1.008, 3: 6.94, 6: 12.011, 7: 14.007, 8: 15.999, 14: 28.0855}
 SYMBOL = {1: "H", 6: "C", 7: "N", 8: "O", 14: "Si"}
@@ -284,6 +286,139 @@ def make_tables(species: list, seed: int = 2024, nx2c: int = 256, n3c: int = 32,
                             ts.cells3c[(a_, b_, c_, P_BCNA, isorp, itheta)] = Cell3c(
                                 nME, n3c, n3c, xmaxA, xmaxA, mu, nu, mv, np.ascontiguousarray(data))
     return ts
+
+
+def munu_S(sp_i: Species, sp_j: Species):
+    """(mu, nu, mvalue) of the shell-averaged ("spherical") tables: one element
+    per shell pair, issh major, mu = issh, nu = jssh, mvalue 0
+    (make_munuS / make_munuS_atom, c.SYSTEM/M_integrals_2c.f90:521-755, and
+    make_munuS_3c, c.SYSTEM/M_integrals_3c.f90:623-753)."""
+    mu, nu = [], []
+    for i in range(sp_i.nssh):
+        for j in range(sp_j.nssh):
+            mu.append(i + 1)
+            nu.append(j + 1)
+    n = len(mu)
+    return np.array(mu, np.int32), np.array(nu, np.int32), np.zeros(n, np.int32)
+
+
+def add_rho_tables(ts: TableSet, seed: int = 4048, with_3c: bool = True) -> TableSet:
+    """Add the McWEDA density tables to ``ts`` (own random stream, so the cells
+    already in ``ts`` do not change):
+
+    * 2c rho_ontopL 15 (isorp 1..nssh(left)), rho_ontopR 16 (isorp
+      1..nssh(right)), rho_atom 17 (isorp 1..nssh(right), both orbitals on the
+      left atom), in the orbital layout of make_munu / make_munu_atom;
+    * 2c rhoS_ontopL 19, rhoS_ontopR 20, rhoS_atom 21, one element per shell
+      pair (make_munuS, make_munuS_atom);
+    * 3c rho_3c 1 (make_munu_3c layout) and rhoS_3c 2 (make_munuS_3c), isorp
+      1..nssh(third atom), five Legendre theta cells.
+
+    Radial shapes as in make_tables; the ontop pairs are mirrored across the
+    two centres the way make_tables mirrors the Vna ontop tables, so the 2c
+    density blocks satisfy rho(j, i) = rho(i, j)^T."""
+    rng = np.random.default_rng(seed)
+    species = ts.species
+    ns = len(species)
+    any2 = next(iter(ts.cells2c.values()))
+    nx2c = any2.nx
+    xmaxA = any2.xmax
+    r = _grid_bohr(nx2c)
+    env = (1.0 - (r / XMAX_BOHR) ** 2) ** 3
+    params = {}
+
+    def radial(key):
+        if key not in params:
+            A = rng.uniform(-2.0, 2.0)
+            a = rng.uniform(0.5, 1.5)
+            params[key] = A * np.exp(-a * r) * env
+        return params[key]
+
+    def parity(l1, l2):
+        return -1.0 if (l1 + l2) % 2 else 1.0
+
+    for a_ in range(1, ns + 1):
+        for b_ in range(1, ns + 1):
+            spA, spB = species[a_ - 1], species[b_ - 1]
+            mu, nu, mv, shells = munu_2c(spA, spB)
+            nME = len(mu)
+            for isorp in range(1, spA.nssh + 1):
+                data = np.empty((nx2c, nME))
+                for k, (i, j, m) in enumerate(shells):
+                    data[:, k] = radial(("rhoP", a_, isorp, a_, i, b_, j, abs(m)))
+                ts.cells2c[(a_, b_, P_RHO_ONTOPL, isorp)] = Cell2c(nME, nx2c, xmaxA, mu, nu, mv, data)
+            for isorp in range(1, spB.nssh + 1):
+                data = np.empty((nx2c, nME))
+                for k, (i, j, m) in enumerate(shells):
+                    li, lj = spA.lssh[i], spB.lssh[j]
+                    data[:, k] = parity(li, lj) * radial(("rhoP", b_, isorp, b_, j, a_, i, abs(m)))
+                ts.cells2c[(a_, b_, P_RHO_ONTOPR, isorp)] = Cell2c(nME, nx2c, xmaxA, mu, nu, mv, data)
+            mua, nua, mva, shells_a = munu_2c(spA, spA)
+            for isorp in range(1, spB.nssh + 1):
+                data = np.empty((nx2c, len(mua)))
+                for k, (i, j, m) in enumerate(shells_a):
+                    data[:, k] = radial(("rhoA", a_, b_, isorp, min(i, j), max(i, j), abs(m)))
+                ts.cells2c[(a_, b_, P_RHO_ATOM, isorp)] = Cell2c(len(mua), nx2c, xmaxA, mua, nua, mva, data)
+            # shell-averaged tables: functions of the shell pair only
+            smu, snu, smv = munu_S(spA, spB)
+            for isorp in range(1, spA.nssh + 1):
+                data = np.empty((nx2c, len(smu)))
+                for k in range(len(smu)):
+                    data[:, k] = radial(("rhoSP", a_, isorp, a_, smu[k] - 1, b_, snu[k] - 1))
+                ts.cells2c[(a_, b_, P_RHOS_ONTOPL, isorp)] = Cell2c(len(smu), nx2c, xmaxA, smu, snu, smv, data)
+            for isorp in range(1, spB.nssh + 1):
+                data = np.empty((nx2c, len(smu)))
+                for k in range(len(smu)):
+                    data[:, k] = radial(("rhoSP", b_, isorp, b_, snu[k] - 1, a_, smu[k] - 1))
+                ts.cells2c[(a_, b_, P_RHOS_ONTOPR, isorp)] = Cell2c(len(smu), nx2c, xmaxA, smu, snu, smv, data)
+            amu, anu, amv = munu_S(spA, spA)
+            for isorp in range(1, spB.nssh + 1):
+                data = np.empty((nx2c, len(amu)))
+                for k in range(len(amu)):
+                    i, j = amu[k] - 1, anu[k] - 1
+                    data[:, k] = radial(("rhoSA", a_, b_, isorp, min(i, j), max(i, j)))
+                ts.cells2c[(a_, b_, P_RHOS_ATOM, isorp)] = Cell2c(len(amu), nx2c, xmaxA, amu, anu, amv, data)
+
+    if with_3c:
+        any3 = next(iter(ts.cells3c.values()))
+        n3c = any3.nx
+        xg = _grid_bohr(n3c)
+        for a_ in range(1, ns + 1):
+            for b_ in range(1, ns + 1):
+                lay = {P_RHO_3C: munu_3c(species[a_ - 1], species[b_ - 1]),
+                       P_RHOS_3C: munu_S(species[a_ - 1], species[b_ - 1])}
+                for c_ in range(1, ns + 1):
+                    for itype in (P_RHO_3C, P_RHOS_3C):
+                        mu, nu, mv = lay[itype]
+                        nME = len(mu)
+                        for isorp in range(1, species[c_ - 1].nssh + 1):
+                            for itheta in range(1, P_MAXTHETA + 1):
+                                A = rng.uniform(-0.5, 0.5, size=nME)
+                                ax = rng.uniform(0.5, 1.5, size=nME)
+                                by = rng.uniform(0.5, 1.5, size=nME)
+                                data = (A[None, None, :] * np.exp(-ax[None, None, :] * xg[None, :, None])
+                                        * np.exp(-by[None, None, :] * xg[:, None, None]))
+                                ts.cells3c[(a_, b_, c_, itype, isorp, itheta)] = Cell3c(
+                                    nME, n3c, n3c, any3.xmax, any3.ymax, mu, nu, mv, np.ascontiguousarray(data))
+    return ts
+
+
+# neutral shell occupations of the synthetic species (info.dat Qneutral)
+QNEUTRAL = {1: [1.0], 6: [2.0, 2.0], 7: [2.0, 3.0], 8: [2.0, 4.0], 14: [2.0, 2.0]}
+
+
+def shell_qin(s: Structure, species, seed=5, scale=0.05):
+    """Input shell charges Qin [natoms][4] = Qneutral + dQ (the DOGS charges of
+    the structure, or a fresh random dQ with ``seed`` when it has none), as
+    c.SYSTEM/M_configuraciones.f90:872-908 sets s%atom(:)%shell(:)%Qin."""
+    rng = np.random.default_rng(seed)
+    q = np.zeros((s.natoms, 4))
+    for ia in range(s.natoms):
+        sp_ = species[s.imass[ia] - 1]
+        qn = np.array(QNEUTRAL.get(sp_.nZ, [1.0] * sp_.nssh)[:sp_.nssh], float)
+        dq = s.dQ[ia, :sp_.nssh] if s.dQ is not None else rng.uniform(-scale, scale, size=sp_.nssh)
+        q[ia, :sp_.nssh] = qn + dq
+    return q
 
 
 # ---------------------------------------------------------------------------
