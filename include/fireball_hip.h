@@ -48,7 +48,11 @@ enum fb_matrix {            /* which block array */
   FB_MAT_T = 1,             /* s%kinetic(:)%neighbors(:)%block */
   FB_MAT_DIP = 2,           /* s%dipole_z(:)%neighbors(:)%block */
   FB_MAT_VNA = 3,           /* s%vna(:)%neighbors(:)%block  (atom term on neigh_self, 3c elsewhere) */
-  FB_MAT_VNAO = 4           /* s%vna(:)%neighbors(:)%blocko (ontop terms) */
+  FB_MAT_VNAO = 4,          /* s%vna(:)%neighbors(:)%blocko (ontop terms) */
+  FB_MAT_RHO_IN = 5,        /* s%rho_in(:)%neighbors(:)%block            (McWEDA, orbital blocks) */
+  FB_MAT_RHO_BOND = 6,      /* s%rho_bond(:)%neighbors(:)%block          (McWEDA, orbital blocks) */
+  FB_MAT_RHOW_IN = 7,       /* s%rho_in_weighted(:)%neighbors(:)%block   (McWEDA, shell blocks) */
+  FB_MAT_RHOW_BOND = 8      /* s%rho_bond_weighted(:)%neighbors(:)%block (McWEDA, shell blocks) */
 };
 
 enum fb_precision {
@@ -150,7 +154,8 @@ int fb_get_device_blocks(fb_ctx *ctx, int matrix, void **dptr, int64_t *len);
  *   vna_atom     forces%vna_atom(3, ineigh) per pair, [npairs][3]
  *   vna_ontop    forces%vna_ontop(3, ineigh) per pair, [npairs][3]
  *   f3naa/b/c    forces%f3naa / f3nab / f3nac per atom, [natoms][3]
- * The force kernels compute in binary64 whatever fb_set_precision says. */
+ * The force kernels compute in binary64; under FB_PREC_MIXED the 3c force
+ * interpolation (values and derivatives of the table stencil) runs in FP32. */
 int fb_set_denmat(fb_ctx *ctx, int istruct, const double *rho);  /* s%denmat(i)%neighbors(k)%block */
 int fb_dassemble_S(fb_ctx *ctx);          /* k.DASSEMBLERS/M_Dassemble_2c_DOGS.f90:95-266   */
 int fb_dassemble_T(fb_ctx *ctx);          /* k.DASSEMBLERS/M_Dassemble_2c_DOGS.f90:292-463  */
@@ -161,6 +166,28 @@ int fb_dassemble_all(fb_ctx *ctx);        /* all five: one 2c launch, the 3c lau
 int fb_get_dblocks(fb_ctx *ctx, int istruct, int matrix, double *out);
 int fb_get_forces(fb_ctx *ctx, int istruct, double *vna_atom, double *vna_ontop, double *f3naa, double *f3nab,
                   double *f3nac);
+
+/* --- McWEDA densities: j.ASSEMBLERS/M_assemble_rho_McWEDA_DOGS.f90 -------
+ * The input density of the McWEDA exchange-correlation terms, assembled from
+ * the same lists and frames as H and S (the triplet consumers of the 3c lists
+ * besides vna_3c).  fb_set_qin gives the input shell charges
+ * s%atom(:)%shell(:)%Qin, [natoms][FB_MAXSH], of one structure; every
+ * structure of the batch needs them before the first density call.  Call order
+ * as in assemble_vxc (j.ASSEMBLERS/M_assemble_vxc_McWEDA_DOGS.f90:148-154):
+ * rho_2c, rho_3c, rho_weighted_2c, rho_weighted_3c (rho_3c adds into the
+ * rho_in blocks of rho_2c, rho_weighted_3c into those of rho_weighted_2c).
+ * Results through fb_get_blocks: FB_MAT_RHO_IN / FB_MAT_RHO_BOND in the block
+ * layout (fb_get_block_len doubles), FB_MAT_RHOW_IN / FB_MAT_RHOW_BOND as
+ * column-major nssh_mu x nssh_nu shell blocks in pair order
+ * (fb_get_shell_block_len doubles).  Binary64, the reference's summation
+ * order; a new list build discards them. */
+int fb_set_qin(fb_ctx *ctx, int istruct, const double *Qin);
+int fb_assemble_rho_2c(fb_ctx *ctx);            /* M_assemble_rho_McWEDA_DOGS.f90:90-341  */
+int fb_assemble_rho_3c(fb_ctx *ctx);            /* M_assemble_rho_McWEDA_DOGS.f90:362-511 */
+int fb_assemble_rho_weighted_2c(fb_ctx *ctx);   /* M_assemble_rho_McWEDA_DOGS.f90:534-762 */
+int fb_assemble_rho_weighted_3c(fb_ctx *ctx);   /* M_assemble_rho_McWEDA_DOGS.f90:783-924 */
+int fb_assemble_rho_all(fb_ctx *ctx);           /* the four in order: one 2c launch, two 3c launches */
+int fb_get_shell_block_len(fb_ctx *ctx, int istruct, int64_t *len);
 
 /* --- consumer: Gamma-point eigenvalues ------------------------------------
  * Folds the blocks of structure istruct at k = 0 into a dense binary64 matrix

@@ -11,6 +11,7 @@
 // k_3c_fp64 is the plain binary64 kernel: one thread per triplet evaluates the
 // reference's interpolation (e.FDATA/M_Fdata_3c.f90:293-517) literally.
 #include "fb_device.h"
+#include "fb_3c_dev.h"
 
 #define T3 128
 #define NB (FB_MO * FB_MO)
@@ -18,82 +19,6 @@
 
 __device__ __forceinline__ int group3c_id(const DView &v, int in1, int in2, int in3, int isub) {
   return v.idx3c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * v.sp.ns + (in3 - 1)) * FB_MAXSUB3 + isub];
-}
-
-// Legendre P0..P4 and sin(theta) (M_Fdata_3c.f90:347-354)
-__device__ __forceinline__ void d_legendre(double cost, double *p, double *sint) {
-  double cost2 = cost * cost;
-  *sint = sqrt(fmax(1.0e-5, 1.0 - cost2));
-  p[0] = 1.0;
-  p[1] = cost;
-  p[2] = 0.5 * (3.0 * cost2 - 1.0);
-  p[3] = 0.5 * cost * (5.0 * cost2 - 3.0);
-  p[4] = 0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2));
-}
-
-// getMEs_Fdata_3c in binary64 with the reference's method selection and
-// evaluation order; writes the recovered molecular-frame block hm (nmu x nnu).
-__device__ inline int d_getMEs_3c(const DView &v, int gid, double x, double y, double cost, int nmu, int nnu,
-                                  double *hm) {
-  if (gid < 0) return ST_NO_TABLE;
-  const DGroup3c g = v.grp3c[gid];
-  const int nME = g.nME, nx = g.nx, ny = g.ny;
-  double p[5], sint;
-  d_legendre(cost, p, &sint);
-  const double dx = g.xmax / (nx - 1.0), dy = g.ymax / (ny - 1.0);
-  int ix = (int)(x / dx) + 1, iy = (int)(y / dy) + 1;
-  if (ix < 2) ix = 2;
-  if (iy < 2) iy = 2;
-  if (ix > nx - 2) ix = nx - 2;
-  if (iy > ny - 2) iy = ny - 2;
-  const double px = x / dx - ix + 1.0, py = y / dy - iy + 1.0;
-  const double *tab = v.tab3c + g.off;
-  const int64_t tstride = (int64_t)ny * nx * nME;
-#define T3D(th, i, j, m) tab[(th) * tstride + ((int64_t)((j) - 1) * nx + ((i) - 1)) * nME + (m)]
-  const double gx = (T3D(0, ix + 1, iy, 0) - T3D(0, ix, iy, 0)) / dx;
-  const double gy = (T3D(0, ix, iy + 1, 0) - T3D(0, ix, iy, 0)) / dy;
-  const double gg = gx * gx + gy * gy;
-  double F[NME3];
-  for (int m = 0; m < nME; m++) F[m] = 0.0;
-  for (int th = 0; th < FB_NTHETA; th++) {
-    for (int m = 0; m < nME; m++) {
-      double interim;
-      if (gg < 1.0e-6) {
-        interim = ((1.0 - px - py) * T3D(th, ix, iy, m) + px * T3D(th, ix + 1, iy, m)) + py * T3D(th, ix, iy + 1, m);
-      } else {
-        double gk[4];
-        for (int k = -1; k <= 2; k++) {
-          double f1m1 = T3D(th, ix + k, iy - 1, m);
-          double f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
-          double f0p3 = 3.0 * T3D(th, ix + k, iy, m), f0p6 = 2.0 * f0p3;
-          double f1p3 = 3.0 * T3D(th, ix + k, iy + 1, m), f1p6 = 2.0 * f1p3;
-          double f2p1 = T3D(th, ix + k, iy + 2, m);
-          double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
-          double bb2 = f1m3 - f0p6 + f1p3;
-          double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
-          double bb0 = f0p6;
-          gk[k + 1] = ((bb3 * py + bb2) * py + bb1) * py + bb0;
-        }
-        double f1m1 = gk[0], f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
-        double f0p3 = 3.0 * gk[1], f0p6 = 2.0 * f0p3;
-        double f1p3 = 3.0 * gk[2], f1p6 = 2.0 * f1p3;
-        double f2p1 = gk[3];
-        double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
-        double bb2 = f1m3 - f0p6 + f1p3;
-        double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
-        double bb0 = f0p6;
-        interim = (((bb3 * px + bb2) * px + bb1) * px + bb0) / 36.0;
-      }
-      F[m] = F[m] + p[th] * interim;
-    }
-  }
-#undef T3D
-  for (int i = 0; i < nmu * nnu; i++) hm[i] = 0.0;
-  for (int k = 0; k < nME; k++) {
-    double val = (v.mv[g.munu + k] != 1) ? F[k] : F[k] * sint;
-    hm[(v.nu[g.munu + k] - 1) * nmu + (v.mu[g.munu + k] - 1)] = val;
-  }
-  return 0;
 }
 
 // triplet geometry, M_assemble_3c_DOGS.f90:175-271
@@ -136,18 +61,6 @@ __device__ inline void d_geo3(const DView &v, int4 tr, Geo3 &G) {
   G.stinky = G.stinky * d_smoother(G.d23, rc2 + rc3, 0.8);
 }
 
-// segment range owned by the workgroup of tile [ts, te)
-__device__ __forceinline__ bool tile_segments(const DView &v, int64_t ts, int64_t te, int64_t &first, int64_t &last,
-                                              int64_t &r0, int64_t &r1) {
-  int64_t sgs = v.trip_seg[ts];
-  first = (v.seg_off[sgs] == ts) ? sgs : sgs + 1;
-  last = v.trip_seg[te - 1];
-  if (first > last) return false;
-  r0 = v.seg_off[first];
-  r1 = v.seg_off[last + 1];
-  return true;
-}
-
 __global__ __launch_bounds__(T3) void k_3c_fp64(DView v) {
   __shared__ double s_c[T3][NB + 1];
   const int64_t ts = (int64_t)blockIdx.x * T3;
@@ -173,7 +86,7 @@ __global__ __launch_bounds__(T3) void k_3c_fp64(DView v) {
       d_pmat<double>(G.eps, pm);
       const int nb = G.nmu * G.nnu;
       int gid = group3c_id(v, G.in1, G.in2, G.indna, 0);
-      status |= d_getMEs_3c(v, gid, G.x, G.z, G.cost, G.nmu, G.nnu, hm);
+      status |= d_getMEs_3c<NME3>(v, gid, G.x, G.z, G.cost, G.nmu, G.nnu, hm);
       d_rotate<double>(v.sp, G.in1, G.in2, pm, hm, G.nmu, bx);
       for (int i = 0; i < nb; i++) c[i] = bx[i] * FB_P_EQ2;
       bool charged = false;
@@ -187,7 +100,7 @@ __global__ __launch_bounds__(T3) void k_3c_fp64(DView v) {
           double dQ = v.dQ[tr.x * FB_MAXSH + isorp - 1];
           if (dQ == 0.0) continue;
           gid = group3c_id(v, G.in1, G.in2, G.indna, isorp);
-          status |= d_getMEs_3c(v, gid, G.x, G.z, G.cost, G.nmu, G.nnu, hm);
+          status |= d_getMEs_3c<NME3>(v, gid, G.x, G.z, G.cost, G.nmu, G.nnu, hm);
           d_rotate<double>(v.sp, G.in1, G.in2, pm, hm, G.nmu, bx);
           for (int i = 0; i < nb; i++) {
             double sterm = sb[i] / 2.0, dterm = db[i] / G.z;

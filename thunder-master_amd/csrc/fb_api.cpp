@@ -120,6 +120,8 @@ struct HStruct {
   std::vector<int> imass;
   double lattice[9];
   std::vector<double> dQ;  // [natoms][FB_MAXSH]
+  std::vector<double> qin; // [natoms][FB_MAXSH] input shell charges Qin (McWEDA densities)
+  bool has_qin = false;
 };
 
 int slot_of(int itype) {
@@ -130,6 +132,12 @@ int slot_of(int itype) {
     case 4: return SLOT_ONR;
     case 5: return SLOT_ATOM;
     case 10: return SLOT_DIP;
+    case 15: return SLOT_RHO_ONL;
+    case 16: return SLOT_RHO_ONR;
+    case 17: return SLOT_RHO_ATOM;
+    case 19: return SLOT_RHOS_ONL;
+    case 20: return SLOT_RHOS_ONR;
+    case 21: return SLOT_RHOS_ATOM;
   }
   return -1;
 }
@@ -203,6 +211,8 @@ struct fb_ctx {
   std::vector<HCell2c> c2;
   std::vector<HCell3c> c3;
   bool tables_ok = false;
+  DevBuf d_idx3r;   // McWEDA density groups (rho_3c, rhoS_3c)
+  int maxsh = 1;    // largest shell count of any species
   DevBuf d_cell2c, d_idx2c, d_grp3c, d_idx3c, d_tab2c, d_tab2c32, d_tab3c, d_tab3c32, d_mu, d_nu, d_mv, d_meth3c;
   DevBuf d_grid2c, d_pden, d_wg2c;
   int canon_sp = 0;
@@ -215,13 +225,13 @@ struct fb_ctx {
   std::vector<DStruct> h_st;
   std::vector<int> h_imass;
   int natoms = 0;
-  DevBuf d_st, d_xl, d_ratom, d_imass, d_atom_struct, d_dQ;
+  DevBuf d_st, d_xl, d_ratom, d_imass, d_atom_struct, d_dQ, d_qin;
 
   DevBuf d_neighn, d_noff, d_neigh_j, d_neigh_b, d_neigh_back, d_neigh_self, d_pair_atom, d_bsz, d_boff;
   DevBuf d_ncommon, d_toff, d_trip_orig, d_tpair_orig, d_trip, d_tpair, d_perm_in, d_perm, d_flag, d_segincl,
       d_seg_off, d_seg_pair, d_trip_seg, d_pair_tgt, d_trip_aux, d_tiles[3];
   DevBuf d_tmp, d_status;
-  DevBuf d_mat[5];
+  DevBuf d_mat[FB_NMAT];
   int64_t npairs = 0, ntrip = 0, nseg = 0, blen = 0;
   std::vector<int64_t> h_noff, h_toff, h_boff_struct;
   bool lists_ok = false;
@@ -231,6 +241,12 @@ struct fb_ctx {
   DevBuf d_rho, d_dmat[3], d_fatom, d_fontop, d_f3a, d_f3b, d_f3c, d_f3, d_jkey, d_jkey_s, d_iota_t, d_perm_j,
       d_inv_perm, d_inv_perm_j, d_ioff, d_joff;
   long forces_gen = -1;          // lists_gen the force buffers and orderings belong to
+  // McWEDA densities: shell-block offsets, per-structure ranges, progress
+  DevBuf d_wbsz, d_wboff;
+  std::vector<int64_t> h_wboff_struct;
+  int64_t wlen = 0;
+  long rho_gen = -1;             // lists_gen the density buffers belong to
+  int rdone[4] = {0, 0, 0, 0};   // rho_2c, rho_3c, rho_weighted_2c, rho_weighted_3c
   std::vector<char> rho_ok;      // per structure: density blocks uploaded
   int ddone[5] = {0, 0, 0, 0, 0};  // Dblock S, T, dipole_z; vna_2c forces; vna_3c forces
   int force_kernel = 1;            // s/p species: 1 register-resident kernels, 0 the generic ones
@@ -302,6 +318,7 @@ struct fb_ctx {
     v.idx2c = d_idx2c.as<int>();
     v.grp3c = d_grp3c.as<DGroup3c>();
     v.idx3c = d_idx3c.as<int>();
+    v.idx3r = d_idx3r.as<int>();
     v.tab2c = d_tab2c.as<double>();
     v.tab2c32 = d_tab2c32.as<float>();
     v.tab3c = d_tab3c.as<double>();
@@ -324,6 +341,7 @@ struct fb_ctx {
     v.imass = d_imass.as<int>();
     v.atom_struct = d_atom_struct.as<int>();
     v.dQ = d_dQ.as<double>();
+    v.qin = d_qin.as<double>();
     v.npairs = npairs;
     v.neighn = d_neighn.as<int>();
     v.noff = d_noff.as<int64_t>();
@@ -333,6 +351,7 @@ struct fb_ctx {
     v.neigh_self = d_neigh_self.as<int>();
     v.pair_atom = d_pair_atom.as<int>();
     v.boff = d_boff.as<int64_t>();
+    v.wboff = d_wboff.as<int64_t>();
     v.ntrip = ntrip;
     v.nseg = nseg;
     v.trip = d_trip.as<int4>();
@@ -343,7 +362,7 @@ struct fb_ctx {
     v.trip_aux = d_trip_aux.as<int>();
     for (int k = 0; k < 3; k++) v.tiles[k] = d_tiles[k].as<int4>();
     v.trip_seg = d_trip_seg.as<int>();
-    for (int i = 0; i < 5; i++) v.mat[i] = d_mat[i].as<double>();
+    for (int i = 0; i < FB_NMAT; i++) v.mat[i] = d_mat[i].as<double>();
     v.mlp = d_mlp.as<DMlp>();
     v.mlp_err = d_mlp_err.as<float>();
     v.mlp_tau = (float)mlp_tau;
@@ -438,7 +457,7 @@ struct fb_ctx {
                       &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
                       &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_pair_tgt, &d_trip_aux, &d_tiles[0], &d_tiles[1], &d_tiles[2], &d_tmp,
                       &d_mlp, &d_mlp_err, &d_mlp_count, &d_meth3c,
-                      &d_status};
+                      &d_status, &d_idx3r, &d_qin, &d_wbsz, &d_wboff};
     for (DevBuf *b : bufs) b->release();
     for (auto &m : d_mat) m.release();
     DevBuf *fbufs[] = {&d_rho, &d_dmat[0], &d_dmat[1], &d_dmat[2], &d_fatom, &d_fontop, &d_f3a, &d_f3b, &d_f3c,
@@ -581,6 +600,8 @@ int fb_set_species(fb_ctx *ctx, int ns, const int *nZ, const int *nssh, const in
     ctx->nZ[i] = nZ ? nZ[i] : 0;
   }
   ctx->sp = sp;
+  ctx->maxsh = 1;
+  for (int i = 0; i < ns; i++) ctx->maxsh = std::max(ctx->maxsh, sp.nssh[i]);
   ctx->species_set = true;
   ctx->tables_ok = false;
   ctx->lists_ok = false;
@@ -613,7 +634,7 @@ int fb_set_table_3c(fb_ctx *ctx, int is, int js, int ks, int itype, int isub, in
                     const double *fdata) {
   if (!ctx || !ctx->species_set) return ctx ? ctx->fail(FB_E_ORDER, "set_table_3c before set_species") : FB_E_ARG;
   if (is < 1 || js < 1 || ks < 1 || is > ctx->sp.ns || js > ctx->sp.ns || ks > ctx->sp.ns || isub < 0 ||
-      isub >= FB_MAXSUB3 || itheta < 1 || itheta > FB_NTHETA || nx < 4 || ny < 4 || nME < 1 || nME > 10 || !mu ||
+      isub >= FB_MAXSUB3 || itheta < 1 || itheta > FB_NTHETA || nx < 4 || ny < 4 || nME < 1 || nME > 16 || !mu ||
       !nu || !mvalue || !fdata)
     return ctx->fail(FB_E_ARG, "set_table_3c: bad arguments");
   HCell3c c;
@@ -640,7 +661,8 @@ int fb_set_table_3c(fb_ctx *ctx, int is, int js, int ks, int itype, int isub, in
 
 namespace {
 bool fdata_type2c(int itype) { return slot_of(itype) >= 0; }
-bool fdata_type3c(int itype) { return itype == 3; }   // bcna, include/interactions_3c.h:8
+// rho_3c 1, rhoS_3c 2, bcna 3 (include/interactions_3c.h:6-8)
+bool fdata_type3c(int itype) { return itype >= 1 && itype <= 3; }
 
 struct CtxSink : FdataSink {
   fb_ctx *ctx;
@@ -681,8 +703,13 @@ int fb_commit_tables(fb_ctx *ctx) {
     int slot = slot_of(c.itype);
     if (slot < 0) continue;  // interactions outside the assembled path
     {
-      // the block of the atom term lives on the first atom's orbitals (make_munu_atom)
-      int rows = ctx->sp.norb[c.is - 1], cols = (slot == SLOT_ATOM) ? rows : ctx->sp.norb[c.js - 1];
+      // the block of an atom term lives on the first atom's orbitals (make_munu_atom);
+      // the shell-averaged density tables index shells (make_munuS, make_munuS_atom)
+      const bool shell = (slot == SLOT_RHOS_ONL || slot == SLOT_RHOS_ONR || slot == SLOT_RHOS_ATOM);
+      const bool atom = (slot == SLOT_ATOM || slot == SLOT_RHO_ATOM || slot == SLOT_RHOS_ATOM);
+      const int *dim = shell ? ctx->sp.nssh : ctx->sp.norb;
+      int rows = dim[c.is - 1], cols = atom ? rows : dim[c.js - 1];
+      if (c.nME > 16) return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: 2c cell with more than 16 elements");
       for (int k = 0; k < c.nME; k++)
         if (c.mu[k] < 1 || c.mu[k] > rows || c.nu[k] < 1 || c.nu[k] > cols)
           return ctx->fail(FB_E_ARG, "commit_tables: 2c mu/nu outside the orbital block");
@@ -719,6 +746,7 @@ int fb_commit_tables(fb_ctx *ctx) {
     for (int t = 0; t < FB_NTHETA; t++)
       if (!th[t]) return ctx->fail(FB_E_NO_TABLE, "commit_tables: a 3c group lacks a theta cell");
     const HCell3c *c1 = th[0];
+    if (c1->nME > 10) return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: bcna cell with more than 10 elements");
     for (int t = 0; t < FB_NTHETA; t++)
       for (int k = 0; k < th[t]->nME; k++)
         if (th[t]->mu[k] < 1 || th[t]->mu[k] > ctx->sp.norb[th[t]->is - 1] || th[t]->nu[k] < 1 ||
@@ -780,6 +808,54 @@ int fb_commit_tables(fb_ctx *ctx) {
     grps.push_back(g);
     grpkey.push_back({is, js, ks, isub});
   }
+  const size_t nbcna = grps.size();
+  // McWEDA density groups: rho_3c (itype 1, orbital blocks, make_munu_3c) and
+  // rhoS_3c (itype 2, shell blocks, make_munuS_3c), subtypes isorp =
+  // 1..nssh(third atom); the binary64 density kernels read the FP64 slab only
+  std::map<std::tuple<int, int, int, int, int>, std::vector<const HCell3c *>> rgroups;
+  for (auto &c : ctx->c3)
+    if (c.itype == 1 || c.itype == 2) rgroups[std::make_tuple(c.itype, c.is, c.js, c.ks, c.isub)].push_back(&c);
+  std::vector<int> idx3r((size_t)ns * ns * ns * 2 * FB_MAXSUB3, -1);
+  for (auto &kv : rgroups) {
+    int itype, is, js, ks, isub;
+    std::tie(itype, is, js, ks, isub) = kv.first;
+    const HCell3c *th[FB_NTHETA] = {nullptr};
+    for (auto *c : kv.second) th[c->itheta - 1] = c;
+    for (int t = 0; t < FB_NTHETA; t++)
+      if (!th[t]) return ctx->fail(FB_E_NO_TABLE, "commit_tables: a 3c density group lacks a theta cell");
+    const HCell3c *c1 = th[0];
+    const int *dim = (itype == 2) ? ctx->sp.nssh : ctx->sp.norb;
+    for (int t = 0; t < FB_NTHETA; t++) {
+      if (th[t]->nME != c1->nME || th[t]->nx != c1->nx || th[t]->ny != c1->ny || th[t]->xmax != c1->xmax ||
+          th[t]->ymax != c1->ymax)
+        return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: theta cells of a 3c group differ in grid");
+      if (th[t]->nME > 16) return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: 3c cell with more than 16 elements");
+      for (int k = 0; k < th[t]->nME; k++)
+        if (th[t]->mu[k] < 1 || th[t]->mu[k] > dim[is - 1] || th[t]->nu[k] < 1 || th[t]->nu[k] > dim[js - 1])
+          return ctx->fail(FB_E_ARG, "commit_tables: 3c mu/nu outside the block");
+    }
+    if (isub < 1 || isub >= FB_MAXSUB3) return ctx->fail(FB_E_ARG, "commit_tables: 3c density subtype out of range");
+    DGroup3c g{};
+    g.nME = c1->nME;
+    g.nx = c1->nx;
+    g.ny = c1->ny;
+    g.xmax = c1->xmax;
+    g.ymax = c1->ymax;
+    g.dx = c1->xmax / (c1->nx - 1.0);
+    g.dy = c1->ymax / (c1->ny - 1.0);
+    g.off = (int64_t)tab3.size();
+    g.off32 = -1;
+    g.moff = -1;
+    const HCell3c *cl = th[FB_NTHETA - 1];
+    g.munu = (int)mu.size();
+    mu.insert(mu.end(), cl->mu.begin(), cl->mu.end());
+    nu.insert(nu.end(), cl->nu.begin(), cl->nu.end());
+    mv.insert(mv.end(), cl->mv.begin(), cl->mv.end());
+    for (int t = 0; t < FB_NTHETA; t++) tab3.insert(tab3.end(), th[t]->data.begin(), th[t]->data.end());
+    idx3r[((((size_t)(is - 1) * ns + (js - 1)) * ns + (ks - 1)) * 2 + (itype - 1)) * FB_MAXSUB3 + isub] =
+        (int)grps.size();
+    grps.push_back(g);
+  }
   // distinct 2c grids and their Lagrange denominators (host binary64, same
   // expression as e.FDATA/M_Fdata_2c.f90:379-385)
   std::vector<DGrid2c> grids;
@@ -815,7 +891,7 @@ int fb_commit_tables(fb_ctx *ctx) {
   if (canon) {
     for (auto &c : ctx->c2) {
       int slot = slot_of(c.itype);
-      if (slot < 0) continue;
+      if (slot < 0 || slot > SLOT_DIP) continue;   // the s/p kernels serve the H+S interactions
       MeList e = (slot == SLOT_ATOM) ? munu2c(shells[c.is - 1], shells[c.is - 1])
                                      : munu2c(shells[c.is - 1], shells[c.js - 1]);
       if (!same_list(e, c.mu, c.nu, c.mv)) canon = 0;
@@ -831,6 +907,7 @@ int fb_commit_tables(fb_ctx *ctx) {
   RC(upload(ctx, ctx->d_idx2c, idx2.data(), idx2.size() * sizeof(int), "upload idx2c"));
   RC(upload(ctx, ctx->d_grp3c, grps.data(), grps.size() * sizeof(DGroup3c), "upload groups3c"));
   RC(upload(ctx, ctx->d_idx3c, idx3.data(), idx3.size() * sizeof(int), "upload idx3c"));
+  RC(upload(ctx, ctx->d_idx3r, idx3r.data(), idx3r.size() * sizeof(int), "upload idx3r"));
   RC(upload(ctx, ctx->d_tab2c, tab2.data(), tab2.size() * sizeof(double), "upload tab2c"));
   RC(upload(ctx, ctx->d_tab2c32, tab2f.data(), tab2f.size() * sizeof(float), "upload tab2c32"));
   RC(upload(ctx, ctx->d_tab3c, tab3.data(), tab3.size() * sizeof(double), "upload tab3c"));
@@ -848,7 +925,7 @@ int fb_commit_tables(fb_ctx *ctx) {
   RC(upload(ctx, ctx->d_nu, nu.data(), nu.size() * sizeof(int), "upload nu"));
   RC(upload(ctx, ctx->d_mv, mv.data(), mv.size() * sizeof(int), "upload mvalue"));
   FB_HIP(hipStreamSynchronize(ctx->stream));
-  ctx->h_grp3c = grps;
+  ctx->h_grp3c.assign(grps.begin(), grps.begin() + nbcna);   // the networks serve bcna groups only
   ctx->h_grpkey = grpkey;
   ctx->mlp_ready = false;   // networks belong to the previous tables
   ctx->mlp_gen++;
@@ -907,12 +984,23 @@ int fb_set_charges(fb_ctx *ctx, int is, const double *dQ) {
   return FB_OK;
 }
 
+int fb_set_qin(fb_ctx *ctx, int is, const double *qin) {
+  if (!ctx || is < 0 || is >= (int)ctx->structs.size() || !qin) return ctx ? ctx->fail(FB_E_ARG, "set_qin arguments") : FB_E_ARG;
+  auto &s = ctx->structs[is];
+  s.qin.assign(qin, qin + (size_t)s.natoms * FB_MAXSH);
+  s.has_qin = true;
+  ctx->structs_dirty = true;
+  for (int i = 0; i < 4; i++) ctx->rdone[i] = 0;   // the densities follow the charges
+  return FB_OK;
+}
+
 static int upload_structures(fb_ctx *ctx) {
   const int ns = (int)ctx->structs.size();
   int natoms = 0;
   for (auto &s : ctx->structs) natoms += s.natoms;
   ctx->natoms = natoms;
   std::vector<double> ratom((size_t)natoms * 3), dq((size_t)natoms * FB_MAXSH), xl((size_t)ns * FB_NIMG * 3, 0.0);
+  std::vector<double> qin((size_t)natoms * FB_MAXSH, 0.0);
   std::vector<int> imass(natoms), astr(natoms);
   ctx->h_st.assign(ns, DStruct{});
   int off = 0;
@@ -936,6 +1024,8 @@ static int upload_structures(fb_ctx *ctx) {
       astr[off + i] = k;
       d.rcmax_all = std::max(d.rcmax_all, ctx->sp.rcmax[s.imass[i] - 1]);
       for (int q = 0; q < FB_MAXSH; q++) dq[(size_t)(off + i) * FB_MAXSH + q] = s.dQ[(size_t)i * FB_MAXSH + q];
+      if (s.has_qin)
+        for (int q = 0; q < FB_MAXSH; q++) qin[(size_t)(off + i) * FB_MAXSH + q] = s.qin[(size_t)i * FB_MAXSH + q];
     }
     off += s.natoms;
   }
@@ -944,6 +1034,7 @@ static int upload_structures(fb_ctx *ctx) {
   RC(upload(ctx, ctx->d_imass, imass.data(), imass.size() * sizeof(int), "upload imass"));
   RC(upload(ctx, ctx->d_atom_struct, astr.data(), astr.size() * sizeof(int), "upload atom_struct"));
   RC(upload(ctx, ctx->d_dQ, dq.data(), dq.size() * sizeof(double), "upload dQ"));
+  RC(upload(ctx, ctx->d_qin, qin.data(), qin.size() * sizeof(double), "upload Qin"));
   RC(upload(ctx, ctx->d_xl, xl.data(), xl.size() * sizeof(double), "upload xl"));
   RC(upload(ctx, ctx->d_st, ctx->h_st.data(), ctx->h_st.size() * sizeof(DStruct), "upload structs"));
   ctx->structs_dirty = false;
@@ -1839,8 +1930,16 @@ int fb_get_block_len(fb_ctx *ctx, int is, int64_t *len) {
 
 int fb_get_blocks(fb_ctx *ctx, int is, int matrix, double *out) {
   if (!ctx || !ctx->lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "lists not built") : FB_E_ARG;
-  if (is < 0 || is >= (int)ctx->structs.size() || matrix < 0 || matrix > 4 || !out)
+  if (is < 0 || is >= (int)ctx->structs.size() || matrix < 0 || matrix >= FB_NMAT || !out)
     return ctx->fail(FB_E_ARG, "get_blocks arguments");
+  if (matrix >= FB_MAT_RHO_IN) {
+    // densities: valid once their two-centre assembler has run on these lists
+    const bool ok = ctx->rho_gen == ctx->lists_gen && ctx->rdone[matrix >= FB_MAT_RHOW_IN ? 2 : 0];
+    if (!ok) return ctx->fail(FB_E_ORDER, "get_blocks: density not assembled");
+    FB_HIP(hipStreamSynchronize(ctx->stream));
+    const std::vector<int64_t> &o = (matrix >= FB_MAT_RHOW_IN) ? ctx->h_wboff_struct : ctx->h_boff_struct;
+    return d2h(ctx, out, ctx->d_mat[matrix].as<double>() + o[is], sizeof(double) * (o[is + 1] - o[is]));
+  }
   if (!ctx->done[matrix]) return ctx->fail(FB_E_ORDER, "get_blocks: matrix not assembled");
   FB_HIP(hipStreamSynchronize(ctx->stream));
   RC(ctx->settle("assemble_all"));
@@ -1852,6 +1951,97 @@ int fb_get_device_blocks(fb_ctx *ctx, int matrix, void **dptr, int64_t *len) {
   if (!ctx || !ctx->lists_ok || matrix < 0 || matrix > 4) return FB_E_ARG;
   if (dptr) *dptr = ctx->d_mat[matrix].p;
   if (len) *len = ctx->blen;
+  return FB_OK;
+}
+
+// ------------------------------------------------------------ McWEDA densities
+extern "C" {
+int fbk_rho_2c(const DView &v, int weighted, int maxsh, hipStream_t st);
+int fbk_rho_3c(const DView &v, int weighted, int maxsh, hipStream_t st);
+int fbk_shell_sizes(const DView &v, int64_t *sz, hipStream_t st);
+}
+
+// shell-block offsets and the four density arrays, once per list generation
+static int prep_rho(fb_ctx *ctx) {
+  if (ctx->rho_gen == ctx->lists_gen) return FB_OK;
+  const int64_t P = ctx->npairs;
+  hipStream_t st = ctx->stream;
+  FB_HIP(ctx->d_wbsz.ensure(sizeof(int64_t) * std::max<int64_t>(P, 1)));
+  FB_HIP(ctx->d_wboff.ensure(sizeof(int64_t) * (P + 1)));
+  DView v = ctx->view();
+  RC(ctx->launch("shell_sizes", [&] { return fbk_shell_sizes(v, ctx->d_wbsz.as<int64_t>(), st); }));
+  RC(scan_i64(ctx, ctx->d_wbsz.as<int64_t>(), ctx->d_wboff.as<int64_t>(), P));
+  FB_HIP(hipMemcpyAsync(&ctx->wlen, ctx->d_wboff.as<int64_t>() + P, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  FB_HIP(hipStreamSynchronize(st));
+  ctx->h_wboff_struct.resize(ctx->structs.size() + 1);
+  for (size_t k = 0; k < ctx->structs.size(); k++) {
+    int64_t p0 = ctx->h_noff[ctx->h_st[k].atom_off];
+    FB_HIP(hipMemcpy(&ctx->h_wboff_struct[k], ctx->d_wboff.as<int64_t>() + p0, sizeof(int64_t), hipMemcpyDeviceToHost));
+  }
+  ctx->h_wboff_struct[ctx->structs.size()] = ctx->wlen;
+  for (int m = FB_MAT_RHO_IN; m <= FB_MAT_RHO_BOND; m++)
+    FB_HIP(ctx->d_mat[m].ensure(sizeof(double) * std::max<int64_t>(ctx->blen, 1)));
+  for (int m = FB_MAT_RHOW_IN; m <= FB_MAT_RHOW_BOND; m++)
+    FB_HIP(ctx->d_mat[m].ensure(sizeof(double) * std::max<int64_t>(ctx->wlen, 1)));
+  for (int i = 0; i < 4; i++) ctx->rdone[i] = 0;
+  ctx->rho_gen = ctx->lists_gen;
+  return FB_OK;
+}
+
+static int pre_rho(fb_ctx *ctx, const char *what) {
+  RC(pre_assemble(ctx, what));
+  for (auto &s : ctx->structs)
+    if (!s.has_qin) return ctx->fail(FB_E_ORDER, (std::string(what) + ": shell charges not set (fb_set_qin)").c_str());
+  return prep_rho(ctx);
+}
+
+static int rho_step(fb_ctx *ctx, int step) {
+  static const char *names[4] = {"assemble_rho_2c", "assemble_rho_3c", "assemble_rho_weighted_2c",
+                                 "assemble_rho_weighted_3c"};
+  // rho_3c adds into the blocks of rho_2c; rho_weighted_3c into those of rho_weighted_2c
+  if ((step == 1 || step == 3) && !ctx->rdone[step - 1])
+    return ctx->fail(FB_E_ORDER, (std::string(names[step]) + " before " + names[step - 1]).c_str());
+  const int weighted = step >= 2;
+  DView v = ctx->view();
+  if (step == 0 || step == 2)
+    RC(ctx->launch(names[step], [&] { return fbk_rho_2c(v, weighted, ctx->maxsh, ctx->stream); }));
+  else
+    RC(ctx->launch(names[step], [&] { return fbk_rho_3c(v, weighted, ctx->maxsh, ctx->stream); }));
+  ctx->rdone[step] = 1;
+  if (step == 0 || step == 2) ctx->rdone[step + 1] = 0;
+  return FB_OK;
+}
+
+static int rho_entry(fb_ctx *ctx, int step) {
+  if (!ctx) return FB_E_ARG;
+  static const char *names[4] = {"assemble_rho_2c", "assemble_rho_3c", "assemble_rho_weighted_2c",
+                                 "assemble_rho_weighted_3c"};
+  RC(pre_rho(ctx, names[step]));
+  RC(rho_step(ctx, step));
+  return ctx->check_status(names[step]);
+}
+
+int fb_assemble_rho_2c(fb_ctx *ctx) { return rho_entry(ctx, 0); }
+int fb_assemble_rho_3c(fb_ctx *ctx) { return rho_entry(ctx, 1); }
+int fb_assemble_rho_weighted_2c(fb_ctx *ctx) { return rho_entry(ctx, 2); }
+int fb_assemble_rho_weighted_3c(fb_ctx *ctx) { return rho_entry(ctx, 3); }
+
+int fb_assemble_rho_all(fb_ctx *ctx) {
+  if (!ctx) return FB_E_ARG;
+  RC(pre_rho(ctx, "assemble_rho_all"));
+  for (int step = 0; step < 4; step++) RC(rho_step(ctx, step));
+  return ctx->check_status("assemble_rho_all");
+}
+
+int fb_get_shell_block_len(fb_ctx *ctx, int is, int64_t *len) {
+  if (!ctx || !ctx->lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "lists not built") : FB_E_ARG;
+  if (is < 0 || is >= (int)ctx->structs.size() || !len) return ctx->fail(FB_E_ARG, "structure index");
+  if (ctx->rho_gen != ctx->lists_gen) {
+    FB_HIP(hipSetDevice(ctx->device));
+    RC(ctx->settle("previous assemble_all"));
+    RC(prep_rho(ctx));
+  }
+  *len = ctx->h_wboff_struct[is + 1] - ctx->h_wboff_struct[is];
   return FB_OK;
 }
 

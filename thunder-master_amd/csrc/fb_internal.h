@@ -10,15 +10,20 @@
 #define FB_NTHETA 5           // include/interactions_3c.h:4
 #define FB_MAXSUB2 10         // include/interactions_2c.h:3
 #define FB_MAXSUB3 7          // include/interactions_3c.h:3
-#define FB_NSLOT2 6           // 2c interactions used on the path
+#define FB_NSLOT2 12          // 2c interactions used by the assemblers
 #define FB_P_EQ2 14.39975     // include/constants.h:4
 #define FB_2C_PCAP 256        // pairs per 2c workgroup pass
 #define FB_2C_SKIP_TGT 16     // 2c mask bit: leave the Vna blocks of 3c targets to the 3c launch
 
 // 2c interaction slots (device-side dense index of the interaction ids of
 // include/interactions_2c.h: overlap 1, kinetic 2, vna_ontopL 3, vna_ontopR 4,
-// vna_atom 5, dipole_z 10)
-enum { SLOT_S = 0, SLOT_T = 1, SLOT_ONL = 2, SLOT_ONR = 3, SLOT_ATOM = 4, SLOT_DIP = 5 };
+// vna_atom 5, dipole_z 10; McWEDA densities rho_ontopL 15, rho_ontopR 16,
+// rho_atom 17, rhoS_ontopL 19, rhoS_ontopR 20, rhoS_atom 21)
+enum {
+  SLOT_S = 0, SLOT_T = 1, SLOT_ONL = 2, SLOT_ONR = 3, SLOT_ATOM = 4, SLOT_DIP = 5,
+  SLOT_RHO_ONL = 6, SLOT_RHO_ONR = 7, SLOT_RHO_ATOM = 8, SLOT_RHOS_ONL = 9, SLOT_RHOS_ONR = 10, SLOT_RHOS_ATOM = 11
+};
+#define FB_NMAT 9             // block arrays: enum fb_matrix
 
 // status bits raised by kernels (one word per context)
 enum {
@@ -79,6 +84,7 @@ struct DView {
   const int *idx2c;             // [ns][ns][FB_NSLOT2][FB_MAXSUB2] -> cell or -1
   const DGroup3c *grp3c;
   const int *idx3c;             // [ns][ns][ns][FB_MAXSUB3] -> group or -1 (itype = bcna)
+  const int *idx3r;             // [ns][ns][ns][2][FB_MAXSUB3] -> group or -1 (itype rho_3c 1, rhoS_3c 2)
   const double *tab2c;
   const float *tab2c32;
   const double *tab3c;
@@ -97,6 +103,7 @@ struct DView {
   const int *imass;             // [natoms]
   const int *atom_struct;       // [natoms]
   const double *dQ;             // [natoms][FB_MAXSH]
+  const double *qin;            // [natoms][FB_MAXSH] input shell charges Qin (McWEDA densities)
   // neighbour lists (CSR over global atoms, per-structure 1-based atom ids)
   int64_t npairs;
   const int *neighn;            // [natoms]
@@ -104,6 +111,7 @@ struct DView {
   const int *neigh_j, *neigh_b, *neigh_back, *neigh_self;
   const int *pair_atom;         // [npairs] global iatom (0-based)
   const int64_t *boff;          // [npairs+1] block offsets (doubles)
+  const int64_t *wboff;         // [npairs+1] shell-block offsets (nssh_mu x nssh_nu per pair)
   const int *wg2c;              // [nwg2c+1] first atom of each 2c workgroup
   int nwg2c;
   // triplets, sorted by target pair (segment order = reference accumulation order)
@@ -123,7 +131,7 @@ struct DView {
   float mlp_tau;                // per-triplet bound above which the table path is used
   unsigned long long *mlp_count;  // [2]: triplets on the network, triplets seen
   // outputs
-  double *mat[5];
+  double *mat[FB_NMAT];
   int *status;
 };
 

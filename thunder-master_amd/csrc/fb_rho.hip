@@ -1,0 +1,296 @@
+// McWEDA input densities, j.ASSEMBLERS/M_assemble_rho_McWEDA_DOGS.f90.
+//
+// rho_in / rho_bond (orbital blocks, block layout) and rho_in_weighted /
+// rho_bond_weighted (shell blocks nssh_mu x nssh_nu): sums over the shells
+// isorp of the contributing atom of Qin(isorp) times an interpolated table,
+// rotated into the crystal frame for the orbital blocks.  The reference adds
+// every (neighbour or triplet, isorp) term into its block as the term is formed;
+// the kernels keep that order for every block:
+//  * the ontop terms of a pair are summed by the pair's lane in sequence;
+//  * the atom terms of every neighbour go to the self block of the atom: each
+//    lane stages the products Qin * term of its neighbour in LDS, then one lane
+//    per block element adds them in neighbour order, isorp inner;
+//  * the 3c terms of a target block are added by one owner thread per target
+//    block in list order (segments of the triplets sorted by target pair), on
+//    top of the two-centre value already in the block.
+// Binary64 throughout; no atomics, deterministic.
+#include "fb_device.h"
+#include "fb_3c_dev.h"
+
+#define WR 64     // lanes per atom (2c)
+#define NBR 16    // elements of one orbital or shell block (at most 4 x 4)
+#define T3R 64    // triplets per 3c tile
+
+__device__ __forceinline__ int cell2c_r(const DView &v, int in1, int in2, int slot, int isub) {
+  return v.idx2c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * FB_NSLOT2 + slot) * FB_MAXSUB2 + isub];
+}
+
+__device__ __forceinline__ int group3c_r(const DView &v, int in1, int in2, int in3, int w, int isub) {
+  return v.idx3r[((((in1 - 1) * v.sp.ns + (in2 - 1)) * v.sp.ns + (in3 - 1)) * 2 + w) * FB_MAXSUB3 + isub];
+}
+
+// getMEs_Fdata_2c of one cell recovered into hm (column-major nr x nc)
+__device__ inline int me2c_block(const DView &v, int cid, int nr, int nc, double x, double *hm) {
+  if (cid < 0) return ST_NO_TABLE;
+  const DCell2c c = v.cell2c[cid];
+  if (c.nME > NBR) return ST_CAPACITY;
+  double F[NBR];
+  int st = d_interp2c(c, v.tab2c, x, F);
+  if (st) return st;
+  for (int i = 0; i < nr * nc; i++) hm[i] = 0.0;
+  for (int k = 0; k < c.nME; k++) hm[(v.nu[c.munu + k] - 1) * nr + (v.mu[c.munu + k] - 1)] = F[k];
+  return 0;
+}
+
+// one term of a block: the table block (rotated for orbital blocks, W = false)
+template <bool W>
+__device__ inline int term2c(const DView &v, int cid, int in1, int in3, int nr, int nc, double z, const double *pm,
+                             double *X) {
+  if (W) return me2c_block(v, cid, nr, nc, z, X);
+  double hm[NBR];
+  int st = me2c_block(v, cid, nr, nc, z, hm);
+  if (st) return st;
+  d_rotate<double>(v.sp, in1, in3, pm, hm, nr, X);
+  return 0;
+}
+
+// assemble_rho_2c (W = false, :90-341) and assemble_rho_weighted_2c (W = true,
+// :534-762): one workgroup of WR lanes per atom, lane k takes the atom's pair k
+// (chunks of WR).  Dynamic LDS: [WR][stride] staged atom terms, isorp major.
+template <bool W>
+__global__ __launch_bounds__(WR) void k_rho_2c(DView v, int stride) {
+  extern __shared__ double s_t[];
+  __shared__ int s_n[WR];
+  const int ia = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int s = v.atom_struct[ia];
+  const DStruct St = v.st[s];
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  const int in1 = v.imass[ia];
+  const int nsi = v.sp.nssh[in1 - 1];
+  const int nmu = W ? nsi : v.sp.norb[in1 - 1];
+  const int n = v.neighn[ia];
+  const int64_t p0 = v.noff[ia];
+  const int kself = v.neigh_self[ia] - 1;
+  const int64_t pself = p0 + kself;
+  const double r1[3] = {v.ratom[3 * ia], v.ratom[3 * ia + 1], v.ratom[3 * ia + 2]};
+  double *out_in = v.mat[W ? FB_MAT_RHOW_IN : FB_MAT_RHO_IN];
+  double *out_bond = v.mat[W ? FB_MAT_RHOW_BOND : FB_MAT_RHO_BOND];
+  const int64_t *off = W ? v.wboff : v.boff;
+  const int nself = nmu * nmu;
+  int status = 0;
+  double acc_in = 0.0, acc_bond = 0.0;   // lane e < nself owns element e of the self blocks
+
+  for (int c0 = 0; c0 < n; c0 += WR) {
+    const int k = c0 + lane;
+    int nsj = 0;
+    if (k < n) {
+      const int64_t p = p0 + k;
+      const int ja = St.atom_off + v.neigh_j[p] - 1;
+      const int mb = v.neigh_b[p];
+      const int in2 = v.imass[ja];
+      nsj = v.sp.nssh[in2 - 1];
+      const int nnu = W ? nsj : v.sp.norb[in2 - 1];
+      const int nb = nmu * nnu;
+      double r2[3];
+      for (int q = 0; q < 3; q++) r2[q] = v.ratom[3 * ja + q] + xls[3 * mb + q];
+      const double z = d_distance(r1, r2);
+      double pm[9];
+      if (!W) {
+        double sighat[3], eps[9];
+        if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+        else for (int q = 0; q < 3; q++) sighat[q] = (r2[q] - r1[q]) / z;
+        d_epsilon(r2, sighat, eps);
+        d_pmat<double>(eps, pm);
+      }
+      double X[NBR];
+      if (k != kself) {
+        // ontop terms: potential of iatom's shells, then of jatom's
+        double acc[NBR];
+        for (int i = 0; i < nb; i++) acc[i] = 0.0;
+        for (int isorp = 1; isorp <= nsi; isorp++) {
+          const double q = v.qin[ia * FB_MAXSH + isorp - 1];
+          status |= term2c<W>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ONL : SLOT_RHO_ONL, isorp), in1, in2, nmu, nnu,
+                              z, pm, X);
+          for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i] * q;
+        }
+        for (int isorp = 1; isorp <= nsj; isorp++) {
+          const double q = v.qin[ja * FB_MAXSH + isorp - 1];
+          status |= term2c<W>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ONR : SLOT_RHO_ONR, isorp), in1, in2, nmu, nnu,
+                              z, pm, X);
+          for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i] * q;
+        }
+        const int64_t bo = off[p];
+        for (int i = 0; i < nb; i++) {
+          out_in[bo + i] = acc[i];
+          out_bond[bo + i] = acc[i];
+        }
+      }
+      // atom terms <i|rho_j|i>, staged as products Qin * term
+      for (int isorp = 1; isorp <= nsj; isorp++) {
+        const double q = v.qin[ja * FB_MAXSH + isorp - 1];
+        status |= term2c<W>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ATOM : SLOT_RHO_ATOM, isorp), in1, in1, nmu, nmu, z,
+                            pm, X);
+        for (int i = 0; i < nself; i++) s_t[lane * stride + (isorp - 1) * NBR + i] = X[i] * q;
+      }
+    }
+    s_n[lane] = nsj;
+    __syncthreads();
+    if (lane < nself) {
+      const int kmax = min(WR, n - c0);
+      for (int kk = 0; kk < kmax; kk++) {
+        const bool self = (c0 + kk == kself);
+        for (int isorp = 0; isorp < s_n[kk]; isorp++) {
+          const double t = s_t[kk * stride + isorp * NBR + lane];
+          acc_in = acc_in + t;
+          if (self) acc_bond = acc_bond + t;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (lane < nself) {
+    out_in[off[pself] + lane] = acc_in;
+    out_bond[off[pself] + lane] = acc_bond;
+  }
+  if (status) atomicOr(v.status, status);
+}
+
+// triplet geometry of the density assemblers, M_assemble_rho_McWEDA_DOGS.f90:451-468
+__device__ inline void rho_geo(const DView &v, int4 tr, double &x, double &z, double &cost, double *pm) {
+  const int ial = tr.x, gia = tr.y, gja = tr.z;
+  const int ib = tr.w & 0xffff, jb = (tr.w >> 16) & 0xffff;
+  const int s = v.atom_struct[ial];
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  double r1[3], r2[3], rna[3], r21[3], sighat[3], rnabc[3], rhat[3], eps[9];
+  for (int k = 0; k < 3; k++) {
+    r1[k] = v.ratom[3 * gia + k] + xls[3 * ib + k];
+    r2[k] = v.ratom[3 * gja + k] + xls[3 * jb + k];
+    rna[k] = v.ratom[3 * ial + k];
+    r21[k] = r2[k] - r1[k];
+  }
+  z = d_distance(r1, r2);
+  if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+  else for (int k = 0; k < 3; k++) sighat[k] = r21[k] / z;
+  for (int k = 0; k < 3; k++) rnabc[k] = rna[k] - (r1[k] + r21[k] / 2.0);
+  x = sqrt(rnabc[0] * rnabc[0] + rnabc[1] * rnabc[1] + rnabc[2] * rnabc[2]);
+  if (x < 1.0e-5) { rhat[0] = 0.0; rhat[1] = 0.0; rhat[2] = 0.0; }
+  else for (int k = 0; k < 3; k++) rhat[k] = rnabc[k] / x;
+  cost = sighat[0] * rhat[0] + sighat[1] * rhat[1] + sighat[2] * rhat[2];
+  d_epsilon(rhat, sighat, eps);
+  d_pmat<double>(eps, pm);
+}
+
+// assemble_rho_3c (W = false, :362-511) and assemble_rho_weighted_3c (W = true,
+// :783-924).  A workgroup owns the target segments that start in its tile of
+// T3R sorted triplets; every thread computes the terms of one triplet per
+// chunk into LDS ([T3R][stride], isorp major), and one owner thread per segment
+// adds them in order onto the two-centre block.
+template <bool W>
+__global__ __launch_bounds__(T3R) void k_rho_3c(DView v, int stride) {
+  extern __shared__ double s_c[];
+  __shared__ int s_n[T3R];
+  const int64_t ts = (int64_t)blockIdx.x * T3R;
+  const int64_t te = min(ts + T3R, v.ntrip);
+  int64_t first, last, r0, r1;
+  if (!tile_segments(v, ts, te, first, last, r0, r1)) return;
+  const int64_t my = first + threadIdx.x;
+  const bool own = my <= last;
+  double *out = v.mat[W ? FB_MAT_RHOW_IN : FB_MAT_RHO_IN];
+  const int64_t *off = W ? v.wboff : v.boff;
+  int64_t a = 0, b = 0, dst = 0;
+  int nb = 0;
+  double acc[NBR];
+  if (own) {
+    a = v.seg_off[my];
+    b = v.seg_off[my + 1];
+    const int p = v.seg_pair[my];
+    const int ia = v.pair_atom[p];
+    const int ja = v.st[v.atom_struct[ia]].atom_off + v.neigh_j[p] - 1;
+    const int in1 = v.imass[ia], in2 = v.imass[ja];
+    nb = W ? v.sp.nssh[in1 - 1] * v.sp.nssh[in2 - 1] : v.sp.norb[in1 - 1] * v.sp.norb[in2 - 1];
+    dst = off[p];
+    for (int i = 0; i < NBR; i++) acc[i] = (i < nb) ? out[dst + i] : 0.0;
+  }
+  int status = 0;
+  for (int64_t c0 = r0; c0 < r1; c0 += T3R) {
+    const int64_t t = c0 + threadIdx.x;
+    int nsorp = 0;
+    if (t < r1) {
+      const int4 tr = v.trip[t];
+      const int in1 = v.imass[tr.y], in2 = v.imass[tr.z], indna = v.imass[tr.x];
+      const int nr = W ? v.sp.nssh[in1 - 1] : v.sp.norb[in1 - 1];
+      const int nc = W ? v.sp.nssh[in2 - 1] : v.sp.norb[in2 - 1];
+      double x, z, cost, pm[9], hm[NBR], X[NBR];
+      rho_geo(v, tr, x, z, cost, pm);
+      nsorp = v.sp.nssh[indna - 1];
+      for (int isorp = 1; isorp <= nsorp; isorp++) {
+        const double q = v.qin[tr.x * FB_MAXSH + isorp - 1];
+        status |= d_getMEs_3c<NBR>(v, group3c_r(v, in1, in2, indna, W ? 1 : 0, isorp), x, z, cost, nr, nc, hm);
+        if (W) {
+          for (int i = 0; i < nr * nc; i++) X[i] = hm[i];
+        } else {
+          d_rotate<double>(v.sp, in1, in2, pm, hm, nr, X);
+        }
+        for (int i = 0; i < nr * nc; i++) s_c[threadIdx.x * stride + (isorp - 1) * NBR + i] = X[i] * q;
+      }
+    }
+    s_n[threadIdx.x] = nsorp;
+    __syncthreads();
+    if (own) {
+      const int64_t k0 = max(a, c0), k1 = min(b, c0 + T3R);
+      for (int64_t kk = k0; kk < k1; kk++) {
+        const int row = (int)(kk - c0);
+        for (int isorp = 0; isorp < s_n[row]; isorp++)
+          for (int i = 0; i < NBR; i++)
+            if (i < nb) acc[i] = acc[i] + s_c[row * stride + isorp * NBR + i];
+      }
+    }
+    __syncthreads();
+  }
+  if (own)
+    for (int i = 0; i < NBR; i++)
+      if (i < nb) out[dst + i] = acc[i];
+  if (status) atomicOr(v.status, status);
+}
+
+// per pair: nssh_mu * nssh_nu (sizes of the shell blocks)
+__global__ void k_shell_sizes(DView v, int64_t *sz) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= v.npairs) return;
+  const int ia = v.pair_atom[p];
+  const int ja = v.st[v.atom_struct[ia]].atom_off + v.neigh_j[p] - 1;
+  sz[p] = (int64_t)v.sp.nssh[v.imass[ia] - 1] * v.sp.nssh[v.imass[ja] - 1];
+}
+
+extern "C" {
+// maxsh: largest shell count of any species (sizes the staged terms)
+int fbk_rho_2c(const DView &v, int weighted, int maxsh, hipStream_t st) {
+  if (v.natoms == 0) return 0;
+  const int stride = maxsh * NBR + 1;
+  const size_t lds = sizeof(double) * WR * stride;
+  if (weighted)
+    hipLaunchKernelGGL(k_rho_2c<true>, dim3(v.natoms), dim3(WR), lds, st, v, stride);
+  else
+    hipLaunchKernelGGL(k_rho_2c<false>, dim3(v.natoms), dim3(WR), lds, st, v, stride);
+  return (int)hipGetLastError();
+}
+
+int fbk_rho_3c(const DView &v, int weighted, int maxsh, hipStream_t st) {
+  if (v.ntrip == 0) return 0;
+  const int stride = maxsh * NBR + 1;
+  const size_t lds = sizeof(double) * T3R * stride;
+  const dim3 grid((unsigned)((v.ntrip + T3R - 1) / T3R));
+  if (weighted)
+    hipLaunchKernelGGL(k_rho_3c<true>, grid, dim3(T3R), lds, st, v, stride);
+  else
+    hipLaunchKernelGGL(k_rho_3c<false>, grid, dim3(T3R), lds, st, v, stride);
+  return (int)hipGetLastError();
+}
+
+int fbk_shell_sizes(const DView &v, int64_t *sz, hipStream_t st) {
+  if (v.npairs == 0) return 0;
+  hipLaunchKernelGGL(k_shell_sizes, dim3((unsigned)((v.npairs + 255) / 256)), dim3(256), 0, st, v, sz);
+  return (int)hipGetLastError();
+}
+}

@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(HERE, "libfbhip.so")
 HEADER = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include", "fireball_hip.h")
 MAXSH = 4
 
-FB_MAT = {"S": 0, "T": 1, "DIP": 2, "VNA": 3, "VNAO": 4}
+FB_MAT = {"S": 0, "T": 1, "DIP": 2, "VNA": 3, "VNAO": 4,
+          "RHO_IN": 5, "RHO_BOND": 6, "RHOW_IN": 7, "RHOW_BOND": 8}
 PREC = {"fp64": 0, "mixed": 1}
 METHOD3C = {"table": 0, "mlp": 1}
 STATUS = {0: "FB_OK", -1: "FB_E_ARG", -2: "FB_E_X_NEGATIVE", -3: "FB_E_X_OUT_OF_RANGE",
@@ -86,6 +87,10 @@ def lib():
             "fb_dassemble_vna_2c": [vp], "fb_dassemble_vna_3c": [vp], "fb_dassemble_all": [vp],
             "fb_get_dblocks": [vp, C.c_int, C.c_int, dp],
             "fb_get_forces": [vp, C.c_int, dp, dp, dp, dp, dp],
+            "fb_set_qin": [vp, C.c_int, dp],
+            "fb_assemble_rho_2c": [vp], "fb_assemble_rho_3c": [vp], "fb_assemble_rho_weighted_2c": [vp],
+            "fb_assemble_rho_weighted_3c": [vp], "fb_assemble_rho_all": [vp],
+            "fb_get_shell_block_len": [vp, C.c_int, i64p],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -299,6 +304,35 @@ class Assembler:
         f3 = [np.zeros((n, 3)) for _ in range(3)]
         self._check(self.L.fb_get_forces(self.h, istruct, _dp(fa), _dp(fo), *(_dp(x) for x in f3)))
         return {"vna_atom": fa[:npairs], "vna_ontop": fo[:npairs], "f3naa": f3[0], "f3nab": f3[1], "f3nac": f3[2]}
+
+    # ------------------------------------------------------ McWEDA densities
+    def set_qin(self, qin, istruct=0):
+        """Input shell charges s%atom(:)%shell(:)%Qin of one structure, [natoms][4]."""
+        qin = np.ascontiguousarray(np.asarray(qin, np.float64)[:, :MAXSH])
+        if qin.shape != (self.structs[istruct].natoms, MAXSH):
+            raise FireballError(-1, f"Qin must be [natoms][{MAXSH}]")
+        self._check(self.L.fb_set_qin(self.h, istruct, _dp(qin)))
+
+    def assemble_rho(self, which=("RHO2C", "RHO3C", "RHOW2C", "RHOW3C")):
+        """assemble_rho_2c, _3c, assemble_rho_weighted_2c, _3c (in this order in
+        j.ASSEMBLERS/M_assemble_vxc_McWEDA_DOGS.f90:148-154)."""
+        fn = {"RHO2C": self.L.fb_assemble_rho_2c, "RHO3C": self.L.fb_assemble_rho_3c,
+              "RHOW2C": self.L.fb_assemble_rho_weighted_2c, "RHOW3C": self.L.fb_assemble_rho_weighted_3c,
+              "ALL": self.L.fb_assemble_rho_all}
+        for w in which:
+            self._check(fn[w](self.h))
+
+    def rho(self, which, istruct=0):
+        """RHO_IN / RHO_BOND (orbital blocks, block layout) or RHOW_IN / RHOW_BOND
+        (shell blocks nssh_mu x nssh_nu per pair, column-major, pair order)."""
+        n = C.c_int64()
+        if which.startswith("RHOW"):
+            self._check(self.L.fb_get_shell_block_len(self.h, istruct, C.byref(n)))
+        else:
+            self._check(self.L.fb_get_block_len(self.h, istruct, C.byref(n)))
+        out = np.zeros(max(n.value, 1))
+        self._check(self.L.fb_get_blocks(self.h, istruct, FB_MAT[which], _dp(out)))
+        return out[:n.value]
 
     def gamma_eigenvalues(self, which="H", istruct=0):
         """Eigenvalues (ascending) of the Gamma-point folded S or
