@@ -166,45 +166,9 @@ extern "C" int fbk_2c(const DView &v, int mask, hipStream_t st) {
 // denominators from the host table, rotations from fb_sp.h.  Bit-identical to
 // the reference in binary64 for neutral atoms (same operations, same order).
 #include "fb_sp.h"
+#include "fb_2c_dev.h"
 
 #define WG2 256
-
-// the seven Lagrange products of M_Fdata_2c.f90:380-392 for window ileft
-__device__ __forceinline__ void lagrange7(const DView &v, const DGrid2c &gr, double x, int ileft, double *prod) {
-  const double *pd = v.pden + gr.pden_off + (int64_t)(ileft - 1) * 49;
-  double xx[7];
-#pragma unroll
-  for (int k = 0; k < 7; k++) xx[k] = (double)(ileft + k - 1) * gr.dx;
-#pragma unroll
-  for (int is = 0; is < 7; is++) {
-    double p = 1.0;
-#pragma unroll
-    for (int ip = 0; ip < 7; ip++)
-      if (ip != is) p = p * (x - xx[ip]) * pd[is * 7 + ip];
-    prod[is] = p;
-  }
-}
-
-// window of M_Fdata_2c.f90:366-375
-__device__ __forceinline__ int window7(double x, double dx, int nx) {
-  int imid = (int)(x / dx) + 1;
-  int ileft = imid - 3;
-  if (ileft < 1) ileft = 1;
-  else if (imid + 3 > nx) ileft = nx - 6;
-  return ileft;
-}
-
-// interpolate one canonical cell with precomputed products; F has nme entries
-template <int NME>
-__device__ __forceinline__ void interp_cell(const DView &v, const DCell2c &c, int ileft, const double *prod, double *F) {
-  const double *base = v.tab2c + c.off + (int64_t)(ileft - 1) * NME;
-#pragma unroll
-  for (int m = 0; m < NME; m++) F[m] = 0.0;
-#pragma unroll
-  for (int is = 0; is < 7; is++)
-#pragma unroll
-    for (int m = 0; m < NME; m++) F[m] = F[m] + base[is * NME + m] * prod[is];
-}
 
 // rotated block of one interaction; returns status bits
 template <int CLS>

@@ -1,0 +1,45 @@
+// Seven-point Lagrange interpolation helpers of the 2c assemblers
+// (e.FDATA/M_Fdata_2c.f90:366-393): window, products with the host-evaluated
+// denominators, and the interpolation of one cell with a compile-time element
+// count.
+#pragma once
+
+#include "fb_device.h"
+
+// the seven Lagrange products of M_Fdata_2c.f90:380-392 for window ileft
+__device__ __forceinline__ void lagrange7(const DView &v, const DGrid2c &gr, double x, int ileft, double *prod) {
+  const double *pd = v.pden + gr.pden_off + (int64_t)(ileft - 1) * 49;
+  double xx[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) xx[k] = (double)(ileft + k - 1) * gr.dx;
+#pragma unroll
+  for (int is = 0; is < 7; is++) {
+    double p = 1.0;
+#pragma unroll
+    for (int ip = 0; ip < 7; ip++)
+      if (ip != is) p = p * (x - xx[ip]) * pd[is * 7 + ip];
+    prod[is] = p;
+  }
+}
+
+// window of M_Fdata_2c.f90:366-375
+__device__ __forceinline__ int window7(double x, double dx, int nx) {
+  int imid = (int)(x / dx) + 1;
+  int ileft = imid - 3;
+  if (ileft < 1) ileft = 1;
+  else if (imid + 3 > nx) ileft = nx - 6;
+  return ileft;
+}
+
+// interpolate one canonical cell with precomputed products; F has nme entries
+template <int NME>
+__device__ __forceinline__ void interp_cell(const DView &v, const DCell2c &c, int ileft, const double *prod, double *F) {
+  const double *base = v.tab2c + c.off + (int64_t)(ileft - 1) * NME;
+#pragma unroll
+  for (int m = 0; m < NME; m++) F[m] = 0.0;
+#pragma unroll
+  for (int is = 0; is < 7; is++)
+#pragma unroll
+    for (int m = 0; m < NME; m++) F[m] = F[m] + base[is * NME + m] * prod[is];
+}
+

@@ -216,6 +216,7 @@ struct fb_ctx {
   DevBuf d_cell2c, d_idx2c, d_grp3c, d_idx3c, d_tab2c, d_tab2c32, d_tab3c, d_tab3c32, d_mu, d_nu, d_mv, d_meth3c;
   DevBuf d_grid2c, d_pden, d_wg2c;
   int canon_sp = 0;
+  int canon_rho = 0;      // s/p species with density tables in make_munu / make_munuS order
   int ncell2c = 0, ngrid2c = 0;
   int64_t ntab3c32 = 0;
   int nwg2c = 0;
@@ -903,6 +904,39 @@ int fb_commit_tables(fb_ctx *ctx) {
     }
   }
   ctx->canon_sp = canon;
+  // density tables: the s/p density kernels need every species s or s+p and the
+  // canonical orders of make_munu / make_munu_atom / make_munuS / make_munu_3c /
+  // make_munuS_3c
+  {
+    int cr = 1;
+    for (int i = 0; i < ns; i++)
+      if (!((shells[i] == std::vector<int>{0}) || (shells[i] == std::vector<int>{0, 1}))) cr = 0;
+    auto munuS = [](int a, int b) {
+      MeList r;
+      for (int i = 1; i <= a; i++)
+        for (int j = 1; j <= b; j++) { r.mu.push_back(i); r.nu.push_back(j); r.mv.push_back(0); }
+      return r;
+    };
+    if (cr) {
+      for (auto &c : ctx->c2) {
+        int slot = slot_of(c.itype);
+        if (slot < SLOT_RHO_ONL) continue;
+        const int si = (int)shells[c.is - 1].size(), sj = (int)shells[c.js - 1].size();
+        MeList e = slot == SLOT_RHO_ATOM ? munu2c(shells[c.is - 1], shells[c.is - 1])
+                 : slot == SLOT_RHOS_ATOM ? munuS(si, si)
+                 : (slot == SLOT_RHOS_ONL || slot == SLOT_RHOS_ONR) ? munuS(si, sj)
+                 : munu2c(shells[c.is - 1], shells[c.js - 1]);
+        if (!same_list(e, c.mu, c.nu, c.mv)) cr = 0;
+      }
+      for (auto &c : ctx->c3) {
+        if (c.itype != 1 && c.itype != 2) continue;
+        MeList e = c.itype == 1 ? munu3c_sp(shells[c.is - 1], shells[c.js - 1])
+                                : munuS((int)shells[c.is - 1].size(), (int)shells[c.js - 1].size());
+        if (!same_list(e, c.mu, c.nu, c.mv)) cr = 0;
+      }
+    }
+    ctx->canon_rho = cr;
+  }
   std::vector<float> tab2f(tab2.begin(), tab2.end());
   RC(upload(ctx, ctx->d_idx2c, idx2.data(), idx2.size() * sizeof(int), "upload idx2c"));
   RC(upload(ctx, ctx->d_grp3c, grps.data(), grps.size() * sizeof(DGroup3c), "upload groups3c"));
@@ -1956,8 +1990,8 @@ int fb_get_device_blocks(fb_ctx *ctx, int matrix, void **dptr, int64_t *len) {
 
 // ------------------------------------------------------------ McWEDA densities
 extern "C" {
-int fbk_rho_2c(const DView &v, int weighted, int maxsh, hipStream_t st);
-int fbk_rho_3c(const DView &v, int weighted, int maxsh, hipStream_t st);
+int fbk_rho_2c(const DView &v, int weighted, int maxsh, int sp, hipStream_t st);
+int fbk_rho_3c(const DView &v, int weighted, int maxsh, int sp, hipStream_t st);
 int fbk_shell_sizes(const DView &v, int64_t *sz, hipStream_t st);
 }
 
@@ -2003,10 +2037,11 @@ static int rho_step(fb_ctx *ctx, int step) {
     return ctx->fail(FB_E_ORDER, (std::string(names[step]) + " before " + names[step - 1]).c_str());
   const int weighted = step >= 2;
   DView v = ctx->view();
+  const int sp = ctx->canon_rho && ctx->force_kernel;   // FB_TUNE_FORCE_KERNEL 0 selects the generic kernels
   if (step == 0 || step == 2)
-    RC(ctx->launch(names[step], [&] { return fbk_rho_2c(v, weighted, ctx->maxsh, ctx->stream); }));
+    RC(ctx->launch(names[step], [&] { return fbk_rho_2c(v, weighted, ctx->maxsh, sp, ctx->stream); }));
   else
-    RC(ctx->launch(names[step], [&] { return fbk_rho_3c(v, weighted, ctx->maxsh, ctx->stream); }));
+    RC(ctx->launch(names[step], [&] { return fbk_rho_3c(v, weighted, ctx->maxsh, sp, ctx->stream); }));
   ctx->rdone[step] = 1;
   if (step == 0 || step == 2) ctx->rdone[step + 1] = 0;
   return FB_OK;

@@ -254,6 +254,280 @@ __global__ __launch_bounds__(T3R) void k_rho_3c(DView v, int stride) {
   if (status) atomicOr(v.status, status);
 }
 
+// ---------------------------------------------------------------------------
+// s/p species with canonical tables: the same sums with compile-time block
+// shapes, the register-resident rotations of fb_sp.h and the Lagrange
+// products of the host-evaluated denominators (the operations and their order
+// are those of the generic kernels).  Pair class cls = (mu side is s+p) * 2 +
+// (nu side is s+p); an s+p species has two shells, an s species one, so the
+// class also fixes the shell-block shape.
+#include "fb_sp.h"
+#include "fb_2c_dev.h"
+
+#define WRS 256   // lanes per 2c workgroup (whole atoms, at most 16)
+
+// shell block from table order (issh major, make_munuS) to column-major
+template <int NSI, int NSJ>
+__device__ __forceinline__ void shell_block(const double *F, double *X) {
+#pragma unroll
+  for (int i = 0; i < NSI; i++)
+#pragma unroll
+    for (int j = 0; j < NSJ; j++) X[j * NSI + i] = F[i * NSJ + j];
+}
+
+// one 2c term of class CLS: orbital block (rotated) or shell block (W)
+template <bool W, int CLS>
+__device__ __forceinline__ int term2c_sp(const DView &v, int cid, double z, const double *pm, int &grid_cached,
+                                         int &ileft, double *prod, double *X) {
+  if (cid < 0) return ST_NO_TABLE;
+  const DCell2c c = v.cell2c[cid];
+  if (z < 0.0) return ST_X_NEG;
+  if (z > c.xmax * (1.0 + 1.0e-5)) return ST_X_RANGE;
+  if (c.grid != grid_cached) {
+    const DGrid2c gr = v.grid2c[c.grid];
+    ileft = window7(z, gr.dx, gr.nx);
+    lagrange7(v, gr, z, ileft, prod);
+    grid_cached = c.grid;
+  }
+  constexpr int NSI = (CLS & 2) ? 2 : 1, NSJ = (CLS & 1) ? 2 : 1;
+  if (W) {
+    double F[NSI * NSJ];
+    interp_cell<NSI * NSJ>(v, c, ileft, prod, F);
+    shell_block<NSI, NSJ>(F, X);
+  } else {
+    double F[nme2c_of(CLS)];
+    interp_cell<nme2c_of(CLS)>(v, c, ileft, prod, F);
+    rot2c_sp<double>(CLS, pm, F, X);
+  }
+  return 0;
+}
+
+template <bool W, int CLS>
+__device__ __forceinline__ int pair_rho_sp(const DView &v, int ia, int ja, int in1, int in2, bool self, double z,
+                                           const double *pm, int64_t bo, double *st_terms) {
+  constexpr int NSI = (CLS & 2) ? 2 : 1, NSJ = (CLS & 1) ? 2 : 1;
+  constexpr int NR = W ? NSI : ((CLS & 2) ? 4 : 1), NC = W ? NSJ : ((CLS & 1) ? 4 : 1), NB = NR * NC;
+  constexpr int CLSA = (CLS & 2) ? CLS_SPSP : CLS_SS;   // atom terms live on iatom's orbitals or shells
+  constexpr int NA = W ? NSI * NSI : ((CLS & 2) ? 16 : 1);
+  int status = 0, grid_cached = -1, ileft = 1;
+  double prod[7], X[16];
+  if (!self) {
+    double acc[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) acc[i] = 0.0;
+#pragma unroll
+    for (int isorp = 1; isorp <= NSI; isorp++) {
+      const double q = v.qin[ia * FB_MAXSH + isorp - 1];
+      status |= term2c_sp<W, CLS>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ONL : SLOT_RHO_ONL, isorp), z, pm,
+                                  grid_cached, ileft, prod, X);
+#pragma unroll
+      for (int i = 0; i < NB; i++) acc[i] = acc[i] + X[i] * q;
+    }
+#pragma unroll
+    for (int isorp = 1; isorp <= NSJ; isorp++) {
+      const double q = v.qin[ja * FB_MAXSH + isorp - 1];
+      status |= term2c_sp<W, CLS>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ONR : SLOT_RHO_ONR, isorp), z, pm,
+                                  grid_cached, ileft, prod, X);
+#pragma unroll
+      for (int i = 0; i < NB; i++) acc[i] = acc[i] + X[i] * q;
+    }
+    double *oi = v.mat[W ? FB_MAT_RHOW_IN : FB_MAT_RHO_IN] + bo;
+    double *ob = v.mat[W ? FB_MAT_RHOW_BOND : FB_MAT_RHO_BOND] + bo;
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      oi[i] = acc[i];
+      ob[i] = acc[i];
+    }
+  }
+#pragma unroll
+  for (int isorp = 1; isorp <= NSJ; isorp++) {
+    const double q = v.qin[ja * FB_MAXSH + isorp - 1];
+    status |= term2c_sp<W, CLSA>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ATOM : SLOT_RHO_ATOM, isorp), z, pm,
+                                 grid_cached, ileft, prod, X);
+#pragma unroll
+    for (int i = 0; i < NA; i++) st_terms[(isorp - 1) * NBR + i] = X[i] * q;
+  }
+  return status;
+}
+
+// one lane per pair over workgroups of whole atoms (at most 16 atoms and
+// FB_2C_PCAP pairs); 16 owner lanes per atom add the staged atom terms of the
+// atom's neighbours into its self blocks.  Dynamic LDS: [WRS][stride].
+template <bool W>
+__global__ __launch_bounds__(WRS) void k_rho_2c_sp(DView v, int stride) {
+  extern __shared__ double s_t[];
+  __shared__ int s_n[WRS];
+  const int tid = threadIdx.x;
+  const int a0 = v.wg2c[blockIdx.x], a1 = v.wg2c[blockIdx.x + 1];
+  const int64_t pb = v.noff[a0], pe = v.noff[a1];
+  const int ra = a0 + (tid >> 4), re = tid & 15;
+  const bool rown = ra < a1;
+  int64_t rp0 = 0, rp1 = 0, rself = -1;
+  int rn = 0;
+  if (rown) {
+    rp0 = v.noff[ra];
+    rp1 = v.noff[ra + 1];
+    rself = rp0 + v.neigh_self[ra] - 1;
+    rn = W ? v.sp.nssh[v.imass[ra] - 1] : v.sp.norb[v.imass[ra] - 1];
+  }
+  double acc_in = 0.0, acc_bond = 0.0;
+  int status = 0;
+  for (int64_t c0 = pb; c0 < pe; c0 += WRS) {
+    const int64_t p = c0 + tid;
+    int nsj = 0;
+    if (p < pe) {
+      const int ia = v.pair_atom[p];
+      const int s = v.atom_struct[ia];
+      const int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
+      const int mb = v.neigh_b[p];
+      const int in1 = v.imass[ia], in2 = v.imass[ja];
+      const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+      double r1[3], r2[3], pm[9];
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        r1[q] = v.ratom[3 * ia + q];
+        r2[q] = v.ratom[3 * ja + q] + xls[3 * mb + q];
+      }
+      const double z = d_distance(r1, r2);
+      if (!W) {
+        double sighat[3], eps[9];
+        if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+        else {
+#pragma unroll
+          for (int q = 0; q < 3; q++) sighat[q] = (r2[q] - r1[q]) / z;
+        }
+        d_epsilon(r2, sighat, eps);
+        d_pmat<double>(eps, pm);
+      }
+      const bool self = (ja == ia && mb == 0);
+      nsj = v.sp.nssh[in2 - 1];
+      const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
+      const int64_t bo = W ? v.wboff[p] : v.boff[p];
+      double *mine = s_t + tid * stride;
+      switch (cls) {
+        case 3: status |= pair_rho_sp<W, 3>(v, ia, ja, in1, in2, self, z, pm, bo, mine); break;
+        case 2: status |= pair_rho_sp<W, 2>(v, ia, ja, in1, in2, self, z, pm, bo, mine); break;
+        case 1: status |= pair_rho_sp<W, 1>(v, ia, ja, in1, in2, self, z, pm, bo, mine); break;
+        default: status |= pair_rho_sp<W, 0>(v, ia, ja, in1, in2, self, z, pm, bo, mine); break;
+      }
+    }
+    s_n[tid] = nsj;
+    __syncthreads();
+    if (rown && re < rn * rn) {
+      const int64_t k0 = max(rp0, c0), k1 = min(rp1, c0 + WRS);
+      for (int64_t k = k0; k < k1; k++) {
+        const int row = (int)(k - c0);
+        for (int isorp = 0; isorp < s_n[row]; isorp++) {
+          const double t = s_t[row * stride + isorp * NBR + re];
+          acc_in = acc_in + t;
+          if (k == rself) acc_bond = acc_bond + t;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (rown && re < rn * rn) {
+    const int64_t o = W ? v.wboff[rself] : v.boff[rself];
+    v.mat[W ? FB_MAT_RHOW_IN : FB_MAT_RHO_IN][o + re] = acc_in;
+    v.mat[W ? FB_MAT_RHOW_BOND : FB_MAT_RHO_BOND][o + re] = acc_bond;
+  }
+  if (status) atomicOr(v.status, status);
+}
+
+// the terms of one triplet of class CLS into its LDS row
+template <bool W, int CLS>
+__device__ __forceinline__ int triplet_rho_sp(const DView &v, int4 tr, int in1, int in2, int indna, double x, double z,
+                                              double cost, const double *pm, double *row) {
+  constexpr int NSI = (CLS & 2) ? 2 : 1, NSJ = (CLS & 1) ? 2 : 1;
+  constexpr int NB = W ? NSI * NSJ : ((CLS & 2) ? 4 : 1) * ((CLS & 1) ? 4 : 1);
+  int status = 0;
+  double X[16];
+  const int nsorp = v.sp.nssh[indna - 1];
+  for (int isorp = 1; isorp <= nsorp; isorp++) {
+    const double q = v.qin[tr.x * FB_MAXSH + isorp - 1];
+    const int gid = group3c_r(v, in1, in2, indna, W ? 1 : 0, isorp);
+    if (W) {
+      double F[NSI * NSJ];
+      status |= getMEs_3c_sp<NSI * NSJ, NSI * NSJ>(v, gid, x, z, cost, F);
+      shell_block<NSI, NSJ>(F, X);
+    } else {
+      double F[nme3c_of(CLS)];
+      status |= getMEs_3c_sp<nme3c_of(CLS)>(v, gid, x, z, cost, F);
+      rot3c_sp<double>(CLS, pm, F, X);
+    }
+#pragma unroll
+    for (int i = 0; i < NB; i++) row[(isorp - 1) * NBR + i] = X[i] * q;
+  }
+  return status;
+}
+
+template <bool W>
+__global__ __launch_bounds__(T3R) void k_rho_3c_sp(DView v, int stride) {
+  extern __shared__ double s_c[];
+  __shared__ int s_n[T3R];
+  const int64_t ts = (int64_t)blockIdx.x * T3R;
+  const int64_t te = min(ts + T3R, v.ntrip);
+  int64_t first, last, r0, r1;
+  if (!tile_segments(v, ts, te, first, last, r0, r1)) return;
+  const int64_t my = first + threadIdx.x;
+  const bool own = my <= last;
+  double *out = v.mat[W ? FB_MAT_RHOW_IN : FB_MAT_RHO_IN];
+  int64_t a = 0, b = 0, dst = 0;
+  int nb = 0;
+  double acc[NBR];
+  if (own) {
+    a = v.seg_off[my];
+    b = v.seg_off[my + 1];
+    const int p = v.seg_pair[my];
+    const int ia = v.pair_atom[p];
+    const int ja = v.st[v.atom_struct[ia]].atom_off + v.neigh_j[p] - 1;
+    const int in1 = v.imass[ia], in2 = v.imass[ja];
+    nb = W ? v.sp.nssh[in1 - 1] * v.sp.nssh[in2 - 1] : v.sp.norb[in1 - 1] * v.sp.norb[in2 - 1];
+    dst = W ? v.wboff[p] : v.boff[p];
+#pragma unroll
+    for (int i = 0; i < NBR; i++) acc[i] = (i < nb) ? out[dst + i] : 0.0;
+  }
+  int status = 0;
+  for (int64_t c0 = r0; c0 < r1; c0 += T3R) {
+    const int64_t t = c0 + threadIdx.x;
+    int nsorp = 0;
+    if (t < r1) {
+      const int4 tr = v.trip[t];
+      const int in1 = v.imass[tr.y], in2 = v.imass[tr.z], indna = v.imass[tr.x];
+      double x, z, cost, pm[9];
+      rho_geo(v, tr, x, z, cost, pm);
+      nsorp = v.sp.nssh[indna - 1];
+      const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
+      double *row = s_c + threadIdx.x * stride;
+      switch (cls) {
+        case 3: status |= triplet_rho_sp<W, 3>(v, tr, in1, in2, indna, x, z, cost, pm, row); break;
+        case 2: status |= triplet_rho_sp<W, 2>(v, tr, in1, in2, indna, x, z, cost, pm, row); break;
+        case 1: status |= triplet_rho_sp<W, 1>(v, tr, in1, in2, indna, x, z, cost, pm, row); break;
+        default: status |= triplet_rho_sp<W, 0>(v, tr, in1, in2, indna, x, z, cost, pm, row); break;
+      }
+    }
+    s_n[threadIdx.x] = nsorp;
+    __syncthreads();
+    if (own) {
+      const int64_t k0 = max(a, c0), k1 = min(b, c0 + T3R);
+      for (int64_t kk = k0; kk < k1; kk++) {
+        const int row = (int)(kk - c0);
+        for (int isorp = 0; isorp < s_n[row]; isorp++)
+#pragma unroll
+          for (int i = 0; i < NBR; i++)
+            if (i < nb) acc[i] = acc[i] + s_c[row * stride + isorp * NBR + i];
+      }
+    }
+    __syncthreads();
+  }
+  if (own) {
+#pragma unroll
+    for (int i = 0; i < NBR; i++)
+      if (i < nb) out[dst + i] = acc[i];
+  }
+  if (status) atomicOr(v.status, status);
+}
+
 // per pair: nssh_mu * nssh_nu (sizes of the shell blocks)
 __global__ void k_shell_sizes(DView v, int64_t *sz) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,10 +538,20 @@ __global__ void k_shell_sizes(DView v, int64_t *sz) {
 }
 
 extern "C" {
-// maxsh: largest shell count of any species (sizes the staged terms)
-int fbk_rho_2c(const DView &v, int weighted, int maxsh, hipStream_t st) {
+// maxsh: largest shell count of any species (sizes the staged terms); sp: the
+// s/p kernels (every species s or s+p, canonical density tables)
+int fbk_rho_2c(const DView &v, int weighted, int maxsh, int sp, hipStream_t st) {
   if (v.natoms == 0) return 0;
   const int stride = maxsh * NBR + 1;
+  if (sp) {
+    if (v.nwg2c == 0) return 0;
+    const size_t lds = sizeof(double) * WRS * stride;
+    if (weighted)
+      hipLaunchKernelGGL(k_rho_2c_sp<true>, dim3(v.nwg2c), dim3(WRS), lds, st, v, stride);
+    else
+      hipLaunchKernelGGL(k_rho_2c_sp<false>, dim3(v.nwg2c), dim3(WRS), lds, st, v, stride);
+    return (int)hipGetLastError();
+  }
   const size_t lds = sizeof(double) * WR * stride;
   if (weighted)
     hipLaunchKernelGGL(k_rho_2c<true>, dim3(v.natoms), dim3(WR), lds, st, v, stride);
@@ -276,11 +560,18 @@ int fbk_rho_2c(const DView &v, int weighted, int maxsh, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
-int fbk_rho_3c(const DView &v, int weighted, int maxsh, hipStream_t st) {
+int fbk_rho_3c(const DView &v, int weighted, int maxsh, int sp, hipStream_t st) {
   if (v.ntrip == 0) return 0;
   const int stride = maxsh * NBR + 1;
   const size_t lds = sizeof(double) * T3R * stride;
   const dim3 grid((unsigned)((v.ntrip + T3R - 1) / T3R));
+  if (sp) {
+    if (weighted)
+      hipLaunchKernelGGL(k_rho_3c_sp<true>, grid, dim3(T3R), lds, st, v, stride);
+    else
+      hipLaunchKernelGGL(k_rho_3c_sp<false>, grid, dim3(T3R), lds, st, v, stride);
+    return (int)hipGetLastError();
+  }
   if (weighted)
     hipLaunchKernelGGL(k_rho_3c<true>, grid, dim3(T3R), lds, st, v, stride);
   else
