@@ -248,6 +248,61 @@ def forces_leg(asm, structs, steps, warmup):
     return out
 
 
+def rho_leg(species, structs, steps, warmup):
+    """One density build = fb_assemble_rho_all (assemble_rho_2c, _3c,
+    assemble_rho_weighted_2c, _3c of the McWEDA module, binary64) on the same
+    structures with the synthetic density tables and shell charges Qin =
+    Qneutral + dQ; its own context, so the H+S measurement above is untouched."""
+    ts = synth.add_rho_tables(synth.make_tables(species), seed=4048)
+    a = Assembler(ts, precision="fp64")
+    for st in structs:
+        a.add_structure(st)
+    a.build_lists()
+    for k, st in enumerate(structs):
+        a.set_qin(synth.shell_qin(st, species, seed=5), k)
+    for _ in range(warmup):
+        a.assemble_rho(("ALL",))
+    a.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a.assemble_rho(("ALL",))
+    a.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    a.enable_timing(True)
+    a.timings(4096)
+    for _ in range(max(1, min(steps, 5))):
+        a.assemble_rho(("ALL",))
+    a.synchronize()
+    kern = {}
+    for name, t in a.timings(4096):
+        kern.setdefault(name, []).append(t)
+    a.enable_timing(False)
+    kern = {k: float(np.mean(v)) for k, v in kern.items() if k != "clear_status"}
+    ntrip = sum(a.list_sizes(k)[1] for k in range(len(structs)))
+    a.close()
+    return {"ms_per_step": ms, "triplets_per_s": ntrip / (ms * 1e-3), "kernels_ms": kern, "dtype": "f64"}
+
+
+def rho_cpu_baseline():
+    """The reference's McWEDA density assemblers (serial flang build) on the
+    bounded Si512 sample of the other baselines."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import refrun
+    if not refrun.available(refrun.HARNESS_RREF):
+        return None
+    st = synth.diamond_si((4, 4, 4), sigma=0.08, seed=0)
+    ts = synth.add_rho_tables(synth.make_tables([synth.SI]), seed=4048)
+    lists = refrun.run(ts, st)
+    nrep = 3
+    r = refrun.run_rho(ts, st, synth.shell_qin(st, [synth.SI], seed=5), lists, nrep=nrep)
+    t = sum(r["times"].values()) / nrep
+    return {"value": r["ntriplets"] / t, "unit": "triplets/s", "cores": 1, "kind": "reference",
+            "sample": f"Si512 4x4x4 diamond (sigma 0.08 A): {r['ntriplets']} triplets, density build "
+                      f"{t * 1e3:.1f} ms (assemble_rho_2c/3c, assemble_rho_weighted_2c/3c, serial flang -O2, "
+                      f"mean of {nrep})",
+            "rho_build_ms": t * 1e3}
+
+
 def force_cpu_baseline():
     """The reference's force assemblers (serial flang build) on the same
     bounded sample as the H+S baseline: Dassemble_S/T/dipole_z/vna_2c/vna_3c."""
@@ -337,6 +392,7 @@ def main():
     ap.add_argument("--no-eig", action="store_true", help="skip the Gamma-point eigenvalue shift")
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned 3c surrogate leg")
     ap.add_argument("--no-forces", action="store_true", help="skip the force-assembler leg")
+    ap.add_argument("--no-rho", action="store_true", help="skip the McWEDA density leg")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one elementwise copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -411,6 +467,17 @@ def main():
         except Exception as e:  # reported, never required
             forces = {"error": str(e)}
 
+    rho = None
+    if not args.no_rho:
+        try:
+            rho = rho_leg(species, structs, args.steps, args.warmup)
+            if not args.no_cpu_baseline:
+                rho["cpu_baseline"] = rho_cpu_baseline()
+            print(f"# densities: {rho['ms_per_step']:.3f} ms per McWEDA density build; kernels {rho['kernels_ms']}",
+                  file=sys.stderr, flush=True)
+        except Exception as e:  # reported, never required
+            rho = {"error": str(e)}
+
     # dominant kernel: the 3c launch
     k3 = [k for k in kern if k.startswith("assemble_3c")]
     dom = max(kern, key=kern.get) if not k3 else k3[0]
@@ -436,7 +503,7 @@ def main():
                    "nstructures": len(structs), "npairs": npairs, "ntriplets": ntrip,
                    "precision": args.precision, "h_s_build_ms": ms, "lists_ms": lists_ms,
                    "chunk3c": chunk, "ms_per_step_with_kernel_events": ms_events,
-                   "kernels_ms": kern, **extra, "mlp_3c": mlp, "forces": forces,
+                   "kernels_ms": kern, **extra, "mlp_3c": mlp, "forces": forces, "rho_mcweda": rho,
                    **({"structures_per_s": len(structs) / (ms * 1e-3)} if len(structs) > 1 else {})},
         "roofline": roof,
         "cpu_baseline": cpu,

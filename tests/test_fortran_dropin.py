@@ -1,6 +1,7 @@
-"""The Fortran drop-in (thunder-master_amd/fortran: modules M_assemble_2c and
-M_assemble_3c over the C ABI) linked into the same harness program as the
-reference's own modules.
+"""The Fortran drop-in (thunder-master_amd/fortran: modules M_assemble_2c,
+M_assemble_3c, M_Dassemble_2c, M_Dassemble_3c and M_assemble_rho_McWEDA over
+the C ABI) linked into the same harness programs as the reference's own
+modules.
 
 The harness (oracle/ref/fb_harness.f90) runs the reference's read_Fdata_2c /
 read_Fdata_3c, make_cells and driver_neighbors, then calls assemble_S, T,
@@ -163,3 +164,56 @@ def test_force_dropin_matches_reference_live_repeated():
         np.testing.assert_allclose(hip[k], ref[k], rtol=0, atol=1e-12 * _scale(ref[k]), err_msg=k)
     for k in F3KEYS:
         np.testing.assert_allclose(hip[k], ref[k], rtol=0, atol=1e-11 * _scale(ref[k]), err_msg=k)
+
+
+# ---------------------------------------------------------- McWEDA densities
+RHIP_HARNESS = os.path.join(os.path.dirname(refrun.HARNESS_REF), "fb_rharness_hip")
+need_rhip = pytest.mark.skipif(not refrun.available(RHIP_HARNESS), reason="make -C oracle hip not built")
+RKEYS = ("RHO_IN", "RHO_BOND", "RHOW_IN", "RHOW_BOND")
+
+
+def _rho_tables(ts, seed):
+    import copy
+    from fireball_hip import synth
+    tsr = copy.copy(ts)
+    tsr.cells2c = dict(ts.cells2c)
+    tsr.cells3c = dict(ts.cells3c)
+    return synth.add_rho_tables(tsr, seed=seed)
+
+
+@need_rhip
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", golden_cases.NAMES)
+def test_rho_dropin_matches_reference_golden(golden, name):
+    """The density harness (oracle/ref/fb_rharness.f90) linked with the drop-in
+    M_assemble_rho_McWEDA: the four density arrays equal the reference's own
+    (golden fixtures) value for value."""
+    print(f"density case {name}", flush=True)
+    g, gr = golden(name), golden(name + "_rho")
+    ts, st = golden_cases.inputs(g)
+    ts = _rho_tables(ts, int(gr["rho_table_seed"]))
+    lists = {k: g[k] for k in ("neighn", "neigh_j")}
+    r = refrun.run_rho(ts, st, gr["qin"], lists, binary=RHIP_HARNESS)
+    for k in RKEYS:
+        assert np.array_equal(r[k][:gr[k].size], gr[k]), k
+        assert float(np.sum(r[k])) == float(gr[k + "_sum"]), k
+
+
+@need_rhip
+@pytest.mark.gpu
+def test_rho_dropin_matches_reference_live_repeated():
+    """Si216 with shell charges, the four density assemblers run twice in one
+    process (the second pass exercises destroy_assemble_rho and the cached
+    lists), against the reference's module on the same input."""
+    if not refrun.available(refrun.HARNESS_RREF):
+        pytest.skip("reference density harness not built")
+    from fireball_hip import synth
+    ts = _rho_tables(synth.make_tables([synth.SI]), 77)
+    st = synth.diamond_si((3, 3, 3), sigma=0.08, seed=5)
+    synth.random_dQ([st], [synth.SI], seed=5)
+    qin = synth.shell_qin(st, [synth.SI])
+    lists = refrun.run(ts, st)
+    ref = refrun.run_rho(ts, st, qin, lists)
+    hip = refrun.run_rho(ts, st, qin, lists, binary=RHIP_HARNESS, nrep=2)
+    for k in RKEYS:
+        assert np.array_equal(hip[k], ref[k]), k

@@ -1023,8 +1023,7 @@ int fb_set_qin(fb_ctx *ctx, int is, const double *qin) {
   auto &s = ctx->structs[is];
   s.qin.assign(qin, qin + (size_t)s.natoms * FB_MAXSH);
   s.has_qin = true;
-  ctx->structs_dirty = true;
-  for (int i = 0; i < 4; i++) ctx->rdone[i] = 0;   // the densities follow the charges
+  ctx->structs_dirty = true;   // the next density assembler reads them, as the reference reads Qin per call
   return FB_OK;
 }
 

@@ -34,11 +34,15 @@
         integer(c_int64_t), save :: npairs_all = 0, ntrip_all = 0
         real(c_double), allocatable, save :: fb_buf(:)
 
-        ! 2c interactions of the assembled path (include/interactions_2c.h:5-14)
-        integer, parameter :: n_itype2c = 6
-        integer, parameter :: itype2c(n_itype2c) = [1, 2, 3, 4, 5, 10]
-        ! 3c: P_bcna (include/interactions_3c.h:8) over five Legendre angles
-        integer, parameter :: itype3c_bcna = 3, ntheta3c = 5
+        ! 2c interactions of the assemblers (include/interactions_2c.h:5-25):
+        ! H and S, then the McWEDA densities rho_* and rhoS_*
+        integer, parameter :: n_itype2c = 12
+        integer, parameter :: itype2c(n_itype2c) = [1, 2, 3, 4, 5, 10, 15, 16, 17, 19, 20, 21]
+        ! 3c: rho_3c, rhoS_3c and bcna (include/interactions_3c.h:6-8) over five
+        ! Legendre angles
+        integer, parameter :: n_itype3c = 3
+        integer, parameter :: itype3c(n_itype3c) = [1, 2, 3]
+        integer, parameter :: ntheta3c = 5
 
         contains
 
@@ -90,14 +94,17 @@
               end do
             end do
             do ks = 1, nsp
-              do isub = 0, species(ks)%nssh
-                do itheta = 1, ntheta3c
-                  idx = Fdata_bundle_3c(is, js, ks)%index_3c(itype3c_bcna, isub, itheta)
-                  if (idx .le. 0) cycle
-                  c3 => Fdata_bundle_3c(is, js, ks)%Fdata_cell_3c(idx)
-                  call fb_check (fb_set_table_3c(fbh, is, js, ks, itype3c_bcna, isub,      &
-     &              itheta, c3%nME, c3%nx, c3%xmax, c3%ny, c3%ymax, c3%mu_3c, c3%nu_3c,    &
-     &              c3%mvalue_3c, c3%Fdata_3c), fbh, 'fb_set_table_3c')
+              do k = 1, n_itype3c
+                itype = itype3c(k)
+                do isub = 0, species(ks)%nssh
+                  do itheta = 1, ntheta3c
+                    idx = Fdata_bundle_3c(is, js, ks)%index_3c(itype, isub, itheta)
+                    if (idx .le. 0) cycle
+                    c3 => Fdata_bundle_3c(is, js, ks)%Fdata_cell_3c(idx)
+                    call fb_check (fb_set_table_3c(fbh, is, js, ks, itype, isub,           &
+     &                itheta, c3%nME, c3%nx, c3%xmax, c3%ny, c3%ymax, c3%mu_3c, c3%nu_3c,  &
+     &                c3%mvalue_3c, c3%Fdata_3c), fbh, 'fb_set_table_3c')
+                  end do
                 end do
               end do
             end do
@@ -170,6 +177,53 @@
         call fb_check (fb_set_charges(fbh, 0, dq), fbh, 'fb_set_charges')
         deallocate (dq)
         end subroutine fb_session_charges
+
+! Hand the input shell charges s%atom(:)%shell(:)%Qin (McWEDA densities) to the library.
+        subroutine fb_session_qin (s)
+        type(T_structure), target :: s
+        real(c_double), allocatable :: q(:, :)
+        integer iatom, issh
+        allocate (q(FB_MAXSH, s%natoms))
+        q = 0.0d0
+        do iatom = 1, s%natoms
+          do issh = 1, min(species(s%atom(iatom)%imass)%nssh, FB_MAXSH)
+            q(issh, iatom) = s%atom(iatom)%shell(issh)%Qin
+          end do
+        end do
+        call fb_check (fb_set_qin(fbh, 0, q), fbh, 'fb_set_qin')
+        deallocate (q)
+        end subroutine fb_session_qin
+
+! Fill the shell blocks (nssh_mu x nssh_nu) of one T_assemble_neighbors array
+! from library matrix imat (allocating them when asked).
+        subroutine fb_session_wstore (s, m, imat, alloc)
+        type(T_structure), target :: s
+        type(T_assemble_neighbors), pointer :: m(:)
+        integer(c_int), intent(in) :: imat
+        logical, intent(in) :: alloc
+        integer iatom, ineigh, jatom, nmu, nnu, nb
+        integer(c_int64_t) :: wlen, off
+        real(c_double), allocatable :: w(:)
+        type(T_assemble_block), pointer :: pb
+        call fb_check (fb_get_shell_block_len(fbh, 0, wlen), fbh, 'fb_get_shell_block_len')
+        allocate (w(max(wlen, 1_8)))
+        call fb_check (fb_get_blocks(fbh, 0, imat, w), fbh, 'fb_get_blocks')
+        off = 0
+        do iatom = 1, s%natoms
+          nmu = species(s%atom(iatom)%imass)%nssh
+          if (alloc) allocate (m(iatom)%neighbors(s%neighbors(iatom)%neighn))
+          do ineigh = 1, s%neighbors(iatom)%neighn
+            jatom = s%neighbors(iatom)%neigh_j(ineigh)
+            nnu = species(s%atom(jatom)%imass)%nssh
+            nb = nmu*nnu
+            pb => m(iatom)%neighbors(ineigh)
+            if (alloc) allocate (pb%block(nmu, nnu))
+            pb%block = reshape(w(off+1:off+nb), [nmu, nnu])
+            off = off + nb
+          end do
+        end do
+        deallocate (w)
+        end subroutine fb_session_wstore
 
         subroutine shell_charges (s, dq)
         type(T_structure), target :: s

@@ -9,6 +9,8 @@
         integer(c_int), parameter :: FB_OK = 0
         integer(c_int), parameter :: FB_MAT_S = 0, FB_MAT_T = 1, FB_MAT_DIP = 2
         integer(c_int), parameter :: FB_MAT_VNA = 3, FB_MAT_VNAO = 4
+        integer(c_int), parameter :: FB_MAT_RHO_IN = 5, FB_MAT_RHO_BOND = 6
+        integer(c_int), parameter :: FB_MAT_RHOW_IN = 7, FB_MAT_RHOW_BOND = 8
         integer(c_int), parameter :: FB_PREC_FP64 = 0, FB_PREC_MIXED = 1
         integer, parameter :: FB_MAXSH = 4
 
@@ -196,6 +198,34 @@
             type(c_ptr), value :: ctx
             integer(c_int), value :: is
             type(c_ptr), value :: vna_atom, vna_ontop, f3naa, f3nab, f3nac
+          end function
+          integer(c_int) function fb_set_qin (ctx, is, qin) bind(c, name='fb_set_qin')
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: is
+            real(c_double) :: qin(*)
+          end function
+          integer(c_int) function fb_assemble_rho_2c (ctx) bind(c, name='fb_assemble_rho_2c')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+          end function
+          integer(c_int) function fb_assemble_rho_3c (ctx) bind(c, name='fb_assemble_rho_3c')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+          end function
+          integer(c_int) function fb_assemble_rho_weighted_2c (ctx) bind(c, name='fb_assemble_rho_weighted_2c')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+          end function
+          integer(c_int) function fb_assemble_rho_weighted_3c (ctx) bind(c, name='fb_assemble_rho_weighted_3c')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+          end function
+          integer(c_int) function fb_get_shell_block_len (ctx, is, wlen) bind(c, name='fb_get_shell_block_len')
+            import :: c_ptr, c_int, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: is
+            integer(c_int64_t) :: wlen
           end function
         end interface
 
