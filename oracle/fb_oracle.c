@@ -48,6 +48,9 @@ struct fbo_tables {
   int norb[16];
   int have_d;
   cell2c **c2; /* [ns][ns][MAXTYPE2][MAXSUB2] */
+  /* pseudopotential shells (species(:)%shell_PP, %rcutoff_PP; shell_PP(:)%cl from clPP.ZZ.dat) */
+  int nssh_pp[16], lssh_pp[16][FBO_MAXSH], norb_pp[16];
+  double rc_pp[16], cl_pp[16][FBO_MAXSH];
   cell3c **c3; /* [ns][ns][ns][MAXTYPE3][MAXSUB3][FBO_MAXTHETA+1] */
 };
 
@@ -86,6 +89,14 @@ struct fbo_system {
   int64_t wlen;
   double *wrho[2];  /* [wlen] rho_in_weighted, rho_bond_weighted */
   int rdone[4];     /* rho_2c, rho_3c, rho_weighted_2c, rho_weighted_3c */
+  /* pseudopotential lists (driver_neighbors_PP) and blocks */
+  int *pp_n, *pp_off, *pp_j, *pp_b, *pp_map, *pp_self;
+  int *px_n, *px_off, *px_j, *px_b, *px_map, *px_point, *px_self;
+  int *pq_n, *pq_off, *pq_j, *pq_b, *pq_self;          /* neighbors_PPp */
+  int *pc_n, *pc_off, *pc_ij, *pc_ib, *pc_jj, *pc_jb, *pc_m;   /* PP common neighbours */
+  int64_t *sv_off, sv_len; double *svnl;              /* per PP pair: norb_mu x norb_PP_nu */
+  int64_t *vl_off, vl_len; double *vnl;               /* per PPp pair: norb_mu x norb_nu */
+  int ppdone[3];    /* svnl, vnl_2c, vnl_3c */
 };
 
 static inline cell2c *C2(const fbo_tables *t, int is, int js, int it, int isub) {
@@ -476,6 +487,11 @@ void fbo_system_free(fbo_system *s) {
   free(s->rho); free(s->fatom); free(s->fontop);
   for (int i = 0; i < 3; i++) { free(s->dmat[i]); free(s->f3[i]); }
   free(s->qin); free(s->wboff);
+  int **pa[] = {&s->pp_n, &s->pp_off, &s->pp_j, &s->pp_b, &s->pp_map, &s->pp_self, &s->px_n, &s->px_off, &s->px_j,
+                &s->px_b, &s->px_map, &s->px_point, &s->px_self, &s->pq_n, &s->pq_off, &s->pq_j, &s->pq_b, &s->pq_self,
+                &s->pc_n, &s->pc_off, &s->pc_ij, &s->pc_ib, &s->pc_jj, &s->pc_jb, &s->pc_m};
+  for (size_t i = 0; i < sizeof(pa) / sizeof(pa[0]); i++) free(*pa[i]);
+  free(s->sv_off); free(s->svnl); free(s->vl_off); free(s->vnl);
   for (int i = 0; i < 2; i++) { free(s->rho2[i]); free(s->wrho[i]); }
   free(s);
 }
@@ -1823,5 +1839,435 @@ int fbo_get_rho(const fbo_system *s, int which, double *out) {
   if (which < 0 || which > 3) return FBO_E_ARG;
   if (which < 2) memcpy(out, s->rho2[which], sizeof(double) * s->blen);
   else memcpy(out, s->wrho[which - 2], sizeof(double) * s->wlen);
+  return FBO_OK;
+}
+
+/* ========================================================== pseudopotential
+ * c.SYSTEM/M_neighbors_PP.f90 (driver_neighbors_PP), b.FUNCTIONS/cl.f90,
+ * c.SYSTEM/rotate_PP.f90, j.ASSEMBLERS/M_assemble_PP_2c.f90 (assemble_svnl,
+ * assemble_vnl_2c) and j.ASSEMBLERS/M_assemble_PP_3c.f90 (assemble_vnl_3c,
+ * mpairnay).  sVNL(mu, ncc) = <phi_mu(iatom) | Psi_ncc(jatom)>, the
+ * Kleinman-Bylander projectors of jatom's PP shells; vnl(iatom, k) =
+ * sum cl(ncc) sVNL(i, .) sVNL(j, .)^T over the projectors of the shared PP atom. */
+#define P_VNL 6   /* include/interactions_2c.h:10 */
+
+int fbo_tables_set_pp(fbo_tables *t, const int *nssh_pp, const int *lssh_pp, const double *rcutoff_pp,
+                      const double *cl) {
+  for (int i = 0; i < t->ns; i++) {
+    if (nssh_pp[i] < 0 || nssh_pp[i] > FBO_MAXSH) return FBO_E_ARG;
+    t->nssh_pp[i] = nssh_pp[i];
+    t->norb_pp[i] = 0;
+    for (int k = 0; k < nssh_pp[i]; k++) {
+      t->lssh_pp[i][k] = lssh_pp[i * FBO_MAXSH + k];
+      t->cl_pp[i][k] = cl[i * FBO_MAXSH + k];
+      t->norb_pp[i] += 2 * t->lssh_pp[i][k] + 1;
+    }
+    t->rc_pp[i] = rcutoff_pp[i];
+  }
+  return FBO_OK;
+}
+
+/* cl values per projector orbital (b.FUNCTIONS/cl.f90:50-100) */
+static void cl_values(const fbo_tables *t, int in, double *cl) {
+  int iorb = 0;
+  for (int issh = 0; issh < t->nssh_pp[in - 1]; issh++)
+    for (int m = 0; m < 2 * t->lssh_pp[in - 1][issh] + 1; m++) cl[iorb++] = t->cl_pp[in - 1][issh];
+}
+
+/* c.SYSTEM/rotate_PP.f90: orbitals of in1 on the left, PP shells of in2 on the right */
+void fbo_rotate_PP(const fbo_tables *t, int in1, int in2, const double *eps, int norb_mu, int norb_nu,
+                   const double *M, double *X) {
+  double dmat[25] = {0}, pmat[9], left[25], right[25];
+  fbo_twister(eps, t->have_d, dmat, pmat);
+  for (int i = 0; i < norb_mu * norb_nu; i++) X[i] = 0.0;
+  int n1 = 0;
+  for (int issh = 0; issh < t->nssh[in1 - 1]; issh++) {
+    int l1 = t->lssh[in1 - 1][issh];
+    chooser(l1, dmat, pmat, left);
+    int n2 = 0;
+    for (int jssh = 0; jssh < t->nssh_pp[in2 - 1]; jssh++) {
+      int l2 = t->lssh_pp[in2 - 1][jssh];
+      chooser(l2, dmat, pmat, right);
+      for (int m2 = 1; m2 <= 2 * l2 + 1; m2++)
+        for (int m1 = 1; m1 <= 2 * l1 + 1; m1++)
+          for (int k2 = 1; k2 <= 2 * l2 + 1; k2++)
+            for (int k1 = 1; k1 <= 2 * l1 + 1; k1++) {
+              double *x = &X[(n2 + k2 - 1) * norb_mu + (n1 + k1 - 1)];
+              *x = *x + left[5 * (m1 - 1) + (k1 - 1)] * M[(n2 + m2 - 1) * norb_mu + (n1 + m1 - 1)] *
+                            right[5 * (m2 - 1) + (k2 - 1)];
+            }
+      n2 += 2 * l2 + 1;
+    }
+    n1 += 2 * l1 + 1;
+  }
+}
+
+/* range list of find_neighbors_PP (mode 0: rc_i + rcPP_j) or find_neighbors_PPx
+ * (mode 1: rcPP_i + rc_j), M_neighbors_PP.f90:409-516, 672-778 */
+static void pp_range_list(fbo_system *s, int mode, int **n_out, int **off_out, int **j_out, int **b_out,
+                          int **self_out, int *err) {
+  const fbo_tables *t = s->t;
+  int N = s->natoms;
+  int *n = calloc(N, sizeof(int)), *off = calloc(N + 1, sizeof(int)), *self = malloc(sizeof(int) * N);
+  int cap = 1024, tot = 0;
+  int *jj = malloc(sizeof(int) * cap), *bb = malloc(sizeof(int) * cap);
+  for (int ia = 1; ia <= N; ia++) {
+    int in1 = s->imass[ia - 1];
+    const double *r1 = RAT(s, ia);
+    double rci = mode == 0 ? rcut_max(t, in1) : t->rc_pp[in1 - 1];
+    for (int mb = 0; mb <= s->mbeta_max; mb++)
+      for (int ja = 1; ja <= N; ja++) {
+        int in2 = s->imass[ja - 1];
+        double r2[3];
+        for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+        double rcj = mode == 0 ? t->rc_pp[in2 - 1] : rcut_max(t, in2);
+        double z = fbo_distance(r1, r2);
+        double range = rci + rcj - 0.01;
+        if (z < range) {
+          if (tot == cap) { cap *= 2; jj = realloc(jj, sizeof(int) * cap); bb = realloc(bb, sizeof(int) * cap); }
+          jj[tot] = ja; bb[tot] = mb; tot++;
+          n[ia - 1]++;
+        }
+      }
+    off[ia] = tot;
+  }
+  for (int ia = 1; ia <= N; ia++) {
+    self[ia - 1] = -999;
+    for (int k = 0; k < n[ia - 1]; k++)
+      if (jj[off[ia - 1] + k] == ia && bb[off[ia - 1] + k] == 0) { self[ia - 1] = k + 1; break; }
+    if (self[ia - 1] == -999) *err = FBO_E_NEIGH_SELF;
+  }
+  *n_out = n; *off_out = off; *j_out = jj; *b_out = bb; *self_out = self;
+}
+
+/* driver_neighbors_PP, M_neighbors_PP.f90:98-151 (find_neighbors_PP, _PPx,
+ * _PPp with the maps and points, find_common_neighbors_PP :1301-1420) */
+int fbo_build_pp_lists(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  if (!s->neighn) return FBO_E_ORDER;
+  int N = s->natoms, err = 0;
+  pp_range_list(s, 0, &s->pp_n, &s->pp_off, &s->pp_j, &s->pp_b, &s->pp_self, &err);
+  pp_range_list(s, 1, &s->px_n, &s->px_off, &s->px_j, &s->px_b, &s->px_self, &err);
+  if (err) return err;
+  /* neighbors_PPp: every (jatom, image of ibeta + jbeta) reached through a PP
+   * neighbour ialpha of iatom and a PPx neighbour jatom of ialpha, first seen first */
+  s->pq_n = calloc(N, sizeof(int));
+  s->pq_off = calloc(N + 1, sizeof(int));
+  s->pq_self = malloc(sizeof(int) * N);
+  int cap = 1024, tot = 0;
+  s->pq_j = malloc(sizeof(int) * cap);
+  s->pq_b = malloc(sizeof(int) * cap);
+  for (int ia = 1; ia <= N; ia++) {
+    int base = tot, num = 0;
+    for (int in = 0; in < s->pp_n[ia - 1]; in++) {
+      int ial = s->pp_j[s->pp_off[ia - 1] + in], ib = s->pp_b[s->pp_off[ia - 1] + in];
+      for (int jn = 0; jn < s->px_n[ial - 1]; jn++) {
+        int ja = s->px_j[s->px_off[ial - 1] + jn], jb = s->px_b[s->px_off[ial - 1] + jn];
+        double r3[3];
+        for (int k = 0; k < 3; k++) r3[k] = XL(s, jb)[k] + XL(s, ib)[k];
+        int mb;
+        for (mb = 0; mb <= s->mbeta_max; mb++)
+          if (fbo_distance(XL(s, mb), r3) < 1.0e-4) break;   /* none found: mbeta_max + 1, as the DO loop leaves it */
+        int flag = 0;
+        for (int k = 0; k < num; k++)
+          if (s->pq_j[base + k] == ja && s->pq_b[base + k] == mb) flag = 1;
+        if (!flag) {
+          if (tot == cap) { cap *= 2; s->pq_j = realloc(s->pq_j, sizeof(int) * cap); s->pq_b = realloc(s->pq_b, sizeof(int) * cap); }
+          s->pq_j[tot] = ja; s->pq_b[tot] = mb; tot++; num++;
+        }
+      }
+    }
+    s->pq_n[ia - 1] = num;
+    s->pq_off[ia] = tot;
+    s->pq_self[ia - 1] = -999;
+    for (int k = 0; k < num; k++)
+      if (s->pq_j[base + k] == ia && s->pq_b[base + k] == 0) { s->pq_self[ia - 1] = k + 1; break; }
+    if (s->pq_self[ia - 1] == -999) return FBO_E_NEIGH_SELF;
+  }
+  /* maps into neighbors_PPp, and the PPx point (index of iatom among jatom's PP neighbours) */
+  s->pp_map = calloc(s->pp_off[N] + 1, sizeof(int));
+  s->px_map = calloc(s->px_off[N] + 1, sizeof(int));
+  s->px_point = calloc(s->px_off[N] + 1, sizeof(int));
+  for (int ia = 1; ia <= N; ia++) {
+    for (int in = 0; in < s->pp_n[ia - 1]; in++) {
+      int p = s->pp_off[ia - 1] + in;
+      for (int k = 0; k < s->pq_n[ia - 1]; k++)
+        if (s->pq_j[s->pq_off[ia - 1] + k] == s->pp_j[p] && s->pq_b[s->pq_off[ia - 1] + k] == s->pp_b[p]) {
+          s->pp_map[p] = k + 1;
+          break;
+        }
+    }
+    for (int in = 0; in < s->px_n[ia - 1]; in++) {
+      int p = s->px_off[ia - 1] + in;
+      for (int k = 0; k < s->pq_n[ia - 1]; k++)
+        if (s->pq_j[s->pq_off[ia - 1] + k] == s->px_j[p] && s->pq_b[s->pq_off[ia - 1] + k] == s->px_b[p]) {
+          s->px_map[p] = k + 1;
+          break;
+        }
+    }
+  }
+  for (int ia = 1; ia <= N; ia++) {
+    const double *r1 = RAT(s, ia);
+    for (int in = 0; in < s->px_n[ia - 1]; in++) {
+      int p = s->px_off[ia - 1] + in;
+      int ja = s->px_j[p], jb = s->px_b[p];
+      for (int k = 0; k < s->pp_n[ja - 1]; k++) {
+        int ka = s->pp_j[s->pp_off[ja - 1] + k], kb = s->pp_b[s->pp_off[ja - 1] + k];
+        double r2[3];
+        for (int q = 0; q < 3; q++) r2[q] = RAT(s, ka)[q] + XL(s, kb)[q] + XL(s, jb)[q];
+        if (fbo_distance(r1, r2) < 1.0e-4) { s->px_point[p] = k + 1; break; }
+      }
+    }
+  }
+  /* common neighbours: ordered pairs of PPx neighbours of ialpha (neither
+   * ialpha itself), kept when jatom is found among iatom's PPp neighbours */
+  s->pc_n = calloc(N, sizeof(int));
+  s->pc_off = calloc(N + 1, sizeof(int));
+  cap = 1024; tot = 0;
+  s->pc_ij = malloc(sizeof(int) * cap); s->pc_ib = malloc(sizeof(int) * cap);
+  s->pc_jj = malloc(sizeof(int) * cap); s->pc_jb = malloc(sizeof(int) * cap); s->pc_m = malloc(sizeof(int) * cap);
+  for (int ial = 1; ial <= N; ial++) {
+    int num = 0;
+    for (int in = 0; in < s->px_n[ial - 1]; in++) {
+      int ia = s->px_j[s->px_off[ial - 1] + in], ib = s->px_b[s->px_off[ial - 1] + in];
+      if (ia == ial && ib == 0) continue;
+      double r1[3];
+      for (int k = 0; k < 3; k++) r1[k] = RAT(s, ia)[k] + XL(s, ib)[k];
+      for (int jn = 0; jn < s->px_n[ial - 1]; jn++) {
+        int ja = s->px_j[s->px_off[ial - 1] + jn], jb = s->px_b[s->px_off[ial - 1] + jn];
+        if ((ja == ial && jb == 0) || in == jn) continue;
+        double r2[3], r[3];
+        for (int k = 0; k < 3; k++) { r2[k] = RAT(s, ja)[k] + XL(s, jb)[k]; r[k] = r2[k] - r1[k]; }
+        int m = -9999;
+        for (int kn = 0; kn < s->pq_n[ia - 1]; kn++) {
+          int ka = s->pq_j[s->pq_off[ia - 1] + kn], kb = s->pq_b[s->pq_off[ia - 1] + kn];
+          if (kb > s->mbeta_max) continue;   /* an image outside xl: the reference would read past it */
+          double r3[3];
+          for (int k = 0; k < 3; k++) r3[k] = XL(s, kb)[k] + RAT(s, ka)[k] - RAT(s, ia)[k];
+          if (fbo_distance(r, r3) < 1.0e-4) { m = kn + 1; break; }
+        }
+        if (m == -9999) continue;
+        if (tot == cap) {
+          cap *= 2;
+          s->pc_ij = realloc(s->pc_ij, sizeof(int) * cap); s->pc_ib = realloc(s->pc_ib, sizeof(int) * cap);
+          s->pc_jj = realloc(s->pc_jj, sizeof(int) * cap); s->pc_jb = realloc(s->pc_jb, sizeof(int) * cap);
+          s->pc_m = realloc(s->pc_m, sizeof(int) * cap);
+        }
+        s->pc_ij[tot] = ia; s->pc_ib[tot] = ib; s->pc_jj[tot] = ja; s->pc_jb[tot] = jb; s->pc_m[tot] = m;
+        tot++; num++;
+      }
+    }
+    s->pc_n[ial - 1] = num;
+    s->pc_off[ial] = tot;
+  }
+  /* block offsets: svnl per PP pair, vnl per PPp pair */
+  s->sv_off = malloc(sizeof(int64_t) * (s->pp_off[N] + 1));
+  int64_t o = 0;
+  for (int ia = 1; ia <= N; ia++)
+    for (int in = 0; in < s->pp_n[ia - 1]; in++) {
+      int p = s->pp_off[ia - 1] + in;
+      s->sv_off[p] = o;
+      o += (int64_t)t->norb[s->imass[ia - 1] - 1] * t->norb_pp[s->imass[s->pp_j[p] - 1] - 1];
+    }
+  s->sv_off[s->pp_off[N]] = o;
+  s->sv_len = o;
+  s->svnl = calloc(o ? o : 1, sizeof(double));
+  s->vl_off = malloc(sizeof(int64_t) * (s->pq_off[N] + 1));
+  o = 0;
+  for (int ia = 1; ia <= N; ia++)
+    for (int in = 0; in < s->pq_n[ia - 1]; in++) {
+      int p = s->pq_off[ia - 1] + in;
+      s->vl_off[p] = o;
+      o += (int64_t)t->norb[s->imass[ia - 1] - 1] * t->norb[s->imass[s->pq_j[p] - 1] - 1];
+    }
+  s->vl_off[s->pq_off[N]] = o;
+  s->vl_len = o;
+  s->vnl = calloc(o ? o : 1, sizeof(double));
+  for (int i = 0; i < 3; i++) s->ppdone[i] = 0;
+  return FBO_OK;
+}
+
+/* assemble_svnl, M_assemble_PP_2c.f90:95-207 */
+int fbo_assemble_svnl(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  if (!s->pp_n) return FBO_E_ORDER;
+  double sm[64], eps[9];
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
+    int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1];
+    for (int in = 0; in < s->pp_n[ia - 1]; in++) {
+      int p = s->pp_off[ia - 1] + in;
+      int mb = s->pp_b[p], ja = s->pp_j[p], in2 = s->imass[ja - 1], nnu = t->norb_pp[in2 - 1];
+      double r2[3];
+      for (int k = 0; k < 3; k++) r2[k] = RAT(s, ja)[k] + XL(s, mb)[k];
+      double z = pair_frame(s, r1, r2, eps);
+      int rc = fbo_getMEs_2c(t, in1, in2, P_VNL, 0, z, nmu, nnu, sm);
+      if (rc) return rc;
+      fbo_rotate_PP(t, in1, in2, eps, nmu, nnu, sm, s->svnl + s->sv_off[p]);
+    }
+  }
+  s->ppdone[0] = 1;
+  s->ppdone[1] = s->ppdone[2] = 0;
+  return FBO_OK;
+}
+
+/* PPx(imu, inu) = sum_ncc cl(ncc) * A(imu, ncc) * B(inu, ncc), loops inu, imu, ncc */
+static void kb_contract(int nmu, int nnu, int ncc, const double *cl, const double *A, const double *B, double *X) {
+  for (int inu = 0; inu < nnu; inu++)
+    for (int imu = 0; imu < nmu; imu++) {
+      double x = 0.0;
+      for (int c = 0; c < ncc; c++) x = x + cl[c] * A[c * nmu + imu] * B[c * nnu + inu];
+      X[inu * nmu + imu] = x;
+    }
+}
+
+/* assemble_vnl_2c, M_assemble_PP_2c.f90:230-496 */
+int fbo_assemble_vnl_2c(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  if (!s->ppdone[0]) return FBO_E_ORDER;
+  memset(s->vnl, 0, sizeof(double) * (s->vl_len ? s->vl_len : 1));
+  double X[64], cl[16];
+  /* atom case: every PP neighbour's projectors into the PPp self block */
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1];
+    double *blk = s->vnl + s->vl_off[s->pq_off[ia - 1] + s->pq_self[ia - 1] - 1];
+    for (int in = 0; in < s->pp_n[ia - 1]; in++) {
+      int p = s->pp_off[ia - 1] + in, in2 = s->imass[s->pp_j[p] - 1];
+      const double *sv = s->svnl + s->sv_off[p];
+      cl_values(t, in2, cl);
+      kb_contract(nmu, nmu, t->norb_pp[in2 - 1], cl, sv, sv, X);
+      for (int i = 0; i < nmu * nmu; i++) blk[i] = blk[i] + X[i];
+    }
+  }
+  /* ontop left: PPx neighbours, the projectors of iatom */
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1];
+    for (int in = 0; in < s->px_n[ia - 1]; in++) {
+      int p = s->px_off[ia - 1] + in, ja = s->px_j[p], mb = s->px_b[p];
+      int in2 = s->imass[ja - 1], nnu = t->norb[in2 - 1];
+      if (ia == ja && mb == 0) {
+        if (s->px_self[ia - 1] != in + 1) return FBO_E_NEIGH_SELF;
+        continue;
+      }
+      cl_values(t, in1, cl);
+      const double *sv1 = s->svnl + s->sv_off[s->pp_off[ia - 1] + s->pp_self[ia - 1] - 1];
+      const double *sv2 = s->svnl + s->sv_off[s->pp_off[ja - 1] + s->px_point[p] - 1];
+      kb_contract(nmu, nnu, t->norb_pp[in1 - 1], cl, sv1, sv2, X);
+      double *blk = s->vnl + s->vl_off[s->pq_off[ia - 1] + s->px_map[p] - 1];
+      for (int i = 0; i < nmu * nnu; i++) blk[i] = blk[i] + X[i];
+    }
+  }
+  /* ontop right: PP neighbours, the projectors of jatom */
+  for (int ia = 1; ia <= s->natoms; ia++) {
+    int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1];
+    for (int in = 0; in < s->pp_n[ia - 1]; in++) {
+      int p = s->pp_off[ia - 1] + in, ja = s->pp_j[p], mb = s->pp_b[p];
+      int in2 = s->imass[ja - 1], nnu = t->norb[in2 - 1];
+      if (ia == ja && mb == 0) {
+        if (s->pp_self[ia - 1] != in + 1) return FBO_E_NEIGH_SELF;
+        continue;
+      }
+      cl_values(t, in2, cl);
+      const double *sv1 = s->svnl + s->sv_off[p];
+      const double *sv2 = s->svnl + s->sv_off[s->pp_off[ja - 1] + s->pp_self[ja - 1] - 1];
+      kb_contract(nmu, nnu, t->norb_pp[in2 - 1], cl, sv1, sv2, X);
+      double *blk = s->vnl + s->vl_off[s->pq_off[ia - 1] + s->pp_map[p] - 1];
+      for (int i = 0; i < nmu * nnu; i++) blk[i] = blk[i] + X[i];
+    }
+  }
+  s->ppdone[1] = 1;
+  s->ppdone[2] = 0;
+  return FBO_OK;
+}
+
+/* mpairnay, M_assemble_PP_3c.f90:266-364: the PP neighbour of iatom that is
+ * jatom at offset r12 from iatom; exactly one match, else the reference stops */
+static int mpairnay(const fbo_system *s, int ia, int ja, const double *r12, int *mpair) {
+  const double *r1 = RAT(s, ia);
+  int imatch = 0;
+  *mpair = 0;
+  for (int in = 0; in < s->pp_n[ia - 1]; in++) {
+    int p = s->pp_off[ia - 1] + in;
+    if (s->pp_j[p] != ja) continue;
+    double r2[3], r21[3];
+    for (int k = 0; k < 3; k++) { r2[k] = RAT(s, ja)[k] + XL(s, s->pp_b[p])[k]; r21[k] = r2[k] - r1[k]; }
+    if (fbo_distance(r12, r21) < 0.0001) { imatch++; *mpair = in + 1; }
+  }
+  return imatch == 1 ? FBO_OK : FBO_E_NEIGH_BACK;
+}
+
+/* assemble_vnl_3c, M_assemble_PP_3c.f90:94-237 */
+int fbo_assemble_vnl_3c(fbo_system *s) {
+  const fbo_tables *t = s->t;
+  if (!s->ppdone[1]) return FBO_E_ORDER;
+  double X[64], cl[16];
+  for (int ial = 1; ial <= s->natoms; ial++) {
+    const double *rna = RAT(s, ial);
+    int indna = s->imass[ial - 1];
+    cl_values(t, indna, cl);
+    for (int in = 0; in < s->pc_n[ial - 1]; in++) {
+      int c = s->pc_off[ial - 1] + in, mneigh = s->pc_m[c];
+      if (mneigh == 0) continue;
+      int ia = s->pc_ij[c], ib = s->pc_ib[c], ja = s->pc_jj[c], jb = s->pc_jb[c];
+      int nmu = t->norb[s->imass[ia - 1] - 1], nnu = t->norb[s->imass[ja - 1] - 1];
+      double r1[3], r2[3], r31[3], r32[3];
+      for (int k = 0; k < 3; k++) {
+        r1[k] = RAT(s, ia)[k] + XL(s, ib)[k];
+        r2[k] = RAT(s, ja)[k] + XL(s, jb)[k];
+        r31[k] = rna[k] - r1[k];
+        r32[k] = rna[k] - r2[k];
+      }
+      int m31, m32, rc;
+      if ((rc = mpairnay(s, ia, ial, r31, &m31))) return rc;
+      if ((rc = mpairnay(s, ja, ial, r32, &m32))) return rc;
+      const double *sv1 = s->svnl + s->sv_off[s->pp_off[ia - 1] + m31 - 1];
+      const double *sv2 = s->svnl + s->sv_off[s->pp_off[ja - 1] + m32 - 1];
+      kb_contract(nmu, nnu, t->norb_pp[indna - 1], cl, sv1, sv2, X);
+      double *blk = s->vnl + s->vl_off[s->pq_off[ia - 1] + mneigh - 1];
+      for (int i = 0; i < nmu * nnu; i++) blk[i] = blk[i] + X[i];
+    }
+  }
+  s->ppdone[2] = 1;
+  return FBO_OK;
+}
+
+/* sizes: [0] PP pairs, [1] PPx pairs, [2] PPp pairs, [3] PP triplets, [4] svnl length, [5] vnl length */
+void fbo_pp_sizes(const fbo_system *s, int64_t *out) {
+  int N = s->natoms;
+  out[0] = s->pp_off ? s->pp_off[N] : 0;
+  out[1] = s->px_off ? s->px_off[N] : 0;
+  out[2] = s->pq_off ? s->pq_off[N] : 0;
+  out[3] = s->pc_off ? s->pc_off[N] : 0;
+  out[4] = s->sv_len;
+  out[5] = s->vl_len;
+}
+
+/* which: 0 PP (neighn, j, b, map), 1 PPx (neighn, j, b, map, point), 2 PPp (neighn, j, b),
+ * 3 common (ncommon, iatom_j, iatom_b, jatom_j, jatom_b, mneigh); self: the *_self array */
+int fbo_get_pp_list(const fbo_system *s, int which, int *n, int *a0, int *a1, int *a2, int *a3, int *a4,
+                    int *self) {
+  int N = s->natoms;
+  const int *cnt[4] = {s->pp_n, s->px_n, s->pq_n, s->pc_n};
+  const int *off[4] = {s->pp_off, s->px_off, s->pq_off, s->pc_off};
+  if (which < 0 || which > 3 || !cnt[which]) return FBO_E_ARG;
+  int tot = off[which][N];
+  memcpy(n, cnt[which], sizeof(int) * N);
+  const int *src[4][5] = {{s->pp_j, s->pp_b, s->pp_map, NULL, NULL},
+                          {s->px_j, s->px_b, s->px_map, s->px_point, NULL},
+                          {s->pq_j, s->pq_b, NULL, NULL, NULL},
+                          {s->pc_ij, s->pc_ib, s->pc_jj, s->pc_jb, s->pc_m}};
+  int *dst[5] = {a0, a1, a2, a3, a4};
+  for (int k = 0; k < 5; k++)
+    if (src[which][k] && dst[k]) memcpy(dst[k], src[which][k], sizeof(int) * tot);
+  const int *sf[4] = {s->pp_self, s->px_self, s->pq_self, NULL};
+  if (self && sf[which]) memcpy(self, sf[which], sizeof(int) * N);
+  return FBO_OK;
+}
+
+/* which: 0 svnl (per PP pair), 1 vnl (per PPp pair) */
+int fbo_get_pp_blocks(const fbo_system *s, int which, double *out) {
+  if (which == 0 && s->svnl) memcpy(out, s->svnl, sizeof(double) * s->sv_len);
+  else if (which == 1 && s->vnl) memcpy(out, s->vnl, sizeof(double) * s->vl_len);
+  else return FBO_E_ARG;
   return FBO_OK;
 }

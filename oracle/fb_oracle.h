@@ -113,4 +113,20 @@ int fbo_assemble_rho_weighted_3c(fbo_system *s);
 int64_t fbo_shell_block_len(const fbo_system *s);
 int fbo_get_rho(const fbo_system *s, int which, double *out);
 
+/* pseudopotential (c.SYSTEM/M_neighbors_PP.f90, j.ASSEMBLERS/M_assemble_PP_2c.f90,
+ * M_assemble_PP_3c.f90).  Per species: nssh_PP, lssh_PP[FBO_MAXSH], rcutoff_PP
+ * (as read from info.dat), cl[FBO_MAXSH] (clPP.ZZ.dat). */
+int fbo_tables_set_pp(fbo_tables *t, const int *nssh_pp, const int *lssh_pp, const double *rcutoff_pp,
+                      const double *cl);
+void fbo_rotate_PP(const fbo_tables *t, int in1, int in2, const double *eps, int norb_mu, int norb_nu,
+                   const double *mmatrix, double *xmatrix);
+int fbo_build_pp_lists(fbo_system *s);
+int fbo_assemble_svnl(fbo_system *s);
+int fbo_assemble_vnl_2c(fbo_system *s);
+int fbo_assemble_vnl_3c(fbo_system *s);
+void fbo_pp_sizes(const fbo_system *s, int64_t *out);
+int fbo_get_pp_list(const fbo_system *s, int which, int *n, int *a0, int *a1, int *a2, int *a3, int *a4,
+                    int *self);
+int fbo_get_pp_blocks(const fbo_system *s, int which, double *out);
+
 #endif

@@ -81,6 +81,13 @@ def lib():
         L.fbo_shell_block_len.restype = i64
         L.fbo_shell_block_len.argtypes = [vp]
         L.fbo_get_rho.argtypes = [vp, C.c_int, dp]
+        L.fbo_tables_set_pp.argtypes = [vp, ip, ip, dp, dp]
+        L.fbo_rotate_PP.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp]
+        for f in ("fbo_build_pp_lists", "fbo_assemble_svnl", "fbo_assemble_vnl_2c", "fbo_assemble_vnl_3c"):
+            getattr(L, f).argtypes = [vp]
+        L.fbo_pp_sizes.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.fbo_get_pp_list.argtypes = [vp, C.c_int, ip, ip, ip, ip, ip, ip, ip]
+        L.fbo_get_pp_blocks.argtypes = [vp, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -126,6 +133,15 @@ class Tables:
             d = np.ascontiguousarray(c.data, np.float64)
             _check(L.fbo_add_cell3c(self.h, a, b, cc, it, isub, ith, c.nME, c.nx, c.ny, c.xmax, c.ymax,
                                     *(_ip(x) for x in arrs), _dp(d)), "add_cell3c")
+        if any(getattr(sp, "nssh_pp", 0) for sp in ts.species):
+            npp = np.array([sp.nssh_pp for sp in ts.species], np.int32)
+            lpp = np.zeros((ns, MAXSH), np.int32)
+            clv = np.zeros((ns, MAXSH))
+            rpp = np.array([sp.rcutoff_pp for sp in ts.species], np.float64)
+            for i, sp in enumerate(ts.species):
+                lpp[i, :sp.nssh_pp] = sp.lssh_pp
+                clv[i, :sp.nssh_pp] = sp.cl_pp
+            _check(L.fbo_tables_set_pp(self.h, _ip(npp), _ip(lpp), _dp(rpp), _dp(clv)), "set_pp")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -345,6 +361,37 @@ class System:
         self.set_qin(qin)
         self.assemble_rho()
         return {k: self.rho(k) for k in self.RHO}
+
+    PP_LISTS = {"pp": (0, ("j", "b", "map")), "ppx": (1, ("j", "b", "map", "point")), "ppp": (2, ("j", "b")),
+                "ppc": (3, ("ij", "ib", "jj", "jb", "m"))}
+
+    def run_pp(self):
+        """driver_neighbors_PP, assemble_svnl, assemble_vnl_2c, assemble_vnl_3c
+        (after build_lists); the PP lists and the sVNL and vnl blocks."""
+        L = lib()
+        for f, w in ((L.fbo_build_pp_lists, "build_pp_lists"), (L.fbo_assemble_svnl, "assemble_svnl"),
+                     (L.fbo_assemble_vnl_2c, "assemble_vnl_2c"), (L.fbo_assemble_vnl_3c, "assemble_vnl_3c")):
+            _check(f(self.h), w)
+        sz = (C.c_int64 * 6)()
+        L.fbo_pp_sizes(self.h, sz)
+        n = self.st.natoms
+        out = {}
+        for name, (which, fields) in self.PP_LISTS.items():
+            tot = max(int(sz[which]), 1)
+            cnt = np.zeros(n, np.int32)
+            arrs = [np.zeros(tot, np.int32) for _ in range(5)]
+            self_ = np.zeros(n, np.int32)
+            _check(L.fbo_get_pp_list(self.h, which, _ip(cnt), *(_ip(a) for a in arrs), _ip(self_)), "get_pp_list")
+            out[name + "_n"] = cnt
+            for k, f in enumerate(fields):
+                out[name + "_" + f] = arrs[k][:int(sz[which])]
+            if which < 3:
+                out[name + "_self"] = self_
+        for which, key, k in ((0, "SVNL", 4), (1, "VNL", 5)):
+            buf = np.zeros(max(int(sz[k]), 1))
+            _check(L.fbo_get_pp_blocks(self.h, which, _dp(buf)), "get_pp_blocks")
+            out[key] = buf[:int(sz[k])]
+        return out
 
     def run_all(self):
         lists = self.build_lists()

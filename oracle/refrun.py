@@ -222,3 +222,75 @@ def run_rho(ts, structure, qin, lists, binary=HARNESS_RREF, nrep=1, workdir=None
         import shutil
         shutil.rmtree(tmp, ignore_errors=True)
     return res
+
+
+HARNESS_PREF = os.path.join(HERE, "_ref", "fb_pharness_ref")
+
+
+def parse_pp(path, species, imass):
+    """PP lists (per list: neighn and the flat per-atom arrays, 1-based as in the
+    reference), sVNL blocks (per PP pair, norb_mu x norb_PP_nu) and vnl blocks
+    (per PPp pair, norb_mu x norb_nu), flat in pair order."""
+    with open(path, "rb") as f:
+        r = _Reader(f.read())
+    natoms, nrep = r.i32(2)
+    times = r.f64(3)
+    out = {"nrep": int(nrep), "times": dict(zip(("SVNL", "VNL2C", "VNL3C"), times))}
+
+    def block_list(prefix, fields):
+        n = r.i32(natoms)
+        cols = {k: [] for k in fields}
+        for ia in range(natoms):
+            for k in fields:
+                cols[k].append(r.i32(int(n[ia])))
+        out[prefix + "_n"] = n
+        for k in fields:
+            out[prefix + "_" + k] = np.concatenate(cols[k]) if cols[k] else np.zeros(0, np.int32)
+
+    block_list("pp", ("j", "b", "map"))
+    out["pp_self"] = r.i32(natoms)
+    block_list("ppx", ("j", "b", "map", "point"))
+    out["ppx_self"] = r.i32(natoms)
+    block_list("ppp", ("j", "b"))
+    out["ppp_self"] = r.i32(natoms)
+    block_list("ppc", ("ij", "ib", "jj", "jb", "m"))
+    norb = np.array([sp.norb for sp in species], np.int64)
+    norb_pp = np.array([sp.norb_pp for sp in species], np.int64)
+    sv, vl = [], []
+    p = 0
+    for ia in range(natoms):
+        for _ in range(int(out["pp_n"][ia])):
+            sv.append(r.f64(int(norb[imass[ia] - 1] * norb_pp[imass[out["pp_j"][p] - 1] - 1])))
+            p += 1
+    p = 0
+    for ia in range(natoms):
+        for _ in range(int(out["ppp_n"][ia])):
+            vl.append(r.f64(int(norb[imass[ia] - 1] * norb[imass[out["ppp_j"][p] - 1] - 1])))
+            p += 1
+    assert r.pos == len(r.buf), "trailing bytes in PP harness output"
+    out["SVNL"] = np.concatenate(sv) if sv else np.zeros(0)
+    out["VNL"] = np.concatenate(vl) if vl else np.zeros(0)
+    return out
+
+
+def run_pp(ts, structure, binary=HARNESS_PREF, nrep=1, workdir=None, timeout=3600):
+    """Run the pseudopotential harness on (tables, structure); the species of
+    ts carry their PP shells (synth.with_pp) and ts the P_vnl tables."""
+    own = workdir is None
+    tmp = tempfile.mkdtemp(prefix="fbpref_") if own else workdir
+    fdir = os.path.join(tmp, "Fdata")
+    if not os.path.isdir(fdir):
+        fdata_io.write_fdata(ts, fdir)
+    inp = os.path.join(tmp, "harness.in")
+    fdata_io.write_harness_input(inp, fdir, ts, structure)
+    ppin = os.path.join(tmp, "pp.in")
+    fdata_io.write_pp_input(ppin, ts)
+    outp = os.path.join(tmp, "pharness.out")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run([binary, inp, ppin, outp, str(nrep)], cwd=tmp, check=True, env=env,
+                   stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=timeout)
+    res = parse_pp(outp, ts.species, structure.imass)
+    if own:
+        import shutil
+        shutil.rmtree(tmp, ignore_errors=True)
+    return res

@@ -6,6 +6,8 @@ build container, where the reference sources are present).
     python tests/golden/make_golden.py --forces   # force fixtures only
     make -C oracle rref                            # McWEDA density harness
     python tests/golden/make_golden.py --rho      # density fixtures only
+    make -C oracle pref                            # pseudopotential harness
+    python tests/golden/make_golden.py --pp       # pseudopotential fixtures only
 
 Each fixture is an .npz holding the inputs that are not re-derivable from a
 seed (explicit positions for the hand-made cases), the reference's neighbour
@@ -153,7 +155,57 @@ def rho_fixture(name, ts, st, keep_atoms):
     print(f"{name}_rho: orbital blocks kept for {npair_keep} pairs")
 
 
+PP_TABLE_SEED = 6066
+PP_LIST_KEYS = ("pp_n", "pp_j", "pp_b", "pp_map", "pp_self", "ppx_n", "ppx_j", "ppx_b", "ppx_map", "ppx_point",
+                "ppx_self", "ppp_n", "ppp_j", "ppp_b", "ppp_self", "ppc_n", "ppc_ij", "ppc_ib", "ppc_jj", "ppc_jb",
+                "ppc_m")
+
+
+def pp_tables(ts, seed=PP_TABLE_SEED):
+    """ts with PP shells on its species and the P_vnl tables (seed)."""
+    import copy
+    tsp = copy.copy(ts)
+    tsp.species = synth.with_pp(ts.species)
+    tsp.cells2c = dict(ts.cells2c)
+    tsp.cells3c = dict(ts.cells3c)
+    return synth.add_pp_tables(tsp, seed=seed)
+
+
+def pp_fixture(name, ts, st, keep_atoms):
+    """<name>_pp.npz: the reference's driver_neighbors_PP lists, assemble_svnl
+    blocks and the vnl blocks after assemble_vnl_2c + assemble_vnl_3c, with the
+    PP shells of synth.PP_SHELLS and the P_vnl tables of
+    synth.add_pp_tables(seed PP_TABLE_SEED); blocks of the first keep_atoms
+    atoms, checksums of all."""
+    tsp = pp_tables(ts)
+    r = refrun.run_pp(tsp, st)
+    d = {"pp_table_seed": np.int64(PP_TABLE_SEED), "pp_tables_digest": np.array(tables_digest(tsp))}
+    for k in PP_LIST_KEYS:
+        d[k] = np.asarray(r[k], np.int32)
+    norb = np.array([sp.norb for sp in tsp.species])
+    norb_pp = np.array([sp.norb_pp for sp in tsp.species])
+    for key, nkey, jkey, dims in (("SVNL", "pp_n", "pp_j", norb_pp), ("VNL", "ppp_n", "ppp_j", norb)):
+        pair_atom = np.repeat(np.arange(st.natoms), r[nkey])
+        sizes = norb[st.imass[pair_atom] - 1] * dims[st.imass[r[jkey] - 1] - 1]
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        npair_keep = len(sizes) if keep_atoms is None else int(np.sum(r[nkey][:keep_atoms]))
+        d[key] = r[key][: offs[npair_keep]]
+        d[key + "_npairs_kept"] = np.int32(npair_keep)
+        d[key + "_sum"] = np.float64(np.sum(r[key]))
+        d[key + "_sumsq"] = np.float64(np.sum(r[key] ** 2))
+        d[key + "_sumabs"] = np.float64(np.sum(np.abs(r[key])))
+    np.savez_compressed(os.path.join(HERE, f"{name}_pp.npz"), **d)
+    print(f"{name}_pp: {len(r['pp_j'])} PP, {len(r['ppx_j'])} PPx, {len(r['ppp_j'])} PPp pairs, "
+          f"{len(r['ppc_m'])} PP triplets")
+
+
 def main():
+    if "--pp" in sys.argv:
+        if not os.path.isfile(refrun.HARNESS_PREF):
+            sys.exit("pseudopotential harness not built: run `make -C oracle pref` first")
+        for name, ts, st, keep_atoms in cases():
+            pp_fixture(name, ts, st, keep_atoms if keep_atoms is not None else (10 if st.natoms > 32 else None))
+        return
     if "--rho" in sys.argv:
         if not os.path.isfile(refrun.HARNESS_RREF):
             sys.exit("density harness not built: run `make -C oracle rref` first")

@@ -8,7 +8,7 @@ Fdata (read by e.FDATA/M_Fdata_2c.f90:149-272 and e.FDATA/M_Fdata_3c.f90:143-264
 * ``TT_munu_2c.Z1.Z2.dat``  mu / nu / mvalue lines, one triple per cell of
   interaction TT in directory order (the reader keeps the unit connected across
   cells of one interaction, so the triples are consumed in sequence)
-* ``clPP.ZZ.dat``    Kleinman-Bylander coefficients (empty: no PP shells)
+* ``clPP.ZZ.dat``    Kleinman-Bylander coefficients, one line per PP shell (empty: no PP shells)
 * ``3c.Z1.Z2.Z3.dat`` / ``.dir`` with ``iindex itype isubtype itheta fname nME nx xmax ny ymax``;
   data files hold ny*nx lines (iy outer, ix inner) of nME values.
 
@@ -44,7 +44,9 @@ def write_fdata(ts: TableSet, outdir: str):
     ns = len(sp)
     for a in range(1, ns + 1):
         za = sp[a - 1].nZ
-        open(os.path.join(outdir, f"clPP.{za:02d}.dat"), "w").close()
+        with open(os.path.join(outdir, f"clPP.{za:02d}.dat"), "w") as f:
+            for c in getattr(sp[a - 1], "cl_pp", []):
+                f.write(_fmt(c) + "\n")
         for b in range(1, ns + 1):
             zb = sp[b - 1].nZ
             keys = sorted(k for k in ts.cells2c if k[0] == a and k[1] == b)
@@ -86,6 +88,16 @@ def write_fdata(ts: TableSet, outdir: str):
                         for c in cells:
                             for arr in (c.mu, c.nu, c.mvalue):
                                 f.write(" ".join(str(int(v)) for v in arr) + "\n")
+
+
+def write_pp_input(path: str, ts: TableSet):
+    """Pseudopotential shells of every species for the PP harness: nssh_PP,
+    then lssh_PP, then rcutoff_PP (the cl values travel in clPP.ZZ.dat)."""
+    with open(path, "w") as f:
+        for sp in ts.species:
+            f.write(f"{sp.nssh_pp}\n")
+            f.write(" ".join(str(l) for l in sp.lssh_pp) + "\n")
+            f.write(_fmt(sp.rcutoff_pp) + "\n")
 
 
 def write_harness_input(path: str, fdata_dir: str, ts: TableSet, s: Structure):

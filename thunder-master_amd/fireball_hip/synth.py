@@ -34,6 +34,7 @@ P_MAXTHETA = 5              # include/interactions_3c.h:4
 
 # interaction ids (include/interactions_2c.h:5-25, interactions_3c.h:6-8)
 P_OVERLAP, P_KINETIC, P_VNA_ONTOPL, P_VNA_ONTOPR, P_VNA_ATOM, P_DIPOLE_Z = 1, 2, 3, 4, 5, 10
+P_VNL = 6
 P_RHO_ONTOPL, P_RHO_ONTOPR, P_RHO_ATOM = 15, 16, 17
 P_RHOS_ONTOPL, P_RHOS_ONTOPR, P_RHOS_ATOM = 19, 20, 21
 P_RHO_3C, P_RHOS_3C, P_BCNA = 1, 2, 3
@@ -48,6 +49,19 @@ class Species:
     nZ: int
     lssh: list            # angular momentum per shell
     rcutoff: list         # shell cutoffs in bohr (as read from info.dat)
+    # pseudopotential shells (info.dat: nssh_PP, lssh_PP, rcutoff_PP) and their
+    # Kleinman-Bylander coefficients (clPP.ZZ.dat); empty: no PP
+    lssh_pp: list = field(default_factory=list)
+    rcutoff_pp: float = 0.0
+    cl_pp: list = field(default_factory=list)
+
+    @property
+    def nssh_pp(self):
+        return len(self.lssh_pp)
+
+    @property
+    def norb_pp(self):
+        return sum(2 * l + 1 for l in self.lssh_pp)
 
     @property
     def nssh(self):
@@ -400,6 +414,56 @@ def add_rho_tables(ts: TableSet, seed: int = 4048, with_3c: bool = True) -> Tabl
                                         * np.exp(-by[None, None, :] * xg[:, None, None]))
                                 ts.cells3c[(a_, b_, c_, itype, isorp, itheta)] = Cell3c(
                                     nME, n3c, n3c, any3.xmax, any3.ymax, mu, nu, mv, np.ascontiguousarray(data))
+    return ts
+
+
+# pseudopotential shells of the synthetic species: (lssh_PP, rcutoff_PP as the
+# neighbour code uses it (Angstrom), Kleinman-Bylander cl per PP shell)
+PP_SHELLS = {1: ([0], 1.05, [-2.1]), 6: ([0, 1], 1.40, [-3.2, 1.7]), 7: ([0, 1], 1.30, [-3.6, 1.9]),
+             8: ([0, 1], 1.25, [-4.1, 2.3]), 14: ([0, 1], 1.90, [-2.6, 1.3])}
+
+
+def with_pp(species):
+    """Copies of ``species`` carrying their pseudopotential shells (PP_SHELLS)."""
+    out = []
+    for sp in species:
+        l, rc, cl = PP_SHELLS[sp.nZ]
+        out.append(Species(sp.nZ, list(sp.lssh), list(sp.rcutoff), list(l), rc, list(cl)))
+    return out
+
+
+def munu_pp(sp_i: Species, sp_j: Species):
+    """(mu, nu, mvalue) of the <phi_i | Psi_PP(j)> tables: the make_munu order with
+    the PP shells of sp_j on the right."""
+    ghost = Species(sp_j.nZ, list(sp_j.lssh_pp), [0.0] * sp_j.nssh_pp)
+    return munu_2c(sp_i, ghost)
+
+
+def add_pp_tables(ts: TableSet, seed: int = 6066) -> TableSet:
+    """Add the non-local pseudopotential tables P_vnl (itype 6, isubtype 0) for
+    every ordered species pair (own random stream): <phi_mu(A) | Psi_ncc(B)>
+    with the orbitals of A on the left and the PP projectors of B on the right,
+    radial shapes as in make_tables.  The species of ``ts`` must carry PP shells
+    (with_pp)."""
+    rng = np.random.default_rng(seed)
+    species = ts.species
+    ns = len(species)
+    any2 = next(iter(ts.cells2c.values()))
+    nx2c, xmaxA = any2.nx, any2.xmax
+    r = _grid_bohr(nx2c)
+    env = (1.0 - (r / XMAX_BOHR) ** 2) ** 3
+    for a_ in range(1, ns + 1):
+        for b_ in range(1, ns + 1):
+            spA, spB = species[a_ - 1], species[b_ - 1]
+            if spB.nssh_pp == 0:
+                continue
+            mu, nu, mv, shells = munu_pp(spA, spB)
+            data = np.empty((nx2c, len(mu)))
+            for k in range(len(mu)):
+                A = rng.uniform(-2.0, 2.0)
+                a = rng.uniform(0.8, 2.0)
+                data[:, k] = A * np.exp(-a * r) * env
+            ts.cells2c[(a_, b_, P_VNL, 0)] = Cell2c(len(mu), nx2c, xmaxA, mu, nu, mv, data)
     return ts
 
 
