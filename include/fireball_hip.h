@@ -189,6 +189,41 @@ int fb_assemble_rho_weighted_3c(fb_ctx *ctx);   /* M_assemble_rho_McWEDA_DOGS.f9
 int fb_assemble_rho_all(fb_ctx *ctx);           /* the four in order: one 2c launch, two 3c launches */
 int fb_get_shell_block_len(fb_ctx *ctx, int istruct, int64_t *len);
 
+/* --- non-local pseudopotential: c.SYSTEM/M_neighbors_PP.f90,
+ * j.ASSEMBLERS/M_assemble_PP_2c.f90, M_assemble_PP_3c.f90 ----------------------
+ * fb_set_species_pp: the PP shells of every species (species(:)%nssh_PP,
+ *   %shell_PP(:)%lssh, %rcutoff_PP as the neighbour code uses it, and the
+ *   Kleinman-Bylander coefficients shell_PP(:)%cl of clPP.ZZ.dat); arrays
+ *   [nspecies] and [nspecies][FB_MAXSH]; s and p PP shells.  Call it before
+ *   fb_commit_tables: the P_vnl tables (interaction 6) are loaded only then.
+ *   fb_read_fdata also reads clPP.ZZ.dat into these coefficients.
+ * fb_build_lists_pp: driver_neighbors_PP (M_neighbors_PP.f90:98-151) for every
+ *   structure: neighbors_PP (rc_i + rcutoff_PP_j), neighbors_PPx (rcutoff_PP_i
+ *   + rc_j), neighbors_PPp with the maps and points, and the PP common
+ *   neighbours (triplets), all in the reference's order.
+ * fb_get_pp_list_sizes: sizes[6] = PP pairs, PPx pairs, PPp pairs, PP triplets,
+ *   sVNL length, vnl length (doubles) of one structure.
+ * fb_get_pp_lists: which 0 neighbors_PP (n = neighn, a0 neigh_j, a1 neigh_b,
+ *   a2 map, self = neighbors_PP_self), 1 neighbors_PPx (a3 point as well),
+ *   2 neighbors_PPp (a0, a1, self), 3 common neighbours (n = ncommon, a0..a4 =
+ *   iatom_common_j, iatom_common_b, jatom_common_j, jatom_common_b,
+ *   neigh_common); 1-based atom and neighbour numbers as in the reference;
+ *   unused outputs may be NULL.
+ * fb_assemble_svnl / _vnl_2c / _vnl_3c: in this order.  Blocks through
+ *   fb_get_pp_blocks: which 0 s%svnl (per PP pair, column-major norb_mu x
+ *   norb_PP_nu), 1 s%vnl (per PPp pair, norb_mu x norb_nu); binary64 with the
+ *   reference's summation order.  A geometry change discards the PP lists. */
+int fb_set_species_pp(fb_ctx *ctx, const int *nssh_pp, const int *lssh_pp, const double *rcutoff_pp,
+                      const double *cl);
+int fb_build_lists_pp(fb_ctx *ctx, int64_t *npp_total, int64_t *nppp_total, int64_t *ntriplets_total);
+int fb_get_pp_list_sizes(fb_ctx *ctx, int istruct, int64_t *sizes);
+int fb_get_pp_lists(fb_ctx *ctx, int istruct, int which, int *n, int *a0, int *a1, int *a2, int *a3, int *a4,
+                    int *self);
+int fb_assemble_svnl(fb_ctx *ctx);      /* j.ASSEMBLERS/M_assemble_PP_2c.f90:95-207   */
+int fb_assemble_vnl_2c(fb_ctx *ctx);    /* j.ASSEMBLERS/M_assemble_PP_2c.f90:230-496  */
+int fb_assemble_vnl_3c(fb_ctx *ctx);    /* j.ASSEMBLERS/M_assemble_PP_3c.f90:94-237   */
+int fb_get_pp_blocks(fb_ctx *ctx, int istruct, int which, double *out);
+
 /* --- consumer: Gamma-point eigenvalues ------------------------------------
  * Folds the blocks of structure istruct at k = 0 into a dense binary64 matrix
  * (build_Hmatrix, h.SOLVESH/M_kspace_Lowdin.f90:180-271, 630-724) -- which = 0:

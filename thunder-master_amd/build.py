@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "fireball_hip", "libfbhip.so")
 SOURCES = ["fb_api.cpp", "fb_fdata.cpp", "fb_lists.hip", "fb_2c.hip", "fb_3c.hip", "fb_3c_mixed.hip", "fb_gamma.hip",
-           "fb_mlp.hip", "fb_forces.hip", "fb_rho.hip"]
+           "fb_mlp.hip", "fb_forces.hip", "fb_rho.hip", "fb_pp.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

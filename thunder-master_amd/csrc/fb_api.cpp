@@ -19,6 +19,7 @@
 
 #include "fb_internal.h"
 #include "fb_forces.h"
+#include "fb_pp.h"
 #include "fb_mlp.h"
 #include "fb_fdata.h"
 #include <array>
@@ -138,6 +139,7 @@ int slot_of(int itype) {
     case 19: return SLOT_RHOS_ONL;
     case 20: return SLOT_RHOS_ONR;
     case 21: return SLOT_RHOS_ATOM;
+    case 6: return SLOT_VNL;
   }
   return -1;
 }
@@ -248,6 +250,56 @@ struct fb_ctx {
   int64_t wlen = 0;
   long rho_gen = -1;             // lists_gen the density buffers belong to
   int rdone[4] = {0, 0, 0, 0};   // rho_2c, rho_3c, rho_weighted_2c, rho_weighted_3c
+
+  // pseudopotential: PP shells of the species, driver_neighbors_PP lists, sVNL / vnl blocks
+  bool pp_species = false;
+  DPP hpp{};                     // species part; pview() adds the device pointers
+  DevBuf d_ppn[3], d_ppoff[3], d_ppj[3], d_ppb[3], d_ppatom[3], d_ppself[3];
+  DevBuf d_pp_map, d_px_map, d_px_point, d_inv_pp, d_inv_px, d_cand_cnt, d_cand_off, d_cand_j, d_cand_b;
+  DevBuf d_pc_n, d_pc_off, d_pc_trip, d_pc_tpair, d_pc_trip_s, d_pc_tpair_s, d_pc_perm_in, d_pc_perm, d_pc_flag,
+      d_pc_segincl, d_pc_seg_off, d_pc_seg_pair, d_pc_trip_seg, d_pc_tgt;
+  DevBuf d_sv_bsz, d_sv_off, d_vl_bsz, d_vl_off, d_svnl, d_vnl;
+  int64_t pp_npairs[3] = {0, 0, 0}, pp_ntrip = 0, pp_nseg = 0, sv_len = 0, vl_len = 0;
+  std::vector<int64_t> h_ppoff[3], h_pcoff, h_sv_struct, h_vl_struct;
+  bool pp_lists_ok = false;
+  int ppdone[3] = {0, 0, 0};     // svnl, vnl_2c, vnl_3c
+
+  DPP pview() const {
+    DPP q = hpp;
+    for (int l = 0; l < 3; l++) {
+      q.n[l] = d_ppn[l].as<int>();
+      q.off[l] = d_ppoff[l].as<int64_t>();
+      q.j[l] = d_ppj[l].as<int>();
+      q.b[l] = d_ppb[l].as<int>();
+      q.atom[l] = d_ppatom[l].as<int>();
+      q.self[l] = d_ppself[l].as<int>();
+      q.npairs[l] = pp_npairs[l];
+    }
+    q.pp_map = d_pp_map.as<int>();
+    q.px_map = d_px_map.as<int>();
+    q.px_point = d_px_point.as<int>();
+    q.inv_pp = d_inv_pp.as<int>();
+    q.inv_px = d_inv_px.as<int>();
+    q.cand_off = d_cand_off.as<int64_t>();
+    q.cand_j = d_cand_j.as<int>();
+    q.cand_b = d_cand_b.as<int>();
+    q.pc_n = d_pc_n.as<int>();
+    q.pc_off = d_pc_off.as<int64_t>();
+    q.trip = d_pc_trip.as<int4>();
+    q.tpair = d_pc_tpair.as<int>();
+    q.trip_s = d_pc_trip_s.as<int4>();
+    q.tpair_s = d_pc_tpair_s.as<int>();
+    q.seg_off = d_pc_seg_off.as<int64_t>();
+    q.seg_pair = d_pc_seg_pair.as<int>();
+    q.trip_seg = d_pc_trip_seg.as<int>();
+    q.ntrip = pp_ntrip;
+    q.nseg = pp_nseg;
+    q.sv_off = d_sv_off.as<int64_t>();
+    q.vl_off = d_vl_off.as<int64_t>();
+    q.svnl = d_svnl.as<double>();
+    q.vnl = d_vnl.as<double>();
+    return q;
+  }
   std::vector<char> rho_ok;      // per structure: density blocks uploaded
   int ddone[5] = {0, 0, 0, 0, 0};  // Dblock S, T, dipole_z; vna_2c forces; vna_3c forces
   int force_kernel = 1;            // s/p species: 1 register-resident kernels, 0 the generic ones
@@ -458,9 +510,15 @@ struct fb_ctx {
                       &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
                       &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_pair_tgt, &d_trip_aux, &d_tiles[0], &d_tiles[1], &d_tiles[2], &d_tmp,
                       &d_mlp, &d_mlp_err, &d_mlp_count, &d_meth3c,
-                      &d_status, &d_idx3r, &d_qin, &d_wbsz, &d_wboff};
+                      &d_status, &d_idx3r, &d_qin, &d_wbsz, &d_wboff, &d_pp_map, &d_px_map, &d_px_point,
+                      &d_inv_pp, &d_inv_px, &d_cand_cnt, &d_cand_off, &d_cand_j, &d_cand_b, &d_pc_n, &d_pc_off,
+                      &d_pc_trip, &d_pc_tpair, &d_pc_trip_s, &d_pc_tpair_s, &d_pc_perm_in, &d_pc_perm, &d_pc_flag,
+                      &d_pc_segincl, &d_pc_seg_off, &d_pc_seg_pair, &d_pc_trip_seg, &d_pc_tgt, &d_sv_bsz, &d_sv_off,
+                      &d_vl_bsz, &d_vl_off, &d_svnl, &d_vnl};
     for (DevBuf *b : bufs) b->release();
     for (auto &m : d_mat) m.release();
+    for (int l = 0; l < 3; l++)
+      for (DevBuf *b : {&d_ppn[l], &d_ppoff[l], &d_ppj[l], &d_ppb[l], &d_ppatom[l], &d_ppself[l]}) b->release();
     DevBuf *fbufs[] = {&d_rho, &d_dmat[0], &d_dmat[1], &d_dmat[2], &d_fatom, &d_fontop, &d_f3a, &d_f3b, &d_f3c,
                        &d_f3, &d_jkey, &d_jkey_s, &d_iota_t, &d_perm_j, &d_inv_perm, &d_inv_perm_j, &d_ioff, &d_joff};
     for (DevBuf *b : fbufs) b->release();
@@ -606,6 +664,7 @@ int fb_set_species(fb_ctx *ctx, int ns, const int *nZ, const int *nssh, const in
   ctx->species_set = true;
   ctx->tables_ok = false;
   ctx->lists_ok = false;
+  ctx->pp_lists_ok = false;
   return FB_OK;
 }
 
@@ -689,6 +748,15 @@ int fb_read_fdata(fb_ctx *ctx, const char *fdata_dir) {
   FdataWanted want{fdata_type2c, fdata_type3c};
   int rc = fb_fdata_read(fdata_dir, ctx->sp.ns, ctx->nZ, want, sink, err);
   if (rc == -1) return ctx->fail(FB_E_NO_TABLE, ("read_fdata: " + err).c_str());
+  if (rc == 0 && ctx->pp_species) {
+    double cl[FB_MAXSPECIES * FB_MAXSH] = {0.0};
+    int nsh[FB_MAXSPECIES] = {0};
+    for (int i = 0; i < ctx->sp.ns; i++) nsh[i] = ctx->hpp.nssh_pp[i];
+    if (fb_fdata_read_cl(fdata_dir, ctx->sp.ns, ctx->nZ, nsh, cl, err))
+      return ctx->fail(FB_E_NO_TABLE, ("read_fdata: " + err).c_str());
+    for (int i = 0; i < ctx->sp.ns; i++)
+      for (int k = 0; k < nsh[i]; k++) ctx->hpp.cl[i][k] = cl[i * FB_MAXSH + k];
+  }
   return rc;
 }
 
@@ -703,6 +771,7 @@ int fb_commit_tables(fb_ctx *ctx) {
   for (auto &c : ctx->c2) {
     int slot = slot_of(c.itype);
     if (slot < 0) continue;  // interactions outside the assembled path
+    if (slot == SLOT_VNL && !ctx->pp_species) continue;   // projector tables need the PP shells (fb_set_species_pp)
     {
       // the block of an atom term lives on the first atom's orbitals (make_munu_atom);
       // the shell-averaged density tables index shells (make_munuS, make_munuS_atom)
@@ -710,6 +779,7 @@ int fb_commit_tables(fb_ctx *ctx) {
       const bool atom = (slot == SLOT_ATOM || slot == SLOT_RHO_ATOM || slot == SLOT_RHOS_ATOM);
       const int *dim = shell ? ctx->sp.nssh : ctx->sp.norb;
       int rows = dim[c.is - 1], cols = atom ? rows : dim[c.js - 1];
+      if (slot == SLOT_VNL) cols = ctx->hpp.norb_pp[c.js - 1];   // PP projectors of the second species
       if (c.nME > 16) return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: 2c cell with more than 16 elements");
       for (int k = 0; k < c.nME; k++)
         if (c.mu[k] < 1 || c.mu[k] > rows || c.nu[k] < 1 || c.nu[k] > cols)
@@ -920,7 +990,7 @@ int fb_commit_tables(fb_ctx *ctx) {
     if (cr) {
       for (auto &c : ctx->c2) {
         int slot = slot_of(c.itype);
-        if (slot < SLOT_RHO_ONL) continue;
+        if (slot < SLOT_RHO_ONL || slot > SLOT_RHOS_ATOM) continue;
         const int si = (int)shells[c.is - 1].size(), sj = (int)shells[c.js - 1].size();
         MeList e = slot == SLOT_RHO_ATOM ? munu2c(shells[c.is - 1], shells[c.is - 1])
                  : slot == SLOT_RHOS_ATOM ? munuS(si, si)
@@ -972,6 +1042,7 @@ int fb_clear_structures(fb_ctx *ctx) {
   ctx->structs.clear();
   ctx->structs_dirty = true;
   ctx->lists_ok = false;
+  ctx->pp_lists_ok = false;
   return FB_OK;
 }
 
@@ -993,6 +1064,7 @@ int fb_add_structure(fb_ctx *ctx, int natoms, const double *ratom, const int *im
   if (istruct) *istruct = (int)ctx->structs.size() - 1;
   ctx->structs_dirty = true;
   ctx->lists_ok = false;
+  ctx->pp_lists_ok = false;
   return FB_OK;
 }
 
@@ -1002,6 +1074,7 @@ int fb_set_positions(fb_ctx *ctx, int is, const double *ratom) {
   std::memcpy(s.ratom.data(), ratom, sizeof(double) * 3 * s.natoms);
   ctx->structs_dirty = true;
   ctx->lists_ok = false;
+  ctx->pp_lists_ok = false;
   return FB_OK;
 }
 
@@ -2077,6 +2150,305 @@ int fb_get_shell_block_len(fb_ctx *ctx, int is, int64_t *len) {
   }
   *len = ctx->h_wboff_struct[is + 1] - ctx->h_wboff_struct[is];
   return FB_OK;
+}
+
+// ---------------------------------------------------------- pseudopotential
+extern "C" int fbk_fill_i32(int *a, int64_t n, int val, hipStream_t st);
+
+int fb_set_species_pp(fb_ctx *ctx, const int *nssh_pp, const int *lssh_pp, const double *rcutoff_pp,
+                      const double *cl) {
+  if (!ctx || !nssh_pp || !lssh_pp || !rcutoff_pp || !cl) return ctx ? ctx->fail(FB_E_ARG, "set_species_pp") : FB_E_ARG;
+  if (!ctx->species_set) return ctx->fail(FB_E_ORDER, "set_species_pp before set_species");
+  DPP h{};
+  for (int i = 0; i < ctx->sp.ns; i++) {
+    if (nssh_pp[i] < 0 || nssh_pp[i] > FB_MAXSH) return ctx->fail(FB_E_ARG, "set_species_pp: nssh_PP out of range");
+    h.nssh_pp[i] = nssh_pp[i];
+    for (int k = 0; k < nssh_pp[i]; k++) {
+      const int l = lssh_pp[i * FB_MAXSH + k];
+      if (l < 0 || l > 1) return ctx->fail(FB_E_UNSUPPORTED, "set_species_pp: PP shells beyond s and p");
+      h.lssh_pp[i][k] = l;
+      h.cl[i][k] = cl[i * FB_MAXSH + k];
+      h.norb_pp[i] += 2 * l + 1;
+    }
+    if (h.norb_pp[i] > 4) return ctx->fail(FB_E_UNSUPPORTED, "set_species_pp: more than 4 projector orbitals");
+    h.rcpp[i] = rcutoff_pp[i];
+  }
+  ctx->hpp = h;
+  ctx->pp_species = true;
+  ctx->pp_lists_ok = false;
+  ctx->tables_ok = false;   // the projector tables are checked against the PP shells at commit
+  return FB_OK;
+}
+
+static int pp_list_pass(fb_ctx *ctx, int list) {
+  const int N = ctx->natoms;
+  hipStream_t st = ctx->stream;
+  FB_HIP(ctx->d_ppn[list].ensure(sizeof(int) * std::max(N, 1)));
+  FB_HIP(ctx->d_ppoff[list].ensure(sizeof(int64_t) * (N + 1)));
+  FB_HIP(ctx->d_ppself[list].ensure(sizeof(int) * std::max(N, 1)));
+  DView v = ctx->view();
+  DPP q = ctx->pview();
+  RC(ctx->launch("pp_neighbors_count", [&] { return fbk_pp_neighbors(v, q, list, 0, v.status, st); }));
+  RC(scan_i32(ctx, ctx->d_ppn[list].as<int>(), ctx->d_ppoff[list].as<int64_t>(), N));
+  ctx->h_ppoff[list].resize(N + 1);
+  FB_HIP(hipMemcpyAsync(ctx->h_ppoff[list].data(), ctx->d_ppoff[list].p, sizeof(int64_t) * (N + 1),
+                        hipMemcpyDeviceToHost, st));
+  FB_HIP(hipStreamSynchronize(st));
+  ctx->pp_npairs[list] = ctx->h_ppoff[list][N];
+  const int64_t P = std::max<int64_t>(ctx->pp_npairs[list], 1);
+  FB_HIP(ctx->d_ppj[list].ensure(sizeof(int) * P));
+  FB_HIP(ctx->d_ppb[list].ensure(sizeof(int) * P));
+  FB_HIP(ctx->d_ppatom[list].ensure(sizeof(int) * P));
+  RC(ctx->launch("fill_self", [&] { return fbk_fill_i32(ctx->d_ppself[list].as<int>(), N, -999, st); }));
+  q = ctx->pview();
+  RC(ctx->launch("pp_neighbors_fill", [&] { return fbk_pp_neighbors(v, q, list, 1, v.status, st); }));
+  std::vector<int> self(N);
+  FB_HIP(hipMemcpyAsync(self.data(), ctx->d_ppself[list].p, sizeof(int) * N, hipMemcpyDeviceToHost, st));
+  FB_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < N; i++)
+    if (self[i] == -999) return ctx->fail(FB_E_NEIGH_SELF, "driver_neighbors_PP: self entry missing");
+  return FB_OK;
+}
+
+int fb_build_lists_pp(fb_ctx *ctx, int64_t *npp, int64_t *nppp, int64_t *ntrip) {
+  if (!ctx) return FB_E_ARG;
+  if (!ctx->tables_ok) return ctx->fail(FB_E_ORDER, "build_lists_pp before commit_tables");
+  if (!ctx->pp_species) return ctx->fail(FB_E_ORDER, "build_lists_pp without PP shells (fb_set_species_pp)");
+  if (ctx->structs.empty()) return ctx->fail(FB_E_ORDER, "build_lists_pp without structures");
+  FB_HIP(hipSetDevice(ctx->device));
+  RC(ctx->settle("previous assemble_all"));
+  RC(upload_structures(ctx));
+  RC(ctx->clear_status());
+  const int N = ctx->natoms;
+  hipStream_t st = ctx->stream;
+  RC(pp_list_pass(ctx, PPL_PP));
+  RC(pp_list_pass(ctx, PPL_PPX));
+  // neighbors_PPp: candidate ranges, first-seen dedup, copy
+  FB_HIP(ctx->d_cand_cnt.ensure(sizeof(int64_t) * std::max(N, 1)));
+  FB_HIP(ctx->d_cand_off.ensure(sizeof(int64_t) * (N + 1)));
+  DView v = ctx->view();
+  DPP q = ctx->pview();
+  RC(ctx->launch("ppp_candidates", [&] { return fbk_ppp_candidates(v, q, ctx->d_cand_cnt.as<int64_t>(), st); }));
+  RC(scan_i64(ctx, ctx->d_cand_cnt.as<int64_t>(), ctx->d_cand_off.as<int64_t>(), N));
+  int64_t ncand = 0;
+  FB_HIP(hipMemcpyAsync(&ncand, ctx->d_cand_off.as<int64_t>() + N, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  FB_HIP(hipStreamSynchronize(st));
+  FB_HIP(ctx->d_cand_j.ensure(sizeof(int) * std::max<int64_t>(ncand, 1)));
+  FB_HIP(ctx->d_cand_b.ensure(sizeof(int) * std::max<int64_t>(ncand, 1)));
+  FB_HIP(ctx->d_ppn[PPL_PPP].ensure(sizeof(int) * std::max(N, 1)));
+  FB_HIP(ctx->d_ppoff[PPL_PPP].ensure(sizeof(int64_t) * (N + 1)));
+  FB_HIP(ctx->d_ppself[PPL_PPP].ensure(sizeof(int) * std::max(N, 1)));
+  q = ctx->pview();
+  RC(ctx->launch("ppp_dedup", [&] { return fbk_ppp_build(v, q, 0, v.status, st); }));
+  RC(scan_i32(ctx, ctx->d_ppn[PPL_PPP].as<int>(), ctx->d_ppoff[PPL_PPP].as<int64_t>(), N));
+  ctx->h_ppoff[PPL_PPP].resize(N + 1);
+  FB_HIP(hipMemcpyAsync(ctx->h_ppoff[PPL_PPP].data(), ctx->d_ppoff[PPL_PPP].p, sizeof(int64_t) * (N + 1),
+                        hipMemcpyDeviceToHost, st));
+  FB_HIP(hipStreamSynchronize(st));
+  ctx->pp_npairs[PPL_PPP] = ctx->h_ppoff[PPL_PPP][N];
+  const int64_t Q = std::max<int64_t>(ctx->pp_npairs[PPL_PPP], 1);
+  FB_HIP(ctx->d_ppj[PPL_PPP].ensure(sizeof(int) * Q));
+  FB_HIP(ctx->d_ppb[PPL_PPP].ensure(sizeof(int) * Q));
+  FB_HIP(ctx->d_ppatom[PPL_PPP].ensure(sizeof(int) * Q));
+  q = ctx->pview();
+  RC(ctx->launch("ppp_fill", [&] { return fbk_ppp_build(v, q, 1, v.status, st); }));
+  RC(ctx->check_status("driver_neighbors_PP (PPp)"));
+  // maps, inverse maps and points
+  FB_HIP(ctx->d_pp_map.ensure(sizeof(int) * std::max<int64_t>(ctx->pp_npairs[PPL_PP], 1)));
+  FB_HIP(ctx->d_px_map.ensure(sizeof(int) * std::max<int64_t>(ctx->pp_npairs[PPL_PPX], 1)));
+  FB_HIP(ctx->d_px_point.ensure(sizeof(int) * std::max<int64_t>(ctx->pp_npairs[PPL_PPX], 1)));
+  FB_HIP(ctx->d_inv_pp.ensure(sizeof(int) * Q));
+  FB_HIP(ctx->d_inv_px.ensure(sizeof(int) * Q));
+  RC(ctx->launch("fill_inv", [&] { return fbk_fill_i32(ctx->d_inv_pp.as<int>(), Q, -1, st); }));
+  RC(ctx->launch("fill_inv", [&] { return fbk_fill_i32(ctx->d_inv_px.as<int>(), Q, -1, st); }));
+  q = ctx->pview();
+  RC(ctx->launch("pp_maps", [&] { return fbk_pp_maps(v, q, v.status, st); }));
+  RC(ctx->check_status("driver_neighbors_PP (maps)"));
+  // PP common neighbours
+  FB_HIP(ctx->d_pc_n.ensure(sizeof(int) * std::max(N, 1)));
+  FB_HIP(ctx->d_pc_off.ensure(sizeof(int64_t) * (N + 1)));
+  q = ctx->pview();
+  RC(ctx->launch("pp_common_count", [&] { return fbk_pp_common(v, q, 0, st); }));
+  RC(scan_i32(ctx, ctx->d_pc_n.as<int>(), ctx->d_pc_off.as<int64_t>(), N));
+  ctx->h_pcoff.resize(N + 1);
+  FB_HIP(hipMemcpyAsync(ctx->h_pcoff.data(), ctx->d_pc_off.p, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost, st));
+  FB_HIP(hipStreamSynchronize(st));
+  const int64_t T = ctx->h_pcoff[N];
+  if (T >= (1ll << 31) || Q >= (1ll << 31)) return ctx->fail(FB_E_CAPACITY, "more than 2^31 PP triplets or pairs");
+  ctx->pp_ntrip = T;
+  const int64_t T1 = std::max<int64_t>(T, 1);
+  FB_HIP(ctx->d_pc_trip.ensure(sizeof(int4) * T1));
+  FB_HIP(ctx->d_pc_tpair.ensure(sizeof(int) * T1));
+  FB_HIP(ctx->d_pc_trip_s.ensure(sizeof(int4) * T1));
+  FB_HIP(ctx->d_pc_tpair_s.ensure(sizeof(int) * T1));
+  FB_HIP(ctx->d_pc_perm_in.ensure(sizeof(int) * T1));
+  FB_HIP(ctx->d_pc_perm.ensure(sizeof(int) * T1));
+  FB_HIP(ctx->d_pc_flag.ensure(sizeof(int) * T1));
+  FB_HIP(ctx->d_pc_segincl.ensure(sizeof(int) * T1));
+  FB_HIP(ctx->d_pc_seg_off.ensure(sizeof(int64_t) * (T1 + 1)));
+  FB_HIP(ctx->d_pc_seg_pair.ensure(sizeof(int) * T1));
+  FB_HIP(ctx->d_pc_trip_seg.ensure(sizeof(int) * T1));
+  FB_HIP(ctx->d_pc_tgt.ensure(sizeof(int) * Q));
+  q = ctx->pview();
+  RC(ctx->launch("pp_common_fill", [&] { return fbk_pp_common(v, q, 1, st); }));
+  ctx->pp_nseg = 0;
+  if (T > 0) {
+    // stable sort of the triplets by target PPp pair: segments in list order
+    int end_bit = 1;
+    while ((1ll << end_bit) <= Q) end_bit++;
+    RC(ctx->launch("iota", [&] { return fbk_iota(ctx->d_pc_perm_in.as<int>(), T, st); }));
+    size_t bytes = 0, b2 = 0;
+    int rc = fbk_sort_pairs(ctx->d_pc_tpair.as<int>(), ctx->d_pc_tpair_s.as<int>(), ctx->d_pc_perm_in.as<int>(),
+                            ctx->d_pc_perm.as<int>(), T, end_bit, nullptr, &bytes, st);
+    if (rc) return ctx->fail(FB_E_HIP, "sort sizing", (hipError_t)rc);
+    rc = fbk_scan_incl_i32(ctx->d_pc_flag.as<int>(), ctx->d_pc_segincl.as<int>(), T, nullptr, &b2, st);
+    if (rc) return ctx->fail(FB_E_HIP, "scan sizing", (hipError_t)rc);
+    RC(ensure_tmp(ctx, std::max(bytes, b2)));
+    bytes = ctx->d_tmp.cap;
+    RC(ctx->launch("pp_sort_triplets", [&] {
+      return fbk_sort_pairs(ctx->d_pc_tpair.as<int>(), ctx->d_pc_tpair_s.as<int>(), ctx->d_pc_perm_in.as<int>(),
+                            ctx->d_pc_perm.as<int>(), T, end_bit, ctx->d_tmp.p, &bytes, st);
+    }));
+    RC(ctx->launch("seg_mark", [&] { return fbk_seg_mark(T, ctx->d_pc_tpair_s.as<int>(), ctx->d_pc_flag.as<int>(), st); }));
+    b2 = ctx->d_tmp.cap;
+    RC(ctx->launch("seg_scan", [&] {
+      return fbk_scan_incl_i32(ctx->d_pc_flag.as<int>(), ctx->d_pc_segincl.as<int>(), T, ctx->d_tmp.p, &b2, st);
+    }));
+    int nseg = 0;
+    FB_HIP(hipMemcpyAsync(&nseg, ctx->d_pc_segincl.as<int>() + (T - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+    RC(ctx->launch("seg_fill", [&] {
+      return fbk_seg_fill(T, ctx->d_pc_tpair_s.as<int>(), ctx->d_pc_segincl.as<int>(), ctx->d_pc_seg_off.as<int64_t>(),
+                          ctx->d_pc_seg_pair.as<int>(), ctx->d_pc_trip_seg.as<int>(), ctx->d_pc_tgt.as<int>(), st);
+    }));
+    RC(ctx->launch("gather_triplets", [&] {
+      return fbk_gather_trip(T, ctx->d_pc_perm.as<int>(), ctx->d_pc_trip.as<int4>(), ctx->d_pc_trip_s.as<int4>(), st);
+    }));
+    FB_HIP(hipStreamSynchronize(st));
+    ctx->pp_nseg = nseg;
+  }
+  // block offsets: sVNL per PP pair, vnl per PPp pair
+  const int64_t P = std::max<int64_t>(ctx->pp_npairs[PPL_PP], 1);
+  FB_HIP(ctx->d_sv_bsz.ensure(sizeof(int64_t) * P));
+  FB_HIP(ctx->d_sv_off.ensure(sizeof(int64_t) * (P + 1)));
+  FB_HIP(ctx->d_vl_bsz.ensure(sizeof(int64_t) * Q));
+  FB_HIP(ctx->d_vl_off.ensure(sizeof(int64_t) * (Q + 1)));
+  q = ctx->pview();
+  RC(ctx->launch("pp_block_sizes", [&] { return fbk_pp_block_sizes(v, q, 0, ctx->d_sv_bsz.as<int64_t>(), st); }));
+  RC(scan_i64(ctx, ctx->d_sv_bsz.as<int64_t>(), ctx->d_sv_off.as<int64_t>(), ctx->pp_npairs[PPL_PP]));
+  RC(ctx->launch("pp_block_sizes", [&] { return fbk_pp_block_sizes(v, q, 1, ctx->d_vl_bsz.as<int64_t>(), st); }));
+  RC(scan_i64(ctx, ctx->d_vl_bsz.as<int64_t>(), ctx->d_vl_off.as<int64_t>(), ctx->pp_npairs[PPL_PPP]));
+  FB_HIP(hipMemcpyAsync(&ctx->sv_len, ctx->d_sv_off.as<int64_t>() + ctx->pp_npairs[PPL_PP], sizeof(int64_t),
+                        hipMemcpyDeviceToHost, st));
+  FB_HIP(hipMemcpyAsync(&ctx->vl_len, ctx->d_vl_off.as<int64_t>() + ctx->pp_npairs[PPL_PPP], sizeof(int64_t),
+                        hipMemcpyDeviceToHost, st));
+  FB_HIP(hipStreamSynchronize(st));
+  FB_HIP(ctx->d_svnl.ensure(sizeof(double) * std::max<int64_t>(ctx->sv_len, 1)));
+  FB_HIP(ctx->d_vnl.ensure(sizeof(double) * std::max<int64_t>(ctx->vl_len, 1)));
+  const size_t ns = ctx->structs.size();
+  ctx->h_sv_struct.assign(ns + 1, 0);
+  ctx->h_vl_struct.assign(ns + 1, 0);
+  for (size_t k = 0; k < ns; k++) {
+    const int a0 = ctx->h_st[k].atom_off;
+    FB_HIP(hipMemcpy(&ctx->h_sv_struct[k], ctx->d_sv_off.as<int64_t>() + ctx->h_ppoff[PPL_PP][a0], sizeof(int64_t),
+                     hipMemcpyDeviceToHost));
+    FB_HIP(hipMemcpy(&ctx->h_vl_struct[k], ctx->d_vl_off.as<int64_t>() + ctx->h_ppoff[PPL_PPP][a0], sizeof(int64_t),
+                     hipMemcpyDeviceToHost));
+  }
+  ctx->h_sv_struct[ns] = ctx->sv_len;
+  ctx->h_vl_struct[ns] = ctx->vl_len;
+  RC(ctx->check_status("driver_neighbors_PP"));
+  for (int i = 0; i < 3; i++) ctx->ppdone[i] = 0;
+  ctx->pp_lists_ok = true;
+  if (npp) *npp = ctx->pp_npairs[PPL_PP];
+  if (nppp) *nppp = ctx->pp_npairs[PPL_PPP];
+  if (ntrip) *ntrip = T;
+  return FB_OK;
+}
+
+int fb_get_pp_list_sizes(fb_ctx *ctx, int is, int64_t *sizes) {
+  if (!ctx || !ctx->pp_lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "PP lists not built") : FB_E_ARG;
+  if (is < 0 || is >= (int)ctx->structs.size() || !sizes) return ctx->fail(FB_E_ARG, "structure index");
+  const int a0 = ctx->h_st[is].atom_off, a1 = a0 + ctx->h_st[is].natoms;
+  for (int l = 0; l < 3; l++) sizes[l] = ctx->h_ppoff[l][a1] - ctx->h_ppoff[l][a0];
+  sizes[3] = ctx->h_pcoff[a1] - ctx->h_pcoff[a0];
+  sizes[4] = ctx->h_sv_struct[is + 1] - ctx->h_sv_struct[is];
+  sizes[5] = ctx->h_vl_struct[is + 1] - ctx->h_vl_struct[is];
+  return FB_OK;
+}
+
+int fb_get_pp_lists(fb_ctx *ctx, int is, int which, int *n, int *a0, int *a1, int *a2, int *a3, int *a4, int *self) {
+  if (!ctx || !ctx->pp_lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "PP lists not built") : FB_E_ARG;
+  if (is < 0 || is >= (int)ctx->structs.size() || which < 0 || which > 3 || !n)
+    return ctx->fail(FB_E_ARG, "get_pp_lists arguments");
+  FB_HIP(hipSetDevice(ctx->device));
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  const int at0 = ctx->h_st[is].atom_off, na = ctx->h_st[is].natoms;
+  if (which < 3) {
+    const int64_t p0 = ctx->h_ppoff[which][at0], p1 = ctx->h_ppoff[which][at0 + na];
+    RC(d2h(ctx, n, ctx->d_ppn[which].as<int>() + at0, sizeof(int) * na));
+    if (a0) RC(d2h(ctx, a0, ctx->d_ppj[which].as<int>() + p0, sizeof(int) * (p1 - p0)));
+    if (a1) RC(d2h(ctx, a1, ctx->d_ppb[which].as<int>() + p0, sizeof(int) * (p1 - p0)));
+    if (a2 && which == PPL_PP) RC(d2h(ctx, a2, ctx->d_pp_map.as<int>() + p0, sizeof(int) * (p1 - p0)));
+    if (a2 && which == PPL_PPX) RC(d2h(ctx, a2, ctx->d_px_map.as<int>() + p0, sizeof(int) * (p1 - p0)));
+    if (a3 && which == PPL_PPX) RC(d2h(ctx, a3, ctx->d_px_point.as<int>() + p0, sizeof(int) * (p1 - p0)));
+    if (self) RC(d2h(ctx, self, ctx->d_ppself[which].as<int>() + at0, sizeof(int) * na));
+    return FB_OK;
+  }
+  // common neighbours: local atom numbers and mneigh (index into iatom's PPp list)
+  const int64_t t0 = ctx->h_pcoff[at0], t1 = ctx->h_pcoff[at0 + na];
+  RC(d2h(ctx, n, ctx->d_pc_n.as<int>() + at0, sizeof(int) * na));
+  std::vector<int4> tr(t1 - t0);
+  std::vector<int> tp(t1 - t0);
+  if (t1 > t0) {
+    RC(d2h(ctx, tr.data(), ctx->d_pc_trip.as<int4>() + t0, sizeof(int4) * (t1 - t0)));
+    RC(d2h(ctx, tp.data(), ctx->d_pc_tpair.as<int>() + t0, sizeof(int) * (t1 - t0)));
+  }
+  for (int64_t t = 0; t < t1 - t0; t++) {
+    const int gia = tr[t].y, gja = tr[t].z;
+    if (a0) a0[t] = gia - at0 + 1;
+    if (a1) a1[t] = tr[t].w & 0xffff;
+    if (a2) a2[t] = gja - at0 + 1;
+    if (a3) a3[t] = (tr[t].w >> 16) & 0xffff;
+    if (a4) a4[t] = (int)(tp[t] - ctx->h_ppoff[PPL_PPP][gia] + 1);
+  }
+  return FB_OK;
+}
+
+static int pp_step(fb_ctx *ctx, int step) {
+  static const char *names[3] = {"assemble_svnl", "assemble_vnl_2c", "assemble_vnl_3c"};
+  if (!ctx) return FB_E_ARG;
+  if (!ctx->pp_lists_ok) return ctx->fail(FB_E_ORDER, (std::string(names[step]) + " before build_lists_pp").c_str());
+  if (step > 0 && !ctx->ppdone[step - 1])
+    return ctx->fail(FB_E_ORDER, (std::string(names[step]) + " before " + names[step - 1]).c_str());
+  FB_HIP(hipSetDevice(ctx->device));
+  RC(ctx->settle("previous assemble_all"));
+  if (ctx->structs_dirty) RC(upload_structures(ctx));
+  RC(ctx->clear_status());
+  DView v = ctx->view();
+  DPP q = ctx->pview();
+  if (step == 0) RC(ctx->launch(names[0], [&] { return fbk_svnl(v, q, ctx->stream); }));
+  if (step == 1) RC(ctx->launch(names[1], [&] { return fbk_vnl_2c(v, q, ctx->stream); }));
+  if (step == 2) RC(ctx->launch(names[2], [&] { return fbk_vnl_3c(v, q, ctx->stream); }));
+  RC(ctx->check_status(names[step]));
+  ctx->ppdone[step] = 1;
+  for (int k = step + 1; k < 3; k++) ctx->ppdone[k] = 0;
+  return FB_OK;
+}
+
+int fb_assemble_svnl(fb_ctx *ctx) { return pp_step(ctx, 0); }
+int fb_assemble_vnl_2c(fb_ctx *ctx) { return pp_step(ctx, 1); }
+int fb_assemble_vnl_3c(fb_ctx *ctx) { return pp_step(ctx, 2); }
+
+int fb_get_pp_blocks(fb_ctx *ctx, int is, int which, double *out) {
+  if (!ctx || !ctx->pp_lists_ok) return ctx ? ctx->fail(FB_E_ORDER, "PP lists not built") : FB_E_ARG;
+  if (is < 0 || is >= (int)ctx->structs.size() || which < 0 || which > 1 || !out)
+    return ctx->fail(FB_E_ARG, "get_pp_blocks arguments");
+  if (!ctx->ppdone[which]) return ctx->fail(FB_E_ORDER, "get_pp_blocks: not assembled");
+  FB_HIP(hipSetDevice(ctx->device));
+  FB_HIP(hipStreamSynchronize(ctx->stream));
+  const std::vector<int64_t> &o = which == 0 ? ctx->h_sv_struct : ctx->h_vl_struct;
+  const double *src = (which == 0 ? ctx->d_svnl : ctx->d_vnl).as<double>();
+  return d2h(ctx, out, src + o[is], sizeof(double) * (o[is + 1] - o[is]));
 }
 
 // ------------------------------------------------------------------ forces

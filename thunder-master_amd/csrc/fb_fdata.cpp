@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "fb_fdata.h"
+#include "../../include/fireball_hip.h"
 
 namespace {
 
@@ -296,5 +297,27 @@ int fb_fdata_read(const std::string &dir, int ns, const int *nZ, const FdataWant
         }
       }
     }
+  return 0;
+}
+
+// Kleinman-Bylander coefficients: clPP.ZZ.dat holds one value per PP shell
+// (e.FDATA/M_Fdata_2c.f90:251-258)
+int fb_fdata_read_cl(const std::string &dir, int ns, const int *nZ, const int *nssh_pp, double *cl,
+                     std::string &err) {
+  for (int a = 0; a < ns; a++) {
+    if (nssh_pp[a] == 0) continue;
+    ListReader r;
+    const std::string path = dir + zname("/clPP.%02d.dat", nZ[a], 0);
+    if (!r.open(path)) {
+      err = "cannot open " + path;
+      return -1;
+    }
+    std::vector<std::string> t;
+    for (int k = 0; k < nssh_pp[a]; k++)
+      if (!r.values(1, t) || !to_double(t[0], cl[a * FB_MAXSH + k])) {
+        err = "short or bad record in " + path;
+        return -1;
+      }
+  }
   return 0;
 }

@@ -25,3 +25,5 @@ struct FdataSink {
 // sink's return code)
 int fb_fdata_read(const std::string &dir, int ns, const int *nZ, const FdataWanted &want, FdataSink &sink,
                   std::string &err);
+int fb_fdata_read_cl(const std::string &dir, int ns, const int *nZ, const int *nssh_pp, double *cl,
+                     std::string &err);

@@ -10,7 +10,7 @@
 #define FB_NTHETA 5           // include/interactions_3c.h:4
 #define FB_MAXSUB2 10         // include/interactions_2c.h:3
 #define FB_MAXSUB3 7          // include/interactions_3c.h:3
-#define FB_NSLOT2 12          // 2c interactions used by the assemblers
+#define FB_NSLOT2 13          // 2c interactions used by the assemblers
 #define FB_P_EQ2 14.39975     // include/constants.h:4
 #define FB_2C_PCAP 256        // pairs per 2c workgroup pass
 #define FB_2C_SKIP_TGT 16     // 2c mask bit: leave the Vna blocks of 3c targets to the 3c launch
@@ -18,10 +18,12 @@
 // 2c interaction slots (device-side dense index of the interaction ids of
 // include/interactions_2c.h: overlap 1, kinetic 2, vna_ontopL 3, vna_ontopR 4,
 // vna_atom 5, dipole_z 10; McWEDA densities rho_ontopL 15, rho_ontopR 16,
-// rho_atom 17, rhoS_ontopL 19, rhoS_ontopR 20, rhoS_atom 21)
+// rho_atom 17, rhoS_ontopL 19, rhoS_ontopR 20, rhoS_atom 21; non-local
+// pseudopotential vnl 6)
 enum {
   SLOT_S = 0, SLOT_T = 1, SLOT_ONL = 2, SLOT_ONR = 3, SLOT_ATOM = 4, SLOT_DIP = 5,
-  SLOT_RHO_ONL = 6, SLOT_RHO_ONR = 7, SLOT_RHO_ATOM = 8, SLOT_RHOS_ONL = 9, SLOT_RHOS_ONR = 10, SLOT_RHOS_ATOM = 11
+  SLOT_RHO_ONL = 6, SLOT_RHO_ONR = 7, SLOT_RHO_ATOM = 8, SLOT_RHOS_ONL = 9, SLOT_RHOS_ONR = 10, SLOT_RHOS_ATOM = 11,
+  SLOT_VNL = 12
 };
 #define FB_NMAT 9             // block arrays: enum fb_matrix
 

@@ -1,0 +1,579 @@
+// Non-local pseudopotential path on the GPU:
+//  * driver_neighbors_PP (c.SYSTEM/M_neighbors_PP.f90): the PP, PPx and PPp
+//    neighbour lists with their maps and points, and the PP common-neighbour
+//    triplets, bit-identical to the reference's order;
+//  * assemble_svnl (j.ASSEMBLERS/M_assemble_PP_2c.f90:95-207): the rotated
+//    <phi_mu(iatom) | Psi_ncc(jatom)> blocks of every PP pair;
+//  * assemble_vnl_2c (:230-496) and assemble_vnl_3c (M_assemble_PP_3c.f90:94-237):
+//    vnl(iatom, k) = sum over the PP atoms alpha near both of
+//    sum_ncc cl(ncc) sVNL(i, alpha)(mu, ncc) sVNL(j, alpha)(nu, ncc), in the
+//    reference's order (atom case, ontop left, ontop right, then the triplets in
+//    list order), one thread per target block or per target segment.
+// Binary64 throughout; contraction off; deterministic.
+#include "fb_device.h"
+
+#define LWG 256
+#include "fb_lists_dev.h"
+#include "fb_pp.h"
+
+#define NBP 16    // elements of one block (at most 4 x 4)
+#define T3P 64    // triplets per vnl_3c tile
+
+// ------------------------------------------------------------------ lists
+// PP (rc_i + rcutoff_PP_j) or PPx (rcutoff_PP_i + rc_j) neighbours, both with
+// the reference's strict test z < range - 0.01 (M_neighbors_PP.f90:466-474,
+// 730-738); FILL = false counts, FILL = true writes.
+template <bool FILL>
+__global__ __launch_bounds__(LWG) void k_pp_neighbors(DView v, DPP q, int list) {
+  __shared__ int s_wave[LWG / 64];
+  const int ia = blockIdx.x;
+  const int s = v.atom_struct[ia];
+  const DStruct S = v.st[s];
+  const int li = ia - S.atom_off;
+  const int in1 = v.imass[ia];
+  const double r1[3] = {v.ratom[3 * ia], v.ratom[3 * ia + 1], v.ratom[3 * ia + 2]};
+  const double rci = (list == PPL_PP) ? v.sp.rcmax[in1 - 1] : q.rcpp[in1 - 1];
+  double maxb = -99.0;
+  for (int k = 0; k < v.sp.ns; k++) maxb = fmax(maxb, (list == PPL_PP) ? q.rcpp[k] : v.sp.rcmax[k]);
+  const double reach = rci + maxb + 1.0e-6;
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  int count = 0;
+  const int64_t base = FILL ? q.off[list][ia] : 0;
+  for (int mb = 0; mb <= S.mbeta_max; mb++) {
+    const double *xlb = xls + 3 * mb;
+    if (!image_may_hit(S, xlb, r1, reach)) continue;
+    for (int j0 = 0; j0 < S.natoms; j0 += LWG) {
+      const int lj = j0 + threadIdx.x;
+      bool pass = false;
+      if (lj < S.natoms) {
+        const int ja = S.atom_off + lj;
+        double r2[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) r2[k] = v.ratom[3 * ja + k] + xlb[k];
+        const double z = d_distance(r1, r2);
+        const double rcj = (list == PPL_PP) ? q.rcpp[v.imass[ja] - 1] : v.sp.rcmax[v.imass[ja] - 1];
+        pass = z < rci + rcj - 0.01;
+      }
+      int tot;
+      const int pos = block_rank(pass, s_wave, &tot);
+      if (FILL && pass) {
+        const int64_t p = base + count + pos;
+        q.j[list][p] = lj + 1;
+        q.b[list][p] = mb;
+        q.atom[list][p] = ia;
+        if (lj == li && mb == 0) q.self[list][ia] = count + pos + 1;
+      }
+      count += tot;
+    }
+  }
+  if (!FILL && threadIdx.x == 0) q.n[list][ia] = count;
+}
+
+// make_cells order of the images (c.SYSTEM/make_cells.f90:130-144): iz outer,
+// iy, ix inner over -4..4 without the origin, which is image 0
+__device__ __forceinline__ void image_coords(int mb, int *c) {
+  if (mb == 0) { c[0] = c[1] = c[2] = 0; return; }
+  const int L = (mb <= 364) ? mb - 1 : mb;
+  c[0] = L % 9 - 4;
+  c[1] = (L / 9) % 9 - 4;
+  c[2] = L / 81 - 4;
+}
+
+// the first image mbeta with |xl(mbeta) - (xl(jbeta) + xl(ibeta))| < 1e-4 in
+// make_cells order, mbeta_max + 1 when none (find_neighbors_PPp,
+// M_neighbors_PP.f90:983-990).  The integer sum of the two image coordinates
+// names the only image that can match; it is checked with the reference's
+// distance, and the full scan covers everything else.
+__device__ inline int image_of_sum(const DStruct &S, const double *xls, int ib, int jb) {
+  double r3[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) r3[k] = xls[3 * jb + k] + xls[3 * ib + k];
+  if (S.mbeta_max == FB_NIMG - 1) {
+    int a[3], c[3];
+    image_coords(ib, a);
+    image_coords(jb, c);
+    const int x = a[0] + c[0], y = a[1] + c[1], z = a[2] + c[2];
+    if (abs(x) <= 4 && abs(y) <= 4 && abs(z) <= 4) {
+      int m = 0;
+      if (x || y || z) {
+        const int L = (z + 4) * 81 + (y + 4) * 9 + (x + 4);
+        m = (L < 364) ? L + 1 : L;
+      }
+      // two images within 1e-4 of one point would need a lattice vector below 2e-4 A
+      if (d_distance(xls + 3 * m, r3) < 1.0e-4) return m;
+    }
+  }
+  for (int mb = 0; mb <= S.mbeta_max; mb++)
+    if (d_distance(xls + 3 * mb, r3) < 1.0e-4) return mb;
+  return S.mbeta_max + 1;
+}
+
+// candidates of the PPp list of each atom: sum over its PP neighbours of their PPx counts
+__global__ void k_ppp_candidates(DView v, DPP q, int64_t *cnt) {
+  const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= v.natoms) return;
+  const int off = v.st[v.atom_struct[ia]].atom_off;
+  int64_t c = 0;
+  for (int64_t p = q.off[PPL_PP][ia]; p < q.off[PPL_PP][ia + 1]; p++) c += q.n[PPL_PPX][off + q.j[PPL_PP][p] - 1];
+  cnt[ia] = c;
+}
+
+// neighbors_PPp (M_neighbors_PP.f90:922-1012): FILL = false walks the
+// candidates (ineigh over PP, jneigh over the PPx of that neighbour) and keeps
+// the first occurrence of every (jatom, image) in the atom's scratch range;
+// FILL = true copies the kept entries to the list and finds the self entry.
+template <bool FILL>
+__global__ void k_ppp_build(DView v, DPP q, int *status) {
+  const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= v.natoms) return;
+  const int s = v.atom_struct[ia];
+  const DStruct S = v.st[s];
+  const int64_t c0 = q.cand_off[ia];
+  if (FILL) {
+    const int n = q.n[PPL_PPP][ia];
+    const int64_t o = q.off[PPL_PPP][ia];
+    const int li = ia - S.atom_off + 1;
+    int self = -999;
+    for (int k = 0; k < n; k++) {
+      const int j = q.cand_j[c0 + k], b = q.cand_b[c0 + k];
+      q.j[PPL_PPP][o + k] = j;
+      q.b[PPL_PPP][o + k] = b;
+      q.atom[PPL_PPP][o + k] = ia;
+      if (self == -999 && j == li && b == 0) self = k + 1;
+    }
+    q.self[PPL_PPP][ia] = self;
+    if (self == -999) atomicOr(status, ST_SELF);
+    return;
+  }
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  int num = 0;
+  for (int64_t p = q.off[PPL_PP][ia]; p < q.off[PPL_PP][ia + 1]; p++) {
+    const int ial = S.atom_off + q.j[PPL_PP][p] - 1;
+    const int ib = q.b[PPL_PP][p];
+    for (int64_t r = q.off[PPL_PPX][ial]; r < q.off[PPL_PPX][ial + 1]; r++) {
+      const int ja = q.j[PPL_PPX][r], jb = q.b[PPL_PPX][r];
+      const int mb = image_of_sum(S, xls, ib, jb);
+      bool seen = false;
+      for (int k = 0; k < num && !seen; k++) seen = (q.cand_j[c0 + k] == ja && q.cand_b[c0 + k] == mb);
+      if (!seen) {
+        q.cand_j[c0 + num] = ja;
+        q.cand_b[c0 + num] = mb;
+        num++;
+      }
+    }
+  }
+  q.n[PPL_PPP][ia] = num;
+}
+
+// maps of the PP and PPx pairs into PPp (first equal entry, M_neighbors_PP.f90:1026-1068)
+// and the PPx points (index of iatom among jatom's PP neighbours, :1071-1088)
+__global__ void k_pp_maps(DView v, DPP q, int *status) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t npp = q.npairs[PPL_PP];
+  if (t >= npp + q.npairs[PPL_PPX]) return;
+  const int list = (t < npp) ? PPL_PP : PPL_PPX;
+  const int64_t p = (t < npp) ? t : t - npp;
+  const int ia = q.atom[list][p];
+  const int j = q.j[list][p], b = q.b[list][p];
+  const int64_t o2 = q.off[PPL_PPP][ia];
+  const int n2 = q.n[PPL_PPP][ia];
+  int m = 0;
+  for (int k = 0; k < n2; k++)
+    if (q.j[PPL_PPP][o2 + k] == j && q.b[PPL_PPP][o2 + k] == b) { m = k + 1; break; }
+  if (m == 0) atomicOr(status, ST_BACK);
+  if (list == PPL_PP) {
+    q.pp_map[p] = m;
+    if (m) q.inv_pp[o2 + m - 1] = (int)p;
+    return;
+  }
+  q.px_map[p] = m;
+  if (m) q.inv_px[o2 + m - 1] = (int)p;
+  const int s = v.atom_struct[ia];
+  const DStruct S = v.st[s];
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  const int ja = S.atom_off + j - 1;
+  const double r1[3] = {v.ratom[3 * ia], v.ratom[3 * ia + 1], v.ratom[3 * ia + 2]};
+  int point = 0;
+  for (int64_t k = q.off[PPL_PP][ja]; k < q.off[PPL_PP][ja + 1]; k++) {
+    const int ka = S.atom_off + q.j[PPL_PP][k] - 1, kb = q.b[PPL_PP][k];
+    double r2[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) r2[c] = v.ratom[3 * ka + c] + xls[3 * kb + c] + xls[3 * b + c];
+    if (d_distance(r1, r2) < 1.0e-4) { point = (int)(k - q.off[PPL_PP][ja]) + 1; break; }
+  }
+  if (point == 0) atomicOr(status, ST_BACK);
+  q.px_point[p] = point;
+}
+
+// find_common_neighbors_PP (M_neighbors_PP.f90:1301-1420): ordered pairs of
+// PPx neighbours of ialpha, neither ialpha itself, kept when the bond vector
+// names an entry of iatom's PPp list (first within 1e-4).  FILL = false counts,
+// FILL = true writes the triplet and its target PPp pair.
+template <bool FILL>
+__global__ __launch_bounds__(LWG) void k_pp_common(DView v, DPP q) {
+  __shared__ int s_wave[LWG / 64];
+  const int ial = blockIdx.x;
+  const int s = v.atom_struct[ial];
+  const DStruct S = v.st[s];
+  const int lal = ial - S.atom_off + 1;
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  const int64_t o = q.off[PPL_PPX][ial];
+  const int n = q.n[PPL_PPX][ial];
+  const int64_t ncand = (int64_t)n * n;
+  int count = 0;
+  const int64_t base = FILL ? q.pc_off[ial] : 0;
+  for (int64_t c0 = 0; c0 < ncand; c0 += LWG) {
+    const int64_t c = c0 + threadIdx.x;
+    bool pass = false;
+    int ia = 0, ib = 0, ja = 0, jb = 0, m = -9999, gia = 0;
+    if (c < ncand) {
+      const int in = (int)(c / n), jn = (int)(c % n);
+      ia = q.j[PPL_PPX][o + in];
+      ib = q.b[PPL_PPX][o + in];
+      ja = q.j[PPL_PPX][o + jn];
+      jb = q.b[PPL_PPX][o + jn];
+      if (!(ia == lal && ib == 0) && !(ja == lal && jb == 0) && in != jn) {
+        gia = S.atom_off + ia - 1;
+        const int gja = S.atom_off + ja - 1;
+        double r1[3], r2[3], r[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          r1[k] = v.ratom[3 * gia + k] + xls[3 * ib + k];
+          r2[k] = v.ratom[3 * gja + k] + xls[3 * jb + k];
+          r[k] = r2[k] - r1[k];
+        }
+        const int64_t io = q.off[PPL_PPP][gia];
+        const int ni = q.n[PPL_PPP][gia];
+        for (int kn = 0; kn < ni; kn++) {
+          const int kb = q.b[PPL_PPP][io + kn];
+          if (kb > S.mbeta_max) continue;   // an image outside xl (the reference reads past its end)
+          const int gka = S.atom_off + q.j[PPL_PPP][io + kn] - 1;
+          double r3[3];
+#pragma unroll
+          for (int k = 0; k < 3; k++) r3[k] = xls[3 * kb + k] + v.ratom[3 * gka + k] - v.ratom[3 * gia + k];
+          if (d_distance(r, r3) < 1.0e-4) {
+            m = kn + 1;
+            break;
+          }
+        }
+        pass = (m != -9999);
+      }
+    }
+    int tot;
+    const int pos = block_rank(pass, s_wave, &tot);
+    if (FILL && pass) {
+      const int64_t t = base + count + pos;
+      const int gja = S.atom_off + ja - 1;
+      q.trip[t] = make_int4(ial, gia, gja, (ib & 0xffff) | (jb << 16));
+      q.tpair[t] = (int)(q.off[PPL_PPP][gia] + m - 1);
+    }
+    count += tot;
+  }
+  if (!FILL && threadIdx.x == 0) q.pc_n[ial] = count;
+}
+
+// block sizes: which 0 sVNL per PP pair (norb_mu x norb_PP_nu), 1 vnl per PPp pair
+__global__ void k_pp_block_sizes(DView v, DPP q, int which, int64_t *sz) {
+  const int list = which == 0 ? PPL_PP : PPL_PPP;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= q.npairs[list]) return;
+  const int ia = q.atom[list][p];
+  const int ja = v.st[v.atom_struct[ia]].atom_off + q.j[list][p] - 1;
+  const int in2 = v.imass[ja];
+  sz[p] = (int64_t)v.sp.norb[v.imass[ia] - 1] * (which == 0 ? q.norb_pp[in2 - 1] : v.sp.norb[in2 - 1]);
+}
+
+// ------------------------------------------------------------------ blocks
+// rotate_PP (c.SYSTEM/rotate_PP.f90): orbital shells of in1 on the left, PP
+// shells of in2 on the right, the loop order and association of rotate
+__device__ inline void d_rotate_pp(const DSpecies &sp, const DPP &q, int in1, int in2, const double *pm,
+                                   const double *M, int nmu, double *X) {
+  const int nnu = q.norb_pp[in2 - 1];
+  for (int i = 0; i < nmu * nnu; i++) X[i] = 0.0;
+  int n1 = 0;
+  for (int issh = 0; issh < sp.nssh[in1 - 1]; issh++) {
+    const int l1 = sp.lssh[in1 - 1][issh];
+    int n2 = 0;
+    for (int jssh = 0; jssh < q.nssh_pp[in2 - 1]; jssh++) {
+      const int l2 = q.lssh_pp[in2 - 1][jssh];
+      for (int m2 = 0; m2 < 2 * l2 + 1; m2++)
+        for (int m1 = 0; m1 < 2 * l1 + 1; m1++)
+          for (int k2 = 0; k2 < 2 * l2 + 1; k2++)
+            for (int k1 = 0; k1 < 2 * l1 + 1; k1++) {
+              const double left = (l1 == 0) ? 1.0 : pm[3 * m1 + k1];
+              const double right = (l2 == 0) ? 1.0 : pm[3 * m2 + k2];
+              double *x = &X[(n2 + k2) * nmu + (n1 + k1)];
+              *x = *x + left * M[(n2 + m2) * nmu + (n1 + m1)] * right;
+            }
+      n2 += 2 * l2 + 1;
+    }
+    n1 += 2 * l1 + 1;
+  }
+}
+
+// assemble_svnl: one lane per PP pair
+__global__ void k_svnl(DView v, DPP q) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= q.npairs[PPL_PP]) return;
+  const int ia = q.atom[PPL_PP][p];
+  const int s = v.atom_struct[ia];
+  const int ja = v.st[s].atom_off + q.j[PPL_PP][p] - 1;
+  const int mb = q.b[PPL_PP][p];
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  const int in1 = v.imass[ia], in2 = v.imass[ja];
+  const int nmu = v.sp.norb[in1 - 1], nnu = q.norb_pp[in2 - 1];
+  double r1[3], r2[3], sighat[3], eps[9], pm[9];
+  for (int k = 0; k < 3; k++) {
+    r1[k] = v.ratom[3 * ia + k];
+    r2[k] = v.ratom[3 * ja + k] + xls[3 * mb + k];
+  }
+  const double z = d_distance(r1, r2);
+  if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+  else for (int k = 0; k < 3; k++) sighat[k] = (r2[k] - r1[k]) / z;
+  d_epsilon(r2, sighat, eps);
+  d_pmat<double>(eps, pm);
+  const int cid = v.idx2c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * FB_NSLOT2 + SLOT_VNL) * FB_MAXSUB2];
+  int status = 0;
+  double hm[NBP], X[NBP];
+  if (cid < 0) {
+    status = ST_NO_TABLE;
+  } else {
+    const DCell2c c = v.cell2c[cid];
+    double F[NBP];
+    status = (c.nME > NBP) ? ST_CAPACITY : d_interp2c(c, v.tab2c, z, F);
+    if (!status) {
+      for (int i = 0; i < nmu * nnu; i++) hm[i] = 0.0;
+      for (int k = 0; k < c.nME; k++) hm[(v.nu[c.munu + k] - 1) * nmu + (v.mu[c.munu + k] - 1)] = F[k];
+      d_rotate_pp(v.sp, q, in1, in2, pm, hm, nmu, X);
+      double *dst = q.svnl + q.sv_off[p];
+      for (int i = 0; i < nmu * nnu; i++) dst[i] = X[i];
+    }
+  }
+  if (status) atomicOr(v.status, status);
+}
+
+// X(mu, nu) = sum_c cl(c) * A(mu, c) * B(nu, c), c ascending from 0 (the
+// PPx / bcnlx loops of M_assemble_PP_2c.f90 and M_assemble_PP_3c.f90);
+// A is nmu x ncc, B nnu x ncc, column-major
+__device__ __forceinline__ void kb_contract(int nmu, int nnu, int in_pp, const DPP &q, const double *A,
+                                            const double *B, double *X) {
+  double cl[NBP];
+  int ncc = 0;
+  for (int sh = 0; sh < q.nssh_pp[in_pp - 1]; sh++)
+    for (int m = 0; m < 2 * q.lssh_pp[in_pp - 1][sh] + 1; m++) cl[ncc++] = q.cl[in_pp - 1][sh];
+  for (int inu = 0; inu < nnu; inu++)
+    for (int imu = 0; imu < nmu; imu++) {
+      double x = 0.0;
+      for (int c = 0; c < ncc; c++) x = x + cl[c] * A[c * nmu + imu] * B[c * nnu + inu];
+      X[inu * nmu + imu] = x;
+    }
+}
+
+// assemble_vnl_2c: one lane per PPp pair.  The self block sums the atom case of
+// every PP neighbour in order; any other block takes its ontop-left term (the
+// PPx pair mapped onto it, projectors of iatom) and then its ontop-right term
+// (the PP pair mapped onto it, projectors of jatom), as the reference's three
+// loops add them.
+__global__ void k_vnl_2c(DView v, DPP q) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= q.npairs[PPL_PPP]) return;
+  const int ia = q.atom[PPL_PPP][k];
+  const int s = v.atom_struct[ia];
+  const int aoff = v.st[s].atom_off;
+  const int ja = aoff + q.j[PPL_PPP][k] - 1;
+  const int in1 = v.imass[ia], in2 = v.imass[ja];
+  const int nmu = v.sp.norb[in1 - 1], nnu = v.sp.norb[in2 - 1], nb = nmu * nnu;
+  double acc[NBP], X[NBP];
+  for (int i = 0; i < nb; i++) acc[i] = 0.0;
+  const int64_t o0 = q.off[PPL_PP][ia];
+  if (k == q.off[PPL_PPP][ia] + q.self[PPL_PPP][ia] - 1) {
+    for (int64_t p = o0; p < q.off[PPL_PP][ia + 1]; p++) {
+      const int inp = v.imass[aoff + q.j[PPL_PP][p] - 1];
+      const double *sv = q.svnl + q.sv_off[p];
+      kb_contract(nmu, nmu, inp, q, sv, sv, X);
+      for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i];
+    }
+  } else {
+    const int px = q.inv_px[k];
+    if (px >= 0) {
+      const double *sv1 = q.svnl + q.sv_off[o0 + q.self[PPL_PP][ia] - 1];
+      const double *sv2 = q.svnl + q.sv_off[q.off[PPL_PP][ja] + q.px_point[px] - 1];
+      kb_contract(nmu, nnu, in1, q, sv1, sv2, X);
+      for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i];
+    }
+    const int pp = q.inv_pp[k];
+    if (pp >= 0) {
+      const double *sv1 = q.svnl + q.sv_off[pp];
+      const double *sv2 = q.svnl + q.sv_off[q.off[PPL_PP][ja] + q.self[PPL_PP][ja] - 1];
+      kb_contract(nmu, nnu, in2, q, sv1, sv2, X);
+      for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i];
+    }
+  }
+  double *dst = q.vnl + q.vl_off[k];
+  for (int i = 0; i < nb; i++) dst[i] = acc[i];
+}
+
+// mpairnay (M_assemble_PP_3c.f90:266-364): the one PP neighbour of iatom that is
+// jatom (1-based local) at offset r12 from iatom's own position; 0 if not unique
+__device__ inline int mpairnay(const DView &v, const DPP &q, const double *xls, int aoff, int gia, int lja,
+                               const double *r12) {
+  int imatch = 0, mpair = 0;
+  const int64_t o = q.off[PPL_PP][gia];
+  const int gja = aoff + lja - 1;
+  for (int k = 0; k < q.n[PPL_PP][gia]; k++) {
+    if (q.j[PPL_PP][o + k] != lja) continue;
+    const int mb = q.b[PPL_PP][o + k];
+    double r21[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) r21[c] = (v.ratom[3 * gja + c] + xls[3 * mb + c]) - v.ratom[3 * gia + c];
+    if (d_distance(r12, r21) < 0.0001) {
+      imatch++;
+      mpair = k + 1;
+    }
+  }
+  return imatch == 1 ? mpair : 0;
+}
+
+// assemble_vnl_3c: the workgroup owns the target segments that start in its
+// tile of sorted triplets; each thread forms bcnlx of one triplet per chunk
+// into LDS, one owner per segment adds them in list order onto the block
+__global__ __launch_bounds__(T3P) void k_vnl_3c(DView v, DPP q) {
+  __shared__ double s_c[T3P][NBP + 1];
+  const int64_t ts = (int64_t)blockIdx.x * T3P;
+  const int64_t te = min(ts + T3P, q.ntrip);
+  const int64_t sgs = q.trip_seg[ts];
+  const int64_t first = (q.seg_off[sgs] == ts) ? sgs : sgs + 1;
+  const int64_t last = q.trip_seg[te - 1];
+  if (first > last) return;
+  const int64_t r0 = q.seg_off[first], r1 = q.seg_off[last + 1];
+  const int64_t my = first + threadIdx.x;
+  const bool own = my <= last;
+  int64_t a = 0, b = 0, dst = 0;
+  int nb = 0;
+  double acc[NBP];
+  if (own) {
+    a = q.seg_off[my];
+    b = q.seg_off[my + 1];
+    const int k = q.seg_pair[my];
+    const int ia = q.atom[PPL_PPP][k];
+    const int ja = v.st[v.atom_struct[ia]].atom_off + q.j[PPL_PPP][k] - 1;
+    nb = v.sp.norb[v.imass[ia] - 1] * v.sp.norb[v.imass[ja] - 1];
+    dst = q.vl_off[k];
+    for (int i = 0; i < NBP; i++) acc[i] = (i < nb) ? q.vnl[dst + i] : 0.0;
+  }
+  int status = 0;
+  for (int64_t c0 = r0; c0 < r1; c0 += T3P) {
+    const int64_t t = c0 + threadIdx.x;
+    double X[NBP];
+    for (int i = 0; i < NBP; i++) X[i] = 0.0;
+    if (t < r1) {
+      const int4 tr = q.trip_s[t];
+      const int ial = tr.x, gia = tr.y, gja = tr.z;
+      const int ib = tr.w & 0xffff, jb = (tr.w >> 16) & 0xffff;
+      const int s = v.atom_struct[ial];
+      const int aoff = v.st[s].atom_off;
+      const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+      double r31[3], r32[3];
+      for (int k = 0; k < 3; k++) {
+        const double rna = v.ratom[3 * ial + k];
+        r31[k] = rna - (v.ratom[3 * gia + k] + xls[3 * ib + k]);
+        r32[k] = rna - (v.ratom[3 * gja + k] + xls[3 * jb + k]);
+      }
+      const int lal = ial - aoff + 1;
+      const int m31 = mpairnay(v, q, xls, aoff, gia, lal, r31);
+      const int m32 = mpairnay(v, q, xls, aoff, gja, lal, r32);
+      if (m31 == 0 || m32 == 0) {
+        status |= ST_BACK;
+      } else {
+        const int nmu = v.sp.norb[v.imass[gia] - 1], nnu = v.sp.norb[v.imass[gja] - 1];
+        const double *sv1 = q.svnl + q.sv_off[q.off[PPL_PP][gia] + m31 - 1];
+        const double *sv2 = q.svnl + q.sv_off[q.off[PPL_PP][gja] + m32 - 1];
+        kb_contract(nmu, nnu, v.imass[ial], q, sv1, sv2, X);
+      }
+    }
+    for (int i = 0; i < NBP; i++) s_c[threadIdx.x][i] = X[i];
+    __syncthreads();
+    if (own) {
+      const int64_t k0 = max(a, c0), k1 = min(b, c0 + T3P);
+      for (int64_t kk = k0; kk < k1; kk++)
+        for (int i = 0; i < NBP; i++)
+          if (i < nb) acc[i] = acc[i] + s_c[kk - c0][i];
+    }
+    __syncthreads();
+  }
+  if (own)
+    for (int i = 0; i < NBP; i++)
+      if (i < nb) q.vnl[dst + i] = acc[i];
+  if (status) atomicOr(v.status, status);
+}
+
+// ------------------------------------------------------------------ launchers
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+extern "C" {
+int fbk_pp_neighbors(const DView &v, const DPP &q, int list, int fill, int *status, hipStream_t st) {
+  (void)status;
+  if (v.natoms == 0) return 0;
+  if (fill)
+    hipLaunchKernelGGL(k_pp_neighbors<true>, dim3(v.natoms), dim3(LWG), 0, st, v, q, list);
+  else
+    hipLaunchKernelGGL(k_pp_neighbors<false>, dim3(v.natoms), dim3(LWG), 0, st, v, q, list);
+  return (int)hipGetLastError();
+}
+
+int fbk_ppp_candidates(const DView &v, const DPP &q, int64_t *cnt, hipStream_t st) {
+  if (v.natoms == 0) return 0;
+  hipLaunchKernelGGL(k_ppp_candidates, dim3(nblk(v.natoms, 128)), dim3(128), 0, st, v, q, cnt);
+  return (int)hipGetLastError();
+}
+
+int fbk_ppp_build(const DView &v, const DPP &q, int fill, int *status, hipStream_t st) {
+  if (v.natoms == 0) return 0;
+  if (fill)
+    hipLaunchKernelGGL(k_ppp_build<true>, dim3(nblk(v.natoms, 64)), dim3(64), 0, st, v, q, status);
+  else
+    hipLaunchKernelGGL(k_ppp_build<false>, dim3(nblk(v.natoms, 64)), dim3(64), 0, st, v, q, status);
+  return (int)hipGetLastError();
+}
+
+int fbk_pp_maps(const DView &v, const DPP &q, int *status, hipStream_t st) {
+  const int64_t n = q.npairs[PPL_PP] + q.npairs[PPL_PPX];
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_pp_maps, dim3(nblk(n, 128)), dim3(128), 0, st, v, q, status);
+  return (int)hipGetLastError();
+}
+
+int fbk_pp_common(const DView &v, const DPP &q, int fill, hipStream_t st) {
+  if (v.natoms == 0) return 0;
+  if (fill)
+    hipLaunchKernelGGL(k_pp_common<true>, dim3(v.natoms), dim3(LWG), 0, st, v, q);
+  else
+    hipLaunchKernelGGL(k_pp_common<false>, dim3(v.natoms), dim3(LWG), 0, st, v, q);
+  return (int)hipGetLastError();
+}
+
+int fbk_pp_block_sizes(const DView &v, const DPP &q, int which, int64_t *sz, hipStream_t st) {
+  const int64_t n = q.npairs[which == 0 ? PPL_PP : PPL_PPP];
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_pp_block_sizes, dim3(nblk(n, 256)), dim3(256), 0, st, v, q, which, sz);
+  return (int)hipGetLastError();
+}
+
+int fbk_svnl(const DView &v, const DPP &q, hipStream_t st) {
+  if (q.npairs[PPL_PP] == 0) return 0;
+  hipLaunchKernelGGL(k_svnl, dim3(nblk(q.npairs[PPL_PP], 128)), dim3(128), 0, st, v, q);
+  return (int)hipGetLastError();
+}
+
+int fbk_vnl_2c(const DView &v, const DPP &q, hipStream_t st) {
+  if (q.npairs[PPL_PPP] == 0) return 0;
+  hipLaunchKernelGGL(k_vnl_2c, dim3(nblk(q.npairs[PPL_PPP], 128)), dim3(128), 0, st, v, q);
+  return (int)hipGetLastError();
+}
+
+int fbk_vnl_3c(const DView &v, const DPP &q, hipStream_t st) {
+  if (q.ntrip == 0) return 0;
+  hipLaunchKernelGGL(k_vnl_3c, dim3(nblk(q.ntrip, T3P)), dim3(T3P), 0, st, v, q);
+  return (int)hipGetLastError();
+}
+}

@@ -91,6 +91,12 @@ def lib():
             "fb_assemble_rho_2c": [vp], "fb_assemble_rho_3c": [vp], "fb_assemble_rho_weighted_2c": [vp],
             "fb_assemble_rho_weighted_3c": [vp], "fb_assemble_rho_all": [vp],
             "fb_get_shell_block_len": [vp, C.c_int, i64p],
+            "fb_set_species_pp": [vp, ip, ip, dp, dp],
+            "fb_build_lists_pp": [vp, i64p, i64p, i64p],
+            "fb_get_pp_list_sizes": [vp, C.c_int, i64p],
+            "fb_get_pp_lists": [vp, C.c_int, C.c_int, ip, ip, ip, ip, ip, ip, ip],
+            "fb_assemble_svnl": [vp], "fb_assemble_vnl_2c": [vp], "fb_assemble_vnl_3c": [vp],
+            "fb_get_pp_blocks": [vp, C.c_int, C.c_int, dp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -168,6 +174,15 @@ class Assembler:
             lssh[i, :sp.nssh] = sp.lssh
             rc[i, :sp.nssh] = sp.rcutoffA
         self._check(self.L.fb_set_species(self.h, ns, _ip(nZ), _ip(nssh), _ip(lssh), _dp(rc)))
+        if any(getattr(sp, "nssh_pp", 0) for sp in ts.species):
+            npp = np.array([sp.nssh_pp for sp in ts.species], np.int32)
+            lpp = np.zeros((ns, MAXSH), np.int32)
+            clv = np.zeros((ns, MAXSH))
+            rpp = np.array([sp.rcutoff_pp for sp in ts.species], np.float64)
+            for i, sp in enumerate(ts.species):
+                lpp[i, :sp.nssh_pp] = sp.lssh_pp
+                clv[i, :sp.nssh_pp] = sp.cl_pp
+            self._check(self.L.fb_set_species_pp(self.h, _ip(npp), _ip(lpp), _dp(rpp), _dp(clv)))
 
     def _set_tables(self, ts):
         for (a, b, it, isub), c in ts.cells2c.items():
@@ -333,6 +348,55 @@ class Assembler:
         out = np.zeros(max(n.value, 1))
         self._check(self.L.fb_get_blocks(self.h, istruct, FB_MAT[which], _dp(out)))
         return out[:n.value]
+
+    # ------------------------------------------------------- pseudopotential
+    PP_LISTS = {"pp": (0, ("j", "b", "map")), "ppx": (1, ("j", "b", "map", "point")), "ppp": (2, ("j", "b")),
+                "ppc": (3, ("ij", "ib", "jj", "jb", "m"))}
+
+    def build_lists_pp(self):
+        """driver_neighbors_PP for every structure; returns (PP pairs, PPp pairs, PP triplets)."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self.L.fb_build_lists_pp(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def pp_sizes(self, istruct=0):
+        sz = (C.c_int64 * 6)()
+        self._check(self.L.fb_get_pp_list_sizes(self.h, istruct, sz))
+        return [int(x) for x in sz]
+
+    def pp_lists(self, istruct=0):
+        """The PP lists of one structure, keyed like the oracle's run_pp()."""
+        sz = self.pp_sizes(istruct)
+        n = self.structs[istruct].natoms
+        out = {}
+        for name, (which, fields) in self.PP_LISTS.items():
+            tot = max(sz[which], 1)
+            cnt = np.zeros(n, np.int32)
+            arrs = [np.zeros(tot, np.int32) for _ in range(5)]
+            self_ = np.zeros(n, np.int32)
+            ptr = [_ip(a) if k < len(fields) else None for k, a in enumerate(arrs)]
+            self._check(self.L.fb_get_pp_lists(self.h, istruct, which, _ip(cnt), *ptr,
+                                               _ip(self_) if which < 3 else None))
+            out[name + "_n"] = cnt
+            for k, f in enumerate(fields):
+                out[name + "_" + f] = arrs[k][:sz[which]]
+            if which < 3:
+                out[name + "_self"] = self_
+        return out
+
+    def assemble_pp(self, which=("SVNL", "VNL2C", "VNL3C")):
+        """assemble_svnl, assemble_vnl_2c, assemble_vnl_3c (in this order)."""
+        fn = {"SVNL": self.L.fb_assemble_svnl, "VNL2C": self.L.fb_assemble_vnl_2c, "VNL3C": self.L.fb_assemble_vnl_3c}
+        for w in which:
+            self._check(fn[w](self.h))
+
+    def pp_blocks(self, which, istruct=0):
+        """SVNL (per PP pair, norb_mu x norb_PP_nu) or VNL (per PPp pair), flat, column-major."""
+        sz = self.pp_sizes(istruct)
+        n = sz[4] if which == "SVNL" else sz[5]
+        out = np.zeros(max(n, 1))
+        self._check(self.L.fb_get_pp_blocks(self.h, istruct, 0 if which == "SVNL" else 1, _dp(out)))
+        return out[:n]
 
     def gamma_eigenvalues(self, which="H", istruct=0):
         """Eigenvalues (ascending) of the Gamma-point folded S or
