@@ -283,6 +283,62 @@ def rho_leg(species, structs, steps, warmup):
     return {"ms_per_step": ms, "triplets_per_s": ntrip / (ms * 1e-3), "kernels_ms": kern, "dtype": "f64"}
 
 
+def pp_leg(species, structs, steps, warmup):
+    """One pseudopotential build = assemble_svnl + assemble_vnl_2c +
+    assemble_vnl_3c (binary64) on the same structures with the synthetic PP
+    shells and P_vnl tables, after driver_neighbors_PP (timed once, like the
+    H+S lists); a context of its own."""
+    sp = synth.with_pp(species)
+    ts = synth.add_pp_tables(synth.make_tables(sp), seed=6066)
+    a = Assembler(ts, precision="fp64")
+    for st in structs:
+        a.add_structure(st)
+    a.build_lists()
+    t0 = time.perf_counter()
+    npp, nppp, ntrip = a.build_lists_pp()
+    lists_ms = (time.perf_counter() - t0) * 1e3
+    for _ in range(warmup):
+        a.assemble_pp()
+    a.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a.assemble_pp()
+    a.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    a.enable_timing(True)
+    a.timings(4096)
+    for _ in range(max(1, min(steps, 5))):
+        a.assemble_pp()
+    a.synchronize()
+    kern = {}
+    for name, t in a.timings(4096):
+        kern.setdefault(name, []).append(t)
+    a.enable_timing(False)
+    kern = {k: float(np.mean(v)) for k, v in kern.items() if k != "clear_status"}
+    a.close()
+    return {"ms_per_step": ms, "pp_triplets_per_s": ntrip / (ms * 1e-3), "kernels_ms": kern, "dtype": "f64",
+            "pp_pairs": npp, "ppp_pairs": nppp, "pp_triplets": ntrip, "lists_pp_ms": lists_ms}
+
+
+def pp_cpu_baseline():
+    """The reference's PP assemblers (serial flang build) on the Si512 sample."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import refrun
+    if not refrun.available(refrun.HARNESS_PREF):
+        return None
+    st = synth.diamond_si((4, 4, 4), sigma=0.08, seed=0)
+    sp = synth.with_pp([synth.SI])
+    ts = synth.add_pp_tables(synth.make_tables(sp), seed=6066)
+    nrep = 3
+    r = refrun.run_pp(ts, st, nrep=nrep)
+    t = sum(r["times"].values()) / nrep
+    ntrip = int(len(r["ppc_m"]))
+    return {"value": ntrip / t, "unit": "PP triplets/s", "cores": 1, "kind": "reference",
+            "sample": f"Si512 4x4x4 diamond (sigma 0.08 A): {ntrip} PP triplets, PP build {t * 1e3:.1f} ms "
+                      f"(assemble_svnl, assemble_vnl_2c, assemble_vnl_3c, serial flang -O2, mean of {nrep})",
+            "pp_build_ms": t * 1e3}
+
+
 def rho_cpu_baseline():
     """The reference's McWEDA density assemblers (serial flang build) on the
     bounded Si512 sample of the other baselines."""
@@ -393,6 +449,7 @@ def main():
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned 3c surrogate leg")
     ap.add_argument("--no-forces", action="store_true", help="skip the force-assembler leg")
     ap.add_argument("--no-rho", action="store_true", help="skip the McWEDA density leg")
+    ap.add_argument("--no-pp", action="store_true", help="skip the pseudopotential leg")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one elementwise copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -478,6 +535,17 @@ def main():
         except Exception as e:  # reported, never required
             rho = {"error": str(e)}
 
+    pp = None
+    if not args.no_pp:
+        try:
+            pp = pp_leg(species, structs, args.steps, args.warmup)
+            if not args.no_cpu_baseline:
+                pp["cpu_baseline"] = pp_cpu_baseline()
+            print(f"# pseudopotential: {pp['ms_per_step']:.3f} ms per PP build ({pp['pp_triplets']} PP triplets, "
+                  f"lists {pp['lists_pp_ms']:.1f} ms); kernels {pp['kernels_ms']}", file=sys.stderr, flush=True)
+        except Exception as e:  # reported, never required
+            pp = {"error": str(e)}
+
     # dominant kernel: the 3c launch
     k3 = [k for k in kern if k.startswith("assemble_3c")]
     dom = max(kern, key=kern.get) if not k3 else k3[0]
@@ -503,7 +571,7 @@ def main():
                    "nstructures": len(structs), "npairs": npairs, "ntriplets": ntrip,
                    "precision": args.precision, "h_s_build_ms": ms, "lists_ms": lists_ms,
                    "chunk3c": chunk, "ms_per_step_with_kernel_events": ms_events,
-                   "kernels_ms": kern, **extra, "mlp_3c": mlp, "forces": forces, "rho_mcweda": rho,
+                   "kernels_ms": kern, **extra, "mlp_3c": mlp, "forces": forces, "rho_mcweda": rho, "pseudopotential": pp,
                    **({"structures_per_s": len(structs) / (ms * 1e-3)} if len(structs) > 1 else {})},
         "roofline": roof,
         "cpu_baseline": cpu,

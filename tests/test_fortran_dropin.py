@@ -217,3 +217,47 @@ def test_rho_dropin_matches_reference_live_repeated():
     hip = refrun.run_rho(ts, st, qin, lists, binary=RHIP_HARNESS, nrep=2)
     for k in RKEYS:
         assert np.array_equal(hip[k], ref[k]), k
+
+
+# ------------------------------------------------------------ pseudopotential
+PHIP_HARNESS = os.path.join(os.path.dirname(refrun.HARNESS_REF), "fb_pharness_hip")
+need_phip = pytest.mark.skipif(not refrun.available(PHIP_HARNESS), reason="make -C oracle hip not built")
+
+
+@need_phip
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", golden_cases.NAMES)
+def test_pp_dropin_matches_reference_golden(golden, name):
+    """The PP harness (oracle/ref/fb_pharness.f90: the caller's
+    driver_neighbors_PP, then assemble_svnl, assemble_vnl_2c, assemble_vnl_3c)
+    linked with the drop-in M_assemble_PP_2c / M_assemble_PP_3c: lists, sVNL
+    and vnl equal the reference's own (golden fixtures)."""
+    from golden.make_golden import PP_LIST_KEYS, pp_tables
+    print(f"pp case {name}", flush=True)
+    g, gp = golden(name), golden(name + "_pp")
+    ts, st = golden_cases.inputs(g)
+    r = refrun.run_pp(pp_tables(ts, int(gp["pp_table_seed"])), st, binary=PHIP_HARNESS)
+    for k in PP_LIST_KEYS:
+        assert np.array_equal(r[k], gp[k]), k
+    for k in ("SVNL", "VNL"):
+        assert np.array_equal(r[k][:gp[k].size], gp[k]), k
+        assert float(np.sum(r[k])) == float(gp[k + "_sum"]), k
+
+
+@need_phip
+@pytest.mark.gpu
+def test_pp_dropin_matches_reference_live_repeated():
+    """Si216, the PP assemblers run twice in one process (destroy_assemble_PP_2c
+    and the cached PP lists), against the reference's modules on the same input."""
+    if not refrun.available(refrun.HARNESS_PREF):
+        pytest.skip("reference PP harness not built")
+    from fireball_hip import synth
+    sp = synth.with_pp([synth.SI])
+    ts = synth.add_pp_tables(synth.make_tables(sp), seed=91)
+    st = synth.diamond_si((3, 3, 3), sigma=0.08, seed=5)
+    ref = refrun.run_pp(ts, st)
+    hip = refrun.run_pp(ts, st, binary=PHIP_HARNESS, nrep=2)
+    for k in ref:
+        if k in ("times", "nrep"):
+            continue
+        assert np.array_equal(hip[k], ref[k]), k

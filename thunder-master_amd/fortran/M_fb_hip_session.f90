@@ -31,13 +31,14 @@
         real(c_double), allocatable, save :: c_ratom(:, :)
         real(c_double), save :: c_lattice(3, 3)
         logical, save :: have_lists = .false.
+        logical, save :: have_pp_lists = .false.
         integer(c_int64_t), save :: npairs_all = 0, ntrip_all = 0
         real(c_double), allocatable, save :: fb_buf(:)
 
         ! 2c interactions of the assemblers (include/interactions_2c.h:5-25):
         ! H and S, then the McWEDA densities rho_* and rhoS_*
-        integer, parameter :: n_itype2c = 12
-        integer, parameter :: itype2c(n_itype2c) = [1, 2, 3, 4, 5, 10, 15, 16, 17, 19, 20, 21]
+        integer, parameter :: n_itype2c = 13
+        integer, parameter :: itype2c(n_itype2c) = [1, 2, 3, 4, 5, 6, 10, 15, 16, 17, 19, 20, 21]
         ! 3c: rho_3c, rhoS_3c and bcna (include/interactions_3c.h:6-8) over five
         ! Legendre angles
         integer, parameter :: n_itype3c = 3
@@ -79,6 +80,7 @@
           end do
         end do
         call fb_check (fb_set_species(fbh, nsp, nZ, nssh, lssh, rcA), fbh, 'fb_set_species')
+        call set_species_pp (nsp)
 
         do is = 1, nsp
           do js = 1, nsp
@@ -164,9 +166,142 @@
         c_ratom = r
         call fb_check (fb_build_lists(fbh, npairs_all, ntrip_all), fbh, 'fb_build_lists')
         have_lists = .true.
+        have_pp_lists = .false.
         call check_lists (s)
         deallocate (r, im)
         end subroutine fb_session_sync
+
+! Pseudopotential shells of the species (species(:)%nssh_PP, shell_PP(:)%lssh,
+! %rcutoff_PP and the Kleinman-Bylander shell_PP(:)%cl of clPP.ZZ.dat), when
+! any species has them.
+        subroutine set_species_pp (nsp)
+        integer, intent(in) :: nsp
+        integer ispecies, issh
+        integer(c_int) :: npp(nsp), lpp(FB_MAXSH, nsp)
+        real(c_double) :: rpp(nsp), clv(FB_MAXSH, nsp)
+        if (all(species(:)%nssh_PP .eq. 0)) return
+        lpp = 0
+        clv = 0.0d0
+        do ispecies = 1, nsp
+          npp(ispecies) = species(ispecies)%nssh_PP
+          if (npp(ispecies) .gt. FB_MAXSH) error stop ' fireball_hip: more than 4 PP shells '
+          rpp(ispecies) = species(ispecies)%rcutoff_PP
+          do issh = 1, species(ispecies)%nssh_PP
+            lpp(issh, ispecies) = species(ispecies)%shell_PP(issh)%lssh
+            clv(issh, ispecies) = species(ispecies)%shell_PP(issh)%cl
+          end do
+        end do
+        call fb_check (fb_set_species_pp(fbh, npp, lpp, rpp, clv), fbh, 'fb_set_species_pp')
+        end subroutine set_species_pp
+
+! The library's pseudopotential lists for the current geometry, checked entry
+! by entry against the caller's driver_neighbors_PP (s%neighbors_PP, _PPx,
+! _PPp with their maps, points and selfs, and the PP common neighbours).
+        subroutine fb_session_pp (s)
+        type(T_structure), target :: s
+        integer(c_int64_t) :: npp, nppp, ntrip, sz(6)
+        integer(c_int), allocatable, target :: n(:), self(:), a(:, :)
+        integer iatom, p, m, which
+        logical ok
+        if (have_pp_lists) return
+        call fb_check (fb_build_lists_pp(fbh, npp, nppp, ntrip), fbh, 'fb_build_lists_pp')
+        call fb_check (fb_get_pp_list_sizes(fbh, 0, sz), fbh, 'fb_get_pp_list_sizes')
+        allocate (n(s%natoms), self(s%natoms))
+        ok = .true.
+        do which = 0, 3
+          allocate (a(max(sz(which+1), 1_8), 5))
+          call fb_check (fb_get_pp_lists(fbh, 0, which, c_loc(n), c_loc(a(1, 1)), c_loc(a(1, 2)),   &
+     &      c_loc(a(1, 3)), c_loc(a(1, 4)), c_loc(a(1, 5)), c_loc(self)), fbh, 'fb_get_pp_lists')
+          p = 0
+          do iatom = 1, s%natoms
+            select case (which)
+            case (0)
+              m = s%neighbors_PP(iatom)%neighn
+              if (m .ne. n(iatom) .or. self(iatom) .ne. s%neighbors_PP_self(iatom)) ok = .false.
+              if (.not. ok) exit
+              if (any(s%neighbors_PP(iatom)%neigh_j(1:m) .ne. a(p+1:p+m, 1))) ok = .false.
+              if (any(s%neighbors_PP(iatom)%neigh_b(1:m) .ne. a(p+1:p+m, 2))) ok = .false.
+              if (any(s%neighbors_PP(iatom)%map(1:m) .ne. a(p+1:p+m, 3))) ok = .false.
+            case (1)
+              m = s%neighbors_PPx(iatom)%neighn
+              if (m .ne. n(iatom) .or. self(iatom) .ne. s%neighbors_PPx_self(iatom)) ok = .false.
+              if (.not. ok) exit
+              if (any(s%neighbors_PPx(iatom)%neigh_j(1:m) .ne. a(p+1:p+m, 1))) ok = .false.
+              if (any(s%neighbors_PPx(iatom)%neigh_b(1:m) .ne. a(p+1:p+m, 2))) ok = .false.
+              if (any(s%neighbors_PPx(iatom)%map(1:m) .ne. a(p+1:p+m, 3))) ok = .false.
+              if (any(s%neighbors_PPx(iatom)%point(1:m) .ne. a(p+1:p+m, 4))) ok = .false.
+            case (2)
+              m = s%neighbors_PPp(iatom)%neighn
+              if (m .ne. n(iatom) .or. self(iatom) .ne. s%neighbors_PPp_self(iatom)) ok = .false.
+              if (.not. ok) exit
+              if (any(s%neighbors_PPp(iatom)%neigh_j(1:m) .ne. a(p+1:p+m, 1))) ok = .false.
+              if (any(s%neighbors_PPp(iatom)%neigh_b(1:m) .ne. a(p+1:p+m, 2))) ok = .false.
+            case (3)
+              m = s%neighbors_PP(iatom)%ncommon
+              if (m .ne. n(iatom)) ok = .false.
+              if (.not. ok) exit
+              if (m .gt. 0) then
+                if (any(s%neighbors_PP(iatom)%iatom_common_j(1:m) .ne. a(p+1:p+m, 1))) ok = .false.
+                if (any(s%neighbors_PP(iatom)%iatom_common_b(1:m) .ne. a(p+1:p+m, 2))) ok = .false.
+                if (any(s%neighbors_PP(iatom)%jatom_common_j(1:m) .ne. a(p+1:p+m, 3))) ok = .false.
+                if (any(s%neighbors_PP(iatom)%jatom_common_b(1:m) .ne. a(p+1:p+m, 4))) ok = .false.
+                if (any(s%neighbors_PP(iatom)%neigh_common(1:m) .ne. a(p+1:p+m, 5))) ok = .false.
+              end if
+            end select
+            p = p + m
+          end do
+          deallocate (a)
+          if (.not. ok) exit
+        end do
+        deallocate (n, self)
+        if (.not. ok) then
+          write (*,*) ' fireball_hip: the GPU PP lists differ from s%neighbors_PP/_PPx/_PPp '
+          error stop ' fireball_hip failure '
+        end if
+        have_pp_lists = .true.
+        end subroutine fb_session_pp
+
+! Fill the blocks of s%svnl (which = 0, per PP pair, norb_mu x norb_PP_nu) or
+! s%vnl (which = 1, per PPp pair, norb_mu x norb_nu) from the library,
+! allocating them when asked.
+        subroutine fb_session_ppstore (s, m, which, alloc)
+        type(T_structure), target :: s
+        type(T_assemble_neighbors), pointer :: m(:)
+        integer, intent(in) :: which
+        logical, intent(in) :: alloc
+        integer iatom, ineigh, jatom, nmu, nnu, nb, nn
+        integer(c_int64_t) :: sz(6), off
+        real(c_double), allocatable :: w(:)
+        type(T_assemble_block), pointer :: pb
+        call fb_check (fb_get_pp_list_sizes(fbh, 0, sz), fbh, 'fb_get_pp_list_sizes')
+        allocate (w(max(sz(5 + which), 1_8)))
+        call fb_check (fb_get_pp_blocks(fbh, 0, which, w), fbh, 'fb_get_pp_blocks')
+        off = 0
+        do iatom = 1, s%natoms
+          nmu = species(s%atom(iatom)%imass)%norb_max
+          if (which .eq. 0) then
+            nn = s%neighbors_PP(iatom)%neighn
+          else
+            nn = s%neighbors_PPp(iatom)%neighn
+          end if
+          if (alloc) allocate (m(iatom)%neighbors(nn))
+          do ineigh = 1, nn
+            if (which .eq. 0) then
+              jatom = s%neighbors_PP(iatom)%neigh_j(ineigh)
+              nnu = species(s%atom(jatom)%imass)%norb_PP_max
+            else
+              jatom = s%neighbors_PPp(iatom)%neigh_j(ineigh)
+              nnu = species(s%atom(jatom)%imass)%norb_max
+            end if
+            nb = nmu*nnu
+            pb => m(iatom)%neighbors(ineigh)
+            if (alloc) allocate (pb%block(nmu, nnu))
+            pb%block = reshape(w(off+1:off+nb), [nmu, nnu])
+            off = off + nb
+          end do
+        end do
+        deallocate (w)
+        end subroutine fb_session_ppstore
 
 ! Hand the current shell charges to the library (Vna only depends on them).
         subroutine fb_session_charges (s)

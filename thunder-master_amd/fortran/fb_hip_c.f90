@@ -227,6 +227,49 @@
             integer(c_int), value :: is
             integer(c_int64_t) :: wlen
           end function
+          integer(c_int) function fb_set_species_pp (ctx, nssh_pp, lssh_pp, rcutoff_pp, cl)       &
+     &        bind(c, name='fb_set_species_pp')
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int) :: nssh_pp(*), lssh_pp(*)
+            real(c_double) :: rcutoff_pp(*), cl(*)
+          end function
+          integer(c_int) function fb_build_lists_pp (ctx, npp, nppp, ntrip) bind(c, name='fb_build_lists_pp')
+            import :: c_ptr, c_int, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t) :: npp, nppp, ntrip
+          end function
+          integer(c_int) function fb_get_pp_list_sizes (ctx, is, sizes) bind(c, name='fb_get_pp_list_sizes')
+            import :: c_ptr, c_int, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: is
+            integer(c_int64_t) :: sizes(6)
+          end function
+          integer(c_int) function fb_get_pp_lists (ctx, is, which, n, a0, a1, a2, a3, a4, self)    &
+     &        bind(c, name='fb_get_pp_lists')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: is, which
+            type(c_ptr), value :: n, a0, a1, a2, a3, a4, self
+          end function
+          integer(c_int) function fb_assemble_svnl (ctx) bind(c, name='fb_assemble_svnl')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+          end function
+          integer(c_int) function fb_assemble_vnl_2c (ctx) bind(c, name='fb_assemble_vnl_2c')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+          end function
+          integer(c_int) function fb_assemble_vnl_3c (ctx) bind(c, name='fb_assemble_vnl_3c')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+          end function
+          integer(c_int) function fb_get_pp_blocks (ctx, is, which, out) bind(c, name='fb_get_pp_blocks')
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: is, which
+            real(c_double) :: out(*)
+          end function
         end interface
 
         contains
