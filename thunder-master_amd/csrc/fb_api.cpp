@@ -257,7 +257,7 @@ struct fb_ctx {
   DevBuf d_ppn[3], d_ppoff[3], d_ppj[3], d_ppb[3], d_ppatom[3], d_ppself[3];
   DevBuf d_pp_map, d_px_map, d_px_point, d_inv_pp, d_inv_px, d_cand_cnt, d_cand_off, d_cand_j, d_cand_b;
   DevBuf d_pc_n, d_pc_off, d_pc_trip, d_pc_tpair, d_pc_trip_s, d_pc_tpair_s, d_pc_perm_in, d_pc_perm, d_pc_flag,
-      d_pc_segincl, d_pc_seg_off, d_pc_seg_pair, d_pc_trip_seg, d_pc_tgt;
+      d_pc_segincl, d_pc_seg_off, d_pc_seg_pair, d_pc_trip_seg, d_pc_tgt, d_pc_mp;
   DevBuf d_sv_bsz, d_sv_off, d_vl_bsz, d_vl_off, d_svnl, d_vnl;
   int64_t pp_npairs[3] = {0, 0, 0}, pp_ntrip = 0, pp_nseg = 0, sv_len = 0, vl_len = 0;
   std::vector<int64_t> h_ppoff[3], h_pcoff, h_sv_struct, h_vl_struct;
@@ -292,6 +292,7 @@ struct fb_ctx {
     q.seg_off = d_pc_seg_off.as<int64_t>();
     q.seg_pair = d_pc_seg_pair.as<int>();
     q.trip_seg = d_pc_trip_seg.as<int>();
+    q.mp = d_pc_mp.as<int2>();
     q.ntrip = pp_ntrip;
     q.nseg = pp_nseg;
     q.sv_off = d_sv_off.as<int64_t>();
@@ -513,7 +514,7 @@ struct fb_ctx {
                       &d_status, &d_idx3r, &d_qin, &d_wbsz, &d_wboff, &d_pp_map, &d_px_map, &d_px_point,
                       &d_inv_pp, &d_inv_px, &d_cand_cnt, &d_cand_off, &d_cand_j, &d_cand_b, &d_pc_n, &d_pc_off,
                       &d_pc_trip, &d_pc_tpair, &d_pc_trip_s, &d_pc_tpair_s, &d_pc_perm_in, &d_pc_perm, &d_pc_flag,
-                      &d_pc_segincl, &d_pc_seg_off, &d_pc_seg_pair, &d_pc_trip_seg, &d_pc_tgt, &d_sv_bsz, &d_sv_off,
+                      &d_pc_segincl, &d_pc_seg_off, &d_pc_seg_pair, &d_pc_trip_seg, &d_pc_tgt, &d_pc_mp, &d_sv_bsz, &d_sv_off,
                       &d_vl_bsz, &d_vl_off, &d_svnl, &d_vnl};
     for (DevBuf *b : bufs) b->release();
     for (auto &m : d_mat) m.release();
@@ -2289,6 +2290,7 @@ int fb_build_lists_pp(fb_ctx *ctx, int64_t *npp, int64_t *nppp, int64_t *ntrip) 
   FB_HIP(ctx->d_pc_seg_pair.ensure(sizeof(int) * T1));
   FB_HIP(ctx->d_pc_trip_seg.ensure(sizeof(int) * T1));
   FB_HIP(ctx->d_pc_tgt.ensure(sizeof(int) * Q));
+  FB_HIP(ctx->d_pc_mp.ensure(sizeof(int2) * T1));
   q = ctx->pview();
   RC(ctx->launch("pp_common_fill", [&] { return fbk_pp_common(v, q, 1, st); }));
   ctx->pp_nseg = 0;
@@ -2325,6 +2327,8 @@ int fb_build_lists_pp(fb_ctx *ctx, int64_t *npp, int64_t *nppp, int64_t *ntrip) 
     }));
     FB_HIP(hipStreamSynchronize(st));
     ctx->pp_nseg = nseg;
+    q = ctx->pview();
+    RC(ctx->launch("pp_mpair", [&] { return fbk_pp_mpair(v, q, st); }));
   }
   // block offsets: sVNL per PP pair, vnl per PPp pair
   const int64_t P = std::max<int64_t>(ctx->pp_npairs[PPL_PP], 1);

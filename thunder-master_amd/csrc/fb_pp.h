@@ -34,6 +34,7 @@ struct DPP {
   int *tpair_s;
   int64_t *seg_off;
   int *seg_pair, *trip_seg;
+  int2 *mp;               // per sorted triplet: mpairnay of (iatom, ialpha) and (jatom, ialpha), 0 when not unique
   int64_t ntrip, nseg;
   // blocks
   int64_t *sv_off, *vl_off;   // [npairs+1] of PP / PPp
@@ -51,4 +52,5 @@ int fbk_pp_block_sizes(const DView &v, const DPP &q, int which, int64_t *sz, hip
 int fbk_svnl(const DView &v, const DPP &q, hipStream_t st);
 int fbk_vnl_2c(const DView &v, const DPP &q, hipStream_t st);
 int fbk_vnl_3c(const DView &v, const DPP &q, hipStream_t st);
+int fbk_pp_mpair(const DView &v, const DPP &q, hipStream_t st);
 }

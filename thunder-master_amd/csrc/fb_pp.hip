@@ -118,51 +118,74 @@ __global__ void k_ppp_candidates(DView v, DPP q, int64_t *cnt) {
   cnt[ia] = c;
 }
 
-// neighbors_PPp (M_neighbors_PP.f90:922-1012): FILL = false walks the
-// candidates (ineigh over PP, jneigh over the PPx of that neighbour) and keeps
-// the first occurrence of every (jatom, image) in the atom's scratch range;
-// FILL = true copies the kept entries to the list and finds the self entry.
-template <bool FILL>
-__global__ void k_ppp_build(DView v, DPP q, int *status) {
-  const int ia = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ia >= v.natoms) return;
+// neighbors_PPp (M_neighbors_PP.f90:922-1012), one workgroup per atom.  The
+// candidates (ineigh over PP, jneigh over the PPx list of that neighbour, in
+// this order) get their key (jatom, image of ibeta + jbeta) in the atom's
+// scratch range (k_ppp_keys); a candidate is kept when no earlier candidate has
+// the same key, which is the reference's first-seen rule, and the kept ones
+// are compacted in candidate order (k_ppp_build: FILL = false counts, FILL =
+// true writes the list and the self entry).
+__global__ __launch_bounds__(LWG) void k_ppp_keys(DView v, DPP q) {
+  const int ia = blockIdx.x;
   const int s = v.atom_struct[ia];
   const DStruct S = v.st[s];
   const int64_t c0 = q.cand_off[ia];
-  if (FILL) {
-    const int n = q.n[PPL_PPP][ia];
-    const int64_t o = q.off[PPL_PPP][ia];
-    const int li = ia - S.atom_off + 1;
-    int self = -999;
-    for (int k = 0; k < n; k++) {
-      const int j = q.cand_j[c0 + k], b = q.cand_b[c0 + k];
-      q.j[PPL_PPP][o + k] = j;
-      q.b[PPL_PPP][o + k] = b;
-      q.atom[PPL_PPP][o + k] = ia;
-      if (self == -999 && j == li && b == 0) self = k + 1;
-    }
-    q.self[PPL_PPP][ia] = self;
-    if (self == -999) atomicOr(status, ST_SELF);
-    return;
-  }
   const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
-  int num = 0;
+  int64_t base = 0;
   for (int64_t p = q.off[PPL_PP][ia]; p < q.off[PPL_PP][ia + 1]; p++) {
     const int ial = S.atom_off + q.j[PPL_PP][p] - 1;
     const int ib = q.b[PPL_PP][p];
-    for (int64_t r = q.off[PPL_PPX][ial]; r < q.off[PPL_PPX][ial + 1]; r++) {
-      const int ja = q.j[PPL_PPX][r], jb = q.b[PPL_PPX][r];
-      const int mb = image_of_sum(S, xls, ib, jb);
-      bool seen = false;
-      for (int k = 0; k < num && !seen; k++) seen = (q.cand_j[c0 + k] == ja && q.cand_b[c0 + k] == mb);
-      if (!seen) {
-        q.cand_j[c0 + num] = ja;
-        q.cand_b[c0 + num] = mb;
-        num++;
-      }
+    const int64_t r0 = q.off[PPL_PPX][ial], nr = q.off[PPL_PPX][ial + 1] - r0;
+    for (int64_t r = threadIdx.x; r < nr; r += LWG) {
+      q.cand_j[c0 + base + r] = q.j[PPL_PPX][r0 + r];
+      q.cand_b[c0 + base + r] = image_of_sum(S, xls, ib, q.b[PPL_PPX][r0 + r]);
     }
+    base += nr;
   }
-  q.n[PPL_PPP][ia] = num;
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(LWG) void k_ppp_build(DView v, DPP q, int *status) {
+  __shared__ int s_wave[LWG / 64];
+  __shared__ int s_self;
+  const int ia = blockIdx.x;
+  const int s = v.atom_struct[ia];
+  const DStruct S = v.st[s];
+  const int64_t c0 = q.cand_off[ia];
+  const int64_t nc = q.cand_off[ia + 1] - c0;
+  const int li = ia - S.atom_off + 1;
+  const int64_t o = FILL ? q.off[PPL_PPP][ia] : 0;
+  if (FILL && threadIdx.x == 0) s_self = -999;
+  int count = 0;
+  for (int64_t k0 = 0; k0 < nc; k0 += LWG) {
+    const int64_t c = k0 + threadIdx.x;
+    bool keep = false;
+    int j = 0, b = 0;
+    if (c < nc) {
+      j = q.cand_j[c0 + c];
+      b = q.cand_b[c0 + c];
+      keep = true;
+      for (int64_t e = 0; e < c && keep; e++) keep = !(q.cand_j[c0 + e] == j && q.cand_b[c0 + e] == b);
+    }
+    int tot;
+    const int pos = block_rank(keep, s_wave, &tot);
+    if (FILL && keep) {
+      q.j[PPL_PPP][o + count + pos] = j;
+      q.b[PPL_PPP][o + count + pos] = b;
+      q.atom[PPL_PPP][o + count + pos] = ia;
+      if (j == li && b == 0) s_self = count + pos + 1;   // a key occurs once in the kept list
+    }
+    count += tot;
+  }
+  if (!FILL) {
+    if (threadIdx.x == 0) q.n[PPL_PPP][ia] = count;
+    return;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    q.self[PPL_PPP][ia] = s_self;
+    if (s_self == -999) atomicOr(status, ST_SELF);
+  }
 }
 
 // maps of the PP and PPx pairs into PPp (first equal entry, M_neighbors_PP.f90:1026-1068)
@@ -369,15 +392,43 @@ __device__ __forceinline__ void kb_contract(int nmu, int nnu, int in_pp, const D
     }
 }
 
-// assemble_vnl_2c: one lane per PPp pair.  The self block sums the atom case of
-// every PP neighbour in order; any other block takes its ontop-left term (the
-// PPx pair mapped onto it, projectors of iatom) and then its ontop-right term
-// (the PP pair mapped onto it, projectors of jatom), as the reference's three
-// loops add them.
+// assemble_vnl_2c, atom case: the self block of iatom sums, over its PP
+// neighbours in order, sum_c cl(c) sVNL(mu, c) sVNL(nu, c) of that neighbour's
+// projectors; one lane per (atom, block element), each element summed in the
+// reference's order.
+__global__ void k_vnl_2c_self(DView v, DPP q) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)v.natoms * NBP) return;
+  const int ia = (int)(t / NBP), e = (int)(t % NBP);
+  const int in1 = v.imass[ia];
+  const int nmu = v.sp.norb[in1 - 1];
+  if (e >= nmu * nmu) return;
+  const int imu = e % nmu, inu = e / nmu;
+  const int aoff = v.st[v.atom_struct[ia]].atom_off;
+  double acc = 0.0;
+  for (int64_t p = q.off[PPL_PP][ia]; p < q.off[PPL_PP][ia + 1]; p++) {
+    const int inp = v.imass[aoff + q.j[PPL_PP][p] - 1];
+    const double *sv = q.svnl + q.sv_off[p];
+    double x = 0.0;
+    int c = 0;
+    for (int sh = 0; sh < q.nssh_pp[inp - 1]; sh++) {
+      const double cl = q.cl[inp - 1][sh];
+      for (int m = 0; m < 2 * q.lssh_pp[inp - 1][sh] + 1; m++, c++) x = x + cl * sv[c * nmu + imu] * sv[c * nmu + inu];
+    }
+    acc = acc + x;
+  }
+  q.vnl[q.vl_off[q.off[PPL_PPP][ia] + q.self[PPL_PPP][ia] - 1] + e] = acc;
+}
+
+// assemble_vnl_2c, ontop cases: one lane per non-self PPp pair, its ontop-left
+// term (the PPx pair mapped onto it, projectors of iatom) and then its
+// ontop-right term (the PP pair mapped onto it, projectors of jatom), as the
+// reference's two loops add them.
 __global__ void k_vnl_2c(DView v, DPP q) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= q.npairs[PPL_PPP]) return;
   const int ia = q.atom[PPL_PPP][k];
+  if (k == q.off[PPL_PPP][ia] + q.self[PPL_PPP][ia] - 1) return;
   const int s = v.atom_struct[ia];
   const int aoff = v.st[s].atom_off;
   const int ja = aoff + q.j[PPL_PPP][k] - 1;
@@ -386,28 +437,19 @@ __global__ void k_vnl_2c(DView v, DPP q) {
   double acc[NBP], X[NBP];
   for (int i = 0; i < nb; i++) acc[i] = 0.0;
   const int64_t o0 = q.off[PPL_PP][ia];
-  if (k == q.off[PPL_PPP][ia] + q.self[PPL_PPP][ia] - 1) {
-    for (int64_t p = o0; p < q.off[PPL_PP][ia + 1]; p++) {
-      const int inp = v.imass[aoff + q.j[PPL_PP][p] - 1];
-      const double *sv = q.svnl + q.sv_off[p];
-      kb_contract(nmu, nmu, inp, q, sv, sv, X);
-      for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i];
-    }
-  } else {
-    const int px = q.inv_px[k];
-    if (px >= 0) {
-      const double *sv1 = q.svnl + q.sv_off[o0 + q.self[PPL_PP][ia] - 1];
-      const double *sv2 = q.svnl + q.sv_off[q.off[PPL_PP][ja] + q.px_point[px] - 1];
-      kb_contract(nmu, nnu, in1, q, sv1, sv2, X);
-      for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i];
-    }
-    const int pp = q.inv_pp[k];
-    if (pp >= 0) {
-      const double *sv1 = q.svnl + q.sv_off[pp];
-      const double *sv2 = q.svnl + q.sv_off[q.off[PPL_PP][ja] + q.self[PPL_PP][ja] - 1];
-      kb_contract(nmu, nnu, in2, q, sv1, sv2, X);
-      for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i];
-    }
+  const int px = q.inv_px[k];
+  if (px >= 0) {
+    const double *sv1 = q.svnl + q.sv_off[o0 + q.self[PPL_PP][ia] - 1];
+    const double *sv2 = q.svnl + q.sv_off[q.off[PPL_PP][ja] + q.px_point[px] - 1];
+    kb_contract(nmu, nnu, in1, q, sv1, sv2, X);
+    for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i];
+  }
+  const int pp = q.inv_pp[k];
+  if (pp >= 0) {
+    const double *sv1 = q.svnl + q.sv_off[pp];
+    const double *sv2 = q.svnl + q.sv_off[q.off[PPL_PP][ja] + q.self[PPL_PP][ja] - 1];
+    kb_contract(nmu, nnu, in2, q, sv1, sv2, X);
+    for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i];
   }
   double *dst = q.vnl + q.vl_off[k];
   for (int i = 0; i < nb; i++) dst[i] = acc[i];
@@ -432,6 +474,27 @@ __device__ inline int mpairnay(const DView &v, const DPP &q, const double *xls, 
     }
   }
   return imatch == 1 ? mpair : 0;
+}
+
+// the two mpairnay searches of every sorted triplet, done once per list build
+// (they depend on the lists and positions only)
+__global__ void k_pp_mpair(DView v, DPP q) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= q.ntrip) return;
+  const int4 tr = q.trip_s[t];
+  const int ial = tr.x, gia = tr.y, gja = tr.z;
+  const int ib = tr.w & 0xffff, jb = (tr.w >> 16) & 0xffff;
+  const int s = v.atom_struct[ial];
+  const int aoff = v.st[s].atom_off;
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  double r31[3], r32[3];
+  for (int k = 0; k < 3; k++) {
+    const double rna = v.ratom[3 * ial + k];
+    r31[k] = rna - (v.ratom[3 * gia + k] + xls[3 * ib + k]);
+    r32[k] = rna - (v.ratom[3 * gja + k] + xls[3 * jb + k]);
+  }
+  const int lal = ial - aoff + 1;
+  q.mp[t] = make_int2(mpairnay(v, q, xls, aoff, gia, lal, r31), mpairnay(v, q, xls, aoff, gja, lal, r32));
 }
 
 // assemble_vnl_3c: the workgroup owns the target segments that start in its
@@ -469,19 +532,8 @@ __global__ __launch_bounds__(T3P) void k_vnl_3c(DView v, DPP q) {
     if (t < r1) {
       const int4 tr = q.trip_s[t];
       const int ial = tr.x, gia = tr.y, gja = tr.z;
-      const int ib = tr.w & 0xffff, jb = (tr.w >> 16) & 0xffff;
-      const int s = v.atom_struct[ial];
-      const int aoff = v.st[s].atom_off;
-      const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
-      double r31[3], r32[3];
-      for (int k = 0; k < 3; k++) {
-        const double rna = v.ratom[3 * ial + k];
-        r31[k] = rna - (v.ratom[3 * gia + k] + xls[3 * ib + k]);
-        r32[k] = rna - (v.ratom[3 * gja + k] + xls[3 * jb + k]);
-      }
-      const int lal = ial - aoff + 1;
-      const int m31 = mpairnay(v, q, xls, aoff, gia, lal, r31);
-      const int m32 = mpairnay(v, q, xls, aoff, gja, lal, r32);
+      const int2 mp = q.mp[t];
+      const int m31 = mp.x, m32 = mp.y;
       if (m31 == 0 || m32 == 0) {
         status |= ST_BACK;
       } else {
@@ -529,10 +581,12 @@ int fbk_ppp_candidates(const DView &v, const DPP &q, int64_t *cnt, hipStream_t s
 
 int fbk_ppp_build(const DView &v, const DPP &q, int fill, int *status, hipStream_t st) {
   if (v.natoms == 0) return 0;
-  if (fill)
-    hipLaunchKernelGGL(k_ppp_build<true>, dim3(nblk(v.natoms, 64)), dim3(64), 0, st, v, q, status);
-  else
-    hipLaunchKernelGGL(k_ppp_build<false>, dim3(nblk(v.natoms, 64)), dim3(64), 0, st, v, q, status);
+  if (fill) {
+    hipLaunchKernelGGL(k_ppp_build<true>, dim3(v.natoms), dim3(LWG), 0, st, v, q, status);
+  } else {
+    hipLaunchKernelGGL(k_ppp_keys, dim3(v.natoms), dim3(LWG), 0, st, v, q);
+    hipLaunchKernelGGL(k_ppp_build<false>, dim3(v.natoms), dim3(LWG), 0, st, v, q, status);
+  }
   return (int)hipGetLastError();
 }
 
@@ -567,7 +621,14 @@ int fbk_svnl(const DView &v, const DPP &q, hipStream_t st) {
 
 int fbk_vnl_2c(const DView &v, const DPP &q, hipStream_t st) {
   if (q.npairs[PPL_PPP] == 0) return 0;
+  hipLaunchKernelGGL(k_vnl_2c_self, dim3(nblk((int64_t)v.natoms * NBP, 256)), dim3(256), 0, st, v, q);
   hipLaunchKernelGGL(k_vnl_2c, dim3(nblk(q.npairs[PPL_PPP], 128)), dim3(128), 0, st, v, q);
+  return (int)hipGetLastError();
+}
+
+int fbk_pp_mpair(const DView &v, const DPP &q, hipStream_t st) {
+  if (q.ntrip == 0) return 0;
+  hipLaunchKernelGGL(k_pp_mpair, dim3(nblk(q.ntrip, 128)), dim3(128), 0, st, v, q);
   return (int)hipGetLastError();
 }
 
