@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libfbhip.so")
 HEADER = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include", "fireball_hip.h")
 MAXSH = 4
 
+HS_MATS = ("S", "T", "DIP", "VNA", "VNAO")        # the H+S build's block arrays
 FB_MAT = {"S": 0, "T": 1, "DIP": 2, "VNA": 3, "VNAO": 4,
           "RHO_IN": 5, "RHO_BOND": 6, "RHOW_IN": 7, "RHOW_BOND": 8}
 PREC = {"fp64": 0, "mixed": 1}
@@ -505,6 +506,6 @@ class Assembler:
     def run_all(self, istruct=0):
         """Lists + every matrix for one structure, as a dict shaped like the oracle's."""
         res = self.lists(istruct)
-        for m in FB_MAT:
+        for m in HS_MATS:
             res[m] = self.blocks(m, istruct)
         return res
