@@ -287,8 +287,8 @@ int fb_mlp_forward(fb_ctx *ctx, int nme, const float *params, const float *oscal
  * FB_TUNE_GRAPH: 1 (default) replays fb_assemble_all from a captured HIP
  * graph, 0 enqueues its launches directly.  FB_TUNE_FORCE_KERNEL: the force
  * kernels for s/p species, 1 (default) = compile-time block shapes, 0 = the
- * generic shell loops; both give the same values.  The McWEDA density
- * kernels follow the same switch.
+ * generic shell loops; both give the same values.  The McWEDA density and
+ * pseudopotential kernels follow the same switch.
  * fb_autotune times every candidate of both on the current lists (nrep
  * launches each, HIP events) and keeps the fastest; it returns the chosen
  * chunk in *chosen. */

@@ -69,6 +69,25 @@ def test_pp_batch_matches_oracle_per_structure():
             _same(asm.pp_blocks(key, k), ref[key], f"structure {k} {key}")
 
 
+@pytest.mark.parametrize("name", ["si64", "cho40_dq", "chain_cluster"])
+def test_pp_sp_kernels_equal_generic_kernels(golden, name):
+    """The s/p PP kernels (compile-time shapes, register rotations, unrolled
+    contractions) against the generic shell-loop kernels."""
+    from fireball_hip.lib import Assembler
+    g, gp = golden(name), golden(name + "_pp")
+    ts, st = golden_cases.inputs(g)
+    tsp = pp_tables(ts, int(gp["pp_table_seed"]))
+    a = _gpu_pp(tsp, [st])
+    b = Assembler(tsp, precision="fp64")
+    b.add_structure(st)
+    b.build_lists()
+    b.set_tuning(3, 0)
+    b.build_lists_pp()
+    b.assemble_pp()
+    for k in ("SVNL", "VNL"):
+        _same(a.pp_blocks(k), b.pp_blocks(k), k)
+
+
 def test_pp_order_errors(golden):
     from fireball_hip.lib import Assembler, FireballError
     g, gp = golden("si8_cell"), golden("si8_cell_pp")

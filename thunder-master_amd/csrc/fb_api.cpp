@@ -266,6 +266,7 @@ struct fb_ctx {
 
   DPP pview() const {
     DPP q = hpp;
+    q.canon = hpp.canon && force_kernel;   // FB_TUNE_FORCE_KERNEL 0 selects the generic kernels
     for (int l = 0; l < 3; l++) {
       q.n[l] = d_ppn[l].as<int>();
       q.off[l] = d_ppoff[l].as<int64_t>();
@@ -1007,6 +1008,20 @@ int fb_commit_tables(fb_ctx *ctx) {
       }
     }
     ctx->canon_rho = cr;
+    // projector tables: s/p species and s/p PP shells, make_munu order with the PP shells on the right
+    int cp = ctx->pp_species ? 1 : 0;
+    std::vector<std::vector<int>> ppsh(ns);
+    for (int i = 0; i < ns && cp; i++) {
+      for (int k = 0; k < ctx->hpp.nssh_pp[i]; k++) ppsh[i].push_back(ctx->hpp.lssh_pp[i][k]);
+      if (!((shells[i] == std::vector<int>{0}) || (shells[i] == std::vector<int>{0, 1}))) cp = 0;
+      if (!(ppsh[i].empty() || ppsh[i] == std::vector<int>{0} || ppsh[i] == std::vector<int>{0, 1})) cp = 0;
+    }
+    for (auto &c : ctx->c2) {
+      if (!cp) break;
+      if (slot_of(c.itype) != SLOT_VNL) continue;
+      if (!same_list(munu2c(shells[c.is - 1], ppsh[c.js - 1]), c.mu, c.nu, c.mv)) cp = 0;
+    }
+    ctx->hpp.canon = cp;
   }
   std::vector<float> tab2f(tab2.begin(), tab2.end());
   RC(upload(ctx, ctx->d_idx2c, idx2.data(), idx2.size() * sizeof(int), "upload idx2c"));
@@ -2173,6 +2188,9 @@ int fb_set_species_pp(fb_ctx *ctx, const int *nssh_pp, const int *lssh_pp, const
     }
     if (h.norb_pp[i] > 4) return ctx->fail(FB_E_UNSUPPORTED, "set_species_pp: more than 4 projector orbitals");
     h.rcpp[i] = rcutoff_pp[i];
+    int c = 0;   // cl per projector orbital, b.FUNCTIONS/cl.f90:84-93
+    for (int k = 0; k < nssh_pp[i]; k++)
+      for (int m = 0; m < 2 * h.lssh_pp[i][k] + 1; m++) h.clo[i][c++] = h.cl[i][k];
   }
   ctx->hpp = h;
   ctx->pp_species = true;

@@ -15,6 +15,8 @@ struct DPP {
   int norb_pp[FB_MAXSPECIES];
   double rcpp[FB_MAXSPECIES];
   double cl[FB_MAXSPECIES][FB_MAXSH];
+  double clo[FB_MAXSPECIES][4];   // cl per projector orbital (b.FUNCTIONS/cl.f90), s/p PP shells
+  int canon;                      // s/p species and PP shells, P_vnl tables in make_munu order
   // lists, CSR over global atoms; neighbour atoms 1-based per structure, images 0..mbeta_max(+1)
   int *n[3];
   int64_t *off[3];        // [natoms+1]

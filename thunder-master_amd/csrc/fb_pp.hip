@@ -559,6 +559,238 @@ __global__ __launch_bounds__(T3P) void k_vnl_3c(DView v, DPP q) {
   if (status) atomicOr(v.status, status);
 }
 
+// ---------------------------------------------------------------------------
+// s/p species and s/p PP shells with canonical P_vnl tables: the same sums
+// with compile-time block shapes.  The projector table of an (orbital, PP)
+// pair has the element order of the 2c s/p tables (make_munu with the PP
+// shells on the right) and rotate_PP is rotate, so the s/p interpolation and
+// rotation of the 2c kernels apply unchanged; the contractions are unrolled.
+#include "fb_sp.h"
+#include "fb_2c_dev.h"
+
+template <int CLS>
+__device__ __forceinline__ int svnl_sp(const DView &v, int cid, double z, const double *pm, double *X) {
+  if (cid < 0) return ST_NO_TABLE;
+  const DCell2c c = v.cell2c[cid];
+  if (z < 0.0) return ST_X_NEG;
+  if (z > c.xmax * (1.0 + 1.0e-5)) return ST_X_RANGE;
+  const DGrid2c gr = v.grid2c[c.grid];
+  const int ileft = window7(z, gr.dx, gr.nx);
+  double prod[7], F[nme2c_of(CLS)];
+  lagrange7(v, gr, z, ileft, prod);
+  interp_cell<nme2c_of(CLS)>(v, c, ileft, prod, F);
+  rot2c_sp<double>(CLS, pm, F, X);
+  return 0;
+}
+
+__global__ void k_svnl_sp(DView v, DPP q) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= q.npairs[PPL_PP]) return;
+  const int ia = q.atom[PPL_PP][p];
+  const int s = v.atom_struct[ia];
+  const int ja = v.st[s].atom_off + q.j[PPL_PP][p] - 1;
+  const int mb = q.b[PPL_PP][p];
+  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+  const int in1 = v.imass[ia], in2 = v.imass[ja];
+  double r1[3], r2[3], sighat[3], eps[9], pm[9];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    r1[k] = v.ratom[3 * ia + k];
+    r2[k] = v.ratom[3 * ja + k] + xls[3 * mb + k];
+  }
+  const double z = d_distance(r1, r2);
+  if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+  else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) sighat[k] = (r2[k] - r1[k]) / z;
+  }
+  d_epsilon(r2, sighat, eps);
+  d_pmat<double>(eps, pm);
+  const int cid = v.idx2c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * FB_NSLOT2 + SLOT_VNL) * FB_MAXSUB2];
+  const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (q.norb_pp[in2 - 1] == 4 ? 1 : 0);
+  double X[16];
+  int status;
+  double *dst = q.svnl + q.sv_off[p];
+  switch (cls) {
+    case 3:
+      status = svnl_sp<3>(v, cid, z, pm, X);
+#pragma unroll
+      for (int i = 0; i < 16; i++) dst[i] = X[i];
+      break;
+    case 2:
+      status = svnl_sp<2>(v, cid, z, pm, X);
+#pragma unroll
+      for (int i = 0; i < 4; i++) dst[i] = X[i];
+      break;
+    case 1:
+      status = svnl_sp<1>(v, cid, z, pm, X);
+#pragma unroll
+      for (int i = 0; i < 4; i++) dst[i] = X[i];
+      break;
+    default:
+      status = svnl_sp<0>(v, cid, z, pm, X);
+      dst[0] = X[0];
+      break;
+  }
+  if (status) atomicOr(v.status, status);
+}
+
+// X(mu, nu) = sum_c cl(c) A(mu, c) B(nu, c), unrolled, the order of kb_contract
+template <int NMU, int NNU, int NCC>
+__device__ __forceinline__ void kb_sp(const double *cl, const double *A, const double *B, double *X) {
+#pragma unroll
+  for (int inu = 0; inu < NNU; inu++)
+#pragma unroll
+    for (int imu = 0; imu < NMU; imu++) {
+      double x = 0.0;
+#pragma unroll
+      for (int c = 0; c < NCC; c++) x = x + cl[c] * A[c * NMU + imu] * B[c * NNU + inu];
+      X[inu * NMU + imu] = x;
+    }
+}
+
+template <int NMU, int NNU, int NCC>
+__device__ __forceinline__ void kb_add(const DPP &q, int in_pp, const double *A, const double *B, double *acc) {
+  double cl[NCC], X[NMU * NNU];
+#pragma unroll
+  for (int c = 0; c < NCC; c++) cl[c] = q.clo[in_pp - 1][c];
+  kb_sp<NMU, NNU, NCC>(cl, A, B, X);
+#pragma unroll
+  for (int i = 0; i < NMU * NNU; i++) acc[i] = acc[i] + X[i];
+}
+
+template <int NMU, int NNU>
+__device__ __forceinline__ void kb_add_any(const DPP &q, int in_pp, const double *A, const double *B, double *acc) {
+  if (q.norb_pp[in_pp - 1] == 4) kb_add<NMU, NNU, 4>(q, in_pp, A, B, acc);
+  else kb_add<NMU, NNU, 1>(q, in_pp, A, B, acc);
+}
+
+template <int NMU, int NNU>
+__device__ __forceinline__ void vnl_pair_sp(const DView &v, const DPP &q, int64_t k, int ia, int ja, int in1, int in2) {
+  double acc[NMU * NNU];
+#pragma unroll
+  for (int i = 0; i < NMU * NNU; i++) acc[i] = 0.0;
+  const int64_t o0 = q.off[PPL_PP][ia];
+  const int px = q.inv_px[k];
+  if (px >= 0) {
+    const double *sv1 = q.svnl + q.sv_off[o0 + q.self[PPL_PP][ia] - 1];
+    const double *sv2 = q.svnl + q.sv_off[q.off[PPL_PP][ja] + q.px_point[px] - 1];
+    kb_add_any<NMU, NNU>(q, in1, sv1, sv2, acc);
+  }
+  const int pp = q.inv_pp[k];
+  if (pp >= 0) {
+    const double *sv1 = q.svnl + q.sv_off[pp];
+    const double *sv2 = q.svnl + q.sv_off[q.off[PPL_PP][ja] + q.self[PPL_PP][ja] - 1];
+    kb_add_any<NMU, NNU>(q, in2, sv1, sv2, acc);
+  }
+  double *dst = q.vnl + q.vl_off[k];
+#pragma unroll
+  for (int i = 0; i < NMU * NNU; i++) dst[i] = acc[i];
+}
+
+__global__ void k_vnl_2c_sp(DView v, DPP q) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= q.npairs[PPL_PPP]) return;
+  const int ia = q.atom[PPL_PPP][k];
+  if (k == q.off[PPL_PPP][ia] + q.self[PPL_PPP][ia] - 1) return;
+  const int ja = v.st[v.atom_struct[ia]].atom_off + q.j[PPL_PPP][k] - 1;
+  const int in1 = v.imass[ia], in2 = v.imass[ja];
+  const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
+  switch (cls) {
+    case 3: vnl_pair_sp<4, 4>(v, q, k, ia, ja, in1, in2); break;
+    case 2: vnl_pair_sp<4, 1>(v, q, k, ia, ja, in1, in2); break;
+    case 1: vnl_pair_sp<1, 4>(v, q, k, ia, ja, in1, in2); break;
+    default: vnl_pair_sp<1, 1>(v, q, k, ia, ja, in1, in2); break;
+  }
+}
+
+// bcnlx of one triplet into its LDS row
+template <int NMU, int NNU>
+__device__ __forceinline__ void triplet_vnl_sp(const DPP &q, int indna, const double *sv1, const double *sv2,
+                                               double *row) {
+  double X[NMU * NNU];
+  if (q.norb_pp[indna - 1] == 4) {
+    double cl[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) cl[c] = q.clo[indna - 1][c];
+    kb_sp<NMU, NNU, 4>(cl, sv1, sv2, X);
+  } else {
+    const double cl[1] = {q.clo[indna - 1][0]};
+    kb_sp<NMU, NNU, 1>(cl, sv1, sv2, X);
+  }
+#pragma unroll
+  for (int i = 0; i < NMU * NNU; i++) row[i] = X[i];
+}
+
+__global__ __launch_bounds__(T3P) void k_vnl_3c_sp(DView v, DPP q) {
+  __shared__ double s_c[T3P][NBP + 1];
+  const int64_t ts = (int64_t)blockIdx.x * T3P;
+  const int64_t te = min(ts + T3P, q.ntrip);
+  const int64_t sgs = q.trip_seg[ts];
+  const int64_t first = (q.seg_off[sgs] == ts) ? sgs : sgs + 1;
+  const int64_t last = q.trip_seg[te - 1];
+  if (first > last) return;
+  const int64_t r0 = q.seg_off[first], r1 = q.seg_off[last + 1];
+  const int64_t my = first + threadIdx.x;
+  const bool own = my <= last;
+  int64_t a = 0, b = 0, dst = 0;
+  int nb = 0;
+  double acc[NBP];
+#pragma unroll
+  for (int i = 0; i < NBP; i++) acc[i] = 0.0;
+  if (own) {
+    a = q.seg_off[my];
+    b = q.seg_off[my + 1];
+    const int k = q.seg_pair[my];
+    const int ia = q.atom[PPL_PPP][k];
+    const int ja = v.st[v.atom_struct[ia]].atom_off + q.j[PPL_PPP][k] - 1;
+    nb = v.sp.norb[v.imass[ia] - 1] * v.sp.norb[v.imass[ja] - 1];
+    dst = q.vl_off[k];
+#pragma unroll
+    for (int i = 0; i < NBP; i++)
+      if (i < nb) acc[i] = q.vnl[dst + i];
+  }
+  int status = 0;
+  for (int64_t c0 = r0; c0 < r1; c0 += T3P) {
+    const int64_t t = c0 + threadIdx.x;
+    double *row = s_c[threadIdx.x];
+    if (t < r1) {
+      const int4 tr = q.trip_s[t];
+      const int ial = tr.x, gia = tr.y, gja = tr.z;
+      const int2 mp = q.mp[t];
+      if (mp.x == 0 || mp.y == 0) {
+        status |= ST_BACK;
+      } else {
+        const int in1 = v.imass[gia], in2 = v.imass[gja];
+        const double *sv1 = q.svnl + q.sv_off[q.off[PPL_PP][gia] + mp.x - 1];
+        const double *sv2 = q.svnl + q.sv_off[q.off[PPL_PP][gja] + mp.y - 1];
+        const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
+        switch (cls) {
+          case 3: triplet_vnl_sp<4, 4>(q, v.imass[ial], sv1, sv2, row); break;
+          case 2: triplet_vnl_sp<4, 1>(q, v.imass[ial], sv1, sv2, row); break;
+          case 1: triplet_vnl_sp<1, 4>(q, v.imass[ial], sv1, sv2, row); break;
+          default: triplet_vnl_sp<1, 1>(q, v.imass[ial], sv1, sv2, row); break;
+        }
+      }
+    }
+    __syncthreads();
+    if (own) {
+      const int64_t k0 = max(a, c0), k1 = min(b, c0 + T3P);
+      for (int64_t kk = k0; kk < k1; kk++)
+#pragma unroll
+        for (int i = 0; i < NBP; i++)
+          if (i < nb) acc[i] = acc[i] + s_c[kk - c0][i];
+    }
+    __syncthreads();
+  }
+  if (own) {
+#pragma unroll
+    for (int i = 0; i < NBP; i++)
+      if (i < nb) q.vnl[dst + i] = acc[i];
+  }
+  if (status) atomicOr(v.status, status);
+}
+
 // ------------------------------------------------------------------ launchers
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -615,14 +847,20 @@ int fbk_pp_block_sizes(const DView &v, const DPP &q, int which, int64_t *sz, hip
 
 int fbk_svnl(const DView &v, const DPP &q, hipStream_t st) {
   if (q.npairs[PPL_PP] == 0) return 0;
-  hipLaunchKernelGGL(k_svnl, dim3(nblk(q.npairs[PPL_PP], 128)), dim3(128), 0, st, v, q);
+  if (q.canon)
+    hipLaunchKernelGGL(k_svnl_sp, dim3(nblk(q.npairs[PPL_PP], 128)), dim3(128), 0, st, v, q);
+  else
+    hipLaunchKernelGGL(k_svnl, dim3(nblk(q.npairs[PPL_PP], 128)), dim3(128), 0, st, v, q);
   return (int)hipGetLastError();
 }
 
 int fbk_vnl_2c(const DView &v, const DPP &q, hipStream_t st) {
   if (q.npairs[PPL_PPP] == 0) return 0;
   hipLaunchKernelGGL(k_vnl_2c_self, dim3(nblk((int64_t)v.natoms * NBP, 256)), dim3(256), 0, st, v, q);
-  hipLaunchKernelGGL(k_vnl_2c, dim3(nblk(q.npairs[PPL_PPP], 128)), dim3(128), 0, st, v, q);
+  if (q.canon)
+    hipLaunchKernelGGL(k_vnl_2c_sp, dim3(nblk(q.npairs[PPL_PPP], 128)), dim3(128), 0, st, v, q);
+  else
+    hipLaunchKernelGGL(k_vnl_2c, dim3(nblk(q.npairs[PPL_PPP], 128)), dim3(128), 0, st, v, q);
   return (int)hipGetLastError();
 }
 
@@ -634,7 +872,10 @@ int fbk_pp_mpair(const DView &v, const DPP &q, hipStream_t st) {
 
 int fbk_vnl_3c(const DView &v, const DPP &q, hipStream_t st) {
   if (q.ntrip == 0) return 0;
-  hipLaunchKernelGGL(k_vnl_3c, dim3(nblk(q.ntrip, T3P)), dim3(T3P), 0, st, v, q);
+  if (q.canon)
+    hipLaunchKernelGGL(k_vnl_3c_sp, dim3(nblk(q.ntrip, T3P)), dim3(T3P), 0, st, v, q);
+  else
+    hipLaunchKernelGGL(k_vnl_3c, dim3(nblk(q.ntrip, T3P)), dim3(T3P), 0, st, v, q);
   return (int)hipGetLastError();
 }
 }
