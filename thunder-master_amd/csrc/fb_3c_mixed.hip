@@ -28,7 +28,6 @@
 
 #define NB (FB_MO * FB_MO)
 #define NME3 10
-#define TH8 8     // theta values per (point, ME) in the FP32 slab: 5 used, 3 padding
 
 struct TPrm {
   unsigned base;              // FP32 slab index (floats) of stencil corner (ix-1, iy-1), neutral group
@@ -85,8 +84,8 @@ __device__ __forceinline__ float fold(__amdgpu_buffer_rsrc_t r, unsigned off, co
 template <int NME>
 __device__ __forceinline__ float stencil(__amdgpu_buffer_rsrc_t r, unsigned base, unsigned rowstride, int m,
                                          const float *wx, const float *wy, const float *w1, const float *P, int m1) {
-  constexpr unsigned PT = NME * TH8 * 4;   // bytes per ix step
-  const unsigned o = (base + (unsigned)m * TH8) * 4u, rb = rowstride * 4u;
+  constexpr unsigned PT = NME * FB_TH32 * 4;   // bytes per ix step
+  const unsigned o = (base + (unsigned)m * FB_TH32) * 4u, rb = rowstride * 4u;
   if (m1) {
     const unsigned c = o + rb + PT;   // (ix, iy)
     float v = w1[0] * fold(r, c, P);
@@ -235,8 +234,8 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       q.m1 = v.meth3c[g.moff + icell];
       q.px = (float)px;
       q.py = (float)py;
-      q.rowstride = (unsigned)(g.nx * g.nME * TH8);
-      q.base = (unsigned)(g.off32 + ((int64_t)(iy - 2) * g.nx + (ix - 2)) * g.nME * TH8);
+      q.rowstride = (unsigned)(g.nx * g.nME * FB_TH32);
+      q.base = (unsigned)(g.off32 + ((int64_t)(iy - 2) * g.nx + (ix - 2)) * g.nME * FB_TH32);
       if (MLP) {
         // per-triplet bound: P_eq2 x the calibrated Frobenius error of the
         // cell's molecular-frame block (bounds every rotated element), with a

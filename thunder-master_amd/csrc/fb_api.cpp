@@ -860,14 +860,14 @@ int fb_commit_tables(fb_ctx *ctx) {
         const double gy = (t0[i00 + (size_t)g.nx * g.nME] - f00) / g.dy;
         meth[g.moff + (size_t)(iy - 1) * g.nx + (ix - 1)] = (gx * gx + gy * gy < 1.0e-6) ? 1 : 0;
       }
-    // FP32 slab [iy][ix][ME][theta], theta padded to 8 so that the five
-    // values of one (point, ME) load as one float4 + one float
+    // FP32 slab [iy][ix][ME][theta]: the five values of one (point, ME) are
+    // contiguous and load as one float4 + one float
     size_t plane = (size_t)g.nx * g.ny;
-    tab3f.resize(tab3f.size() + plane * g.nME * 8, 0.0f);
+    tab3f.resize(tab3f.size() + plane * g.nME * FB_TH32, 0.0f);
     float *dst = tab3f.data() + g.off32;
     for (size_t q = 0; q < plane; q++)
       for (int m = 0; m < g.nME; m++)
-        for (int t = 0; t < FB_NTHETA; t++) dst[(q * g.nME + m) * 8 + t] = (float)th[t]->data[q * g.nME + m];
+        for (int t = 0; t < FB_NTHETA; t++) dst[(q * g.nME + m) * FB_TH32 + t] = (float)th[t]->data[q * g.nME + m];
     int is, js, ks, isub;
     std::tie(is, js, ks, isub) = kv.first;
     // the mixed kernel addresses every charge subtype with the neutral group's stencil

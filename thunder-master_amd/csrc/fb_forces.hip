@@ -1520,12 +1520,11 @@ __device__ __forceinline__ void xzc3(const DView &v, int4 tr, bool charged, doub
 
 // ---------------------------------------------------------------------------
 // Mixed precision (fb_set_precision FB_PREC_MIXED): the interpolation of phase
-// A in FP32 over the FP32 slab [iy][ix][ME][theta8] -- cubic Lagrange weights in
+// A in FP32 over the FP32 slab [iy][ix][ME][theta] -- cubic Lagrange weights in
 // x and y and their derivatives (the Horner form of M_Fdata_3c.f90:636-713 is
 // this polynomial times 36), the Legendre fold of the value and of its cos theta
 // derivative at every stencil point.  The cell, the method, the interpolation
 // arguments and everything of phase B stay binary64.
-#define TH8F 8
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t slab32(const float *tab, int64_t nfloats) {
   return __builtin_amdgcn_make_buffer_rsrc((void *)tab, (short)0, (int)(nfloats * 4), 0x00020000);
@@ -1589,9 +1588,9 @@ __device__ inline int stage_q_mx(const DView &v, int gid, int nme, double x, dou
   dP[3] = (float)((15.0 * cost2 - 3.0) / 2.0);
   dP[4] = (float)(0.5 * cost * (35.0 * cost2 - 15.0));
   const __amdgpu_buffer_rsrc_t r = slab32(v.tab3c32, v.ntab3c32);
-  const unsigned PT = (unsigned)(nme * TH8F * 4);            // bytes per ix step
-  const unsigned RB = (unsigned)(nx * nme * TH8F * 4);       // bytes per iy step
-  const unsigned base = (unsigned)(g.off32 + ((int64_t)(iy - 2) * nx + (ix - 2)) * nme * TH8F) * 4u;
+  const unsigned PT = (unsigned)(nme * FB_TH32 * 4);            // bytes per ix step
+  const unsigned RB = (unsigned)(nx * nme * FB_TH32 * 4);       // bytes per iy step
+  const unsigned base = (unsigned)(g.off32 + ((int64_t)(iy - 2) * nx + (ix - 2)) * nme * FB_TH32) * 4u;
   const int nm0 = (nme == 10) ? 6 : (nme == 3 ? 2 : 1);
   float wx[4], wy[4], dwx[4], dwy[4];
   cubic_wd((float)px, wx, dwx);
@@ -1610,7 +1609,7 @@ __device__ inline int stage_q_mx(const DView &v, int gid, int nme, double x, dou
   }
   const float idx = (float)(1.0 / dx), idy = (float)(1.0 / dy);
   for (int m = q; m < nme; m += 4) {
-    const unsigned o = base + (unsigned)(m * TH8F * 4);
+    const unsigned o = base + (unsigned)(m * FB_TH32 * 4);
     float fv, fp, fx, fy;
     if (three) {
       const unsigned c = o + RB + PT;   // (ix, iy)

@@ -61,12 +61,16 @@ struct DGrid2c {
   int64_t pden_off;
 };
 
+// theta values per (point, ME) in the FP32 3c slab, packed without padding:
+// one float4 + one float per load, dword aligned
+#define FB_TH32 5
+
 // one (is, js, ks, isub) group of the five Legendre theta cells
 struct DGroup3c {
   int nME, nx, ny, munu;
   double xmax, ymax, dx, dy;
   int64_t off;       // FP64 slab, layout [theta][ny][nx][nME]
-  int64_t off32;     // FP32 slab, layout [ny][nx][ME][theta8]
+  int64_t off32;     // FP32 slab, layout [ny][nx][ME][theta], FB_TH32 values per (point, ME)
   int64_t moff;      // method table [ny][nx] (uint8, 1 = 3-point rule), e.FDATA/M_Fdata_3c.f90:383-385
 };
 
