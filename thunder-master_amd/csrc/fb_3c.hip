@@ -63,7 +63,7 @@ __device__ inline void d_geo3(const DView &v, int4 tr, Geo3 &G) {
 
 __global__ __launch_bounds__(T3) void k_3c_fp64(DView v) {
   __shared__ double s_c[T3][NB + 1];
-  const int64_t ts = (int64_t)blockIdx.x * T3;
+  const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3;
   const int64_t te = min(ts + T3, v.ntrip);
   int64_t first, last, r0, r1;
   if (!tile_segments(v, ts, te, first, last, r0, r1)) return;
@@ -177,7 +177,7 @@ __device__ __forceinline__ int triplet_fp64_sp(const DView &v, int64_t t, const 
 
 __global__ __launch_bounds__(T3) void k_3c_fp64_sp(DView v) {
   __shared__ double s_c[T3][NB + 1];
-  const int64_t ts = (int64_t)blockIdx.x * T3;
+  const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3;
   const int64_t te = min(ts + T3, v.ntrip);
   int64_t first, last, r0, r1;
   if (!tile_segments(v, ts, te, first, last, r0, r1)) return;

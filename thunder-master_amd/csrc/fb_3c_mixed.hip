@@ -121,7 +121,7 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   static_assert(sizeof(TPrm) >= (NB + 1) * sizeof(float), "contribution row must fit in a parameter row");
   const int tid = threadIdx.x;
   // segment ownership of this tile, from the list build
-  const int4 td = v.tiles[CH == 64 ? 0 : (CH == 128 ? 1 : 2)][blockIdx.x];
+  const int4 td = v.tiles[CH == 64 ? 0 : (CH == 128 ? 1 : 2)][xcd_tile(blockIdx.x, gridDim.x)];
   const int64_t first = td.x, last = td.y, r0 = td.z, r1 = td.w;
   if (first > last) return;
   // owned segments [first, last] (at most CH: each starts inside the tile).

@@ -15,6 +15,16 @@
 
 #define FB_MO 4   // orbitals per species handled by the kernels (s and p shells)
 
+// Tile of workgroup b in a grid of n.  The dispatcher hands workgroups to the
+// eight XCDs round-robin (workgroup b to XCD b mod 8) and each XCD has its own
+// L2; renumbering so that XCD x runs one contiguous range of tiles keeps the
+// gathers of neighbouring tiles (the same atoms' blocks and positions) in one
+// L2.  A bijection of [0, n): any tile order is valid for the kernels that use it.
+__device__ __forceinline__ int64_t xcd_tile(unsigned b, unsigned n) {
+  const unsigned x = b & 7u, q = b >> 3, per = n >> 3, rem = n & 7u;
+  return x < rem ? (int64_t)x * (per + 1) + q : (int64_t)rem * (per + 1) + (int64_t)(x - rem) * per + q;
+}
+
 __device__ __forceinline__ double d_distance(const double *a, const double *b) {
   double d1 = b[0] - a[0], d2 = b[1] - a[1], d3 = b[2] - a[2];
   return sqrt(d1 * d1 + d2 * d2 + d3 * d3);

@@ -554,7 +554,7 @@ __device__ inline void d_f3_rot(const DView &v, const DGroup3c &g, int in1, int 
 #define T3D 64
 
 __global__ __launch_bounds__(T3D) void k_d3c(DView v, DForce F) {
-  const int64_t t = (int64_t)blockIdx.x * T3D + threadIdx.x;
+  const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * T3D + threadIdx.x;
   if (t >= v.ntrip) return;
   const int4 tr = F.trip[t];
   const int tp = F.tpair[t];
@@ -1803,7 +1803,7 @@ template <bool MX>
 __global__ __launch_bounds__(T3Q) void k_d3c_sp(DView v, DForce Fo) {
   __shared__ double sI[T3Q / 4][SIROW];
   const int slot = threadIdx.x >> 2, q = threadIdx.x & 3;
-  const int64_t t = (int64_t)blockIdx.x * (T3Q / 4) + slot;
+  const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * (T3Q / 4) + slot;
   if (t >= v.ntrip) return;
   const int4 tr = Fo.trip[t];
   const int in1 = v.imass[tr.y], in2 = v.imass[tr.z], indna = v.imass[tr.x];
@@ -1843,7 +1843,7 @@ __global__ __launch_bounds__(T3Q) void k_d3c_sp(DView v, DForce Fo) {
 __global__ __launch_bounds__(T3Q) void k_d3c_sp_q(DView v, DForce Fo) {
   __shared__ double sI[T3Q / 4][SIROW];
   const int slot = threadIdx.x >> 2, q = threadIdx.x & 3;
-  const int64_t t = (int64_t)blockIdx.x * (T3Q / 4) + slot;
+  const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * (T3Q / 4) + slot;
   if (t >= v.ntrip) return;
   const int4 tr = Fo.trip[t];
   const int indna = v.imass[tr.x];

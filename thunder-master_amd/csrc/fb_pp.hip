@@ -502,7 +502,7 @@ __global__ void k_pp_mpair(DView v, DPP q) {
 // into LDS, one owner per segment adds them in list order onto the block
 __global__ __launch_bounds__(T3P) void k_vnl_3c(DView v, DPP q) {
   __shared__ double s_c[T3P][NBP + 1];
-  const int64_t ts = (int64_t)blockIdx.x * T3P;
+  const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3P;
   const int64_t te = min(ts + T3P, q.ntrip);
   const int64_t sgs = q.trip_seg[ts];
   const int64_t first = (q.seg_off[sgs] == ts) ? sgs : sgs + 1;
@@ -724,7 +724,7 @@ __device__ __forceinline__ void triplet_vnl_sp(const DPP &q, int indna, const do
 
 __global__ __launch_bounds__(T3P) void k_vnl_3c_sp(DView v, DPP q) {
   __shared__ double s_c[T3P][NBP + 1];
-  const int64_t ts = (int64_t)blockIdx.x * T3P;
+  const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3P;
   const int64_t te = min(ts + T3P, q.ntrip);
   const int64_t sgs = q.trip_seg[ts];
   const int64_t first = (q.seg_off[sgs] == ts) ? sgs : sgs + 1;

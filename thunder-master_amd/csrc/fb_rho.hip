@@ -190,7 +190,7 @@ template <bool W>
 __global__ __launch_bounds__(T3R) void k_rho_3c(DView v, int stride) {
   extern __shared__ double s_c[];
   __shared__ int s_n[T3R];
-  const int64_t ts = (int64_t)blockIdx.x * T3R;
+  const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3R;
   const int64_t te = min(ts + T3R, v.ntrip);
   int64_t first, last, r0, r1;
   if (!tile_segments(v, ts, te, first, last, r0, r1)) return;
@@ -465,7 +465,7 @@ template <bool W>
 __global__ __launch_bounds__(T3R) void k_rho_3c_sp(DView v, int stride) {
   extern __shared__ double s_c[];
   __shared__ int s_n[T3R];
-  const int64_t ts = (int64_t)blockIdx.x * T3R;
+  const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3R;
   const int64_t te = min(ts + T3R, v.ntrip);
   int64_t first, last, r0, r1;
   if (!tile_segments(v, ts, te, first, last, r0, r1)) return;
