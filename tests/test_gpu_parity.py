@@ -183,6 +183,46 @@ def test_si4096_mixed_vs_fp64():
     assert np.array_equal(back[back], np.arange(len(back)))
 
 
+def test_mixed_segments_longer_than_chunk():
+    """A dense Si cluster in which every pair is a neighbour pair and every
+    other atom a common neighbour: each Vna block takes N-2 = 268 triplets,
+    more than any workgroup chunk (64, 128, 256), so every workgroup sums its
+    segment over several chunks.  The three chunk sizes split the same
+    binary64 sums differently and must agree bit for bit; against binary64 the
+    bound is the FP32 rounding of 268 contributions per element (ten times
+    the per-element bar of the benchmark workloads, whose blocks take ~14),
+    far below one lost or repeated contribution."""
+    from fireball_hip import synth
+    rng = np.random.default_rng(7)
+    pos = np.zeros((0, 3))
+    while len(pos) < 270:          # ball of 2.8 A: every distance below 2 rc = 5.7 A
+        p = rng.uniform(-2.8, 2.8, 3)
+        if np.linalg.norm(p) <= 2.8 and (len(pos) == 0 or np.min(np.linalg.norm(pos - p, axis=1)) >= 0.55):
+            pos = np.vstack([pos, p])
+    pos = pos + 12.0
+    st = synth.Structure(pos, np.ones(len(pos), np.int32), np.eye(3) * 40.0, 1)
+    ts = synth.make_tables([synth.SI])
+    asm = _gpu(ts, [st], precision="fp64")
+    L = asm.lists(0)
+    n = len(pos)
+    assert np.all(L["neighn"] == n) and np.all(L["ncommon"] == (n - 1) * (n - 2))
+    off = np.concatenate([[0], np.cumsum(L["neighn"])])
+    seg = np.bincount(off[L["com_iatom_j"] - 1] + L["com_mneigh"] - 1)
+    assert seg[seg > 0].min() == n - 2 > 256
+    ref = asm.blocks("VNA")
+    asm.set_precision("mixed")
+    got = {}
+    for chunk in (64, 128, 256):
+        asm.set_chunk3c(chunk)
+        asm.assemble_all()
+        got[chunk] = asm.blocks("VNA")
+        err = float(np.max(np.abs(got[chunk] - ref)))
+        print(f"dense cluster chunk {chunk}: mixed vs fp64 VNA max-abs {err:.3e} eV "
+              f"(max |VNA| {float(np.max(np.abs(ref))):.3e})", flush=True)
+        assert err <= 10 * TOL_MIXED, chunk
+    assert np.array_equal(got[64], got[128]) and np.array_equal(got[64], got[256])
+
+
 def test_generic_shells_against_oracle():
     """Species outside the s / sp fast path (two s shells) run the generic kernels."""
     from fireball_hip import synth

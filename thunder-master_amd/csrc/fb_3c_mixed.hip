@@ -1,10 +1,10 @@
 // Three-centre bcna assembly, mixed precision (M_assemble_3c_DOGS.f90:86-340),
 // for s/p species with canonical tables.
 //
-// Same segment ownership as the binary64 kernels (a workgroup owns the target
-// blocks whose triplet segment starts in its tile and sums each segment in the
-// reference's order), but the work inside a chunk of triplets is spread over
-// all lanes in four phases:
+// A workgroup owns whole target segments (the triplets of one Vna block, in
+// the reference's order), packed at list build into tiles of at most one
+// chunk of triplets, so that it usually makes a single pass.  The work inside
+// a chunk is spread over all lanes in four phases:
 //   A  one lane per triplet: geometry.  Positions, bond and third-atom vectors,
 //      x = |r_na - bond centre| and y = bond length are binary64 with the
 //      reference's expressions, and so are the decisions taken on them: the
@@ -19,7 +19,7 @@
 //   C  one lane per triplet: sin(theta) on mvalue = 1 elements, Slater-Koster
 //      rotation (fb_sp.h) in FP32, times P_eq2, charged terms.
 //   D  one lane per (owned segment, block element): binary64 accumulation of
-//      the FP32 contributions in segment order in LDS; one store per element.
+//      the FP32 contributions in segment order from LDS, stored to the block.
 // Binary64 adds run at the FP32 rate on gfx950 without packed math, so the
 // accumulation is plain binary64 rather than a compensated FP32 sum.
 #include "fb_device.h"
@@ -113,6 +113,8 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   __shared__ float sF[CH][NME3 + 1];
   __shared__ TChg sx[CHARGED ? CH : 1];
   __shared__ int64_t sSeg[CH + 1];
+  __shared__ int64_t sBo[CH];
+  __shared__ int sNb[CH];
   __shared__ float sU[MLP ? CH : 1][4];
   __shared__ int sG[MLP ? CH : 1];
   __shared__ unsigned sCnt[2];
@@ -120,18 +122,24 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   float(*sC)[NB + 1] = reinterpret_cast<float(*)[NB + 1]>(&sq[0]);
   static_assert(sizeof(TPrm) >= (NB + 1) * sizeof(float), "contribution row must fit in a parameter row");
   const int tid = threadIdx.x;
-  // segment ownership of this tile, from the list build
-  const int4 td = v.tiles[CH == 64 ? 0 : (CH == 128 ? 1 : 2)][xcd_tile(blockIdx.x, gridDim.x)];
+  // the tile's whole segments [first, last], from the list build: at most CH
+  // triplets (one pass) unless a single segment is longer
+  const int kt = CH == 64 ? 0 : (CH == 128 ? 1 : 2);
+  const int4 td = v.tiles[kt][xcd_tile(blockIdx.x, gridDim.x)];
   const int64_t first = td.x, last = td.y, r0 = td.z, r1 = td.w;
   if (first > last) return;
-  // owned segments [first, last] (at most CH: each starts inside the tile).
   // Accumulator slot sl = (segment, block element); lane tid owns the slots
-  // tid + j*CH, j < NB, in registers.
+  // tid + j*CH, j < NB.  The workgroup owns the target blocks, so a segment
+  // that spans chunks keeps its binary64 partial sum in the block itself.
   const int nown = (int)(last - first + 1);
   for (int k = tid; k <= nown; k += CH) sSeg[k] = v.seg_off[first + k];
-  double acc[NB];
-#pragma unroll
-  for (int j = 0; j < NB; j++) acc[j] = 0.0;
+  for (int k = tid; k < nown; k += CH) {
+    const int p = v.seg_pair[first + k];
+    const int64_t b0 = v.boff[p];
+    sBo[k] = b0;
+    sNb[k] = (int)(v.boff[p + 1] - b0);
+  }
+  double *const vna = v.mat[FB_MAT_VNA];
   int status = 0;
   const __amdgpu_buffer_rsrc_t tab = slab_rsrc(v.tab3c32, v.ntab3c32);
   if (MLP && tid < 2) sCnt[tid] = 0u;
@@ -406,32 +414,25 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
     }
     __syncthreads();
     // ---------------- phase D: ordered binary64 accumulation, one lane per
-    // (owned segment, block element), segment order preserved
+    // (owned segment, block element), segment order preserved: the sum starts
+    // from zero in the segment's first chunk and continues from the block
+    // (written by this lane in the previous chunk) in a later one
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       const int sl = tid + j * CH;
       if (sl < nown * NB) {
         const int sg = sl / NB, e = sl - sg * NB;
-        const int64_t k0 = max(sSeg[sg], c0), k1 = min(sSeg[sg + 1], c0 + nt);
-        double a = acc[j];
-        for (int64_t k = k0; k < k1; k++) a += (double)sC[k - c0][e];
-        acc[j] = a;
+        const int64_t s0 = sSeg[sg];
+        const int64_t k0 = max(s0, c0), k1 = min(sSeg[sg + 1], c0 + nt);
+        if (k0 < k1 && e < sNb[sg]) {
+          double *const o = vna + sBo[sg] + e;
+          double a = (s0 < c0) ? *o : 0.0;
+          for (int64_t k = k0; k < k1; k++) a += (double)sC[k - c0][e];
+          *o = a;
+        }
       }
     }
     __syncthreads();
-  }
-#pragma unroll
-  for (int j = 0; j < NB; j++) {
-    const int sl = tid + j * CH;
-    if (sl < nown * NB) {
-      const int sg = sl / NB, e = sl - sg * NB;
-      const int p = v.seg_pair[first + sg];
-      const int ia = v.pair_atom[p];
-      const int s = v.atom_struct[ia];
-      const int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
-      const int nb = v.sp.norb[v.imass[ia] - 1] * v.sp.norb[v.imass[ja] - 1];
-      if (e < nb) v.mat[FB_MAT_VNA][v.boff[p] + e] = acc[j];
-    }
   }
   if (MLP) {
     __syncthreads();
@@ -443,31 +444,27 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   if (status) atomicOr(v.status, status);
 }
 
-template <bool CHARGED>
+template <bool CHARGED, bool MLP>
 static int launch_mixed(const DView &v, int chunk, hipStream_t st) {
-  if (chunk == 256) {
-    hipLaunchKernelGGL((k_3c_mixed<256, CHARGED>), dim3((unsigned)((v.ntrip + 255) / 256)), dim3(256), 0, st, v);
-  } else if (chunk == 64) {
-    hipLaunchKernelGGL((k_3c_mixed<64, CHARGED>), dim3((unsigned)((v.ntrip + 63) / 64)), dim3(64), 0, st, v);
+  const int kt = chunk == 64 ? 0 : (chunk == 256 ? 2 : 1);
+  if (v.ntiles[kt] == 0) return 0;
+  const dim3 grid((unsigned)v.ntiles[kt]);
+  if (kt == 2) {
+    hipLaunchKernelGGL((k_3c_mixed<256, CHARGED, MLP>), grid, dim3(256), 0, st, v);
+  } else if (kt == 0) {
+    hipLaunchKernelGGL((k_3c_mixed<64, CHARGED, MLP>), grid, dim3(64), 0, st, v);
   } else {
-    hipLaunchKernelGGL((k_3c_mixed<128, CHARGED>), dim3((unsigned)((v.ntrip + 127) / 128)), dim3(128), 0, st, v);
+    hipLaunchKernelGGL((k_3c_mixed<128, CHARGED, MLP>), grid, dim3(128), 0, st, v);
   }
   return (int)hipGetLastError();
 }
 
 extern "C" int fbk_3c_mixed_mlp(const DView &v, int chunk, hipStream_t st) {
   if (v.ntrip == 0) return 0;
-  if (chunk == 256) {
-    hipLaunchKernelGGL((k_3c_mixed<256, false, true>), dim3((unsigned)((v.ntrip + 255) / 256)), dim3(256), 0, st, v);
-  } else if (chunk == 64) {
-    hipLaunchKernelGGL((k_3c_mixed<64, false, true>), dim3((unsigned)((v.ntrip + 63) / 64)), dim3(64), 0, st, v);
-  } else {
-    hipLaunchKernelGGL((k_3c_mixed<128, false, true>), dim3((unsigned)((v.ntrip + 127) / 128)), dim3(128), 0, st, v);
-  }
-  return (int)hipGetLastError();
+  return launch_mixed<false, true>(v, chunk, st);
 }
 
 extern "C" int fbk_3c_mixed(const DView &v, int chunk, int charged, hipStream_t st) {
   if (v.ntrip == 0) return 0;
-  return charged ? launch_mixed<true>(v, chunk, st) : launch_mixed<false>(v, chunk, st);
+  return charged ? launch_mixed<true, false>(v, chunk, st) : launch_mixed<false, false>(v, chunk, st);
 }

@@ -45,7 +45,6 @@ int fbk_seg_fill(int64_t ntrip, const int *sorted_pair, const int *segid_incl, i
                  int *trip_seg, int *pair_tgt, hipStream_t st);
 int fbk_gather_trip(int64_t ntrip, const int *perm, const int4 *in, int4 *out, hipStream_t st);
 int fbk_trip_aux(const DView &v, int *aux, hipStream_t st);
-int fbk_tiles(const DView &v, int ch, int4 *tiles, hipStream_t st);
 int fbk_fill_i32(int *a, int64_t n, int val, hipStream_t st);
 int fbk_2c(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64(const DView &v, hipStream_t st);
@@ -233,6 +232,7 @@ struct fb_ctx {
   DevBuf d_neighn, d_noff, d_neigh_j, d_neigh_b, d_neigh_back, d_neigh_self, d_pair_atom, d_bsz, d_boff;
   DevBuf d_ncommon, d_toff, d_trip_orig, d_tpair_orig, d_trip, d_tpair, d_perm_in, d_perm, d_flag, d_segincl,
       d_seg_off, d_seg_pair, d_trip_seg, d_pair_tgt, d_trip_aux, d_tiles[3];
+  int ntiles[3] = {0, 0, 0};
   DevBuf d_tmp, d_status;
   DevBuf d_mat[FB_NMAT];
   int64_t npairs = 0, ntrip = 0, nseg = 0, blen = 0;
@@ -415,7 +415,10 @@ struct fb_ctx {
     v.seg_pair = d_seg_pair.as<int>();
     v.pair_tgt = d_pair_tgt.as<int>();
     v.trip_aux = d_trip_aux.as<int>();
-    for (int k = 0; k < 3; k++) v.tiles[k] = d_tiles[k].as<int4>();
+    for (int k = 0; k < 3; k++) {
+      v.tiles[k] = d_tiles[k].as<int4>();
+      v.ntiles[k] = ntiles[k];
+    }
     v.trip_seg = d_trip_seg.as<int>();
     for (int i = 0; i < FB_NMAT; i++) v.mat[i] = d_mat[i].as<double>();
     v.mlp = d_mlp.as<DMlp>();
@@ -1256,6 +1259,7 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
                            v.status, st);
   }));
   ctx->nseg = 0;
+  for (int k = 0; k < 3; k++) ctx->ntiles[k] = 0;
   if (T > 0) {
     // stable radix sort of triplet ids by target pair
     int end_bit = 1;
@@ -1297,12 +1301,29 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
     v.ntrip = T;
     v.nseg = nseg;
     RC(ctx->launch("trip_aux", [&] { return fbk_trip_aux(v, ctx->d_trip_aux.as<int>(), st); }));
-    const int chs[3] = {64, 128, 256};
-    for (int k = 0; k < 3; k++) {
-      FB_HIP(ctx->d_tiles[k].ensure(sizeof(int4) * ((T + chs[k] - 1) / chs[k])));
-      RC(ctx->launch("tiles", [&] { return fbk_tiles(v, chs[k], ctx->d_tiles[k].as<int4>(), st); }));
-    }
+    // mixed-kernel tiles: whole segments packed greedily, at most one chunk of
+    // triplets per tile unless a single segment is longer, so that a
+    // workgroup makes one pass over its triplets
+    std::vector<int64_t> hso((size_t)nseg + 1);
+    FB_HIP(hipMemcpyAsync(hso.data(), ctx->d_seg_off.as<int64_t>(), sizeof(int64_t) * (nseg + 1), hipMemcpyDeviceToHost,
+                          st));
     FB_HIP(hipStreamSynchronize(st));
+    const int chs[3] = {64, 128, 256};
+    std::vector<int4> ht;
+    for (int k = 0; k < 3; k++) {
+      ht.clear();
+      for (int s0 = 0; s0 < nseg;) {
+        int e = s0;
+        while (e + 1 < nseg && hso[e + 2] - hso[s0] <= chs[k]) e++;
+        ht.push_back(make_int4(s0, e, (int)hso[s0], (int)hso[e + 1]));
+        s0 = e + 1;
+      }
+      ctx->ntiles[k] = (int)ht.size();
+      FB_HIP(ctx->d_tiles[k].ensure(sizeof(int4) * std::max<size_t>(ht.size(), 1)));
+      if (!ht.empty())
+        FB_HIP(hipMemcpyAsync(ctx->d_tiles[k].p, ht.data(), sizeof(int4) * ht.size(), hipMemcpyHostToDevice, st));
+      FB_HIP(hipStreamSynchronize(st));
+    }
   }
   // block offsets
   v = ctx->view();

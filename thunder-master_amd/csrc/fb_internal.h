@@ -128,8 +128,9 @@ struct DView {
   const int *seg_pair;          // [nseg] target pair of each segment
   const int *pair_tgt;          // [npairs] 1 where some triplet accumulates into the pair's Vna block
   const int *trip_aux;          // [ntrip] neutral 3c group | structure << 16 (species and cells of the record)
-  const int4 *tiles[3];         // per 3c chunk of 64, 128, 256 triplets: owned segments first, last and
-                                // their triplet range r0, r1 (first > last: the tile owns none)
+  const int4 *tiles[3];         // per 3c chunk of 64, 128, 256 triplets: segments first..last packed whole,
+                                // their triplet range r0, r1 at most one chunk unless one segment is longer
+  int ntiles[3];
   const int *trip_seg;          // [ntrip] segment of each sorted triplet
   // learned 3c surrogate (one DMlp per 3c group; ready = 0 where untrained)
   const struct DMlp *mlp;

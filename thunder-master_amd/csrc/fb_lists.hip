@@ -335,32 +335,9 @@ __global__ void k_trip_aux(DView v, int *aux) {
 
 // segment ownership of every tile of ch consecutive sorted triplets: the
 // segments that start inside the tile, and the triplets they span
-__global__ void k_tiles(DView v, int ch, int4 *tiles) {
-  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t ntile = (v.ntrip + ch - 1) / ch;
-  if (b >= ntile) return;
-  const int64_t ts = b * ch, te = min(ts + (int64_t)ch, v.ntrip);
-  const int64_t sgs = v.trip_seg[ts];
-  const int64_t first = (v.seg_off[sgs] == ts) ? sgs : sgs + 1;
-  const int64_t last = v.trip_seg[te - 1];
-  int4 d = make_int4((int)first, (int)last, 0, 0);
-  if (first <= last) {
-    d.z = (int)v.seg_off[first];
-    d.w = (int)v.seg_off[last + 1];
-  }
-  tiles[b] = d;
-}
-
 extern "C" int fbk_trip_aux(const DView &v, int *aux, hipStream_t st) {
   if (v.ntrip == 0) return 0;
   hipLaunchKernelGGL(k_trip_aux, dim3((unsigned)((v.ntrip + 255) / 256)), dim3(256), 0, st, v, aux);
-  return (int)hipGetLastError();
-}
-
-extern "C" int fbk_tiles(const DView &v, int ch, int4 *tiles, hipStream_t st) {
-  const int64_t ntile = (v.ntrip + ch - 1) / ch;
-  if (ntile == 0) return 0;
-  hipLaunchKernelGGL(k_tiles, dim3((unsigned)((ntile + 255) / 256)), dim3(256), 0, st, v, ch, tiles);
   return (int)hipGetLastError();
 }
 
