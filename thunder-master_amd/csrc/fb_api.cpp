@@ -814,6 +814,22 @@ int fb_commit_tables(fb_ctx *ctx) {
   std::vector<DGroup3c> grps;
   std::vector<std::array<int, 4>> grpkey;
   std::vector<unsigned char> meth;
+  // method of every interior cell of a group: gradient of ME 1 of theta cell
+  // 1 (data t0, layout [ny][nx][nME]) against 1e-6 (e.FDATA/M_Fdata_3c.f90:
+  // 383-385), the same binary64 expression the kernels would evaluate per
+  // triplet; 1 = the 3-point rule
+  auto method_table = [&](DGroup3c &g, const double *t0) {
+    g.moff = (int64_t)meth.size();
+    meth.resize(meth.size() + (size_t)g.nx * g.ny, 0);
+    for (int iy = 2; iy <= g.ny - 2; iy++)
+      for (int ix = 2; ix <= g.nx - 2; ix++) {
+        const size_t i00 = ((size_t)(iy - 1) * g.nx + (ix - 1)) * g.nME;
+        const double f00 = t0[i00];
+        const double gx = (t0[i00 + g.nME] - f00) / g.dx;
+        const double gy = (t0[i00 + (size_t)g.nx * g.nME] - f00) / g.dy;
+        meth[g.moff + (size_t)(iy - 1) * g.nx + (ix - 1)] = (gx * gx + gy * gy < 1.0e-6) ? 1 : 0;
+      }
+  };
   std::vector<double> tab3;
   std::vector<float> tab3f;
   for (auto &kv : groups) {
@@ -849,20 +865,7 @@ int fb_commit_tables(fb_ctx *ctx) {
     nu.insert(nu.end(), cl->nu.begin(), cl->nu.end());
     mv.insert(mv.end(), cl->mv.begin(), cl->mv.end());
     for (int t = 0; t < FB_NTHETA; t++) tab3.insert(tab3.end(), th[t]->data.begin(), th[t]->data.end());
-    // method of every interior cell: gradient of ME 1 of theta cell 1 against
-    // 1e-6 (e.FDATA/M_Fdata_3c.f90:383-385), the same binary64 expression the
-    // kernels would evaluate per triplet
-    g.moff = (int64_t)meth.size();
-    meth.resize(meth.size() + (size_t)g.nx * g.ny, 0);
-    for (int iy = 2; iy <= g.ny - 2; iy++)
-      for (int ix = 2; ix <= g.nx - 2; ix++) {
-        const double *t0 = th[0]->data.data();
-        const size_t i00 = ((size_t)(iy - 1) * g.nx + (ix - 1)) * g.nME;
-        const double f00 = t0[i00];
-        const double gx = (t0[i00 + g.nME] - f00) / g.dx;
-        const double gy = (t0[i00 + (size_t)g.nx * g.nME] - f00) / g.dy;
-        meth[g.moff + (size_t)(iy - 1) * g.nx + (ix - 1)] = (gx * gx + gy * gy < 1.0e-6) ? 1 : 0;
-      }
+    method_table(g, th[0]->data.data());
     // FP32 slab [iy][ix][ME][theta]: the five values of one (point, ME) are
     // contiguous and load as one float4 + one float
     size_t plane = (size_t)g.nx * g.ny;
@@ -921,7 +924,7 @@ int fb_commit_tables(fb_ctx *ctx) {
     g.dy = c1->ymax / (c1->ny - 1.0);
     g.off = (int64_t)tab3.size();
     g.off32 = -1;
-    g.moff = -1;
+    method_table(g, th[0]->data.data());
     const HCell3c *cl = th[FB_NTHETA - 1];
     g.munu = (int)mu.size();
     mu.insert(mu.end(), cl->mu.begin(), cl->mu.end());

@@ -434,96 +434,220 @@ __global__ __launch_bounds__(WRS) void k_rho_2c_sp(DView v, int stride) {
   if (status) atomicOr(v.status, status);
 }
 
-// the terms of one triplet of class CLS into its LDS row
-template <bool W, int CLS>
-__device__ __forceinline__ int triplet_rho_sp(const DView &v, int4 tr, int in1, int in2, int indna, double x, double z,
-                                              double cost, const double *pm, double *row) {
-  constexpr int NSI = (CLS & 2) ? 2 : 1, NSJ = (CLS & 1) ? 2 : 1;
-  constexpr int NB = W ? NSI * NSJ : ((CLS & 2) ? 4 : 1) * ((CLS & 1) ? 4 : 1);
-  int status = 0;
-  double X[16];
-  const int nsorp = v.sp.nssh[indna - 1];
-  for (int isorp = 1; isorp <= nsorp; isorp++) {
-    const double q = v.qin[tr.x * FB_MAXSH + isorp - 1];
-    const int gid = group3c_r(v, in1, in2, indna, W ? 1 : 0, isorp);
-    if (W) {
-      double F[NSI * NSJ];
-      status |= getMEs_3c_sp<NSI * NSJ, NSI * NSJ>(v, gid, x, z, cost, F);
-      shell_block<NSI, NSJ>(F, X);
-    } else {
-      double F[nme3c_of(CLS)];
-      status |= getMEs_3c_sp<nme3c_of(CLS)>(v, gid, x, z, cost, F);
-      rot3c_sp<double>(CLS, pm, F, X);
-    }
-#pragma unroll
-    for (int i = 0; i < NB; i++) row[(isorp - 1) * NBR + i] = X[i] * q;
-  }
-  return status;
-}
+// assemble_rho_3c / assemble_rho_weighted_3c for s/p species with canonical
+// density tables.  A workgroup of RW lanes owns whole target segments, packed
+// at list build into tiles of at most RT triplets (the mixed kernel's 64-triplet
+// tiles), and works through them in four phases:
+//   A  one lane per triplet: binary64 geometry (rho_geo), Legendre P0..P4 and
+//      sin(theta), and per shell isorp of the third atom the group, the grid
+//      cell, px, py, the method of the cell and Qin, into LDS.
+//   B  one lane per (triplet, isorp, matrix element): the interpolation of
+//      getMEs_3c_sp for that element -- the five theta cells in order, each
+//      with the 16-point form or the 3-point rule, summed with P_theta --
+//      times sin(theta) where mvalue = 1.
+//   C  one lane per (triplet, isorp): rotation (or the shell-block reorder of
+//      the weighted density) times Qin into the triplet's LDS row.
+//   D  one lane per (owned segment, block element): the rows added onto the
+//      two-centre block in list order, isorp inner, as the reference does.
+// Every element goes through the same binary64 operations in the same order
+// as the one-lane-per-triplet form, so the blocks are bit-identical to it.
+#define RT 64           // triplets per tile
+#define RW 256          // lanes per workgroup
+#define RSO 2           // shells of an s/p third atom
+
+struct RTri {           // per triplet
+  double p[5], sint, pm[9];
+  int cls, nsorp;
+};
+struct RShell {         // per (triplet, isorp)
+  double px, py, q;
+  int64_t base;         // FP64 slab index of stencil corner (ix-1, iy-1), theta cell 1, ME 1
+  int rs, ts, nme, m1;  // row and theta strides (elements), ME count (0: no table), 3-point rule
+};
 
 template <bool W>
-__global__ __launch_bounds__(T3R) void k_rho_3c_sp(DView v, int stride) {
-  extern __shared__ double s_c[];
-  __shared__ int s_n[T3R];
-  const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3R;
-  const int64_t te = min(ts + T3R, v.ntrip);
-  int64_t first, last, r0, r1;
-  if (!tile_segments(v, ts, te, first, last, r0, r1)) return;
-  const int64_t my = first + threadIdx.x;
-  const bool own = my <= last;
-  double *out = v.mat[W ? FB_MAT_RHOW_IN : FB_MAT_RHO_IN];
-  int64_t a = 0, b = 0, dst = 0;
-  int nb = 0;
-  double acc[NBR];
-  if (own) {
-    a = v.seg_off[my];
-    b = v.seg_off[my + 1];
-    const int p = v.seg_pair[my];
-    const int ia = v.pair_atom[p];
-    const int ja = v.st[v.atom_struct[ia]].atom_off + v.neigh_j[p] - 1;
-    const int in1 = v.imass[ia], in2 = v.imass[ja];
-    nb = W ? v.sp.nssh[in1 - 1] * v.sp.nssh[in2 - 1] : v.sp.norb[in1 - 1] * v.sp.norb[in2 - 1];
-    dst = W ? v.wboff[p] : v.boff[p];
-#pragma unroll
-    for (int i = 0; i < NBR; i++) acc[i] = (i < nb) ? out[dst + i] : 0.0;
+__global__ __launch_bounds__(RW) void k_rho_3c_ph(DView v) {
+  constexpr int NMX = W ? 4 : 10;          // matrix elements per (triplet, isorp)
+  constexpr int NBX = W ? 4 : 16;          // block elements
+  constexpr int ROW = NBX + 1;             // LDS row stride (doubles)
+  constexpr size_t SZ_A = sizeof(RTri) * RT + sizeof(RShell) * RT * RSO + sizeof(double) * RT * RSO * NMX;
+  constexpr size_t SZ_R = sizeof(double) * RT * RSO * ROW;
+  __shared__ __align__(16) unsigned char sbuf[SZ_A > SZ_R ? SZ_A : SZ_R];
+  __shared__ int64_t sSeg[RT + 1], sBo[RT];
+  __shared__ int sNb[RT], sNs[RT];
+  RTri *sT = reinterpret_cast<RTri *>(sbuf);
+  RShell *sS = reinterpret_cast<RShell *>(sbuf + sizeof(RTri) * RT);
+  double *sF = reinterpret_cast<double *>(sbuf + sizeof(RTri) * RT + sizeof(RShell) * RT * RSO);
+  double *sR = reinterpret_cast<double *>(sbuf);   // rows, after phase C has read A and B
+  const int tid = threadIdx.x;
+  const int4 td = v.tiles[0][xcd_tile(blockIdx.x, gridDim.x)];
+  const int64_t first = td.x, last = td.y, r0 = td.z, r1 = td.w;
+  if (first > last) return;
+  const int nown = (int)(last - first + 1);
+  double *const out = v.mat[W ? FB_MAT_RHOW_IN : FB_MAT_RHO_IN];
+  const int64_t *const off = W ? v.wboff : v.boff;
+  for (int k = tid; k <= nown; k += RW) sSeg[k] = v.seg_off[first + k];
+  for (int k = tid; k < nown; k += RW) {
+    const int p = v.seg_pair[first + k];
+    const int64_t b0 = off[p];
+    sBo[k] = b0;
+    sNb[k] = (int)(off[p + 1] - b0);
   }
   int status = 0;
-  for (int64_t c0 = r0; c0 < r1; c0 += T3R) {
-    const int64_t t = c0 + threadIdx.x;
-    int nsorp = 0;
-    if (t < r1) {
-      const int4 tr = v.trip[t];
+  for (int64_t c0 = r0; c0 < r1; c0 += RT) {
+    const int nt = (int)min((int64_t)RT, r1 - c0);
+    // ---------------- phase A
+    if (tid < nt) {
+      const int4 tr = v.trip[c0 + tid];
       const int in1 = v.imass[tr.y], in2 = v.imass[tr.z], indna = v.imass[tr.x];
-      double x, z, cost, pm[9];
-      rho_geo(v, tr, x, z, cost, pm);
-      nsorp = v.sp.nssh[indna - 1];
-      const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
-      double *row = s_c + threadIdx.x * stride;
-      switch (cls) {
-        case 3: status |= triplet_rho_sp<W, 3>(v, tr, in1, in2, indna, x, z, cost, pm, row); break;
-        case 2: status |= triplet_rho_sp<W, 2>(v, tr, in1, in2, indna, x, z, cost, pm, row); break;
-        case 1: status |= triplet_rho_sp<W, 1>(v, tr, in1, in2, indna, x, z, cost, pm, row); break;
-        default: status |= triplet_rho_sp<W, 0>(v, tr, in1, in2, indna, x, z, cost, pm, row); break;
+      RTri &T = sT[tid];
+      double x, z, cost;
+      rho_geo(v, tr, x, z, cost, T.pm);
+      d_legendre(cost, T.p, &T.sint);
+      T.cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
+      int nsorp = v.sp.nssh[indna - 1];
+      if (nsorp > RSO) {
+        status |= ST_NO_TABLE;
+        nsorp = RSO;
+      }
+      T.nsorp = nsorp;
+      sNs[tid] = nsorp;
+      for (int so = 0; so < nsorp; so++) {
+        RShell &S = sS[tid * RSO + so];
+        const int gid = group3c_r(v, in1, in2, indna, W ? 1 : 0, so + 1);
+        S.q = v.qin[tr.x * FB_MAXSH + so];
+        if (gid < 0) {
+          status |= ST_NO_TABLE;
+          S.nme = 0;
+          continue;
+        }
+        const DGroup3c g = v.grp3c[gid];
+        const int nx = g.nx, ny = g.ny;
+        // e.FDATA/M_Fdata_3c.f90:367-381, as getMEs_3c_sp
+        const double dx = g.xmax / (nx - 1.0), dy = g.ymax / (ny - 1.0);
+        int ix = (int)(x / dx) + 1, iy = (int)(z / dy) + 1;
+        if (ix < 2) ix = 2;
+        if (iy < 2) iy = 2;
+        if (ix > nx - 2) ix = nx - 2;
+        if (iy > ny - 2) iy = ny - 2;
+        S.px = x / dx - ix + 1.0;
+        S.py = z / dy - iy + 1.0;
+        S.nme = g.nME;
+        S.rs = nx * g.nME;
+        S.ts = ny * nx * g.nME;
+        S.base = g.off + ((int64_t)(iy - 2) * nx + (ix - 2)) * g.nME;
+        S.m1 = v.meth3c[g.moff + (int64_t)(iy - 1) * nx + (ix - 1)];
       }
     }
-    s_n[threadIdx.x] = nsorp;
     __syncthreads();
-    if (own) {
-      const int64_t k0 = max(a, c0), k1 = min(b, c0 + T3R);
-      for (int64_t kk = k0; kk < k1; kk++) {
-        const int row = (int)(kk - c0);
-        for (int isorp = 0; isorp < s_n[row]; isorp++)
+    // ---------------- phase B
+    for (int it = tid; it < nt * RSO * NMX; it += RW) {
+      const int t = it / (RSO * NMX), r = it - t * (RSO * NMX), so = r / NMX, m = r - so * NMX;
+      const RTri &T = sT[t];
+      const RShell &S = sS[t * RSO + so];
+      if (so >= T.nsorp || m >= S.nme) continue;
+      const double px = S.px, py = S.py;
+      const int rs = S.rs, nm = S.nme;
+      const double *tab = v.tab3c + S.base + m;
+      double F = 0.0;
+#pragma unroll 1
+      for (int th = 0; th < FB_NTHETA; th++) {
+        const double *tt = tab + (int64_t)th * S.ts;
+        double interim;
+        if (S.m1) {
+          interim = ((1.0 - px - py) * tt[rs + nm] + px * tt[rs + 2 * nm]) + py * tt[2 * rs + nm];
+        } else {
+          double gk[4];
 #pragma unroll
-          for (int i = 0; i < NBR; i++)
-            if (i < nb) acc[i] = acc[i] + s_c[row * stride + isorp * NBR + i];
+          for (int k = 0; k < 4; k++) {
+            const double *col = tt + k * nm;
+            double f1m1 = col[0];
+            double f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+            double f0p3 = 3.0 * col[rs], f0p6 = 2.0 * f0p3;
+            double f1p3 = 3.0 * col[2 * rs], f1p6 = 2.0 * f1p3;
+            double f2p1 = col[3 * rs];
+            double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+            double bb2 = f1m3 - f0p6 + f1p3;
+            double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+            double bb0 = f0p6;
+            gk[k] = ((bb3 * py + bb2) * py + bb1) * py + bb0;
+          }
+          double f1m1 = gk[0], f1m2 = 2.0 * f1m1, f1m3 = 3.0 * f1m1;
+          double f0p3 = 3.0 * gk[1], f0p6 = 2.0 * f0p3;
+          double f1p3 = 3.0 * gk[2], f1p6 = 2.0 * f1p3;
+          double f2p1 = gk[3];
+          double bb3 = -f1m1 + f0p3 - f1p3 + f2p1;
+          double bb2 = f1m3 - f0p6 + f1p3;
+          double bb1 = -f1m2 - f0p3 + f1p6 - f2p1;
+          double bb0 = f0p6;
+          interim = (((bb3 * px + bb2) * px + bb1) * px + bb0) / 36.0;
+        }
+        F = F + T.p[th] * interim;
+      }
+      // sin(theta) on the mvalue = 1 elements: all but the leading 6 / 2 / 1
+      // of the canonical order (none in the weighted shell blocks)
+      if (!W && m >= (nm == 10 ? 6 : (nm == 3 ? 2 : 1))) F = F * T.sint;
+      sF[(t * RSO + so) * NMX + m] = F;
+    }
+    __syncthreads();
+    // ---------------- phase C
+    double X[NBX];
+    int nbc = 0;
+    double q = 0.0;
+    const int tc = tid / RSO, soc = tid - tc * RSO;
+    if (tc < nt && soc < sT[tc].nsorp) {
+      const RTri &T = sT[tc];
+      const RShell &S = sS[tc * RSO + soc];
+      double F[NMX];
+#pragma unroll
+      for (int m = 0; m < NMX; m++) F[m] = (m < S.nme) ? sF[(tc * RSO + soc) * NMX + m] : 0.0;
+      q = S.q;
+      if (W) {
+        switch (T.cls) {
+          case 3: shell_block<2, 2>(F, X); nbc = 4; break;
+          case 2: shell_block<2, 1>(F, X); nbc = 2; break;
+          case 1: shell_block<1, 2>(F, X); nbc = 2; break;
+          default: shell_block<1, 1>(F, X); nbc = 1; break;
+        }
+      } else {
+        double pm[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) pm[k] = T.pm[k];
+        switch (T.cls) {
+          case 3: rot3c_sp<double>(3, pm, F, X); nbc = 16; break;
+          case 2: rot3c_sp<double>(2, pm, F, X); nbc = 4; break;
+          case 1: rot3c_sp<double>(1, pm, F, X); nbc = 4; break;
+          default: rot3c_sp<double>(0, pm, F, X); nbc = 1; break;
+        }
+      }
+      if (S.nme == 0) nbc = 0;
+    }
+    __syncthreads();   // phase A and B data read; the rows overwrite them
+    if (tc < nt && soc < RSO) {
+      double *row = sR + (tc * RSO + soc) * ROW;
+#pragma unroll
+      for (int i = 0; i < NBX; i++) row[i] = (i < nbc) ? X[i] * q : 0.0;
+    }
+    __syncthreads();
+    // ---------------- phase D
+#pragma unroll
+    for (int j = 0; j < (RT * NBX + RW - 1) / RW; j++) {
+      const int sl = tid + j * RW;
+      if (sl < nown * NBX) {
+        const int sg = sl / NBX, e = sl - sg * NBX;
+        const int64_t k0 = max(sSeg[sg], c0), k1 = min(sSeg[sg + 1], c0 + nt);
+        if (k0 < k1 && e < sNb[sg]) {
+          double *const o = out + sBo[sg] + e;
+          double a = *o;
+          for (int64_t k = k0; k < k1; k++) {
+            const int row = (int)(k - c0);
+            for (int so = 0; so < sNs[row]; so++) a = a + sR[(row * RSO + so) * ROW + e];
+          }
+          *o = a;
+        }
       }
     }
     __syncthreads();
-  }
-  if (own) {
-#pragma unroll
-    for (int i = 0; i < NBR; i++)
-      if (i < nb) out[dst + i] = acc[i];
   }
   if (status) atomicOr(v.status, status);
 }
@@ -566,10 +690,12 @@ int fbk_rho_3c(const DView &v, int weighted, int maxsh, int sp, hipStream_t st) 
   const size_t lds = sizeof(double) * T3R * stride;
   const dim3 grid((unsigned)((v.ntrip + T3R - 1) / T3R));
   if (sp) {
+    if (v.ntiles[0] == 0) return 0;
+    const dim3 g((unsigned)v.ntiles[0]);
     if (weighted)
-      hipLaunchKernelGGL(k_rho_3c_sp<true>, grid, dim3(T3R), lds, st, v, stride);
+      hipLaunchKernelGGL(k_rho_3c_ph<true>, g, dim3(RW), 0, st, v);
     else
-      hipLaunchKernelGGL(k_rho_3c_sp<false>, grid, dim3(T3R), lds, st, v, stride);
+      hipLaunchKernelGGL(k_rho_3c_ph<false>, g, dim3(RW), 0, st, v);
     return (int)hipGetLastError();
   }
   if (weighted)
