@@ -223,6 +223,27 @@ def test_mixed_segments_longer_than_chunk():
     assert np.array_equal(got[64], got[128]) and np.array_equal(got[64], got[256])
 
 
+@pytest.mark.parametrize("name", ["cho40_dq", "si64"])
+def test_2c_kernel_variants_same_bits(golden, name):
+    """The two s/p 2c kernels (table cells staged in LDS, register-resident
+    with per-lane gathers) give the same bits on every matrix, neutral and
+    charged."""
+    print(f"2c variants {name}", flush=True)
+    g = golden(name)
+    ts, st = golden_cases.inputs(g)
+    asm = _gpu(ts, [st])
+    got = {}
+    for var in (0, 1):
+        asm.set_tuning(1, var)
+        asm.assemble_all()
+        got[var] = {m: asm.blocks(m) for m in MATS}
+    for m in MATS:
+        assert np.array_equal(got[0][m], got[1][m]), m
+    from fireball_hip.lib import FireballError
+    with pytest.raises(FireballError):
+        asm.set_tuning(1, 2)
+
+
 def test_generic_shells_against_oracle():
     """Species outside the s / sp fast path (two s shells) run the generic kernels."""
     from fireball_hip import synth

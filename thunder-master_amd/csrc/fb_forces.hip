@@ -1358,26 +1358,8 @@ __device__ __forceinline__ double drot3c_el(int row, int col, const double *pm, 
   return 0.0;
 }
 
-// element (row, col) of the rotated 3c value block (rot3c_sp)
-template <int CLS>
-__device__ __forceinline__ double rot3c_el(int row, int col, const double *pm, const double *F) {
-  if (row == 0 && col == 0) return F[0];
-  if (CLS == CLS_SPSP) {
-    if (row == 0) return F[1] * PP_(col - 1, 1) + F[6] * PP_(col - 1, 2);
-    if (col == 0) return PP_(row - 1, 1) * F[2] + PP_(row - 1, 2) * F[7];
-    const int k = row - 1, l = col - 1;
-    double x = (PP_(k, 0) * F[3]) * PP_(l, 0);
-    x = x + (PP_(k, 1) * F[4]) * PP_(l, 1);
-    x = x + (PP_(k, 2) * F[8]) * PP_(l, 1);
-    x = x + (PP_(k, 1) * F[9]) * PP_(l, 2);
-    return x + (PP_(k, 2) * F[5]) * PP_(l, 2);
-  }
-  if (CLS == CLS_SSP) return F[1] * PP_(col - 1, 1) + F[2] * PP_(col - 1, 2);
-  if (CLS == CLS_SPS) return PP_(row - 1, 1) * F[1] + PP_(row - 1, 2) * F[2];
-  return 0.0;
 #undef PP_
 #undef DD_
-}
 
 // ---------------------------------------------------------------------------
 // Four lanes per triplet.  Phase A: lane q interpolates matrix elements q, q+4,

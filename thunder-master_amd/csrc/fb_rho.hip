@@ -591,42 +591,79 @@ __global__ __launch_bounds__(RW) void k_rho_3c_ph(DView v) {
     }
     __syncthreads();
     // ---------------- phase C
-    double X[NBX];
-    int nbc = 0;
-    double q = 0.0;
-    const int tc = tid / RSO, soc = tid - tc * RSO;
-    if (tc < nt && soc < sT[tc].nsorp) {
-      const RTri &T = sT[tc];
-      const RShell &S = sS[tc * RSO + soc];
-      double F[NMX];
+    if (W) {
+      // shell blocks: one lane per (triplet, isorp)
+      double X[NBX];
+      int nbc = 0;
+      double q = 0.0;
+      const int tc = tid / RSO, soc = tid - tc * RSO;
+      if (tc < nt && soc < sT[tc].nsorp) {
+        const RShell &S = sS[tc * RSO + soc];
+        double F[NMX];
 #pragma unroll
-      for (int m = 0; m < NMX; m++) F[m] = (m < S.nme) ? sF[(tc * RSO + soc) * NMX + m] : 0.0;
-      q = S.q;
-      if (W) {
-        switch (T.cls) {
+        for (int m = 0; m < NMX; m++) F[m] = (m < S.nme) ? sF[(tc * RSO + soc) * NMX + m] : 0.0;
+        q = S.q;
+        switch (sT[tc].cls) {
           case 3: shell_block<2, 2>(F, X); nbc = 4; break;
           case 2: shell_block<2, 1>(F, X); nbc = 2; break;
           case 1: shell_block<1, 2>(F, X); nbc = 2; break;
           default: shell_block<1, 1>(F, X); nbc = 1; break;
         }
-      } else {
-        double pm[9];
+        if (S.nme == 0) nbc = 0;
+      }
+      __syncthreads();   // phase A and B data read; the rows overwrite them
+      if (tc < nt && soc < RSO) {
+        double *row = sR + (tc * RSO + soc) * ROW;
 #pragma unroll
-        for (int k = 0; k < 9; k++) pm[k] = T.pm[k];
-        switch (T.cls) {
-          case 3: rot3c_sp<double>(3, pm, F, X); nbc = 16; break;
-          case 2: rot3c_sp<double>(2, pm, F, X); nbc = 4; break;
-          case 1: rot3c_sp<double>(1, pm, F, X); nbc = 4; break;
-          default: rot3c_sp<double>(0, pm, F, X); nbc = 1; break;
+        for (int i = 0; i < NBX; i++) row[i] = (i < nbc) ? X[i] * q : 0.0;
+      }
+    } else {
+      // rotated blocks: one lane per (triplet, isorp, block column), the
+      // elements of rot3c_sp one at a time (rot3c_el: same terms, same order)
+      constexpr int NCI = (RT * RSO * 4 + RW - 1) / RW;
+      double Xc[NCI][4];
+      int dst[NCI], nr[NCI];
+#pragma unroll
+      for (int k = 0; k < NCI; k++) {
+        const int it = tid + k * RW, pr = it >> 2, col = it & 3, tc = pr / RSO, soc = pr - tc * RSO;
+        dst[k] = -1;
+        nr[k] = 0;
+        if (tc < nt && soc < sT[tc].nsorp) {
+          const RTri &T = sT[tc];
+          const RShell &S = sS[pr];
+          const int cls = T.cls, nmu = (cls & 2) ? 4 : 1, nnu = (cls & 1) ? 4 : 1;
+          if (S.nme > 0 && col < nnu) {
+            double F[NMX], pm[9];
+#pragma unroll
+            for (int m = 0; m < NMX; m++) F[m] = (m < S.nme) ? sF[pr * NMX + m] : 0.0;
+#pragma unroll
+            for (int e = 0; e < 9; e++) pm[e] = T.pm[e];
+            const double q = S.q;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              double x = 0.0;
+              if (r < nmu) {
+                switch (cls) {
+                  case 3: x = rot3c_el<3>(r, col, pm, F); break;
+                  case 2: x = rot3c_el<2>(r, col, pm, F); break;
+                  case 1: x = rot3c_el<1>(r, col, pm, F); break;
+                  default: x = rot3c_el<0>(r, col, pm, F); break;
+                }
+              }
+              Xc[k][r] = x * q;
+            }
+            dst[k] = pr * ROW + nmu * col;
+            nr[k] = nmu;
+          }
         }
       }
-      if (S.nme == 0) nbc = 0;
-    }
-    __syncthreads();   // phase A and B data read; the rows overwrite them
-    if (tc < nt && soc < RSO) {
-      double *row = sR + (tc * RSO + soc) * ROW;
+      __syncthreads();   // phase A and B data read; the rows overwrite them
 #pragma unroll
-      for (int i = 0; i < NBX; i++) row[i] = (i < nbc) ? X[i] * q : 0.0;
+      for (int k = 0; k < NCI; k++)
+        if (dst[k] >= 0)
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            if (r < nr[k]) sR[dst[k] + r] = Xc[k][r];
     }
     __syncthreads();
     // ---------------- phase D

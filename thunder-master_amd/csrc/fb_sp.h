@@ -93,4 +93,23 @@ __device__ __forceinline__ void rot3c_sp(int cls, const T *pm, const T *F, T *X)
     X[0] = F[0];
   }
 }
+
+// element (row, col) of the rotated 3c value block (rot3c_sp)
+template <int CLS>
+__device__ __forceinline__ double rot3c_el(int row, int col, const double *pm, const double *F) {
+  if (row == 0 && col == 0) return F[0];
+  if (CLS == CLS_SPSP) {
+    if (row == 0) return F[1] * PP(col - 1, 1) + F[6] * PP(col - 1, 2);
+    if (col == 0) return PP(row - 1, 1) * F[2] + PP(row - 1, 2) * F[7];
+    const int k = row - 1, l = col - 1;
+    double x = (PP(k, 0) * F[3]) * PP(l, 0);
+    x = x + (PP(k, 1) * F[4]) * PP(l, 1);
+    x = x + (PP(k, 2) * F[8]) * PP(l, 1);
+    x = x + (PP(k, 1) * F[9]) * PP(l, 2);
+    return x + (PP(k, 2) * F[5]) * PP(l, 2);
+  }
+  if (CLS == CLS_SSP) return F[1] * PP(col - 1, 1) + F[2] * PP(col - 1, 2);
+  if (CLS == CLS_SPS) return PP(row - 1, 1) * F[1] + PP(row - 1, 2) * F[2];
+  return 0.0;
+}
 #undef PP
