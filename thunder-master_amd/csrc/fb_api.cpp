@@ -257,7 +257,8 @@ struct fb_ctx {
   DevBuf d_ppn[3], d_ppoff[3], d_ppj[3], d_ppb[3], d_ppatom[3], d_ppself[3];
   DevBuf d_pp_map, d_px_map, d_px_point, d_inv_pp, d_inv_px, d_cand_cnt, d_cand_off, d_cand_j, d_cand_b;
   DevBuf d_pc_n, d_pc_off, d_pc_trip, d_pc_tpair, d_pc_trip_s, d_pc_tpair_s, d_pc_perm_in, d_pc_perm, d_pc_flag,
-      d_pc_segincl, d_pc_seg_off, d_pc_seg_pair, d_pc_trip_seg, d_pc_tgt, d_pc_mp;
+      d_pc_segincl, d_pc_seg_off, d_pc_seg_pair, d_pc_trip_seg, d_pc_tgt, d_pc_mp, d_pc_tiles;
+  int pp_ntiles = 0;
   DevBuf d_sv_bsz, d_sv_off, d_vl_bsz, d_vl_off, d_svnl, d_vnl;
   int64_t pp_npairs[3] = {0, 0, 0}, pp_ntrip = 0, pp_nseg = 0, sv_len = 0, vl_len = 0;
   std::vector<int64_t> h_ppoff[3], h_pcoff, h_sv_struct, h_vl_struct;
@@ -294,6 +295,8 @@ struct fb_ctx {
     q.seg_pair = d_pc_seg_pair.as<int>();
     q.trip_seg = d_pc_trip_seg.as<int>();
     q.mp = d_pc_mp.as<int2>();
+    q.tiles = d_pc_tiles.as<int4>();
+    q.ntiles = pp_ntiles;
     q.ntrip = pp_ntrip;
     q.nseg = pp_nseg;
     q.sv_off = d_sv_off.as<int64_t>();
@@ -518,7 +521,7 @@ struct fb_ctx {
                       &d_status, &d_idx3r, &d_qin, &d_wbsz, &d_wboff, &d_pp_map, &d_px_map, &d_px_point,
                       &d_inv_pp, &d_inv_px, &d_cand_cnt, &d_cand_off, &d_cand_j, &d_cand_b, &d_pc_n, &d_pc_off,
                       &d_pc_trip, &d_pc_tpair, &d_pc_trip_s, &d_pc_tpair_s, &d_pc_perm_in, &d_pc_perm, &d_pc_flag,
-                      &d_pc_segincl, &d_pc_seg_off, &d_pc_seg_pair, &d_pc_trip_seg, &d_pc_tgt, &d_pc_mp, &d_sv_bsz, &d_sv_off,
+                      &d_pc_segincl, &d_pc_seg_off, &d_pc_seg_pair, &d_pc_trip_seg, &d_pc_tgt, &d_pc_mp, &d_pc_tiles, &d_sv_bsz, &d_sv_off,
                       &d_vl_bsz, &d_vl_off, &d_svnl, &d_vnl};
     for (DevBuf *b : bufs) b->release();
     for (auto &m : d_mat) m.release();
@@ -1193,6 +1196,19 @@ static int scan_i64(fb_ctx *ctx, const int64_t *in, int64_t *out, int64_t n) {
   return ctx->launch("scan", [&] { return fbk_scan_i64(in, out, n, ctx->d_tmp.p, &bytes, ctx->stream); });
 }
 
+// Whole target segments packed greedily into tiles of at most ch triplets
+// (a longer segment gets a tile of its own): (first, last segment, first
+// triplet, end triplet) per tile, from the segment offsets so[0..nseg].
+static void pack_tiles(const std::vector<int64_t> &so, int nseg, int ch, std::vector<int4> &out) {
+  out.clear();
+  for (int s0 = 0; s0 < nseg;) {
+    int e = s0;
+    while (e + 1 < nseg && so[e + 2] - so[s0] <= ch) e++;
+    out.push_back(make_int4(s0, e, (int)so[s0], (int)so[e + 1]));
+    s0 = e + 1;
+  }
+}
+
 int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
   if (!ctx) return FB_E_ARG;
   if (!ctx->tables_ok) return ctx->fail(FB_E_ORDER, "build_lists before commit_tables");
@@ -1314,13 +1330,7 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
     const int chs[3] = {64, 128, 256};
     std::vector<int4> ht;
     for (int k = 0; k < 3; k++) {
-      ht.clear();
-      for (int s0 = 0; s0 < nseg;) {
-        int e = s0;
-        while (e + 1 < nseg && hso[e + 2] - hso[s0] <= chs[k]) e++;
-        ht.push_back(make_int4(s0, e, (int)hso[s0], (int)hso[e + 1]));
-        s0 = e + 1;
-      }
+      pack_tiles(hso, nseg, chs[k], ht);
       ctx->ntiles[k] = (int)ht.size();
       FB_HIP(ctx->d_tiles[k].ensure(sizeof(int4) * std::max<size_t>(ht.size(), 1)));
       if (!ht.empty())
@@ -2336,6 +2346,7 @@ int fb_build_lists_pp(fb_ctx *ctx, int64_t *npp, int64_t *nppp, int64_t *ntrip) 
   q = ctx->pview();
   RC(ctx->launch("pp_common_fill", [&] { return fbk_pp_common(v, q, 1, st); }));
   ctx->pp_nseg = 0;
+  ctx->pp_ntiles = 0;
   if (T > 0) {
     // stable sort of the triplets by target PPp pair: segments in list order
     int end_bit = 1;
@@ -2369,6 +2380,14 @@ int fb_build_lists_pp(fb_ctx *ctx, int64_t *npp, int64_t *nppp, int64_t *ntrip) 
     }));
     FB_HIP(hipStreamSynchronize(st));
     ctx->pp_nseg = nseg;
+    // one-pass tiles of the s/p vnl_3c kernel (whole segments, at most 64 triplets)
+    std::vector<int64_t> hso((size_t)nseg + 1);
+    FB_HIP(hipMemcpy(hso.data(), ctx->d_pc_seg_off.p, sizeof(int64_t) * (nseg + 1), hipMemcpyDeviceToHost));
+    std::vector<int4> ht;
+    pack_tiles(hso, nseg, 64, ht);
+    FB_HIP(ctx->d_pc_tiles.ensure(sizeof(int4) * std::max<size_t>(ht.size(), 1)));
+    if (!ht.empty()) FB_HIP(hipMemcpy(ctx->d_pc_tiles.p, ht.data(), sizeof(int4) * ht.size(), hipMemcpyHostToDevice));
+    ctx->pp_ntiles = (int)ht.size();
     q = ctx->pview();
     RC(ctx->launch("pp_mpair", [&] { return fbk_pp_mpair(v, q, st); }));
   }

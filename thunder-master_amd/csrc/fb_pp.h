@@ -37,7 +37,9 @@ struct DPP {
   int64_t *seg_off;
   int *seg_pair, *trip_seg;
   int2 *mp;               // per sorted triplet: mpairnay of (iatom, ialpha) and (jatom, ialpha), 0 when not unique
+  int4 *tiles;            // whole segments packed into tiles of at most 64 triplets: first, last segment, r0, r1
   int64_t ntrip, nseg;
+  int ntiles;
   // blocks
   int64_t *sv_off, *vl_off;   // [npairs+1] of PP / PPp
   double *svnl, *vnl;

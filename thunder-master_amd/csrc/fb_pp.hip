@@ -722,15 +722,14 @@ __device__ __forceinline__ void triplet_vnl_sp(const DPP &q, int indna, const do
   for (int i = 0; i < NMU * NNU; i++) row[i] = X[i];
 }
 
+// The workgroup owns the whole segments of one tile from the list build (at
+// most T3P triplets unless one segment is longer), so it usually makes a
+// single pass over its triplets.
 __global__ __launch_bounds__(T3P) void k_vnl_3c_sp(DView v, DPP q) {
   __shared__ double s_c[T3P][NBP + 1];
-  const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3P;
-  const int64_t te = min(ts + T3P, q.ntrip);
-  const int64_t sgs = q.trip_seg[ts];
-  const int64_t first = (q.seg_off[sgs] == ts) ? sgs : sgs + 1;
-  const int64_t last = q.trip_seg[te - 1];
+  const int4 td = q.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const int64_t first = td.x, last = td.y, r0 = td.z, r1 = td.w;
   if (first > last) return;
-  const int64_t r0 = q.seg_off[first], r1 = q.seg_off[last + 1];
   const int64_t my = first + threadIdx.x;
   const bool own = my <= last;
   int64_t a = 0, b = 0, dst = 0;
@@ -872,10 +871,11 @@ int fbk_pp_mpair(const DView &v, const DPP &q, hipStream_t st) {
 
 int fbk_vnl_3c(const DView &v, const DPP &q, hipStream_t st) {
   if (q.ntrip == 0) return 0;
-  if (q.canon)
-    hipLaunchKernelGGL(k_vnl_3c_sp, dim3(nblk(q.ntrip, T3P)), dim3(T3P), 0, st, v, q);
-  else
+  if (q.canon) {
+    if (q.ntiles > 0) hipLaunchKernelGGL(k_vnl_3c_sp, dim3((unsigned)q.ntiles), dim3(T3P), 0, st, v, q);
+  } else {
     hipLaunchKernelGGL(k_vnl_3c, dim3(nblk(q.ntrip, T3P)), dim3(T3P), 0, st, v, q);
+  }
   return (int)hipGetLastError();
 }
 }
