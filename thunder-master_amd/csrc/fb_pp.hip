@@ -688,6 +688,54 @@ __device__ __forceinline__ void vnl_pair_sp(const DView &v, const DPP &q, int64_
   for (int i = 0; i < NMU * NNU; i++) dst[i] = acc[i];
 }
 
+// atom case of assemble_vnl_2c for s/p PP shells: k_vnl_2c_self's sums (same
+// terms, same order) with the loads of four PP neighbours in flight per step.
+// The projector count of a block comes from its sVNL offsets, so the block
+// loads wait for sv_off only, beside the neighbour-species chain.
+__global__ void k_vnl_2c_self_sp(DView v, DPP q) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)v.natoms * NBP) return;
+  const int ia = (int)(t / NBP), e = (int)(t % NBP);
+  const int nmu = v.sp.norb[v.imass[ia] - 1];
+  if (e >= nmu * nmu) return;
+  const int imu = e % nmu, inu = e / nmu;
+  const int aoff = v.st[v.atom_struct[ia]].atom_off;
+  const int64_t p0 = q.off[PPL_PP][ia], p1 = q.off[PPL_PP][ia + 1];
+  double acc = 0.0;
+  for (int64_t pb = p0; pb < p1; pb += 4) {
+    int inp[4], npp[4];
+    double A[4][4], B[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = pb + u;
+      inp[u] = 1;
+      npp[u] = 0;
+      if (p < p1) {
+        const int64_t so = q.sv_off[p];
+        npp[u] = (int)(q.sv_off[p + 1] - so) / nmu;
+        inp[u] = v.imass[aoff + q.j[PPL_PP][p] - 1];
+        const double *sv = q.svnl + so;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          if (c < npp[u]) {
+            A[u][c] = sv[c * nmu + imu];
+            B[u][c] = sv[c * nmu + inu];
+          }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (pb + u < p1) {
+        double x = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          if (c < npp[u]) x = x + q.clo[inp[u] - 1][c] * A[u][c] * B[u][c];
+        acc = acc + x;
+      }
+  }
+  q.vnl[q.vl_off[q.off[PPL_PPP][ia] + q.self[PPL_PPP][ia] - 1] + e] = acc;
+}
+
 __global__ void k_vnl_2c_sp(DView v, DPP q) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= q.npairs[PPL_PPP]) return;
@@ -855,7 +903,10 @@ int fbk_svnl(const DView &v, const DPP &q, hipStream_t st) {
 
 int fbk_vnl_2c(const DView &v, const DPP &q, hipStream_t st) {
   if (q.npairs[PPL_PPP] == 0) return 0;
-  hipLaunchKernelGGL(k_vnl_2c_self, dim3(nblk((int64_t)v.natoms * NBP, 256)), dim3(256), 0, st, v, q);
+  if (q.canon)
+    hipLaunchKernelGGL(k_vnl_2c_self_sp, dim3(nblk((int64_t)v.natoms * NBP, 256)), dim3(256), 0, st, v, q);
+  else
+    hipLaunchKernelGGL(k_vnl_2c_self, dim3(nblk((int64_t)v.natoms * NBP, 256)), dim3(256), 0, st, v, q);
   if (q.canon)
     hipLaunchKernelGGL(k_vnl_2c_sp, dim3(nblk(q.npairs[PPL_PPP], 128)), dim3(128), 0, st, v, q);
   else
