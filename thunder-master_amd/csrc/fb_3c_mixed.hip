@@ -32,14 +32,13 @@
 struct TPrm {
   unsigned base;              // FP32 slab index (floats) of stencil corner (ix-1, iy-1), neutral group
   unsigned rowstride;         // floats per iy row of the slab
-  int p;                      // target pair
   unsigned char cls, nme, m1, pad_;   // pair class, ME count, method-1 flag
   float px, py;               // position in the cell; phase B forms the weights
-  float P[5], sint;
+  float P[4], sint;           // Legendre P1..P4 (P0 = 1)
   float pm[9];
 };
-// 21 words: an odd row stride, so lane-per-triplet accesses hit distinct banks
-static_assert(sizeof(TPrm) == 21 * 4, "TPrm row stride must stay odd in words");
+// 19 words: an odd row stride, so lane-per-triplet accesses hit distinct banks
+static_assert(sizeof(TPrm) == 19 * 4, "TPrm row stride must stay odd in words");
 
 struct TChg {                 // charged third atom (DOGS terms)
   float dq[FB_MAXSH];
@@ -110,11 +109,11 @@ __device__ __forceinline__ float stencil(__amdgpu_buffer_rsrc_t r, unsigned base
 template <int CH, bool CHARGED, bool MLP = false>
 __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   __shared__ TPrm sq[CH];
-  __shared__ float sF[CH][NME3 + 1];
+  __shared__ float sF[CH][NME3];
   __shared__ TChg sx[CHARGED ? CH : 1];
-  __shared__ int64_t sSeg[CH + 1];
+  __shared__ int sSeg[CH + 1];          // triplet offsets (below 2^31: the list build checks)
   __shared__ int64_t sBo[CH];
-  __shared__ int sNb[CH];
+  __shared__ unsigned char sNb[CH];
   __shared__ float sU[MLP ? CH : 1][4];
   __shared__ int sG[MLP ? CH : 1];
   __shared__ unsigned sCnt[2];
@@ -132,12 +131,12 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   // tid + j*CH, j < NB.  The workgroup owns the target blocks, so a segment
   // that spans chunks keeps its binary64 partial sum in the block itself.
   const int nown = (int)(last - first + 1);
-  for (int k = tid; k <= nown; k += CH) sSeg[k] = v.seg_off[first + k];
+  for (int k = tid; k <= nown; k += CH) sSeg[k] = (int)v.seg_off[first + k];
   for (int k = tid; k < nown; k += CH) {
     const int p = v.seg_pair[first + k];
     const int64_t b0 = v.boff[p];
     sBo[k] = b0;
-    sNb[k] = (int)(v.boff[p + 1] - b0);
+    sNb[k] = (unsigned char)(v.boff[p + 1] - b0);
   }
   double *const vna = v.mat[FB_MAT_VNA];
   int status = 0;
@@ -165,7 +164,6 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
         r21[k] = r2v[k] - r1v[k];
       }
       const int in1 = v.imass[gia], in2 = v.imass[gja], indna = v.imass[ial];
-      q.p = v.trip_pair[t];
       q.cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
       const double z = d_distance(r1v, r2v);
       if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
@@ -214,11 +212,10 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       }
       const double cost2 = cost * cost;
       q.sint = (float)sqrt(fmax(1.0e-5, 1.0 - cost2));
-      q.P[0] = 1.f;
-      q.P[1] = (float)cost;
-      q.P[2] = (float)(0.5 * (3.0 * cost2 - 1.0));
-      q.P[3] = (float)(0.5 * cost * (5.0 * cost2 - 3.0));
-      q.P[4] = (float)(0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2)));
+      q.P[0] = (float)cost;
+      q.P[1] = (float)(0.5 * (3.0 * cost2 - 1.0));
+      q.P[2] = (float)(0.5 * cost * (5.0 * cost2 - 3.0));
+      q.P[3] = (float)(0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2)));
       int gid = aux & 0xffff;
       if (gid == 0xffff) gid = -1;
       if (gid < 0) {
@@ -305,7 +302,8 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       w1[1] = px;
       w1[2] = py;
 #pragma unroll
-      for (int k = 0; k < 5; k++) P[k] = q.P[k];
+      for (int k = 0; k < 4; k++) P[k + 1] = q.P[k];
+      P[0] = 1.f;
       const unsigned base = q.base, rs = q.rowstride;
       float F;
       switch (q.nme) {
@@ -356,7 +354,7 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
 #pragma unroll
                 for (int th = 0; th < 5; th++) {
                   const int r = ml * 5 + th, o = 16 * (r >> 2) + 4 * qg + (r & 3);
-                  F = __builtin_fmaf(q.P[th], out[r] * w->oscale[o] * env, F);
+                  F = __builtin_fmaf(th == 0 ? 1.f : q.P[th - 1], out[r] * w->oscale[o] * env, F);
                 }
                 sF[t][m] = F;
               }
@@ -376,7 +374,7 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       const TPrm &q = sq[tid];
       float F[NME3], pm[9];
       cls = q.cls;
-      p = q.p;
+      if (CHARGED) p = v.trip_pair[c0 + tid];
       const int nm0 = (cls == 3) ? 6 : (cls == 0 ? 1 : 2);
       const float sint = q.sint;
 #pragma unroll
@@ -423,7 +421,7 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       if (sl < nown * NB) {
         const int sg = sl / NB, e = sl - sg * NB;
         const int64_t s0 = sSeg[sg];
-        const int64_t k0 = max(s0, c0), k1 = min(sSeg[sg + 1], c0 + nt);
+        const int64_t k0 = max(s0, c0), k1 = min((int64_t)sSeg[sg + 1], c0 + nt);
         if (k0 < k1 && e < sNb[sg]) {
           double *const o = vna + sBo[sg] + e;
           double a = (s0 < c0) ? *o : 0.0;
