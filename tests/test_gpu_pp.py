@@ -72,7 +72,8 @@ def test_pp_batch_matches_oracle_per_structure():
 @pytest.mark.parametrize("name", ["si64", "cho40_dq", "chain_cluster"])
 def test_pp_sp_kernels_equal_generic_kernels(golden, name):
     """The s/p PP kernels (compile-time shapes, register rotations, unrolled
-    contractions) against the generic shell-loop kernels."""
+    contractions, batched atom-case loads, one-pass 3c tiles) against the
+    generic shell-loop kernels."""
     from fireball_hip.lib import Assembler
     g, gp = golden(name), golden(name + "_pp")
     ts, st = golden_cases.inputs(g)

@@ -92,9 +92,9 @@ def test_rho_stepwise_entry_points_equal_rho_all(golden):
 
 @pytest.mark.parametrize("name", ["si64_dq", "cho40_dq", "chain_cluster"])
 def test_rho_sp_kernels_equal_generic_kernels(golden, name):
-    """The s/p density kernels (compile-time block shapes, register-resident
-    rotations) against the generic shell-loop kernels: same operations, same
-    order."""
+    """The s/p density kernels (compile-time block shapes; the 3c one in four
+    phases, its elements spread over lanes) against the generic shell-loop
+    kernels: same operations, same order, so the same bits."""
     g, gr = golden(name), golden(name + "_rho")
     ts, st = golden_cases.inputs(g)
     ts = _with_rho_tables(ts, int(gr["rho_table_seed"]))
