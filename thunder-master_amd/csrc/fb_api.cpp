@@ -28,6 +28,9 @@
 // kernel launchers (fb_lists.hip, fb_2c.hip, fb_3c.hip, fb_3c_mixed.hip)
 extern "C" {
 int fbk_neighbors_count(const DView &v, int *neighn, int *status, hipStream_t st);
+int fbk_pair_disp(const DView &v, double *disp, hipStream_t st);
+int fbk_cell_count(const DView &v, int *cnt, int *atom_cell, hipStream_t st);
+int fbk_cell_fill(const DView &v, const int *atom_cell, int *cursor, int *cell_atoms, hipStream_t st);
 int fbk_neighbors_fill(const DView &v, const int64_t *noff, int *neigh_j, int *neigh_b, int *neigh_self,
                        int *pair_atom, int *status, hipStream_t st);
 int fbk_neigh_back(const DView &v, int *neigh_back, int *status, hipStream_t st);
@@ -216,6 +219,10 @@ struct fb_ctx {
   int maxsh = 1;    // largest shell count of any species
   DevBuf d_cell2c, d_idx2c, d_grp3c, d_idx3c, d_tab2c, d_tab2c32, d_tab3c, d_tab3c32, d_mu, d_nu, d_mv, d_meth3c;
   DevBuf d_grid2c, d_pden, d_wg2c;
+  DevBuf d_cell_cnt, d_cell_start, d_cell_atoms, d_atom_cell;   // cell lists of the neighbour search
+  DevBuf d_pair_disp;   // [npairs][3] neighbour displacements of the common-neighbour search
+  int64_t ncells = 0;
+  int cell_bits = 0;
   int canon_sp = 0;
   int canon_rho = 0;      // s/p species with density tables in make_munu / make_munuS order
   int ncell2c = 0, ngrid2c = 0;
@@ -391,6 +398,10 @@ struct fb_ctx {
     v.canon_sp = canon_sp;
     v.wg2c = d_wg2c.as<int>();
     v.nwg2c = nwg2c;
+    v.pair_disp = d_pair_disp.as<double>();
+    v.cell_start = d_cell_start.as<int64_t>();
+    v.cell_atoms = d_cell_atoms.as<int>();
+    v.cell_bits = cell_bits;
     v.nstruct = (int)structs.size();
     v.natoms = natoms;
     v.st = d_st.as<DStruct>();
@@ -513,7 +524,7 @@ struct fb_ctx {
 
   ~fb_ctx() {
     DevBuf *bufs[] = {&d_cell2c, &d_idx2c, &d_grp3c, &d_idx3c, &d_tab2c, &d_tab2c32, &d_tab3c, &d_tab3c32,
-                      &d_mu, &d_nu, &d_mv, &d_grid2c, &d_pden, &d_wg2c, &d_st, &d_xl, &d_ratom, &d_imass, &d_atom_struct, &d_dQ,
+                      &d_mu, &d_nu, &d_mv, &d_grid2c, &d_pden, &d_wg2c, &d_cell_cnt, &d_cell_start, &d_cell_atoms, &d_atom_cell, &d_pair_disp, &d_st, &d_xl, &d_ratom, &d_imass, &d_atom_struct, &d_dQ,
                       &d_neighn, &d_noff, &d_neigh_j, &d_neigh_b, &d_neigh_back, &d_neigh_self, &d_pair_atom,
                       &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
                       &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_pair_tgt, &d_trip_aux, &d_tiles[0], &d_tiles[1], &d_tiles[2], &d_tmp,
@@ -1134,7 +1145,8 @@ static int upload_structures(fb_ctx *ctx) {
   std::vector<double> qin((size_t)natoms * FB_MAXSH, 0.0);
   std::vector<int> imass(natoms), astr(natoms);
   ctx->h_st.assign(ns, DStruct{});
-  int off = 0;
+  int off = 0, max_atoms = 0;
+  int64_t ncells = 0;
   for (int k = 0; k < ns; k++) {
     const HStruct &s = ctx->structs[k];
     DStruct &d = ctx->h_st[k];
@@ -1158,8 +1170,35 @@ static int upload_structures(fb_ctx *ctx) {
       if (s.has_qin)
         for (int q = 0; q < FB_MAXSH; q++) qin[(size_t)(off + i) * FB_MAXSH + q] = s.qin[(size_t)i * FB_MAXSH + q];
     }
+    // cell grid: width at least the largest cutoff of any list (H+S: rc_i + rc_j,
+    // PP and PPx: rc + rcutoff_PP), at most 2^22 cells per structure
+    double rcpp_max = 0.0;
+    if (ctx->pp_species)
+      for (int i = 0; i < ctx->sp.ns; i++) rcpp_max = std::max(rcpp_max, ctx->hpp.rcpp[i]);
+    d.crange = 2.0 * std::max(d.rcmax_all, rcpp_max) + 1.0e-6;
+    double ext[3];
+    for (int q = 0; q < 3; q++) {
+      ext[q] = s.natoms > 0 ? d.bmax[q] - d.bmin[q] : 0.0;
+      d.clo[q] = s.natoms > 0 ? d.bmin[q] : 0.0;
+      d.cn[q] = (int)std::min(1024.0, std::floor(ext[q] / d.crange) + 1.0);
+    }
+    while ((int64_t)d.cn[0] * d.cn[1] * d.cn[2] > (1ll << 22)) {
+      const int q = (d.cn[0] >= d.cn[1] && d.cn[0] >= d.cn[2]) ? 0 : (d.cn[1] >= d.cn[2] ? 1 : 2);
+      d.cn[q] = (d.cn[q] + 1) / 2;
+    }
+    // widths strictly above crange: the box point +- crange then spans at most
+    // three cells per axis after rounding
+    const double wmin = d.crange * (1.0 + 1.0e-9);
+    for (int q = 0; q < 3; q++)
+      d.cw[q] = d.cn[q] > 1 ? std::max(wmin, ext[q] / (d.cn[q] - 0.5)) : std::max(wmin, ext[q] + 1.0);
+    d.cell_off = (int)ncells;
+    ncells += (int64_t)d.cn[0] * d.cn[1] * d.cn[2];
+    max_atoms = std::max(max_atoms, s.natoms);
     off += s.natoms;
   }
+  ctx->ncells = ncells;
+  // per-image bitmap of the neighbour search in LDS (32 KB at most), else the full scan
+  ctx->cell_bits = (max_atoms <= 262144 && ncells < (1ll << 31)) ? (max_atoms + 31) / 32 : 0;
   ctx->h_imass = imass;
   RC(upload(ctx, ctx->d_ratom, ratom.data(), ratom.size() * sizeof(double), "upload ratom"));
   RC(upload(ctx, ctx->d_imass, imass.data(), imass.size() * sizeof(int), "upload imass"));
@@ -1221,6 +1260,26 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
   FB_HIP(ctx->d_neighn.ensure(sizeof(int) * N));
   FB_HIP(ctx->d_noff.ensure(sizeof(int64_t) * (N + 1)));
   FB_HIP(ctx->d_neigh_self.ensure(sizeof(int) * N));
+  if (ctx->cell_bits > 0) {
+    // atoms binned into their structure's cells (counting sort; order inside a
+    // cell is free: the search orders its results through a bitmap)
+    const int64_t C = ctx->ncells;
+    FB_HIP(ctx->d_cell_cnt.ensure(sizeof(int) * C));
+    FB_HIP(ctx->d_cell_start.ensure(sizeof(int64_t) * (C + 1)));
+    FB_HIP(ctx->d_cell_atoms.ensure(sizeof(int) * std::max(N, 1)));
+    FB_HIP(ctx->d_atom_cell.ensure(sizeof(int) * std::max(N, 1)));
+    DView vb = ctx->view();
+    RC(ctx->launch("cell_clear", [&] { return fbk_fill_i32(ctx->d_cell_cnt.as<int>(), C, 0, st); }));
+    RC(ctx->launch("cell_count", [&] {
+      return fbk_cell_count(vb, ctx->d_cell_cnt.as<int>(), ctx->d_atom_cell.as<int>(), st);
+    }));
+    RC(scan_i32(ctx, ctx->d_cell_cnt.as<int>(), ctx->d_cell_start.as<int64_t>(), C));
+    RC(ctx->launch("cell_clear", [&] { return fbk_fill_i32(ctx->d_cell_cnt.as<int>(), C, 0, st); }));
+    vb = ctx->view();
+    RC(ctx->launch("cell_fill", [&] {
+      return fbk_cell_fill(vb, ctx->d_atom_cell.as<int>(), ctx->d_cell_cnt.as<int>(), ctx->d_cell_atoms.as<int>(), st);
+    }));
+  }
   DView v = ctx->view();
   RC(ctx->launch("neighbors_count", [&] { return fbk_neighbors_count(v, ctx->d_neighn.as<int>(), v.status, st); }));
   RC(scan_i32(ctx, ctx->d_neighn.as<int>(), ctx->d_noff.as<int64_t>(), N));
@@ -1242,6 +1301,9 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
                               ctx->d_neigh_self.as<int>(), ctx->d_pair_atom.as<int>(), v.status, st);
   }));
   RC(ctx->launch("neigh_back", [&] { return fbk_neigh_back(v, ctx->d_neigh_back.as<int>(), v.status, st); }));
+  FB_HIP(ctx->d_pair_disp.ensure(sizeof(double) * 3 * std::max<int64_t>(P, 1)));
+  v = ctx->view();
+  RC(ctx->launch("pair_disp", [&] { return fbk_pair_disp(v, ctx->d_pair_disp.as<double>(), st); }));
   {
     std::vector<int> self(N);
     FB_HIP(hipMemcpyAsync(self.data(), ctx->d_neigh_self.p, sizeof(int) * N, hipMemcpyDeviceToHost, st));

@@ -79,6 +79,11 @@ struct DStruct {
   int64_t pair_off, trip_off;
   double bmin[3], bmax[3];
   double rcmax_all;
+  // cell list of the structure's atoms (home positions): cn cells per axis of
+  // width cw >= crange from clo; every atom within crange of a point lies in a
+  // cell that overlaps the box point +- crange
+  double clo[3], cw[3], crange;
+  int cn[3], cell_off;
 };
 
 // all device pointers of a context, passed by value to kernels
@@ -132,6 +137,10 @@ struct DView {
                                 // their triplet range r0, r1 at most one chunk unless one segment is longer
   int ntiles[3];
   const int *trip_seg;          // [ntrip] segment of each sorted triplet
+  const double *pair_disp;      // [npairs][3] xl(jbeta) + r_j - r_i of each neighbour pair (common-neighbour search)
+  const int64_t *cell_start;    // [ncells+1] first entry of each cell in cell_atoms (cells of all structures)
+  const int *cell_atoms;        // [natoms] local atom index, grouped by cell
+  int cell_bits;                // 32-bit words of the per-image atom bitmap (0: no cell list, scan all atoms)
   // learned 3c surrogate (one DMlp per 3c group; ready = 0 where untrained)
   const struct DMlp *mlp;
   const float *mlp_err;         // calibrated per-cell error of each network

@@ -15,12 +15,39 @@
 #define LWG 256
 #include "fb_lists_dev.h"
 
+// atoms binned by cell (counting sort of the neighbour search)
+__global__ void k_cell_count(DView v, int *cnt, int *atom_cell) {
+  const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= v.natoms) return;
+  const DStruct S = v.st[v.atom_struct[ia]];
+  const double r[3] = {v.ratom[3 * ia], v.ratom[3 * ia + 1], v.ratom[3 * ia + 2]};
+  const int c = S.cell_off + cell_of(S, r);
+  atom_cell[ia] = c;
+  atomicAdd(&cnt[c], 1);
+}
+
+__global__ void k_cell_fill(DView v, const int *atom_cell, int *cursor, int *cell_atoms) {
+  const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= v.natoms) return;
+  const int c = atom_cell[ia];
+  const int k = atomicAdd(&cursor[c], 1);
+  cell_atoms[v.cell_start[c] + k] = ia - v.st[v.atom_struct[ia]].atom_off;
+}
+
 // FILL = false: count neighbours (find_neigh_max); FILL = true: write them.
+// With a cell list (v.cell_bits > 0) the candidates of each image are the
+// atoms of the cells around r1 - xl(mbeta); the accepted ones are marked in a
+// bitmap over jatom and written in jatom order, so the list order is the
+// reference's either way.  Dynamic LDS: v.cell_bits words.
 template <bool FILL>
 __global__ __launch_bounds__(LWG) void k_neighbors(DView v, int *neighn_out, const int64_t *noff,
                                                    int *neigh_j, int *neigh_b, int *neigh_self,
                                                    int *pair_atom, int *status) {
+  extern __shared__ unsigned s_bits[];
+  __shared__ int64_t s_cs[27];
+  __shared__ int s_pre[28];
   __shared__ int s_wave[LWG / 64];
+  __shared__ int s_img[FB_NIMG];
   const int ia = blockIdx.x;  // global atom
   const int s = v.atom_struct[ia];
   const DStruct S = v.st[s];
@@ -32,9 +59,58 @@ __global__ __launch_bounds__(LWG) void k_neighbors(DView v, int *neighn_out, con
   const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
   int count = 0;
   int64_t base = FILL ? noff[ia] : 0;
-  for (int mb = 0; mb <= S.mbeta_max; mb++) {
+  if (v.cell_bits > 0) {
+    const int nw = (S.natoms + 31) >> 5;
+    const int nimg = hit_images(S, xls, r1, reach, s_img, s_wave);
+    for (int ki = 0; ki < nimg; ki++) {
+      const int mb = s_img[ki];
+      const double *xlb = xls + 3 * mb;
+      const double q[3] = {r1[0] - xlb[0], r1[1] - xlb[1], r1[2] - xlb[2]};
+      const int nc = query_cells(v, S, q, s_cs, s_pre);
+      if (nc == 0) continue;
+      for (int w = threadIdx.x; w < nw; w += LWG) s_bits[w] = 0u;
+      __syncthreads();
+      for (int t = threadIdx.x; t < s_pre[nc]; t += LWG) {
+        const int lj = cell_candidate(v, s_cs, s_pre, nc, t);
+        const int ja = S.atom_off + lj;
+        double r2[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) r2[k] = v.ratom[3 * ja + k] + xlb[k];
+        const double z = d_distance(r1, r2);
+        const double range_max = rci + v.sp.rcmax[v.imass[ja] - 1] - 0.01;
+        if (lj != li && z < 0.5) atomicOr(status, ST_TOO_CLOSE);
+        if (z < range_max) atomicOr(&s_bits[lj >> 5], 1u << (lj & 31));
+      }
+      __syncthreads();
+      for (int w0 = 0; w0 < nw; w0 += LWG) {
+        const int w = w0 + threadIdx.x;
+        const unsigned bits = (w < nw) ? s_bits[w] : 0u;
+        int tot;
+        const int pos = block_scan_count(__popc(bits), s_wave, &tot);
+        if (FILL) {
+          unsigned b = bits;
+          int r = 0;
+          while (b) {
+            const int lj = w * 32 + __ffs(b) - 1;
+            b &= b - 1u;
+            const int64_t p = base + count + pos + r;
+            neigh_j[p] = lj + 1;
+            neigh_b[p] = mb;
+            pair_atom[p] = ia;
+            if (lj == li && mb == 0) neigh_self[ia] = count + pos + r + 1;
+            r++;
+          }
+        }
+        count += tot;
+      }
+    }
+    if (!FILL && threadIdx.x == 0) neighn_out[ia] = count;
+    return;
+  }
+  const int nimg = hit_images(S, xls, r1, reach, s_img, s_wave);
+  for (int ki = 0; ki < nimg; ki++) {
+    const int mb = s_img[ki];
     const double *xlb = xls + 3 * mb;
-    if (!image_may_hit(S, xlb, r1, reach)) continue;
     for (int j0 = 0; j0 < S.natoms; j0 += LWG) {
       int lj = j0 + threadIdx.x;
       bool pass = false;
@@ -61,6 +137,19 @@ __global__ __launch_bounds__(LWG) void k_neighbors(DView v, int *neighn_out, con
     }
   }
   if (!FILL && threadIdx.x == 0) neighn_out[ia] = count;
+}
+
+// displacement of every neighbour pair, xl(jbeta) + r_j - r_i with the
+// reference's evaluation order (M_neighbors.f90:766-769 compares it with r2 - r1)
+__global__ void k_pair_disp(DView v, double *disp) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= v.npairs) return;
+  const int ia = v.pair_atom[p];
+  const int s = v.atom_struct[ia];
+  const int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
+  const double *xlb = v.xl + (int64_t)s * FB_NIMG * 3 + 3 * v.neigh_b[p];
+#pragma unroll
+  for (int k = 0; k < 3; k++) disp[3 * p + k] = xlb[k] + v.ratom[3 * ja + k] - v.ratom[3 * ia + k];
 }
 
 // neigh_back, c.SYSTEM/M_neighbors.f90:469-486: the jneigh of jatom that
@@ -130,20 +219,23 @@ __global__ __launch_bounds__(LWG) void k_common(DView v, int *ncommon_out, const
         double z = d_distance(r1, r2);
         double range_max = v.sp.rcmax[v.imass[gia] - 1] + v.sp.rcmax[v.imass[gja] - 1] - 0.01;
         if (z < range_max) {
-          int64_t io = v.noff[gia];
-          int ni = v.neighn[gia];
-          for (int kn = 0; kn < ni; kn++) {
-            int ka = v.neigh_j[io + kn];
-            int kb = v.neigh_b[io + kn];
-            int gka = S.atom_off + ka - 1;
-            double r3[3];
+          // mneigh: the first kneigh of iatom whose displacement r3 = xl(kbeta)
+          // + r_k - r_i matches r2 - r1 within 1e-4 per component; eight
+          // displacements loaded at a time, tested in order
+          const double *dp = v.pair_disp + 3 * v.noff[gia];
+          const int ni = v.neighn[gia];
+          for (int k0 = 0; k0 < ni && m == -9999; k0 += 8) {
+            double r3[8][3];
 #pragma unroll
-            for (int k = 0; k < 3; k++) r3[k] = xls[3 * kb + k] + v.ratom[3 * gka + k] - v.ratom[3 * gia + k];
-            if (fabs(r3[0] - r2[0] + r1[0]) < 1.0e-4 && fabs(r3[1] - r2[1] + r1[1]) < 1.0e-4 &&
-                fabs(r3[2] - r2[2] + r1[2]) < 1.0e-4) {
-              m = kn + 1;
-              break;
-            }
+            for (int u = 0; u < 8; u++)
+              if (k0 + u < ni)
+#pragma unroll
+                for (int k = 0; k < 3; k++) r3[u][k] = dp[3 * (k0 + u) + k];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+              if (m == -9999 && k0 + u < ni && fabs(r3[u][0] - r2[0] + r1[0]) < 1.0e-4 &&
+                  fabs(r3[u][1] - r2[1] + r1[1]) < 1.0e-4 && fabs(r3[u][2] - r2[2] + r1[2]) < 1.0e-4)
+                m = k0 + u + 1;
           }
           pass = (m != -9999);
         }
@@ -210,16 +302,37 @@ __global__ void k_block_sizes(DView v, int64_t *sz) {
 
 extern "C" {
 
+int fbk_cell_count(const DView &v, int *cnt, int *atom_cell, hipStream_t st) {
+  if (v.natoms == 0) return 0;
+  hipLaunchKernelGGL(k_cell_count, dim3((unsigned)((v.natoms + 255) / 256)), dim3(256), 0, st, v, cnt, atom_cell);
+  return (int)hipGetLastError();
+}
+
+int fbk_cell_fill(const DView &v, const int *atom_cell, int *cursor, int *cell_atoms, hipStream_t st) {
+  if (v.natoms == 0) return 0;
+  hipLaunchKernelGGL(k_cell_fill, dim3((unsigned)((v.natoms + 255) / 256)), dim3(256), 0, st, v, atom_cell, cursor,
+                     cell_atoms);
+  return (int)hipGetLastError();
+}
+
 int fbk_neighbors_count(const DView &v, int *neighn, int *status, hipStream_t st) {
-  hipLaunchKernelGGL(k_neighbors<false>, dim3(v.natoms), dim3(LWG), 0, st, v, neighn, (const int64_t *)nullptr,
+  const size_t lds = sizeof(unsigned) * (size_t)v.cell_bits;
+  hipLaunchKernelGGL(k_neighbors<false>, dim3(v.natoms), dim3(LWG), lds, st, v, neighn, (const int64_t *)nullptr,
                      (int *)nullptr, (int *)nullptr, (int *)nullptr, (int *)nullptr, status);
   return (int)hipGetLastError();
 }
 
 int fbk_neighbors_fill(const DView &v, const int64_t *noff, int *neigh_j, int *neigh_b, int *neigh_self,
                        int *pair_atom, int *status, hipStream_t st) {
-  hipLaunchKernelGGL(k_neighbors<true>, dim3(v.natoms), dim3(LWG), 0, st, v, (int *)nullptr, noff, neigh_j,
+  const size_t lds = sizeof(unsigned) * (size_t)v.cell_bits;
+  hipLaunchKernelGGL(k_neighbors<true>, dim3(v.natoms), dim3(LWG), lds, st, v, (int *)nullptr, noff, neigh_j,
                      neigh_b, neigh_self, pair_atom, status);
+  return (int)hipGetLastError();
+}
+
+int fbk_pair_disp(const DView &v, double *disp, hipStream_t st) {
+  if (v.npairs == 0) return 0;
+  hipLaunchKernelGGL(k_pair_disp, dim3((unsigned)((v.npairs + 255) / 256)), dim3(256), 0, st, v, disp);
   return (int)hipGetLastError();
 }
 

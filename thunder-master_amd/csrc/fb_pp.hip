@@ -23,9 +23,17 @@
 // PP (rc_i + rcutoff_PP_j) or PPx (rcutoff_PP_i + rc_j) neighbours, both with
 // the reference's strict test z < range - 0.01 (M_neighbors_PP.f90:466-474,
 // 730-738); FILL = false counts, FILL = true writes.
+// With the cell list of the H+S search (when its cells are at least as wide as
+// this list's reach) the candidates of each image come from the cells around
+// r1 - xl(mbeta) and the accepted ones are written in jatom order through a
+// bitmap, as in k_neighbors.  Dynamic LDS: v.cell_bits words.
 template <bool FILL>
 __global__ __launch_bounds__(LWG) void k_pp_neighbors(DView v, DPP q, int list) {
+  extern __shared__ unsigned s_bits[];
+  __shared__ int64_t s_cs[27];
+  __shared__ int s_pre[28];
   __shared__ int s_wave[LWG / 64];
+  __shared__ int s_img[FB_NIMG];
   const int ia = blockIdx.x;
   const int s = v.atom_struct[ia];
   const DStruct S = v.st[s];
@@ -39,9 +47,57 @@ __global__ __launch_bounds__(LWG) void k_pp_neighbors(DView v, DPP q, int list) 
   const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
   int count = 0;
   const int64_t base = FILL ? q.off[list][ia] : 0;
-  for (int mb = 0; mb <= S.mbeta_max; mb++) {
+  if (v.cell_bits > 0 && reach <= S.crange) {
+    const int nw = (S.natoms + 31) >> 5;
+    const int nimg = hit_images(S, xls, r1, reach, s_img, s_wave);
+    for (int ki = 0; ki < nimg; ki++) {
+      const int mb = s_img[ki];
+      const double *xlb = xls + 3 * mb;
+      const double qp[3] = {r1[0] - xlb[0], r1[1] - xlb[1], r1[2] - xlb[2]};
+      const int nc = query_cells(v, S, qp, s_cs, s_pre);
+      if (nc == 0) continue;
+      for (int w = threadIdx.x; w < nw; w += LWG) s_bits[w] = 0u;
+      __syncthreads();
+      for (int t = threadIdx.x; t < s_pre[nc]; t += LWG) {
+        const int lj = cell_candidate(v, s_cs, s_pre, nc, t);
+        const int ja = S.atom_off + lj;
+        double r2[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) r2[k] = v.ratom[3 * ja + k] + xlb[k];
+        const double z = d_distance(r1, r2);
+        const double rcj = (list == PPL_PP) ? q.rcpp[v.imass[ja] - 1] : v.sp.rcmax[v.imass[ja] - 1];
+        if (z < rci + rcj - 0.01) atomicOr(&s_bits[lj >> 5], 1u << (lj & 31));
+      }
+      __syncthreads();
+      for (int w0 = 0; w0 < nw; w0 += LWG) {
+        const int w = w0 + threadIdx.x;
+        const unsigned bits = (w < nw) ? s_bits[w] : 0u;
+        int tot;
+        const int pos = block_scan_count(__popc(bits), s_wave, &tot);
+        if (FILL) {
+          unsigned b = bits;
+          int r = 0;
+          while (b) {
+            const int lj = w * 32 + __ffs(b) - 1;
+            b &= b - 1u;
+            const int64_t p = base + count + pos + r;
+            q.j[list][p] = lj + 1;
+            q.b[list][p] = mb;
+            q.atom[list][p] = ia;
+            if (lj == li && mb == 0) q.self[list][ia] = count + pos + r + 1;
+            r++;
+          }
+        }
+        count += tot;
+      }
+    }
+    if (!FILL && threadIdx.x == 0) q.n[list][ia] = count;
+    return;
+  }
+  const int nimg = hit_images(S, xls, r1, reach, s_img, s_wave);
+  for (int ki = 0; ki < nimg; ki++) {
+    const int mb = s_img[ki];
     const double *xlb = xls + 3 * mb;
-    if (!image_may_hit(S, xlb, r1, reach)) continue;
     for (int j0 = 0; j0 < S.natoms; j0 += LWG) {
       const int lj = j0 + threadIdx.x;
       bool pass = false;
@@ -845,10 +901,11 @@ extern "C" {
 int fbk_pp_neighbors(const DView &v, const DPP &q, int list, int fill, int *status, hipStream_t st) {
   (void)status;
   if (v.natoms == 0) return 0;
+  const size_t lds = sizeof(unsigned) * (size_t)v.cell_bits;
   if (fill)
-    hipLaunchKernelGGL(k_pp_neighbors<true>, dim3(v.natoms), dim3(LWG), 0, st, v, q, list);
+    hipLaunchKernelGGL(k_pp_neighbors<true>, dim3(v.natoms), dim3(LWG), lds, st, v, q, list);
   else
-    hipLaunchKernelGGL(k_pp_neighbors<false>, dim3(v.natoms), dim3(LWG), 0, st, v, q, list);
+    hipLaunchKernelGGL(k_pp_neighbors<false>, dim3(v.natoms), dim3(LWG), lds, st, v, q, list);
   return (int)hipGetLastError();
 }
 
