@@ -28,6 +28,7 @@ struct DPP {
   int64_t *cand_off;      // [natoms+1] PPp candidate ranges
   int *cand_j, *cand_b;   // PPp construction scratch
   // PP triplets: list order (common-neighbour order of each ialpha) and sorted by target
+  double *ppp_disp;       // [PPp pairs][3] xl(kbeta) + r_k - r_i; NaN where kbeta lies beyond the images
   int *pc_n;
   int64_t *pc_off;        // [natoms+1]
   int4 *trip;             // (ialpha, iatom, jatom global; ibeta | jbeta << 16)
@@ -52,6 +53,7 @@ int fbk_ppp_candidates(const DView &v, const DPP &q, int64_t *cnt, hipStream_t s
 int fbk_ppp_build(const DView &v, const DPP &q, int fill, int *status, hipStream_t st);
 int fbk_pp_maps(const DView &v, const DPP &q, int *status, hipStream_t st);
 int fbk_pp_common(const DView &v, const DPP &q, int fill, hipStream_t st);
+int fbk_ppp_disp(const DView &v, const DPP &q, hipStream_t st);
 int fbk_pp_block_sizes(const DView &v, const DPP &q, int which, int64_t *sz, hipStream_t st);
 int fbk_svnl(const DView &v, const DPP &q, hipStream_t st);
 int fbk_vnl_2c(const DView &v, const DPP &q, hipStream_t st);

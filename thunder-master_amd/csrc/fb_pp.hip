@@ -200,10 +200,22 @@ __global__ __launch_bounds__(LWG) void k_ppp_keys(DView v, DPP q) {
   }
 }
 
+// First occurrences by key through an LDS hash table: every candidate inserts
+// its key (linear probing) and lowers the key's smallest candidate index; a
+// candidate is kept when it holds that minimum.  A workgroup whose keys
+// overflow the table, or whose candidates or atoms exceed its encoding, falls
+// back to comparing each candidate with all earlier ones.
+#define PPH 2048   // hash slots per workgroup
+
+__device__ __forceinline__ unsigned ppp_hash(unsigned key) { return (key * 2654435761u) >> (32 - 11); }
+
 template <bool FILL>
 __global__ __launch_bounds__(LWG) void k_ppp_build(DView v, DPP q, int *status) {
   __shared__ int s_wave[LWG / 64];
   __shared__ int s_self;
+  __shared__ unsigned s_key[PPH];
+  __shared__ int s_min[PPH];
+  __shared__ int s_over;
   const int ia = blockIdx.x;
   const int s = v.atom_struct[ia];
   const DStruct S = v.st[s];
@@ -212,6 +224,31 @@ __global__ __launch_bounds__(LWG) void k_ppp_build(DView v, DPP q, int *status) 
   const int li = ia - S.atom_off + 1;
   const int64_t o = FILL ? q.off[PPL_PPP][ia] : 0;
   if (FILL && threadIdx.x == 0) s_self = -999;
+  // key = jatom * 1024 + image (images 0..729): unique below 2^22 atoms
+  const bool hashed = S.natoms < (1 << 22) && nc < (1ll << 31);
+  if (hashed) {
+    for (int k = threadIdx.x; k < PPH; k += LWG) {
+      s_key[k] = 0xffffffffu;
+      s_min[k] = 0x7fffffff;
+    }
+    if (threadIdx.x == 0) s_over = 0;
+    __syncthreads();
+    for (int64_t c = threadIdx.x; c < nc; c += LWG) {
+      const unsigned key = (unsigned)q.cand_j[c0 + c] * 1024u + (unsigned)q.cand_b[c0 + c];
+      unsigned h = ppp_hash(key);
+      int probe = 0;
+      while (probe < PPH) {
+        const unsigned old = atomicCAS(&s_key[h], 0xffffffffu, key);
+        if (old == 0xffffffffu || old == key) break;
+        h = (h + 1) & (PPH - 1);
+        probe++;
+      }
+      if (probe == PPH) s_over = 1;
+      else atomicMin(&s_min[h], (int)c);
+    }
+    __syncthreads();
+  }
+  const bool use_hash = hashed && s_over == 0;
   int count = 0;
   for (int64_t k0 = 0; k0 < nc; k0 += LWG) {
     const int64_t c = k0 + threadIdx.x;
@@ -220,8 +257,24 @@ __global__ __launch_bounds__(LWG) void k_ppp_build(DView v, DPP q, int *status) 
     if (c < nc) {
       j = q.cand_j[c0 + c];
       b = q.cand_b[c0 + c];
-      keep = true;
-      for (int64_t e = 0; e < c && keep; e++) keep = !(q.cand_j[c0 + e] == j && q.cand_b[c0 + e] == b);
+      bool found = false;
+      if (use_hash) {
+        // every key was inserted above; the probe is bounded all the same
+        const unsigned key = (unsigned)j * 1024u + (unsigned)b;
+        unsigned h = ppp_hash(key);
+        for (int probe = 0; probe < PPH; probe++) {
+          if (s_key[h] == key) {
+            keep = s_min[h] == (int)c;
+            found = true;
+            break;
+          }
+          h = (h + 1) & (PPH - 1);
+        }
+      }
+      if (!found) {
+        keep = true;
+        for (int64_t e = 0; e < c && keep; e++) keep = !(q.cand_j[c0 + e] == j && q.cand_b[c0 + e] == b);
+      }
     }
     int tot;
     const int pos = block_rank(keep, s_wave, &tot);
@@ -284,6 +337,26 @@ __global__ void k_pp_maps(DView v, DPP q, int *status) {
   q.px_point[p] = point;
 }
 
+// displacement of every PPp pair, xl(kbeta) + r_k - r_i in the reference's
+// order (M_neighbors_PP.f90 common-neighbour match), NaN beyond the images
+__global__ void k_ppp_disp(DView v, DPP q) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= q.npairs[PPL_PPP]) return;
+  const int ia = q.atom[PPL_PPP][p];
+  const int s = v.atom_struct[ia];
+  const DStruct S = v.st[s];
+  const int kb = q.b[PPL_PPP][p];
+  double *d = q.ppp_disp + 3 * p;
+  if (kb > S.mbeta_max) {
+    d[0] = d[1] = d[2] = __builtin_nan("");
+    return;
+  }
+  const int ka = S.atom_off + q.j[PPL_PPP][p] - 1;
+  const double *xlb = v.xl + (int64_t)s * FB_NIMG * 3 + 3 * kb;
+#pragma unroll
+  for (int k = 0; k < 3; k++) d[k] = xlb[k] + v.ratom[3 * ka + k] - v.ratom[3 * ia + k];
+}
+
 // find_common_neighbors_PP (M_neighbors_PP.f90:1301-1420): ordered pairs of
 // PPx neighbours of ialpha, neither ialpha itself, kept when the bond vector
 // names an entry of iatom's PPp list (first within 1e-4).  FILL = false counts,
@@ -321,19 +394,21 @@ __global__ __launch_bounds__(LWG) void k_pp_common(DView v, DPP q) {
           r2[k] = v.ratom[3 * gja + k] + xls[3 * jb + k];
           r[k] = r2[k] - r1[k];
         }
-        const int64_t io = q.off[PPL_PPP][gia];
+        // first kneigh of iatom's PPp list whose displacement lies within
+        // 1e-4 of r2 - r1 (entries with an image outside xl are NaN: never
+        // a match, as the reference's search skips them); eight at a time
+        const double *dp = q.ppp_disp + 3 * q.off[PPL_PPP][gia];
         const int ni = q.n[PPL_PPP][gia];
-        for (int kn = 0; kn < ni; kn++) {
-          const int kb = q.b[PPL_PPP][io + kn];
-          if (kb > S.mbeta_max) continue;   // an image outside xl (the reference reads past its end)
-          const int gka = S.atom_off + q.j[PPL_PPP][io + kn] - 1;
-          double r3[3];
+        for (int k0 = 0; k0 < ni && m == -9999; k0 += 8) {
+          double r3[8][3];
 #pragma unroll
-          for (int k = 0; k < 3; k++) r3[k] = xls[3 * kb + k] + v.ratom[3 * gka + k] - v.ratom[3 * gia + k];
-          if (d_distance(r, r3) < 1.0e-4) {
-            m = kn + 1;
-            break;
-          }
+          for (int u = 0; u < 8; u++)
+            if (k0 + u < ni)
+#pragma unroll
+              for (int k = 0; k < 3; k++) r3[u][k] = dp[3 * (k0 + u) + k];
+#pragma unroll
+          for (int u = 0; u < 8; u++)
+            if (m == -9999 && k0 + u < ni && d_distance(r, r3[u]) < 1.0e-4) m = k0 + u + 1;
         }
         pass = (m != -9999);
       }
@@ -930,6 +1005,12 @@ int fbk_pp_maps(const DView &v, const DPP &q, int *status, hipStream_t st) {
   const int64_t n = q.npairs[PPL_PP] + q.npairs[PPL_PPX];
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_pp_maps, dim3(nblk(n, 128)), dim3(128), 0, st, v, q, status);
+  return (int)hipGetLastError();
+}
+
+int fbk_ppp_disp(const DView &v, const DPP &q, hipStream_t st) {
+  if (q.npairs[PPL_PPP] == 0) return 0;
+  hipLaunchKernelGGL(k_ppp_disp, dim3(nblk(q.npairs[PPL_PPP], 256)), dim3(256), 0, st, v, q);
   return (int)hipGetLastError();
 }
 
