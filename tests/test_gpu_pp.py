@@ -69,6 +69,44 @@ def test_pp_batch_matches_oracle_per_structure():
             _same(asm.pp_blocks(key, k), ref[key], f"structure {k} {key}")
 
 
+def _triclinic(n=32, seed=11):
+    """Si atoms at random in a skewed periodic cell, at least 2.0 A apart
+    across the cell's images."""
+    rng = np.random.default_rng(seed)
+    lat = np.array([[8.2, 0.0, 0.0], [2.6, 7.6, 0.0], [1.4, 2.3, 7.1]])
+    shifts = np.array([i * lat[0] + j * lat[1] + k * lat[2] for i in (-1, 0, 1) for j in (-1, 0, 1)
+                       for k in (-1, 0, 1)])
+    pos = np.zeros((0, 3))
+    while len(pos) < n:
+        p = rng.uniform(0.0, 1.0, 3) @ lat
+        if len(pos) == 0 or np.min(np.linalg.norm(p - pos[:, None, :] - shifts[None, :, :], axis=2)) >= 2.0:
+            pos = np.vstack([pos, p])
+    return synth.Structure(pos, np.ones(n, np.int32), lat, 0, name="tri32")
+
+
+def test_lists_triclinic_cell_against_oracle():
+    """A skewed periodic cell (the cell-list neighbour search over images whose
+    boxes are not axis-aligned with the lattice): the H lists, the PP, PPx and
+    PPp lists with their maps and triplets, and the sVNL / vnl blocks against
+    the oracle, bit for bit."""
+    from golden.make_golden import LIST_KEYS
+    sp = synth.with_pp([synth.SI])
+    ts = synth.add_pp_tables(synth.make_tables(sp, seed=41), seed=43)
+    st = _triclinic()
+    o = oracle.System(oracle.Tables(ts), st)
+    ref = o.run_all()
+    ref_pp = o.run_pp()
+    asm = _gpu_pp(ts, [st])
+    got = asm.lists(0)
+    for k in LIST_KEYS:
+        _same(got[k], ref[k], k)
+    L = asm.pp_lists()
+    for k in PP_LIST_KEYS:
+        _same(L[k], ref_pp[k], k)
+    for k in ("SVNL", "VNL"):
+        _same(asm.pp_blocks(k), ref_pp[k], k)
+
+
 @pytest.mark.parametrize("name", ["si64", "cho40_dq", "chain_cluster"])
 def test_pp_sp_kernels_equal_generic_kernels(golden, name):
     """The s/p PP kernels (compile-time shapes, register rotations, unrolled
