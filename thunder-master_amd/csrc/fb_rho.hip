@@ -353,8 +353,10 @@ __device__ __forceinline__ int pair_rho_sp(const DView &v, int ia, int ja, int i
 // one lane per pair over workgroups of whole atoms (at most 16 atoms and
 // FB_2C_PCAP pairs); 16 owner lanes per atom add the staged atom terms of the
 // atom's neighbours into its self blocks.  Dynamic LDS: [WRS][stride].
+// two waves per SIMD (the LDS rows of 256 lanes allow no more): the
+// unweighted form then fits in VGPRs alone, without AGPR spill slots
 template <bool W>
-__global__ __launch_bounds__(WRS) void k_rho_2c_sp(DView v, int stride) {
+__global__ __launch_bounds__(WRS, 2) void k_rho_2c_sp(DView v, int stride) {
   extern __shared__ double s_t[];
   __shared__ int s_n[WRS];
   const int tid = threadIdx.x;
