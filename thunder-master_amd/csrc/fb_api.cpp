@@ -264,7 +264,8 @@ struct fb_ctx {
   DevBuf d_ppn[3], d_ppoff[3], d_ppj[3], d_ppb[3], d_ppatom[3], d_ppself[3];
   DevBuf d_pp_map, d_px_map, d_px_point, d_inv_pp, d_inv_px, d_cand_cnt, d_cand_off, d_cand_j, d_cand_b;
   DevBuf d_pc_n, d_pc_off, d_pc_trip, d_pc_tpair, d_pc_trip_s, d_pc_tpair_s, d_pc_perm_in, d_pc_perm, d_pc_flag,
-      d_pc_segincl, d_pc_seg_off, d_pc_seg_pair, d_pc_trip_seg, d_pc_tgt, d_pc_mp, d_pc_tiles, d_ppp_disp;
+      d_pc_segincl, d_pc_seg_off, d_pc_seg_pair, d_pc_trip_seg, d_pc_tgt, d_pc_mp, d_pc_tiles, d_ppp_disp, d_pk_size,
+      d_pk_off, d_pk_key, d_pk_idx;
   int pp_ntiles = 0;
   DevBuf d_sv_bsz, d_sv_off, d_vl_bsz, d_vl_off, d_svnl, d_vnl;
   int64_t pp_npairs[3] = {0, 0, 0}, pp_ntrip = 0, pp_nseg = 0, sv_len = 0, vl_len = 0;
@@ -304,6 +305,9 @@ struct fb_ctx {
     q.mp = d_pc_mp.as<int2>();
     q.tiles = d_pc_tiles.as<int4>();
     q.ppp_disp = d_ppp_disp.as<double>();
+    q.pk_key = d_pk_key.as<unsigned>();
+    q.pk_idx = d_pk_idx.as<int>();
+    q.pk_off = d_pk_off.as<int64_t>();
     q.ntiles = pp_ntiles;
     q.ntrip = pp_ntrip;
     q.nseg = pp_nseg;
@@ -533,7 +537,7 @@ struct fb_ctx {
                       &d_status, &d_idx3r, &d_qin, &d_wbsz, &d_wboff, &d_pp_map, &d_px_map, &d_px_point,
                       &d_inv_pp, &d_inv_px, &d_cand_cnt, &d_cand_off, &d_cand_j, &d_cand_b, &d_pc_n, &d_pc_off,
                       &d_pc_trip, &d_pc_tpair, &d_pc_trip_s, &d_pc_tpair_s, &d_pc_perm_in, &d_pc_perm, &d_pc_flag,
-                      &d_pc_segincl, &d_pc_seg_off, &d_pc_seg_pair, &d_pc_trip_seg, &d_pc_tgt, &d_pc_mp, &d_pc_tiles, &d_ppp_disp, &d_sv_bsz, &d_sv_off,
+                      &d_pc_segincl, &d_pc_seg_off, &d_pc_seg_pair, &d_pc_trip_seg, &d_pc_tgt, &d_pc_mp, &d_pc_tiles, &d_ppp_disp, &d_pk_size, &d_pk_off, &d_pk_key, &d_pk_idx, &d_sv_bsz, &d_sv_off,
                       &d_vl_bsz, &d_vl_off, &d_svnl, &d_vnl};
     for (DevBuf *b : bufs) b->release();
     for (auto &m : d_mat) m.release();
@@ -2372,6 +2376,21 @@ int fb_build_lists_pp(fb_ctx *ctx, int64_t *npp, int64_t *nppp, int64_t *ntrip) 
   FB_HIP(ctx->d_ppp_disp.ensure(sizeof(double) * 3 * Q));
   q = ctx->pview();
   RC(ctx->launch("ppp_disp", [&] { return fbk_ppp_disp(v, q, st); }));
+  // per-atom key tables of the PPp lists (the common-neighbour match)
+  {
+    FB_HIP(ctx->d_pk_size.ensure(sizeof(int) * std::max(N, 1)));
+    FB_HIP(ctx->d_pk_off.ensure(sizeof(int64_t) * (N + 1)));
+    RC(ctx->launch("ppk_size", [&] { return fbk_ppk_size(v, q, ctx->d_pk_size.as<int>(), st); }));
+    RC(scan_i32(ctx, ctx->d_pk_size.as<int>(), ctx->d_pk_off.as<int64_t>(), N));
+    int64_t nslot = 0;
+    FB_HIP(hipMemcpyAsync(&nslot, ctx->d_pk_off.as<int64_t>() + N, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    FB_HIP(hipStreamSynchronize(st));
+    FB_HIP(ctx->d_pk_key.ensure(sizeof(unsigned) * std::max<int64_t>(nslot, 1)));
+    FB_HIP(ctx->d_pk_idx.ensure(sizeof(int) * std::max<int64_t>(nslot, 1)));
+    RC(ctx->launch("ppk_clear", [&] { return fbk_fill_i32(ctx->d_pk_key.as<int>(), nslot, -1, st); }));
+    q = ctx->pview();
+    RC(ctx->launch("ppk_insert", [&] { return fbk_ppk_insert(v, q, st); }));
+  }
   // maps, inverse maps and points
   FB_HIP(ctx->d_pp_map.ensure(sizeof(int) * std::max<int64_t>(ctx->pp_npairs[PPL_PP], 1)));
   FB_HIP(ctx->d_px_map.ensure(sizeof(int) * std::max<int64_t>(ctx->pp_npairs[PPL_PPX], 1)));

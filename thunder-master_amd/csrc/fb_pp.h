@@ -29,6 +29,11 @@ struct DPP {
   int *cand_j, *cand_b;   // PPp construction scratch
   // PP triplets: list order (common-neighbour order of each ialpha) and sorted by target
   double *ppp_disp;       // [PPp pairs][3] xl(kbeta) + r_k - r_i; NaN where kbeta lies beyond the images
+  // per-atom open-addressing tables of the PPp keys (jatom * 1024 + image):
+  // slots pk_off[i]..pk_off[i+1] (a power of two), key and list index per slot
+  unsigned *pk_key;
+  int *pk_idx;
+  int64_t *pk_off;
   int *pc_n;
   int64_t *pc_off;        // [natoms+1]
   int4 *trip;             // (ialpha, iatom, jatom global; ibeta | jbeta << 16)
@@ -54,6 +59,8 @@ int fbk_ppp_build(const DView &v, const DPP &q, int fill, int *status, hipStream
 int fbk_pp_maps(const DView &v, const DPP &q, int *status, hipStream_t st);
 int fbk_pp_common(const DView &v, const DPP &q, int fill, hipStream_t st);
 int fbk_ppp_disp(const DView &v, const DPP &q, hipStream_t st);
+int fbk_ppk_size(const DView &v, const DPP &q, int *sz, hipStream_t st);
+int fbk_ppk_insert(const DView &v, const DPP &q, hipStream_t st);
 int fbk_pp_block_sizes(const DView &v, const DPP &q, int which, int64_t *sz, hipStream_t st);
 int fbk_svnl(const DView &v, const DPP &q, hipStream_t st);
 int fbk_vnl_2c(const DView &v, const DPP &q, hipStream_t st);

@@ -357,6 +357,44 @@ __global__ void k_ppp_disp(DView v, DPP q) {
   for (int k = 0; k < 3; k++) d[k] = xlb[k] + v.ratom[3 * ka + k] - v.ratom[3 * ia + k];
 }
 
+__device__ __forceinline__ unsigned ppk_mix(unsigned k) {
+  k ^= k >> 16;
+  k *= 0x45d9f3bu;
+  k ^= k >> 16;
+  return k;
+}
+
+// slots of each atom's PPp key table: a power of two, at least twice the list
+__global__ void k_ppk_size(DView v, DPP q, int *sz) {
+  const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= v.natoms) return;
+  const int n = q.n[PPL_PPP][ia];
+  int t = 2;
+  while (t < 2 * n) t <<= 1;
+  sz[ia] = t;
+}
+
+// each PPp entry inserts its key into its atom's table (the keys of one list
+// are distinct: the list keeps first occurrences)
+__global__ void k_ppk_insert(DView v, DPP q) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= q.npairs[PPL_PPP]) return;
+  const int ia = q.atom[PPL_PPP][p];
+  const unsigned key = (unsigned)q.j[PPL_PPP][p] * 1024u + (unsigned)q.b[PPL_PPP][p];
+  const int64_t o = q.pk_off[ia];
+  const unsigned mask = (unsigned)(q.pk_off[ia + 1] - o - 1);
+  unsigned h = ppk_mix(key) & mask;
+  for (unsigned probe = 0; probe <= mask; probe++) {
+    const unsigned old = atomicCAS(&q.pk_key[o + h], 0xffffffffu, key);
+    if (old == 0xffffffffu) {
+      q.pk_idx[o + h] = (int)(p - q.off[PPL_PPP][ia]);
+      return;
+    }
+    if (old == key) return;
+    h = (h + 1) & mask;
+  }
+}
+
 // find_common_neighbors_PP (M_neighbors_PP.f90:1301-1420): ordered pairs of
 // PPx neighbours of ialpha, neither ialpha itself, kept when the bond vector
 // names an entry of iatom's PPp list (first within 1e-4).  FILL = false counts,
@@ -396,9 +434,41 @@ __global__ __launch_bounds__(LWG) void k_pp_common(DView v, DPP q) {
         }
         // first kneigh of iatom's PPp list whose displacement lies within
         // 1e-4 of r2 - r1 (entries with an image outside xl are NaN: never
-        // a match, as the reference's search skips them); eight at a time
+        // a match, as the reference's search skips them).  The entry that
+        // can match is (jatom, image with the coordinates of jbeta minus
+        // ibeta): looked up by key and accepted with the reference's test.
+        // No other entry can pass that test (two entries within 2e-4 A of
+        // one point would be coincident atoms), so it is the first match.
+        // Otherwise the list is scanned in order, eight at a time.
         const double *dp = q.ppp_disp + 3 * q.off[PPL_PPP][gia];
         const int ni = q.n[PPL_PPP][gia];
+        {
+          int a[3], c[3];
+          image_coords(ib, a);
+          image_coords(jb, c);
+          const int x = c[0] - a[0], y = c[1] - a[1], z = c[2] - a[2];
+          if (abs(x) <= 4 && abs(y) <= 4 && abs(z) <= 4 && (S.mbeta_max == FB_NIMG - 1 || (x | y | z) == 0)) {
+            int mb = 0;
+            if (x || y || z) {
+              const int L = (z + 4) * 81 + (y + 4) * 9 + (x + 4);
+              mb = (L < 364) ? L + 1 : L;
+            }
+            const unsigned key = (unsigned)ja * 1024u + (unsigned)mb;
+            const int64_t to = q.pk_off[gia];
+            const unsigned mask = (unsigned)(q.pk_off[gia + 1] - to - 1);
+            unsigned h = ppk_mix(key) & mask;
+            for (unsigned probe = 0; probe <= mask; probe++) {
+              const unsigned kk = q.pk_key[to + h];
+              if (kk == 0xffffffffu) break;
+              if (kk == key) {
+                const int k = q.pk_idx[to + h];
+                if (d_distance(r, dp + 3 * k) < 1.0e-4) m = k + 1;
+                break;
+              }
+              h = (h + 1) & mask;
+            }
+          }
+        }
         for (int k0 = 0; k0 < ni && m == -9999; k0 += 8) {
           double r3[8][3];
 #pragma unroll
@@ -1005,6 +1075,18 @@ int fbk_pp_maps(const DView &v, const DPP &q, int *status, hipStream_t st) {
   const int64_t n = q.npairs[PPL_PP] + q.npairs[PPL_PPX];
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_pp_maps, dim3(nblk(n, 128)), dim3(128), 0, st, v, q, status);
+  return (int)hipGetLastError();
+}
+
+int fbk_ppk_size(const DView &v, const DPP &q, int *sz, hipStream_t st) {
+  if (v.natoms == 0) return 0;
+  hipLaunchKernelGGL(k_ppk_size, dim3(nblk(v.natoms, 256)), dim3(256), 0, st, v, q, sz);
+  return (int)hipGetLastError();
+}
+
+int fbk_ppk_insert(const DView &v, const DPP &q, hipStream_t st) {
+  if (q.npairs[PPL_PPP] == 0) return 0;
+  hipLaunchKernelGGL(k_ppk_insert, dim3(nblk(q.npairs[PPL_PPP], 256)), dim3(256), 0, st, v, q);
   return (int)hipGetLastError();
 }
 
