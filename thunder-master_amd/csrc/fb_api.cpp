@@ -29,6 +29,9 @@
 extern "C" {
 int fbk_neighbors_count(const DView &v, int *neighn, int *status, hipStream_t st);
 int fbk_pair_disp(const DView &v, double *disp, hipStream_t st);
+int fbk_tiles_atom(int natoms, const int64_t *pair_off, const int64_t *seg_off, const int *seg_pair, int64_t nseg,
+                   int ch, int fill, int *cnt, const int64_t *tile_off, int4 *tiles, hipStream_t st);
+int fbk_gather_i64(int n, const int64_t *src, const int64_t *idx, int64_t *dst, hipStream_t st);
 int fbk_cell_count(const DView &v, int *cnt, int *atom_cell, hipStream_t st);
 int fbk_cell_fill(const DView &v, const int *atom_cell, int *cursor, int *cell_atoms, hipStream_t st);
 int fbk_neighbors_fill(const DView &v, const int64_t *noff, int *neigh_j, int *neigh_b, int *neigh_self,
@@ -221,6 +224,7 @@ struct fb_ctx {
   DevBuf d_grid2c, d_pden, d_wg2c;
   DevBuf d_cell_cnt, d_cell_start, d_cell_atoms, d_atom_cell;   // cell lists of the neighbour search
   DevBuf d_pair_disp;   // [npairs][3] neighbour displacements of the common-neighbour search
+  DevBuf d_tile_cnt, d_tile_off, d_gidx, d_gout;   // tile packing per atom; gathers of per-structure offsets
   int64_t ncells = 0;
   int cell_bits = 0;
   int canon_sp = 0;
@@ -529,7 +533,7 @@ struct fb_ctx {
 
   ~fb_ctx() {
     DevBuf *bufs[] = {&d_cell2c, &d_idx2c, &d_grp3c, &d_idx3c, &d_tab2c, &d_tab2c32, &d_tab3c, &d_tab3c32,
-                      &d_mu, &d_nu, &d_mv, &d_grid2c, &d_pden, &d_wg2c, &d_cell_cnt, &d_cell_start, &d_cell_atoms, &d_atom_cell, &d_pair_disp, &d_st, &d_xl, &d_ratom, &d_imass, &d_atom_struct, &d_dQ,
+                      &d_mu, &d_nu, &d_mv, &d_grid2c, &d_pden, &d_wg2c, &d_cell_cnt, &d_cell_start, &d_cell_atoms, &d_atom_cell, &d_pair_disp, &d_tile_cnt, &d_tile_off, &d_gidx, &d_gout, &d_st, &d_xl, &d_ratom, &d_imass, &d_atom_struct, &d_dQ,
                       &d_neighn, &d_noff, &d_neigh_j, &d_neigh_b, &d_neigh_back, &d_neigh_self, &d_pair_atom,
                       &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
                       &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_pair_tgt, &d_trip_aux, &d_tiles[0], &d_tiles[1], &d_tiles[2], &d_tmp,
@@ -1240,17 +1244,45 @@ static int scan_i64(fb_ctx *ctx, const int64_t *in, int64_t *out, int64_t n) {
   return ctx->launch("scan", [&] { return fbk_scan_i64(in, out, n, ctx->d_tmp.p, &bytes, ctx->stream); });
 }
 
-// Whole target segments packed greedily into tiles of at most ch triplets
-// (a longer segment gets a tile of its own): (first, last segment, first
-// triplet, end triplet) per tile, from the segment offsets so[0..nseg].
-static void pack_tiles(const std::vector<int64_t> &so, int nseg, int ch, std::vector<int4> &out) {
-  out.clear();
-  for (int s0 = 0; s0 < nseg;) {
-    int e = s0;
-    while (e + 1 < nseg && so[e + 2] - so[s0] <= ch) e++;
-    out.push_back(make_int4(s0, e, (int)so[s0], (int)so[e + 1]));
-    s0 = e + 1;
-  }
+// Tiles of whole target segments, at most ch triplets each (a longer segment
+// alone), packed per atom on the device: count, scan, fill.  Returns the
+// number of tiles in *ntiles.
+static int build_tiles(fb_ctx *ctx, const int64_t *pair_off, const int64_t *seg_off, const int *seg_pair, int64_t nseg,
+                       int ch, DevBuf &tiles, int *ntiles) {
+  const int N = ctx->natoms;
+  hipStream_t st = ctx->stream;
+  FB_HIP(ctx->d_tile_cnt.ensure(sizeof(int) * std::max(N, 1)));
+  FB_HIP(ctx->d_tile_off.ensure(sizeof(int64_t) * (N + 1)));
+  RC(ctx->launch("tiles_count", [&] {
+    return fbk_tiles_atom(N, pair_off, seg_off, seg_pair, nseg, ch, 0, ctx->d_tile_cnt.as<int>(), nullptr, nullptr, st);
+  }));
+  RC(scan_i32(ctx, ctx->d_tile_cnt.as<int>(), ctx->d_tile_off.as<int64_t>(), N));
+  int64_t n = 0;
+  FB_HIP(hipMemcpyAsync(&n, ctx->d_tile_off.as<int64_t>() + N, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  FB_HIP(hipStreamSynchronize(st));
+  FB_HIP(tiles.ensure(sizeof(int4) * std::max<int64_t>(n, 1)));
+  RC(ctx->launch("tiles_fill", [&] {
+    return fbk_tiles_atom(N, pair_off, seg_off, seg_pair, nseg, ch, 1, nullptr, ctx->d_tile_off.as<int64_t>(),
+                          tiles.as<int4>(), st);
+  }));
+  *ntiles = (int)n;
+  return FB_OK;
+}
+
+// out[k] = src[idx[k]] from a device array, one gather and one copy
+static int gather_to_host(fb_ctx *ctx, const int64_t *src, const std::vector<int64_t> &idx, int64_t *out) {
+  const int n = (int)idx.size();
+  if (n == 0) return FB_OK;
+  hipStream_t st = ctx->stream;
+  FB_HIP(ctx->d_gidx.ensure(sizeof(int64_t) * n));
+  FB_HIP(ctx->d_gout.ensure(sizeof(int64_t) * n));
+  FB_HIP(hipMemcpyAsync(ctx->d_gidx.p, idx.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, st));
+  RC(ctx->launch("gather", [&] {
+    return fbk_gather_i64(n, src, ctx->d_gidx.as<int64_t>(), ctx->d_gout.as<int64_t>(), st);
+  }));
+  FB_HIP(hipMemcpyAsync(out, ctx->d_gout.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st));
+  FB_HIP(hipStreamSynchronize(st));
+  return FB_OK;
 }
 
 int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
@@ -1387,23 +1419,13 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
     v.ntrip = T;
     v.nseg = nseg;
     RC(ctx->launch("trip_aux", [&] { return fbk_trip_aux(v, ctx->d_trip_aux.as<int>(), st); }));
-    // mixed-kernel tiles: whole segments packed greedily, at most one chunk of
-    // triplets per tile unless a single segment is longer, so that a
-    // workgroup makes one pass over its triplets
-    std::vector<int64_t> hso((size_t)nseg + 1);
-    FB_HIP(hipMemcpyAsync(hso.data(), ctx->d_seg_off.as<int64_t>(), sizeof(int64_t) * (nseg + 1), hipMemcpyDeviceToHost,
-                          st));
-    FB_HIP(hipStreamSynchronize(st));
+    // mixed-kernel tiles: whole segments, at most one chunk of triplets per
+    // tile unless a single segment is longer, so that a workgroup makes one
+    // pass over its triplets
     const int chs[3] = {64, 128, 256};
-    std::vector<int4> ht;
-    for (int k = 0; k < 3; k++) {
-      pack_tiles(hso, nseg, chs[k], ht);
-      ctx->ntiles[k] = (int)ht.size();
-      FB_HIP(ctx->d_tiles[k].ensure(sizeof(int4) * std::max<size_t>(ht.size(), 1)));
-      if (!ht.empty())
-        FB_HIP(hipMemcpyAsync(ctx->d_tiles[k].p, ht.data(), sizeof(int4) * ht.size(), hipMemcpyHostToDevice, st));
-      FB_HIP(hipStreamSynchronize(st));
-    }
+    for (int k = 0; k < 3; k++)
+      RC(build_tiles(ctx, ctx->d_noff.as<int64_t>(), ctx->d_seg_off.as<int64_t>(), ctx->d_seg_pair.as<int>(), nseg,
+                     chs[k], ctx->d_tiles[k], &ctx->ntiles[k]));
   }
   // block offsets
   v = ctx->view();
@@ -1412,9 +1434,10 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
   FB_HIP(hipMemcpyAsync(&ctx->blen, ctx->d_boff.as<int64_t>() + P, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   RC(ctx->check_status("find_common_neighbors"));
   ctx->h_boff_struct.resize(ctx->structs.size() + 1);
-  for (size_t k = 0; k < ctx->structs.size(); k++) {
-    int64_t p0 = ctx->h_noff[ctx->h_st[k].atom_off];
-    FB_HIP(hipMemcpy(&ctx->h_boff_struct[k], ctx->d_boff.as<int64_t>() + p0, sizeof(int64_t), hipMemcpyDeviceToHost));
+  {
+    std::vector<int64_t> idx(ctx->structs.size());
+    for (size_t k = 0; k < idx.size(); k++) idx[k] = ctx->h_noff[ctx->h_st[k].atom_off];
+    RC(gather_to_host(ctx, ctx->d_boff.as<int64_t>(), idx, ctx->h_boff_struct.data()));
   }
   ctx->h_boff_struct[ctx->structs.size()] = ctx->blen;
   for (int i = 0; i < 5; i++) {
@@ -2197,9 +2220,10 @@ static int prep_rho(fb_ctx *ctx) {
   FB_HIP(hipMemcpyAsync(&ctx->wlen, ctx->d_wboff.as<int64_t>() + P, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   FB_HIP(hipStreamSynchronize(st));
   ctx->h_wboff_struct.resize(ctx->structs.size() + 1);
-  for (size_t k = 0; k < ctx->structs.size(); k++) {
-    int64_t p0 = ctx->h_noff[ctx->h_st[k].atom_off];
-    FB_HIP(hipMemcpy(&ctx->h_wboff_struct[k], ctx->d_wboff.as<int64_t>() + p0, sizeof(int64_t), hipMemcpyDeviceToHost));
+  {
+    std::vector<int64_t> idx(ctx->structs.size());
+    for (size_t k = 0; k < idx.size(); k++) idx[k] = ctx->h_noff[ctx->h_st[k].atom_off];
+    RC(gather_to_host(ctx, ctx->d_wboff.as<int64_t>(), idx, ctx->h_wboff_struct.data()));
   }
   ctx->h_wboff_struct[ctx->structs.size()] = ctx->wlen;
   for (int m = FB_MAT_RHO_IN; m <= FB_MAT_RHO_BOND; m++)
@@ -2466,13 +2490,8 @@ int fb_build_lists_pp(fb_ctx *ctx, int64_t *npp, int64_t *nppp, int64_t *ntrip) 
     FB_HIP(hipStreamSynchronize(st));
     ctx->pp_nseg = nseg;
     // one-pass tiles of the s/p vnl_3c kernel (whole segments, at most 64 triplets)
-    std::vector<int64_t> hso((size_t)nseg + 1);
-    FB_HIP(hipMemcpy(hso.data(), ctx->d_pc_seg_off.p, sizeof(int64_t) * (nseg + 1), hipMemcpyDeviceToHost));
-    std::vector<int4> ht;
-    pack_tiles(hso, nseg, 64, ht);
-    FB_HIP(ctx->d_pc_tiles.ensure(sizeof(int4) * std::max<size_t>(ht.size(), 1)));
-    if (!ht.empty()) FB_HIP(hipMemcpy(ctx->d_pc_tiles.p, ht.data(), sizeof(int4) * ht.size(), hipMemcpyHostToDevice));
-    ctx->pp_ntiles = (int)ht.size();
+    RC(build_tiles(ctx, ctx->d_ppoff[PPL_PPP].as<int64_t>(), ctx->d_pc_seg_off.as<int64_t>(),
+                   ctx->d_pc_seg_pair.as<int>(), nseg, 64, ctx->d_pc_tiles, &ctx->pp_ntiles));
     q = ctx->pview();
     RC(ctx->launch("pp_mpair", [&] { return fbk_pp_mpair(v, q, st); }));
   }
@@ -2497,12 +2516,15 @@ int fb_build_lists_pp(fb_ctx *ctx, int64_t *npp, int64_t *nppp, int64_t *ntrip) 
   const size_t ns = ctx->structs.size();
   ctx->h_sv_struct.assign(ns + 1, 0);
   ctx->h_vl_struct.assign(ns + 1, 0);
-  for (size_t k = 0; k < ns; k++) {
-    const int a0 = ctx->h_st[k].atom_off;
-    FB_HIP(hipMemcpy(&ctx->h_sv_struct[k], ctx->d_sv_off.as<int64_t>() + ctx->h_ppoff[PPL_PP][a0], sizeof(int64_t),
-                     hipMemcpyDeviceToHost));
-    FB_HIP(hipMemcpy(&ctx->h_vl_struct[k], ctx->d_vl_off.as<int64_t>() + ctx->h_ppoff[PPL_PPP][a0], sizeof(int64_t),
-                     hipMemcpyDeviceToHost));
+  {
+    std::vector<int64_t> isv(ns), ivl(ns);
+    for (size_t k = 0; k < ns; k++) {
+      const int a0 = ctx->h_st[k].atom_off;
+      isv[k] = ctx->h_ppoff[PPL_PP][a0];
+      ivl[k] = ctx->h_ppoff[PPL_PPP][a0];
+    }
+    RC(gather_to_host(ctx, ctx->d_sv_off.as<int64_t>(), isv, ctx->h_sv_struct.data()));
+    RC(gather_to_host(ctx, ctx->d_vl_off.as<int64_t>(), ivl, ctx->h_vl_struct.data()));
   }
   ctx->h_sv_struct[ns] = ctx->sv_len;
   ctx->h_vl_struct[ns] = ctx->vl_len;

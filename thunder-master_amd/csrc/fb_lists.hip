@@ -297,10 +297,67 @@ __global__ void k_block_sizes(DView v, int64_t *sz) {
   sz[p] = (int64_t)v.sp.norb[v.imass[ia] - 1] * v.sp.norb[v.imass[ja] - 1];
 }
 
+// Tiles of whole target segments for the segment-owning kernels, packed per
+// atom (a segment's target pair belongs to one atom, so a tile never needs to
+// span atoms): greedily at most ch triplets per tile, a longer segment alone.
+// One thread per atom over its segments, found by binary search of its pair
+// range in the sorted segment pairs; FILL = false counts, FILL = true writes
+// (first, last segment, first, end triplet).  Any partition into whole
+// segments gives the kernels the same per-segment order, hence the same bits.
+template <bool FILL>
+__global__ void k_tiles_atom(int natoms, const int64_t *pair_off, const int64_t *seg_off, const int *seg_pair,
+                             int64_t nseg, int ch, int *cnt, const int64_t *tile_off, int4 *tiles) {
+  const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= natoms) return;
+  auto lower = [&](int64_t key) {
+    int64_t lo = 0, hi = nseg;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if ((int64_t)seg_pair[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  const int64_t s0 = lower(pair_off[ia]), s1 = lower(pair_off[ia + 1]);
+  int count = 0;
+  for (int64_t s = s0; s < s1;) {
+    int64_t e = s;
+    while (e + 1 < s1 && seg_off[e + 2] - seg_off[s] <= ch) e++;
+    if (FILL) tiles[tile_off[ia] + count] = make_int4((int)s, (int)e, (int)seg_off[s], (int)seg_off[e + 1]);
+    count++;
+    s = e + 1;
+  }
+  if (!FILL) cnt[ia] = count;
+}
+
+__global__ void k_gather_i64(int n, const int64_t *src, const int64_t *idx, int64_t *dst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[idx[k]];
+}
+
 // ----------------------------------------------------------------- launchers
 #include <hipcub/hipcub.hpp>
 
 extern "C" {
+
+int fbk_tiles_atom(int natoms, const int64_t *pair_off, const int64_t *seg_off, const int *seg_pair, int64_t nseg,
+                   int ch, int fill, int *cnt, const int64_t *tile_off, int4 *tiles, hipStream_t st) {
+  if (natoms == 0) return 0;
+  const dim3 g((unsigned)((natoms + 255) / 256));
+  if (fill)
+    hipLaunchKernelGGL(k_tiles_atom<true>, g, dim3(256), 0, st, natoms, pair_off, seg_off, seg_pair, nseg, ch, cnt,
+                       tile_off, tiles);
+  else
+    hipLaunchKernelGGL(k_tiles_atom<false>, g, dim3(256), 0, st, natoms, pair_off, seg_off, seg_pair, nseg, ch, cnt,
+                       tile_off, tiles);
+  return (int)hipGetLastError();
+}
+
+int fbk_gather_i64(int n, const int64_t *src, const int64_t *idx, int64_t *dst, hipStream_t st) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_gather_i64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, src, idx, dst);
+  return (int)hipGetLastError();
+}
 
 int fbk_cell_count(const DView &v, int *cnt, int *atom_cell, hipStream_t st) {
   if (v.natoms == 0) return 0;
