@@ -380,3 +380,48 @@ def test_full_size_workloads_mixed_vs_fp64(cfg):
         assert err <= TOL_MIXED, m
     for k in ks:
         _list_invariants(asm, k, structs[k])
+
+
+def test_atoms_and_structures_without_triplets():
+    """Tiles are packed per atom from each atom's range of target segments.
+    An isolated atom and the two atoms of a dimer own no segment (no two
+    distinct non-self neighbours), and a lone dimer is a whole structure with
+    no triplet at all; it comes first in the batch so that the per-structure
+    tile and block offsets start from an empty structure.  Lists must equal
+    the oracle's bit for bit and the blocks its values, in binary64 and in the
+    mixed path at every workgroup chunk."""
+    from fireball_hip import synth
+    box = np.eye(3) * 60.0
+    dimer = np.array([[10.0, 10.0, 10.0], [12.3, 10.0, 10.0]])
+    st0 = synth.Structure(dimer.copy(), np.ones(2, np.int32), box, 1)
+    si8 = synth.diamond_si((1, 1, 1), sigma=0.05, seed=3).ratom + 10.0
+    pos1 = np.vstack([si8, [[45.0, 45.0, 45.0]], dimer + 25.0])
+    st1 = synth.Structure(pos1, np.ones(len(pos1), np.int32), box, 1)
+    st2 = synth.diamond_si((1, 1, 1), sigma=0.06, seed=5)
+    structs = [st0, st1, st2]
+    ts = synth.make_tables([synth.SI])
+    asm = _gpu(ts, structs, precision="fp64")
+    tab = oracle.Tables(ts)
+    ref = [oracle.System(tab, st).run_all() for st in structs]
+    assert np.all(ref[0]["ncommon"] == 0)
+    assert np.all(ref[1]["ncommon"][8:] == 0) and np.all(ref[1]["ncommon"][:8] > 0)
+    fp64 = []
+    for k in range(len(structs)):
+        r = asm.run_all(k)
+        for key in LIST_KEYS:
+            assert np.array_equal(r[key], ref[k][key]), (k, key)
+        for m in MATS:
+            np.testing.assert_allclose(r[m], ref[k][m], rtol=0, atol=TOL64, err_msg=f"{k} {m}")
+        fp64.append(r)
+    asm.set_precision("mixed")
+    got = {}
+    for chunk in (64, 128, 256):
+        print(f"structures without triplets: mixed, chunk {chunk}", flush=True)
+        asm.set_chunk3c(chunk)
+        asm.assemble_all()
+        got[chunk] = [asm.run_all(k)["VNA"] for k in range(len(structs))]
+        for k in range(len(structs)):
+            err = float(np.max(np.abs(got[chunk][k] - ref[k]["VNA"])))
+            assert err <= TOL_MIXED, (chunk, k, err)
+    for k in range(len(structs)):
+        assert np.array_equal(got[64][k], got[128][k]) and np.array_equal(got[64][k], got[256][k]), k
