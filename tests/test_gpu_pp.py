@@ -200,3 +200,33 @@ def test_pp_full_size_kb_operator(cfg):
         assert set(ref) == set(row), f"atom {i}: block pattern"
         for j in ref:
             np.testing.assert_allclose(row[j], ref[j], atol=1e-12 * scale, err_msg=f"atom {i} block {j}")
+
+
+def test_pp_atoms_and_structures_without_triplets():
+    """PP triplets are tiled per atom like the 3c ones.  A lone dimer (first
+    in the batch, so every per-structure offset starts from it), and a cluster
+    with an isolated atom and a distant dimer, give atoms that own no PP
+    triplet; lists and blocks must equal the oracle's bit for bit."""
+    sp = synth.with_pp([synth.SI])
+    ts = synth.add_pp_tables(synth.make_tables(sp, seed=5), seed=6)
+    box = np.eye(3) * 60.0
+    dimer = np.array([[10.0, 10.0, 10.0], [12.3, 10.0, 10.0]])
+    si8 = synth.diamond_si((1, 1, 1), sigma=0.05, seed=3).ratom + 10.0
+    pos1 = np.vstack([si8, [[45.0, 45.0, 45.0]], dimer + 25.0])
+    structs = [synth.Structure(dimer.copy(), np.ones(2, np.int32), box, 1),
+               synth.Structure(pos1, np.ones(len(pos1), np.int32), box, 1),
+               synth.diamond_si((1, 1, 1), sigma=0.06, seed=5)]
+    tab = oracle.Tables(ts)
+    refs = []
+    for st in structs:
+        o = oracle.System(tab, st)
+        o.build_lists()
+        refs.append(o.run_pp())
+    asm = _gpu_pp(ts, structs)
+    for k, ref in enumerate(refs):
+        print(f"structure {k} natoms={structs[k].natoms}", flush=True)
+        L = asm.pp_lists(k)
+        for key in PP_LIST_KEYS:
+            _same(L[key], ref[key], f"structure {k} {key}")
+        for key in ("SVNL", "VNL"):
+            _same(asm.pp_blocks(key, k), ref[key], f"structure {k} {key}")
