@@ -35,6 +35,7 @@
         integer in1, in2, ntot_pairs, ntot_common, nmu, nnu
         integer(kind=8) :: c0, c1, crate
         real t_S, t_T, t_dip, t_vna2c, t_vna3c
+        real(kind=8) dqscale
         character(len=512) :: infile, rhofile, outfile, arg
         character(len=128) :: floc
 
@@ -46,6 +47,13 @@
         if (nargs .ge. 4) then
           call get_command_argument(4, arg)
           read (arg, *) nrep
+        end if
+        ! optional: scale every shell charge after the H+S assembly, as a
+        ! charge mixer does between the last assemble_vna and the force calls
+        dqscale = 1.0d0
+        if (nargs .ge. 5) then
+          call get_command_argument(5, arg)
+          read (arg, *) dqscale
         end if
 
         open (unit = ilogfile, file = 'fb_dharness.log', status = 'unknown')
@@ -102,6 +110,13 @@
         call assemble_dipole_z (st)
         call assemble_vna_2c (st)
         call assemble_vna_3c (st)
+        if (dqscale .ne. 1.0d0) then
+          do iatom = 1, st%natoms
+            do issh = 1, species(st%atom(iatom)%imass)%nssh
+              st%atom(iatom)%shell(issh)%dQ = st%atom(iatom)%shell(issh)%dQ*dqscale
+            end do
+          end do
+        end if
 
         ! density matrix blocks, same layout as the H/S blocks
         allocate (st%denmat(st%natoms))

@@ -14,10 +14,13 @@
 ! of this project that call the GPU library; the output files are then
 ! comparable byte for byte in layout.
 !
-! Usage:  fb_harness <input> <output> [nrep]
+! Usage:  fb_harness <input> <output> [nrep [vnabump]]
 !   input  : written by fireball_hip.fdata_io.write_harness_input
 !   output : stream-unformatted dump of lists, blocks and timings
 !   nrep   : repeat the assemble_* sequence nrep times for timing (default 1)
+!   vnabump: add this value to every element of s%vna(:)%neighbors(:)%block
+!            between assemble_vna_2c and assemble_vna_3c (a caller that edits
+!            the blocks in between; default 0)
         program fb_harness
         use M_precision
         use M_species
@@ -34,6 +37,7 @@
         integer in1, in2, ialpha, ntot_pairs, ntot_common
         integer(kind=8) :: c0, c1, crate
         real t_lists, t_S, t_T, t_dip, t_vna2c, t_vna3c
+        real(kind=8) vnabump
         character(len=512) :: infile, outfile, arg
         character(len=128) :: floc
 
@@ -44,6 +48,11 @@
         if (nargs .ge. 3) then
           call get_command_argument(3, arg)
           read (arg, *) nrep
+        end if
+        vnabump = 0.0d0
+        if (nargs .ge. 4) then
+          call get_command_argument(4, arg)
+          read (arg, *) vnabump
         end if
 
         open (unit = ilogfile, file = 'fb_harness.log', status = 'unknown')
@@ -123,6 +132,14 @@
           call assemble_vna_2c (st)
           call system_clock (c1)
           t_vna2c = t_vna2c + real(c1 - c0)/real(crate)
+          if (vnabump .ne. 0.0d0) then
+            do iatom = 1, st%natoms
+              do ineigh = 1, st%neighbors(iatom)%neighn
+                st%vna(iatom)%neighbors(ineigh)%block =                        &
+     &            st%vna(iatom)%neighbors(ineigh)%block + vnabump
+              end do
+            end do
+          end if
           call system_clock (c0)
           call assemble_vna_3c (st)
           call system_clock (c1)

@@ -85,8 +85,10 @@ def parse_output(path, species, imass):
     return out
 
 
-def run(ts, structure, binary=HARNESS_REF, nrep=1, workdir=None, timeout=3600):
-    """Write tables + structure, run the harness, return the parsed output."""
+def run(ts, structure, binary=HARNESS_REF, nrep=1, workdir=None, timeout=3600, vna_bump=None):
+    """Write tables + structure, run the harness, return the parsed output.
+    vna_bump: add it to every Vna block element between assemble_vna_2c and
+    assemble_vna_3c."""
     own = workdir is None
     tmp = tempfile.mkdtemp(prefix="fbref_") if own else workdir
     fdir = os.path.join(tmp, "Fdata")
@@ -96,7 +98,8 @@ def run(ts, structure, binary=HARNESS_REF, nrep=1, workdir=None, timeout=3600):
     fdata_io.write_harness_input(inp, fdir, ts, structure)
     outp = os.path.join(tmp, "harness.out")
     env = dict(os.environ, OMP_NUM_THREADS="1")
-    subprocess.run([binary, inp, outp, str(nrep)], cwd=tmp, check=True, env=env,
+    args = [binary, inp, outp, str(nrep)] + ([repr(float(vna_bump))] if vna_bump is not None else [])
+    subprocess.run(args, cwd=tmp, check=True, env=env,
                    stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=timeout)
     res = parse_output(outp, ts.species, structure.imass)
     if own:
@@ -142,9 +145,12 @@ def parse_forces(path, species, imass, neighn, neigh_j):
     return out
 
 
-def run_forces(ts, structure, rho, lists, binary=HARNESS_DREF, nrep=1, workdir=None, timeout=3600):
+def run_forces(ts, structure, rho, lists, binary=HARNESS_DREF, nrep=1, workdir=None, timeout=3600,
+               dq_scale=None):
     """Run the force harness on (tables, structure) with density blocks rho
-    (pair order of `lists`, the reference's neighbour lists)."""
+    (pair order of `lists`, the reference's neighbour lists).  dq_scale:
+    multiply every shell charge by it between the H+S assembly and the force
+    assemblers (what a charge mixer does in an SCF loop)."""
     own = workdir is None
     tmp = tempfile.mkdtemp(prefix="fbdref_") if own else workdir
     fdir = os.path.join(tmp, "Fdata")
@@ -156,7 +162,8 @@ def run_forces(ts, structure, rho, lists, binary=HARNESS_DREF, nrep=1, workdir=N
     np.ascontiguousarray(rho, np.float64).tofile(rhof)
     outp = os.path.join(tmp, "dharness.out")
     env = dict(os.environ, OMP_NUM_THREADS="1")
-    subprocess.run([binary, inp, rhof, outp, str(nrep)], cwd=tmp, check=True, env=env,
+    args = [binary, inp, rhof, outp, str(nrep)] + ([repr(float(dq_scale))] if dq_scale is not None else [])
+    subprocess.run(args, cwd=tmp, check=True, env=env,
                    stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=timeout)
     res = parse_forces(outp, ts.species, structure.imass, lists["neighn"], lists["neigh_j"])
     if own:

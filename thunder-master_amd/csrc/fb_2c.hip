@@ -8,7 +8,10 @@
 // every needed table cell (7-point Lagrange), rotates and stores the blocks.
 // The Vna atom term <i|V_j|i> of every neighbour is summed into the self block
 // of iatom in neighbour order (M_assemble_2c_DOGS.f90:710-816) through LDS, so
-// the sum is deterministic and in the reference's order.
+// the sum is deterministic and in the reference's order.  With shell charges
+// a neighbour contributes one term per charged shell besides the neutral one,
+// and the reference adds each to the block on its own (:790-812): the terms
+// are kept apart in LDS and added one by one.
 //
 // The kernel is bound by the block stores (HBM); its arithmetic stays binary64.
 #include "fb_device.h"
@@ -40,7 +43,8 @@ __device__ inline int me2c_rotated(const DView &v, int cid, int in1, int in3, in
 // mask bits: 1 = S, 2 = T, 4 = dipole_z, 8 = vna (ontop into blocko, atom into
 // block(neigh_self), zero non-self block for the 3c accumulation)
 __global__ __launch_bounds__(W2C) void k_2c(DView v, int mask) {
-  __shared__ double s_atom[W2C][NB + 1];
+  // atom terms of the chunk's neighbours: [0] neutral, [isorp] charged shell isorp
+  __shared__ double s_atom[1 + FB_MAXSH][W2C][NB + 1];
   const int ia = blockIdx.x;
   const int lane = threadIdx.x;
   const int s = v.atom_struct[ia];
@@ -61,6 +65,8 @@ __global__ __launch_bounds__(W2C) void k_2c(DView v, int mask) {
     const bool active = k < n;
     double atom_c[NB];
     for (int i = 0; i < NB; i++) atom_c[i] = 0.0;
+    for (int q = 1; q <= FB_MAXSH; q++)
+      for (int i = 0; i < NB; i++) s_atom[q][lane][i] = 0.0;
     if (active) {
       const int64_t p = p0 + k;
       const int ja = St.atom_off + v.neigh_j[p] - 1;
@@ -133,19 +139,24 @@ __global__ __launch_bounds__(W2C) void k_2c(DView v, int mask) {
             status |= me2c_rotated(v, cell2c_id(v, in1, in2, SLOT_ATOM, isorp), in1, in1, nmu, z, pm, X);
             for (int i = 0; i < na; i++) {
               double emnpl = (z > 1.0e-4) ? sself[i] / z : 0.0;
-              atom_c[i] = atom_c[i] + FB_P_EQ2 * dQ * ((1.0 - smooth) * emnpl + smooth * X[i]);
+              s_atom[isorp][lane][i] = FB_P_EQ2 * dQ * ((1.0 - smooth) * emnpl + smooth * X[i]);
             }
           }
         }
       }
     }
     if (mask & 8) {
-      // ordered sum over this chunk's neighbours into the self block
-      for (int i = 0; i < NB; i++) s_atom[lane][i] = atom_c[i];
+      // ordered sum over this chunk's neighbours into the self block, each
+      // neighbour's terms one at a time (an absent charged term is an exact zero)
+      for (int i = 0; i < NB; i++) s_atom[0][lane][i] = atom_c[i];
       __syncthreads();
       if (lane < nmu * nmu) {
         int kmax = min(W2C, n - c0);
-        for (int kk = 0; kk < kmax; kk++) acc_atom = acc_atom + s_atom[kk][lane];
+        const int nq = v.any_dq ? FB_MAXSH : 0;
+        for (int kk = 0; kk < kmax; kk++) {
+          acc_atom = acc_atom + s_atom[0][kk][lane];
+          for (int q = 1; q <= nq; q++) acc_atom = acc_atom + s_atom[q][kk][lane];
+        }
       }
       __syncthreads();
     }
@@ -191,9 +202,14 @@ __device__ __forceinline__ int block2c(const DView &v, int cid, double x, const 
   return 0;
 }
 
-template <int CLS>
+// atom_c receives the neutral atom term; with KQ > 0 the charged terms of
+// shells 1..KQ go to qrow[(isorp - 1) * QROW + i] (LDS rows that the caller
+// cleared), so that the self-block sum can add each term on its own
+#define QROW (WG2 * 17)
+template <int CLS, int KQ>
 __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, int in1, int in2, bool self,
-                                      bool keep_vna, double z, const double *pm, int64_t bo, double *atom_c) {
+                                      bool keep_vna, double z, const double *pm, int64_t bo, double *atom_c,
+                                      double *qrow) {
   constexpr int NMU = (CLS & 2) ? 4 : 1, NNU = (CLS & 1) ? 4 : 1, NBL = NMU * NNU;
   constexpr int CLSA = (CLS & 2) ? CLS_SPSP : CLS_SS;   // atom term lives on iatom's orbitals
   int status = 0, grid_cached = -1, ileft = 1;
@@ -262,7 +278,9 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
 #pragma unroll
         for (int i = 0; i < NA; i++) {
           double emnpl = (z > 1.0e-4) ? sself[i] / z : 0.0;
-          atom_c[i] = atom_c[i] + FB_P_EQ2 * dQ * ((1.0 - smooth) * emnpl + smooth * X[i]);
+          const double t = FB_P_EQ2 * dQ * ((1.0 - smooth) * emnpl + smooth * X[i]);
+          if (KQ > 0 && isorp <= KQ) qrow[(isorp - 1) * QROW + i] = t;
+          else atom_c[i] = atom_c[i] + t;
         }
       }
     }
@@ -270,8 +288,11 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
   return status;
 }
 
+// KQ: charged atom-term rows kept apart (0 for neutral builds, 2 = the most
+// shells of an s/p species, for charged ones)
+template <int KQ>
 __global__ __launch_bounds__(WG2, 3) void k_2c_sp(DView v, int mask) {
-  __shared__ double s_at[WG2][17];
+  __shared__ double s_at[1 + KQ][WG2][17];
   const int tid = threadIdx.x;
   const int a0 = v.wg2c[blockIdx.x], a1 = v.wg2c[blockIdx.x + 1];
   const int64_t pb = v.noff[a0], pe = v.noff[a1];
@@ -292,6 +313,10 @@ __global__ __launch_bounds__(WG2, 3) void k_2c_sp(DView v, int mask) {
     double atom_c[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) atom_c[i] = 0.0;
+    if (KQ > 0 && (mask & 8))
+      for (int q = 1; q <= KQ; q++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) s_at[q][tid][i] = 0.0;
     if (p < pe) {
       const int ia = v.pair_atom[p];
       const int s = v.atom_struct[ia];
@@ -318,20 +343,26 @@ __global__ __launch_bounds__(WG2, 3) void k_2c_sp(DView v, int mask) {
       const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
       const int64_t bo = v.boff[p];
       const bool keep = (mask & FB_2C_SKIP_TGT) && v.pair_tgt[p];
+      double *qrow = KQ > 0 ? &s_at[KQ > 0 ? 1 : 0][tid][0] : nullptr;
       switch (cls) {
-        case 3: status |= pair2c<3>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c); break;
-        case 2: status |= pair2c<2>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c); break;
-        case 1: status |= pair2c<1>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c); break;
-        default: status |= pair2c<0>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c); break;
+        case 3: status |= pair2c<3, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow); break;
+        case 2: status |= pair2c<2, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow); break;
+        case 1: status |= pair2c<1, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow); break;
+        default: status |= pair2c<0, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow); break;
       }
     }
     if (mask & 8) {
 #pragma unroll
-      for (int i = 0; i < 16; i++) s_at[tid][i] = atom_c[i];
+      for (int i = 0; i < 16; i++) s_at[0][tid][i] = atom_c[i];
       __syncthreads();
       if (rown && re < rna * rna) {
+        // each neighbour's terms one at a time, as the reference adds them
         const int64_t k0 = max(rp0, c0), k1 = min(rp1, c0 + WG2);
-        for (int64_t k = k0; k < k1; k++) racc = racc + s_at[k - c0][re];
+        for (int64_t k = k0; k < k1; k++) {
+          racc = racc + s_at[0][k - c0][re];
+#pragma unroll
+          for (int q = 1; q <= KQ; q++) racc = racc + s_at[q][k - c0][re];
+        }
       }
       __syncthreads();
     }
@@ -345,7 +376,10 @@ __global__ __launch_bounds__(WG2, 3) void k_2c_sp(DView v, int mask) {
 
 extern "C" int fbk_2c_sp(const DView &v, int mask, hipStream_t st) {
   if (v.nwg2c == 0) return 0;
-  hipLaunchKernelGGL(k_2c_sp, dim3(v.nwg2c), dim3(WG2), 0, st, v, mask);
+  if (v.any_dq && (mask & 8))
+    hipLaunchKernelGGL(k_2c_sp<2>, dim3(v.nwg2c), dim3(WG2), 0, st, v, mask);
+  else
+    hipLaunchKernelGGL(k_2c_sp<0>, dim3(v.nwg2c), dim3(WG2), 0, st, v, mask);
   return (int)hipGetLastError();
 }
 
@@ -655,6 +689,10 @@ __global__ __launch_bounds__(WG2) void k_2c_lds(DView v, int mask) {
 
 extern "C" int fbk_2c_lds(const DView &v, int mask, hipStream_t st) {
   if (v.nwg2c == 0) return 0;
+  // this variant sums a neighbour's charged atom terms before adding them to
+  // the self block; charged Vna builds take the s/p kernel, which adds each
+  // term on its own as the reference does
+  if (v.any_dq && (mask & 8)) return fbk_2c_sp(v, mask, st);
   hipLaunchKernelGGL(k_2c_lds, dim3(v.nwg2c), dim3(WG2), 0, st, v, mask);
   return (int)hipGetLastError();
 }

@@ -6,7 +6,11 @@
 // order).  A workgroup owns the segments that start inside its tile of
 // consecutive triplets: it computes the contributions of their triplets in
 // chunks, and one thread per segment adds them in order and stores the block
-// once.  No atomics; results are deterministic.
+// once.  No atomics; results are deterministic.  A triplet contributes one
+// term (neutral) or, with shell charges on its third atom, one more term per
+// charged shell; the reference adds each term to the block on its own
+// (M_assemble_3c_DOGS.f90:298-318), so the terms are kept apart and the
+// segment owner adds them one by one in that order.
 //
 // k_3c_fp64 is the plain binary64 kernel: one thread per triplet evaluates the
 // reference's interpolation (e.FDATA/M_Fdata_3c.f90:293-517) literally.
@@ -61,10 +65,15 @@ __device__ inline void d_geo3(const DView &v, int4 tr, Geo3 &G) {
   G.stinky = G.stinky * d_smoother(G.d23, rc2 + rc3, 0.8);
 }
 
-__global__ __launch_bounds__(T3) void k_3c_fp64(DView v) {
-  __shared__ double s_c[T3][NB + 1];
-  const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3;
-  const int64_t te = min(ts + T3, v.ntrip);
+// terms per triplet: the neutral one plus one per charged shell of the third atom
+#define KT_GEN (1 + FB_MAXSH)
+#define T3G 64   // triplets per tile of the generic kernel (its LDS holds KT_GEN terms each)
+
+__global__ __launch_bounds__(T3G) void k_3c_fp64(DView v) {
+  __shared__ double s_c[T3G][KT_GEN][NB + 1];
+  __shared__ int s_n[T3G];
+  const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3G;
+  const int64_t te = min(ts + T3G, v.ntrip);
   int64_t first, last, r0, r1;
   if (!tile_segments(v, ts, te, first, last, r0, r1)) return;
   const int64_t my = first + threadIdx.x;
@@ -74,10 +83,9 @@ __global__ __launch_bounds__(T3) void k_3c_fp64(DView v) {
   double acc[NB];
   for (int i = 0; i < NB; i++) acc[i] = 0.0;
   int status = 0;
-  for (int64_t c0 = r0; c0 < r1; c0 += T3) {
+  for (int64_t c0 = r0; c0 < r1; c0 += T3G) {
     const int64_t t = c0 + threadIdx.x;
-    double c[NB];
-    for (int i = 0; i < NB; i++) c[i] = 0.0;
+    int nt = 0;
     if (t < r1) {
       const int4 tr = v.trip[t];
       Geo3 G;
@@ -88,7 +96,8 @@ __global__ __launch_bounds__(T3) void k_3c_fp64(DView v) {
       int gid = group3c_id(v, G.in1, G.in2, G.indna, 0);
       status |= d_getMEs_3c<NME3>(v, gid, G.x, G.z, G.cost, G.nmu, G.nnu, hm);
       d_rotate<double>(v.sp, G.in1, G.in2, pm, hm, G.nmu, bx);
-      for (int i = 0; i < nb; i++) c[i] = bx[i] * FB_P_EQ2;
+      for (int i = 0; i < nb; i++) s_c[threadIdx.x][0][i] = bx[i] * FB_P_EQ2;
+      nt = 1;
       bool charged = false;
       for (int isorp = 1; isorp <= v.sp.nssh[G.indna - 1]; isorp++)
         charged |= (v.dQ[tr.x * FB_MAXSH + isorp - 1] != 0.0);
@@ -98,24 +107,26 @@ __global__ __launch_bounds__(T3) void k_3c_fp64(DView v) {
         const double *db = v.mat[FB_MAT_DIP] + v.boff[p];
         for (int isorp = 1; isorp <= v.sp.nssh[G.indna - 1]; isorp++) {
           double dQ = v.dQ[tr.x * FB_MAXSH + isorp - 1];
-          if (dQ == 0.0) continue;
+          if (dQ == 0.0) continue;   // the reference adds an exact zero here
           gid = group3c_id(v, G.in1, G.in2, G.indna, isorp);
           status |= d_getMEs_3c<NME3>(v, gid, G.x, G.z, G.cost, G.nmu, G.nnu, hm);
           d_rotate<double>(v.sp, G.in1, G.in2, pm, hm, G.nmu, bx);
           for (int i = 0; i < nb; i++) {
             double sterm = sb[i] / 2.0, dterm = db[i] / G.z;
             double emnpl = (sterm - dterm) / G.d13 + (sterm + dterm) / G.d23;
-            c[i] = c[i] + dQ * (G.stinky * bx[i] + (1.0 - G.stinky) * emnpl) * FB_P_EQ2;
+            s_c[threadIdx.x][nt][i] = dQ * (G.stinky * bx[i] + (1.0 - G.stinky) * emnpl) * FB_P_EQ2;
           }
+          nt++;
         }
       }
     }
-    for (int i = 0; i < NB; i++) s_c[threadIdx.x][i] = c[i];
+    s_n[threadIdx.x] = nt;
     __syncthreads();
     if (own) {
-      int64_t k0 = max(a, c0), k1 = min(b, c0 + T3);
+      int64_t k0 = max(a, c0), k1 = min(b, c0 + T3G);
       for (int64_t k = k0; k < k1; k++)
-        for (int i = 0; i < NB; i++) acc[i] = acc[i] + s_c[k - c0][i];
+        for (int j = 0; j < s_n[k - c0]; j++)
+          for (int i = 0; i < NB; i++) acc[i] = acc[i] + s_c[k - c0][j][i];
     }
     __syncthreads();
   }
@@ -133,7 +144,7 @@ __global__ __launch_bounds__(T3) void k_3c_fp64(DView v) {
 
 extern "C" int fbk_3c_fp64(const DView &v, hipStream_t st) {
   if (v.ntrip == 0) return 0;
-  hipLaunchKernelGGL(k_3c_fp64, dim3((unsigned)((v.ntrip + T3 - 1) / T3)), dim3(T3), 0, st, v);
+  hipLaunchKernelGGL(k_3c_fp64, dim3((unsigned)((v.ntrip + T3G - 1) / T3G)), dim3(T3G), 0, st, v);
   return (int)hipGetLastError();
 }
 
@@ -143,8 +154,14 @@ extern "C" int fbk_3c_fp64(const DView &v, hipStream_t st) {
 // operations and order as the generic path, without private-memory arrays).
 #include "fb_sp.h"
 
+// terms per triplet of s/p species: neutral, plus at most two charged shells
+#define KT_SP 3
+
+// the triplet's terms into c[k][*], k < returned count (neutral first, then
+// the charged shells in order)
 template <int CLS>
-__device__ __forceinline__ int triplet_fp64_sp(const DView &v, int64_t t, const int4 tr, const Geo3 &G, double *c) {
+__device__ __forceinline__ int triplet_fp64_sp(const DView &v, int64_t t, const int4 tr, const Geo3 &G,
+                                               double (*c)[NB + 1], int *nterms, int *status_out) {
   constexpr int NME = nme3c_of(CLS);
   constexpr int NBL = ((CLS & 2) ? 4 : 1) * ((CLS & 1) ? 4 : 1);
   double pm[9], F[NME], bx[16];
@@ -152,31 +169,36 @@ __device__ __forceinline__ int triplet_fp64_sp(const DView &v, int64_t t, const 
   int status = getMEs_3c_sp<NME>(v, group3c_id(v, G.in1, G.in2, G.indna, 0), G.x, G.z, G.cost, F);
   rot3c_sp<double>(CLS, pm, F, bx);
 #pragma unroll
-  for (int i = 0; i < NBL; i++) c[i] = bx[i] * FB_P_EQ2;
+  for (int i = 0; i < NBL; i++) c[0][i] = bx[i] * FB_P_EQ2;
+  int nt = 1;
   bool charged = false;
   for (int isorp = 1; isorp <= v.sp.nssh[G.indna - 1]; isorp++) charged |= (v.dQ[tr.x * FB_MAXSH + isorp - 1] != 0.0);
   if (charged) {
     const int p = v.trip_pair[t];
     const double *sb = v.mat[FB_MAT_S] + v.boff[p];
     const double *db = v.mat[FB_MAT_DIP] + v.boff[p];
-    for (int isorp = 1; isorp <= v.sp.nssh[G.indna - 1]; isorp++) {
+    for (int isorp = 1; isorp <= v.sp.nssh[G.indna - 1] && nt < KT_SP; isorp++) {
       double dQ = v.dQ[tr.x * FB_MAXSH + isorp - 1];
-      if (dQ == 0.0) continue;
+      if (dQ == 0.0) continue;   // the reference adds an exact zero here
       status |= getMEs_3c_sp<NME>(v, group3c_id(v, G.in1, G.in2, G.indna, isorp), G.x, G.z, G.cost, F);
       rot3c_sp<double>(CLS, pm, F, bx);
 #pragma unroll
       for (int i = 0; i < NBL; i++) {
         double sterm = sb[i] / 2.0, dterm = db[i] / G.z;
         double emnpl = (sterm - dterm) / G.d13 + (sterm + dterm) / G.d23;
-        c[i] = c[i] + dQ * (G.stinky * bx[i] + (1.0 - G.stinky) * emnpl) * FB_P_EQ2;
+        c[nt][i] = dQ * (G.stinky * bx[i] + (1.0 - G.stinky) * emnpl) * FB_P_EQ2;
       }
+      nt++;
     }
   }
-  return status;
+  *nterms = nt;
+  *status_out |= status;
+  return 0;
 }
 
 __global__ __launch_bounds__(T3) void k_3c_fp64_sp(DView v) {
-  __shared__ double s_c[T3][NB + 1];
+  __shared__ double s_c[T3][KT_SP][NB + 1];
+  __shared__ unsigned char s_n[T3];
   const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3;
   const int64_t te = min(ts + T3, v.ntrip);
   int64_t first, last, r0, r1;
@@ -191,29 +213,29 @@ __global__ __launch_bounds__(T3) void k_3c_fp64_sp(DView v) {
   int status = 0;
   for (int64_t c0 = r0; c0 < r1; c0 += T3) {
     const int64_t t = c0 + threadIdx.x;
-    double c[NB];
-#pragma unroll
-    for (int i = 0; i < NB; i++) c[i] = 0.0;
+    int nt = 0;
     if (t < r1) {
       const int4 tr = v.trip[t];
       Geo3 G;
       d_geo3(v, tr, G);
       const int cls = (G.nmu == 4 ? 2 : 0) + (G.nnu == 4 ? 1 : 0);
       switch (cls) {
-        case 3: status |= triplet_fp64_sp<3>(v, t, tr, G, c); break;
-        case 2: status |= triplet_fp64_sp<2>(v, t, tr, G, c); break;
-        case 1: status |= triplet_fp64_sp<1>(v, t, tr, G, c); break;
-        default: status |= triplet_fp64_sp<0>(v, t, tr, G, c); break;
+        case 3: triplet_fp64_sp<3>(v, t, tr, G, s_c[threadIdx.x], &nt, &status); break;
+        case 2: triplet_fp64_sp<2>(v, t, tr, G, s_c[threadIdx.x], &nt, &status); break;
+        case 1: triplet_fp64_sp<1>(v, t, tr, G, s_c[threadIdx.x], &nt, &status); break;
+        default: triplet_fp64_sp<0>(v, t, tr, G, s_c[threadIdx.x], &nt, &status); break;
       }
     }
-#pragma unroll
-    for (int i = 0; i < NB; i++) s_c[threadIdx.x][i] = c[i];
+    s_n[threadIdx.x] = (unsigned char)nt;
     __syncthreads();
     if (own) {
       int64_t k0 = max(a, c0), k1 = min(b, c0 + T3);
-      for (int64_t k = k0; k < k1; k++)
+      for (int64_t k = k0; k < k1; k++) {
+        const int n = s_n[k - c0];
+        for (int j = 0; j < n; j++)
 #pragma unroll
-        for (int i = 0; i < NB; i++) acc[i] = acc[i] + s_c[k - c0][i];
+          for (int i = 0; i < NB; i++) acc[i] = acc[i] + s_c[k - c0][j][i];
+      }
     }
     __syncthreads();
   }

@@ -386,6 +386,9 @@ struct fb_ctx {
   DView view() const {
     DView v{};
     v.sp = sp;
+    for (auto &st : structs)
+      for (double q : st.dQ)
+        if (q != 0.0) v.any_dq = 1;
     v.cell2c = d_cell2c.as<DCell2c>();
     v.ncell2c = ncell2c;
     v.ngrid2c = ngrid2c;
@@ -1774,8 +1777,21 @@ int fb_gamma_eigenvalues(fb_ctx *ctx, int is, int which, double *w, int64_t nw, 
 // ------------------------------------------------------- learned 3c surrogate
 namespace {
 
+// work arrays of one forward + backward pass over a batch of B samples:
+// Z1, A1, Z2, A2, dA, dZ2, dZ1 [B][64], Y, dY [B][64], rowloss [B], part
+// [64][64][64] (split-K partial products), ones [1] = 1.0f
+struct MlpWork {
+  float *Z1, *A1, *Z2, *A2, *Y, *dY, *dA, *dZ2, *dZ1, *rowloss, *part;
+  const float *ones;
+};
+
 struct MlpBufs {
   DevBuf X, T, osc, inv, Z1, A1, Z2, A2, Y, dY, dA, dZ2, dZ1, rowloss, P, G, M, V, part, ones;
+  MlpWork work() {
+    return MlpWork{Z1.as<float>(), A1.as<float>(), Z2.as<float>(), A2.as<float>(), Y.as<float>(), dY.as<float>(),
+                   dA.as<float>(), dZ2.as<float>(), dZ1.as<float>(), rowloss.as<float>(), part.as<float>(),
+                   ones.as<float>()};
+  }
   ~MlpBufs() {
     DevBuf *b[] = {&X, &T, &osc, &inv, &Z1, &A1, &Z2, &A2, &Y, &dY, &dA, &dZ2, &dZ1, &rowloss, &P, &G, &M, &V, &part,
                    &ones};
@@ -1808,14 +1824,15 @@ int mlp_alloc_batch(fb_ctx *ctx, MlpBufs &b, int64_t B) {
   return FB_OK;
 }
 
-// forward + backward of one batch (X [B][3], T [B][64]); gradients into b.G
-int mlp_fwd_bwd(fb_ctx *ctx, MlpBufs &b, int nme, int64_t B, const float *X, const float *T) {
+// forward + backward of one batch (X [B][3], T [B][64]) with parameters P;
+// gradients into G, per-row losses into w.rowloss.  Every array of w is
+// written before it is read, so the work area needs no clearing.
+int mlp_fwd_bwd(fb_ctx *ctx, const MlpWork &w, const float *P, float *G, int nme, int64_t B, const float *X,
+                const float *T) {
   hipStream_t st = ctx->stream;
-  float *P = b.P.as<float>(), *G = b.G.as<float>();
-  float *Z1 = b.Z1.as<float>(), *A1 = b.A1.as<float>(), *Z2 = b.Z2.as<float>(), *A2 = b.A2.as<float>();
-  float *Y = b.Y.as<float>(), *dY = b.dY.as<float>(), *dA = b.dA.as<float>();
-  float *dZ2 = b.dZ2.as<float>(), *dZ1 = b.dZ1.as<float>(), *part = b.part.as<float>();
-  const float *ones = b.ones.as<float>();
+  float *Z1 = w.Z1, *A1 = w.A1, *Z2 = w.Z2, *A2 = w.A2, *Y = w.Y, *dY = w.dY, *dA = w.dA;
+  float *dZ2 = w.dZ2, *dZ1 = w.dZ1, *part = w.part;
+  const float *ones = w.ones;
   const int Bi = (int)B;
   const int nz = (int)std::min<int64_t>(64, std::max<int64_t>(1, B / 256));
   int nvalid = 0;
@@ -1829,7 +1846,7 @@ int mlp_fwd_bwd(fb_ctx *ctx, MlpBufs &b, int nme, int64_t B, const float *X, con
   MLP_RC(fbk_mlp_gemm(Bi, MLP_H, MLP_H, A1, MLP_H, 1, P + MLP_P_W2, 1, MLP_H, Z2, MLP_H, 1, 1, nullptr, st));
   MLP_RC(fbk_mlp_bias_silu(B, Z2, A2, P + MLP_P_B2, st));
   MLP_RC(fbk_mlp_gemm(Bi, MLP_O, MLP_H, A2, MLP_H, 1, P + MLP_P_W3, 1, MLP_H, Y, MLP_O, 1, 1, nullptr, st));
-  MLP_RC(fbk_mlp_out(B, nme, Y, P + MLP_P_B3, T, dY, b.rowloss.as<float>(), 1.0f / ((float)B * (float)nvalid), st));
+  MLP_RC(fbk_mlp_out(B, nme, Y, P + MLP_P_B3, T, dY, w.rowloss, 1.0f / ((float)B * (float)nvalid), st));
   // backward
   MLP_RC(fbk_mlp_gemm(MLP_O, MLP_H, Bi, dY, 1, MLP_O, A2, MLP_H, 1, G + MLP_P_W3, MLP_H, 1, nz, part, st));
   MLP_RC(fbk_mlp_gemm(1, MLP_O, Bi, ones, 0, 0, dY, MLP_O, 1, G + MLP_P_B3, MLP_O, 1, nz, part, st));
@@ -1987,7 +2004,8 @@ int fb_mlp_train(fb_ctx *ctx, int64_t nsamples, int nsteps, int batch, double lr
     double loss0 = -1.0, loss = -1.0;
     for (int step = 0; step < nsteps; step++) {
       const int64_t off = (step % nb) * batch;
-      RC(mlp_fwd_bwd(ctx, b, g.nME, batch, b.X.as<float>() + off * MLP_IN, b.T.as<float>() + off * MLP_O));
+      RC(mlp_fwd_bwd(ctx, b.work(), b.P.as<float>(), b.G.as<float>(), g.nME, batch, b.X.as<float>() + off * MLP_IN,
+                     b.T.as<float>() + off * MLP_O));
       if (step == 0) loss0 = mlp_batch_loss(ctx, b, g.nME, batch);
       const double lr_t = lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(M_PI * step / nsteps)));
       const double c1 = 1.0 / (1.0 - std::pow(b1, step + 1)), c2 = 1.0 / (1.0 - std::pow(b2, step + 1));
@@ -1995,7 +2013,7 @@ int fb_mlp_train(fb_ctx *ctx, int64_t nsamples, int nsteps, int batch, double lr
                           (float)b1, (float)b2, (float)c1, (float)c2, 1e-8f, ctx->stream));
     }
     // loss of the trained network on the first batch
-    RC(mlp_fwd_bwd(ctx, b, g.nME, batch, b.X.as<float>(), b.T.as<float>()));
+    RC(mlp_fwd_bwd(ctx, b.work(), b.P.as<float>(), b.G.as<float>(), g.nME, batch, b.X.as<float>(), b.T.as<float>()));
     loss = mlp_batch_loss(ctx, b, g.nME, batch);
     FB_HIP(hipMemcpy(p.data(), b.P.p, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost));
     ctx->h_mlp_params[gid] = p;
@@ -2130,7 +2148,7 @@ int fb_mlp_gradients(fb_ctx *ctx, int nme, const float *params, int n, const flo
   FB_HIP(hipMemcpyAsync(b.P.p, params, sizeof(float) * MLP_NP, hipMemcpyHostToDevice, ctx->stream));
   FB_HIP(hipMemcpyAsync(b.X.p, X, sizeof(float) * (size_t)n * MLP_IN, hipMemcpyHostToDevice, ctx->stream));
   FB_HIP(hipMemcpyAsync(b.T.p, T, sizeof(float) * (size_t)n * MLP_O, hipMemcpyHostToDevice, ctx->stream));
-  RC(mlp_fwd_bwd(ctx, b, nme, n, b.X.as<float>(), b.T.as<float>()));
+  RC(mlp_fwd_bwd(ctx, b.work(), b.P.as<float>(), b.G.as<float>(), nme, n, b.X.as<float>(), b.T.as<float>()));
   const double l = mlp_batch_loss(ctx, b, nme, n);
   if (loss) *loss = l;
   FB_HIP(hipMemcpy(grads, b.G.p, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost));
@@ -2863,6 +2881,11 @@ int fb_get_tuning(fb_ctx *ctx, int param, int *value) {
   return ctx->fail(FB_E_ARG, "unknown tuning parameter");
 }
 
+// Times the launch shapes of the kernels that the context's precision and
+// species use, on the current lists, and keeps the fastest.  The timed
+// launches leave the blocks half-built (a 2c candidate rewrites every Vna
+// block, 3c targets included), so the whole H+S build is run again with the
+// chosen shapes before returning: the blocks then hold a complete build.
 int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   if (!ctx) return FB_E_ARG;
   if (!ctx->lists_ok || !ctx->done[FB_MAT_S] || !ctx->done[FB_MAT_DIP] || !ctx->done[FB_MAT_VNA])
@@ -2870,61 +2893,72 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   if (nrep < 1) nrep = 5;
   FB_HIP(hipSetDevice(ctx->device));
   RC(ctx->settle("assemble_all"));
-  const int cand[3] = {64, 128, 256};
-  float best = 1e30f;
-  int pick = ctx->chunk3c;
-  hipEvent_t a, b;
-  FB_HIP(hipEventCreate(&a));
-  FB_HIP(hipEventCreate(&b));
+  if (ctx->structs_dirty) RC(upload_structures(ctx));
+  struct Events {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } ev;
+  FB_HIP(hipEventCreate(&ev.a));
+  FB_HIP(hipEventCreate(&ev.b));
   DView v = ctx->view();
   const int charged = any_charge(ctx);
-  const bool mlp = ctx->precision == FB_PREC_MIXED && ctx->method3c == FB_3C_MLP && ctx->mlp_ready && !charged;
-  auto run3 = [&](int c) { return mlp ? fbk_3c_mixed_mlp(v, c, ctx->stream) : fbk_3c_mixed(v, c, charged, ctx->stream); };
-  auto run2 = [&](int var) {
-    if (!charged) return launch_2c(ctx, v, 15, var, ctx->stream);
-    int rc = launch_2c(ctx, v, 7, var, ctx->stream);
-    return rc ? rc : launch_2c(ctx, v, 8, var, ctx->stream);
+  // the chunk only shapes the mixed s/p 3c kernel; the binary64 and generic
+  // 3c kernels have no launch parameter to tune
+  const bool mixed3c = ctx->canon_sp && ctx->precision == FB_PREC_MIXED;
+  const bool mlp = mixed3c && ctx->method3c == FB_3C_MLP && ctx->mlp_ready && !charged;
+  // one candidate: a warm-up launch, then nrep launches between two events
+  auto timed = [&](auto &&fn, float *ms) -> hipError_t {
+    hipError_t e = (hipError_t)fn();
+    if (e == hipSuccess) e = hipEventRecord(ev.a, ctx->stream);
+    for (int r = 0; r < nrep && e == hipSuccess; r++) e = (hipError_t)fn();
+    if (e == hipSuccess) e = hipEventRecord(ev.b, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev.b);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, ev.a, ev.b);
+    return e;
   };
-  auto timed = [&](auto &&fn, float *ms) -> int {
-    int rc = fn();  // warm-up
-    if (rc) return rc;
-    hipError_t e = hipEventRecord(a, ctx->stream);
-    for (int r = 0; r < nrep && e == hipSuccess && rc == 0; r++) rc = fn();
-    if (rc) return rc;
-    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(b);
-    if (e == hipSuccess) e = hipEventElapsedTime(ms, a, b);
-    return (int)e;
-  };
-  for (int c : cand) {
-    float ms = 0.f;
-    int rc = timed([&] { return run3(c); }, &ms);
-    if (rc) return ctx->fail(FB_E_HIP, "autotune 3c", (hipError_t)rc);
-    if (ms < best) {
-      best = ms;
-      pick = c;
+  int pick = ctx->chunk3c;
+  if (mixed3c) {
+    const int cand[3] = {64, 128, 256};
+    float best = 1e30f;
+    for (int c : cand) {
+      float ms = 0.f;
+      const hipError_t e = timed([&] { return mlp ? fbk_3c_mixed_mlp(v, c, ctx->stream)
+                                                  : fbk_3c_mixed(v, c, charged, ctx->stream); }, &ms);
+      if (e != hipSuccess) return ctx->fail(FB_E_HIP, "autotune 3c", e);
+      if (ms < best) {
+        best = ms;
+        pick = c;
+      }
     }
   }
+  int pick2 = ctx->k2c;
   if (ctx->canon_sp) {
     float best2 = 1e30f;
-    int pick2 = ctx->k2c;
     for (int var = 0; var < 2; var++) {
       float ms = 0.f;
-      int rc = timed([&] { return run2(var); }, &ms);
-      if (rc) return ctx->fail(FB_E_HIP, "autotune 2c", (hipError_t)rc);
+      const hipError_t e = timed([&] {
+        if (!charged) return launch_2c(ctx, v, 15, var, ctx->stream);
+        const int rc = launch_2c(ctx, v, 7, var, ctx->stream);
+        return rc ? rc : launch_2c(ctx, v, 8, var, ctx->stream);
+      }, &ms);
+      if (e != hipSuccess) return ctx->fail(FB_E_HIP, "autotune 2c", e);
       if (ms < best2) {
         best2 = ms;
         pick2 = var;
       }
     }
-    ctx->k2c = pick2;
   }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
   ctx->chunk3c = pick;
+  ctx->k2c = pick2;
   ctx->mlp_gen++;   // captured graphs hold the previous launch shapes
+  RC(enqueue_build(ctx, charged));
+  ctx->pending = true;
+  RC(ctx->settle("autotune"));
+  for (int i = 0; i < 5; i++) ctx->done[i] = 1;
   if (chosen) *chosen = pick;
-  RC(ctx->check_status("autotune"));
   return FB_OK;
 }
 

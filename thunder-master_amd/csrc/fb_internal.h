@@ -114,6 +114,7 @@ struct DView {
   const int *imass;             // [natoms]
   const int *atom_struct;       // [natoms]
   const double *dQ;             // [natoms][FB_MAXSH]
+  int any_dq;                   // some shell charge of the batch is non-zero
   const double *qin;            // [natoms][FB_MAXSH] input shell charges Qin (McWEDA densities)
   // neighbour lists (CSR over global atoms, per-structure 1-based atom ids)
   int64_t npairs;

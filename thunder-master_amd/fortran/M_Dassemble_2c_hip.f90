@@ -44,7 +44,9 @@
         end subroutine Dassemble_dipole_z
 
 ! k.DASSEMBLERS/M_Dassemble_2c_DOGS.f90:679-1199: ontop and atom cases of the
-! neutral-atom force, contracted with s%denmat on the GPU
+! neutral-atom force, contracted with s%denmat on the GPU.  The shell charges
+! are read at call time, as the reference reads s%atom(:)%shell(:)%dQ here
+! (a charge mixer may have changed them since assemble_vna_2c).
         subroutine Dassemble_vna_2c (s)
         type(T_structure), target :: s
         integer iatom, n
@@ -53,6 +55,7 @@
         real(c_double), allocatable, target :: fa(:, :), fo(:, :)
         integer p
         call fb_session_sync (s)
+        call fb_session_charges (s)
         call fb_session_denmat (s)
         call fb_check (fb_dassemble_vna_2c(fbh), fbh, 'Dassemble_vna_2c')
         call fb_check (fb_get_list_sizes(fbh, 0, npairs, ntrip, mb), fbh, 'fb_get_list_sizes')

@@ -3,9 +3,10 @@
 !
 ! Dassemble_vna_3c adds the library's per-atom f3naa (on ialpha), f3nab (on
 ! iatom) and f3nac (on jatom) to s%forces(:), as the reference accumulates them
-! (M_Dassemble_3c_DOGS.f90:136-793), from the density matrix in s%denmat.  It
-! needs assemble_S, assemble_dipole_z, Dassemble_S and Dassemble_dipole_z of
-! this session first (the charged terms use their blocks and Dblocks).
+! (M_Dassemble_3c_DOGS.f90:136-793), from the density matrix in s%denmat and
+! the shell charges s%atom(:)%shell(:)%dQ as they are at call time.  It needs
+! assemble_S, assemble_dipole_z, Dassemble_S and Dassemble_dipole_z of this
+! session first (the charged terms use their blocks and Dblocks).
         module M_Dassemble_3c
         use iso_c_binding
         use fb_hip_c
@@ -22,6 +23,7 @@
         real(c_double), allocatable, target :: fa(:, :), fb(:, :), fc(:, :)
         integer iatom
         call fb_session_sync (s)
+        call fb_session_charges (s)
         call fb_session_denmat (s)
         call fb_check (fb_dassemble_vna_3c(fbh), fbh, 'Dassemble_vna_3c')
         allocate (fa(3, s%natoms), fb(3, s%natoms), fc(3, s%natoms))

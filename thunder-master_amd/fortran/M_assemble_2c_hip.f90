@@ -58,6 +58,7 @@
         allocate (s%vna(s%natoms))
         call fb_session_fetch (FB_MAT_VNA)
         call fb_session_store (s, s%vna, 0, .true.)
+        call fb_session_keep_vna ()
         call fb_session_fetch (FB_MAT_VNAO)
         call fb_session_store (s, s%vna, 1, .true.)
         end subroutine assemble_vna_2c

@@ -34,6 +34,8 @@
         logical, save :: have_pp_lists = .false.
         integer(c_int64_t), save :: npairs_all = 0, ntrip_all = 0
         real(c_double), allocatable, save :: fb_buf(:)
+        ! the Vna blocks (%block) as last handed to the caller
+        real(c_double), allocatable, save :: vna_last(:)
 
         ! 2c interactions of the assemblers (include/interactions_2c.h:5-25):
         ! H and S, then the McWEDA densities rho_* and rhoS_*
@@ -468,6 +470,36 @@
           end do
         end do
         end subroutine fb_session_store
+
+! Remember the Vna blocks just stored (the session buffer) as the caller's.
+        subroutine fb_session_keep_vna ()
+        if (allocated(vna_last)) deallocate (vna_last)
+        allocate (vna_last(size(fb_buf, kind=8)))
+        vna_last = fb_buf
+        end subroutine fb_session_keep_vna
+
+! Add the change of the library's Vna blocks (session buffer) since they were
+! last handed over into the caller's s%vna(:)%neighbors(:)%block.
+        subroutine fb_session_add_vna (s)
+        type(T_structure), target :: s
+        integer iatom, ineigh, jatom, nmu, nnu, nb
+        integer(c_int64_t) :: off
+        type(T_assemble_block), pointer :: pb
+        if (.not. allocated(vna_last)) error stop ' fireball_hip: assemble_vna_3c before assemble_vna_2c '
+        off = 0
+        do iatom = 1, s%natoms
+          nmu = species(s%atom(iatom)%imass)%norb_max
+          do ineigh = 1, s%neighbors(iatom)%neighn
+            jatom = s%neighbors(iatom)%neigh_j(ineigh)
+            nnu = species(s%atom(jatom)%imass)%norb_max
+            nb = nmu*nnu
+            pb => s%vna(iatom)%neighbors(ineigh)
+            pb%block = pb%block + reshape(fb_buf(off+1:off+nb) - vna_last(off+1:off+nb), [nmu, nnu])
+            off = off + nb
+          end do
+        end do
+        vna_last = fb_buf
+        end subroutine fb_session_add_vna
 
 ! Hand the caller's density matrix (s%denmat, the blocks the force assemblers
 ! contract with) to the library, in the pair order of s%neighbors.
