@@ -278,6 +278,26 @@ int fb_mlp_gradients(fb_ctx *ctx, int nme, const float *params, int n, const flo
 int fb_mlp_forward(fb_ctx *ctx, int nme, const float *params, const float *oscale, int n, const float *X,
                    float *Y);
 
+/* --- allocation-free training step -----------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, training loops that forbid
+ * hipMalloc.  The forward and backward pass of fb_mlp_gradients on one batch
+ * with every argument in device memory: params [8576] (fb_mlp_gradients
+ * layout), X [n][3] and T [n][64] in; grads [8576] and loss (one float: the
+ * mean squared error over the valid output slots) out.  The work arrays
+ * (activations, their gradients, per-row losses, split-K partial products)
+ * live in a fixed scratch array of the library, about 5.8 MB per GPU, held
+ * until the process ends; the call allocates nothing, copies nothing to or
+ * from the host and does not synchronize (work is enqueued on the context's
+ * stream).  n is at most FB_MLP_NOALLOC_MAX_BATCH (FB_E_CAPACITY beyond).
+ * Every call on one GPU shares that GPU's scratch array, from any context:
+ * calls that overlap in time on one GPU race, so order them on one stream or
+ * synchronize between them.  The gradients equal those of fb_mlp_gradients
+ * on the same batch. */
+#define FB_MLP_NOALLOC_MAX_BATCH 2048
+int fb_mlp_gradients_noalloc(fb_ctx *ctx, int nme, const float *params, int n, const float *X, const float *T,
+                             float *grads, float *loss);
+
 /* --- tuning ------------------------------------------------------------
  * No counterpart in the reference (its assembler has no launch parameters).
  * FB_TUNE_CHUNK3C: triplets per workgroup chunk of the mixed 3c kernel (64,

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -76,6 +77,7 @@ int fbk_mlp_silu_back(int64_t n, const float *dA, const float *Z, float *dZ, hip
 int fbk_mlp_adam(int np, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float c1,
                  float c2, float eps, hipStream_t st);
 int fbk_mlp_infer(const DMlp *w, int64_t n, const float *X, float *Y, hipStream_t st);
+int fbk_mlp_loss(int64_t n, const float *rowloss, float scale, float *loss, hipStream_t st);
 int fbk_mlp_calib(const DView &v, int gid, const DMlp *w, int64_t n, unsigned long long seed, unsigned *err,
                   hipStream_t st);
 }
@@ -203,6 +205,7 @@ bool same_list(const MeList &a, const std::vector<int> &mu, const std::vector<in
 
 struct fb_ctx {
   int device = 0;
+  float *scratch = nullptr;   // this device's scratch array (fb_mlp_gradients_noalloc)
   hipStream_t own_stream = nullptr, stream = nullptr;
   hipStream_t side = nullptr;          // second stream: 2c beside 3c in a build
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -602,6 +605,42 @@ static int make_cells(const double *lat, int icluster, double *xl) {
 extern "C" {
 
 // why the last fb_create of this thread failed (fb_last_error(NULL) returns it)
+// ---------------------------------------------- scratch of the allocation-free entry point
+// fb_mlp_gradients_noalloc keeps the work arrays of one training batch (the
+// MlpWork arrays for FB_MLP_NOALLOC_MAX_BATCH samples, laid out at fixed
+// offsets) in this file-scope device array instead of allocating them.  Each
+// GPU holds its own copy until the process ends.  The host never passes the
+// array's name to a kernel: fb_create looks up its device address once per
+// device (with that device current) and every use goes through that address.
+namespace {
+constexpr int64_t NA_B = FB_MLP_NOALLOC_MAX_BATCH;
+constexpr int64_t NA_HID = NA_B * MLP_H;            // floats of one [B][64] array
+constexpr int64_t NA_OFF_Y = 7 * NA_HID;            // Z1, A1, Z2, A2, dA, dZ2, dZ1 first
+constexpr int64_t NA_OFF_DY = NA_OFF_Y + NA_B * MLP_O;
+constexpr int64_t NA_OFF_ROW = NA_OFF_DY + NA_B * MLP_O;
+constexpr int64_t NA_OFF_PART = NA_OFF_ROW + NA_B;
+constexpr int64_t NA_OFF_ONES = NA_OFF_PART + 64LL * MLP_H * MLP_O;
+constexpr int64_t NA_FLOATS = NA_OFF_ONES + 64;     // ones[1], padded to 256 bytes
+constexpr int NA_MAXDEV = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_dev[NA_MAXDEV] = {};
+}  // namespace
+static __device__ __align__(256) float scratch[NA_FLOATS];
+
+// device address of this GPU's scratch array; the device must be current
+static hipError_t scratch_address(int device, float **out) {
+  if (device < 0 || device >= NA_MAXDEV) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = static_cast<float *>(ptr);
+  }
+  *out = g_scratch_dev[device];
+  return hipSuccess;
+}
+
 static thread_local std::string g_create_err = "null context";
 
 static int create_failed(fb_ctx *ctx, const char *what, hipError_t e) {
@@ -622,6 +661,8 @@ int fb_create(fb_ctx **out, int device) {
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) return create_failed(ctx, "streams", e);
+  e = scratch_address(device, &ctx->scratch);
+  if (e != hipSuccess) return create_failed(ctx, "scratch address", e);
   ctx->stream = ctx->own_stream;
   e = ctx->d_status.ensure(sizeof(int) * 4);
   if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_status, sizeof(int) * 4);
@@ -2152,6 +2193,35 @@ int fb_mlp_gradients(fb_ctx *ctx, int nme, const float *params, int n, const flo
   const double l = mlp_batch_loss(ctx, b, nme, n);
   if (loss) *loss = l;
   FB_HIP(hipMemcpy(grads, b.G.p, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost));
+  return FB_OK;
+}
+
+// The forward and backward pass of fb_mlp_gradients on device arrays, with
+// the work arrays in the scratch array of this file: no allocation, no host
+// copy, no synchronization, so it can be captured in a HIP graph.
+int fb_mlp_gradients_noalloc(fb_ctx *ctx, int nme, const float *params, int n, const float *X, const float *T,
+                             float *grads, float *loss) {
+  if (!ctx) return FB_E_ARG;
+  if (!params || !X || !T || !grads || !loss || n < 1 || nme < 1 || nme > 12)
+    return ctx->fail(FB_E_ARG, "mlp_gradients_noalloc arguments");
+  if (n > FB_MLP_NOALLOC_MAX_BATCH)
+    return ctx->fail(FB_E_CAPACITY, "mlp_gradients_noalloc: batch larger than FB_MLP_NOALLOC_MAX_BATCH (2048)");
+  FB_HIP(hipSetDevice(ctx->device));
+  float *const sc = ctx->scratch;
+  const MlpWork w{sc, sc + NA_HID, sc + 2 * NA_HID, sc + 3 * NA_HID, sc + NA_OFF_Y, sc + NA_OFF_DY,
+                  sc + 4 * NA_HID, sc + 5 * NA_HID, sc + 6 * NA_HID, sc + NA_OFF_ROW, sc + NA_OFF_PART,
+                  sc + NA_OFF_ONES};
+  // 1.0f, written by a kernel (no host copy, nothing a graph replays wrongly)
+  int rc = fbk_fill_i32(reinterpret_cast<int *>(sc + NA_OFF_ONES), 1, 0x3f800000, ctx->stream);
+  if (rc) return ctx->fail(FB_E_HIP, "mlp_gradients_noalloc fill", (hipError_t)rc);
+  RC(mlp_fwd_bwd(ctx, w, params, grads, nme, n, X, T));
+  int nvalid = 0;
+  for (int o = 0; o < MLP_O; o++) {
+    int th, m;
+    nvalid += mlp_slot(o, nme, &th, &m) ? 1 : 0;
+  }
+  rc = fbk_mlp_loss(n, w.rowloss, 1.0f / ((float)n * (float)nvalid), loss, ctx->stream);
+  if (rc) return ctx->fail(FB_E_HIP, "mlp_gradients_noalloc loss", (hipError_t)rc);
   return FB_OK;
 }
 

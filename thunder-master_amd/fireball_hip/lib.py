@@ -83,6 +83,7 @@ def lib():
             "fb_mlp_stats": [vp, i64p, i64p],
             "fb_mlp_gradients": [vp, C.c_int, fp, C.c_int, fp, fp, fp, dp],
             "fb_mlp_forward": [vp, C.c_int, fp, fp, C.c_int, fp, fp],
+            "fb_mlp_gradients_noalloc": [vp, C.c_int, vp, C.c_int, vp, vp, vp, vp],
             "fb_set_denmat": [vp, C.c_int, dp],
             "fb_dassemble_S": [vp], "fb_dassemble_T": [vp], "fb_dassemble_dipole_z": [vp],
             "fb_dassemble_vna_2c": [vp], "fb_dassemble_vna_3c": [vp], "fb_dassemble_all": [vp],
@@ -459,6 +460,14 @@ class Assembler:
         self._check(self.L.fb_mlp_gradients(self.h, int(nme), _fp(p), X.shape[0], _fp(X), _fp(T), _fp(g),
                                             C.byref(loss)))
         return loss.value, g
+
+    def mlp_gradients_noalloc(self, nme, params_ptr, n, X_ptr, T_ptr, grads_ptr, loss_ptr):
+        """fb_mlp_gradients_noalloc on device arrays (integer device addresses:
+        params [8576], X [n][3], T [n][64], grads [8576], loss [1], float32).
+        Enqueued on the context's stream; call synchronize() before reading."""
+        self._check(self.L.fb_mlp_gradients_noalloc(self.h, int(nme), C.c_void_p(params_ptr), int(n),
+                                                    C.c_void_p(X_ptr), C.c_void_p(T_ptr),
+                                                    C.c_void_p(grads_ptr), C.c_void_p(loss_ptr)))
 
     def mlp_forward(self, nme, params, X, oscale=None):
         p = np.ascontiguousarray(params, np.float32)
