@@ -238,23 +238,26 @@ int fb_gamma_eigenvalues(fb_ctx *ctx, int istruct, int which, double *w, int64_t
 
 /* --- learned 3c surrogate ------------------------------------------------
  * No counterpart in the reference.  One network per neutral 3c group (species
- * triple): inputs (x, y, cos theta), two hidden layers of 64 SiLU units,
- * outputs the five Legendre theta cells of every matrix element that
- * getMEs_Fdata_3c interpolates (e.FDATA/M_Fdata_3c.f90:391-489).
+ * triple): inputs (x, y, cos theta), two hidden layers of 64 units with the
+ * activation z s(z) (s a quadratic sigmoid, the shape of SiLU without a
+ * transcendental), outputs the five Legendre theta cells of every matrix
+ * element that getMEs_Fdata_3c interpolates (e.FDATA/M_Fdata_3c.f90:391-489).
  * fb_mlp_train samples nsamples points of every neutral group from the
  * binary64 table path, trains in FP32 on the GPU (Adam, batch, nsteps, cosine
- * learning-rate decay from lr), stores BF16 weights, and calibrates, for
- * every table cell, the largest Frobenius norm of the block error
+ * learning-rate decay from lr), stores BF16 MFMA operand fragments (inputs,
+ * first-layer and last-layer weights split into BF16 high and low parts),
+ * and calibrates, for every sub-cell of a grid four times finer than the
+ * table's in each direction, the largest Frobenius norm of the block error
  * |M_network - M_table| (theta cells folded at the sampled cos theta) over
  * fresh samples; orthogonal rotations keep that norm.  With
  * fb_set_3c_method(FB_3C_MLP) and mixed precision, the 3c launch evaluates
  * the network on MFMA for each triplet whose bound
- *   2 * P_eq2 * (calibrated Frobenius error of the cell's molecular-frame block)
- * is at most the tolerance (fb_mlp_set_tolerance, eV, default 5e-5: with it the
- * measured H-block error stays below 1e-4 eV on the test workloads) and the
- * table stencil for every other triplet.  Charged builds use the tables.
+ *   2 * P_eq2 * (calibrated Frobenius error of the sub-cell's molecular-frame block)
+ * is at most the tolerance (fb_mlp_set_tolerance, eV, default 2.5e-5: with it
+ * the measured H-block error stays below 1e-4 eV on the test workloads) and
+ * the table stencil for every other triplet.  Charged builds use the tables.
  * fb_mlp_info: 12 values per group: is, js, ks, isub, trained, nsamples,
- *   nsteps, loss before, loss after (normalised MSE), max cell error (eV,
+ *   nsteps, loss before, loss after (normalised MSE), max sub-cell error (eV,
  *   Frobenius),
  *   training ms, calibration samples.
  * fb_mlp_stats: triplets that took the network and triplets seen by MLP 3c

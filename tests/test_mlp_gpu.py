@@ -6,8 +6,14 @@
 * the BF16 MFMA inference equals the oracle's emulation of its arithmetic
   (this also pins the MFMA operand layout);
 * after training, MLP assembly stays within 1e-4 eV of the binary64 table
-  path, serves triplets the untrained network cannot, and with tolerance 0
-  reproduces the table kernel exactly."""
+  path and with tolerance 0 reproduces the table kernel exactly;
+* against the table method on an amorphous-carbon box (the benchmark
+  workload's density, cutoffs and tables): with the default tolerance every
+  Vna element stays within 1e-4 eV of binary64; the fraction of triplets on
+  the network and the 3c kernel times of both methods are measured and
+  printed (the network phase does not beat the table stencil here: 97 % of
+  these triplets fall in cells where the reference interpolates with its
+  3-point rule)."""
 import numpy as np
 import pytest
 
@@ -87,7 +93,7 @@ def test_mfma_inference_matches_bf16_emulation():
     assert np.max(d) < 2e-2 * np.max(np.abs(ref)), np.max(d)
 
 
-def test_trained_surrogate_within_tolerance_and_gains_over_untrained(golden):
+def test_trained_surrogate_within_tolerance_and_exact_fallback(golden):
     g = golden("si64")
     ts, st = golden_cases.inputs(g)
 
@@ -112,19 +118,65 @@ def test_trained_surrogate_within_tolerance_and_gains_over_untrained(golden):
     tab.assemble_all()
     vna_tab = tab.blocks("VNA")
 
-    asm, vna, frac = run(4000, 5e-5)
+    asm, vna, frac = run(20000, 2.5e-5)
     info = asm.mlp_info(0)
     err = float(np.max(np.abs(vna - vna64)))
     print(f"trained: loss {info['loss_before']:.3e} -> {info['loss_after']:.3e}, "
           f"max cell err {info['max_cell_err_eV']:.3e} eV, network fraction {frac:.3f}, "
           f"max |VNA - fp64| {err:.3e} eV", flush=True)
-    _, vna_u, frac_u = run(1, 5e-5)
-    err_u = float(np.max(np.abs(vna_u - vna64)))
-    print(f"untrained: network fraction {frac_u:.3f}, max |VNA - fp64| {err_u:.3e} eV", flush=True)
-    assert info["trained"] == 1 and info["loss_after"] < 0.1 * info["loss_before"]
-    assert err <= 1e-4 and err_u <= 1e-4
-    assert frac > frac_u
+    assert info["trained"] == 1 and info["loss_after"] < 0.01 * info["loss_before"]
+    assert err <= 1e-4
+    assert frac > 0.2
     # tolerance 0: every triplet falls back, identical to the table kernel
     _, vna0, frac0 = run(1, 0.0)
     assert frac0 == 0.0
     assert np.array_equal(vna0, vna_tab)
+
+
+def test_network_against_table_path():
+    """The learned surrogate against the method without it (the table
+    stencil) on a 2048-atom amorphous-carbon box: every Vna element within
+    1e-4 eV of binary64 at the default per-triplet tolerance, a fraction of
+    the triplets served by the network, and the 3c kernel times of both
+    methods on the same lists (printed: the comparison the benchmark reports
+    on the 8192-atom box)."""
+    from fireball_hip import synth
+    sp = [synth.C_SP]
+    ts = synth.make_tables(sp)
+    st = synth.random_box({6: 2048}, 2.0, 1.2, 5, sp)
+    ref = _asm(ts, precision="fp64")
+    ref.add_structure(st)
+    ref.build_lists()
+    ref.assemble_all()
+    vna64 = ref.blocks("VNA")
+    asm = _asm(ts, precision="mixed", method="table")
+    asm.add_structure(st)
+    npairs, ntrip = asm.build_lists()
+    print(f"aC2048: {npairs} pairs, {ntrip} triplets", flush=True)
+
+    def kernel_ms(name, n=10):
+        asm.assemble_all()
+        asm.enable_timing(True)
+        asm.timings(4096)
+        for _ in range(n):
+            asm.assemble_all()
+        t = [ms for k, ms in asm.timings(4096) if k == name]
+        asm.enable_timing(False)
+        return float(np.median(t))
+
+    t_table = kernel_ms("assemble_3c_mixed")
+    vna_tab = asm.blocks("VNA")
+    asm.set_3c_method("mlp")
+    asm.train_mlp(nsamples=1 << 21, nsteps=40000, batch=8192, lr=3e-3, seed=7)
+    asm.mlp_stats()
+    asm.assemble_all()
+    nn, seen = asm.mlp_stats()
+    frac = nn / max(seen, 1)
+    vna = asm.blocks("VNA")
+    err = float(np.max(np.abs(vna - vna64)))
+    t_mlp = kernel_ms("assemble_3c_mlp")
+    print(f"network fraction {frac:.3f}, max |VNA - fp64| {err:.3e} eV (table path {np.max(np.abs(vna_tab - vna64)):.3e}), "
+          f"3c kernel {t_mlp:.3f} ms vs table {t_table:.3f} ms ({t_table / t_mlp:.2f}x)", flush=True)
+    assert seen == ntrip
+    assert 0.0 < frac <= 1.0
+    assert err <= 1e-4

@@ -22,6 +22,14 @@
 //      the FP32 contributions in segment order from LDS, stored to the block.
 // Binary64 adds run at the FP32 rate on gfx950 without packed math, so the
 // accumulation is plain binary64 rather than a compensated FP32 sum.
+//
+// k_3c_mlp is the learned-surrogate form: a persistent launch of 256-thread
+// workgroups (four per CU) that pull one-pass tiles of up to 256 triplets from
+// a work queue (one range per XCD, the others when it runs dry).  Phase B
+// runs the stencil only for triplets whose calibrated network error exceeds
+// the bound; phase M evaluates the network for the rest on MFMA, 16 triplets
+// per wavefront pass, reading each weight fragment from global memory (L1)
+// at its MFMA.
 #include "fb_device.h"
 #include "fb_mlp.h"
 #include "fb_sp.h"
@@ -104,50 +112,52 @@ __device__ __forceinline__ float stencil(__amdgpu_buffer_rsrc_t r, unsigned base
   return acc;
 }
 
-// MLP: the learned surrogate replaces the stencil of phase B for every triplet
-// whose calibrated bound passes (neutral builds only); the others keep it.
-template <int CH, bool CHARGED, bool MLP = false>
-__global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
-  __shared__ TPrm sq[CH];
-  __shared__ float sF[CH][NME3];
-  __shared__ TChg sx[CHARGED ? CH : 1];
-  __shared__ int sSeg[CH + 1];          // triplet offsets (below 2^31: the list build checks)
-  __shared__ int64_t sBo[CH];
-  __shared__ unsigned char sNb[CH];
-  __shared__ float sU[MLP ? CH : 1][4];
-  __shared__ int sG[MLP ? CH : 1];
-  __shared__ unsigned sCnt[2];
+// LDS of one tile pass
+template <int CH, bool CHARGED, bool MLP>
+struct S3c {
+  TPrm sq[CH];
+  float sF[CH][NME3];
+  TChg sx[CHARGED ? CH : 1];
+  int sSeg[CH + 1];                   // triplet offsets (below 2^31: the list build checks)
+  int64_t sBo[CH];
+  unsigned char sNb[CH];
+  float sU[MLP ? CH : 1][4];          // network inputs and envelope
+  int sG[MLP ? CH : 1];               // network group of the triplet, -1: table stencil
+  short sNet[MLP ? CH : 1], sFb[MLP ? CH : 1];   // chunk triplets on the network / on the stencil, in order
+  int sWn[MLP ? CH / 64 : 1][2];      // per-wavefront counts of both lists
+};
+
+// One tile of whole target segments (first, last, r0, r1), in chunks of at
+// most CH triplets.  MLP: the learned surrogate replaces the stencil of phase
+// B for every triplet whose calibrated bound passes (neutral builds only).
+template <int CH, bool CHARGED, bool MLP>
+__device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CHARGED, MLP> &S, unsigned *sCnt,
+                                       int &status) {
   // contributions reuse the parameter storage, rows padded to 17 words
-  float(*sC)[NB + 1] = reinterpret_cast<float(*)[NB + 1]>(&sq[0]);
+  float(*sC)[NB + 1] = reinterpret_cast<float(*)[NB + 1]>(&S.sq[0]);
   static_assert(sizeof(TPrm) >= (NB + 1) * sizeof(float), "contribution row must fit in a parameter row");
   const int tid = threadIdx.x;
-  // the tile's whole segments [first, last], from the list build: at most CH
-  // triplets (one pass) unless a single segment is longer
-  const int kt = CH == 64 ? 0 : (CH == 128 ? 1 : 2);
-  const int4 td = v.tiles[kt][xcd_tile(blockIdx.x, gridDim.x)];
   const int64_t first = td.x, last = td.y, r0 = td.z, r1 = td.w;
   if (first > last) return;
   // Accumulator slot sl = (segment, block element); lane tid owns the slots
   // tid + j*CH, j < NB.  The workgroup owns the target blocks, so a segment
   // that spans chunks keeps its binary64 partial sum in the block itself.
   const int nown = (int)(last - first + 1);
-  for (int k = tid; k <= nown; k += CH) sSeg[k] = (int)v.seg_off[first + k];
+  for (int k = tid; k <= nown; k += CH) S.sSeg[k] = (int)v.seg_off[first + k];
   for (int k = tid; k < nown; k += CH) {
     const int p = v.seg_pair[first + k];
     const int64_t b0 = v.boff[p];
-    sBo[k] = b0;
-    sNb[k] = (unsigned char)(v.boff[p + 1] - b0);
+    S.sBo[k] = b0;
+    S.sNb[k] = (unsigned char)(v.boff[p + 1] - b0);
   }
   double *const vna = v.mat[FB_MAT_VNA];
-  int status = 0;
   const __amdgpu_buffer_rsrc_t tab = slab_rsrc(v.tab3c32, v.ntab3c32);
-  if (MLP && tid < 2) sCnt[tid] = 0u;
 
   for (int64_t c0 = r0; c0 < r1; c0 += CH) {
     const int nt = (int)min((int64_t)CH, r1 - c0);
     // ---------------- phase A: geometry and stencil parameters
     if (tid < nt) {
-      TPrm &q = sq[tid];
+      TPrm &q = S.sq[tid];
       const int64_t t = c0 + tid;
       const int4 tr = v.trip[t];
       const int aux = v.trip_aux[t];
@@ -243,24 +253,24 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       q.base = (unsigned)(g.off32 + ((int64_t)(iy - 2) * g.nx + (ix - 2)) * g.nME * FB_TH32);
       if (MLP) {
         // per-triplet bound: P_eq2 x the calibrated Frobenius error of the
-        // cell's molecular-frame block (bounds every rotated element), with a
-        // factor 2 of margin over the sampled maximum
+        // sub-cell's molecular-frame block (bounds every rotated element),
+        // with a factor 2 of margin over the sampled maximum
         const DMlp *w = v.mlp + gid;
         bool ok = w->ready && x <= g.xmax && z <= g.ymax;
         if (ok) {
-          const float ce = v.mlp_err[w->err_off + (int64_t)(iy - 1) * g.nx + (ix - 1)];
+          const float ce = v.mlp_err[w->err_off + mlp_subcell(x, z, g.xmax, g.ymax, g.nx, g.ny)];
           ok = 2.0f * (float)FB_P_EQ2 * ce <= v.mlp_tau;
         }
-        sU[tid][0] = (float)(2.0 * x / g.xmax - 1.0);
-        sU[tid][1] = (float)(2.0 * z / g.ymax - 1.0);
-        sU[tid][2] = (float)cost;
-        sU[tid][3] = ok ? mlp_envelope(w, (float)x, (float)z) : 0.f;
-        sG[tid] = ok ? gid : -1;
+        S.sU[tid][0] = (float)(2.0 * x / g.xmax - 1.0);
+        S.sU[tid][1] = (float)(2.0 * z / g.ymax - 1.0);
+        S.sU[tid][2] = (float)cost;
+        S.sU[tid][3] = ok ? mlp_envelope(w, (float)x, (float)z) : 0.f;
+        S.sG[tid] = ok ? gid : -1;
         atomicAdd(&sCnt[0], ok ? 1u : 0u);
         atomicAdd(&sCnt[1], 1u);
       }
       if (CHARGED) {
-        TChg &c = sx[tid];
+        TChg &c = S.sx[tid];
         c.dqs = 0.f;
 #pragma unroll
         for (int k = 0; k < FB_MAXSH; k++) { c.dq[k] = 0.f; c.gk[k] = 0; c.m1k[k] = 0; }
@@ -286,14 +296,45 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
         c.d23 = (float)d23;
         c.z = (float)z;
       }
+    } else if (MLP && tid < CH) {
+      S.sG[tid] = -1;   // lanes past the chunk end take no network pass
+    }
+    // the chunk's triplets split into a network list and a stencil list, each
+    // dense and in triplet order, so that each phase costs in proportion to
+    // its share
+    int nnet = 0, nfb = nt;
+    if (MLP) {
+      const int wv = tid >> 6, l = tid & 63;
+      const bool on = tid < nt && S.sG[tid] >= 0, fb = tid < nt && !on;
+      const unsigned long long bn = __ballot(on), bf = __ballot(fb);
+      if (l == 0) {
+        S.sWn[wv][0] = __popcll(bn);
+        S.sWn[wv][1] = __popcll(bf);
+      }
+      __syncthreads();
+      int pn = 0, pf = 0;
+      nnet = 0;
+      nfb = 0;
+#pragma unroll
+      for (int w = 0; w < CH / 64; w++) {
+        if (w < wv) {
+          pn += S.sWn[w][0];
+          pf += S.sWn[w][1];
+        }
+        nnet += S.sWn[w][0];
+        nfb += S.sWn[w][1];
+      }
+      const unsigned long long below = (l == 0) ? 0ull : (~0ull >> (64 - l));
+      if (on) S.sNet[pn + __popcll(bn & below)] = (short)tid;
+      if (fb) S.sFb[pf + __popcll(bf & below)] = (short)tid;
     }
     __syncthreads();
     // ---------------- phase B: interpolation, one lane per (triplet, ME)
-    for (int it = tid; it < nt * NME3; it += CH) {
-      const int t = it / NME3, m = it - t * NME3;
-      const TPrm &q = sq[t];
+    for (int it = tid; it < nfb * NME3; it += CH) {
+      const int k = it / NME3, m = it - k * NME3;
+      const int t = MLP ? (int)S.sFb[k] : k;
+      const TPrm &q = S.sq[t];
       if (m >= q.nme) continue;
-      if (MLP && sG[t] >= 0) continue;
       float wx[4], wy[4], w1[3], P[5];
       const float px = q.px, py = q.py;
       cubic_w(px, wx);
@@ -312,7 +353,7 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
         default: F = stencil<1>(tab, base, rs, m, wx, wy, w1, P, q.m1); break;
       }
       if (CHARGED) {
-        const TChg &c = sx[t];
+        const TChg &c = S.sx[t];
         for (int k = 0; k < FB_MAXSH; k++) {
           const float dq = c.dq[k];
           if (dq == 0.f) continue;
@@ -326,37 +367,40 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
           F = __builtin_fmaf(dq * c.stinky, Fk, F);
         }
       }
-      sF[t][m] = F;
+      S.sF[t][m] = F;
     }
     if (MLP) {
-      // network evaluation, 16 triplets per wavefront pass; lane group l/16
-      // folds the matrix elements m = l/16 + 4k of its triplet
+      // keeps the weight loads of phase M from being hoisted into the stencil
+      // loop (their registers would overlap)
+      __syncthreads();
+      // ---------------- phase M: network evaluation, 16 triplets per
+      // wavefront pass; lane group l/16 folds the matrix elements l/16 + 4k of
+      // its triplet (the theta cells times P_theta(cos) and the envelope)
       const int wv = tid >> 6, l = tid & 63, qg = l >> 4;
-      for (int blk = wv; blk * 16 < nt; blk += CH / 64) {
-        const int t = blk * 16 + (l & 15);
-        const int gsel = (t < nt) ? sG[t] : -1;
+      for (int blk = wv; blk * 16 < nnet; blk += CH / 64) {
+        const int k = blk * 16 + (l & 15);
+        const int t = (k < nnet) ? (int)S.sNet[k] : 0;
+        const int gsel = (k < nnet) ? S.sG[t] : -1;
         unsigned long long pend = __ballot(gsel >= 0);
         while (pend) {
           const int lead = __ffsll((long long)pend) - 1;
-          const int gcur = __shfl(gsel, lead);
-          const bool mine = (gsel == gcur);
-          const DMlp *w = v.mlp + gcur;
+          const int gnow = __shfl(gsel, lead);
+          const MlpW *w = &v.mlp[gnow].w;
+          const bool mine = (gsel == gnow);
           float out[16];
-          mlp16(w, mine ? sU[t][0] : 0.f, mine ? sU[t][1] : 0.f, mine ? sU[t][2] : 0.f, out);
+          mlp_eval16(w, mine ? S.sU[t][0] : 0.f, mine ? S.sU[t][1] : 0.f, mine ? S.sU[t][2] : 0.f, out);
           if (mine) {
-            const TPrm &q = sq[t];
-            const float env = sU[t][3];
+            const TPrm &q = S.sq[t];
+            const float env = S.sU[t][3];
+            const float P[5] = {env, q.P[0] * env, q.P[1] * env, q.P[2] * env, q.P[3] * env};
 #pragma unroll
             for (int ml = 0; ml < 3; ml++) {
               const int m = qg + 4 * ml;
               if (m < q.nme) {
-                float F = 0.f;
+                float F = P[0] * out[ml * 5];
 #pragma unroll
-                for (int th = 0; th < 5; th++) {
-                  const int r = ml * 5 + th, o = 16 * (r >> 2) + 4 * qg + (r & 3);
-                  F = __builtin_fmaf(th == 0 ? 1.f : q.P[th - 1], out[r] * w->oscale[o] * env, F);
-                }
-                sF[t][m] = F;
+                for (int th = 1; th < 5; th++) F = __builtin_fmaf(P[th], out[ml * 5 + th], F);
+                S.sF[t][m] = F;
               }
             }
           }
@@ -371,14 +415,14 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
 #pragma unroll
     for (int i = 0; i < 16; i++) X[i] = 0.f;
     if (tid < nt) {
-      const TPrm &q = sq[tid];
+      const TPrm &q = S.sq[tid];
       float F[NME3], pm[9];
       cls = q.cls;
       if (CHARGED) p = v.trip_pair[c0 + tid];
       const int nm0 = (cls == 3) ? 6 : (cls == 0 ? 1 : 2);
       const float sint = q.sint;
 #pragma unroll
-      for (int m = 0; m < NME3; m++) F[m] = (m < q.nme) ? sF[tid][m] * (m >= nm0 ? sint : 1.f) : 0.f;
+      for (int m = 0; m < NME3; m++) F[m] = (m < q.nme) ? S.sF[tid][m] * (m >= nm0 ? sint : 1.f) : 0.f;
 #pragma unroll
       for (int k = 0; k < 9; k++) pm[k] = q.pm[k];
       switch (cls) {
@@ -392,7 +436,7 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
     if (tid < nt) {
       const int nb = (cls & 2 ? 4 : 1) * (cls & 1 ? 4 : 1);
       if (CHARGED) {
-        const TChg &c = sx[tid];
+        const TChg &c = S.sx[tid];
         const double *sbp = v.mat[FB_MAT_S] + v.boff[p];
         const double *dbp = v.mat[FB_MAT_DIP] + v.boff[p];
 #pragma unroll
@@ -420,10 +464,10 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
       const int sl = tid + j * CH;
       if (sl < nown * NB) {
         const int sg = sl / NB, e = sl - sg * NB;
-        const int64_t s0 = sSeg[sg];
-        const int64_t k0 = max(s0, c0), k1 = min((int64_t)sSeg[sg + 1], c0 + nt);
-        if (k0 < k1 && e < sNb[sg]) {
-          double *const o = vna + sBo[sg] + e;
+        const int64_t s0 = S.sSeg[sg];
+        const int64_t k0 = max(s0, c0), k1 = min((int64_t)S.sSeg[sg + 1], c0 + nt);
+        if (k0 < k1 && e < S.sNb[sg]) {
+          double *const o = vna + S.sBo[sg] + e;
           double a = (s0 < c0) ? *o : 0.0;
           for (int64_t k = k0; k < k1; k++) a += (double)sC[k - c0][e];
           *o = a;
@@ -432,37 +476,87 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
     }
     __syncthreads();
   }
-  if (MLP) {
+}
+
+// table stencil: one tile per workgroup
+template <int CH, bool CHARGED>
+__global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
+  __shared__ S3c<CH, CHARGED, false> S;
+  const int kt = CH == 64 ? 0 : (CH == 128 ? 1 : 2);
+  int status = 0;
+  tile3c<CH, CHARGED, false>(v, v.tiles[kt][xcd_tile(blockIdx.x, gridDim.x)], S, nullptr, status);
+  if (status) atomicOr(v.status, status);
+}
+
+// learned surrogate with the stencil as fallback: persistent workgroups over a
+// queue of the 256-triplet tiles.  queue[x] counts the tiles taken from the
+// range of XCD x (tiles numbered contiguously per XCD, as xcd_tile numbers
+// them); a workgroup takes from its own XCD's range first (workgroup b runs on
+// XCD b mod 8), then from the others, and exits when all are empty.
+__global__ __launch_bounds__(256, 4) void k_3c_mlp(DView v, int *queue) {
+  __shared__ S3c<256, false, true> S;
+  __shared__ int s_tile;
+  __shared__ unsigned sCnt[2];
+  const int tid = threadIdx.x;
+  if (tid < 2) sCnt[tid] = 0u;
+  const unsigned nt = (unsigned)v.ntiles[2], per = nt >> 3, rem = nt & 7u;
+  const unsigned x0 = blockIdx.x & 7u;
+  int status = 0;
+  for (;;) {
     __syncthreads();
     if (tid == 0) {
-      atomicAdd(&v.mlp_count[0], (unsigned long long)sCnt[0]);
-      atomicAdd(&v.mlp_count[1], (unsigned long long)sCnt[1]);
+      int t = -1;
+      for (unsigned k = 0; k < 8 && t < 0; k++) {
+        const unsigned x = (x0 + k) & 7u;
+        const unsigned lo = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+        const unsigned cnt = per + (x < rem ? 1u : 0u);
+        if (cnt == 0) continue;
+        const unsigned i = (unsigned)atomicAdd(&queue[x], 1);
+        if (i < cnt) t = (int)(lo + i);
+      }
+      s_tile = t;
     }
+    __syncthreads();
+    const int t = s_tile;
+    if (t < 0) break;
+    tile3c<256, false, true>(v, v.tiles[2][t], S, sCnt, status);
+  }
+  if (tid == 0) {
+    atomicAdd(&v.mlp_count[0], (unsigned long long)sCnt[0]);
+    atomicAdd(&v.mlp_count[1], (unsigned long long)sCnt[1]);
   }
   if (status) atomicOr(v.status, status);
 }
 
-template <bool CHARGED, bool MLP>
+template <bool CHARGED>
 static int launch_mixed(const DView &v, int chunk, hipStream_t st) {
   const int kt = chunk == 64 ? 0 : (chunk == 256 ? 2 : 1);
   if (v.ntiles[kt] == 0) return 0;
   const dim3 grid((unsigned)v.ntiles[kt]);
   if (kt == 2) {
-    hipLaunchKernelGGL((k_3c_mixed<256, CHARGED, MLP>), grid, dim3(256), 0, st, v);
+    hipLaunchKernelGGL((k_3c_mixed<256, CHARGED>), grid, dim3(256), 0, st, v);
   } else if (kt == 0) {
-    hipLaunchKernelGGL((k_3c_mixed<64, CHARGED, MLP>), grid, dim3(64), 0, st, v);
+    hipLaunchKernelGGL((k_3c_mixed<64, CHARGED>), grid, dim3(64), 0, st, v);
   } else {
-    hipLaunchKernelGGL((k_3c_mixed<128, CHARGED, MLP>), grid, dim3(128), 0, st, v);
+    hipLaunchKernelGGL((k_3c_mixed<128, CHARGED>), grid, dim3(128), 0, st, v);
   }
   return (int)hipGetLastError();
 }
 
-extern "C" int fbk_3c_mixed_mlp(const DView &v, int chunk, hipStream_t st) {
-  if (v.ntrip == 0) return 0;
-  return launch_mixed<false, true>(v, chunk, st);
+extern "C" int fbk_fill_i32(int *a, int64_t n, int val, hipStream_t st);
+
+// queue: 8 ints of device memory, cleared here by a fill kernel; nwg:
+// persistent workgroups (four per CU)
+extern "C" int fbk_3c_mlp(const DView &v, int *queue, int nwg, hipStream_t st) {
+  if (v.ntrip == 0 || v.ntiles[2] == 0) return 0;
+  int rc = fbk_fill_i32(queue, 8, 0, st);
+  if (rc) return rc;
+  const dim3 grid((unsigned)std::max(1, std::min(nwg, v.ntiles[2])));
+  hipLaunchKernelGGL(k_3c_mlp, grid, dim3(256), 0, st, v, queue);
+  return (int)hipGetLastError();
 }
 
 extern "C" int fbk_3c_mixed(const DView &v, int chunk, int charged, hipStream_t st) {
   if (v.ntrip == 0) return 0;
-  return charged ? launch_mixed<true, false>(v, chunk, st) : launch_mixed<false, false>(v, chunk, st);
+  return charged ? launch_mixed<true>(v, chunk, st) : launch_mixed<false>(v, chunk, st);
 }

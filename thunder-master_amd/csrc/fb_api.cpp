@@ -61,7 +61,7 @@ int fbk_2c_lds(const DView &v, int mask, hipStream_t st);
 int fbk_3c_fp64_sp(const DView &v, hipStream_t st);
 int fbk_gamma_fold(const DView &v, int a0, int natoms, int which, const int *orb_off, int n, double *A,
                    hipStream_t st);
-int fbk_3c_mixed_mlp(const DView &v, int chunk, hipStream_t st);
+int fbk_3c_mlp(const DView &v, int *queue, int nwg, hipStream_t st);
 int fbk_mlp_samples(const DView &v, int gid, int64_t n, unsigned long long seed, float *X, float *T, hipStream_t st);
 int fbk_mlp_colmaxabs(int64_t n, int ncol, const float *A, float *out, hipStream_t st);
 int fbk_mlp_rowmax(int64_t n, const float *T, float *M, hipStream_t st);
@@ -70,10 +70,10 @@ int fbk_mlp_unenvelope(int64_t n, const float *X, float *T, float c, float kx, f
 int fbk_mlp_scale_cols(int64_t n, int ncol, float *A, const float *inv, hipStream_t st);
 int fbk_mlp_gemm(int M, int N, int K, const float *A, int64_t sai, int64_t sak, const float *B, int64_t sbk,
                  int64_t sbj, float *C, int64_t sci, int64_t scj, int nz, float *partial, hipStream_t st);
-int fbk_mlp_bias_silu(int64_t n, float *Z, float *A, const float *b, hipStream_t st);
+int fbk_mlp_bias_act(int64_t n, float *Z, float *A, const float *b, hipStream_t st);
 int fbk_mlp_out(int64_t n, int nme, float *Y, const float *b3, const float *T, float *dY, float *rowloss,
                 float inv_count, hipStream_t st);
-int fbk_mlp_silu_back(int64_t n, const float *dA, const float *Z, float *dZ, hipStream_t st);
+int fbk_mlp_act_back(int64_t n, const float *dA, const float *Z, float *dZ, hipStream_t st);
 int fbk_mlp_adam(int np, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float c1,
                  float c2, float eps, hipStream_t st);
 int fbk_mlp_infer(const DMlp *w, int64_t n, const float *X, float *Y, hipStream_t st);
@@ -206,6 +206,7 @@ bool same_list(const MeList &a, const std::vector<int> &mu, const std::vector<in
 struct fb_ctx {
   int device = 0;
   float *scratch = nullptr;   // this device's scratch array (fb_mlp_gradients_noalloc)
+  int ncu = 256;              // compute units of the device
   hipStream_t own_stream = nullptr, stream = nullptr;
   hipStream_t side = nullptr;          // second stream: 2c beside 3c in a build
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -341,12 +342,12 @@ struct fb_ctx {
   // learned 3c surrogate
   std::vector<DGroup3c> h_grp3c;
   std::vector<std::array<int, 4>> h_grpkey;   // (is, js, ks, isub) of each group
-  DevBuf d_mlp, d_mlp_err, d_mlp_count;
+  DevBuf d_mlp, d_mlp_err, d_mlp_count, d_queue;
   std::vector<DMlp> h_mlp;
   std::vector<std::vector<float>> h_mlp_params, h_mlp_err;
   std::vector<std::array<double, 8>> mlp_stat;
   bool mlp_ready = false;
-  double mlp_tau = 5.0e-5;
+  double mlp_tau = 2.5e-5;
   long mlp_gen = 0;
 
   bool timing = false;
@@ -543,7 +544,7 @@ struct fb_ctx {
                       &d_neighn, &d_noff, &d_neigh_j, &d_neigh_b, &d_neigh_back, &d_neigh_self, &d_pair_atom,
                       &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
                       &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_pair_tgt, &d_trip_aux, &d_tiles[0], &d_tiles[1], &d_tiles[2], &d_tmp,
-                      &d_mlp, &d_mlp_err, &d_mlp_count, &d_meth3c,
+                      &d_mlp, &d_mlp_err, &d_mlp_count, &d_queue, &d_meth3c,
                       &d_status, &d_idx3r, &d_qin, &d_wbsz, &d_wboff, &d_pp_map, &d_px_map, &d_px_point,
                       &d_inv_pp, &d_inv_px, &d_cand_cnt, &d_cand_off, &d_cand_j, &d_cand_b, &d_pc_n, &d_pc_off,
                       &d_pc_trip, &d_pc_tpair, &d_pc_trip_s, &d_pc_tpair_s, &d_pc_perm_in, &d_pc_perm, &d_pc_flag,
@@ -663,6 +664,8 @@ int fb_create(fb_ctx **out, int device) {
   if (e != hipSuccess) return create_failed(ctx, "streams", e);
   e = scratch_address(device, &ctx->scratch);
   if (e != hipSuccess) return create_failed(ctx, "scratch address", e);
+  e = hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess) return create_failed(ctx, "device attributes", e);
   ctx->stream = ctx->own_stream;
   e = ctx->d_status.ensure(sizeof(int) * 4);
   if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_status, sizeof(int) * 4);
@@ -1606,7 +1609,9 @@ static int run_3c(fb_ctx *ctx) {
     // the learned surrogate serves the neutral bcna term of the mixed path;
     // charged builds keep the table stencil for every subtype
     if (ctx->precision == FB_PREC_MIXED && ctx->method3c == FB_3C_MLP && ctx->mlp_ready && !any_charge(ctx))
-      return ctx->launch("assemble_3c_mlp", [&] { return fbk_3c_mixed_mlp(v, ctx->chunk3c, ctx->stream); });
+      return ctx->launch("assemble_3c_mlp", [&] {
+        return fbk_3c_mlp(v, ctx->d_queue.as<int>(), 4 * ctx->ncu, ctx->stream);
+      });
     if (ctx->precision == FB_PREC_MIXED)
       return ctx->launch("assemble_3c_mixed", [&] { return fbk_3c_mixed(v, ctx->chunk3c, any_charge(ctx), ctx->stream); });
     return ctx->launch("assemble_3c_fp64", [&] { return fbk_3c_fp64_sp(v, ctx->stream); });
@@ -1883,20 +1888,20 @@ int mlp_fwd_bwd(fb_ctx *ctx, const MlpWork &w, const float *P, float *G, int nme
   }
   // forward
   MLP_RC(fbk_mlp_gemm(Bi, MLP_H, MLP_IN, X, MLP_IN, 1, P + MLP_P_W1, 1, MLP_IN, Z1, MLP_H, 1, 1, nullptr, st));
-  MLP_RC(fbk_mlp_bias_silu(B, Z1, A1, P + MLP_P_B1, st));
+  MLP_RC(fbk_mlp_bias_act(B, Z1, A1, P + MLP_P_B1, st));
   MLP_RC(fbk_mlp_gemm(Bi, MLP_H, MLP_H, A1, MLP_H, 1, P + MLP_P_W2, 1, MLP_H, Z2, MLP_H, 1, 1, nullptr, st));
-  MLP_RC(fbk_mlp_bias_silu(B, Z2, A2, P + MLP_P_B2, st));
+  MLP_RC(fbk_mlp_bias_act(B, Z2, A2, P + MLP_P_B2, st));
   MLP_RC(fbk_mlp_gemm(Bi, MLP_O, MLP_H, A2, MLP_H, 1, P + MLP_P_W3, 1, MLP_H, Y, MLP_O, 1, 1, nullptr, st));
   MLP_RC(fbk_mlp_out(B, nme, Y, P + MLP_P_B3, T, dY, w.rowloss, 1.0f / ((float)B * (float)nvalid), st));
   // backward
   MLP_RC(fbk_mlp_gemm(MLP_O, MLP_H, Bi, dY, 1, MLP_O, A2, MLP_H, 1, G + MLP_P_W3, MLP_H, 1, nz, part, st));
   MLP_RC(fbk_mlp_gemm(1, MLP_O, Bi, ones, 0, 0, dY, MLP_O, 1, G + MLP_P_B3, MLP_O, 1, nz, part, st));
   MLP_RC(fbk_mlp_gemm(Bi, MLP_H, MLP_O, dY, MLP_O, 1, P + MLP_P_W3, MLP_H, 1, dA, MLP_H, 1, 1, nullptr, st));
-  MLP_RC(fbk_mlp_silu_back(B, dA, Z2, dZ2, st));
+  MLP_RC(fbk_mlp_act_back(B, dA, Z2, dZ2, st));
   MLP_RC(fbk_mlp_gemm(MLP_H, MLP_H, Bi, dZ2, 1, MLP_H, A1, MLP_H, 1, G + MLP_P_W2, MLP_H, 1, nz, part, st));
   MLP_RC(fbk_mlp_gemm(1, MLP_H, Bi, ones, 0, 0, dZ2, MLP_H, 1, G + MLP_P_B2, MLP_H, 1, nz, part, st));
   MLP_RC(fbk_mlp_gemm(Bi, MLP_H, MLP_H, dZ2, MLP_H, 1, P + MLP_P_W2, MLP_H, 1, dA, MLP_H, 1, 1, nullptr, st));
-  MLP_RC(fbk_mlp_silu_back(B, dA, Z1, dZ1, st));
+  MLP_RC(fbk_mlp_act_back(B, dA, Z1, dZ1, st));
   MLP_RC(fbk_mlp_gemm(MLP_H, MLP_IN, Bi, dZ1, 1, MLP_H, X, MLP_IN, 1, G + MLP_P_W1, MLP_IN, 1, nz, part, st));
   MLP_RC(fbk_mlp_gemm(1, MLP_H, Bi, ones, 0, 0, dZ1, MLP_H, 1, G + MLP_P_B1, MLP_H, 1, nz, part, st));
   return FB_OK;
@@ -1916,41 +1921,73 @@ double mlp_batch_loss(fb_ctx *ctx, MlpBufs &b, int nme, int64_t B) {
   return s / ((double)B * nvalid);
 }
 
+// weights and biases uniform in +-1/sqrt(fan in): the first layer's units
+// then switch at points spread over the input square
 void mlp_init_params(std::vector<float> &p, unsigned long long seed) {
   std::mt19937_64 rng(seed);
   std::uniform_real_distribution<double> u(-1.0, 1.0);
   p.assign(MLP_NP, 0.0f);
-  const double l1 = std::sqrt(6.0 / (MLP_IN + MLP_H)), l2 = std::sqrt(6.0 / (MLP_H + MLP_H)),
-               l3 = std::sqrt(6.0 / (MLP_H + MLP_O));
+  const double l1 = 1.0 / std::sqrt((double)MLP_IN), l2 = 1.0 / std::sqrt((double)MLP_H);
   for (int i = 0; i < MLP_H * MLP_IN; i++) p[MLP_P_W1 + i] = (float)(l1 * u(rng));
+  for (int i = 0; i < MLP_H; i++) p[MLP_P_B1 + i] = (float)(l1 * u(rng));
   for (int i = 0; i < MLP_H * MLP_H; i++) p[MLP_P_W2 + i] = (float)(l2 * u(rng));
-  for (int i = 0; i < MLP_O * MLP_H; i++) p[MLP_P_W3 + i] = (float)(l3 * u(rng));
+  for (int i = 0; i < MLP_H; i++) p[MLP_P_B2 + i] = (float)(l2 * u(rng));
+  for (int i = 0; i < MLP_O * MLP_H; i++) p[MLP_P_W3 + i] = (float)(l2 * u(rng));
+  for (int i = 0; i < MLP_O; i++) p[MLP_P_B3 + i] = (float)(l2 * u(rng));
 }
 
-// device image of a network: FP32 first layer and biases, BF16 operand tiles
+// device image of a network: MFMA operand fragments in the K-slot order of
+// mlp_eval16 (fb_mlp.h), the slot scale folded into the last layer
 void mlp_image(const std::vector<float> &p, const float *oscale, const DGroup3c &g, DMlp &w) {
   std::memset(&w, 0, sizeof(w));
-  std::memcpy(w.w1, &p[MLP_P_W1], sizeof(w.w1));
-  std::memcpy(w.b1, &p[MLP_P_B1], sizeof(w.b1));
-  std::memcpy(w.b2, &p[MLP_P_B2], sizeof(w.b2));
-  std::memcpy(w.b3, &p[MLP_P_B3], sizeof(w.b3));
   for (int o = 0; o < MLP_O; o++) w.oscale[o] = oscale ? oscale[o] : 1.0f;
   w.xmax = (float)g.xmax;
   w.ymax = (float)g.ymax;
-  w.xs = (float)(2.0 / g.xmax);
-  w.ys = (float)(2.0 / g.ymax);
   w.nme = g.nME;
   w.nx = g.nx;
   w.ny = g.ny;
   w.ready = 1;
+  auto hi = [](float f) { return mlp_bf16(f); };
+  auto lo = [](float f) { return mlp_bf16(f - mlp_bf16f(mlp_bf16(f))); };
+  auto pack2 = [](const unsigned short h[4]) {
+    return uint2{(unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16)};
+  };
+  auto pack4 = [](const unsigned short h[8]) {
+    return uint4{(unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16),
+                 (unsigned)h[4] | ((unsigned)h[5] << 16), (unsigned)h[6] | ((unsigned)h[7] << 16)};
+  };
+  // hidden unit of K slot (q, i) of K block kb: accumulator tiles 2kb, 2kb+1
+  auto unit = [](int kb, int q, int i) { return i < 4 ? 32 * kb + 4 * q + i : 32 * kb + 16 + 4 * q + (i - 4); };
   for (int mt = 0; mt < 4; mt++)
-    for (int kb = 0; kb < 4; kb++)
-      for (int l = 0; l < 64; l++)
-        for (int j = 0; j < 4; j++) {
-          const int row = mt * 16 + (l & 15), col = kb * 16 + 4 * (l >> 4) + j;
-          w.w2op[mt][kb][l][j] = mlp_bf16(p[MLP_P_W2 + row * MLP_H + col]);
-          w.w3op[mt][kb][l][j] = mlp_bf16(p[MLP_P_W3 + row * MLP_H + col]);
+    for (int l = 0; l < 64; l++) {
+      const int q = l >> 4, r = 16 * mt + (l & 15);
+      // layer 1: slots q0 (w0h, w0h, w1h, w1h), q1 (w2h, w2h, w0l, w1l), q2 (w2l, 0, 0, 0)
+      const float *w1 = &p[MLP_P_W1 + r * MLP_IN];
+      unsigned short s1[4] = {0, 0, 0, 0};
+      if (q == 0) { s1[0] = hi(w1[0]); s1[1] = hi(w1[0]); s1[2] = hi(w1[1]); s1[3] = hi(w1[1]); }
+      if (q == 1) { s1[0] = hi(w1[2]); s1[1] = hi(w1[2]); s1[2] = lo(w1[0]); s1[3] = lo(w1[1]); }
+      if (q == 2) { s1[0] = lo(w1[2]); }
+      w.w.a1[mt][l] = pack2(s1);
+      const float sc = w.oscale[r];
+      for (int kb = 0; kb < 2; kb++) {
+        unsigned short s2[8], s3h[8], s3l[8];
+        for (int i = 0; i < 8; i++) {
+          const int c = unit(kb, q, i);
+          s2[i] = hi(p[MLP_P_W2 + r * MLP_H + c]);
+          const float v3 = p[MLP_P_W3 + r * MLP_H + c] * sc;
+          s3h[i] = hi(v3);
+          s3l[i] = lo(v3);
         }
+        w.w.a2[mt][kb][l] = pack4(s2);
+        w.w.a3h[mt][kb][l] = pack4(s3h);
+        w.w.a3l[mt][kb][l] = pack4(s3l);
+      }
+    }
+  for (int u = 0; u < MLP_H; u++) {
+    w.w.b1[u] = p[MLP_P_B1 + u];
+    w.w.b2[u] = p[MLP_P_B2 + u];
+  }
+  for (int o = 0; o < MLP_O; o++) w.w.b3[o] = p[MLP_P_B3 + o] * w.oscale[o];
 }
 
 }  // namespace
@@ -2065,32 +2102,32 @@ int fb_mlp_train(fb_ctx *ctx, int64_t nsamples, int nsteps, int batch, double lr
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctx->mlp_stat[gid] = {1.0, (double)nsamples, (double)nsteps, loss0, loss, 0.0, ms, 0.0};
   }
-  // calibration: per grid cell, the largest |network - table| of any theta cell
+  // calibration: per sub-cell of the calibration grid, the largest block
+  // error |network - table| (Frobenius) of the sampled points
   FB_HIP(hipSetDevice(ctx->device));
   RC(upload(ctx, ctx->d_mlp, ctx->h_mlp.data(), sizeof(DMlp) * std::max(ng, 1), "upload networks"));
   std::vector<float> errs;
   for (int gid = 0; gid < ng; gid++) {
     if (!ctx->h_mlp[gid].ready) continue;
     const DGroup3c &g = ctx->h_grp3c[gid];
-    const size_t ncell = (size_t)g.nx * g.ny;
+    const size_t nsub = (size_t)(g.nx - 1) * MLP_CSUB * (size_t)(g.ny - 1) * MLP_CSUB;
     DevBuf de;
-    FB_HIP(de.ensure(sizeof(unsigned) * ncell));
-    FB_HIP(hipMemsetAsync(de.p, 0, sizeof(unsigned) * ncell, ctx->stream));
-    const int64_t ncal = std::max<int64_t>(1 << 20, (int64_t)ncell * 1024);
+    FB_HIP(de.ensure(sizeof(unsigned) * nsub));
+    FB_HIP(hipMemsetAsync(de.p, 0, sizeof(unsigned) * nsub, ctx->stream));
+    const int64_t ncal = std::max<int64_t>(1 << 20, (int64_t)nsub * 256);
     int rc = fbk_mlp_calib(v, gid, ctx->d_mlp.as<DMlp>() + gid, ncal, seed ^ 0xca1b7a7eull, de.as<unsigned>(),
                            ctx->stream);
-    std::vector<float> ce(ncell);
+    std::vector<float> ce(nsub);
     if (rc == 0) rc = (int)hipStreamSynchronize(ctx->stream);
-    if (rc == 0) rc = (int)hipMemcpy(ce.data(), de.p, sizeof(float) * ncell, hipMemcpyDeviceToHost);
+    if (rc == 0) rc = (int)hipMemcpy(ce.data(), de.p, sizeof(float) * nsub, hipMemcpyDeviceToHost);
     de.release();
     if (rc) return ctx->fail(FB_E_HIP, "network calibration", (hipError_t)rc);
-    // cells no calibration sample reached (the clamped border rows) never
-    // take the network
-    for (float &x : ce)
-      if (!(x > 0.f)) x = 3.0e38f;
+    // sub-cells no calibration sample reached never take the network
     float emax = 0.f;
-    for (int iy = 2; iy <= g.ny - 2; iy++)
-      for (int ix = 2; ix <= g.nx - 2; ix++) emax = std::max(emax, ce[(size_t)(iy - 1) * g.nx + (ix - 1)]);
+    for (float &x : ce) {
+      if (!(x > 0.f)) x = 3.0e38f;
+      else emax = std::max(emax, x);
+    }
     ctx->mlp_stat[gid][5] = emax;
     ctx->mlp_stat[gid][7] = (double)ncal;
     ctx->h_mlp[gid].err_off = (int64_t)errs.size();
@@ -2101,6 +2138,8 @@ int fb_mlp_train(fb_ctx *ctx, int64_t nsamples, int nsteps, int batch, double lr
   RC(upload(ctx, ctx->d_mlp, ctx->h_mlp.data(), sizeof(DMlp) * std::max(ng, 1), "upload networks"));
   RC(upload(ctx, ctx->d_mlp_err, errs.data(), sizeof(float) * errs.size(), "upload calibration"));
   FB_HIP(ctx->d_mlp_count.ensure(2 * sizeof(unsigned long long)));
+  FB_HIP(ctx->d_queue.ensure(8 * sizeof(int)));
+
   FB_HIP(hipMemsetAsync(ctx->d_mlp_count.p, 0, 2 * sizeof(unsigned long long), ctx->stream));
   FB_HIP(hipStreamSynchronize(ctx->stream));
   ctx->mlp_ready = true;
@@ -2977,8 +3016,11 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   const int charged = any_charge(ctx);
   // the chunk only shapes the mixed s/p 3c kernel; the binary64 and generic
   // 3c kernels have no launch parameter to tune
-  const bool mixed3c = ctx->canon_sp && ctx->precision == FB_PREC_MIXED;
-  const bool mlp = mixed3c && ctx->method3c == FB_3C_MLP && ctx->mlp_ready && !charged;
+  // (the learned surrogate's launch has no chunk: it always takes the
+  // 256-triplet tiles)
+  const bool mlp = ctx->canon_sp && ctx->precision == FB_PREC_MIXED && ctx->method3c == FB_3C_MLP &&
+                   ctx->mlp_ready && !charged;
+  const bool mixed3c = ctx->canon_sp && ctx->precision == FB_PREC_MIXED && !mlp;
   // one candidate: a warm-up launch, then nrep launches between two events
   auto timed = [&](auto &&fn, float *ms) -> hipError_t {
     hipError_t e = (hipError_t)fn();
@@ -2995,8 +3037,7 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
     float best = 1e30f;
     for (int c : cand) {
       float ms = 0.f;
-      const hipError_t e = timed([&] { return mlp ? fbk_3c_mixed_mlp(v, c, ctx->stream)
-                                                  : fbk_3c_mixed(v, c, charged, ctx->stream); }, &ms);
+      const hipError_t e = timed([&] { return fbk_3c_mixed(v, c, charged, ctx->stream); }, &ms);
       if (e != hipSuccess) return ctx->fail(FB_E_HIP, "autotune 3c", e);
       if (ms < best) {
         best = ms;

@@ -416,7 +416,7 @@ class Assembler:
     MLP_INFO = ("is", "js", "ks", "isub", "trained", "nsamples", "nsteps", "loss_before", "loss_after",
                 "max_cell_err_eV", "train_ms", "calib_samples")
 
-    def train_mlp(self, nsamples=1 << 21, nsteps=4000, batch=8192, lr=3e-3, seed=7):
+    def train_mlp(self, nsamples=1 << 21, nsteps=40000, batch=8192, lr=3e-3, seed=7):
         """Train one network per neutral 3c group on samples of the table path
         (GPU forward/backward/Adam kernels), then calibrate its error."""
         self._check(self.L.fb_mlp_train(self.h, int(nsamples), int(nsteps), int(batch), float(lr), int(seed)))
