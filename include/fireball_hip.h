@@ -70,7 +70,7 @@ typedef struct fb_ctx fb_ctx;
 /* --- context ------------------------------------------------------------ */
 int fb_create(fb_ctx **ctx, int device);
 int fb_destroy(fb_ctx *ctx);
-const char *fb_last_error(const fb_ctx *ctx);          /* NULL: why this thread's last fb_create failed */
+const char *fb_last_error(const fb_ctx *ctx);          /* NULL: this thread's last context-free failure (fb_create, fb_read_positions) */
 int fb_set_stream(fb_ctx *ctx, void *hip_stream);      /* NULL: the context's own stream */
 int fb_synchronize(fb_ctx *ctx);
 int fb_set_precision(fb_ctx *ctx, int mode);          /* enum fb_precision */
@@ -110,6 +110,22 @@ int fb_read_fdata(fb_ctx *ctx, const char *fdata_dir);
  * leaves them; imass 1-based species; lattice rows a1, a2, a3; dQ
  * [natoms][FB_MAXSH] shell charges (NULL: neutral).  Several structures form
  * one batch: lists and assembly then cover all of them in single launches. */
+/* fb_read_positions: the in-scope steps of read_positions
+ * (c.SYSTEM/M_configuraciones.f90:604-797) on a structure file in the
+ * reference's layout ("natoms icluster"; three lattice vectors; one "Z x y z"
+ * record per atom; list-directed): species by atomic number (the first of the
+ * nspecies entries of nZ with that Z, 1-based into imass, :666-681); the
+ * origin shift, every atom moved by (pi, 1/e, sqrt 2) when some atom lies
+ * within 1e-4 A of the origin (:757-775, *shifted = 1 then); norbitals, the
+ * sum of norb over the atoms (:703-708); iblock_slot, each atom's 0-based
+ * orbital offset (:778-783, may be NULL).  Outputs are ready for
+ * fb_add_structure (lattice rows a1, a2, a3; ratom [natoms][3]).  max_atoms
+ * bounds ratom/imass/iblock_slot (FB_E_CAPACITY beyond, *natoms still set).
+ * An atomic number missing from nZ fails with FB_E_ARG where the reference
+ * stops.  Needs no context and no GPU; fb_last_error(NULL) gives the reason. */
+int fb_read_positions(const char *basisfile, int nspecies, const int *nZ, const int *norb, int max_atoms,
+                      int *natoms, int *icluster, double *lattice, double *ratom, int *imass, int *norbitals,
+                      int *iblock_slot, int *shifted);
 int fb_clear_structures(fb_ctx *ctx);
 int fb_add_structure(fb_ctx *ctx, int natoms, const double *ratom, const int *imass,
                      const double *lattice, int icluster, const double *dQ, int *istruct);

@@ -301,3 +301,38 @@ def run_pp(ts, structure, binary=HARNESS_PREF, nrep=1, workdir=None, timeout=360
         import shutil
         shutil.rmtree(tmp, ignore_errors=True)
     return res
+
+
+HARNESS_POSREF = os.path.join(HERE, "_ref", "fb_posharness_ref")
+
+
+def run_positions(species, basis_file, binary=HARNESS_POSREF, timeout=600):
+    """The reference's read_positions on basis_file (oracle/ref/fb_posharness.f90):
+    natoms, icluster, shifted (ishiftO), norbitals, lattice [3][3], imass,
+    iblock_slot and ratom [natoms][3] as read_positions leaves them."""
+    tmp = tempfile.mkdtemp(prefix="fbpos_")
+    spf = os.path.join(tmp, "species.in")
+    with open(spf, "w") as f:
+        f.write(f"{len(species)}\n")
+        for sp in species:
+            f.write(f"{sp.nZ} {float(getattr(sp, 'xmass', 2.0 * sp.nZ))!r} {sp.norb}\n")
+    outp = os.path.join(tmp, "pos.out")
+    # the reference keeps the file name in 25 characters (T_structure%basisfile)
+    import shutil
+    shutil.copyfile(basis_file, os.path.join(tmp, "s.inp"))
+    subprocess.run([binary, spf, "s.inp", outp], cwd=tmp, check=True,
+                   stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=timeout)
+    with open(outp, "rb") as f:
+        r = _Reader(f.read())
+    natoms, icluster, shifted, norbitals = (int(v) for v in r.i32(4))
+    lattice = r.f64(9).reshape(3, 3)
+    imass = np.zeros(natoms, np.int32)
+    slot = np.zeros(natoms, np.int32)
+    ratom = np.zeros((natoms, 3))
+    for ia in range(natoms):
+        imass[ia], slot[ia] = r.i32(2)
+        ratom[ia] = r.f64(3)
+    import shutil
+    shutil.rmtree(tmp, ignore_errors=True)
+    return {"natoms": natoms, "icluster": icluster, "shifted": shifted, "norbitals": norbitals,
+            "lattice": lattice, "imass": imass, "iblock_slot": slot, "ratom": ratom}

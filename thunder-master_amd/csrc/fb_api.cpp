@@ -685,6 +685,22 @@ int fb_destroy(fb_ctx *ctx) {
 
 const char *fb_last_error(const fb_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 
+int fb_read_positions(const char *basisfile, int nspecies, const int *nZ, const int *norb, int max_atoms,
+                      int *natoms, int *icluster, double *lattice, double *ratom, int *imass, int *norbitals,
+                      int *iblock_slot, int *shifted) {
+  if (!basisfile || nspecies < 1 || !nZ || !norb || max_atoms < 0 || !natoms || !icluster || !lattice || !ratom ||
+      !imass || !norbitals || !shifted) {
+    g_create_err = "read_positions arguments";
+    return FB_E_ARG;
+  }
+  std::string err;
+  const int rc = fb_positions_read(basisfile, nspecies, nZ, norb, max_atoms, natoms, icluster, lattice, ratom, imass,
+                                   norbitals, iblock_slot, shifted, err);
+  if (rc == 0) return FB_OK;
+  g_create_err = "read_positions: " + err;
+  return rc == -2 ? FB_E_CAPACITY : FB_E_ARG;
+}
+
 int fb_set_stream(fb_ctx *ctx, void *s) {
   if (!ctx) return FB_E_ARG;
   ctx->stream = s ? (hipStream_t)s : ctx->own_stream;

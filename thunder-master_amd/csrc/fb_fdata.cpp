@@ -25,6 +25,7 @@
 //    connected unit; directories written by create.x keep each interaction's
 //    cells together, so this case does not arise there).
 #include <cctype>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -319,5 +320,84 @@ int fb_fdata_read_cl(const std::string &dir, int ns, const int *nZ, const int *n
         return -1;
       }
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------- positions
+// The in-scope steps of read_positions (c.SYSTEM/M_configuraciones.f90:604-797)
+// on a structure file in the reference's layout: "natoms icluster", three
+// lattice vectors, then one "Z x y z" record per atom, all list-directed.
+int fb_positions_read(const std::string &path, int ns, const int *nZ, const int *norb, int max_atoms, int *natoms,
+                      int *icluster, double *lattice, double *ratom, int *imass, int *norbitals, int *iblock_slot,
+                      int *shifted, std::string &err) {
+  ListReader r;
+  if (!r.open(path)) {
+    err = "cannot open " + path;
+    return -1;
+  }
+  std::vector<std::string> t;
+  int n = 0, ic = 0;
+  if (!r.values(2, t) || !to_int(t[0], n) || !to_int(t[1], ic) || n < 0) {
+    err = "bad natoms/icluster record in " + path;
+    return -1;
+  }
+  *natoms = n;
+  *icluster = ic;
+  if (n > max_atoms) {
+    err = "more atoms than max_atoms in " + path;
+    return -2;
+  }
+  for (int k = 0; k < 3; k++) {
+    if (!r.values(3, t)) {
+      err = "short lattice record in " + path;
+      return -1;
+    }
+    for (int c = 0; c < 3; c++)
+      if (!to_double(t[c], lattice[3 * k + c])) {
+        err = "bad lattice value in " + path;
+        return -1;
+      }
+  }
+  for (int i = 0; i < n; i++) {
+    int z = 0;
+    if (!r.values(4, t) || !to_int(t[0], z)) {
+      err = "bad atom record in " + path;
+      return -1;
+    }
+    for (int c = 0; c < 3; c++)
+      if (!to_double(t[1 + c], ratom[3 * i + c])) {
+        err = "bad coordinate in " + path;
+        return -1;
+      }
+    // species matching by atomic number: the first species with that Z (:666-681)
+    int sp = 0;
+    for (int k = 0; k < ns && !sp; k++)
+      if (nZ[k] == z) sp = k + 1;
+    if (!sp) {
+      err = "atomic number " + std::to_string(z) + " of " + path + " is not among the species";
+      return -3;
+    }
+    imass[i] = sp;
+  }
+  // norbitals (:703-708) and the orbital offset of each atom (:778-783)
+  int no = 0;
+  for (int i = 0; i < n; i++) {
+    if (iblock_slot) iblock_slot[i] = no;
+    no += norb[imass[i] - 1];
+  }
+  *norbitals = no;
+  // shift every atom by (pi, 1/e, sqrt 2) when one lies within 1e-4 of the
+  // origin (:757-775); magnitude is b.FUNCTIONS/magnitude.f90
+  int shift = 0;
+  for (int i = 0; i < n && !shift; i++) {
+    const double *a = ratom + 3 * i;
+    if (std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]) < 1.0e-4) shift = 1;
+  }
+  if (shift) {
+    const double shifter[3] = {4.0 * std::atan(1.0), 1.0 / std::exp(1.0), std::sqrt(2.0)};
+    for (int i = 0; i < n; i++)
+      for (int c = 0; c < 3; c++) ratom[3 * i + c] = ratom[3 * i + c] + shifter[c];
+  }
+  *shifted = shift;
   return 0;
 }

@@ -27,3 +27,9 @@ int fb_fdata_read(const std::string &dir, int ns, const int *nZ, const FdataWant
                   std::string &err);
 int fb_fdata_read_cl(const std::string &dir, int ns, const int *nZ, const int *nssh_pp, double *cl,
                      std::string &err);
+
+// the in-scope steps of read_positions on one structure file (see
+// fb_fdata.cpp); 0, or -1 (format), -2 (capacity), -3 (unknown Z) with err set
+int fb_positions_read(const std::string &path, int ns, const int *nZ, const int *norb, int max_atoms, int *natoms,
+                      int *icluster, double *lattice, double *ratom, int *imass, int *norbitals, int *iblock_slot,
+                      int *shifted, std::string &err);

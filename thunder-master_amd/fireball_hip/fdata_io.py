@@ -120,6 +120,20 @@ def write_harness_input(path: str, fdata_dir: str, ts: TableSet, s: Structure):
             f.write(" ".join(_fmt(v) for v in dq) + "\n")
 
 
+def write_basis_file(path: str, species, s: Structure, z=None):
+    """A structure file in the layout read_positions reads
+    (c.SYSTEM/M_configuraciones.f90:651-681): "natoms icluster", the three
+    lattice vectors, one "Z x y z" record per atom (positions as given, before
+    any origin shift).  z overrides the atomic numbers per atom."""
+    with open(path, "w") as f:
+        f.write(f"{s.natoms} {s.icluster}\n")
+        for i in range(3):
+            f.write(" ".join(_fmt(v) for v in s.lattice[i]) + "\n")
+        for ia in range(s.natoms):
+            zi = species[s.imass[ia] - 1].nZ if z is None else int(z[ia])
+            f.write(f"{zi} " + " ".join(_fmt(v) for v in s.ratom[ia]) + "\n")
+
+
 # ---------------------------------------------------------------- reader
 class _ListReader:
     """Fortran list-directed input over one text file: every read starts at a

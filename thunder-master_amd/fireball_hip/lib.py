@@ -84,6 +84,7 @@ def lib():
             "fb_mlp_gradients": [vp, C.c_int, fp, C.c_int, fp, fp, fp, dp],
             "fb_mlp_forward": [vp, C.c_int, fp, fp, C.c_int, fp, fp],
             "fb_mlp_gradients_noalloc": [vp, C.c_int, vp, C.c_int, vp, vp, vp, vp],
+            "fb_read_positions": [C.c_char_p, C.c_int, ip, ip, C.c_int, ip, ip, dp, dp, ip, ip, ip, ip],
             "fb_set_denmat": [vp, C.c_int, dp],
             "fb_dassemble_S": [vp], "fb_dassemble_T": [vp], "fb_dassemble_dipole_z": [vp],
             "fb_dassemble_vna_2c": [vp], "fb_dassemble_vna_3c": [vp], "fb_dassemble_all": [vp],
@@ -125,6 +126,34 @@ def _fp(a):
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def read_positions(path, species, max_atoms=None):
+    """fb_read_positions: the in-scope steps of the reference's read_positions
+    on a structure file (species matched by atomic number, origin shift,
+    norbitals, iblock_slot).  No GPU is needed."""
+    L = lib()
+    if max_atoms is None:
+        with open(path) as f:
+            max_atoms = int(f.readline().replace(",", " ").split()[0])
+    ns = len(species)
+    nZ = np.array([sp.nZ for sp in species], np.int32)
+    norb = np.array([sp.norb for sp in species], np.int32)
+    n, ic, no, sh = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    lat = np.zeros(9)
+    m = max(int(max_atoms), 1)
+    ratom = np.zeros(3 * m)
+    imass = np.zeros(m, np.int32)
+    slot = np.zeros(m, np.int32)
+    rc = L.fb_read_positions(str(path).encode(), ns, _ip(nZ), _ip(norb), int(max_atoms), C.byref(n), C.byref(ic),
+                             _dp(lat), _dp(ratom), _ip(imass), C.byref(no), _ip(slot), C.byref(sh))
+    if rc != 0:
+        msg = L.fb_last_error(None)
+        raise FireballError(rc, msg.decode() if msg else "")
+    k = n.value
+    return {"natoms": k, "icluster": ic.value, "shifted": sh.value, "norbitals": no.value,
+            "lattice": lat.reshape(3, 3), "imass": imass[:k].copy(), "iblock_slot": slot[:k].copy(),
+            "ratom": ratom[:3 * k].reshape(k, 3).copy()}
 
 
 class Assembler:
