@@ -150,8 +150,10 @@ int fb_assemble_vna_2c(fb_ctx *ctx);     /* j.ASSEMBLERS/M_assemble_2c_DOGS.f90:
 int fb_assemble_vna_3c(fb_ctx *ctx);     /* j.ASSEMBLERS/M_assemble_3c_DOGS.f90:86-340   */
 /* S, T, dipole_z and vna_2c in one 2c launch, then vna_3c: the H+S build.
  * Asynchronous: it returns once the build is enqueued (replayed from a
- * captured HIP graph unless timing is on); a kernel-detected error is returned
- * by the next fb_synchronize, fb_get_blocks or assemble call. */
+ * captured HIP graph unless timing is on), without waiting for an earlier
+ * build; a kernel-detected error of the builds enqueued since the last check
+ * is returned by the next fb_synchronize, fb_get_blocks or other call that
+ * reads results or assembles separately. */
 int fb_assemble_all(fb_ctx *ctx);
 
 /* --- results ------------------------------------------------------------ */
@@ -328,10 +330,18 @@ int fb_mlp_gradients_noalloc(fb_ctx *ctx, int nme, const float *params, int n, c
  * kernels for s/p species, 1 (default) = compile-time block shapes, 0 = the
  * generic shell loops; both give the same values.  The McWEDA density and
  * pseudopotential kernels follow the same switch.
- * fb_autotune times every candidate of both on the current lists (nrep
- * launches each, HIP events) and keeps the fastest; it returns the chosen
- * chunk in *chosen. */
-enum fb_tune { FB_TUNE_CHUNK3C = 0, FB_TUNE_2C_KERNEL = 1, FB_TUNE_GRAPH = 2, FB_TUNE_FORCE_KERNEL = 3 };
+ * FB_TUNE_FUSED: 0 runs the neutral mixed-precision H+S build as two
+ * launches (2c, then 3c, the 2c one on a second stream), 3 or 4 as one
+ * launch whose first blocks are the 2c atom groups and the rest the
+ * 256-triplet 3c tiles (register allocation for 3 or 4 waves per SIMD);
+ * charged, binary64 and learned-surrogate builds always take separate
+ * launches.
+ * fb_autotune times whole H+S builds (nrep each, HIP events) over every
+ * launch shape that applies to the context -- the fused launch at both
+ * occupancies, and the separate launches with every chunk and both 2c
+ * kernels -- and keeps the fastest; it returns the chosen chunk in *chosen.
+ * The blocks hold a complete build afterwards. */
+enum fb_tune { FB_TUNE_CHUNK3C = 0, FB_TUNE_2C_KERNEL = 1, FB_TUNE_GRAPH = 2, FB_TUNE_FORCE_KERNEL = 3, FB_TUNE_FUSED = 4 };
 int fb_set_tuning(fb_ctx *ctx, int param, int value);
 int fb_get_tuning(fb_ctx *ctx, int param, int *value);
 int fb_autotune(fb_ctx *ctx, int nrep, int *chosen);

@@ -166,6 +166,30 @@ def test_full_size_fp64_digest(idx):
         print(f"{m}: values identical, bytes {'identical' if g['sha256'] == rec['sha256'] else 'differ in zero signs'}")
 
 
+@pytest.mark.parametrize("waves", [3, 4])
+def test_fused_build_matches_separate_launches(waves):
+    """One persistent launch for the whole neutral mixed-precision H+S build
+    (2c atom groups, then 3c tiles, from a queue) gives the same bits as the
+    separate 2c and 3c launches, on a batch with a structure without
+    triplets, and from a replayed graph as well as eagerly."""
+    from fireball_hip import synth
+    box = np.eye(3) * 60.0
+    dimer = synth.Structure(np.array([[10.0, 10.0, 10.0], [12.3, 10.0, 10.0]]), np.ones(2, np.int32), box, 1)
+    structs = [dimer, synth.diamond_si((3, 3, 3), sigma=0.08, seed=4), synth.diamond_si((1, 2, 1), sigma=0.06, seed=5)]
+    ts = synth.make_tables([synth.SI])
+    asm = _gpu(ts, structs, precision="mixed")
+    ref = [{m: asm.blocks(m, k) for m in MATS} for k in range(len(structs))]
+    asm.set_tuning(4, waves)
+    for graph in (1, 0):
+        asm.set_tuning(2, graph)
+        print(f"fused waves={waves} graph={graph}", flush=True)
+        for _ in range(2):
+            asm.assemble_all()
+            for k in range(len(structs)):
+                for m in MATS:
+                    assert_bits(asm.blocks(m, k), ref[k][m], f"{k} {m}")
+
+
 def test_error_atoms_too_close():
     from fireball_hip import synth
     from fireball_hip.lib import Assembler, FireballError

@@ -15,14 +15,12 @@
 //
 // The kernel is bound by the block stores (HBM); its arithmetic stays binary64.
 #include "fb_device.h"
+#include "fb_2c_sp.h"
 
 #define W2C 64
 #define NB (FB_MO * FB_MO)
 #define NME2 16   // max non-zero 2c matrix elements for s/p species
 
-__device__ __forceinline__ int cell2c_id(const DView &v, int in1, int in2, int slot, int isub) {
-  return v.idx2c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * FB_NSLOT2 + slot) * FB_MAXSUB2 + isub];
-}
 
 // interpolate + recover + rotate one interaction into X (column-major nmu x nnu)
 __device__ inline int me2c_rotated(const DView &v, int cid, int in1, int in3, int nmu, double x, const double *pm,
@@ -172,206 +170,11 @@ extern "C" int fbk_2c(const DView &v, int mask, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// s/p species with canonical tables: one lane per pair, workgroups packed by
-// whole atoms (at most 16 atoms and 256 pairs, or one larger atom), Lagrange
-// denominators from the host table, rotations from fb_sp.h.  Bit-identical to
-// the reference in binary64 for neutral atoms (same operations, same order).
-#include "fb_sp.h"
-#include "fb_2c_dev.h"
-
-#define WG2 256
-
-// rotated block of one interaction; returns status bits
-template <int CLS>
-__device__ __forceinline__ int block2c(const DView &v, int cid, double x, const double *pm, int &grid_cached,
-                                       int &ileft, double *prod, double *X) {
-  if (cid < 0) return ST_NO_TABLE;
-  const DCell2c c = v.cell2c[cid];
-  if (x < 0.0) return ST_X_NEG;
-  if (x > c.xmax * (1.0 + 1.0e-5)) return ST_X_RANGE;
-  if (c.grid != grid_cached) {
-    const DGrid2c gr = v.grid2c[c.grid];
-    ileft = window7(x, gr.dx, gr.nx);
-    lagrange7(v, gr, x, ileft, prod);
-    grid_cached = c.grid;
-  }
-  constexpr int NME = nme2c_of(CLS);
-  double F[NME];
-  interp_cell<NME>(v, c, ileft, prod, F);
-  rot2c_sp<double>(CLS, pm, F, X);
-  return 0;
-}
-
-// atom_c receives the neutral atom term; with KQ > 0 the charged terms of
-// shells 1..KQ go to qrow[(isorp - 1) * QROW + i] (LDS rows that the caller
-// cleared), so that the self-block sum can add each term on its own
-#define QROW (WG2 * 17)
-template <int CLS, int KQ>
-__device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, int in1, int in2, bool self,
-                                      bool keep_vna, double z, const double *pm, int64_t bo, double *atom_c,
-                                      double *qrow) {
-  constexpr int NMU = (CLS & 2) ? 4 : 1, NNU = (CLS & 1) ? 4 : 1, NBL = NMU * NNU;
-  constexpr int CLSA = (CLS & 2) ? CLS_SPSP : CLS_SS;   // atom term lives on iatom's orbitals
-  int status = 0, grid_cached = -1, ileft = 1;
-  double prod[7], X[16];
-  if (mask & 1) {
-    status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_S, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-    for (int i = 0; i < NBL; i++) v.mat[FB_MAT_S][bo + i] = X[i];
-  }
-  if (mask & 2) {
-    status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_T, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-    for (int i = 0; i < NBL; i++) v.mat[FB_MAT_T][bo + i] = X[i];
-  }
-  if (mask & 4) {
-    status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_DIP, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-    for (int i = 0; i < NBL; i++) v.mat[FB_MAT_DIP][bo + i] = X[i];
-  }
-  if (mask & 8) {
-    double acc[NBL];
-#pragma unroll
-    for (int i = 0; i < NBL; i++) acc[i] = 0.0;
-    if (!self) {
-      status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_ONL, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-      for (int i = 0; i < NBL; i++) acc[i] = acc[i] + X[i] * FB_P_EQ2;
-      for (int isorp = 1; isorp <= v.sp.nssh[in1 - 1]; isorp++) {
-        double dQ = v.dQ[ia * FB_MAXSH + isorp - 1];
-        if (dQ == 0.0) continue;
-        status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_ONL, isorp), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-        for (int i = 0; i < NBL; i++) acc[i] = acc[i] + dQ * X[i] * FB_P_EQ2;
-      }
-      status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_ONR, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-      for (int i = 0; i < NBL; i++) acc[i] = acc[i] + X[i] * FB_P_EQ2;
-      for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
-        double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
-        if (dQ == 0.0) continue;
-        status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_ONR, isorp), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-        for (int i = 0; i < NBL; i++) acc[i] = acc[i] + dQ * X[i] * FB_P_EQ2;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < NBL; i++) v.mat[FB_MAT_VNAO][bo + i] = acc[i];
-    if (!self && !keep_vna) {
-#pragma unroll
-      for (int i = 0; i < NBL; i++) v.mat[FB_MAT_VNA][bo + i] = 0.0;
-    }
-    // atom term <i|V_j|i>
-    constexpr int NA = NMU * NMU;
-    status |= block2c<CLSA>(v, cell2c_id(v, in1, in2, SLOT_ATOM, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-    for (int i = 0; i < NA; i++) atom_c[i] = X[i] * FB_P_EQ2;
-    bool charged = false;
-    for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) charged |= (v.dQ[ja * FB_MAXSH + isorp - 1] != 0.0);
-    if (charged) {
-      const int64_t pself = v.noff[ia] + v.neigh_self[ia] - 1;
-      const double *sself = v.mat[FB_MAT_S] + v.boff[pself];
-      const double smooth = d_smoother(z, v.sp.rcmin[in1 - 1] + v.sp.rcmin[in2 - 1], 0.8);
-      for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
-        double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
-        status |= block2c<CLSA>(v, cell2c_id(v, in1, in2, SLOT_ATOM, isorp), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-        for (int i = 0; i < NA; i++) {
-          double emnpl = (z > 1.0e-4) ? sself[i] / z : 0.0;
-          const double t = FB_P_EQ2 * dQ * ((1.0 - smooth) * emnpl + smooth * X[i]);
-          if (KQ > 0 && isorp <= KQ) qrow[(isorp - 1) * QROW + i] = t;
-          else atom_c[i] = atom_c[i] + t;
-        }
-      }
-    }
-  }
-  return status;
-}
-
-// KQ: charged atom-term rows kept apart (0 for neutral builds, 2 = the most
-// shells of an s/p species, for charged ones)
+// s/p species with canonical tables (fb_2c_sp.h)
 template <int KQ>
 __global__ __launch_bounds__(WG2, 3) void k_2c_sp(DView v, int mask) {
-  __shared__ double s_at[1 + KQ][WG2][17];
-  const int tid = threadIdx.x;
-  const int a0 = v.wg2c[blockIdx.x], a1 = v.wg2c[blockIdx.x + 1];
-  const int64_t pb = v.noff[a0], pe = v.noff[a1];
-  // reduction slot of this thread: (atom a0 + tid/16, element tid%16)
-  const int ra = a0 + (tid >> 4), re = tid & 15;
-  const bool rown = ra < a1;
-  int64_t rp0 = 0, rp1 = 0;
-  int rna = 0;
-  if (rown) {
-    rp0 = v.noff[ra];
-    rp1 = v.noff[ra + 1];
-    rna = v.sp.norb[v.imass[ra] - 1];
-  }
-  double racc = 0.0;
-  int status = 0;
-  for (int64_t c0 = pb; c0 < pe; c0 += WG2) {
-    const int64_t p = c0 + tid;
-    double atom_c[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) atom_c[i] = 0.0;
-    if (KQ > 0 && (mask & 8))
-      for (int q = 1; q <= KQ; q++)
-#pragma unroll
-        for (int i = 0; i < 16; i++) s_at[q][tid][i] = 0.0;
-    if (p < pe) {
-      const int ia = v.pair_atom[p];
-      const int s = v.atom_struct[ia];
-      const DStruct St = v.st[s];
-      const int ja = St.atom_off + v.neigh_j[p] - 1;
-      const int mb = v.neigh_b[p];
-      const int in1 = v.imass[ia], in2 = v.imass[ja];
-      const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
-      double r1[3], r2[3], sighat[3], eps[9], pm[9];
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        r1[q] = v.ratom[3 * ia + q];
-        r2[q] = v.ratom[3 * ja + q] + xls[3 * mb + q];
-      }
-      const double z = d_distance(r1, r2);
-      if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
-      else {
-#pragma unroll
-        for (int q = 0; q < 3; q++) sighat[q] = (r2[q] - r1[q]) / z;
-      }
-      d_epsilon(r2, sighat, eps);
-      d_pmat<double>(eps, pm);
-      const bool self = (ja == ia && mb == 0);
-      const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
-      const int64_t bo = v.boff[p];
-      const bool keep = (mask & FB_2C_SKIP_TGT) && v.pair_tgt[p];
-      double *qrow = KQ > 0 ? &s_at[KQ > 0 ? 1 : 0][tid][0] : nullptr;
-      switch (cls) {
-        case 3: status |= pair2c<3, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow); break;
-        case 2: status |= pair2c<2, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow); break;
-        case 1: status |= pair2c<1, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow); break;
-        default: status |= pair2c<0, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow); break;
-      }
-    }
-    if (mask & 8) {
-#pragma unroll
-      for (int i = 0; i < 16; i++) s_at[0][tid][i] = atom_c[i];
-      __syncthreads();
-      if (rown && re < rna * rna) {
-        // each neighbour's terms one at a time, as the reference adds them
-        const int64_t k0 = max(rp0, c0), k1 = min(rp1, c0 + WG2);
-        for (int64_t k = k0; k < k1; k++) {
-          racc = racc + s_at[0][k - c0][re];
-#pragma unroll
-          for (int q = 1; q <= KQ; q++) racc = racc + s_at[q][k - c0][re];
-        }
-      }
-      __syncthreads();
-    }
-  }
-  if ((mask & 8) && rown && re < rna * rna) {
-    const int64_t pself = v.noff[ra] + v.neigh_self[ra] - 1;
-    v.mat[FB_MAT_VNA][v.boff[pself] + re] = racc;
-  }
-  if (status) atomicOr(v.status, status);
+  __shared__ S2c<KQ> S;
+  wg2c_sp<KQ>(v, mask, blockIdx.x, S);
 }
 
 extern "C" int fbk_2c_sp(const DView &v, int mask, hipStream_t st) {

@@ -33,6 +33,7 @@
 #include "fb_device.h"
 #include "fb_mlp.h"
 #include "fb_sp.h"
+#include "fb_2c_sp.h"
 
 #define NB (FB_MO * FB_MO)
 #define NME3 10
@@ -488,34 +489,37 @@ __global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
   if (status) atomicOr(v.status, status);
 }
 
-// learned surrogate with the stencil as fallback: persistent workgroups over a
-// queue of the 256-triplet tiles.  queue[x] counts the tiles taken from the
-// range of XCD x (tiles numbered contiguously per XCD, as xcd_tile numbers
-// them); a workgroup takes from its own XCD's range first (workgroup b runs on
-// XCD b mod 8), then from the others, and exits when all are empty.
+// Next tile of a persistent workgroup from per-XCD ranges of the 256-triplet
+// tiles: queue[x] counts the tiles taken from the range of XCD x (numbered
+// contiguously per XCD, as xcd_tile numbers them); the workgroup's own XCD
+// (b mod 8) first, then the others; -1 when all are empty.  Thread 0 only.
+__device__ __forceinline__ int next_tile(int *queue, unsigned ntiles) {
+  const unsigned per = ntiles >> 3, rem = ntiles & 7u, x0 = blockIdx.x & 7u;
+  for (unsigned k = 0; k < 8; k++) {
+    const unsigned x = (x0 + k) & 7u;
+    const unsigned lo = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+    const unsigned cnt = per + (x < rem ? 1u : 0u);
+    if (cnt == 0) continue;
+    const unsigned i = (unsigned)atomicAdd(&queue[x], 1);
+    if (i < cnt) return (int)(lo + i);
+  }
+  return -1;
+}
+
+// learned surrogate with the stencil as fallback: persistent workgroups over
+// the per-XCD queue of the 256-triplet tiles (next_tile); a workgroup exits
+// when all ranges are empty.
 __global__ __launch_bounds__(256, 4) void k_3c_mlp(DView v, int *queue) {
   __shared__ S3c<256, false, true> S;
   __shared__ int s_tile;
   __shared__ unsigned sCnt[2];
   const int tid = threadIdx.x;
   if (tid < 2) sCnt[tid] = 0u;
-  const unsigned nt = (unsigned)v.ntiles[2], per = nt >> 3, rem = nt & 7u;
-  const unsigned x0 = blockIdx.x & 7u;
+  const unsigned nt = (unsigned)v.ntiles[2];
   int status = 0;
   for (;;) {
     __syncthreads();
-    if (tid == 0) {
-      int t = -1;
-      for (unsigned k = 0; k < 8 && t < 0; k++) {
-        const unsigned x = (x0 + k) & 7u;
-        const unsigned lo = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
-        const unsigned cnt = per + (x < rem ? 1u : 0u);
-        if (cnt == 0) continue;
-        const unsigned i = (unsigned)atomicAdd(&queue[x], 1);
-        if (i < cnt) t = (int)(lo + i);
-      }
-      s_tile = t;
-    }
+    if (tid == 0) s_tile = next_tile(queue, nt);
     __syncthreads();
     const int t = s_tile;
     if (t < 0) break;
@@ -525,6 +529,36 @@ __global__ __launch_bounds__(256, 4) void k_3c_mlp(DView v, int *queue) {
     atomicAdd(&v.mlp_count[0], (unsigned long long)sCnt[0]);
     atomicAdd(&v.mlp_count[1], (unsigned long long)sCnt[1]);
   }
+  if (status) atomicOr(v.status, status);
+}
+
+// The whole neutral H+S build of s/p species in one launch: blocks
+// [0, n2p) are the 2c workgroup items of v.wg2c (S, T, dipole_z, the Vna
+// ontop and atom terms, fb_2c_sp.h; the Vna blocks of 3c targets are left to
+// the 3c tiles), n2p = v.nwg2c rounded up to a multiple of 8 so that the XCD
+// of a block stays blockIdx mod 8; the blocks after them are the 256-triplet
+// 3c tiles, numbered contiguously per XCD.  The 2c blocks are dispatched
+// first, so their long per-lane chains (one pair per lane, six interactions)
+// start at once and the 3c tiles fill the issue slots they leave idle.
+// Neutral builds only: charged 3c terms read the S and dipole_z blocks of the
+// 2c items.
+union SFused {
+  S2c<0> s2;
+  S3c<256, false, false> s3;
+};
+
+// W: waves per SIMD the register allocation targets (3: the 2c body's 159
+// VGPRs fit; 4: the 3c tiles' occupancy, the 2c body spills)
+template <int W>
+__global__ __launch_bounds__(256, W) void k_hs_fused(DView v, int n2p) {
+  __shared__ SFused S;
+  const int b = blockIdx.x;
+  if (b < n2p) {
+    if (b < v.nwg2c) wg2c_sp<0>(v, 15 | FB_2C_SKIP_TGT, b, S.s2);
+    return;
+  }
+  int status = 0;
+  tile3c<256, false, false>(v, v.tiles[2][xcd_tile(b - n2p, gridDim.x - n2p)], S.s3, nullptr, status);
   if (status) atomicOr(v.status, status);
 }
 
@@ -559,4 +593,16 @@ extern "C" int fbk_3c_mlp(const DView &v, int *queue, int nwg, hipStream_t st) {
 extern "C" int fbk_3c_mixed(const DView &v, int chunk, int charged, hipStream_t st) {
   if (v.ntrip == 0) return 0;
   return charged ? launch_mixed<true>(v, chunk, st) : launch_mixed<false>(v, chunk, st);
+}
+
+// waves: 3 or 4 per SIMD (the register allocation of k_hs_fused)
+extern "C" int fbk_hs_fused(const DView &v, int waves, hipStream_t st) {
+  if (v.nwg2c == 0 && v.ntiles[2] == 0) return 0;
+  const int n2p = (v.nwg2c + 7) / 8 * 8;
+  const dim3 grid((unsigned)(n2p + v.ntiles[2]));
+  if (waves == 3)
+    hipLaunchKernelGGL(k_hs_fused<3>, grid, dim3(256), 0, st, v, n2p);
+  else
+    hipLaunchKernelGGL(k_hs_fused<4>, grid, dim3(256), 0, st, v, n2p);
+  return (int)hipGetLastError();
 }

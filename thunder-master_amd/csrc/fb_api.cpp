@@ -62,6 +62,7 @@ int fbk_3c_fp64_sp(const DView &v, hipStream_t st);
 int fbk_gamma_fold(const DView &v, int a0, int natoms, int which, const int *orb_off, int n, double *A,
                    hipStream_t st);
 int fbk_3c_mlp(const DView &v, int *queue, int nwg, hipStream_t st);
+int fbk_hs_fused(const DView &v, int waves, hipStream_t st);
 int fbk_mlp_samples(const DView &v, int gid, int64_t n, unsigned long long seed, float *X, float *T, hipStream_t st);
 int fbk_mlp_colmaxabs(int64_t n, int ncol, const float *A, float *out, hipStream_t st);
 int fbk_mlp_rowmax(int64_t n, const float *T, float *M, hipStream_t st);
@@ -207,6 +208,7 @@ struct fb_ctx {
   int device = 0;
   float *scratch = nullptr;   // this device's scratch array (fb_mlp_gradients_noalloc)
   int ncu = 256;              // compute units of the device
+  int fused = 0;              // FB_TUNE_FUSED: 0 separate 2c/3c launches, 3 or 4 one persistent launch
   hipStream_t own_stream = nullptr, stream = nullptr;
   hipStream_t side = nullptr;          // second stream: 2c beside 3c in a build
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -510,13 +512,22 @@ struct fb_ctx {
     return decode_status(st, where);
   }
 
-  // result of the last asynchronous build, if one is outstanding
+  // result of the asynchronous builds enqueued since the last check, if any:
+  // their kernels OR their error bits into the status word, which stays set
+  // until read here and cleared
   int settle(const char *where) {
     if (!pending) return FB_OK;
-    hipError_t e = hipStreamSynchronize(stream);
-    if (e != hipSuccess) return fail(FB_E_HIP, where, e);
     pending = false;
-    return decode_status(*h_status, where);
+    hipError_t e = hipMemcpyAsync(h_status, d_status.p, sizeof(int), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return fail(FB_E_HIP, where, e);
+    const int st = *h_status;
+    if (st) {
+      e = hipMemsetAsync(d_status.p, 0, sizeof(int), stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+      if (e != hipSuccess) return fail(FB_E_HIP, where, e);
+    }
+    return decode_status(st, where);
   }
 
   int decode_status(int st, const char *where) {
@@ -668,6 +679,7 @@ int fb_create(fb_ctx **out, int device) {
   if (e != hipSuccess) return create_failed(ctx, "device attributes", e);
   ctx->stream = ctx->own_stream;
   e = ctx->d_status.ensure(sizeof(int) * 4);
+  if (e == hipSuccess) e = hipMemset(ctx->d_status.p, 0, sizeof(int) * 4);
   if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->h_status, sizeof(int) * 4);
   if (e != hipSuccess) return create_failed(ctx, "status buffers", e);
   ctx->h_status[0] = 0;
@@ -1688,8 +1700,24 @@ int fb_assemble_vna_3c(fb_ctx *ctx) {
 // read (all of 2c when neutral; the Vna part when charged, whose terms read
 // S and dipole_z) runs on a second stream beside the 3c launch; it then leaves
 // the Vna blocks of 3c targets to the 3c launch, which overwrites them.
+// the fused persistent launch serves neutral mixed-precision table builds of
+// s/p species
+static bool fused_build(const fb_ctx *ctx, int charged) {
+  return ctx->fused && !charged && ctx->canon_sp && ctx->precision == FB_PREC_MIXED &&
+         !(ctx->method3c == FB_3C_MLP && ctx->mlp_ready);
+}
+
+// The status word is not cleared here: a build only ORs error bits into it,
+// and settle() reads and clears it (so back-to-back builds need no host
+// round trip between them).
 static int enqueue_build(fb_ctx *ctx, int charged) {
-  RC(ctx->launch("clear_status", [&] { return fbk_fill_i32(ctx->d_status.as<int>(), 1, 0, ctx->stream); }));
+  if (fused_build(ctx, charged)) {
+    DView v = ctx->view();
+    RC(ctx->launch("assemble_hs_fused", [&] {
+      return fbk_hs_fused(v, ctx->fused, ctx->stream);
+    }));
+    return FB_OK;
+  }
   const bool fork = !ctx->timing && ctx->side != nullptr;
   if (charged) RC(run_2c(ctx, 7, "assemble_2c_STD"));
   if (fork) {
@@ -1703,28 +1731,30 @@ static int enqueue_build(fb_ctx *ctx, int charged) {
     RC(run_2c(ctx, charged ? 8 : 15, charged ? "assemble_2c_vna" : "assemble_2c"));
     RC(run_3c(ctx));
   }
-  FB_HIP(hipMemcpyAsync(ctx->h_status, ctx->d_status.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   return FB_OK;
 }
 
-// Asynchronous: returns once the build is enqueued.  A status raised by the
-// kernels (x beyond a table, missing table) is returned by the next
-// fb_synchronize, fb_get_blocks or assemble call.  Without per-kernel timing
-// the build is replayed from a captured HIP graph.
+// Asynchronous: returns once the build is enqueued, without waiting for an
+// earlier build.  A status raised by the kernels of the builds enqueued since
+// the last check (x beyond a table, missing table) is returned by the next
+// fb_synchronize, fb_get_blocks or other call that reads results.  Without
+// per-kernel timing the build is replayed from a captured HIP graph.
 int fb_assemble_all(fb_ctx *ctx) {
   if (!ctx) return FB_E_ARG;
   if (!ctx->lists_ok) return ctx->fail(FB_E_ORDER, "assemble_all before build_lists");
   FB_HIP(hipSetDevice(ctx->device));
-  RC(ctx->settle("previous assemble_all"));
-  if (ctx->structs_dirty) RC(upload_structures(ctx));
+  if (ctx->structs_dirty) {
+    RC(ctx->settle("previous assemble_all"));
+    RC(upload_structures(ctx));
+  }
   const int charged = any_charge(ctx);
   if (ctx->timing || !ctx->use_graph || ctx->stream == nullptr) {
     RC(enqueue_build(ctx, charged));
   } else {
     // the graph holds kernel arguments by value: any change of lists, precision,
     // charge state, chunk, 3c method or networks needs a new capture
-    const long key = (((((ctx->lists_gen * 4 + ctx->precision) * 2 + charged) * 512 + ctx->chunk3c) * 2 +
-                       ctx->method3c) * 4096) + (ctx->mlp_gen & 4095);
+    const long key = ((((((ctx->lists_gen * 4 + ctx->precision) * 2 + charged) * 512 + ctx->chunk3c) * 2 +
+                        ctx->method3c) * 8 + ctx->fused) * 4096) + (ctx->mlp_gen & 4095);
     if (ctx->graph_key != key || !ctx->gexec) {
       if (ctx->gexec) { (void)hipGraphExecDestroy(ctx->gexec); ctx->gexec = nullptr; }
       if (ctx->graph) { (void)hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
@@ -2982,6 +3012,11 @@ int fb_set_tuning(fb_ctx *ctx, int param, int value) {
     ctx->force_kernel = value;
     return FB_OK;
   }
+  if (param == FB_TUNE_FUSED) {
+    if (value != 0 && value != 3 && value != 4) return ctx->fail(FB_E_ARG, "fused must be 0, 3 or 4");
+    ctx->fused = value;
+    return FB_OK;
+  }
   return ctx->fail(FB_E_ARG, "unknown tuning parameter");
 }
 
@@ -3003,14 +3038,20 @@ int fb_get_tuning(fb_ctx *ctx, int param, int *value) {
     *value = ctx->force_kernel;
     return FB_OK;
   }
+  if (param == FB_TUNE_FUSED) {
+    *value = ctx->fused;
+    return FB_OK;
+  }
   return ctx->fail(FB_E_ARG, "unknown tuning parameter");
 }
 
-// Times the launch shapes of the kernels that the context's precision and
-// species use, on the current lists, and keeps the fastest.  The timed
-// launches leave the blocks half-built (a 2c candidate rewrites every Vna
-// block, 3c targets included), so the whole H+S build is run again with the
-// chosen shapes before returning: the blocks then hold a complete build.
+// Times whole H+S builds (enqueued directly, not from a graph) over every
+// launch shape that applies to the context and keeps the fastest: the fused
+// persistent launch at 3 and 4 waves per SIMD (neutral mixed-precision s/p
+// builds), and separate 2c and 3c launches with every 3c chunk (mixed s/p
+// builds; it sets the triplets and the LDS footprint of a workgroup) and both
+// s/p 2c kernels.  The last candidate leaves the blocks of its own build, so
+// the build is run once more with the chosen shape before returning.
 int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   if (!ctx) return FB_E_ARG;
   if (!ctx->lists_ok || !ctx->done[FB_MAT_S] || !ctx->done[FB_MAT_DIP] || !ctx->done[FB_MAT_VNA])
@@ -3028,64 +3069,57 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   } ev;
   FB_HIP(hipEventCreate(&ev.a));
   FB_HIP(hipEventCreate(&ev.b));
-  DView v = ctx->view();
   const int charged = any_charge(ctx);
-  // the chunk only shapes the mixed s/p 3c kernel; the binary64 and generic
-  // 3c kernels have no launch parameter to tune
-  // (the learned surrogate's launch has no chunk: it always takes the
-  // 256-triplet tiles)
-  const bool mlp = ctx->canon_sp && ctx->precision == FB_PREC_MIXED && ctx->method3c == FB_3C_MLP &&
-                   ctx->mlp_ready && !charged;
-  const bool mixed3c = ctx->canon_sp && ctx->precision == FB_PREC_MIXED && !mlp;
-  // one candidate: a warm-up launch, then nrep launches between two events
-  auto timed = [&](auto &&fn, float *ms) -> hipError_t {
-    hipError_t e = (hipError_t)fn();
-    if (e == hipSuccess) e = hipEventRecord(ev.a, ctx->stream);
-    for (int r = 0; r < nrep && e == hipSuccess; r++) e = (hipError_t)fn();
-    if (e == hipSuccess) e = hipEventRecord(ev.b, ctx->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(ev.b);
-    if (e == hipSuccess) e = hipEventElapsedTime(ms, ev.a, ev.b);
-    return e;
+  const bool timing = ctx->timing;
+  ctx->timing = false;   // enqueue_build then takes the forked two-stream form
+  struct Cfg {
+    int chunk, k2c, fused;
   };
-  int pick = ctx->chunk3c;
-  if (mixed3c) {
-    const int cand[3] = {64, 128, 256};
-    float best = 1e30f;
-    for (int c : cand) {
-      float ms = 0.f;
-      const hipError_t e = timed([&] { return fbk_3c_mixed(v, c, charged, ctx->stream); }, &ms);
-      if (e != hipSuccess) return ctx->fail(FB_E_HIP, "autotune 3c", e);
-      if (ms < best) {
-        best = ms;
-        pick = c;
-      }
+  std::vector<Cfg> cand;
+  const bool mixed_sp = ctx->canon_sp && ctx->precision == FB_PREC_MIXED;
+  const bool mlp = mixed_sp && ctx->method3c == FB_3C_MLP && ctx->mlp_ready && !charged;
+  if (mixed_sp && !charged && !mlp)
+    for (int w : {3, 4}) cand.push_back({ctx->chunk3c, ctx->k2c, w});
+  for (int c : {64, 128, 256}) {
+    if (!mixed_sp && c != ctx->chunk3c) continue;   // the chunk only shapes the mixed s/p 3c kernel
+    for (int k : {0, 1}) {
+      if (!ctx->canon_sp && k != ctx->k2c) continue;   // the s/p 2c kernels
+      cand.push_back({c, k, 0});
     }
   }
-  int pick2 = ctx->k2c;
-  if (ctx->canon_sp) {
-    float best2 = 1e30f;
-    for (int var = 0; var < 2; var++) {
-      float ms = 0.f;
-      const hipError_t e = timed([&] {
-        if (!charged) return launch_2c(ctx, v, 15, var, ctx->stream);
-        const int rc = launch_2c(ctx, v, 7, var, ctx->stream);
-        return rc ? rc : launch_2c(ctx, v, 8, var, ctx->stream);
-      }, &ms);
-      if (e != hipSuccess) return ctx->fail(FB_E_HIP, "autotune 2c", e);
-      if (ms < best2) {
-        best2 = ms;
-        pick2 = var;
-      }
+  Cfg best{ctx->chunk3c, ctx->k2c, ctx->fused};
+  float tbest = 1e30f;
+  hipError_t e = hipSuccess;
+  int rc = FB_OK;
+  for (const Cfg &c : cand) {
+    ctx->chunk3c = c.chunk;
+    ctx->k2c = c.k2c;
+    ctx->fused = c.fused;
+    rc = enqueue_build(ctx, charged);   // warm-up
+    if (rc == FB_OK) e = hipEventRecord(ev.a, ctx->stream);
+    for (int r = 0; r < nrep && rc == FB_OK && e == hipSuccess; r++) rc = enqueue_build(ctx, charged);
+    if (rc == FB_OK && e == hipSuccess) e = hipEventRecord(ev.b, ctx->stream);
+    if (rc == FB_OK && e == hipSuccess) e = hipEventSynchronize(ev.b);
+    float ms = 0.f;
+    if (rc == FB_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, ev.a, ev.b);
+    if (rc != FB_OK || e != hipSuccess) break;
+    if (ms < tbest) {
+      tbest = ms;
+      best = c;
     }
   }
-  ctx->chunk3c = pick;
-  ctx->k2c = pick2;
+  ctx->chunk3c = best.chunk;
+  ctx->k2c = best.k2c;
+  ctx->fused = best.fused;
   ctx->mlp_gen++;   // captured graphs hold the previous launch shapes
-  RC(enqueue_build(ctx, charged));
+  if (rc == FB_OK && e == hipSuccess) rc = enqueue_build(ctx, charged);
+  ctx->timing = timing;
+  if (e != hipSuccess) return ctx->fail(FB_E_HIP, "autotune", e);
+  RC(rc);
   ctx->pending = true;
   RC(ctx->settle("autotune"));
   for (int i = 0; i < 5; i++) ctx->done[i] = 1;
-  if (chosen) *chosen = pick;
+  if (chosen) *chosen = best.chunk;
   return FB_OK;
 }
 
