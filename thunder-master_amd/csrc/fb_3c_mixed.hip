@@ -8,14 +8,17 @@
 //   A  one lane per triplet: geometry.  Positions, bond and third-atom vectors,
 //      x = |r_na - bond centre| and y = bond length are binary64 with the
 //      reference's expressions, and so are the decisions taken on them: the
-//      grid cell (ix, iy) = int(x/dx)+1, int(y/dy)+1 and the method choice
+//      grid cell (ix, iy) = int(x/dx)+1, int(y/dy)+1, the method choice
 //      (gradient of ME 1 of theta cell 1 of each table against 1e-6, from the
-//      binary64 table).  Unit vectors and the frame epsilon(rhat, sighat) are
-//      binary64 with reciprocal multiplies; weights, Legendre P0..P4 and the
-//      p rotation go to LDS in FP32.
-//   B  one lane per (triplet, matrix element): the 16 (or 3) stencil points of
-//      the FP32 slab [iy][ix][ME][theta], one float4 + one float per point,
-//      Legendre-folded at each point, then the x/y Lagrange weights.
+//      binary64 table) and the thresholds of the frame.  The frame
+//      epsilon(rhat, sighat), cos theta and sin theta only feed FP32 values:
+//      they come from the binary64 cross and dot products of the bond and
+//      third-atom vectors, scaled in FP32; triplets whose threshold decisions
+//      the products cannot settle take the reference's binary64 steps.
+//   B  one lane per (triplet, quad of matrix elements): the 16 (or 3)
+//      stencil points of the FP32 quad slab [iy][ix][theta][ME quad], five
+//      float4 per point, Legendre-folded at each point, then the x/y
+//      Lagrange weights.
 //   C  one lane per triplet: sin(theta) on mvalue = 1 elements, Slater-Koster
 //      rotation (fb_sp.h) in FP32, times P_eq2, charged terms.
 //   D  one lane per (owned segment, block element): binary64 accumulation of
@@ -39,15 +42,44 @@
 #define NME3 10
 
 struct TPrm {
-  unsigned base;              // FP32 slab index (floats) of stencil corner (ix-1, iy-1), neutral group
-  unsigned rowstride;         // floats per iy row of the slab
-  unsigned char cls, nme, m1, pad_;   // pair class, ME count, method-1 flag
+  unsigned base;              // quad slab index (floats) of stencil corner (ix-1, iy-1), neutral group
+  unsigned rowstride;         // floats per iy row of the quad slab
+  unsigned char cls, nme, m1, nq;     // pair class, ME count, method-1 flag, ME quads
   float px, py;               // position in the cell; phase B forms the weights
   float P[4], sint;           // Legendre P1..P4 (P0 = 1)
   float pm[9];
 };
 // 19 words: an odd row stride, so lane-per-triplet accesses hit distinct banks
 static_assert(sizeof(TPrm) == 19 * 4, "TPrm row stride must stay odd in words");
+
+// the group table through the constant address space: uniform indices load
+// with scalar loads
+typedef const __attribute__((address_space(4))) DGroup3c CGroup3c;
+
+// The constants of group gid (per lane), by scalar loads: one pass per
+// distinct group of the wavefront (the group table is read-only during a
+// launch).  The empty asm uses the loaded fields inside the loop, where the
+// index is a scalar; without it the loads sink past the loop and address
+// with the per-lane index.
+__device__ __forceinline__ DGroup3c group_scalar(const DView &v, int gid) {
+#ifndef __HIP_DEVICE_COMPILE__
+  return v.grp3c[gid];   // the host pass only parses device code
+#else
+  const CGroup3c *cg = (const CGroup3c *)v.grp3c;
+  DGroup3c g;
+  for (;;) {
+    const int gc = __builtin_amdgcn_readfirstlane(gid);
+    const DGroup3c t = cg[gc];
+    asm volatile("" ::"s"(t.nME), "s"(t.nx), "s"(t.ny), "s"(t.cls), "s"(t.nq), "s"(t.xmax), "s"(t.ymax), "s"(t.dx),
+                 "s"(t.dy), "s"(t.rdx), "s"(t.rdy), "s"(t.off32q), "s"(t.moff));
+    if (gid == gc) {
+      g = t;
+      break;
+    }
+  }
+  return g;
+#endif
+}
 
 struct TChg {                 // charged third atom (DOGS terms)
   float dq[FB_MAXSH];
@@ -76,57 +108,144 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t slab_rsrc(const float *tab, in
   return __builtin_amdgcn_make_buffer_rsrc((void *)tab, (short)0, (int)(nfloats * 4), 0x00020000);
 }
 
-// Legendre fold of one (point, ME): sum_theta P_theta T_theta; off in bytes
-__device__ __forceinline__ float fold(__amdgpu_buffer_rsrc_t r, unsigned off, const float *P) {
-  const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
-  const float b = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(off + 16u), 0, 0));
-  float v = P[0] * a.x;
-  v = __builtin_fmaf(P[1], a.y, v);
-  v = __builtin_fmaf(P[2], a.z, v);
-  v = __builtin_fmaf(P[3], a.w, v);
-  return __builtin_fmaf(P[4], b, v);
+__device__ __forceinline__ float4 ld4(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
 }
 
-// one (triplet, ME): 16-point tensor stencil (method 3) or 3-point rule (method 1);
-// base and rowstride in floats
-template <int NME>
-__device__ __forceinline__ float stencil(__amdgpu_buffer_rsrc_t r, unsigned base, unsigned rowstride, int m,
-                                         const float *wx, const float *wy, const float *w1, const float *P, int m1) {
-  constexpr unsigned PT = NME * FB_TH32 * 4;   // bytes per ix step
-  const unsigned o = (base + (unsigned)m * FB_TH32) * 4u, rb = rowstride * 4u;
-  if (m1) {
-    const unsigned c = o + rb + PT;   // (ix, iy)
-    float v = w1[0] * fold(r, c, P);
-    v = __builtin_fmaf(w1[1], fold(r, c + PT, P), v);
-    return __builtin_fmaf(w1[2], fold(r, c + rb, P), v);
-  }
-  float acc = 0.f;
+// Legendre fold of one stencil point for one ME quad: sum_theta P_theta
+// T[theta][quad]; off in bytes, TS bytes per theta row of a point
+template <unsigned TS>
+__device__ __forceinline__ float4 fold4(__amdgpu_buffer_rsrc_t r, unsigned off, const float *P) {
+  float4 v = ld4(r, off);
 #pragma unroll
+  for (int t = 1; t < 5; t++) {
+    const float4 a = ld4(r, off + t * TS);
+    v.x = __builtin_fmaf(P[t], a.x, v.x);
+    v.y = __builtin_fmaf(P[t], a.y, v.y);
+    v.z = __builtin_fmaf(P[t], a.z, v.z);
+    v.w = __builtin_fmaf(P[t], a.w, v.w);
+  }
+  return v;
+}
+
+__device__ __forceinline__ float4 axpy4(float w, float4 a, float4 y) {
+  return make_float4(__builtin_fmaf(w, a.x, y.x), __builtin_fmaf(w, a.y, y.y), __builtin_fmaf(w, a.z, y.z),
+                     __builtin_fmaf(w, a.w, y.w));
+}
+
+// one (triplet, ME quad): 16-point tensor stencil (method 3) or 3-point rule
+// (M1, method 1); o: byte offset of the quad at stencil corner (ix-1, iy-1),
+// rb: bytes per iy row; NQ quads per point.  Offsets within a row are
+// immediates of the loads.
+template <int NQ, bool M1>
+__device__ __forceinline__ float4 stencil4(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned rb, float px, float py,
+                                           const float *P) {
+  constexpr unsigned TS = 16 * NQ, PT = 5 * TS;   // bytes per theta row, per ix step
+  if (M1) {
+    const unsigned c = o + rb + PT;   // (ix, iy)
+    const float4 f0 = fold4<TS>(r, c, P), f1 = fold4<TS>(r, c + PT, P), f2 = fold4<TS>(r, c + rb, P);
+    const float w0 = (1.f - px) - py;
+    return axpy4(py, f2, axpy4(px, f1, make_float4(w0 * f0.x, w0 * f0.y, w0 * f0.z, w0 * f0.w)));
+  }
+  float wx[4], wy[4];
+  cubic_w(px, wx);
+  cubic_w(py, wy);
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int j = 0; j < 4; j++) {
     const unsigned row = o + (unsigned)j * rb;
-    float g = wx[0] * fold(r, row, P);
-    g = __builtin_fmaf(wx[1], fold(r, row + PT, P), g);
-    g = __builtin_fmaf(wx[2], fold(r, row + 2 * PT, P), g);
-    g = __builtin_fmaf(wx[3], fold(r, row + 3 * PT, P), g);
-    acc = __builtin_fmaf(wy[j], g, acc);
+    const float4 g0 = fold4<TS>(r, row, P), g1 = fold4<TS>(r, row + PT, P);
+    float4 g = make_float4(wx[0] * g0.x, wx[0] * g0.y, wx[0] * g0.z, wx[0] * g0.w);
+    g = axpy4(wx[1], g1, g);
+    const float4 g2 = fold4<TS>(r, row + 2 * PT, P), g3 = fold4<TS>(r, row + 3 * PT, P);
+    g = axpy4(wx[2], g2, g);
+    g = axpy4(wx[3], g3, g);
+    acc = axpy4(wy[j], g, acc);
   }
   return acc;
 }
+
+template <int NQ>
+__device__ __forceinline__ float4 stencil4(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned rb, float px, float py,
+                                           const float *P, int m1) {
+  return m1 ? stencil4<NQ, true>(r, o, rb, px, py, P) : stencil4<NQ, false>(r, o, rb, px, py, P);
+}
+
+// One point (ix-2+i, iy-2+j) of the 16-point stencil times its weights,
+// wx_i wy_j fold(i, j); o: byte offset of the quad at the stencil corner
+template <int NQ>
+__device__ __forceinline__ float4 stencil_point(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned rb, float px, float py,
+                                                int i, int j, const float *P) {
+  constexpr unsigned TS = 16 * NQ, PT = 5 * TS;
+  float wx[4], wy[4];
+  cubic_w(px, wx);
+  cubic_w(py, wy);
+  const float w = (i == 0 ? wx[0] : (i == 1 ? wx[1] : (i == 2 ? wx[2] : wx[3]))) *
+                  (j == 0 ? wy[0] : (j == 1 ? wy[1] : (j == 2 ? wy[2] : wy[3])));
+  const float4 f = fold4<TS>(r, o + (unsigned)j * rb + (unsigned)i * PT, P);
+  return make_float4(w * f.x, w * f.y, w * f.z, w * f.w);
+}
+
+// sum over the 16 lanes 16m..16m+15 of the wavefront, every lane receiving it
+__device__ __forceinline__ float sum16(float v) {
+  v += __shfl_xor(v, 1);
+  v += __shfl_xor(v, 2);
+  v += __shfl_xor(v, 4);
+  return v + __shfl_xor(v, 8);
+}
+__device__ __forceinline__ float4 sum16(float4 v) { return make_float4(sum16(v.x), sum16(v.y), sum16(v.z), sum16(v.w)); }
 
 // LDS of one tile pass
 template <int CH, bool CHARGED, bool MLP>
 struct S3c {
   TPrm sq[CH];
-  float sF[CH][NME3];
+  float sF[CH][12];                   // molecular-frame MEs, whole quads
   TChg sx[CHARGED ? CH : 1];
   int sSeg[CH + 1];                   // triplet offsets (below 2^31: the list build checks)
   int64_t sBo[CH];
   unsigned char sNb[CH];
   float sU[MLP ? CH : 1][4];          // network inputs and envelope
-  int sG[MLP ? CH : 1];               // network group of the triplet, -1: table stencil
-  short sNet[MLP ? CH : 1], sFb[MLP ? CH : 1];   // chunk triplets on the network / on the stencil, in order
-  int sWn[MLP ? CH / 64 : 1][2];      // per-wavefront counts of both lists
+  short sG[MLP ? CH : 1];             // network group of the triplet, -1: table stencil
+  short sNet[MLP ? CH : 1];           // chunk triplets on the network, in order
+  short sFb[CH];                      // chunk triplets on the stencil, in four classes (see tile3c)
+  int sWn[CH / 64][5];                // per-wavefront counts of the network list and the four classes
 };
+
+// Phase B over one class of the stencil list, entries [kb, kb + n) of sFb,
+// NQ lanes per triplet (one per ME quad).  M1, the 3-point rule: one lane per
+// quad.  The 16-point stencil: 16 lanes per quad, one per stencil point,
+// whose weighted folds are summed across the lanes (a point's five loads
+// are then all the latency a lane waits for).  Charged subtypes choose
+// their method per table.
+template <int NQ, bool M1, int CH, bool CHARGED, bool MLP>
+__device__ __forceinline__ void interp_class(S3c<CH, CHARGED, MLP> &S, __amdgpu_buffer_rsrc_t tab, int kb, int n) {
+  constexpr int R = M1 ? 1 : 16;   // lanes per (triplet, quad)
+  for (int it = threadIdx.x; it < R * NQ * n; it += CH) {
+    const int kq = it / R, p = it - R * kq;
+    const int k = kq / NQ, qd = kq - NQ * k;
+    const int t = S.sFb[kb + k];
+    const TPrm &q = S.sq[t];
+    const float P[5] = {1.f, q.P[0], q.P[1], q.P[2], q.P[3]};
+    const float px = q.px, py = q.py;
+    const unsigned o = (q.base + 4u * (unsigned)qd) * 4u, rb = q.rowstride * 4u;
+    float4 F = M1 ? stencil4<NQ, true>(tab, o, rb, px, py, P)
+                  : sum16(stencil_point<NQ>(tab, o, rb, px, py, p & 3, p >> 2, P));
+    if (CHARGED) {
+      const TChg &c = S.sx[t];
+      for (int kk = 0; kk < FB_MAXSH; kk++) {
+        const float dq = c.dq[kk];
+        if (dq == 0.f) continue;
+        const unsigned ok = o + (unsigned)c.gk[kk] * 4u;
+        float4 Fk;
+        if (M1) Fk = stencil4<NQ>(tab, ok, rb, px, py, P, c.m1k[kk]);
+        // all 16 lanes of the quad take the same branch: the subtype's method is per triplet
+        else if (c.m1k[kk]) Fk = stencil4<NQ, true>(tab, ok, rb, px, py, P);
+        else Fk = sum16(stencil_point<NQ>(tab, ok, rb, px, py, p & 3, p >> 2, P));
+        F = axpy4(dq * c.stinky, Fk, F);
+      }
+    }
+    if (p == 0) *reinterpret_cast<float4 *>(&S.sF[t][4 * qd]) = F;
+  }
+}
 
 // One tile of whole target segments (first, last, r0, r1), in chunks of at
 // most CH triplets.  MLP: the learned surrogate replaces the stencil of phase
@@ -152,7 +271,7 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
     S.sNb[k] = (unsigned char)(v.boff[p + 1] - b0);
   }
   double *const vna = v.mat[FB_MAT_VNA];
-  const __amdgpu_buffer_rsrc_t tab = slab_rsrc(v.tab3c32, v.ntab3c32);
+  const __amdgpu_buffer_rsrc_t tab = slab_rsrc(v.tab3c32q, v.ntab3c32q);
 
   for (int64_t c0 = r0; c0 < r1; c0 += CH) {
     const int nt = (int)min((int64_t)CH, r1 - c0);
@@ -174,31 +293,71 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
         rna[k] = v.ratom[3 * ial + k];
         r21[k] = r2v[k] - r1v[k];
       }
-      const int in1 = v.imass[gia], in2 = v.imass[gja], indna = v.imass[ial];
-      q.cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
-      const double z = d_distance(r1v, r2v);
-      if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
-      else {
-        const double iz = 1.0 / z;
-#pragma unroll
-        for (int k = 0; k < 3; k++) sighat[k] = r21[k] * iz;
+      int gid = aux & 0xffff;
+      if (gid == 0xffff) gid = -1;
+      if (gid < 0) {
+        status |= ST_NO_TABLE;
+        gid = 0;
       }
+      const DGroup3c g = group_scalar(v, gid);
+      q.cls = (unsigned char)g.cls;
+      q.nme = (unsigned char)g.nME;
+      q.nq = (unsigned char)g.nq;
+      const double z = d_distance(r1v, r2v);
 #pragma unroll
       for (int k = 0; k < 3; k++) rnabc[k] = rna[k] - (r1v[k] + r21[k] * 0.5);
       const double x = sqrt(rnabc[0] * rnabc[0] + rnabc[1] * rnabc[1] + rnabc[2] * rnabc[2]);
-      if (x < 1.0e-5) { rhat[0] = 0.0; rhat[1] = 0.0; rhat[2] = 0.0; }
-      else {
-        const double ix_ = 1.0 / x;
+      // |sighat x rhat| = |c| / (z x) with c = r21 x rnabc, and cos theta =
+      // (r21 . rnabc) / (z x).  epsilon's collinearity branch (|sighat x
+      // rhat| < 1e-4) is c.c < 1e-8 (z x)^2; within a relative 1e-6 of that
+      // bound, and when z or x is below the 1e-5 of the unit vectors, the
+      // reference's own binary64 steps decide.
+      double cr[3];
+      d_cross(r21, rnabc, cr);
+      const double c2 = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];
+      const double zx = z * x, lim = 1.0e-8 * (zx * zx);
+      const bool fast = z >= 1.0e-5 && x >= 1.0e-5 && c2 > lim && c2 - lim > 1.0e-6 * lim;
+      float eps[9], cosf_, sint;
+      if (fast) {
+        const float izx = __builtin_amdgcn_rcpf((float)zx), iz = __builtin_amdgcn_rcpf((float)z);
+        const float ic = __builtin_amdgcn_rsqf((float)c2);
+        const float dt = (float)(r21[0] * rnabc[0] + r21[1] * rnabc[1] + r21[2] * rnabc[2]);
+        float zp[3], yp[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) rhat[k] = rnabc[k] * ix_;
-      }
-      const double cost = sighat[0] * rhat[0] + sighat[1] * rhat[1] + sighat[2] * rhat[2];
-      {
+        for (int k = 0; k < 3; k++) {
+          zp[k] = (float)r21[k] * iz;
+          yp[k] = (float)cr[k] * ic;
+        }
+        eps[0] = yp[1] * zp[2] - yp[2] * zp[1];
+        eps[1] = yp[2] * zp[0] - yp[0] * zp[2];
+        eps[2] = yp[0] * zp[1] - yp[1] * zp[0];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          eps[3 + k] = yp[k];
+          eps[6 + k] = zp[k];
+        }
+        cosf_ = dt * izx;
+        // sqrt(max(1e-5, 1 - cos^2)) of M_Fdata_3c.f90:349 as max(sqrt(1e-5), |sin|)
+        sint = fmaxf(__builtin_amdgcn_sqrtf((float)c2) * izx, 3.16227766e-3f);
+      } else {
+        if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+        else {
+          const double iz_ = 1.0 / z;
+#pragma unroll
+          for (int k = 0; k < 3; k++) sighat[k] = r21[k] * iz_;
+        }
+        if (x < 1.0e-5) { rhat[0] = 0.0; rhat[1] = 0.0; rhat[2] = 0.0; }
+        else {
+          const double ix_ = 1.0 / x;
+#pragma unroll
+          for (int k = 0; k < 3; k++) rhat[k] = rnabc[k] * ix_;
+        }
+        const double cost = sighat[0] * rhat[0] + sighat[1] * rhat[1] + sighat[2] * rhat[2];
         // epsilon(rhat, sighat), b.FUNCTIONS/epsilon.f90:120-177, with the
         // reference's thresholds (sighat is a unit vector, so the r2 = 0 branch
-        // cannot occur)
-        // sighat and rhat are unit vectors already (rhat = 0 when x < 1e-5),
-        // so the normalisations of epsilon.f90:136-145 change them by rounding only
+        // cannot occur); sighat and rhat are unit vectors already (rhat = 0
+        // when x < 1e-5), so the normalisations of epsilon.f90:136-145 change
+        // them by rounding only
         double zp[3] = {sighat[0], sighat[1], sighat[2]}, yp[3] = {0.0, 0.0, 0.0}, xp[3];
         double ypm = 0.0;
         if (x >= 1.0e-5) {
@@ -215,29 +374,41 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
 #pragma unroll
         for (int k = 0; k < 3; k++) yp[k] *= iy_;
         d_cross(yp, zp, xp);
-        const double eps[9] = {xp[0], xp[1], xp[2], yp[0], yp[1], yp[2], zp[0], zp[1], zp[2]};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          eps[k] = (float)xp[k];
+          eps[3 + k] = (float)yp[k];
+          eps[6 + k] = (float)zp[k];
+        }
+        cosf_ = (float)cost;
+        sint = (float)sqrt(fmax(1.0e-5, 1.0 - cost * cost));
+      }
+      {
         float pmf[9];
         d_pmat<float>(eps, pmf);
 #pragma unroll
         for (int k = 0; k < 9; k++) q.pm[k] = pmf[k];
       }
-      const double cost2 = cost * cost;
-      q.sint = (float)sqrt(fmax(1.0e-5, 1.0 - cost2));
-      q.P[0] = (float)cost;
-      q.P[1] = (float)(0.5 * (3.0 * cost2 - 1.0));
-      q.P[2] = (float)(0.5 * cost * (5.0 * cost2 - 3.0));
-      q.P[3] = (float)(0.125 * (3.0 - cost2 * (30.0 - 35.0 * cost2)));
-      int gid = aux & 0xffff;
-      if (gid == 0xffff) gid = -1;
-      if (gid < 0) {
-        status |= ST_NO_TABLE;
-        gid = 0;
+      const float cost = cosf_, cost2 = cost * cost;
+      q.sint = sint;
+      q.P[0] = cost;
+      q.P[1] = 0.5f * (3.f * cost2 - 1.f);
+      q.P[2] = 0.5f * cost * (5.f * cost2 - 3.f);
+      q.P[3] = 0.125f * (3.f - cost2 * (30.f - 35.f * cost2));
+      // grid cell and method, M_Fdata_3c.f90:367-385: ix = int(x/dx) + 1.
+      // x (1/dx) lies within 4e-16 relative of x/dx, so its integer part is
+      // that of the quotient unless it falls within 1e-12 of an integer; then
+      // the division is taken.
+      double qx = x * g.rdx, qy = z * g.rdy;
+      {
+        const double fx = qx - floor(qx), fy = qy - floor(qy);
+        const bool nx_ = fx < 1.0e-12 || fx > 1.0 - 1.0e-12, ny_ = fy < 1.0e-12 || fy > 1.0 - 1.0e-12;
+        // a branch of the wavefront (not a select): the quotients are rare
+        if (__builtin_amdgcn_ballot_w64(nx_ || ny_) != 0) {
+          if (nx_) qx = x / g.dx;
+          if (ny_) qy = z / g.dy;
+        }
       }
-      const DGroup3c g = v.grp3c[gid];
-      q.nme = g.nME;
-      // grid cell and method, binary64 as M_Fdata_3c.f90:367-385
-      const double dx = g.xmax / (g.nx - 1.0), dy = g.ymax / (g.ny - 1.0);
-      const double qx = x / dx, qy = z / dy;
       int ix = (int)qx + 1, iy = (int)qy + 1;
       if (ix < 2) ix = 2;
       if (iy < 2) iy = 2;
@@ -250,8 +421,8 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
       q.m1 = v.meth3c[g.moff + icell];
       q.px = (float)px;
       q.py = (float)py;
-      q.rowstride = (unsigned)(g.nx * g.nME * FB_TH32);
-      q.base = (unsigned)(g.off32 + ((int64_t)(iy - 2) * g.nx + (ix - 2)) * g.nME * FB_TH32);
+      q.rowstride = (unsigned)(g.nx * 20 * g.nq);
+      q.base = (unsigned)(g.off32q + ((int64_t)(iy - 2) * g.nx + (ix - 2)) * 20 * g.nq);
       if (MLP) {
         // per-triplet bound: P_eq2 x the calibrated Frobenius error of the
         // sub-cell's molecular-frame block (bounds every rotated element),
@@ -264,13 +435,14 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
         }
         S.sU[tid][0] = (float)(2.0 * x / g.xmax - 1.0);
         S.sU[tid][1] = (float)(2.0 * z / g.ymax - 1.0);
-        S.sU[tid][2] = (float)cost;
+        S.sU[tid][2] = cost;
         S.sU[tid][3] = ok ? mlp_envelope(w, (float)x, (float)z) : 0.f;
-        S.sG[tid] = ok ? gid : -1;
+        S.sG[tid] = (short)(ok ? gid : -1);
         atomicAdd(&sCnt[0], ok ? 1u : 0u);
         atomicAdd(&sCnt[1], 1u);
       }
       if (CHARGED) {
+        const int in1 = v.imass[gia], in2 = v.imass[gja], indna = v.imass[ial];
         TChg &c = S.sx[tid];
         c.dqs = 0.f;
 #pragma unroll
@@ -286,7 +458,7 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
           const DGroup3c gq = v.grp3c[gk];
           c.dq[isorp - 1] = (float)dq;
           c.dqs += (float)dq;
-          c.gk[isorp - 1] = (int)(gq.off32 - g.off32);
+          c.gk[isorp - 1] = (int)(gq.off32q - g.off32q);
           // getMEs_Fdata_3c chooses the method from each table's own gradient
           c.m1k[isorp - 1] = v.meth3c[gq.moff + icell];
         }
@@ -300,76 +472,63 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
     } else if (MLP && tid < CH) {
       S.sG[tid] = -1;   // lanes past the chunk end take no network pass
     }
-    // the chunk's triplets split into a network list and a stencil list, each
+    // The chunk's triplets split into a network list and a stencil list, each
     // dense and in triplet order, so that each phase costs in proportion to
-    // its share
-    int nnet = 0, nfb = nt;
-    if (MLP) {
+    // its share.  The stencil list holds four classes one after another:
+    // 3-point cells with 3 ME quads, then with 1, then 16-point cells with 3
+    // and with 1.  Phase B then gives each triplet as many lanes as it has
+    // quads, and a wavefront rarely mixes a 16-point cell (five times the
+    // work) or the other quad count into its lanes.
+    int nnet = 0, nc[4] = {0, 0, 0, 0};
+    {
       const int wv = tid >> 6, l = tid & 63;
-      const bool on = tid < nt && S.sG[tid] >= 0, fb = tid < nt && !on;
-      const unsigned long long bn = __ballot(on), bf = __ballot(fb);
-      if (l == 0) {
-        S.sWn[wv][0] = __popcll(bn);
-        S.sWn[wv][1] = __popcll(bf);
-      }
-      __syncthreads();
-      int pn = 0, pf = 0;
-      nnet = 0;
-      nfb = 0;
+      const bool on = MLP && tid < nt && S.sG[tid] >= 0, st = tid < nt && !on;
+      const int cl = st ? (S.sq[tid].m1 ? 0 : 2) + (S.sq[tid].nq == 3 ? 0 : 1) : -1;
+      const unsigned long long bn = __ballot(on);
+      unsigned long long bc[4];
 #pragma unroll
-      for (int w = 0; w < CH / 64; w++) {
-        if (w < wv) {
-          pn += S.sWn[w][0];
-          pf += S.sWn[w][1];
+      for (int c = 0; c < 4; c++) bc[c] = __ballot(cl == c);
+      int pn = 0, pc[4] = {0, 0, 0, 0};
+      if (CH == 64) {
+        nnet = __popcll(bn);
+#pragma unroll
+        for (int c = 0; c < 4; c++) nc[c] = __popcll(bc[c]);
+      } else {
+        if (l == 0) {
+          S.sWn[wv][0] = __popcll(bn);
+#pragma unroll
+          for (int c = 0; c < 4; c++) S.sWn[wv][1 + c] = __popcll(bc[c]);
         }
-        nnet += S.sWn[w][0];
-        nfb += S.sWn[w][1];
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < CH / 64; w++) {
+          if (w < wv) {
+            pn += S.sWn[w][0];
+#pragma unroll
+            for (int c = 0; c < 4; c++) pc[c] += S.sWn[w][1 + c];
+          }
+          nnet += S.sWn[w][0];
+#pragma unroll
+          for (int c = 0; c < 4; c++) nc[c] += S.sWn[w][1 + c];
+        }
       }
       const unsigned long long below = (l == 0) ? 0ull : (~0ull >> (64 - l));
       if (on) S.sNet[pn + __popcll(bn & below)] = (short)tid;
-      if (fb) S.sFb[pf + __popcll(bf & below)] = (short)tid;
+      if (cl >= 0) {
+        int base = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+          if (c < cl) base += nc[c];
+        S.sFb[base + pc[cl] + __popcll(bc[cl] & below)] = (short)tid;
+      }
     }
     __syncthreads();
-    // ---------------- phase B: interpolation, one lane per (triplet, ME)
-    for (int it = tid; it < nfb * NME3; it += CH) {
-      const int k = it / NME3, m = it - k * NME3;
-      const int t = MLP ? (int)S.sFb[k] : k;
-      const TPrm &q = S.sq[t];
-      if (m >= q.nme) continue;
-      float wx[4], wy[4], w1[3], P[5];
-      const float px = q.px, py = q.py;
-      cubic_w(px, wx);
-      cubic_w(py, wy);
-      w1[0] = (1.f - px) - py;
-      w1[1] = px;
-      w1[2] = py;
-#pragma unroll
-      for (int k = 0; k < 4; k++) P[k + 1] = q.P[k];
-      P[0] = 1.f;
-      const unsigned base = q.base, rs = q.rowstride;
-      float F;
-      switch (q.nme) {
-        case 10: F = stencil<10>(tab, base, rs, m, wx, wy, w1, P, q.m1); break;
-        case 3: F = stencil<3>(tab, base, rs, m, wx, wy, w1, P, q.m1); break;
-        default: F = stencil<1>(tab, base, rs, m, wx, wy, w1, P, q.m1); break;
-      }
-      if (CHARGED) {
-        const TChg &c = S.sx[t];
-        for (int k = 0; k < FB_MAXSH; k++) {
-          const float dq = c.dq[k];
-          if (dq == 0.f) continue;
-          const unsigned bk = base + (unsigned)c.gk[k];
-          float Fk;
-          switch (q.nme) {
-            case 10: Fk = stencil<10>(tab, bk, rs, m, wx, wy, w1, P, c.m1k[k]); break;
-            case 3: Fk = stencil<3>(tab, bk, rs, m, wx, wy, w1, P, c.m1k[k]); break;
-            default: Fk = stencil<1>(tab, bk, rs, m, wx, wy, w1, P, c.m1k[k]); break;
-          }
-          F = __builtin_fmaf(dq * c.stinky, Fk, F);
-        }
-      }
-      S.sF[t][m] = F;
-    }
+    // ---------------- phase B: interpolation, one lane per (triplet, ME
+    // quad), the four classes of the stencil list one after another
+    interp_class<3, true>(S, tab, 0, nc[0]);
+    interp_class<1, true>(S, tab, nc[0], nc[1]);
+    interp_class<3, false>(S, tab, nc[0] + nc[1], nc[2]);
+    interp_class<1, false>(S, tab, nc[0] + nc[1] + nc[2], nc[3]);
     if (MLP) {
       // keeps the weight loads of phase M from being hoisted into the stencil
       // loop (their registers would overlap)
@@ -460,19 +619,17 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
     // (owned segment, block element), segment order preserved: the sum starts
     // from zero in the segment's first chunk and continues from the block
     // (written by this lane in the previous chunk) in a later one
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      const int sl = tid + j * CH;
-      if (sl < nown * NB) {
-        const int sg = sl / NB, e = sl - sg * NB;
-        const int64_t s0 = S.sSeg[sg];
-        const int64_t k0 = max(s0, c0), k1 = min((int64_t)S.sSeg[sg + 1], c0 + nt);
-        if (k0 < k1 && e < S.sNb[sg]) {
-          double *const o = vna + S.sBo[sg] + e;
-          double a = (s0 < c0) ? *o : 0.0;
-          for (int64_t k = k0; k < k1; k++) a += (double)sC[k - c0][e];
-          *o = a;
-        }
+    for (int sl = tid; sl < nown * NB; sl += CH) {
+      const int sg = sl / NB, e = sl - sg * NB;
+      // the segment's triplets relative to the chunk start
+      const int s0 = (int)(S.sSeg[sg] - c0), s1 = (int)(S.sSeg[sg + 1] - c0);
+      const int k0 = max(s0, 0), k1 = min(s1, nt);
+      if (k0 < k1 && e < S.sNb[sg]) {
+        double *const o = vna + S.sBo[sg] + e;
+        double a = (s0 < 0) ? *o : 0.0;
+#pragma unroll 4
+        for (int k = k0; k < k1; k++) a += (double)sC[k][e];
+        *o = a;
       }
     }
     __syncthreads();

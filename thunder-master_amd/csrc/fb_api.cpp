@@ -208,7 +208,7 @@ struct fb_ctx {
   int device = 0;
   float *scratch = nullptr;   // this device's scratch array (fb_mlp_gradients_noalloc)
   int ncu = 256;              // compute units of the device
-  int fused = 0;              // FB_TUNE_FUSED: 0 separate 2c/3c launches, 3 or 4 one persistent launch
+  int fused = 0;              // FB_TUNE_FUSED: 0 separate 2c/3c launches, 3 or 4 one launch (waves per SIMD)
   hipStream_t own_stream = nullptr, stream = nullptr;
   hipStream_t side = nullptr;          // second stream: 2c beside 3c in a build
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -226,7 +226,8 @@ struct fb_ctx {
   bool tables_ok = false;
   DevBuf d_idx3r;   // McWEDA density groups (rho_3c, rhoS_3c)
   int maxsh = 1;    // largest shell count of any species
-  DevBuf d_cell2c, d_idx2c, d_grp3c, d_idx3c, d_tab2c, d_tab2c32, d_tab3c, d_tab3c32, d_mu, d_nu, d_mv, d_meth3c;
+  DevBuf d_cell2c, d_idx2c, d_grp3c, d_idx3c, d_tab2c, d_tab2c32, d_tab3c, d_tab3c32, d_tab3c32q, d_mu, d_nu, d_mv,
+      d_meth3c;
   DevBuf d_grid2c, d_pden, d_wg2c;
   DevBuf d_cell_cnt, d_cell_start, d_cell_atoms, d_atom_cell;   // cell lists of the neighbour search
   DevBuf d_pair_disp;   // [npairs][3] neighbour displacements of the common-neighbour search
@@ -236,7 +237,7 @@ struct fb_ctx {
   int canon_sp = 0;
   int canon_rho = 0;      // s/p species with density tables in make_munu / make_munuS order
   int ncell2c = 0, ngrid2c = 0;
-  int64_t ntab3c32 = 0;
+  int64_t ntab3c32 = 0, ntab3c32q = 0;
   int nwg2c = 0;
 
   std::vector<HStruct> structs;
@@ -408,6 +409,8 @@ struct fb_ctx {
     v.tab3c32 = d_tab3c32.as<float>();
     v.meth3c = d_meth3c.as<unsigned char>();
     v.ntab3c32 = ntab3c32;
+    v.tab3c32q = d_tab3c32q.as<float>();
+    v.ntab3c32q = ntab3c32q;
     v.mu = d_mu.as<int>();
     v.nu = d_nu.as<int>();
     v.mv = d_mv.as<int>();
@@ -550,7 +553,7 @@ struct fb_ctx {
   }
 
   ~fb_ctx() {
-    DevBuf *bufs[] = {&d_cell2c, &d_idx2c, &d_grp3c, &d_idx3c, &d_tab2c, &d_tab2c32, &d_tab3c, &d_tab3c32,
+    DevBuf *bufs[] = {&d_cell2c, &d_idx2c, &d_grp3c, &d_idx3c, &d_tab2c, &d_tab2c32, &d_tab3c, &d_tab3c32, &d_tab3c32q,
                       &d_mu, &d_nu, &d_mv, &d_grid2c, &d_pden, &d_wg2c, &d_cell_cnt, &d_cell_start, &d_cell_atoms, &d_atom_cell, &d_pair_disp, &d_tile_cnt, &d_tile_off, &d_gidx, &d_gout, &d_st, &d_xl, &d_ratom, &d_imass, &d_atom_struct, &d_dQ,
                       &d_neighn, &d_noff, &d_neigh_j, &d_neigh_b, &d_neigh_back, &d_neigh_self, &d_pair_atom,
                       &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
@@ -929,7 +932,7 @@ int fb_commit_tables(fb_ctx *ctx) {
       }
   };
   std::vector<double> tab3;
-  std::vector<float> tab3f;
+  std::vector<float> tab3f, tab3q;
   for (auto &kv : groups) {
     const HCell3c *th[FB_NTHETA] = {nullptr};
     for (auto *c : kv.second) th[c->itheta - 1] = c;
@@ -954,8 +957,13 @@ int fb_commit_tables(fb_ctx *ctx) {
     g.ymax = c1->ymax;
     g.dx = c1->xmax / (c1->nx - 1.0);  // e.FDATA/M_Fdata_3c.f90:367-368
     g.dy = c1->ymax / (c1->ny - 1.0);
+    g.rdx = 1.0 / g.dx;
+    g.rdy = 1.0 / g.dy;
+    g.cls = (ctx->sp.norb[c1->is - 1] == 4 ? 2 : 0) + (ctx->sp.norb[c1->js - 1] == 4 ? 1 : 0);
+    g.nq = (c1->nME + 3) / 4;
     g.off = (int64_t)tab3.size();
     g.off32 = (int64_t)tab3f.size();
+    g.off32q = (int64_t)tab3q.size();
     // the reference recovers the block with the mu/nu/mvalue of the last theta cell
     const HCell3c *cl = th[FB_NTHETA - 1];
     g.munu = (int)mu.size();
@@ -972,6 +980,14 @@ int fb_commit_tables(fb_ctx *ctx) {
     for (size_t q = 0; q < plane; q++)
       for (int m = 0; m < g.nME; m++)
         for (int t = 0; t < FB_NTHETA; t++) dst[(q * g.nME + m) * FB_TH32 + t] = (float)th[t]->data[q * g.nME + m];
+    // FP32 quad slab [iy][ix][theta][4 nq]: one float4 holds four MEs of one
+    // theta cell at one point
+    const size_t qrow = (size_t)4 * g.nq;
+    tab3q.resize(tab3q.size() + plane * FB_NTHETA * qrow, 0.0f);
+    float *dq = tab3q.data() + g.off32q;
+    for (size_t q = 0; q < plane; q++)
+      for (int t = 0; t < FB_NTHETA; t++)
+        for (int m = 0; m < g.nME; m++) dq[(q * FB_NTHETA + t) * qrow + m] = (float)th[t]->data[q * g.nME + m];
     int is, js, ks, isub;
     std::tie(is, js, ks, isub) = kv.first;
     // the mixed kernel addresses every charge subtype with the neutral group's stencil
@@ -1140,6 +1156,10 @@ int fb_commit_tables(fb_ctx *ctx) {
     return ctx->fail(FB_E_CAPACITY, "commit_tables: FP32 3c slab above 2 GiB");
   RC(upload(ctx, ctx->d_tab3c32, tab3f.data(), tab3f.size() * sizeof(float), "upload tab3c32"));
   ctx->ntab3c32 = (int64_t)tab3f.size();
+  if (tab3q.size() * sizeof(float) >= (size_t)INT32_MAX)
+    return ctx->fail(FB_E_CAPACITY, "commit_tables: FP32 3c quad slab above 2 GiB");
+  RC(upload(ctx, ctx->d_tab3c32q, tab3q.data(), tab3q.size() * sizeof(float), "upload tab3c32q"));
+  ctx->ntab3c32q = (int64_t)tab3q.size();
   RC(upload(ctx, ctx->d_cell2c, cells.data(), cells.size() * sizeof(DCell2c), "upload cells2c"));
   ctx->ncell2c = (int)cells.size();
   ctx->ngrid2c = (int)grids.size();
@@ -1700,8 +1720,8 @@ int fb_assemble_vna_3c(fb_ctx *ctx) {
 // read (all of 2c when neutral; the Vna part when charged, whose terms read
 // S and dipole_z) runs on a second stream beside the 3c launch; it then leaves
 // the Vna blocks of 3c targets to the 3c launch, which overwrites them.
-// the fused persistent launch serves neutral mixed-precision table builds of
-// s/p species
+// the one-launch H+S build serves neutral mixed-precision table builds of s/p
+// species
 static bool fused_build(const fb_ctx *ctx, int charged) {
   return ctx->fused && !charged && ctx->canon_sp && ctx->precision == FB_PREC_MIXED &&
          !(ctx->method3c == FB_3C_MLP && ctx->mlp_ready);
@@ -3046,8 +3066,8 @@ int fb_get_tuning(fb_ctx *ctx, int param, int *value) {
 }
 
 // Times whole H+S builds (enqueued directly, not from a graph) over every
-// launch shape that applies to the context and keeps the fastest: the fused
-// persistent launch at 3 and 4 waves per SIMD (neutral mixed-precision s/p
+// launch shape that applies to the context and keeps the fastest: the
+// one-launch build at 3 and 4 waves per SIMD (neutral mixed-precision s/p
 // builds), and separate 2c and 3c launches with every 3c chunk (mixed s/p
 // builds; it sets the triplets and the LDS footprint of a workgroup) and both
 // s/p 2c kernels.  The last candidate leaves the blocks of its own build, so

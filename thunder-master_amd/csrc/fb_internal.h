@@ -68,9 +68,13 @@ struct DGrid2c {
 // one (is, js, ks, isub) group of the five Legendre theta cells
 struct DGroup3c {
   int nME, nx, ny, munu;
+  int cls;           // s/p pair class of (is, js): 2 if is has p, + 1 if js has p
+  int nq;            // ME quads of the FP32 quad slab, ceil(nME / 4)
   double xmax, ymax, dx, dy;
+  double rdx, rdy;   // 1/dx, 1/dy
   int64_t off;       // FP64 slab, layout [theta][ny][nx][nME]
   int64_t off32;     // FP32 slab, layout [ny][nx][ME][theta], FB_TH32 values per (point, ME)
+  int64_t off32q;    // FP32 quad slab, layout [ny][nx][theta][4 nq] (MEs zero-padded to whole quads)
   int64_t moff;      // method table [ny][nx] (uint8, 1 = 3-point rule), e.FDATA/M_Fdata_3c.f90:383-385
 };
 
@@ -100,8 +104,10 @@ struct DView {
   const float *tab2c32;
   const double *tab3c;
   const float *tab3c32;
+  const float *tab3c32q;        // FP32 quad slab of the bcna groups (DGroup3c::off32q)
   const unsigned char *meth3c;  // per-cell method choice of every 3c group
   int64_t ntab3c32;             // floats in the FP32 3c slab
+  int64_t ntab3c32q;            // floats in the FP32 quad slab
   const int *mu, *nu, *mv;
   const DGrid2c *grid2c;
   const double *pden;
