@@ -333,7 +333,8 @@ int fb_mlp_gradients_noalloc(fb_ctx *ctx, int nme, const float *params, int n, c
  * FB_TUNE_FUSED: 0 runs the neutral mixed-precision H+S build as two
  * launches (2c, then 3c, the 2c one on a second stream), 3 or 4 as one
  * launch whose first blocks are the 2c atom groups and the rest the
- * 256-triplet 3c tiles (register allocation for 3 or 4 waves per SIMD);
+ * 64-triplet 3c tiles, four per workgroup (register allocation for 3 or 4
+ * waves per SIMD);
  * charged, binary64 and learned-surrogate builds always take separate
  * launches.
  * fb_autotune times whole H+S builds (nrep each, HIP events) over every

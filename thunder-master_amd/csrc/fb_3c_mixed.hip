@@ -41,16 +41,54 @@
 #define NB (FB_MO * FB_MO)
 #define NME3 10
 
+// The rotation of rot3c_sp (fb_sp.h) in FP32 with fused multiply-adds, the
+// p-p block as (P M) P^T: 42 operations instead of 126 for an sp-sp block
+#define PPF(k, m) pm[3 * (m) + (k)]
+template <int CLS>
+__device__ __forceinline__ void rot3c_fma(const float *pm, const float *F, float *X) {
+  X[0] = F[0];
+  if (CLS == CLS_SPSP) {
+    const float b = F[1], c = F[2], d = F[3], e = F[4], f = F[5], g = F[6], h = F[7], i = F[8], j = F[9];
+#pragma unroll
+    for (int l = 0; l < 3; l++) X[(1 + l) * 4] = __builtin_fmaf(g, PPF(l, 2), b * PPF(l, 1));
+#pragma unroll
+    for (int k = 0; k < 3; k++) X[1 + k] = __builtin_fmaf(PPF(k, 2), h, PPF(k, 1) * c);
+    float T0[3], T1[3], T2[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      T0[k] = PPF(k, 0) * d;
+      T1[k] = __builtin_fmaf(PPF(k, 2), i, PPF(k, 1) * e);
+      T2[k] = __builtin_fmaf(PPF(k, 2), f, PPF(k, 1) * j);
+    }
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        X[(1 + l) * 4 + 1 + k] = __builtin_fmaf(T2[k], PPF(l, 2), __builtin_fmaf(T1[k], PPF(l, 1), T0[k] * PPF(l, 0)));
+  } else if (CLS == CLS_SSP) {
+#pragma unroll
+    for (int l = 0; l < 3; l++) X[1 + l] = __builtin_fmaf(F[2], PPF(l, 2), F[1] * PPF(l, 1));
+  } else if (CLS == CLS_SPS) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) X[1 + k] = __builtin_fmaf(PPF(k, 2), F[2], PPF(k, 1) * F[1]);
+  }
+}
+#undef PPF
+
 struct TPrm {
   unsigned base;              // quad slab index (floats) of stencil corner (ix-1, iy-1), neutral group
-  unsigned rowstride;         // floats per iy row of the quad slab
-  unsigned char cls, nme, m1, nq;     // pair class, ME count, method-1 flag, ME quads
+  unsigned rsc;               // floats per iy row of the quad slab (bits 0-23), pair class (24-25), method-1 flag (26)
   float px, py;               // position in the cell; phase B forms the weights
   float P[4], sint;           // Legendre P1..P4 (P0 = 1)
-  float pm[9];
+  float pm[6];                // columns 0 and 1 of the p rotation; column 2 is their cross product
+  __device__ int rowstride() const { return (int)(rsc & 0xffffffu); }
+  __device__ int cls() const { return (int)(rsc >> 24) & 3; }
+  __device__ bool m1() const { return (rsc >> 26) & 1u; }
+  __device__ int nme() const { return nme3c_of(cls()); }   // canonical s/p tables
+  __device__ int nq() const { return cls() == CLS_SPSP ? 3 : 1; }
 };
-// 19 words: an odd row stride, so lane-per-triplet accesses hit distinct banks
-static_assert(sizeof(TPrm) == 19 * 4, "TPrm row stride must stay odd in words");
+// 15 words: an odd row stride, so lane-per-triplet accesses hit distinct banks
+static_assert(sizeof(TPrm) == 15 * 4, "TPrm row stride must stay odd in words");
 
 // the group table through the constant address space: uniform indices load
 // with scalar loads
@@ -194,11 +232,31 @@ __device__ __forceinline__ float sum16(float v) {
 }
 __device__ __forceinline__ float4 sum16(float4 v) { return make_float4(sum16(v.x), sum16(v.y), sum16(v.z), sum16(v.w)); }
 
+// Barrier of one tile pass: a 64-triplet tile is one wavefront, which needs
+// only ordering of its LDS accesses (so that several such tiles can share a
+// workgroup, one per wavefront, each with its own pass count)
+template <int CH>
+__device__ __forceinline__ void tile_sync() {
+  if (CH == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// lane of a tile: a 64-triplet tile is one wavefront of its workgroup
+template <int CH>
+__device__ __forceinline__ int tile_tid() {
+  return CH == 64 ? (int)(threadIdx.x & 63u) : (int)threadIdx.x;
+}
+
 // LDS of one tile pass
 template <int CH, bool CHARGED, bool MLP>
 struct S3c {
   TPrm sq[CH];
-  float sF[CH][12];                   // molecular-frame MEs, whole quads
+  float sF[CH][NME3];                 // molecular-frame MEs (phase C's contributions reuse sq and sF)
   TChg sx[CHARGED ? CH : 1];
   int sSeg[CH + 1];                   // triplet offsets (below 2^31: the list build checks)
   int64_t sBo[CH];
@@ -219,14 +277,14 @@ struct S3c {
 template <int NQ, bool M1, int CH, bool CHARGED, bool MLP>
 __device__ __forceinline__ void interp_class(S3c<CH, CHARGED, MLP> &S, __amdgpu_buffer_rsrc_t tab, int kb, int n) {
   constexpr int R = M1 ? 1 : 16;   // lanes per (triplet, quad)
-  for (int it = threadIdx.x; it < R * NQ * n; it += CH) {
+  for (int it = tile_tid<CH>(); it < R * NQ * n; it += CH) {
     const int kq = it / R, p = it - R * kq;
     const int k = kq / NQ, qd = kq - NQ * k;
     const int t = S.sFb[kb + k];
     const TPrm &q = S.sq[t];
     const float P[5] = {1.f, q.P[0], q.P[1], q.P[2], q.P[3]};
     const float px = q.px, py = q.py;
-    const unsigned o = (q.base + 4u * (unsigned)qd) * 4u, rb = q.rowstride * 4u;
+    const unsigned o = (q.base + 4u * (unsigned)qd) * 4u, rb = (unsigned)q.rowstride() * 4u;
     float4 F = M1 ? stencil4<NQ, true>(tab, o, rb, px, py, P)
                   : sum16(stencil_point<NQ>(tab, o, rb, px, py, p & 3, p >> 2, P));
     if (CHARGED) {
@@ -243,7 +301,12 @@ __device__ __forceinline__ void interp_class(S3c<CH, CHARGED, MLP> &S, __amdgpu_
         F = axpy4(dq * c.stinky, Fk, F);
       }
     }
-    if (p == 0) *reinterpret_cast<float4 *>(&S.sF[t][4 * qd]) = F;
+    if (p == 0) {
+      // quad 2 of an sp-sp triplet holds MEs 8, 9 and two padding zeros
+      float2 *const d = reinterpret_cast<float2 *>(&S.sF[t][4 * qd]);
+      d[0] = make_float2(F.x, F.y);
+      if (qd < 2) d[1] = make_float2(F.z, F.w);
+    }
   }
 }
 
@@ -255,8 +318,10 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
                                        int &status) {
   // contributions reuse the parameter storage, rows padded to 17 words
   float(*sC)[NB + 1] = reinterpret_cast<float(*)[NB + 1]>(&S.sq[0]);
-  static_assert(sizeof(TPrm) >= (NB + 1) * sizeof(float), "contribution row must fit in a parameter row");
-  const int tid = threadIdx.x;
+  typedef S3c<CH, CHARGED, MLP> S_t;
+  static_assert(offsetof(S_t, sF) == sizeof(S.sq) && sizeof(S.sq) + sizeof(S.sF) >= CH * (NB + 1) * sizeof(float),
+                "the contribution rows must fit in the parameter and ME rows");
+  const int tid = tile_tid<CH>();
   const int64_t first = td.x, last = td.y, r0 = td.z, r1 = td.w;
   if (first > last) return;
   // Accumulator slot sl = (segment, block element); lane tid owns the slots
@@ -300,9 +365,6 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
         gid = 0;
       }
       const DGroup3c g = group_scalar(v, gid);
-      q.cls = (unsigned char)g.cls;
-      q.nme = (unsigned char)g.nME;
-      q.nq = (unsigned char)g.nq;
       const double z = d_distance(r1v, r2v);
 #pragma unroll
       for (int k = 0; k < 3; k++) rnabc[k] = rna[k] - (r1v[k] + r21[k] * 0.5);
@@ -387,7 +449,7 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
         float pmf[9];
         d_pmat<float>(eps, pmf);
 #pragma unroll
-        for (int k = 0; k < 9; k++) q.pm[k] = pmf[k];
+        for (int k = 0; k < 6; k++) q.pm[k] = pmf[k];
       }
       const float cost = cosf_, cost2 = cost * cost;
       q.sint = sint;
@@ -418,10 +480,9 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
       // method of the cell, precomputed from the binary64 table with the
       // reference's gradient test (M_Fdata_3c.f90:383-385)
       const int64_t icell = (int64_t)(iy - 1) * g.nx + (ix - 1);
-      q.m1 = v.meth3c[g.moff + icell];
+      q.rsc = (unsigned)(g.nx * 20 * g.nq) | ((unsigned)g.cls << 24) | ((unsigned)v.meth3c[g.moff + icell] << 26);
       q.px = (float)px;
       q.py = (float)py;
-      q.rowstride = (unsigned)(g.nx * 20 * g.nq);
       q.base = (unsigned)(g.off32q + ((int64_t)(iy - 2) * g.nx + (ix - 2)) * 20 * g.nq);
       if (MLP) {
         // per-triplet bound: P_eq2 x the calibrated Frobenius error of the
@@ -483,7 +544,7 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
     {
       const int wv = tid >> 6, l = tid & 63;
       const bool on = MLP && tid < nt && S.sG[tid] >= 0, st = tid < nt && !on;
-      const int cl = st ? (S.sq[tid].m1 ? 0 : 2) + (S.sq[tid].nq == 3 ? 0 : 1) : -1;
+      const int cl = st ? (S.sq[tid].m1() ? 0 : 2) + (S.sq[tid].nq() == 3 ? 0 : 1) : -1;
       const unsigned long long bn = __ballot(on);
       unsigned long long bc[4];
 #pragma unroll
@@ -499,7 +560,7 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
 #pragma unroll
           for (int c = 0; c < 4; c++) S.sWn[wv][1 + c] = __popcll(bc[c]);
         }
-        __syncthreads();
+        tile_sync<CH>();
 #pragma unroll
         for (int w = 0; w < CH / 64; w++) {
           if (w < wv) {
@@ -522,7 +583,7 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
         S.sFb[base + pc[cl] + __popcll(bc[cl] & below)] = (short)tid;
       }
     }
-    __syncthreads();
+    tile_sync<CH>();
     // ---------------- phase B: interpolation, one lane per (triplet, ME
     // quad), the four classes of the stencil list one after another
     interp_class<3, true>(S, tab, 0, nc[0]);
@@ -532,7 +593,7 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
     if (MLP) {
       // keeps the weight loads of phase M from being hoisted into the stencil
       // loop (their registers would overlap)
-      __syncthreads();
+      tile_sync<CH>();
       // ---------------- phase M: network evaluation, 16 triplets per
       // wavefront pass; lane group l/16 folds the matrix elements l/16 + 4k of
       // its triplet (the theta cells times P_theta(cos) and the envelope)
@@ -556,7 +617,7 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
 #pragma unroll
             for (int ml = 0; ml < 3; ml++) {
               const int m = qg + 4 * ml;
-              if (m < q.nme) {
+              if (m < q.nme()) {
                 float F = P[0] * out[ml * 5];
 #pragma unroll
                 for (int th = 1; th < 5; th++) F = __builtin_fmaf(P[th], out[ml * 5 + th], F);
@@ -568,8 +629,9 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
         }
       }
     }
-    __syncthreads();
-    // ---------------- phase C: rotation, one lane per triplet
+    tile_sync<CH>();
+    // ---------------- phase C: rotation, one lane per triplet; P_eq2 (and
+    // sin theta on the mvalue = 1 elements) scale the MEs before it
     float X[16];
     int cls = 0, p = 0;
 #pragma unroll
@@ -577,22 +639,27 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
     if (tid < nt) {
       const TPrm &q = S.sq[tid];
       float F[NME3], pm[9];
-      cls = q.cls;
+      cls = q.cls();
       if (CHARGED) p = v.trip_pair[c0 + tid];
       const int nm0 = (cls == 3) ? 6 : (cls == 0 ? 1 : 2);
-      const float sint = q.sint;
+      const float s0 = (float)FB_P_EQ2, s1 = (float)FB_P_EQ2 * q.sint;
+      const int nme = q.nme();
 #pragma unroll
-      for (int m = 0; m < NME3; m++) F[m] = (m < q.nme) ? S.sF[tid][m] * (m >= nm0 ? sint : 1.f) : 0.f;
+      for (int m = 0; m < NME3; m++) F[m] = (m < nme) ? S.sF[tid][m] * (m >= nm0 ? s1 : s0) : 0.f;
 #pragma unroll
-      for (int k = 0; k < 9; k++) pm[k] = q.pm[k];
+      for (int k = 0; k < 6; k++) pm[k] = q.pm[k];
+      // column 2 = column 0 x column 1 (a rotation; (y, z, x) is a cyclic order)
+      pm[6] = pm[1] * pm[5] - pm[2] * pm[4];
+      pm[7] = pm[2] * pm[3] - pm[0] * pm[5];
+      pm[8] = pm[0] * pm[4] - pm[1] * pm[3];
       switch (cls) {
-        case 3: rot3c_sp<float>(3, pm, F, X); break;
-        case 2: rot3c_sp<float>(2, pm, F, X); break;
-        case 1: rot3c_sp<float>(1, pm, F, X); break;
-        default: rot3c_sp<float>(0, pm, F, X); break;
+        case 3: rot3c_fma<3>(pm, F, X); break;
+        case 2: rot3c_fma<2>(pm, F, X); break;
+        case 1: rot3c_fma<1>(pm, F, X); break;
+        default: X[0] = F[0]; break;
       }
     }
-    __syncthreads();   // every lane has read its parameters before sC overwrites them
+    tile_sync<CH>();   // every lane has read its parameters before sC overwrites them
     if (tid < nt) {
       const int nb = (cls & 2 ? 4 : 1) * (cls & 1 ? 4 : 1);
       if (CHARGED) {
@@ -605,16 +672,16 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
           if (e < nb) {
             double sterm = sbp[e] * 0.5, dterm = dbp[e] / (double)c.z;
             double emnpl = (sterm - dterm) / (double)c.d13 + (sterm + dterm) / (double)c.d23;
-            val = X[e] * (float)FB_P_EQ2 + (float)(c.dqs * (1.0 - c.stinky) * emnpl * FB_P_EQ2);
+            val = X[e] + (float)(c.dqs * (1.0 - c.stinky) * emnpl * FB_P_EQ2);
           }
           sC[tid][e] = val;
         }
       } else {
 #pragma unroll
-        for (int e = 0; e < 16; e++) sC[tid][e] = X[e] * (float)FB_P_EQ2;
+        for (int e = 0; e < 16; e++) sC[tid][e] = X[e];
       }
     }
-    __syncthreads();
+    tile_sync<CH>();
     // ---------------- phase D: ordered binary64 accumulation, one lane per
     // (owned segment, block element), segment order preserved: the sum starts
     // from zero in the segment's first chunk and continues from the block
@@ -632,13 +699,13 @@ __device__ __forceinline__ void tile3c(const DView &v, const int4 td, S3c<CH, CH
         *o = a;
       }
     }
-    __syncthreads();
+    tile_sync<CH>();
   }
 }
 
 // table stencil: one tile per workgroup
 template <int CH, bool CHARGED>
-__global__ __launch_bounds__(CH) void k_3c_mixed(DView v) {
+__global__ __launch_bounds__(CH, 4) void k_3c_mixed(DView v) {
   __shared__ S3c<CH, CHARGED, false> S;
   const int kt = CH == 64 ? 0 : (CH == 128 ? 1 : 2);
   int status = 0;
@@ -693,15 +760,17 @@ __global__ __launch_bounds__(256, 4) void k_3c_mlp(DView v, int *queue) {
 // [0, n2p) are the 2c workgroup items of v.wg2c (S, T, dipole_z, the Vna
 // ontop and atom terms, fb_2c_sp.h; the Vna blocks of 3c targets are left to
 // the 3c tiles), n2p = v.nwg2c rounded up to a multiple of 8 so that the XCD
-// of a block stays blockIdx mod 8; the blocks after them are the 256-triplet
-// 3c tiles, numbered contiguously per XCD.  The 2c blocks are dispatched
+// of a block stays blockIdx mod 8; each block after them runs four
+// 64-triplet 3c tiles, one per wavefront, numbered contiguously per XCD (the
+// tiles that win as separate launches, without a workgroup barrier between
+// the waves of a block).  The 2c blocks are dispatched
 // first, so their long per-lane chains (one pair per lane, six interactions)
 // start at once and the 3c tiles fill the issue slots they leave idle.
 // Neutral builds only: charged 3c terms read the S and dipole_z blocks of the
 // 2c items.
 union SFused {
   S2c<0> s2;
-  S3c<256, false, false> s3;
+  S3c<64, false, false> s3[4];
 };
 
 // W: waves per SIMD the register allocation targets (3: the 2c body's 159
@@ -714,8 +783,13 @@ __global__ __launch_bounds__(256, W) void k_hs_fused(DView v, int n2p) {
     if (b < v.nwg2c) wg2c_sp<0>(v, 15 | FB_2C_SKIP_TGT, b, S.s2);
     return;
   }
+  const int b3 = b - n2p;
+  // four 64-triplet tiles per workgroup, one per wavefront
+  const int wv = threadIdx.x >> 6;
+  const int64_t tile = 4 * xcd_tile(b3, gridDim.x - n2p) + wv;
+  if (tile >= v.ntiles[0]) return;
   int status = 0;
-  tile3c<256, false, false>(v, v.tiles[2][xcd_tile(b - n2p, gridDim.x - n2p)], S.s3, nullptr, status);
+  tile3c<64, false, false>(v, v.tiles[0][tile], S.s3[wv], nullptr, status);
   if (status) atomicOr(v.status, status);
 }
 
@@ -754,9 +828,9 @@ extern "C" int fbk_3c_mixed(const DView &v, int chunk, int charged, hipStream_t 
 
 // waves: 3 or 4 per SIMD (the register allocation of k_hs_fused)
 extern "C" int fbk_hs_fused(const DView &v, int waves, hipStream_t st) {
-  if (v.nwg2c == 0 && v.ntiles[2] == 0) return 0;
+  if (v.nwg2c == 0 && v.ntiles[0] == 0) return 0;
   const int n2p = (v.nwg2c + 7) / 8 * 8;
-  const dim3 grid((unsigned)(n2p + v.ntiles[2]));
+  const dim3 grid((unsigned)(n2p + (v.ntiles[0] + 3) / 4));
   if (waves == 3)
     hipLaunchKernelGGL(k_hs_fused<3>, grid, dim3(256), 0, st, v, n2p);
   else
