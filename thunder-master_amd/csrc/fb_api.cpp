@@ -58,7 +58,7 @@ int fbk_3c_fp64(const DView &v, hipStream_t st);
 int fbk_3c_mixed(const DView &v, int chunk, int charged, hipStream_t st);
 int fbk_2c_sp(const DView &v, int mask, hipStream_t st);
 int fbk_2c_lds(const DView &v, int mask, hipStream_t st);
-int fbk_3c_fp64_sp(const DView &v, hipStream_t st);
+int fbk_3c_fp64_ph(const DView &v, int charged, hipStream_t st);
 int fbk_gamma_fold(const DView &v, int a0, int natoms, int which, const int *orb_off, int n, double *A,
                    hipStream_t st);
 int fbk_3c_mlp(const DView &v, int *queue, int nwg, hipStream_t st);
@@ -1662,7 +1662,7 @@ static int run_3c(fb_ctx *ctx) {
       });
     if (ctx->precision == FB_PREC_MIXED)
       return ctx->launch("assemble_3c_mixed", [&] { return fbk_3c_mixed(v, ctx->chunk3c, any_charge(ctx), ctx->stream); });
-    return ctx->launch("assemble_3c_fp64", [&] { return fbk_3c_fp64_sp(v, ctx->stream); });
+    return ctx->launch("assemble_3c_fp64", [&] { return fbk_3c_fp64_ph(v, any_charge(ctx), ctx->stream); });
   }
   // generic shells: binary64 path only
   return ctx->launch("assemble_3c_fp64_generic", [&] { return fbk_3c_fp64(v, ctx->stream); });
