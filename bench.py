@@ -481,6 +481,44 @@ def copy_calibration(mib):
     del a, b
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline_openmp(steps=5):
+    """The reference's H+S assemblers with the OpenMP loop of
+    M_assemble_3c_DOGS.f90 on every host thread (oracle/_ref/fb_harness_nbomp:
+    the 2c assemblers have no OpenMP and run serially), on the Si4096 workload
+    itself; its neighbour lists come from the drop-in driver_neighbors (GPU),
+    outside the timed assemblers.  Median of `steps` repetitions."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import refrun
+    binary = os.path.join(os.path.dirname(refrun.HARNESS_REF), "fb_harness_nbomp")
+    if not refrun.available(binary):
+        return {"error": "make -C oracle nbhip not built"}
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    species, structs = synth.config(1)
+    st = structs[0]
+    r = refrun.run(synth.make_tables(species), st, binary=binary, nrep=steps, threads=threads)
+    reps = np.array(r["rep_times"])
+    build = np.median(reps.sum(axis=1))
+    t3 = np.median(reps[:, 4])
+    return {"value": r["ntriplets"] / build, "unit": "triplets/s", "cores": threads, "kind": "reference",
+            "cpu_model": _cpu_model(),
+            "sample": f"Si4096 ({r['ntriplets']} triplets): H+S build {build * 1e3:.1f} ms, median of {steps} "
+                      f"(assemble_S/T/dipole_z/vna_2c serial, vna_3c OpenMP, flang -O2 -fopenmp, "
+                      f"OMP_PROC_BIND=close); vna_3c alone {t3 * 1e3:.1f} ms",
+            "h_s_build_ms": build * 1e3, "vna_3c_ms": t3 * 1e3,
+            "rep_ms": [[round(x * 1e3, 3) for x in rep] for rep in r["rep_times"]]}
+
+
 def cpu_baseline():
     """The reference's own Fortran (serial flang build, oracle/_ref) on a
     bounded sample: a 512-atom Si cell (4x4x4 diamond, sigma 0.08 A)."""
@@ -493,6 +531,7 @@ def cpu_baseline():
         r = refrun.run(ts, st, nrep=nrep)
         t = sum(r["times"][k] for k in ("S", "T", "DIP", "VNA2C", "VNA3C")) / nrep
         return {"value": r["ntriplets"] / t, "unit": "triplets/s", "cores": 1, "kind": "reference",
+                "cpu_model": _cpu_model(),
                 "sample": f"Si512 4x4x4 diamond (sigma 0.08 A): {r['ntriplets']} triplets, H+S build "
                           f"{t * 1e3:.1f} ms (assemble_S/T/dipole_z/vna_2c/vna_3c, serial flang -O2, "
                           f"mean of {nrep}); neighbour lists {r['times']['lists']:.2f} s not included",
@@ -656,6 +695,12 @@ def main():
             cpu = cpu_baseline()
         except Exception as e:  # the baseline is reported, never required
             cpu = {"error": str(e)}
+        try:
+            cpu["openmp"] = cpu_baseline_openmp()
+            print(f"# cpu baseline (OpenMP, {cpu['openmp'].get('cores')} threads): {cpu['openmp'].get('sample')}",
+                  file=sys.stderr, flush=True)
+        except Exception as e:  # noqa: BLE001
+            cpu["openmp"] = {"error": str(e)}
 
     out = {
         "metric": METRIC, "value": ntrip / (ms * 1e-3), "unit": "triplets/s", "n_gpus": 1,

@@ -37,6 +37,7 @@
         integer in1, in2, ialpha, ntot_pairs, ntot_common
         integer(kind=8) :: c0, c1, crate
         real t_lists, t_S, t_T, t_dip, t_vna2c, t_vna3c
+        real t_rep(5)
         real(kind=8) vnabump
         character(len=512) :: infile, outfile, arg
         character(len=128) :: floc
@@ -112,6 +113,7 @@
 
         t_S = 0.0d0; t_T = 0.0d0; t_dip = 0.0d0; t_vna2c = 0.0d0; t_vna3c = 0.0d0
         do irep = 1, nrep
+          t_rep = (/t_S, t_T, t_dip, t_vna2c, t_vna3c/)
           if (irep .gt. 1) then
             call destroy_assemble_2c (st)
             call destroy_assemble_vna (st)
@@ -144,6 +146,9 @@
           call assemble_vna_3c (st)
           call system_clock (c1)
           t_vna3c = t_vna3c + real(c1 - c0)/real(crate)
+          ! this repetition's times (seconds): S T dip vna2c vna3c
+          write (*, '(A, I6, 5F14.6)') ' rep ', irep,                         &
+     &      (/t_S, t_T, t_dip, t_vna2c, t_vna3c/) - t_rep
         end do
 
         ntot_pairs = 0

@@ -20,6 +20,8 @@ sys.path.insert(0, os.path.join(REPO, "thunder-master_amd"))
 from fireball_hip import fdata_io  # noqa: E402
 
 HARNESS_REF = os.path.join(HERE, "_ref", "fb_harness_ref")
+# the same harness with the OpenMP loop of M_assemble_3c_DOGS.f90 (make -C oracle ompref)
+HARNESS_OMP = os.path.join(HERE, "_ref", "fb_harness_omp")
 
 
 def available(binary=HARNESS_REF):
@@ -85,10 +87,12 @@ def parse_output(path, species, imass):
     return out
 
 
-def run(ts, structure, binary=HARNESS_REF, nrep=1, workdir=None, timeout=3600, vna_bump=None):
+def run(ts, structure, binary=HARNESS_REF, nrep=1, workdir=None, timeout=3600, vna_bump=None, threads=1):
     """Write tables + structure, run the harness, return the parsed output.
     vna_bump: add it to every Vna block element between assemble_vna_2c and
-    assemble_vna_3c."""
+    assemble_vna_3c.  threads: OMP_NUM_THREADS (the OpenMP harness's 3c loop).
+    res["rep_times"] holds each repetition's (S, T, dipole_z, vna_2c, vna_3c)
+    seconds."""
     own = workdir is None
     tmp = tempfile.mkdtemp(prefix="fbref_") if own else workdir
     fdir = os.path.join(tmp, "Fdata")
@@ -97,11 +101,15 @@ def run(ts, structure, binary=HARNESS_REF, nrep=1, workdir=None, timeout=3600, v
     inp = os.path.join(tmp, "harness.in")
     fdata_io.write_harness_input(inp, fdir, ts, structure)
     outp = os.path.join(tmp, "harness.out")
-    env = dict(os.environ, OMP_NUM_THREADS="1")
+    env = dict(os.environ, OMP_NUM_THREADS=str(int(threads)))
+    if threads > 1:
+        env.setdefault("OMP_PROC_BIND", "close")
     args = [binary, inp, outp, str(nrep)] + ([repr(float(vna_bump))] if vna_bump is not None else [])
-    subprocess.run(args, cwd=tmp, check=True, env=env,
-                   stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=timeout)
+    proc = subprocess.run(args, cwd=tmp, check=True, env=env,
+                          stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=timeout)
     res = parse_output(outp, ts.species, structure.imass)
+    res["rep_times"] = [tuple(float(x) for x in line.split()[2:7])
+                        for line in proc.stdout.decode().splitlines() if line.split()[:1] == ["rep"]]
     if own:
         import shutil
         shutil.rmtree(tmp, ignore_errors=True)

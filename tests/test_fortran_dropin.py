@@ -27,10 +27,13 @@ MATS = ("S", "T", "DIP", "VNA", "VNAO")
 TOL_MIXED = 1e-6
 
 need_hip = pytest.mark.skipif(not refrun.available(HIP_HARNESS), reason="make -C oracle hip not built")
+# the reference's assemblers with the drop-in driver_neighbors (M_neighbors_hip.f90)
+NB_HARNESS = os.path.join(os.path.dirname(refrun.HARNESS_REF), "fb_harness_nbhip")
+need_nb = pytest.mark.skipif(not refrun.available(NB_HARNESS), reason="make -C oracle nbhip not built")
 
 
-def _run_hip(ts, st, nrep=1, env=None):
-    print(f"hip harness natoms={st.natoms} nrep={nrep} env={env}", flush=True)
+def _run_hip(ts, st, nrep=1, env=None, binary=HIP_HARNESS):
+    print(f"{os.path.basename(binary)} natoms={st.natoms} nrep={nrep} env={env}", flush=True)
     full = dict(os.environ, **(env or {}))
     with tempfile.TemporaryDirectory(prefix="fbhip_") as d:
         fdir = os.path.join(d, "Fdata")
@@ -39,7 +42,7 @@ def _run_hip(ts, st, nrep=1, env=None):
         inp = os.path.join(d, "harness.in")
         fdata_io.write_harness_input(inp, fdir, ts, st)
         outp = os.path.join(d, "harness.out")
-        r = subprocess.run([HIP_HARNESS, inp, outp, str(nrep)], cwd=d, env=full, capture_output=True,
+        r = subprocess.run([binary, inp, outp, str(nrep)], cwd=d, env=full, capture_output=True,
                            text=True, timeout=300)
         if r.returncode != 0:
             raise RuntimeError(f"hip harness exit {r.returncode}: {r.stdout[-2000:]} {r.stderr[-2000:]}")
@@ -314,3 +317,39 @@ def test_pp_dropin_matches_reference_live_repeated():
         if k in ("times", "nrep"):
             continue
         assert np.array_equal(hip[k], ref[k]), k
+
+
+@need_nb
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", golden_cases.NAMES)
+def test_driver_neighbors_dropin_golden(golden, name):
+    """driver_neighbors from the GPU lists under the reference's own
+    assemblers: every list array and every block equal to the reference's run
+    (the blocks can only match if the lists match entry for entry, in order)."""
+    print(f"case {name}", flush=True)
+    g = golden(name)
+    ts, st = golden_cases.inputs(g)
+    r = _run_hip(ts, st, binary=NB_HARNESS)
+    for k in LIST_KEYS:
+        assert np.array_equal(r[k], g[k]), k
+    n = len(g["S"])
+    for m in MATS:
+        assert np.array_equal(r[m][:n], g[m]), (m, float(np.max(np.abs(r[m][:n] - g[m]))))
+
+
+@need_nb
+@pytest.mark.gpu
+def test_driver_neighbors_dropin_chno_box():
+    """A 300-atom C/H/N/O box (four species, mixed s and sp orbitals): lists and
+    blocks of the drop-in run equal the reference harness's, run here on the
+    same input."""
+    from fireball_hip import synth
+    sp = synth.CHNO_SPECIES
+    ts = synth.make_tables(sp)
+    st = synth.random_box(synth.chno_counts(300), 1.0, 0.9, 7, sp)
+    ref = refrun.run(ts, st)
+    r = _run_hip(ts, st, binary=NB_HARNESS)
+    for k in LIST_KEYS:
+        assert np.array_equal(r[k], ref[k]), k
+    for m in MATS:
+        assert np.array_equal(r[m], ref[m]), m

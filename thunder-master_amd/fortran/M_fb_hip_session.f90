@@ -121,6 +121,13 @@
 ! Mirror the caller's structure; rebuild and check the lists when it changed.
         subroutine fb_session_sync (s)
         type(T_structure), target :: s
+        if (fb_session_build(s)) call check_lists (s)
+        end subroutine fb_session_sync
+
+! Mirror the caller's structure and rebuild the lists on the GPU when it
+! changed; .true. when they were rebuilt.
+        logical function fb_session_build (s) result (rebuilt)
+        type(T_structure), target :: s
         integer iatom, i
         logical same_atoms, same_pos
         integer(c_int) :: idx
@@ -142,6 +149,7 @@
         end if
         same_pos = .false.
         if (same_atoms) same_pos = all(r .eq. c_ratom)
+        rebuilt = .false.
         if (same_atoms .and. same_pos .and. have_lists) then
           deallocate (r, im)
           return
@@ -169,9 +177,9 @@
         call fb_check (fb_build_lists(fbh, npairs_all, ntrip_all), fbh, 'fb_build_lists')
         have_lists = .true.
         have_pp_lists = .false.
-        call check_lists (s)
+        rebuilt = .true.
         deallocate (r, im)
-        end subroutine fb_session_sync
+        end function fb_session_build
 
 ! Pseudopotential shells of the species (species(:)%nssh_PP, shell_PP(:)%lssh,
 ! %rcutoff_PP and the Kleinman-Bylander shell_PP(:)%cl of clPP.ZZ.dat), when
