@@ -8,6 +8,7 @@ build container, where the reference sources are present).
     python tests/golden/make_golden.py --rho      # density fixtures only
     make -C oracle pref                            # pseudopotential harness
     python tests/golden/make_golden.py --pp       # pseudopotential fixtures only
+    python tests/golden/make_golden.py --d        # H/S fixtures of species with d shells
 
 Each fixture is an .npz holding the inputs that are not re-derivable from a
 seed (explicit positions for the hand-made cases), the reference's neighbour
@@ -88,6 +89,24 @@ def cases():
     out.append(("cho40_dq", ts3, box, None))
     ch = chain_cluster()
     out.append(("chain_cluster", ts3, ch, None))
+    return out
+
+
+def cases_d():
+    """Species with d shells (s, p, d: 9 orbitals) beside s and s/p ones: a
+    charged cluster and a small periodic cell with many images."""
+    spd = [synth.FE_SPD, synth.SI, synth.H_SP]
+    ts = synth.make_tables(spd, seed=77)
+    ts.seed = 77
+    out = []
+    cl = synth.random_box({26: 4, 14: 4, 1: 4}, 1.5, 1.6, 21, spd)
+    cl.icluster = 1
+    synth.random_dQ([cl], spd, seed=22)
+    out.append(("spd_cluster_dq", ts, cl, None))
+    lat = np.array([[4.1, 0.0, 0.0], [0.3, 3.9, 0.0], [0.2, 0.4, 4.3]])
+    pos = np.array([[0.1, 0.2, 0.3], [2.0, 1.8, 2.2], [1.1, 2.9, 0.9], [3.0, 0.7, 3.1]])
+    cell = synth.Structure(pos, np.array([1, 1, 2, 3], np.int32), lat, 0, name="spd_cell")
+    out.append(("spd_cell", ts, cell, None))
     return out
 
 
@@ -221,7 +240,8 @@ def main():
         for name, ts, st, keep_atoms in cases():
             forces_fixture(name, ts, st, keep_atoms if keep_atoms is not None else (10 if st.natoms > 32 else None))
         return
-    for name, ts, st, keep_atoms in cases():
+    todo = cases_d() if "--d" in sys.argv else cases()
+    for name, ts, st, keep_atoms in todo:
         r = refrun.run(ts, st)
         d = {k: np.asarray(r[k], np.int32) for k in LIST_KEYS}
         d["natoms"] = np.int32(st.natoms)
@@ -251,7 +271,7 @@ def main():
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **d)
         print(f"{name}: natoms={st.natoms} pairs={r['npairs']} triplets={r['ntriplets']} "
               f"blocks kept for {npair_keep} pairs")
-        if want_forces:
+        if want_forces and "--d" not in sys.argv:
             forces_fixture(name, ts, st, keep_atoms if keep_atoms is not None else (10 if st.natoms > 32 else None))
 
 

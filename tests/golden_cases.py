@@ -4,8 +4,10 @@ import numpy as np
 
 from fireball_hip import synth
 
-SPECIES_BY_Z = {sp.nZ: sp for sp in (synth.SI, synth.C_SP, synth.H_SP, synth.N_SP, synth.O_SP)}
+SPECIES_BY_Z = {sp.nZ: sp for sp in (synth.SI, synth.C_SP, synth.H_SP, synth.N_SP, synth.O_SP, synth.FE_SPD)}
 NAMES = ("si64", "si64_dq", "si8_cell", "cho40_dq", "chain_cluster")
+# species with d shells: the H+S assemblers only (generic binary64 kernels)
+NAMES_D = ("spd_cluster_dq", "spd_cell")
 
 
 def inputs(g):

@@ -37,7 +37,7 @@ def _gpu(ts, structs, precision="fp64"):
     return asm
 
 
-@pytest.mark.parametrize("name", golden_cases.NAMES)
+@pytest.mark.parametrize("name", golden_cases.NAMES + golden_cases.NAMES_D)
 def test_golden_fp64(golden, name):
     print(f"case {name}", flush=True)
     g = golden(name)
@@ -521,3 +521,30 @@ def test_atoms_and_structures_without_triplets():
             assert err <= TOL_MIXED, (chunk, k, err)
     for k in range(len(structs)):
         assert np.array_equal(got[64][k], got[128][k]) and np.array_equal(got[64][k], got[256][k]), k
+
+
+@pytest.mark.parametrize("name", golden_cases.NAMES_D)
+def test_d_shells_mixed_context_takes_binary64_path(golden, name):
+    """Species with d shells go through the generic binary64 kernels in either
+    precision mode: the mixed-precision context gives the reference's bits."""
+    g = golden(name)
+    ts, st = golden_cases.inputs(g)
+    r = _gpu(ts, [st], precision="mixed").run_all(0)
+    n = len(g["S"])
+    for m in MATS:
+        assert_bits(r[m][:n], g[m], m + " vs golden")
+
+
+def test_d_shells_rejected_outside_h_s(golden):
+    """The density, force and pseudopotential kernels hold s and p blocks only:
+    with a d species they stop with FB_E_UNSUPPORTED instead of computing."""
+    from fireball_hip import synth
+    from fireball_hip.lib import FireballError
+    g = golden("spd_cluster_dq")
+    ts, st = golden_cases.inputs(g)
+    asm = _gpu(ts, [st])
+    L = asm.lists(0)
+    rho = synth.denmat(ts.species, st.imass, L["neighn"], L["neigh_j"], L["neigh_back"], seed=3)
+    with pytest.raises(FireballError) as e:
+        asm.set_denmat(rho)
+    assert e.value.code == -11

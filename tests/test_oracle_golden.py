@@ -15,27 +15,27 @@ MATS = ("S", "T", "DIP", "VNA", "VNAO")
 @pytest.fixture(scope="module")
 def oracle_runs(golden):
     out = {}
-    for name in golden_cases.NAMES:
+    for name in golden_cases.NAMES + golden_cases.NAMES_D:
         g = golden(name)
         ts, st = golden_cases.inputs(g)
         out[name] = (g, ts, oracle.System(oracle.Tables(ts), st).run_all())
     return out
 
 
-@pytest.mark.parametrize("name", golden_cases.NAMES)
+@pytest.mark.parametrize("name", golden_cases.NAMES + golden_cases.NAMES_D)
 def test_tables_regenerate(oracle_runs, name):
     g, ts, _ = oracle_runs[name]
     assert str(g["tables_digest"]) == tables_digest(ts)
 
 
-@pytest.mark.parametrize("name", golden_cases.NAMES)
+@pytest.mark.parametrize("name", golden_cases.NAMES + golden_cases.NAMES_D)
 def test_lists_bit_exact(oracle_runs, name):
     g, _, o = oracle_runs[name]
     for k in LIST_KEYS:
         assert np.array_equal(np.asarray(o[k]), g[k]), k
 
 
-@pytest.mark.parametrize("name", golden_cases.NAMES)
+@pytest.mark.parametrize("name", golden_cases.NAMES + golden_cases.NAMES_D)
 def test_blocks_match_reference(oracle_runs, name):
     g, _, o = oracle_runs[name]
     n = len(g["S"])

@@ -17,30 +17,39 @@
 #include "fb_device.h"
 #include "fb_2c_sp.h"
 
-#define W2C 64
-#define NB (FB_MO * FB_MO)
-#define NME2 16   // max non-zero 2c matrix elements for s/p species
-
+// the generic kernel's sizes: s/p species (MO 4), or with d shells (MO 9)
+template <int MO> struct Gen2c;
+template <> struct Gen2c<FB_MO> {
+  static constexpr int W = 64;        // lanes (pairs) per central atom and pass
+  static constexpr int NME = 16;      // non-zero 2c matrix elements
+};
+template <> struct Gen2c<FB_MOD> {
+  static constexpr int W = 16;        // the LDS of the atom terms grows with MO^2
+  static constexpr int NME = FB_NME2_WIDE;
+};
 
 // interpolate + recover + rotate one interaction into X (column-major nmu x nnu)
+template <int MO>
 __device__ inline int me2c_rotated(const DView &v, int cid, int in1, int in3, int nmu, double x, const double *pm,
-                                   double *X) {
+                                   const double *dm, double *X) {
   if (cid < 0) return ST_NO_TABLE;
   const DCell2c c = v.cell2c[cid];
-  double F[NME2];
+  double F[Gen2c<MO>::NME];
   int st = d_interp2c(c, v.tab2c, x, F);
   if (st) return st;
-  double hm[NB];
+  double hm[MO * MO];
   int nnu = v.sp.norb[in3 - 1];
   for (int i = 0; i < nmu * nnu; i++) hm[i] = 0.0;
   for (int k = 0; k < c.nME; k++) hm[(v.nu[c.munu + k] - 1) * nmu + (v.mu[c.munu + k] - 1)] = F[k];
-  d_rotate<double>(v.sp, in1, in3, pm, hm, nmu, X);
+  d_rotate<double>(v.sp, in1, in3, pm, hm, nmu, X, dm);
   return 0;
 }
 
 // mask bits: 1 = S, 2 = T, 4 = dipole_z, 8 = vna (ontop into blocko, atom into
 // block(neigh_self), zero non-self block for the 3c accumulation)
-__global__ __launch_bounds__(W2C) void k_2c(DView v, int mask) {
+template <int MO>
+__global__ __launch_bounds__(Gen2c<MO>::W) void k_2c(DView v, int mask) {
+  constexpr int W2C = Gen2c<MO>::W, NB = MO * MO;
   // atom terms of the chunk's neighbours: [0] neutral, [isorp] charged shell isorp
   __shared__ double s_atom[1 + FB_MAXSH][W2C][NB + 1];
   const int ia = blockIdx.x;
@@ -55,7 +64,10 @@ __global__ __launch_bounds__(W2C) void k_2c(DView v, int mask) {
   const double r1[3] = {v.ratom[3 * ia], v.ratom[3 * ia + 1], v.ratom[3 * ia + 2]};
   const int64_t pself = p0 + v.neigh_self[ia] - 1;
   int status = 0;
-  double acc_atom = 0.0;  // lane e < nmu*nmu owns element e of the self block
+  // lane owns elements lane + j W2C (j < NE) of the self block
+  constexpr int NE = (NB + W2C - 1) / W2C;
+  double acc_atom[NE];
+  for (int j = 0; j < NE; j++) acc_atom[j] = 0.0;
   const double rc1min = v.sp.rcmin[in1 - 1];
 
   for (int c0 = 0; c0 < n; c0 += W2C) {
@@ -74,23 +86,24 @@ __global__ __launch_bounds__(W2C) void k_2c(DView v, int mask) {
       const int nb = nmu * nnu;
       double r2[3];
       for (int q = 0; q < 3; q++) r2[q] = v.ratom[3 * ja + q] + xls[3 * mb + q];
-      double z = d_distance(r1, r2), sighat[3], eps[9], pm[9];
+      double z = d_distance(r1, r2), sighat[3], eps[9], pm[9], dm[MO == FB_MOD ? 25 : 1];
       if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
       else for (int q = 0; q < 3; q++) sighat[q] = (r2[q] - r1[q]) / z;
       d_epsilon(r2, sighat, eps);
       d_pmat<double>(eps, pm);
+      if (MO == FB_MOD) d_dmat(eps, dm);
       const int64_t bo = v.boff[p];
       double X[NB];
       if (mask & 1) {
-        status |= me2c_rotated(v, cell2c_id(v, in1, in2, SLOT_S, 0), in1, in2, nmu, z, pm, X);
+        status |= me2c_rotated<MO>(v, cell2c_id(v, in1, in2, SLOT_S, 0), in1, in2, nmu, z, pm, dm, X);
         for (int i = 0; i < nb; i++) v.mat[FB_MAT_S][bo + i] = X[i];
       }
       if (mask & 2) {
-        status |= me2c_rotated(v, cell2c_id(v, in1, in2, SLOT_T, 0), in1, in2, nmu, z, pm, X);
+        status |= me2c_rotated<MO>(v, cell2c_id(v, in1, in2, SLOT_T, 0), in1, in2, nmu, z, pm, dm, X);
         for (int i = 0; i < nb; i++) v.mat[FB_MAT_T][bo + i] = X[i];
       }
       if (mask & 4) {
-        status |= me2c_rotated(v, cell2c_id(v, in1, in2, SLOT_DIP, 0), in1, in2, nmu, z, pm, X);
+        status |= me2c_rotated<MO>(v, cell2c_id(v, in1, in2, SLOT_DIP, 0), in1, in2, nmu, z, pm, dm, X);
         for (int i = 0; i < nb; i++) v.mat[FB_MAT_DIP][bo + i] = X[i];
       }
       if (mask & 8) {
@@ -99,20 +112,20 @@ __global__ __launch_bounds__(W2C) void k_2c(DView v, int mask) {
         for (int i = 0; i < nb; i++) acc[i] = 0.0;
         const bool self = (v.neigh_j[p] == ia - St.atom_off + 1 && mb == 0);
         if (!self) {
-          status |= me2c_rotated(v, cell2c_id(v, in1, in2, SLOT_ONL, 0), in1, in2, nmu, z, pm, X);
+          status |= me2c_rotated<MO>(v, cell2c_id(v, in1, in2, SLOT_ONL, 0), in1, in2, nmu, z, pm, dm, X);
           for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i] * FB_P_EQ2;
           for (int isorp = 1; isorp <= v.sp.nssh[in1 - 1]; isorp++) {
             double dQ = v.dQ[ia * FB_MAXSH + isorp - 1];
             if (dQ == 0.0) continue;
-            status |= me2c_rotated(v, cell2c_id(v, in1, in2, SLOT_ONL, isorp), in1, in2, nmu, z, pm, X);
+            status |= me2c_rotated<MO>(v, cell2c_id(v, in1, in2, SLOT_ONL, isorp), in1, in2, nmu, z, pm, dm, X);
             for (int i = 0; i < nb; i++) acc[i] = acc[i] + dQ * X[i] * FB_P_EQ2;
           }
-          status |= me2c_rotated(v, cell2c_id(v, in1, in2, SLOT_ONR, 0), in1, in2, nmu, z, pm, X);
+          status |= me2c_rotated<MO>(v, cell2c_id(v, in1, in2, SLOT_ONR, 0), in1, in2, nmu, z, pm, dm, X);
           for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i] * FB_P_EQ2;
           for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
             double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
             if (dQ == 0.0) continue;
-            status |= me2c_rotated(v, cell2c_id(v, in1, in2, SLOT_ONR, isorp), in1, in2, nmu, z, pm, X);
+            status |= me2c_rotated<MO>(v, cell2c_id(v, in1, in2, SLOT_ONR, isorp), in1, in2, nmu, z, pm, dm, X);
             for (int i = 0; i < nb; i++) acc[i] = acc[i] + dQ * X[i] * FB_P_EQ2;
           }
         }
@@ -122,7 +135,7 @@ __global__ __launch_bounds__(W2C) void k_2c(DView v, int mask) {
         // atom term <i|V_j|i> (M_assemble_2c_DOGS.f90:763-812)
         double rend = rc1min + v.sp.rcmin[in2 - 1];
         double smooth = d_smoother(z, rend, 0.8);
-        status |= me2c_rotated(v, cell2c_id(v, in1, in2, SLOT_ATOM, 0), in1, in1, nmu, z, pm, X);
+        status |= me2c_rotated<MO>(v, cell2c_id(v, in1, in2, SLOT_ATOM, 0), in1, in1, nmu, z, pm, dm, X);
         const int na = nmu * nmu;
         for (int i = 0; i < na; i++) atom_c[i] = X[i] * FB_P_EQ2;
         bool charged = false;
@@ -134,7 +147,7 @@ __global__ __launch_bounds__(W2C) void k_2c(DView v, int mask) {
           const double *sself = v.mat[FB_MAT_S] + v.boff[pself];
           for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
             double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
-            status |= me2c_rotated(v, cell2c_id(v, in1, in2, SLOT_ATOM, isorp), in1, in1, nmu, z, pm, X);
+            status |= me2c_rotated<MO>(v, cell2c_id(v, in1, in2, SLOT_ATOM, isorp), in1, in1, nmu, z, pm, dm, X);
             for (int i = 0; i < na; i++) {
               double emnpl = (z > 1.0e-4) ? sself[i] / z : 0.0;
               s_atom[isorp][lane][i] = FB_P_EQ2 * dQ * ((1.0 - smooth) * emnpl + smooth * X[i]);
@@ -148,24 +161,33 @@ __global__ __launch_bounds__(W2C) void k_2c(DView v, int mask) {
       // neighbour's terms one at a time (an absent charged term is an exact zero)
       for (int i = 0; i < NB; i++) s_atom[0][lane][i] = atom_c[i];
       __syncthreads();
-      if (lane < nmu * nmu) {
-        int kmax = min(W2C, n - c0);
-        const int nq = v.any_dq ? FB_MAXSH : 0;
+      const int kmax = min(W2C, n - c0);
+      const int nq = v.any_dq ? FB_MAXSH : 0;
+      for (int j = 0; j < NE; j++) {
+        const int e = lane + j * W2C;
+        if (e >= nmu * nmu) break;
         for (int kk = 0; kk < kmax; kk++) {
-          acc_atom = acc_atom + s_atom[0][kk][lane];
-          for (int q = 1; q <= nq; q++) acc_atom = acc_atom + s_atom[q][kk][lane];
+          acc_atom[j] = acc_atom[j] + s_atom[0][kk][e];
+          for (int q = 1; q <= nq; q++) acc_atom[j] = acc_atom[j] + s_atom[q][kk][e];
         }
       }
       __syncthreads();
     }
   }
-  if ((mask & 8) && lane < nmu * nmu) v.mat[FB_MAT_VNA][v.boff[pself] + lane] = acc_atom;
+  if (mask & 8)
+    for (int j = 0; j < NE; j++) {
+      const int e = lane + j * W2C;
+      if (e < nmu * nmu) v.mat[FB_MAT_VNA][v.boff[pself] + e] = acc_atom[j];
+    }
   if (status) atomicOr(v.status, status);
 }
 
 extern "C" int fbk_2c(const DView &v, int mask, hipStream_t st) {
   if (v.natoms == 0) return 0;
-  hipLaunchKernelGGL(k_2c, dim3(v.natoms), dim3(W2C), 0, st, v, mask);
+  if (fb_wide_species(v.sp))
+    hipLaunchKernelGGL(k_2c<FB_MOD>, dim3(v.natoms), dim3(Gen2c<FB_MOD>::W), 0, st, v, mask);
+  else
+    hipLaunchKernelGGL(k_2c<FB_MO>, dim3(v.natoms), dim3(Gen2c<FB_MO>::W), 0, st, v, mask);
   return (int)hipGetLastError();
 }
 

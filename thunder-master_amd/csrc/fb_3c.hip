@@ -67,10 +67,22 @@ __device__ inline void d_geo3(const DView &v, int4 tr, Geo3 &G) {
 
 // terms per triplet: the neutral one plus one per charged shell of the third atom
 #define KT_GEN (1 + FB_MAXSH)
-#define T3G 64   // triplets per tile of the generic kernel (its LDS holds KT_GEN terms each)
 
-__global__ __launch_bounds__(T3G) void k_3c_fp64(DView v) {
-  __shared__ double s_c[T3G][KT_GEN][NB + 1];
+// the generic kernel's sizes: s/p species (MO 4), or with d shells (MO 9)
+template <int MO> struct Gen3c;
+template <> struct Gen3c<FB_MO> {
+  static constexpr int T = 64;     // triplets per tile (the LDS holds KT_GEN terms of each)
+  static constexpr int NME = 10;   // non-zero bcna matrix elements
+};
+template <> struct Gen3c<FB_MOD> {
+  static constexpr int T = 16;
+  static constexpr int NME = FB_NME3_WIDE;
+};
+
+template <int MO>
+__global__ __launch_bounds__(Gen3c<MO>::T) void k_3c_fp64(DView v) {
+  constexpr int T3G = Gen3c<MO>::T, NBG = MO * MO;
+  __shared__ double s_c[T3G][KT_GEN][NBG + 1];
   __shared__ int s_n[T3G];
   const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3G;
   const int64_t te = min(ts + T3G, v.ntrip);
@@ -80,8 +92,8 @@ __global__ __launch_bounds__(T3G) void k_3c_fp64(DView v) {
   const bool own = my <= last;
   int64_t a = 0, b = 0;
   if (own) { a = v.seg_off[my]; b = v.seg_off[my + 1]; }
-  double acc[NB];
-  for (int i = 0; i < NB; i++) acc[i] = 0.0;
+  double acc[NBG];
+  for (int i = 0; i < NBG; i++) acc[i] = 0.0;
   int status = 0;
   for (int64_t c0 = r0; c0 < r1; c0 += T3G) {
     const int64_t t = c0 + threadIdx.x;
@@ -90,12 +102,13 @@ __global__ __launch_bounds__(T3G) void k_3c_fp64(DView v) {
       const int4 tr = v.trip[t];
       Geo3 G;
       d_geo3(v, tr, G);
-      double pm[9], hm[NB], bx[NB];
+      double pm[9], dm[MO == FB_MOD ? 25 : 1], hm[NBG], bx[NBG];
       d_pmat<double>(G.eps, pm);
+      if (MO == FB_MOD) d_dmat(G.eps, dm);
       const int nb = G.nmu * G.nnu;
       int gid = group3c_id(v, G.in1, G.in2, G.indna, 0);
-      status |= d_getMEs_3c<NME3>(v, gid, G.x, G.z, G.cost, G.nmu, G.nnu, hm);
-      d_rotate<double>(v.sp, G.in1, G.in2, pm, hm, G.nmu, bx);
+      status |= d_getMEs_3c<Gen3c<MO>::NME>(v, gid, G.x, G.z, G.cost, G.nmu, G.nnu, hm);
+      d_rotate<double>(v.sp, G.in1, G.in2, pm, hm, G.nmu, bx, dm);
       for (int i = 0; i < nb; i++) s_c[threadIdx.x][0][i] = bx[i] * FB_P_EQ2;
       nt = 1;
       bool charged = false;
@@ -109,8 +122,8 @@ __global__ __launch_bounds__(T3G) void k_3c_fp64(DView v) {
           double dQ = v.dQ[tr.x * FB_MAXSH + isorp - 1];
           if (dQ == 0.0) continue;   // the reference adds an exact zero here
           gid = group3c_id(v, G.in1, G.in2, G.indna, isorp);
-          status |= d_getMEs_3c<NME3>(v, gid, G.x, G.z, G.cost, G.nmu, G.nnu, hm);
-          d_rotate<double>(v.sp, G.in1, G.in2, pm, hm, G.nmu, bx);
+          status |= d_getMEs_3c<Gen3c<MO>::NME>(v, gid, G.x, G.z, G.cost, G.nmu, G.nnu, hm);
+          d_rotate<double>(v.sp, G.in1, G.in2, pm, hm, G.nmu, bx, dm);
           for (int i = 0; i < nb; i++) {
             double sterm = sb[i] / 2.0, dterm = db[i] / G.z;
             double emnpl = (sterm - dterm) / G.d13 + (sterm + dterm) / G.d23;
@@ -126,7 +139,7 @@ __global__ __launch_bounds__(T3G) void k_3c_fp64(DView v) {
       int64_t k0 = max(a, c0), k1 = min(b, c0 + T3G);
       for (int64_t k = k0; k < k1; k++)
         for (int j = 0; j < s_n[k - c0]; j++)
-          for (int i = 0; i < NB; i++) acc[i] = acc[i] + s_c[k - c0][j][i];
+          for (int i = 0; i < NBG; i++) acc[i] = acc[i] + s_c[k - c0][j][i];
     }
     __syncthreads();
   }
@@ -144,7 +157,13 @@ __global__ __launch_bounds__(T3G) void k_3c_fp64(DView v) {
 
 extern "C" int fbk_3c_fp64(const DView &v, hipStream_t st) {
   if (v.ntrip == 0) return 0;
-  hipLaunchKernelGGL(k_3c_fp64, dim3((unsigned)((v.ntrip + T3G - 1) / T3G)), dim3(T3G), 0, st, v);
+  if (fb_wide_species(v.sp)) {
+    constexpr int T = Gen3c<FB_MOD>::T;
+    hipLaunchKernelGGL(k_3c_fp64<FB_MOD>, dim3((unsigned)((v.ntrip + T - 1) / T)), dim3(T), 0, st, v);
+  } else {
+    constexpr int T = Gen3c<FB_MO>::T;
+    hipLaunchKernelGGL(k_3c_fp64<FB_MO>, dim3((unsigned)((v.ntrip + T - 1) / T)), dim3(T), 0, st, v);
+  }
   return (int)hipGetLastError();
 }
 

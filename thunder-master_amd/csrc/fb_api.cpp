@@ -761,7 +761,7 @@ int fb_set_species(fb_ctx *ctx, int ns, const int *nZ, const int *nssh, const in
       sp.rcmax[i] = std::max(sp.rcmax[i], rcA[i * FB_MAXSH + k]);
       sp.rcmin[i] = std::min(sp.rcmin[i], rcA[i * FB_MAXSH + k]);
     }
-    if (sp.norb[i] > 4) return ctx->fail(FB_E_UNSUPPORTED, "species: more than 4 orbitals (s and p shells only)");
+    if (sp.norb[i] > FB_MOD) return ctx->fail(FB_E_UNSUPPORTED, "species: more than 9 orbitals");
     ctx->nZ[i] = nZ ? nZ[i] : 0;
   }
   ctx->sp = sp;
@@ -778,7 +778,7 @@ int fb_set_table_2c(fb_ctx *ctx, int is, int js, int itype, int isub, int nME, i
                     const int *nu, const int *mvalue, const double *fdata) {
   if (!ctx || !ctx->species_set) return ctx ? ctx->fail(FB_E_ORDER, "set_table_2c before set_species") : FB_E_ARG;
   if (is < 1 || js < 1 || is > ctx->sp.ns || js > ctx->sp.ns || isub < 0 || isub >= FB_MAXSUB2 || nx < 7 ||
-      nME < 1 || nME > 16 || !mu || !nu || !mvalue || !fdata)
+      nME < 1 || nME > FB_NME3_WIDE || !mu || !nu || !mvalue || !fdata)
     return ctx->fail(FB_E_ARG, "set_table_2c: bad arguments");
   HCell2c c;
   c.is = is; c.js = js; c.itype = itype; c.isub = isub; c.nME = nME; c.nx = nx; c.xmax = xmax;
@@ -800,7 +800,7 @@ int fb_set_table_3c(fb_ctx *ctx, int is, int js, int ks, int itype, int isub, in
                     const double *fdata) {
   if (!ctx || !ctx->species_set) return ctx ? ctx->fail(FB_E_ORDER, "set_table_3c before set_species") : FB_E_ARG;
   if (is < 1 || js < 1 || ks < 1 || is > ctx->sp.ns || js > ctx->sp.ns || ks > ctx->sp.ns || isub < 0 ||
-      isub >= FB_MAXSUB3 || itheta < 1 || itheta > FB_NTHETA || nx < 4 || ny < 4 || nME < 1 || nME > 16 || !mu ||
+      isub >= FB_MAXSUB3 || itheta < 1 || itheta > FB_NTHETA || nx < 4 || ny < 4 || nME < 1 || nME > FB_NME3_WIDE || !mu ||
       !nu || !mvalue || !fdata)
     return ctx->fail(FB_E_ARG, "set_table_3c: bad arguments");
   HCell3c c;
@@ -870,6 +870,7 @@ int fb_commit_tables(fb_ctx *ctx) {
   if (!ctx || !ctx->species_set) return ctx ? ctx->fail(FB_E_ORDER, "commit_tables before set_species") : FB_E_ARG;
   FB_HIP(hipSetDevice(ctx->device));
   const int ns = ctx->sp.ns;
+  const bool wide = fb_wide_species(ctx->sp);
   std::vector<int> idx2((size_t)ns * ns * FB_NSLOT2 * FB_MAXSUB2, -1);
   std::vector<DCell2c> cells;
   std::vector<double> tab2;
@@ -886,7 +887,8 @@ int fb_commit_tables(fb_ctx *ctx) {
       const int *dim = shell ? ctx->sp.nssh : ctx->sp.norb;
       int rows = dim[c.is - 1], cols = atom ? rows : dim[c.js - 1];
       if (slot == SLOT_VNL) cols = ctx->hpp.norb_pp[c.js - 1];   // PP projectors of the second species
-      if (c.nME > 16) return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: 2c cell with more than 16 elements");
+      if (c.nME > (wide ? FB_NME2_WIDE : 16))
+        return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: 2c cell with more matrix elements than the kernels hold");
       for (int k = 0; k < c.nME; k++)
         if (c.mu[k] < 1 || c.mu[k] > rows || c.nu[k] < 1 || c.nu[k] > cols)
           return ctx->fail(FB_E_ARG, "commit_tables: 2c mu/nu outside the orbital block");
@@ -939,7 +941,8 @@ int fb_commit_tables(fb_ctx *ctx) {
     for (int t = 0; t < FB_NTHETA; t++)
       if (!th[t]) return ctx->fail(FB_E_NO_TABLE, "commit_tables: a 3c group lacks a theta cell");
     const HCell3c *c1 = th[0];
-    if (c1->nME > 10) return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: bcna cell with more than 10 elements");
+    if (c1->nME > (wide ? FB_NME3_WIDE : 10))
+      return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: bcna cell with more matrix elements than the kernels hold");
     for (int t = 0; t < FB_NTHETA; t++)
       for (int k = 0; k < th[t]->nME; k++)
         if (th[t]->mu[k] < 1 || th[t]->mu[k] > ctx->sp.norb[th[t]->is - 1] || th[t]->nu[k] < 1 ||
@@ -1022,7 +1025,8 @@ int fb_commit_tables(fb_ctx *ctx) {
       if (th[t]->nME != c1->nME || th[t]->nx != c1->nx || th[t]->ny != c1->ny || th[t]->xmax != c1->xmax ||
           th[t]->ymax != c1->ymax)
         return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: theta cells of a 3c group differ in grid");
-      if (th[t]->nME > 16) return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: 3c cell with more than 16 elements");
+      if (th[t]->nME > (wide ? FB_NME3_WIDE : 16))
+        return ctx->fail(FB_E_UNSUPPORTED, "commit_tables: 3c cell with more matrix elements than the kernels hold");
       for (int k = 0; k < th[t]->nME; k++)
         if (th[t]->mu[k] < 1 || th[t]->mu[k] > dim[is - 1] || th[t]->nu[k] < 1 || th[t]->nu[k] > dim[js - 1])
           return ctx->fail(FB_E_ARG, "commit_tables: 3c mu/nu outside the block");
@@ -1623,6 +1627,16 @@ int fb_get_common(fb_ctx *ctx, int is, int *ncommon, int *iatom_j, int *iatom_b,
 }
 
 // ---------------------------------------------------------------- assembly
+// The density, pseudopotential and force kernels hold blocks of s and p
+// species (at most FB_MO orbitals); species with d shells (or more than four
+// orbitals) are assembled by S, T, dipole_z, vna_2c and vna_3c only.
+static int sp_orbitals_only(fb_ctx *ctx, const char *what) {
+  if (!fb_wide_species(ctx->sp)) return FB_OK;
+  return ctx->fail(FB_E_UNSUPPORTED, (std::string(what) +
+                                      ": a species has more than 4 orbitals (d shells); only S, T, dipole_z, "
+                                      "vna_2c and vna_3c take them").c_str());
+}
+
 static int pre_assemble(fb_ctx *ctx, const char *what) {
   if (!ctx) return FB_E_ARG;
   if (!ctx->lists_ok) return ctx->fail(FB_E_ORDER, (std::string(what) + " before build_lists").c_str());
@@ -2430,6 +2444,7 @@ static int prep_rho(fb_ctx *ctx) {
 
 static int pre_rho(fb_ctx *ctx, const char *what) {
   RC(pre_assemble(ctx, what));
+  RC(sp_orbitals_only(ctx, what));
   for (auto &s : ctx->structs)
     if (!s.has_qin) return ctx->fail(FB_E_ORDER, (std::string(what) + ": shell charges not set (fb_set_qin)").c_str());
   return prep_rho(ctx);
@@ -2493,6 +2508,7 @@ int fb_set_species_pp(fb_ctx *ctx, const int *nssh_pp, const int *lssh_pp, const
                       const double *cl) {
   if (!ctx || !nssh_pp || !lssh_pp || !rcutoff_pp || !cl) return ctx ? ctx->fail(FB_E_ARG, "set_species_pp") : FB_E_ARG;
   if (!ctx->species_set) return ctx->fail(FB_E_ORDER, "set_species_pp before set_species");
+  RC(sp_orbitals_only(ctx, "set_species_pp"));
   DPP h{};
   for (int i = 0; i < ctx->sp.ns; i++) {
     if (nssh_pp[i] < 0 || nssh_pp[i] > FB_MAXSH) return ctx->fail(FB_E_ARG, "set_species_pp: nssh_PP out of range");
@@ -2876,6 +2892,7 @@ static int prep_forces(fb_ctx *ctx) {
 
 int fb_set_denmat(fb_ctx *ctx, int is, const double *rho) {
   if (!ctx) return FB_E_ARG;
+  RC(sp_orbitals_only(ctx, "set_denmat"));
   if (!ctx->lists_ok) return ctx->fail(FB_E_ORDER, "set_denmat before build_lists");
   if (is < 0 || is >= (int)ctx->structs.size() || !rho) return ctx->fail(FB_E_ARG, "set_denmat arguments");
   FB_HIP(hipSetDevice(ctx->device));
@@ -2892,6 +2909,7 @@ int fb_set_denmat(fb_ctx *ctx, int is, const double *rho) {
 
 static int pre_dassemble(fb_ctx *ctx, const char *what) {
   RC(pre_assemble(ctx, what));
+  RC(sp_orbitals_only(ctx, what));
   RC(prep_forces(ctx));
   return FB_OK;
 }

@@ -13,7 +13,6 @@
 
 #include "fb_internal.h"
 
-#define FB_MO 4   // orbitals per species handled by the kernels (s and p shells)
 
 // Tile of workgroup b in a grid of n.  The dispatcher hands workgroups to the
 // eight XCDs round-robin (workgroup b to XCD b mod 8) and each XCD has its own
@@ -98,12 +97,50 @@ __device__ __forceinline__ void d_pmat(const E *eps, T *pm) {
 #undef EPS
 }
 
-// X = L M R^T shell pair by shell pair (s and p shells), loop order of
-// c.SYSTEM/M_rotations.f90:141-164.  M and X are column-major with leading
-// dimension nmu.
+// The d-orbital rotation of twister (c.SYSTEM/M_rotations.f90:272-300):
+// column-major dm[5 (m - 1) + (k - 1)] = dmat(k, m) from eps (column-major
+// eps(i, j) = eps[3 (j - 1) + (i - 1)]), with the A matrices of
+// include/constants.h:26-35 and the reference's sums in its order (terms with
+// a zero A entry skipped, as the reference skips them).
+__device__ inline void d_dmat(const double *eps, double *dm) {
+  // amat(ix, jx, imu), column-major, imu = xy, yz, z^2, xz, x^2-y^2
+  const double r3 = 1.0 / (2.0 * sqrt(3.0));
+  const double amat[5][3][3] = {
+      {{0.0, 0.5, 0.0}, {0.5, 0.0, 0.0}, {0.0, 0.0, 0.0}},
+      {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.5}, {0.0, 0.5, 0.0}},
+      {{-r3, 0.0, 0.0}, {0.0, -r3, 0.0}, {0.0, 0.0, 2.0 / (2.0 * sqrt(3.0))}},
+      {{0.0, 0.0, 0.5}, {0.0, 0.0, 0.0}, {0.5, 0.0, 0.0}},
+      {{0.5, 0.0, 0.0}, {0.0, -0.5, 0.0}, {0.0, 0.0, 0.0}}};
+#define EPSD(i, j) eps[3 * ((j) - 1) + (i) - 1]
+  for (int imu = 0; imu < 5; imu++) {
+    double l11 = 0.0, l12 = 0.0, l13 = 0.0, l32 = 0.0, l33 = 0.0;
+    for (int jx = 1; jx <= 3; jx++)
+      for (int ix = 1; ix <= 3; ix++) {
+        const double a = amat[imu][jx - 1][ix - 1];
+        if (a != 0.0) {
+          l11 = l11 + a * EPSD(jx, 1) * EPSD(ix, 1);
+          l12 = l12 + a * EPSD(jx, 1) * EPSD(ix, 2);
+          l13 = l13 + a * EPSD(jx, 1) * EPSD(ix, 3);
+          l32 = l32 + a * EPSD(jx, 3) * EPSD(ix, 2);
+          l33 = l33 + a * EPSD(jx, 3) * EPSD(ix, 3);
+        }
+      }
+    dm[0 * 5 + imu] = 2.0 * l12;
+    dm[1 * 5 + imu] = 2.0 * l32;
+    dm[2 * 5 + imu] = sqrt(3.0) * l33;
+    dm[3 * 5 + imu] = 2.0 * l13;
+    dm[4 * 5 + imu] = 2.0 * l11 + l33;
+  }
+#undef EPSD
+}
+
+// X = L M R^T shell pair by shell pair, loop order of
+// c.SYSTEM/M_rotations.f90:141-164: L and R are 1 (s), pm (p, column-major
+// pm[3 m + k]) or dm (d, d_dmat; only read when a species has a d shell).  M
+// and X are column-major with leading dimension nmu.
 template <typename T>
 __device__ inline void d_rotate(const DSpecies &sp, int in1, int in3, const T *pm, const T *M, int nmu,
-                                T *X) {
+                                T *X, const T *dm = nullptr) {
   int nnu = sp.norb[in3 - 1];
   for (int i = 0; i < nmu * nnu; i++) X[i] = (T)0;
   int n1 = 0;
@@ -116,8 +153,8 @@ __device__ inline void d_rotate(const DSpecies &sp, int in1, int in3, const T *p
         for (int m1 = 0; m1 < 2 * l1 + 1; m1++)
           for (int k2 = 0; k2 < 2 * l2 + 1; k2++)
             for (int k1 = 0; k1 < 2 * l1 + 1; k1++) {
-              T left = (l1 == 0) ? (T)1 : pm[3 * m1 + k1];
-              T right = (l2 == 0) ? (T)1 : pm[3 * m2 + k2];
+              T left = (l1 == 0) ? (T)1 : (l1 == 1 ? pm[3 * m1 + k1] : dm[5 * m1 + k1]);
+              T right = (l2 == 0) ? (T)1 : (l2 == 1 ? pm[3 * m2 + k2] : dm[5 * m2 + k2]);
               T *x = &X[(n2 + k2) * nmu + (n1 + k1)];
               *x = *x + left * M[(n2 + m2) * nmu + (n1 + m1)] * right;
             }

@@ -43,6 +43,19 @@ struct DSpecies {
   double rcmin[FB_MAXSPECIES];   // min shell rcutoffA (smoothing)
 };
 
+#define FB_MO 4           // orbitals per species of the s/p kernels
+#define FB_MOD 9          // orbitals per species of the generic binary64 kernels (s, p and d shells)
+#define FB_NME2_WIDE 25   // 2c matrix elements the generic kernels hold with d shells (19 for s, p, d)
+#define FB_NME3_WIDE 48   // 3c matrix elements likewise (45 for s, p, d)
+
+// some species has more than FB_MO orbitals: the generic kernels take their
+// FB_MOD sizes, and only the H+S assemblers accept the species
+inline bool fb_wide_species(const DSpecies &sp) {
+  for (int i = 0; i < sp.ns; i++)
+    if (sp.norb[i] > FB_MO) return true;
+  return false;
+}
+
 struct DCell2c {
   int nME, nx;
   double xmax, dx;

@@ -39,8 +39,8 @@ P_RHO_ONTOPL, P_RHO_ONTOPR, P_RHO_ATOM = 15, 16, 17
 P_RHOS_ONTOPL, P_RHOS_ONTOPR, P_RHOS_ATOM = 19, 20, 21
 P_RHO_3C, P_RHOS_3C, P_BCNA = 1, 2, 3
 
-MASS = {1: 1.008, 3: 6.94, 6: 12.011, 7: 14.007, 8: 15.999, 14: 28.0855}
-SYMBOL = {1: "H", 6: "C", 7: "N", 8: "O", 14: "Si"}
+MASS = {1: 1.008, 3: 6.94, 6: 12.011, 7: 14.007, 8: 15.999, 14: 28.0855, 26: 55.845}
+SYMBOL = {1: "H", 6: "C", 7: "N", 8: "O", 14: "Si", 26: "Fe"}
 AMU_G = 1.66053906660e-24
 
 
@@ -493,6 +493,8 @@ H_SP = Species(1, [0], [4.5])
 C_SP = Species(6, [0, 1], [5.0, 5.0])
 N_SP = Species(7, [0, 1], [5.0, 5.0])
 O_SP = Species(8, [0, 1], [5.0, 5.0])
+# a transition-metal-like species with s, p and d shells (9 orbitals)
+FE_SPD = Species(26, [0, 1, 2], [5.0, 5.0, 4.5])
 
 DIAMOND_BASIS = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0],
                           [.25, .25, .25], [.25, .75, .75], [.75, .25, .75], [.75, .75, .25]])
