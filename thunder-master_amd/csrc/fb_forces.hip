@@ -951,10 +951,14 @@ __device__ __forceinline__ void dcontract2c_sp(const double *pm, const double *d
   }
 }
 
-template <int CLS>
+// PART 0: the Dblock of interaction d (0 S, 1 T, 2 dipole_z) into dst
+// (Dblock(3, norb_mu, norb_nu) column-major); PART 1: the vna_ontop force of
+// the pair; PART 2: the neutral vna_atom term; PART 3: the charged vna_atom
+// terms, added to what PART 2 stored
+template <int CLS, int PART>
 __device__ __forceinline__ int pair_d2c_sp(const DView &v, const DForce &Fo, int mask, int64_t p, int ia, int ja,
                                            int in1, int in2, bool self, double z, const double *pm,
-                                           const double *deps, const double *eta) {
+                                           const double *deps, const double *eta, int d = 0, double *dst = nullptr) {
   constexpr int NMU = (CLS & 2) ? 4 : 1, NNU = (CLS & 1) ? 4 : 1, NBL = NMU * NNU;
   constexpr int NME = nme2c_of(CLS);
   constexpr int CLSA = (CLS & 2) ? CLS_SPSP : CLS_SS;   // atom term on iatom's orbitals
@@ -965,14 +969,11 @@ __device__ __forceinline__ int pair_d2c_sp(const DView &v, const DForce &Fo, int
   L.ileft = 1;
   double F[6], dF[6], X[16], DM[6], dpm[9];
   const int64_t bo = v.boff[p];
-  const int slots[3] = {SLOT_S, SLOT_T, SLOT_DIP};
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    if (!(mask & (1 << d))) continue;
-    int st = dinterp2c_sp<NME>(v, cell2c_of(v, in1, in2, slots[d], 0), z, L, F, dF);
+  if (PART == 0) {
+    const int slot = d == 0 ? SLOT_S : (d == 1 ? SLOT_T : SLOT_DIP);
+    int st = dinterp2c_sp<NME>(v, cell2c_of(v, in1, in2, slot, 0), z, L, F, dF);
     status |= st;
-    if (st) continue;
-    double *D = Fo.dmat[d] + 3 * bo;
+    if (st) return status;
 #pragma unroll
     for (int ix = 0; ix < 3; ix++) {
 #pragma unroll
@@ -980,11 +981,13 @@ __device__ __forceinline__ int pair_d2c_sp(const DView &v, const DForce &Fo, int
       d_dpmat(deps, ix + 1, dpm);
       drot2c_sp<CLS>(pm, dpm, F, DM, X);
 #pragma unroll
-      for (int i = 0; i < NBL; i++) D[ix + 3 * i] = X[i];
+      for (int i = 0; i < NBL; i++) dst[ix + 3 * i] = X[i];
     }
+    return status;
   }
-  if (mask & 8) {
-    double fo[3] = {0.0, 0.0, 0.0}, fa[3] = {0.0, 0.0, 0.0};
+  if (PART == 1) {
+    // vna_ontop (M_Dassemble_2c_DOGS.f90:679-863)
+    double fo[3] = {0.0, 0.0, 0.0};
     if (!self) {
       double rho[NBL];
 #pragma unroll
@@ -1006,16 +1009,30 @@ __device__ __forceinline__ int pair_d2c_sp(const DView &v, const DForce &Fo, int
         }
       }
     }
+#pragma unroll
+    for (int q = 0; q < 3; q++) Fo.fontop[3 * p + q] = fo[q];
+    return status;
+  }
+  {
+    // vna_atom (M_Dassemble_2c_DOGS.f90:866-1199): PART 2 the neutral term,
+    // PART 3 continues the sum with the charged shells of jatom
+    double fa[3] = {0.0, 0.0, 0.0};
     const int64_t pself = v.noff[ia] + v.neigh_self[ia] - 1;
     double rs[NA];
 #pragma unroll
     for (int i = 0; i < NA; i++) rs[i] = Fo.rho[v.boff[pself] + i];
-    int st = dinterp2c_sp<NMEA>(v, cell2c_of(v, in1, in2, SLOT_ATOM, 0), z, L, F, dF);
-    status |= st;
-    if (!st) dcontract2c_sp<CLSA>(pm, deps, eta, z, 1.0e-3, F, dF, rs, 0.0, false, fa);
+    int st = 0;
+    if (PART == 2) {
+      st = dinterp2c_sp<NMEA>(v, cell2c_of(v, in1, in2, SLOT_ATOM, 0), z, L, F, dF);
+      status |= st;
+      if (!st) dcontract2c_sp<CLSA>(pm, deps, eta, z, 1.0e-3, F, dF, rs, 0.0, false, fa);
+    }
     bool charged = false;
-    for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) charged |= (v.dQ[ja * FB_MAXSH + isorp - 1] != 0.0);
+    if (PART == 3)
+      for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) charged |= (v.dQ[ja * FB_MAXSH + isorp - 1] != 0.0);
     if (charged) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) fa[q] = Fo.fatom[3 * p + q];
       const double rend = v.sp.rcmin[in1 - 1] + v.sp.rcmin[in2 - 1];
       const double smooth = d_smoother(z, rend, 0.8), Dsmooth = d_Dsmoother(z, rend, 0.8);
       const double *sself = v.mat[FB_MAT_S] + v.boff[pself];
@@ -1044,51 +1061,86 @@ __device__ __forceinline__ int pair_d2c_sp(const DView &v, const DForce &Fo, int
         }
       }
     }
+    if (PART == 2 || charged)
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-      Fo.fontop[3 * p + q] = fo[q];
-      Fo.fatom[3 * p + q] = fa[q];
-    }
+      for (int q = 0; q < 3; q++) Fo.fatom[3 * p + q] = fa[q];
   }
   return status;
 }
 
 #define WD2S 128
 
+// Three kernels over the pairs, one lane per pair.  PART 0 forms the Dblocks
+// of S, T and dipole_z (mask bits 1, 2, 4) one interaction at a time into
+// LDS, packed as in the output, and the workgroup stores the contiguous block
+// range of its pairs with consecutive lanes on consecutive addresses (a lane
+// writing its own 48-value block would leave partly written cache lines
+// behind).  PARTs 1 to 3 form the vna_ontop force and the neutral and charged
+// vna_atom terms (bit 8; PART 3 only for builds with shell charges).
+// Splitting the work keeps each kernel's registers below the spill point.
+template <int PART>
 __global__ __launch_bounds__(WD2S) void k_d2c_sp(DView v, DForce Fo, int mask) {
-  const int64_t p = (int64_t)blockIdx.x * WD2S + threadIdx.x;
-  if (p >= v.npairs) return;
-  const int ia = v.pair_atom[p];
-  const int s = v.atom_struct[ia];
-  const int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
-  const int mb = v.neigh_b[p];
-  const int in1 = v.imass[ia], in2 = v.imass[ja];
-  const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
-  double r1[3], r2[3], sighat[3], eps[9], pm[9], deps[27], eta[3];
+  __shared__ double sD[PART == 0 ? 3 * 16 * WD2S : 1];
+  const int64_t p0 = (int64_t)blockIdx.x * WD2S, pe = min(p0 + (int64_t)WD2S, v.npairs);
+  const int64_t p = p0 + threadIdx.x;
+  const bool act = p < pe;
+  int ia = 0, ja = 0, in1 = 1, in2 = 1, cls = 0;
+  bool self = false;
+  double z = 0.0, pm[9], deps[27], eta[3];
+  if (act) {
+    ia = v.pair_atom[p];
+    const int s = v.atom_struct[ia];
+    ja = v.st[s].atom_off + v.neigh_j[p] - 1;
+    const int mb = v.neigh_b[p];
+    in1 = v.imass[ia];
+    in2 = v.imass[ja];
+    const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+    double r1[3], r2[3], sighat[3], eps[9];
 #pragma unroll
-  for (int q = 0; q < 3; q++) {
-    r1[q] = v.ratom[3 * ia + q];
-    r2[q] = v.ratom[3 * ja + q] + xls[3 * mb + q];
+    for (int q = 0; q < 3; q++) {
+      r1[q] = v.ratom[3 * ia + q];
+      r2[q] = v.ratom[3 * ja + q] + xls[3 * mb + q];
+    }
+    z = d_distance(r1, r2);
+    if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
+    else {
+#pragma unroll
+      for (int q = 0; q < 3; q++) sighat[q] = (r2[q] - r1[q]) / z;
+    }
+    d_epsilon(r2, sighat, eps);
+    d_Depsilon_2c(r1, r2, z, eps, deps);
+    d_pmat<double>(eps, pm);
+#pragma unroll
+    for (int q = 0; q < 3; q++) eta[q] = eps[6 + q];
+    self = (ja == ia && mb == 0);
+    cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
   }
-  const double z = d_distance(r1, r2);
-  if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
-  else {
-#pragma unroll
-    for (int q = 0; q < 3; q++) sighat[q] = (r2[q] - r1[q]) / z;
-  }
-  d_epsilon(r2, sighat, eps);
-  d_Depsilon_2c(r1, r2, z, eps, deps);
-  d_pmat<double>(eps, pm);
-#pragma unroll
-  for (int q = 0; q < 3; q++) eta[q] = eps[6 + q];
-  const bool self = (ja == ia && mb == 0);
-  const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
-  int status;
-  switch (cls) {
-    case 3: status = pair_d2c_sp<3>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta); break;
-    case 2: status = pair_d2c_sp<2>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta); break;
-    case 1: status = pair_d2c_sp<1>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta); break;
-    default: status = pair_d2c_sp<0>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta); break;
+  int status = 0;
+  if (PART == 0) {
+    const int64_t b0 = v.boff[p0], nb = 3 * (v.boff[pe] - b0);
+    double *const dst = act ? sD + 3 * (v.boff[p] - b0) : nullptr;
+    for (int d = 0; d < 3; d++) {
+      if (!(mask & (1 << d))) continue;
+      if (act) {
+        switch (cls) {
+          case 3: status |= pair_d2c_sp<3, 0>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta, d, dst); break;
+          case 2: status |= pair_d2c_sp<2, 0>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta, d, dst); break;
+          case 1: status |= pair_d2c_sp<1, 0>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta, d, dst); break;
+          default: status |= pair_d2c_sp<0, 0>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta, d, dst); break;
+        }
+      }
+      __syncthreads();
+      double *const out = Fo.dmat[d] + 3 * b0;
+      for (int64_t k = threadIdx.x; k < nb; k += WD2S) out[k] = sD[k];
+      __syncthreads();
+    }
+  } else if (act) {
+    switch (cls) {
+      case 3: status = pair_d2c_sp<3, PART>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta); break;
+      case 2: status = pair_d2c_sp<2, PART>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta); break;
+      case 1: status = pair_d2c_sp<1, PART>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta); break;
+      default: status = pair_d2c_sp<0, PART>(v, Fo, mask, p, ia, ja, in1, in2, self, z, pm, deps, eta); break;
+    }
   }
   if (status) atomicOr(v.status, status);
 }
@@ -1885,7 +1937,13 @@ __global__ __launch_bounds__(T3Q) void k_d3c_sp_q(DView v, DForce Fo) {
 
 extern "C" int fbk_d2c_sp(const DView &v, const DForce &f, int mask, hipStream_t st) {
   if (v.npairs == 0) return 0;
-  hipLaunchKernelGGL(k_d2c_sp, dim3((unsigned)((v.npairs + WD2S - 1) / WD2S)), dim3(WD2S), 0, st, v, f, mask);
+  const dim3 grid((unsigned)((v.npairs + WD2S - 1) / WD2S));
+  if (mask & 7) hipLaunchKernelGGL(k_d2c_sp<0>, grid, dim3(WD2S), 0, st, v, f, mask);
+  if (mask & 8) {
+    hipLaunchKernelGGL(k_d2c_sp<1>, grid, dim3(WD2S), 0, st, v, f, mask);
+    hipLaunchKernelGGL(k_d2c_sp<2>, grid, dim3(WD2S), 0, st, v, f, mask);
+    if (v.any_dq) hipLaunchKernelGGL(k_d2c_sp<3>, grid, dim3(WD2S), 0, st, v, f, mask);
+  }
   return (int)hipGetLastError();
 }
 

@@ -467,10 +467,11 @@ struct RShell {         // per (triplet, isorp)
   int rs, ts, nme, m1;  // row and theta strides (elements), ME count (0: no table), 3-point rule
 };
 
-// four waves per SIMD: the binary64 rotation of the unweighted density then
-// spills 48 bytes per lane, which costs less than the lost occupancy
+// the unweighted density at three waves per SIMD (140 VGPRs): at four its
+// binary64 rotation spills 48 bytes per lane, whose scratch traffic doubles
+// the kernel's HBM bytes for a 5 % shorter run; the weighted one fits four
 template <bool W>
-__global__ __launch_bounds__(RW) __attribute__((amdgpu_waves_per_eu(4))) void k_rho_3c_ph(DView v) {
+__global__ __launch_bounds__(RW) __attribute__((amdgpu_waves_per_eu(W ? 4 : 3))) void k_rho_3c_ph(DView v) {
   constexpr int NMX = W ? 4 : 10;          // matrix elements per (triplet, isorp)
   constexpr int NBX = W ? 4 : 16;          // block elements
   constexpr int ROW = NBX + 1;             // LDS row stride (doubles)
