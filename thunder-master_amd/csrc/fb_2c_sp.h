@@ -37,6 +37,46 @@ __device__ __forceinline__ int block2c(const DView &v, int cid, double x, const 
   return 0;
 }
 
+// Block stores of a wavefront whose active lanes all hold sp-sp pairs (16
+// doubles each, so the wavefront's blocks are one contiguous range starting
+// at boff of its first pair): each lane writes its block to an LDS row of 17
+// doubles, and the lanes then store the range with consecutive lanes on
+// consecutive doubles.  A lane storing its own 128-byte block instead touches
+// a separate cache line per lane and element group.  Other wavefronts store
+// per lane.
+struct WStore {
+  bool uni;        // every active lane of the wavefront holds an sp-sp pair
+  int lane, nact;  // lane, active lanes (a prefix of the wavefront)
+  int64_t wb0;     // block offset of the wavefront's first pair
+  double *ws;      // the wavefront's 64 LDS rows
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int NBL>
+__device__ __forceinline__ void put_block(double *m, int64_t bo, const double *X, const WStore &W) {
+  if (NBL == 16 && W.uni) {
+    double *row = W.ws + W.lane * 17;
+#pragma unroll
+    for (int i = 0; i < 16; i++) row[i] = X[i];
+    wave_sync();
+    double *dst = m + W.wb0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int o = k * W.nact + W.lane;
+      dst[o] = W.ws[(o >> 4) * 17 + (o & 15)];
+    }
+    wave_sync();
+  } else {
+#pragma unroll
+    for (int i = 0; i < NBL; i++) m[bo + i] = X[i];
+  }
+}
+
 // atom_c receives the neutral atom term; with KQ > 0 the charged terms of
 // shells 1..KQ go to qrow[(isorp - 1) * QROW + i] (LDS rows that the caller
 // cleared), so that the self-block sum can add each term on its own
@@ -44,25 +84,22 @@ __device__ __forceinline__ int block2c(const DView &v, int cid, double x, const 
 template <int CLS, int KQ>
 __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, int in1, int in2, bool self,
                                       bool keep_vna, double z, const double *pm, int64_t bo, double *atom_c,
-                                      double *qrow) {
+                                      double *qrow, const WStore &W) {
   constexpr int NMU = (CLS & 2) ? 4 : 1, NNU = (CLS & 1) ? 4 : 1, NBL = NMU * NNU;
   constexpr int CLSA = (CLS & 2) ? CLS_SPSP : CLS_SS;   // atom term lives on iatom's orbitals
   int status = 0, grid_cached = -1, ileft = 1;
   double prod[7], X[16];
   if (mask & 1) {
     status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_S, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-    for (int i = 0; i < NBL; i++) v.mat[FB_MAT_S][bo + i] = X[i];
+    put_block<NBL>(v.mat[FB_MAT_S], bo, X, W);
   }
   if (mask & 2) {
     status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_T, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-    for (int i = 0; i < NBL; i++) v.mat[FB_MAT_T][bo + i] = X[i];
+    put_block<NBL>(v.mat[FB_MAT_T], bo, X, W);
   }
   if (mask & 4) {
     status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_DIP, 0), z, pm, grid_cached, ileft, prod, X);
-#pragma unroll
-    for (int i = 0; i < NBL; i++) v.mat[FB_MAT_DIP][bo + i] = X[i];
+    put_block<NBL>(v.mat[FB_MAT_DIP], bo, X, W);
   }
   if (mask & 8) {
     double acc[NBL];
@@ -90,9 +127,16 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
         for (int i = 0; i < NBL; i++) acc[i] = acc[i] + dQ * X[i] * FB_P_EQ2;
       }
     }
+    put_block<NBL>(v.mat[FB_MAT_VNAO], bo, acc, W);
+    const bool clr = !self && !keep_vna;
+    if (NBL == 16 && W.uni) {
+      double *dst = v.mat[FB_MAT_VNA] + W.wb0;
 #pragma unroll
-    for (int i = 0; i < NBL; i++) v.mat[FB_MAT_VNAO][bo + i] = acc[i];
-    if (!self && !keep_vna) {
+      for (int k = 0; k < 16; k++) {
+        const int o = k * W.nact + W.lane;
+        if (__shfl((int)clr, o >> 4)) dst[o] = 0.0;
+      }
+    } else if (clr) {
 #pragma unroll
       for (int i = 0; i < NBL; i++) v.mat[FB_MAT_VNA][bo + i] = 0.0;
     }
@@ -124,7 +168,8 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
 }
 
 // LDS of one workgroup item: the atom terms of its pairs, [0] neutral and
-// [1..KQ] the charged shells
+// [1..KQ] the charged shells.  Before the atom terms are written, the rows of
+// [0] serve each wavefront's block stores (WStore).
 template <int KQ>
 struct S2c {
   double s_at[1 + KQ][WG2][17];
@@ -151,6 +196,9 @@ __device__ __forceinline__ void wg2c_sp(const DView &v, int mask, int item, S2c<
   }
   double racc = 0.0;
   int status = 0;
+  WStore W;
+  W.lane = tid & 63;
+  W.ws = &s_at[0][0][0] + (tid & ~63) * 17;
   for (int64_t c0 = pb; c0 < pe; c0 += WG2) {
     const int64_t p = c0 + tid;
     double atom_c[16];
@@ -185,13 +233,18 @@ __device__ __forceinline__ void wg2c_sp(const DView &v, int mask, int item, S2c<
       const bool self = (ja == ia && mb == 0);
       const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
       const int64_t bo = v.boff[p];
+      // the active lanes are a prefix of the wavefront
+      const int64_t p0 = c0 + (tid & ~63);
+      W.nact = (int)min((int64_t)64, pe - p0);
+      W.uni = __ballot(cls != 3) == 0;
+      W.wb0 = v.boff[p0];
       const bool keep = (mask & FB_2C_SKIP_TGT) && v.pair_tgt[p];
       double *qrow = KQ > 0 ? &s_at[KQ > 0 ? 1 : 0][tid][0] : nullptr;
       switch (cls) {
-        case 3: status |= pair2c<3, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow); break;
-        case 2: status |= pair2c<2, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow); break;
-        case 1: status |= pair2c<1, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow); break;
-        default: status |= pair2c<0, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow); break;
+        case 3: status |= pair2c<3, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow, W); break;
+        case 2: status |= pair2c<2, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow, W); break;
+        case 1: status |= pair2c<1, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow, W); break;
+        default: status |= pair2c<0, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow, W); break;
       }
     }
     if (mask & 8) {
