@@ -57,6 +57,25 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// the block already in the lane's LDS row
+template <int NBL>
+__device__ __forceinline__ void put_row(double *m, int64_t bo, const WStore &W) {
+  const double *row = W.ws + W.lane * 17;
+  if (NBL == 16 && W.uni) {
+    wave_sync();
+    double *dst = m + W.wb0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int o = k * W.nact + W.lane;
+      dst[o] = W.ws[(o >> 4) * 17 + (o & 15)];
+    }
+    wave_sync();
+  } else {
+#pragma unroll
+    for (int i = 0; i < NBL; i++) m[bo + i] = row[i];
+  }
+}
+
 template <int NBL>
 __device__ __forceinline__ void put_block(double *m, int64_t bo, const double *X, const WStore &W) {
   if (NBL == 16 && W.uni) {
@@ -102,7 +121,8 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
     put_block<NBL>(v.mat[FB_MAT_DIP], bo, X, W);
   }
   if (mask & 8) {
-    double acc[NBL];
+    // the ontop sum accumulates in the lane's LDS row
+    double *acc = W.ws + W.lane * 17;
 #pragma unroll
     for (int i = 0; i < NBL; i++) acc[i] = 0.0;
     if (!self) {
@@ -127,7 +147,7 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
         for (int i = 0; i < NBL; i++) acc[i] = acc[i] + dQ * X[i] * FB_P_EQ2;
       }
     }
-    put_block<NBL>(v.mat[FB_MAT_VNAO], bo, acc, W);
+    put_row<NBL>(v.mat[FB_MAT_VNAO], bo, W);
     const bool clr = !self && !keep_vna;
     if (NBL == 16 && W.uni) {
       double *dst = v.mat[FB_MAT_VNA] + W.wb0;
@@ -144,7 +164,7 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
     constexpr int NA = NMU * NMU;
     status |= block2c<CLSA>(v, cell2c_id(v, in1, in2, SLOT_ATOM, 0), z, pm, grid_cached, ileft, prod, X);
 #pragma unroll
-    for (int i = 0; i < NA; i++) atom_c[i] = X[i] * FB_P_EQ2;
+    for (int i = 0; i < 16; i++) atom_c[i] = i < NA ? X[i] * FB_P_EQ2 : 0.0;
     bool charged = false;
     for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) charged |= (v.dQ[ja * FB_MAXSH + isorp - 1] != 0.0);
     if (charged) {
@@ -187,18 +207,11 @@ __device__ __forceinline__ void wg2c_sp(const DView &v, int mask, int item, S2c<
   // reduction slot of this thread: (atom a0 + tid/16, element tid%16)
   const int ra = a0 + (tid >> 4), re = tid & 15;
   const bool rown = ra < a1;
-  int64_t rp0 = 0, rp1 = 0;
-  int rna = 0;
-  if (rown) {
-    rp0 = v.noff[ra];
-    rp1 = v.noff[ra + 1];
-    rna = v.sp.norb[v.imass[ra] - 1];
-  }
   double racc = 0.0;
   int status = 0;
   WStore W;
   W.lane = tid & 63;
-  W.ws = &s_at[0][0][0] + (tid & ~63) * 17;
+  W.ws = &s_at[0][0][0] + __builtin_amdgcn_readfirstlane(tid & ~63) * 17;
   for (int64_t c0 = pb; c0 < pe; c0 += WG2) {
     const int64_t p = c0 + tid;
     double atom_c[16];
@@ -234,10 +247,12 @@ __device__ __forceinline__ void wg2c_sp(const DView &v, int mask, int item, S2c<
       const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
       const int64_t bo = v.boff[p];
       // the active lanes are a prefix of the wavefront
-      const int64_t p0 = c0 + (tid & ~63);
+      const int64_t p0 = c0 + __builtin_amdgcn_readfirstlane(tid & ~63);
       W.nact = (int)min((int64_t)64, pe - p0);
       W.uni = __ballot(cls != 3) == 0;
-      W.wb0 = v.boff[p0];
+      const int64_t wb0 = v.boff[p0];
+      W.wb0 = ((int64_t)__builtin_amdgcn_readfirstlane((int)(wb0 >> 32)) << 32) |
+              (uint32_t)__builtin_amdgcn_readfirstlane((int)wb0);
       const bool keep = (mask & FB_2C_SKIP_TGT) && v.pair_tgt[p];
       double *qrow = KQ > 0 ? &s_at[KQ > 0 ? 1 : 0][tid][0] : nullptr;
       switch (cls) {
@@ -251,9 +266,10 @@ __device__ __forceinline__ void wg2c_sp(const DView &v, int mask, int item, S2c<
 #pragma unroll
       for (int i = 0; i < 16; i++) s_at[0][tid][i] = atom_c[i];
       __syncthreads();
+      const int rna = rown ? v.sp.norb[v.imass[ra] - 1] : 0;
       if (rown && re < rna * rna) {
         // each neighbour's terms one at a time, as the reference adds them
-        const int64_t k0 = max(rp0, c0), k1 = min(rp1, c0 + WG2);
+        const int64_t k0 = max(v.noff[ra], c0), k1 = min(v.noff[ra + 1], c0 + WG2);
         for (int64_t k = k0; k < k1; k++) {
           racc = racc + s_at[0][k - c0][re];
 #pragma unroll
@@ -263,7 +279,7 @@ __device__ __forceinline__ void wg2c_sp(const DView &v, int mask, int item, S2c<
       __syncthreads();
     }
   }
-  if ((mask & 8) && rown && re < rna * rna) {
+  if ((mask & 8) && rown && re < v.sp.norb[v.imass[ra] - 1] * v.sp.norb[v.imass[ra] - 1]) {
     const int64_t pself = v.noff[ra] + v.neigh_self[ra] - 1;
     v.mat[FB_MAT_VNA][v.boff[pself] + re] = racc;
   }
