@@ -289,18 +289,11 @@ def test_mixed_segments_longer_than_chunk():
     the per-element bar of the benchmark workloads, whose blocks take ~14),
     far below one lost or repeated contribution."""
     from fireball_hip import synth
-    rng = np.random.default_rng(7)
-    pos = np.zeros((0, 3))
-    while len(pos) < 270:          # ball of 2.8 A: every distance below 2 rc = 5.7 A
-        p = rng.uniform(-2.8, 2.8, 3)
-        if np.linalg.norm(p) <= 2.8 and (len(pos) == 0 or np.min(np.linalg.norm(pos - p, axis=1)) >= 0.55):
-            pos = np.vstack([pos, p])
-    pos = pos + 12.0
-    st = synth.Structure(pos, np.ones(len(pos), np.int32), np.eye(3) * 40.0, 1)
+    st = _dense_si_cluster()       # ball of 2.8 A: every distance below 2 rc = 5.7 A
     ts = synth.make_tables([synth.SI])
     asm = _gpu(ts, [st], precision="fp64")
     L = asm.lists(0)
-    n = len(pos)
+    n = st.natoms
     assert np.all(L["neighn"] == n) and np.all(L["ncommon"] == (n - 1) * (n - 2))
     off = np.concatenate([[0], np.cumsum(L["neighn"])])
     seg = np.bincount(off[L["com_iatom_j"] - 1] + L["com_mneigh"] - 1)
@@ -317,6 +310,56 @@ def test_mixed_segments_longer_than_chunk():
               f"(max |VNA| {float(np.max(np.abs(ref))):.3e})", flush=True)
         assert err <= 10 * TOL_MIXED, chunk
     assert np.array_equal(got[64], got[128]) and np.array_equal(got[64], got[256])
+
+
+def _dense_si_cluster(n=270, radius=2.8, seed=7):
+    """n Si atoms in a ball of the given radius (closest approach 0.55 A):
+    every pair is a neighbour pair, so every atom has n neighbours."""
+    from fireball_hip import synth
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((0, 3))
+    while len(pos) < n:
+        p = rng.uniform(-radius, radius, 3)
+        if np.linalg.norm(p) <= radius and (len(pos) == 0 or np.min(np.linalg.norm(pos - p, axis=1)) >= 0.55):
+            pos = np.vstack([pos, p])
+    return synth.Structure(pos + 12.0, np.ones(len(pos), np.int32), np.eye(3) * 40.0, 1)
+
+
+def test_2c_passes_and_partial_wavefronts_same_bits():
+    """Atoms with more neighbours (270) than a 2c workgroup pass (256 pairs):
+    each atom's workgroup runs a full pass of four wavefronts and a pass whose
+    only wavefront holds 14 pairs, so the wavefront-coalesced block stores see
+    full and partial wavefronts and a second pass of the self-block sum.  The
+    register kernel (coalesced stores), the LDS-staged kernel (workgroup
+    stores) and the one-launch build must give the same bits on every
+    matrix."""
+    from fireball_hip import synth
+    st = _dense_si_cluster()
+    ts = synth.make_tables([synth.SI])
+    asm = _gpu(ts, [st], precision="fp64")
+    L = asm.lists(0)
+    assert np.all(L["neighn"] == 270)
+    got = {}
+    for var in (0, 1):
+        asm.set_tuning(1, var)
+        asm.assemble_all()
+        got[var] = {m: asm.blocks(m) for m in MATS}
+    for m in MATS:
+        assert np.array_equal(got[0][m], got[1][m]), m
+    # mixed precision: two launches against the one-launch build at 3 and 4
+    # waves per SIMD (the 2c matrices are binary64 in every shape)
+    asm.set_precision("mixed")
+    asm.set_tuning(1, 1)
+    shapes = {}
+    for shape in (0, 3, 4):
+        asm.set_tuning(4, shape)
+        asm.assemble_all()
+        shapes[shape] = {m: asm.blocks(m) for m in MATS}
+    for m in MATS:
+        assert np.array_equal(shapes[0][m], shapes[3][m]), m
+        assert np.array_equal(shapes[0][m], shapes[4][m]), m
+        if m != "VNA":
+            assert np.array_equal(shapes[0][m], got[1][m]), m
 
 
 @pytest.mark.parametrize("name", ["cho40_dq", "si64"])
