@@ -261,22 +261,6 @@ __device__ __forceinline__ void pair_geometry(const DView &v, int64_t p, Pair2c 
 #define CELLCAP 4096   // doubles of one staged cell (nx * nME)
 #define ROWP 17        // doubles per LDS block row (odd: conflict-free lane rows)
 
-// window weights of M_Fdata_2c.f90:377-390: xx_k = (ileft+k-1)*dx,
-// pden = 1/(xx_is - xx_ip), prod = prod*(x - xx_ip)*pden in ip order
-__device__ __forceinline__ void lagrange7_div(double x, double dx, int ileft, double *prod) {
-  double xx[7];
-#pragma unroll
-  for (int k = 0; k < 7; k++) xx[k] = (double)(ileft + k - 1) * dx;
-#pragma unroll
-  for (int is = 0; is < 7; is++) {
-    double p = 1.0;
-#pragma unroll
-    for (int ip = 0; ip < 7; ip++)
-      if (ip != is) p = p * (x - xx[ip]) * (1.0 / (xx[is] - xx[ip]));
-    prod[is] = p;
-  }
-}
-
 template <int CLS>
 __device__ __forceinline__ void interp_rot(const double *base, const double *prod, const double *pm, double *X) {
   constexpr int NME = nme2c_of(CLS);

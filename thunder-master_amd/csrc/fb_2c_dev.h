@@ -22,6 +22,34 @@ __device__ __forceinline__ void lagrange7(const DView &v, const DGrid2c &gr, dou
   }
 }
 
+// The same products with the reciprocals formed in-kernel with the
+// reference's expression, pden = 1/(xx_is - xx_ip) (correctly rounded
+// binary64 division, as on the host).  fl(a - b) = -fl(b - a) and
+// 1/(-y) = -(1/y) exactly, so the 21 divisions of is < ip give the other 21
+// by negation.  A lane computes them instead of gathering the 49 host values
+// of its window: the gathers of a wavefront's different windows cost more
+// than the divisions.
+__device__ __forceinline__ void lagrange7_div(double x, double dx, int ileft, double *prod) {
+  double xx[7], r[7][7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) xx[k] = (double)(ileft + k - 1) * dx;
+#pragma unroll
+  for (int is = 0; is < 7; is++)
+#pragma unroll
+    for (int ip = is + 1; ip < 7; ip++) {
+      r[is][ip] = 1.0 / (xx[is] - xx[ip]);
+      r[ip][is] = -r[is][ip];
+    }
+#pragma unroll
+  for (int is = 0; is < 7; is++) {
+    double p = 1.0;
+#pragma unroll
+    for (int ip = 0; ip < 7; ip++)
+      if (ip != is) p = p * (x - xx[ip]) * r[is][ip];
+    prod[is] = p;
+  }
+}
+
 // window of M_Fdata_2c.f90:366-375
 __device__ __forceinline__ int window7(double x, double dx, int nx) {
   int imid = (int)(x / dx) + 1;

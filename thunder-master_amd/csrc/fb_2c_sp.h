@@ -1,9 +1,10 @@
 // Two-centre assembly for s/p species with canonical tables (S, T, dipole_z,
 // Vna ontop and atom terms, j.ASSEMBLERS/M_assemble_2c_DOGS.f90:87-829): one
 // lane per pair, workgroups of WG2 threads packed by whole atoms (at most 16
-// atoms and 256 pairs, or one larger atom), Lagrange denominators from the
-// host table, rotations from fb_sp.h.  Bit-identical to the reference in
-// binary64.  Shared by the 2c kernel and the fused H+S kernel.
+// atoms and 256 pairs, or one larger atom), Lagrange reciprocals formed
+// in-kernel with the reference's division, rotations from fb_sp.h.
+// Bit-identical to the reference in binary64.  Shared by the 2c kernel and
+// the fused H+S kernel.
 #pragma once
 
 #include "fb_device.h"
@@ -27,7 +28,7 @@ __device__ __forceinline__ int block2c(const DView &v, int cid, double x, const 
   if (c.grid != grid_cached) {
     const DGrid2c gr = v.grid2c[c.grid];
     ileft = window7(x, gr.dx, gr.nx);
-    lagrange7(v, gr, x, ileft, prod);
+    lagrange7_div(x, gr.dx, ileft, prod);
     grid_cached = c.grid;
   }
   constexpr int NME = nme2c_of(CLS);
