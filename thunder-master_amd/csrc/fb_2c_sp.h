@@ -223,18 +223,15 @@ __device__ __forceinline__ void wg2c_sp(const DView &v, int mask, int item, S2c<
 #pragma unroll
         for (int i = 0; i < 16; i++) s_at[q][tid][i] = 0.0;
     if (p < pe) {
-      const int ia = v.pair_atom[p];
-      const int s = v.atom_struct[ia];
-      const DStruct St = v.st[s];
-      const int ja = St.atom_off + v.neigh_j[p] - 1;
-      const int mb = v.neigh_b[p];
-      const int in1 = v.imass[ia], in2 = v.imass[ja];
-      const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
+      const int4 pr = v.pair_rec[p];
+      const int ia = pr.x, ja = pr.y;
+      const int in1 = pr.w & 0xffff, in2 = pr.w >> 16;
+      const double *xlp = v.xl + (int64_t)pr.z * 3;   // xl(:, mbeta) of the pair's structure
       double r1[3], r2[3], sighat[3], eps[9], pm[9];
 #pragma unroll
       for (int q = 0; q < 3; q++) {
         r1[q] = v.ratom[3 * ia + q];
-        r2[q] = v.ratom[3 * ja + q] + xls[3 * mb + q];
+        r2[q] = v.ratom[3 * ja + q] + xlp[q];
       }
       const double z = d_distance(r1, r2);
       if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
@@ -244,7 +241,7 @@ __device__ __forceinline__ void wg2c_sp(const DView &v, int mask, int item, S2c<
       }
       d_epsilon(r2, sighat, eps);
       d_pmat<double>(eps, pm);
-      const bool self = (ja == ia && mb == 0);
+      const bool self = (ja == ia && pr.z % FB_NIMG == 0);   // jatom = iatom in the home cell
       const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
       const int64_t bo = v.boff[p];
       // the active lanes are a prefix of the wavefront

@@ -287,14 +287,17 @@ __global__ void k_iota(int64_t n, int *a) {
   if (t < n) a[t] = (int)t;
 }
 
-// block offsets: boff[p] = sum of norb_i * norb_j over earlier pairs
-__global__ void k_block_sizes(DView v, int64_t *sz) {
+// block offsets: boff[p] = sum of norb_i * norb_j over earlier pairs; and
+// the pair's record (the atoms, the image and the species in one int4)
+__global__ void k_block_sizes(DView v, int64_t *sz, int4 *rec) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= v.npairs) return;
   int ia = v.pair_atom[p];
   int s = v.atom_struct[ia];
   int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
-  sz[p] = (int64_t)v.sp.norb[v.imass[ia] - 1] * v.sp.norb[v.imass[ja] - 1];
+  const int in1 = v.imass[ia], in2 = v.imass[ja];
+  sz[p] = (int64_t)v.sp.norb[in1 - 1] * v.sp.norb[in2 - 1];
+  rec[p] = make_int4(ia, ja, s * FB_NIMG + v.neigh_b[p], in1 | (in2 << 16));
 }
 
 // Tiles of whole target segments for the segment-owning kernels, packed per
@@ -413,9 +416,9 @@ int fbk_common_fill(const DView &v, const int64_t *toff, int4 *trip, int *trip_p
   return (int)hipGetLastError();
 }
 
-int fbk_block_sizes(const DView &v, int64_t *sz, hipStream_t st) {
+int fbk_block_sizes(const DView &v, int64_t *sz, int4 *rec, hipStream_t st) {
   if (v.npairs == 0) return 0;
-  hipLaunchKernelGGL(k_block_sizes, dim3((unsigned)((v.npairs + 255) / 256)), dim3(256), 0, st, v, sz);
+  hipLaunchKernelGGL(k_block_sizes, dim3((unsigned)((v.npairs + 255) / 256)), dim3(256), 0, st, v, sz, rec);
   return (int)hipGetLastError();
 }
 
