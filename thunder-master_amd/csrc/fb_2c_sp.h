@@ -17,16 +17,57 @@ __device__ __forceinline__ int cell2c_id(const DView &v, int in1, int in2, int s
   return v.idx2c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * FB_NSLOT2 + slot) * FB_MAXSUB2 + isub];
 }
 
+typedef const __attribute__((address_space(4))) int CInt;
+typedef const __attribute__((address_space(4))) DCell2c CCell2c;
+typedef const __attribute__((address_space(4))) DGrid2c CGrid2c;
+
+// Cell of interaction (slot, isub) for the species pair sp2 = in1 | in2 << 16,
+// read through the scalar cache: a waterfall over the distinct (sp2, isub)
+// of the active lanes (one pass when the wavefront's pairs share their
+// species, as in a one-element system).  The empty asm keeps the loads at the
+// scalar index inside the loop.  Returns the cell index (negative: no table).
+__device__ __forceinline__ int cell_scalar(const DView &v, int sp2, int slot, int isub, DCell2c &c) {
+#ifndef __HIP_DEVICE_COMPILE__
+  const int cid = cell2c_id(v, sp2 & 0xffff, sp2 >> 16, slot, isub);   // the host pass only parses device code
+  if (cid >= 0) c = v.cell2c[cid];
+  return cid;
+#else
+  int cid = -1;
+  for (;;) {
+    const int k = __builtin_amdgcn_readfirstlane(sp2), ks = __builtin_amdgcn_readfirstlane(isub);
+    const int a = k & 0xffff, b = k >> 16;
+    const int id = ((CInt *)v.idx2c)[(((a - 1) * v.sp.ns + (b - 1)) * FB_NSLOT2 + slot) * FB_MAXSUB2 + ks];
+    DCell2c t{};
+    if (id >= 0) t = ((CCell2c *)v.cell2c)[id];
+    asm volatile("" ::"s"(id), "s"(t.xmax), "s"(t.off), "s"(t.grid));
+    if (sp2 == k && isub == ks) {
+      cid = id;
+      c = t;
+      break;
+    }
+  }
+  return cid;
+#endif
+}
+
+__device__ __forceinline__ DGrid2c grid_scalar(const DView &v, int g) {
+#ifndef __HIP_DEVICE_COMPILE__
+  return v.grid2c[g];
+#else
+  return ((CGrid2c *)v.grid2c)[g];   // g comes from a scalar cell header
+#endif
+}
+
 // rotated block of one interaction; returns status bits
 template <int CLS>
-__device__ __forceinline__ int block2c(const DView &v, int cid, double x, const double *pm, int &grid_cached,
-                                       int &ileft, double *prod, double *X) {
-  if (cid < 0) return ST_NO_TABLE;
-  const DCell2c c = v.cell2c[cid];
+__device__ __forceinline__ int block2c(const DView &v, int sp2, int slot, int isub, double x, const double *pm,
+                                       int &grid_cached, int &ileft, double *prod, double *X) {
+  DCell2c c;
+  if (cell_scalar(v, sp2, slot, isub, c) < 0) return ST_NO_TABLE;
   if (x < 0.0) return ST_X_NEG;
   if (x > c.xmax * (1.0 + 1.0e-5)) return ST_X_RANGE;
   if (c.grid != grid_cached) {
-    const DGrid2c gr = v.grid2c[c.grid];
+    const DGrid2c gr = grid_scalar(v, c.grid);
     ileft = window7(x, gr.dx, gr.nx);
     lagrange7_div(x, gr.dx, ileft, prod);
     grid_cached = c.grid;
@@ -109,16 +150,17 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
   constexpr int CLSA = (CLS & 2) ? CLS_SPSP : CLS_SS;   // atom term lives on iatom's orbitals
   int status = 0, grid_cached = -1, ileft = 1;
   double prod[7], X[16];
+  const int sp2 = in1 | (in2 << 16);
   if (mask & 1) {
-    status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_S, 0), z, pm, grid_cached, ileft, prod, X);
+    status |= block2c<CLS>(v, sp2, SLOT_S, 0, z, pm, grid_cached, ileft, prod, X);
     put_block<NBL>(v.mat[FB_MAT_S], bo, X, W);
   }
   if (mask & 2) {
-    status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_T, 0), z, pm, grid_cached, ileft, prod, X);
+    status |= block2c<CLS>(v, sp2, SLOT_T, 0, z, pm, grid_cached, ileft, prod, X);
     put_block<NBL>(v.mat[FB_MAT_T], bo, X, W);
   }
   if (mask & 4) {
-    status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_DIP, 0), z, pm, grid_cached, ileft, prod, X);
+    status |= block2c<CLS>(v, sp2, SLOT_DIP, 0, z, pm, grid_cached, ileft, prod, X);
     put_block<NBL>(v.mat[FB_MAT_DIP], bo, X, W);
   }
   if (mask & 8) {
@@ -127,23 +169,23 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
 #pragma unroll
     for (int i = 0; i < NBL; i++) acc[i] = 0.0;
     if (!self) {
-      status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_ONL, 0), z, pm, grid_cached, ileft, prod, X);
+      status |= block2c<CLS>(v, sp2, SLOT_ONL, 0, z, pm, grid_cached, ileft, prod, X);
 #pragma unroll
       for (int i = 0; i < NBL; i++) acc[i] = acc[i] + X[i] * FB_P_EQ2;
       for (int isorp = 1; isorp <= v.sp.nssh[in1 - 1]; isorp++) {
         double dQ = v.dQ[ia * FB_MAXSH + isorp - 1];
         if (dQ == 0.0) continue;
-        status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_ONL, isorp), z, pm, grid_cached, ileft, prod, X);
+        status |= block2c<CLS>(v, sp2, SLOT_ONL, isorp, z, pm, grid_cached, ileft, prod, X);
 #pragma unroll
         for (int i = 0; i < NBL; i++) acc[i] = acc[i] + dQ * X[i] * FB_P_EQ2;
       }
-      status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_ONR, 0), z, pm, grid_cached, ileft, prod, X);
+      status |= block2c<CLS>(v, sp2, SLOT_ONR, 0, z, pm, grid_cached, ileft, prod, X);
 #pragma unroll
       for (int i = 0; i < NBL; i++) acc[i] = acc[i] + X[i] * FB_P_EQ2;
       for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
         double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
         if (dQ == 0.0) continue;
-        status |= block2c<CLS>(v, cell2c_id(v, in1, in2, SLOT_ONR, isorp), z, pm, grid_cached, ileft, prod, X);
+        status |= block2c<CLS>(v, sp2, SLOT_ONR, isorp, z, pm, grid_cached, ileft, prod, X);
 #pragma unroll
         for (int i = 0; i < NBL; i++) acc[i] = acc[i] + dQ * X[i] * FB_P_EQ2;
       }
@@ -163,7 +205,7 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
     }
     // atom term <i|V_j|i>
     constexpr int NA = NMU * NMU;
-    status |= block2c<CLSA>(v, cell2c_id(v, in1, in2, SLOT_ATOM, 0), z, pm, grid_cached, ileft, prod, X);
+    status |= block2c<CLSA>(v, sp2, SLOT_ATOM, 0, z, pm, grid_cached, ileft, prod, X);
 #pragma unroll
     for (int i = 0; i < 16; i++) atom_c[i] = i < NA ? X[i] * FB_P_EQ2 : 0.0;
     bool charged = false;
@@ -174,7 +216,7 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
       const double smooth = d_smoother(z, v.sp.rcmin[in1 - 1] + v.sp.rcmin[in2 - 1], 0.8);
       for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
         double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
-        status |= block2c<CLSA>(v, cell2c_id(v, in1, in2, SLOT_ATOM, isorp), z, pm, grid_cached, ileft, prod, X);
+        status |= block2c<CLSA>(v, sp2, SLOT_ATOM, isorp, z, pm, grid_cached, ileft, prod, X);
 #pragma unroll
         for (int i = 0; i < NA; i++) {
           double emnpl = (z > 1.0e-4) ? sself[i] / z : 0.0;
