@@ -172,7 +172,8 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
       status |= block2c<CLS>(v, sp2, SLOT_ONL, 0, z, pm, grid_cached, ileft, prod, X);
 #pragma unroll
       for (int i = 0; i < NBL; i++) acc[i] = acc[i] + X[i] * FB_P_EQ2;
-      for (int isorp = 1; isorp <= v.sp.nssh[in1 - 1]; isorp++) {
+      // (no charged shell in the batch: the subtype loops add nothing)
+      for (int isorp = 1; v.any_dq && isorp <= v.sp.nssh[in1 - 1]; isorp++) {
         double dQ = v.dQ[ia * FB_MAXSH + isorp - 1];
         if (dQ == 0.0) continue;
         status |= block2c<CLS>(v, sp2, SLOT_ONL, isorp, z, pm, grid_cached, ileft, prod, X);
@@ -182,7 +183,7 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
       status |= block2c<CLS>(v, sp2, SLOT_ONR, 0, z, pm, grid_cached, ileft, prod, X);
 #pragma unroll
       for (int i = 0; i < NBL; i++) acc[i] = acc[i] + X[i] * FB_P_EQ2;
-      for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
+      for (int isorp = 1; v.any_dq && isorp <= v.sp.nssh[in2 - 1]; isorp++) {
         double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
         if (dQ == 0.0) continue;
         status |= block2c<CLS>(v, sp2, SLOT_ONR, isorp, z, pm, grid_cached, ileft, prod, X);
@@ -209,7 +210,8 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
 #pragma unroll
     for (int i = 0; i < 16; i++) atom_c[i] = i < NA ? X[i] * FB_P_EQ2 : 0.0;
     bool charged = false;
-    for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) charged |= (v.dQ[ja * FB_MAXSH + isorp - 1] != 0.0);
+    for (int isorp = 1; v.any_dq && isorp <= v.sp.nssh[in2 - 1]; isorp++)
+      charged |= (v.dQ[ja * FB_MAXSH + isorp - 1] != 0.0);
     if (charged) {
       const int64_t pself = v.noff[ia] + v.neigh_self[ia] - 1;
       const double *sself = v.mat[FB_MAT_S] + v.boff[pself];
