@@ -50,6 +50,49 @@ __device__ __forceinline__ void lagrange7_div(double x, double dx, int ileft, do
   }
 }
 
+typedef const __attribute__((address_space(4))) int CInt;
+typedef const __attribute__((address_space(4))) DCell2c CCell2c;
+typedef const __attribute__((address_space(4))) DGrid2c CGrid2c;
+
+// Cell of interaction (slot, isub) for the species pair sp2 = in1 | in2 << 16,
+// read through the scalar cache: a waterfall over the distinct (sp2, isub)
+// of the active lanes (one pass when the wavefront's pairs share their
+// species, as in a one-element system).  The empty asm keeps the loads at the
+// scalar index inside the loop.  Returns the cell index (negative: no table).
+__device__ __forceinline__ int cell_scalar(const DView &v, int sp2, int slot, int isub, DCell2c &c) {
+#ifndef __HIP_DEVICE_COMPILE__
+  // the host pass only parses device code
+  const int cid = v.idx2c[(((sp2 & 0xffff) - 1) * v.sp.ns + ((sp2 >> 16) - 1)) * FB_NSLOT2 * FB_MAXSUB2 +
+                         slot * FB_MAXSUB2 + isub];
+  if (cid >= 0) c = v.cell2c[cid];
+  return cid;
+#else
+  int cid = -1;
+  for (;;) {
+    const int k = __builtin_amdgcn_readfirstlane(sp2), ks = __builtin_amdgcn_readfirstlane(isub);
+    const int a = k & 0xffff, b = k >> 16;
+    const int id = ((CInt *)v.idx2c)[(((a - 1) * v.sp.ns + (b - 1)) * FB_NSLOT2 + slot) * FB_MAXSUB2 + ks];
+    DCell2c t{};
+    if (id >= 0) t = ((CCell2c *)v.cell2c)[id];
+    asm volatile("" ::"s"(id), "s"(t.xmax), "s"(t.off), "s"(t.grid));
+    if (sp2 == k && isub == ks) {
+      cid = id;
+      c = t;
+      break;
+    }
+  }
+  return cid;
+#endif
+}
+
+__device__ __forceinline__ DGrid2c grid_scalar(const DView &v, int g) {
+#ifndef __HIP_DEVICE_COMPILE__
+  return v.grid2c[g];
+#else
+  return ((CGrid2c *)v.grid2c)[g];   // g comes from a scalar cell header
+#endif
+}
+
 // window of M_Fdata_2c.f90:366-375
 __device__ __forceinline__ int window7(double x, double dx, int nx) {
   int imid = (int)(x / dx) + 1;

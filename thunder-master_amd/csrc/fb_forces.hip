@@ -755,6 +755,7 @@ extern "C" int fbk_trip_jkey(int64_t ntrip, const int4 *trip, int *key, hipStrea
 // are skipped), so the two paths agree to the last bit apart from signs of
 // zero.
 #include "fb_sp.h"
+#include "fb_2c_dev.h"
 
 #define PP_(k, m) pm[3 * (m) + (k)]
 #define DD_(k, m) dpm[3 * (m) + (k)]
@@ -862,23 +863,31 @@ struct DLag7 {
 };
 
 __device__ __forceinline__ void dlag7_build(const DView &v, int grid, double x, DLag7 &L) {
-  const DGrid2c gr = v.grid2c[grid];
+  const DGrid2c gr = grid_scalar(v, grid);
   int imid = (int)(x / gr.dx) + 1;
   int ileft = imid - 3;
   if (ileft < 1) ileft = 1;
   else if (imid + 3 > gr.nx) ileft = gr.nx - 6;
-  const double *pd = v.pden + gr.pden_off + (int64_t)(ileft - 1) * 49;
-  double xx[7];
+  double xx[7], r[7][7];
 #pragma unroll
   for (int k = 0; k < 7; k++) xx[k] = (double)(ileft + k - 1) * gr.dx;
+  // pden = 1/(xx_is - xx_ip) with the reference's division; the pairs ip < is
+  // by exact negation (fl(a - b) = -fl(b - a))
+#pragma unroll
+  for (int is = 0; is < 7; is++)
+#pragma unroll
+    for (int ip = is + 1; ip < 7; ip++) {
+      r[is][ip] = 1.0 / (xx[is] - xx[ip]);
+      r[ip][is] = -r[is][ip];
+    }
 #pragma unroll
   for (int is = 0; is < 7; is++) {
     double p = 1.0, q = 1.0;
 #pragma unroll
     for (int ip = 0; ip < 7; ip++)
       if (ip != is) {
-        q = q * pd[is * 7 + ip];
-        p = p * (x - xx[ip]) * pd[is * 7 + ip];
+        q = q * r[is][ip];
+        p = p * (x - xx[ip]) * r[is][ip];
       }
     L.prod[is] = p;
     L.pdenom[is] = q;
@@ -903,9 +912,10 @@ __device__ __forceinline__ void dlag7_build(const DView &v, int grid, double x, 
 
 // values and x-derivatives of one canonical 2c cell (getDMEs_Fdata_2c)
 template <int NME>
-__device__ __forceinline__ int dinterp2c_sp(const DView &v, int cid, double x, DLag7 &L, double *F, double *dF) {
-  if (cid < 0) return ST_NO_TABLE;
-  const DCell2c c = v.cell2c[cid];
+__device__ __forceinline__ int dinterp2c_sp(const DView &v, int sp2, int slot, int isub, double x, DLag7 &L, double *F,
+                                            double *dF) {
+  DCell2c c;
+  if (cell_scalar(v, sp2, slot, isub, c) < 0) return ST_NO_TABLE;
   if (x < 0.0) return ST_X_NEG;
   if (x > c.xmax * (1.0 + 1.0e-5)) return ST_X_RANGE;
   const double *tab = v.tab2c + c.off;
@@ -971,7 +981,7 @@ __device__ __forceinline__ int pair_d2c_sp(const DView &v, const DForce &Fo, int
   const int64_t bo = v.boff[p];
   if (PART == 0) {
     const int slot = d == 0 ? SLOT_S : (d == 1 ? SLOT_T : SLOT_DIP);
-    int st = dinterp2c_sp<NME>(v, cell2c_of(v, in1, in2, slot, 0), z, L, F, dF);
+    int st = dinterp2c_sp<NME>(v, in1 | (in2 << 16), slot, 0, z, L, F, dF);
     status |= st;
     if (st) return status;
 #pragma unroll
@@ -997,13 +1007,13 @@ __device__ __forceinline__ int pair_d2c_sp(const DView &v, const DForce &Fo, int
         const int slot = side ? SLOT_ONR : SLOT_ONL;
         const int qa = side ? ja : ia;
         const int inq = side ? in2 : in1;
-        int st = dinterp2c_sp<NME>(v, cell2c_of(v, in1, in2, slot, 0), z, L, F, dF);
+        int st = dinterp2c_sp<NME>(v, in1 | (in2 << 16), slot, 0, z, L, F, dF);
         status |= st;
         if (!st) dcontract2c_sp<CLS>(pm, deps, eta, z, 1.0e-3, F, dF, rho, 0.0, false, fo);
         for (int isorp = 1; isorp <= v.sp.nssh[inq - 1]; isorp++) {
           const double dQ = v.dQ[qa * FB_MAXSH + isorp - 1];
           if (dQ == 0.0) continue;
-          st = dinterp2c_sp<NME>(v, cell2c_of(v, in1, in2, slot, isorp), z, L, F, dF);
+          st = dinterp2c_sp<NME>(v, in1 | (in2 << 16), slot, isorp, z, L, F, dF);
           status |= st;
           if (!st) dcontract2c_sp<CLS>(pm, deps, eta, z, 1.0e-3, F, dF, rho, dQ, true, fo);
         }
@@ -1023,7 +1033,7 @@ __device__ __forceinline__ int pair_d2c_sp(const DView &v, const DForce &Fo, int
     for (int i = 0; i < NA; i++) rs[i] = Fo.rho[v.boff[pself] + i];
     int st = 0;
     if (PART == 2) {
-      st = dinterp2c_sp<NMEA>(v, cell2c_of(v, in1, in2, SLOT_ATOM, 0), z, L, F, dF);
+      st = dinterp2c_sp<NMEA>(v, in1 | (in2 << 16), SLOT_ATOM, 0, z, L, F, dF);
       status |= st;
       if (!st) dcontract2c_sp<CLSA>(pm, deps, eta, z, 1.0e-3, F, dF, rs, 0.0, false, fa);
     }
@@ -1039,7 +1049,7 @@ __device__ __forceinline__ int pair_d2c_sp(const DView &v, const DForce &Fo, int
       for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
         const double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
         if (dQ == 0.0) continue;
-        st = dinterp2c_sp<NMEA>(v, cell2c_of(v, in1, in2, SLOT_ATOM, isorp), z, L, F, dF);
+        st = dinterp2c_sp<NMEA>(v, in1 | (in2 << 16), SLOT_ATOM, isorp, z, L, F, dF);
         status |= st;
         if (st) continue;
         double bx[16];

@@ -256,8 +256,9 @@ __global__ __launch_bounds__(T3R) void k_rho_3c(DView v, int stride) {
 
 // ---------------------------------------------------------------------------
 // s/p species with canonical tables: the same sums with compile-time block
-// shapes, the register-resident rotations of fb_sp.h and the Lagrange
-// products of the host-evaluated denominators (the operations and their order
+// shapes, the register-resident rotations of fb_sp.h, the Lagrange products
+// with the reciprocals formed in-kernel by the reference's division, and the
+// cell headers read through the scalar cache (the operations and their order
 // are those of the generic kernels).  Pair class cls = (mu side is s+p) * 2 +
 // (nu side is s+p); an s+p species has two shells, an s species one, so the
 // class also fixes the shell-block shape.
@@ -277,16 +278,16 @@ __device__ __forceinline__ void shell_block(const double *F, double *X) {
 
 // one 2c term of class CLS: orbital block (rotated) or shell block (W)
 template <bool W, int CLS>
-__device__ __forceinline__ int term2c_sp(const DView &v, int cid, double z, const double *pm, int &grid_cached,
-                                         int &ileft, double *prod, double *X) {
-  if (cid < 0) return ST_NO_TABLE;
-  const DCell2c c = v.cell2c[cid];
+__device__ __forceinline__ int term2c_sp(const DView &v, int sp2, int slot, int isub, double z, const double *pm,
+                                         int &grid_cached, int &ileft, double *prod, double *X) {
+  DCell2c c;
+  if (cell_scalar(v, sp2, slot, isub, c) < 0) return ST_NO_TABLE;
   if (z < 0.0) return ST_X_NEG;
   if (z > c.xmax * (1.0 + 1.0e-5)) return ST_X_RANGE;
   if (c.grid != grid_cached) {
-    const DGrid2c gr = v.grid2c[c.grid];
+    const DGrid2c gr = grid_scalar(v, c.grid);
     ileft = window7(z, gr.dx, gr.nx);
-    lagrange7(v, gr, z, ileft, prod);
+    lagrange7_div(z, gr.dx, ileft, prod);
     grid_cached = c.grid;
   }
   constexpr int NSI = (CLS & 2) ? 2 : 1, NSJ = (CLS & 1) ? 2 : 1;
@@ -318,7 +319,7 @@ __device__ __forceinline__ int pair_rho_sp(const DView &v, int ia, int ja, int i
 #pragma unroll
     for (int isorp = 1; isorp <= NSI; isorp++) {
       const double q = v.qin[ia * FB_MAXSH + isorp - 1];
-      status |= term2c_sp<W, CLS>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ONL : SLOT_RHO_ONL, isorp), z, pm,
+      status |= term2c_sp<W, CLS>(v, in1 | (in2 << 16), W ? SLOT_RHOS_ONL : SLOT_RHO_ONL, isorp, z, pm,
                                   grid_cached, ileft, prod, X);
 #pragma unroll
       for (int i = 0; i < NB; i++) acc[i] = acc[i] + X[i] * q;
@@ -326,7 +327,7 @@ __device__ __forceinline__ int pair_rho_sp(const DView &v, int ia, int ja, int i
 #pragma unroll
     for (int isorp = 1; isorp <= NSJ; isorp++) {
       const double q = v.qin[ja * FB_MAXSH + isorp - 1];
-      status |= term2c_sp<W, CLS>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ONR : SLOT_RHO_ONR, isorp), z, pm,
+      status |= term2c_sp<W, CLS>(v, in1 | (in2 << 16), W ? SLOT_RHOS_ONR : SLOT_RHO_ONR, isorp, z, pm,
                                   grid_cached, ileft, prod, X);
 #pragma unroll
       for (int i = 0; i < NB; i++) acc[i] = acc[i] + X[i] * q;
@@ -342,7 +343,7 @@ __device__ __forceinline__ int pair_rho_sp(const DView &v, int ia, int ja, int i
 #pragma unroll
   for (int isorp = 1; isorp <= NSJ; isorp++) {
     const double q = v.qin[ja * FB_MAXSH + isorp - 1];
-    status |= term2c_sp<W, CLSA>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ATOM : SLOT_RHO_ATOM, isorp), z, pm,
+    status |= term2c_sp<W, CLSA>(v, in1 | (in2 << 16), W ? SLOT_RHOS_ATOM : SLOT_RHO_ATOM, isorp, z, pm,
                                  grid_cached, ileft, prod, X);
 #pragma unroll
     for (int i = 0; i < NA; i++) st_terms[(isorp - 1) * NBR + i] = X[i] * q;
