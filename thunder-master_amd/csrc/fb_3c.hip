@@ -32,10 +32,11 @@ struct Geo3 {
   int in1, in2, indna, nmu, nnu;
 };
 
-__device__ inline void d_geo3(const DView &v, int4 tr, Geo3 &G) {
+// aux: the triplet's trip_aux word (neutral group | structure << 16)
+__device__ inline void d_geo3(const DView &v, int4 tr, int aux, Geo3 &G) {
   const int ial = tr.x, gia = tr.y, gja = tr.z;
   const int ib = tr.w & 0xffff, jb = (tr.w >> 16) & 0xffff;
-  const int s = v.atom_struct[ial];
+  const int s = aux >> 16;
   const double *xls = v.xl + (int64_t)s * FB_NIMG * 3;
   double r1[3], r2[3], rna[3], r21[3], sighat[3], rnabc[3], rhat[3];
   for (int k = 0; k < 3; k++) {
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(Gen3c<MO>::T) void k_3c_fp64(DView v) {
     if (t < r1) {
       const int4 tr = v.trip[t];
       Geo3 G;
-      d_geo3(v, tr, G);
+      d_geo3(v, tr, v.trip_aux[t], G);
       double pm[9], dm[MO == FB_MOD ? 25 : 1], hm[NBG], bx[NBG];
       d_pmat<double>(G.eps, pm);
       if (MO == FB_MOD) d_dmat(G.eps, dm);
@@ -295,8 +296,9 @@ __global__ __launch_bounds__(CH64) void k_3c_fp64_ph(DView v) {
       TPrm64 &q = S.sq[tid];
       const int64_t t = c0 + tid;
       const int4 tr = v.trip[t];
+      const int aux = v.trip_aux[t];
       Geo3 G;
-      d_geo3(v, tr, G);
+      d_geo3(v, tr, aux, G);
       double pm[9];
       d_pmat<double>(G.eps, pm);
 #pragma unroll
@@ -307,12 +309,13 @@ __global__ __launch_bounds__(CH64) void k_3c_fp64_ph(DView v) {
       q.sint = sint;
 #pragma unroll
       for (int k = 0; k < 4; k++) q.p[k] = p[k + 1];
-      int gid = group3c_id(v, G.in1, G.in2, G.indna, 0);
-      if (gid < 0) {
+      // the neutral group of the triplet, from its list-build word
+      int gid = aux & 0xffff;
+      if (gid == 0xffff) {
         status |= ST_NO_TABLE;
         gid = 0;
       }
-      const DGroup3c g = v.grp3c[gid];
+      const DGroup3c g = group3c_scalar(v, gid);
       const int nx = g.nx, ny = g.ny;
       const double dx = g.xmax / (nx - 1.0), dy = g.ymax / (ny - 1.0);
       int ix = (int)(G.x / dx) + 1, iy = (int)(G.z / dy) + 1;

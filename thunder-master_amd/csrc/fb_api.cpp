@@ -1179,6 +1179,12 @@ int fb_commit_tables(fb_ctx *ctx) {
   ctx->mlp_ready = false;   // networks belong to the previous tables
   ctx->mlp_gen++;
   ctx->tables_ok = true;
+  // the per-triplet group index of existing lists names the new tables' groups
+  if (ctx->lists_ok && ctx->ntrip > 0) {
+    DView v = ctx->view();
+    RC(ctx->launch("trip_aux", [&] { return fbk_trip_aux(v, ctx->d_trip_aux.as<int>(), ctx->stream); }));
+    FB_HIP(hipStreamSynchronize(ctx->stream));
+  }
   return FB_OK;
 }
 
