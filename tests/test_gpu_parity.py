@@ -399,6 +399,34 @@ def test_generic_shells_against_oracle():
         assert_bits(r[m], o[m], m)
 
 
+def test_tables_committed_after_lists():
+    """Lists built under a table set that lacks the 3c groups with a carbon
+    third atom, then the full set committed: the per-triplet group index
+    written at list build must follow the new tables (their group numbering
+    shifts), in binary64 and mixed precision, without rebuilding the lists."""
+    import dataclasses
+    from fireball_hip import synth
+    from fireball_hip.lib import Assembler
+    sp = [synth.SI, synth.C_SP]
+    ts = synth.make_tables(sp, seed=5)
+    part = dataclasses.replace(ts, cells3c={k: c for k, c in ts.cells3c.items() if k[2] != 2})
+    assert len(part.cells3c) < len(ts.cells3c)
+    st = synth.random_box({14: 8, 6: 8}, 0.8, 1.0, 23, sp)
+    o = oracle.System(oracle.Tables(ts), st).run_all()
+    for precision in ("fp64", "mixed"):
+        asm = Assembler(part, precision=precision)
+        asm.add_structure(st)
+        asm.build_lists()
+        asm._set_tables(ts)        # the missing groups are added and the table set committed again
+        asm.assemble_all()
+        for m in MATS:
+            got = asm.blocks(m)
+            if precision == "fp64":
+                assert_bits(got, o[m], m)
+            else:
+                np.testing.assert_allclose(got, o[m], rtol=0, atol=TOL_MIXED, err_msg=m)
+
+
 def test_graph_replays_match_eager():
     """assemble_all replays a captured HIP graph; every replay, with eager
     builds and host synchronisations in between, must equal the eager result."""
