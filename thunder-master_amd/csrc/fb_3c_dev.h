@@ -5,30 +5,6 @@
 
 #include "fb_device.h"
 
-// Header of 3c group gid read through the scalar cache: a waterfall over
-// the distinct group indices of the active lanes (one pass when a
-// wavefront's triplets share their group, as in a one-element system).  The
-// empty asm keeps every field's load at the scalar index inside the loop.
-typedef const __attribute__((address_space(4))) DGroup3c CGrp3c;
-__device__ __forceinline__ DGroup3c group3c_scalar(const DView &v, int gid) {
-#ifndef __HIP_DEVICE_COMPILE__
-  return v.grp3c[gid];   // the host pass only parses device code
-#else
-  DGroup3c g;
-  for (;;) {
-    const int gc = __builtin_amdgcn_readfirstlane(gid);
-    const DGroup3c t = ((CGrp3c *)v.grp3c)[gc];
-    asm volatile("" ::"s"(t.nME), "s"(t.nx), "s"(t.ny), "s"(t.munu), "s"(t.cls), "s"(t.nq), "s"(t.xmax),
-                 "s"(t.ymax), "s"(t.dx), "s"(t.dy), "s"(t.off), "s"(t.moff));
-    if (gid == gc) {
-      g = t;
-      break;
-    }
-  }
-  return g;
-#endif
-}
-
 // Legendre P0..P4 and sin(theta) (M_Fdata_3c.f90:347-354)
 __device__ __forceinline__ void d_legendre(double cost, double *p, double *sint) {
   double cost2 = cost * cost;
