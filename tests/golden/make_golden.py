@@ -9,6 +9,7 @@ build container, where the reference sources are present).
     make -C oracle pref                            # pseudopotential harness
     python tests/golden/make_golden.py --pp       # pseudopotential fixtures only
     python tests/golden/make_golden.py --d        # H/S fixtures of species with d shells
+    python tests/golden/make_golden.py --rho --d  # density fixtures of species with d shells
 
 Each fixture is an .npz holding the inputs that are not re-derivable from a
 seed (explicit positions for the hand-made cases), the reference's neighbour
@@ -228,7 +229,7 @@ def main():
     if "--rho" in sys.argv:
         if not os.path.isfile(refrun.HARNESS_RREF):
             sys.exit("density harness not built: run `make -C oracle rref` first")
-        for name, ts, st, keep_atoms in cases():
+        for name, ts, st, keep_atoms in (cases_d() if "--d" in sys.argv else cases()):
             rho_fixture(name, ts, st, keep_atoms if keep_atoms is not None else (10 if st.natoms > 32 else None))
         return
     if not refrun.available():

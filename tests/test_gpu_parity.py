@@ -607,8 +607,9 @@ def test_d_shells_mixed_context_takes_binary64_path(golden, name):
 
 
 def test_d_shells_rejected_outside_h_s(golden):
-    """The density, force and pseudopotential kernels hold s and p blocks only:
-    with a d species they stop with FB_E_UNSUPPORTED instead of computing."""
+    """The force and pseudopotential kernels hold s and p blocks only: with a
+    d species they stop with FB_E_UNSUPPORTED instead of computing (the
+    density kernels take d shells: tests/test_gpu_rho.py)."""
     from fireball_hip import synth
     from fireball_hip.lib import FireballError
     g = golden("spd_cluster_dq")

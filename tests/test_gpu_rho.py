@@ -48,7 +48,7 @@ def _same(got, ref, what):
     assert bad.size == 0, f"{what}: {bad.size} values differ, first {bad[:5]}, max abs {np.abs(got - ref).max():.3e}"
 
 
-@pytest.mark.parametrize("name", golden_cases.NAMES)
+@pytest.mark.parametrize("name", golden_cases.NAMES + golden_cases.NAMES_D)
 def test_rho_golden(golden, name):
     print(f"case {name}", flush=True)
     g, gr = golden(name), golden(name + "_rho")

@@ -26,7 +26,7 @@ def rho_inputs(g, gr):
 @pytest.fixture(scope="module")
 def rho_runs(golden):
     out = {}
-    for name in golden_cases.NAMES:
+    for name in golden_cases.NAMES + golden_cases.NAMES_D:
         g, gr = golden(name), golden(name + "_rho")
         ts, st = rho_inputs(g, gr)
         o = oracle.System(oracle.Tables(ts), st)
@@ -35,13 +35,13 @@ def rho_runs(golden):
     return out
 
 
-@pytest.mark.parametrize("name", golden_cases.NAMES)
+@pytest.mark.parametrize("name", golden_cases.NAMES + golden_cases.NAMES_D)
 def test_rho_tables_regenerate(rho_runs, name):
     gr, ts, _ = rho_runs[name]
     assert tables_digest(ts) == str(gr["rho_tables_digest"])
 
 
-@pytest.mark.parametrize("name", golden_cases.NAMES)
+@pytest.mark.parametrize("name", golden_cases.NAMES + golden_cases.NAMES_D)
 def test_rho_bit_exact_against_reference(rho_runs, name):
     gr, _, r = rho_runs[name]
     for k in RKEYS:

@@ -1644,7 +1644,7 @@ static int sp_orbitals_only(fb_ctx *ctx, const char *what) {
   if (!fb_wide_species(ctx->sp)) return FB_OK;
   return ctx->fail(FB_E_UNSUPPORTED, (std::string(what) +
                                       ": a species has more than 4 orbitals (d shells); only S, T, dipole_z, "
-                                      "vna_2c and vna_3c take them").c_str());
+                                      "vna_2c, vna_3c and the McWEDA densities take them").c_str());
 }
 
 static int pre_assemble(fb_ctx *ctx, const char *what) {
@@ -2454,7 +2454,6 @@ static int prep_rho(fb_ctx *ctx) {
 
 static int pre_rho(fb_ctx *ctx, const char *what) {
   RC(pre_assemble(ctx, what));
-  RC(sp_orbitals_only(ctx, what));
   for (auto &s : ctx->structs)
     if (!s.has_qin) return ctx->fail(FB_E_ORDER, (std::string(what) + ": shell charges not set (fb_set_qin)").c_str());
   return prep_rho(ctx);

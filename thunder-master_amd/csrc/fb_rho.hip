@@ -17,9 +17,19 @@
 #include "fb_device.h"
 #include "fb_3c_dev.h"
 
-#define WR 64     // lanes per atom (2c)
-#define NBR 16    // elements of one orbital or shell block (at most 4 x 4)
-#define T3R 64    // triplets per 3c tile
+#define NBR 16    // elements of one orbital or shell block of s/p species (at most 4 x 4)
+
+// the generic kernels' sizes: s/p species (MO 4), or with d shells (MO 9,
+// blocks of up to 81 elements; fewer lanes keep the staged terms in LDS small)
+template <int MO> struct GenR;
+template <> struct GenR<FB_MO> {
+  static constexpr int W = 64;    // lanes (pairs) per atom and pass (2c)
+  static constexpr int T = 64;    // triplets per 3c tile
+};
+template <> struct GenR<FB_MOD> {
+  static constexpr int W = 16;
+  static constexpr int T = 16;
+};
 
 __device__ __forceinline__ int cell2c_r(const DView &v, int in1, int in2, int slot, int isub) {
   return v.idx2c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * FB_NSLOT2 + slot) * FB_MAXSUB2 + isub];
@@ -30,11 +40,12 @@ __device__ __forceinline__ int group3c_r(const DView &v, int in1, int in2, int i
 }
 
 // getMEs_Fdata_2c of one cell recovered into hm (column-major nr x nc)
+template <int MO>
 __device__ inline int me2c_block(const DView &v, int cid, int nr, int nc, double x, double *hm) {
   if (cid < 0) return ST_NO_TABLE;
   const DCell2c c = v.cell2c[cid];
-  if (c.nME > NBR) return ST_CAPACITY;
-  double F[NBR];
+  if (c.nME > MO * MO) return ST_CAPACITY;
+  double F[MO * MO];
   int st = d_interp2c(c, v.tab2c, x, F);
   if (st) return st;
   for (int i = 0; i < nr * nc; i++) hm[i] = 0.0;
@@ -42,23 +53,26 @@ __device__ inline int me2c_block(const DView &v, int cid, int nr, int nc, double
   return 0;
 }
 
-// one term of a block: the table block (rotated for orbital blocks, W = false)
-template <bool W>
+// one term of a block: the table block (rotated for orbital blocks, W = false;
+// dm: the d-shell rotation when MO = 9)
+template <bool W, int MO>
 __device__ inline int term2c(const DView &v, int cid, int in1, int in3, int nr, int nc, double z, const double *pm,
-                             double *X) {
-  if (W) return me2c_block(v, cid, nr, nc, z, X);
-  double hm[NBR];
-  int st = me2c_block(v, cid, nr, nc, z, hm);
+                             const double *dm, double *X) {
+  if (W) return me2c_block<MO>(v, cid, nr, nc, z, X);
+  double hm[MO * MO];
+  int st = me2c_block<MO>(v, cid, nr, nc, z, hm);
   if (st) return st;
-  d_rotate<double>(v.sp, in1, in3, pm, hm, nr, X);
+  d_rotate<double>(v.sp, in1, in3, pm, hm, nr, X, MO == FB_MOD ? dm : nullptr);
   return 0;
 }
 
 // assemble_rho_2c (W = false, :90-341) and assemble_rho_weighted_2c (W = true,
 // :534-762): one workgroup of WR lanes per atom, lane k takes the atom's pair k
-// (chunks of WR).  Dynamic LDS: [WR][stride] staged atom terms, isorp major.
-template <bool W>
-__global__ __launch_bounds__(WR) void k_rho_2c(DView v, int stride) {
+// (chunks of WR).  Dynamic LDS: [WR][stride] staged atom terms, isorp major,
+// NB elements per term.
+template <bool W, int MO>
+__global__ __launch_bounds__(GenR<MO>::W) void k_rho_2c(DView v, int stride) {
+  constexpr int WR = GenR<MO>::W, NB = MO * MO, NE = (NB + WR - 1) / WR;
   extern __shared__ double s_t[];
   __shared__ int s_n[WR];
   const int ia = blockIdx.x;
@@ -79,7 +93,12 @@ __global__ __launch_bounds__(WR) void k_rho_2c(DView v, int stride) {
   const int64_t *off = W ? v.wboff : v.boff;
   const int nself = nmu * nmu;
   int status = 0;
-  double acc_in = 0.0, acc_bond = 0.0;   // lane e < nself owns element e of the self blocks
+  // the lane owns elements lane + j WR (j < NE) of the self blocks
+  double acc_in[NE], acc_bond[NE];
+  for (int j = 0; j < NE; j++) {
+    acc_in[j] = 0.0;
+    acc_bond[j] = 0.0;
+  }
 
   for (int c0 = 0; c0 < n; c0 += WR) {
     const int k = c0 + lane;
@@ -95,29 +114,30 @@ __global__ __launch_bounds__(WR) void k_rho_2c(DView v, int stride) {
       double r2[3];
       for (int q = 0; q < 3; q++) r2[q] = v.ratom[3 * ja + q] + xls[3 * mb + q];
       const double z = d_distance(r1, r2);
-      double pm[9];
+      double pm[9], dm[MO == FB_MOD ? 25 : 1];
       if (!W) {
         double sighat[3], eps[9];
         if (z < 1.0e-5) { sighat[0] = 0.0; sighat[1] = 0.0; sighat[2] = 1.0; }
         else for (int q = 0; q < 3; q++) sighat[q] = (r2[q] - r1[q]) / z;
         d_epsilon(r2, sighat, eps);
         d_pmat<double>(eps, pm);
+        if (MO == FB_MOD) d_dmat(eps, dm);
       }
-      double X[NBR];
+      double X[NB];
       if (k != kself) {
         // ontop terms: potential of iatom's shells, then of jatom's
-        double acc[NBR];
+        double acc[NB];
         for (int i = 0; i < nb; i++) acc[i] = 0.0;
         for (int isorp = 1; isorp <= nsi; isorp++) {
           const double q = v.qin[ia * FB_MAXSH + isorp - 1];
-          status |= term2c<W>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ONL : SLOT_RHO_ONL, isorp), in1, in2, nmu, nnu,
-                              z, pm, X);
+          status |= term2c<W, MO>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ONL : SLOT_RHO_ONL, isorp), in1, in2, nmu,
+                                  nnu, z, pm, dm, X);
           for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i] * q;
         }
         for (int isorp = 1; isorp <= nsj; isorp++) {
           const double q = v.qin[ja * FB_MAXSH + isorp - 1];
-          status |= term2c<W>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ONR : SLOT_RHO_ONR, isorp), in1, in2, nmu, nnu,
-                              z, pm, X);
+          status |= term2c<W, MO>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ONR : SLOT_RHO_ONR, isorp), in1, in2, nmu,
+                                  nnu, z, pm, dm, X);
           for (int i = 0; i < nb; i++) acc[i] = acc[i] + X[i] * q;
         }
         const int64_t bo = off[p];
@@ -129,35 +149,40 @@ __global__ __launch_bounds__(WR) void k_rho_2c(DView v, int stride) {
       // atom terms <i|rho_j|i>, staged as products Qin * term
       for (int isorp = 1; isorp <= nsj; isorp++) {
         const double q = v.qin[ja * FB_MAXSH + isorp - 1];
-        status |= term2c<W>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ATOM : SLOT_RHO_ATOM, isorp), in1, in1, nmu, nmu, z,
-                            pm, X);
-        for (int i = 0; i < nself; i++) s_t[lane * stride + (isorp - 1) * NBR + i] = X[i] * q;
+        status |= term2c<W, MO>(v, cell2c_r(v, in1, in2, W ? SLOT_RHOS_ATOM : SLOT_RHO_ATOM, isorp), in1, in1, nmu,
+                                nmu, z, pm, dm, X);
+        for (int i = 0; i < nself; i++) s_t[lane * stride + (isorp - 1) * NB + i] = X[i] * q;
       }
     }
     s_n[lane] = nsj;
     __syncthreads();
-    if (lane < nself) {
-      const int kmax = min(WR, n - c0);
+    const int kmax = min(WR, n - c0);
+    for (int j = 0; j < NE; j++) {
+      const int e = lane + j * WR;
+      if (e >= nself) break;
       for (int kk = 0; kk < kmax; kk++) {
         const bool self = (c0 + kk == kself);
         for (int isorp = 0; isorp < s_n[kk]; isorp++) {
-          const double t = s_t[kk * stride + isorp * NBR + lane];
-          acc_in = acc_in + t;
-          if (self) acc_bond = acc_bond + t;
+          const double t = s_t[kk * stride + isorp * NB + e];
+          acc_in[j] = acc_in[j] + t;
+          if (self) acc_bond[j] = acc_bond[j] + t;
         }
       }
     }
     __syncthreads();
   }
-  if (lane < nself) {
-    out_in[off[pself] + lane] = acc_in;
-    out_bond[off[pself] + lane] = acc_bond;
+  for (int j = 0; j < NE; j++) {
+    const int e = lane + j * WR;
+    if (e >= nself) break;
+    out_in[off[pself] + e] = acc_in[j];
+    out_bond[off[pself] + e] = acc_bond[j];
   }
   if (status) atomicOr(v.status, status);
 }
 
 // triplet geometry of the density assemblers, M_assemble_rho_McWEDA_DOGS.f90:451-468
-__device__ inline void rho_geo(const DView &v, int4 tr, double &x, double &z, double &cost, double *pm) {
+__device__ inline void rho_geo(const DView &v, int4 tr, double &x, double &z, double &cost, double *pm,
+                               double *eps_out = nullptr) {
   const int ial = tr.x, gia = tr.y, gja = tr.z;
   const int ib = tr.w & 0xffff, jb = (tr.w >> 16) & 0xffff;
   const int s = v.atom_struct[ial];
@@ -179,6 +204,8 @@ __device__ inline void rho_geo(const DView &v, int4 tr, double &x, double &z, do
   cost = sighat[0] * rhat[0] + sighat[1] * rhat[1] + sighat[2] * rhat[2];
   d_epsilon(rhat, sighat, eps);
   d_pmat<double>(eps, pm);
+  if (eps_out)
+    for (int k = 0; k < 9; k++) eps_out[k] = eps[k];
 }
 
 // assemble_rho_3c (W = false, :362-511) and assemble_rho_weighted_3c (W = true,
@@ -186,8 +213,9 @@ __device__ inline void rho_geo(const DView &v, int4 tr, double &x, double &z, do
 // T3R sorted triplets; every thread computes the terms of one triplet per
 // chunk into LDS ([T3R][stride], isorp major), and one owner thread per segment
 // adds them in order onto the two-centre block.
-template <bool W>
-__global__ __launch_bounds__(T3R) void k_rho_3c(DView v, int stride) {
+template <bool W, int MO>
+__global__ __launch_bounds__(GenR<MO>::T) void k_rho_3c(DView v, int stride) {
+  constexpr int T3R = GenR<MO>::T, NB = MO * MO;
   extern __shared__ double s_c[];
   __shared__ int s_n[T3R];
   const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3R;
@@ -200,7 +228,7 @@ __global__ __launch_bounds__(T3R) void k_rho_3c(DView v, int stride) {
   const int64_t *off = W ? v.wboff : v.boff;
   int64_t a = 0, b = 0, dst = 0;
   int nb = 0;
-  double acc[NBR];
+  double acc[NB];
   if (own) {
     a = v.seg_off[my];
     b = v.seg_off[my + 1];
@@ -210,7 +238,7 @@ __global__ __launch_bounds__(T3R) void k_rho_3c(DView v, int stride) {
     const int in1 = v.imass[ia], in2 = v.imass[ja];
     nb = W ? v.sp.nssh[in1 - 1] * v.sp.nssh[in2 - 1] : v.sp.norb[in1 - 1] * v.sp.norb[in2 - 1];
     dst = off[p];
-    for (int i = 0; i < NBR; i++) acc[i] = (i < nb) ? out[dst + i] : 0.0;
+    for (int i = 0; i < NB; i++) acc[i] = (i < nb) ? out[dst + i] : 0.0;
   }
   int status = 0;
   for (int64_t c0 = r0; c0 < r1; c0 += T3R) {
@@ -221,18 +249,19 @@ __global__ __launch_bounds__(T3R) void k_rho_3c(DView v, int stride) {
       const int in1 = v.imass[tr.y], in2 = v.imass[tr.z], indna = v.imass[tr.x];
       const int nr = W ? v.sp.nssh[in1 - 1] : v.sp.norb[in1 - 1];
       const int nc = W ? v.sp.nssh[in2 - 1] : v.sp.norb[in2 - 1];
-      double x, z, cost, pm[9], hm[NBR], X[NBR];
-      rho_geo(v, tr, x, z, cost, pm);
+      double x, z, cost, pm[9], eps[9], dm[MO == FB_MOD ? 25 : 1], hm[NB], X[NB];
+      rho_geo(v, tr, x, z, cost, pm, eps);
+      if (MO == FB_MOD && !W) d_dmat(eps, dm);
       nsorp = v.sp.nssh[indna - 1];
       for (int isorp = 1; isorp <= nsorp; isorp++) {
         const double q = v.qin[tr.x * FB_MAXSH + isorp - 1];
-        status |= d_getMEs_3c<NBR>(v, group3c_r(v, in1, in2, indna, W ? 1 : 0, isorp), x, z, cost, nr, nc, hm);
+        status |= d_getMEs_3c<NB>(v, group3c_r(v, in1, in2, indna, W ? 1 : 0, isorp), x, z, cost, nr, nc, hm);
         if (W) {
           for (int i = 0; i < nr * nc; i++) X[i] = hm[i];
         } else {
-          d_rotate<double>(v.sp, in1, in2, pm, hm, nr, X);
+          d_rotate<double>(v.sp, in1, in2, pm, hm, nr, X, MO == FB_MOD ? dm : nullptr);
         }
-        for (int i = 0; i < nr * nc; i++) s_c[threadIdx.x * stride + (isorp - 1) * NBR + i] = X[i] * q;
+        for (int i = 0; i < nr * nc; i++) s_c[threadIdx.x * stride + (isorp - 1) * NB + i] = X[i] * q;
       }
     }
     s_n[threadIdx.x] = nsorp;
@@ -242,14 +271,14 @@ __global__ __launch_bounds__(T3R) void k_rho_3c(DView v, int stride) {
       for (int64_t kk = k0; kk < k1; kk++) {
         const int row = (int)(kk - c0);
         for (int isorp = 0; isorp < s_n[row]; isorp++)
-          for (int i = 0; i < NBR; i++)
-            if (i < nb) acc[i] = acc[i] + s_c[row * stride + isorp * NBR + i];
+          for (int i = 0; i < NB; i++)
+            if (i < nb) acc[i] = acc[i] + s_c[row * stride + isorp * NB + i];
       }
     }
     __syncthreads();
   }
   if (own)
-    for (int i = 0; i < NBR; i++)
+    for (int i = 0; i < NB; i++)
       if (i < nb) out[dst + i] = acc[i];
   if (status) atomicOr(v.status, status);
 }
@@ -704,6 +733,30 @@ __global__ void k_shell_sizes(DView v, int64_t *sz) {
   sz[p] = (int64_t)v.sp.nssh[v.imass[ia] - 1] * v.sp.nssh[v.imass[ja] - 1];
 }
 
+// generic kernels, sized by the orbital count of the widest species
+template <int MO>
+static void launch_rho_2c(const DView &v, int weighted, int maxsh, hipStream_t st) {
+  constexpr int WR = GenR<MO>::W;
+  const int stride = maxsh * MO * MO + 1;
+  const size_t lds = sizeof(double) * WR * stride;
+  if (weighted)
+    hipLaunchKernelGGL((k_rho_2c<true, MO>), dim3(v.natoms), dim3(WR), lds, st, v, stride);
+  else
+    hipLaunchKernelGGL((k_rho_2c<false, MO>), dim3(v.natoms), dim3(WR), lds, st, v, stride);
+}
+
+template <int MO>
+static void launch_rho_3c(const DView &v, int weighted, int maxsh, hipStream_t st) {
+  constexpr int T3R = GenR<MO>::T;
+  const int stride = maxsh * MO * MO + 1;
+  const size_t lds = sizeof(double) * T3R * stride;
+  const dim3 grid((unsigned)((v.ntrip + T3R - 1) / T3R));
+  if (weighted)
+    hipLaunchKernelGGL((k_rho_3c<true, MO>), grid, dim3(T3R), lds, st, v, stride);
+  else
+    hipLaunchKernelGGL((k_rho_3c<false, MO>), grid, dim3(T3R), lds, st, v, stride);
+}
+
 extern "C" {
 // maxsh: largest shell count of any species (sizes the staged terms); sp: the
 // s/p kernels (every species s or s+p, canonical density tables)
@@ -719,19 +772,13 @@ int fbk_rho_2c(const DView &v, int weighted, int maxsh, int sp, hipStream_t st) 
       hipLaunchKernelGGL(k_rho_2c_sp<false>, dim3(v.nwg2c), dim3(WRS), lds, st, v, stride);
     return (int)hipGetLastError();
   }
-  const size_t lds = sizeof(double) * WR * stride;
-  if (weighted)
-    hipLaunchKernelGGL(k_rho_2c<true>, dim3(v.natoms), dim3(WR), lds, st, v, stride);
-  else
-    hipLaunchKernelGGL(k_rho_2c<false>, dim3(v.natoms), dim3(WR), lds, st, v, stride);
+  if (fb_wide_species(v.sp)) launch_rho_2c<FB_MOD>(v, weighted, maxsh, st);
+  else launch_rho_2c<FB_MO>(v, weighted, maxsh, st);
   return (int)hipGetLastError();
 }
 
 int fbk_rho_3c(const DView &v, int weighted, int maxsh, int sp, hipStream_t st) {
   if (v.ntrip == 0) return 0;
-  const int stride = maxsh * NBR + 1;
-  const size_t lds = sizeof(double) * T3R * stride;
-  const dim3 grid((unsigned)((v.ntrip + T3R - 1) / T3R));
   if (sp) {
     if (v.ntiles[0] == 0) return 0;
     const dim3 g((unsigned)v.ntiles[0]);
@@ -741,10 +788,8 @@ int fbk_rho_3c(const DView &v, int weighted, int maxsh, int sp, hipStream_t st) 
       hipLaunchKernelGGL(k_rho_3c_ph<false>, g, dim3(RW), 0, st, v);
     return (int)hipGetLastError();
   }
-  if (weighted)
-    hipLaunchKernelGGL(k_rho_3c<true>, grid, dim3(T3R), lds, st, v, stride);
-  else
-    hipLaunchKernelGGL(k_rho_3c<false>, grid, dim3(T3R), lds, st, v, stride);
+  if (fb_wide_species(v.sp)) launch_rho_3c<FB_MOD>(v, weighted, maxsh, st);
+  else launch_rho_3c<FB_MO>(v, weighted, maxsh, st);
   return (int)hipGetLastError();
 }
 
