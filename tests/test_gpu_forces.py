@@ -43,7 +43,7 @@ def _close(got, ref, rel, what):
     np.testing.assert_allclose(got, ref, rtol=0, atol=rel * scale, err_msg=what)
 
 
-@pytest.mark.parametrize("name", golden_cases.NAMES)
+@pytest.mark.parametrize("name", golden_cases.NAMES + golden_cases.NAMES_D)
 def test_forces_golden(golden, name):
     print(f"case {name}", flush=True)
     g, gf = golden(name), golden(name + "_forces")

@@ -606,17 +606,17 @@ def test_d_shells_mixed_context_takes_binary64_path(golden, name):
         assert_bits(r[m][:n], g[m], m + " vs golden")
 
 
-def test_d_shells_rejected_outside_h_s(golden):
-    """The force and pseudopotential kernels hold s and p blocks only: with a
-    d species they stop with FB_E_UNSUPPORTED instead of computing (the
-    density kernels take d shells: tests/test_gpu_rho.py)."""
+def test_d_shells_rejected_by_pp(golden):
+    """The pseudopotential kernels hold s and p orbital blocks only: a species
+    with d shells stops with FB_E_UNSUPPORTED when PP shells are set (the H+S,
+    density and force kernels take d shells: the golden tests of
+    test_gpu_parity.py, test_gpu_rho.py and test_gpu_forces.py)."""
+    import dataclasses
     from fireball_hip import synth
-    from fireball_hip.lib import FireballError
+    from fireball_hip.lib import Assembler, FireballError
     g = golden("spd_cluster_dq")
-    ts, st = golden_cases.inputs(g)
-    asm = _gpu(ts, [st])
-    L = asm.lists(0)
-    rho = synth.denmat(ts.species, st.imass, L["neighn"], L["neigh_j"], L["neigh_back"], seed=3)
+    ts, _ = golden_cases.inputs(g)
+    species = [synth.Species(sp.nZ, list(sp.lssh), list(sp.rcutoff), [0, 1], 1.6, [-3.0, 1.5]) for sp in ts.species]
     with pytest.raises(FireballError) as e:
-        asm.set_denmat(rho)
+        Assembler(dataclasses.replace(ts, species=species))
     assert e.value.code == -11

@@ -18,7 +18,7 @@ FKEYS = ("vna_atom", "vna_ontop", "f3naa", "f3nab", "f3nac")
 @pytest.fixture(scope="module")
 def force_runs(golden):
     out = {}
-    for name in golden_cases.NAMES:
+    for name in golden_cases.NAMES + golden_cases.NAMES_D:
         g = golden(name)
         gf = golden(name + "_forces")
         ts, st = golden_cases.inputs(g)
@@ -30,7 +30,7 @@ def force_runs(golden):
     return out
 
 
-@pytest.mark.parametrize("name", golden_cases.NAMES)
+@pytest.mark.parametrize("name", golden_cases.NAMES + golden_cases.NAMES_D)
 def test_forces_match_reference(force_runs, name):
     gf, o = force_runs[name]
     for k in DKEYS:

@@ -1643,8 +1643,8 @@ int fb_get_common(fb_ctx *ctx, int is, int *ncommon, int *iatom_j, int *iatom_b,
 static int sp_orbitals_only(fb_ctx *ctx, const char *what) {
   if (!fb_wide_species(ctx->sp)) return FB_OK;
   return ctx->fail(FB_E_UNSUPPORTED, (std::string(what) +
-                                      ": a species has more than 4 orbitals (d shells); only S, T, dipole_z, "
-                                      "vna_2c, vna_3c and the McWEDA densities take them").c_str());
+                                      ": a species has more than 4 orbitals (d shells); the pseudopotential "
+                                      "assemblers hold s and p orbital blocks only").c_str());
 }
 
 static int pre_assemble(fb_ctx *ctx, const char *what) {
@@ -2901,7 +2901,6 @@ static int prep_forces(fb_ctx *ctx) {
 
 int fb_set_denmat(fb_ctx *ctx, int is, const double *rho) {
   if (!ctx) return FB_E_ARG;
-  RC(sp_orbitals_only(ctx, "set_denmat"));
   if (!ctx->lists_ok) return ctx->fail(FB_E_ORDER, "set_denmat before build_lists");
   if (is < 0 || is >= (int)ctx->structs.size() || !rho) return ctx->fail(FB_E_ARG, "set_denmat arguments");
   FB_HIP(hipSetDevice(ctx->device));
@@ -2918,7 +2917,6 @@ int fb_set_denmat(fb_ctx *ctx, int is, const double *rho) {
 
 static int pre_dassemble(fb_ctx *ctx, const char *what) {
   RC(pre_assemble(ctx, what));
-  RC(sp_orbitals_only(ctx, what));
   RC(prep_forces(ctx));
   return FB_OK;
 }

@@ -24,7 +24,6 @@
 #include "fb_forces.h"
 
 #define WD2 64
-#define NBK (FB_MO * FB_MO)
 #define NME3F 10
 
 // Levi-Civita and Kronecker tensors of include/constants.h (xlevi, delk), 1-based
@@ -153,11 +152,47 @@ __device__ __forceinline__ void d_dpmat(const double *deps, int ix, double *dpm)
   dpm[2] = DEP_(deps, ix, 1, 2); dpm[5] = DEP_(deps, ix, 1, 3); dpm[8] = DEP_(deps, ix, 1, 1);
 }
 
+// derivative (component ix, 1-based) of the d rotation: Ddmat(ix, :, :) of
+// Dtwister (c.SYSTEM/M_Drotations.f90:203-298) with the A matrices of
+// include/constants.h:26-35, the sums in the reference's order; column-major
+// ddm[5 (m - 1) + (k - 1)] = Ddmat(ix, k, m)
+__device__ inline void d_ddmat(const double *eps, const double *deps, int ix, double *ddm) {
+  const double r3 = 1.0 / (2.0 * sqrt(3.0));
+  // amat(i, j, imu) = amat(j, i, imu): imu = xy, yz, z^2, xz, x^2-y^2
+  const double amat[5][3][3] = {
+      {{0.0, 0.5, 0.0}, {0.5, 0.0, 0.0}, {0.0, 0.0, 0.0}},
+      {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.5}, {0.0, 0.5, 0.0}},
+      {{-r3, 0.0, 0.0}, {0.0, -r3, 0.0}, {0.0, 0.0, 2.0 / (2.0 * sqrt(3.0))}},
+      {{0.0, 0.0, 0.5}, {0.0, 0.0, 0.0}, {0.5, 0.0, 0.0}},
+      {{0.5, 0.0, 0.0}, {0.0, -0.5, 0.0}, {0.0, 0.0, 0.0}}};
+  for (int imu = 0; imu < 5; imu++) {
+    double a12 = 0.0, a32 = 0.0, a33 = 0.0, a13 = 0.0, a11 = 0.0;
+    for (int i = 1; i <= 3; i++)
+      for (int j = 1; j <= 3; j++) {
+        const double a = amat[imu][j - 1][i - 1];
+        if (a != 0.0) {
+          a12 = a12 + a * (DEP_(deps, ix, i, 1) * EPS_(eps, j, 2) + EPS_(eps, i, 1) * DEP_(deps, ix, j, 2));
+          a32 = a32 + a * (DEP_(deps, ix, i, 3) * EPS_(eps, j, 2) + EPS_(eps, i, 3) * DEP_(deps, ix, j, 2));
+          a33 = a33 + a * (DEP_(deps, ix, i, 3) * EPS_(eps, j, 3) + EPS_(eps, i, 3) * DEP_(deps, ix, j, 3));
+          a13 = a13 + a * (DEP_(deps, ix, i, 1) * EPS_(eps, j, 3) + EPS_(eps, i, 1) * DEP_(deps, ix, j, 3));
+          a11 = a11 + a * (DEP_(deps, ix, i, 1) * EPS_(eps, j, 1) + EPS_(eps, i, 1) * DEP_(deps, ix, j, 1));
+        }
+      }
+    ddm[0 * 5 + imu] = 2.0 * a12;
+    ddm[1 * 5 + imu] = 2.0 * a32;
+    ddm[2 * 5 + imu] = sqrt(3.0) * a33;
+    ddm[3 * 5 + imu] = 2.0 * a13;
+    ddm[4 * 5 + imu] = 2.0 * a11 + a33;
+  }
+}
+
 // Drotate, one derivative component: X = dL M R^T + L M dR^T + L dM R^T shell
 // pair by shell pair with the reference's loop order (m2, k2, m1, k1) and the
-// s-shell rotation 1, derivative 0 of Dchooser.  M, DM, X column-major, ld nmu.
+// s-shell rotation 1, derivative 0 of Dchooser; dm, ddm: the d rotation and
+// its derivative (read only for d shells).  M, DM, X column-major, ld nmu.
 __device__ inline void d_Drot(const DSpecies &sp, int in1, int in2, const double *pm, const double *dpm,
-                              const double *M, const double *DM, int nmu, double *X) {
+                              const double *dm, const double *ddm, const double *M, const double *DM, int nmu,
+                              double *X) {
   const int nnu = sp.norb[in2 - 1];
   for (int i = 0; i < nmu * nnu; i++) X[i] = 0.0;
   int n1 = 0;
@@ -170,10 +205,10 @@ __device__ inline void d_Drot(const DSpecies &sp, int in1, int in2, const double
         for (int k2 = 0; k2 < 2 * l2 + 1; k2++)
           for (int m1 = 0; m1 < 2 * l1 + 1; m1++)
             for (int k1 = 0; k1 < 2 * l1 + 1; k1++) {
-              const double lf = (l1 == 0) ? 1.0 : pm[3 * m1 + k1];
-              const double dl = (l1 == 0) ? 0.0 : dpm[3 * m1 + k1];
-              const double rt = (l2 == 0) ? 1.0 : pm[3 * m2 + k2];
-              const double dr = (l2 == 0) ? 0.0 : dpm[3 * m2 + k2];
+              const double lf = (l1 == 0) ? 1.0 : (l1 == 1 ? pm[3 * m1 + k1] : dm[5 * m1 + k1]);
+              const double dl = (l1 == 0) ? 0.0 : (l1 == 1 ? dpm[3 * m1 + k1] : ddm[5 * m1 + k1]);
+              const double rt = (l2 == 0) ? 1.0 : (l2 == 1 ? pm[3 * m2 + k2] : dm[5 * m2 + k2]);
+              const double dr = (l2 == 0) ? 0.0 : (l2 == 1 ? dpm[3 * m2 + k2] : ddm[5 * m2 + k2]);
               const int o = (n2 + m2) * nmu + (n1 + m1);
               double *x = &X[(n2 + k2) * nmu + (n1 + k1)];
               *x = *x + dl * rt * M[o] + lf * dr * M[o] + lf * rt * DM[o];
@@ -240,12 +275,23 @@ __device__ __forceinline__ int cell2c_of(const DView &v, int in1, int in2, int s
   return v.idx2c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * FB_NSLOT2 + slot) * FB_MAXSUB2 + isub];
 }
 
+// the generic kernels' sizes: s/p species (MO 4), or with d shells (MO 9)
+template <int MO> struct GenF;
+template <> struct GenF<FB_MO> {
+  static constexpr int NME2 = 16, NME3 = NME3F;
+};
+template <> struct GenF<FB_MOD> {
+  static constexpr int NME2 = FB_NME2_WIDE, NME3 = FB_NME3_WIDE;
+};
+
 // value and r1-derivative blocks (molecular frame) of one 2c interaction:
 // hm = getDMEs value, vd(ix) = -eta(ix) dF/dx where z > zcut (else 0)
+template <int MO>
 __device__ inline int d_dblock2c(const DView &v, int cid, int nmu, int nnu, double z, double *hm, double *dhm) {
   if (cid < 0) return ST_NO_TABLE;
   const DCell2c c = v.cell2c[cid];
-  double F[16], dF[16];
+  if (c.nME > GenF<MO>::NME2) return ST_CAPACITY;
+  double F[GenF<MO>::NME2], dF[GenF<MO>::NME2];
   int st = d_getD2c(c, v.tab2c, z, F, dF);
   if (st) return st;
   for (int i = 0; i < nmu * nnu; i++) { hm[i] = 0.0; dhm[i] = 0.0; }
@@ -258,16 +304,19 @@ __device__ inline int d_dblock2c(const DView &v, int cid, int nmu, int nnu, doub
 }
 
 // f(ix) -= rho(o) [dq] X_ix(o) P_eq2 over o, with X_ix the Drotate of (hm, -eta dhm)
+template <int MO>
 __device__ inline void d_contract_2c(const DView &v, int in1, int in2, int nmu, double z, double zcut,
-                                     const double *pm, const double *deps, const double *eta, const double *hm,
-                                     const double *dhm, const double *rho, double dq, bool use_dq, double *f) {
+                                     const double *pm, const double *dm, const double *eps, const double *deps,
+                                     const double *eta, const double *hm, const double *dhm, const double *rho,
+                                     double dq, bool use_dq, double *f) {
   const int nb = nmu * v.sp.norb[in2 - 1];
-  double DM[NBK], X[NBK], dpm[9];
+  double DM[MO * MO], X[MO * MO], dpm[9], ddm[MO == FB_MOD ? 25 : 1];
 #pragma unroll
   for (int ix = 0; ix < 3; ix++) {
     for (int i = 0; i < nb; i++) DM[i] = (z > zcut) ? -eta[ix] * dhm[i] : 0.0;
     d_dpmat(deps, ix + 1, dpm);
-    d_Drot(v.sp, in1, in2, pm, dpm, hm, DM, nmu, X);
+    if (MO == FB_MOD) d_ddmat(eps, deps, ix + 1, ddm);
+    d_Drot(v.sp, in1, in2, pm, dpm, dm, ddm, hm, DM, nmu, X);
     for (int i = 0; i < nb; i++) {
       if (use_dq) f[ix] = f[ix] - rho[i] * dq * X[i] * FB_P_EQ2;
       else f[ix] = f[ix] - rho[i] * X[i] * FB_P_EQ2;
@@ -276,7 +325,9 @@ __device__ inline void d_contract_2c(const DView &v, int in1, int in2, int nmu, 
 }
 
 // mask bits: 1 Dblock S, 2 Dblock T, 4 Dblock dipole_z, 8 vna_ontop / vna_atom
+template <int MO>
 __global__ __launch_bounds__(WD2) void k_d2c(DView v, DForce F, int mask) {
+  constexpr int NB = MO * MO;
   const int ia = blockIdx.x;
   const int lane = threadIdx.x;
   const int s = v.atom_struct[ia];
@@ -304,20 +355,23 @@ __global__ __launch_bounds__(WD2) void k_d2c(DView v, DForce F, int mask) {
     d_epsilon(r2, sighat, eps);
     d_Depsilon_2c(r1, r2, z, eps, deps);
     d_pmat<double>(eps, pm);
+    double dm[MO == FB_MOD ? 25 : 1], ddm[MO == FB_MOD ? 25 : 1];
+    if (MO == FB_MOD) d_dmat(eps, dm);
     for (int q = 0; q < 3; q++) eta[q] = eps[6 + q];
     const int64_t bo = v.boff[p];
-    double hm[NBK], dhm[NBK], DM[NBK], X[NBK], dpm[9];
+    double hm[NB], dhm[NB], DM[NB], X[NB], dpm[9];
     const int slots[3] = {SLOT_S, SLOT_T, SLOT_DIP};
     for (int d = 0; d < 3; d++) {
       if (!(mask & (1 << d))) continue;
-      int st = d_dblock2c(v, cell2c_of(v, in1, in2, slots[d], 0), nmu, nnu, z, hm, dhm);
+      int st = d_dblock2c<MO>(v, cell2c_of(v, in1, in2, slots[d], 0), nmu, nnu, z, hm, dhm);
       status |= st;
       if (st) continue;
       double *D = F.dmat[d] + 3 * bo;
       for (int ix = 0; ix < 3; ix++) {
         for (int i = 0; i < nb; i++) DM[i] = (z > 1.0e-3) ? -eta[ix] * dhm[i] : 0.0;
         d_dpmat(deps, ix + 1, dpm);
-        d_Drot(v.sp, in1, in2, pm, dpm, hm, DM, nmu, X);
+        if (MO == FB_MOD) d_ddmat(eps, deps, ix + 1, ddm);
+        d_Drot(v.sp, in1, in2, pm, dpm, dm, ddm, hm, DM, nmu, X);
         for (int i = 0; i < nb; i++) D[ix + 3 * i] = X[i];
       }
     }
@@ -331,24 +385,24 @@ __global__ __launch_bounds__(WD2) void k_d2c(DView v, DForce F, int mask) {
           const int slot = side ? SLOT_ONR : SLOT_ONL;
           const int qa = side ? ja : ia;
           const int inq = side ? in2 : in1;
-          int st = d_dblock2c(v, cell2c_of(v, in1, in2, slot, 0), nmu, nnu, z, hm, dhm);
+          int st = d_dblock2c<MO>(v, cell2c_of(v, in1, in2, slot, 0), nmu, nnu, z, hm, dhm);
           status |= st;
-          if (!st) d_contract_2c(v, in1, in2, nmu, z, 1.0e-3, pm, deps, eta, hm, dhm, rho, 0.0, false, fo);
+          if (!st) d_contract_2c<MO>(v, in1, in2, nmu, z, 1.0e-3, pm, dm, eps, deps, eta, hm, dhm, rho, 0.0, false, fo);
           for (int isorp = 1; isorp <= v.sp.nssh[inq - 1]; isorp++) {
             const double dQ = v.dQ[qa * FB_MAXSH + isorp - 1];
             if (dQ == 0.0) continue;
-            st = d_dblock2c(v, cell2c_of(v, in1, in2, slot, isorp), nmu, nnu, z, hm, dhm);
+            st = d_dblock2c<MO>(v, cell2c_of(v, in1, in2, slot, isorp), nmu, nnu, z, hm, dhm);
             status |= st;
-            if (!st) d_contract_2c(v, in1, in2, nmu, z, 1.0e-3, pm, deps, eta, hm, dhm, rho, dQ, true, fo);
+            if (!st) d_contract_2c<MO>(v, in1, in2, nmu, z, 1.0e-3, pm, dm, eps, deps, eta, hm, dhm, rho, dQ, true, fo);
           }
         }
       }
       // atom term <i|V_j|i>, contracted with the density block of neigh_self (:1005-1190)
       double fa[3] = {0.0, 0.0, 0.0};
       const double *rho_s = F.rho + v.boff[pself];
-      int st = d_dblock2c(v, cell2c_of(v, in1, in2, SLOT_ATOM, 0), nmu, nmu, z, hm, dhm);
+      int st = d_dblock2c<MO>(v, cell2c_of(v, in1, in2, SLOT_ATOM, 0), nmu, nmu, z, hm, dhm);
       status |= st;
-      if (!st) d_contract_2c(v, in1, in1, nmu, z, 1.0e-3, pm, deps, eta, hm, dhm, rho_s, 0.0, false, fa);
+      if (!st) d_contract_2c<MO>(v, in1, in1, nmu, z, 1.0e-3, pm, dm, eps, deps, eta, hm, dhm, rho_s, 0.0, false, fa);
       bool charged = false;
       for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) charged |= (v.dQ[ja * FB_MAXSH + isorp - 1] != 0.0);
       if (charged) {
@@ -359,15 +413,16 @@ __global__ __launch_bounds__(WD2) void k_d2c(DView v, DForce F, int mask) {
         for (int isorp = 1; isorp <= v.sp.nssh[in2 - 1]; isorp++) {
           const double dQ = v.dQ[ja * FB_MAXSH + isorp - 1];
           if (dQ == 0.0) continue;
-          st = d_dblock2c(v, cell2c_of(v, in1, in2, SLOT_ATOM, isorp), nmu, nmu, z, hm, dhm);
+          st = d_dblock2c<MO>(v, cell2c_of(v, in1, in2, SLOT_ATOM, isorp), nmu, nmu, z, hm, dhm);
           status |= st;
           if (st) continue;
-          double bx[NBK];
-          d_rotate<double>(v.sp, in1, in1, pm, hm, nmu, bx);
+          double bx[NB];
+          d_rotate<double>(v.sp, in1, in1, pm, hm, nmu, bx, MO == FB_MOD ? dm : nullptr);
           for (int ix = 0; ix < 3; ix++) {
             for (int i = 0; i < na; i++) DM[i] = (z > 1.0e-4) ? -eta[ix] * dhm[i] : 0.0;
             d_dpmat(deps, ix + 1, dpm);
-            d_Drot(v.sp, in1, in1, pm, dpm, hm, DM, nmu, X);
+            if (MO == FB_MOD) d_ddmat(eps, deps, ix + 1, ddm);
+            d_Drot(v.sp, in1, in1, pm, dpm, dm, ddm, hm, DM, nmu, X);
             for (int i = 0; i < na; i++) {
               const double emnpl = (z > 1.0e-4) ? sself[i] / z : 0.0;
               const double vdem = (z > 1.0e-4) ? eta[ix] * sself[i] / (z * z) : 0.0;
@@ -395,11 +450,13 @@ __device__ __forceinline__ int group3c_of(const DView &v, int in1, int in2, int 
 
 // getDMEs_Fdata_3c: value, d/dcos, d/dx, d/dy blocks (molecular frame).  The
 // 3-point rule takes the slopes of ME 1 of theta cell 1 for every element.
+template <int MO>
 __device__ inline int d_getD3c(const DView &v, int gid, double x, double y, double cost, int nmu, int nnu,
                                double *hm, double *dpm_, double *dxm, double *dym) {
   if (gid < 0) return ST_NO_TABLE;
   const DGroup3c g = v.grp3c[gid];
   const int nME = g.nME, nx = g.nx, ny = g.ny;
+  if (nME > GenF<MO>::NME3) return ST_CAPACITY;
   double p[5], Dp[5];
   const double cost2 = cost * cost;
   const double sint = sqrt(fmax(1.0e-5, 1.0 - cost2));
@@ -426,7 +483,7 @@ __device__ inline int d_getD3c(const DView &v, int gid, double x, double y, doub
   const double gx = (T3F(0, ix + 1, iy, 0) - T3F(0, ix, iy, 0)) / dx;
   const double gy = (T3F(0, ix, iy + 1, 0) - T3F(0, ix, iy, 0)) / dy;
   const double gg = gx * gx + gy * gy;
-  double Fv[NME3F], DpF[NME3F], DxF[NME3F], DyF[NME3F];
+  double Fv[GenF<MO>::NME3], DpF[GenF<MO>::NME3], DxF[GenF<MO>::NME3], DyF[GenF<MO>::NME3];
   for (int m = 0; m < nME; m++) { Fv[m] = 0.0; DpF[m] = 0.0; DxF[m] = 0.0; DyF[m] = 0.0; }
   for (int th = 0; th < FB_NTHETA; th++) {
     const double pt = p[th], dpt = Dp[th];
@@ -534,11 +591,12 @@ __device__ inline void d_dgeo3(const DView &v, int4 tr, bool charged, DGeo3 &G) 
 
 // f3naMa / f3naMb of component ix (M_Dassemble_3c_DOGS.f90:388-399) on the
 // elements of the group's cell, rotated into Xa / Xb
+template <int MO>
 __device__ inline void d_f3_rot(const DView &v, const DGroup3c &g, int in1, int in2, int nmu, const DGeo3 &G,
-                                const double *pm, int ix, const double *hm, const double *dp, const double *dxm,
-                                const double *dym, double *Xa, double *Xb) {
+                                const double *pm, const double *dm, int ix, const double *hm, const double *dp,
+                                const double *dxm, const double *dym, double *Xa, double *Xb) {
   const int nb = nmu * v.sp.norb[in2 - 1];
-  double Ma[NBK], Mb[NBK], dpm[9];
+  double Ma[MO * MO], Mb[MO * MO], dpm[9], ddm[MO == FB_MOD ? 25 : 1];
   for (int i = 0; i < nb; i++) { Ma[i] = 0.0; Mb[i] = 0.0; }
   for (int k = 0; k < g.nME; k++) {
     const int o = (v.nu[g.munu + k] - 1) * nmu + (v.mu[g.munu + k] - 1);
@@ -546,14 +604,18 @@ __device__ inline void d_f3_rot(const DView &v, const DGroup3c &g, int in1, int 
     Mb[o] = -G.sighat[ix] * dym[o] + G.bmt[ix] * dp[o] - Ma[o] / 2.0;
   }
   d_dpmat(G.depsA, ix + 1, dpm);
-  d_Drot(v.sp, in1, in2, pm, dpm, hm, Ma, nmu, Xa);
+  if (MO == FB_MOD) d_ddmat(G.eps, G.depsA, ix + 1, ddm);
+  d_Drot(v.sp, in1, in2, pm, dpm, dm, ddm, hm, Ma, nmu, Xa);
   d_dpmat(G.depsB, ix + 1, dpm);
-  d_Drot(v.sp, in1, in2, pm, dpm, hm, Mb, nmu, Xb);
+  if (MO == FB_MOD) d_ddmat(G.eps, G.depsB, ix + 1, ddm);
+  d_Drot(v.sp, in1, in2, pm, dpm, dm, ddm, hm, Mb, nmu, Xb);
 }
 
 #define T3D 64
 
+template <int MO>
 __global__ __launch_bounds__(T3D) void k_d3c(DView v, DForce F) {
+  constexpr int NB = MO * MO;
   const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * T3D + threadIdx.x;
   if (t >= v.ntrip) return;
   const int4 tr = F.trip[t];
@@ -564,18 +626,19 @@ __global__ __launch_bounds__(T3D) void k_d3c(DView v, DForce F) {
   const double *rho = F.rho + bo;
   int status = 0;
   double fa[3] = {0.0, 0.0, 0.0}, fb[3] = {0.0, 0.0, 0.0}, fc[3] = {0.0, 0.0, 0.0};
-  double hm[NBK], dp[NBK], dxm[NBK], dym[NBK], Xa[NBK], Xb[NBK], pm[9];
+  double hm[NB], dp[NB], dxm[NB], dym[NB], Xa[NB], Xb[NB], pm[9], dm[MO == FB_MOD ? 25 : 1];
   {
     DGeo3 G;
     d_dgeo3(v, tr, false, G);
     d_pmat<double>(G.eps, pm);
+    if (MO == FB_MOD) d_dmat(G.eps, dm);
     const int gid = group3c_of(v, in1, in2, indna, 0);
-    int st = d_getD3c(v, gid, G.x, G.z, G.cost, nmu, nnu, hm, dp, dxm, dym);
+    int st = d_getD3c<MO>(v, gid, G.x, G.z, G.cost, nmu, nnu, hm, dp, dxm, dym);
     status |= st;
     if (!st) {
       const DGroup3c g = v.grp3c[gid];
       for (int ix = 0; ix < 3; ix++) {
-        d_f3_rot(v, g, in1, in2, nmu, G, pm, ix, hm, dp, dxm, dym, Xa, Xb);
+        d_f3_rot<MO>(v, g, in1, in2, nmu, G, pm, dm, ix, hm, dp, dxm, dym, Xa, Xb);
         for (int i = 0; i < nb; i++) {
           const double xc = -Xa[i] - Xb[i];
           fa[ix] = fa[ix] - rho[i] * Xa[i] * FB_P_EQ2;
@@ -592,6 +655,7 @@ __global__ __launch_bounds__(T3D) void k_d3c(DView v, DForce F) {
     DGeo3 G;
     d_dgeo3(v, tr, true, G);
     d_pmat<double>(G.eps, pm);
+    if (MO == FB_MOD) d_dmat(G.eps, dm);
     double rA1[3] = {0.0, 0.0, 0.0}, rA2[3] = {0.0, 0.0, 0.0};
     if (G.d13 > 1.0e-5) for (int k = 0; k < 3; k++) rA1[k] = (G.rna[k] - G.r1[k]) / G.d13;
     if (G.d23 > 1.0e-5) for (int k = 0; k < 3; k++) rA2[k] = (G.rna[k] - G.r2[k]) / G.d23;
@@ -613,14 +677,14 @@ __global__ __launch_bounds__(T3D) void k_d3c(DView v, DForce F) {
       const double dQ = v.dQ[tr.x * FB_MAXSH + isorp - 1];
       if (dQ == 0.0) continue;
       const int gid = group3c_of(v, in1, in2, indna, isorp);
-      int st = d_getD3c(v, gid, G.x, G.z, G.cost, nmu, nnu, hm, dp, dxm, dym);
+      int st = d_getD3c<MO>(v, gid, G.x, G.z, G.cost, nmu, nnu, hm, dp, dxm, dym);
       status |= st;
       if (st) continue;
       const DGroup3c g = v.grp3c[gid];
-      double bx[NBK];
-      d_rotate<double>(v.sp, in1, in2, pm, hm, nmu, bx);
+      double bx[NB];
+      d_rotate<double>(v.sp, in1, in2, pm, hm, nmu, bx, MO == FB_MOD ? dm : nullptr);
       for (int ix = 0; ix < 3; ix++) {
-        d_f3_rot(v, g, in1, in2, nmu, G, pm, ix, hm, dp, dxm, dym, Xa, Xb);
+        d_f3_rot<MO>(v, g, in1, in2, nmu, G, pm, dm, ix, hm, dp, dxm, dym, Xa, Xb);
         for (int i = 0; i < nb; i++) {
           const double sterm = sb[i] / 2.0, dterm = db[i] / G.z;
           const double emnpl = (sterm - dterm) / G.d13 + (sterm + dterm) / G.d23;
@@ -711,13 +775,20 @@ __global__ void k_trip_jkey(int64_t ntrip, const int4 *trip, int *key) {
 
 extern "C" int fbk_d2c(const DView &v, const DForce &f, int mask, hipStream_t st) {
   if (v.natoms == 0) return 0;
-  hipLaunchKernelGGL(k_d2c, dim3(v.natoms), dim3(WD2), 0, st, v, f, mask);
+  if (fb_wide_species(v.sp))
+    hipLaunchKernelGGL(k_d2c<FB_MOD>, dim3(v.natoms), dim3(WD2), 0, st, v, f, mask);
+  else
+    hipLaunchKernelGGL(k_d2c<FB_MO>, dim3(v.natoms), dim3(WD2), 0, st, v, f, mask);
   return (int)hipGetLastError();
 }
 
 extern "C" int fbk_d3c(const DView &v, const DForce &f, hipStream_t st) {
   if (v.ntrip == 0) return 0;
-  hipLaunchKernelGGL(k_d3c, dim3((unsigned)((v.ntrip + T3D - 1) / T3D)), dim3(T3D), 0, st, v, f);
+  const dim3 grid((unsigned)((v.ntrip + T3D - 1) / T3D));
+  if (fb_wide_species(v.sp))
+    hipLaunchKernelGGL(k_d3c<FB_MOD>, grid, dim3(T3D), 0, st, v, f);
+  else
+    hipLaunchKernelGGL(k_d3c<FB_MO>, grid, dim3(T3D), 0, st, v, f);
   return (int)hipGetLastError();
 }
 
