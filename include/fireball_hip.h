@@ -40,7 +40,7 @@ enum fb_status {
   FB_E_ORDER = -8,            /* assemble_* called before its inputs exist */
   FB_E_HIP = -9,              /* HIP runtime error */
   FB_E_CAPACITY = -10,        /* a list exceeds a kernel limit */
-  FB_E_UNSUPPORTED = -11      /* e.g. a shell with l > 2, or d shells in the PP assemblers */
+  FB_E_UNSUPPORTED = -11      /* e.g. a shell with l > 2, or a PP projector shell beyond p */
 };
 
 enum fb_matrix {            /* which block array */
@@ -81,11 +81,10 @@ int fb_set_3c_method(fb_ctx *ctx, int method);        /* enum fb_method3c */
  * reference's shell(issh)%rcutoffA).  Shells s, p and d (l <= 2), at most 9
  * orbitals per species.  Species with d shells (or more than 4 orbitals) are
  * assembled by the H+S entry points (S, T, dipole_z, vna_2c, vna_3c, their
- * lists and blocks), the McWEDA density and the force entry points in
- * binary64 whatever the precision mode -- the d rotation of twister and its
- * derivative, c.SYSTEM/M_rotations.f90:272-300, M_Drotations.f90:203-298 --
- * and fb_set_species_pp then returns FB_E_UNSUPPORTED (the pseudopotential
- * assemblers hold s and p orbital blocks). */
+ * lists and blocks), the McWEDA density, force and pseudopotential entry
+ * points in binary64 whatever the precision mode -- the d rotation of twister
+ * and its derivative, c.SYSTEM/M_rotations.f90:272-300,
+ * M_Drotations.f90:203-298.  Pseudopotential projector shells are s or p. */
 int fb_set_species(fb_ctx *ctx, int nspecies, const int *nZ, const int *nssh, const int *lssh,
                    const double *rcutoffA);
 

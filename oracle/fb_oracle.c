@@ -2091,7 +2091,7 @@ int fbo_build_pp_lists(fbo_system *s) {
 int fbo_assemble_svnl(fbo_system *s) {
   const fbo_tables *t = s->t;
   if (!s->pp_n) return FBO_E_ORDER;
-  double sm[64], eps[9];
+  double sm[81], eps[9];
   for (int ia = 1; ia <= s->natoms; ia++) {
     double r1[3] = {RAT(s, ia)[0], RAT(s, ia)[1], RAT(s, ia)[2]};
     int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1];
@@ -2126,7 +2126,7 @@ int fbo_assemble_vnl_2c(fbo_system *s) {
   const fbo_tables *t = s->t;
   if (!s->ppdone[0]) return FBO_E_ORDER;
   memset(s->vnl, 0, sizeof(double) * (s->vl_len ? s->vl_len : 1));
-  double X[64], cl[16];
+  double X[81], cl[16];
   /* atom case: every PP neighbour's projectors into the PPp self block */
   for (int ia = 1; ia <= s->natoms; ia++) {
     int in1 = s->imass[ia - 1], nmu = t->norb[in1 - 1];
@@ -2200,7 +2200,7 @@ static int mpairnay(const fbo_system *s, int ia, int ja, const double *r12, int 
 int fbo_assemble_vnl_3c(fbo_system *s) {
   const fbo_tables *t = s->t;
   if (!s->ppdone[1]) return FBO_E_ORDER;
-  double X[64], cl[16];
+  double X[81], cl[16];
   for (int ial = 1; ial <= s->natoms; ial++) {
     const double *rna = RAT(s, ial);
     int indna = s->imass[ial - 1];

@@ -11,6 +11,7 @@ build container, where the reference sources are present).
     python tests/golden/make_golden.py --d        # H/S fixtures of species with d shells
     python tests/golden/make_golden.py --rho --d  # density fixtures of species with d shells
     python tests/golden/make_golden.py --forces --d  # force fixtures of species with d shells
+    python tests/golden/make_golden.py --pp --d   # pseudopotential fixtures of species with d shells
 
 Each fixture is an .npz holding the inputs that are not re-derivable from a
 seed (explicit positions for the hand-made cases), the reference's neighbour
@@ -224,7 +225,7 @@ def main():
     if "--pp" in sys.argv:
         if not os.path.isfile(refrun.HARNESS_PREF):
             sys.exit("pseudopotential harness not built: run `make -C oracle pref` first")
-        for name, ts, st, keep_atoms in cases():
+        for name, ts, st, keep_atoms in (cases_d() if "--d" in sys.argv else cases()):
             pp_fixture(name, ts, st, keep_atoms if keep_atoms is not None else (10 if st.natoms > 32 else None))
         return
     if "--rho" in sys.argv:

@@ -606,17 +606,15 @@ def test_d_shells_mixed_context_takes_binary64_path(golden, name):
         assert_bits(r[m][:n], g[m], m + " vs golden")
 
 
-def test_d_shells_rejected_by_pp(golden):
-    """The pseudopotential kernels hold s and p orbital blocks only: a species
-    with d shells stops with FB_E_UNSUPPORTED when PP shells are set (the H+S,
-    density and force kernels take d shells: the golden tests of
-    test_gpu_parity.py, test_gpu_rho.py and test_gpu_forces.py)."""
+def test_pp_shells_beyond_p_rejected(golden):
+    """The pseudopotential kernels take s and p projector shells (and orbital
+    shells up to d): a d projector shell stops with FB_E_UNSUPPORTED."""
     import dataclasses
     from fireball_hip import synth
     from fireball_hip.lib import Assembler, FireballError
     g = golden("spd_cluster_dq")
     ts, _ = golden_cases.inputs(g)
-    species = [synth.Species(sp.nZ, list(sp.lssh), list(sp.rcutoff), [0, 1], 1.6, [-3.0, 1.5]) for sp in ts.species]
+    species = [synth.Species(sp.nZ, list(sp.lssh), list(sp.rcutoff), [0, 2], 1.6, [-3.0, 1.5]) for sp in ts.species]
     with pytest.raises(FireballError) as e:
         Assembler(dataclasses.replace(ts, species=species))
     assert e.value.code == -11

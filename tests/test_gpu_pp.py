@@ -33,7 +33,7 @@ def _same(got, ref, what):
     assert bad.size == 0, f"{what}: {bad.size} values differ, first {bad[:5]}"
 
 
-@pytest.mark.parametrize("name", golden_cases.NAMES)
+@pytest.mark.parametrize("name", golden_cases.NAMES + golden_cases.NAMES_D)
 def test_pp_golden(golden, name):
     print(f"case {name}", flush=True)
     g, gp = golden(name), golden(name + "_pp")

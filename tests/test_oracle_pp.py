@@ -23,7 +23,7 @@ from golden.make_golden import PP_LIST_KEYS, pp_tables, tables_digest
 @pytest.fixture(scope="module")
 def pp_runs(golden):
     out = {}
-    for name in golden_cases.NAMES:
+    for name in golden_cases.NAMES + golden_cases.NAMES_D:
         g, gp = golden(name), golden(name + "_pp")
         ts, st = golden_cases.inputs(g)
         tsp = pp_tables(ts, int(gp["pp_table_seed"]))
@@ -33,13 +33,13 @@ def pp_runs(golden):
     return out
 
 
-@pytest.mark.parametrize("name", golden_cases.NAMES)
+@pytest.mark.parametrize("name", golden_cases.NAMES + golden_cases.NAMES_D)
 def test_pp_tables_regenerate(pp_runs, name):
     gp, tsp, _, _ = pp_runs[name]
     assert tables_digest(tsp) == str(gp["pp_tables_digest"])
 
 
-@pytest.mark.parametrize("name", golden_cases.NAMES)
+@pytest.mark.parametrize("name", golden_cases.NAMES + golden_cases.NAMES_D)
 def test_pp_bit_exact_against_reference(pp_runs, name):
     gp, _, _, r = pp_runs[name]
     for k in PP_LIST_KEYS:

@@ -1640,13 +1640,6 @@ int fb_get_common(fb_ctx *ctx, int is, int *ncommon, int *iatom_j, int *iatom_b,
 // The density, pseudopotential and force kernels hold blocks of s and p
 // species (at most FB_MO orbitals); species with d shells (or more than four
 // orbitals) are assembled by S, T, dipole_z, vna_2c and vna_3c only.
-static int sp_orbitals_only(fb_ctx *ctx, const char *what) {
-  if (!fb_wide_species(ctx->sp)) return FB_OK;
-  return ctx->fail(FB_E_UNSUPPORTED, (std::string(what) +
-                                      ": a species has more than 4 orbitals (d shells); the pseudopotential "
-                                      "assemblers hold s and p orbital blocks only").c_str());
-}
-
 static int pre_assemble(fb_ctx *ctx, const char *what) {
   if (!ctx) return FB_E_ARG;
   if (!ctx->lists_ok) return ctx->fail(FB_E_ORDER, (std::string(what) + " before build_lists").c_str());
@@ -2517,7 +2510,6 @@ int fb_set_species_pp(fb_ctx *ctx, const int *nssh_pp, const int *lssh_pp, const
                       const double *cl) {
   if (!ctx || !nssh_pp || !lssh_pp || !rcutoff_pp || !cl) return ctx ? ctx->fail(FB_E_ARG, "set_species_pp") : FB_E_ARG;
   if (!ctx->species_set) return ctx->fail(FB_E_ORDER, "set_species_pp before set_species");
-  RC(sp_orbitals_only(ctx, "set_species_pp"));
   DPP h{};
   for (int i = 0; i < ctx->sp.ns; i++) {
     if (nssh_pp[i] < 0 || nssh_pp[i] > FB_MAXSH) return ctx->fail(FB_E_ARG, "set_species_pp: nssh_PP out of range");

@@ -509,9 +509,10 @@ __global__ void k_pp_block_sizes(DView v, DPP q, int which, int64_t *sz) {
 
 // ------------------------------------------------------------------ blocks
 // rotate_PP (c.SYSTEM/rotate_PP.f90): orbital shells of in1 on the left, PP
-// shells of in2 on the right, the loop order and association of rotate
+// shells of in2 (s and p) on the right, the loop order and association of
+// rotate; dm: the d rotation, read only for a d orbital shell
 __device__ inline void d_rotate_pp(const DSpecies &sp, const DPP &q, int in1, int in2, const double *pm,
-                                   const double *M, int nmu, double *X) {
+                                   const double *dm, const double *M, int nmu, double *X) {
   const int nnu = q.norb_pp[in2 - 1];
   for (int i = 0; i < nmu * nnu; i++) X[i] = 0.0;
   int n1 = 0;
@@ -524,7 +525,7 @@ __device__ inline void d_rotate_pp(const DSpecies &sp, const DPP &q, int in1, in
         for (int m1 = 0; m1 < 2 * l1 + 1; m1++)
           for (int k2 = 0; k2 < 2 * l2 + 1; k2++)
             for (int k1 = 0; k1 < 2 * l1 + 1; k1++) {
-              const double left = (l1 == 0) ? 1.0 : pm[3 * m1 + k1];
+              const double left = (l1 == 0) ? 1.0 : (l1 == 1 ? pm[3 * m1 + k1] : dm[5 * m1 + k1]);
               const double right = (l2 == 0) ? 1.0 : pm[3 * m2 + k2];
               double *x = &X[(n2 + k2) * nmu + (n1 + k1)];
               *x = *x + left * M[(n2 + m2) * nmu + (n1 + m1)] * right;
@@ -535,8 +536,11 @@ __device__ inline void d_rotate_pp(const DSpecies &sp, const DPP &q, int in1, in
   }
 }
 
-// assemble_svnl: one lane per PP pair
+// assemble_svnl: one lane per PP pair (the generic kernels take MO orbitals
+// per species: 4 for s/p species, 9 with d shells)
+template <int MO>
 __global__ void k_svnl(DView v, DPP q) {
+  constexpr int NB = MO * MO;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= q.npairs[PPL_PP]) return;
   const int ia = q.atom[PPL_PP][p];
@@ -556,19 +560,21 @@ __global__ void k_svnl(DView v, DPP q) {
   else for (int k = 0; k < 3; k++) sighat[k] = (r2[k] - r1[k]) / z;
   d_epsilon(r2, sighat, eps);
   d_pmat<double>(eps, pm);
+  double dm[MO == FB_MOD ? 25 : 1];
+  if (MO == FB_MOD) d_dmat(eps, dm);
   const int cid = v.idx2c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * FB_NSLOT2 + SLOT_VNL) * FB_MAXSUB2];
   int status = 0;
-  double hm[NBP], X[NBP];
+  double hm[NB], X[NB];
   if (cid < 0) {
     status = ST_NO_TABLE;
   } else {
     const DCell2c c = v.cell2c[cid];
-    double F[NBP];
-    status = (c.nME > NBP) ? ST_CAPACITY : d_interp2c(c, v.tab2c, z, F);
+    double F[NB];
+    status = (c.nME > NB) ? ST_CAPACITY : d_interp2c(c, v.tab2c, z, F);
     if (!status) {
       for (int i = 0; i < nmu * nnu; i++) hm[i] = 0.0;
       for (int k = 0; k < c.nME; k++) hm[(v.nu[c.munu + k] - 1) * nmu + (v.mu[c.munu + k] - 1)] = F[k];
-      d_rotate_pp(v.sp, q, in1, in2, pm, hm, nmu, X);
+      d_rotate_pp(v.sp, q, in1, in2, pm, dm, hm, nmu, X);
       double *dst = q.svnl + q.sv_off[p];
       for (int i = 0; i < nmu * nnu; i++) dst[i] = X[i];
     }
@@ -597,10 +603,12 @@ __device__ __forceinline__ void kb_contract(int nmu, int nnu, int in_pp, const D
 // neighbours in order, sum_c cl(c) sVNL(mu, c) sVNL(nu, c) of that neighbour's
 // projectors; one lane per (atom, block element), each element summed in the
 // reference's order.
+template <int MO>
 __global__ void k_vnl_2c_self(DView v, DPP q) {
+  constexpr int NB = MO * MO;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)v.natoms * NBP) return;
-  const int ia = (int)(t / NBP), e = (int)(t % NBP);
+  if (t >= (int64_t)v.natoms * NB) return;
+  const int ia = (int)(t / NB), e = (int)(t % NB);
   const int in1 = v.imass[ia];
   const int nmu = v.sp.norb[in1 - 1];
   if (e >= nmu * nmu) return;
@@ -625,7 +633,9 @@ __global__ void k_vnl_2c_self(DView v, DPP q) {
 // term (the PPx pair mapped onto it, projectors of iatom) and then its
 // ontop-right term (the PP pair mapped onto it, projectors of jatom), as the
 // reference's two loops add them.
+template <int MO>
 __global__ void k_vnl_2c(DView v, DPP q) {
+  constexpr int NB = MO * MO;
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= q.npairs[PPL_PPP]) return;
   const int ia = q.atom[PPL_PPP][k];
@@ -635,7 +645,7 @@ __global__ void k_vnl_2c(DView v, DPP q) {
   const int ja = aoff + q.j[PPL_PPP][k] - 1;
   const int in1 = v.imass[ia], in2 = v.imass[ja];
   const int nmu = v.sp.norb[in1 - 1], nnu = v.sp.norb[in2 - 1], nb = nmu * nnu;
-  double acc[NBP], X[NBP];
+  double acc[NB], X[NB];
   for (int i = 0; i < nb; i++) acc[i] = 0.0;
   const int64_t o0 = q.off[PPL_PP][ia];
   const int px = q.inv_px[k];
@@ -701,8 +711,10 @@ __global__ void k_pp_mpair(DView v, DPP q) {
 // assemble_vnl_3c: the workgroup owns the target segments that start in its
 // tile of sorted triplets; each thread forms bcnlx of one triplet per chunk
 // into LDS, one owner per segment adds them in list order onto the block
+template <int MO>
 __global__ __launch_bounds__(T3P) void k_vnl_3c(DView v, DPP q) {
-  __shared__ double s_c[T3P][NBP + 1];
+  constexpr int NB = MO * MO;
+  __shared__ double s_c[T3P][NB + 1];
   const int64_t ts = xcd_tile(blockIdx.x, gridDim.x) * T3P;
   const int64_t te = min(ts + T3P, q.ntrip);
   const int64_t sgs = q.trip_seg[ts];
@@ -714,7 +726,7 @@ __global__ __launch_bounds__(T3P) void k_vnl_3c(DView v, DPP q) {
   const bool own = my <= last;
   int64_t a = 0, b = 0, dst = 0;
   int nb = 0;
-  double acc[NBP];
+  double acc[NB];
   if (own) {
     a = q.seg_off[my];
     b = q.seg_off[my + 1];
@@ -723,13 +735,13 @@ __global__ __launch_bounds__(T3P) void k_vnl_3c(DView v, DPP q) {
     const int ja = v.st[v.atom_struct[ia]].atom_off + q.j[PPL_PPP][k] - 1;
     nb = v.sp.norb[v.imass[ia] - 1] * v.sp.norb[v.imass[ja] - 1];
     dst = q.vl_off[k];
-    for (int i = 0; i < NBP; i++) acc[i] = (i < nb) ? q.vnl[dst + i] : 0.0;
+    for (int i = 0; i < NB; i++) acc[i] = (i < nb) ? q.vnl[dst + i] : 0.0;
   }
   int status = 0;
   for (int64_t c0 = r0; c0 < r1; c0 += T3P) {
     const int64_t t = c0 + threadIdx.x;
-    double X[NBP];
-    for (int i = 0; i < NBP; i++) X[i] = 0.0;
+    double X[NB];
+    for (int i = 0; i < NB; i++) X[i] = 0.0;
     if (t < r1) {
       const int4 tr = q.trip_s[t];
       const int ial = tr.x, gia = tr.y, gja = tr.z;
@@ -744,18 +756,18 @@ __global__ __launch_bounds__(T3P) void k_vnl_3c(DView v, DPP q) {
         kb_contract(nmu, nnu, v.imass[ial], q, sv1, sv2, X);
       }
     }
-    for (int i = 0; i < NBP; i++) s_c[threadIdx.x][i] = X[i];
+    for (int i = 0; i < NB; i++) s_c[threadIdx.x][i] = X[i];
     __syncthreads();
     if (own) {
       const int64_t k0 = max(a, c0), k1 = min(b, c0 + T3P);
       for (int64_t kk = k0; kk < k1; kk++)
-        for (int i = 0; i < NBP; i++)
+        for (int i = 0; i < NB; i++)
           if (i < nb) acc[i] = acc[i] + s_c[kk - c0][i];
     }
     __syncthreads();
   }
   if (own)
-    for (int i = 0; i < NBP; i++)
+    for (int i = 0; i < NB; i++)
       if (i < nb) q.vnl[dst + i] = acc[i];
   if (status) atomicOr(v.status, status);
 }
@@ -1116,8 +1128,10 @@ int fbk_svnl(const DView &v, const DPP &q, hipStream_t st) {
   if (q.npairs[PPL_PP] == 0) return 0;
   if (q.canon)
     hipLaunchKernelGGL(k_svnl_sp, dim3(nblk(q.npairs[PPL_PP], 128)), dim3(128), 0, st, v, q);
+  else if (fb_wide_species(v.sp))
+    hipLaunchKernelGGL(k_svnl<FB_MOD>, dim3(nblk(q.npairs[PPL_PP], 128)), dim3(128), 0, st, v, q);
   else
-    hipLaunchKernelGGL(k_svnl, dim3(nblk(q.npairs[PPL_PP], 128)), dim3(128), 0, st, v, q);
+    hipLaunchKernelGGL(k_svnl<FB_MO>, dim3(nblk(q.npairs[PPL_PP], 128)), dim3(128), 0, st, v, q);
   return (int)hipGetLastError();
 }
 
@@ -1125,12 +1139,17 @@ int fbk_vnl_2c(const DView &v, const DPP &q, hipStream_t st) {
   if (q.npairs[PPL_PPP] == 0) return 0;
   if (q.canon)
     hipLaunchKernelGGL(k_vnl_2c_self_sp, dim3(nblk((int64_t)v.natoms * NBP, 256)), dim3(256), 0, st, v, q);
+  else if (fb_wide_species(v.sp))
+    hipLaunchKernelGGL(k_vnl_2c_self<FB_MOD>, dim3(nblk((int64_t)v.natoms * FB_MOD * FB_MOD, 256)), dim3(256), 0, st,
+                       v, q);
   else
-    hipLaunchKernelGGL(k_vnl_2c_self, dim3(nblk((int64_t)v.natoms * NBP, 256)), dim3(256), 0, st, v, q);
+    hipLaunchKernelGGL(k_vnl_2c_self<FB_MO>, dim3(nblk((int64_t)v.natoms * NBP, 256)), dim3(256), 0, st, v, q);
   if (q.canon)
     hipLaunchKernelGGL(k_vnl_2c_sp, dim3(nblk(q.npairs[PPL_PPP], 128)), dim3(128), 0, st, v, q);
+  else if (fb_wide_species(v.sp))
+    hipLaunchKernelGGL(k_vnl_2c<FB_MOD>, dim3(nblk(q.npairs[PPL_PPP], 128)), dim3(128), 0, st, v, q);
   else
-    hipLaunchKernelGGL(k_vnl_2c, dim3(nblk(q.npairs[PPL_PPP], 128)), dim3(128), 0, st, v, q);
+    hipLaunchKernelGGL(k_vnl_2c<FB_MO>, dim3(nblk(q.npairs[PPL_PPP], 128)), dim3(128), 0, st, v, q);
   return (int)hipGetLastError();
 }
 
@@ -1144,8 +1163,10 @@ int fbk_vnl_3c(const DView &v, const DPP &q, hipStream_t st) {
   if (q.ntrip == 0) return 0;
   if (q.canon) {
     if (q.ntiles > 0) hipLaunchKernelGGL(k_vnl_3c_sp, dim3((unsigned)q.ntiles), dim3(T3P), 0, st, v, q);
+  } else if (fb_wide_species(v.sp)) {
+    hipLaunchKernelGGL(k_vnl_3c<FB_MOD>, dim3(nblk(q.ntrip, T3P)), dim3(T3P), 0, st, v, q);
   } else {
-    hipLaunchKernelGGL(k_vnl_3c, dim3(nblk(q.ntrip, T3P)), dim3(T3P), 0, st, v, q);
+    hipLaunchKernelGGL(k_vnl_3c<FB_MO>, dim3(nblk(q.ntrip, T3P)), dim3(T3P), 0, st, v, q);
   }
   return (int)hipGetLastError();
 }

@@ -420,7 +420,7 @@ def add_rho_tables(ts: TableSet, seed: int = 4048, with_3c: bool = True) -> Tabl
 # pseudopotential shells of the synthetic species: (lssh_PP, rcutoff_PP as the
 # neighbour code uses it (Angstrom), Kleinman-Bylander cl per PP shell)
 PP_SHELLS = {1: ([0], 1.05, [-2.1]), 6: ([0, 1], 1.40, [-3.2, 1.7]), 7: ([0, 1], 1.30, [-3.6, 1.9]),
-             8: ([0, 1], 1.25, [-4.1, 2.3]), 14: ([0, 1], 1.90, [-2.6, 1.3])}
+             8: ([0, 1], 1.25, [-4.1, 2.3]), 14: ([0, 1], 1.90, [-2.6, 1.3]), 26: ([0, 1], 1.70, [-3.4, 1.6])}
 
 
 def with_pp(species):
