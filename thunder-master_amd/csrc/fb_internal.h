@@ -48,8 +48,8 @@ struct DSpecies {
 #define FB_NME2_WIDE 25   // 2c matrix elements the generic kernels hold with d shells (19 for s, p, d)
 #define FB_NME3_WIDE 48   // 3c matrix elements likewise (45 for s, p, d)
 
-// some species has more than FB_MO orbitals: the generic kernels take their
-// FB_MOD sizes, and only the H+S assemblers accept the species
+// some species has more than FB_MO orbitals: every assembler then runs its
+// generic kernels at their FB_MOD sizes
 inline bool fb_wide_species(const DSpecies &sp) {
   for (int i = 0; i < sp.ns; i++)
     if (sp.norb[i] > FB_MO) return true;
