@@ -52,6 +52,12 @@ MATS = ("S", "T", "DIP", "VNA", "VNAO")
 LAUNCH = {0: "2c and 3c launches on two streams", 3: "one launch, 3 waves/SIMD", 4: "one launch, 4 waves/SIMD"}
 
 
+def launch_name(asm):
+    """The H+S launch shape the autotuner chose (FB_TUNE_FUSED, FB_TUNE_PERSIST)."""
+    fused = asm.tuning(4)
+    return LAUNCH[fused] + (", 3c waves on a tile queue" if fused and asm.tuning(5) else "")
+
+
 def algorithmic_bytes_3c(asm, nstruct):
     """Algorithmic HBM bytes of one 3c launch: 24 B per triplet (six int32:
     ialpha, iatom, jatom, the two cells and the target pair) plus one binary64
@@ -126,7 +132,7 @@ def batch_leg(steps, warmup, config=3):
                 "natoms": int(sum(s.natoms for s in structs)), "npairs": npairs, "ntriplets": ntrip,
                 "ms_per_step": ms, "structures_per_s": len(structs) / (ms * 1e-3),
                 "triplets_per_s": ntrip / (ms * 1e-3), "chunk3c": chunk,
-                "kernel_2c": ("LDS-staged", "register")[asm.tuning(1)], "launch": LAUNCH[asm.tuning(4)]}
+                "kernel_2c": ("LDS-staged", "register")[asm.tuning(1)], "launch": launch_name(asm)}
     finally:
         asm.close()
 
@@ -591,7 +597,7 @@ def main():
         asm.assemble_all()
         chunk = asm.autotune(5)
         print(f"# autotuned 3c chunk: {chunk} triplets per workgroup; 2c kernel "
-              f"{('LDS-staged', 'register')[asm.tuning(1)]}; {LAUNCH[asm.tuning(4)]}", file=sys.stderr, flush=True)
+              f"{('LDS-staged', 'register')[asm.tuning(1)]}; {launch_name(asm)}", file=sys.stderr, flush=True)
     # per-kernel times from separate 2c and 3c launches (the one-launch form
     # has no 3c kernel of its own to put on the roofline); the build time is
     # that of the tuned launch shape
@@ -711,7 +717,7 @@ def main():
                    "nstructures": len(structs), "npairs": npairs, "ntriplets": ntrip,
                    "precision": args.precision, "h_s_build_ms": ms, "lists_ms": lists_ms,
                    "lists_rebuild_ms": lists_rebuild_ms,
-                   "chunk3c": chunk, "launch": LAUNCH[asm.tuning(4)] if chunk else None,
+                   "chunk3c": chunk, "launch": launch_name(asm) if chunk else None,
                    "ms_per_step_with_kernel_events": ms_events,
                    "kernels_ms": kern, **extra, "mlp_3c": mlp, "batch": batch, "forces": forces, "rho_mcweda": rho, "pseudopotential": pp,
                    **({"structures_per_s": len(structs) / (ms * 1e-3)} if len(structs) > 1 else {})},

@@ -340,15 +340,19 @@ int fb_mlp_gradients_noalloc(fb_ctx *ctx, int nme, const float *params, int n, c
  * launches (2c, then 3c, the 2c one on a second stream), 3 or 4 as one
  * launch whose first blocks are the 2c atom groups and the rest the
  * 64-triplet 3c tiles, four per workgroup (register allocation for 3 or 4
- * waves per SIMD);
+ * waves per SIMD).  FB_TUNE_PERSIST: 0 (default) gives that launch one
+ * workgroup per four tiles; 1 makes its 3c part persistent, waves x CUs
+ * workgroups whose wavefronts take the tiles one at a time from per-XCD
+ * queues (same blocks, same bits);
  * charged, binary64 and learned-surrogate builds always take separate
  * launches.
  * fb_autotune times whole H+S builds (nrep each, HIP events) over every
  * launch shape that applies to the context -- the fused launch at both
- * occupancies, and the separate launches with every chunk and both 2c
+ * occupancies, with and without the queue, and the separate launches with every chunk and both 2c
  * kernels -- and keeps the fastest; it returns the chosen chunk in *chosen.
  * The blocks hold a complete build afterwards. */
-enum fb_tune { FB_TUNE_CHUNK3C = 0, FB_TUNE_2C_KERNEL = 1, FB_TUNE_GRAPH = 2, FB_TUNE_FORCE_KERNEL = 3, FB_TUNE_FUSED = 4 };
+enum fb_tune { FB_TUNE_CHUNK3C = 0, FB_TUNE_2C_KERNEL = 1, FB_TUNE_GRAPH = 2, FB_TUNE_FORCE_KERNEL = 3, FB_TUNE_FUSED = 4,
+              FB_TUNE_PERSIST = 5 };
 int fb_set_tuning(fb_ctx *ctx, int param, int value);
 int fb_get_tuning(fb_ctx *ctx, int param, int *value);
 int fb_autotune(fb_ctx *ctx, int nrep, int *chosen);

@@ -166,10 +166,12 @@ def test_full_size_fp64_digest(idx):
         print(f"{m}: values identical, bytes {'identical' if g['sha256'] == rec['sha256'] else 'differ in zero signs'}")
 
 
+@pytest.mark.parametrize("persist", [0, 1])
 @pytest.mark.parametrize("waves", [3, 4])
-def test_fused_build_matches_separate_launches(waves):
-    """One persistent launch for the whole neutral mixed-precision H+S build
-    (2c atom groups, then 3c tiles, from a queue) gives the same bits as the
+def test_fused_build_matches_separate_launches(waves, persist):
+    """One launch for the whole neutral mixed-precision H+S build (2c atom
+    groups, then 3c tiles: four per workgroup, or taken one per wavefront
+    from per-XCD queues by persistent workgroups) gives the same bits as the
     separate 2c and 3c launches, on a batch with a structure without
     triplets, and from a replayed graph as well as eagerly."""
     from fireball_hip import synth
@@ -180,9 +182,10 @@ def test_fused_build_matches_separate_launches(waves):
     asm = _gpu(ts, structs, precision="mixed")
     ref = [{m: asm.blocks(m, k) for m in MATS} for k in range(len(structs))]
     asm.set_tuning(4, waves)
+    asm.set_tuning(5, persist)
     for graph in (1, 0):
         asm.set_tuning(2, graph)
-        print(f"fused waves={waves} graph={graph}", flush=True)
+        print(f"fused waves={waves} persist={persist} graph={graph}", flush=True)
         for _ in range(2):
             asm.assemble_all()
             for k in range(len(structs)):

@@ -714,9 +714,10 @@ __global__ __launch_bounds__(CH, 4) void k_3c_mixed(DView v) {
 }
 
 // Next tile of a persistent workgroup from per-XCD ranges of the 256-triplet
-// tiles: queue[x] counts the tiles taken from the range of XCD x (numbered
+// tiles: queue[x * FB_QSTRIDE] counts the tiles taken from the range of XCD x (numbered
 // contiguously per XCD, as xcd_tile numbers them); the workgroup's own XCD
-// (b mod 8) first, then the others; -1 when all are empty.  Thread 0 only.
+// (b mod 8) first, then the others; -1 when all are empty.  One lane per
+// taker (workgroup or wavefront).
 __device__ __forceinline__ int next_tile(int *queue, unsigned ntiles) {
   const unsigned per = ntiles >> 3, rem = ntiles & 7u, x0 = blockIdx.x & 7u;
   for (unsigned k = 0; k < 8; k++) {
@@ -724,7 +725,7 @@ __device__ __forceinline__ int next_tile(int *queue, unsigned ntiles) {
     const unsigned lo = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
     const unsigned cnt = per + (x < rem ? 1u : 0u);
     if (cnt == 0) continue;
-    const unsigned i = (unsigned)atomicAdd(&queue[x], 1);
+    const unsigned i = (unsigned)atomicAdd(&queue[x * FB_QSTRIDE], 1);
     if (i < cnt) return (int)(lo + i);
   }
   return -1;
@@ -793,6 +794,33 @@ __global__ __launch_bounds__(256, W) void k_hs_fused(DView v, int n2p) {
   if (status) atomicOr(v.status, status);
 }
 
+// Persistent form of k_hs_fused: the same 2c blocks first, then a fixed
+// number of blocks whose wavefronts each take 64-triplet tiles from the
+// per-XCD queue (next_tile over v.tiles[0]) until every range is empty.
+// Every wavefront's loop ends: each pass either takes an unused tile or
+// finds all eight ranges exhausted.
+template <int W>
+__global__ __launch_bounds__(256, W) void k_hs_persist(DView v, int n2p, int *queue) {
+  __shared__ SFused S;
+  const int b = blockIdx.x;
+  if (b < n2p) {
+    if (b < v.nwg2c) wg2c_sp<0>(v, 15 | FB_2C_SKIP_TGT, b, S.s2);
+    return;
+  }
+  const int wv = threadIdx.x >> 6;
+  const unsigned nt = (unsigned)v.ntiles[0];
+  int status = 0;
+  for (;;) {
+    int t = 0;
+    if ((threadIdx.x & 63) == 0) t = next_tile(queue, nt);
+    t = __shfl(t, 0);
+    if (t < 0) break;
+    tile3c<64, false, false>(v, v.tiles[0][t], S.s3[wv], nullptr, status);
+    tile_sync<64>();   // the next tile reuses the wavefront's LDS rows
+  }
+  if (status) atomicOr(v.status, status);
+}
+
 template <bool CHARGED>
 static int launch_mixed(const DView &v, int chunk, hipStream_t st) {
   const int kt = chunk == 64 ? 0 : (chunk == 256 ? 2 : 1);
@@ -810,11 +838,11 @@ static int launch_mixed(const DView &v, int chunk, hipStream_t st) {
 
 extern "C" int fbk_fill_i32(int *a, int64_t n, int val, hipStream_t st);
 
-// queue: 8 ints of device memory, cleared here by a fill kernel; nwg:
+// queue: FB_QUEUE_INTS ints of device memory, cleared here by a fill kernel; nwg:
 // persistent workgroups (four per CU)
 extern "C" int fbk_3c_mlp(const DView &v, int *queue, int nwg, hipStream_t st) {
   if (v.ntrip == 0 || v.ntiles[2] == 0) return 0;
-  int rc = fbk_fill_i32(queue, 8, 0, st);
+  int rc = fbk_fill_i32(queue, FB_QUEUE_INTS, 0, st);
   if (rc) return rc;
   const dim3 grid((unsigned)std::max(1, std::min(nwg, v.ntiles[2])));
   hipLaunchKernelGGL(k_3c_mlp, grid, dim3(256), 0, st, v, queue);
@@ -826,10 +854,23 @@ extern "C" int fbk_3c_mixed(const DView &v, int chunk, int charged, hipStream_t 
   return charged ? launch_mixed<true>(v, chunk, st) : launch_mixed<false>(v, chunk, st);
 }
 
-// waves: 3 or 4 per SIMD (the register allocation of k_hs_fused)
-extern "C" int fbk_hs_fused(const DView &v, int waves, hipStream_t st) {
+// waves: 3 or 4 per SIMD (the register allocation of k_hs_fused); queue:
+// null for one block per four tiles, else FB_QUEUE_INTS ints of device memory (cleared
+// here by a fill kernel) for the persistent form with waves x ncu 3c blocks
+extern "C" int fbk_hs_fused(const DView &v, int waves, int *queue, int ncu, hipStream_t st) {
   if (v.nwg2c == 0 && v.ntiles[0] == 0) return 0;
   const int n2p = (v.nwg2c + 7) / 8 * 8;
+  if (queue) {
+    int rc = fbk_fill_i32(queue, FB_QUEUE_INTS, 0, st);
+    if (rc) return rc;
+    const int nq = std::max(1, std::min(waves * ncu, (v.ntiles[0] + 3) / 4));
+    const dim3 grid((unsigned)(n2p + nq));
+    if (waves == 3)
+      hipLaunchKernelGGL(k_hs_persist<3>, grid, dim3(256), 0, st, v, n2p, queue);
+    else
+      hipLaunchKernelGGL(k_hs_persist<4>, grid, dim3(256), 0, st, v, n2p, queue);
+    return (int)hipGetLastError();
+  }
   const dim3 grid((unsigned)(n2p + (v.ntiles[0] + 3) / 4));
   if (waves == 3)
     hipLaunchKernelGGL(k_hs_fused<3>, grid, dim3(256), 0, st, v, n2p);

@@ -62,7 +62,7 @@ int fbk_3c_fp64_ph(const DView &v, int charged, hipStream_t st);
 int fbk_gamma_fold(const DView &v, int a0, int natoms, int which, const int *orb_off, int n, double *A,
                    hipStream_t st);
 int fbk_3c_mlp(const DView &v, int *queue, int nwg, hipStream_t st);
-int fbk_hs_fused(const DView &v, int waves, hipStream_t st);
+int fbk_hs_fused(const DView &v, int waves, int *queue, int ncu, hipStream_t st);
 int fbk_mlp_samples(const DView &v, int gid, int64_t n, unsigned long long seed, float *X, float *T, hipStream_t st);
 int fbk_mlp_colmaxabs(int64_t n, int ncol, const float *A, float *out, hipStream_t st);
 int fbk_mlp_rowmax(int64_t n, const float *T, float *M, hipStream_t st);
@@ -209,6 +209,7 @@ struct fb_ctx {
   float *scratch = nullptr;   // this device's scratch array (fb_mlp_gradients_noalloc)
   int ncu = 256;              // compute units of the device
   int fused = 0;              // FB_TUNE_FUSED: 0 separate 2c/3c launches, 3 or 4 one launch (waves per SIMD)
+  int persist = 0;            // FB_TUNE_PERSIST: the one launch's 3c waves take tiles from a queue
   hipStream_t own_stream = nullptr, stream = nullptr;
   hipStream_t side = nullptr;          // second stream: 2c beside 3c in a build
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -1751,7 +1752,7 @@ static int enqueue_build(fb_ctx *ctx, int charged) {
   if (fused_build(ctx, charged)) {
     DView v = ctx->view();
     RC(ctx->launch("assemble_hs_fused", [&] {
-      return fbk_hs_fused(v, ctx->fused, ctx->stream);
+      return fbk_hs_fused(v, ctx->fused, ctx->persist ? ctx->d_queue.as<int>() : nullptr, ctx->ncu, ctx->stream);
     }));
     return FB_OK;
   }
@@ -2221,7 +2222,7 @@ int fb_mlp_train(fb_ctx *ctx, int64_t nsamples, int nsteps, int batch, double lr
   RC(upload(ctx, ctx->d_mlp, ctx->h_mlp.data(), sizeof(DMlp) * std::max(ng, 1), "upload networks"));
   RC(upload(ctx, ctx->d_mlp_err, errs.data(), sizeof(float) * errs.size(), "upload calibration"));
   FB_HIP(ctx->d_mlp_count.ensure(2 * sizeof(unsigned long long)));
-  FB_HIP(ctx->d_queue.ensure(8 * sizeof(int)));
+  FB_HIP(ctx->d_queue.ensure(FB_QUEUE_INTS * sizeof(int)));
 
   FB_HIP(hipMemsetAsync(ctx->d_mlp_count.p, 0, 2 * sizeof(unsigned long long), ctx->stream));
   FB_HIP(hipStreamSynchronize(ctx->stream));
@@ -3054,6 +3055,14 @@ int fb_set_tuning(fb_ctx *ctx, int param, int value) {
     ctx->fused = value;
     return FB_OK;
   }
+  if (param == FB_TUNE_PERSIST) {
+    if (value != 0 && value != 1) return ctx->fail(FB_E_ARG, "persist must be 0 or 1");
+    FB_HIP(hipSetDevice(ctx->device));
+    if (value) FB_HIP(ctx->d_queue.ensure(FB_QUEUE_INTS * sizeof(int)));
+    ctx->persist = value;
+    ctx->mlp_gen++;   // captured graphs hold the old launch
+    return FB_OK;
+  }
   return ctx->fail(FB_E_ARG, "unknown tuning parameter");
 }
 
@@ -3077,6 +3086,10 @@ int fb_get_tuning(fb_ctx *ctx, int param, int *value) {
   }
   if (param == FB_TUNE_FUSED) {
     *value = ctx->fused;
+    return FB_OK;
+  }
+  if (param == FB_TUNE_PERSIST) {
+    *value = ctx->persist;
     return FB_OK;
   }
   return ctx->fail(FB_E_ARG, "unknown tuning parameter");
@@ -3110,21 +3123,24 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   const bool timing = ctx->timing;
   ctx->timing = false;   // enqueue_build then takes the forked two-stream form
   struct Cfg {
-    int chunk, k2c, fused;
+    int chunk, k2c, fused, persist;
   };
   std::vector<Cfg> cand;
   const bool mixed_sp = ctx->canon_sp && ctx->precision == FB_PREC_MIXED;
   const bool mlp = mixed_sp && ctx->method3c == FB_3C_MLP && ctx->mlp_ready && !charged;
-  if (mixed_sp && !charged && !mlp)
-    for (int w : {3, 4}) cand.push_back({ctx->chunk3c, ctx->k2c, w});
+  if (mixed_sp && !charged && !mlp) {
+    FB_HIP(ctx->d_queue.ensure(FB_QUEUE_INTS * sizeof(int)));
+    for (int w : {3, 4})
+      for (int q : {0, 1}) cand.push_back({ctx->chunk3c, ctx->k2c, w, q});
+  }
   for (int c : {64, 128, 256}) {
     if (!mixed_sp && c != ctx->chunk3c) continue;   // the chunk only shapes the mixed s/p 3c kernel
     for (int k : {0, 1}) {
       if (!ctx->canon_sp && k != ctx->k2c) continue;   // the s/p 2c kernels
-      cand.push_back({c, k, 0});
+      cand.push_back({c, k, 0, 0});
     }
   }
-  Cfg best{ctx->chunk3c, ctx->k2c, ctx->fused};
+  Cfg best{ctx->chunk3c, ctx->k2c, ctx->fused, ctx->persist};
   float tbest = 1e30f;
   hipError_t e = hipSuccess;
   int rc = FB_OK;
@@ -3132,6 +3148,7 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
     ctx->chunk3c = c.chunk;
     ctx->k2c = c.k2c;
     ctx->fused = c.fused;
+    ctx->persist = c.persist;
     rc = enqueue_build(ctx, charged);   // warm-up
     if (rc == FB_OK) e = hipEventRecord(ev.a, ctx->stream);
     for (int r = 0; r < nrep && rc == FB_OK && e == hipSuccess; r++) rc = enqueue_build(ctx, charged);
@@ -3148,6 +3165,7 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   ctx->chunk3c = best.chunk;
   ctx->k2c = best.k2c;
   ctx->fused = best.fused;
+  ctx->persist = best.persist;
   ctx->mlp_gen++;   // captured graphs hold the previous launch shapes
   if (rc == FB_OK && e == hipSuccess) rc = enqueue_build(ctx, charged);
   ctx->timing = timing;

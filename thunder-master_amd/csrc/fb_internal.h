@@ -48,6 +48,11 @@ struct DSpecies {
 #define FB_NME2_WIDE 25   // 2c matrix elements the generic kernels hold with d shells (19 for s, p, d)
 #define FB_NME3_WIDE 48   // 3c matrix elements likewise (45 for s, p, d)
 
+// tile queues of the persistent 3c launches: one counter per XCD, each in a
+// 256-byte line of its own (counters sharing a line serialise their atomics)
+#define FB_QSTRIDE 64
+#define FB_QUEUE_INTS (8 * FB_QSTRIDE)
+
 // some species has more than FB_MO orbitals: every assembler then runs its
 // generic kernels at their FB_MOD sizes
 inline bool fb_wide_species(const DSpecies &sp) {
