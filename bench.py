@@ -137,6 +137,33 @@ def batch_leg(steps, warmup, config=3):
         asm.close()
 
 
+def config_leg(steps, warmup, config=2):
+    """Another single-structure workload (config 2, CHNO3000) beside the main
+    one: the autotuned mixed-precision H+S build and the binary64 build, both
+    replayed from their captured graphs, and the largest block difference
+    between the two."""
+    species, structs = synth.config(config)
+    asm = Assembler(synth.make_tables(species), precision="mixed")
+    try:
+        for st in structs:
+            asm.add_structure(st)
+        npairs, ntrip = asm.build_lists()
+        asm.assemble_all()
+        chunk = asm.autotune(5)
+        ms = time_graph(asm, steps, warmup)
+        mixed = {m: asm.blocks(m) for m in MATS}
+        launch = launch_name(asm)
+        asm.set_precision("fp64")
+        ms64 = time_graph(asm, steps, warmup)
+        err = {m: float(np.max(np.abs(mixed[m] - asm.blocks(m)))) for m in MATS}
+        return {"workload": WORKLOADS[config], "natoms": int(sum(s.natoms for s in structs)),
+                "npairs": npairs, "ntriplets": ntrip, "ms_per_step": ms, "triplets_per_s": ntrip / (ms * 1e-3),
+                "chunk3c": chunk, "launch": launch, "fp64_ms_per_step": ms64, "speedup_vs_fp64": ms64 / ms,
+                "max_abs_err_vs_fp64_eV": err}
+    finally:
+        asm.close()
+
+
 def gamma_eigen_shift(asm, precision_a, precision_b, max_norb=20000):
     """Eigenvalue shift of the Gamma-point folded H = T + Vna (block + blocko)
     of structure 0 between two precisions, from the library's own fold and
@@ -566,7 +593,7 @@ def main():
     ap.add_argument("--no-forces", action="store_true", help="skip the force-assembler leg")
     ap.add_argument("--no-rho", action="store_true", help="skip the McWEDA density leg")
     ap.add_argument("--no-pp", action="store_true", help="skip the pseudopotential leg")
-    ap.add_argument("--no-batch", action="store_true", help="skip the 512-structure batch leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-structure batch and CHNO3000 legs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one elementwise copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -642,6 +669,16 @@ def main():
                   file=sys.stderr, flush=True)
         except Exception as e:  # noqa: BLE001 -- reported, not fatal to the main metric
             mlp = {"error": str(e)}
+
+    chno = None
+    if not args.no_batch and args.config != 2:
+        try:
+            chno = config_leg(args.steps, args.warmup)
+            print(f"# CHNO3000: {chno['ms_per_step']:.3f} ms per build ({chno['launch']}, chunk {chno['chunk3c']}), "
+                  f"binary64 {chno['fp64_ms_per_step']:.3f} ms, VNA err {chno['max_abs_err_vs_fp64_eV']['VNA']:.2e} eV",
+                  file=sys.stderr, flush=True)
+        except Exception as e:  # noqa: BLE001 -- reported, not fatal to the main metric
+            chno = {"error": str(e)}
 
     batch = None
     if not args.no_batch and args.config != 3:
@@ -719,7 +756,7 @@ def main():
                    "lists_rebuild_ms": lists_rebuild_ms,
                    "chunk3c": chunk, "launch": launch_name(asm) if chunk else None,
                    "ms_per_step_with_kernel_events": ms_events,
-                   "kernels_ms": kern, **extra, "mlp_3c": mlp, "batch": batch, "forces": forces, "rho_mcweda": rho, "pseudopotential": pp,
+                   "kernels_ms": kern, **extra, "mlp_3c": mlp, "chno3000": chno, "batch": batch, "forces": forces, "rho_mcweda": rho, "pseudopotential": pp,
                    **({"structures_per_s": len(structs) / (ms * 1e-3)} if len(structs) > 1 else {})},
         "roofline": roof,
         "cpu_baseline": cpu,
