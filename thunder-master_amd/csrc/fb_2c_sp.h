@@ -48,6 +48,7 @@ __device__ __forceinline__ int block2c(const DView &v, int sp2, int slot, int is
 struct WStore {
   bool uni;        // every active lane of the wavefront holds an sp-sp pair
   int lane, nact;  // lane, active lanes (a prefix of the wavefront)
+  int row;         // LDS row of the lane's pair: its index within the wavefront's group
   int64_t wb0;     // block offset of the wavefront's first pair
   double *ws;      // the wavefront's 64 LDS rows
 };
@@ -61,7 +62,7 @@ __device__ __forceinline__ void wave_sync() {
 // the block already in the lane's LDS row
 template <int NBL>
 __device__ __forceinline__ void put_row(double *m, int64_t bo, const WStore &W) {
-  const double *row = W.ws + W.lane * 17;
+  const double *row = W.ws + W.row * 17;
   if (NBL == 16 && W.uni) {
     wave_sync();
     double *dst = m + W.wb0;
@@ -80,7 +81,7 @@ __device__ __forceinline__ void put_row(double *m, int64_t bo, const WStore &W) 
 template <int NBL>
 __device__ __forceinline__ void put_block(double *m, int64_t bo, const double *X, const WStore &W) {
   if (NBL == 16 && W.uni) {
-    double *row = W.ws + W.lane * 17;
+    double *row = W.ws + W.row * 17;
 #pragma unroll
     for (int i = 0; i < 16; i++) row[i] = X[i];
     wave_sync();
@@ -123,8 +124,8 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
     put_block<NBL>(v.mat[FB_MAT_DIP], bo, X, W);
   }
   if (mask & 8) {
-    // the ontop sum accumulates in the lane's LDS row
-    double *acc = W.ws + W.lane * 17;
+    // the ontop sum accumulates in the pair's LDS row
+    double *acc = W.ws + W.row * 17;
 #pragma unroll
     for (int i = 0; i < NBL; i++) acc[i] = 0.0;
     if (!self) {
@@ -150,14 +151,15 @@ __device__ __forceinline__ int pair2c(const DView &v, int mask, int ia, int ja, 
         for (int i = 0; i < NBL; i++) acc[i] = acc[i] + dQ * X[i] * FB_P_EQ2;
       }
     }
-    put_row<NBL>(v.mat[FB_MAT_VNAO], bo, W);
     const bool clr = !self && !keep_vna;
+    acc[16] = clr ? 1.0 : 0.0;   // the row's pad element carries the pair's flag
+    put_row<NBL>(v.mat[FB_MAT_VNAO], bo, W);
     if (NBL == 16 && W.uni) {
       double *dst = v.mat[FB_MAT_VNA] + W.wb0;
 #pragma unroll
       for (int k = 0; k < 16; k++) {
         const int o = k * W.nact + W.lane;
-        if (__shfl((int)clr, o >> 4)) dst[o] = 0.0;
+        if (W.ws[(o >> 4) * 17 + 16] != 0.0) dst[o] = 0.0;
       }
     } else if (clr) {
 #pragma unroll
@@ -217,7 +219,12 @@ __device__ __forceinline__ void wg2c_sp(const DView &v, int mask, int item, S2c<
   W.lane = tid & 63;
   W.ws = &s_at[0][0][0] + __builtin_amdgcn_readfirstlane(tid & ~63) * 17;
   for (int64_t c0 = pb; c0 < pe; c0 += WG2) {
-    const int64_t p = c0 + tid;
+    // the lane at slot c0 + tid takes pair p of its wavefront's group (in
+    // the order of v.pair_perm); blocks, LDS rows and sums go by p
+    const int64_t slot = c0 + tid;
+    const int64_t p0 = c0 + __builtin_amdgcn_readfirstlane(tid & ~63);
+    const int64_t p = (slot < pe && v.pair_perm) ? p0 + v.pair_perm[slot] : slot;
+    const int srow = (int)(p - c0);
     double atom_c[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) atom_c[i] = 0.0;
@@ -225,7 +232,7 @@ __device__ __forceinline__ void wg2c_sp(const DView &v, int mask, int item, S2c<
       for (int q = 1; q <= KQ; q++)
 #pragma unroll
         for (int i = 0; i < 16; i++) s_at[q][tid][i] = 0.0;
-    if (p < pe) {
+    if (slot < pe) {
       const int4 pr = v.pair_rec[p];
       const int ia = pr.x, ja = pr.y;
       const int in1 = pr.w & 0xffff, in2 = pr.w >> 16;
@@ -248,14 +255,14 @@ __device__ __forceinline__ void wg2c_sp(const DView &v, int mask, int item, S2c<
       const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
       const int64_t bo = v.boff[p];
       // the active lanes are a prefix of the wavefront
-      const int64_t p0 = c0 + __builtin_amdgcn_readfirstlane(tid & ~63);
       W.nact = (int)min((int64_t)64, pe - p0);
+      W.row = (int)(p - p0);
       W.uni = __ballot(cls != 3) == 0;
       const int64_t wb0 = v.boff[p0];
       W.wb0 = ((int64_t)__builtin_amdgcn_readfirstlane((int)(wb0 >> 32)) << 32) |
               (uint32_t)__builtin_amdgcn_readfirstlane((int)wb0);
       const bool keep = (mask & FB_2C_SKIP_TGT) && v.pair_tgt[p];
-      double *qrow = KQ > 0 ? &s_at[KQ > 0 ? 1 : 0][tid][0] : nullptr;
+      double *qrow = KQ > 0 ? &s_at[KQ > 0 ? 1 : 0][srow][0] : nullptr;
       switch (cls) {
         case 3: status |= pair2c<3, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow, W); break;
         case 2: status |= pair2c<2, KQ>(v, mask, ia, ja, in1, in2, self, keep, z, pm, bo, atom_c, qrow, W); break;
@@ -265,7 +272,7 @@ __device__ __forceinline__ void wg2c_sp(const DView &v, int mask, int item, S2c<
     }
     if (mask & 8) {
 #pragma unroll
-      for (int i = 0; i < 16; i++) s_at[0][tid][i] = atom_c[i];
+      for (int i = 0; i < 16; i++) s_at[0][srow][i] = atom_c[i];
       __syncthreads();
       const int rna = rown ? v.sp.norb[v.imass[ra] - 1] : 0;
       if (rown && re < rna * rna) {

@@ -41,6 +41,7 @@ int fbk_neigh_back(const DView &v, int *neigh_back, int *status, hipStream_t st)
 int fbk_common_count(const DView &v, int *ncommon, int *status, hipStream_t st);
 int fbk_common_fill(const DView &v, const int64_t *toff, int4 *trip, int *trip_pair, int *status, hipStream_t st);
 int fbk_block_sizes(const DView &v, int64_t *sz, int4 *rec, hipStream_t st);
+int fbk_pair_perm(const DView &v, unsigned char *perm, hipStream_t st);
 int fbk_scan_i32(const int *in, int64_t *out, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t st);
 int fbk_scan_i64(const int64_t *in, int64_t *out, int64_t n, void *tmp, size_t *tmp_bytes, hipStream_t st);
 int fbk_sort_pairs(const int *keys_in, int *keys_out, const int *vals_in, int *vals_out, int64_t n, int end_bit,
@@ -248,7 +249,7 @@ struct fb_ctx {
   int natoms = 0;
   DevBuf d_st, d_xl, d_ratom, d_imass, d_atom_struct, d_dQ, d_qin;
 
-  DevBuf d_neighn, d_noff, d_neigh_j, d_neigh_b, d_neigh_back, d_neigh_self, d_pair_atom, d_pair_rec, d_bsz, d_boff;
+  DevBuf d_neighn, d_noff, d_neigh_j, d_neigh_b, d_neigh_back, d_neigh_self, d_pair_atom, d_pair_rec, d_pair_perm, d_bsz, d_boff;
   DevBuf d_ncommon, d_toff, d_trip_orig, d_tpair_orig, d_trip, d_tpair, d_perm_in, d_perm, d_flag, d_segincl,
       d_seg_off, d_seg_pair, d_trip_seg, d_pair_tgt, d_trip_aux, d_tiles[3];
   int ntiles[3] = {0, 0, 0};
@@ -443,6 +444,7 @@ struct fb_ctx {
     v.pair_atom = d_pair_atom.as<int>();
     v.boff = d_boff.as<int64_t>();
     v.pair_rec = d_pair_rec.as<int4>();
+    v.pair_perm = lists_ok ? d_pair_perm.as<unsigned char>() : nullptr;
     v.wboff = d_wboff.as<int64_t>();
     v.ntrip = ntrip;
     v.nseg = nseg;
@@ -557,7 +559,7 @@ struct fb_ctx {
   ~fb_ctx() {
     DevBuf *bufs[] = {&d_cell2c, &d_idx2c, &d_grp3c, &d_idx3c, &d_tab2c, &d_tab2c32, &d_tab3c, &d_tab3c32, &d_tab3c32q,
                       &d_mu, &d_nu, &d_mv, &d_grid2c, &d_pden, &d_wg2c, &d_cell_cnt, &d_cell_start, &d_cell_atoms, &d_atom_cell, &d_pair_disp, &d_tile_cnt, &d_tile_off, &d_gidx, &d_gout, &d_st, &d_xl, &d_ratom, &d_imass, &d_atom_struct, &d_dQ,
-                      &d_neighn, &d_noff, &d_neigh_j, &d_neigh_b, &d_neigh_back, &d_neigh_self, &d_pair_atom, &d_pair_rec,
+                      &d_neighn, &d_noff, &d_neigh_j, &d_neigh_b, &d_neigh_back, &d_neigh_self, &d_pair_atom, &d_pair_rec, &d_pair_perm,
                       &d_bsz, &d_boff, &d_ncommon, &d_toff, &d_trip_orig, &d_tpair_orig, &d_trip, &d_tpair,
                       &d_perm_in, &d_perm, &d_flag, &d_segincl, &d_seg_off, &d_seg_pair, &d_trip_seg, &d_pair_tgt, &d_trip_aux, &d_tiles[0], &d_tiles[1], &d_tiles[2], &d_tmp,
                       &d_mlp, &d_mlp_err, &d_mlp_count, &d_queue, &d_meth3c,
@@ -1437,6 +1439,7 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
   FB_HIP(ctx->d_neigh_back.ensure(sizeof(int) * P));
   FB_HIP(ctx->d_pair_atom.ensure(sizeof(int) * P));
   FB_HIP(ctx->d_pair_rec.ensure(sizeof(int4) * std::max<int64_t>(P, 1)));
+  FB_HIP(ctx->d_pair_perm.ensure(std::max<int64_t>(P, 1)));
   FB_HIP(ctx->d_bsz.ensure(sizeof(int64_t) * P));
   FB_HIP(ctx->d_boff.ensure(sizeof(int64_t) * (P + 1)));
   v = ctx->view();
@@ -1574,6 +1577,8 @@ int fb_build_lists(fb_ctx *ctx, int64_t *npairs_total, int64_t *ntrip_total) {
     wg.push_back(N);
     ctx->nwg2c = (int)wg.size() - 1;
     RC(upload(ctx, ctx->d_wg2c, wg.data(), wg.size() * sizeof(int), "upload wg2c"));
+    v = ctx->view();
+    RC(ctx->launch("pair_perm", [&] { return fbk_pair_perm(v, ctx->d_pair_perm.as<unsigned char>(), st); }));
     FB_HIP(hipStreamSynchronize(st));
   }
   ctx->lists_ok = true;

@@ -148,6 +148,7 @@ struct DView {
   const int *pair_atom;         // [npairs] global iatom (0-based)
   const int4 *pair_rec;         // [npairs] global iatom, global jatom (0-based), structure * FB_NIMG + mbeta,
                                 //          imass_i | imass_j << 16: the pair in one load
+  const unsigned char *pair_perm;   // [npairs] lane order of the 2c workgroups (k_pair_perm), or null
   const int64_t *boff;          // [npairs+1] block offsets (doubles)
   const int64_t *wboff;         // [npairs+1] shell-block offsets (nssh_mu x nssh_nu per pair)
   const int *wg2c;              // [nwg2c+1] first atom of each 2c workgroup

@@ -416,6 +416,54 @@ int fbk_common_fill(const DView &v, const int64_t *toff, int4 *trip, int *trip_p
   return (int)hipGetLastError();
 }
 
+// Lane order of the 2c workgroups (fb_2c_sp.h): within each 64-pair
+// wavefront group of a workgroup item (the groups that pass c0 of item b
+// forms: c0 + 64 w), the pairs by species pair, then bond length, then index,
+// so that neighbouring lanes interpolate nearby rows of the same table.  perm[slot] is the
+// group-relative index of the pair that the lane at slot takes.  Blocks and
+// sums are placed by pair index, so any permutation gives the same bits.
+__global__ __launch_bounds__(FB_2C_PCAP) void k_pair_perm(DView v, unsigned char *perm) {
+  __shared__ double s_x[FB_2C_PCAP];
+  __shared__ int s_k[FB_2C_PCAP];
+  const int tid = threadIdx.x;
+  const int64_t pb = v.noff[v.wg2c[blockIdx.x]], pe = v.noff[v.wg2c[blockIdx.x + 1]];
+  for (int64_t c0 = pb; c0 < pe; c0 += FB_2C_PCAP) {
+    const int64_t slot = c0 + tid;
+    double x = 0.0;
+    int key = 0;
+    if (slot < pe) {
+      const int4 pr = v.pair_rec[slot];
+      key = pr.w;
+      const double *xlp = v.xl + (int64_t)pr.z * 3;
+      for (int q = 0; q < 3; q++) {
+        const double d = v.ratom[3 * pr.y + q] + xlp[q] - v.ratom[3 * pr.x + q];
+        x = x + d * d;
+      }
+    }
+    __syncthreads();   // the previous pass has read s_x
+    s_x[tid] = x;
+    s_k[tid] = key;
+    __syncthreads();
+    if (slot < pe) {
+      const int g0 = tid & ~63;
+      const int n = (int)min((int64_t)64, pe - (c0 + g0));
+      int rank = 0;
+      for (int j = 0; j < n; j++) {
+        const double xj = s_x[g0 + j];
+        const int kj = s_k[g0 + j];
+        rank += (kj < key || (kj == key && (xj < x || (xj == x && g0 + j < tid)))) ? 1 : 0;
+      }
+      perm[c0 + g0 + rank] = (unsigned char)(tid & 63);
+    }
+  }
+}
+
+int fbk_pair_perm(const DView &v, unsigned char *perm, hipStream_t st) {
+  if (v.nwg2c == 0) return 0;
+  hipLaunchKernelGGL(k_pair_perm, dim3((unsigned)v.nwg2c), dim3(FB_2C_PCAP), 0, st, v, perm);
+  return (int)hipGetLastError();
+}
+
 int fbk_block_sizes(const DView &v, int64_t *sz, int4 *rec, hipStream_t st) {
   if (v.npairs == 0) return 0;
   hipLaunchKernelGGL(k_block_sizes, dim3((unsigned)((v.npairs + 255) / 256)), dim3(256), 0, st, v, sz, rec);
