@@ -126,7 +126,7 @@ def batch_leg(steps, warmup, config=3):
             asm.add_structure(st)
         npairs, ntrip = asm.build_lists()
         asm.assemble_all()
-        chunk = asm.autotune(5)
+        chunk = asm.autotune(20)
         ms = time_graph(asm, steps, warmup)
         return {"workload": WORKLOADS[config], "nstructures": len(structs),
                 "natoms": int(sum(s.natoms for s in structs)), "npairs": npairs, "ntriplets": ntrip,
@@ -149,7 +149,7 @@ def config_leg(steps, warmup, config=2):
             asm.add_structure(st)
         npairs, ntrip = asm.build_lists()
         asm.assemble_all()
-        chunk = asm.autotune(5)
+        chunk = asm.autotune(20)
         ms = time_graph(asm, steps, warmup)
         mixed = {m: asm.blocks(m) for m in MATS}
         launch = launch_name(asm)
@@ -229,7 +229,7 @@ def mlp_leg(steps, warmup, config=4):
         asm.add_structure(st)
     npairs, ntrip = asm.build_lists()
     asm.assemble_all()
-    asm.autotune(3)
+    asm.autotune(20)
     table_ms = time_graph(asm, steps, warmup)
     table_kernel_ms = _kernel_ms(asm, "assemble_3c_mixed", steps)
     t0 = time.perf_counter()
@@ -622,7 +622,7 @@ def main():
     chunk = None
     if args.precision == "mixed":
         asm.assemble_all()
-        chunk = asm.autotune(5)
+        chunk = asm.autotune(20)
         print(f"# autotuned 3c chunk: {chunk} triplets per workgroup; 2c kernel "
               f"{('LDS-staged', 'register')[asm.tuning(1)]}; {launch_name(asm)}", file=sys.stderr, flush=True)
     # per-kernel times from separate 2c and 3c launches (the one-launch form

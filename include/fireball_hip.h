@@ -346,7 +346,8 @@ int fb_mlp_gradients_noalloc(fb_ctx *ctx, int nme, const float *params, int n, c
  * queues (same blocks, same bits);
  * charged, binary64 and learned-surrogate builds always take separate
  * launches.
- * fb_autotune times whole H+S builds (nrep each, HIP events) over every
+ * fb_autotune times whole H+S builds as fb_assemble_all runs them (graph
+ * replays unless FB_TUNE_GRAPH is 0; nrep each, two sweeps, HIP events) over every
  * launch shape that applies to the context -- the fused launch at both
  * occupancies, with and without the queue, and the separate launches with every chunk and both 2c
  * kernels -- and keeps the fastest; it returns the chosen chunk in *chosen.

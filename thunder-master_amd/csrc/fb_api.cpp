@@ -1777,6 +1777,36 @@ static int enqueue_build(fb_ctx *ctx, int charged) {
   return FB_OK;
 }
 
+// One H+S build as fb_assemble_all runs it: replayed from the captured graph
+// (captured again when its key changes), or enqueued directly with per-kernel
+// timing, with graphs off or on the null stream.
+static int launch_build(fb_ctx *ctx, int charged) {
+  if (ctx->timing || !ctx->use_graph || ctx->stream == nullptr) return enqueue_build(ctx, charged);
+  // the graph holds kernel arguments by value: any change of lists, precision,
+  // charge state, chunk, 3c method, launch shape or networks needs a new capture
+  const long key = ((((((((ctx->lists_gen * 4 + ctx->precision) * 2 + charged) * 512 + ctx->chunk3c) * 2 +
+                         ctx->method3c) * 8 + ctx->fused) * 2 + ctx->persist) * 2 + ctx->k2c) * 4096) +
+                   (ctx->mlp_gen & 4095);
+  if (ctx->graph_key != key || !ctx->gexec) {
+    if (ctx->gexec) { (void)hipGraphExecDestroy(ctx->gexec); ctx->gexec = nullptr; }
+    if (ctx->graph) { (void)hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
+    FB_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_build(ctx, charged);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc != FB_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return ctx->fail(FB_E_HIP, "graph capture", e);
+    ctx->graph = g;
+    FB_HIP(hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
+    ctx->graph_key = key;
+  }
+  FB_HIP(hipGraphLaunch(ctx->gexec, ctx->stream));
+  return FB_OK;
+}
+
 // Asynchronous: returns once the build is enqueued, without waiting for an
 // earlier build.  A status raised by the kernels of the builds enqueued since
 // the last check (x beyond a table, missing table) is returned by the next
@@ -1790,32 +1820,7 @@ int fb_assemble_all(fb_ctx *ctx) {
     RC(ctx->settle("previous assemble_all"));
     RC(upload_structures(ctx));
   }
-  const int charged = any_charge(ctx);
-  if (ctx->timing || !ctx->use_graph || ctx->stream == nullptr) {
-    RC(enqueue_build(ctx, charged));
-  } else {
-    // the graph holds kernel arguments by value: any change of lists, precision,
-    // charge state, chunk, 3c method or networks needs a new capture
-    const long key = ((((((ctx->lists_gen * 4 + ctx->precision) * 2 + charged) * 512 + ctx->chunk3c) * 2 +
-                        ctx->method3c) * 8 + ctx->fused) * 4096) + (ctx->mlp_gen & 4095);
-    if (ctx->graph_key != key || !ctx->gexec) {
-      if (ctx->gexec) { (void)hipGraphExecDestroy(ctx->gexec); ctx->gexec = nullptr; }
-      if (ctx->graph) { (void)hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
-      FB_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_build(ctx, charged);
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-      if (rc != FB_OK) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-      }
-      if (e != hipSuccess) return ctx->fail(FB_E_HIP, "graph capture", e);
-      ctx->graph = g;
-      FB_HIP(hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
-      ctx->graph_key = key;
-    }
-    FB_HIP(hipGraphLaunch(ctx->gexec, ctx->stream));
-  }
+  RC(launch_build(ctx, any_charge(ctx)));
   ctx->pending = true;
   for (int i = 0; i < 5; i++) ctx->done[i] = 1;
   return FB_OK;
@@ -3100,13 +3105,15 @@ int fb_get_tuning(fb_ctx *ctx, int param, int *value) {
   return ctx->fail(FB_E_ARG, "unknown tuning parameter");
 }
 
-// Times whole H+S builds (enqueued directly, not from a graph) over every
+// Times whole H+S builds, run as fb_assemble_all runs them (graph replays
+// unless graphs are off), over every
 // launch shape that applies to the context and keeps the fastest: the
 // one-launch build at 3 and 4 waves per SIMD (neutral mixed-precision s/p
 // builds), and separate 2c and 3c launches with every 3c chunk (mixed s/p
 // builds; it sets the triplets and the LDS footprint of a workgroup) and both
-// s/p 2c kernels.  The last candidate leaves the blocks of its own build, so
-// the build is run once more with the chosen shape before returning.
+// s/p 2c kernels, and the one launch with the 3c tile queue.  The last
+// candidate leaves the blocks of its own build, so the build is run once more
+// with the chosen shape before returning.
 int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   if (!ctx) return FB_E_ARG;
   if (!ctx->lists_ok || !ctx->done[FB_MAT_S] || !ctx->done[FB_MAT_DIP] || !ctx->done[FB_MAT_VNA])
@@ -3126,7 +3133,7 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   FB_HIP(hipEventCreate(&ev.b));
   const int charged = any_charge(ctx);
   const bool timing = ctx->timing;
-  ctx->timing = false;   // enqueue_build then takes the forked two-stream form
+  ctx->timing = false;   // builds then take the graph (or, without graphs, the forked two-stream form)
   struct Cfg {
     int chunk, k2c, fused, persist;
   };
@@ -3149,14 +3156,18 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   float tbest = 1e30f;
   hipError_t e = hipSuccess;
   int rc = FB_OK;
-  for (const Cfg &c : cand) {
+  // two sweeps over the candidates, the best time of each counting, so that
+  // clock ramp-up in the first sweep does not decide
+  std::vector<Cfg> sweep(cand);
+  sweep.insert(sweep.end(), cand.begin(), cand.end());
+  for (const Cfg &c : sweep) {
     ctx->chunk3c = c.chunk;
     ctx->k2c = c.k2c;
     ctx->fused = c.fused;
     ctx->persist = c.persist;
-    rc = enqueue_build(ctx, charged);   // warm-up
+    rc = launch_build(ctx, charged);   // warm-up (and the capture of this shape's graph)
     if (rc == FB_OK) e = hipEventRecord(ev.a, ctx->stream);
-    for (int r = 0; r < nrep && rc == FB_OK && e == hipSuccess; r++) rc = enqueue_build(ctx, charged);
+    for (int r = 0; r < nrep && rc == FB_OK && e == hipSuccess; r++) rc = launch_build(ctx, charged);
     if (rc == FB_OK && e == hipSuccess) e = hipEventRecord(ev.b, ctx->stream);
     if (rc == FB_OK && e == hipSuccess) e = hipEventSynchronize(ev.b);
     float ms = 0.f;
@@ -3172,7 +3183,7 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
   ctx->fused = best.fused;
   ctx->persist = best.persist;
   ctx->mlp_gen++;   // captured graphs hold the previous launch shapes
-  if (rc == FB_OK && e == hipSuccess) rc = enqueue_build(ctx, charged);
+  if (rc == FB_OK && e == hipSuccess) rc = launch_build(ctx, charged);
   ctx->timing = timing;
   if (e != hipSuccess) return ctx->fail(FB_E_HIP, "autotune", e);
   RC(rc);
