@@ -12,6 +12,7 @@
 #include "fb_2c_dev.h"
 
 #define WG2 256
+static_assert(WG2 == FB_2C_PCAP, "the 2c lane order (k_pair_perm) is laid out for passes of FB_2C_PCAP pairs");
 
 __device__ __forceinline__ int cell2c_id(const DView &v, int in1, int in2, int slot, int isub) {
   return v.idx2c[(((in1 - 1) * v.sp.ns + (in2 - 1)) * FB_NSLOT2 + slot) * FB_MAXSUB2 + isub];

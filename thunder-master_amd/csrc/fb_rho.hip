@@ -295,6 +295,7 @@ __global__ __launch_bounds__(GenR<MO>::T) void k_rho_3c(DView v, int stride) {
 #include "fb_2c_dev.h"
 
 #define WRS 256   // lanes per 2c workgroup (whole atoms, at most 16)
+static_assert(WRS == FB_2C_PCAP, "the 2c lane order (k_pair_perm) is laid out for passes of FB_2C_PCAP pairs");
 
 // shell block from table order (issh major, make_munuS) to column-major
 template <int NSI, int NSJ>
@@ -405,9 +406,13 @@ __global__ __launch_bounds__(WRS, 2) void k_rho_2c_sp(DView v, int stride) {
   double acc_in = 0.0, acc_bond = 0.0;
   int status = 0;
   for (int64_t c0 = pb; c0 < pe; c0 += WRS) {
-    const int64_t p = c0 + tid;
+    // the lane at slot c0 + tid takes pair p of its wavefront's group (the
+    // 2c lane order of v.pair_perm); LDS rows and sums go by p
+    const int64_t slot = c0 + tid;
+    const int64_t p = (slot < pe && v.pair_perm) ? c0 + (tid & ~63) + v.pair_perm[slot] : slot;
+    const int srow = (int)(p - c0);
     int nsj = 0;
-    if (p < pe) {
+    if (slot < pe) {
       const int ia = v.pair_atom[p];
       const int s = v.atom_struct[ia];
       const int ja = v.st[s].atom_off + v.neigh_j[p] - 1;
@@ -435,7 +440,7 @@ __global__ __launch_bounds__(WRS, 2) void k_rho_2c_sp(DView v, int stride) {
       nsj = v.sp.nssh[in2 - 1];
       const int cls = (v.sp.norb[in1 - 1] == 4 ? 2 : 0) + (v.sp.norb[in2 - 1] == 4 ? 1 : 0);
       const int64_t bo = W ? v.wboff[p] : v.boff[p];
-      double *mine = s_t + tid * stride;
+      double *mine = s_t + srow * stride;
       switch (cls) {
         case 3: status |= pair_rho_sp<W, 3>(v, ia, ja, in1, in2, self, z, pm, bo, mine); break;
         case 2: status |= pair_rho_sp<W, 2>(v, ia, ja, in1, in2, self, z, pm, bo, mine); break;
@@ -443,7 +448,7 @@ __global__ __launch_bounds__(WRS, 2) void k_rho_2c_sp(DView v, int stride) {
         default: status |= pair_rho_sp<W, 0>(v, ia, ja, in1, in2, self, z, pm, bo, mine); break;
       }
     }
-    s_n[tid] = nsj;
+    s_n[srow] = nsj;
     __syncthreads();
     if (rown && re < rn * rn) {
       const int64_t k0 = max(rp0, c0), k1 = min(rp1, c0 + WRS);
