@@ -3153,7 +3153,8 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
     }
   }
   Cfg best{ctx->chunk3c, ctx->k2c, ctx->fused, ctx->persist};
-  float tbest = 1e30f;
+  Cfg best_sep = best;   // the fastest separate-launch shape: its chunk and 2c kernel stay set
+  float tbest = 1e30f, tsep = 1e30f;
   hipError_t e = hipSuccess;
   int rc = FB_OK;
   // two sweeps over the candidates, the best time of each counting, so that
@@ -3177,6 +3178,16 @@ int fb_autotune(fb_ctx *ctx, int nrep, int *chosen) {
       tbest = ms;
       best = c;
     }
+    if (c.fused == 0 && ms < tsep) {
+      tsep = ms;
+      best_sep = c;
+    }
+  }
+  // a one-launch winner leaves chunk and 2c kernel to the separate launches
+  // (per-kernel timing, charged builds), which take the best separate shape's
+  if (best.fused && tsep < 1e30f) {
+    best.chunk = best_sep.chunk;
+    best.k2c = best_sep.k2c;
   }
   ctx->chunk3c = best.chunk;
   ctx->k2c = best.k2c;
