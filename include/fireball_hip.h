@@ -350,7 +350,9 @@ int fb_mlp_gradients_noalloc(fb_ctx *ctx, int nme, const float *params, int n, c
  * replays unless FB_TUNE_GRAPH is 0; nrep each, two sweeps, HIP events) over every
  * launch shape that applies to the context -- the fused launch at both
  * occupancies, with and without the queue, and the separate launches with every chunk and both 2c
- * kernels -- and keeps the fastest; it returns the chosen chunk in *chosen.
+ * kernels -- and keeps the fastest; when that is the one launch, the chunk
+ * and 2c kernel of the separate launches are those of the fastest separate
+ * shape.  It returns the chosen chunk in *chosen.
  * The blocks hold a complete build afterwards. */
 enum fb_tune { FB_TUNE_CHUNK3C = 0, FB_TUNE_2C_KERNEL = 1, FB_TUNE_GRAPH = 2, FB_TUNE_FORCE_KERNEL = 3, FB_TUNE_FUSED = 4,
               FB_TUNE_PERSIST = 5 };
